@@ -1,0 +1,421 @@
+"""Structured tensor-product meshes, face tables and seeded states.
+
+The reference reads gmsh meshes and matches periodic faces
+(src/simulation/mesh/mesh_readers.c, mesh/mesh_periodic.c:102) and stores the
+left/right face correspondence as ``neigh_info{volume, ind_lf, ind_ord}``
+(src/simulation/computational_elements/face.h:32-70).  Here meshes are
+generated in memory and flattened into the arrays the C-ABI takes:
+
+* ``xyz``   (E, d, Np) float64 -- isoparametric geometry nodes at the GLL solution
+  nodes, r-fastest ordering (element_operators_tp.c:90-131).
+* ``faces`` (F, 6) int32 -- ``eL, fL, eR, fR, ord, bc`` with TP local faces
+  ``2*dim + side`` (side 0 = -1, 1 = +1) and ``ord`` the orientation bits
+  (bit0 transpose, bit1 flip first face coordinate, bit2 flip second).
+
+Face node ordering: the face coordinates are the remaining volume dimensions in
+increasing order, first fastest.  ``perm[ord][i]`` is the right-face index of the
+node that coincides with left-face node ``i``.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+BC_INTERIOR, BC_RIEMANN, BC_SLIPWALL = 0, 1, 2
+PDE_ADVECTION, PDE_EULER = 0, 1
+FLUX_UPWIND, FLUX_LF, FLUX_ROE = 0, 1, 2
+GAMMA, GM1 = 1.4, 0.4
+
+
+# --------------------------------------------------------------------------- nodes
+def gauss_lobatto(n: int) -> tuple[np.ndarray, np.ndarray]:
+    """GLL nodes/weights on [-1, 1] (interior nodes: roots of P_{n-1}')."""
+    N = n - 1
+    P = np.polynomial.legendre.Legendre.basis(N)
+    x = np.concatenate([[-1.0], np.sort(P.deriv().roots().real), [1.0]])
+    # polish the interior roots with Newton on P_N'
+    dP, d2P = P.deriv(), P.deriv(2)
+    for _ in range(3):
+        x[1:-1] -= dP(x[1:-1]) / d2P(x[1:-1])
+    w = 2.0 / (N * (N + 1) * P(x) ** 2)
+    x = 0.5 * (x - x[::-1])
+    w = 0.5 * (w + w[::-1])
+    return x, w
+
+
+def gauss_legendre(n: int) -> tuple[np.ndarray, np.ndarray]:
+    x, w = np.polynomial.legendre.leggauss(n)
+    x = 0.5 * (x - x[::-1])
+    w = 0.5 * (w + w[::-1])
+    return x, w
+
+
+def _tp_coords(N: int, d: int) -> np.ndarray:
+    """(Np, d) integer multi-indices, dimension 0 fastest."""
+    idx = np.arange(N ** d)
+    return np.stack([(idx // N ** k) % N for k in range(d)], axis=1)
+
+
+def face_node_volume_index(d: int, N: int, lf: int) -> np.ndarray:
+    """Volume node index of each face-local node of local face lf."""
+    fd, side = lf // 2, lf % 2
+    nf = N ** (d - 1)
+    out = np.empty(nf, dtype=np.int64)
+    rem = [k for k in range(d) if k != fd]
+    for i in range(nf):
+        ab = [(i // N ** j) % N for j in range(d - 1)]
+        idx = [0] * d
+        for j, k in enumerate(rem):
+            idx[k] = ab[j]
+        idx[fd] = N - 1 if side else 0
+        out[i] = sum(idx[k] * N ** k for k in range(d))
+    return out
+
+
+def face_perm(d: int, N: int, ord_: int) -> np.ndarray:
+    nf = N ** (d - 1)
+    perm = np.empty(nf, dtype=np.int64)
+    if d == 1:
+        perm[0] = 0
+        return perm
+    if d == 2:
+        for a in range(N):
+            perm[a] = N - 1 - a if ord_ & 2 else a
+        return perm
+    for b in range(N):
+        for a in range(N):
+            ap, bp = (b, a) if ord_ & 1 else (a, b)
+            if ord_ & 2:
+                ap = N - 1 - ap
+            if ord_ & 4:
+                bp = N - 1 - bp
+            perm[a + N * b] = ap + N * bp
+    return perm
+
+
+def n_orientations(d: int) -> int:
+    return {1: 1, 2: 4, 3: 8}[d]
+
+
+# --------------------------------------------------------------------------- rng
+class SplitMix64:
+    """splitmix64 (Steele, Lea & Flood 2014) -- the seeded generator for all synthetic inputs."""
+
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed: int):
+        self.state = seed & self.MASK
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & self.MASK
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & self.MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & self.MASK
+        return z ^ (z >> 31)
+
+    def uniform(self, n: int, lo: float = -1.0, hi: float = 1.0) -> np.ndarray:
+        """n doubles in [lo, hi) from the top 53 bits; vectorised for large n."""
+        if n <= 4096:
+            v = np.array([(self.next_u64() >> 11) * (1.0 / (1 << 53)) for _ in range(n)])
+            return lo + (hi - lo) * v
+        # vectorised splitmix64 over a counter: state_k = s0 + k*gamma
+        g = np.uint64(0x9E3779B97F4A7C15)
+        with np.errstate(over="ignore"):
+            ks = np.arange(1, n + 1, dtype=np.uint64)
+            z = np.uint64(self.state) + ks * g
+            self.state = int(z[-1])
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        v = (z >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+        return lo + (hi - lo) * v
+
+
+SEED = 20261202
+
+
+# --------------------------------------------------------------------------- mesh
+@dataclass
+class Mesh:
+    d: int
+    p: int
+    xyz: np.ndarray            # (E, d, Np)
+    faces: np.ndarray          # (F, 6) int32
+    periods: tuple = ()
+    dims: tuple = ()
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def N(self) -> int:
+        return self.p + 1
+
+    @property
+    def E(self) -> int:
+        return self.xyz.shape[0]
+
+    @property
+    def F(self) -> int:
+        return self.faces.shape[0]
+
+    @property
+    def Np(self) -> int:
+        return self.N ** self.d
+
+
+def _match_faces(d, N, xyz, pairs, periods):
+    """Given element pairs (eL, eR) that share a face, find (fL, fR, ord) geometrically."""
+    nf = N ** (d - 1)
+    fidx = [face_node_volume_index(d, N, lf) for lf in range(2 * d)]
+    perms = [face_perm(d, N, o) for o in range(n_orientations(d))]
+    per = np.array([p if p else 0.0 for p in periods] + [0.0] * (d - len(periods)))
+
+    def wrapdiff(a, b):
+        diff = a - b
+        for k in range(d):
+            if per[k] > 0:
+                diff[..., k] -= per[k] * np.round(diff[..., k] / per[k])
+        return diff
+
+    out = []
+    for eL, eR, k in pairs:
+        cL = [xyz[eL][:, fidx[lf]].T for lf in range(2 * d)]  # (nf, d)
+        cR = [xyz[eR][:, fidx[lf]].T for lf in range(2 * d)]
+        found = None
+        # the left face is the one of eL lying furthest along +k
+        fL_hi = int(np.argmax([c[:, k].mean() for c in cL]))
+        for fL in (fL_hi,):
+            mL = cL[fL].mean(0)
+            for fR in range(2 * d):
+                if eL == eR and fL == fR:
+                    continue
+                if np.abs(wrapdiff(mL, cR[fR].mean(0))).max() > 1e-9:
+                    continue
+                for o, pm in enumerate(perms):
+                    if np.abs(wrapdiff(cL[fL], cR[fR][pm])).max() < 1e-9:
+                        found = (fL, fR, o)
+                        break
+                if found:
+                    break
+            if found:
+                break
+        if found is None:
+            raise ValueError(f"faces of elements {eL},{eR} do not match")
+        out.append(found)
+    return out
+
+
+def structured_tp_mesh(d: int, p: int, n: tuple, lo: tuple, hi: tuple, periodic: tuple,
+                       curve=None, bc_low=None, bc_high=None, rotate=None) -> Mesh:
+    """Structured QUAD/HEX mesh of prod(n) elements, i fastest.
+
+    curve(x: (..., d)) -> (..., d) maps straight node coordinates to curved ones.
+    rotate: optional array (E,) of orientation codes; element local numbering is
+    permuted (2-D: 90-degree turns, 3-D: axis flips/permutations) to exercise
+    general face orientations.
+    """
+    N = p + 1
+    x1, _ = gauss_lobatto(N)
+    ref = x1[_tp_coords(N, d)]                    # (Np, d) in [-1, 1]
+    E = int(np.prod(n))
+    eidx = _tp_coords_var(n)                      # (E, d)
+    h = np.array([(hi[k] - lo[k]) / n[k] for k in range(d)])
+    base = np.array(lo)[None, :] + eidx * h[None, :]
+    pts = base[:, None, :] + 0.5 * (ref[None, :, :] + 1.0) * h[None, None, :]   # (E, Np, d)
+    if curve is not None:
+        pts = curve(pts)
+    if rotate is not None:
+        pts = _apply_rotation(d, N, pts, rotate)
+    xyz = np.ascontiguousarray(pts.transpose(0, 2, 1))
+
+    def eid(c):
+        return int(sum(c[k] * int(np.prod(n[:k])) for k in range(d)))
+
+    pairs, bfaces = [], []
+    for e in range(E):
+        c = list(eidx[e])
+        for k in range(d):
+            cn = list(c)
+            cn[k] += 1
+            if cn[k] == n[k]:
+                if periodic[k]:
+                    cn[k] = 0
+                else:
+                    bfaces.append((e, k, 1))
+                    continue
+            pairs.append((e, eid(cn), k))
+            if c[k] == 0 and not periodic[k]:
+                bfaces.append((e, k, 0))
+    periods = tuple((hi[k] - lo[k]) if periodic[k] else 0.0 for k in range(d))
+    matched = _match_faces(d, N, xyz, pairs, periods)
+    rows = [(eL, fL, eR, fR, o, BC_INTERIOR) for (eL, eR, _), (fL, fR, o) in zip(pairs, matched)]
+    if bfaces and rotate is not None:
+        raise ValueError("rotated elements are only supported on fully periodic meshes")
+    for e, k, side in bfaces:
+        rows.append((e, 2 * k + side, -1, -1, 0, (bc_high if side else bc_low)[k]))
+    faces = np.array(rows, dtype=np.int32).reshape(-1, 6)
+    return Mesh(d, p, xyz, faces, periods, tuple(n))
+
+
+def _tp_coords_var(n: tuple) -> np.ndarray:
+    E = int(np.prod(n))
+    idx = np.arange(E)
+    out, stride = [], 1
+    for k in range(len(n)):
+        out.append((idx // stride) % n[k])
+        stride *= n[k]
+    return np.stack(out, axis=1)
+
+
+def _apply_rotation(d, N, pts, rotate):
+    """Renumber element-local nodes: pts[e, new_local] = pts[e, old_local(new_local)]."""
+    coords = _tp_coords(N, d)
+    out = pts.copy()
+    for e in range(pts.shape[0]):
+        r = int(rotate[e])
+        if r == 0:
+            continue
+        c = coords.copy()
+        if d == 2:
+            for _ in range(r % 4):                 # 90-degree turn of the local frame
+                c = np.stack([N - 1 - c[:, 1], c[:, 0]], axis=1)
+        else:
+            if r & 1:
+                c = c[:, [1, 0, 2]]
+            if r & 2:
+                c = c[:, [0, 2, 1]]
+            if r & 4:
+                c[:, 0] = N - 1 - c[:, 0]
+        src = sum(c[:, k] * N ** k for k in range(d))
+        out[e] = pts[e][src]
+    return out
+
+
+# --------------------------------------------------------------------------- states
+def prim_to_cons(prim: np.ndarray, d: int) -> np.ndarray:
+    """prim[..., (rho, u.., p)] -> conserved."""
+    rho = prim[..., 0]
+    vel = prim[..., 1:1 + d]
+    p = prim[..., d + 1]
+    out = np.empty_like(prim)
+    out[..., 0] = rho
+    out[..., 1:1 + d] = rho[..., None] * vel
+    out[..., d + 1] = p / GM1 + 0.5 * rho * (vel ** 2).sum(-1)
+    return out
+
+
+def nodes_xyz(mesh: Mesh) -> np.ndarray:
+    """(E, Np, d) physical coordinates of the solution nodes (isoparametric)."""
+    return mesh.xyz.transpose(0, 2, 1)
+
+
+def pack_state(cons_nodes: np.ndarray) -> np.ndarray:
+    """(E, Np, nv) -> flat reference layout per element (Np, nv) column-major."""
+    return np.ascontiguousarray(cons_nodes.transpose(0, 2, 1)).ravel()
+
+
+# --------------------------------------------------------------------------- configs
+def config1_mesh(n: int = 16, p: int = 2) -> Mesh:
+    """2-D advection, periodic [-1,1]^2 QUAD."""
+    return structured_tp_mesh(2, p, (n, n), (-1.0, -1.0), (1.0, 1.0), (True, True))
+
+
+def config1_state(mesh: Mesh) -> np.ndarray:
+    x = nodes_xyz(mesh)
+    u = np.sin(np.pi * x[..., 0]) * np.sin(np.pi * x[..., 1])
+    return pack_state(u[..., None])
+
+
+def config2_mesh(n: int = 128, p: int = 3) -> Mesh:
+    """2-D Euler, periodic [0,10]^2 QUAD."""
+    return structured_tp_mesh(2, p, (n, n), (0.0, 0.0), (10.0, 10.0), (True, True))
+
+
+def isentropic_vortex(x: np.ndarray, beta=5.0, xc=(5.0, 5.0), mach=0.5) -> np.ndarray:
+    """Isentropic vortex (strength beta, unit radius) on a Mach `mach` free stream along x; primitive."""
+    dx = x[..., 0] - xc[0]
+    dy = x[..., 1] - xc[1]
+    r2 = dx * dx + dy * dy
+    T = 1.0 - GM1 * beta ** 2 / (8.0 * GAMMA * math.pi ** 2) * np.exp(1.0 - r2)
+    rho = T ** (1.0 / GM1)
+    p = rho ** GAMMA
+    uinf = mach * math.sqrt(GAMMA)             # rho_inf = p_inf = 1 -> c_inf = sqrt(gamma)
+    u = uinf - beta / (2.0 * math.pi) * np.exp(0.5 * (1.0 - r2)) * dy
+    v = beta / (2.0 * math.pi) * np.exp(0.5 * (1.0 - r2)) * dx
+    return np.stack([rho, u, v, p], axis=-1)
+
+
+def config2_state(mesh: Mesh, noise: float = 1e-3, seed: int = SEED) -> np.ndarray:
+    x = nodes_xyz(mesh)
+    cons = prim_to_cons(isentropic_vortex(x), 2)
+    rng = SplitMix64(seed + 2)
+    cons = cons + noise * rng.uniform(cons.size).reshape(cons.shape)
+    return pack_state(cons)
+
+
+def curve3d(amp: float = 0.05):
+    def f(pts):
+        x, y, z = pts[..., 0], pts[..., 1], pts[..., 2]
+        s = 2.0 * np.pi
+        out = pts.copy()
+        out[..., 0] = x + amp * np.sin(s * y) * np.sin(s * z)
+        out[..., 1] = y + amp * np.sin(s * z) * np.sin(s * x)
+        out[..., 2] = z + amp * np.sin(s * x) * np.sin(s * y)
+        return out
+    return f
+
+
+def config3_mesh(n: int = 64, p: int = 4, amp: float = 0.05) -> Mesh:
+    """3-D Euler, periodic [0,1]^3 HEX, curved by x += amp sin(2 pi y) sin(2 pi z) (cyclic)."""
+    return structured_tp_mesh(3, p, (n, n, n), (0.0,) * 3, (1.0,) * 3, (True,) * 3, curve=curve3d(amp))
+
+
+def smooth_state_3d(x: np.ndarray, seed: int = SEED) -> np.ndarray:
+    """rho = 1+0.1 s1, (u,v,w) = 0.28 (s2,s3,s4), p = 1 + 0.1 s5 with s_k = prod_c sin(2 pi (c + phi_kc))."""
+    rng = SplitMix64(seed + 3)
+    phi = rng.uniform(15, 0.0, 1.0).reshape(5, 3)
+    d = x.shape[-1]
+    s = []
+    for k in range(5):
+        v = np.ones(x.shape[:-1])
+        for c in range(d):
+            v = v * np.sin(2.0 * np.pi * (x[..., c] + phi[k, c]))
+        s.append(v)
+    prim = np.stack([1.0 + 0.1 * s[0], 0.28 * s[1], 0.28 * s[2], 0.28 * s[3], 1.0 + 0.1 * s[4]], axis=-1)
+    if d == 2:
+        prim = prim[..., [0, 1, 2, 4]]
+    return prim
+
+
+def config3_state(mesh: Mesh, seed: int = SEED) -> np.ndarray:
+    x = nodes_xyz(mesh)
+    return pack_state(prim_to_cons(smooth_state_3d(x, seed), mesh.d))
+
+
+def channel_mesh(nx: int = 64, ny: int = 64, p: int = 3) -> Mesh:
+    """[-1.5,1.5] x [0,0.8] channel, bottom y = 0.0625 exp(-25 x^2); Riemann in/outflow, slip walls."""
+    H = 0.8
+
+    def curve(pts):
+        out = pts.copy()
+        x, eta = pts[..., 0], pts[..., 1] / H
+        b = 0.0625 * np.exp(-25.0 * x * x)
+        out[..., 1] = b + eta * (H - b)
+        return out
+
+    return structured_tp_mesh(2, p, (nx, ny), (-1.5, 0.0), (1.5, H), (False, False), curve=curve,
+                              bc_low=(BC_RIEMANN, BC_SLIPWALL), bc_high=(BC_RIEMANN, BC_SLIPWALL))
+
+
+def freestream_prim(d: int, mach: float = 0.5) -> np.ndarray:
+    prim = np.zeros(d + 2)
+    prim[0] = 1.0
+    prim[1] = mach * math.sqrt(GAMMA)
+    prim[d + 1] = 1.0
+    return prim
+
+
+def uniform_state(mesh: Mesh, prim: np.ndarray) -> np.ndarray:
+    cons = prim_to_cons(prim[None, None, :], mesh.d)
+    return pack_state(np.broadcast_to(cons, (mesh.E, mesh.Np, mesh.d + 2)).copy())

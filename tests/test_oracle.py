@@ -1,0 +1,386 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+Each check is chosen so that a plausible slip (dropped term, wrong sign/index,
+transposed operand) fails: reference golden data, closed forms, conservation and
+free-stream invariants, and complex-step vs finite-difference consistency.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from dpgsolver_amd import mesh as M
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+EPS = 1e-15
+
+
+def _tp(x1, d):
+    N = len(x1)
+    idx = np.arange(N ** d)
+    return np.stack([x1[(idx // N ** k) % N] for k in range(d)], axis=1)
+
+
+# ----------------------------------------------------------------------------- nodes (nodes_tp.data)
+@pytest.mark.parametrize("name", ["Nodes_d1_p3_GLoL", "Nodes_d1_p3_GLe", "Nodes_d2_p4_GLoL", "Nodes_d2_p4_GLe",
+                                  "Nodes_d3_p3_GLoL", "Nodes_d3_p3_GLe"])
+def test_nodes_against_reference_golden(name):
+    g = json.load(open(os.path.join(GOLD, "nodes_tp.json")))["nodes"][name]
+    d = int(name.split("_")[1][1:])
+    p = int(name.split("_")[2][1:])
+    x1, w1 = (O.gauss_lobatto if "GLoL" in name else O.gauss_legendre)(p + 1)
+    rst = _tp(x1, d)
+    w = np.prod(_tp(w1, d), axis=1)
+    np.testing.assert_allclose(rst, np.array(g["rst"]), atol=2 * EPS, rtol=0)
+    np.testing.assert_allclose(w, np.array(g["w"]), atol=4 * EPS, rtol=0)
+
+
+@pytest.mark.parametrize("n", range(2, 9))
+def test_quadrature_exactness(n):
+    xg, wg = O.gauss_legendre(n)
+    xl, wl = O.gauss_lobatto(n)
+    for k in range(2 * n):          # GL exact to degree 2n-1, GLL to 2n-3
+        exact = 0.0 if k % 2 else 2.0 / (k + 1)
+        assert abs(wg @ xg ** k - exact) < 1e-14
+        if k <= 2 * n - 3:
+            assert abs(wl @ xl ** k - exact) < 1e-14
+    # GLL degree 2n-2 must NOT be exact (pins that these are Lobatto, not Legendre, points)
+    assert abs(wl @ xl ** (2 * n - 2) - 2.0 / (2 * n - 1)) > 1e-6
+
+
+def test_product_package_nodes_agree_with_oracle():
+    for n in range(2, 9):
+        np.testing.assert_allclose(M.gauss_lobatto(n)[0], O.gauss_lobatto(n)[0], atol=2e-15)
+        np.testing.assert_allclose(M.gauss_legendre(n)[1], O.gauss_legendre(n)[1], atol=2e-15)
+
+
+def test_lagrange_interp_and_derivative_exact_on_polynomials():
+    xl, _ = O.gauss_lobatto(6)
+    xg, _ = O.gauss_legendre(6)
+    I, D = O.lagrange(xl, xg)
+    for k in range(6):
+        np.testing.assert_allclose(I @ xl ** k, xg ** k, atol=1e-14)
+        np.testing.assert_allclose(D @ xl ** k, k * xg ** max(k - 1, 0) if k else 0 * xg, atol=1e-13)
+    I2, _ = O.lagrange(xl, xl)
+    np.testing.assert_allclose(I2, np.eye(6), atol=1e-15)
+
+
+# ----------------------------------------------------------------------------- operators_tp.data
+@pytest.mark.parametrize("elem", ["quad", "hex"])
+def test_kronecker_ordering_against_reference_golden(elem):
+    g = json.load(open(os.path.join(GOLD, "operators_tp.json")))["operators"]
+    ops = [np.array(m) for m in g[f"{elem}_ops_tp"]]
+    std = O.kron(ops)
+    np.testing.assert_allclose(std, np.array(g[f"{elem}_op_std"]), atol=5e-16 + 1e-15, rtol=0)
+    b = np.array(g[f"{elem}_b"])
+    c = np.array(g[f"{elem}_c"])
+    np.testing.assert_allclose(std @ b, c, rtol=1e-9, atol=1e-9)
+
+
+def test_kronecker_transposed_ordering_fails():
+    g = json.load(open(os.path.join(GOLD, "operators_tp.json")))["operators"]
+    ops = [np.array(m) for m in g["quad_ops_tp"]]
+    assert not np.allclose(O.kron(ops[::-1]).shape, np.array(g["quad_op_std"]).shape) or \
+        not np.allclose(O.kron(ops[::-1]), np.array(g["quad_op_std"]))
+
+
+# ----------------------------------------------------------------------------- fluxes
+def _rand_state(rng, d):
+    rho = 1.0 + 0.2 * rng.random()
+    vel = 0.4 * (rng.random(d) - 0.5)
+    p = 1.0 + 0.2 * rng.random()
+    return M.prim_to_cons(np.concatenate([[rho], vel, [p]]), d)
+
+
+def _rot(d, rng):
+    q, _ = np.linalg.qr(rng.standard_normal((d, d)))
+    return q
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_euler_flux_rotational_invariance_and_structure(d):
+    rng = np.random.default_rng(1)
+    s = _rand_state(rng, d)
+    f = O.euler_flux(d, s).reshape(d + 2, d)          # f[eq, dim]
+    R = _rot(d, rng)
+    sr = s.copy()
+    sr[1:1 + d] = R @ s[1:1 + d]
+    fr = O.euler_flux(d, sr).reshape(d + 2, d)
+    # F(R s) = R_eq F(s) R^T: mass/energy rows rotate as vectors, momentum block as a tensor
+    np.testing.assert_allclose(fr[0], R @ f[0], atol=1e-14)
+    np.testing.assert_allclose(fr[d + 1], R @ f[d + 1], atol=1e-14)
+    np.testing.assert_allclose(fr[1:1 + d], R @ f[1:1 + d] @ R.T, atol=1e-14)
+    # state at rest: only pressure on the momentum diagonal
+    s0 = M.prim_to_cons(np.array([1.2] + [0.0] * d + [0.7]), d)
+    f0 = O.euler_flux(d, s0).reshape(d + 2, d)
+    np.testing.assert_allclose(f0[1:1 + d], 0.7 * np.eye(d), atol=1e-15)
+    assert np.abs(f0[0]).max() == 0.0 and np.abs(f0[d + 1]).max() == 0.0
+
+
+@pytest.mark.parametrize("d", [2, 3])
+@pytest.mark.parametrize("flux", [M.FLUX_LF, M.FLUX_ROE])
+def test_numflux_consistency_and_conservation(d, flux):
+    rng = np.random.default_rng(2 + d + flux)
+    for _ in range(20):
+        sL, sR = _rand_state(rng, d), _rand_state(rng, d)
+        n = rng.standard_normal(d)
+        n /= np.linalg.norm(n)
+        F = O.euler_flux(d, sL).reshape(d + 2, d) @ n
+        np.testing.assert_allclose(O.numflux(d, 1, flux, sL, sL, n), F, atol=2e-15 * 10)
+        swapped = -O.numflux(d, 1, flux, sR, sL, -n)
+        if flux == M.FLUX_LF:
+            np.testing.assert_allclose(O.numflux(d, 1, flux, sL, sR, n), swapped, atol=1e-14)
+
+
+def test_roe_entropy_fix_is_side_asymmetric():
+    """Qu 2015 eq. 35 as used by the reference takes l1 from the LEFT and l5 from the RIGHT state
+    (numerical_flux_euler_T.c:561-563), so F(sL,sR,n) != -F(sR,sL,-n) in general; the DG face
+    term stays conservative because each face flux is evaluated once and applied with both signs."""
+    rng = np.random.default_rng(11)
+    d = 2
+    diffs = []
+    for _ in range(20):
+        sL, sR = _rand_state(rng, d), _rand_state(rng, d)
+        n = np.array([1.0, 0.0])
+        diffs.append(np.abs(O.numflux(d, 1, M.FLUX_ROE, sL, sR, n)
+                            + O.numflux(d, 1, M.FLUX_ROE, sR, sL, -n)).max())
+    assert max(diffs) > 1e-6
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_roe_preserves_stationary_contact_lf_does_not(d):
+    sL = M.prim_to_cons(np.array([1.0] + [0.0] * d + [1.0]), d)
+    sR = M.prim_to_cons(np.array([0.5] + [0.0] * d + [1.0]), d)
+    n = np.zeros(d)
+    n[0] = 1.0
+    exact = np.zeros(d + 2)
+    exact[1] = 1.0
+    np.testing.assert_allclose(O.numflux(d, 1, M.FLUX_ROE, sL, sR, n), exact, atol=1e-15)
+    lf = O.numflux(d, 1, M.FLUX_LF, sL, sR, n)
+    # LF adds 0.5*lambda*(rhoL-rhoR) with lambda = max(|V|+c) = c_R (|V| = 0)
+    lam = max(np.sqrt(1.4 * 1.0 / 1.0), np.sqrt(1.4 * 1.0 / 0.5))
+    assert abs(lf[0] - 0.5 * lam * 0.5) < 1e-14
+
+
+def test_lf_uses_speed_magnitude_not_normal_speed():
+    # tangential flow: V.n = 0 but |V| > 0, so lambda must include |V| (numerical_flux_euler_T.c:137-172)
+    d = 2
+    sL = M.prim_to_cons(np.array([1.0, 0.0, 0.5, 1.0]), d)
+    sR = M.prim_to_cons(np.array([0.9, 0.0, 0.5, 1.0]), d)
+    n = np.array([1.0, 0.0])
+    f = O.numflux(d, 1, M.FLUX_LF, sL, sR, n)
+    lam = max(0.5 + np.sqrt(1.4), 0.5 + np.sqrt(1.4 / 0.9))
+    central = 0.5 * (O.euler_flux(d, sL).reshape(4, 2) @ n + O.euler_flux(d, sR).reshape(4, 2) @ n)
+    np.testing.assert_allclose(f, central + 0.5 * lam * (sL - sR), atol=1e-14)
+
+
+def test_upwind_selects_by_sign_of_b_dot_n():
+    b = np.array([1.0, 0.5, 0.0])
+    n = np.array([0.6, 0.8])
+    assert O.numflux(2, 0, 0, [2.0], [3.0], n, b)[0] == pytest.approx((0.6 + 0.4) * 2.0)
+    assert O.numflux(2, 0, 0, [2.0], [3.0], -n, b)[0] == pytest.approx(-(0.6 + 0.4) * 3.0)
+
+
+@pytest.mark.parametrize("d", [2, 3])
+@pytest.mark.parametrize("flux", [M.FLUX_LF, M.FLUX_ROE])
+def test_numflux_complex_step_matches_finite_difference(d, flux):
+    rng = np.random.default_rng(7)
+    sL, sR = _rand_state(rng, d), _rand_state(rng, d)
+    n = rng.standard_normal(d)
+    n /= np.linalg.norm(n)
+    for side in (0, 1):
+        for v in range(d + 2):
+            e = np.zeros(d + 2)
+            e[v] = 1.0
+            dL, dR = (e, 0 * e) if side == 0 else (0 * e, e)
+            _, cs = O.numflux_cs(d, 1, flux, sL, sR, n, dL, dR)
+            h = 1e-6
+            fd = (O.numflux(d, 1, flux, sL + h * dL, sR + h * dR, n)
+                  - O.numflux(d, 1, flux, sL - h * dL, sR - h * dR, n)) / (2 * h)
+            np.testing.assert_allclose(cs, fd, atol=2e-8)
+
+
+def test_slipwall_state_reflects_normal_momentum():
+    d = 2
+    s = M.prim_to_cons(np.array([1.1, 0.3, -0.2, 0.9]), d)
+    n = np.array([0.6, 0.8])
+    sb = O.bc_state(d, M.BC_SLIPWALL, s, n, np.zeros(5))
+    mL, mB = s[1:3], sb[1:3]
+    assert abs((mL + mB) @ n) < 1e-15                     # average momentum tangential
+    np.testing.assert_allclose(mB - (mB @ n) * n, mL - (mL @ n) * n, atol=1e-15)
+    assert sb[0] == s[0] and sb[3] == s[3]
+
+
+def test_riemann_bc_returns_freestream_for_freestream_input():
+    d = 2
+    prim = M.freestream_prim(d)
+    s = M.prim_to_cons(prim, d)
+    for n in (np.array([-1.0, 0.0]), np.array([1.0, 0.0]), np.array([0.0, 1.0])):
+        np.testing.assert_allclose(O.bc_state(d, M.BC_RIEMANN, s, n, prim), s, atol=1e-13)
+
+
+# ----------------------------------------------------------------------------- residual invariants
+def _curved2d(n=4, p=3, amp=0.04):
+    def curve(pts):
+        out = pts.copy()
+        x, y = pts[..., 0], pts[..., 1]
+        out[..., 0] = x + amp * np.sin(2 * np.pi * y)
+        out[..., 1] = y + amp * np.sin(2 * np.pi * x)
+        return out
+    return M.structured_tp_mesh(2, p, (n, n), (0.0, 0.0), (1.0, 1.0), (True, True), curve=curve)
+
+
+@pytest.mark.parametrize("flux", [M.FLUX_LF, M.FLUX_ROE])
+@pytest.mark.parametrize("case", ["2d", "3d", "2d_rot"])
+def test_free_stream_preservation_curved(case, flux):
+    if case == "2d":
+        mesh = _curved2d()
+    elif case == "2d_rot":
+        rot = np.random.default_rng(3).integers(0, 4, 16)
+        mesh = M.structured_tp_mesh(2, 3, (4, 4), (0, 0), (1, 1), (True, True), rotate=rot)
+    else:
+        mesh = M.config3_mesh(3, 3)
+    d = mesh.d
+    orc = O.Oracle(mesh, M.PDE_EULER, flux)
+    prim = np.array([1.0, 0.3, -0.2, 0.1, 1.0])[: d + 2] if d == 3 else np.array([1.0, 0.3, -0.2, 1.0])
+    u = M.uniform_state(mesh, prim)
+    r = orc.rhs(u)
+    scale = np.abs(orc.rhs(u, "volume")).max()
+    assert scale > 1e-3                    # the volume term alone is O(1): cancellation is real
+    assert np.abs(r).max() < 2e1 * EPS * 10, np.abs(r).max()   # test_integration_free_stream.c:138 (x10 for 3-D sums)
+
+
+@pytest.mark.parametrize("case", ["2d", "3d", "2d_rot", "3d_rot"])
+def test_conservation_sum_of_rhs_is_zero(case):
+    if case == "2d":
+        mesh = _curved2d()
+    elif case == "3d":
+        mesh = M.config3_mesh(3, 2)
+    elif case == "2d_rot":
+        mesh = M.structured_tp_mesh(2, 3, (4, 4), (0, 0), (1, 1), (True, True),
+                                    rotate=np.random.default_rng(4).integers(0, 4, 16))
+    else:
+        mesh = M.structured_tp_mesh(3, 2, (3, 3, 3), (0,) * 3, (1,) * 3, (True,) * 3,
+                                    rotate=np.random.default_rng(5).integers(0, 8, 27))
+    orc = O.Oracle(mesh, M.PDE_EULER, M.FLUX_ROE)
+    x = M.nodes_xyz(mesh)
+    u = M.pack_state(M.prim_to_cons(M.smooth_state_3d(x), mesh.d))
+    r = orc.rhs(u).reshape(mesh.E, orc.nv, orc.Np)
+    tot = r.sum(axis=(0, 2))              # sum_m phi_m = 1 -> integral of R against the constant
+    assert np.abs(tot).max() < 5e1 * EPS * mesh.E, tot    # test_integration_conservation.c:104 per element
+    assert np.abs(r).max() > 1e-2
+
+
+def test_rotated_mesh_gives_same_residual_as_unrotated():
+    """The same physical mesh with renumbered element-local nodes must give the same nodal residual."""
+    rot = np.random.default_rng(6).integers(0, 4, 9)
+    m0 = M.structured_tp_mesh(2, 2, (3, 3), (0, 0), (1, 1), (True, True))
+    m1 = M.structured_tp_mesh(2, 2, (3, 3), (0, 0), (1, 1), (True, True), rotate=rot)
+    u0 = M.pack_state(M.prim_to_cons(M.smooth_state_3d(M.nodes_xyz(m0)), 2))
+    u1 = M.pack_state(M.prim_to_cons(M.smooth_state_3d(M.nodes_xyz(m1)), 2))
+    r0 = O.Oracle(m0, 1, M.FLUX_ROE).rhs(u0).reshape(9, 4, 9)
+    r1 = O.Oracle(m1, 1, M.FLUX_ROE).rhs(u1).reshape(9, 4, 9)
+    x0 = M.nodes_xyz(m0)
+    x1 = M.nodes_xyz(m1)
+    for e in range(9):
+        # match nodes by coordinates
+        dmat = np.abs(x0[e][:, None, :] - x1[e][None, :, :]).sum(-1)
+        idx = dmat.argmin(1)
+        np.testing.assert_allclose(r0[e], r1[e][:, idx], atol=1e-13)
+
+
+def test_mass_matrix_integrates_area_and_is_spd():
+    mesh = _curved2d(3, 3)
+    orc = O.Oracle(mesh, M.PDE_ADVECTION, M.FLUX_UPWIND, b_adv=(1.0, 0.5))
+    tot = 0.0
+    for e in range(mesh.E):
+        Mm = orc.mass(e)
+        np.testing.assert_allclose(Mm, Mm.T, atol=1e-16)
+        assert np.linalg.eigvalsh(Mm).min() > 0
+        tot += Mm.sum()
+    assert abs(tot - 1.0) < 1e-13          # area of the periodic unit square (curving is area-preserving)
+
+
+def _linear_operator(orc):
+    n = orc.ndof
+    L = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        L[:, j] = orc.apply_minv(orc.rhs(e))
+    return L
+
+
+def test_ssp_rk3_closed_form_on_linear_advection():
+    mesh = M.config1_mesh(4, 2)
+    orc = O.Oracle(mesh, M.PDE_ADVECTION, M.FLUX_UPWIND, b_adv=(1.0, 0.5))
+    L = _linear_operator(orc)
+    u0 = M.config1_state(mesh)
+    dt = 0.01
+    u1, _, err = orc.ssp_rk3_step(u0, dt, positivity=False)
+    A = dt * L
+    np.testing.assert_allclose(u1, u0 + A @ u0 + A @ A @ u0 / 2 + A @ A @ A @ u0 / 6, atol=1e-14)
+    assert err == 0
+
+
+def test_advection_convergence_order():
+    """config-1 physics: p=2 upwind DG, SSP-RK3, order >= p+1-0.165 (test_integration_convergence_support.c:49)."""
+    errs = []
+    for n in (8, 16):
+        mesh = M.config1_mesh(n, 2)
+        orc = O.Oracle(mesh, M.PDE_ADVECTION, M.FLUX_UPWIND, b_adv=(1.0, 0.5))
+        u = M.config1_state(mesh)
+        T, h = 0.2, 2.0 / n
+        steps = int(np.ceil(T / (0.1 * h / 5)))
+        dt = T / steps
+        for _ in range(steps):
+            u, _, _ = orc.ssp_rk3_step(u, dt, positivity=False)
+        x = M.nodes_xyz(mesh)
+        ex = np.sin(np.pi * (x[..., 0] - T)) * np.sin(np.pi * (x[..., 1] - 0.5 * T))
+        diff = u.reshape(mesh.E, 9) - ex
+        # L2 with the mass matrix
+        e2 = sum(diff[e] @ orc.mass(e) @ diff[e] for e in range(mesh.E))
+        errs.append(np.sqrt(e2))
+    order = np.log2(errs[0] / errs[1])
+    assert order >= 3 - 0.165, (errs, order)
+
+
+def test_positivity_limiter_restores_positivity_and_keeps_means():
+    mesh = M.config2_mesh(2, 3)
+    orc = O.Oracle(mesh, M.PDE_EULER, M.FLUX_ROE)
+    u = M.uniform_state(mesh, np.array([1.0, 0.2, 0.1, 1.0])).reshape(mesh.E, 4, 16)
+    u[0, 0, 5] = -0.3                     # negative density at one node of element 0
+    u2, st = orc.positivity(u.ravel())
+    assert st == 0
+    u2 = u2.reshape(mesh.E, 4, 16)
+    np.testing.assert_allclose(u2[1:], u[1:], atol=1e-14)       # untouched elements: round trip only
+    # mean density (Bezier-coefficient mean) is preserved and the element is positive at the nodes
+    assert u2[0, 0].min() > 0
+    # the cell average (mean of the Bezier coefficients = exact integral / area) is preserved
+    xl, _ = O.gauss_lobatto(4)
+    xg, wg = O.gauss_legendre(4)
+    I1, _ = O.lagrange(xl, xg)
+    Iq = O.kron([I1, I1])
+    wq = np.kron(wg, wg) / 4.0
+    np.testing.assert_allclose(wq @ (Iq @ u2[0, 0]), wq @ (Iq @ u[0, 0]), atol=1e-14)
+    _, st_bad = orc.positivity(np.full(mesh.E * 64, -1.0))
+    assert st_bad == -1
+
+
+def test_complex_step_jacobian_matches_finite_difference():
+    mesh = M.structured_tp_mesh(2, 2, (3, 3), (0, 0), (1, 1), (True, True))
+    orc = O.Oracle(mesh, M.PDE_EULER, M.FLUX_ROE)
+    u = M.pack_state(M.prim_to_cons(M.smooth_state_3d(M.nodes_xyz(mesh)), 2))
+    nb = orc.Np * orc.nv
+    for er, ec in ((4, 4), (4, 5)):
+        blk = orc.jacobian_block(u, er, ec)
+        fd = np.zeros_like(blk)
+        h = 1e-6
+        for j in range(nb):
+            up, um = u.copy(), u.copy()
+            up[ec * nb + j] += h
+            um[ec * nb + j] -= h
+            fd[:, j] = (orc.rhs(up) - orc.rhs(um))[er * nb:(er + 1) * nb] / (2 * h)
+        np.testing.assert_allclose(blk, fd, atol=1e-6 * np.abs(blk).max())  # kinks of |.| in the entropy fix
