@@ -1,0 +1,195 @@
+"""Residual-evaluation throughput of the MI355X DG path (GDOF/s), one JSON line on stdout.
+
+Workload: 3-D Euler, DG p=4 on a 64^3 periodic HEX mesh of [0,1]^3 (262,144 elements, 125 nodes x
+5 variables each, 163.84 M DOF), curved by x += 0.05 sin(2 pi y) sin(2 pi z) (cyclic),
+Lax-Friedrichs flux, seeded smooth state rho in [0.9,1.1], |u| <= 0.485, p in [0.9,1.1].
+A "step" is one residual evaluation R(u) (volume + face terms, compute_rlhs_common_dg) with the
+state already resident in HBM.  `value` is the FP32 residual (FP32 state, FP32 geometry shadow);
+the FP64 residual is timed beside it and its error is stated against the FP64 path.
+
+CPU baseline: the FP64 oracle (a C restatement of the reference's dense per-element operator
+application) on one host core, timed on a bounded sample of the same workload (an 8^3 periodic
+mesh of the same curved HEX p=4 elements).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _state_on_device(torch, xyz_dev, seed):
+    """config3_state (mesh.py) evaluated with torch on the device; returns [E][5][Np] float64."""
+    from dpgsolver_amd import mesh as M
+
+    rng = M.SplitMix64(seed + 3)
+    phi = rng.uniform(15, 0.0, 1.0).reshape(5, 3)
+    s = []
+    sx = [torch.sin(2.0 * math.pi * xyz_dev[:, c, :]) for c in range(3)]
+    cx = [torch.cos(2.0 * math.pi * xyz_dev[:, c, :]) for c in range(3)]
+    for k in range(5):
+        v = None
+        for c in range(3):
+            t = sx[c] * math.cos(2.0 * math.pi * phi[k, c]) + cx[c] * math.sin(2.0 * math.pi * phi[k, c])
+            v = t if v is None else v * t
+        s.append(v)
+    rho = 1.0 + 0.1 * s[0]
+    vel = [0.28 * s[1 + c] for c in range(3)]
+    p = 1.0 + 0.1 * s[4]
+    E, Np = xyz_dev.shape[0], xyz_dev.shape[2]
+    out = torch.empty((E, 5, Np), dtype=torch.float64, device=xyz_dev.device)
+    out[:, 0] = rho
+    ke = torch.zeros_like(rho)
+    for c in range(3):
+        out[:, 1 + c] = rho * vel[c]
+        ke += vel[c] * vel[c]
+    out[:, 4] = p / M.GM1 + 0.5 * rho * ke
+    return out.reshape(-1)
+
+
+def _traffic_from_profiles(kernel_tag):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        data = json.load(open(path))
+        return data.get(kernel_tag, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(target_seconds=12.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from dpgsolver_amd import mesh as M
+
+    mesh = M.config3_mesh(8, 4)
+    u = M.config3_state(mesh)
+    orc = pyoracle.Oracle(mesh, M.PDE_EULER, M.FLUX_LF)
+    orc.rhs(u)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        orc.rhs(u)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= target_seconds or n >= 200:
+            break
+    per = el / n
+    dof = mesh.E * 125 * 5
+    return {"value": dof / per / 1e9, "unit": "GDOF/s", "cores": 1, "kind": "port",
+            "sample": f"oracle dense-operator FP64 residual, 8^3 curved HEX p=4 periodic mesh ({mesh.E} elements), "
+                      f"{n} evaluations in {el:.1f} s on 1 host core"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=64, help="elements per direction (64 = the named workload)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "sharding is a design, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from dpgsolver_amd import _lib, mesh as M
+
+    dev = torch.device("cuda:0")
+    n, p = args.n, 4
+    t_setup = time.perf_counter()
+    mesh = M.config3_mesh(n, p)
+    ctx = _lib.Context(3, p, M.PDE_EULER, M.FLUX_LF)
+    stream = torch.cuda.current_stream(dev)
+    ctx.set_stream(stream.cuda_stream)
+    ctx.upload_mesh(mesh.xyz, mesh.faces)
+    xyz_dev = torch.from_numpy(mesh.xyz).to(dev)
+    u64 = _state_on_device(torch, xyz_dev, M.SEED)
+    del xyz_dev
+    ndof = u64.numel()
+    u32 = torch.empty(ndof, dtype=torch.float32, device=dev)
+    ctx.convert_f64_f32(u64.data_ptr(), u32.data_ptr(), ndof)
+    r64 = torch.empty_like(u64)
+    r32 = torch.empty_like(u32)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t_setup
+    print(f"setup: mesh {mesh.E} elements, {ndof} DOF, {setup_s:.1f} s", file=sys.stderr, flush=True)
+
+    def timed(fn, k):
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        start.record(stream)
+        for _ in range(k):
+            fn()
+        stop.record(stream)
+        torch.cuda.synchronize()
+        return start.elapsed_time(stop) / k   # ms per launch
+
+    f32 = lambda: ctx.compute_rhs_f32(u32.data_ptr(), r32.data_ptr())
+    f64 = lambda: ctx.compute_rhs(u64.data_ptr(), r64.data_ptr())
+    for _ in range(args.warmup):
+        f32()
+        f64()
+    torch.cuda.synchronize()
+    ms32 = timed(f32, args.steps)
+    print(f"fp32 residual: {ms32:.4f} ms", file=sys.stderr, flush=True)
+    ms64 = timed(f64, args.steps)
+    print(f"fp64 residual: {ms64:.4f} ms", file=sys.stderr, flush=True)
+
+    err32 = ((r32.double() - r64).abs().max() / r64.abs().max()).item()
+    cons = (r64.view(mesh.E, 5, -1).sum(dim=(0, 2)).abs().max() / r64.abs().max()).item()
+
+    E, Np, nv, d, Nfc = mesh.E, 125, 5, 3, 25
+    bytes32 = E * (2 * nv * Np * 4 + Np * d * d * 4 + d * Nfc * d * 4)
+    bytes64 = E * (2 * nv * Np * 8 + Np * d * d * 8 + d * Nfc * d * 8)
+    gdof32 = ndof / (ms32 * 1e-3) / 1e9
+    gdof64 = ndof / (ms64 * 1e-3) / 1e9
+    ach32 = bytes32 / (ms32 * 1e-3) / 1e9
+    ach64 = bytes64 / (ms64 * 1e-3) / 1e9
+
+    cpu = None if args.no_cpu else cpu_baseline()
+
+    out = {
+        "metric": "GDOF/s residual eval (FP64 vs FP32-MFMA); time-to-1e-10 steady residual",
+        "value": round(gdof32, 3),
+        "unit": "GDOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms32, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"3-D Euler DG p=4, {n}^3 curved periodic HEX, Lax-Friedrichs, residual eval",
+                   "elements": E, "dof": ndof, "p": p, "num_flux": "Lax-Friedrichs"},
+        "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": _traffic_from_profiles("rhs_fp32")},
+        "fp64": {"value": round(gdof64, 3), "ms_per_step": round(ms64, 5),
+                 "roofline": {"bound": "hbm", "achieved": round(ach64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(ach64 / HBM_PEAK_GBS, 4),
+                              "traffic": _traffic_from_profiles("rhs_fp64")}},
+        "fp32_rel_err_vs_fp64": err32,
+        "fp64_conservation_rel": cons,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

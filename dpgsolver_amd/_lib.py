@@ -1,0 +1,196 @@
+"""ctypes binding of libdgx.so (the HIP C-ABI declared in include/dgx.h).
+
+Pointers passed to the ``*_dev`` methods are HIP device addresses (e.g. ``tensor.data_ptr()``
+of a torch tensor on ``cuda:0``).  The library is required: importing this module without a
+built ``libdgx.so`` raises, and there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdgx.so")
+
+DGX_OK = 0
+_ERRORS = {-1: "DGX_ERR_ARG", -2: "DGX_ERR_HIP", -3: "DGX_ERR_UNSUPPORTED", -4: "DGX_ERR_POSITIVITY",
+           -5: "DGX_ERR_NONFINITE", -6: "DGX_ERR_NOT_CONVERGED", -7: "DGX_ERR_NO_MESH"}
+
+# every symbol include/dgx.h declares
+EXPORTS = [
+    "dgx_version", "dgx_device_count", "dgx_create", "dgx_destroy", "dgx_set_stream", "dgx_synchronize",
+    "dgx_sizes", "dgx_upload_mesh", "dgx_get_geometry", "dgx_compute_rhs", "dgx_compute_rhs_f32",
+    "dgx_compute_rhs_explicit", "dgx_apply_minv", "dgx_enforce_positivity", "dgx_ssp_rk3_step", "dgx_max_abs",
+    "dgx_convert_f64_f32", "dgx_convert_f32_f64", "dgx_assemble_diag_blocks", "dgx_jacobian_vector",
+    "dgx_block_lu_factor", "dgx_block_lu_solve",
+]
+
+
+class DgxError(RuntimeError):
+    def __init__(self, fn: str, code: int):
+        super().__init__(f"{fn} failed: {_ERRORS.get(code, code)}")
+        self.code = code
+
+
+class DgxConfig(C.Structure):
+    _fields_ = [("d", C.c_int), ("p", C.c_int), ("pde", C.c_int), ("num_flux", C.c_int),
+                ("b_adv", C.c_double * 3), ("bc_prim", C.c_double * 5)]
+
+
+class DgxMesh(C.Structure):
+    _fields_ = [("E", C.c_int), ("F", C.c_int), ("xyz", C.POINTER(C.c_double)), ("faces", C.POINTER(C.c_int))]
+
+
+_lib = None
+vp = C.c_void_p
+dp = C.POINTER(C.c_double)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libdgx.so not built ({LIB_PATH}); run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    L.dgx_version.restype = C.c_char_p
+    sig = {
+        "dgx_device_count": [C.POINTER(C.c_int)],
+        "dgx_create": [C.POINTER(vp), C.POINTER(DgxConfig)],
+        "dgx_destroy": [vp],
+        "dgx_set_stream": [vp, vp],
+        "dgx_synchronize": [vp],
+        "dgx_sizes": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_longlong)],
+        "dgx_upload_mesh": [vp, C.POINTER(DgxMesh)],
+        "dgx_get_geometry": [vp, dp, dp, dp],
+        "dgx_compute_rhs": [vp, vp, vp],
+        "dgx_compute_rhs_f32": [vp, vp, vp],
+        "dgx_compute_rhs_explicit": [vp, vp, vp, dp],
+        "dgx_apply_minv": [vp, vp],
+        "dgx_enforce_positivity": [vp, vp],
+        "dgx_ssp_rk3_step": [vp, vp, C.c_double, C.c_int, dp],
+        "dgx_max_abs": [vp, vp, C.c_longlong, dp],
+        "dgx_convert_f64_f32": [vp, vp, vp, C.c_longlong],
+        "dgx_convert_f32_f64": [vp, vp, vp, C.c_longlong],
+        "dgx_assemble_diag_blocks": [vp, vp, C.c_double, vp],
+        "dgx_jacobian_vector": [vp, vp, vp, vp],
+        "dgx_block_lu_factor": [vp, vp, vp, C.c_int, C.c_int],
+        "dgx_block_lu_solve": [vp, vp, vp, vp, C.c_int, C.c_int],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    _lib = L
+    return L
+
+
+def check(fn: str, code: int):
+    if code != DGX_OK:
+        raise DgxError(fn, code)
+
+
+class Context:
+    """One DG discretisation on the device (owns geometry; state buffers are the caller's)."""
+
+    def __init__(self, d: int, p: int, pde: int, num_flux: int, b_adv=(0.0, 0.0, 0.0), bc_prim=None):
+        L = load()
+        cfg = DgxConfig()
+        cfg.d, cfg.p, cfg.pde, cfg.num_flux = d, p, pde, num_flux
+        for k, v in enumerate(b_adv):
+            cfg.b_adv[k] = float(v)
+        if bc_prim is not None:
+            for k, v in enumerate(bc_prim):
+                cfg.bc_prim[k] = float(v)
+        h = vp()
+        check("dgx_create", L.dgx_create(C.byref(h), C.byref(cfg)))
+        self.h = h
+        self.L = L
+        self.d, self.p = d, p
+        Np, nv, Nfc, nd = C.c_int(), C.c_int(), C.c_int(), C.c_longlong()
+        check("dgx_sizes", L.dgx_sizes(h, C.byref(Np), C.byref(nv), C.byref(Nfc), C.byref(nd)))
+        self.Np, self.nv, self.Nfc = Np.value, nv.value, Nfc.value
+        self.E = self.F = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.dgx_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    @property
+    def ndof(self) -> int:
+        return self.E * self.Np * self.nv
+
+    def set_stream(self, stream_ptr: int):
+        check("dgx_set_stream", self.L.dgx_set_stream(self.h, vp(stream_ptr)))
+
+    def synchronize(self):
+        check("dgx_synchronize", self.L.dgx_synchronize(self.h))
+
+    def upload_mesh(self, xyz: np.ndarray, faces: np.ndarray):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        faces = np.ascontiguousarray(faces, dtype=np.int32)
+        m = DgxMesh()
+        m.E, m.F = xyz.shape[0], faces.shape[0]
+        m.xyz = xyz.ctypes.data_as(dp)
+        m.faces = faces.ctypes.data_as(C.POINTER(C.c_int))
+        check("dgx_upload_mesh", self.L.dgx_upload_mesh(self.h, C.byref(m)))
+        self.E, self.F = m.E, m.F
+
+    def geometry(self):
+        met = np.zeros(self.E * self.d * self.d * self.Np)
+        dj = np.zeros(self.E * self.Np)
+        nJ = np.zeros(max(self.F, 1) * self.Nfc * self.d)
+        check("dgx_get_geometry", self.L.dgx_get_geometry(self.h, met.ctypes.data_as(dp), dj.ctypes.data_as(dp),
+                                                          nJ.ctypes.data_as(dp)))
+        return met, dj, nJ[: self.F * self.Nfc * self.d]
+
+    # --- device-pointer entry points
+    def compute_rhs(self, u: int, r: int):
+        check("dgx_compute_rhs", self.L.dgx_compute_rhs(self.h, vp(u), vp(r)))
+
+    def compute_rhs_f32(self, u: int, r: int):
+        check("dgx_compute_rhs_f32", self.L.dgx_compute_rhs_f32(self.h, vp(u), vp(r)))
+
+    def compute_rhs_explicit(self, u: int, r: int) -> float:
+        mx = C.c_double()
+        check("dgx_compute_rhs_explicit", self.L.dgx_compute_rhs_explicit(self.h, vp(u), vp(r), C.byref(mx)))
+        return mx.value
+
+    def apply_minv(self, r: int):
+        check("dgx_apply_minv", self.L.dgx_apply_minv(self.h, vp(r)))
+
+    def enforce_positivity(self, u: int) -> int:
+        return self.L.dgx_enforce_positivity(self.h, vp(u))
+
+    def ssp_rk3_step(self, u: int, dt: float, positivity: bool = True) -> float:
+        mx = C.c_double()
+        check("dgx_ssp_rk3_step", self.L.dgx_ssp_rk3_step(self.h, vp(u), dt, int(positivity), C.byref(mx)))
+        return mx.value
+
+    def max_abs(self, x: int, n: int) -> float:
+        out = C.c_double()
+        check("dgx_max_abs", self.L.dgx_max_abs(self.h, vp(x), n, C.byref(out)))
+        return out.value
+
+    def convert_f64_f32(self, src: int, dst: int, n: int):
+        check("dgx_convert_f64_f32", self.L.dgx_convert_f64_f32(self.h, vp(src), vp(dst), n))
+
+    def convert_f32_f64(self, src: int, dst: int, n: int):
+        check("dgx_convert_f32_f64", self.L.dgx_convert_f32_f64(self.h, vp(src), vp(dst), n))
+
+    def assemble_diag_blocks(self, u: int, mass_scale: float, blocks: int):
+        check("dgx_assemble_diag_blocks", self.L.dgx_assemble_diag_blocks(self.h, vp(u), mass_scale, vp(blocks)))
+
+    def jacobian_vector(self, u: int, v: int, jv: int):
+        check("dgx_jacobian_vector", self.L.dgx_jacobian_vector(self.h, vp(u), vp(v), vp(jv)))
+
+    def block_lu_factor(self, blocks: int, piv: int, E: int, nb: int):
+        check("dgx_block_lu_factor", self.L.dgx_block_lu_factor(self.h, vp(blocks), vp(piv), E, nb))
+
+    def block_lu_solve(self, lu: int, piv: int, x: int, E: int, nb: int):
+        check("dgx_block_lu_solve", self.L.dgx_block_lu_solve(self.h, vp(lu), vp(piv), vp(x), E, nb))

@@ -1,0 +1,336 @@
+// dgx_api.hip -- the C ABI of libdgx (declared in include/dgx.h).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "dgx_internal.h"
+
+struct dgx_ctx {
+	dgx::Ctx c;
+};
+
+using dgx::Ctx;
+
+namespace {
+
+dgx_status hip_status(hipError_t e) { return e == hipSuccess ? DGX_OK : DGX_ERR_HIP; }
+
+#define DGX_TRY(expr)                                  \
+	do {                                               \
+		hipError_t e_ = (expr);                        \
+		if (e_ != hipSuccess) return DGX_ERR_HIP;      \
+	} while (0)
+
+bool supported(int d, int p, int pde)
+{
+	if (d == 2) return p >= 1 && p <= 5;
+	if (d == 3) return pde == DGX_PDE_EULER && p >= 1 && p <= 4;
+	return false;
+}
+
+template <typename T>
+void release(T*& p)
+{
+	if (p) (void)hipFree((void*)p);
+	p = nullptr;
+}
+
+void free_mesh(Ctx& c)
+{
+	release(c.xyz); release(c.faces); release(c.elem_faces); release(c.metrics_vc); release(c.metrics_vc32);
+	release(c.detJ_vc); release(c.detJ_vc32); release(c.nJ_fc); release(c.nJ_fc32); release(c.u0); release(c.rtmp);
+	c.mesh_ready = false;
+}
+
+size_t ndof(const Ctx& c) { return (size_t)c.E * c.Np * c.nv; }
+
+}  // namespace
+
+extern "C" {
+
+const char* dgx_version(void) { return "dgx 0.1 (gfx950)"; }
+
+dgx_status dgx_device_count(int* n)
+{
+	if (!n) return DGX_ERR_ARG;
+	return hip_status(hipGetDeviceCount(n));
+}
+
+dgx_status dgx_create(dgx_ctx** out, const dgx_config* cfg)
+{
+	if (!out || !cfg) return DGX_ERR_ARG;
+	*out = nullptr;
+	if (cfg->pde != DGX_PDE_EULER && cfg->pde != DGX_PDE_ADVECTION) return DGX_ERR_ARG;
+	if (cfg->pde == DGX_PDE_EULER && cfg->num_flux != DGX_FLUX_LF && cfg->num_flux != DGX_FLUX_ROE) return DGX_ERR_ARG;
+	if (cfg->pde == DGX_PDE_ADVECTION && cfg->num_flux != DGX_FLUX_UPWIND) return DGX_ERR_ARG;
+	if (!supported(cfg->d, cfg->p, cfg->pde)) return DGX_ERR_UNSUPPORTED;
+	dgx_ctx* h = new (std::nothrow) dgx_ctx();
+	if (!h) return DGX_ERR_ARG;
+	Ctx& c = h->c;
+	c.cfg = *cfg;
+	c.d = cfg->d;
+	c.p = cfg->p;
+	c.N = cfg->p + 1;
+	c.Np = c.N * c.N * (c.d == 3 ? c.N : 1);
+	c.Nc = c.Np;
+	c.Nfc = (c.d == 3) ? c.N * c.N : c.N;
+	c.nv = (cfg->pde == DGX_PDE_EULER) ? c.d + 2 : 1;
+	if (hipGetDevice(&c.device) != hipSuccess) { delete h; return DGX_ERR_HIP; }
+	dgx::build_ops(c.d, c.p, c.h_ops64, c.h_ops32);
+	if (hipMalloc(&c.ops64, sizeof(dgx::Ops1D<double>)) != hipSuccess ||
+	    hipMalloc(&c.ops32, sizeof(dgx::Ops1D<float>)) != hipSuccess ||
+	    hipMalloc(&c.dmax, sizeof(unsigned long long) * 4) != hipSuccess ||
+	    hipMalloc(&c.dstatus, sizeof(int) * 4) != hipSuccess) {
+		dgx_destroy(h);
+		return DGX_ERR_HIP;
+	}
+	if (hipMemcpy(c.ops64, &c.h_ops64, sizeof(c.h_ops64), hipMemcpyHostToDevice) != hipSuccess ||
+	    hipMemcpy(c.ops32, &c.h_ops32, sizeof(c.h_ops32), hipMemcpyHostToDevice) != hipSuccess) {
+		dgx_destroy(h);
+		return DGX_ERR_HIP;
+	}
+	*out = h;
+	return DGX_OK;
+}
+
+dgx_status dgx_destroy(dgx_ctx* h)
+{
+	if (!h) return DGX_OK;
+	Ctx& c = h->c;
+	free_mesh(c);
+	release(c.ops64); release(c.ops32); release(c.dmax); release(c.dstatus);
+	release(c.blocks); release(c.lu); release(c.piv);
+	delete h;
+	return DGX_OK;
+}
+
+dgx_status dgx_set_stream(dgx_ctx* h, void* stream)
+{
+	if (!h) return DGX_ERR_ARG;
+	h->c.stream = (hipStream_t)stream;
+	return DGX_OK;
+}
+
+dgx_status dgx_synchronize(dgx_ctx* h)
+{
+	if (!h) return DGX_ERR_ARG;
+	return hip_status(hipStreamSynchronize(h->c.stream));
+}
+
+dgx_status dgx_sizes(const dgx_ctx* h, int* Np, int* nv, int* Nfc, long long* nd)
+{
+	if (!h) return DGX_ERR_ARG;
+	if (Np) *Np = h->c.Np;
+	if (nv) *nv = h->c.nv;
+	if (Nfc) *Nfc = h->c.Nfc;
+	if (nd) *nd = (long long)ndof(h->c);
+	return DGX_OK;
+}
+
+dgx_status dgx_upload_mesh(dgx_ctx* h, const dgx_mesh* m)
+{
+	if (!h || !m || m->E <= 0 || m->F < 0 || !m->xyz || (m->F > 0 && !m->faces)) return DGX_ERR_ARG;
+	Ctx& c = h->c;
+	const int d = c.d, NFACE = 2 * d;
+	// validate the face table and build the per-element face records
+	std::vector<dgx::ElemFace> ef((size_t)m->E * NFACE, dgx::ElemFace{ -1, -1, -1, 0 });
+	for (int f = 0; f < m->F; ++f) {
+		const int* r = m->faces + 6 * f;
+		const int eL = r[0], fL = r[1], eR = r[2], fR = r[3], ord = r[4], bc = r[5];
+		if (eL < 0 || eL >= m->E || fL < 0 || fL >= NFACE || ord < 0 || ord > 7) return DGX_ERR_ARG;
+		if (bc == DGX_BC_INTERIOR) {
+			if (eR < 0 || eR >= m->E || fR < 0 || fR >= NFACE) return DGX_ERR_ARG;
+		} else {
+			if (c.cfg.pde != DGX_PDE_EULER || (bc != DGX_BC_RIEMANN && bc != DGX_BC_SLIPWALL)) return DGX_ERR_ARG;
+		}
+		dgx::ElemFace& L = ef[(size_t)eL * NFACE + fL];
+		if (L.f >= 0) return DGX_ERR_ARG;  // local face used twice
+		L = dgx::ElemFace{ f, bc == DGX_BC_INTERIOR ? eR : -1, (bc == DGX_BC_INTERIOR ? fR : 0) | (0 << 4) | (ord << 8) | (bc << 12), 0 };
+		if (bc == DGX_BC_INTERIOR) {
+			dgx::ElemFace& R = ef[(size_t)eR * NFACE + fR];
+			if (R.f >= 0) return DGX_ERR_ARG;
+			R = dgx::ElemFace{ f, eL, fL | (1 << 4) | (ord << 8), 0 };
+		}
+	}
+	for (const auto& x : ef)
+		if (x.f < 0) return DGX_ERR_ARG;  // open mesh: every local face needs a face record
+	free_mesh(c);
+	c.E = m->E;
+	c.F = m->F;
+	const size_t nx = (size_t)c.E * d * c.Np, nm = (size_t)c.E * d * d * c.Nc, nj = (size_t)c.E * c.Nc,
+	             nn = (size_t)(c.F > 0 ? c.F : 1) * c.Nfc * d;
+	DGX_TRY(hipMalloc(&c.xyz, sizeof(double) * nx));
+	DGX_TRY(hipMalloc(&c.faces, sizeof(int) * 6 * (size_t)(c.F > 0 ? c.F : 1)));
+	DGX_TRY(hipMalloc(&c.elem_faces, sizeof(dgx::ElemFace) * ef.size()));
+	DGX_TRY(hipMalloc(&c.metrics_vc, sizeof(double) * nm));
+	DGX_TRY(hipMalloc(&c.metrics_vc32, sizeof(float) * nm));
+	DGX_TRY(hipMalloc(&c.detJ_vc, sizeof(double) * nj));
+	DGX_TRY(hipMalloc(&c.detJ_vc32, sizeof(float) * nj));
+	DGX_TRY(hipMalloc(&c.nJ_fc, sizeof(double) * nn));
+	DGX_TRY(hipMalloc(&c.nJ_fc32, sizeof(float) * nn));
+	DGX_TRY(hipMemcpy(c.xyz, m->xyz, sizeof(double) * nx, hipMemcpyHostToDevice));
+	if (c.F > 0) DGX_TRY(hipMemcpy(c.faces, m->faces, sizeof(int) * 6 * (size_t)c.F, hipMemcpyHostToDevice));
+	DGX_TRY(hipMemcpy(c.elem_faces, ef.data(), sizeof(dgx::ElemFace) * ef.size(), hipMemcpyHostToDevice));
+	DGX_TRY(dgx::launch_geometry(c));
+	DGX_TRY(dgx::launch_convert(c.metrics_vc, c.metrics_vc32, nm, c.stream));
+	DGX_TRY(dgx::launch_convert(c.detJ_vc, c.detJ_vc32, nj, c.stream));
+	DGX_TRY(dgx::launch_convert(c.nJ_fc, c.nJ_fc32, nn, c.stream));
+	DGX_TRY(hipStreamSynchronize(c.stream));
+	c.mesh_ready = true;
+	return DGX_OK;
+}
+
+dgx_status dgx_get_geometry(dgx_ctx* h, double* met, double* dj, double* nJ)
+{
+	if (!h) return DGX_ERR_ARG;
+	Ctx& c = h->c;
+	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	DGX_TRY(hipStreamSynchronize(c.stream));
+	if (met) DGX_TRY(hipMemcpy(met, c.metrics_vc, sizeof(double) * (size_t)c.E * c.d * c.d * c.Nc, hipMemcpyDeviceToHost));
+	if (dj) DGX_TRY(hipMemcpy(dj, c.detJ_vc, sizeof(double) * (size_t)c.E * c.Nc, hipMemcpyDeviceToHost));
+	if (nJ && c.F > 0) DGX_TRY(hipMemcpy(nJ, c.nJ_fc, sizeof(double) * (size_t)c.F * c.Nfc * c.d, hipMemcpyDeviceToHost));
+	return DGX_OK;
+}
+
+dgx_status dgx_compute_rhs(dgx_ctx* h, const double* u, double* r)
+{
+	if (!h || !u || !r) return DGX_ERR_ARG;
+	if (!h->c.mesh_ready) return DGX_ERR_NO_MESH;
+	return hip_status(dgx::launch_rhs64(h->c, u, r, dgx::MODE_RHS, nullptr));
+}
+
+dgx_status dgx_compute_rhs_f32(dgx_ctx* h, const float* u, float* r)
+{
+	if (!h || !u || !r) return DGX_ERR_ARG;
+	if (!h->c.mesh_ready) return DGX_ERR_NO_MESH;
+	return hip_status(dgx::launch_rhs32(h->c, u, r));
+}
+
+static dgx_status read_max(Ctx& c, double* out)
+{
+	unsigned long long bits = 0;
+	DGX_TRY(hipMemcpyAsync(&bits, c.dmax, sizeof(bits), hipMemcpyDeviceToHost, c.stream));
+	DGX_TRY(hipStreamSynchronize(c.stream));
+	double v;
+	std::memcpy(&v, &bits, sizeof(v));
+	*out = v;
+	return (v != v || v > 1.7e308) ? DGX_ERR_NONFINITE : DGX_OK;
+}
+
+dgx_status dgx_compute_rhs_explicit(dgx_ctx* h, const double* u, double* r, double* max_abs_r)
+{
+	if (!h || !u || !r) return DGX_ERR_ARG;
+	Ctx& c = h->c;
+	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	DGX_TRY(dgx::launch_fill_u64(c.stream, c.dmax, 1, 0ull));
+	DGX_TRY(dgx::launch_rhs64(c, u, r, dgx::MODE_RHS_MINV, c.dmax));
+	if (max_abs_r) return read_max(c, max_abs_r);
+	return DGX_OK;
+}
+
+dgx_status dgx_apply_minv(dgx_ctx* h, double* r)
+{
+	if (!h || !r) return DGX_ERR_ARG;
+	if (!h->c.mesh_ready) return DGX_ERR_NO_MESH;
+	return hip_status(dgx::launch_minv(h->c, r));
+}
+
+dgx_status dgx_enforce_positivity(dgx_ctx* h, double* u)
+{
+	if (!h || !u) return DGX_ERR_ARG;
+	Ctx& c = h->c;
+	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (c.cfg.pde != DGX_PDE_EULER) return DGX_OK;  // test_case.c:63-77: Euler/NS only
+	DGX_TRY(dgx::launch_fill_i32(c.stream, c.dstatus, 1, 0));
+	DGX_TRY(dgx::launch_positivity(c, u, c.dstatus));
+	int st = 0;
+	DGX_TRY(hipMemcpyAsync(&st, c.dstatus, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+	DGX_TRY(hipStreamSynchronize(c.stream));
+	return st ? DGX_ERR_POSITIVITY : DGX_OK;
+}
+
+dgx_status dgx_ssp_rk3_step(dgx_ctx* h, double* u, double dt, int positivity, double* max_abs_r)
+{
+	if (!h || !u) return DGX_ERR_ARG;
+	Ctx& c = h->c;
+	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	const size_t n = ndof(c);
+	if (!c.u0) DGX_TRY(hipMalloc(&c.u0, sizeof(double) * n));
+	if (!c.rtmp) DGX_TRY(hipMalloc(&c.rtmp, sizeof(double) * n));
+	const bool pos = positivity && c.cfg.pde == DGX_PDE_EULER;
+	if (pos) DGX_TRY(dgx::launch_fill_i32(c.stream, c.dstatus, 1, 0));
+	for (int rk = 0; rk < 3; ++rk) {
+		DGX_TRY(dgx::launch_fill_u64(c.stream, c.dmax, 1, 0ull));
+		DGX_TRY(dgx::launch_rhs64(c, u, c.rtmp, dgx::MODE_RHS_MINV, c.dmax));
+		DGX_TRY(dgx::launch_rk_stage(c, u, c.rtmp, dt, rk));
+		if (pos) DGX_TRY(dgx::launch_positivity(c, u, c.dstatus));
+	}
+	dgx_status st = DGX_OK;
+	if (max_abs_r) st = read_max(c, max_abs_r);
+	if (pos) {
+		int bad = 0;
+		DGX_TRY(hipMemcpyAsync(&bad, c.dstatus, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+		DGX_TRY(hipStreamSynchronize(c.stream));
+		if (bad) return DGX_ERR_POSITIVITY;
+	}
+	return st;
+}
+
+dgx_status dgx_max_abs(dgx_ctx* h, const double* x, long long n, double* out)
+{
+	if (!h || !x || !out || n < 0) return DGX_ERR_ARG;
+	Ctx& c = h->c;
+	DGX_TRY(dgx::launch_fill_u64(c.stream, c.dmax, 1, 0ull));
+	DGX_TRY(dgx::launch_maxabs(c, x, (size_t)n, c.dmax));
+	return read_max(c, out);
+}
+
+dgx_status dgx_convert_f64_f32(dgx_ctx* h, const double* src, float* dst, long long n)
+{
+	if (!h || !src || !dst || n < 0) return DGX_ERR_ARG;
+	return hip_status(dgx::launch_convert(src, dst, (size_t)n, h->c.stream));
+}
+
+dgx_status dgx_convert_f32_f64(dgx_ctx* h, const float* src, double* dst, long long n)
+{
+	if (!h || !src || !dst || n < 0) return DGX_ERR_ARG;
+	return hip_status(dgx::launch_convert_back(src, dst, (size_t)n, h->c.stream));
+}
+
+dgx_status dgx_assemble_diag_blocks(dgx_ctx* h, const double* u, double mass_scale, double* blocks)
+{
+	if (!h || !u || !blocks) return DGX_ERR_ARG;
+	Ctx& c = h->c;
+	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
+	const int nb = c.Np * c.nv;
+	for (int col = 0; col < nb; ++col) DGX_TRY(dgx::launch_diag_col(c, u, blocks, col));
+	if (mass_scale != 0.0) DGX_TRY(dgx::launch_sub_mass(c, blocks, mass_scale));
+	return DGX_OK;
+}
+
+dgx_status dgx_jacobian_vector(dgx_ctx* h, const double* u, const double* v, double* jv)
+{
+	if (!h || !u || !v || !jv) return DGX_ERR_ARG;
+	Ctx& c = h->c;
+	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
+	return hip_status(dgx::launch_jvp(c, u, v, jv));
+}
+
+dgx_status dgx_block_lu_factor(dgx_ctx* h, double* blocks, int* piv, int E, int nb)
+{
+	if (!h || !blocks || !piv || E < 0 || nb <= 0 || nb > 90) return DGX_ERR_ARG;  // nb^2 doubles in LDS
+	return hip_status(dgx::launch_block_lu(h->c.stream, blocks, piv, E, nb));
+}
+
+dgx_status dgx_block_lu_solve(dgx_ctx* h, const double* lu, const int* piv, double* x, int E, int nb)
+{
+	if (!h || !lu || !piv || !x || E < 0 || nb <= 0 || nb > 90) return DGX_ERR_ARG;
+	return hip_status(dgx::launch_block_lu_solve(h->c.stream, lu, piv, x, E, nb));
+}
+
+}  // extern "C"

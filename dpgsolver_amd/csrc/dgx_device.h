@@ -1,0 +1,254 @@
+// dgx_device.h -- device-side scalar types and physics shared by the gfx950 kernels.
+//
+// Physics restated from the reference (paths relative to the DPGSolver source tree):
+//   Euler flux                 src/simulation/test_case/flux/flux_euler_T.c:71-143, 259-297
+//   Lax-Friedrichs             src/simulation/solvers/numerical_flux/numerical_flux_euler_T.c:60-327
+//   Roe-Pike + entropy fix     numerical_flux_euler_T.c:470-778 (Qu et al. 2015 eq. 35)
+//   upwind                     numerical_flux_advection_T.c:39-124
+//   Riemann / slip-wall states src/simulation/test_case/boundary/boundary_euler_T.c:99-180, 494-530
+// The arithmetic order follows the reference's expressions so that the FP64 path agrees
+// with the oracle to rounding.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "dgx_internal.h"
+
+namespace dgx {
+
+// Forward-mode dual number: exact directional derivatives of the residual
+// (the analytic Jacobians of the reference, flux_euler_T.c:298-...,
+// numerical_flux_euler_T.c:329-468, 780-1648, follow from the same chain rule).
+struct Dual {
+	double v, d;
+	Dual() = default;
+	__host__ __device__ Dual(double a) : v(a), d(0.0) {}
+	__host__ __device__ Dual(double a, double b) : v(a), d(b) {}
+};
+__device__ inline Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.d + b.d); }
+__device__ inline Dual operator-(Dual a, Dual b) { return Dual(a.v - b.v, a.d - b.d); }
+__device__ inline Dual operator-(Dual a) { return Dual(-a.v, -a.d); }
+__device__ inline Dual operator*(Dual a, Dual b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+__device__ inline Dual operator/(Dual a, Dual b)
+{
+	const double q = a.v / b.v;
+	return Dual(q, (a.d - q * b.d) / b.v);
+}
+__device__ inline Dual& operator+=(Dual& a, Dual b) { a = a + b; return a; }
+__device__ inline Dual& operator-=(Dual& a, Dual b) { a = a - b; return a; }
+__device__ inline Dual& operator*=(Dual& a, Dual b) { a = a * b; return a; }
+
+__device__ inline double rv(double a) { return a; }
+__device__ inline float rv(float a) { return a; }
+__device__ inline double rv(Dual a) { return a.v; }
+
+__device__ inline double tsqrt(double a) { return sqrt(a); }
+__device__ inline float tsqrt(float a) { return sqrtf(a); }
+__device__ inline Dual tsqrt(Dual a)
+{
+	const double s = sqrt(a.v);
+	return Dual(s, a.d / (2.0 * s));
+}
+__device__ inline double tpow(double a, double b) { return pow(a, b); }
+__device__ inline float tpow(float a, double b) { return powf(a, (float)b); }
+__device__ inline Dual tpow(Dual a, double b)
+{
+	const double pv = pow(a.v, b);
+	return Dual(pv, b * pow(a.v, b - 1.0) * a.d);
+}
+// drop the derivative part (neighbour traces held fixed for diagonal blocks)
+__device__ inline double strip(double a) { return a; }
+__device__ inline float strip(float a) { return a; }
+__device__ inline Dual strip(Dual a) { return Dual(a.v, 0.0); }
+
+template <typename T> struct RealOf { using type = T; };
+template <> struct RealOf<Dual> { using type = double; };
+
+constexpr __host__ __device__ int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
+
+// ------------------------------------------------------------------------------------------ fluxes
+// f[k + D*eq] (flux_T.h:18-19 layout (node, dim, eq))
+template <typename T, int D>
+__device__ inline void euler_flux(const T* s, T* f)
+{
+	const T rho = s[0], E = s[D + 1];
+	const T rho_inv = T(1.0) / rho;
+	T uvw[D];
+#pragma unroll
+	for (int k = 0; k < D; ++k) uvw[k] = rho_inv * s[1 + k];
+	T V2 = uvw[0] * uvw[0];
+#pragma unroll
+	for (int k = 1; k < D; ++k) V2 += uvw[k] * uvw[k];
+	const T p = T(kGM1) * (E - T(0.5) * rho * V2);
+#pragma unroll
+	for (int k = 0; k < D; ++k) f[k] = s[1 + k];
+#pragma unroll
+	for (int i = 0; i < D; ++i)
+#pragma unroll
+		for (int k = 0; k < D; ++k) f[k + D * (1 + i)] = (i == k) ? s[1 + i] * uvw[k] + p : s[1 + i] * uvw[k];
+#pragma unroll
+	for (int k = 0; k < D; ++k) f[k + D * (D + 1)] = (E + p) * uvw[k];
+}
+
+template <typename T, int D>
+__device__ inline void numflux_lf(const T* WL, const T* WR, const T* n, T* nnf)
+{
+	constexpr int NV = D + 2;
+	T VL, VR, cL, cR;
+	{
+		const T rho_inv = T(1.0) / WL[0];
+		T u0 = WL[1] * rho_inv;
+		T V2 = u0 * u0;
+#pragma unroll
+		for (int k = 1; k < D; ++k) { const T u = WL[1 + k] * rho_inv; V2 += u * u; }
+		VL = tsqrt(V2);
+		const T p = T(kGM1) * (WL[D + 1] - T(0.5) * WL[0] * V2);
+		cL = tsqrt(T(kGamma) * p / WL[0]);
+	}
+	{
+		const T rho_inv = T(1.0) / WR[0];
+		T u0 = WR[1] * rho_inv;
+		T V2 = u0 * u0;
+#pragma unroll
+		for (int k = 1; k < D; ++k) { const T u = WR[1 + k] * rho_inv; V2 += u * u; }
+		VR = tsqrt(V2);
+		const T p = T(kGM1) * (WR[D + 1] - T(0.5) * WR[0] * V2);
+		cR = tsqrt(T(kGamma) * p / WR[0]);
+	}
+	const T maxlL = VL + cL, maxlR = VR + cR;
+	const T maxV = (rv(maxlL) > rv(maxlR)) ? maxlL : maxlR;
+	T FL[D * NV], FR[D * NV];
+	euler_flux<T, D>(WL, FL);
+	euler_flux<T, D>(WR, FR);
+#pragma unroll
+	for (int eq = 0; eq < NV; ++eq) {
+		T acc = n[0] * (FL[0 + D * eq] + FR[0 + D * eq]);
+#pragma unroll
+		for (int k = 1; k < D; ++k) acc += n[k] * (FL[k + D * eq] + FR[k + D * eq]);
+		nnf[eq] = T(0.5) * (acc + maxV * (WL[eq] - WR[eq]));
+	}
+}
+
+template <typename T>
+__device__ inline T min_abs_real(T a, T b)
+{
+	T c = (fabs(rv(a)) < fabs(rv(b))) ? a : b;
+	if (rv(c) < 0.0) c = T(-1.0) * c;
+	return c;
+}
+template <typename T>
+__device__ inline T max_abs_real(T a, T b)
+{
+	T c = (fabs(rv(a)) > fabs(rv(b))) ? a : b;
+	if (rv(c) < 0.0) c = T(-1.0) * c;
+	return c;
+}
+
+template <typename T, int D>
+__device__ inline void numflux_roe(const T* WL, const T* WR, const T* n, T* nnf)
+{
+	const T n1 = n[0], n2 = (D > 1) ? n[1] : T(0.0), n3 = (D > 2) ? n[2] : T(0.0);
+	const T rhoL = WL[0], rhouL = WL[1], rhovL = (D > 1) ? WL[2] : T(0.0), rhowL = (D > 2) ? WL[3] : T(0.0);
+	const T EL = WL[D + 1];
+	const T rhoR = WR[0], rhouR = WR[1], rhovR = (D > 1) ? WR[2] : T(0.0), rhowR = (D > 2) ? WR[3] : T(0.0);
+	const T ER = WR[D + 1];
+	const T rhoL_inv = T(1.0) / rhoL, rhoR_inv = T(1.0) / rhoR;
+	const T uL = rhouL * rhoL_inv, vL = rhovL * rhoL_inv, wL = rhowL * rhoL_inv;
+	const T uR = rhouR * rhoR_inv, vR = rhovR * rhoR_inv, wR = rhowR * rhoR_inv;
+	T V2L = uL * uL, V2R = uR * uR, VnL = n1 * uL, VnR = n1 * uR;
+	if (D > 1) { V2L += vL * vL; V2R += vR * vR; VnL += n2 * vL; VnR += n2 * vR; }
+	if (D > 2) { V2L += wL * wL; V2R += wR * wR; VnL += n3 * wL; VnR += n3 * wR; }
+	const T pL = T(kGM1) * (EL - T(0.5) * rhoL * V2L), pR = T(kGM1) * (ER - T(0.5) * rhoR * V2R);
+	const T HL = (EL + pL) * rhoL_inv, HR = (ER + pR) * rhoR_inv;
+
+	const T r = tsqrt(rhoR / rhoL), rP1 = r + T(1.0);
+	const T rho = r * rhoL;
+	const T u = (r * uR + uL) / rP1, v = (r * vR + vL) / rP1, w = (r * wR + wL) / rP1;
+	const T H = (r * HR + HL) / rP1;
+	T Vn = n1 * u, V2 = u * u;
+	if (D > 1) { Vn += n2 * v; V2 += v * v; }
+	if (D > 2) { Vn += n3 * w; V2 += w * w; }
+	const T c2 = T(kGM1) * (H - T(0.5) * V2), c = tsqrt(c2);
+
+	const T l1 = min_abs_real(VnL - c, Vn - c);
+	const T l5 = max_abs_real(VnR + c, Vn + c);
+	const T l234 = (rv(Vn) > 0.0) ? Vn : T(-1.0) * Vn;
+
+	const T drho = rhoR - rhoL;
+	const T drhou = rhoR * uR - rhoL * uL, drhov = rhoR * vR - rhoL * vL, drhow = rhoR * wR - rhoL * wL;
+	const T dE = ER - EL, dp = pR - pL, dVn = VnR - VnL;
+	const T lc1 = T(0.5) * (l5 + l1) - l234, lc2 = T(0.5) * (l5 - l1);
+	const T disInter1 = lc1 * dp / (c * c) + lc2 * rho * dVn / c;
+	const T disInter2 = lc1 * rho * dVn + lc2 * dp / c;
+
+	const T rhoVnL = rhoL * VnL, rhoVnR = rhoR * VnR, pLR = pL + pR;
+	int k = 0;
+	nnf[k++] = T(0.5) * ((rhoVnL + rhoVnR) - (l234 * drho + disInter1));
+	nnf[k++] = T(0.5) * ((rhoVnL * uL + rhoVnR * uR + n1 * pLR) - (l234 * drhou + disInter1 * u + disInter2 * n1));
+	if (D > 1)
+		nnf[k++] = T(0.5) * ((rhoVnL * vL + rhoVnR * vR + n2 * pLR) - (l234 * drhov + disInter1 * v + disInter2 * n2));
+	if (D > 2)
+		nnf[k++] = T(0.5) * ((rhoVnL * wL + rhoVnR * wR + n3 * pLR) - (l234 * drhow + disInter1 * w + disInter2 * n3));
+	nnf[k++] = T(0.5) * ((VnL * (EL + pL) + VnR * (ER + pR)) - (l234 * dE + disInter1 * H + disInter2 * Vn));
+}
+
+// boundary state, n is the unit normal (primitive free stream in bc_prim)
+template <typename T, int D>
+__device__ inline void bc_state(int bc, const T* sL, const T* n, const double* bc_prim, T* sR)
+{
+	if (bc == 2) {  // slip wall
+		T rVn = n[0] * sL[1];
+#pragma unroll
+		for (int k = 1; k < D; ++k) rVn += n[k] * sL[1 + k];
+		sR[0] = sL[0];
+		sR[D + 1] = sL[D + 1];
+#pragma unroll
+		for (int k = 0; k < D; ++k) sR[1 + k] = sL[1 + k] - T(2.0) * rVn * n[k];
+		return;
+	}
+	const T rho_l = sL[0];
+	T uvw_l[D];
+	T V2l = T(0.0);
+#pragma unroll
+	for (int k = 0; k < D; ++k) { uvw_l[k] = sL[1 + k] / rho_l; V2l += uvw_l[k] * uvw_l[k]; }
+	const T p_l = T(kGM1) * (sL[D + 1] - T(0.5) * rho_l * V2l);
+	const T rho_r = T(bc_prim[0]), p_r = T(bc_prim[D + 1]);
+	T uvw_r[D];
+#pragma unroll
+	for (int k = 0; k < D; ++k) uvw_r[k] = T(bc_prim[1 + k]);
+	const T c_l = tsqrt(T(kGamma) * p_l / rho_l);
+	const T c_r = tsqrt(T(kGamma) * p_r / rho_r);
+	T Vn_l = T(0.0), Vn_r = T(0.0);
+#pragma unroll
+	for (int k = 0; k < D; ++k) { Vn_l += n[k] * uvw_l[k]; Vn_r += n[k] * uvw_r[k]; }
+	const T R_l = Vn_l + T(2.0 / kGM1) * c_l;
+	const T R_r = Vn_r - T(2.0 / kGM1) * c_r;
+	const T Vn = T(0.5) * (R_l + R_r), cc = T(0.25 * kGM1) * (R_l - R_r);
+	T rho, p, uvw[D];
+	if (fabs(rv(Vn)) >= fabs(rv(cc))) {
+		if (rv(Vn) < 0.0) { rho = rho_r; for (int k = 0; k < D; ++k) uvw[k] = uvw_r[k]; p = p_r; }
+		else              { rho = rho_l; for (int k = 0; k < D; ++k) uvw[k] = uvw_l[k]; p = p_l; }
+	} else {
+		T uvw_t[D];
+		if (rv(Vn) < 0.0) {
+			const T s_r = p_r / tpow(rho_r, kGamma);
+			rho = tpow(T(1.0 / kGamma) * cc * cc / s_r, 1.0 / kGM1);
+			for (int k = 0; k < D; ++k) uvw_t[k] = uvw_r[k] - Vn_r * n[k];
+		} else {
+			const T s_l = p_l / tpow(rho_l, kGamma);
+			rho = tpow(T(1.0 / kGamma) * cc * cc / s_l, 1.0 / kGM1);
+			for (int k = 0; k < D; ++k) uvw_t[k] = uvw_l[k] - Vn_l * n[k];
+		}
+		for (int k = 0; k < D; ++k) uvw[k] = Vn * n[k] + uvw_t[k];
+		p = T(1.0 / kGamma) * cc * cc * rho;
+	}
+	T V2 = T(0.0);
+#pragma unroll
+	for (int k = 0; k < D; ++k) V2 += uvw[k] * uvw[k];
+	sR[0] = rho;
+#pragma unroll
+	for (int k = 0; k < D; ++k) sR[1 + k] = rho * uvw[k];
+	sR[D + 1] = p / T(kGM1) + T(0.5) * rho * V2;
+}
+
+}  // namespace dgx

@@ -1,0 +1,112 @@
+// dgx_internal.h -- shared definitions of the MI355X DG residual library (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/dgx.h"
+
+namespace dgx {
+
+constexpr int kMaxN = 8;          // p <= 7
+constexpr int kMaxNF = kMaxN * kMaxN;
+
+constexpr double kGamma = 1.4;    // definitions_physics.h:23-25
+constexpr double kGM1 = 0.4;
+constexpr double kEps = 1.0e-15;  // definitions_tol.h:23
+constexpr double kEpsPhys = 1.0e-13;
+
+// 1-D operator tables for one polynomial order, row-major [out][in].
+template <typename T>
+struct Ops1D {
+	T I[kMaxN * kMaxN];      // GLL -> GL interpolation
+	T IT[kMaxN * kMaxN];     // transpose of I (test-function side)
+	T DhT[kMaxN * kMaxN];    // transpose of the GL-collocated derivative
+	T Iinv[kMaxN * kMaxN];   // inverse of I (GL -> GLL)
+	T Dg[kMaxN * kMaxN];     // GLL-collocated derivative (metric nodes)
+	T B[kMaxN * kMaxN];      // Bernstein basis at the GLL nodes (Bezier -> nodal)
+	T Binv[kMaxN * kMaxN];   // nodal -> Bezier
+	T Ext[2][kMaxN];         // GLL basis at r = -1 / +1 (unit vectors)
+	T Lift[2][kMaxN];        // I^{-T} e_side : face lift expressed at the GL volume nodes
+	T w[kMaxN];              // GL weights
+	T wgll[kMaxN];           // GLL weights
+	int perm[8][kMaxNF];     // face orientation permutations
+};
+
+// Per-element face record: f, neighbour element (-1 on a boundary), and packed
+// (neighbour local face | side << 4 | ord << 8 | bc << 12).
+struct ElemFace {
+	int f;
+	int en;
+	int packed;
+	int pad;
+};
+
+struct Ctx {
+	dgx_config cfg{};
+	int d = 0, p = 0, N = 0, Np = 0, Nc = 0, Nfc = 0, nv = 0;
+	int E = 0, F = 0;
+	hipStream_t stream = nullptr;
+	int device = 0;
+	bool mesh_ready = false;
+	double* xyz = nullptr;          // E*d*Np
+	int* faces = nullptr;           // F*6
+	ElemFace* elem_faces = nullptr; // E*2d
+	double* metrics_vc = nullptr;   // E*d*d*Nc  (m[(i + d*j)*Nc + n] = J dxi_j/dx_i)
+	float* metrics_vc32 = nullptr;
+	double* detJ_vc = nullptr;      // E*Nc
+	float* detJ_vc32 = nullptr;
+	double* nJ_fc = nullptr;        // F*Nfc*d   (unit normal * face Jacobian, left element)
+	float* nJ_fc32 = nullptr;
+	Ops1D<double>* ops64 = nullptr;
+	Ops1D<float>* ops32 = nullptr;
+	Ops1D<double> h_ops64{};
+	Ops1D<float> h_ops32{};
+	double* u0 = nullptr;           // RK stage storage (E*Np*nv)
+	double* rtmp = nullptr;         // residual scratch (E*Np*nv)
+	unsigned long long* dmax = nullptr;  // device reduction slot(s)
+	int* dstatus = nullptr;
+	double* rk_uout = nullptr;
+	// implicit path
+	double* blocks = nullptr;       // E*(Np*nv)^2 diagonal blocks
+	double* lu = nullptr;           // factored blocks
+	int* piv = nullptr;
+	size_t nblk = 0;
+	std::vector<double> h_tmp;
+};
+
+// host-side helpers implemented in dgx_ops.cpp
+void build_ops(int d, int p, Ops1D<double>& o64, Ops1D<float>& o32);
+void gauss_legendre(int n, double* x, double* w);
+void gauss_lobatto(int n, double* x, double* w);
+
+// launchers (each returns hipError_t)
+hipError_t launch_geometry(Ctx& c);
+hipError_t launch_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned long long* dmax);
+hipError_t launch_rhs32(Ctx& c, const float* u, float* r);
+hipError_t launch_minv(Ctx& c, double* r);
+hipError_t launch_positivity(Ctx& c, double* u, int* status);
+hipError_t launch_maxabs(Ctx& c, const double* r, size_t n, unsigned long long* dmax);
+hipError_t launch_convert(const double* src, float* dst, size_t n, hipStream_t s);
+hipError_t launch_convert_back(const float* src, double* dst, size_t n, hipStream_t s);
+hipError_t launch_rk_stage(Ctx& c, double* u, const double* r, double dt, int stage);
+hipError_t launch_sub_mass(Ctx& c, double* blocks, double scale);
+hipError_t launch_block_lu(hipStream_t s, double* blocks, int* piv, int E, int nb);
+hipError_t launch_block_lu_solve(hipStream_t s, const double* lu, const int* piv, double* x, int E, int nb);
+hipError_t launch_diag_col(Ctx& c, const double* u, double* blocks, int col);
+hipError_t launch_jvp(Ctx& c, const double* u, const double* v, double* jv);
+hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, int n, unsigned long long v);
+hipError_t launch_fill_i32(hipStream_t s, int* p, int n, int v);
+
+// residual modes
+enum { MODE_RHS = 0, MODE_RHS_MINV = 1 };
+
+}  // namespace dgx
+
+#define DGX_HIP_CHECK(expr)                                  \
+	do {                                                     \
+		hipError_t e_ = (expr);                              \
+		if (e_ != hipSuccess) return e_;                     \
+	} while (0)

@@ -1,0 +1,53 @@
+// k_dual.hip -- forward-mode (dual number) instantiations of the fused residual kernel:
+// exact Jacobian-vector products and columns of the element diagonal blocks
+// (the quantities constructor_lhs_v_1_T / constructor_lhs_f_1_T assemble,
+//  compute_volume_rlhs_T.c:179-220, compute_face_rlhs_T.c:163-220).
+#include "k_residual.h"
+
+namespace dgx {
+
+template <int MODE>
+static hipError_t dispatch_dual(Ctx& c, const RhsArgs<Dual, double>& a)
+{
+	const int d = c.d, p = c.p, pde = c.cfg.pde, flux = c.cfg.num_flux;
+#define DGX_DCASE(NN)                                                                                    \
+	if (d == 2 && p + 1 == NN) {                                                                         \
+		if (pde == DGX_PDE_ADVECTION) return run_rhs<Dual, double, 2, NN, DGX_PDE_ADVECTION, DGX_FLUX_UPWIND, MODE>(a, c.stream); \
+		if (flux == DGX_FLUX_LF) return run_rhs<Dual, double, 2, NN, DGX_PDE_EULER, DGX_FLUX_LF, MODE>(a, c.stream);            \
+		return run_rhs<Dual, double, 2, NN, DGX_PDE_EULER, DGX_FLUX_ROE, MODE>(a, c.stream);                                    \
+	}
+	DGX_DCASE(2) DGX_DCASE(3) DGX_DCASE(4) DGX_DCASE(5)
+#undef DGX_DCASE
+	return hipErrorInvalidValue;
+}
+
+static RhsArgs<Dual, double> make_dual(Ctx& c, const double* u, const double* v, double* out, int col)
+{
+	RhsArgs<Dual, double> a{};
+	a.u = u;
+	a.v = v;
+	a.out = out;
+	a.met = c.metrics_vc;
+	a.detJ = c.detJ_vc;
+	a.nJ = c.nJ_fc;
+	a.ef = c.elem_faces;
+	a.ops = c.ops64;
+	for (int k = 0; k < 5; ++k) a.bc_prim[k] = c.cfg.bc_prim[k];
+	for (int k = 0; k < 3; ++k) a.b_adv[k] = c.cfg.b_adv[k];
+	a.E = c.E;
+	a.col = col;
+	a.dmax = nullptr;
+	return a;
+}
+
+hipError_t launch_jvp(Ctx& c, const double* u, const double* v, double* jv)
+{
+	return dispatch_dual<MODE_JVP>(c, make_dual(c, u, v, jv, 0));
+}
+
+hipError_t launch_diag_col(Ctx& c, const double* u, double* blocks, int col)
+{
+	return dispatch_dual<MODE_DIAG>(c, make_dual(c, u, nullptr, blocks, col));
+}
+
+}  // namespace dgx

@@ -1,0 +1,397 @@
+// k_residual.h -- the fused element-centric DG residual kernel for QUAD/HEX (gfx950).
+//
+// One pass per element computes the whole rhs of compute_rlhs_common_dg
+// (src/simulation/solvers/dg/solve_dg.c:182-189):
+//   volume  rhs += sum_d tw1_vt_vc[d] fr_d          compute_volume_rlhs_T.c:145-177
+//   faces   rhs -= tw0_vt_fc (J_f * nnf)             compute_face_rlhs_T.c:297-359
+// but sum-factorised: with Ibar = I (x) I (x) I the GLL->GL interpolation and
+// Dhat the GL-collocated derivative, tw1_d = Ibar^T Dhat_d^T W, so
+//   rhs = Ibar^T [ sum_d Dhat_d^T (W fr_d) - sum_faces (Lift_side (x) W_f nnf) ],
+//   Lift_side = I^{-T} e_side,
+// and M^-1 rhs = Ibar^{-1} [ ... ] / (W detJ) for the explicit path (mass inverse
+// for free).  Every face flux is evaluated by both neighbours from bitwise identical
+// inputs (left trace, right trace permuted to the left ordering, left n*J_f), the way
+// the reference evaluates it once per face, so the face coupling stays conservative.
+//
+// Data layout in HBM (reference layout, per element contiguous):
+//   u, rhs     [E][nv][Np]               (sol_coef (Np, nv) column-major)
+//   metrics    [E][d*d][Nc]              (metrics_vc (Nc, d, d))
+//   nJ         [F][Nfc][d]               (normals_fc * jacobian_det_fc, left element)
+//   elem faces [E][2d] {f, en, lfN | side<<4 | ord<<8 | bc<<12}
+// Work mapping: one thread per volume node, EPB elements per workgroup, 1-D operators
+// and face orientation tables in LDS, pencil sweeps through LDS (a thread owns one
+// N-long line, reads it to registers, writes its N outputs).
+#pragma once
+
+#include "dgx_device.h"
+#include "dgx_internal.h"
+
+namespace dgx {
+
+__device__ inline Dual operator*(double a, Dual b) { return Dual(a * b.v, a * b.d); }
+
+enum { MODE_JVP = 5, MODE_DIAG = 6 };
+
+template <typename S, typename R>
+struct RhsArgs {
+	using Real = typename RealOf<S>::type;
+	const Real* u;        // state
+	const Real* v;        // JVP direction
+	Real* out;            // residual (or diagonal blocks in MODE_DIAG)
+	const R* met;
+	const R* detJ;
+	const R* nJ;
+	const ElemFace* ef;
+	const Ops1D<R>* ops;
+	double bc_prim[5];
+	double b_adv[3];
+	int E;
+	int col;              // MODE_DIAG column (local dof)
+	unsigned long long* dmax;
+};
+
+template <int D, int N>
+__device__ inline int face_vol_node(int lf, int i)
+{
+	const int fd = lf >> 1, fix = (lf & 1) ? N - 1 : 0;
+	if (D == 2) return fd == 0 ? fix + N * i : i + N * fix;
+	const int a = i % N, b = i / N;
+	if (fd == 0) return fix + N * (a + N * b);
+	if (fd == 1) return a + N * (fix + N * b);
+	return a + N * (b + N * fix);
+}
+
+// In-place sweep along `dir` over NB blocks of N^DD values.
+template <typename S, typename R, int DD, int N, int NB, int NT>
+__device__ inline void sweep_inplace(S* buf, const R* M, int dir, int t)
+{
+	constexpr int BL = ipow(N, DD);
+	constexpr int NL = ipow(N, DD - 1);
+	constexpr int NPEN = NB * NL;
+	constexpr int PPT = (NPEN + NT - 1) / NT;
+	const int Sd = (dir == 0) ? 1 : (dir == 1 ? N : N * N);
+	S in[PPT][N];
+	int base[PPT];
+#pragma unroll
+	for (int k = 0; k < PPT; ++k) {
+		const int pen = t + k * NT;
+		base[k] = -1;
+		if (pen < NPEN) {
+			const int b = pen / NL, l = pen - b * NL;
+			base[k] = b * BL + (l % Sd) + (l / Sd) * Sd * N;
+#pragma unroll
+			for (int q = 0; q < N; ++q) in[k][q] = buf[base[k] + q * Sd];
+		}
+	}
+	__syncthreads();
+#pragma unroll
+	for (int k = 0; k < PPT; ++k) {
+		if (base[k] >= 0) {
+#pragma unroll
+			for (int q = 0; q < N; ++q) {
+				S acc = M[q * N] * in[k][0];
+#pragma unroll
+				for (int j = 1; j < N; ++j) acc += M[q * N + j] * in[k][j];
+				buf[base[k] + q * Sd] = acc;
+			}
+		}
+	}
+	__syncthreads();
+}
+
+// dst (+)= sweep(src) along dir over NB blocks of N^DD values (src != dst).
+template <typename S, typename R, int DD, int N, int NB, int NT, bool ACC>
+__device__ inline void sweep_to(const S* src, S* dst, const R* M, int dir, int t)
+{
+	constexpr int BL = ipow(N, DD);
+	constexpr int NL = ipow(N, DD - 1);
+	constexpr int NPEN = NB * NL;
+	constexpr int PPT = (NPEN + NT - 1) / NT;
+	const int Sd = (dir == 0) ? 1 : (dir == 1 ? N : N * N);
+#pragma unroll
+	for (int k = 0; k < PPT; ++k) {
+		const int pen = t + k * NT;
+		if (pen < NPEN) {
+			const int b = pen / NL, l = pen - b * NL;
+			const int base = b * BL + (l % Sd) + (l / Sd) * Sd * N;
+			S in[N];
+#pragma unroll
+			for (int q = 0; q < N; ++q) in[q] = src[base + q * Sd];
+#pragma unroll
+			for (int q = 0; q < N; ++q) {
+				S acc = M[q * N] * in[0];
+#pragma unroll
+				for (int j = 1; j < N; ++j) acc += M[q * N + j] * in[j];
+				if (ACC) dst[base + q * Sd] += acc;
+				else dst[base + q * Sd] = acc;
+			}
+		}
+	}
+	__syncthreads();
+}
+
+template <typename S, int D, int N, int NV>
+constexpr int rhs_epb()
+{
+	constexpr int Np = ipow(N, D), NF = ipow(N, D - 1), NFACE = 2 * D;
+	constexpr int bytes = (int)sizeof(S) * (NV * Np * (1 + D) + 2 * NFACE * NV * NF) + NFACE * 16;
+	constexpr int by_lds = (56 * 1024) / bytes;
+	constexpr int by_thr = 256 / Np;
+	constexpr int e = by_lds < by_thr ? by_lds : by_thr;
+	return e > 0 ? e : 1;
+}
+
+template <typename S, typename R, int D, int N, int PDE, int FLUX, int MODE>
+__global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
+{
+	constexpr int Np = ipow(N, D), NF = ipow(N, D - 1), NV = (PDE == DGX_PDE_EULER) ? D + 2 : 1;
+	constexpr int NFACE = 2 * D, NT = Np;
+	constexpr int EPB = rhs_epb<S, D, N, NV>();
+	constexpr bool MINV = (MODE == MODE_RHS_MINV);
+	using Real = typename RealOf<S>::type;
+
+	__shared__ R sI[N * N], sIT[N * N], sDhT[N * N], sIinv[N * N], sLift[2][N], sW[N];
+	__shared__ int sPerm[8][NF];
+	__shared__ S sU[EPB][NV * Np];
+	__shared__ S sFR[EPB][D * NV * Np];
+	__shared__ S sTR[EPB][2 * NFACE * NV * NF];
+	__shared__ ElemFace sEF[EPB][NFACE];
+	__shared__ unsigned long long sMax;
+
+	const int slot = threadIdx.x / NT, t = threadIdx.x - slot * NT;
+	const int e = blockIdx.x * EPB + slot;
+	const bool valid = (slot < EPB) && (e < a.E);
+	const int sl = slot < EPB ? slot : 0;
+
+	for (int i = threadIdx.x; i < N * N; i += blockDim.x) {
+		sI[i] = a.ops->I[i];
+		sIT[i] = a.ops->IT[i];
+		sDhT[i] = a.ops->DhT[i];
+		sIinv[i] = a.ops->Iinv[i];
+	}
+	for (int i = threadIdx.x; i < N; i += blockDim.x) {
+		sLift[0][i] = a.ops->Lift[0][i];
+		sLift[1][i] = a.ops->Lift[1][i];
+		sW[i] = a.ops->w[i];
+	}
+	for (int i = threadIdx.x; i < 8 * NF; i += blockDim.x) sPerm[i / NF][i % NF] = a.ops->perm[i / NF][i % NF];
+	if (threadIdx.x == 0) sMax = 0ull;
+	if (valid && t < NFACE) sEF[sl][t] = a.ef[(size_t)e * NFACE + t];
+
+	S* U = sU[sl];
+	S* FR = sFR[sl];
+	S* TR = sTR[sl];
+	const size_t ebase = (size_t)e * NV * Np;
+	if (valid) {
+		for (int i = t; i < NV * Np; i += NT) {
+			if constexpr (MODE == MODE_JVP) U[i] = S(a.u[ebase + i], a.v[ebase + i]);
+			else if constexpr (MODE == MODE_DIAG) U[i] = S(a.u[ebase + i], i == a.col ? 1.0 : 0.0);
+			else U[i] = S(a.u[ebase + i]);
+		}
+	}
+	__syncthreads();
+
+	// ---- traces at the GLL face nodes: own (from LDS) and neighbour (gathered from HBM)
+	if (valid) {
+		for (int item = t; item < NFACE * NV * NF; item += NT) {
+			const int lf = item / (NV * NF), rem = item - lf * (NV * NF);
+			const int v = rem / NF, i = rem - v * NF;
+			TR[lf * NV * NF + rem] = U[v * Np + face_vol_node<D, N>(lf, i)];
+			const ElemFace ef = sEF[sl][lf];
+			S nb = S(0.0);
+			if (ef.en >= 0) {
+				const size_t gi = (size_t)ef.en * NV * Np + v * Np + face_vol_node<D, N>(ef.packed & 15, i);
+				if constexpr (MODE == MODE_JVP) nb = S(a.u[gi], a.v[gi]);
+				else nb = S(a.u[gi]);
+			}
+			TR[(NFACE + lf) * NV * NF + rem] = nb;
+		}
+	}
+	// ---- GLL -> GL: volume (D sweeps) and face traces (D-1 sweeps)
+#pragma unroll
+	for (int dir = 0; dir < D; ++dir) sweep_inplace<S, R, D, N, NV, NT>(U, sI, dir, t);
+#pragma unroll
+	for (int dir = 0; dir < D - 1; ++dir) sweep_inplace<S, R, D - 1, N, 2 * NFACE * NV, NT>(TR, sI, dir, t);
+
+	// ---- physical flux at the volume cubature, contravariant and weighted: FR[d][v][q]
+	if (valid) {
+		const int q = t;
+		S s[NV];
+#pragma unroll
+		for (int v = 0; v < NV; ++v) s[v] = U[v * Np + q];
+		S f[D * NV];
+		if constexpr (PDE == DGX_PDE_EULER) euler_flux<S, D>(s, f);
+		else {
+#pragma unroll
+			for (int k = 0; k < D; ++k) f[k] = R(a.b_adv[k]) * s[0];
+		}
+		R m[D * D];
+		const R* mp = a.met + (size_t)e * D * D * Np;
+#pragma unroll
+		for (int i = 0; i < D * D; ++i) m[i] = mp[i * Np + q];
+		const int c0 = q % N, c1 = (q / N) % N, c2 = (D == 3) ? q / (N * N) : 0;
+		R wq = sW[c0] * sW[c1];
+		if (D == 3) wq = wq * sW[c2];
+#pragma unroll
+		for (int d0 = 0; d0 < D; ++d0)
+#pragma unroll
+			for (int v = 0; v < NV; ++v) {
+				S acc = m[0 + D * d0] * f[0 + D * v];
+#pragma unroll
+				for (int d1 = 1; d1 < D; ++d1) acc += m[d1 + D * d0] * f[d1 + D * v];
+				FR[(d0 * NV + v) * Np + q] = wq * acc;
+			}
+	}
+
+	// ---- numerical flux at the face cubature (canonical = left ordering)
+	constexpr int NIT = NFACE * NF;
+	constexpr int IPT = (NIT + NT - 1) / NT;
+	S res[IPT][NV];
+	int dst[IPT];
+#pragma unroll
+	for (int k = 0; k < IPT; ++k) {
+		const int item = t + k * NT;
+		dst[k] = -1;
+		if (valid && item < NIT) {
+			const int lf = item / NF, i = item - lf * NF;
+			const ElemFace ef = sEF[sl][lf];
+			const int side = (ef.packed >> 4) & 1, ord = (ef.packed >> 8) & 7, bc = (ef.packed >> 12) & 15;
+			const S* own = TR + lf * NV * NF;
+			const S* nbr = TR + (NFACE + lf) * NV * NF;
+			const int pi = sPerm[ord][i];
+			S sL[NV], sR[NV];
+			if (side == 0) {
+#pragma unroll
+				for (int v = 0; v < NV; ++v) { sL[v] = own[v * NF + i]; sR[v] = nbr[v * NF + pi]; }
+			} else {
+#pragma unroll
+				for (int v = 0; v < NV; ++v) { sL[v] = nbr[v * NF + i]; sR[v] = own[v * NF + pi]; }
+			}
+			const R* np_ = a.nJ + ((size_t)ef.f * NF + i) * D;
+			R nJv[D];
+#pragma unroll
+			for (int k2 = 0; k2 < D; ++k2) nJv[k2] = np_[k2];
+			R J2 = nJv[0] * nJv[0];
+#pragma unroll
+			for (int k2 = 1; k2 < D; ++k2) J2 += nJv[k2] * nJv[k2];
+			const R Jf = sqrt(J2);
+			S nu[D];
+#pragma unroll
+			for (int k2 = 0; k2 < D; ++k2) nu[k2] = S(nJv[k2] / Jf);
+			S nnf[NV];
+			if constexpr (PDE == DGX_PDE_EULER) {
+				if (ef.en < 0) bc_state<S, D>(bc, sL, nu, a.bc_prim, sR);
+				if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<S, D>(sL, sR, nu, nnf);
+				else numflux_roe<S, D>(sL, sR, nu, nnf);
+			} else {
+				double bn = a.b_adv[0] * rv(nu[0]);
+#pragma unroll
+				for (int k2 = 1; k2 < D; ++k2) bn += a.b_adv[k2] * rv(nu[k2]);
+				nnf[0] = (bn > kEps) ? R(bn) * sL[0] : R(bn) * sR[0];
+			}
+#pragma unroll
+			for (int v = 0; v < NV; ++v) res[k][v] = (side == 0) ? Jf * nnf[v] : R(-1.0) * (Jf * nnf[v]);
+			dst[k] = lf * NV * NF + (side == 0 ? i : pi);
+		}
+	}
+	__syncthreads();
+#pragma unroll
+	for (int k = 0; k < IPT; ++k)
+		if (dst[k] >= 0) {
+#pragma unroll
+			for (int v = 0; v < NV; ++v) TR[dst[k] + v * NF] = res[k][v];
+		}
+	__syncthreads();
+
+	// ---- weak divergence in GL space: U = sum_d Dhat_d^T FR_d
+	sweep_to<S, R, D, N, NV, NT, false>(FR, U, sDhT, 0, t);
+	sweep_to<S, R, D, N, NV, NT, true>(FR + NV * Np, U, sDhT, 1, t);
+	if constexpr (D == 3) sweep_to<S, R, D, N, NV, NT, true>(FR + 2 * NV * Np, U, sDhT, 2, t);
+
+	// ---- face lifts (and the mass inverse) at the GL nodes
+	if (valid) {
+		const int q = t;
+		const int cc[3] = { q % N, (q / N) % N, (D == 3) ? q / (N * N) : 0 };
+		R scale = R(1.0);
+		if constexpr (MINV) {
+			R wq = sW[cc[0]] * sW[cc[1]];
+			if (D == 3) wq = wq * sW[cc[2]];
+			scale = R(1.0) / (wq * a.detJ[(size_t)e * Np + q]);
+		}
+#pragma unroll
+		for (int v = 0; v < NV; ++v) {
+			S acc = U[v * Np + q];
+#pragma unroll
+			for (int lf = 0; lf < NFACE; ++lf) {
+				const int fd = lf >> 1;
+				int fa, fb;
+				if (fd == 0) { fa = cc[1]; fb = cc[2]; }
+				else if (fd == 1) { fa = cc[0]; fb = cc[2]; }
+				else { fa = cc[0]; fb = cc[1]; }
+				const int qf = (D == 3) ? fa + N * fb : fa;
+				R wf = sW[fa];
+				if (D == 3) wf = wf * sW[fb];
+				acc -= (sLift[lf & 1][cc[fd]] * wf) * TR[lf * NV * NF + v * NF + qf];
+			}
+			if constexpr (MINV) acc = scale * acc;
+			U[v * Np + q] = acc;
+		}
+	}
+	__syncthreads();
+	// ---- back to the GLL test functions: Ibar^T (rhs) or Ibar^{-1} (M^-1 rhs)
+#pragma unroll
+	for (int dir = 0; dir < D; ++dir) sweep_inplace<S, R, D, N, NV, NT>(U, MINV ? sIinv : sIT, dir, t);
+
+	if (valid) {
+		double lmax = 0.0;
+		for (int i = t; i < NV * Np; i += NT) {
+			if constexpr (MODE == MODE_JVP) a.out[ebase + i] = U[i].d;
+			else if constexpr (MODE == MODE_DIAG)
+				a.out[(size_t)e * (NV * Np) * (NV * Np) + (size_t)i * (NV * Np) + a.col] = U[i].d;
+			else {
+				a.out[ebase + i] = U[i];
+				lmax = fmax(lmax, fabs((double)rv(U[i])));
+			}
+		}
+		if constexpr (MODE == MODE_RHS_MINV) {
+			if (a.dmax) atomicMax(&sMax, (unsigned long long)__double_as_longlong(lmax));
+		}
+	}
+	if constexpr (MODE == MODE_RHS_MINV) {
+		__syncthreads();
+		if (threadIdx.x == 0 && a.dmax) atomicMax(a.dmax, sMax);
+	}
+}
+
+template <typename S, typename R, int D, int N, int PDE, int FLUX, int MODE>
+inline hipError_t run_rhs(const RhsArgs<S, R>& a, hipStream_t st)
+{
+	constexpr int Np = ipow(N, D), NV = (PDE == DGX_PDE_EULER) ? D + 2 : 1;
+	constexpr int EPB = rhs_epb<S, D, N, NV>();
+	const int grid = (a.E + EPB - 1) / EPB;
+	if (grid == 0) return hipSuccess;
+	hipLaunchKernelGGL((k_rhs<S, R, D, N, PDE, FLUX, MODE>), dim3(grid), dim3(EPB * Np), 0, st, a);
+	return hipGetLastError();
+}
+
+// runtime (d, p, pde, flux) -> template instantiation
+template <typename S, typename R, int MODE>
+hipError_t dispatch_rhs(int d, int p, int pde, int flux, const RhsArgs<S, R>& a, hipStream_t st)
+{
+#define DGX_CASE_EULER(DD, NN)                                                              \
+	if (d == DD && p + 1 == NN && pde == DGX_PDE_EULER) {                                   \
+		if (flux == DGX_FLUX_LF) return run_rhs<S, R, DD, NN, DGX_PDE_EULER, DGX_FLUX_LF, MODE>(a, st); \
+		return run_rhs<S, R, DD, NN, DGX_PDE_EULER, DGX_FLUX_ROE, MODE>(a, st);             \
+	}
+#define DGX_CASE_ADV(DD, NN)                                                                \
+	if (d == DD && p + 1 == NN && pde == DGX_PDE_ADVECTION)                                 \
+		return run_rhs<S, R, DD, NN, DGX_PDE_ADVECTION, DGX_FLUX_UPWIND, MODE>(a, st);
+	DGX_CASE_EULER(2, 2) DGX_CASE_EULER(2, 3) DGX_CASE_EULER(2, 4) DGX_CASE_EULER(2, 5) DGX_CASE_EULER(2, 6)
+	DGX_CASE_EULER(3, 2) DGX_CASE_EULER(3, 3) DGX_CASE_EULER(3, 4) DGX_CASE_EULER(3, 5)
+	DGX_CASE_ADV(2, 2) DGX_CASE_ADV(2, 3) DGX_CASE_ADV(2, 4) DGX_CASE_ADV(2, 5) DGX_CASE_ADV(2, 6)
+#undef DGX_CASE_EULER
+#undef DGX_CASE_ADV
+	return hipErrorInvalidValue;
+}
+
+}  // namespace dgx

@@ -1,0 +1,32 @@
+// k_rhs64.hip -- FP64 instantiations of the fused residual kernel (outer residual, explicit path).
+#include "k_residual.h"
+
+namespace dgx {
+
+static RhsArgs<double, double> make_args64(Ctx& c, const double* u, double* r, unsigned long long* dmax)
+{
+	RhsArgs<double, double> a{};
+	a.u = u;
+	a.v = nullptr;
+	a.out = r;
+	a.met = c.metrics_vc;
+	a.detJ = c.detJ_vc;
+	a.nJ = c.nJ_fc;
+	a.ef = c.elem_faces;
+	a.ops = c.ops64;
+	for (int k = 0; k < 5; ++k) a.bc_prim[k] = c.cfg.bc_prim[k];
+	for (int k = 0; k < 3; ++k) a.b_adv[k] = c.cfg.b_adv[k];
+	a.E = c.E;
+	a.col = 0;
+	a.dmax = dmax;
+	return a;
+}
+
+hipError_t launch_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned long long* dmax)
+{
+	const RhsArgs<double, double> a = make_args64(c, u, r, dmax);
+	if (mode == MODE_RHS) return dispatch_rhs<double, double, MODE_RHS>(c.d, c.p, c.cfg.pde, c.cfg.num_flux, a, c.stream);
+	return dispatch_rhs<double, double, MODE_RHS_MINV>(c.d, c.p, c.cfg.pde, c.cfg.num_flux, a, c.stream);
+}
+
+}  // namespace dgx

@@ -229,27 +229,30 @@ def structured_tp_mesh(d: int, p: int, n: tuple, lo: tuple, hi: tuple, periodic:
         pts = _apply_rotation(d, N, pts, rotate)
     xyz = np.ascontiguousarray(pts.transpose(0, 2, 1))
 
-    def eid(c):
-        return int(sum(c[k] * int(np.prod(n[:k])) for k in range(d)))
-
     pairs, bfaces = [], []
-    for e in range(E):
-        c = list(eidx[e])
-        for k in range(d):
-            cn = list(c)
-            cn[k] += 1
-            if cn[k] == n[k]:
-                if periodic[k]:
-                    cn[k] = 0
-                else:
-                    bfaces.append((e, k, 1))
-                    continue
-            pairs.append((e, eid(cn), k))
-            if c[k] == 0 and not periodic[k]:
-                bfaces.append((e, k, 0))
+    strides = [int(np.prod(n[:k])) for k in range(d)]
+    all_e = np.arange(E)
+    for k in range(d):
+        ck = eidx[:, k]
+        nxt = ck + 1
+        inside = nxt < n[k]
+        en = all_e + strides[k]
+        if periodic[k]:
+            en = np.where(inside, en, all_e - (n[k] - 1) * strides[k])
+            sel = np.ones(E, dtype=bool)
+        else:
+            sel = inside
+            bfaces += [(int(e), k, 1) for e in all_e[~inside]]
+            bfaces += [(int(e), k, 0) for e in all_e[ck == 0]]
+        pairs += list(zip(all_e[sel].tolist(), en[sel].tolist(), [k] * int(sel.sum())))
+    pairs.sort(key=lambda t: (t[0], t[2]))
     periods = tuple((hi[k] - lo[k]) if periodic[k] else 0.0 for k in range(d))
-    matched = _match_faces(d, N, xyz, pairs, periods)
-    rows = [(eL, fL, eR, fR, o, BC_INTERIOR) for (eL, eR, _), (fL, fR, o) in zip(pairs, matched)]
+    if rotate is None:
+        # unrotated structured numbering: the +k face of eL meets the -k face of eR, same orientation
+        rows = [(eL, 2 * k + 1, eR, 2 * k, 0, BC_INTERIOR) for (eL, eR, k) in pairs]
+    else:
+        matched = _match_faces(d, N, xyz, pairs, periods)
+        rows = [(eL, fL, eR, fR, o, BC_INTERIOR) for (eL, eR, _), (fL, fR, o) in zip(pairs, matched)]
     if bfaces and rotate is not None:
         raise ValueError("rotated elements are only supported on fully periodic meshes")
     for e, k, side in bfaces:
@@ -376,11 +379,15 @@ def smooth_state_3d(x: np.ndarray, seed: int = SEED) -> np.ndarray:
     rng = SplitMix64(seed + 3)
     phi = rng.uniform(15, 0.0, 1.0).reshape(5, 3)
     d = x.shape[-1]
+    # sin(2 pi (x + phi)) = sin(2 pi x) cos(2 pi phi) + cos(2 pi x) sin(2 pi phi): 2d transcendentals in total
+    sx = [np.sin(2.0 * np.pi * x[..., c]) for c in range(d)]
+    cx = [np.cos(2.0 * np.pi * x[..., c]) for c in range(d)]
     s = []
     for k in range(5):
-        v = np.ones(x.shape[:-1])
+        v = None
         for c in range(d):
-            v = v * np.sin(2.0 * np.pi * (x[..., c] + phi[k, c]))
+            t = sx[c] * math.cos(2.0 * math.pi * phi[k, c]) + cx[c] * math.sin(2.0 * math.pi * phi[k, c])
+            v = t if v is None else v * t
         s.append(v)
     prim = np.stack([1.0 + 0.1 * s[0], 0.28 * s[1], 0.28 * s[2], 0.28 * s[3], 1.0 + 0.1 * s[4]], axis=-1)
     if d == 2:
@@ -389,8 +396,32 @@ def smooth_state_3d(x: np.ndarray, seed: int = SEED) -> np.ndarray:
 
 
 def config3_state(mesh: Mesh, seed: int = SEED) -> np.ndarray:
-    x = nodes_xyz(mesh)
-    return pack_state(prim_to_cons(smooth_state_3d(x, seed), mesh.d))
+    """smooth_state_3d in conserved variables, built directly in the packed [E][nv][Np] layout."""
+    d = mesh.d
+    rng = SplitMix64(seed + 3)
+    phi = rng.uniform(15, 0.0, 1.0).reshape(5, 3)
+    xc = [np.ascontiguousarray(mesh.xyz[:, c, :]) for c in range(d)]
+    sx = [np.sin(2.0 * np.pi * a) for a in xc]
+    cx = [np.cos(2.0 * np.pi * a) for a in xc]
+    s = []
+    for k in range(5):
+        v = None
+        for c in range(d):
+            t = sx[c] * math.cos(2.0 * math.pi * phi[k, c]) + cx[c] * math.sin(2.0 * math.pi * phi[k, c])
+            v = t if v is None else v * t
+        s.append(v)
+    del sx, cx, xc
+    rho = 1.0 + 0.1 * s[0]
+    vel = [0.28 * s[1 + c] for c in range(d)]
+    p = 1.0 + 0.1 * s[4]
+    out = np.empty((mesh.E, d + 2, mesh.Np))
+    out[:, 0] = rho
+    ke = np.zeros_like(rho)
+    for c in range(d):
+        out[:, 1 + c] = rho * vel[c]
+        ke += vel[c] * vel[c]
+    out[:, d + 1] = p / GM1 + 0.5 * rho * ke
+    return out.ravel()
 
 
 def channel_mesh(nx: int = 64, ny: int = 64, p: int = 3) -> Mesh:

@@ -1,0 +1,124 @@
+/*
+ * dgx.h -- C ABI of the MI355X-native DG residual library (libdgx.so).
+ *
+ * Drop-in boundary for DPGSolver's DG hot path.  The reference entry points each
+ * function replaces are cited per declaration (paths relative to the DPGSolver
+ * source tree).  All device pointers are HIP device memory on the context's device;
+ * every call is asynchronous on the context's stream unless stated otherwise and
+ * returns a dgx_status (never aborts, unlike the reference's EXIT_ERROR/assert).
+ *
+ * State layout at the ABI is the reference's: per element the (Np, nv) column-major
+ * block sol_coef (index var*Np + node, nodes r-fastest), elements concatenated in
+ * ind_dof order (src/simulation/solvers/solve_T.c:184-223).
+ */
+#ifndef DGX_H
+#define DGX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int dgx_status;
+enum {
+	DGX_OK = 0,
+	DGX_ERR_ARG = -1,          /* bad argument / size */
+	DGX_ERR_HIP = -2,          /* HIP runtime error */
+	DGX_ERR_UNSUPPORTED = -3,  /* (d, p, pde) not compiled */
+	DGX_ERR_POSITIVITY = -4,   /* mean density or pressure <= EPS_PHYS (solve.c:204-208) */
+	DGX_ERR_NONFINITE = -5,    /* NaN/Inf in a residual norm */
+	DGX_ERR_NOT_CONVERGED = -6,/* linear solve hit its iteration cap (KSP DIVERGED_ITS) */
+	DGX_ERR_NO_MESH = -7
+};
+
+enum { DGX_PDE_ADVECTION = 0, DGX_PDE_EULER = 1 };
+/* num_flux_1st: upwind | Lax-Friedrichs | Roe-Pike (numerical_flux/definitions_numerical_flux.h:22-32) */
+enum { DGX_FLUX_UPWIND = 0, DGX_FLUX_LF = 1, DGX_FLUX_ROE = 2 };
+enum { DGX_BC_INTERIOR = 0, DGX_BC_RIEMANN = 1, DGX_BC_SLIPWALL = 2 };
+
+typedef struct dgx_config {
+	int d;              /* 2 or 3 (DIM, a compile-time constant in the reference) */
+	int p;              /* polynomial order, 1..7 (p_ref) */
+	int pde;            /* DGX_PDE_* (pde_name) */
+	int num_flux;       /* DGX_FLUX_* (num_flux_1st) */
+	double b_adv[3];    /* advection velocity (solution.data b_adv) */
+	double bc_prim[5];  /* free-stream primitive state for Riemann BCs (rho, u, v, [w], p) */
+} dgx_config;
+
+typedef struct dgx_mesh {
+	int E;              /* elements (QUAD/HEX, one p) */
+	int F;              /* faces */
+	const double* xyz;  /* HOST, E x d x Np: isoparametric geometry nodes at the GLL solution nodes */
+	const int* faces;   /* HOST, F x 6: eL, fL, eR, fR, ord, bc  (face.h:32-70 neigh_info) */
+} dgx_mesh;
+
+typedef struct dgx_ctx dgx_ctx;
+
+/* Library/version and device helpers. */
+const char* dgx_version(void);
+dgx_status dgx_device_count(int* n);
+
+/* Context for one discretisation.  Replaces constructor_Simulation's solver set-up
+ * (src/simulation/simulation.h:214-216) for the DG hot path. */
+dgx_status dgx_create(dgx_ctx** ctx, const dgx_config* cfg);
+dgx_status dgx_destroy(dgx_ctx* ctx);
+/* hipStream_t; NULL = the default stream */
+dgx_status dgx_set_stream(dgx_ctx* ctx, void* stream);
+dgx_status dgx_synchronize(dgx_ctx* ctx);
+dgx_status dgx_sizes(const dgx_ctx* ctx, int* Np, int* nv, int* Nfc, long long* ndof);
+
+/* Mesh upload + FP64 geometry on the GPU: volume metrics (2-D cofactors, 3-D symmetric
+ * curl form), det J at the volume cubature, n*J_f at the face cubature, FP32 shadows.
+ * Replaces set_up_solver_geometry_T / compute_geometry_volume_T / compute_geometry_face_T
+ * (src/simulation/geometry/geometry_T.c:165, 208-325, 734-801;
+ *  deprecated/src/setup_geom_factors.c:157-243 for the 3-D curl form). Synchronous. */
+dgx_status dgx_upload_mesh(dgx_ctx* ctx, const dgx_mesh* mesh);
+/* Copies of the device geometry into HOST arrays (any may be NULL). Synchronous. */
+dgx_status dgx_get_geometry(dgx_ctx* ctx, double* metrics_vc, double* detJ_vc, double* nJ_fc);
+
+/* FP64 residual r = R(u): volume + face terms, no M^-1.  Replaces compute_rlhs_common_dg
+ * (solvers/dg/solve_dg.c:182-189) = compute_volume_rlhs_dg + compute_face_rlhs_dg with
+ * 'e' parameters (solvers/dg/compute_volume_rlhs_dg_T.c:53-72, compute_face_rlhs_dg_T.c:88-114).
+ * r is fully overwritten (no pre-zeroing needed, solve_T.c:159-182). */
+dgx_status dgx_compute_rhs(dgx_ctx* ctx, const double* u, double* r);
+/* FP32 residual (defect-correction inner residual): FP32 state, FP32 geometry shadow,
+ * FP32 arithmetic.  Same semantics as dgx_compute_rhs. */
+dgx_status dgx_compute_rhs_f32(dgx_ctx* ctx, const float* u, float* r);
+/* compute_rhs_dg (solve_dg.c:130-136): r = M^-1 R(u), and max|r|_inf to *max_abs_r
+ * (HOST pointer, may be NULL; synchronous when given) (compute_rlhs.c:59-71). */
+dgx_status dgx_compute_rhs_explicit(dgx_ctx* ctx, const double* u, double* r, double* max_abs_r);
+/* r <- M^-1 r, M = cv0^T diag(w J) cv0 (scale_rhs_by_m_inv_std, solve_dg.c:265-272). */
+dgx_status dgx_apply_minv(dgx_ctx* ctx, double* r);
+/* Positivity limiter in place (enforce_positivity_highorder, solve.c:187-233).
+ * Returns DGX_ERR_POSITIVITY if an element mean density/pressure is below EPS_PHYS. Synchronous. */
+dgx_status dgx_enforce_positivity(dgx_ctx* ctx, double* u);
+/* One SSP-RK3 step in place (time_step_ssp_rk_33, solve_explicit.c:252-301), positivity
+ * after every stage when enforce_positivity != 0; *max_abs_r (HOST, may be NULL) receives
+ * max|M^-1 R| of the last stage.  Synchronous when max_abs_r is given. */
+dgx_status dgx_ssp_rk3_step(dgx_ctx* ctx, double* u, double dt, int enforce_positivity, double* max_abs_r);
+/* max |x|_inf over n doubles (HOST result). Synchronous. */
+dgx_status dgx_max_abs(dgx_ctx* ctx, const double* x, long long n, double* out);
+/* FP64 <-> FP32 state conversion on the device. */
+dgx_status dgx_convert_f64_f32(dgx_ctx* ctx, const double* src, float* dst, long long n);
+dgx_status dgx_convert_f32_f64(dgx_ctx* ctx, const float* src, double* dst, long long n);
+
+/* Implicit path.
+ * Diagonal blocks A_e = dR_e/du_e - mass_scale * M_e, (Np*nv)^2 each, row-major, row eq*Np+node,
+ * column var*Np+node: volume lhs_v_1 + face lhs_ll/lhs_rr contributions + CFL term
+ * (constructor_lhs_v_1_T compute_volume_rlhs_T.c:179-220, constructor_lhs_f_1_T
+ *  compute_face_rlhs_T.c:163-220, compute_CFL_ramping solve_dg.c:223-248 with mass_scale = 1/dt). */
+dgx_status dgx_assemble_diag_blocks(dgx_ctx* ctx, const double* u, double mass_scale, double* blocks);
+/* Exact Jacobian-vector product jv = (dR/du)(u) v by forward-mode (dual-number) evaluation of the
+ * residual: the matrix-free operator of the Newton system (replaces the PETSc AIJ Mat assembled by
+ * add_to_petsc_Mat, solve.c:321-338). */
+dgx_status dgx_jacobian_vector(dgx_ctx* ctx, const double* u, const double* v, double* jv);
+/* Batched LU (partial pivoting) of E blocks of size nb <= 90 in place; piv: E*nb ints (device). */
+dgx_status dgx_block_lu_factor(dgx_ctx* ctx, double* blocks, int* piv, int E, int nb);
+/* x <- A_e^{-1} x for every element block (x: E*nb). */
+dgx_status dgx_block_lu_solve(dgx_ctx* ctx, const double* lu, const int* piv, double* x, int E, int nb);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
