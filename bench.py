@@ -97,6 +97,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=64, help="elements per direction (64 = the named workload)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one 1 GiB read + 1 GiB write elementwise kernel (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -150,7 +152,27 @@ def main():
     print(f"fp64 residual: {ms64:.4f} ms", file=sys.stderr, flush=True)
 
     err32 = ((r32.double() - r64).abs().max() / r64.abs().max()).item()
-    cons = (r64.view(mesh.E, 5, -1).sum(dim=(0, 2)).abs().max() / r64.abs().max()).item()
+    # input rounding alone: FP64 residual of the FP32-rounded state
+    u64r = u32.double()
+    r64r = torch.empty_like(u64)
+    ctx.compute_rhs(u64r.data_ptr(), r64r.data_ptr())
+    err_in = ((r64r - r64).abs().max() / r64.abs().max()).item()
+    del u64r, r64r
+    # size-independent property at full size: free-stream preservation on the curved mesh
+    prim = torch.tensor([1.0, 0.3, -0.2, 0.1, 1.0], dtype=torch.float64, device=dev)
+    cons0 = torch.stack([prim[0], prim[0] * prim[1], prim[0] * prim[2], prim[0] * prim[3],
+                         prim[4] / M.GM1 + 0.5 * prim[0] * (prim[1:4] ** 2).sum()])
+    ufs = cons0.view(1, 5, 1).expand(mesh.E, 5, 125).contiguous().view(-1)
+    ctx.compute_rhs(ufs.data_ptr(), r64.data_ptr())
+    torch.cuda.synchronize()
+    free_stream = r64.abs().max().item()
+    del ufs
+    if args.copy_probe:
+        a = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        torch.mul(a, 2.0, out=b)   # one elementwise kernel: reads 1 GiB, writes 1 GiB
+        torch.cuda.synchronize()
+        del a, b
 
     E, Np, nv, d, Nfc = mesh.E, 125, 5, 3, 25
     bytes32 = E * (2 * nv * Np * 4 + Np * d * d * 4 + d * Nfc * d * 4)
@@ -184,7 +206,8 @@ def main():
                               "frac": round(ach64 / HBM_PEAK_GBS, 4),
                               "traffic": _traffic_from_profiles("rhs_fp64")}},
         "fp32_rel_err_vs_fp64": err32,
-        "fp64_conservation_rel": cons,
+        "fp32_input_rounding_rel_err": err_in,
+        "fp64_free_stream_max_abs": free_stream,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

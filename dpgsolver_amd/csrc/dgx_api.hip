@@ -40,7 +40,7 @@ void release(T*& p)
 
 void free_mesh(Ctx& c)
 {
-	release(c.xyz); release(c.faces); release(c.elem_faces); release(c.metrics_vc); release(c.metrics_vc32);
+	release(c.xyz); release(c.faces); release(c.elem_faces); release(c.order); release(c.metrics_vc); release(c.metrics_vc32);
 	release(c.detJ_vc); release(c.detJ_vc32); release(c.nJ_fc); release(c.nJ_fc32); release(c.u0); release(c.rtmp);
 	c.mesh_ready = false;
 }
@@ -174,6 +174,11 @@ dgx_status dgx_upload_mesh(dgx_ctx* h, const dgx_mesh* m)
 	DGX_TRY(hipMemcpy(c.xyz, m->xyz, sizeof(double) * nx, hipMemcpyHostToDevice));
 	if (c.F > 0) DGX_TRY(hipMemcpy(c.faces, m->faces, sizeof(int) * 6 * (size_t)c.F, hipMemcpyHostToDevice));
 	DGX_TRY(hipMemcpy(c.elem_faces, ef.data(), sizeof(dgx::ElemFace) * ef.size(), hipMemcpyHostToDevice));
+	{
+		const std::vector<int> order = dgx::morton_order(d, c.E, c.Np, m->xyz);
+		DGX_TRY(hipMalloc(&c.order, sizeof(int) * order.size()));
+		DGX_TRY(hipMemcpy(c.order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
+	}
 	DGX_TRY(dgx::launch_geometry(c));
 	DGX_TRY(dgx::launch_convert(c.metrics_vc, c.metrics_vc32, nm, c.stream));
 	DGX_TRY(dgx::launch_convert(c.detJ_vc, c.detJ_vc32, nj, c.stream));

@@ -56,6 +56,13 @@ __device__ inline Dual tpow(Dual a, double b)
 	const double pv = pow(a.v, b);
 	return Dual(pv, b * pow(a.v, b - 1.0) * a.d);
 }
+__device__ inline double trcp(double a) { return 1.0 / a; }
+__device__ inline float trcp(float a) { return 1.0f / a; }
+__device__ inline Dual trcp(Dual a)
+{
+	const double r = 1.0 / a.v;
+	return Dual(r, -a.d * r * r);
+}
 // drop the derivative part (neighbour traces held fixed for diagonal blocks)
 __device__ inline double strip(double a) { return a; }
 __device__ inline float strip(float a) { return a; }
@@ -90,43 +97,44 @@ __device__ inline void euler_flux(const T* s, T* f)
 	for (int k = 0; k < D; ++k) f[k + D * (D + 1)] = (E + p) * uvw[k];
 }
 
+// n . F(W) for a state with precomputed primitives: [rho Vn, rho u_k Vn + p n_k, (E+p) Vn]
+template <typename T, int D>
+__device__ inline void euler_normal_flux(const T* W, const T* uvw, T p, const T* n, T* Fn)
+{
+	T Vn = uvw[0] * n[0];
+#pragma unroll
+	for (int k = 1; k < D; ++k) Vn += uvw[k] * n[k];
+	Fn[0] = W[0] * Vn;
+#pragma unroll
+	for (int k = 0; k < D; ++k) Fn[1 + k] = W[1 + k] * Vn + p * n[k];
+	Fn[D + 1] = (W[D + 1] + p) * Vn;
+}
+
+// Lax-Friedrichs: 0.5 [n.(F_L + F_R) + lambda (W_L - W_R)], lambda = max(|V| + c) over the sides
 template <typename T, int D>
 __device__ inline void numflux_lf(const T* WL, const T* WR, const T* n, T* nnf)
 {
 	constexpr int NV = D + 2;
-	T VL, VR, cL, cR;
-	{
-		const T rho_inv = T(1.0) / WL[0];
-		T u0 = WL[1] * rho_inv;
-		T V2 = u0 * u0;
+	T uL[D], uR[D];
+	const T riL = trcp(WL[0]), riR = trcp(WR[0]);
+	T V2L = T(0.0), V2R = T(0.0);
 #pragma unroll
-		for (int k = 1; k < D; ++k) { const T u = WL[1 + k] * rho_inv; V2 += u * u; }
-		VL = tsqrt(V2);
-		const T p = T(kGM1) * (WL[D + 1] - T(0.5) * WL[0] * V2);
-		cL = tsqrt(T(kGamma) * p / WL[0]);
+	for (int k = 0; k < D; ++k) {
+		uL[k] = WL[1 + k] * riL;
+		uR[k] = WR[1 + k] * riR;
+		V2L += uL[k] * uL[k];
+		V2R += uR[k] * uR[k];
 	}
-	{
-		const T rho_inv = T(1.0) / WR[0];
-		T u0 = WR[1] * rho_inv;
-		T V2 = u0 * u0;
-#pragma unroll
-		for (int k = 1; k < D; ++k) { const T u = WR[1 + k] * rho_inv; V2 += u * u; }
-		VR = tsqrt(V2);
-		const T p = T(kGM1) * (WR[D + 1] - T(0.5) * WR[0] * V2);
-		cR = tsqrt(T(kGamma) * p / WR[0]);
-	}
-	const T maxlL = VL + cL, maxlR = VR + cR;
+	const T pL = T(kGM1) * (WL[D + 1] - T(0.5) * WL[0] * V2L);
+	const T pR = T(kGM1) * (WR[D + 1] - T(0.5) * WR[0] * V2R);
+	const T maxlL = tsqrt(V2L) + tsqrt(T(kGamma) * pL * riL);
+	const T maxlR = tsqrt(V2R) + tsqrt(T(kGamma) * pR * riR);
 	const T maxV = (rv(maxlL) > rv(maxlR)) ? maxlL : maxlR;
-	T FL[D * NV], FR[D * NV];
-	euler_flux<T, D>(WL, FL);
-	euler_flux<T, D>(WR, FR);
+	T FL[NV], FR[NV];
+	euler_normal_flux<T, D>(WL, uL, pL, n, FL);
+	euler_normal_flux<T, D>(WR, uR, pR, n, FR);
 #pragma unroll
-	for (int eq = 0; eq < NV; ++eq) {
-		T acc = n[0] * (FL[0 + D * eq] + FR[0 + D * eq]);
-#pragma unroll
-		for (int k = 1; k < D; ++k) acc += n[k] * (FL[k + D * eq] + FR[k + D * eq]);
-		nnf[eq] = T(0.5) * (acc + maxV * (WL[eq] - WR[eq]));
-	}
+	for (int eq = 0; eq < NV; ++eq) nnf[eq] = T(0.5) * ((FL[eq] + FR[eq]) + maxV * (WL[eq] - WR[eq]));
 }
 
 template <typename T>
@@ -152,7 +160,7 @@ __device__ inline void numflux_roe(const T* WL, const T* WR, const T* n, T* nnf)
 	const T EL = WL[D + 1];
 	const T rhoR = WR[0], rhouR = WR[1], rhovR = (D > 1) ? WR[2] : T(0.0), rhowR = (D > 2) ? WR[3] : T(0.0);
 	const T ER = WR[D + 1];
-	const T rhoL_inv = T(1.0) / rhoL, rhoR_inv = T(1.0) / rhoR;
+	const T rhoL_inv = trcp(rhoL), rhoR_inv = trcp(rhoR);
 	const T uL = rhouL * rhoL_inv, vL = rhovL * rhoL_inv, wL = rhowL * rhoL_inv;
 	const T uR = rhouR * rhoR_inv, vR = rhovR * rhoR_inv, wR = rhowR * rhoR_inv;
 	T V2L = uL * uL, V2R = uR * uR, VnL = n1 * uL, VnR = n1 * uR;
@@ -161,10 +169,10 @@ __device__ inline void numflux_roe(const T* WL, const T* WR, const T* n, T* nnf)
 	const T pL = T(kGM1) * (EL - T(0.5) * rhoL * V2L), pR = T(kGM1) * (ER - T(0.5) * rhoR * V2R);
 	const T HL = (EL + pL) * rhoL_inv, HR = (ER + pR) * rhoR_inv;
 
-	const T r = tsqrt(rhoR / rhoL), rP1 = r + T(1.0);
+	const T r = tsqrt(rhoR * rhoL_inv), rP1i = trcp(r + T(1.0));
 	const T rho = r * rhoL;
-	const T u = (r * uR + uL) / rP1, v = (r * vR + vL) / rP1, w = (r * wR + wL) / rP1;
-	const T H = (r * HR + HL) / rP1;
+	const T u = (r * uR + uL) * rP1i, v = (r * vR + vL) * rP1i, w = (r * wR + wL) * rP1i;
+	const T H = (r * HR + HL) * rP1i;
 	T Vn = n1 * u, V2 = u * u;
 	if (D > 1) { Vn += n2 * v; V2 += v * v; }
 	if (D > 2) { Vn += n3 * w; V2 += w * w; }
@@ -178,8 +186,9 @@ __device__ inline void numflux_roe(const T* WL, const T* WR, const T* n, T* nnf)
 	const T drhou = rhoR * uR - rhoL * uL, drhov = rhoR * vR - rhoL * vL, drhow = rhoR * wR - rhoL * wL;
 	const T dE = ER - EL, dp = pR - pL, dVn = VnR - VnL;
 	const T lc1 = T(0.5) * (l5 + l1) - l234, lc2 = T(0.5) * (l5 - l1);
-	const T disInter1 = lc1 * dp / (c * c) + lc2 * rho * dVn / c;
-	const T disInter2 = lc1 * rho * dVn + lc2 * dp / c;
+	const T ci = trcp(c);
+	const T disInter1 = lc1 * dp * (ci * ci) + lc2 * rho * dVn * ci;
+	const T disInter2 = lc1 * rho * dVn + lc2 * dp * ci;
 
 	const T rhoVnL = rhoL * VnL, rhoVnR = rhoR * VnR, pLR = pL + pR;
 	int k = 0;
@@ -196,59 +205,88 @@ __device__ inline void numflux_roe(const T* WL, const T* WR, const T* n, T* nnf)
 template <typename T, int D>
 __device__ inline void bc_state(int bc, const T* sL, const T* n, const double* bc_prim, T* sR)
 {
+	// results in scalars first, one straight-line store sequence at the end (no dynamic indexing)
+	T o_rho, o_E, o_m[D];
 	if (bc == 2) {  // slip wall
 		T rVn = n[0] * sL[1];
 #pragma unroll
 		for (int k = 1; k < D; ++k) rVn += n[k] * sL[1 + k];
-		sR[0] = sL[0];
-		sR[D + 1] = sL[D + 1];
+		o_rho = sL[0];
+		o_E = sL[D + 1];
 #pragma unroll
-		for (int k = 0; k < D; ++k) sR[1 + k] = sL[1 + k] - T(2.0) * rVn * n[k];
-		return;
-	}
-	const T rho_l = sL[0];
-	T uvw_l[D];
-	T V2l = T(0.0);
-#pragma unroll
-	for (int k = 0; k < D; ++k) { uvw_l[k] = sL[1 + k] / rho_l; V2l += uvw_l[k] * uvw_l[k]; }
-	const T p_l = T(kGM1) * (sL[D + 1] - T(0.5) * rho_l * V2l);
-	const T rho_r = T(bc_prim[0]), p_r = T(bc_prim[D + 1]);
-	T uvw_r[D];
-#pragma unroll
-	for (int k = 0; k < D; ++k) uvw_r[k] = T(bc_prim[1 + k]);
-	const T c_l = tsqrt(T(kGamma) * p_l / rho_l);
-	const T c_r = tsqrt(T(kGamma) * p_r / rho_r);
-	T Vn_l = T(0.0), Vn_r = T(0.0);
-#pragma unroll
-	for (int k = 0; k < D; ++k) { Vn_l += n[k] * uvw_l[k]; Vn_r += n[k] * uvw_r[k]; }
-	const T R_l = Vn_l + T(2.0 / kGM1) * c_l;
-	const T R_r = Vn_r - T(2.0 / kGM1) * c_r;
-	const T Vn = T(0.5) * (R_l + R_r), cc = T(0.25 * kGM1) * (R_l - R_r);
-	T rho, p, uvw[D];
-	if (fabs(rv(Vn)) >= fabs(rv(cc))) {
-		if (rv(Vn) < 0.0) { rho = rho_r; for (int k = 0; k < D; ++k) uvw[k] = uvw_r[k]; p = p_r; }
-		else              { rho = rho_l; for (int k = 0; k < D; ++k) uvw[k] = uvw_l[k]; p = p_l; }
+		for (int k = 0; k < D; ++k) o_m[k] = sL[1 + k] - T(2.0) * rVn * n[k];
 	} else {
-		T uvw_t[D];
-		if (rv(Vn) < 0.0) {
-			const T s_r = p_r / tpow(rho_r, kGamma);
-			rho = tpow(T(1.0 / kGamma) * cc * cc / s_r, 1.0 / kGM1);
-			for (int k = 0; k < D; ++k) uvw_t[k] = uvw_r[k] - Vn_r * n[k];
+		const T rho_l = sL[0];
+		T uvw_l[D];
+		T V2l = T(0.0);
+#pragma unroll
+		for (int k = 0; k < D; ++k) { uvw_l[k] = sL[1 + k] / rho_l; V2l += uvw_l[k] * uvw_l[k]; }
+		const T p_l = T(kGM1) * (sL[D + 1] - T(0.5) * rho_l * V2l);
+		const T rho_r = T(bc_prim[0]), p_r = T(bc_prim[D + 1]);
+		T uvw_r[D];
+#pragma unroll
+		for (int k = 0; k < D; ++k) uvw_r[k] = T(bc_prim[1 + k]);
+		const T c_l = tsqrt(T(kGamma) * p_l / rho_l);
+		const T c_r = tsqrt(T(kGamma) * p_r / rho_r);
+		T Vn_l = T(0.0), Vn_r = T(0.0);
+#pragma unroll
+		for (int k = 0; k < D; ++k) { Vn_l += n[k] * uvw_l[k]; Vn_r += n[k] * uvw_r[k]; }
+		const T R_l = Vn_l + T(2.0 / kGM1) * c_l;
+		const T R_r = Vn_r - T(2.0 / kGM1) * c_r;
+		const T Vn = T(0.5) * (R_l + R_r), cc = T(0.25 * kGM1) * (R_l - R_r);
+		T rho, p, uvw[D];
+		const bool super = fabs(rv(Vn)) >= fabs(rv(cc));
+		const bool inlet = rv(Vn) < 0.0;
+		if (super) {
+			rho = inlet ? rho_r : rho_l;
+			p = inlet ? p_r : p_l;
+#pragma unroll
+			for (int k = 0; k < D; ++k) uvw[k] = inlet ? uvw_r[k] : uvw_l[k];
 		} else {
-			const T s_l = p_l / tpow(rho_l, kGamma);
-			rho = tpow(T(1.0 / kGamma) * cc * cc / s_l, 1.0 / kGM1);
-			for (int k = 0; k < D; ++k) uvw_t[k] = uvw_l[k] - Vn_l * n[k];
+			const T s_b = inlet ? p_r / tpow(rho_r, kGamma) : p_l / tpow(rho_l, kGamma);
+			rho = tpow(T(1.0 / kGamma) * cc * cc / s_b, 1.0 / kGM1);
+#pragma unroll
+			for (int k = 0; k < D; ++k) {
+				const T ut = inlet ? uvw_r[k] - Vn_r * n[k] : uvw_l[k] - Vn_l * n[k];
+				uvw[k] = Vn * n[k] + ut;
+			}
+			p = T(1.0 / kGamma) * cc * cc * rho;
 		}
-		for (int k = 0; k < D; ++k) uvw[k] = Vn * n[k] + uvw_t[k];
-		p = T(1.0 / kGamma) * cc * cc * rho;
+		T V2 = uvw[0] * uvw[0];
+#pragma unroll
+		for (int k = 1; k < D; ++k) V2 += uvw[k] * uvw[k];
+		o_rho = rho;
+#pragma unroll
+		for (int k = 0; k < D; ++k) o_m[k] = rho * uvw[k];
+		o_E = p / T(kGM1) + T(0.5) * rho * V2;
 	}
+	sR[0] = o_rho;
+#pragma unroll
+	for (int k = 0; k < D; ++k) sR[1 + k] = o_m[k];
+	sR[D + 1] = o_E;
+}
+
+// contravariant Euler fluxes fr[d0] = sum_k m[k + D d0] F_k at one node, weighted by wq
+template <typename T, typename R, int D>
+__device__ inline void euler_contravariant_flux(const T* s, const R* m, R wq, T (*fr)[D + 2])
+{
+	const T ri = trcp(s[0]);
+	T uvw[D];
 	T V2 = T(0.0);
 #pragma unroll
-	for (int k = 0; k < D; ++k) V2 += uvw[k] * uvw[k];
-	sR[0] = rho;
+	for (int k = 0; k < D; ++k) { uvw[k] = s[1 + k] * ri; V2 += uvw[k] * uvw[k]; }
+	const T p = T(kGM1) * (s[D + 1] - T(0.5) * s[0] * V2);
+	const T Ep = s[D + 1] + p;
 #pragma unroll
-	for (int k = 0; k < D; ++k) sR[1 + k] = rho * uvw[k];
-	sR[D + 1] = p / T(kGM1) + T(0.5) * rho * V2;
+	for (int d0 = 0; d0 < D; ++d0) {
+		T Vm = m[0 + D * d0] * uvw[0];
+#pragma unroll
+		for (int k = 1; k < D; ++k) Vm += m[k + D * d0] * uvw[k];
+		fr[d0][0] = wq * (s[0] * Vm);
+#pragma unroll
+		for (int k = 0; k < D; ++k) fr[d0][1 + k] = wq * (s[1 + k] * Vm + p * m[k + D * d0]);
+		fr[d0][D + 1] = wq * (Ep * Vm);
+	}
 }
 
 }  // namespace dgx

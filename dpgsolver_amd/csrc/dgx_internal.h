@@ -54,6 +54,7 @@ struct Ctx {
 	double* xyz = nullptr;          // E*d*Np
 	int* faces = nullptr;           // F*6
 	ElemFace* elem_faces = nullptr; // E*2d
+	int* order = nullptr;           // E: element processing order (Morton order of centroids)
 	double* metrics_vc = nullptr;   // E*d*d*Nc  (m[(i + d*j)*Nc + n] = J dxi_j/dx_i)
 	float* metrics_vc32 = nullptr;
 	double* detJ_vc = nullptr;      // E*Nc
@@ -81,6 +82,7 @@ struct Ctx {
 void build_ops(int d, int p, Ops1D<double>& o64, Ops1D<float>& o32);
 void gauss_legendre(int n, double* x, double* w);
 void gauss_lobatto(int n, double* x, double* w);
+std::vector<int> morton_order(int d, int E, int Np, const double* xyz);
 
 // launchers (each returns hipError_t)
 hipError_t launch_geometry(Ctx& c);

@@ -5,6 +5,8 @@
 // nodes per the interp_tp GLL control setting).  Lagrange basis on GLL nodes
 // (basis_sol lagrange), Bernstein basis for the positivity limiter (solve.c:187-233).
 #include <cmath>
+#include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <vector>
 
@@ -190,6 +192,43 @@ void build_ops(int d, int p, Ops1D<double>& o, Ops1D<float>& o32)
 	cp(o.Ext[0], o32.Ext[0], 2 * kMaxN); cp(o.Lift[0], o32.Lift[0], 2 * kMaxN);
 	cp(o.w, o32.w, kMaxN); cp(o.wgll, o32.wgll, kMaxN);
 	std::memcpy(o32.perm, o.perm, sizeof(o.perm));
+}
+
+// Z-order (Morton) sort of element centroids: consecutive positions are spatial neighbours, so
+// an XCD walking a contiguous range of positions re-reads neighbour traces from its own L2.
+std::vector<int> morton_order(int d, int E, int Np, const double* xyz)
+{
+	std::vector<double> c((size_t)E * d);
+	double lo[3] = { 1e300, 1e300, 1e300 }, hi[3] = { -1e300, -1e300, -1e300 };
+	for (int e = 0; e < E; ++e)
+		for (int k = 0; k < d; ++k) {
+			double s = 0.0;
+			for (int n = 0; n < Np; ++n) s += xyz[((size_t)e * d + k) * Np + n];
+			s /= Np;
+			c[(size_t)e * d + k] = s;
+			lo[k] = std::min(lo[k], s);
+			hi[k] = std::max(hi[k], s);
+		}
+	const int bits = (d == 3) ? 21 : 31;
+	const double scale = (double)((1u << (bits > 30 ? 30 : bits)) - 1);
+	std::vector<std::pair<uint64_t, int>> key((size_t)E);
+	for (int e = 0; e < E; ++e) {
+		uint64_t q[3] = { 0, 0, 0 };
+		for (int k = 0; k < d; ++k) {
+			const double span = hi[k] - lo[k];
+			const double t = span > 0 ? (c[(size_t)e * d + k] - lo[k]) / span : 0.0;
+			q[k] = (uint64_t)(t * scale + 0.5);
+		}
+		uint64_t m = 0;
+		const int nb = (d == 3) ? 21 : 30;
+		for (int b = nb - 1; b >= 0; --b)
+			for (int k = d - 1; k >= 0; --k) m = (m << 1) | ((q[k] >> b) & 1u);
+		key[(size_t)e] = { m, e };
+	}
+	std::stable_sort(key.begin(), key.end());
+	std::vector<int> order((size_t)E);
+	for (int i = 0; i < E; ++i) order[(size_t)i] = key[(size_t)i].second;
+	return order;
 }
 
 }  // namespace dgx

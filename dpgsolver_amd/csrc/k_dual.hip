@@ -31,6 +31,7 @@ static RhsArgs<Dual, double> make_dual(Ctx& c, const double* u, const double* v,
 	a.detJ = c.detJ_vc;
 	a.nJ = c.nJ_fc;
 	a.ef = c.elem_faces;
+	a.order = c.order;
 	a.ops = c.ops64;
 	for (int k = 0; k < 5; ++k) a.bc_prim[k] = c.cfg.bc_prim[k];
 	for (int k = 0; k < 3; ++k) a.b_adv[k] = c.cfg.b_adv[k];

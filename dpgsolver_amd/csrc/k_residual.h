@@ -42,6 +42,7 @@ struct RhsArgs {
 	const R* detJ;
 	const R* nJ;
 	const ElemFace* ef;
+	const int* order;     // processing order (Morton order of element centroids)
 	const Ops1D<R>* ops;
 	double bc_prim[5];
 	double b_adv[3];
@@ -99,6 +100,85 @@ __device__ inline void sweep_inplace(S* buf, const R* M, int dir, int t)
 	__syncthreads();
 }
 
+// Two independent in-place sweeps over one buffer: region A (NB1 blocks of N^D1 at offset 0) and
+// region B (NB2 blocks of N^D2 at offset OFFB), sharing one read / barrier / write / barrier cycle.
+template <typename S, typename R, int D1, int N, int NB1, int D2, int NB2, int OFFB, int NT>
+__device__ inline void sweep_inplace2(S* buf, const R* M, int dir, int t)
+{
+	constexpr int BL1 = ipow(N, D1), NL1 = ipow(N, D1 - 1), NP1 = NB1 * NL1;
+	constexpr int BL2 = ipow(N, D2), NL2 = ipow(N, D2 - 1), NP2 = NB2 * NL2;
+	constexpr int NPEN = NP1 + NP2;
+	constexpr int PPT = (NPEN + NT - 1) / NT;
+	const int Sd = (dir == 0) ? 1 : (dir == 1 ? N : N * N);
+	S in[PPT][N];
+	int base[PPT];
+#pragma unroll
+	for (int k = 0; k < PPT; ++k) {
+		const int pen = t + k * NT;
+		int bs = -1;
+		if (pen < NP1) {
+			const int bb = pen / NL1, l = pen - bb * NL1;
+			bs = bb * BL1 + (l % Sd) + (l / Sd) * Sd * N;
+		} else if (pen < NPEN) {
+			const int p2 = pen - NP1;
+			const int bb = p2 / NL2, l = p2 - bb * NL2;
+			bs = OFFB + bb * BL2 + (l % Sd) + (l / Sd) * Sd * N;
+		}
+		base[k] = bs;
+		if (bs >= 0) {
+#pragma unroll
+			for (int q = 0; q < N; ++q) in[k][q] = buf[bs + q * Sd];
+		}
+	}
+	__syncthreads();
+#pragma unroll
+	for (int k = 0; k < PPT; ++k) {
+		if (base[k] >= 0) {
+#pragma unroll
+			for (int q = 0; q < N; ++q) {
+				S acc = M[q * N] * in[k][0];
+#pragma unroll
+				for (int j = 1; j < N; ++j) acc += M[q * N + j] * in[k][j];
+				buf[base[k] + q * Sd] = acc;
+			}
+		}
+	}
+	__syncthreads();
+}
+
+// 2-D interpolation (I along a, then along b) of NB blocks of N x N values in place, one block per
+// thread (the last NB threads of the element), no barrier needed: each block is private to its thread.
+template <typename S, typename R, int N, int NB, int NT>
+__device__ inline void face_block_interp(S* buf, const R* M, int t)
+{
+	const int bi = t - (NT - NB);
+	if (bi < 0) return;
+	S* blk = buf + bi * N * N;
+	S T[N][N];  // T[q][b] = sum_j M[q][j] blk[j + N b]
+#pragma unroll
+	for (int b = 0; b < N; ++b) {
+		S row[N];
+#pragma unroll
+		for (int j = 0; j < N; ++j) row[j] = blk[j + N * b];
+#pragma unroll
+		for (int q = 0; q < N; ++q) {
+			S acc = M[q * N] * row[0];
+#pragma unroll
+			for (int j = 1; j < N; ++j) acc += M[q * N + j] * row[j];
+			T[q][b] = acc;
+		}
+	}
+#pragma unroll
+	for (int p = 0; p < N; ++p)
+#pragma unroll
+		for (int q = 0; q < N; ++q) {
+			S acc = M[p * N] * T[q][0];
+#pragma unroll
+			for (int k = 1; k < N; ++k) acc += M[p * N + k] * T[q][k];
+			blk[q + N * p] = acc;
+		}
+}
+
 // dst (+)= sweep(src) along dir over NB blocks of N^DD values (src != dst).
 template <typename S, typename R, int DD, int N, int NB, int NT, bool ACC>
 __device__ inline void sweep_to(const S* src, S* dst, const R* M, int dir, int t)
@@ -134,7 +214,8 @@ template <typename S, int D, int N, int NV>
 constexpr int rhs_epb()
 {
 	constexpr int Np = ipow(N, D), NF = ipow(N, D - 1), NFACE = 2 * D;
-	constexpr int bytes = (int)sizeof(S) * (NV * Np * (1 + D) + 2 * NFACE * NV * NF) + NFACE * 16;
+	constexpr int trsz = NFACE * NV * NF + (NFACE * NF > Np ? NFACE * NF : Np) * NV;
+	constexpr int bytes = (int)sizeof(S) * (NV * Np + trsz) + NFACE * 16;
 	constexpr int by_lds = (56 * 1024) / bytes;
 	constexpr int by_thr = 256 / Np;
 	constexpr int e = by_lds < by_thr ? by_lds : by_thr;
@@ -152,15 +233,22 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 
 	__shared__ R sI[N * N], sIT[N * N], sDhT[N * N], sIinv[N * N], sLift[2][N], sW[N];
 	__shared__ int sPerm[8][NF];
-	__shared__ S sU[EPB][NV * Np];
-	__shared__ S sFR[EPB][D * NV * Np];
-	__shared__ S sTR[EPB][2 * NFACE * NV * NF];
+	// traces: own half, then the neighbour half which is reused as the flux staging buffer
+	constexpr int TRSZ = NFACE * NV * NF + (NFACE * NF > Np ? NFACE * NF : Np) * NV;
+	__shared__ S sBuf[EPB][NV * Np + TRSZ];   // volume block U, then the face traces TR
 	__shared__ ElemFace sEF[EPB][NFACE];
 	__shared__ unsigned long long sMax;
 
+	// XCD-aware schedule: the grid is a multiple of 8 and consecutive workgroups land on different
+	// XCDs, so XCD x takes the x-th contiguous range of the (Morton-ordered) element sequence and
+	// neighbour traces are read from its own L2.
+	const int per_xcd = gridDim.x >> 3;
+	const int lblk = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+	if (lblk * EPB >= a.E) return;  // whole workgroup idle (padding), before any barrier
 	const int slot = threadIdx.x / NT, t = threadIdx.x - slot * NT;
-	const int e = blockIdx.x * EPB + slot;
-	const bool valid = (slot < EPB) && (e < a.E);
+	const int pos = lblk * EPB + slot;
+	const bool valid = (slot < EPB) && (pos < a.E);
+	const int e = valid ? a.order[pos] : 0;
 	const int sl = slot < EPB ? slot : 0;
 
 	for (int i = threadIdx.x; i < N * N; i += blockDim.x) {
@@ -178,9 +266,9 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 	if (threadIdx.x == 0) sMax = 0ull;
 	if (valid && t < NFACE) sEF[sl][t] = a.ef[(size_t)e * NFACE + t];
 
-	S* U = sU[sl];
-	S* FR = sFR[sl];
-	S* TR = sTR[sl];
+	S* U = sBuf[sl];
+	S* TR = U + NV * Np;
+	S* BUF = TR + NFACE * NV * NF;   // neighbour-trace half, reused for the flux components
 	const size_t ebase = (size_t)e * NV * Np;
 	if (valid) {
 		for (int i = t; i < NV * Np; i += NT) {
@@ -191,56 +279,45 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 	}
 	__syncthreads();
 
-	// ---- traces at the GLL face nodes: own (from LDS) and neighbour (gathered from HBM)
+	// ---- traces at the GLL face nodes: own (from LDS) and neighbour (gathered from HBM);
+	//      one (face, node) item per thread, the variable loop reuses the address arithmetic
 	if (valid) {
-		for (int item = t; item < NFACE * NV * NF; item += NT) {
-			const int lf = item / (NV * NF), rem = item - lf * (NV * NF);
-			const int v = rem / NF, i = rem - v * NF;
-			TR[lf * NV * NF + rem] = U[v * Np + face_vol_node<D, N>(lf, i)];
+		for (int item = t; item < NFACE * NF; item += NT) {
+			const int lf = item / NF, i = item - lf * NF;
+			const int vown = face_vol_node<D, N>(lf, i);
 			const ElemFace ef = sEF[sl][lf];
-			S nb = S(0.0);
+			S* to = TR + lf * NV * NF + i;
+			S* tn = TR + (NFACE + lf) * NV * NF + i;
 			if (ef.en >= 0) {
-				const size_t gi = (size_t)ef.en * NV * Np + v * Np + face_vol_node<D, N>(ef.packed & 15, i);
-				if constexpr (MODE == MODE_JVP) nb = S(a.u[gi], a.v[gi]);
-				else nb = S(a.u[gi]);
+				const size_t gi = (size_t)ef.en * NV * Np + face_vol_node<D, N>(ef.packed & 15, i);
+#pragma unroll
+				for (int v = 0; v < NV; ++v) {
+					to[v * NF] = U[v * Np + vown];
+					if constexpr (MODE == MODE_JVP) tn[v * NF] = S(a.u[gi + v * Np], a.v[gi + v * Np]);
+					else tn[v * NF] = S(a.u[gi + v * Np]);
+				}
+			} else {
+#pragma unroll
+				for (int v = 0; v < NV; ++v) {
+					to[v * NF] = U[v * Np + vown];
+					tn[v * NF] = S(0.0);
+				}
 			}
-			TR[(NFACE + lf) * NV * NF + rem] = nb;
 		}
 	}
-	// ---- GLL -> GL: volume (D sweeps) and face traces (D-1 sweeps)
+	__syncthreads();  // traces written by other threads are read by the first sweep
+	// ---- GLL -> GL: volume (D sweeps) and face traces (D-1 sweeps) sharing barrier cycles
+	if constexpr (D == 3 && 2 * NFACE * NV <= NT) {
+		// face traces: one N x N block per thread (in place, private), then the volume sweeps
+		face_block_interp<S, R, N, 2 * NFACE * NV, NT>(TR, sI, t);
 #pragma unroll
-	for (int dir = 0; dir < D; ++dir) sweep_inplace<S, R, D, N, NV, NT>(U, sI, dir, t);
+		for (int dir = 0; dir < D; ++dir) sweep_inplace<S, R, D, N, NV, NT>(U, sI, dir, t);
+	} else {
 #pragma unroll
-	for (int dir = 0; dir < D - 1; ++dir) sweep_inplace<S, R, D - 1, N, 2 * NFACE * NV, NT>(TR, sI, dir, t);
-
-	// ---- physical flux at the volume cubature, contravariant and weighted: FR[d][v][q]
-	if (valid) {
-		const int q = t;
-		S s[NV];
-#pragma unroll
-		for (int v = 0; v < NV; ++v) s[v] = U[v * Np + q];
-		S f[D * NV];
-		if constexpr (PDE == DGX_PDE_EULER) euler_flux<S, D>(s, f);
-		else {
-#pragma unroll
-			for (int k = 0; k < D; ++k) f[k] = R(a.b_adv[k]) * s[0];
+		for (int dir = 0; dir < D; ++dir) {
+			if (dir < D - 1) sweep_inplace2<S, R, D, N, NV, D - 1, 2 * NFACE * NV, NV * Np, NT>(U, sI, dir, t);
+			else sweep_inplace<S, R, D, N, NV, NT>(U, sI, dir, t);
 		}
-		R m[D * D];
-		const R* mp = a.met + (size_t)e * D * D * Np;
-#pragma unroll
-		for (int i = 0; i < D * D; ++i) m[i] = mp[i * Np + q];
-		const int c0 = q % N, c1 = (q / N) % N, c2 = (D == 3) ? q / (N * N) : 0;
-		R wq = sW[c0] * sW[c1];
-		if (D == 3) wq = wq * sW[c2];
-#pragma unroll
-		for (int d0 = 0; d0 < D; ++d0)
-#pragma unroll
-			for (int v = 0; v < NV; ++v) {
-				S acc = m[0 + D * d0] * f[0 + D * v];
-#pragma unroll
-				for (int d1 = 1; d1 < D; ++d1) acc += m[d1 + D * d0] * f[d1 + D * v];
-				FR[(d0 * NV + v) * Np + q] = wq * acc;
-			}
 	}
 
 	// ---- numerical flux at the face cubature (canonical = left ordering)
@@ -275,9 +352,10 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 #pragma unroll
 			for (int k2 = 1; k2 < D; ++k2) J2 += nJv[k2] * nJv[k2];
 			const R Jf = sqrt(J2);
+			const R Jfi = R(1.0) / Jf;
 			S nu[D];
 #pragma unroll
-			for (int k2 = 0; k2 < D; ++k2) nu[k2] = S(nJv[k2] / Jf);
+			for (int k2 = 0; k2 < D; ++k2) nu[k2] = S(nJv[k2] * Jfi);
 			S nnf[NV];
 			if constexpr (PDE == DGX_PDE_EULER) {
 				if (ef.en < 0) bc_state<S, D>(bc, sL, nu, a.bc_prim, sR);
@@ -301,12 +379,45 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 #pragma unroll
 			for (int v = 0; v < NV; ++v) TR[dst[k] + v * NF] = res[k][v];
 		}
-	__syncthreads();
 
-	// ---- weak divergence in GL space: U = sum_d Dhat_d^T FR_d
-	sweep_to<S, R, D, N, NV, NT, false>(FR, U, sDhT, 0, t);
-	sweep_to<S, R, D, N, NV, NT, true>(FR + NV * Np, U, sDhT, 1, t);
-	if constexpr (D == 3) sweep_to<S, R, D, N, NV, NT, true>(FR + 2 * NV * Np, U, sDhT, 2, t);
+	// ---- physical flux at the volume cubature, contravariant and weighted, kept in registers
+	S fr[D][NV];
+	if (valid) {
+		const int q = t;
+		S s[NV];
+#pragma unroll
+		for (int v = 0; v < NV; ++v) s[v] = U[v * Np + q];
+		R m[D * D];
+		const R* mp = a.met + (size_t)e * D * D * Np;
+#pragma unroll
+		for (int i = 0; i < D * D; ++i) m[i] = mp[i * Np + q];
+		const int c0 = q % N, c1 = (q / N) % N, c2 = (D == 3) ? q / (N * N) : 0;
+		R wq = sW[c0] * sW[c1];
+		if (D == 3) wq = wq * sW[c2];
+		if constexpr (PDE == DGX_PDE_EULER) {
+			euler_contravariant_flux<S, R, D>(s, m, wq, fr);
+		} else {
+#pragma unroll
+			for (int d0 = 0; d0 < D; ++d0) {
+				R bm = R(a.b_adv[0]) * m[0 + D * d0];
+#pragma unroll
+				for (int k = 1; k < D; ++k) bm += R(a.b_adv[k]) * m[k + D * d0];
+				fr[d0][0] = wq * (bm * s[0]);
+			}
+		}
+	}
+
+	// ---- weak divergence in GL space: U = sum_d Dhat_d^T fr_d, one direction at a time through BUF
+#pragma unroll
+	for (int d0 = 0; d0 < D; ++d0) {
+		if (valid) {
+#pragma unroll
+			for (int v = 0; v < NV; ++v) BUF[v * Np + t] = fr[d0][v];
+		}
+		__syncthreads();
+		if (d0 == 0) sweep_to<S, R, D, N, NV, NT, false>(BUF, U, sDhT, 0, t);
+		else sweep_to<S, R, D, N, NV, NT, true>(BUF, U, sDhT, d0, t);
+	}
 
 	// ---- face lifts (and the mass inverse) at the GL nodes
 	if (valid) {
@@ -318,21 +429,26 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 			if (D == 3) wq = wq * sW[cc[2]];
 			scale = R(1.0) / (wq * a.detJ[(size_t)e * Np + q]);
 		}
+		// per-face coefficient Lift_side[c_fd] * W_f and face-node offset, shared by all variables
+		R cf[NFACE];
+		int of[NFACE];
+#pragma unroll
+		for (int lf = 0; lf < NFACE; ++lf) {
+			const int fd = lf >> 1;
+			int fa, fb;
+			if (fd == 0) { fa = cc[1]; fb = cc[2]; }
+			else if (fd == 1) { fa = cc[0]; fb = cc[2]; }
+			else { fa = cc[0]; fb = cc[1]; }
+			R wf = sW[fa];
+			if (D == 3) wf = wf * sW[fb];
+			cf[lf] = sLift[lf & 1][cc[fd]] * wf;
+			of[lf] = lf * NV * NF + ((D == 3) ? fa + N * fb : fa);
+		}
 #pragma unroll
 		for (int v = 0; v < NV; ++v) {
 			S acc = U[v * Np + q];
 #pragma unroll
-			for (int lf = 0; lf < NFACE; ++lf) {
-				const int fd = lf >> 1;
-				int fa, fb;
-				if (fd == 0) { fa = cc[1]; fb = cc[2]; }
-				else if (fd == 1) { fa = cc[0]; fb = cc[2]; }
-				else { fa = cc[0]; fb = cc[1]; }
-				const int qf = (D == 3) ? fa + N * fb : fa;
-				R wf = sW[fa];
-				if (D == 3) wf = wf * sW[fb];
-				acc -= (sLift[lf & 1][cc[fd]] * wf) * TR[lf * NV * NF + v * NF + qf];
-			}
+			for (int lf = 0; lf < NFACE; ++lf) acc -= cf[lf] * TR[of[lf] + v * NF];
 			if constexpr (MINV) acc = scale * acc;
 			U[v * Np + q] = acc;
 		}
@@ -368,8 +484,8 @@ inline hipError_t run_rhs(const RhsArgs<S, R>& a, hipStream_t st)
 {
 	constexpr int Np = ipow(N, D), NV = (PDE == DGX_PDE_EULER) ? D + 2 : 1;
 	constexpr int EPB = rhs_epb<S, D, N, NV>();
-	const int grid = (a.E + EPB - 1) / EPB;
-	if (grid == 0) return hipSuccess;
+	const int grid = (((a.E + EPB - 1) / EPB) + 7) / 8 * 8;
+	if (a.E == 0) return hipSuccess;
 	hipLaunchKernelGGL((k_rhs<S, R, D, N, PDE, FLUX, MODE>), dim3(grid), dim3(EPB * Np), 0, st, a);
 	return hipGetLastError();
 }

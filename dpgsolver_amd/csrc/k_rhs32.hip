@@ -14,6 +14,7 @@ hipError_t launch_rhs32(Ctx& c, const float* u, float* r)
 	a.detJ = c.detJ_vc32;
 	a.nJ = c.nJ_fc32;
 	a.ef = c.elem_faces;
+	a.order = c.order;
 	a.ops = c.ops32;
 	for (int k = 0; k < 5; ++k) a.bc_prim[k] = c.cfg.bc_prim[k];
 	for (int k = 0; k < 3; ++k) a.b_adv[k] = c.cfg.b_adv[k];

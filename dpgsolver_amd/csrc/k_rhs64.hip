@@ -13,6 +13,7 @@ static RhsArgs<double, double> make_args64(Ctx& c, const double* u, double* r, u
 	a.detJ = c.detJ_vc;
 	a.nJ = c.nJ_fc;
 	a.ef = c.elem_faces;
+	a.order = c.order;
 	a.ops = c.ops64;
 	for (int k = 0; k < 5; ++k) a.bc_prim[k] = c.cfg.bc_prim[k];
 	for (int k = 0; k < 3; ++k) a.b_adv[k] = c.cfg.b_adv[k];
