@@ -24,8 +24,11 @@ EXPORTS = [
     "dgx_sizes", "dgx_upload_mesh", "dgx_get_geometry", "dgx_compute_rhs", "dgx_compute_rhs_f32",
     "dgx_compute_rhs_explicit", "dgx_apply_minv", "dgx_enforce_positivity", "dgx_ssp_rk3_step", "dgx_max_abs",
     "dgx_convert_f64_f32", "dgx_convert_f32_f64", "dgx_assemble_diag_blocks", "dgx_jacobian_vector",
-    "dgx_block_lu_factor", "dgx_block_lu_solve",
+    "dgx_block_lu_factor", "dgx_block_lu_solve", "dgx_cfl_dt", "dgx_mass_sub", "dgx_solve_implicit",
 ]
+
+PC_NONE, PC_BLOCK_JACOBI, PC_LEARNED = 0, 1, 2
+HISTORY_MAX = 256
 
 
 class DgxError(RuntimeError):
@@ -41,6 +44,18 @@ class DgxConfig(C.Structure):
 
 class DgxMesh(C.Structure):
     _fields_ = [("E", C.c_int), ("F", C.c_int), ("xyz", C.POINTER(C.c_double)), ("faces", C.POINTER(C.c_int))]
+
+
+class ImplicitParams(C.Structure):
+    _fields_ = [("cfl_initial", C.c_double), ("exit_tol", C.c_double), ("exit_ratio", C.c_double),
+                ("max_newton", C.c_int), ("gmres_restart", C.c_int), ("gmres_rtol", C.c_double),
+                ("gmres_max_iter", C.c_int), ("precond", C.c_int)]
+
+
+class ImplicitStats(C.Structure):
+    _fields_ = [("newton_steps", C.c_int), ("gmres_iters", C.c_int), ("gmres_not_converged", C.c_int),
+                ("converged", C.c_int), ("max_rhs0", C.c_double), ("max_rhs", C.c_double),
+                ("seconds", C.c_double), ("history", C.c_double * HISTORY_MAX)]
 
 
 _lib = None
@@ -74,7 +89,10 @@ def load():
         "dgx_max_abs": [vp, vp, C.c_longlong, dp],
         "dgx_convert_f64_f32": [vp, vp, vp, C.c_longlong],
         "dgx_convert_f32_f64": [vp, vp, vp, C.c_longlong],
-        "dgx_assemble_diag_blocks": [vp, vp, C.c_double, vp],
+        "dgx_assemble_diag_blocks": [vp, vp, vp, vp],
+        "dgx_cfl_dt": [vp, vp, C.c_double, vp],
+        "dgx_mass_sub": [vp, vp, vp, vp],
+        "dgx_solve_implicit": [vp, vp, C.POINTER(ImplicitParams), C.POINTER(ImplicitStats)],
         "dgx_jacobian_vector": [vp, vp, vp, vp],
         "dgx_block_lu_factor": [vp, vp, vp, C.c_int, C.c_int],
         "dgx_block_lu_solve": [vp, vp, vp, vp, C.c_int, C.c_int],
@@ -183,8 +201,31 @@ class Context:
     def convert_f32_f64(self, src: int, dst: int, n: int):
         check("dgx_convert_f32_f64", self.L.dgx_convert_f32_f64(self.h, vp(src), vp(dst), n))
 
-    def assemble_diag_blocks(self, u: int, mass_scale: float, blocks: int):
-        check("dgx_assemble_diag_blocks", self.L.dgx_assemble_diag_blocks(self.h, vp(u), mass_scale, vp(blocks)))
+    def assemble_diag_blocks(self, u: int, mass_scale: int | None, blocks: int):
+        """mass_scale: device pointer to E doubles (1/dt per element) or None."""
+        check("dgx_assemble_diag_blocks",
+              self.L.dgx_assemble_diag_blocks(self.h, vp(u), vp(mass_scale or 0), vp(blocks)))
+
+    def cfl_dt(self, u: int, cfl: float, inv_dt: int):
+        check("dgx_cfl_dt", self.L.dgx_cfl_dt(self.h, vp(u), cfl, vp(inv_dt)))
+
+    def mass_sub(self, x: int, scale: int | None, y: int):
+        check("dgx_mass_sub", self.L.dgx_mass_sub(self.h, vp(x), vp(scale or 0), vp(y)))
+
+    def solve_implicit(self, u: int, cfl_initial=10.0, exit_tol=1e-10, exit_ratio=0.0, max_newton=100,
+                       gmres_restart=60, gmres_rtol=1e-4, gmres_max_iter=600, precond=PC_BLOCK_JACOBI,
+                       raise_on_error=True):
+        p = ImplicitParams(cfl_initial, exit_tol, exit_ratio, max_newton, gmres_restart, gmres_rtol, gmres_max_iter,
+                           precond)
+        st = ImplicitStats()
+        rc = self.L.dgx_solve_implicit(self.h, vp(u), C.byref(p), C.byref(st))
+        if raise_on_error and rc != DGX_OK:
+            raise DgxError("dgx_solve_implicit", rc)
+        out = {k: getattr(st, k) for k in ("newton_steps", "gmres_iters", "gmres_not_converged", "converged",
+                                           "max_rhs0", "max_rhs", "seconds")}
+        out["history"] = list(st.history[: min(st.newton_steps + 1, HISTORY_MAX)])
+        out["status"] = rc
+        return out
 
     def jacobian_vector(self, u: int, v: int, jv: int):
         check("dgx_jacobian_vector", self.L.dgx_jacobian_vector(self.h, vp(u), vp(v), vp(jv)))

@@ -8,10 +8,6 @@
 
 #include "dgx_internal.h"
 
-struct dgx_ctx {
-	dgx::Ctx c;
-};
-
 using dgx::Ctx;
 
 namespace {
@@ -40,7 +36,7 @@ void release(T*& p)
 
 void free_mesh(Ctx& c)
 {
-	release(c.xyz); release(c.faces); release(c.elem_faces); release(c.order); release(c.metrics_vc); release(c.metrics_vc32);
+	release(c.xyz); release(c.faces); release(c.elem_faces); release(c.order); release(c.hmin); release(c.metrics_vc); release(c.metrics_vc32);
 	release(c.detJ_vc); release(c.detJ_vc32); release(c.nJ_fc); release(c.nJ_fc32); release(c.u0); release(c.rtmp);
 	c.mesh_ready = false;
 }
@@ -178,6 +174,9 @@ dgx_status dgx_upload_mesh(dgx_ctx* h, const dgx_mesh* m)
 		const std::vector<int> order = dgx::morton_order(d, c.E, c.Np, m->xyz);
 		DGX_TRY(hipMalloc(&c.order, sizeof(int) * order.size()));
 		DGX_TRY(hipMemcpy(c.order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
+		const std::vector<double> hm = dgx::min_edge_lengths(d, c.N, c.E, m->xyz);
+		DGX_TRY(hipMalloc(&c.hmin, sizeof(double) * hm.size()));
+		DGX_TRY(hipMemcpy(c.hmin, hm.data(), sizeof(double) * hm.size(), hipMemcpyHostToDevice));
 	}
 	DGX_TRY(dgx::launch_geometry(c));
 	DGX_TRY(dgx::launch_convert(c.metrics_vc, c.metrics_vc32, nm, c.stream));
@@ -305,7 +304,7 @@ dgx_status dgx_convert_f32_f64(dgx_ctx* h, const float* src, double* dst, long l
 	return hip_status(dgx::launch_convert_back(src, dst, (size_t)n, h->c.stream));
 }
 
-dgx_status dgx_assemble_diag_blocks(dgx_ctx* h, const double* u, double mass_scale, double* blocks)
+dgx_status dgx_assemble_diag_blocks(dgx_ctx* h, const double* u, const double* mass_scale, double* blocks)
 {
 	if (!h || !u || !blocks) return DGX_ERR_ARG;
 	Ctx& c = h->c;
@@ -313,8 +312,26 @@ dgx_status dgx_assemble_diag_blocks(dgx_ctx* h, const double* u, double mass_sca
 	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
 	const int nb = c.Np * c.nv;
 	for (int col = 0; col < nb; ++col) DGX_TRY(dgx::launch_diag_col(c, u, blocks, col));
-	if (mass_scale != 0.0) DGX_TRY(dgx::launch_sub_mass(c, blocks, mass_scale));
+	if (mass_scale) DGX_TRY(dgx::launch_sub_mass(c, blocks, mass_scale, 0.0));
 	return DGX_OK;
+}
+
+dgx_status dgx_cfl_dt(dgx_ctx* h, const double* u, double cfl, double* inv_dt)
+{
+	if (!h || !u || !inv_dt || !(cfl > 0.0)) return DGX_ERR_ARG;
+	Ctx& c = h->c;
+	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (c.d != 2 || c.nv != 4) return DGX_ERR_UNSUPPORTED;
+	return hip_status(dgx::launch_cfl_dt(c, u, cfl, inv_dt));
+}
+
+dgx_status dgx_mass_sub(dgx_ctx* h, const double* x, const double* scale, double* y)
+{
+	if (!h || !x || !y) return DGX_ERR_ARG;
+	Ctx& c = h->c;
+	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
+	return hip_status(dgx::launch_mass_sub(c, x, scale, y));
 }
 
 dgx_status dgx_jacobian_vector(dgx_ctx* h, const double* u, const double* v, double* jv)

@@ -1,0 +1,565 @@
+// dgx_implicit.hip -- pseudo-time Newton-GMRES for steady problems (solve_implicit.c:95-110, 232-281).
+//
+// Per Newton step: r = R(u); max|r| exit test (check_exit, solve_implicit.c:258-281); CFL-ramped
+// per-element dt = cfl * h_min / max wave speed (compute_dt_cfl_constrained, solve_dg.c:284-319);
+// linear system (dR/du - M/dt) du = -r (compute_CFL_ramping solve_dg.c:223-248, fill_petsc_Vec_b_dg
+// :203-221) solved by right-preconditioned restarted GMRES (the reference uses PETSc GMRES(60),
+// rtol 1e-4, external/petsc/options_files/petsc_options_gmres_default.txt) with a block-Jacobi LU or a
+// learned preconditioner; update u += relax du with positivity under-relaxation (update_coef /
+// compute_relaxation, solve_implicit.c:780-873).  The operator is matrix-free: exact J v by dual numbers,
+// M v sum-factorised.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "dgx_device.h"
+#include "dgx_internal.h"
+
+namespace dgx {
+
+template <int D, int N>
+__device__ inline void coords2(int n, int* c)
+{
+	c[0] = n % N;
+	c[1] = (D > 1) ? (n / N) % N : 0;
+	c[2] = (D > 2) ? n / (N * N) : 0;
+}
+template <int D, int N>
+__device__ inline double apply1d_(const double* M, const double* buf, int n, int k)
+{
+	int c[3];
+	coords2<D, N>(n, c);
+	double acc = 0.0;
+	for (int j = 0; j < N; ++j) {
+		int cc[3] = { c[0], c[1], c[2] };
+		cc[k] = j;
+		acc += M[c[k] * N + j] * buf[cc[0] + N * (cc[1] + N * cc[2])];
+	}
+	return acc;
+}
+
+// y_e -= s_e * M_e x_e with M_e = Ibar^T diag(w J) Ibar per variable
+template <int D, int N, int NV>
+__global__ void __launch_bounds__(128) k_mass_sub(const double* __restrict__ x, const double* __restrict__ detJ,
+                                                  const double* __restrict__ scale, const Ops1D<double>* ops, int E,
+                                                  double* y)
+{
+	constexpr int Np = ipow(N, D);
+	__shared__ double B0[NV * Np], B1[NV * Np], sI[N * N], sIT[N * N], sW[N];
+	const int e = blockIdx.x, n = threadIdx.x;
+	for (int i = threadIdx.x; i < N * N; i += blockDim.x) { sI[i] = ops->I[i]; sIT[i] = ops->IT[i]; }
+	for (int i = threadIdx.x; i < N; i += blockDim.x) sW[i] = ops->w[i];
+	for (int i = threadIdx.x; i < NV * Np; i += blockDim.x) B0[i] = x[(size_t)e * NV * Np + i];
+	__syncthreads();
+	double* cur = B0;
+	double* nxt = B1;
+	for (int pass = 0; pass < 2 * D; ++pass) {
+		const double* M = pass < D ? sI : sIT;
+		if (n < Np)
+			for (int v = 0; v < NV; ++v) nxt[v * Np + n] = apply1d_<D, N>(M, cur + v * Np, n, pass % D);
+		__syncthreads();
+		if (pass == D - 1 && n < Np) {
+			int c[3];
+			coords2<D, N>(n, c);
+			double wq = sW[c[0]] * sW[c[1]];
+			if (D == 3) wq *= sW[c[2]];
+			const double s = wq * detJ[(size_t)e * Np + n];
+			for (int v = 0; v < NV; ++v) nxt[v * Np + n] *= s;
+		}
+		__syncthreads();
+		double* t = cur; cur = nxt; nxt = t;
+	}
+	const double s = scale ? scale[e] : 1.0;
+	for (int i = threadIdx.x; i < NV * Np; i += blockDim.x) y[(size_t)e * NV * Np + i] -= s * cur[i];
+}
+
+// per-element dt = cfl * h_min / max(|V| + c) over the Bezier coefficients (solve_dg.c:284-319)
+template <int D, int N>
+__global__ void __launch_bounds__(128) k_cfl_dt(const double* __restrict__ u, const double* __restrict__ hmin,
+                                                const Ops1D<double>* ops, int E, double cfl, double* inv_dt)
+{
+	constexpr int Np = ipow(N, D), NV = D + 2;
+	__shared__ double B0[NV * Np], B1[NV * Np], sBinv[N * N], sWs[Np];
+	const int e = blockIdx.x, n = threadIdx.x;
+	for (int i = threadIdx.x; i < N * N; i += blockDim.x) sBinv[i] = ops->Binv[i];
+	for (int i = threadIdx.x; i < NV * Np; i += blockDim.x) B0[i] = u[(size_t)e * NV * Np + i];
+	__syncthreads();
+	double* cur = B0;
+	double* nxt = B1;
+	for (int k = 0; k < D; ++k) {
+		if (n < Np)
+			for (int v = 0; v < NV; ++v) nxt[v * Np + n] = apply1d_<D, N>(sBinv, cur + v * Np, n, k);
+		__syncthreads();
+		double* t = cur; cur = nxt; nxt = t;
+	}
+	if (n < Np) {
+		const double rho = cur[n], ri = 1.0 / rho;
+		double V2 = 0.0;
+		for (int k = 0; k < D; ++k) { const double w = cur[(1 + k) * Np + n] * ri; V2 += w * w; }
+		const double p = kGM1 * (cur[(D + 1) * Np + n] - 0.5 * rho * V2);
+		sWs[n] = sqrt(V2) + sqrt(kGamma * p * ri);
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		double m = 0.0;
+		for (int i = 0; i < Np; ++i) m = fmax(m, sWs[i]);
+		inv_dt[e] = m / (cfl * hmin[e]);
+	}
+}
+
+// u_e += relax_e du_e, relax halved until the Bezier-coefficient means of rho and p exceed EPS_PHYS
+// (compute_relaxation, solve_implicit.c:823-873)
+template <int D, int N>
+__global__ void __launch_bounds__(128) k_relaxed_update(double* u, const double* __restrict__ du,
+                                                       const Ops1D<double>* ops, int E, int* status,
+                                                       double* relax_out)
+{
+	constexpr int Np = ipow(N, D), NV = D + 2;
+	__shared__ double Bu[NV * Np], Bd[NV * Np], T[NV * Np], sBinv[N * N];
+	__shared__ double sRelax;
+	const int e = blockIdx.x, n = threadIdx.x;
+	for (int i = threadIdx.x; i < N * N; i += blockDim.x) sBinv[i] = ops->Binv[i];
+	for (int i = threadIdx.x; i < NV * Np; i += blockDim.x) {
+		Bu[i] = u[(size_t)e * NV * Np + i];
+		Bd[i] = du[(size_t)e * NV * Np + i];
+	}
+	__syncthreads();
+	for (int arr = 0; arr < 2; ++arr) {
+		double* X = arr == 0 ? Bu : Bd;
+		for (int k = 0; k < D; ++k) {
+			if (n < Np)
+				for (int v = 0; v < NV; ++v) T[v * Np + n] = apply1d_<D, N>(sBinv, X + v * Np, n, k);
+			__syncthreads();
+			if (n < Np)
+				for (int v = 0; v < NV; ++v) X[v * Np + n] = T[v * Np + n];
+			__syncthreads();
+		}
+	}
+	if (threadIdx.x == 0) {
+		double relax = 1.0;
+		for (;;) {
+			double ar = 0.0, ap = 0.0;
+			for (int i = 0; i < Np; ++i) {
+				const double rho = Bu[i] + relax * Bd[i];
+				double V2 = 0.0;
+				for (int k = 0; k < D; ++k) {
+					const double w = (Bu[(1 + k) * Np + i] + relax * Bd[(1 + k) * Np + i]) / rho;
+					V2 += w * w;
+				}
+				ar += rho;
+				ap += kGM1 * ((Bu[(D + 1) * Np + i] + relax * Bd[(D + 1) * Np + i]) - 0.5 * rho * V2);
+			}
+			ar /= Np;
+			ap /= Np;
+			if (!(ar < kEpsPhys || ap < kEpsPhys)) break;
+			relax *= 0.5;
+			if (relax < kEps) { atomicOr(status, 1); break; }
+		}
+		sRelax = relax;
+		if (relax_out) relax_out[e] = relax;
+	}
+	__syncthreads();
+	const double r = sRelax;
+	for (int i = threadIdx.x; i < NV * Np; i += blockDim.x) u[(size_t)e * NV * Np + i] += r * du[(size_t)e * NV * Np + i];
+}
+
+// ------------------------------------------------------------------------------- BLAS-1 (deterministic)
+constexpr int kChunks = 64;
+
+// partial[j*kChunks + c] = sum over chunk c of V_j . w   (j < k)
+__global__ void __launch_bounds__(256) k_dots_partial(const double* __restrict__ V, size_t ldv,
+                                                      const double* __restrict__ w, size_t n, double* partial)
+{
+	const int j = blockIdx.y, c = blockIdx.x;
+	const size_t chunk = (n + kChunks - 1) / kChunks;
+	const size_t lo = (size_t)c * chunk, hi = (lo + chunk < n) ? lo + chunk : n;
+	const double* vj = V + (size_t)j * ldv;
+	double s = 0.0;
+	for (size_t i = lo + threadIdx.x; i < hi; i += blockDim.x) s += vj[i] * w[i];
+	__shared__ double red[256];
+	red[threadIdx.x] = s;
+	__syncthreads();
+	for (int off = 128; off > 0; off >>= 1) {
+		if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) partial[(size_t)j * kChunks + c] = red[0];
+}
+__global__ void k_dots_final(const double* __restrict__ partial, int k, double* out)
+{
+	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= k) return;
+	double s = 0.0;
+	for (int c = 0; c < kChunks; ++c) s += partial[(size_t)j * kChunks + c];
+	out[j] = s;
+}
+// w -= V h (k vectors)
+__global__ void k_gemv_sub(const double* __restrict__ V, size_t ldv, const double* __restrict__ h, int k, double* w,
+                           size_t n)
+{
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+		double s = 0.0;
+		for (int j = 0; j < k; ++j) s += V[(size_t)j * ldv + i] * h[j];
+		w[i] -= s;
+	}
+}
+// y = a x + b y
+__global__ void k_axpby(double a, const double* __restrict__ x, double b, double* y, size_t n)
+{
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+		y[i] = a * x[i] + (b == 0.0 ? 0.0 : b * y[i]);
+}
+// y = V c (k vectors), accumulate
+__global__ void k_gemv_add(const double* __restrict__ V, size_t ldv, const double* __restrict__ c, int k, double* y,
+                           size_t n)
+{
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+		double s = 0.0;
+		for (int j = 0; j < k; ++j) s += V[(size_t)j * ldv + i] * c[j];
+		y[i] += s;
+	}
+}
+
+static int grid_for(size_t n) { return (int)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096); }
+
+struct Blas {
+	hipStream_t s;
+	double* partial = nullptr;  // kChunks * kmax
+	double* dvec = nullptr;     // kmax
+	std::vector<double> h;
+	int kmax = 0;
+	hipError_t init(hipStream_t st, int k)
+	{
+		s = st;
+		kmax = k;
+		h.resize(k);
+		DGX_HIP_CHECK(hipMalloc(&partial, sizeof(double) * kChunks * k));
+		DGX_HIP_CHECK(hipMalloc(&dvec, sizeof(double) * k));
+		return hipSuccess;
+	}
+	void release()
+	{
+		if (partial) (void)hipFree(partial);
+		if (dvec) (void)hipFree(dvec);
+		partial = dvec = nullptr;
+	}
+	// host h[0..k) = V^T w
+	hipError_t dots(const double* V, size_t ldv, int k, const double* w, size_t n, double* hout)
+	{
+		hipLaunchKernelGGL(k_dots_partial, dim3(kChunks, k), dim3(256), 0, s, V, ldv, w, n, partial);
+		hipLaunchKernelGGL(k_dots_final, dim3((k + 63) / 64), dim3(64), 0, s, partial, k, dvec);
+		DGX_HIP_CHECK(hipGetLastError());
+		DGX_HIP_CHECK(hipMemcpyAsync(hout, dvec, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+		return hipStreamSynchronize(s);
+	}
+	hipError_t gemv_sub(const double* V, size_t ldv, const double* hh, int k, double* w, size_t n)
+	{
+		DGX_HIP_CHECK(hipMemcpyAsync(dvec, hh, sizeof(double) * k, hipMemcpyHostToDevice, s));
+		hipLaunchKernelGGL(k_gemv_sub, dim3(grid_for(n)), dim3(256), 0, s, V, ldv, dvec, k, w, n);
+		return hipGetLastError();
+	}
+	hipError_t gemv_add(const double* V, size_t ldv, const double* cc, int k, double* y, size_t n)
+	{
+		DGX_HIP_CHECK(hipMemcpyAsync(dvec, cc, sizeof(double) * k, hipMemcpyHostToDevice, s));
+		hipLaunchKernelGGL(k_gemv_add, dim3(grid_for(n)), dim3(256), 0, s, V, ldv, dvec, k, y, n);
+		return hipGetLastError();
+	}
+	hipError_t axpby(double a, const double* x, double b, double* y, size_t n)
+	{
+		hipLaunchKernelGGL(k_axpby, dim3(grid_for(n)), dim3(256), 0, s, a, x, b, y, n);
+		return hipGetLastError();
+	}
+	hipError_t norm(const double* x, size_t n, double* out)
+	{
+		double v;
+		DGX_HIP_CHECK(dots(x, 0, 1, x, n, &v));
+		*out = std::sqrt(v);
+		return hipSuccess;
+	}
+};
+
+#define DGX_PER_ELEM2(KNAME, BLK, ...)                                                               \
+	switch (c.d * 16 + c.N) {                                                                       \
+	case 2 * 16 + 2: hipLaunchKernelGGL((KNAME<2, 2 __VA_ARGS__>), dim3(c.E), dim3(BLK), 0, c.stream, ARGS); break; \
+	case 2 * 16 + 3: hipLaunchKernelGGL((KNAME<2, 3 __VA_ARGS__>), dim3(c.E), dim3(BLK), 0, c.stream, ARGS); break; \
+	case 2 * 16 + 4: hipLaunchKernelGGL((KNAME<2, 4 __VA_ARGS__>), dim3(c.E), dim3(BLK), 0, c.stream, ARGS); break; \
+	case 2 * 16 + 5: hipLaunchKernelGGL((KNAME<2, 5 __VA_ARGS__>), dim3(c.E), dim3(BLK), 0, c.stream, ARGS); break; \
+	case 2 * 16 + 6: hipLaunchKernelGGL((KNAME<2, 6 __VA_ARGS__>), dim3(c.E), dim3(BLK), 0, c.stream, ARGS); break; \
+	default: return hipErrorInvalidValue;                                                       \
+	}
+
+hipError_t launch_mass_sub(Ctx& c, const double* x, const double* scale, double* y)
+{
+	if (c.E == 0) return hipSuccess;
+	if (c.d != 2) return hipErrorInvalidValue;
+#define ARGS x, c.detJ_vc, scale, c.ops64, c.E, y
+	if (c.nv == 1) { DGX_PER_ELEM2(k_mass_sub, 64, , 1) }
+	else { DGX_PER_ELEM2(k_mass_sub, 64, , 4) }
+#undef ARGS
+	return hipGetLastError();
+}
+
+hipError_t launch_cfl_dt(Ctx& c, const double* u, double cfl, double* inv_dt)
+{
+	if (c.d != 2 || c.nv != 4) return hipErrorInvalidValue;
+#define ARGS u, c.hmin, c.ops64, c.E, cfl, inv_dt
+	DGX_PER_ELEM2(k_cfl_dt, 64, )
+#undef ARGS
+	return hipGetLastError();
+}
+
+hipError_t launch_relaxed_update(Ctx& c, double* u, const double* du, int* status)
+{
+	if (c.d != 2 || c.nv != 4) return hipErrorInvalidValue;
+#define ARGS u, du, c.ops64, c.E, status, (double*)nullptr
+	DGX_PER_ELEM2(k_relaxed_update, 64, )
+#undef ARGS
+	return hipGetLastError();
+}
+
+}  // namespace dgx
+
+// ------------------------------------------------------------------------------- preconditioners
+namespace dgx {
+
+struct Precond {
+	virtual ~Precond() {}
+	virtual hipError_t setup(Ctx& c, const double* u, const double* inv_dt) = 0;
+	virtual hipError_t apply(Ctx& c, const double* v, double* z) = 0;  // z = P^-1 v
+};
+
+struct PrecondIdentity : Precond {
+	hipError_t setup(Ctx&, const double*, const double*) override { return hipSuccess; }
+	hipError_t apply(Ctx& c, const double* v, double* z) override
+	{
+		return hipMemcpyAsync(z, v, sizeof(double) * (size_t)c.E * c.Np * c.nv, hipMemcpyDeviceToDevice, c.stream);
+	}
+};
+
+// exact LU of the element diagonal blocks of (dR/du - M/dt): the block-Jacobi baseline
+struct PrecondBlockJacobi : Precond {
+	double* blocks = nullptr;
+	int* piv = nullptr;
+	~PrecondBlockJacobi() override
+	{
+		if (blocks) (void)hipFree(blocks);
+		if (piv) (void)hipFree(piv);
+	}
+	hipError_t setup(Ctx& c, const double* u, const double* inv_dt) override
+	{
+		const int nb = c.Np * c.nv;
+		if (!blocks) {
+			DGX_HIP_CHECK(hipMalloc(&blocks, sizeof(double) * (size_t)c.E * nb * nb));
+			DGX_HIP_CHECK(hipMalloc(&piv, sizeof(int) * (size_t)c.E * nb));
+		}
+		for (int col = 0; col < nb; ++col) DGX_HIP_CHECK(launch_diag_col(c, u, blocks, col));
+		DGX_HIP_CHECK(launch_sub_mass(c, blocks, inv_dt, 0.0));
+		return launch_block_lu(c.stream, blocks, piv, c.E, nb);
+	}
+	hipError_t apply(Ctx& c, const double* v, double* z) override
+	{
+		const int nb = c.Np * c.nv;
+		DGX_HIP_CHECK(hipMemcpyAsync(z, v, sizeof(double) * (size_t)c.E * nb, hipMemcpyDeviceToDevice, c.stream));
+		return launch_block_lu_solve(c.stream, blocks, piv, z, c.E, nb);
+	}
+};
+
+Precond* make_learned_precond(Ctx& c);  // dgx_mlp.hip
+
+// ------------------------------------------------------------------------------- GMRES
+struct Gmres {
+	int m = 60;
+	double* V = nullptr;  // (m+1) x n
+	double* w = nullptr;
+	double* z = nullptr;
+	double* x = nullptr;
+	size_t n = 0;
+	Blas blas;
+	std::vector<double> H, g, cs, sn, y, h1, h2;
+	hipError_t init(Ctx& c, int restart)
+	{
+		m = restart;
+		n = (size_t)c.E * c.Np * c.nv;
+		DGX_HIP_CHECK(hipMalloc(&V, sizeof(double) * (size_t)(m + 1) * n));
+		DGX_HIP_CHECK(hipMalloc(&w, sizeof(double) * n));
+		DGX_HIP_CHECK(hipMalloc(&z, sizeof(double) * n));
+		DGX_HIP_CHECK(hipMalloc(&x, sizeof(double) * n));
+		H.assign((size_t)(m + 1) * m, 0.0);
+		g.assign(m + 1, 0.0);
+		cs.assign(m, 0.0);
+		sn.assign(m, 0.0);
+		y.assign(m, 0.0);
+		h1.assign(m + 1, 0.0);
+		h2.assign(m + 1, 0.0);
+		return blas.init(c.stream, m + 1);
+	}
+	void release()
+	{
+		for (double* p : { V, w, z, x })
+			if (p) (void)hipFree(p);
+		V = w = z = x = nullptr;
+		blas.release();
+	}
+	// w = A v with A = dR/du(u) - M/dt
+	hipError_t op(Ctx& c, const double* u, const double* inv_dt, const double* v, double* out)
+	{
+		DGX_HIP_CHECK(launch_jvp(c, u, v, out));
+		return launch_mass_sub(c, v, inv_dt, out);
+	}
+	// right-preconditioned restarted GMRES on A x = b, x0 = 0; returns iterations, sets converged
+	hipError_t solve(Ctx& c, const double* u, const double* inv_dt, const double* b, Precond& P, double rtol,
+	                 int max_it, int* iters, bool* converged)
+	{
+		DGX_HIP_CHECK(blas.axpby(0.0, b, 0.0, x, n));
+		double bnorm;
+		DGX_HIP_CHECK(blas.norm(b, n, &bnorm));
+		*iters = 0;
+		*converged = (bnorm == 0.0);
+		if (*converged) return hipSuccess;
+		int total = 0;
+		while (total < max_it) {
+			// r = b - A x  (x = 0 on the first cycle)
+			if (total == 0) {
+				DGX_HIP_CHECK(hipMemcpyAsync(V, b, sizeof(double) * n, hipMemcpyDeviceToDevice, c.stream));
+			} else {
+				DGX_HIP_CHECK(op(c, u, inv_dt, x, w));
+				DGX_HIP_CHECK(blas.axpby(1.0, b, 0.0, V, n));
+				DGX_HIP_CHECK(blas.axpby(-1.0, w, 1.0, V, n));
+			}
+			double beta;
+			DGX_HIP_CHECK(blas.norm(V, n, &beta));
+			if (beta <= rtol * bnorm) { *converged = true; break; }
+			DGX_HIP_CHECK(blas.axpby(1.0 / beta, V, 0.0, V, n));
+			std::fill(g.begin(), g.end(), 0.0);
+			g[0] = beta;
+			int j = 0;
+			for (; j < m && total < max_it; ++j, ++total) {
+				DGX_HIP_CHECK(P.apply(c, V + (size_t)j * n, z));
+				DGX_HIP_CHECK(op(c, u, inv_dt, z, w));
+				// classical Gram-Schmidt, twice (CGS2)
+				DGX_HIP_CHECK(blas.dots(V, n, j + 1, w, n, h1.data()));
+				DGX_HIP_CHECK(blas.gemv_sub(V, n, h1.data(), j + 1, w, n));
+				DGX_HIP_CHECK(blas.dots(V, n, j + 1, w, n, h2.data()));
+				DGX_HIP_CHECK(blas.gemv_sub(V, n, h2.data(), j + 1, w, n));
+				double hn;
+				DGX_HIP_CHECK(blas.norm(w, n, &hn));
+				for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = h1[i] + h2[i];
+				H[(size_t)(j + 1) * m + j] = hn;
+				if (hn > 0.0) DGX_HIP_CHECK(blas.axpby(1.0 / hn, w, 0.0, V + (size_t)(j + 1) * n, n));
+				for (int i = 0; i < j; ++i) {
+					const double a = H[(size_t)i * m + j], bb = H[(size_t)(i + 1) * m + j];
+					H[(size_t)i * m + j] = cs[i] * a + sn[i] * bb;
+					H[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * bb;
+				}
+				const double a = H[(size_t)j * m + j], bb = H[(size_t)(j + 1) * m + j];
+				const double r = std::sqrt(a * a + bb * bb);
+				cs[j] = r > 0 ? a / r : 1.0;
+				sn[j] = r > 0 ? bb / r : 0.0;
+				H[(size_t)j * m + j] = r;
+				H[(size_t)(j + 1) * m + j] = 0.0;
+				g[j + 1] = -sn[j] * g[j];
+				g[j] = cs[j] * g[j];
+				if (std::fabs(g[j + 1]) <= rtol * bnorm || hn == 0.0) { ++j; ++total; *converged = true; break; }
+			}
+			// y = H^-1 g, x += P^-1 (V y)
+			for (int i = j - 1; i >= 0; --i) {
+				double s = g[i];
+				for (int k = i + 1; k < j; ++k) s -= H[(size_t)i * m + k] * y[k];
+				y[i] = s / H[(size_t)i * m + i];
+			}
+			DGX_HIP_CHECK(blas.axpby(0.0, V, 0.0, w, n));
+			DGX_HIP_CHECK(blas.gemv_add(V, n, y.data(), j, w, n));
+			DGX_HIP_CHECK(P.apply(c, w, z));
+			DGX_HIP_CHECK(blas.axpby(1.0, z, 1.0, x, n));
+			if (*converged) break;
+		}
+		*iters = total;
+		return hipSuccess;
+	}
+};
+
+}  // namespace dgx
+
+using dgx::Ctx;
+
+extern "C" dgx_status dgx_solve_implicit(dgx_ctx* h, double* u, const dgx_implicit_params* prm, dgx_implicit_stats* st)
+{
+	if (!h || !u || !prm || !st) return DGX_ERR_ARG;
+	Ctx& c = h->c;
+	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (c.d != 2 || c.cfg.pde != DGX_PDE_EULER) return DGX_ERR_UNSUPPORTED;
+	if (prm->gmres_restart < 1 || prm->gmres_restart > 200 || prm->max_newton < 1 || !(prm->cfl_initial > 0.0))
+		return DGX_ERR_ARG;
+	std::memset(st, 0, sizeof(*st));
+	const auto t0 = std::chrono::steady_clock::now();
+	const size_t n = (size_t)c.E * c.Np * c.nv;
+	double *r = nullptr, *inv_dt = nullptr;
+	int* status = nullptr;
+	dgx::Gmres gm;
+	dgx::Precond* P = nullptr;
+	dgx_status ret = DGX_OK;
+	auto fail = [&](dgx_status s) { ret = s; };
+	do {
+		if (hipMalloc(&r, sizeof(double) * n) != hipSuccess || hipMalloc(&inv_dt, sizeof(double) * c.E) != hipSuccess ||
+		    hipMalloc(&status, sizeof(int)) != hipSuccess || gm.init(c, prm->gmres_restart) != hipSuccess) {
+			fail(DGX_ERR_HIP);
+			break;
+		}
+		if (prm->precond == DGX_PC_BLOCK_JACOBI) P = new dgx::PrecondBlockJacobi();
+		else if (prm->precond == DGX_PC_LEARNED) P = dgx::make_learned_precond(c);
+		else P = new dgx::PrecondIdentity();
+		if (!P) { fail(DGX_ERR_UNSUPPORTED); break; }
+		double max_rhs0 = 0.0;
+		for (int it = 0; it < prm->max_newton; ++it) {
+			if (dgx::launch_rhs64(c, u, r, dgx::MODE_RHS, nullptr) != hipSuccess) { fail(DGX_ERR_HIP); break; }
+			double mx = 0.0;
+			if (dgx::launch_fill_u64(c.stream, c.dmax, 1, 0ull) != hipSuccess ||
+			    dgx::launch_maxabs(c, r, n, c.dmax) != hipSuccess) { fail(DGX_ERR_HIP); break; }
+			unsigned long long bits = 0;
+			if (hipMemcpy(&bits, c.dmax, sizeof(bits), hipMemcpyDeviceToHost) != hipSuccess) { fail(DGX_ERR_HIP); break; }
+			std::memcpy(&mx, &bits, sizeof(mx));
+			if (!(mx == mx) || mx > 1e300) { fail(DGX_ERR_NONFINITE); break; }
+			if (it == 0) max_rhs0 = mx;
+			if (it < DGX_HISTORY_MAX) st->history[it] = mx;
+			st->newton_steps = it;
+			st->max_rhs = mx;
+			st->max_rhs0 = max_rhs0;
+			if (mx < prm->exit_tol || (prm->exit_ratio > 0.0 && mx / max_rhs0 < prm->exit_ratio)) {
+				st->converged = 1;
+				break;
+			}
+			const double cfl = prm->cfl_initial * (max_rhs0 / mx < 1.0 ? 1.0 : max_rhs0 / mx);
+			if (dgx::launch_cfl_dt(c, u, cfl, inv_dt) != hipSuccess) { fail(DGX_ERR_HIP); break; }
+			// b = -r
+			if (gm.blas.axpby(-1.0, r, 0.0, r, n) != hipSuccess) { fail(DGX_ERR_HIP); break; }
+			if (P->setup(c, u, inv_dt) != hipSuccess) { fail(DGX_ERR_HIP); break; }
+			int its = 0;
+			bool conv = false;
+			if (gm.solve(c, u, inv_dt, r, *P, prm->gmres_rtol, prm->gmres_max_iter, &its, &conv) != hipSuccess) {
+				fail(DGX_ERR_HIP);
+				break;
+			}
+			st->gmres_iters += its;
+			if (!conv) st->gmres_not_converged += 1;  // KSP DIVERGED_ITS: warn and continue
+			if (dgx::launch_fill_i32(c.stream, status, 1, 0) != hipSuccess ||
+			    dgx::launch_relaxed_update(c, u, gm.x, status) != hipSuccess) { fail(DGX_ERR_HIP); break; }
+			int bad = 0;
+			if (hipMemcpy(&bad, status, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) { fail(DGX_ERR_HIP); break; }
+			if (bad) { fail(DGX_ERR_POSITIVITY); break; }
+			st->newton_steps = it + 1;
+		}
+	} while (0);
+	(void)hipStreamSynchronize(c.stream);
+	delete P;
+	gm.release();
+	if (r) (void)hipFree(r);
+	if (inv_dt) (void)hipFree(inv_dt);
+	if (status) (void)hipFree(status);
+	st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+	if (ret == DGX_OK && !st->converged) ret = DGX_ERR_NOT_CONVERGED;
+	return ret;
+}

@@ -55,6 +55,7 @@ struct Ctx {
 	int* faces = nullptr;           // F*6
 	ElemFace* elem_faces = nullptr; // E*2d
 	int* order = nullptr;           // E: element processing order (Morton order of centroids)
+	double* hmin = nullptr;         // E: minimum straight edge length (compute_min_length_measure)
 	double* metrics_vc = nullptr;   // E*d*d*Nc  (m[(i + d*j)*Nc + n] = J dxi_j/dx_i)
 	float* metrics_vc32 = nullptr;
 	double* detJ_vc = nullptr;      // E*Nc
@@ -94,7 +95,11 @@ hipError_t launch_maxabs(Ctx& c, const double* r, size_t n, unsigned long long* 
 hipError_t launch_convert(const double* src, float* dst, size_t n, hipStream_t s);
 hipError_t launch_convert_back(const float* src, double* dst, size_t n, hipStream_t s);
 hipError_t launch_rk_stage(Ctx& c, double* u, const double* r, double dt, int stage);
-hipError_t launch_sub_mass(Ctx& c, double* blocks, double scale);
+hipError_t launch_sub_mass(Ctx& c, double* blocks, const double* scale_e, double scale);
+hipError_t launch_mass_sub(Ctx& c, const double* x, const double* scale, double* y);
+hipError_t launch_cfl_dt(Ctx& c, const double* u, double cfl, double* inv_dt);
+hipError_t launch_relaxed_update(Ctx& c, double* u, const double* du, int* status);
+std::vector<double> min_edge_lengths(int d, int N, int E, const double* xyz);
 hipError_t launch_block_lu(hipStream_t s, double* blocks, int* piv, int E, int nb);
 hipError_t launch_block_lu_solve(hipStream_t s, const double* lu, const int* piv, double* x, int E, int nb);
 hipError_t launch_diag_col(Ctx& c, const double* u, double* blocks, int col);
@@ -106,6 +111,10 @@ hipError_t launch_fill_i32(hipStream_t s, int* p, int n, int v);
 enum { MODE_RHS = 0, MODE_RHS_MINV = 1 };
 
 }  // namespace dgx
+
+struct dgx_ctx {
+	dgx::Ctx c;
+};
 
 #define DGX_HIP_CHECK(expr)                                  \
 	do {                                                     \

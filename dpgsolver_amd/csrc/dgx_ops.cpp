@@ -231,4 +231,33 @@ std::vector<int> morton_order(int d, int E, int Np, const double* xyz)
 	return order;
 }
 
+// minimum straight edge (vertex-to-vertex) length per element (compute_min_length_measure,
+// solve_dg.c:323-379); TP vertices are the GLL corner nodes
+std::vector<double> min_edge_lengths(int d, int N, int E, const double* xyz)
+{
+	const int Np = (d == 3) ? N * N * N : N * N;
+	auto vnode = [&](int v) {
+		const int b0 = v & 1, b1 = (v >> 1) & 1, b2 = (v >> 2) & 1;
+		return b0 * (N - 1) + N * (b1 * (N - 1) + N * (b2 * (N - 1)));
+	};
+	std::vector<double> out((size_t)E);
+	for (int e = 0; e < E; ++e) {
+		double m = 1e300;
+		const int nv = 1 << d;
+		for (int a = 0; a < nv; ++a)
+			for (int k = 0; k < d; ++k) {
+				const int b = a ^ (1 << k);
+				if (b < a) continue;  // each edge once: vertices differing in one bit
+				double s = 0.0;
+				for (int x = 0; x < d; ++x) {
+					const double t = xyz[((size_t)e * d + x) * Np + vnode(a)] - xyz[((size_t)e * d + x) * Np + vnode(b)];
+					s += t * t;
+				}
+				m = std::min(m, std::sqrt(s));
+			}
+		out[(size_t)e] = m;
+	}
+	return out;
+}
+
 }  // namespace dgx

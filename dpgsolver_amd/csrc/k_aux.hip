@@ -406,8 +406,9 @@ hipError_t launch_rk_stage(Ctx& c, double* u, const double* r, double dt, int st
 // (the CFL-ramping term A += -M/dt, solve_dg.c:223-248, 284-319)
 template <int D, int N>
 __global__ void k_sub_mass(double* blocks, const double* __restrict__ detJ, const Ops1D<double>* ops, int E, int nv,
-                           double scale)
+                           const double* scale_e, double scale_c)
 {
+	const double scale = scale_e ? scale_e[blockIdx.x] : scale_c;
 	constexpr int Np = ipow(N, D);
 	__shared__ double sI[N * N], sW[N], wJ[Np];
 	const int e = blockIdx.x;
@@ -440,10 +441,10 @@ __global__ void k_sub_mass(double* blocks, const double* __restrict__ detJ, cons
 			blocks[(size_t)e * nb * nb + (size_t)(v * Np + i) * nb + (v * Np + j)] -= scale * acc;
 	}
 }
-hipError_t launch_sub_mass(Ctx& c, double* blocks, double scale)
+hipError_t launch_sub_mass(Ctx& c, double* blocks, const double* scale_e, double scale)
 {
 	if (c.E == 0) return hipSuccess;
-#define ARGS blocks, c.detJ_vc, c.ops64, c.E, c.nv, scale
+#define ARGS blocks, c.detJ_vc, c.ops64, c.E, c.nv, scale_e, scale
 	DGX_PER_ELEM(k_sub_mass, )
 #undef ARGS
 	return hipGetLastError();

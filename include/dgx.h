@@ -104,11 +104,16 @@ dgx_status dgx_convert_f64_f32(dgx_ctx* ctx, const double* src, float* dst, long
 dgx_status dgx_convert_f32_f64(dgx_ctx* ctx, const float* src, double* dst, long long n);
 
 /* Implicit path.
- * Diagonal blocks A_e = dR_e/du_e - mass_scale * M_e, (Np*nv)^2 each, row-major, row eq*Np+node,
+ * Diagonal blocks A_e = dR_e/du_e - mass_scale[e] * M_e (mass_scale: device array of E, or NULL for none), (Np*nv)^2 each, row-major, row eq*Np+node,
  * column var*Np+node: volume lhs_v_1 + face lhs_ll/lhs_rr contributions + CFL term
  * (constructor_lhs_v_1_T compute_volume_rlhs_T.c:179-220, constructor_lhs_f_1_T
  *  compute_face_rlhs_T.c:163-220, compute_CFL_ramping solve_dg.c:223-248 with mass_scale = 1/dt). */
-dgx_status dgx_assemble_diag_blocks(dgx_ctx* ctx, const double* u, double mass_scale, double* blocks);
+dgx_status dgx_assemble_diag_blocks(dgx_ctx* ctx, const double* u, const double* mass_scale, double* blocks);
+/* CFL-ramped pseudo-time step per element, returned as inv_dt[e] = max wave speed / (cfl * h_min)
+ * (compute_dt_cfl_constrained / compute_min_length_measure, solve_dg.c:284-379). */
+dgx_status dgx_cfl_dt(dgx_ctx* ctx, const double* u, double cfl, double* inv_dt);
+/* y_e -= scale[e] * M_e x_e (scale NULL = 1): the -M/dt term of the CFL-ramped system. */
+dgx_status dgx_mass_sub(dgx_ctx* ctx, const double* x, const double* scale, double* y);
 /* Exact Jacobian-vector product jv = (dR/du)(u) v by forward-mode (dual-number) evaluation of the
  * residual: the matrix-free operator of the Newton system (replaces the PETSc AIJ Mat assembled by
  * add_to_petsc_Mat, solve.c:321-338). */
@@ -117,6 +122,33 @@ dgx_status dgx_jacobian_vector(dgx_ctx* ctx, const double* u, const double* v, d
 dgx_status dgx_block_lu_factor(dgx_ctx* ctx, double* blocks, int* piv, int E, int nb);
 /* x <- A_e^{-1} x for every element block (x: E*nb). */
 dgx_status dgx_block_lu_solve(dgx_ctx* ctx, const double* lu, const int* piv, double* x, int E, int nb);
+
+/* Steady solve by pseudo-time Newton-GMRES (solve_implicit, solve_implicit.c:95-110, 232-281, 393-470).
+ * Names mirror the reference's test_case.data keys (cfl_initial, exit_tol_i, exit_ratio_i) and the PETSc
+ * options file (GMRES restart 60, rtol 1e-4).  u (device) is updated in place. */
+enum { DGX_PC_NONE = 0, DGX_PC_BLOCK_JACOBI = 1, DGX_PC_LEARNED = 2 };
+#define DGX_HISTORY_MAX 256
+typedef struct dgx_implicit_params {
+	double cfl_initial;   /* cfl_initial */
+	double exit_tol;      /* exit_tol_i: stop when max|R| < exit_tol */
+	double exit_ratio;    /* exit_ratio_i: stop when max|R|/max|R_0| < exit_ratio (0 = off) */
+	int max_newton;
+	int gmres_restart;    /* -ksp_gmres_restart */
+	double gmres_rtol;    /* -ksp_rtol */
+	int gmres_max_iter;   /* per Newton step; hitting it is DIVERGED_ITS: counted, Newton continues */
+	int precond;          /* DGX_PC_* */
+} dgx_implicit_params;
+typedef struct dgx_implicit_stats {
+	int newton_steps;
+	int gmres_iters;
+	int gmres_not_converged;
+	int converged;
+	double max_rhs0;
+	double max_rhs;
+	double seconds;
+	double history[DGX_HISTORY_MAX];
+} dgx_implicit_stats;
+dgx_status dgx_solve_implicit(dgx_ctx* ctx, double* u, const dgx_implicit_params* params, dgx_implicit_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,208 @@
+"""GPU parity of the explicit stepper and the implicit path against the CPU oracle.
+
+* SSP-RK3 steps (with the Bezier positivity limiter) against or_ssp_rk3_step.
+* Element diagonal Jacobian blocks (forward-mode dual numbers on the GPU) against the oracle's
+  complex-step blocks: the reference's own linearization check (analytic vs complex step,
+  test_integration_linearization.c:377, 4e-13).
+* Exact Jacobian-vector products against complex step of the full oracle residual.
+* Batched block LU against numpy; CFL dt against its closed form on a uniform state.
+* A steady channel solve to 1e-10, whose converged state is re-checked by the oracle residual.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from dpgsolver_amd import mesh as M
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda:0")
+
+
+def _ctx(mesh, pde, flux, b_adv=(0.0, 0.0, 0.0), bc_prim=None):
+    from dpgsolver_amd import _lib
+
+    ctx = _lib.Context(mesh.d, mesh.p, pde, flux, b_adv=b_adv, bc_prim=bc_prim)
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    ctx.upload_mesh(mesh.xyz, mesh.faces)
+    return ctx
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _channel_state(mesh, amp=0.02):
+    prim = M.freestream_prim(2)
+    x = M.nodes_xyz(mesh)
+    pert = amp * np.sin(3 * x[..., :1]) * np.cos(2 * x[..., 1:2]) * np.array([1, 1, 0.5, 1])
+    return M.pack_state(M.prim_to_cons(prim[None, None, :] + pert, 2)), prim
+
+
+# ----------------------------------------------------------------------------- explicit
+@pytest.mark.parametrize("case", ["adv", "euler_vortex"])
+def test_ssp_rk3_steps_match_oracle(dev, case):
+    if case == "adv":
+        mesh = M.config1_mesh(8, 2)
+        u = M.config1_state(mesh)
+        pde, flux, b, dt, pos = M.PDE_ADVECTION, M.FLUX_UPWIND, (1.0, 0.5, 0.0), 0.2 * 0.25 / (5 * 1.0), False
+    else:
+        mesh = M.config2_mesh(8, 3)
+        u = M.config2_state(mesh)
+        pde, flux, b, dt, pos = M.PDE_EULER, M.FLUX_ROE, (0.0, 0.0, 0.0), 0.01, True
+    ctx = _ctx(mesh, pde, flux, b)
+    orc = O.Oracle(mesh, pde, flux, b)
+    ud = torch.from_numpy(u).to(dev)
+    uo = u.copy()
+    for _ in range(5):
+        mx = ctx.ssp_rk3_step(ud.data_ptr(), dt, positivity=pos)
+        uo, mxo, err = orc.ssp_rk3_step(uo, dt, positivity=pos)
+        assert err == 0
+        assert abs(mx - mxo) <= 1e-10 * mxo
+    torch.cuda.synchronize()
+    assert _rel(ud.cpu().numpy(), uo) < 1e-12
+    ctx.close()
+
+
+def test_positivity_limiter_matches_oracle(dev):
+    mesh = M.config2_mesh(3, 3)
+    u = M.uniform_state(mesh, np.array([1.0, 0.2, 0.1, 1.0])).reshape(mesh.E, 4, 16)
+    u[0, 0, 5] = -0.3
+    u[4, 3, 7] = -0.5        # negative pressure contribution in another element
+    u = u.ravel()
+    ctx = _ctx(mesh, M.PDE_EULER, M.FLUX_ROE)
+    ud = torch.from_numpy(u).to(dev)
+    assert ctx.enforce_positivity(ud.data_ptr()) == 0
+    ref, st = O.Oracle(mesh, M.PDE_EULER, M.FLUX_ROE).positivity(u)
+    assert st == 0
+    assert _rel(ud.cpu().numpy(), ref) < 1e-12
+    bad = torch.full_like(ud, -1.0)
+    assert ctx.enforce_positivity(bad.data_ptr()) == -4   # DGX_ERR_POSITIVITY, not an abort
+    ctx.close()
+
+
+# ----------------------------------------------------------------------------- implicit building blocks
+def _jac_cases():
+    out = []
+    m = M.structured_tp_mesh(2, 2, (3, 3), (0, 0), (1, 1), (True, True))
+    u = M.pack_state(M.prim_to_cons(M.smooth_state_3d(M.nodes_xyz(m)), 2))
+    out.append(("periodic_roe_p2", m, M.FLUX_ROE, None, u))
+    m = M.structured_tp_mesh(2, 3, (3, 3), (0, 0), (1, 1), (True, True),
+                             rotate=np.random.default_rng(2).integers(0, 4, 9))
+    u = M.pack_state(M.prim_to_cons(M.smooth_state_3d(M.nodes_xyz(m)), 2))
+    out.append(("rotated_lf_p3", m, M.FLUX_LF, None, u))
+    m = M.channel_mesh(4, 3, 2)
+    u, prim = _channel_state(m)
+    out.append(("channel_bc_roe_p2", m, M.FLUX_ROE, prim, u))
+    return out
+
+
+JAC = _jac_cases()
+
+
+@pytest.mark.parametrize("case", JAC, ids=[c[0] for c in JAC])
+def test_diag_blocks_match_complex_step(dev, case):
+    name, mesh, flux, bcp, u = case
+    ctx = _ctx(mesh, M.PDE_EULER, flux, bc_prim=bcp)
+    nb = ctx.Np * ctx.nv
+    ud = torch.from_numpy(u).to(dev)
+    blocks = torch.zeros(mesh.E * nb * nb, dtype=torch.float64, device=dev)
+    ctx.assemble_diag_blocks(ud.data_ptr(), None, blocks.data_ptr())
+    torch.cuda.synchronize()
+    g = blocks.cpu().numpy().reshape(mesh.E, nb, nb)
+    orc = O.Oracle(mesh, M.PDE_EULER, flux, bc_prim=bcp)
+    ref = orc.diag_blocks(u)
+    for e in range(mesh.E):
+        diff = np.abs(g[e] - ref[e]).max()
+        assert diff <= 4e-13 * max(1.0, np.abs(ref[e]).max()), (e, diff)
+    # with the CFL mass term: A_e - s_e M_e (x) I
+    s = np.linspace(0.5, 2.0, mesh.E)
+    sd = torch.from_numpy(s).to(dev)
+    ctx.assemble_diag_blocks(ud.data_ptr(), sd.data_ptr(), blocks.data_ptr())
+    torch.cuda.synchronize()
+    g2 = blocks.cpu().numpy().reshape(mesh.E, nb, nb)
+    for e in (0, mesh.E - 1):
+        Mm = np.kron(np.eye(ctx.nv), orc.mass(e))
+        assert np.abs(g2[e] - (ref[e] - s[e] * Mm)).max() <= 1e-12 * np.abs(ref[e]).max()
+    ctx.close()
+
+
+@pytest.mark.parametrize("case", JAC, ids=[c[0] for c in JAC])
+def test_jacobian_vector_matches_complex_step(dev, case):
+    name, mesh, flux, bcp, u = case
+    ctx = _ctx(mesh, M.PDE_EULER, flux, bc_prim=bcp)
+    v = np.random.default_rng(9).standard_normal(u.size)
+    ud, vd = torch.from_numpy(u).to(dev), torch.from_numpy(v).to(dev)
+    jv = torch.empty_like(ud)
+    ctx.jacobian_vector(ud.data_ptr(), vd.data_ptr(), jv.data_ptr())
+    torch.cuda.synchronize()
+    h = 1e-30
+    _, im = O.Oracle(mesh, M.PDE_EULER, flux, bc_prim=bcp).rhs_complex(u, h * v)
+    assert _rel(jv.cpu().numpy(), im / h) < 1e-12
+    ctx.close()
+
+
+def test_block_lu_solve_matches_numpy(dev):
+    from dpgsolver_amd import _lib
+
+    rng = np.random.default_rng(3)
+    E, nb = 37, 64
+    A = rng.standard_normal((E, nb, nb)) + 4 * np.eye(nb)
+    x = rng.standard_normal((E, nb))
+    ctx = _lib.Context(2, 3, M.PDE_EULER, M.FLUX_ROE)
+    Ad = torch.from_numpy(A.copy()).to(dev)
+    piv = torch.zeros(E * nb, dtype=torch.int32, device=dev)
+    xd = torch.from_numpy(x.copy()).to(dev)
+    ctx.block_lu_factor(Ad.data_ptr(), piv.data_ptr(), E, nb)
+    ctx.block_lu_solve(Ad.data_ptr(), piv.data_ptr(), xd.data_ptr(), E, nb)
+    torch.cuda.synchronize()
+    ref = np.linalg.solve(A, x[..., None])[..., 0]
+    assert _rel(xd.cpu().numpy(), ref) < 1e-12
+    with pytest.raises(_lib.DgxError):
+        ctx.block_lu_factor(Ad.data_ptr(), piv.data_ptr(), E, 91)   # nb^2 doubles must fit in LDS
+    ctx.close()
+
+
+def test_cfl_dt_closed_form_on_uniform_state(dev):
+    mesh = M.channel_mesh(6, 4, 3)
+    prim = M.freestream_prim(2)
+    u = M.uniform_state(mesh, prim)
+    ctx = _ctx(mesh, M.PDE_EULER, M.FLUX_ROE, bc_prim=prim)
+    ud = torch.from_numpy(u).to(dev)
+    inv = torch.empty(mesh.E, dtype=torch.float64, device=dev)
+    ctx.cfl_dt(ud.data_ptr(), 7.0, inv.data_ptr())
+    torch.cuda.synchronize()
+    ws = abs(prim[1]) + np.sqrt(1.4 * prim[3] / prim[0])
+    N = mesh.p + 1
+    corners = [0, N - 1, N * (N - 1), N * N - 1]
+    hmin = []
+    for e in range(mesh.E):
+        P = mesh.xyz[e][:, corners].T
+        edges = [(0, 1), (2, 3), (0, 2), (1, 3)]
+        hmin.append(min(np.linalg.norm(P[a] - P[b]) for a, b in edges))
+    np.testing.assert_allclose(inv.cpu().numpy(), ws / (7.0 * np.array(hmin)), rtol=1e-13)
+    ctx.close()
+
+
+# ----------------------------------------------------------------------------- steady solve
+def test_steady_channel_newton_gmres_block_jacobi(dev):
+    from dpgsolver_amd import _lib
+
+    mesh = M.channel_mesh(12, 6, 2)
+    prim = M.freestream_prim(2)
+    u = M.uniform_state(mesh, prim)
+    ctx = _ctx(mesh, M.PDE_EULER, M.FLUX_ROE, bc_prim=prim)
+    ud = torch.from_numpy(u).to(dev)
+    st = ctx.solve_implicit(ud.data_ptr(), cfl_initial=10.0, exit_tol=1e-10, max_newton=80,
+                            precond=_lib.PC_BLOCK_JACOBI)
+    print("newton", st["newton_steps"], "gmres", st["gmres_iters"], "max_rhs", st["max_rhs"], st["seconds"])
+    assert st["converged"] == 1 and st["max_rhs"] < 1e-10
+    r = O.Oracle(mesh, M.PDE_EULER, M.FLUX_ROE, bc_prim=prim).rhs(ud.cpu().numpy())
+    assert np.abs(r).max() < 1e-9          # the oracle agrees that this is the steady state
+    ctx.close()
