@@ -25,6 +25,8 @@ EXPORTS = [
     "dgx_compute_rhs_explicit", "dgx_apply_minv", "dgx_enforce_positivity", "dgx_ssp_rk3_step", "dgx_max_abs",
     "dgx_convert_f64_f32", "dgx_convert_f32_f64", "dgx_assemble_diag_blocks", "dgx_jacobian_vector",
     "dgx_block_lu_factor", "dgx_block_lu_solve", "dgx_cfl_dt", "dgx_mass_sub", "dgx_solve_implicit",
+    "dgx_mlp_create", "dgx_mlp_destroy", "dgx_mlp_sizes", "dgx_mlp_get_params", "dgx_mlp_set_params",
+    "dgx_mlp_features", "dgx_mlp_loss_grad", "dgx_mlp_train", "dgx_mlp_predict", "dgx_attach_mlp",
 ]
 
 PC_NONE, PC_BLOCK_JACOBI, PC_LEARNED = 0, 1, 2
@@ -49,13 +51,14 @@ class DgxMesh(C.Structure):
 class ImplicitParams(C.Structure):
     _fields_ = [("cfl_initial", C.c_double), ("exit_tol", C.c_double), ("exit_ratio", C.c_double),
                 ("max_newton", C.c_int), ("gmres_restart", C.c_int), ("gmres_rtol", C.c_double),
-                ("gmres_max_iter", C.c_int), ("precond", C.c_int)]
+                ("gmres_max_iter", C.c_int), ("precond", C.c_int), ("capture_blocks", C.c_void_p),
+                ("capture_features", C.c_void_p), ("capture_sigma", C.c_void_p), ("capture_capacity", C.c_longlong)]
 
 
 class ImplicitStats(C.Structure):
     _fields_ = [("newton_steps", C.c_int), ("gmres_iters", C.c_int), ("gmres_not_converged", C.c_int),
                 ("converged", C.c_int), ("max_rhs0", C.c_double), ("max_rhs", C.c_double),
-                ("seconds", C.c_double), ("history", C.c_double * HISTORY_MAX)]
+                ("seconds", C.c_double), ("captured", C.c_longlong), ("history", C.c_double * HISTORY_MAX)]
 
 
 _lib = None
@@ -94,6 +97,16 @@ def load():
         "dgx_mass_sub": [vp, vp, vp, vp],
         "dgx_solve_implicit": [vp, vp, C.POINTER(ImplicitParams), C.POINTER(ImplicitStats)],
         "dgx_jacobian_vector": [vp, vp, vp, vp],
+        "dgx_mlp_create": [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong],
+        "dgx_mlp_destroy": [vp],
+        "dgx_mlp_sizes": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_longlong)],
+        "dgx_mlp_get_params": [vp, C.POINTER(C.c_float)],
+        "dgx_mlp_set_params": [vp, C.POINTER(C.c_float)],
+        "dgx_mlp_features": [vp, vp, vp, vp, vp, vp],
+        "dgx_mlp_loss_grad": [vp, vp, vp, vp, C.c_int, dp, C.POINTER(C.c_float)],
+        "dgx_mlp_train": [vp, vp, vp, vp, C.c_longlong, C.c_int, C.c_int, C.c_float, dp],
+        "dgx_mlp_predict": [vp, vp, C.c_int, vp],
+        "dgx_attach_mlp": [vp, vp],
         "dgx_block_lu_factor": [vp, vp, vp, C.c_int, C.c_int],
         "dgx_block_lu_solve": [vp, vp, vp, vp, C.c_int, C.c_int],
     }
@@ -201,6 +214,9 @@ class Context:
     def convert_f32_f64(self, src: int, dst: int, n: int):
         check("dgx_convert_f32_f64", self.L.dgx_convert_f32_f64(self.h, vp(src), vp(dst), n))
 
+    def attach_mlp(self, mlp: "MLP | None"):
+        check("dgx_attach_mlp", self.L.dgx_attach_mlp(self.h, mlp.h if mlp is not None else None))
+
     def assemble_diag_blocks(self, u: int, mass_scale: int | None, blocks: int):
         """mass_scale: device pointer to E doubles (1/dt per element) or None."""
         check("dgx_assemble_diag_blocks",
@@ -214,15 +230,17 @@ class Context:
 
     def solve_implicit(self, u: int, cfl_initial=10.0, exit_tol=1e-10, exit_ratio=0.0, max_newton=100,
                        gmres_restart=60, gmres_rtol=1e-4, gmres_max_iter=600, precond=PC_BLOCK_JACOBI,
-                       raise_on_error=True):
+                       raise_on_error=True, capture=None):
+        """capture: optional (blocks_ptr, features_ptr, sigma_ptr, capacity_elements) device buffers."""
+        cb, cf, cs, cc = capture if capture else (0, 0, 0, 0)
         p = ImplicitParams(cfl_initial, exit_tol, exit_ratio, max_newton, gmres_restart, gmres_rtol, gmres_max_iter,
-                           precond)
+                           precond, cb or None, cf or None, cs or None, cc)
         st = ImplicitStats()
         rc = self.L.dgx_solve_implicit(self.h, vp(u), C.byref(p), C.byref(st))
         if raise_on_error and rc != DGX_OK:
             raise DgxError("dgx_solve_implicit", rc)
         out = {k: getattr(st, k) for k in ("newton_steps", "gmres_iters", "gmres_not_converged", "converged",
-                                           "max_rhs0", "max_rhs", "seconds")}
+                                           "max_rhs0", "max_rhs", "seconds", "captured")}
         out["history"] = list(st.history[: min(st.newton_steps + 1, HISTORY_MAX)])
         out["status"] = rc
         return out
@@ -235,3 +253,57 @@ class Context:
 
     def block_lu_solve(self, lu: int, piv: int, x: int, E: int, nb: int):
         check("dgx_block_lu_solve", self.L.dgx_block_lu_solve(self.h, vp(lu), vp(piv), vp(x), E, nb))
+
+
+class MLP:
+    """The learned block preconditioner (dgx_mlp_*)."""
+
+    def __init__(self, nb: int, Np: int, hidden: int = 256, rank: int = 8, seed: int = 20261202):
+        self.L = load()
+        h = vp()
+        check("dgx_mlp_create", self.L.dgx_mlp_create(C.byref(h), nb, Np, hidden, rank, seed))
+        self.h = h
+        nf, nout, npar = C.c_int(), C.c_int(), C.c_longlong()
+        check("dgx_mlp_sizes", self.L.dgx_mlp_sizes(h, C.byref(nf), C.byref(nout), C.byref(npar)))
+        self.nf, self.nout, self.nparam = nf.value, nout.value, npar.value
+        self.nb, self.Np, self.hidden, self.rank = nb, Np, hidden, rank
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.dgx_mlp_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.nparam, dtype=np.float32)
+        check("dgx_mlp_get_params", self.L.dgx_mlp_get_params(self.h, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def set_params(self, w: np.ndarray):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == self.nparam
+        check("dgx_mlp_set_params", self.L.dgx_mlp_set_params(self.h, w.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def features(self, ctx: "Context", u: int, inv_dt: int, feat: int, sigma: int):
+        check("dgx_mlp_features", self.L.dgx_mlp_features(ctx.h, self.h, vp(u), vp(inv_dt), vp(feat), vp(sigma)))
+
+    def loss_grad(self, feat: int, sigma: int | None, blocks: int, B: int, want_grad=True):
+        loss = C.c_double()
+        g = np.zeros(self.nparam, dtype=np.float32) if want_grad else None
+        check("dgx_mlp_loss_grad", self.L.dgx_mlp_loss_grad(
+            self.h, vp(feat), vp(sigma or 0), vp(blocks), B, C.byref(loss),
+            g.ctypes.data_as(C.POINTER(C.c_float)) if want_grad else None))
+        return loss.value, g
+
+    def train(self, feat: int, sigma: int | None, blocks: int, n: int, epochs: int, batch: int, lr: float) -> float:
+        loss = C.c_double()
+        check("dgx_mlp_train", self.L.dgx_mlp_train(self.h, vp(feat), vp(sigma or 0), vp(blocks), n, epochs, batch,
+                                                   lr, C.byref(loss)))
+        return loss.value
+
+    def predict(self, feat: int, B: int, out: int):
+        check("dgx_mlp_predict", self.L.dgx_mlp_predict(self.h, vp(feat), B, vp(out)))
+
+    def attach(self, ctx: "Context"):
+        check("dgx_attach_mlp", self.L.dgx_attach_mlp(ctx.h, self.h))

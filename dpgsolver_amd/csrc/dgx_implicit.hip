@@ -326,12 +326,6 @@ hipError_t launch_relaxed_update(Ctx& c, double* u, const double* du, int* statu
 // ------------------------------------------------------------------------------- preconditioners
 namespace dgx {
 
-struct Precond {
-	virtual ~Precond() {}
-	virtual hipError_t setup(Ctx& c, const double* u, const double* inv_dt) = 0;
-	virtual hipError_t apply(Ctx& c, const double* v, double* z) = 0;  // z = P^-1 v
-};
-
 struct PrecondIdentity : Precond {
 	hipError_t setup(Ctx&, const double*, const double*) override { return hipSuccess; }
 	hipError_t apply(Ctx& c, const double* v, double* z) override
@@ -341,9 +335,16 @@ struct PrecondIdentity : Precond {
 };
 
 // exact LU of the element diagonal blocks of (dR/du - M/dt): the block-Jacobi baseline
+hipError_t launch_features(Ctx& c, const double* u, const double* inv_dt, float* feat, int nf, float* sigma);
+
 struct PrecondBlockJacobi : Precond {
 	double* blocks = nullptr;
 	int* piv = nullptr;
+	// optional training-set capture
+	double* cap_blocks = nullptr;
+	float* cap_feat = nullptr;
+	float* cap_sigma = nullptr;
+	long long cap_capacity = 0, cap_count = 0;
 	~PrecondBlockJacobi() override
 	{
 		if (blocks) (void)hipFree(blocks);
@@ -358,6 +359,14 @@ struct PrecondBlockJacobi : Precond {
 		}
 		for (int col = 0; col < nb; ++col) DGX_HIP_CHECK(launch_diag_col(c, u, blocks, col));
 		DGX_HIP_CHECK(launch_sub_mass(c, blocks, inv_dt, 0.0));
+		if (cap_blocks && cap_feat && cap_sigma && cap_count + c.E <= cap_capacity) {
+			const int nf = (nb + c.Np + 1 + 15) / 16 * 16;
+			DGX_HIP_CHECK(hipMemcpyAsync(cap_blocks + (size_t)cap_count * nb * nb, blocks,
+			                             sizeof(double) * (size_t)c.E * nb * nb, hipMemcpyDeviceToDevice, c.stream));
+			DGX_HIP_CHECK(launch_features(c, u, inv_dt, cap_feat + (size_t)cap_count * nf, nf,
+			                              cap_sigma + (size_t)cap_count));
+			cap_count += c.E;
+		}
 		return launch_block_lu(c.stream, blocks, piv, c.E, nb);
 	}
 	hipError_t apply(Ctx& c, const double* v, double* z) override
@@ -368,7 +377,6 @@ struct PrecondBlockJacobi : Precond {
 	}
 };
 
-Precond* make_learned_precond(Ctx& c);  // dgx_mlp.hip
 
 // ------------------------------------------------------------------------------- GMRES
 struct Gmres {
@@ -509,7 +517,14 @@ extern "C" dgx_status dgx_solve_implicit(dgx_ctx* h, double* u, const dgx_implic
 			fail(DGX_ERR_HIP);
 			break;
 		}
-		if (prm->precond == DGX_PC_BLOCK_JACOBI) P = new dgx::PrecondBlockJacobi();
+		if (prm->precond == DGX_PC_BLOCK_JACOBI) {
+			auto* bj = new dgx::PrecondBlockJacobi();
+			bj->cap_blocks = prm->capture_blocks;
+			bj->cap_feat = prm->capture_features;
+			bj->cap_sigma = prm->capture_sigma;
+			bj->cap_capacity = prm->capture_capacity;
+			P = bj;
+		}
 		else if (prm->precond == DGX_PC_LEARNED) P = dgx::make_learned_precond(c);
 		else P = new dgx::PrecondIdentity();
 		if (!P) { fail(DGX_ERR_UNSUPPORTED); break; }
@@ -554,6 +569,7 @@ extern "C" dgx_status dgx_solve_implicit(dgx_ctx* h, double* u, const dgx_implic
 		}
 	} while (0);
 	(void)hipStreamSynchronize(c.stream);
+	if (prm->precond == DGX_PC_BLOCK_JACOBI && P) st->captured = static_cast<dgx::PrecondBlockJacobi*>(P)->cap_count;
 	delete P;
 	gm.release();
 	if (r) (void)hipFree(r);

@@ -56,6 +56,7 @@ struct Ctx {
 	ElemFace* elem_faces = nullptr; // E*2d
 	int* order = nullptr;           // E: element processing order (Morton order of centroids)
 	double* hmin = nullptr;         // E: minimum straight edge length (compute_min_length_measure)
+	dgx_mlp* mlp = nullptr;         // attached learned preconditioner (not owned)
 	double* metrics_vc = nullptr;   // E*d*d*Nc  (m[(i + d*j)*Nc + n] = J dxi_j/dx_i)
 	float* metrics_vc32 = nullptr;
 	double* detJ_vc = nullptr;      // E*Nc
@@ -78,6 +79,14 @@ struct Ctx {
 	size_t nblk = 0;
 	std::vector<double> h_tmp;
 };
+
+// preconditioner interface of the Newton-GMRES driver: z = P^-1 v
+struct Precond {
+	virtual ~Precond() {}
+	virtual hipError_t setup(Ctx& c, const double* u, const double* inv_dt) = 0;
+	virtual hipError_t apply(Ctx& c, const double* v, double* z) = 0;
+};
+Precond* make_learned_precond(Ctx& c);  // nullptr when no model is attached
 
 // host-side helpers implemented in dgx_ops.cpp
 void build_ops(int d, int p, Ops1D<double>& o64, Ops1D<float>& o32);

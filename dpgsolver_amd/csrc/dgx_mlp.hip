@@ -1,7 +1,598 @@
-// dgx_mlp.hip -- learned block preconditioner (placeholder until the MLP kernels land).
+// dgx_mlp.hip -- learned block preconditioner: a 3-layer MLP (BF16 weights, FP32 accumulation on
+// MFMA) maps per-element features to P_e^-1 = diag(d_e) + U_e V_e^T (rank R), an approximate inverse of
+// the element diagonal block A_e = dR_e/du_e - M_e/dt_e of the pseudo-time Newton system.
+//
+// P_e^-1 = sigma_e (diag(d_e) + U_e V_e^T) with the analytic scale sigma_e = 1 / (mean(w detJ)/dt_e +
+// 0.25 h_e lambda_e) so that the network outputs stay O(1) along the CFL ramp.
+// Training data are the solver's own blocks captured along its Newton iterations; the loss is
+//   L = (1/B) sum_e ||P_e A_e - I||_F^2 / nb,
+// minimised with Adam.  Forward, backward and the loss gradient are hand-written kernels; the GEMMs use
+// v_mfma_f32_16x16x16_bf16 (one 16x16 output tile per wavefront, K in steps of 16).
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "dgx_device.h"
 #include "dgx_internal.h"
 
 namespace dgx {
-struct Precond;
-Precond* make_learned_precond(Ctx&) { return nullptr; }
+
+using bf16x4 = __attribute__((__vector_size__(4 * sizeof(short)))) short;
+using f32x4 = __attribute__((__vector_size__(4 * sizeof(float)))) float;
+
+__device__ inline short f2bf(float x)
+{
+	// round-to-nearest-even float -> bf16 bits
+	unsigned u = __float_as_uint(x);
+	if ((u & 0x7f800000u) == 0x7f800000u) return (short)(u >> 16);  // inf/nan
+	u += 0x7fffu + ((u >> 16) & 1u);
+	return (short)(u >> 16);
+}
+__device__ inline float bf2f(short b) { return __uint_as_float(((unsigned)(unsigned short)b) << 16); }
+
+// C[M x N] (+)= op(A)[M x K] * op(B)[K x N], A/B given as FP32 and rounded to bf16 on load.
+// A element (i,k) at A[i*sa0 + k*sa1]; B element (k,j) at B[k*sb0 + j*sb1]; C row-major ldc.
+// One wavefront per 16x16 output tile; out-of-range rows/cols/ks read as zero.
+__global__ void __launch_bounds__(64) k_gemm_bf16(const float* __restrict__ A, long sa0, long sa1,
+                                                  const float* __restrict__ B, long sb0, long sb1, float* C, long ldc,
+                                                  int M, int N, int K, int accumulate, const float* __restrict__ bias,
+                                                  int act)
+{
+	const int l = threadIdx.x;
+	const int i0 = blockIdx.y * 16, j0 = blockIdx.x * 16;
+	const int row = i0 + (l & 15), col = j0 + (l & 15), kq = (l >> 4) * 4;
+	f32x4 acc = { 0.f, 0.f, 0.f, 0.f };
+	for (int k0 = 0; k0 < K; k0 += 16) {
+		bf16x4 a, b;
+#pragma unroll
+		for (int t = 0; t < 4; ++t) {
+			const int k = k0 + kq + t;
+			const float av = (row < M && k < K) ? A[(long)row * sa0 + (long)k * sa1] : 0.f;
+			const float bv = (col < N && k < K) ? B[(long)k * sb0 + (long)col * sb1] : 0.f;
+			a[t] = f2bf(av);
+			b[t] = f2bf(bv);
+		}
+		acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+	}
+#pragma unroll
+	for (int t = 0; t < 4; ++t) {
+		const int r = i0 + (l >> 4) * 4 + t, c = j0 + (l & 15);
+		if (r < M && c < N) {
+			float v = acc[t];
+			if (accumulate) v += C[(long)r * ldc + c];
+			if (bias) v += bias[c];
+			if (act == 1) v = tanhf(v);
+			C[(long)r * ldc + c] = v;
+		}
+	}
+}
+
+static hipError_t gemm(hipStream_t s, const float* A, long sa0, long sa1, const float* B, long sb0, long sb1, float* C,
+                       long ldc, int M, int N, int K, int accumulate, const float* bias, int act)
+{
+	dim3 grid((N + 15) / 16, (M + 15) / 16);
+	hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(64), 0, s, A, sa0, sa1, B, sb0, sb1, C, ldc, M, N, K, accumulate,
+	                   bias, act);
+	return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------- loss and its gradient
+// Per sample e: out = [d (nb) | U (nb x R, row-major) | V (nb x R)], A_e (nb x nb row-major), sigma_e.
+// P = sigma (diag(d) + U V^T), Rm = P A - I, L_e = ||Rm||_F^2 / nb.
+// dL/dP = 2 Rm A^T / nb; with G = sigma dL/dP: dd = diag(G), dU = G V, dV = G^T U  (times gscale).
+template <int NB, int RK>
+__global__ void __launch_bounds__(256) k_block_loss(const float* __restrict__ out, const float* __restrict__ sigma,
+                                                    const double* __restrict__ A, float* grad, double* loss_partial,
+                                                    float gscale)
+{
+	constexpr int NO = NB + 2 * NB * RK;
+	__shared__ float sA[NB * NB], sP[NB * NB], sR[NB * NB];
+	__shared__ float sD[NB], sU[NB * RK], sV[NB * RK];
+	__shared__ double red[256];
+	const int e = blockIdx.x, t = threadIdx.x;
+	const float sg = sigma ? sigma[e] : 1.f;
+	const float* o = out + (size_t)e * NO;
+	for (int i = t; i < NB * NB; i += blockDim.x) sA[i] = (float)A[(size_t)e * NB * NB + i];
+	for (int i = t; i < NB; i += blockDim.x) sD[i] = o[i];
+	for (int i = t; i < NB * RK; i += blockDim.x) { sU[i] = o[NB + i]; sV[i] = o[NB + NB * RK + i]; }
+	__syncthreads();
+	for (int ij = t; ij < NB * NB; ij += blockDim.x) {
+		const int i = ij / NB, j = ij % NB;
+		float p = (i == j) ? sD[i] : 0.f;
+		for (int r = 0; r < RK; ++r) p += sU[i * RK + r] * sV[j * RK + r];
+		sP[ij] = sg * p;
+	}
+	__syncthreads();
+	double lsum = 0.0;
+	for (int ij = t; ij < NB * NB; ij += blockDim.x) {
+		const int i = ij / NB, j = ij % NB;
+		float acc = (i == j) ? -1.f : 0.f;
+		for (int k = 0; k < NB; ++k) acc += sP[i * NB + k] * sA[k * NB + j];
+		sR[ij] = acc;
+		lsum += (double)acc * acc;
+	}
+	red[t] = lsum;
+	__syncthreads();
+	for (int off = 128; off > 0; off >>= 1) {
+		if (t < off) red[t] += red[t + off];
+		__syncthreads();
+	}
+	if (t == 0) loss_partial[e] = red[0] / NB;
+	if (!grad) return;
+	// G = sigma * 2 Rm A^T / nb * gscale  (stored in sP, which is no longer needed)
+	for (int ij = t; ij < NB * NB; ij += blockDim.x) {
+		const int i = ij / NB, j = ij % NB;
+		float acc = 0.f;
+		for (int k = 0; k < NB; ++k) acc += sR[i * NB + k] * sA[j * NB + k];
+		sP[ij] = acc * (2.0f / NB) * gscale * sg;
+	}
+	__syncthreads();
+	float* g = grad + (size_t)e * NO;
+	for (int i = t; i < NB; i += blockDim.x) g[i] = sP[i * NB + i];
+	for (int ir = t; ir < NB * RK; ir += blockDim.x) {
+		const int i = ir / RK, r = ir % RK;
+		float du = 0.f, dv = 0.f;
+		for (int j = 0; j < NB; ++j) {
+			du += sP[i * NB + j] * sV[j * RK + r];
+			dv += sP[j * NB + i] * sU[j * RK + r];
+		}
+		g[NB + ir] = du;
+		g[NB + NB * RK + ir] = dv;
+	}
+}
+
+// z_e = sigma_e (d_e * v_e + U_e (V_e^T v_e))   (FP64 vectors, FP32 factors)
+template <int NB, int RK>
+__global__ void __launch_bounds__(64) k_lowrank_apply(const float* __restrict__ out, const float* __restrict__ sigma,
+                                                      const double* __restrict__ v, double* z, int E)
+{
+	constexpr int NO = NB + 2 * NB * RK;
+	__shared__ double sv[NB], sw[RK];
+	const int e = blockIdx.x, t = threadIdx.x;
+	const float* o = out + (size_t)e * NO;
+	const double sg = sigma[e];
+	for (int i = t; i < NB; i += blockDim.x) sv[i] = v[(size_t)e * NB + i];
+	__syncthreads();
+	for (int r = t; r < RK; r += blockDim.x) {
+		double s = 0.0;
+		for (int j = 0; j < NB; ++j) s += (double)o[NB + NB * RK + j * RK + r] * sv[j];
+		sw[r] = s;
+	}
+	__syncthreads();
+	for (int i = t; i < NB; i += blockDim.x) {
+		double s = (double)o[i] * sv[i];
+		for (int r = 0; r < RK; ++r) s += (double)o[NB + i * RK + r] * sw[r];
+		z[(size_t)e * NB + i] = sg * s;
+	}
+}
+
+// features [u / u_ref (nb) | detJ / mean (Np) | log10(h inv_dt / lambda) | 0 pad] and sigma_e
+template <int D, int N>
+__global__ void __launch_bounds__(64) k_features(const double* __restrict__ u, const double* __restrict__ detJ,
+                                                 const Ops1D<double>* ops, const double* __restrict__ hmin,
+                                                 const double* __restrict__ inv_dt, int E, float* feat, int nf,
+                                                 double uref0, double uref1, double uref2, float* sigma)
+{
+	constexpr int Np = ipow(N, D), NV = D + 2, nb = NV * Np;
+	__shared__ double sWJ[Np], sWs[Np];
+	const int e = blockIdx.x, t = threadIdx.x;
+	for (int q = t; q < Np; q += blockDim.x) {
+		const int c0 = q % N, c1 = (q / N) % N, c2 = (D == 3) ? q / (N * N) : 0;
+		double w = ops->w[c0] * ops->w[c1];
+		if (D == 3) w *= ops->w[c2];
+		sWJ[q] = w * detJ[(size_t)e * Np + q];
+		const double* ue = u + (size_t)e * nb;
+		const double rho = ue[q], ri = 1.0 / rho;
+		double V2 = 0.0;
+		for (int k = 0; k < D; ++k) { const double w2 = ue[(1 + k) * Np + q] * ri; V2 += w2 * w2; }
+		const double p = kGM1 * (ue[(D + 1) * Np + q] - 0.5 * rho * V2);
+		sWs[q] = sqrt(V2) + sqrt(fmax(kGamma * p * ri, 0.0));
+	}
+	__syncthreads();
+	__shared__ double mj, mdj, ws;
+	if (t == 0) {
+		double a = 0.0, b = 0.0, m = 0.0;
+		for (int q = 0; q < Np; ++q) { a += sWJ[q]; b += detJ[(size_t)e * Np + q]; m = fmax(m, sWs[q]); }
+		mj = a / Np;
+		mdj = b / Np;
+		ws = m;
+		sigma[e] = (float)(1.0 / (mj * inv_dt[e] + 0.25 * hmin[e] * m));
+	}
+	__syncthreads();
+	float* f = feat + (size_t)e * nf;
+	for (int i = t; i < nb; i += blockDim.x) {
+		const int v = i / Np;
+		const double ref = (v == 0) ? uref0 : (v == NV - 1 ? uref2 : uref1);
+		f[i] = (float)(u[(size_t)e * nb + i] / ref);
+	}
+	for (int i = t; i < Np; i += blockDim.x) f[nb + i] = (float)(detJ[(size_t)e * Np + i] / mdj);
+	if (t == 0) f[nb + Np] = (float)log10(hmin[e] * inv_dt[e] / ws + 1e-30);
+	for (int i = nb + Np + 1 + t; i < nf; i += blockDim.x) f[i] = 0.f;
+}
+
+__global__ void k_tanh_backward(const float* __restrict__ h, float* dh, size_t n)
+{
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+		dh[i] *= (1.f - h[i] * h[i]);
+}
+// out[j] = sum_b g[b][j]  (deterministic, one thread per column)
+__global__ void k_colsum(const float* __restrict__ g, int B, int N, float* out)
+{
+	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= N) return;
+	float s = 0.f;
+	for (int b = 0; b < B; ++b) s += g[(size_t)b * N + j];
+	out[j] = s;
+}
+__global__ void k_adam(float* w, const float* __restrict__ g, float* m, float* v, size_t n, float lr, float b1,
+                       float b2, float c1, float c2, float eps)
+{
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+		const float gi = g[i];
+		m[i] = b1 * m[i] + (1.f - b1) * gi;
+		v[i] = b2 * v[i] + (1.f - b2) * gi * gi;
+		w[i] -= lr * (m[i] / c1) / (sqrtf(v[i] / c2) + eps);
+	}
+}
+// the forward pass always sees the BF16 values of the FP32 master weights
+__global__ void k_round_bf16(const float* __restrict__ w, float* wb, size_t n)
+{
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+		wb[i] = bf2f(f2bf(w[i]));
+}
+
+static int gridn(size_t n) { return (int)((n + 255) / 256 < 8192 ? (n + 255) / 256 : 8192); }
+
 }  // namespace dgx
+
+// ------------------------------------------------------------------------------- the model
+struct dgx_mlp {
+	int nf = 0, nhid = 0, nout = 0, nb = 0, rank = 0, Np = 0;
+	float* w = nullptr;   // FP32 master parameters W1,b1,W2,b2,W3,b3
+	float* wb = nullptr;  // BF16-rounded copy used by the forward pass
+	float* gw = nullptr;
+	float* am = nullptr;
+	float* av = nullptr;
+	size_t nparam = 0;
+	size_t off[6] = { 0, 0, 0, 0, 0, 0 };
+	int step = 0;
+	int cap = 0;
+	float *h1 = nullptr, *h2 = nullptr, *o = nullptr, *go = nullptr, *gh2 = nullptr, *gh1 = nullptr;
+	double* lpart = nullptr;
+	hipStream_t s = nullptr;
+};
+
+namespace dgx {
+
+static void mlp_free_work(dgx_mlp* m)
+{
+	for (float* p : { m->h1, m->h2, m->o, m->go, m->gh2, m->gh1 })
+		if (p) (void)hipFree(p);
+	if (m->lpart) (void)hipFree(m->lpart);
+	m->h1 = m->h2 = m->o = m->go = m->gh2 = m->gh1 = nullptr;
+	m->lpart = nullptr;
+	m->cap = 0;
+}
+
+static hipError_t mlp_reserve(dgx_mlp* m, int B)
+{
+	if (B <= m->cap) return hipSuccess;
+	mlp_free_work(m);
+	DGX_HIP_CHECK(hipMalloc(&m->h1, sizeof(float) * (size_t)B * m->nhid));
+	DGX_HIP_CHECK(hipMalloc(&m->h2, sizeof(float) * (size_t)B * m->nhid));
+	DGX_HIP_CHECK(hipMalloc(&m->o, sizeof(float) * (size_t)B * m->nout));
+	DGX_HIP_CHECK(hipMalloc(&m->go, sizeof(float) * (size_t)B * m->nout));
+	DGX_HIP_CHECK(hipMalloc(&m->gh2, sizeof(float) * (size_t)B * m->nhid));
+	DGX_HIP_CHECK(hipMalloc(&m->gh1, sizeof(float) * (size_t)B * m->nhid));
+	DGX_HIP_CHECK(hipMalloc(&m->lpart, sizeof(double) * (size_t)B));
+	m->cap = B;
+	return hipSuccess;
+}
+
+static hipError_t mlp_forward(dgx_mlp* m, const float* X, int B)
+{
+	DGX_HIP_CHECK(mlp_reserve(m, B));
+	const float *W1 = m->wb + m->off[0], *b1 = m->wb + m->off[1], *W2 = m->wb + m->off[2], *b2 = m->wb + m->off[3],
+	            *W3 = m->wb + m->off[4], *b3 = m->wb + m->off[5];
+	DGX_HIP_CHECK(gemm(m->s, X, m->nf, 1, W1, 1, m->nf, m->h1, m->nhid, B, m->nhid, m->nf, 0, b1, 1));
+	DGX_HIP_CHECK(gemm(m->s, m->h1, m->nhid, 1, W2, 1, m->nhid, m->h2, m->nhid, B, m->nhid, m->nhid, 0, b2, 1));
+	return gemm(m->s, m->h2, m->nhid, 1, W3, 1, m->nhid, m->o, m->nout, B, m->nout, m->nhid, 0, b3, 0);
+}
+
+static hipError_t mlp_backward(dgx_mlp* m, const float* X, int B)
+{
+	const float *W2 = m->wb + m->off[2], *W3 = m->wb + m->off[4];
+	float *gW1 = m->gw + m->off[0], *gb1 = m->gw + m->off[1], *gW2 = m->gw + m->off[2], *gb2 = m->gw + m->off[3],
+	      *gW3 = m->gw + m->off[4], *gb3 = m->gw + m->off[5];
+	DGX_HIP_CHECK(gemm(m->s, m->go, 1, m->nout, m->h2, m->nhid, 1, gW3, m->nhid, m->nout, m->nhid, B, 0, nullptr, 0));
+	hipLaunchKernelGGL(k_colsum, dim3((m->nout + 255) / 256), dim3(256), 0, m->s, m->go, B, m->nout, gb3);
+	DGX_HIP_CHECK(gemm(m->s, m->go, m->nout, 1, W3, m->nhid, 1, m->gh2, m->nhid, B, m->nhid, m->nout, 0, nullptr, 0));
+	hipLaunchKernelGGL(k_tanh_backward, dim3(gridn((size_t)B * m->nhid)), dim3(256), 0, m->s, m->h2, m->gh2,
+	                   (size_t)B * m->nhid);
+	DGX_HIP_CHECK(gemm(m->s, m->gh2, 1, m->nhid, m->h1, m->nhid, 1, gW2, m->nhid, m->nhid, m->nhid, B, 0, nullptr, 0));
+	hipLaunchKernelGGL(k_colsum, dim3((m->nhid + 255) / 256), dim3(256), 0, m->s, m->gh2, B, m->nhid, gb2);
+	DGX_HIP_CHECK(gemm(m->s, m->gh2, m->nhid, 1, W2, m->nhid, 1, m->gh1, m->nhid, B, m->nhid, m->nhid, 0, nullptr, 0));
+	hipLaunchKernelGGL(k_tanh_backward, dim3(gridn((size_t)B * m->nhid)), dim3(256), 0, m->s, m->h1, m->gh1,
+	                   (size_t)B * m->nhid);
+	DGX_HIP_CHECK(gemm(m->s, m->gh1, 1, m->nhid, X, m->nf, 1, gW1, m->nf, m->nhid, m->nf, B, 0, nullptr, 0));
+	hipLaunchKernelGGL(k_colsum, dim3((m->nhid + 255) / 256), dim3(256), 0, m->s, m->gh1, B, m->nhid, gb1);
+	return hipGetLastError();
+}
+
+static hipError_t mlp_loss(dgx_mlp* m, const float* sigma, const double* A, int B, double* loss, float gscale,
+                           bool want_grad)
+{
+	float* g = want_grad ? m->go : nullptr;
+	if (m->nb == 64 && m->rank == 8)
+		hipLaunchKernelGGL((k_block_loss<64, 8>), dim3(B), dim3(256), 0, m->s, m->o, sigma, A, g, m->lpart, gscale);
+	else if (m->nb == 36 && m->rank == 8)
+		hipLaunchKernelGGL((k_block_loss<36, 8>), dim3(B), dim3(256), 0, m->s, m->o, sigma, A, g, m->lpart, gscale);
+	else
+		return hipErrorInvalidValue;
+	DGX_HIP_CHECK(hipGetLastError());
+	if (loss) {
+		std::vector<double> lp(B);
+		DGX_HIP_CHECK(hipMemcpyAsync(lp.data(), m->lpart, sizeof(double) * B, hipMemcpyDeviceToHost, m->s));
+		DGX_HIP_CHECK(hipStreamSynchronize(m->s));
+		double sum = 0.0;
+		for (double x : lp) sum += x;
+		*loss = sum / B;
+	}
+	return hipSuccess;
+}
+
+static hipError_t mlp_sync_bf16(dgx_mlp* m)
+{
+	hipLaunchKernelGGL(k_round_bf16, dim3(gridn(m->nparam)), dim3(256), 0, m->s, m->w, m->wb, m->nparam);
+	return hipGetLastError();
+}
+
+struct PrecondLearned : Precond {
+	dgx_mlp* mlp = nullptr;
+	float* feat = nullptr;
+	float* sigma = nullptr;
+	~PrecondLearned() override
+	{
+		if (feat) (void)hipFree(feat);
+		if (sigma) (void)hipFree(sigma);
+	}
+	hipError_t setup(Ctx& c, const double* u, const double* inv_dt) override;
+	hipError_t apply(Ctx& c, const double* v, double* z) override
+	{
+		if (mlp->nb == 64 && mlp->rank == 8)
+			hipLaunchKernelGGL((k_lowrank_apply<64, 8>), dim3(c.E), dim3(64), 0, c.stream, mlp->o, sigma, v, z, c.E);
+		else if (mlp->nb == 36 && mlp->rank == 8)
+			hipLaunchKernelGGL((k_lowrank_apply<36, 8>), dim3(c.E), dim3(64), 0, c.stream, mlp->o, sigma, v, z, c.E);
+		else
+			return hipErrorInvalidValue;
+		return hipGetLastError();
+	}
+};
+
+hipError_t launch_features(Ctx& c, const double* u, const double* inv_dt, float* feat, int nf, float* sigma)
+{
+	if (c.d != 2 || c.nv != 4) return hipErrorInvalidValue;
+	const double* bp = c.cfg.bc_prim;
+	const double rho = bp[0] > 0 ? bp[0] : 1.0;
+	const double vel = std::sqrt(bp[1] * bp[1] + bp[2] * bp[2]);
+	const double uref1 = rho * (vel > 0 ? vel : 1.0);
+	const double pres = bp[3] > 0 ? bp[3] : 1.0;
+	const double uref2 = pres / kGM1 + 0.5 * rho * vel * vel;
+#define DGX_FEAT(NN)                                                                                              \
+	if (c.N == NN) {                                                                                              \
+		hipLaunchKernelGGL((k_features<2, NN>), dim3(c.E), dim3(64), 0, c.stream, u, c.detJ_vc, c.ops64, c.hmin,   \
+		                   inv_dt, c.E, feat, nf, rho, uref1, uref2, sigma);                                       \
+		return hipGetLastError();                                                                                 \
+	}
+	DGX_FEAT(2) DGX_FEAT(3) DGX_FEAT(4) DGX_FEAT(5) DGX_FEAT(6)
+#undef DGX_FEAT
+	return hipErrorInvalidValue;
+}
+
+hipError_t PrecondLearned::setup(Ctx& c, const double* u, const double* inv_dt)
+{
+	if (!feat) {
+		DGX_HIP_CHECK(hipMalloc(&feat, sizeof(float) * (size_t)c.E * mlp->nf));
+		DGX_HIP_CHECK(hipMalloc(&sigma, sizeof(float) * (size_t)c.E));
+	}
+	// inference runs on the solver's stream; the model keeps its own stream for training calls
+	const hipStream_t saved = mlp->s;
+	mlp->s = c.stream;
+	hipError_t err = launch_features(c, u, inv_dt, feat, mlp->nf, sigma);
+	if (err == hipSuccess) err = mlp_forward(mlp, feat, c.E);
+	mlp->s = saved;
+	return err;
+}
+
+Precond* make_learned_precond(Ctx& c)
+{
+	if (!c.mlp || c.mlp->nb != c.Np * c.nv || c.mlp->Np != c.Np) return nullptr;
+	PrecondLearned* p = new PrecondLearned();
+	p->mlp = c.mlp;
+	return p;
+}
+
+static uint64_t splitmix64(uint64_t& s)
+{
+	uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+	z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+	z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+	return z ^ (z >> 31);
+}
+
+}  // namespace dgx
+
+extern "C" {
+
+dgx_status dgx_mlp_create(dgx_mlp** out, int nb, int Np, int hidden, int rank, unsigned long long seed)
+{
+	if (!out || nb <= 0 || Np <= 0 || hidden < 16 || hidden > 1024 || rank != 8 || !(nb == 64 || nb == 36))
+		return DGX_ERR_ARG;
+	dgx_mlp* m = new dgx_mlp();
+	m->nb = nb;
+	m->Np = Np;
+	m->rank = rank;
+	m->nhid = hidden;
+	m->nf = ((nb + Np + 1) + 15) / 16 * 16;
+	m->nout = nb + 2 * nb * rank;
+	const size_t sizes[6] = { (size_t)m->nhid * m->nf, (size_t)m->nhid, (size_t)m->nhid * m->nhid, (size_t)m->nhid,
+	                          (size_t)m->nout * m->nhid, (size_t)m->nout };
+	size_t acc = 0;
+	for (int i = 0; i < 6; ++i) { m->off[i] = acc; acc += sizes[i]; }
+	m->nparam = acc;
+	std::vector<float> w(acc, 0.f);
+	uint64_t st = seed ? seed : 20261202ull;
+	auto uni = [&](float a) { return (float)(((dgx::splitmix64(st) >> 11) * (1.0 / 9007199254740992.0)) * 2.0 - 1.0) * a; };
+	const int fan[3][2] = { { m->nf, m->nhid }, { m->nhid, m->nhid }, { m->nhid, m->nout } };
+	for (int l = 0; l < 3; ++l) {
+		const float a = std::sqrt(6.0f / (fan[l][0] + fan[l][1])) * (l == 2 ? 0.1f : 1.0f);
+		for (size_t i = 0; i < sizes[2 * l]; ++i) w[m->off[2 * l] + i] = uni(a);
+	}
+	for (int i = 0; i < nb; ++i) w[m->off[5] + i] = -1.0f;  // start from a (negative) scaled identity
+	if (hipMalloc(&m->w, sizeof(float) * acc) != hipSuccess || hipMalloc(&m->wb, sizeof(float) * acc) != hipSuccess ||
+	    hipMalloc(&m->gw, sizeof(float) * acc) != hipSuccess || hipMalloc(&m->am, sizeof(float) * acc) != hipSuccess ||
+	    hipMalloc(&m->av, sizeof(float) * acc) != hipSuccess) {
+		dgx_mlp_destroy(m);
+		return DGX_ERR_HIP;
+	}
+	std::vector<float> z(acc, 0.f);
+	if (hipMemcpy(m->w, w.data(), sizeof(float) * acc, hipMemcpyHostToDevice) != hipSuccess ||
+	    hipMemcpy(m->am, z.data(), sizeof(float) * acc, hipMemcpyHostToDevice) != hipSuccess ||
+	    hipMemcpy(m->av, z.data(), sizeof(float) * acc, hipMemcpyHostToDevice) != hipSuccess ||
+	    dgx::mlp_sync_bf16(m) != hipSuccess || hipStreamSynchronize(m->s) != hipSuccess) {
+		dgx_mlp_destroy(m);
+		return DGX_ERR_HIP;
+	}
+	*out = m;
+	return DGX_OK;
+}
+
+dgx_status dgx_mlp_destroy(dgx_mlp* m)
+{
+	if (!m) return DGX_OK;
+	dgx::mlp_free_work(m);
+	for (float* p : { m->w, m->wb, m->gw, m->am, m->av })
+		if (p) (void)hipFree(p);
+	delete m;
+	return DGX_OK;
+}
+
+dgx_status dgx_mlp_sizes(const dgx_mlp* m, int* nf, int* nout, long long* nparam)
+{
+	if (!m) return DGX_ERR_ARG;
+	if (nf) *nf = m->nf;
+	if (nout) *nout = m->nout;
+	if (nparam) *nparam = (long long)m->nparam;
+	return DGX_OK;
+}
+
+dgx_status dgx_mlp_get_params(dgx_mlp* m, float* host)
+{
+	if (!m || !host) return DGX_ERR_ARG;
+	return hipMemcpy(host, m->w, sizeof(float) * m->nparam, hipMemcpyDeviceToHost) == hipSuccess ? DGX_OK : DGX_ERR_HIP;
+}
+
+dgx_status dgx_mlp_set_params(dgx_mlp* m, const float* host)
+{
+	if (!m || !host) return DGX_ERR_ARG;
+	if (hipMemcpy(m->w, host, sizeof(float) * m->nparam, hipMemcpyHostToDevice) != hipSuccess) return DGX_ERR_HIP;
+	if (dgx::mlp_sync_bf16(m) != hipSuccess || hipStreamSynchronize(m->s) != hipSuccess) return DGX_ERR_HIP;
+	return DGX_OK;
+}
+
+dgx_status dgx_mlp_features(dgx_ctx* h, const dgx_mlp* m, const double* u, const double* inv_dt, float* feat,
+                            float* sigma)
+{
+	if (!h || !m || !u || !inv_dt || !feat || !sigma) return DGX_ERR_ARG;
+	if (!h->c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (m->nb != h->c.Np * h->c.nv || m->Np != h->c.Np) return DGX_ERR_ARG;
+	return dgx::launch_features(h->c, u, inv_dt, feat, m->nf, sigma) == hipSuccess ? DGX_OK : DGX_ERR_HIP;
+}
+
+dgx_status dgx_mlp_loss_grad(dgx_mlp* m, const float* feat, const float* sigma, const double* blocks, int B,
+                             double* loss, float* grad_host)
+{
+	if (!m || !feat || !blocks || B <= 0) return DGX_ERR_ARG;
+	if (dgx::mlp_forward(m, feat, B) != hipSuccess) return DGX_ERR_HIP;
+	if (dgx::mlp_loss(m, sigma, blocks, B, loss, 1.0f / B, grad_host != nullptr) != hipSuccess) return DGX_ERR_HIP;
+	if (grad_host) {
+		if (dgx::mlp_backward(m, feat, B) != hipSuccess) return DGX_ERR_HIP;
+		if (hipMemcpyAsync(grad_host, m->gw, sizeof(float) * m->nparam, hipMemcpyDeviceToHost, m->s) != hipSuccess ||
+		    hipStreamSynchronize(m->s) != hipSuccess)
+			return DGX_ERR_HIP;
+	}
+	return DGX_OK;
+}
+
+dgx_status dgx_mlp_train(dgx_mlp* m, const float* feat, const float* sigma, const double* blocks, long long n,
+                         int epochs, int batch, float lr, double* loss)
+{
+	if (!m || !feat || !blocks || n <= 0 || epochs < 0 || batch <= 0 || !(lr > 0.f)) return DGX_ERR_ARG;
+	const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+	const long long nbatch = (n + batch - 1) / batch;
+	std::vector<long long> order(nbatch);
+	for (long long i = 0; i < nbatch; ++i) order[i] = i;
+	uint64_t rs = 0x5EEDull + (uint64_t)m->step;
+	for (int ep = 0; ep < epochs; ++ep) {
+		// visit the mini-batches (contiguous runs of samples) in a fresh seeded order each epoch
+		for (long long i = nbatch - 1; i > 0; --i) {
+			const long long j = (long long)(dgx::splitmix64(rs) % (uint64_t)(i + 1));
+			const long long t = order[i];
+			order[i] = order[j];
+			order[j] = t;
+		}
+		for (long long oi = 0; oi < nbatch; ++oi) {
+			const long long bi = order[oi];
+			const long long s0 = bi * batch;
+			const int B = (int)((s0 + batch <= n) ? batch : n - s0);
+			const float* X = feat + (size_t)s0 * m->nf;
+			const float* sg = sigma ? sigma + s0 : nullptr;
+			const double* A = blocks + (size_t)s0 * m->nb * m->nb;
+			if (dgx::mlp_forward(m, X, B) != hipSuccess) return DGX_ERR_HIP;
+			if (dgx::mlp_loss(m, sg, A, B, nullptr, 1.0f / B, true) != hipSuccess) return DGX_ERR_HIP;
+			if (dgx::mlp_backward(m, X, B) != hipSuccess) return DGX_ERR_HIP;
+			++m->step;
+			const float c1 = 1.f - std::pow(b1, (float)m->step), c2 = 1.f - std::pow(b2, (float)m->step);
+			hipLaunchKernelGGL(dgx::k_adam, dim3(dgx::gridn(m->nparam)), dim3(256), 0, m->s, m->w, m->gw, m->am, m->av,
+			                   m->nparam, lr, b1, b2, c1, c2, eps);
+			if (hipGetLastError() != hipSuccess || dgx::mlp_sync_bf16(m) != hipSuccess) return DGX_ERR_HIP;
+		}
+	}
+	if (loss) {
+		// full-set loss in chunks
+		double tot = 0.0;
+		for (long long bi = 0; bi < nbatch; ++bi) {
+			const long long s0 = bi * batch;
+			const int B = (int)((s0 + batch <= n) ? batch : n - s0);
+			double l = 0.0;
+			if (dgx::mlp_forward(m, feat + (size_t)s0 * m->nf, B) != hipSuccess ||
+			    dgx::mlp_loss(m, sigma ? sigma + s0 : nullptr, blocks + (size_t)s0 * m->nb * m->nb, B, &l, 1.0f, false) !=
+			        hipSuccess)
+				return DGX_ERR_HIP;
+			tot += l * B;
+		}
+		*loss = tot / n;
+	}
+	return hipStreamSynchronize(m->s) == hipSuccess ? DGX_OK : DGX_ERR_HIP;
+}
+
+dgx_status dgx_mlp_predict(dgx_mlp* m, const float* feat, int B, float* out)
+{
+	if (!m || !feat || !out || B <= 0) return DGX_ERR_ARG;
+	if (dgx::mlp_forward(m, feat, B) != hipSuccess) return DGX_ERR_HIP;
+	return hipMemcpyAsync(out, m->o, sizeof(float) * (size_t)B * m->nout, hipMemcpyDeviceToDevice, m->s) == hipSuccess
+	           ? DGX_OK
+	           : DGX_ERR_HIP;
+}
+
+dgx_status dgx_attach_mlp(dgx_ctx* h, dgx_mlp* m)
+{
+	if (!h) return DGX_ERR_ARG;
+	h->c.mlp = m;
+	return DGX_OK;
+}
+
+}  // extern "C"

@@ -54,6 +54,7 @@ typedef struct dgx_mesh {
 } dgx_mesh;
 
 typedef struct dgx_ctx dgx_ctx;
+typedef struct dgx_mlp dgx_mlp;
 
 /* Library/version and device helpers. */
 const char* dgx_version(void);
@@ -136,7 +137,14 @@ typedef struct dgx_implicit_params {
 	int gmres_restart;    /* -ksp_gmres_restart */
 	double gmres_rtol;    /* -ksp_rtol */
 	int gmres_max_iter;   /* per Newton step; hitting it is DIVERGED_ITS: counted, Newton continues */
-	int precond;          /* DGX_PC_* */
+	int precond;          /* DGX_PC_* (DGX_PC_LEARNED needs dgx_attach_mlp) */
+	/* optional capture of the training set along the iterations (DGX_PC_BLOCK_JACOBI only): per Newton
+	 * step the E diagonal blocks (E x nb x nb doubles) and features (E x nf floats) are appended while
+	 * capacity (in elements) lasts; device pointers or NULL */
+	double* capture_blocks;
+	float* capture_features;   /* nf = round_up(nb + Np + 1, 16) floats per element */
+	float* capture_sigma;      /* 1 float per element */
+	long long capture_capacity;
 } dgx_implicit_params;
 typedef struct dgx_implicit_stats {
 	int newton_steps;
@@ -146,9 +154,36 @@ typedef struct dgx_implicit_stats {
 	double max_rhs0;
 	double max_rhs;
 	double seconds;
+	long long captured;   /* elements appended to the capture buffers */
 	double history[DGX_HISTORY_MAX];
 } dgx_implicit_stats;
 dgx_status dgx_solve_implicit(dgx_ctx* ctx, double* u, const dgx_implicit_params* params, dgx_implicit_stats* stats);
+
+/* Learned block preconditioner (no counterpart in the reference; the reference's preconditioner is PETSc
+ * ILU(1)).  A 3-layer MLP with BF16 weights and FP32 accumulation maps element features
+ * [state / u_ref (nb) | det J / mean (Np) | log10(h / (dt * wave speed))] to
+ * P_e^-1 = sigma_e (diag(d_e) + U_e V_e^T), rank 8, sigma_e = 1 / (mean(w det J)/dt_e + 0.25 h_e lambda_e).
+ * Trained with its own forward/backward kernels on the solver's diagonal blocks. */
+dgx_status dgx_mlp_create(dgx_mlp** m, int nb, int Np, int hidden, int rank, unsigned long long seed);
+dgx_status dgx_mlp_destroy(dgx_mlp* m);
+/* nf: feature count (padded to 16), nout: outputs per element, nparam: FP32 master parameter count */
+dgx_status dgx_mlp_sizes(const dgx_mlp* m, int* nf, int* nout, long long* nparam);
+dgx_status dgx_mlp_get_params(dgx_mlp* m, float* host);        /* FP32 master weights W1,b1,W2,b2,W3,b3 */
+dgx_status dgx_mlp_set_params(dgx_mlp* m, const float* host);
+/* features (E x nf floats, device) and the per-element scale sigma (E floats, device) for state u and inv_dt */
+dgx_status dgx_mlp_features(dgx_ctx* ctx, const dgx_mlp* m, const double* u, const double* inv_dt, float* feat,
+                            float* sigma);
+/* mean loss and (optionally) the parameter gradient (HOST, nparam floats) on B samples (device inputs;
+ * sigma may be NULL = 1): the quantities the training step uses, exposed for gradient checks */
+dgx_status dgx_mlp_loss_grad(dgx_mlp* m, const float* feat, const float* sigma, const double* blocks, int B,
+                             double* loss, float* grad_host);
+/* Adam on n samples in mini-batches of `batch` for `epochs` passes; *loss receives the last full-set loss */
+dgx_status dgx_mlp_train(dgx_mlp* m, const float* feat, const float* sigma, const double* blocks, long long n,
+                         int epochs, int batch, float lr, double* loss);
+/* outputs (B x nout floats, device) for given features */
+dgx_status dgx_mlp_predict(dgx_mlp* m, const float* feat, int B, float* out);
+/* use m as the DGX_PC_LEARNED preconditioner of ctx (m must outlive the solves) */
+dgx_status dgx_attach_mlp(dgx_ctx* ctx, dgx_mlp* m);
 
 #ifdef __cplusplus
 }
