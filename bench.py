@@ -142,16 +142,22 @@ def main():
 
     f32 = lambda: ctx.compute_rhs_f32(u32.data_ptr(), r32.data_ptr())
     f64 = lambda: ctx.compute_rhs(u64.data_ptr(), r64.data_ptr())
+    rmx = torch.empty_like(u32)
+    fmx = lambda: ctx.compute_rhs_mixed(u64.data_ptr(), rmx.data_ptr())
     for _ in range(args.warmup):
         f32()
         f64()
+        fmx()
     torch.cuda.synchronize()
     ms32 = timed(f32, args.steps)
     print(f"fp32 residual: {ms32:.4f} ms", file=sys.stderr, flush=True)
     ms64 = timed(f64, args.steps)
     print(f"fp64 residual: {ms64:.4f} ms", file=sys.stderr, flush=True)
+    msmx = timed(fmx, args.steps)
+    print(f"mixed residual: {msmx:.4f} ms", file=sys.stderr, flush=True)
 
     err32 = ((r32.double() - r64).abs().max() / r64.abs().max()).item()
+    errmx = ((rmx.double() - r64).abs().max() / r64.abs().max()).item()
     # input rounding alone: FP64 residual of the FP32-rounded state
     u64r = u32.double()
     r64r = torch.empty_like(u64)
@@ -177,10 +183,13 @@ def main():
     E, Np, nv, d, Nfc = mesh.E, 125, 5, 3, 25
     bytes32 = E * (2 * nv * Np * 4 + Np * d * d * 4 + d * Nfc * d * 4)
     bytes64 = E * (2 * nv * Np * 8 + Np * d * d * 8 + d * Nfc * d * 8)
+    bytesmx = E * (nv * Np * 8 + nv * Np * 4 + Np * d * d * 4 + d * Nfc * d * 4)
     gdof32 = ndof / (ms32 * 1e-3) / 1e9
     gdof64 = ndof / (ms64 * 1e-3) / 1e9
     ach32 = bytes32 / (ms32 * 1e-3) / 1e9
     ach64 = bytes64 / (ms64 * 1e-3) / 1e9
+    achmx = bytesmx / (msmx * 1e-3) / 1e9
+    gdofmx = ndof / (msmx * 1e-3) / 1e9
 
     cpu = None if args.no_cpu else cpu_baseline()
 
@@ -205,6 +214,10 @@ def main():
                  "roofline": {"bound": "hbm", "achieved": round(ach64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": round(ach64 / HBM_PEAK_GBS, 4),
                               "traffic": _traffic_from_profiles("rhs_fp64")}},
+        "mixed": {"value": round(gdofmx, 3), "ms_per_step": round(msmx, 5), "rel_err_vs_fp64": errmx,
+                  "roofline": {"bound": "hbm", "achieved": round(achmx, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": round(achmx / HBM_PEAK_GBS, 4),
+                               "traffic": _traffic_from_profiles("rhs_mixed")}},
         "fp32_rel_err_vs_fp64": err32,
         "fp32_input_rounding_rel_err": err_in,
         "fp64_free_stream_max_abs": free_stream,

@@ -21,7 +21,7 @@ _ERRORS = {-1: "DGX_ERR_ARG", -2: "DGX_ERR_HIP", -3: "DGX_ERR_UNSUPPORTED", -4: 
 # every symbol include/dgx.h declares
 EXPORTS = [
     "dgx_version", "dgx_device_count", "dgx_create", "dgx_destroy", "dgx_set_stream", "dgx_synchronize",
-    "dgx_sizes", "dgx_upload_mesh", "dgx_get_geometry", "dgx_compute_rhs", "dgx_compute_rhs_f32",
+    "dgx_sizes", "dgx_upload_mesh", "dgx_get_geometry", "dgx_compute_rhs", "dgx_compute_rhs_f32", "dgx_compute_rhs_mixed",
     "dgx_compute_rhs_explicit", "dgx_apply_minv", "dgx_enforce_positivity", "dgx_ssp_rk3_step", "dgx_max_abs",
     "dgx_convert_f64_f32", "dgx_convert_f32_f64", "dgx_assemble_diag_blocks", "dgx_jacobian_vector",
     "dgx_block_lu_factor", "dgx_block_lu_solve", "dgx_cfl_dt", "dgx_mass_sub", "dgx_solve_implicit",
@@ -85,6 +85,7 @@ def load():
         "dgx_get_geometry": [vp, dp, dp, dp],
         "dgx_compute_rhs": [vp, vp, vp],
         "dgx_compute_rhs_f32": [vp, vp, vp],
+        "dgx_compute_rhs_mixed": [vp, vp, vp],
         "dgx_compute_rhs_explicit": [vp, vp, vp, dp],
         "dgx_apply_minv": [vp, vp],
         "dgx_enforce_positivity": [vp, vp],
@@ -186,6 +187,9 @@ class Context:
 
     def compute_rhs_f32(self, u: int, r: int):
         check("dgx_compute_rhs_f32", self.L.dgx_compute_rhs_f32(self.h, vp(u), vp(r)))
+
+    def compute_rhs_mixed(self, u: int, r: int):
+        check("dgx_compute_rhs_mixed", self.L.dgx_compute_rhs_mixed(self.h, vp(u), vp(r)))
 
     def compute_rhs_explicit(self, u: int, r: int) -> float:
         mx = C.c_double()

@@ -213,6 +213,14 @@ dgx_status dgx_compute_rhs_f32(dgx_ctx* h, const float* u, float* r)
 	return hip_status(dgx::launch_rhs32(h->c, u, r));
 }
 
+dgx_status dgx_compute_rhs_mixed(dgx_ctx* h, const double* u, float* r)
+{
+	if (!h || !u || !r) return DGX_ERR_ARG;
+	if (!h->c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (h->c.cfg.pde != DGX_PDE_EULER) return DGX_ERR_UNSUPPORTED;
+	return hip_status(dgx::launch_rhs_mixed(h->c, u, r));
+}
+
 static dgx_status read_max(Ctx& c, double* out)
 {
 	unsigned long long bits = 0;

@@ -43,7 +43,8 @@ __device__ inline float rv(float a) { return a; }
 __device__ inline double rv(Dual a) { return a.v; }
 
 __device__ inline double tsqrt(double a) { return sqrt(a); }
-__device__ inline float tsqrt(float a) { return sqrtf(a); }
+// FP32: the hardware square root and reciprocal (1 ulp), not the IEEE-rounded sequences
+__device__ inline float tsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
 __device__ inline Dual tsqrt(Dual a)
 {
 	const double s = sqrt(a.v);
@@ -57,7 +58,7 @@ __device__ inline Dual tpow(Dual a, double b)
 	return Dual(pv, b * pow(a.v, b - 1.0) * a.d);
 }
 __device__ inline double trcp(double a) { return 1.0 / a; }
-__device__ inline float trcp(float a) { return 1.0f / a; }
+__device__ inline float trcp(float a) { return __builtin_amdgcn_rcpf(a); }
 __device__ inline Dual trcp(Dual a)
 {
 	const double r = 1.0 / a.v;
@@ -79,7 +80,7 @@ template <typename T, int D>
 __device__ inline void euler_flux(const T* s, T* f)
 {
 	const T rho = s[0], E = s[D + 1];
-	const T rho_inv = T(1.0) / rho;
+	const T rho_inv = trcp(rho);
 	T uvw[D];
 #pragma unroll
 	for (int k = 0; k < D; ++k) uvw[k] = rho_inv * s[1 + k];

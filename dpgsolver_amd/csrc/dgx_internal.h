@@ -98,6 +98,7 @@ std::vector<int> morton_order(int d, int E, int Np, const double* xyz);
 hipError_t launch_geometry(Ctx& c);
 hipError_t launch_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned long long* dmax);
 hipError_t launch_rhs32(Ctx& c, const float* u, float* r);
+hipError_t launch_rhs_mixed(Ctx& c, const double* u, float* r);
 hipError_t launch_minv(Ctx& c, double* r);
 hipError_t launch_positivity(Ctx& c, double* u, int* status);
 hipError_t launch_maxabs(Ctx& c, const double* r, size_t n, unsigned long long* dmax);

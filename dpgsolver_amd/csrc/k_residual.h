@@ -30,12 +30,19 @@ namespace dgx {
 
 __device__ inline Dual operator*(double a, Dual b) { return Dual(a * b.v, a * b.d); }
 
-enum { MODE_JVP = 5, MODE_DIAG = 6 };
+// MODE_PERT: FP32 residual of an FP64 state in per-element perturbation form.  With c_e the state at
+// the element's first node, the kernel interpolates and contracts u - c_e and F(u) - F(c_e) in FP32
+// while every pointwise flux (volume and numerical) is evaluated in FP64 from c_e + (u - c_e).
+// The constant part F(c_e) drops out exactly: its volume and face contributions cancel per test
+// function (free-stream preservation), so the result is the full residual, and the FP32 rounding
+// acts only on the small within-element variation instead of on the whole flux.
+enum { MODE_JVP = 5, MODE_DIAG = 6, MODE_PERT = 7 };
 
 template <typename S, typename R>
 struct RhsArgs {
 	using Real = typename RealOf<S>::type;
 	const Real* u;        // state
+	const double* u64;    // FP64 state (MODE_PERT)
 	const Real* v;        // JVP direction
 	Real* out;            // residual (or diagonal blocks in MODE_DIAG)
 	const R* met;
@@ -238,6 +245,8 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 	__shared__ S sBuf[EPB][NV * Np + TRSZ];   // volume block U, then the face traces TR
 	__shared__ ElemFace sEF[EPB][NFACE];
 	__shared__ unsigned long long sMax;
+	constexpr bool PERT = (MODE == MODE_PERT);
+	__shared__ double sC[EPB][PERT ? NV : 1];   // MODE_PERT reference state c_e
 
 	// XCD-aware schedule: the grid is a multiple of 8 and consecutive workgroups land on different
 	// XCDs, so XCD x takes the x-th contiguous range of the (Morton-ordered) element sequence and
@@ -271,8 +280,12 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 	S* BUF = TR + NFACE * NV * NF;   // neighbour-trace half, reused for the flux components
 	const size_t ebase = (size_t)e * NV * Np;
 	if (valid) {
+		if constexpr (PERT) {
+			if (t < NV) sC[sl][t] = a.u64[ebase + (size_t)t * Np];
+		}
 		for (int i = t; i < NV * Np; i += NT) {
-			if constexpr (MODE == MODE_JVP) U[i] = S(a.u[ebase + i], a.v[ebase + i]);
+			if constexpr (PERT) U[i] = S(a.u64[ebase + i] - a.u64[ebase + (size_t)(i / Np) * Np]);
+			else if constexpr (MODE == MODE_JVP) U[i] = S(a.u[ebase + i], a.v[ebase + i]);
 			else if constexpr (MODE == MODE_DIAG) U[i] = S(a.u[ebase + i], i == a.col ? 1.0 : 0.0);
 			else U[i] = S(a.u[ebase + i]);
 		}
@@ -293,7 +306,8 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 #pragma unroll
 				for (int v = 0; v < NV; ++v) {
 					to[v * NF] = U[v * Np + vown];
-					if constexpr (MODE == MODE_JVP) tn[v * NF] = S(a.u[gi + v * Np], a.v[gi + v * Np]);
+					if constexpr (PERT) tn[v * NF] = S(a.u64[gi + v * Np] - sC[sl][v]);
+					else if constexpr (MODE == MODE_JVP) tn[v * NF] = S(a.u[gi + v * Np], a.v[gi + v * Np]);
 					else tn[v * NF] = S(a.u[gi + v * Np]);
 				}
 			} else {
@@ -351,13 +365,35 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 			R J2 = nJv[0] * nJv[0];
 #pragma unroll
 			for (int k2 = 1; k2 < D; ++k2) J2 += nJv[k2] * nJv[k2];
-			const R Jf = sqrt(J2);
-			const R Jfi = R(1.0) / Jf;
+			const R Jf = tsqrt(J2);
+			const R Jfi = trcp(Jf);
 			S nu[D];
 #pragma unroll
 			for (int k2 = 0; k2 < D; ++k2) nu[k2] = S(nJv[k2] * Jfi);
 			S nnf[NV];
-			if constexpr (PDE == DGX_PDE_EULER) {
+			if constexpr (PERT && PDE == DGX_PDE_EULER) {
+				// FP64 states c_e + delta, FP64 numerical flux minus the normal flux of c_e
+				double cL[NV], cR[NV], nd[D], fd[NV], cu[D], cn[NV];
+#pragma unroll
+				for (int v = 0; v < NV; ++v) {
+					cL[v] = sC[sl][v] + (double)sL[v];
+					cR[v] = sC[sl][v] + (double)sR[v];
+				}
+#pragma unroll
+				for (int k2 = 0; k2 < D; ++k2) nd[k2] = (double)nJv[k2] / (double)Jf;
+				if (ef.en < 0) bc_state<double, D>(bc, cL, nd, a.bc_prim, cR);
+				if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<double, D>(cL, cR, nd, fd);
+				else numflux_roe<double, D>(cL, cR, nd, fd);
+				const double* c0 = sC[sl];
+				const double ci = 1.0 / c0[0];
+				double V2c = 0.0;
+#pragma unroll
+				for (int k2 = 0; k2 < D; ++k2) { cu[k2] = c0[1 + k2] * ci; V2c += cu[k2] * cu[k2]; }
+				const double pc = kGM1 * (c0[D + 1] - 0.5 * c0[0] * V2c);
+				euler_normal_flux<double, D>(c0, cu, pc, nd, cn);
+#pragma unroll
+				for (int v = 0; v < NV; ++v) nnf[v] = S(fd[v] - cn[v]);
+			} else if constexpr (PDE == DGX_PDE_EULER) {
 				if (ef.en < 0) bc_state<S, D>(bc, sL, nu, a.bc_prim, sR);
 				if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<S, D>(sL, sR, nu, nnf);
 				else numflux_roe<S, D>(sL, sR, nu, nnf);
@@ -394,7 +430,27 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 		const int c0 = q % N, c1 = (q / N) % N, c2 = (D == 3) ? q / (N * N) : 0;
 		R wq = sW[c0] * sW[c1];
 		if (D == 3) wq = wq * sW[c2];
-		if constexpr (PDE == DGX_PDE_EULER) {
+		if constexpr (PERT && PDE == DGX_PDE_EULER) {
+			// physical fluxes of c_e + delta and of c_e in FP64, their difference contracted in FP32
+			double cs[NV], fu[D * NV], fc[D * NV];
+#pragma unroll
+			for (int v = 0; v < NV; ++v) cs[v] = sC[sl][v] + (double)s[v];
+			euler_flux<double, D>(cs, fu);
+			euler_flux<double, D>(sC[sl], fc);
+#pragma unroll
+			for (int v = 0; v < NV; ++v) {
+				S df[D];
+#pragma unroll
+				for (int k = 0; k < D; ++k) df[k] = S(fu[k + D * v] - fc[k + D * v]);
+#pragma unroll
+				for (int d0 = 0; d0 < D; ++d0) {
+					S acc = m[0 + D * d0] * df[0];
+#pragma unroll
+					for (int k = 1; k < D; ++k) acc += m[k + D * d0] * df[k];
+					fr[d0][v] = wq * acc;
+				}
+			}
+		} else if constexpr (PDE == DGX_PDE_EULER) {
 			euler_contravariant_flux<S, R, D>(s, m, wq, fr);
 		} else {
 #pragma unroll
@@ -427,7 +483,7 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 		if constexpr (MINV) {
 			R wq = sW[cc[0]] * sW[cc[1]];
 			if (D == 3) wq = wq * sW[cc[2]];
-			scale = R(1.0) / (wq * a.detJ[(size_t)e * Np + q]);
+			scale = trcp(wq * a.detJ[(size_t)e * Np + q]);
 		}
 		// per-face coefficient Lift_side[c_fd] * W_f and face-node offset, shared by all variables
 		R cf[NFACE];
@@ -504,7 +560,9 @@ hipError_t dispatch_rhs(int d, int p, int pde, int flux, const RhsArgs<S, R>& a,
 		return run_rhs<S, R, DD, NN, DGX_PDE_ADVECTION, DGX_FLUX_UPWIND, MODE>(a, st);
 	DGX_CASE_EULER(2, 2) DGX_CASE_EULER(2, 3) DGX_CASE_EULER(2, 4) DGX_CASE_EULER(2, 5) DGX_CASE_EULER(2, 6)
 	DGX_CASE_EULER(3, 2) DGX_CASE_EULER(3, 3) DGX_CASE_EULER(3, 4) DGX_CASE_EULER(3, 5)
-	DGX_CASE_ADV(2, 2) DGX_CASE_ADV(2, 3) DGX_CASE_ADV(2, 4) DGX_CASE_ADV(2, 5) DGX_CASE_ADV(2, 6)
+	if constexpr (MODE != MODE_PERT) {
+		DGX_CASE_ADV(2, 2) DGX_CASE_ADV(2, 3) DGX_CASE_ADV(2, 4) DGX_CASE_ADV(2, 5) DGX_CASE_ADV(2, 6)
+	}
 #undef DGX_CASE_EULER
 #undef DGX_CASE_ADV
 	return hipErrorInvalidValue;

@@ -86,6 +86,12 @@ dgx_status dgx_compute_rhs(dgx_ctx* ctx, const double* u, double* r);
 /* FP32 residual (defect-correction inner residual): FP32 state, FP32 geometry shadow,
  * FP32 arithmetic.  Same semantics as dgx_compute_rhs. */
 dgx_status dgx_compute_rhs_f32(dgx_ctx* ctx, const float* u, float* r);
+/* FP32 residual of an FP64 state (Euler only), same semantics as dgx_compute_rhs: the kernel forms
+ * u - c_e and F(u) - F(c_e) per element (c_e = the state at the element's first node), evaluates the
+ * pointwise physical and numerical fluxes in FP64 and runs the interpolations, contractions and lifts
+ * in FP32 on those perturbations.  The F(c_e) part cancels exactly (free-stream preservation), so
+ * FP32 rounding acts only on the within-element variation. */
+dgx_status dgx_compute_rhs_mixed(dgx_ctx* ctx, const double* u, float* r);
 /* compute_rhs_dg (solve_dg.c:130-136): r = M^-1 R(u), and max|r|_inf to *max_abs_r
  * (HOST pointer, may be NULL; synchronous when given) (compute_rlhs.c:59-71). */
 dgx_status dgx_compute_rhs_explicit(dgx_ctx* ctx, const double* u, double* r, double* max_abs_r);

@@ -153,6 +153,48 @@ def test_fp32_residual_close_to_fp64(dev):
     ctx.close()
 
 
+EULER = [c for c in CASES if c[2] == M.PDE_EULER]
+
+
+@pytest.mark.parametrize("case", EULER, ids=[c[0] for c in EULER])
+def test_mixed_residual_matches_oracle(dev, case):
+    """FP32 contractions in per-element perturbation form, FP64 state and pointwise fluxes."""
+    name, mesh, pde, flux, b, bcp, u = case
+    ctx = _ctx(mesh, pde, flux, b, bcp)
+    ud = torch.from_numpy(u).to(dev)
+    r = torch.empty(ud.numel(), dtype=torch.float32, device=dev)
+    ctx.compute_rhs_mixed(ud.data_ptr(), r.data_ptr())
+    torch.cuda.synchronize()
+    ref = O.Oracle(mesh, pde, flux, b, bcp).rhs(u)
+    err = _rel(r.double().cpu().numpy(), ref)
+    print(f"{name}: mixed vs oracle {err:.3e}")
+    assert err < 2e-5
+    ctx.close()
+
+
+def test_mixed_residual_error_on_curved_hex(dev):
+    """At 16^3 curved HEX p=4 the perturbation form keeps the FP32 error below 1e-6 (the pure FP32
+    path, whose state is rounded to FP32, does not)."""
+    mesh = M.config3_mesh(16, 4)
+    u = M.config3_state(mesh)
+    ctx = _ctx(mesh, M.PDE_EULER, M.FLUX_LF)
+    ud = torch.from_numpy(u).to(dev)
+    r64 = torch.empty_like(ud)
+    ctx.compute_rhs(ud.data_ptr(), r64.data_ptr())
+    rm = torch.empty(ud.numel(), dtype=torch.float32, device=dev)
+    ctx.compute_rhs_mixed(ud.data_ptr(), rm.data_ptr())
+    u32 = torch.empty(ud.numel(), dtype=torch.float32, device=dev)
+    ctx.convert_f64_f32(ud.data_ptr(), u32.data_ptr(), ud.numel())
+    r32 = torch.empty_like(u32)
+    ctx.compute_rhs_f32(u32.data_ptr(), r32.data_ptr())
+    torch.cuda.synchronize()
+    ref = r64.cpu().numpy()
+    em, e32 = _rel(rm.double().cpu().numpy(), ref), _rel(r32.double().cpu().numpy(), ref)
+    print(f"16^3 HEX p=4: mixed {em:.3e}, fp32 {e32:.3e}")
+    assert em < 1e-6 and em < 0.2 * e32
+    ctx.close()
+
+
 def test_free_stream_preserved_on_curved_hex_gpu(dev):
     mesh = M.config3_mesh(6, 4)
     prim = np.array([1.0, 0.3, -0.2, 0.1, 1.0])
