@@ -98,7 +98,7 @@ def load():
         "dgx_mass_sub": [vp, vp, vp, vp],
         "dgx_solve_implicit": [vp, vp, C.POINTER(ImplicitParams), C.POINTER(ImplicitStats)],
         "dgx_jacobian_vector": [vp, vp, vp, vp],
-        "dgx_mlp_create": [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong],
+        "dgx_mlp_create": [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong],
         "dgx_mlp_destroy": [vp],
         "dgx_mlp_sizes": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_longlong)],
         "dgx_mlp_get_params": [vp, C.POINTER(C.c_float)],
@@ -259,13 +259,20 @@ class Context:
         check("dgx_block_lu_solve", self.L.dgx_block_lu_solve(self.h, vp(lu), vp(piv), vp(x), E, nb))
 
 
+MLP_LOWRANK, MLP_KRON = 0, 1
+
+
 class MLP:
     """The learned block preconditioner (dgx_mlp_*)."""
 
-    def __init__(self, nb: int, Np: int, hidden: int = 256, rank: int = 8, seed: int = 20261202):
+    def __init__(self, nb: int, Np: int, hidden: int = 256, rank: int | None = None, seed: int = 20261202,
+                 kind: int = 0):
         self.L = load()
         h = vp()
-        check("dgx_mlp_create", self.L.dgx_mlp_create(C.byref(h), nb, Np, hidden, rank, seed))
+        if rank is None:
+            rank = 8 if kind == MLP_LOWRANK else 4
+        check("dgx_mlp_create", self.L.dgx_mlp_create(C.byref(h), kind, nb, Np, hidden, rank, seed))
+        self.kind = kind
         self.h = h
         nf, nout, npar = C.c_int(), C.c_int(), C.c_longlong()
         check("dgx_mlp_sizes", self.L.dgx_mlp_sizes(h, C.byref(nf), C.byref(nout), C.byref(npar)))

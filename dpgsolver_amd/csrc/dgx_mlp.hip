@@ -1,9 +1,11 @@
 // dgx_mlp.hip -- learned block preconditioner: a 3-layer MLP (BF16 weights, FP32 accumulation on
-// MFMA) maps per-element features to P_e^-1 = diag(d_e) + U_e V_e^T (rank R), an approximate inverse of
-// the element diagonal block A_e = dR_e/du_e - M_e/dt_e of the pseudo-time Newton system.
-//
-// P_e^-1 = sigma_e (diag(d_e) + U_e V_e^T) with the analytic scale sigma_e = 1 / (mean(w detJ)/dt_e +
-// 0.25 h_e lambda_e) so that the network outputs stay O(1) along the CFL ramp.
+// MFMA) maps per-element features to an approximate inverse P_e of the element diagonal block
+// A_e = dR_e/du_e - M_e/dt_e of the pseudo-time Newton system, in one of two structures:
+//   LOWRANK: P_e = sigma_e (diag(d_e) + U_e V_e^T), rank R (= 8);
+//   KRON:    P_e = sigma_e sum_{r<R} S_r (x) Q_r, S_r nv x nv (variables), Q_r Np x Np (nodes), R = 4,
+//            which follows the (variables) x (nodes) Kronecker structure of the DG block;
+// sigma_e = 1 / (mean(w detJ)/dt_e + 0.25 h_e lambda_e) is analytic, so the network outputs stay O(1)
+// along the CFL ramp.
 // Training data are the solver's own blocks captured along its Newton iterations; the loss is
 //   L = (1/B) sum_e ||P_e A_e - I||_F^2 / nb,
 // minimised with Adam.  Forward, backward and the loss gradient are hand-written kernels; the GEMMs use
@@ -169,6 +171,107 @@ __global__ void __launch_bounds__(64) k_lowrank_apply(const float* __restrict__ 
 	}
 }
 
+// KRON structure.  out = [S_0 .. S_{RK-1} (NV x NV each) | Q_0 .. Q_{RK-1} (NPN x NPN each)], row-major;
+// P[(a,i),(b,j)] = sigma sum_r S_r[a,b] Q_r[i,j] (rows var-major like the state).
+// With G = sigma dL/dP: dS_r[a,b] = sum_ij G[(a,i),(b,j)] Q_r[i,j], dQ_r[i,j] = sum_ab G[(a,i),(b,j)] S_r[a,b].
+template <int NV, int NPN, int RK>
+__global__ void __launch_bounds__(256) k_block_loss_kron(const float* __restrict__ out,
+                                                         const float* __restrict__ sigma,
+                                                         const double* __restrict__ A, float* grad,
+                                                         double* loss_partial, float gscale)
+{
+	constexpr int NB = NV * NPN, NS = NV * NV, NQ = NPN * NPN, NO = RK * (NS + NQ);
+	__shared__ float sA[NB * NB], sP[NB * NB], sR[NB * NB];
+	__shared__ float sO[NO];
+	__shared__ double red[256];
+	const int e = blockIdx.x, t = threadIdx.x;
+	const float sg = sigma ? sigma[e] : 1.f;
+	const float* o = out + (size_t)e * NO;
+	for (int i = t; i < NB * NB; i += blockDim.x) sA[i] = (float)A[(size_t)e * NB * NB + i];
+	for (int i = t; i < NO; i += blockDim.x) sO[i] = o[i];
+	__syncthreads();
+	const float* S = sO;
+	const float* Q = sO + RK * NS;
+	for (int ij = t; ij < NB * NB; ij += blockDim.x) {
+		const int row = ij / NB, col = ij % NB;
+		const int a = row / NPN, i = row % NPN, b = col / NPN, j = col % NPN;
+		float p = 0.f;
+		for (int r = 0; r < RK; ++r) p += S[r * NS + a * NV + b] * Q[r * NQ + i * NPN + j];
+		sP[ij] = sg * p;
+	}
+	__syncthreads();
+	double lsum = 0.0;
+	for (int ij = t; ij < NB * NB; ij += blockDim.x) {
+		const int i = ij / NB, j = ij % NB;
+		float acc = (i == j) ? -1.f : 0.f;
+		for (int k = 0; k < NB; ++k) acc += sP[i * NB + k] * sA[k * NB + j];
+		sR[ij] = acc;
+		lsum += (double)acc * acc;
+	}
+	red[t] = lsum;
+	__syncthreads();
+	for (int off = 128; off > 0; off >>= 1) {
+		if (t < off) red[t] += red[t + off];
+		__syncthreads();
+	}
+	if (t == 0) loss_partial[e] = red[0] / NB;
+	if (!grad) return;
+	for (int ij = t; ij < NB * NB; ij += blockDim.x) {
+		const int i = ij / NB, j = ij % NB;
+		float acc = 0.f;
+		for (int k = 0; k < NB; ++k) acc += sR[i * NB + k] * sA[j * NB + k];
+		sP[ij] = acc * (2.0f / NB) * gscale * sg;
+	}
+	__syncthreads();
+	float* g = grad + (size_t)e * NO;
+	for (int x = t; x < RK * NS; x += blockDim.x) {
+		const int r = x / NS, a = (x % NS) / NV, b = x % NV;
+		float acc = 0.f;
+		for (int i = 0; i < NPN; ++i)
+			for (int j = 0; j < NPN; ++j) acc += sP[(a * NPN + i) * NB + b * NPN + j] * Q[r * NQ + i * NPN + j];
+		g[x] = acc;
+	}
+	for (int x = t; x < RK * NQ; x += blockDim.x) {
+		const int r = x / NQ, i = (x % NQ) / NPN, j = x % NPN;
+		float acc = 0.f;
+		for (int a = 0; a < NV; ++a)
+			for (int b = 0; b < NV; ++b) acc += sP[(a * NPN + i) * NB + b * NPN + j] * S[r * NS + a * NV + b];
+		g[RK * NS + x] = acc;
+	}
+}
+
+// z_e[a,i] = sigma_e sum_r sum_b S_r[a,b] sum_j Q_r[i,j] v_e[b,j]   (FP64 vectors, FP32 factors)
+template <int NV, int NPN, int RK>
+__global__ void __launch_bounds__(64) k_kron_apply(const float* __restrict__ out, const float* __restrict__ sigma,
+                                                   const double* __restrict__ v, double* z, int E)
+{
+	constexpr int NB = NV * NPN, NS = NV * NV, NQ = NPN * NPN, NO = RK * (NS + NQ);
+	__shared__ float sO[NO];
+	__shared__ double sv[NB], sW[RK * NB];
+	const int e = blockIdx.x, t = threadIdx.x;
+	const float* o = out + (size_t)e * NO;
+	for (int i = t; i < NO; i += blockDim.x) sO[i] = o[i];
+	for (int i = t; i < NB; i += blockDim.x) sv[i] = v[(size_t)e * NB + i];
+	__syncthreads();
+	const float* S = sO;
+	const float* Q = sO + RK * NS;
+	for (int x = t; x < RK * NB; x += blockDim.x) {   // W_r[b][i] = sum_j Q_r[i,j] v[b,j]
+		const int r = x / NB, b = (x % NB) / NPN, i = x % NPN;
+		double acc = 0.0;
+		for (int j = 0; j < NPN; ++j) acc += (double)Q[r * NQ + i * NPN + j] * sv[b * NPN + j];
+		sW[x] = acc;
+	}
+	__syncthreads();
+	const double sg = sigma[e];
+	for (int x = t; x < NB; x += blockDim.x) {
+		const int a = x / NPN, i = x % NPN;
+		double acc = 0.0;
+		for (int r = 0; r < RK; ++r)
+			for (int b = 0; b < NV; ++b) acc += (double)S[r * NS + a * NV + b] * sW[r * NB + b * NPN + i];
+		z[(size_t)e * NB + x] = sg * acc;
+	}
+}
+
 // features [u / u_ref (nb) | detJ / mean (Np) | log10(h inv_dt / lambda) | 0 pad] and sigma_e
 template <int D, int N>
 __global__ void __launch_bounds__(64) k_features(const double* __restrict__ u, const double* __restrict__ detJ,
@@ -250,6 +353,7 @@ static int gridn(size_t n) { return (int)((n + 255) / 256 < 8192 ? (n + 255) / 2
 
 // ------------------------------------------------------------------------------- the model
 struct dgx_mlp {
+	int kind = 0;
 	int nf = 0, nhid = 0, nout = 0, nb = 0, rank = 0, Np = 0;
 	float* w = nullptr;   // FP32 master parameters W1,b1,W2,b2,W3,b3
 	float* wb = nullptr;  // BF16-rounded copy used by the forward pass
@@ -326,10 +430,16 @@ static hipError_t mlp_loss(dgx_mlp* m, const float* sigma, const double* A, int 
                            bool want_grad)
 {
 	float* g = want_grad ? m->go : nullptr;
-	if (m->nb == 64 && m->rank == 8)
+	if (m->kind == DGX_MLP_LOWRANK && m->nb == 64 && m->rank == 8)
 		hipLaunchKernelGGL((k_block_loss<64, 8>), dim3(B), dim3(256), 0, m->s, m->o, sigma, A, g, m->lpart, gscale);
-	else if (m->nb == 36 && m->rank == 8)
+	else if (m->kind == DGX_MLP_LOWRANK && m->nb == 36 && m->rank == 8)
 		hipLaunchKernelGGL((k_block_loss<36, 8>), dim3(B), dim3(256), 0, m->s, m->o, sigma, A, g, m->lpart, gscale);
+	else if (m->kind == DGX_MLP_KRON && m->Np == 16 && m->rank == 4)
+		hipLaunchKernelGGL((k_block_loss_kron<4, 16, 4>), dim3(B), dim3(256), 0, m->s, m->o, sigma, A, g, m->lpart,
+		                   gscale);
+	else if (m->kind == DGX_MLP_KRON && m->Np == 9 && m->rank == 4)
+		hipLaunchKernelGGL((k_block_loss_kron<4, 9, 4>), dim3(B), dim3(256), 0, m->s, m->o, sigma, A, g, m->lpart,
+		                   gscale);
 	else
 		return hipErrorInvalidValue;
 	DGX_HIP_CHECK(hipGetLastError());
@@ -362,6 +472,15 @@ struct PrecondLearned : Precond {
 	hipError_t setup(Ctx& c, const double* u, const double* inv_dt) override;
 	hipError_t apply(Ctx& c, const double* v, double* z) override
 	{
+		if (mlp->kind == DGX_MLP_KRON) {
+			if (mlp->Np == 16)
+				hipLaunchKernelGGL((k_kron_apply<4, 16, 4>), dim3(c.E), dim3(64), 0, c.stream, mlp->o, sigma, v, z, c.E);
+			else if (mlp->Np == 9)
+				hipLaunchKernelGGL((k_kron_apply<4, 9, 4>), dim3(c.E), dim3(64), 0, c.stream, mlp->o, sigma, v, z, c.E);
+			else
+				return hipErrorInvalidValue;
+			return hipGetLastError();
+		}
 		if (mlp->nb == 64 && mlp->rank == 8)
 			hipLaunchKernelGGL((k_lowrank_apply<64, 8>), dim3(c.E), dim3(64), 0, c.stream, mlp->o, sigma, v, z, c.E);
 		else if (mlp->nb == 36 && mlp->rank == 8)
@@ -427,17 +546,21 @@ static uint64_t splitmix64(uint64_t& s)
 
 extern "C" {
 
-dgx_status dgx_mlp_create(dgx_mlp** out, int nb, int Np, int hidden, int rank, unsigned long long seed)
+dgx_status dgx_mlp_create(dgx_mlp** out, int kind, int nb, int Np, int hidden, int rank, unsigned long long seed)
 {
-	if (!out || nb <= 0 || Np <= 0 || hidden < 16 || hidden > 1024 || rank != 8 || !(nb == 64 || nb == 36))
-		return DGX_ERR_ARG;
+	if (!out || hidden < 16 || hidden > 1024) return DGX_ERR_ARG;
+	// 2-D Euler blocks: nv = 4 variables, p = 2 (Np = 9) or p = 3 (Np = 16)
+	if (!((nb == 64 && Np == 16) || (nb == 36 && Np == 9))) return DGX_ERR_ARG;
+	if (!((kind == DGX_MLP_LOWRANK && rank == 8) || (kind == DGX_MLP_KRON && rank == 4))) return DGX_ERR_ARG;
+	const int nv = nb / Np;
 	dgx_mlp* m = new dgx_mlp();
+	m->kind = kind;
 	m->nb = nb;
 	m->Np = Np;
 	m->rank = rank;
 	m->nhid = hidden;
 	m->nf = ((nb + Np + 1) + 15) / 16 * 16;
-	m->nout = nb + 2 * nb * rank;
+	m->nout = (kind == DGX_MLP_LOWRANK) ? nb + 2 * nb * rank : rank * (nv * nv + Np * Np);
 	const size_t sizes[6] = { (size_t)m->nhid * m->nf, (size_t)m->nhid, (size_t)m->nhid * m->nhid, (size_t)m->nhid,
 	                          (size_t)m->nout * m->nhid, (size_t)m->nout };
 	size_t acc = 0;
@@ -451,7 +574,13 @@ dgx_status dgx_mlp_create(dgx_mlp** out, int nb, int Np, int hidden, int rank, u
 		const float a = std::sqrt(6.0f / (fan[l][0] + fan[l][1])) * (l == 2 ? 0.1f : 1.0f);
 		for (size_t i = 0; i < sizes[2 * l]; ++i) w[m->off[2 * l] + i] = uni(a);
 	}
-	for (int i = 0; i < nb; ++i) w[m->off[5] + i] = -1.0f;  // start from a (negative) scaled identity
+	// start from the (negative) scaled identity P = -sigma I
+	if (kind == DGX_MLP_LOWRANK) {
+		for (int i = 0; i < nb; ++i) w[m->off[5] + i] = -1.0f;
+	} else {
+		for (int a = 0; a < nv; ++a) w[m->off[5] + a * nv + a] = -1.0f;                         // S_0 = -I
+		for (int i = 0; i < Np; ++i) w[m->off[5] + rank * nv * nv + i * Np + i] = 1.0f;          // Q_0 = I
+	}
 	if (hipMalloc(&m->w, sizeof(float) * acc) != hipSuccess || hipMalloc(&m->wb, sizeof(float) * acc) != hipSuccess ||
 	    hipMalloc(&m->gw, sizeof(float) * acc) != hipSuccess || hipMalloc(&m->am, sizeof(float) * acc) != hipSuccess ||
 	    hipMalloc(&m->av, sizeof(float) * acc) != hipSuccess) {

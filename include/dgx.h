@@ -167,10 +167,14 @@ dgx_status dgx_solve_implicit(dgx_ctx* ctx, double* u, const dgx_implicit_params
 
 /* Learned block preconditioner (no counterpart in the reference; the reference's preconditioner is PETSc
  * ILU(1)).  A 3-layer MLP with BF16 weights and FP32 accumulation maps element features
- * [state / u_ref (nb) | det J / mean (Np) | log10(h / (dt * wave speed))] to
- * P_e^-1 = sigma_e (diag(d_e) + U_e V_e^T), rank 8, sigma_e = 1 / (mean(w det J)/dt_e + 0.25 h_e lambda_e).
+ * [state / u_ref (nb) | det J / mean (Np) | log10(h / (dt * wave speed))] to an approximate inverse of the
+ * element diagonal block, scaled by sigma_e = 1 / (mean(w det J)/dt_e + 0.25 h_e lambda_e):
+ *   DGX_MLP_LOWRANK: P_e = sigma_e (diag(d_e) + U_e V_e^T), rank 8 (outputs nb + 2 nb 8);
+ *   DGX_MLP_KRON:    P_e = sigma_e sum_{r<4} S_r (x) Q_r, S_r nv x nv, Q_r Np x Np (outputs 4 (nv^2 + Np^2)).
+ * Both start from P_e = -sigma_e I.  2-D Euler with p = 2 (nb 36) or p = 3 (nb 64).
  * Trained with its own forward/backward kernels on the solver's diagonal blocks. */
-dgx_status dgx_mlp_create(dgx_mlp** m, int nb, int Np, int hidden, int rank, unsigned long long seed);
+enum { DGX_MLP_LOWRANK = 0, DGX_MLP_KRON = 1 };
+dgx_status dgx_mlp_create(dgx_mlp** m, int kind, int nb, int Np, int hidden, int rank, unsigned long long seed);
 dgx_status dgx_mlp_destroy(dgx_mlp* m);
 /* nf: feature count (padded to 16), nout: outputs per element, nparam: FP32 master parameter count */
 dgx_status dgx_mlp_sizes(const dgx_mlp* m, int* nf, int* nout, long long* nparam);

@@ -12,8 +12,11 @@ exact block-Jacobi preconditioner applies.
 
 Model (shared with dgx_mlp.hip):
   features x_e (nf)  ->  h1 = tanh(W1 x + b1)  ->  h2 = tanh(W2 h1 + b2)  ->  o = W3 h2 + b3
-  o = [d (nb) | U (nb x R, row-major) | V (nb x R, row-major)]
-  P_e = sigma_e (diag(d) + U V^T),   L = (1/B) sum_e ||P_e A_e - I||_F^2 / nb.
+  kind 0 (low rank): o = [d (nb) | U (nb x R, row-major) | V (nb x R, row-major)],
+                     P_e = sigma_e (diag(d) + U V^T);
+  kind 1 (Kronecker): o = [S_0..S_{R-1} (nv x nv) | Q_0..Q_{R-1} (Np x Np)],
+                     P_e = sigma_e sum_r S_r (x) Q_r  (rows variable-major, like the state);
+  L = (1/B) sum_e ||P_e A_e - I||_F^2 / nb.
 With ``emulate_bf16`` every GEMM operand is rounded to bfloat16 (round to nearest even) as the MFMA
 kernels do, while accumulation stays float64; biases and tanh derivatives use the unrounded values, as
 on the GPU.
@@ -21,7 +24,8 @@ on the GPU.
 Pins (tests/test_mlp_oracle.py):
   * the gradient against central finite differences of the float64 loss;
   * L = 1 exactly when the network outputs zero (P = 0, P A - I = -I);
-  * L = 0 when A is the numpy inverse of sigma (diag(d) + U V^T) (fixes the U/V layout and sigma);
+  * L = 0 when A is the numpy inverse of sigma (diag(d) + U V^T), or of sigma sum_r np.kron(S_r, Q_r)
+    (fixes the output layouts and sigma);
   * bf16 rounding against torch's float32 -> bfloat16 conversion.
 """
 from __future__ import annotations
@@ -76,37 +80,66 @@ def split_output(o, nb, R):
     return d, U, V
 
 
-def precond_blocks(o, sigma, nb, R):
-    """P_e = sigma_e (diag(d_e) + U_e V_e^T), shape (B, nb, nb)."""
-    d, U, V = split_output(o, nb, R)
-    P = np.einsum("bir,bjr->bij", U, V)
-    P[:, np.arange(nb), np.arange(nb)] += d
+def n_out(nb, R, kind=0, Np=None):
+    if kind == 0:
+        return nb + 2 * nb * R
+    nv = nb // Np
+    return R * (nv * nv + Np * Np)
+
+
+def split_kron(o, nb, R, Np):
+    nv = nb // Np
+    S = o[:, :R * nv * nv].reshape(-1, R, nv, nv)
+    Q = o[:, R * nv * nv:].reshape(-1, R, Np, Np)
+    return S, Q
+
+
+def precond_blocks(o, sigma, nb, R, kind=0, Np=None):
+    """P_e of the chosen structure, shape (B, nb, nb)."""
+    if kind == 0:
+        d, U, V = split_output(o, nb, R)
+        P = np.einsum("bir,bjr->bij", U, V)
+        P[:, np.arange(nb), np.arange(nb)] += d
+    else:
+        S, Q = split_kron(o, nb, R, Np)
+        P = np.zeros((len(o), nb, nb))
+        for e in range(len(o)):
+            for r in range(R):
+                P[e] += np.kron(S[e, r], Q[e, r])
     return P * np.asarray(sigma, dtype=np.float64)[:, None, None]
 
 
-def block_loss(o, sigma, A, nb, R):
+def block_loss(o, sigma, A, nb, R, kind=0, Np=None):
     """Per-sample L_e = ||P_e A_e - I||_F^2 / nb and dL_e/do (B x nout)."""
-    d, U, V = split_output(o, nb, R)
     sg = np.asarray(sigma, dtype=np.float64)
-    P = precond_blocks(o, sg, nb, R)
+    P = precond_blocks(o, sg, nb, R, kind, Np)
     Rm = P @ A - np.eye(nb)[None]
     L = (Rm ** 2).sum(axis=(1, 2)) / nb
     G = 2.0 / nb * Rm @ np.transpose(A, (0, 2, 1)) * sg[:, None, None]    # sigma dL/dP
-    go = np.concatenate([np.diagonal(G, axis1=1, axis2=2),
-                         (G @ V).reshape(len(o), -1),
-                         (np.transpose(G, (0, 2, 1)) @ U).reshape(len(o), -1)], axis=1)
+    if kind == 0:
+        d, U, V = split_output(o, nb, R)
+        go = np.concatenate([np.diagonal(G, axis1=1, axis2=2),
+                             (G @ V).reshape(len(o), -1),
+                             (np.transpose(G, (0, 2, 1)) @ U).reshape(len(o), -1)], axis=1)
+    else:
+        nv = nb // Np
+        S, Q = split_kron(o, nb, R, Np)
+        G4 = G.reshape(len(o), nv, Np, nv, Np)                # [e, a, i, b, j]
+        dS = np.einsum("eaibj,erij->erab", G4, Q)
+        dQ = np.einsum("eaibj,erab->erij", G4, S)
+        go = np.concatenate([dS.reshape(len(o), -1), dQ.reshape(len(o), -1)], axis=1)
     return L, go
 
 
-def loss_and_grad(params, X, sigma, A, nb, R, nhid, emulate_bf16=False):
+def loss_and_grad(params, X, sigma, A, nb, R, nhid, emulate_bf16=False, kind=0, Np=None):
     """Mean loss over the batch and its gradient with respect to the flat parameter vector."""
     B, nf = X.shape
-    nout = nb + 2 * nb * R
+    nout = n_out(nb, R, kind, Np)
     e = emulate_bf16
     Wb = unpack(_r(params, e), nf, nhid, nout)
     W2, W3 = Wb[2], Wb[4]
     h1, h2, o = forward(params, X, nf, nhid, nout, e)
-    L, go = block_loss(o, sigma, A, nb, R)
+    L, go = block_loss(o, sigma, A, nb, R, kind, Np)
     go = go / B
     gW3 = _r(go, e).T @ _r(h2, e)
     gb3 = go.sum(0)

@@ -11,8 +11,8 @@ import pytest
 import mlp_oracle as MO
 
 
-def _rand_problem(rng, B=5, nf=7, nhid=6, nb=5, R=2):
-    nout = nb + 2 * nb * R
+def _rand_problem(rng, B=5, nf=7, nhid=6, nb=5, R=2, kind=0, Np=None):
+    nout = MO.n_out(nb, R, kind, Np)
     _, _, n = MO.layout(nf, nhid, nout)
     params = rng.standard_normal(n) * 0.4
     X = rng.standard_normal((B, nf))
@@ -33,6 +33,37 @@ def test_gradient_matches_finite_differences():
         fd[i] = (MO.loss_and_grad(params + e, X, sigma, A, nb, R, nhid)[0]
                  - MO.loss_and_grad(params - e, X, sigma, A, nb, R, nhid)[0]) / (2 * h)
     assert np.abs(g - fd).max() <= 1e-7 * max(1.0, np.abs(fd).max())
+
+
+def test_kron_gradient_matches_finite_differences():
+    rng = np.random.default_rng(17)
+    params, X, sigma, A, nb, R, nhid = _rand_problem(rng, nb=6, R=2, kind=1, Np=3)
+    kw = dict(kind=1, Np=3)
+    _, g = MO.loss_and_grad(params, X, sigma, A, nb, R, nhid, **kw)
+    h = 1e-6
+    fd = np.zeros_like(g)
+    for i in range(params.size):
+        e = np.zeros_like(params)
+        e[i] = h
+        fd[i] = (MO.loss_and_grad(params + e, X, sigma, A, nb, R, nhid, **kw)[0]
+                 - MO.loss_and_grad(params - e, X, sigma, A, nb, R, nhid, **kw)[0]) / (2 * h)
+    assert np.abs(g - fd).max() <= 1e-7 * max(1.0, np.abs(fd).max())
+
+
+def test_kron_exact_inverse_gives_zero_loss():
+    rng = np.random.default_rng(18)
+    nv, Np, R = 2, 3, 2
+    nb = nv * Np
+    S = rng.standard_normal((R, nv, nv)) + 2 * np.eye(nv)
+    Q = rng.standard_normal((R, Np, Np)) * 0.2 + np.eye(Np)
+    o = np.concatenate([S.ravel(), Q.ravel()])[None]
+    P = 1.3 * sum(np.kron(S[r], Q[r]) for r in range(R))
+    A = np.linalg.inv(P)[None]
+    L, go = MO.block_loss(o, np.array([1.3]), A, nb, R, kind=1, Np=Np)
+    assert L[0] < 1e-22 and np.abs(go).max() < 1e-10
+    # the swapped factor order (Q (x) S) is a different matrix
+    Pt = 1.3 * sum(np.kron(Q[r], S[r]) for r in range(R))
+    assert ((Pt @ A[0] - np.eye(nb)) ** 2).sum() / nb > 1e-4
 
 
 def test_gradient_plants_fail():
