@@ -24,7 +24,7 @@ EXPORTS = [
     "dgx_sizes", "dgx_upload_mesh", "dgx_get_geometry", "dgx_compute_rhs", "dgx_compute_rhs_f32", "dgx_compute_rhs_mixed",
     "dgx_compute_rhs_explicit", "dgx_apply_minv", "dgx_enforce_positivity", "dgx_ssp_rk3_step", "dgx_max_abs",
     "dgx_convert_f64_f32", "dgx_convert_f32_f64", "dgx_assemble_diag_blocks", "dgx_jacobian_vector",
-    "dgx_block_lu_factor", "dgx_block_lu_solve", "dgx_cfl_dt", "dgx_mass_sub", "dgx_solve_implicit",
+    "dgx_assemble_face_blocks", "dgx_block_lu_factor", "dgx_block_lu_solve", "dgx_cfl_dt", "dgx_mass_sub", "dgx_solve_implicit",
     "dgx_mlp_create", "dgx_mlp_destroy", "dgx_mlp_sizes", "dgx_mlp_get_params", "dgx_mlp_set_params",
     "dgx_mlp_features", "dgx_mlp_loss_grad", "dgx_mlp_train", "dgx_mlp_predict", "dgx_attach_mlp",
 ]
@@ -98,6 +98,7 @@ def load():
         "dgx_mass_sub": [vp, vp, vp, vp],
         "dgx_solve_implicit": [vp, vp, C.POINTER(ImplicitParams), C.POINTER(ImplicitStats)],
         "dgx_jacobian_vector": [vp, vp, vp, vp],
+        "dgx_assemble_face_blocks": [vp, vp, vp],
         "dgx_mlp_create": [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong],
         "dgx_mlp_destroy": [vp],
         "dgx_mlp_sizes": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_longlong)],
@@ -225,6 +226,9 @@ class Context:
         """mass_scale: device pointer to E doubles (1/dt per element) or None."""
         check("dgx_assemble_diag_blocks",
               self.L.dgx_assemble_diag_blocks(self.h, vp(u), vp(mass_scale or 0), vp(blocks)))
+
+    def assemble_face_blocks(self, u: int, blocks: int):
+        check("dgx_assemble_face_blocks", self.L.dgx_assemble_face_blocks(self.h, vp(u), vp(blocks)))
 
     def cfl_dt(self, u: int, cfl: float, inv_dt: int):
         check("dgx_cfl_dt", self.L.dgx_cfl_dt(self.h, vp(u), cfl, vp(inv_dt)))

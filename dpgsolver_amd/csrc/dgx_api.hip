@@ -324,6 +324,17 @@ dgx_status dgx_assemble_diag_blocks(dgx_ctx* h, const double* u, const double* m
 	return DGX_OK;
 }
 
+dgx_status dgx_assemble_face_blocks(dgx_ctx* h, const double* u, double* blocks)
+{
+	if (!h || !u || !blocks) return DGX_ERR_ARG;
+	Ctx& c = h->c;
+	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
+	const int nb = c.Np * c.nv;
+	for (int col = 0; col < nb; ++col) DGX_TRY(dgx::launch_offd_col(c, u, blocks, col));
+	return DGX_OK;
+}
+
 dgx_status dgx_cfl_dt(dgx_ctx* h, const double* u, double cfl, double* inv_dt)
 {
 	if (!h || !u || !inv_dt || !(cfl > 0.0)) return DGX_ERR_ARG;

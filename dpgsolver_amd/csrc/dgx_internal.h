@@ -113,6 +113,7 @@ std::vector<double> min_edge_lengths(int d, int N, int E, const double* xyz);
 hipError_t launch_block_lu(hipStream_t s, double* blocks, int* piv, int E, int nb);
 hipError_t launch_block_lu_solve(hipStream_t s, const double* lu, const int* piv, double* x, int E, int nb);
 hipError_t launch_diag_col(Ctx& c, const double* u, double* blocks, int col);
+hipError_t launch_offd_col(Ctx& c, const double* u, double* blocks, int col);
 hipError_t launch_jvp(Ctx& c, const double* u, const double* v, double* jv);
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, int n, unsigned long long v);
 hipError_t launch_fill_i32(hipStream_t s, int* p, int n, int v);

@@ -1,5 +1,5 @@
 // k_dual.hip -- forward-mode (dual number) instantiations of the fused residual kernel:
-// exact Jacobian-vector products and columns of the element diagonal blocks
+// exact Jacobian-vector products, columns of the element diagonal blocks and of the face blocks
 // (the quantities constructor_lhs_v_1_T / constructor_lhs_f_1_T assemble,
 //  compute_volume_rlhs_T.c:179-220, compute_face_rlhs_T.c:163-220).
 #include "k_residual.h"
@@ -49,6 +49,11 @@ hipError_t launch_jvp(Ctx& c, const double* u, const double* v, double* jv)
 hipError_t launch_diag_col(Ctx& c, const double* u, double* blocks, int col)
 {
 	return dispatch_dual<MODE_DIAG>(c, make_dual(c, u, nullptr, blocks, col));
+}
+
+hipError_t launch_offd_col(Ctx& c, const double* u, double* blocks, int col)
+{
+	return dispatch_dual<MODE_OFFD>(c, make_dual(c, u, nullptr, blocks, col));
 }
 
 }  // namespace dgx

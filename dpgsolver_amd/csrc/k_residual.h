@@ -36,7 +36,11 @@ __device__ inline Dual operator*(double a, Dual b) { return Dual(a * b.v, a * b.
 // The constant part F(c_e) drops out exactly: its volume and face contributions cancel per test
 // function (free-stream preservation), so the result is the full residual, and the FP32 rounding
 // acts only on the small within-element variation instead of on the whole flux.
-enum { MODE_JVP = 5, MODE_DIAG = 6, MODE_PERT = 7 };
+// MODE_OFFD: column `col` of the face (off-diagonal) blocks dR_e/du_n for every neighbour n: the seed
+// sits on the neighbour's local dof `col`, the element's own state carries none, so only the face flux
+// of the face shared with n has a derivative; the faces are lifted one at a time into
+// out[(e * 2d + lf) * nb * nb + row * nb + col].
+enum { MODE_JVP = 5, MODE_DIAG = 6, MODE_PERT = 7, MODE_OFFD = 8 };
 
 template <typename S, typename R>
 struct RhsArgs {
@@ -307,6 +311,9 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 				for (int v = 0; v < NV; ++v) {
 					to[v * NF] = U[v * Np + vown];
 					if constexpr (PERT) tn[v * NF] = S(a.u64[gi + v * Np] - sC[sl][v]);
+					else if constexpr (MODE == MODE_OFFD)
+						tn[v * NF] = S(a.u[gi + v * Np],
+						               (face_vol_node<D, N>(ef.packed & 15, i) + v * Np == a.col) ? 1.0 : 0.0);
 					else if constexpr (MODE == MODE_JVP) tn[v * NF] = S(a.u[gi + v * Np], a.v[gi + v * Np]);
 					else tn[v * NF] = S(a.u[gi + v * Np]);
 				}
@@ -418,7 +425,7 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 
 	// ---- physical flux at the volume cubature, contravariant and weighted, kept in registers
 	S fr[D][NV];
-	if (valid) {
+	if (valid && MODE != MODE_OFFD) {
 		const int q = t;
 		S s[NV];
 #pragma unroll
@@ -464,8 +471,9 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 	}
 
 	// ---- weak divergence in GL space: U = sum_d Dhat_d^T fr_d, one direction at a time through BUF
+	//      (MODE_OFFD: the volume term has no derivative and is skipped)
 #pragma unroll
-	for (int d0 = 0; d0 < D; ++d0) {
+	for (int d0 = 0; d0 < (MODE == MODE_OFFD ? 0 : D); ++d0) {
 		if (valid) {
 #pragma unroll
 			for (int v = 0; v < NV; ++v) BUF[v * Np + t] = fr[d0][v];
@@ -476,18 +484,18 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 	}
 
 	// ---- face lifts (and the mass inverse) at the GL nodes
+	// per-face coefficient Lift_side[c_fd] * W_f and face-node offset, shared by all variables
+	R cf[NFACE];
+	int of[NFACE];
+	R scale = R(1.0);
 	if (valid) {
 		const int q = t;
 		const int cc[3] = { q % N, (q / N) % N, (D == 3) ? q / (N * N) : 0 };
-		R scale = R(1.0);
 		if constexpr (MINV) {
 			R wq = sW[cc[0]] * sW[cc[1]];
 			if (D == 3) wq = wq * sW[cc[2]];
 			scale = trcp(wq * a.detJ[(size_t)e * Np + q]);
 		}
-		// per-face coefficient Lift_side[c_fd] * W_f and face-node offset, shared by all variables
-		R cf[NFACE];
-		int of[NFACE];
 #pragma unroll
 		for (int lf = 0; lf < NFACE; ++lf) {
 			const int fd = lf >> 1;
@@ -500,6 +508,27 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 			cf[lf] = sLift[lf & 1][cc[fd]] * wf;
 			of[lf] = lf * NV * NF + ((D == 3) ? fa + N * fb : fa);
 		}
+	}
+	if constexpr (MODE == MODE_OFFD) {
+		constexpr int NB = NV * Np;
+		for (int lf = 0; lf < NFACE; ++lf) {
+			if (valid) {
+#pragma unroll
+				for (int v = 0; v < NV; ++v) U[v * Np + t] = S(0.0) - cf[lf] * TR[of[lf] + v * NF];
+			}
+			__syncthreads();
+#pragma unroll
+			for (int dir = 0; dir < D; ++dir) sweep_inplace<S, R, D, N, NV, NT>(U, sIT, dir, t);
+			if (valid) {
+				double* ob = a.out + ((size_t)e * NFACE + lf) * NB * NB;
+				for (int i = t; i < NB; i += NT) ob[(size_t)i * NB + a.col] = U[i].d;
+			}
+			__syncthreads();
+		}
+		return;
+	}
+	if (valid) {
+		const int q = t;
 #pragma unroll
 		for (int v = 0; v < NV; ++v) {
 			S acc = U[v * Np + q];
@@ -520,7 +549,9 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 			if constexpr (MODE == MODE_JVP) a.out[ebase + i] = U[i].d;
 			else if constexpr (MODE == MODE_DIAG)
 				a.out[(size_t)e * (NV * Np) * (NV * Np) + (size_t)i * (NV * Np) + a.col] = U[i].d;
-			else {
+			else if constexpr (MODE == MODE_OFFD) {
+				// stored per face above
+			} else {
 				a.out[ebase + i] = U[i];
 				lmax = fmax(lmax, fabs((double)rv(U[i])));
 			}

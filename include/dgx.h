@@ -116,6 +116,11 @@ dgx_status dgx_convert_f32_f64(dgx_ctx* ctx, const float* src, double* dst, long
  * (constructor_lhs_v_1_T compute_volume_rlhs_T.c:179-220, constructor_lhs_f_1_T
  *  compute_face_rlhs_T.c:163-220, compute_CFL_ramping solve_dg.c:223-248 with mass_scale = 1/dt). */
 dgx_status dgx_assemble_diag_blocks(dgx_ctx* ctx, const double* u, const double* mass_scale, double* blocks);
+/* Face (off-diagonal) blocks: blocks[(e * 2d + lf) * nb^2 + row * nb + col] = dR_e[row] / du_n[col], n the
+ * neighbour across local face lf (col in n's local numbering var*Np+node); zero for boundary faces.
+ * These are lhs_lr / lhs_rl of constructor_lhs_f_1_T (compute_face_rlhs_T.c:163-220, added at
+ * compute_face_rlhs_dg.c:198-213); with the diagonal blocks they are the whole block-sparse Jacobian. */
+dgx_status dgx_assemble_face_blocks(dgx_ctx* ctx, const double* u, double* blocks);
 /* CFL-ramped pseudo-time step per element, returned as inv_dt[e] = max wave speed / (cfl * h_min)
  * (compute_dt_cfl_constrained / compute_min_length_measure, solve_dg.c:284-379). */
 dgx_status dgx_cfl_dt(dgx_ctx* ctx, const double* u, double cfl, double* inv_dt);

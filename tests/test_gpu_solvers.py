@@ -148,6 +148,54 @@ def test_jacobian_vector_matches_complex_step(dev, case):
     ctx.close()
 
 
+def _neighbours(mesh):
+    """(e, lf) -> neighbour element, from the face table rows (eL, fL, eR, fR, ord, bc)."""
+    nb = {}
+    for eL, fL, eR, fR, _, _ in mesh.faces:
+        if eR >= 0:
+            nb[(int(eL), int(fL))] = int(eR)
+            nb[(int(eR), int(fR))] = int(eL)
+    return nb
+
+
+@pytest.mark.parametrize("case", JAC, ids=[c[0] for c in JAC])
+def test_face_blocks_match_complex_step(dev, case):
+    """dR_e/du_n for the face neighbours n against the oracle's complex-step block (the reference's
+    lhs_lr / lhs_rl), and the whole block-sparse Jacobian applied to v against the exact J v."""
+    name, mesh, flux, bcp, u = case
+    ctx = _ctx(mesh, M.PDE_EULER, flux, bc_prim=bcp)
+    nb, nf = ctx.Np * ctx.nv, 2 * mesh.d
+    ud = torch.from_numpy(u).to(dev)
+    fb = torch.zeros(mesh.E * nf * nb * nb, dtype=torch.float64, device=dev)
+    db = torch.zeros(mesh.E * nb * nb, dtype=torch.float64, device=dev)
+    ctx.assemble_face_blocks(ud.data_ptr(), fb.data_ptr())
+    ctx.assemble_diag_blocks(ud.data_ptr(), None, db.data_ptr())
+    v = np.random.default_rng(5).standard_normal(u.size)
+    vd = torch.from_numpy(v).to(dev)
+    jv = torch.empty_like(ud)
+    ctx.jacobian_vector(ud.data_ptr(), vd.data_ptr(), jv.data_ptr())
+    torch.cuda.synchronize()
+    F = fb.cpu().numpy().reshape(mesh.E, nf, nb, nb)
+    Dg = db.cpu().numpy().reshape(mesh.E, nb, nb)
+    nbr = _neighbours(mesh)
+    orc = O.Oracle(mesh, M.PDE_EULER, flux, bc_prim=bcp)
+    for e in range(min(mesh.E, 3)):
+        for n in sorted({nbr[(e, lf)] for lf in range(nf) if (e, lf) in nbr} - {e}):
+            got = sum(F[e, lf] for lf in range(nf) if nbr.get((e, lf)) == n)
+            ref = orc.jacobian_block(u, e, n)
+            assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), (e, n)
+    for e in range(mesh.E):                       # boundary faces carry no block
+        for lf in range(nf):
+            if (e, lf) not in nbr:
+                assert not F[e, lf].any()
+    V = v.reshape(mesh.E, nb)
+    y = np.einsum("eij,ej->ei", Dg, V)
+    for (e, lf), n in nbr.items():
+        y[e] += F[e, lf] @ V[n]
+    assert _rel(y.ravel(), jv.cpu().numpy()) < 1e-12
+    ctx.close()
+
+
 def test_block_lu_solve_matches_numpy(dev):
     from dpgsolver_amd import _lib
 
