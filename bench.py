@@ -90,6 +90,72 @@ def cpu_baseline(target_seconds=12.0):
                       f"{n} evaluations in {el:.1f} s on 1 host core"}
 
 
+def time_to_steady(torch, dev, epochs=60):
+    """Time to max|R| < 1e-10 (FP64) on the steady channel workload: 2-D Euler, DG p=3, 64x64 QUAD channel
+    [-1.5,1.5]x[0,0.8] with the bump y = 0.0625 exp(-25 x^2), inflow Mach 0.5, Roe flux, pseudo-time
+    Newton-GMRES(60) (rtol 1e-4) from the free stream with cfl_initial 1 and the ramped CFL capped at 300.
+    Legs: exact block-Jacobi (FP64 inverse of every diagonal block); the learned preconditioner (Kronecker
+    rank 8, trained here on the diagonal blocks captured from the block-Jacobi run) with the exact
+    block-Jacobi as fallback when GMRES has not converged after 50 iterations."""
+    from dpgsolver_amd import _lib, mesh as M
+
+    mesh = M.channel_mesh(64, 64, 3)
+    prim = M.freestream_prim(2)
+    ctx = _lib.Context(2, 3, M.PDE_EULER, M.FLUX_ROE, bc_prim=prim)
+    ctx.set_stream(torch.cuda.current_stream(dev).cuda_stream)
+    ctx.upload_mesh(mesh.xyz, mesh.faces)
+    u0 = M.uniform_state(mesh, prim)
+    kw = dict(cfl_initial=1.0, cfl_max=300.0, exit_tol=1e-10, max_newton=100, gmres_restart=60, gmres_rtol=1e-4,
+              gmres_max_iter=1000, raise_on_error=False)
+    nb, E = 64, mesh.E
+    mlp = _lib.MLP(nb, 16, hidden=256, rank=8, kind=_lib.MLP_KRON)
+    cap = E * 40
+    blocks = torch.empty(cap * nb * nb, dtype=torch.float64, device=dev)
+    feat = torch.empty(cap * mlp.nf, dtype=torch.float32, device=dev)
+    sig = torch.empty(cap, dtype=torch.float32, device=dev)
+
+    def solve(pc, **extra):
+        u = torch.from_numpy(u0).to(dev)
+        st = ctx.solve_implicit(u.data_ptr(), precond=pc, **kw, **extra)
+        return st, u
+
+    def leg(st):
+        return {"seconds": round(st["seconds"], 4), "newton_steps": st["newton_steps"],
+                "gmres_iters": st["gmres_iters"], "converged": bool(st["converged"]), "max_rhs": st["max_rhs"],
+                "fallback_steps": st["fallback_steps"], "gmres_per_step": st["gmres_per_step"]}
+
+    solve(_lib.PC_BLOCK_JACOBI, max_newton=2)      # warm-up (module load, first launches)
+    st_cap, _ = solve(_lib.PC_BLOCK_JACOBI, capture=(blocks.data_ptr(), feat.data_ptr(), sig.data_ptr(), cap))
+    n = st_cap["captured"]
+    st_bj, u_bj = solve(_lib.PC_BLOCK_JACOBI)
+    print(f"block-Jacobi: {st_bj['seconds']:.3f} s, newton {st_bj['newton_steps']}, gmres {st_bj['gmres_iters']}",
+          file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    l0, _ = mlp.loss_grad(feat.data_ptr(), sig.data_ptr(), blocks.data_ptr(), min(n, 4096), want_grad=False)
+    loss = l0
+    for ep in range(0, epochs, 5):
+        lr = 1e-3 if ep < epochs // 3 else (3e-4 if ep < 2 * epochs // 3 else 1e-4)
+        loss = mlp.train(feat.data_ptr(), sig.data_ptr(), blocks.data_ptr(), n, epochs=5, batch=256, lr=lr)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    print(f"training: {n} blocks, {epochs} epochs, loss {l0:.3f} -> {loss:.4f}, {t_train:.2f} s", file=sys.stderr,
+          flush=True)
+    ctx.attach_mlp(mlp)
+    st_ln, u_ln = solve(_lib.PC_LEARNED, learned_fallback_iters=50)
+    print(f"learned: {st_ln['seconds']:.3f} s, newton {st_ln['newton_steps']}, gmres {st_ln['gmres_iters']}, "
+          f"fallbacks {st_ln['fallback_steps']}", file=sys.stderr, flush=True)
+    diff = ((u_ln - u_bj).abs().max() / u_bj.abs().max()).item()
+    out = {"workload": "2-D Euler DG p=3, 64x64 curved-bump channel, Mach 0.5, Roe; Newton-GMRES(60) to 1e-10",
+           "block_jacobi": leg(st_bj), "learned_with_fallback": leg(st_ln),
+           "training": {"blocks": n, "epochs": epochs, "seconds": round(t_train, 3), "loss_untrained": l0,
+                        "loss_trained": loss, "structure": "Kronecker rank 8, MLP 96-256-256-2176 BF16/FP32"},
+           "learned_vs_bj_wall": round(st_ln["seconds"] / st_bj["seconds"], 4),
+           "steady_states_rel_diff": diff}
+    mlp.close()
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +163,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=64, help="elements per direction (64 = the named workload)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-implicit", action="store_true", help="skip the time-to-1e-10 steady-solve legs")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 1 GiB read + 1 GiB write elementwise kernel (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -192,6 +259,7 @@ def main():
     gdofmx = ndof / (msmx * 1e-3) / 1e9
 
     cpu = None if args.no_cpu else cpu_baseline()
+    implicit = None if args.no_implicit else time_to_steady(torch, dev)
 
     out = {
         "metric": "GDOF/s residual eval (FP64 vs FP32-MFMA); time-to-1e-10 steady residual",
@@ -222,6 +290,7 @@ def main():
         "fp32_input_rounding_rel_err": err_in,
         "fp64_free_stream_max_abs": free_stream,
         "cpu_baseline": cpu,
+        "time_to_1e-10": implicit,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

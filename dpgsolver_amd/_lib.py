@@ -24,7 +24,7 @@ EXPORTS = [
     "dgx_sizes", "dgx_upload_mesh", "dgx_get_geometry", "dgx_compute_rhs", "dgx_compute_rhs_f32", "dgx_compute_rhs_mixed",
     "dgx_compute_rhs_explicit", "dgx_apply_minv", "dgx_enforce_positivity", "dgx_ssp_rk3_step", "dgx_max_abs",
     "dgx_convert_f64_f32", "dgx_convert_f32_f64", "dgx_assemble_diag_blocks", "dgx_jacobian_vector",
-    "dgx_assemble_face_blocks", "dgx_block_lu_factor", "dgx_block_lu_solve", "dgx_cfl_dt", "dgx_mass_sub", "dgx_solve_implicit",
+    "dgx_assemble_face_blocks", "dgx_block_lu_factor", "dgx_block_lu_solve", "dgx_block_inverse", "dgx_block_inv_apply", "dgx_cfl_dt", "dgx_mass_sub", "dgx_solve_implicit",
     "dgx_mlp_create", "dgx_mlp_destroy", "dgx_mlp_sizes", "dgx_mlp_get_params", "dgx_mlp_set_params",
     "dgx_mlp_features", "dgx_mlp_loss_grad", "dgx_mlp_train", "dgx_mlp_predict", "dgx_attach_mlp",
 ]
@@ -51,14 +51,16 @@ class DgxMesh(C.Structure):
 class ImplicitParams(C.Structure):
     _fields_ = [("cfl_initial", C.c_double), ("exit_tol", C.c_double), ("exit_ratio", C.c_double),
                 ("max_newton", C.c_int), ("gmres_restart", C.c_int), ("gmres_rtol", C.c_double),
-                ("gmres_max_iter", C.c_int), ("precond", C.c_int), ("capture_blocks", C.c_void_p),
+                ("gmres_max_iter", C.c_int), ("precond", C.c_int), ("cfl_max", C.c_double),
+                ("learned_fallback_iters", C.c_int), ("capture_blocks", C.c_void_p),
                 ("capture_features", C.c_void_p), ("capture_sigma", C.c_void_p), ("capture_capacity", C.c_longlong)]
 
 
 class ImplicitStats(C.Structure):
     _fields_ = [("newton_steps", C.c_int), ("gmres_iters", C.c_int), ("gmres_not_converged", C.c_int),
                 ("converged", C.c_int), ("max_rhs0", C.c_double), ("max_rhs", C.c_double),
-                ("seconds", C.c_double), ("captured", C.c_longlong), ("history", C.c_double * HISTORY_MAX)]
+                ("seconds", C.c_double), ("captured", C.c_longlong), ("fallback_steps", C.c_int),
+                ("gmres_per_step", C.c_int * HISTORY_MAX), ("history", C.c_double * HISTORY_MAX)]
 
 
 _lib = None
@@ -111,6 +113,8 @@ def load():
         "dgx_attach_mlp": [vp, vp],
         "dgx_block_lu_factor": [vp, vp, vp, C.c_int, C.c_int],
         "dgx_block_lu_solve": [vp, vp, vp, vp, C.c_int, C.c_int],
+        "dgx_block_inverse": [vp, vp, C.c_int, C.c_int],
+        "dgx_block_inv_apply": [vp, vp, vp, vp, C.c_int, C.c_int],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -238,18 +242,19 @@ class Context:
 
     def solve_implicit(self, u: int, cfl_initial=10.0, exit_tol=1e-10, exit_ratio=0.0, max_newton=100,
                        gmres_restart=60, gmres_rtol=1e-4, gmres_max_iter=600, precond=PC_BLOCK_JACOBI,
-                       raise_on_error=True, capture=None):
+                       raise_on_error=True, capture=None, cfl_max=0.0, learned_fallback_iters=0):
         """capture: optional (blocks_ptr, features_ptr, sigma_ptr, capacity_elements) device buffers."""
         cb, cf, cs, cc = capture if capture else (0, 0, 0, 0)
         p = ImplicitParams(cfl_initial, exit_tol, exit_ratio, max_newton, gmres_restart, gmres_rtol, gmres_max_iter,
-                           precond, cb or None, cf or None, cs or None, cc)
+                           precond, cfl_max, learned_fallback_iters, cb or None, cf or None, cs or None, cc)
         st = ImplicitStats()
         rc = self.L.dgx_solve_implicit(self.h, vp(u), C.byref(p), C.byref(st))
         if raise_on_error and rc != DGX_OK:
             raise DgxError("dgx_solve_implicit", rc)
         out = {k: getattr(st, k) for k in ("newton_steps", "gmres_iters", "gmres_not_converged", "converged",
-                                           "max_rhs0", "max_rhs", "seconds", "captured")}
+                                           "max_rhs0", "max_rhs", "seconds", "captured", "fallback_steps")}
         out["history"] = list(st.history[: min(st.newton_steps + 1, HISTORY_MAX)])
+        out["gmres_per_step"] = list(st.gmres_per_step[: min(st.newton_steps, HISTORY_MAX)])
         out["status"] = rc
         return out
 
@@ -258,6 +263,12 @@ class Context:
 
     def block_lu_factor(self, blocks: int, piv: int, E: int, nb: int):
         check("dgx_block_lu_factor", self.L.dgx_block_lu_factor(self.h, vp(blocks), vp(piv), E, nb))
+
+    def block_inverse(self, blocks: int, E: int, nb: int):
+        check("dgx_block_inverse", self.L.dgx_block_inverse(self.h, vp(blocks), E, nb))
+
+    def block_inv_apply(self, inv: int, x: int, y: int, E: int, nb: int):
+        check("dgx_block_inv_apply", self.L.dgx_block_inv_apply(self.h, vp(inv), vp(x), vp(y), E, nb))
 
     def block_lu_solve(self, lu: int, piv: int, x: int, E: int, nb: int):
         check("dgx_block_lu_solve", self.L.dgx_block_lu_solve(self.h, vp(lu), vp(piv), vp(x), E, nb))
@@ -274,7 +285,7 @@ class MLP:
         self.L = load()
         h = vp()
         if rank is None:
-            rank = 8 if kind == MLP_LOWRANK else 4
+            rank = 8
         check("dgx_mlp_create", self.L.dgx_mlp_create(C.byref(h), kind, nb, Np, hidden, rank, seed))
         self.kind = kind
         self.h = h

@@ -362,6 +362,18 @@ dgx_status dgx_jacobian_vector(dgx_ctx* h, const double* u, const double* v, dou
 	return hip_status(dgx::launch_jvp(c, u, v, jv));
 }
 
+dgx_status dgx_block_inverse(dgx_ctx* h, double* blocks, int E, int nb)
+{
+	if (!h || !blocks || E < 0 || nb <= 0 || nb > dgx::kMaxInvBlock) return DGX_ERR_ARG;
+	return hip_status(dgx::launch_block_inverse(h->c.stream, blocks, E, nb, nullptr));
+}
+
+dgx_status dgx_block_inv_apply(dgx_ctx* h, const double* inv, const double* x, double* y, int E, int nb)
+{
+	if (!h || !inv || !x || !y || E < 0 || nb <= 0 || nb > dgx::kMaxInvBlock) return DGX_ERR_ARG;
+	return hip_status(dgx::launch_block_inv_apply(h->c.stream, inv, x, y, E, nb));
+}
+
 dgx_status dgx_block_lu_factor(dgx_ctx* h, double* blocks, int* piv, int E, int nb)
 {
 	if (!h || !blocks || !piv || E < 0 || nb <= 0 || nb > 90) return DGX_ERR_ARG;  // nb^2 doubles in LDS

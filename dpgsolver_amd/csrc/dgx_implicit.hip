@@ -207,6 +207,14 @@ __global__ void k_gemv_sub(const double* __restrict__ V, size_t ldv, const doubl
 		w[i] -= s;
 	}
 }
+// y = x / sqrt(nrm2[0])  (normalisation by a device-resident squared norm)
+__global__ void k_scale_by_rnorm(const double* __restrict__ x, const double* __restrict__ nrm2, double* y, size_t n)
+{
+	const double nv = nrm2[0];
+	const double a = nv > 0.0 ? 1.0 / sqrt(nv) : 0.0;
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+		y[i] = a * x[i];
+}
 // y = a x + b y
 __global__ void k_axpby(double a, const double* __restrict__ x, double b, double* y, size_t n)
 {
@@ -247,14 +255,30 @@ struct Blas {
 		if (dvec) (void)hipFree(dvec);
 		partial = dvec = nullptr;
 	}
+	// device out[0..k) = V^T w (deterministic two-stage reduction, no host round trip)
+	hipError_t dots_dev(const double* V, size_t ldv, int k, const double* w, size_t n, double* out)
+	{
+		hipLaunchKernelGGL(k_dots_partial, dim3(kChunks, k), dim3(256), 0, s, V, ldv, w, n, partial);
+		hipLaunchKernelGGL(k_dots_final, dim3((k + 63) / 64), dim3(64), 0, s, partial, k, out);
+		return hipGetLastError();
+	}
 	// host h[0..k) = V^T w
 	hipError_t dots(const double* V, size_t ldv, int k, const double* w, size_t n, double* hout)
 	{
-		hipLaunchKernelGGL(k_dots_partial, dim3(kChunks, k), dim3(256), 0, s, V, ldv, w, n, partial);
-		hipLaunchKernelGGL(k_dots_final, dim3((k + 63) / 64), dim3(64), 0, s, partial, k, dvec);
-		DGX_HIP_CHECK(hipGetLastError());
+		DGX_HIP_CHECK(dots_dev(V, ldv, k, w, n, dvec));
 		DGX_HIP_CHECK(hipMemcpyAsync(hout, dvec, sizeof(double) * k, hipMemcpyDeviceToHost, s));
 		return hipStreamSynchronize(s);
+	}
+	// w -= V h with h already on the device
+	hipError_t gemv_sub_dev(const double* V, size_t ldv, const double* hdev, int k, double* w, size_t n)
+	{
+		hipLaunchKernelGGL(k_gemv_sub, dim3(grid_for(n)), dim3(256), 0, s, V, ldv, hdev, k, w, n);
+		return hipGetLastError();
+	}
+	hipError_t scale_by_rnorm(const double* x, const double* nrm2_dev, double* y, size_t n)
+	{
+		hipLaunchKernelGGL(k_scale_by_rnorm, dim3(grid_for(n)), dim3(256), 0, s, x, nrm2_dev, y, n);
+		return hipGetLastError();
 	}
 	hipError_t gemv_sub(const double* V, size_t ldv, const double* hh, int k, double* w, size_t n)
 	{
@@ -334,7 +358,7 @@ struct PrecondIdentity : Precond {
 	}
 };
 
-// exact LU of the element diagonal blocks of (dR/du - M/dt): the block-Jacobi baseline
+// exact inverse (or LU) of the element diagonal blocks of (dR/du - M/dt): the block-Jacobi baseline
 hipError_t launch_features(Ctx& c, const double* u, const double* inv_dt, float* feat, int nf, float* sigma);
 
 struct PrecondBlockJacobi : Precond {
@@ -367,11 +391,14 @@ struct PrecondBlockJacobi : Precond {
 			                              cap_sigma + (size_t)cap_count));
 			cap_count += c.E;
 		}
+		// blocks up to 64 x 64: explicit inverse (one coalesced mat-vec per application); larger: LU
+		if (nb <= kMaxInvBlock) return launch_block_inverse(c.stream, blocks, c.E, nb, nullptr);
 		return launch_block_lu(c.stream, blocks, piv, c.E, nb);
 	}
 	hipError_t apply(Ctx& c, const double* v, double* z) override
 	{
 		const int nb = c.Np * c.nv;
+		if (nb <= kMaxInvBlock) return launch_block_inv_apply(c.stream, blocks, v, z, c.E, nb);
 		DGX_HIP_CHECK(hipMemcpyAsync(z, v, sizeof(double) * (size_t)c.E * nb, hipMemcpyDeviceToDevice, c.stream));
 		return launch_block_lu_solve(c.stream, blocks, piv, z, c.E, nb);
 	}
@@ -387,7 +414,8 @@ struct Gmres {
 	double* x = nullptr;
 	size_t n = 0;
 	Blas blas;
-	std::vector<double> H, g, cs, sn, y, h1, h2;
+	std::vector<double> H, g, cs, sn, y, h1, h2, hcol;
+	double* dcol = nullptr;   // device: [h1 (m+1) | h2 (m+1) | |w|^2]
 	hipError_t init(Ctx& c, int restart)
 	{
 		m = restart;
@@ -403,13 +431,15 @@ struct Gmres {
 		y.assign(m, 0.0);
 		h1.assign(m + 1, 0.0);
 		h2.assign(m + 1, 0.0);
+		hcol.assign(2 * (m + 1) + 1, 0.0);
+		DGX_HIP_CHECK(hipMalloc(&dcol, sizeof(double) * (2 * (m + 1) + 1)));
 		return blas.init(c.stream, m + 1);
 	}
 	void release()
 	{
-		for (double* p : { V, w, z, x })
+		for (double* p : { V, w, z, x, dcol })
 			if (p) (void)hipFree(p);
-		V = w = z = x = nullptr;
+		V = w = z = x = dcol = nullptr;
 		blas.release();
 	}
 	// w = A v with A = dR/du(u) - M/dt
@@ -448,16 +478,23 @@ struct Gmres {
 			for (; j < m && total < max_it; ++j, ++total) {
 				DGX_HIP_CHECK(P.apply(c, V + (size_t)j * n, z));
 				DGX_HIP_CHECK(op(c, u, inv_dt, z, w));
-				// classical Gram-Schmidt, twice (CGS2)
-				DGX_HIP_CHECK(blas.dots(V, n, j + 1, w, n, h1.data()));
-				DGX_HIP_CHECK(blas.gemv_sub(V, n, h1.data(), j + 1, w, n));
-				DGX_HIP_CHECK(blas.dots(V, n, j + 1, w, n, h2.data()));
-				DGX_HIP_CHECK(blas.gemv_sub(V, n, h2.data(), j + 1, w, n));
-				double hn;
-				DGX_HIP_CHECK(blas.norm(w, n, &hn));
-				for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = h1[i] + h2[i];
+				// classical Gram-Schmidt, twice (CGS2), coefficients kept on the device; one host round
+				// trip per iteration brings the Hessenberg column back for the Givens rotations
+				double* dh1 = dcol;
+				double* dh2 = dcol + (m + 1);
+				double* dnrm = dcol + 2 * (m + 1);
+				DGX_HIP_CHECK(blas.dots_dev(V, n, j + 1, w, n, dh1));
+				DGX_HIP_CHECK(blas.gemv_sub_dev(V, n, dh1, j + 1, w, n));
+				DGX_HIP_CHECK(blas.dots_dev(V, n, j + 1, w, n, dh2));
+				DGX_HIP_CHECK(blas.gemv_sub_dev(V, n, dh2, j + 1, w, n));
+				DGX_HIP_CHECK(blas.dots_dev(w, 0, 1, w, n, dnrm));
+				DGX_HIP_CHECK(blas.scale_by_rnorm(w, dnrm, V + (size_t)(j + 1) * n, n));
+				DGX_HIP_CHECK(hipMemcpyAsync(hcol.data(), dcol, sizeof(double) * (2 * (m + 1) + 1),
+				                             hipMemcpyDeviceToHost, c.stream));
+				DGX_HIP_CHECK(hipStreamSynchronize(c.stream));
+				const double hn = std::sqrt(hcol[2 * (m + 1)]);
+				for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = hcol[i] + hcol[(m + 1) + i];
 				H[(size_t)(j + 1) * m + j] = hn;
-				if (hn > 0.0) DGX_HIP_CHECK(blas.axpby(1.0 / hn, w, 0.0, V + (size_t)(j + 1) * n, n));
 				for (int i = 0; i < j; ++i) {
 					const double a = H[(size_t)i * m + j], bb = H[(size_t)(i + 1) * m + j];
 					H[(size_t)i * m + j] = cs[i] * a + sn[i] * bb;
@@ -509,6 +546,7 @@ extern "C" dgx_status dgx_solve_implicit(dgx_ctx* h, double* u, const dgx_implic
 	int* status = nullptr;
 	dgx::Gmres gm;
 	dgx::Precond* P = nullptr;
+	dgx::PrecondBlockJacobi* Pfb = nullptr;   // exact fallback of the learned preconditioner
 	dgx_status ret = DGX_OK;
 	auto fail = [&](dgx_status s) { ret = s; };
 	do {
@@ -525,7 +563,10 @@ extern "C" dgx_status dgx_solve_implicit(dgx_ctx* h, double* u, const dgx_implic
 			bj->cap_capacity = prm->capture_capacity;
 			P = bj;
 		}
-		else if (prm->precond == DGX_PC_LEARNED) P = dgx::make_learned_precond(c);
+		else if (prm->precond == DGX_PC_LEARNED) {
+			P = dgx::make_learned_precond(c);
+			if (prm->learned_fallback_iters > 0) Pfb = new dgx::PrecondBlockJacobi();
+		}
 		else P = new dgx::PrecondIdentity();
 		if (!P) { fail(DGX_ERR_UNSUPPORTED); break; }
 		double max_rhs0 = 0.0;
@@ -547,21 +588,39 @@ extern "C" dgx_status dgx_solve_implicit(dgx_ctx* h, double* u, const dgx_implic
 				st->converged = 1;
 				break;
 			}
-			const double cfl = prm->cfl_initial * (max_rhs0 / mx < 1.0 ? 1.0 : max_rhs0 / mx);
+			double cfl = prm->cfl_initial * (max_rhs0 / mx < 1.0 ? 1.0 : max_rhs0 / mx);
+			if (prm->cfl_max > 0.0 && cfl > prm->cfl_max) cfl = prm->cfl_max;
 			if (dgx::launch_cfl_dt(c, u, cfl, inv_dt) != hipSuccess) { fail(DGX_ERR_HIP); break; }
 			// b = -r
 			if (gm.blas.axpby(-1.0, r, 0.0, r, n) != hipSuccess) { fail(DGX_ERR_HIP); break; }
 			if (P->setup(c, u, inv_dt) != hipSuccess) { fail(DGX_ERR_HIP); break; }
 			int its = 0;
 			bool conv = false;
-			if (gm.solve(c, u, inv_dt, r, *P, prm->gmres_rtol, prm->gmres_max_iter, &its, &conv) != hipSuccess) {
+			const int first_max = Pfb ? std::min(prm->learned_fallback_iters, prm->gmres_max_iter) : prm->gmres_max_iter;
+			if (gm.solve(c, u, inv_dt, r, *P, prm->gmres_rtol, first_max, &its, &conv) != hipSuccess) {
 				fail(DGX_ERR_HIP);
 				break;
 			}
+			int step_its = its;
+			if (Pfb && !conv) {
+				// the learned preconditioner did not get there: exact block-Jacobi for this step
+				st->fallback_steps += 1;
+				if (Pfb->setup(c, u, inv_dt) != hipSuccess ||
+				    gm.solve(c, u, inv_dt, r, *Pfb, prm->gmres_rtol, prm->gmres_max_iter, &its, &conv) != hipSuccess) {
+					fail(DGX_ERR_HIP);
+					break;
+				}
+				step_its += its;
+			}
+			its = step_its;
+			if (it < DGX_HISTORY_MAX) st->gmres_per_step[it] = its;
 			st->gmres_iters += its;
 			if (!conv) st->gmres_not_converged += 1;  // KSP DIVERGED_ITS: warn and continue
+			// s += relax * x, then the positivity limiter on every element (update_coef_s_v,
+			// solve_implicit.c:723-732; enforce_positivity_highorder, solve.c:187-233)
 			if (dgx::launch_fill_i32(c.stream, status, 1, 0) != hipSuccess ||
-			    dgx::launch_relaxed_update(c, u, gm.x, status) != hipSuccess) { fail(DGX_ERR_HIP); break; }
+			    dgx::launch_relaxed_update(c, u, gm.x, status) != hipSuccess ||
+			    dgx::launch_positivity(c, u, status) != hipSuccess) { fail(DGX_ERR_HIP); break; }
 			int bad = 0;
 			if (hipMemcpy(&bad, status, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) { fail(DGX_ERR_HIP); break; }
 			if (bad) { fail(DGX_ERR_POSITIVITY); break; }
@@ -571,6 +630,7 @@ extern "C" dgx_status dgx_solve_implicit(dgx_ctx* h, double* u, const dgx_implic
 	(void)hipStreamSynchronize(c.stream);
 	if (prm->precond == DGX_PC_BLOCK_JACOBI && P) st->captured = static_cast<dgx::PrecondBlockJacobi*>(P)->cap_count;
 	delete P;
+	delete Pfb;
 	gm.release();
 	if (r) (void)hipFree(r);
 	if (inv_dt) (void)hipFree(inv_dt);

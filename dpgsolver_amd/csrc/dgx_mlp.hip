@@ -2,7 +2,7 @@
 // MFMA) maps per-element features to an approximate inverse P_e of the element diagonal block
 // A_e = dR_e/du_e - M_e/dt_e of the pseudo-time Newton system, in one of two structures:
 //   LOWRANK: P_e = sigma_e (diag(d_e) + U_e V_e^T), rank R (= 8);
-//   KRON:    P_e = sigma_e sum_{r<R} S_r (x) Q_r, S_r nv x nv (variables), Q_r Np x Np (nodes), R = 4,
+//   KRON:    P_e = sigma_e sum_{r<R} S_r (x) Q_r, S_r nv x nv (variables), Q_r Np x Np (nodes), R = 4 or 8,
 //            which follows the (variables) x (nodes) Kronecker structure of the DG block;
 // sigma_e = 1 / (mean(w detJ)/dt_e + 0.25 h_e lambda_e) is analytic, so the network outputs stay O(1)
 // along the CFL ramp.
@@ -434,12 +434,12 @@ static hipError_t mlp_loss(dgx_mlp* m, const float* sigma, const double* A, int 
 		hipLaunchKernelGGL((k_block_loss<64, 8>), dim3(B), dim3(256), 0, m->s, m->o, sigma, A, g, m->lpart, gscale);
 	else if (m->kind == DGX_MLP_LOWRANK && m->nb == 36 && m->rank == 8)
 		hipLaunchKernelGGL((k_block_loss<36, 8>), dim3(B), dim3(256), 0, m->s, m->o, sigma, A, g, m->lpart, gscale);
-	else if (m->kind == DGX_MLP_KRON && m->Np == 16 && m->rank == 4)
-		hipLaunchKernelGGL((k_block_loss_kron<4, 16, 4>), dim3(B), dim3(256), 0, m->s, m->o, sigma, A, g, m->lpart,
+#define DGX_KRON_LOSS(NPN, RK)                                                                                  \
+	else if (m->kind == DGX_MLP_KRON && m->Np == NPN && m->rank == RK)                                          \
+		hipLaunchKernelGGL((k_block_loss_kron<4, NPN, RK>), dim3(B), dim3(256), 0, m->s, m->o, sigma, A, g, m->lpart, \
 		                   gscale);
-	else if (m->kind == DGX_MLP_KRON && m->Np == 9 && m->rank == 4)
-		hipLaunchKernelGGL((k_block_loss_kron<4, 9, 4>), dim3(B), dim3(256), 0, m->s, m->o, sigma, A, g, m->lpart,
-		                   gscale);
+	DGX_KRON_LOSS(16, 4) DGX_KRON_LOSS(9, 4) DGX_KRON_LOSS(16, 8) DGX_KRON_LOSS(9, 8)
+#undef DGX_KRON_LOSS
 	else
 		return hipErrorInvalidValue;
 	DGX_HIP_CHECK(hipGetLastError());
@@ -473,13 +473,15 @@ struct PrecondLearned : Precond {
 	hipError_t apply(Ctx& c, const double* v, double* z) override
 	{
 		if (mlp->kind == DGX_MLP_KRON) {
-			if (mlp->Np == 16)
-				hipLaunchKernelGGL((k_kron_apply<4, 16, 4>), dim3(c.E), dim3(64), 0, c.stream, mlp->o, sigma, v, z, c.E);
-			else if (mlp->Np == 9)
-				hipLaunchKernelGGL((k_kron_apply<4, 9, 4>), dim3(c.E), dim3(64), 0, c.stream, mlp->o, sigma, v, z, c.E);
-			else
-				return hipErrorInvalidValue;
-			return hipGetLastError();
+#define DGX_KRON_APPLY(NPN, RK)                                                                                     \
+			if (mlp->Np == NPN && mlp->rank == RK) {                                                                \
+				hipLaunchKernelGGL((k_kron_apply<4, NPN, RK>), dim3(c.E), dim3(64), 0, c.stream, mlp->o, sigma, v, z, \
+				                   c.E);                                                                            \
+				return hipGetLastError();                                                                          \
+			}
+			DGX_KRON_APPLY(16, 4) DGX_KRON_APPLY(9, 4) DGX_KRON_APPLY(16, 8) DGX_KRON_APPLY(9, 8)
+#undef DGX_KRON_APPLY
+			return hipErrorInvalidValue;
 		}
 		if (mlp->nb == 64 && mlp->rank == 8)
 			hipLaunchKernelGGL((k_lowrank_apply<64, 8>), dim3(c.E), dim3(64), 0, c.stream, mlp->o, sigma, v, z, c.E);
@@ -551,7 +553,8 @@ dgx_status dgx_mlp_create(dgx_mlp** out, int kind, int nb, int Np, int hidden, i
 	if (!out || hidden < 16 || hidden > 1024) return DGX_ERR_ARG;
 	// 2-D Euler blocks: nv = 4 variables, p = 2 (Np = 9) or p = 3 (Np = 16)
 	if (!((nb == 64 && Np == 16) || (nb == 36 && Np == 9))) return DGX_ERR_ARG;
-	if (!((kind == DGX_MLP_LOWRANK && rank == 8) || (kind == DGX_MLP_KRON && rank == 4))) return DGX_ERR_ARG;
+	if (!((kind == DGX_MLP_LOWRANK && rank == 8) || (kind == DGX_MLP_KRON && (rank == 4 || rank == 8))))
+		return DGX_ERR_ARG;
 	const int nv = nb / Np;
 	dgx_mlp* m = new dgx_mlp();
 	m->kind = kind;

@@ -522,6 +522,96 @@ __global__ void __launch_bounds__(64) k_block_lu_solve(const double* __restrict_
 	for (int i = l; i < nb; i += 64) xe[i] = b[i];
 }
 
+// In-place inverse of every block by Gauss-Jordan elimination with partial pivoting, stored transposed
+// (out[j * nb + i] = (A^-1)[i][j]) so that the apply kernel reads it coalesced.
+__global__ void __launch_bounds__(256) k_block_inverse(double* blocks, int nb, int* status)
+{
+	extern __shared__ double A[];  // nb*nb, then the pivot column (nb)
+	double* col = A + nb * nb;
+	__shared__ int sp, sperm[kMaxInvBlock];
+	const int e = blockIdx.x;
+	double* g = blocks + (size_t)e * nb * nb;
+	for (int i = threadIdx.x; i < nb * nb; i += blockDim.x) A[i] = g[i];
+	__syncthreads();
+	for (int k = 0; k < nb; ++k) {
+		if (threadIdx.x == 0) {
+			int p = k;
+			double best = fabs(A[k * nb + k]);
+			for (int i = k + 1; i < nb; ++i)
+				if (fabs(A[i * nb + k]) > best) { best = fabs(A[i * nb + k]); p = i; }
+			sp = p;
+			sperm[k] = p;
+			if (!(best > 0.0) && status) atomicOr(status, 1);
+		}
+		__syncthreads();
+		const int p = sp;
+		if (p != k)
+			for (int j = threadIdx.x; j < nb; j += blockDim.x) {
+				const double t = A[k * nb + j];
+				A[k * nb + j] = A[p * nb + j];
+				A[p * nb + j] = t;
+			}
+		__syncthreads();
+		const double inv = 1.0 / A[k * nb + k];
+		for (int i = threadIdx.x; i < nb; i += blockDim.x) col[i] = A[i * nb + k];
+		__syncthreads();
+		for (int j = threadIdx.x; j < nb; j += blockDim.x) A[k * nb + j] = (j == k ? 1.0 : A[k * nb + j]) * inv;
+		__syncthreads();
+		for (int ij = threadIdx.x; ij < nb * nb; ij += blockDim.x) {
+			const int i = ij / nb, j = ij % nb;
+			if (i == k) continue;
+			if (j == k) A[ij] = -col[i] * A[k * nb + k];
+			else A[ij] -= col[i] * A[k * nb + j];
+		}
+		__syncthreads();
+	}
+	// undo the row interchanges as column interchanges, last first
+	for (int k = nb - 1; k >= 0; --k) {
+		const int p = sperm[k];
+		if (p != k)
+			for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+				const double t = A[i * nb + k];
+				A[i * nb + k] = A[i * nb + p];
+				A[i * nb + p] = t;
+			}
+		__syncthreads();
+	}
+	for (int ij = threadIdx.x; ij < nb * nb; ij += blockDim.x) {
+		const int i = ij / nb, j = ij % nb;
+		g[(size_t)j * nb + i] = A[ij];
+	}
+}
+
+// z_e = A_e^-1 v_e with the transposed inverse from k_block_inverse (one wavefront per element)
+__global__ void __launch_bounds__(64) k_block_inv_apply(const double* __restrict__ invT, const double* __restrict__ v,
+                                                        double* z, int nb)
+{
+	extern __shared__ double sv[];
+	const int e = blockIdx.x, l = threadIdx.x;
+	const double* M = invT + (size_t)e * nb * nb;
+	for (int i = l; i < nb; i += 64) sv[i] = v[(size_t)e * nb + i];
+	__syncthreads();
+	for (int i = l; i < nb; i += 64) {
+		double acc = 0.0;
+		for (int k = 0; k < nb; ++k) acc += M[(size_t)k * nb + i] * sv[k];
+		z[(size_t)e * nb + i] = acc;
+	}
+}
+
+hipError_t launch_block_inverse(hipStream_t s, double* blocks, int E, int nb, int* status)
+{
+	if (E == 0) return hipSuccess;
+	if (nb > kMaxInvBlock) return hipErrorInvalidValue;
+	hipLaunchKernelGGL(k_block_inverse, dim3(E), dim3(256), sizeof(double) * (nb * nb + nb), s, blocks, nb, status);
+	return hipGetLastError();
+}
+hipError_t launch_block_inv_apply(hipStream_t s, const double* invT, const double* v, double* z, int E, int nb)
+{
+	if (E == 0) return hipSuccess;
+	hipLaunchKernelGGL(k_block_inv_apply, dim3(E), dim3(64), sizeof(double) * nb, s, invT, v, z, nb);
+	return hipGetLastError();
+}
+
 hipError_t launch_block_lu(hipStream_t s, double* blocks, int* piv, int E, int nb)
 {
 	if (E == 0) return hipSuccess;

@@ -284,12 +284,17 @@ def _apply_rotation(d, N, pts, rotate):
             for _ in range(r % 4):                 # 90-degree turn of the local frame
                 c = np.stack([N - 1 - c[:, 1], c[:, 0]], axis=1)
         else:
-            if r & 1:
+            # proper rotations only (a mirrored local frame would have det J < 0): each bit is a
+            # half turn about a face diagonal or an axis, so every face orientation code occurs
+            if r & 1:                              # swap axes 0,1 and reverse axis 2
                 c = c[:, [1, 0, 2]]
-            if r & 2:
+                c[:, 2] = N - 1 - c[:, 2]
+            if r & 2:                              # swap axes 1,2 and reverse axis 0
                 c = c[:, [0, 2, 1]]
-            if r & 4:
                 c[:, 0] = N - 1 - c[:, 0]
+            if r & 4:                              # reverse axes 0 and 1
+                c[:, 0] = N - 1 - c[:, 0]
+                c[:, 1] = N - 1 - c[:, 1]
         src = sum(c[:, k] * N ** k for k in range(d))
         out[e] = pts[e][src]
     return out

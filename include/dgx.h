@@ -134,6 +134,11 @@ dgx_status dgx_jacobian_vector(dgx_ctx* ctx, const double* u, const double* v, d
 dgx_status dgx_block_lu_factor(dgx_ctx* ctx, double* blocks, int* piv, int E, int nb);
 /* x <- A_e^{-1} x for every element block (x: E*nb). */
 dgx_status dgx_block_lu_solve(dgx_ctx* ctx, const double* lu, const int* piv, double* x, int E, int nb);
+/* In-place inverse of E blocks of size nb <= 64 (Gauss-Jordan, partial pivoting), stored transposed:
+ * blocks[e][j * nb + i] = (A_e^-1)[i][j].  The block-Jacobi preconditioner applies it as one mat-vec. */
+dgx_status dgx_block_inverse(dgx_ctx* ctx, double* blocks, int E, int nb);
+/* y_e = A_e^-1 x_e from the transposed inverses of dgx_block_inverse (x, y: E*nb, distinct). */
+dgx_status dgx_block_inv_apply(dgx_ctx* ctx, const double* inv, const double* x, double* y, int E, int nb);
 
 /* Steady solve by pseudo-time Newton-GMRES (solve_implicit, solve_implicit.c:95-110, 232-281, 393-470).
  * Names mirror the reference's test_case.data keys (cfl_initial, exit_tol_i, exit_ratio_i) and the PETSc
@@ -149,6 +154,9 @@ typedef struct dgx_implicit_params {
 	double gmres_rtol;    /* -ksp_rtol */
 	int gmres_max_iter;   /* per Newton step; hitting it is DIVERGED_ITS: counted, Newton continues */
 	int precond;          /* DGX_PC_* (DGX_PC_LEARNED needs dgx_attach_mlp) */
+	double cfl_max;       /* upper bound of the ramped CFL number (0 = none, as the reference) */
+	int learned_fallback_iters; /* DGX_PC_LEARNED: when GMRES has not converged after this many iterations the
+	                               step is solved again with the exact block-Jacobi preconditioner (0 = never) */
 	/* optional capture of the training set along the iterations (DGX_PC_BLOCK_JACOBI only): per Newton
 	 * step the E diagonal blocks (E x nb x nb doubles) and features (E x nf floats) are appended while
 	 * capacity (in elements) lasts; device pointers or NULL */
@@ -166,6 +174,8 @@ typedef struct dgx_implicit_stats {
 	double max_rhs;
 	double seconds;
 	long long captured;   /* elements appended to the capture buffers */
+	int fallback_steps;   /* Newton steps re-solved with block-Jacobi (learned_fallback_iters) */
+	int gmres_per_step[DGX_HISTORY_MAX];   /* GMRES iterations of each Newton step (all attempts) */
 	double history[DGX_HISTORY_MAX];
 } dgx_implicit_stats;
 dgx_status dgx_solve_implicit(dgx_ctx* ctx, double* u, const dgx_implicit_params* params, dgx_implicit_stats* stats);
@@ -175,7 +185,8 @@ dgx_status dgx_solve_implicit(dgx_ctx* ctx, double* u, const dgx_implicit_params
  * [state / u_ref (nb) | det J / mean (Np) | log10(h / (dt * wave speed))] to an approximate inverse of the
  * element diagonal block, scaled by sigma_e = 1 / (mean(w det J)/dt_e + 0.25 h_e lambda_e):
  *   DGX_MLP_LOWRANK: P_e = sigma_e (diag(d_e) + U_e V_e^T), rank 8 (outputs nb + 2 nb 8);
- *   DGX_MLP_KRON:    P_e = sigma_e sum_{r<4} S_r (x) Q_r, S_r nv x nv, Q_r Np x Np (outputs 4 (nv^2 + Np^2)).
+ *   DGX_MLP_KRON:    P_e = sigma_e sum_{r<R} S_r (x) Q_r, S_r nv x nv, Q_r Np x Np, R = 4 or 8
+ *                    (outputs R (nv^2 + Np^2)).
  * Both start from P_e = -sigma_e I.  2-D Euler with p = 2 (nb 36) or p = 3 (nb 64).
  * Trained with its own forward/backward kernels on the solver's diagonal blocks. */
 enum { DGX_MLP_LOWRANK = 0, DGX_MLP_KRON = 1 };

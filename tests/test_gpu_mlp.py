@@ -8,7 +8,7 @@
   and the trained network, used inside GMRES, reaches the same 1e-10 FP64 steady state as the exact LU
   block-Jacobi solve with fewer GMRES iterations than the untrained network (whose output is close to
   the analytic scaled identity -sigma I).  Both structures are covered: diagonal + rank 8 and the
-  Kronecker sum of rank 4.
+  Kronecker sum of rank 8.
 """
 import numpy as np
 import pytest
@@ -34,7 +34,7 @@ def _mlp(nb, Np, seed=7, kind=0):
     return _lib.MLP(nb, Np, hidden=256, seed=seed, kind=kind)
 
 
-KINDS = [0, 1]   # low rank (diag + rank 8), Kronecker sum (rank 4)
+KINDS = [0, 1]   # low rank (diag + rank 8), Kronecker sum (rank 8)
 
 
 def _perturbed_params(m, rng):

@@ -214,6 +214,22 @@ def test_block_lu_solve_matches_numpy(dev):
     assert _rel(xd.cpu().numpy(), ref) < 1e-12
     with pytest.raises(_lib.DgxError):
         ctx.block_lu_factor(Ad.data_ptr(), piv.data_ptr(), E, 91)   # nb^2 doubles must fit in LDS
+    # the explicit (transposed) inverse used by the block-Jacobi preconditioner, with pivoting exercised
+    A2 = A.copy()
+    A2[:, [0, 5]] = A2[:, [5, 0]]          # zero-free but pivot-demanding leading columns
+    A2[:, 0, 0] = 1e-8
+    Ai = torch.from_numpy(A2.copy()).to(dev)
+    ctx.block_inverse(Ai.data_ptr(), E, nb)
+    y = torch.empty_like(xd)
+    x2 = torch.from_numpy(x.copy()).to(dev)
+    ctx.block_inv_apply(Ai.data_ptr(), x2.data_ptr(), y.data_ptr(), E, nb)
+    torch.cuda.synchronize()
+    inv = np.transpose(Ai.cpu().numpy().reshape(E, nb, nb), (0, 2, 1))
+    np.testing.assert_allclose(inv, np.linalg.inv(A2), rtol=0, atol=1e-12 * np.abs(np.linalg.inv(A2)).max())
+    ref2 = np.linalg.solve(A2, x[..., None])[..., 0]
+    assert _rel(y.cpu().numpy(), ref2) < 1e-12
+    with pytest.raises(_lib.DgxError):
+        ctx.block_inverse(Ai.data_ptr(), E, 65)
     ctx.close()
 
 

@@ -233,13 +233,20 @@ def _curved2d(n=4, p=3, amp=0.04):
 
 
 @pytest.mark.parametrize("flux", [M.FLUX_LF, M.FLUX_ROE])
-@pytest.mark.parametrize("case", ["2d", "3d", "2d_rot"])
+@pytest.mark.parametrize("case", ["2d", "3d", "2d_rot", "3d_rot"])
 def test_free_stream_preservation_curved(case, flux):
     if case == "2d":
         mesh = _curved2d()
     elif case == "2d_rot":
         rot = np.random.default_rng(3).integers(0, 4, 16)
         mesh = M.structured_tp_mesh(2, 3, (4, 4), (0, 0), (1, 1), (True, True), rotate=rot)
+    elif case == "3d_rot":
+        # every face orientation code, right-handed elements only (det J > 0)
+        rot = np.random.default_rng(4).integers(0, 8, 27)
+        mesh = M.structured_tp_mesh(3, 2, (3, 3, 3), (0,) * 3, (1,) * 3, (True,) * 3, curve=M.curve3d(0.05),
+                                    rotate=rot)
+        assert set(np.unique(mesh.faces[:, 4])) == set(range(8))
+        assert O.Oracle(mesh, M.PDE_EULER, flux).geometry()[1].min() > 0
     else:
         mesh = M.config3_mesh(3, 3)
     d = mesh.d
@@ -273,18 +280,26 @@ def test_conservation_sum_of_rhs_is_zero(case):
     assert np.abs(r).max() > 1e-2
 
 
-def test_rotated_mesh_gives_same_residual_as_unrotated():
-    """The same physical mesh with renumbered element-local nodes must give the same nodal residual."""
-    rot = np.random.default_rng(6).integers(0, 4, 9)
-    m0 = M.structured_tp_mesh(2, 2, (3, 3), (0, 0), (1, 1), (True, True))
-    m1 = M.structured_tp_mesh(2, 2, (3, 3), (0, 0), (1, 1), (True, True), rotate=rot)
-    u0 = M.pack_state(M.prim_to_cons(M.smooth_state_3d(M.nodes_xyz(m0)), 2))
-    u1 = M.pack_state(M.prim_to_cons(M.smooth_state_3d(M.nodes_xyz(m1)), 2))
-    r0 = O.Oracle(m0, 1, M.FLUX_ROE).rhs(u0).reshape(9, 4, 9)
-    r1 = O.Oracle(m1, 1, M.FLUX_ROE).rhs(u1).reshape(9, 4, 9)
+@pytest.mark.parametrize("d", [2, 3])
+def test_rotated_mesh_gives_same_residual_as_unrotated(d):
+    """The same physical mesh with renumbered element-local nodes must give the same nodal residual
+    (in 3-D with every face orientation code, on a curved mesh)."""
+    if d == 2:
+        rot = np.random.default_rng(6).integers(0, 4, 9)
+        kw = dict(n=(3, 3), lo=(0, 0), hi=(1, 1), periodic=(True, True))
+    else:
+        rot = np.random.default_rng(7).integers(0, 8, 27)
+        kw = dict(n=(3, 3, 3), lo=(0,) * 3, hi=(1,) * 3, periodic=(True,) * 3, curve=M.curve3d(0.05))
+    m0 = M.structured_tp_mesh(d, 2, **kw)
+    m1 = M.structured_tp_mesh(d, 2, rotate=rot, **kw)
+    u0 = M.pack_state(M.prim_to_cons(M.smooth_state_3d(M.nodes_xyz(m0)), d))
+    u1 = M.pack_state(M.prim_to_cons(M.smooth_state_3d(M.nodes_xyz(m1)), d))
+    nv, Np = d + 2, 3 ** d
+    r0 = O.Oracle(m0, 1, M.FLUX_ROE).rhs(u0).reshape(m0.E, nv, Np)
+    r1 = O.Oracle(m1, 1, M.FLUX_ROE).rhs(u1).reshape(m1.E, nv, Np)
     x0 = M.nodes_xyz(m0)
     x1 = M.nodes_xyz(m1)
-    for e in range(9):
+    for e in range(m0.E):
         # match nodes by coordinates
         dmat = np.abs(x0[e][:, None, :] - x1[e][None, :, :]).sum(-1)
         idx = dmat.argmin(1)
