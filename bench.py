@@ -116,7 +116,7 @@ def time_to_steady(torch, dev, epochs=60):
 
     def solve(pc, **extra):
         u = torch.from_numpy(u0).to(dev)
-        st = ctx.solve_implicit(u.data_ptr(), precond=pc, **kw, **extra)
+        st = ctx.solve_implicit(u.data_ptr(), precond=pc, **{**kw, **extra})
         return st, u
 
     def leg(st):
