@@ -56,15 +56,17 @@ def _state_on_device(torch, xyz_dev, seed):
 
 
 def _traffic_from_profiles(kernel_tag):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
-    path = os.path.join(ROOT, "profiles", "traffic.json")
-    if not os.path.exists(path):
-        return None
+    """HBM bytes per launch (calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE) from the committed summary, if present:
+    lines `traffic <tag> <bytes>` in profiles/rhs_hex_p4_64cube_traffic.md."""
+    path = os.path.join(ROOT, "profiles", "rhs_hex_p4_64cube_traffic.md")
     try:
-        data = json.load(open(path))
-        return data.get(kernel_tag, {}).get("bytes_per_launch")
+        for line in open(path):
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic" and parts[1] == kernel_tag:
+                return int(parts[2])
     except (OSError, ValueError):
         return None
+    return None
 
 
 def cpu_baseline(target_seconds=12.0):
@@ -164,6 +166,8 @@ def main():
     ap.add_argument("--n", type=int, default=64, help="elements per direction (64 = the named workload)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-implicit", action="store_true", help="skip the time-to-1e-10 steady-solve legs")
+    ap.add_argument("--implicit-only", action="store_true",
+                    help="run only the time-to-1e-10 legs and print their JSON (profiling aid)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 1 GiB read + 1 GiB write elementwise kernel (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -178,6 +182,9 @@ def main():
     from dpgsolver_amd import _lib, mesh as M
 
     dev = torch.device("cuda:0")
+    if args.implicit_only:
+        print(json.dumps({"time_to_1e-10": time_to_steady(torch, dev)}), flush=True)
+        return
     n, p = args.n, 4
     t_setup = time.perf_counter()
     mesh = M.config3_mesh(n, p)
