@@ -25,6 +25,7 @@ EXPORTS = [
     "dgx_compute_rhs_explicit", "dgx_apply_minv", "dgx_enforce_positivity", "dgx_ssp_rk3_step", "dgx_max_abs",
     "dgx_convert_f64_f32", "dgx_convert_f32_f64", "dgx_assemble_diag_blocks", "dgx_jacobian_vector",
     "dgx_assemble_face_blocks", "dgx_block_lu_factor", "dgx_block_lu_solve", "dgx_block_inverse", "dgx_block_inv_apply", "dgx_cfl_dt", "dgx_mass_sub", "dgx_solve_implicit",
+    "dgx_mesh_read_gmsh", "dgx_mesh_free",
     "dgx_mlp_create", "dgx_mlp_destroy", "dgx_mlp_sizes", "dgx_mlp_get_params", "dgx_mlp_set_params",
     "dgx_mlp_features", "dgx_mlp_loss_grad", "dgx_mlp_train", "dgx_mlp_predict", "dgx_attach_mlp",
 ]
@@ -42,6 +43,13 @@ class DgxError(RuntimeError):
 class DgxConfig(C.Structure):
     _fields_ = [("d", C.c_int), ("p", C.c_int), ("pde", C.c_int), ("num_flux", C.c_int),
                 ("b_adv", C.c_double * 3), ("bc_prim", C.c_double * 5)]
+
+
+class DgxHostMesh(C.Structure):
+    _fields_ = [("d", C.c_int), ("p", C.c_int), ("E", C.c_int), ("F", C.c_int),
+                ("xyz", C.POINTER(C.c_double)), ("faces", C.POINTER(C.c_int)), ("v_to_v", C.POINTER(C.c_int)),
+                ("v_to_lf", C.POINTER(C.c_int)), ("v_to_lf_wp", C.POINTER(C.c_int)),
+                ("n_unsupported_bc", C.c_int), ("n_inverted", C.c_int)]
 
 
 class DgxMesh(C.Structure):
@@ -101,6 +109,7 @@ def load():
         "dgx_solve_implicit": [vp, vp, C.POINTER(ImplicitParams), C.POINTER(ImplicitStats)],
         "dgx_jacobian_vector": [vp, vp, vp, vp],
         "dgx_assemble_face_blocks": [vp, vp, vp],
+        "dgx_mesh_read_gmsh": [C.c_char_p, C.c_int, C.c_void_p],
         "dgx_mlp_create": [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong],
         "dgx_mlp_destroy": [vp],
         "dgx_mlp_sizes": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_longlong)],
@@ -333,3 +342,25 @@ class MLP:
 
     def attach(self, ctx: "Context"):
         check("dgx_attach_mlp", self.L.dgx_attach_mlp(ctx.h, self.h))
+
+
+def read_gmsh(path: str, p: int) -> dict:
+    """dgx_mesh_read_gmsh: a gmsh 2.2 QUAD mesh as numpy arrays (xyz, faces, v_to_v, v_to_lf, v_to_lf_wp)."""
+    L = load()
+    L.dgx_mesh_free.argtypes = [C.c_void_p]
+    L.dgx_mesh_free.restype = None
+    m = DgxHostMesh()
+    check("dgx_mesh_read_gmsh", L.dgx_mesh_read_gmsh(str(path).encode(), p, C.byref(m)))
+    try:
+        Np = (p + 1) ** m.d
+        nf = 2 * m.d
+        out = {"d": m.d, "p": m.p, "E": m.E, "F": m.F,
+               "xyz": np.ctypeslib.as_array(m.xyz, (m.E, m.d, Np)).copy(),
+               "faces": np.ctypeslib.as_array(m.faces, (m.F, 6)).copy(),
+               "v_to_v": np.ctypeslib.as_array(m.v_to_v, (m.E, nf)).copy(),
+               "v_to_lf": np.ctypeslib.as_array(m.v_to_lf, (m.E, nf)).copy(),
+               "v_to_lf_wp": np.ctypeslib.as_array(m.v_to_lf_wp, (m.E, nf)).copy(),
+               "n_unsupported_bc": m.n_unsupported_bc, "n_inverted": m.n_inverted}
+    finally:
+        L.dgx_mesh_free(C.byref(m))
+    return out

@@ -1,0 +1,292 @@
+// dgx_mesh.cpp -- gmsh 2.2 ASCII mesh ingest for 2-D QUAD meshes (host only, no HIP calls).
+//
+// Follows the reference's reader and connectivity (src/simulation/mesh/mesh_readers.c,
+// mesh_connectivity.c:285-465, mesh_periodic.c):
+//  * volumes are the 2-D elements in file order; gmsh's counter-clockwise QUAD vertices (a, b, c, d) are
+//    taken in tensor-product order (a, b, d, c), r fastest;
+//  * local faces 0..3 are r = -1, r = +1, s = -1, s = +1 with their vertices in increasing face coordinate;
+//  * periodic faces are the boundary line elements with a periodic tag (x051..x056 translation, x061/x062
+//    reflected in y); masters (x051, x053, x055, x061) and slaves (x052, x054, x056, x062) are paired in the
+//    order of their centroids along the face (the slave's reflected for x062), and a volume face lying on a
+//    slave takes the master's vertices (correct_f_ve_for_periodic, mesh_periodic.c:102-135, 240-440);
+//  * faces match when their vertex sets then agree; v_to_lf_wp keeps only matches whose raw vertex sets agree
+//    and holds the periodic boundary tags elsewhere;
+//  * unmatched faces take the physical tag of the boundary line element with the same vertices (add_bc_info).
+// The geometry is the straight-sided (bilinear) map of the vertices evaluated at the GLL nodes of degree p.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/dgx.h"
+
+namespace {
+
+struct Elem {
+	int type = 0;
+	std::vector<int> tags;
+	std::vector<int> nodes;   // gmsh node ids
+};
+
+void gll_nodes(int n, std::vector<double>& x)
+{
+	// Gauss-Lobatto-Legendre nodes on [-1, 1]: the end points and the roots of P'_{n-1}
+	x.assign(n, 0.0);
+	x[0] = -1.0;
+	x[n - 1] = 1.0;
+	const int m = n - 1;
+	for (int i = 1; i < m; ++i) {
+		double z = -std::cos(M_PI * i / m);
+		for (int it = 0; it < 100; ++it) {
+			// P_m and its first two derivatives by the three-term recurrence
+			double p0 = 1.0, p1 = z;
+			for (int k = 2; k <= m; ++k) {
+				const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+				p0 = p1;
+				p1 = p2;
+			}
+			const double dp = m * (z * p1 - p0) / (z * z - 1.0);
+			const double d2p = (2.0 * z * dp - m * (m + 1.0) * p1) / (1.0 - z * z);
+			const double dz = dp / d2p;
+			z -= dz;
+			if (std::fabs(dz) < 1e-16) break;
+		}
+		x[i] = z;
+	}
+	std::sort(x.begin(), x.end());
+}
+
+int bc_code(int tag)
+{
+	switch (tag % 10000) {
+	case 101: return DGX_BC_RIEMANN;
+	case 102: return DGX_BC_SLIPWALL;
+	default: return -1;
+	}
+}
+
+}  // namespace
+
+extern "C" {
+
+dgx_status dgx_mesh_read_gmsh(const char* path, int p, dgx_host_mesh* out)
+{
+	if (!path || !out || p < 1 || p > 5) return DGX_ERR_ARG;
+	std::memset(out, 0, sizeof(*out));
+	std::ifstream in(path);
+	if (!in) return DGX_ERR_ARG;
+	std::map<int, std::array<double, 2>> nodes;
+	std::vector<Elem> elems;
+	std::string line;
+	while (std::getline(in, line)) {
+		if (line.rfind("$MeshFormat", 0) == 0) {
+			std::getline(in, line);
+			std::istringstream ss(line);
+			double ver = 0;
+			int ft = 1;
+			ss >> ver >> ft;
+			if (ver < 2.0 || ver >= 3.0 || ft != 0) return DGX_ERR_UNSUPPORTED;   // gmsh 2.x ASCII only
+		} else if (line.rfind("$Nodes", 0) == 0) {
+			int n = 0;
+			in >> n;
+			for (int i = 0; i < n; ++i) {
+				int id;
+				double x, y, z;
+				in >> id >> x >> y >> z;
+				nodes[id] = { x, y };
+			}
+		} else if (line.rfind("$Elements", 0) == 0) {
+			int n = 0;
+			in >> n;
+			std::getline(in, line);
+			for (int i = 0; i < n; ++i) {
+				std::getline(in, line);
+				std::istringstream ss(line);
+				int id, type, ntag;
+				ss >> id >> type >> ntag;
+				Elem e;
+				e.type = type;
+				e.tags.resize(ntag);
+				for (int t = 0; t < ntag; ++t) ss >> e.tags[t];
+				int v;
+				while (ss >> v) e.nodes.push_back(v);
+				elems.push_back(e);
+			}
+		}
+	}
+	if (nodes.empty() || elems.empty()) return DGX_ERR_ARG;
+	for (const Elem& e : elems)
+		if (e.type == 2 || e.type == 4 || e.type == 5) return DGX_ERR_UNSUPPORTED;   // TRI / 3-D volumes
+	auto key = [](std::array<int, 2> x) { return std::make_pair(std::min(x[0], x[1]), std::max(x[0], x[1])); };
+
+	// periodic boundary faces: pair masters and slaves by (direction, centroid along the face)
+	struct PFace {
+		int dir;
+		double c;
+		std::array<int, 2> v;
+		int tag;
+	};
+	std::vector<PFace> pm, ps;
+	for (const Elem& e : elems) {
+		if (e.type != 1 || e.tags.empty()) continue;
+		const int base = e.tags[0] % 10000;
+		const bool m = (base == 51 || base == 53 || base == 55 || base == 61);
+		const bool sl = (base == 52 || base == 54 || base == 56 || base == 62);
+		if (!m && !sl) continue;
+		if (base == 55 || base == 56) return DGX_ERR_UNSUPPORTED;   // z-periodic: 3-D only
+		PFace f;
+		f.dir = (base == 53 || base == 54) ? 1 : 0;
+		const int t = 1 - f.dir;   // the coordinate along the face
+		const double scale = (sl && base == 62) ? -1.0 : 1.0;
+		f.c = 0.5 * scale * (nodes[e.nodes[0]][t] + nodes[e.nodes[1]][t]);
+		f.v = { e.nodes[0], e.nodes[1] };
+		f.tag = e.tags[0];
+		(m ? pm : ps).push_back(f);
+	}
+	if (pm.size() != ps.size()) return DGX_ERR_ARG;
+	auto by_pos = [](const PFace& a, const PFace& b) { return a.dir != b.dir ? a.dir < b.dir : a.c < b.c; };
+	std::sort(pm.begin(), pm.end(), by_pos);
+	std::sort(ps.begin(), ps.end(), by_pos);
+	// slave face (raw vertex pair) -> its vertices' master counterparts, matched along the face
+	std::map<std::pair<int, int>, std::map<int, int>> slave_map;
+	for (size_t i = 0; i < ps.size(); ++i) {
+		const PFace &M = pm[i], &S = ps[i];
+		if (M.dir != S.dir) return DGX_ERR_ARG;
+		const int t = 1 - S.dir;
+		const double sc = (S.tag % 10000 == 62) ? -1.0 : 1.0;
+		const double s0 = sc * nodes[S.v[0]][t], m0 = nodes[M.v[0]][t], m1 = nodes[M.v[1]][t];
+		std::map<int, int> mp;
+		if (std::fabs(s0 - m0) <= std::fabs(s0 - m1)) { mp[S.v[0]] = M.v[0]; mp[S.v[1]] = M.v[1]; }
+		else { mp[S.v[0]] = M.v[1]; mp[S.v[1]] = M.v[0]; }
+		slave_map[key(S.v)] = mp;
+	}
+
+	// volumes and their local faces
+	std::vector<std::array<int, 4>> vol;   // tensor-product vertex order
+	for (const Elem& e : elems)
+		if (e.type == 3) vol.push_back({ e.nodes[0], e.nodes[1], e.nodes[3], e.nodes[2] });
+	const int E = (int)vol.size();
+	if (E == 0) return DGX_ERR_UNSUPPORTED;
+	static const int fv[4][2] = { { 0, 2 }, { 1, 3 }, { 0, 1 }, { 2, 3 } };
+	const int nf = 4 * E;
+	std::vector<std::array<int, 2>> raw(nf), cor(nf);
+	for (int v = 0; v < E; ++v)
+		for (int lf = 0; lf < 4; ++lf) {
+			const int a = vol[v][fv[lf][0]], b = vol[v][fv[lf][1]];
+			raw[4 * v + lf] = { a, b };
+			auto it = slave_map.find(key({ a, b }));
+			if (it == slave_map.end()) cor[4 * v + lf] = { a, b };
+			else cor[4 * v + lf] = { it->second.at(a), it->second.at(b) };
+		}
+	std::map<std::pair<int, int>, std::vector<int>> groups;
+	for (int f = 0; f < nf; ++f) groups[key(cor[f])].push_back(f);
+	std::map<std::pair<int, int>, int> btag;   // boundary line elements by raw vertex pair
+	for (const Elem& e : elems)
+		if (e.type == 1 && !e.tags.empty()) btag[key({ e.nodes[0], e.nodes[1] })] = e.tags[0];
+
+	std::vector<int> v2v(nf, -1), v2lf(nf, -1), v2lfwp(nf, -1);
+	std::vector<int> faces;
+	int n_unsupported = 0;
+	for (const auto& g : groups) {
+		const std::vector<int>& fs = g.second;
+		if (fs.size() == 2) {
+			const int f0 = std::min(fs[0], fs[1]), f1 = std::max(fs[0], fs[1]);
+			v2v[f0] = f1 / 4;
+			v2lf[f0] = f1 % 4;
+			v2v[f1] = f0 / 4;
+			v2lf[f1] = f0 % 4;
+			if (key(raw[f0]) == key(raw[f1])) {
+				v2lfwp[f0] = v2lf[f0];
+				v2lfwp[f1] = v2lf[f1];
+			}
+			// same direction along the face (after the periodic map) or reversed
+			const int ord = (cor[f0][0] == cor[f1][0]) ? 0 : 2;
+			const int row[6] = { f0 / 4, f0 % 4, f1 / 4, f1 % 4, ord, DGX_BC_INTERIOR };
+			faces.insert(faces.end(), row, row + 6);
+		} else if (fs.size() == 1) {
+			const int f = fs[0];
+			auto it = btag.find(key(raw[f]));
+			const int tag = (it == btag.end()) ? -1 : it->second;
+			v2lf[f] = tag;
+			const int bc = bc_code(tag);
+			if (bc < 0) ++n_unsupported;
+			const int row[6] = { f / 4, f % 4, -1, -1, 0, bc < 0 ? DGX_BC_RIEMANN : bc };
+			faces.insert(faces.end(), row, row + 6);
+		} else {
+			return DGX_ERR_UNSUPPORTED;   // non-manifold face
+		}
+	}
+	// periodic boundary tags in v_to_lf_wp (add_bc_info with include_periodic)
+	for (int f = 0; f < nf; ++f)
+		if (v2lfwp[f] == -1) {
+			auto it = btag.find(key(raw[f]));
+			if (it != btag.end()) v2lfwp[f] = it->second;
+		}
+
+	// straight-sided geometry at the GLL nodes of degree p
+	const int N = p + 1, Np = N * N;
+	std::vector<double> r;
+	gll_nodes(N, r);
+	double* xyz = static_cast<double*>(std::malloc(sizeof(double) * (size_t)E * 2 * Np));
+	int* fout = static_cast<int*>(std::malloc(sizeof(int) * faces.size()));
+	int* a = static_cast<int*>(std::malloc(sizeof(int) * nf));
+	int* b = static_cast<int*>(std::malloc(sizeof(int) * nf));
+	int* c = static_cast<int*>(std::malloc(sizeof(int) * nf));
+	if (!xyz || !fout || !a || !b || !c) {
+		std::free(xyz); std::free(fout); std::free(a); std::free(b); std::free(c);
+		return DGX_ERR_ARG;
+	}
+	int n_inverted = 0;
+	for (int v = 0; v < E; ++v) {
+		double X[4][2];
+		for (int k = 0; k < 4; ++k) {
+			X[k][0] = nodes[vol[v][k]][0];
+			X[k][1] = nodes[vol[v][k]][1];
+		}
+		for (int j = 0; j < N; ++j)
+			for (int i = 0; i < N; ++i) {
+				const double s0 = 0.5 * (1 - r[i]), s1 = 0.5 * (1 + r[i]), t0 = 0.5 * (1 - r[j]), t1 = 0.5 * (1 + r[j]);
+				for (int dd = 0; dd < 2; ++dd)
+					xyz[((size_t)v * 2 + dd) * Np + i + N * j] =
+					    s0 * t0 * X[0][dd] + s1 * t0 * X[1][dd] + s0 * t1 * X[2][dd] + s1 * t1 * X[3][dd];
+			}
+		// orientation: det J at the first vertex of the bilinear map
+		const double j00 = X[1][0] - X[0][0], j01 = X[2][0] - X[0][0], j10 = X[1][1] - X[0][1], j11 = X[2][1] - X[0][1];
+		if (j00 * j11 - j01 * j10 <= 0.0) ++n_inverted;
+	}
+	std::copy(faces.begin(), faces.end(), fout);
+	std::copy(v2v.begin(), v2v.end(), a);
+	std::copy(v2lf.begin(), v2lf.end(), b);
+	std::copy(v2lfwp.begin(), v2lfwp.end(), c);
+	out->d = 2;
+	out->p = p;
+	out->E = E;
+	out->F = (int)(faces.size() / 6);
+	out->xyz = xyz;
+	out->faces = fout;
+	out->v_to_v = a;
+	out->v_to_lf = b;
+	out->v_to_lf_wp = c;
+	out->n_unsupported_bc = n_unsupported;
+	out->n_inverted = n_inverted;
+	return DGX_OK;
+}
+
+void dgx_mesh_free(dgx_host_mesh* m)
+{
+	if (!m) return;
+	std::free(m->xyz);
+	std::free(m->faces);
+	std::free(m->v_to_v);
+	std::free(m->v_to_lf);
+	std::free(m->v_to_lf_wp);
+	std::memset(m, 0, sizeof(*m));
+}
+
+}  // extern "C"

@@ -56,6 +56,18 @@ typedef struct dgx_mesh {
 typedef struct dgx_ctx dgx_ctx;
 typedef struct dgx_mlp dgx_mlp;
 
+/* Host mesh read from a file (dgx_mesh_read_gmsh); arrays are malloc'ed, release with dgx_mesh_free. */
+typedef struct dgx_host_mesh {
+	int d, p, E, F;
+	double* xyz;          /* E x d x (p+1)^d straight-sided geometry at the GLL nodes (dgx_mesh.xyz) */
+	int* faces;           /* F x 6 {eL, fL, eR, fR, ord, bc} (dgx_mesh.faces) */
+	int* v_to_v;          /* E x 2d: neighbour across each local face, -1 on a boundary (Mesh_Connectivity) */
+	int* v_to_lf;         /* E x 2d: the neighbour's local face, or the boundary's physical tag */
+	int* v_to_lf_wp;      /* E x 2d: as v_to_lf, with periodic faces holding their periodic boundary tag */
+	int n_unsupported_bc; /* boundary faces whose tag is neither Riemann (x101) nor slip wall (x102) */
+	int n_inverted;       /* elements numbered clockwise (det J <= 0 in the straight map) */
+} dgx_host_mesh;
+
 /* Library/version and device helpers. */
 const char* dgx_version(void);
 dgx_status dgx_device_count(int* n);
@@ -75,6 +87,11 @@ dgx_status dgx_sizes(const dgx_ctx* ctx, int* Np, int* nv, int* Nfc, long long* 
  * (src/simulation/geometry/geometry_T.c:165, 208-325, 734-801;
  *  deprecated/src/setup_geom_factors.c:157-243 for the 3-D curl form). Synchronous. */
 dgx_status dgx_upload_mesh(dgx_ctx* ctx, const dgx_mesh* mesh);
+/* gmsh 2.2 ASCII reader for 2-D QUAD meshes (mesh/mesh_readers.c, mesh_connectivity.c:285-465, mesh_periodic.c):
+ * volumes in file order, $Periodic node correspondences, boundary tags from the line elements; geometry is
+ * the straight-sided map at the GLL nodes of degree p.  TRI or 3-D volumes: DGX_ERR_UNSUPPORTED.  Host only. */
+dgx_status dgx_mesh_read_gmsh(const char* path, int p, dgx_host_mesh* out);
+void dgx_mesh_free(dgx_host_mesh* m);
 /* Copies of the device geometry into HOST arrays (any may be NULL). Synchronous. */
 dgx_status dgx_get_geometry(dgx_ctx* ctx, double* metrics_vc, double* detJ_vc, double* nJ_fc);
 
