@@ -25,7 +25,7 @@ EXPORTS = [
     "dgx_compute_rhs_explicit", "dgx_apply_minv", "dgx_enforce_positivity", "dgx_ssp_rk3_step", "dgx_max_abs",
     "dgx_convert_f64_f32", "dgx_convert_f32_f64", "dgx_assemble_diag_blocks", "dgx_jacobian_vector",
     "dgx_assemble_face_blocks", "dgx_block_lu_factor", "dgx_block_lu_solve", "dgx_block_inverse", "dgx_block_inv_apply", "dgx_cfl_dt", "dgx_mass_sub", "dgx_solve_implicit",
-    "dgx_mesh_read_gmsh", "dgx_mesh_free",
+    "dgx_mesh_read_gmsh", "dgx_mesh_free", "dgx_restart_write_gmsh", "dgx_restart_read_solution",
     "dgx_mlp_create", "dgx_mlp_destroy", "dgx_mlp_sizes", "dgx_mlp_get_params", "dgx_mlp_set_params",
     "dgx_mlp_features", "dgx_mlp_loss_grad", "dgx_mlp_train", "dgx_mlp_predict", "dgx_attach_mlp",
 ]
@@ -110,6 +110,8 @@ def load():
         "dgx_jacobian_vector": [vp, vp, vp, vp],
         "dgx_assemble_face_blocks": [vp, vp, vp],
         "dgx_mesh_read_gmsh": [C.c_char_p, C.c_int, C.c_void_p],
+        "dgx_restart_write_gmsh": [C.c_char_p, C.POINTER(DgxConfig), C.POINTER(DgxMesh), dp],
+        "dgx_restart_read_solution": [C.c_char_p, C.POINTER(DgxConfig), C.c_int, dp],
         "dgx_mlp_create": [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong],
         "dgx_mlp_destroy": [vp],
         "dgx_mlp_sizes": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_longlong)],
@@ -364,3 +366,33 @@ def read_gmsh(path: str, p: int) -> dict:
     finally:
         L.dgx_mesh_free(C.byref(m))
     return out
+
+
+def _config(d, p, pde, num_flux=0):
+    cfg = DgxConfig()
+    cfg.d, cfg.p, cfg.pde, cfg.num_flux = d, p, pde, num_flux
+    return cfg
+
+
+def restart_write(path: str, d: int, p: int, pde: int, xyz: np.ndarray, faces: np.ndarray, u: np.ndarray):
+    """dgx_restart_write_gmsh for a host mesh and host state."""
+    L = load()
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    faces = np.ascontiguousarray(faces, dtype=np.int32)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    m = DgxMesh(xyz.shape[0], faces.shape[0], xyz.ctypes.data_as(C.POINTER(C.c_double)),
+                faces.ctypes.data_as(C.POINTER(C.c_int)))
+    cfg = _config(d, p, pde)
+    check("dgx_restart_write_gmsh", L.dgx_restart_write_gmsh(str(path).encode(), C.byref(cfg), C.byref(m),
+                                                             u.ctypes.data_as(C.POINTER(C.c_double))))
+
+
+def restart_read(path: str, d: int, p: int, pde: int, E: int) -> np.ndarray:
+    """dgx_restart_read_solution: nodal state from a restart file."""
+    L = load()
+    nv = d + 2 if pde == 1 else 1
+    u = np.zeros(E * nv * (p + 1) ** d)
+    cfg = _config(d, p, pde)
+    check("dgx_restart_read_solution", L.dgx_restart_read_solution(str(path).encode(), C.byref(cfg), E,
+                                                                   u.ctypes.data_as(C.POINTER(C.c_double))))
+    return u

@@ -92,6 +92,13 @@ dgx_status dgx_upload_mesh(dgx_ctx* ctx, const dgx_mesh* mesh);
  * the straight-sided map at the GLL nodes of degree p.  TRI or 3-D volumes: DGX_ERR_UNSUPPORTED.  Host only. */
 dgx_status dgx_mesh_read_gmsh(const char* path, int p, dgx_host_mesh* out);
 void dgx_mesh_free(dgx_host_mesh* m);
+/* The reference's restart file (restart_writers.c:57-171, 301-412): gmsh 2.2 vertices, boundary faces tagged
+ * 10101 (Riemann) / 10102 (slip wall), volumes, and $SolutionCoefficients with each volume's Bezier
+ * coefficients (dof fastest).  u: HOST state in the dgx_compute_rhs layout.  Host only. */
+dgx_status dgx_restart_write_gmsh(const char* path, const dgx_config* cfg, const dgx_mesh* mesh, const double* u);
+/* Reads $SolutionCoefficients back into nodal values (initialize_volumes_sol_coef, restart_T.c:317-420);
+ * a different degree in the file: DGX_ERR_UNSUPPORTED. */
+dgx_status dgx_restart_read_solution(const char* path, const dgx_config* cfg, int E, double* u);
 /* Copies of the device geometry into HOST arrays (any may be NULL). Synchronous. */
 dgx_status dgx_get_geometry(dgx_ctx* ctx, double* metrics_vc, double* detJ_vc, double* nJ_fc);
 
