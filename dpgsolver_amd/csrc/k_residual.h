@@ -221,6 +221,67 @@ __device__ inline void sweep_to(const S* src, S* dst, const R* M, int dir, int t
 	__syncthreads();
 }
 
+// One pencil per thread (NB * N^(DD-1) <= NT): the pencil's base offset for each direction is computed once
+// per thread and reused by every sweep of the kernel.
+template <int DD, int N, int NB>
+__device__ inline int pencil_base(int pen, int dir)
+{
+	constexpr int BL = ipow(N, DD), NL = ipow(N, DD - 1);
+	if (pen >= NB * NL) return -1;
+	const int Sd = (dir == 0) ? 1 : (dir == 1 ? N : N * N);
+	const int b = pen / NL, l = pen - b * NL;
+	return b * BL + (l % Sd) + (l / Sd) * Sd * N;
+}
+
+// in-place sweep along DIR of the thread's pencil (base < 0: idle thread, still takes both barriers)
+template <typename S, typename R, int N, int DIR>
+__device__ inline void sweep_inplace_pb(S* buf, const R* M, int base)
+{
+	constexpr int Sd = (DIR == 0) ? 1 : (DIR == 1 ? N : N * N);
+	S in[N];
+	if (base >= 0) {
+#pragma unroll
+		for (int q = 0; q < N; ++q) in[q] = buf[base + q * Sd];
+	}
+	__syncthreads();
+	if (base >= 0) {
+#pragma unroll
+		for (int q = 0; q < N; ++q) {
+			S acc = M[q * N] * in[0];
+#pragma unroll
+			for (int j = 1; j < N; ++j) acc += M[q * N + j] * in[j];
+			buf[base + q * Sd] = acc;
+		}
+	}
+	__syncthreads();
+}
+
+// dst (+)= sweep(src) along DIR for the thread's pencil (src != dst); the accumulation starts from dst
+template <typename S, typename R, int N, int DIR, bool ACC>
+__device__ inline void sweep_to_pb(const S* src, S* dst, const R* M, int base)
+{
+	constexpr int Sd = (DIR == 0) ? 1 : (DIR == 1 ? N : N * N);
+	if (base >= 0) {
+		S in[N];
+#pragma unroll
+		for (int q = 0; q < N; ++q) in[q] = src[base + q * Sd];
+#pragma unroll
+		for (int q = 0; q < N; ++q) {
+			S acc;
+			if (ACC) {
+				acc = dst[base + q * Sd];
+				acc += M[q * N] * in[0];
+			} else {
+				acc = M[q * N] * in[0];
+			}
+#pragma unroll
+			for (int j = 1; j < N; ++j) acc += M[q * N + j] * in[j];
+			dst[base + q * Sd] = acc;
+		}
+	}
+	__syncthreads();
+}
+
 template <typename S, int D, int N, int NV>
 constexpr int rhs_epb()
 {
@@ -282,6 +343,12 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 	S* U = sBuf[sl];
 	S* TR = U + NV * Np;
 	S* BUF = TR + NFACE * NV * NF;   // neighbour-trace half, reused for the flux components
+	constexpr bool PB = (NV * ipow(N, D - 1) <= NT);   // one volume pencil per thread
+	int pb[3] = { -1, -1, -1 };
+	if constexpr (PB) {
+#pragma unroll
+		for (int dir = 0; dir < D; ++dir) pb[dir] = pencil_base<D, N, NV>(t, dir);
+	}
 	const size_t ebase = (size_t)e * NV * Np;
 	if (valid) {
 		if constexpr (PERT) {
@@ -331,8 +398,14 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 	if constexpr (D == 3 && 2 * NFACE * NV <= NT) {
 		// face traces: one N x N block per thread (in place, private), then the volume sweeps
 		face_block_interp<S, R, N, 2 * NFACE * NV, NT>(TR, sI, t);
+		if constexpr (PB) {
+			sweep_inplace_pb<S, R, N, 0>(U, sI, pb[0]);
+			sweep_inplace_pb<S, R, N, 1>(U, sI, pb[1]);
+			sweep_inplace_pb<S, R, N, 2>(U, sI, pb[2]);
+		} else {
 #pragma unroll
-		for (int dir = 0; dir < D; ++dir) sweep_inplace<S, R, D, N, NV, NT>(U, sI, dir, t);
+			for (int dir = 0; dir < D; ++dir) sweep_inplace<S, R, D, N, NV, NT>(U, sI, dir, t);
+		}
 	} else {
 #pragma unroll
 		for (int dir = 0; dir < D; ++dir) {
@@ -479,8 +552,14 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 			for (int v = 0; v < NV; ++v) BUF[v * Np + t] = fr[d0][v];
 		}
 		__syncthreads();
-		if (d0 == 0) sweep_to<S, R, D, N, NV, NT, false>(BUF, U, sDhT, 0, t);
-		else sweep_to<S, R, D, N, NV, NT, true>(BUF, U, sDhT, d0, t);
+		if constexpr (PB && D == 3) {
+			if (d0 == 0) sweep_to_pb<S, R, N, 0, false>(BUF, U, sDhT, pb[0]);
+			else if (d0 == 1) sweep_to_pb<S, R, N, 1, true>(BUF, U, sDhT, pb[1]);
+			else sweep_to_pb<S, R, N, 2, true>(BUF, U, sDhT, pb[2]);
+		} else {
+			if (d0 == 0) sweep_to<S, R, D, N, NV, NT, false>(BUF, U, sDhT, 0, t);
+			else sweep_to<S, R, D, N, NV, NT, true>(BUF, U, sDhT, d0, t);
+		}
 	}
 
 	// ---- face lifts (and the mass inverse) at the GL nodes
@@ -540,8 +619,14 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 	}
 	__syncthreads();
 	// ---- back to the GLL test functions: Ibar^T (rhs) or Ibar^{-1} (M^-1 rhs)
+	if constexpr (PB && D == 3) {
+		sweep_inplace_pb<S, R, N, 0>(U, MINV ? sIinv : sIT, pb[0]);
+		sweep_inplace_pb<S, R, N, 1>(U, MINV ? sIinv : sIT, pb[1]);
+		sweep_inplace_pb<S, R, N, 2>(U, MINV ? sIinv : sIT, pb[2]);
+	} else {
 #pragma unroll
-	for (int dir = 0; dir < D; ++dir) sweep_inplace<S, R, D, N, NV, NT>(U, MINV ? sIinv : sIT, dir, t);
+		for (int dir = 0; dir < D; ++dir) sweep_inplace<S, R, D, N, NV, NT>(U, MINV ? sIinv : sIT, dir, t);
+	}
 
 	if (valid) {
 		double lmax = 0.0;
