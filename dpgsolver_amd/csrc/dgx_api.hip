@@ -353,6 +353,15 @@ dgx_status dgx_mass_sub(dgx_ctx* h, const double* x, const double* scale, double
 	return hip_status(dgx::launch_mass_sub(c, x, scale, y));
 }
 
+dgx_status dgx_newton_operator(dgx_ctx* h, const double* u, const double* mass_scale, const double* v, double* out)
+{
+	if (!h || !u || !mass_scale || !v || !out) return DGX_ERR_ARG;
+	Ctx& c = h->c;
+	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
+	return hip_status(dgx::launch_jvp(c, u, v, out, mass_scale));
+}
+
 dgx_status dgx_jacobian_vector(dgx_ctx* h, const double* u, const double* v, double* jv)
 {
 	if (!h || !u || !v || !jv) return DGX_ERR_ARG;

@@ -406,8 +406,48 @@ struct PrecondBlockJacobi : Precond {
 
 
 // ------------------------------------------------------------------------------- GMRES
+// One Arnoldi column on the device (single thread): H(0..j, j) = h1 + h2, H(j+1, j) = |w|, the previous
+// Givens rotations applied to the new column, the new rotation, and the residual estimate |g(j+1)|.
+// st[0] = |g(j+1)|, st[1] = first column count at which |g| <= tol (or -1), st[2] = |w|.
+__global__ void k_arnoldi_step(double* H, double* cs, double* sn, double* g, const double* dcol, int j, int m,
+                               double tol, double* st)
+{
+	if (threadIdx.x != 0 || blockIdx.x != 0) return;
+	const double* h1 = dcol;
+	const double* h2 = dcol + (m + 1);
+	const double hn = sqrt(dcol[2 * (m + 1)]);
+	double* col = H + (size_t)j * (m + 1);   // column j, rows 0..m
+	for (int i = 0; i <= j; ++i) col[i] = h1[i] + h2[i];
+	col[j + 1] = hn;
+	for (int i = 0; i < j; ++i) {
+		const double a = col[i], b = col[i + 1];
+		col[i] = cs[i] * a + sn[i] * b;
+		col[i + 1] = -sn[i] * a + cs[i] * b;
+	}
+	const double a = col[j], b = col[j + 1];
+	const double r = sqrt(a * a + b * b);
+	cs[j] = r > 0 ? a / r : 1.0;
+	sn[j] = r > 0 ? b / r : 0.0;
+	col[j] = r;
+	col[j + 1] = 0.0;
+	g[j + 1] = -sn[j] * g[j];
+	g[j] = cs[j] * g[j];
+	st[0] = fabs(g[j + 1]);
+	if (st[1] < 0.0 && (st[0] <= tol || hn == 0.0)) st[1] = (double)(j + 1);
+	st[2] = hn;
+}
+
+// g = (beta, 0, ...), st = (beta, -1, 1) at the start of a restart cycle
+__global__ void k_arnoldi_start(double* g, int m, double beta, double* st)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i <= m) g[i] = (i == 0) ? beta : 0.0;
+	if (i == 0) { st[0] = beta; st[1] = -1.0; st[2] = 1.0; }
+}
+
 struct Gmres {
 	int m = 60;
+	static constexpr int kCheck = 4;   // Arnoldi steps between host convergence checks
 	double* V = nullptr;  // (m+1) x n
 	double* w = nullptr;
 	double* z = nullptr;
@@ -416,6 +456,7 @@ struct Gmres {
 	Blas blas;
 	std::vector<double> H, g, cs, sn, y, h1, h2, hcol;
 	double* dcol = nullptr;   // device: [h1 (m+1) | h2 (m+1) | |w|^2]
+	double* dls = nullptr;    // device: H ((m+1) x m, column-major) | cs (m) | sn (m) | g (m+1) | st (3)
 	hipError_t init(Ctx& c, int restart)
 	{
 		m = restart;
@@ -433,22 +474,27 @@ struct Gmres {
 		h2.assign(m + 1, 0.0);
 		hcol.assign(2 * (m + 1) + 1, 0.0);
 		DGX_HIP_CHECK(hipMalloc(&dcol, sizeof(double) * (2 * (m + 1) + 1)));
+		DGX_HIP_CHECK(hipMalloc(&dls, sizeof(double) * ls_size()));
 		return blas.init(c.stream, m + 1);
 	}
 	void release()
 	{
-		for (double* p : { V, w, z, x, dcol })
+		for (double* p : { V, w, z, x, dcol, dls })
 			if (p) (void)hipFree(p);
-		V = w = z = x = dcol = nullptr;
+		V = w = z = x = dcol = dls = nullptr;
 		blas.release();
 	}
+	size_t ls_size() const { return (size_t)(m + 1) * m + 2 * (size_t)m + (m + 1) + 3; }
 	// w = A v with A = dR/du(u) - M/dt
 	hipError_t op(Ctx& c, const double* u, const double* inv_dt, const double* v, double* out)
 	{
-		DGX_HIP_CHECK(launch_jvp(c, u, v, out));
-		return launch_mass_sub(c, v, inv_dt, out);
+		// dual-number J v with the -M/dt term added in the same kernel (at the GL nodes, before Ibar^T)
+		return launch_jvp(c, u, v, out, inv_dt);
 	}
-	// right-preconditioned restarted GMRES on A x = b, x0 = 0; returns iterations, sets converged
+	// right-preconditioned restarted GMRES on A x = b, x0 = 0; returns iterations, sets converged.
+	// The Hessenberg column and its Givens rotation are formed on the device; the host reads the residual
+	// estimate every kCheck steps (steps past convergence only enlarge the Krylov space) and brings H and g
+	// back once per restart cycle for the triangular solve.
 	hipError_t solve(Ctx& c, const double* u, const double* inv_dt, const double* b, Precond& P, double rtol,
 	                 int max_it, int* iters, bool* converged)
 	{
@@ -458,6 +504,16 @@ struct Gmres {
 		*iters = 0;
 		*converged = (bnorm == 0.0);
 		if (*converged) return hipSuccess;
+		const double tol = rtol * bnorm;
+		double* dH = dls;
+		double* dcs = dH + (size_t)(m + 1) * m;
+		double* dsn = dcs + m;
+		double* dg = dsn + m;
+		double* dst = dg + (m + 1);
+		double* dh1 = dcol;
+		double* dh2 = dcol + (m + 1);
+		double* dnrm = dcol + 2 * (m + 1);
+		std::vector<double> hls(ls_size());
 		int total = 0;
 		while (total < max_it) {
 			// r = b - A x  (x = 0 on the first cycle)
@@ -470,54 +526,43 @@ struct Gmres {
 			}
 			double beta;
 			DGX_HIP_CHECK(blas.norm(V, n, &beta));
-			if (beta <= rtol * bnorm) { *converged = true; break; }
+			if (beta <= tol) { *converged = true; break; }
 			DGX_HIP_CHECK(blas.axpby(1.0 / beta, V, 0.0, V, n));
-			std::fill(g.begin(), g.end(), 0.0);
-			g[0] = beta;
-			int j = 0;
+			hipLaunchKernelGGL(k_arnoldi_start, dim3((m + 64) / 64), dim3(64), 0, c.stream, dg, m, beta, dst);
+			DGX_HIP_CHECK(hipGetLastError());
+			int j = 0, jconv = -1;
 			for (; j < m && total < max_it; ++j, ++total) {
 				DGX_HIP_CHECK(P.apply(c, V + (size_t)j * n, z));
 				DGX_HIP_CHECK(op(c, u, inv_dt, z, w));
-				// classical Gram-Schmidt, twice (CGS2), coefficients kept on the device; one host round
-				// trip per iteration brings the Hessenberg column back for the Givens rotations
-				double* dh1 = dcol;
-				double* dh2 = dcol + (m + 1);
-				double* dnrm = dcol + 2 * (m + 1);
+				// classical Gram-Schmidt, twice (CGS2), coefficients on the device
 				DGX_HIP_CHECK(blas.dots_dev(V, n, j + 1, w, n, dh1));
 				DGX_HIP_CHECK(blas.gemv_sub_dev(V, n, dh1, j + 1, w, n));
 				DGX_HIP_CHECK(blas.dots_dev(V, n, j + 1, w, n, dh2));
 				DGX_HIP_CHECK(blas.gemv_sub_dev(V, n, dh2, j + 1, w, n));
 				DGX_HIP_CHECK(blas.dots_dev(w, 0, 1, w, n, dnrm));
 				DGX_HIP_CHECK(blas.scale_by_rnorm(w, dnrm, V + (size_t)(j + 1) * n, n));
-				DGX_HIP_CHECK(hipMemcpyAsync(hcol.data(), dcol, sizeof(double) * (2 * (m + 1) + 1),
-				                             hipMemcpyDeviceToHost, c.stream));
-				DGX_HIP_CHECK(hipStreamSynchronize(c.stream));
-				const double hn = std::sqrt(hcol[2 * (m + 1)]);
-				for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = hcol[i] + hcol[(m + 1) + i];
-				H[(size_t)(j + 1) * m + j] = hn;
-				for (int i = 0; i < j; ++i) {
-					const double a = H[(size_t)i * m + j], bb = H[(size_t)(i + 1) * m + j];
-					H[(size_t)i * m + j] = cs[i] * a + sn[i] * bb;
-					H[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * bb;
+				hipLaunchKernelGGL(k_arnoldi_step, dim3(1), dim3(1), 0, c.stream, dH, dcs, dsn, dg, dcol, j, m, tol, dst);
+				DGX_HIP_CHECK(hipGetLastError());
+				if ((j + 1) % kCheck == 0 || j + 1 == m || total + 1 >= max_it) {
+					double st[3];
+					DGX_HIP_CHECK(hipMemcpyAsync(st, dst, sizeof(st), hipMemcpyDeviceToHost, c.stream));
+					DGX_HIP_CHECK(hipStreamSynchronize(c.stream));
+					if (st[1] >= 0.0) { jconv = (int)st[1]; ++j; ++total; *converged = true; break; }
 				}
-				const double a = H[(size_t)j * m + j], bb = H[(size_t)(j + 1) * m + j];
-				const double r = std::sqrt(a * a + bb * bb);
-				cs[j] = r > 0 ? a / r : 1.0;
-				sn[j] = r > 0 ? bb / r : 0.0;
-				H[(size_t)j * m + j] = r;
-				H[(size_t)(j + 1) * m + j] = 0.0;
-				g[j + 1] = -sn[j] * g[j];
-				g[j] = cs[j] * g[j];
-				if (std::fabs(g[j + 1]) <= rtol * bnorm || hn == 0.0) { ++j; ++total; *converged = true; break; }
 			}
-			// y = H^-1 g, x += P^-1 (V y)
-			for (int i = j - 1; i >= 0; --i) {
-				double s = g[i];
-				for (int k = i + 1; k < j; ++k) s -= H[(size_t)i * m + k] * y[k];
-				y[i] = s / H[(size_t)i * m + i];
+			// y = H^-1 g over the first jj columns, x += P^-1 (V y)
+			const int jj = (jconv > 0) ? jconv : j;
+			DGX_HIP_CHECK(hipMemcpyAsync(hls.data(), dls, sizeof(double) * hls.size(), hipMemcpyDeviceToHost, c.stream));
+			DGX_HIP_CHECK(hipStreamSynchronize(c.stream));
+			const double* Hh = hls.data();
+			const double* gh = Hh + (size_t)(m + 1) * m + 2 * (size_t)m;
+			for (int i = jj - 1; i >= 0; --i) {
+				double sacc = gh[i];
+				for (int k = i + 1; k < jj; ++k) sacc -= Hh[(size_t)k * (m + 1) + i] * y[k];
+				y[i] = sacc / Hh[(size_t)i * (m + 1) + i];
 			}
 			DGX_HIP_CHECK(blas.axpby(0.0, V, 0.0, w, n));
-			DGX_HIP_CHECK(blas.gemv_add(V, n, y.data(), j, w, n));
+			DGX_HIP_CHECK(blas.gemv_add(V, n, y.data(), jj, w, n));
 			DGX_HIP_CHECK(P.apply(c, w, z));
 			DGX_HIP_CHECK(blas.axpby(1.0, z, 1.0, x, n));
 			if (*converged) break;

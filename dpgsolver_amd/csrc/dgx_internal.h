@@ -117,7 +117,8 @@ hipError_t launch_block_lu(hipStream_t s, double* blocks, int* piv, int E, int n
 hipError_t launch_block_lu_solve(hipStream_t s, const double* lu, const int* piv, double* x, int E, int nb);
 hipError_t launch_diag_col(Ctx& c, const double* u, double* blocks, int col);
 hipError_t launch_offd_col(Ctx& c, const double* u, double* blocks, int col);
-hipError_t launch_jvp(Ctx& c, const double* u, const double* v, double* jv);
+// jv = J v, or J v - mass_scale[e] M_e v when mass_scale is not NULL (the pseudo-time Newton operator)
+hipError_t launch_jvp(Ctx& c, const double* u, const double* v, double* jv, const double* mass_scale = nullptr);
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, int n, unsigned long long v);
 hipError_t launch_fill_i32(hipStream_t s, int* p, int n, int v);
 

@@ -41,9 +41,11 @@ static RhsArgs<Dual, double> make_dual(Ctx& c, const double* u, const double* v,
 	return a;
 }
 
-hipError_t launch_jvp(Ctx& c, const double* u, const double* v, double* jv)
+hipError_t launch_jvp(Ctx& c, const double* u, const double* v, double* jv, const double* mass_scale)
 {
-	return dispatch_dual<MODE_JVP>(c, make_dual(c, u, v, jv, 0));
+	RhsArgs<Dual, double> a = make_dual(c, u, v, jv, 0);
+	a.mass_scale = mass_scale;
+	return dispatch_dual<MODE_JVP>(c, a);
 }
 
 hipError_t launch_diag_col(Ctx& c, const double* u, double* blocks, int col)

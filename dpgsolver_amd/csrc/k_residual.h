@@ -60,6 +60,7 @@ struct RhsArgs {
 	int E;
 	int col;              // MODE_DIAG column (local dof)
 	unsigned long long* dmax;
+	const double* mass_scale;   // MODE_JVP: out = J v - mass_scale[e] M_e v when not NULL
 };
 
 template <int D, int N>
@@ -498,11 +499,16 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 
 	// ---- physical flux at the volume cubature, contravariant and weighted, kept in registers
 	S fr[D][NV];
+	double vgl[(MODE == MODE_JVP) ? NV : 1];   // MODE_JVP: (Ibar v) at this node, for the mass term
 	if (valid && MODE != MODE_OFFD) {
 		const int q = t;
 		S s[NV];
 #pragma unroll
 		for (int v = 0; v < NV; ++v) s[v] = U[v * Np + q];
+		if constexpr (MODE == MODE_JVP) {
+#pragma unroll
+			for (int v = 0; v < NV; ++v) vgl[v] = s[v].d;
+		}
 		R m[D * D];
 		const R* mp = a.met + (size_t)e * D * D * Np;
 #pragma unroll
@@ -608,12 +614,22 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 	}
 	if (valid) {
 		const int q = t;
+		double msc = 0.0;   // MODE_JVP mass term: - s_e W_q J_q (Ibar v)_q before the final Ibar^T
+		if constexpr (MODE == MODE_JVP) {
+			if (a.mass_scale) {
+				const int c0 = q % N, c1 = (q / N) % N, c2 = (D == 3) ? q / (N * N) : 0;
+				double wq = sW[c0] * sW[c1];
+				if (D == 3) wq *= sW[c2];
+				msc = a.mass_scale[e] * wq * a.detJ[(size_t)e * Np + q];
+			}
+		}
 #pragma unroll
 		for (int v = 0; v < NV; ++v) {
 			S acc = U[v * Np + q];
 #pragma unroll
 			for (int lf = 0; lf < NFACE; ++lf) acc -= cf[lf] * TR[of[lf] + v * NF];
 			if constexpr (MINV) acc = scale * acc;
+			if constexpr (MODE == MODE_JVP) acc.d -= msc * vgl[v];
 			U[v * Np + q] = acc;
 		}
 	}

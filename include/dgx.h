@@ -154,6 +154,9 @@ dgx_status dgx_mass_sub(dgx_ctx* ctx, const double* x, const double* scale, doub
  * residual: the matrix-free operator of the Newton system (replaces the PETSc AIJ Mat assembled by
  * add_to_petsc_Mat, solve.c:321-338). */
 dgx_status dgx_jacobian_vector(dgx_ctx* ctx, const double* u, const double* v, double* jv);
+/* out = (dR/du)(u) v - mass_scale[e] M_e v_e (mass_scale: device, E doubles): the operator of the pseudo-time
+ * Newton system, J v and the -M/dt term in one dual-number launch (the GMRES operator of dgx_solve_implicit). */
+dgx_status dgx_newton_operator(dgx_ctx* ctx, const double* u, const double* mass_scale, const double* v, double* out);
 /* Batched LU (partial pivoting) of E blocks of size nb <= 90 in place; piv: E*nb ints (device). */
 dgx_status dgx_block_lu_factor(dgx_ctx* ctx, double* blocks, int* piv, int E, int nb);
 /* x <- A_e^{-1} x for every element block (x: E*nb). */

@@ -196,6 +196,26 @@ def test_face_blocks_match_complex_step(dev, case):
     ctx.close()
 
 
+@pytest.mark.parametrize("case", JAC, ids=[c[0] for c in JAC])
+def test_newton_operator_fuses_the_mass_term(dev, case):
+    """dgx_newton_operator (J v - s_e M_e v in one dual-number launch, the GMRES operator) equals
+    dgx_jacobian_vector followed by dgx_mass_sub."""
+    name, mesh, flux, bcp, u = case
+    ctx = _ctx(mesh, M.PDE_EULER, flux, bc_prim=bcp)
+    rng = np.random.default_rng(19)
+    v = rng.standard_normal(u.size)
+    sc = rng.uniform(0.5, 30.0, mesh.E)
+    ud, vd, sd = (torch.from_numpy(a).to(dev) for a in (u, v, sc))
+    a = torch.empty_like(ud)
+    b = torch.empty_like(ud)
+    ctx.newton_operator(ud.data_ptr(), sd.data_ptr(), vd.data_ptr(), a.data_ptr())
+    ctx.jacobian_vector(ud.data_ptr(), vd.data_ptr(), b.data_ptr())
+    ctx.mass_sub(vd.data_ptr(), sd.data_ptr(), b.data_ptr())
+    torch.cuda.synchronize()
+    assert _rel(a.cpu().numpy(), b.cpu().numpy()) < 1e-13
+    ctx.close()
+
+
 def test_block_lu_solve_matches_numpy(dev):
     from dpgsolver_amd import _lib
 
