@@ -24,13 +24,13 @@ EXPORTS = [
     "dgx_sizes", "dgx_upload_mesh", "dgx_get_geometry", "dgx_compute_rhs", "dgx_compute_rhs_f32", "dgx_compute_rhs_mixed",
     "dgx_compute_rhs_explicit", "dgx_apply_minv", "dgx_enforce_positivity", "dgx_ssp_rk3_step", "dgx_max_abs",
     "dgx_convert_f64_f32", "dgx_convert_f32_f64", "dgx_assemble_diag_blocks", "dgx_jacobian_vector",
-    "dgx_assemble_face_blocks", "dgx_newton_operator", "dgx_block_lu_factor", "dgx_block_lu_solve", "dgx_block_inverse", "dgx_block_inv_apply", "dgx_cfl_dt", "dgx_mass_sub", "dgx_solve_implicit",
+    "dgx_assemble_face_blocks", "dgx_newton_operator", "dgx_block_lu_factor", "dgx_block_lu_solve", "dgx_block_inverse", "dgx_block_inv_apply", "dgx_cfl_dt", "dgx_mass_sub", "dgx_precond_apply", "dgx_solve_implicit",
     "dgx_mesh_read_gmsh", "dgx_mesh_free", "dgx_restart_write_gmsh", "dgx_restart_read_solution",
     "dgx_mlp_create", "dgx_mlp_destroy", "dgx_mlp_sizes", "dgx_mlp_get_params", "dgx_mlp_set_params",
     "dgx_mlp_features", "dgx_mlp_loss_grad", "dgx_mlp_train", "dgx_mlp_predict", "dgx_attach_mlp",
 ]
 
-PC_NONE, PC_BLOCK_JACOBI, PC_LEARNED = 0, 1, 2
+PC_NONE, PC_BLOCK_JACOBI, PC_LEARNED, PC_BLOCK_SGS = 0, 1, 2, 3
 HISTORY_MAX = 256
 
 
@@ -110,6 +110,7 @@ def load():
         "dgx_jacobian_vector": [vp, vp, vp, vp],
         "dgx_assemble_face_blocks": [vp, vp, vp],
         "dgx_newton_operator": [vp, vp, vp, vp, vp],
+        "dgx_precond_apply": [vp, C.c_int, vp, vp, vp, vp],
         "dgx_mesh_read_gmsh": [C.c_char_p, C.c_int, C.c_void_p],
         "dgx_restart_write_gmsh": [C.c_char_p, C.POINTER(DgxConfig), C.POINTER(DgxMesh), dp],
         "dgx_restart_read_solution": [C.c_char_p, C.POINTER(DgxConfig), C.c_int, dp],
@@ -245,6 +246,9 @@ class Context:
 
     def newton_operator(self, u: int, mass_scale: int, v: int, out: int):
         check("dgx_newton_operator", self.L.dgx_newton_operator(self.h, vp(u), vp(mass_scale), vp(v), vp(out)))
+
+    def precond_apply(self, precond: int, u: int, inv_dt: int, v: int, z: int):
+        check("dgx_precond_apply", self.L.dgx_precond_apply(self.h, precond, vp(u), vp(inv_dt), vp(v), vp(z)))
 
     def assemble_face_blocks(self, u: int, blocks: int):
         check("dgx_assemble_face_blocks", self.L.dgx_assemble_face_blocks(self.h, vp(u), vp(blocks)))

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -38,6 +39,8 @@ void free_mesh(Ctx& c)
 {
 	release(c.xyz); release(c.faces); release(c.elem_faces); release(c.order); release(c.hmin); release(c.metrics_vc); release(c.metrics_vc32);
 	release(c.detJ_vc); release(c.detJ_vc32); release(c.nJ_fc); release(c.nJ_fc32); release(c.u0); release(c.rtmp);
+	release(c.color_elems);
+	c.color_off.clear();
 	c.mesh_ready = false;
 }
 
@@ -177,6 +180,27 @@ dgx_status dgx_upload_mesh(dgx_ctx* h, const dgx_mesh* m)
 		const std::vector<double> hm = dgx::min_edge_lengths(d, c.N, c.E, m->xyz);
 		DGX_TRY(hipMalloc(&c.hmin, sizeof(double) * hm.size()));
 		DGX_TRY(hipMemcpy(c.hmin, hm.data(), sizeof(double) * hm.size(), hipMemcpyHostToDevice));
+		// greedy colouring of the face-neighbour graph (multicolour block Gauss-Seidel)
+		std::vector<int> color(c.E, -1);
+		int ncol = 0;
+		for (int e = 0; e < c.E; ++e) {
+			unsigned used = 0;
+			for (int lf = 0; lf < NFACE; ++lf) {
+				const int n = ef[(size_t)e * NFACE + lf].en;
+				if (n >= 0 && n != e && color[n] >= 0 && color[n] < 32) used |= 1u << color[n];
+			}
+			int k = 0;
+			while (k < 32 && (used >> k) & 1u) ++k;
+			color[e] = k;
+			ncol = std::max(ncol, k + 1);
+		}
+		c.color_off.assign(ncol + 1, 0);
+		for (int e = 0; e < c.E; ++e) c.color_off[color[e] + 1] += 1;
+		for (int k = 0; k < ncol; ++k) c.color_off[k + 1] += c.color_off[k];
+		std::vector<int> elems(c.E), fill(c.color_off.begin(), c.color_off.end() - 1);
+		for (int e = 0; e < c.E; ++e) elems[fill[color[e]]++] = e;
+		DGX_TRY(hipMalloc(&c.color_elems, sizeof(int) * c.E));
+		DGX_TRY(hipMemcpy(c.color_elems, elems.data(), sizeof(int) * c.E, hipMemcpyHostToDevice));
 	}
 	DGX_TRY(dgx::launch_geometry(c));
 	DGX_TRY(dgx::launch_convert(c.metrics_vc, c.metrics_vc32, nm, c.stream));
@@ -330,9 +354,15 @@ dgx_status dgx_assemble_face_blocks(dgx_ctx* h, const double* u, double* blocks)
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
 	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
-	const int nb = c.Np * c.nv;
-	for (int col = 0; col < nb; ++col) DGX_TRY(dgx::launch_offd_col(c, u, blocks, col));
-	return DGX_OK;
+	const int nfv = c.nv * c.N;   // face-node rows / columns of a 2-D face block
+	double* cT = nullptr;
+	DGX_TRY(hipMalloc(&cT, sizeof(double) * (size_t)c.E * 4 * nfv * nfv));
+	hipError_t e = hipSuccess;
+	for (int col = 0; col < nfv && e == hipSuccess; ++col) e = dgx::launch_offd_col(c, u, cT, col);
+	if (e == hipSuccess) e = dgx::launch_expand_face_blocks(c, cT, blocks);
+	const hipError_t es = hipStreamSynchronize(c.stream);
+	(void)hipFree(cT);
+	return hip_status(e != hipSuccess ? e : es);
 }
 
 dgx_status dgx_cfl_dt(dgx_ctx* h, const double* u, double cfl, double* inv_dt)

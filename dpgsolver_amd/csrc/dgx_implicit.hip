@@ -350,6 +350,57 @@ hipError_t launch_relaxed_update(Ctx& c, double* u, const double* du, int* statu
 // ------------------------------------------------------------------------------- preconditioners
 namespace dgx {
 
+// 2-D face-node map: node of local face lf at face position a (faces r-, r+, s-, s+)
+__device__ inline int sgs_face_node(int lf, int a, int N)
+{
+	const int fix = (lf & 1) ? N - 1 : 0;
+	return (lf >> 1) == 0 ? fix + N * a : a + N * fix;
+}
+
+// One colour of a block Gauss-Seidel sweep on a 2-D mesh: for every element e of the colour,
+// z_e = A_ee^-1 (v_e - sum_faces A_e,n z_n).  A_ee^-1 is stored transposed (nb x nb); the face blocks A_e,n
+// compactly, transposed, on the nfv = nv N face-node rows and columns (the MODE_OFFD layout), so a face
+// costs an nfv x nfv mat-vec on the neighbour's trace.  Elements of one colour share no face, so their
+// updates are independent.  One 64-lane workgroup per element; nb <= 64 and 4 nfv <= 64.
+__global__ void __launch_bounds__(64) k_bgs_color(const int* __restrict__ elems, const ElemFace* __restrict__ ef,
+                                                  const double* __restrict__ invT, const double* __restrict__ offT,
+                                                  const double* __restrict__ rhs, double* z, int N, int nv)
+{
+	__shared__ double sz[64], sf[64], sr[64];
+	const int Np = N * N, nb = Np * nv, nfv = nv * N;
+	const int e = elems[blockIdx.x], t = threadIdx.x;
+	if (t < 4 * nfv) {   // neighbour traces, in each neighbour's own face-node order
+		const int lf = t / nfv, c = t - lf * nfv, w = c / N;
+		const ElemFace f = ef[(size_t)e * 4 + lf];
+		sz[t] = (f.en >= 0) ? z[(size_t)f.en * nb + w * Np + sgs_face_node(f.packed & 15, c - w * N, N)] : 0.0;
+	}
+	__syncthreads();
+	if (t < 4 * nfv) {   // face contributions on the element's face-node rows
+		const int lf = t / nfv, row = t - lf * nfv;
+		const double* C = offT + ((size_t)e * 4 + lf) * nfv * nfv;
+		double acc = 0.0;
+		for (int c = 0; c < nfv; ++c) acc += C[(size_t)c * nfv + row] * sz[lf * nfv + c];
+		sf[t] = acc;
+	}
+	__syncthreads();
+	if (t < nb) {
+		const int v = t / Np, node = t - v * Np, i0 = node % N, i1 = node / N;
+		double r = rhs[(size_t)e * nb + t];
+		if (i0 == 0) r -= sf[0 * nfv + v * N + i1];
+		if (i0 == N - 1) r -= sf[1 * nfv + v * N + i1];
+		if (i1 == 0) r -= sf[2 * nfv + v * N + i0];
+		if (i1 == N - 1) r -= sf[3 * nfv + v * N + i0];
+		sr[t] = r;
+	}
+	__syncthreads();
+	if (t < nb) {
+		const double* M = invT + (size_t)e * nb * nb;
+		double acc = 0.0;
+		for (int k = 0; k < nb; ++k) acc += M[(size_t)k * nb + t] * sr[k];
+		z[(size_t)e * nb + t] = acc;
+	}
+}
+
 struct PrecondIdentity : Precond {
 	hipError_t setup(Ctx&, const double*, const double*) override { return hipSuccess; }
 	hipError_t apply(Ctx& c, const double* v, double* z) override
@@ -401,6 +452,45 @@ struct PrecondBlockJacobi : Precond {
 		if (nb <= kMaxInvBlock) return launch_block_inv_apply(c.stream, blocks, v, z, c.E, nb);
 		DGX_HIP_CHECK(hipMemcpyAsync(z, v, sizeof(double) * (size_t)c.E * nb, hipMemcpyDeviceToDevice, c.stream));
 		return launch_block_lu_solve(c.stream, blocks, piv, z, c.E, nb);
+	}
+};
+
+// Multicolour block symmetric Gauss-Seidel on the block-sparse Jacobian (diagonal blocks with the -M/dt
+// term, exact inverses; compact face blocks from the dual-number MODE_OFFD columns): forward over the
+// colours, then backward, from z = 0.  A fixed linear operator, so plain right-preconditioned GMRES applies.
+struct PrecondBlockSGS : Precond {
+	PrecondBlockJacobi diag;   // assembles and inverts A_ee - M_e/dt
+	double* offT = nullptr;    // E * 4 * nfv^2 compact transposed face blocks
+	~PrecondBlockSGS() override
+	{
+		if (offT) (void)hipFree(offT);
+	}
+	hipError_t setup(Ctx& c, const double* u, const double* inv_dt) override
+	{
+		const int nb = c.Np * c.nv, nfv = c.nv * c.N;
+		if (c.d != 2 || nb > kMaxInvBlock || 4 * nfv > 64) return hipErrorInvalidValue;
+		DGX_HIP_CHECK(diag.setup(c, u, inv_dt));
+		if (!offT) DGX_HIP_CHECK(hipMalloc(&offT, sizeof(double) * (size_t)c.E * 4 * nfv * nfv));
+		for (int col = 0; col < nfv; ++col) DGX_HIP_CHECK(launch_offd_col(c, u, offT, col));
+		return hipSuccess;
+	}
+	hipError_t sweep(Ctx& c, const double* v, double* z, int k)
+	{
+		const int n0 = c.color_off[k], cnt = c.color_off[k + 1] - n0;
+		if (cnt == 0) return hipSuccess;
+		hipLaunchKernelGGL(k_bgs_color, dim3(cnt), dim3(64), 0, c.stream, c.color_elems + n0, c.elem_faces,
+		                   diag.blocks, offT, v, z, c.N, c.nv);
+		return hipGetLastError();
+	}
+	hipError_t apply(Ctx& c, const double* v, double* z) override
+	{
+		const size_t n = (size_t)c.E * c.Np * c.nv;
+		DGX_HIP_CHECK(launch_fill_u64(c.stream, reinterpret_cast<unsigned long long*>(z), (int)n, 0ull));
+		const int nc = (int)c.color_off.size() - 1;
+		for (int k = 0; k < nc; ++k) DGX_HIP_CHECK(sweep(c, v, z, k));
+		// backward: the last colour's inputs have not changed since its forward update, so it is not redone
+		for (int k = nc - 2; k >= 0; --k) DGX_HIP_CHECK(sweep(c, v, z, k));
+		return hipSuccess;
 	}
 };
 
@@ -576,6 +666,27 @@ struct Gmres {
 
 using dgx::Ctx;
 
+extern "C" dgx_status dgx_precond_apply(dgx_ctx* h, int precond, const double* u, const double* inv_dt,
+                                        const double* v, double* z)
+{
+	if (!h || !u || !inv_dt || !v || !z || v == z) return DGX_ERR_ARG;
+	Ctx& c = h->c;
+	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (c.d != 2 || c.cfg.pde != DGX_PDE_EULER) return DGX_ERR_UNSUPPORTED;
+	dgx::Precond* P = nullptr;
+	if (precond == DGX_PC_NONE) P = new dgx::PrecondIdentity();
+	else if (precond == DGX_PC_BLOCK_JACOBI) P = new dgx::PrecondBlockJacobi();
+	else if (precond == DGX_PC_BLOCK_SGS) P = new dgx::PrecondBlockSGS();
+	else if (precond == DGX_PC_LEARNED) P = dgx::make_learned_precond(c);
+	if (!P) return precond == DGX_PC_LEARNED ? DGX_ERR_UNSUPPORTED : DGX_ERR_ARG;
+	hipError_t e = P->setup(c, u, inv_dt);
+	if (e == hipSuccess) e = P->apply(c, v, z);
+	if (e == hipSuccess) e = hipStreamSynchronize(c.stream);   // the preconditioner's buffers go with it
+	delete P;
+	if (e == hipErrorInvalidValue) return DGX_ERR_UNSUPPORTED;
+	return e == hipSuccess ? DGX_OK : DGX_ERR_HIP;
+}
+
 extern "C" dgx_status dgx_solve_implicit(dgx_ctx* h, double* u, const dgx_implicit_params* prm, dgx_implicit_stats* st)
 {
 	if (!h || !u || !prm || !st) return DGX_ERR_ARG;
@@ -608,6 +719,7 @@ extern "C" dgx_status dgx_solve_implicit(dgx_ctx* h, double* u, const dgx_implic
 			bj->cap_capacity = prm->capture_capacity;
 			P = bj;
 		}
+		else if (prm->precond == DGX_PC_BLOCK_SGS) P = new dgx::PrecondBlockSGS();
 		else if (prm->precond == DGX_PC_LEARNED) {
 			P = dgx::make_learned_precond(c);
 			if (prm->learned_fallback_iters > 0) Pfb = new dgx::PrecondBlockJacobi();

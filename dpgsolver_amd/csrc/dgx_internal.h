@@ -56,6 +56,8 @@ struct Ctx {
 	ElemFace* elem_faces = nullptr; // E*2d
 	int* order = nullptr;           // E: element processing order (Morton order of centroids)
 	double* hmin = nullptr;         // E: minimum straight edge length (compute_min_length_measure)
+	int* color_elems = nullptr;     // E: elements grouped by colour (no two face neighbours share a colour)
+	std::vector<int> color_off;     // n_colors + 1 offsets into color_elems
 	dgx_mlp* mlp = nullptr;         // attached learned preconditioner (not owned)
 	double* metrics_vc = nullptr;   // E*d*d*Nc  (m[(i + d*j)*Nc + n] = J dxi_j/dx_i)
 	float* metrics_vc32 = nullptr;
@@ -116,7 +118,10 @@ hipError_t launch_block_inv_apply(hipStream_t s, const double* invT, const doubl
 hipError_t launch_block_lu(hipStream_t s, double* blocks, int* piv, int E, int nb);
 hipError_t launch_block_lu_solve(hipStream_t s, const double* lu, const int* piv, double* x, int E, int nb);
 hipError_t launch_diag_col(Ctx& c, const double* u, double* blocks, int col);
+// column col < nv * N^(d-1) of the compact face blocks: blocks[((e * 2d + lf) * nfv + col) * nfv + row], nfv =
+// nv * N^(d-1), rows and columns on the face nodes (see MODE_OFFD)
 hipError_t launch_offd_col(Ctx& c, const double* u, double* blocks, int col);
+hipError_t launch_expand_face_blocks(Ctx& c, const double* cT, double* full);   // compact -> full nb x nb (2-D)
 // jv = J v, or J v - mass_scale[e] M_e v when mass_scale is not NULL (the pseudo-time Newton operator)
 hipError_t launch_jvp(Ctx& c, const double* u, const double* v, double* jv, const double* mass_scale = nullptr);
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, int n, unsigned long long v);

@@ -180,7 +180,7 @@ hipError_t launch_geometry(Ctx& c)
 	DGX_GEOM(3, 2) DGX_GEOM(3, 3) DGX_GEOM(3, 4) DGX_GEOM(3, 5)
 #undef DGX_GEOM
 	hipError_t e2 = hipStreamSynchronize(c.stream);
-	hipFree(mvm);
+	(void)hipFree(mvm);
 	return err != hipSuccess ? err : e2;
 }
 
@@ -609,6 +609,45 @@ hipError_t launch_block_inv_apply(hipStream_t s, const double* invT, const doubl
 {
 	if (E == 0) return hipSuccess;
 	hipLaunchKernelGGL(k_block_inv_apply, dim3(E), dim3(64), sizeof(double) * nb, s, invT, v, z, nb);
+	return hipGetLastError();
+}
+
+// 2-D face-node map: node of local face lf at face position a (faces r-, r+, s-, s+)
+__device__ inline int face_node_2d(int lf, int a, int N)
+{
+	const int fix = (lf & 1) ? N - 1 : 0;
+	return (lf >> 1) == 0 ? fix + N * a : a + N * fix;
+}
+
+// Compact transposed 2-D face blocks (nfv = nv N face-node rows and columns, [col][row], as the MODE_OFFD
+// launches store them) scattered into full row-major nb x nb blocks, zero off the face nodes and on
+// boundary faces.  One workgroup per (element, local face).
+__global__ void k_expand_face_blocks(const double* __restrict__ cT, const ElemFace* __restrict__ ef, double* full,
+                                     int N, int nv)
+{
+	const int Np = N * N, nb = Np * nv, nfv = nv * N;
+	const size_t b = blockIdx.x;
+	const int lf = (int)(b % 4);
+	const ElemFace f = ef[b];
+	double* out = full + b * nb * nb;
+	for (int k = threadIdx.x; k < nb * nb; k += blockDim.x) out[k] = 0.0;
+	__syncthreads();
+	if (f.en < 0) return;
+	const int fn = f.packed & 15;
+	for (int k = threadIdx.x; k < nfv * nfv; k += blockDim.x) {
+		const int col = k / nfv, row = k - col * nfv;
+		const int v = row / N, w = col / N;
+		out[(size_t)(v * Np + face_node_2d(lf, row - v * N, N)) * nb + w * Np + face_node_2d(fn, col - w * N, N)] =
+			cT[b * nfv * nfv + k];
+	}
+}
+
+hipError_t launch_expand_face_blocks(Ctx& c, const double* cT, double* full)
+{
+	if (c.d != 2) return hipErrorInvalidValue;
+	if (c.E == 0) return hipSuccess;
+	hipLaunchKernelGGL(k_expand_face_blocks, dim3((unsigned)(c.E * 4)), dim3(256), 0, c.stream, cT, c.elem_faces, full,
+	                   c.N, c.nv);
 	return hipGetLastError();
 }
 

@@ -36,10 +36,14 @@ __device__ inline Dual operator*(double a, Dual b) { return Dual(a * b.v, a * b.
 // The constant part F(c_e) drops out exactly: its volume and face contributions cancel per test
 // function (free-stream preservation), so the result is the full residual, and the FP32 rounding
 // acts only on the small within-element variation instead of on the whole flux.
-// MODE_OFFD: column `col` of the face (off-diagonal) blocks dR_e/du_n for every neighbour n: the seed
-// sits on the neighbour's local dof `col`, the element's own state carries none, so only the face flux
-// of the face shared with n has a derivative; the faces are lifted one at a time into
-// out[(e * 2d + lf) * nb * nb + row * nb + col].
+// MODE_OFFD: column `col` of the face (off-diagonal) blocks dR_e/du_n for every neighbour n.  The
+// neighbour's state enters R_e only through its trace on the shared face, and the face integral tests
+// only with the element's own face nodes (GLL nodal basis), so each block is dense on NFV = nv * N^(d-1)
+// face-node rows and columns and zero elsewhere.  The seed sits on the neighbour's face node
+// col = v * NF + i (i in the neighbour's face ordering face_vol_node(fn, i)); the element's own state
+// carries none, so only the face flux of the face shared with n has a derivative; the faces are lifted
+// one at a time and the face-node rows rho = v * NF + a (node face_vol_node(lf, a)) are stored
+// transposed: out[((e * 2d + lf) * NFV + col) * NFV + rho].
 enum { MODE_JVP = 5, MODE_DIAG = 6, MODE_PERT = 7, MODE_OFFD = 8 };
 
 template <typename S, typename R>
@@ -302,7 +306,6 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 	constexpr int NFACE = 2 * D, NT = Np;
 	constexpr int EPB = rhs_epb<S, D, N, NV>();
 	constexpr bool MINV = (MODE == MODE_RHS_MINV);
-	using Real = typename RealOf<S>::type;
 
 	__shared__ R sI[N * N], sIT[N * N], sDhT[N * N], sIinv[N * N], sLift[2][N], sW[N];
 	__shared__ int sPerm[8][NF];
@@ -380,8 +383,7 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 					to[v * NF] = U[v * Np + vown];
 					if constexpr (PERT) tn[v * NF] = S(a.u64[gi + v * Np] - sC[sl][v]);
 					else if constexpr (MODE == MODE_OFFD)
-						tn[v * NF] = S(a.u[gi + v * Np],
-						               (face_vol_node<D, N>(ef.packed & 15, i) + v * Np == a.col) ? 1.0 : 0.0);
+						tn[v * NF] = S(a.u[gi + v * Np], (v * NF + i == a.col) ? 1.0 : 0.0);
 					else if constexpr (MODE == MODE_JVP) tn[v * NF] = S(a.u[gi + v * Np], a.v[gi + v * Np]);
 					else tn[v * NF] = S(a.u[gi + v * Np]);
 				}
@@ -595,7 +597,7 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 		}
 	}
 	if constexpr (MODE == MODE_OFFD) {
-		constexpr int NB = NV * Np;
+		constexpr int NFV = NV * NF;
 		for (int lf = 0; lf < NFACE; ++lf) {
 			if (valid) {
 #pragma unroll
@@ -605,8 +607,11 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 #pragma unroll
 			for (int dir = 0; dir < D; ++dir) sweep_inplace<S, R, D, N, NV, NT>(U, sIT, dir, t);
 			if (valid) {
-				double* ob = a.out + ((size_t)e * NFACE + lf) * NB * NB;
-				for (int i = t; i < NB; i += NT) ob[(size_t)i * NB + a.col] = U[i].d;
+				double* ob = a.out + (((size_t)e * NFACE + lf) * NFV + a.col) * NFV;
+				for (int r = t; r < NFV; r += NT) {
+					const int v = r / NF;
+					ob[r] = U[v * Np + face_vol_node<D, N>(lf, r - v * NF)].d;
+				}
 			}
 			__syncthreads();
 		}

@@ -141,7 +141,9 @@ dgx_status dgx_convert_f32_f64(dgx_ctx* ctx, const float* src, double* dst, long
  *  compute_face_rlhs_T.c:163-220, compute_CFL_ramping solve_dg.c:223-248 with mass_scale = 1/dt). */
 dgx_status dgx_assemble_diag_blocks(dgx_ctx* ctx, const double* u, const double* mass_scale, double* blocks);
 /* Face (off-diagonal) blocks: blocks[(e * 2d + lf) * nb^2 + row * nb + col] = dR_e[row] / du_n[col], n the
- * neighbour across local face lf (col in n's local numbering var*Np+node); zero for boundary faces.
+ * neighbour across local face lf (col in n's local numbering var*Np+node); zero for boundary faces.  Only the
+ * nv*N face-node rows and columns are non-zero (the neighbour enters through its trace, the face integral tests
+ * with the element's face nodes); they are computed compactly, nv*N dual-number launches, and scattered.
  * These are lhs_lr / lhs_rl of constructor_lhs_f_1_T (compute_face_rlhs_T.c:163-220, added at
  * compute_face_rlhs_dg.c:198-213); with the diagonal blocks they are the whole block-sparse Jacobian. */
 dgx_status dgx_assemble_face_blocks(dgx_ctx* ctx, const double* u, double* blocks);
@@ -170,7 +172,17 @@ dgx_status dgx_block_inv_apply(dgx_ctx* ctx, const double* inv, const double* x,
 /* Steady solve by pseudo-time Newton-GMRES (solve_implicit, solve_implicit.c:95-110, 232-281, 393-470).
  * Names mirror the reference's test_case.data keys (cfl_initial, exit_tol_i, exit_ratio_i) and the PETSc
  * options file (GMRES restart 60, rtol 1e-4).  u (device) is updated in place. */
-enum { DGX_PC_NONE = 0, DGX_PC_BLOCK_JACOBI = 1, DGX_PC_LEARNED = 2 };
+/* Preconditioners of the GMRES solves.  DGX_PC_BLOCK_JACOBI: exact inverses of the element diagonal blocks of
+ * dR/du - M/dt.  DGX_PC_BLOCK_SGS (nb <= 64): one forward and one backward multicolour block Gauss-Seidel sweep
+ * over the diagonal and face blocks from z = 0, the coupled-block counterpart of the reference's default
+ * PETSc ILU(1) with RCM ordering (external/petsc/options_files/petsc_options_gmres_default.txt:6-8), which
+ * factors the assembled matrix sequentially.  DGX_PC_LEARNED: the MLP below. */
+enum { DGX_PC_NONE = 0, DGX_PC_BLOCK_JACOBI = 1, DGX_PC_LEARNED = 2, DGX_PC_BLOCK_SGS = 3 };
+/* z = P^-1 v for one preconditioner set up at state u with the pseudo-time scale inv_dt (E doubles, as
+ * dgx_cfl_dt writes it): the operator that dgx_solve_implicit applies inside GMRES, exposed for callers that
+ * run their own Krylov solver.  All pointers device, v and z distinct; synchronous. */
+dgx_status dgx_precond_apply(dgx_ctx* ctx, int precond, const double* u, const double* inv_dt, const double* v,
+                             double* z);
 #define DGX_HISTORY_MAX 256
 typedef struct dgx_implicit_params {
 	double cfl_initial;   /* cfl_initial */
