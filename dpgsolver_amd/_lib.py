@@ -22,7 +22,8 @@ _ERRORS = {-1: "DGX_ERR_ARG", -2: "DGX_ERR_HIP", -3: "DGX_ERR_UNSUPPORTED", -4: 
 EXPORTS = [
     "dgx_version", "dgx_device_count", "dgx_create", "dgx_destroy", "dgx_set_stream", "dgx_synchronize",
     "dgx_sizes", "dgx_upload_mesh", "dgx_get_geometry", "dgx_compute_rhs", "dgx_compute_rhs_f32", "dgx_compute_rhs_mixed",
-    "dgx_compute_rhs_explicit", "dgx_apply_minv", "dgx_enforce_positivity", "dgx_ssp_rk3_step", "dgx_max_abs",
+    "dgx_compute_rhs_explicit", "dgx_apply_minv", "dgx_enforce_positivity", "dgx_ssp_rk3_step",
+    "dgx_graph_create_rhs", "dgx_graph_create_rk3", "dgx_graph_launch", "dgx_graph_sync", "dgx_graph_destroy", "dgx_max_abs",
     "dgx_convert_f64_f32", "dgx_convert_f32_f64", "dgx_assemble_diag_blocks", "dgx_jacobian_vector",
     "dgx_assemble_face_blocks", "dgx_newton_operator", "dgx_block_lu_factor", "dgx_block_lu_solve", "dgx_block_inverse", "dgx_block_inv_apply", "dgx_cfl_dt", "dgx_mass_sub", "dgx_precond_apply", "dgx_solve_implicit",
     "dgx_mesh_read_gmsh", "dgx_mesh_free", "dgx_restart_write_gmsh", "dgx_restart_read_solution",
@@ -100,6 +101,11 @@ def load():
         "dgx_apply_minv": [vp, vp],
         "dgx_enforce_positivity": [vp, vp],
         "dgx_ssp_rk3_step": [vp, vp, C.c_double, C.c_int, dp],
+        "dgx_graph_create_rhs": [vp, C.c_int, vp, vp, C.POINTER(vp)],
+        "dgx_graph_create_rk3": [vp, vp, C.c_double, C.c_int, C.POINTER(vp)],
+        "dgx_graph_launch": [vp, C.c_int],
+        "dgx_graph_sync": [vp, dp],
+        "dgx_graph_destroy": [vp],
         "dgx_max_abs": [vp, vp, C.c_longlong, dp],
         "dgx_convert_f64_f32": [vp, vp, vp, C.c_longlong],
         "dgx_convert_f32_f64": [vp, vp, vp, C.c_longlong],
@@ -225,6 +231,18 @@ class Context:
         check("dgx_ssp_rk3_step", self.L.dgx_ssp_rk3_step(self.h, vp(u), dt, int(positivity), C.byref(mx)))
         return mx.value
 
+    def graph_rhs(self, kind: int, u: int, r: int) -> "Graph":
+        """One residual evaluation recorded as a HIP graph (kind GRAPH_RHS_FP64 / _FP32 / _MIXED)."""
+        g = vp()
+        check("dgx_graph_create_rhs", self.L.dgx_graph_create_rhs(self.h, kind, vp(u), vp(r), C.byref(g)))
+        return Graph(self, g)
+
+    def graph_rk3(self, u: int, dt: float, positivity: bool = True) -> "Graph":
+        """One SSP-RK3 step (as ssp_rk3_step) recorded as a HIP graph."""
+        g = vp()
+        check("dgx_graph_create_rk3", self.L.dgx_graph_create_rk3(self.h, vp(u), dt, int(positivity), C.byref(g)))
+        return Graph(self, g)
+
     def max_abs(self, x: int, n: int) -> float:
         out = C.c_double()
         check("dgx_max_abs", self.L.dgx_max_abs(self.h, vp(x), n, C.byref(out)))
@@ -294,6 +312,31 @@ class Context:
 
 
 MLP_LOWRANK, MLP_KRON = 0, 1
+
+
+GRAPH_RHS_FP64, GRAPH_RHS_FP32, GRAPH_RHS_MIXED = 0, 1, 2
+
+
+class Graph:
+    """A recorded launch sequence (dgx_graph_*); keeps its context alive."""
+
+    def __init__(self, ctx: Context, h):
+        self.ctx, self.L, self.h = ctx, ctx.L, h
+
+    def launch(self, n: int = 1):
+        check("dgx_graph_launch", self.L.dgx_graph_launch(self.h, n))
+
+    def sync(self) -> float:
+        mx = C.c_double()
+        check("dgx_graph_sync", self.L.dgx_graph_sync(self.h, C.byref(mx)))
+        return mx.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.dgx_graph_destroy(self.h)
+            self.h = None
+
+    __del__ = close
 
 
 class MLP:

@@ -315,6 +315,141 @@ dgx_status dgx_ssp_rk3_step(dgx_ctx* h, double* u, double dt, int positivity, do
 	return st;
 }
 
+}  // extern "C"
+
+struct dgx_graph {
+	dgx_ctx* ctx = nullptr;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	bool rk3 = false, positivity = false;
+};
+
+namespace {
+
+// Records what `record` launches on the context's stream into g (a private non-blocking stream stands in for
+// the context's stream during the capture, so capture also works for a context on the null stream).
+template <typename F>
+dgx_status capture_graph(dgx_ctx* h, dgx_graph* g, F record)
+{
+	Ctx& c = h->c;
+	hipStream_t cs = nullptr;
+	DGX_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+	const hipStream_t saved = c.stream;
+	c.stream = cs;
+	hipGraph_t graph = nullptr;
+	hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+	if (e == hipSuccess) {
+		const hipError_t er = record();
+		e = hipStreamEndCapture(cs, &graph);
+		if (e == hipSuccess) e = er;
+	}
+	c.stream = saved;
+	(void)hipStreamDestroy(cs);
+	if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+	if (e != hipSuccess) {
+		if (graph) (void)hipGraphDestroy(graph);
+		return e == hipErrorInvalidValue ? DGX_ERR_UNSUPPORTED : DGX_ERR_HIP;
+	}
+	g->graph = graph;
+	g->ctx = h;
+	return DGX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+dgx_status dgx_graph_create_rhs(dgx_ctx* h, int kind, const void* u, void* r, dgx_graph** out)
+{
+	if (!h || !u || !r || !out || kind < DGX_GRAPH_RHS_FP64 || kind > DGX_GRAPH_RHS_MIXED) return DGX_ERR_ARG;
+	Ctx& c = h->c;
+	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (kind == DGX_GRAPH_RHS_MIXED && c.cfg.pde != DGX_PDE_EULER) return DGX_ERR_UNSUPPORTED;
+	dgx_graph* g = new (std::nothrow) dgx_graph();
+	if (!g) return DGX_ERR_HIP;
+	const dgx_status st = capture_graph(h, g, [&]() -> hipError_t {
+		if (kind == DGX_GRAPH_RHS_FP64)
+			return dgx::launch_rhs64(c, static_cast<const double*>(u), static_cast<double*>(r), dgx::MODE_RHS, nullptr);
+		if (kind == DGX_GRAPH_RHS_FP32)
+			return dgx::launch_rhs32(c, static_cast<const float*>(u), static_cast<float*>(r));
+		return dgx::launch_rhs_mixed(c, static_cast<const double*>(u), static_cast<float*>(r));
+	});
+	if (st != DGX_OK) {
+		delete g;
+		return st;
+	}
+	*out = g;
+	return DGX_OK;
+}
+
+dgx_status dgx_graph_create_rk3(dgx_ctx* h, double* u, double dt, int positivity, dgx_graph** out)
+{
+	if (!h || !u || !out) return DGX_ERR_ARG;
+	Ctx& c = h->c;
+	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	const size_t n = ndof(c);
+	if (!c.u0) DGX_TRY(hipMalloc(&c.u0, sizeof(double) * n));     // stage buffers exist before the capture
+	if (!c.rtmp) DGX_TRY(hipMalloc(&c.rtmp, sizeof(double) * n));
+	dgx_graph* g = new (std::nothrow) dgx_graph();
+	if (!g) return DGX_ERR_HIP;
+	g->rk3 = true;
+	g->positivity = positivity && c.cfg.pde == DGX_PDE_EULER;
+	const dgx_status st = capture_graph(h, g, [&]() -> hipError_t {
+		for (int rk = 0; rk < 3; ++rk) {
+			DGX_HIP_CHECK(dgx::launch_fill_u64(c.stream, c.dmax, 1, 0ull));
+			DGX_HIP_CHECK(dgx::launch_rhs64(c, u, c.rtmp, dgx::MODE_RHS_MINV, c.dmax));
+			DGX_HIP_CHECK(dgx::launch_rk_stage(c, u, c.rtmp, dt, rk));
+			if (g->positivity) DGX_HIP_CHECK(dgx::launch_positivity(c, u, c.dstatus));
+		}
+		return hipSuccess;
+	});
+	if (st != DGX_OK) {
+		delete g;
+		return st;
+	}
+	// the limiter flag accumulates over replays until dgx_graph_sync reads and clears it
+	if (g->positivity && dgx::launch_fill_i32(c.stream, c.dstatus, 1, 0) != hipSuccess) {
+		(void)dgx_graph_destroy(g);
+		return DGX_ERR_HIP;
+	}
+	*out = g;
+	return DGX_OK;
+}
+
+dgx_status dgx_graph_launch(dgx_graph* g, int n)
+{
+	if (!g || !g->exec || n < 0) return DGX_ERR_ARG;
+	for (int i = 0; i < n; ++i) DGX_TRY(hipGraphLaunch(g->exec, g->ctx->c.stream));
+	return DGX_OK;
+}
+
+dgx_status dgx_graph_sync(dgx_graph* g, double* max_abs_r)
+{
+	if (!g || !g->exec) return DGX_ERR_ARG;
+	Ctx& c = g->ctx->c;
+	DGX_TRY(hipStreamSynchronize(c.stream));
+	if (!g->rk3) return DGX_OK;
+	dgx_status st = DGX_OK;
+	if (max_abs_r) st = read_max(c, max_abs_r);
+	if (g->positivity) {
+		int bad = 0;
+		DGX_TRY(hipMemcpyAsync(&bad, c.dstatus, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+		DGX_TRY(dgx::launch_fill_i32(c.stream, c.dstatus, 1, 0));
+		DGX_TRY(hipStreamSynchronize(c.stream));
+		if (bad) return DGX_ERR_POSITIVITY;
+	}
+	return st;
+}
+
+dgx_status dgx_graph_destroy(dgx_graph* g)
+{
+	if (!g) return DGX_OK;
+	if (g->exec) (void)hipGraphExecDestroy(g->exec);
+	if (g->graph) (void)hipGraphDestroy(g->graph);
+	delete g;
+	return DGX_OK;
+}
+
 dgx_status dgx_max_abs(dgx_ctx* h, const double* x, long long n, double* out)
 {
 	if (!h || !x || !out || n < 0) return DGX_ERR_ARG;

@@ -128,6 +128,25 @@ dgx_status dgx_enforce_positivity(dgx_ctx* ctx, double* u);
  * after every stage when enforce_positivity != 0; *max_abs_r (HOST, may be NULL) receives
  * max|M^-1 R| of the last stage.  Synchronous when max_abs_r is given. */
 dgx_status dgx_ssp_rk3_step(dgx_ctx* ctx, double* u, double dt, int enforce_positivity, double* max_abs_r);
+
+/* HIP graphs of the launch sequences above, recorded once and replayed with one hipGraphLaunch each: the
+ * explicit time loop (solve_explicit.c:78-118 calling time_step_ssp_rk_33 every step) and repeated residual
+ * evaluations then cost no per-kernel launch overhead.  A graph binds the pointers and dt it was recorded
+ * with; it replays on the context's stream (dgx_set_stream) as of dgx_graph_launch. */
+typedef struct dgx_graph dgx_graph;
+enum { DGX_GRAPH_RHS_FP64 = 0, DGX_GRAPH_RHS_FP32 = 1, DGX_GRAPH_RHS_MIXED = 2 };
+/* one residual evaluation: kind DGX_GRAPH_RHS_FP64 (dgx_compute_rhs: double u, r), _FP32 (dgx_compute_rhs_f32:
+ * float u, r) or _MIXED (dgx_compute_rhs_mixed: double u, float r) */
+dgx_status dgx_graph_create_rhs(dgx_ctx* ctx, int kind, const void* u, void* r, dgx_graph** g);
+/* one dgx_ssp_rk3_step(ctx, u, dt, enforce_positivity, ...) */
+dgx_status dgx_graph_create_rk3(dgx_ctx* ctx, double* u, double dt, int enforce_positivity, dgx_graph** g);
+/* n replays, asynchronous (n >= 0) */
+dgx_status dgx_graph_launch(dgx_graph* g, int n);
+/* waits for the replays; for an RK3 graph *max_abs_r (HOST, may be NULL) receives max|M^-1 R| of the last stage
+ * of the last replay, and DGX_ERR_POSITIVITY is returned if a limiter failed in any replay since the previous
+ * dgx_graph_sync (dgx_ssp_rk3_step and dgx_enforce_positivity on the same context clear that flag too) */
+dgx_status dgx_graph_sync(dgx_graph* g, double* max_abs_r);
+dgx_status dgx_graph_destroy(dgx_graph* g);
 /* max |x|_inf over n doubles (HOST result). Synchronous. */
 dgx_status dgx_max_abs(dgx_ctx* ctx, const double* x, long long n, double* out);
 /* FP64 <-> FP32 state conversion on the device. */
