@@ -221,8 +221,15 @@ def main():
         torch.cuda.synchronize()
         return start.elapsed_time(stop) / k   # ms per launch
 
-    # each evaluation is one replay of a HIP graph holding the residual launch (dgx_graph_create_rhs)
+    # launch shapes from the autotuner (one workgroup per element block, or a persistent grid), then each
+    # evaluation is one replay of a HIP graph holding the residual launch (dgx_graph_create_rhs)
     rmx = torch.empty_like(u32)
+    tuned = {}
+    for name, kind, ui, ri in (("fp32", _lib.GRAPH_RHS_FP32, u32, r32), ("fp64", _lib.GRAPH_RHS_FP64, u64, r64),
+                               ("mixed", _lib.GRAPH_RHS_MIXED, u64, rmx)):
+        grid, ms = ctx.autotune_rhs(kind, ui.data_ptr(), ri.data_ptr())
+        tuned[name] = {"grid": grid, "ms": round(ms, 5)}
+    print(f"autotuned grids: {tuned}", file=sys.stderr, flush=True)
     g32 = ctx.graph_rhs(_lib.GRAPH_RHS_FP32, u32.data_ptr(), r32.data_ptr())
     g64 = ctx.graph_rhs(_lib.GRAPH_RHS_FP64, u64.data_ptr(), r64.data_ptr())
     gmx = ctx.graph_rhs(_lib.GRAPH_RHS_MIXED, u64.data_ptr(), rmx.data_ptr())
@@ -306,6 +313,7 @@ def main():
                                "frac": round(achmx / HBM_PEAK_GBS, 4),
                                "traffic": _traffic_from_profiles("rhs_mixed")}},
         "fp32_ms_per_step_eager_launch": round(ms32_eager, 5),
+        "launch_grids": tuned,
         "fp32_rel_err_vs_fp64": err32,
         "fp32_input_rounding_rel_err": err_in,
         "fp64_free_stream_max_abs": free_stream,

@@ -23,7 +23,8 @@ EXPORTS = [
     "dgx_version", "dgx_device_count", "dgx_create", "dgx_destroy", "dgx_set_stream", "dgx_synchronize",
     "dgx_sizes", "dgx_upload_mesh", "dgx_get_geometry", "dgx_compute_rhs", "dgx_compute_rhs_f32", "dgx_compute_rhs_mixed",
     "dgx_compute_rhs_explicit", "dgx_apply_minv", "dgx_enforce_positivity", "dgx_ssp_rk3_step",
-    "dgx_graph_create_rhs", "dgx_graph_create_rk3", "dgx_graph_launch", "dgx_graph_sync", "dgx_graph_destroy", "dgx_max_abs",
+    "dgx_graph_create_rhs", "dgx_graph_create_rk3", "dgx_graph_launch", "dgx_graph_sync", "dgx_graph_destroy",
+    "dgx_set_rhs_grid", "dgx_autotune_rhs", "dgx_max_abs",
     "dgx_convert_f64_f32", "dgx_convert_f32_f64", "dgx_assemble_diag_blocks", "dgx_jacobian_vector",
     "dgx_assemble_face_blocks", "dgx_newton_operator", "dgx_block_lu_factor", "dgx_block_lu_solve", "dgx_block_inverse", "dgx_block_inv_apply", "dgx_cfl_dt", "dgx_mass_sub", "dgx_precond_apply", "dgx_solve_implicit",
     "dgx_mesh_read_gmsh", "dgx_mesh_free", "dgx_restart_write_gmsh", "dgx_restart_read_solution",
@@ -106,6 +107,8 @@ def load():
         "dgx_graph_launch": [vp, C.c_int],
         "dgx_graph_sync": [vp, dp],
         "dgx_graph_destroy": [vp],
+        "dgx_set_rhs_grid": [vp, C.c_int, C.c_int],
+        "dgx_autotune_rhs": [vp, C.c_int, vp, vp, C.POINTER(C.c_int), dp],
         "dgx_max_abs": [vp, vp, C.c_longlong, dp],
         "dgx_convert_f64_f32": [vp, vp, vp, C.c_longlong],
         "dgx_convert_f32_f64": [vp, vp, vp, C.c_longlong],
@@ -243,6 +246,15 @@ class Context:
         check("dgx_graph_create_rk3", self.L.dgx_graph_create_rk3(self.h, vp(u), dt, int(positivity), C.byref(g)))
         return Graph(self, g)
 
+    def set_rhs_grid(self, kind: int, grid: int):
+        check("dgx_set_rhs_grid", self.L.dgx_set_rhs_grid(self.h, kind, grid))
+
+    def autotune_rhs(self, kind: int, u: int, r: int) -> tuple[int, float]:
+        """Picks the residual launch grid of `kind` (GRAPH_RHS_* or TUNE_RHS_MINV); returns (grid, ms)."""
+        g, ms = C.c_int(), C.c_double()
+        check("dgx_autotune_rhs", self.L.dgx_autotune_rhs(self.h, kind, vp(u), vp(r), C.byref(g), C.byref(ms)))
+        return g.value, ms.value
+
     def max_abs(self, x: int, n: int) -> float:
         out = C.c_double()
         check("dgx_max_abs", self.L.dgx_max_abs(self.h, vp(x), n, C.byref(out)))
@@ -314,7 +326,7 @@ class Context:
 MLP_LOWRANK, MLP_KRON = 0, 1
 
 
-GRAPH_RHS_FP64, GRAPH_RHS_FP32, GRAPH_RHS_MIXED = 0, 1, 2
+GRAPH_RHS_FP64, GRAPH_RHS_FP32, GRAPH_RHS_MIXED, TUNE_RHS_MINV = 0, 1, 2, 3
 
 
 class Graph:

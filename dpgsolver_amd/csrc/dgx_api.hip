@@ -41,6 +41,7 @@ void free_mesh(Ctx& c)
 	release(c.detJ_vc); release(c.detJ_vc32); release(c.nJ_fc); release(c.nJ_fc32); release(c.u0); release(c.rtmp);
 	release(c.color_elems);
 	c.color_off.clear();
+	for (int& g : c.rhs_grid) g = 0;   // launch shapes are tuned per mesh
 	c.mesh_ready = false;
 }
 
@@ -447,6 +448,76 @@ dgx_status dgx_graph_destroy(dgx_graph* g)
 	if (g->exec) (void)hipGraphExecDestroy(g->exec);
 	if (g->graph) (void)hipGraphDestroy(g->graph);
 	delete g;
+	return DGX_OK;
+}
+
+dgx_status dgx_set_rhs_grid(dgx_ctx* h, int kind, int grid)
+{
+	if (!h || kind < DGX_GRAPH_RHS_FP64 || kind > DGX_TUNE_RHS_MINV || grid < 0) return DGX_ERR_ARG;
+	h->c.rhs_grid[kind] = grid;
+	return DGX_OK;
+}
+
+static hipError_t launch_rhs_kind(Ctx& c, int kind, const void* u, void* r)
+{
+	if (kind == dgx::TUNE_RHS64)
+		return dgx::launch_rhs64(c, static_cast<const double*>(u), static_cast<double*>(r), dgx::MODE_RHS, nullptr);
+	if (kind == dgx::TUNE_RHS32) return dgx::launch_rhs32(c, static_cast<const float*>(u), static_cast<float*>(r));
+	if (kind == dgx::TUNE_MIXED) return dgx::launch_rhs_mixed(c, static_cast<const double*>(u), static_cast<float*>(r));
+	return dgx::launch_rhs64(c, static_cast<const double*>(u), static_cast<double*>(r), dgx::MODE_RHS_MINV, nullptr);
+}
+
+dgx_status dgx_autotune_rhs(dgx_ctx* h, int kind, const void* u, void* r, int* grid_out, double* ms_out)
+{
+	if (!h || !u || !r || kind < DGX_GRAPH_RHS_FP64 || kind > DGX_TUNE_RHS_MINV) return DGX_ERR_ARG;
+	Ctx& c = h->c;
+	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (kind == DGX_GRAPH_RHS_MIXED && c.cfg.pde != DGX_PDE_EULER) return DGX_ERR_UNSUPPORTED;
+	int dev = 0, ncu = 0;
+	DGX_TRY(hipGetDevice(&dev));
+	DGX_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+	// one workgroup per element block, then persistent grids, then the first again (the clocks may still be
+	// ramping up during the first measurement)
+	std::vector<int> cand = { 0 };
+	for (int m : { 1, 2, 3, 4, 6, 8, 12, 16 }) cand.push_back(m * ncu);
+	cand.push_back(0);
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+		if (e0) (void)hipEventDestroy(e0);
+		return DGX_ERR_HIP;
+	}
+	constexpr int kReps = 5;
+	const int saved = c.rhs_grid[kind];
+	int best = 0;
+	float best_ms = 0.0f, full_ms = 0.0f;
+	hipError_t err = hipSuccess;
+	for (size_t k = 0; k < cand.size() && err == hipSuccess; ++k) {
+		c.rhs_grid[kind] = cand[k];
+		err = launch_rhs_kind(c, kind, u, r);   // warm-up
+		if (err == hipSuccess) err = hipEventRecord(e0, c.stream);
+		for (int rep = 0; rep < kReps && err == hipSuccess; ++rep) err = launch_rhs_kind(c, kind, u, r);
+		if (err == hipSuccess) err = hipEventRecord(e1, c.stream);
+		if (err == hipSuccess) err = hipEventSynchronize(e1);
+		float ms = 0.0f;
+		if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+		ms /= kReps;
+		if (k == 0) full_ms = best_ms = ms;
+		else if (cand[k] == 0) {
+			full_ms = std::min(full_ms, ms);
+			if (ms < best_ms) { best_ms = ms; best = 0; }
+		} else if (ms < best_ms) { best_ms = ms; best = cand[k]; }
+	}
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	if (err != hipSuccess) {
+		c.rhs_grid[kind] = saved;
+		return DGX_ERR_HIP;
+	}
+	// a persistent grid is kept only when it beats one-workgroup-per-block by more than 1%
+	if (best != 0 && best_ms > 0.99f * full_ms) { best = 0; best_ms = full_ms; }
+	c.rhs_grid[kind] = best;
+	if (grid_out) *grid_out = best;
+	if (ms_out) *ms_out = best_ms;
 	return DGX_OK;
 }
 

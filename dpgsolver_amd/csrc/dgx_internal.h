@@ -59,6 +59,7 @@ struct Ctx {
 	int* color_elems = nullptr;     // E: elements grouped by colour (no two face neighbours share a colour)
 	std::vector<int> color_off;     // n_colors + 1 offsets into color_elems
 	dgx_mlp* mlp = nullptr;         // attached learned preconditioner (not owned)
+	int rhs_grid[4] = { 0, 0, 0, 0 };   // residual launch grids by TUNE_* (0 = one workgroup per element block)
 	double* metrics_vc = nullptr;   // E*d*d*Nc  (m[(i + d*j)*Nc + n] = J dxi_j/dx_i)
 	float* metrics_vc32 = nullptr;
 	double* detJ_vc = nullptr;      // E*Nc
@@ -129,6 +130,8 @@ hipError_t launch_fill_i32(hipStream_t s, int* p, int n, int v);
 
 // residual modes
 enum { MODE_RHS = 0, MODE_RHS_MINV = 1 };
+// residual kernels with a tunable launch grid (the first three match DGX_GRAPH_RHS_*)
+enum { TUNE_RHS64 = 0, TUNE_RHS32 = 1, TUNE_MIXED = 2, TUNE_RHS64_MINV = 3 };
 
 }  // namespace dgx
 

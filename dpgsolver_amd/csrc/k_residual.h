@@ -63,6 +63,7 @@ struct RhsArgs {
 	double b_adv[3];
 	int E;
 	int col;              // MODE_DIAG column (local dof)
+	int grid;             // workgroups of the launch: 0 = one per element block, else a persistent grid
 	unsigned long long* dmax;
 	const double* mass_scale;   // MODE_JVP: out = J v - mass_scale[e] M_e v when not NULL
 };
@@ -302,399 +303,52 @@ constexpr int rhs_epb()
 template <typename S, typename R, int D, int N, int PDE, int FLUX, int MODE>
 __global__ void __launch_bounds__(256) k_rhs(RhsArgs<S, R> a)
 {
-	constexpr int Np = ipow(N, D), NF = ipow(N, D - 1), NV = (PDE == DGX_PDE_EULER) ? D + 2 : 1;
-	constexpr int NFACE = 2 * D, NT = Np;
-	constexpr int EPB = rhs_epb<S, D, N, NV>();
-	constexpr bool MINV = (MODE == MODE_RHS_MINV);
-
-	__shared__ R sI[N * N], sIT[N * N], sDhT[N * N], sIinv[N * N], sLift[2][N], sW[N];
-	__shared__ int sPerm[8][NF];
-	// traces: own half, then the neighbour half which is reused as the flux staging buffer
-	constexpr int TRSZ = NFACE * NV * NF + (NFACE * NF > Np ? NFACE * NF : Np) * NV;
-	__shared__ S sBuf[EPB][NV * Np + TRSZ];   // volume block U, then the face traces TR
-	__shared__ ElemFace sEF[EPB][NFACE];
-	__shared__ unsigned long long sMax;
-	constexpr bool PERT = (MODE == MODE_PERT);
-	__shared__ double sC[EPB][PERT ? NV : 1];   // MODE_PERT reference state c_e
-
-	// XCD-aware schedule: the grid is a multiple of 8 and consecutive workgroups land on different
-	// XCDs, so XCD x takes the x-th contiguous range of the (Morton-ordered) element sequence and
-	// neighbour traces are read from its own L2.
-	const int per_xcd = gridDim.x >> 3;
-	const int lblk = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-	if (lblk * EPB >= a.E) return;  // whole workgroup idle (padding), before any barrier
-	const int slot = threadIdx.x / NT, t = threadIdx.x - slot * NT;
-	const int pos = lblk * EPB + slot;
-	const bool valid = (slot < EPB) && (pos < a.E);
-	const int e = valid ? a.order[pos] : 0;
-	const int sl = slot < EPB ? slot : 0;
-
-	for (int i = threadIdx.x; i < N * N; i += blockDim.x) {
-		sI[i] = a.ops->I[i];
-		sIT[i] = a.ops->IT[i];
-		sDhT[i] = a.ops->DhT[i];
-		sIinv[i] = a.ops->Iinv[i];
-	}
-	for (int i = threadIdx.x; i < N; i += blockDim.x) {
-		sLift[0][i] = a.ops->Lift[0][i];
-		sLift[1][i] = a.ops->Lift[1][i];
-		sW[i] = a.ops->w[i];
-	}
-	for (int i = threadIdx.x; i < 8 * NF; i += blockDim.x) sPerm[i / NF][i % NF] = a.ops->perm[i / NF][i % NF];
-	if (threadIdx.x == 0) sMax = 0ull;
-	if (valid && t < NFACE) sEF[sl][t] = a.ef[(size_t)e * NFACE + t];
-
-	S* U = sBuf[sl];
-	S* TR = U + NV * Np;
-	S* BUF = TR + NFACE * NV * NF;   // neighbour-trace half, reused for the flux components
-	constexpr bool PB = (NV * ipow(N, D - 1) <= NT);   // one volume pencil per thread
-	int pb[3] = { -1, -1, -1 };
-	if constexpr (PB) {
-#pragma unroll
-		for (int dir = 0; dir < D; ++dir) pb[dir] = pencil_base<D, N, NV>(t, dir);
-	}
-	const size_t ebase = (size_t)e * NV * Np;
-	if (valid) {
-		if constexpr (PERT) {
-			if (t < NV) sC[sl][t] = a.u64[ebase + (size_t)t * Np];
-		}
-		for (int i = t; i < NV * Np; i += NT) {
-			if constexpr (PERT) U[i] = S(a.u64[ebase + i] - a.u64[ebase + (size_t)(i / Np) * Np]);
-			else if constexpr (MODE == MODE_JVP) U[i] = S(a.u[ebase + i], a.v[ebase + i]);
-			else if constexpr (MODE == MODE_DIAG) U[i] = S(a.u[ebase + i], i == a.col ? 1.0 : 0.0);
-			else U[i] = S(a.u[ebase + i]);
-		}
-	}
-	__syncthreads();
-
-	// ---- traces at the GLL face nodes: own (from LDS) and neighbour (gathered from HBM);
-	//      one (face, node) item per thread, the variable loop reuses the address arithmetic
-	if (valid) {
-		for (int item = t; item < NFACE * NF; item += NT) {
-			const int lf = item / NF, i = item - lf * NF;
-			const int vown = face_vol_node<D, N>(lf, i);
-			const ElemFace ef = sEF[sl][lf];
-			S* to = TR + lf * NV * NF + i;
-			S* tn = TR + (NFACE + lf) * NV * NF + i;
-			if (ef.en >= 0) {
-				const size_t gi = (size_t)ef.en * NV * Np + face_vol_node<D, N>(ef.packed & 15, i);
-#pragma unroll
-				for (int v = 0; v < NV; ++v) {
-					to[v * NF] = U[v * Np + vown];
-					if constexpr (PERT) tn[v * NF] = S(a.u64[gi + v * Np] - sC[sl][v]);
-					else if constexpr (MODE == MODE_OFFD)
-						tn[v * NF] = S(a.u[gi + v * Np], (v * NF + i == a.col) ? 1.0 : 0.0);
-					else if constexpr (MODE == MODE_JVP) tn[v * NF] = S(a.u[gi + v * Np], a.v[gi + v * Np]);
-					else tn[v * NF] = S(a.u[gi + v * Np]);
-				}
-			} else {
-#pragma unroll
-				for (int v = 0; v < NV; ++v) {
-					to[v * NF] = U[v * Np + vown];
-					tn[v * NF] = S(0.0);
-				}
-			}
-		}
-	}
-	__syncthreads();  // traces written by other threads are read by the first sweep
-	// ---- GLL -> GL: volume (D sweeps) and face traces (D-1 sweeps) sharing barrier cycles
-	if constexpr (D == 3 && 2 * NFACE * NV <= NT) {
-		// face traces: one N x N block per thread (in place, private), then the volume sweeps
-		face_block_interp<S, R, N, 2 * NFACE * NV, NT>(TR, sI, t);
-		if constexpr (PB) {
-			sweep_inplace_pb<S, R, N, 0>(U, sI, pb[0]);
-			sweep_inplace_pb<S, R, N, 1>(U, sI, pb[1]);
-			sweep_inplace_pb<S, R, N, 2>(U, sI, pb[2]);
-		} else {
-#pragma unroll
-			for (int dir = 0; dir < D; ++dir) sweep_inplace<S, R, D, N, NV, NT>(U, sI, dir, t);
-		}
-	} else {
-#pragma unroll
-		for (int dir = 0; dir < D; ++dir) {
-			if (dir < D - 1) sweep_inplace2<S, R, D, N, NV, D - 1, 2 * NFACE * NV, NV * Np, NT>(U, sI, dir, t);
-			else sweep_inplace<S, R, D, N, NV, NT>(U, sI, dir, t);
-		}
-	}
-
-	// ---- numerical flux at the face cubature (canonical = left ordering)
-	constexpr int NIT = NFACE * NF;
-	constexpr int IPT = (NIT + NT - 1) / NT;
-	S res[IPT][NV];
-	int dst[IPT];
-#pragma unroll
-	for (int k = 0; k < IPT; ++k) {
-		const int item = t + k * NT;
-		dst[k] = -1;
-		if (valid && item < NIT) {
-			const int lf = item / NF, i = item - lf * NF;
-			const ElemFace ef = sEF[sl][lf];
-			const int side = (ef.packed >> 4) & 1, ord = (ef.packed >> 8) & 7, bc = (ef.packed >> 12) & 15;
-			const S* own = TR + lf * NV * NF;
-			const S* nbr = TR + (NFACE + lf) * NV * NF;
-			const int pi = sPerm[ord][i];
-			S sL[NV], sR[NV];
-			if (side == 0) {
-#pragma unroll
-				for (int v = 0; v < NV; ++v) { sL[v] = own[v * NF + i]; sR[v] = nbr[v * NF + pi]; }
-			} else {
-#pragma unroll
-				for (int v = 0; v < NV; ++v) { sL[v] = nbr[v * NF + i]; sR[v] = own[v * NF + pi]; }
-			}
-			const R* np_ = a.nJ + ((size_t)ef.f * NF + i) * D;
-			R nJv[D];
-#pragma unroll
-			for (int k2 = 0; k2 < D; ++k2) nJv[k2] = np_[k2];
-			R J2 = nJv[0] * nJv[0];
-#pragma unroll
-			for (int k2 = 1; k2 < D; ++k2) J2 += nJv[k2] * nJv[k2];
-			const R Jf = tsqrt(J2);
-			const R Jfi = trcp(Jf);
-			S nu[D];
-#pragma unroll
-			for (int k2 = 0; k2 < D; ++k2) nu[k2] = S(nJv[k2] * Jfi);
-			S nnf[NV];
-			if constexpr (PERT && PDE == DGX_PDE_EULER) {
-				// FP64 states c_e + delta, FP64 numerical flux minus the normal flux of c_e
-				double cL[NV], cR[NV], nd[D], fd[NV], cu[D], cn[NV];
-#pragma unroll
-				for (int v = 0; v < NV; ++v) {
-					cL[v] = sC[sl][v] + (double)sL[v];
-					cR[v] = sC[sl][v] + (double)sR[v];
-				}
-#pragma unroll
-				for (int k2 = 0; k2 < D; ++k2) nd[k2] = (double)nJv[k2] / (double)Jf;
-				if (ef.en < 0) bc_state<double, D>(bc, cL, nd, a.bc_prim, cR);
-				if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<double, D>(cL, cR, nd, fd);
-				else numflux_roe<double, D>(cL, cR, nd, fd);
-				const double* c0 = sC[sl];
-				const double ci = 1.0 / c0[0];
-				double V2c = 0.0;
-#pragma unroll
-				for (int k2 = 0; k2 < D; ++k2) { cu[k2] = c0[1 + k2] * ci; V2c += cu[k2] * cu[k2]; }
-				const double pc = kGM1 * (c0[D + 1] - 0.5 * c0[0] * V2c);
-				euler_normal_flux<double, D>(c0, cu, pc, nd, cn);
-#pragma unroll
-				for (int v = 0; v < NV; ++v) nnf[v] = S(fd[v] - cn[v]);
-			} else if constexpr (PDE == DGX_PDE_EULER) {
-				if (ef.en < 0) bc_state<S, D>(bc, sL, nu, a.bc_prim, sR);
-				if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<S, D>(sL, sR, nu, nnf);
-				else numflux_roe<S, D>(sL, sR, nu, nnf);
-			} else {
-				double bn = a.b_adv[0] * rv(nu[0]);
-#pragma unroll
-				for (int k2 = 1; k2 < D; ++k2) bn += a.b_adv[k2] * rv(nu[k2]);
-				nnf[0] = (bn > kEps) ? R(bn) * sL[0] : R(bn) * sR[0];
-			}
-#pragma unroll
-			for (int v = 0; v < NV; ++v) res[k][v] = (side == 0) ? Jf * nnf[v] : R(-1.0) * (Jf * nnf[v]);
-			dst[k] = lf * NV * NF + (side == 0 ? i : pi);
-		}
-	}
-	__syncthreads();
-#pragma unroll
-	for (int k = 0; k < IPT; ++k)
-		if (dst[k] >= 0) {
-#pragma unroll
-			for (int v = 0; v < NV; ++v) TR[dst[k] + v * NF] = res[k][v];
-		}
-
-	// ---- physical flux at the volume cubature, contravariant and weighted, kept in registers
-	S fr[D][NV];
-	double vgl[(MODE == MODE_JVP) ? NV : 1];   // MODE_JVP: (Ibar v) at this node, for the mass term
-	if (valid && MODE != MODE_OFFD) {
-		const int q = t;
-		S s[NV];
-#pragma unroll
-		for (int v = 0; v < NV; ++v) s[v] = U[v * Np + q];
-		if constexpr (MODE == MODE_JVP) {
-#pragma unroll
-			for (int v = 0; v < NV; ++v) vgl[v] = s[v].d;
-		}
-		R m[D * D];
-		const R* mp = a.met + (size_t)e * D * D * Np;
-#pragma unroll
-		for (int i = 0; i < D * D; ++i) m[i] = mp[i * Np + q];
-		const int c0 = q % N, c1 = (q / N) % N, c2 = (D == 3) ? q / (N * N) : 0;
-		R wq = sW[c0] * sW[c1];
-		if (D == 3) wq = wq * sW[c2];
-		if constexpr (PERT && PDE == DGX_PDE_EULER) {
-			// physical fluxes of c_e + delta and of c_e in FP64, their difference contracted in FP32
-			double cs[NV], fu[D * NV], fc[D * NV];
-#pragma unroll
-			for (int v = 0; v < NV; ++v) cs[v] = sC[sl][v] + (double)s[v];
-			euler_flux<double, D>(cs, fu);
-			euler_flux<double, D>(sC[sl], fc);
-#pragma unroll
-			for (int v = 0; v < NV; ++v) {
-				S df[D];
-#pragma unroll
-				for (int k = 0; k < D; ++k) df[k] = S(fu[k + D * v] - fc[k + D * v]);
-#pragma unroll
-				for (int d0 = 0; d0 < D; ++d0) {
-					S acc = m[0 + D * d0] * df[0];
-#pragma unroll
-					for (int k = 1; k < D; ++k) acc += m[k + D * d0] * df[k];
-					fr[d0][v] = wq * acc;
-				}
-			}
-		} else if constexpr (PDE == DGX_PDE_EULER) {
-			euler_contravariant_flux<S, R, D>(s, m, wq, fr);
-		} else {
-#pragma unroll
-			for (int d0 = 0; d0 < D; ++d0) {
-				R bm = R(a.b_adv[0]) * m[0 + D * d0];
-#pragma unroll
-				for (int k = 1; k < D; ++k) bm += R(a.b_adv[k]) * m[k + D * d0];
-				fr[d0][0] = wq * (bm * s[0]);
-			}
-		}
-	}
-
-	// ---- weak divergence in GL space: U = sum_d Dhat_d^T fr_d, one direction at a time through BUF
-	//      (MODE_OFFD: the volume term has no derivative and is skipped)
-#pragma unroll
-	for (int d0 = 0; d0 < (MODE == MODE_OFFD ? 0 : D); ++d0) {
-		if (valid) {
-#pragma unroll
-			for (int v = 0; v < NV; ++v) BUF[v * Np + t] = fr[d0][v];
-		}
-		__syncthreads();
-		if constexpr (PB && D == 3) {
-			if (d0 == 0) sweep_to_pb<S, R, N, 0, false>(BUF, U, sDhT, pb[0]);
-			else if (d0 == 1) sweep_to_pb<S, R, N, 1, true>(BUF, U, sDhT, pb[1]);
-			else sweep_to_pb<S, R, N, 2, true>(BUF, U, sDhT, pb[2]);
-		} else {
-			if (d0 == 0) sweep_to<S, R, D, N, NV, NT, false>(BUF, U, sDhT, 0, t);
-			else sweep_to<S, R, D, N, NV, NT, true>(BUF, U, sDhT, d0, t);
-		}
-	}
-
-	// ---- face lifts (and the mass inverse) at the GL nodes
-	// per-face coefficient Lift_side[c_fd] * W_f and face-node offset, shared by all variables
-	R cf[NFACE];
-	int of[NFACE];
-	R scale = R(1.0);
-	if (valid) {
-		const int q = t;
-		const int cc[3] = { q % N, (q / N) % N, (D == 3) ? q / (N * N) : 0 };
-		if constexpr (MINV) {
-			R wq = sW[cc[0]] * sW[cc[1]];
-			if (D == 3) wq = wq * sW[cc[2]];
-			scale = trcp(wq * a.detJ[(size_t)e * Np + q]);
-		}
-#pragma unroll
-		for (int lf = 0; lf < NFACE; ++lf) {
-			const int fd = lf >> 1;
-			int fa, fb;
-			if (fd == 0) { fa = cc[1]; fb = cc[2]; }
-			else if (fd == 1) { fa = cc[0]; fb = cc[2]; }
-			else { fa = cc[0]; fb = cc[1]; }
-			R wf = sW[fa];
-			if (D == 3) wf = wf * sW[fb];
-			cf[lf] = sLift[lf & 1][cc[fd]] * wf;
-			of[lf] = lf * NV * NF + ((D == 3) ? fa + N * fb : fa);
-		}
-	}
-	if constexpr (MODE == MODE_OFFD) {
-		constexpr int NFV = NV * NF;
-		for (int lf = 0; lf < NFACE; ++lf) {
-			if (valid) {
-#pragma unroll
-				for (int v = 0; v < NV; ++v) U[v * Np + t] = S(0.0) - cf[lf] * TR[of[lf] + v * NF];
-			}
-			__syncthreads();
-#pragma unroll
-			for (int dir = 0; dir < D; ++dir) sweep_inplace<S, R, D, N, NV, NT>(U, sIT, dir, t);
-			if (valid) {
-				double* ob = a.out + (((size_t)e * NFACE + lf) * NFV + a.col) * NFV;
-				for (int r = t; r < NFV; r += NT) {
-					const int v = r / NF;
-					ob[r] = U[v * Np + face_vol_node<D, N>(lf, r - v * NF)].d;
-				}
-			}
-			__syncthreads();
-		}
-		return;
-	}
-	if (valid) {
-		const int q = t;
-		double msc = 0.0;   // MODE_JVP mass term: - s_e W_q J_q (Ibar v)_q before the final Ibar^T
-		if constexpr (MODE == MODE_JVP) {
-			if (a.mass_scale) {
-				const int c0 = q % N, c1 = (q / N) % N, c2 = (D == 3) ? q / (N * N) : 0;
-				double wq = sW[c0] * sW[c1];
-				if (D == 3) wq *= sW[c2];
-				msc = a.mass_scale[e] * wq * a.detJ[(size_t)e * Np + q];
-			}
-		}
-#pragma unroll
-		for (int v = 0; v < NV; ++v) {
-			S acc = U[v * Np + q];
-#pragma unroll
-			for (int lf = 0; lf < NFACE; ++lf) acc -= cf[lf] * TR[of[lf] + v * NF];
-			if constexpr (MINV) acc = scale * acc;
-			if constexpr (MODE == MODE_JVP) acc.d -= msc * vgl[v];
-			U[v * Np + q] = acc;
-		}
-	}
-	__syncthreads();
-	// ---- back to the GLL test functions: Ibar^T (rhs) or Ibar^{-1} (M^-1 rhs)
-	if constexpr (PB && D == 3) {
-		sweep_inplace_pb<S, R, N, 0>(U, MINV ? sIinv : sIT, pb[0]);
-		sweep_inplace_pb<S, R, N, 1>(U, MINV ? sIinv : sIT, pb[1]);
-		sweep_inplace_pb<S, R, N, 2>(U, MINV ? sIinv : sIT, pb[2]);
-	} else {
-#pragma unroll
-		for (int dir = 0; dir < D; ++dir) sweep_inplace<S, R, D, N, NV, NT>(U, MINV ? sIinv : sIT, dir, t);
-	}
-
-	if (valid) {
-		double lmax = 0.0;
-		for (int i = t; i < NV * Np; i += NT) {
-			if constexpr (MODE == MODE_JVP) a.out[ebase + i] = U[i].d;
-			else if constexpr (MODE == MODE_DIAG)
-				a.out[(size_t)e * (NV * Np) * (NV * Np) + (size_t)i * (NV * Np) + a.col] = U[i].d;
-			else if constexpr (MODE == MODE_OFFD) {
-				// stored per face above
-			} else {
-				a.out[ebase + i] = U[i];
-				lmax = fmax(lmax, fabs((double)rv(U[i])));
-			}
-		}
-		if constexpr (MODE == MODE_RHS_MINV) {
-			if (a.dmax) atomicMax(&sMax, (unsigned long long)__double_as_longlong(lmax));
-		}
-	}
-	if constexpr (MODE == MODE_RHS_MINV) {
-		__syncthreads();
-		if (threadIdx.x == 0 && a.dmax) atomicMax(a.dmax, sMax);
-	}
+#define DGX_RHS_PERSIST 0
+#include "k_rhs_body.inc"
+#undef DGX_RHS_PERSIST
 }
 
+// The same residual over a persistent grid (a.grid workgroups), each workgroup striding over its XCD's
+// range of element blocks.  It holds values across the block loop and needs more registers (HEX p=4 FP32:
+// about 120 VGPRs against 77), so the autotuner keeps it only where it measures faster.
 template <typename S, typename R, int D, int N, int PDE, int FLUX, int MODE>
+__global__ void __launch_bounds__(256) k_rhs_persistent(RhsArgs<S, R> a)
+{
+#define DGX_RHS_PERSIST 1
+#include "k_rhs_body.inc"
+#undef DGX_RHS_PERSIST
+}
+
+template <typename S, typename R, int D, int N, int PDE, int FLUX, int MODE, bool PERSIST = false>
 inline hipError_t run_rhs(const RhsArgs<S, R>& a, hipStream_t st)
 {
 	constexpr int Np = ipow(N, D), NV = (PDE == DGX_PDE_EULER) ? D + 2 : 1;
 	constexpr int EPB = rhs_epb<S, D, N, NV>();
 	const int grid = (((a.E + EPB - 1) / EPB) + 7) / 8 * 8;
 	if (a.E == 0) return hipSuccess;
-	hipLaunchKernelGGL((k_rhs<S, R, D, N, PDE, FLUX, MODE>), dim3(grid), dim3(EPB * Np), 0, st, a);
+	if constexpr (PERSIST) {
+		const int pgrid = (a.grid > 0 && a.grid < grid) ? (a.grid + 7) / 8 * 8 : grid;
+		hipLaunchKernelGGL((k_rhs_persistent<S, R, D, N, PDE, FLUX, MODE>), dim3(pgrid), dim3(EPB * Np), 0, st, a);
+	} else {
+		hipLaunchKernelGGL((k_rhs<S, R, D, N, PDE, FLUX, MODE>), dim3(grid), dim3(EPB * Np), 0, st, a);
+	}
 	return hipGetLastError();
 }
 
-// runtime (d, p, pde, flux) -> template instantiation
-template <typename S, typename R, int MODE>
+// runtime (d, p, pde, flux) -> template instantiation.  PERSIST selects k_rhs_persistent; it is
+// instantiated only in k_rhs_persistent.hip, because compiling both kernels of a configuration into one
+// code object changes the register allocation of k_rhs (HEX p=4 FP64: 128 -> 167 VGPRs).
+template <typename S, typename R, int MODE, bool PERSIST = false>
 hipError_t dispatch_rhs(int d, int p, int pde, int flux, const RhsArgs<S, R>& a, hipStream_t st)
 {
 #define DGX_CASE_EULER(DD, NN)                                                              \
 	if (d == DD && p + 1 == NN && pde == DGX_PDE_EULER) {                                   \
-		if (flux == DGX_FLUX_LF) return run_rhs<S, R, DD, NN, DGX_PDE_EULER, DGX_FLUX_LF, MODE>(a, st); \
-		return run_rhs<S, R, DD, NN, DGX_PDE_EULER, DGX_FLUX_ROE, MODE>(a, st);             \
+		if (flux == DGX_FLUX_LF) return run_rhs<S, R, DD, NN, DGX_PDE_EULER, DGX_FLUX_LF, MODE, PERSIST>(a, st); \
+		return run_rhs<S, R, DD, NN, DGX_PDE_EULER, DGX_FLUX_ROE, MODE, PERSIST>(a, st);             \
 	}
 #define DGX_CASE_ADV(DD, NN)                                                                \
 	if (d == DD && p + 1 == NN && pde == DGX_PDE_ADVECTION)                                 \
-		return run_rhs<S, R, DD, NN, DGX_PDE_ADVECTION, DGX_FLUX_UPWIND, MODE>(a, st);
+		return run_rhs<S, R, DD, NN, DGX_PDE_ADVECTION, DGX_FLUX_UPWIND, MODE, PERSIST>(a, st);
 	DGX_CASE_EULER(2, 2) DGX_CASE_EULER(2, 3) DGX_CASE_EULER(2, 4) DGX_CASE_EULER(2, 5) DGX_CASE_EULER(2, 6)
 	DGX_CASE_EULER(3, 2) DGX_CASE_EULER(3, 3) DGX_CASE_EULER(3, 4) DGX_CASE_EULER(3, 5)
 	if constexpr (MODE != MODE_PERT) {
@@ -704,5 +358,9 @@ hipError_t dispatch_rhs(int d, int p, int pde, int flux, const RhsArgs<S, R>& a,
 #undef DGX_CASE_ADV
 	return hipErrorInvalidValue;
 }
+
+// persistent-grid launches (defined in k_rhs_persistent.hip)
+hipError_t dispatch_rhs_persistent(const Ctx& c, const RhsArgs<double, double>& a, int mode);
+hipError_t dispatch_rhs_persistent(const Ctx& c, const RhsArgs<float, float>& a, int mode);
 
 }  // namespace dgx

@@ -25,7 +25,9 @@ static RhsArgs<double, double> make_args64(Ctx& c, const double* u, double* r, u
 
 hipError_t launch_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned long long* dmax)
 {
-	const RhsArgs<double, double> a = make_args64(c, u, r, dmax);
+	RhsArgs<double, double> a = make_args64(c, u, r, dmax);
+	a.grid = c.rhs_grid[mode == MODE_RHS ? TUNE_RHS64 : TUNE_RHS64_MINV];
+	if (a.grid > 0) return dispatch_rhs_persistent(c, a, mode);
 	if (mode == MODE_RHS) return dispatch_rhs<double, double, MODE_RHS>(c.d, c.p, c.cfg.pde, c.cfg.num_flux, a, c.stream);
 	return dispatch_rhs<double, double, MODE_RHS_MINV>(c.d, c.p, c.cfg.pde, c.cfg.num_flux, a, c.stream);
 }

@@ -147,6 +147,20 @@ dgx_status dgx_graph_launch(dgx_graph* g, int n);
  * dgx_graph_sync (dgx_ssp_rk3_step and dgx_enforce_positivity on the same context clear that flag too) */
 dgx_status dgx_graph_sync(dgx_graph* g, double* max_abs_r);
 dgx_status dgx_graph_destroy(dgx_graph* g);
+/* Launch shape of the residual kernels.  The grid is either one workgroup per element block (the default,
+ * grid 0) or persistent: `grid` workgroups, each XCD's share striding over that XCD's contiguous range of
+ * element blocks with the reference-element operators staged in LDS once.  kind: DGX_GRAPH_RHS_FP64, _FP32,
+ * _MIXED, or DGX_TUNE_RHS_MINV (the M^-1 R launch of the explicit step).  The persistent kernel is a separate
+ * compilation of the same code; its results agree with the default kernel's to rounding (the compiler may
+ * pair multiplies and adds into FMAs differently), not bit for bit. */
+enum { DGX_TUNE_RHS_MINV = 3 };
+dgx_status dgx_set_rhs_grid(dgx_ctx* ctx, int kind, int grid);
+/* Autotuner: times the kernel of `kind` on (u, r) (device buffers of that kernel's types) for grid 0 and for
+ * persistent grids of 1, 2, 3, 4, 6, 8, 12 and 16 workgroups per CU, keeps the fastest for this context and
+ * mesh (a persistent grid only when more than 1% faster) and returns it in *grid and its time per launch (ms)
+ * in *ms (HOST, either may be NULL).  Later launches and graph recordings of that kernel use it.  Synchronous;
+ * not during a graph capture. */
+dgx_status dgx_autotune_rhs(dgx_ctx* ctx, int kind, const void* u, void* r, int* grid, double* ms);
 /* max |x|_inf over n doubles (HOST result). Synchronous. */
 dgx_status dgx_max_abs(dgx_ctx* ctx, const double* x, long long n, double* out);
 /* FP64 <-> FP32 state conversion on the device. */
