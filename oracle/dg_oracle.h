@@ -81,6 +81,8 @@ void or_face_perm(const or_ctx* c, int ord, int* perm);
 
 /* residual (volume + face, no M^-1) -- compute_rlhs_common_dg */
 void or_compute_rhs_d(const or_ctx* c, const double* u, double* rhs);
+/* the same on nthreads OpenMP threads (faces lifted by colour); agrees with or_compute_rhs_d to rounding */
+void or_compute_rhs_d_threads(const or_ctx* c, const double* u, double* rhs, int nthreads);
 /* complex twin: u = ure + i*uim */
 void or_compute_rhs_c(const or_ctx* c, const double* ure, const double* uim, double* rre, double* rim);
 /* volume-only and face-only parts */

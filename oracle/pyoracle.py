@@ -36,6 +36,7 @@ def lib():
         L.or_destroy.argtypes = [C.c_void_p]
         for name in ("or_compute_rhs_d", "or_volume_rhs_d", "or_face_rhs_d"):
             getattr(L, name).argtypes = [C.c_void_p, dp, dp]
+        L.or_compute_rhs_d_threads.argtypes = [C.c_void_p, dp, dp, C.c_int]
         L.or_compute_rhs_c.argtypes = [C.c_void_p, dp, dp, dp, dp]
         L.or_geometry.argtypes = [C.c_void_p, dp, dp, dp]
         L.or_apply_minv.argtypes = [C.c_void_p, dp]
@@ -175,6 +176,13 @@ class Oracle:
         r = np.zeros(self.ndof)
         fn = {"all": lib().or_compute_rhs_d, "volume": lib().or_volume_rhs_d, "face": lib().or_face_rhs_d}[part]
         fn(self.ctx, _d(u), _d(r))
+        return r
+
+    def rhs_threads(self, u, nthreads):
+        """The residual on nthreads OpenMP threads (faces lifted by colour); equal to rhs() to rounding."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        r = np.zeros(self.ndof)
+        lib().or_compute_rhs_d_threads(self.ctx, _d(u), _d(r), int(nthreads))
         return r
 
     def rhs_complex(self, u_re, u_im):

@@ -259,6 +259,26 @@ def test_free_stream_preservation_curved(case, flux):
     assert np.abs(r).max() < 2e1 * EPS * 10, np.abs(r).max()   # test_integration_free_stream.c:138 (x10 for 3-D sums)
 
 
+@pytest.mark.parametrize("case", ["3d_curved_lf", "2d_channel_bc_roe"])
+def test_threaded_residual_equals_sequential(case):
+    """or_compute_rhs_d_threads (the CPU baseline's all-cores residual) equals the sequential residual to
+    rounding; the face colouring must give every face lift exactly once."""
+    if case == "3d_curved_lf":
+        mesh = M.config3_mesh(4, 3)
+        u, orc = M.config3_state(mesh), O.Oracle(mesh, M.PDE_EULER, M.FLUX_LF)
+    else:
+        mesh = M.channel_mesh(6, 4, 2)
+        prim = M.freestream_prim(2)
+        x = M.nodes_xyz(mesh)
+        pert = 0.02 * np.sin(3 * x[..., :1]) * np.cos(2 * x[..., 1:2]) * np.array([1, 1, 0.5, 1])
+        u = M.pack_state(M.prim_to_cons(prim[None, None, :] + pert, 2))
+        orc = O.Oracle(mesh, M.PDE_EULER, M.FLUX_ROE, bc_prim=prim)
+    r1 = orc.rhs(u)
+    for nt in (1, 3, 8):
+        rt = orc.rhs_threads(u, nt)
+        assert np.abs(rt - r1).max() <= 1e-13 * np.abs(r1).max(), nt
+
+
 @pytest.mark.parametrize("case", ["2d", "3d", "2d_rot", "3d_rot"])
 def test_conservation_sum_of_rhs_is_zero(case):
     if case == "2d":
