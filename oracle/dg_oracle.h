@@ -81,6 +81,15 @@ void or_face_perm(const or_ctx* c, int ord, int* perm);
 
 /* residual (volume + face, no M^-1) -- compute_rlhs_common_dg */
 void or_compute_rhs_d(const or_ctx* c, const double* u, double* rhs);
+/* Navier-Stokes with BR2 (real only, interior faces only: periodic meshes).  or_set_viscous switches an Euler
+ * context to Navier-Stokes with constant viscosity mu and Prandtl number Pr. */
+void or_set_viscous(or_ctx* c, double mu, double Pr);
+/* negated viscous flux fv[dim + d*eq] for state s[nv] and gradient g[v + nv*dim] */
+void or_viscous_flux(int d, const double* s, const double* g, double mu, double Pr, double* fv);
+/* BR2 weak gradient coefficients: g, gv: E x d x nv x Np; gf: F x 2 x d x nv x Np.  0 or -1 */
+int or_ns_gradient(or_ctx* c, const double* u, double* g, double* gv, double* gf);
+/* full Navier-Stokes residual (no M^-1).  0 or -1 */
+int or_compute_rhs_ns(or_ctx* c, const double* u, double* rhs);
 /* the same on nthreads OpenMP threads (faces lifted by colour); agrees with or_compute_rhs_d to rounding */
 void or_compute_rhs_d_threads(const or_ctx* c, const double* u, double* rhs, int nthreads);
 /* complex twin: u = ure + i*uim */

@@ -37,6 +37,12 @@ def lib():
         for name in ("or_compute_rhs_d", "or_volume_rhs_d", "or_face_rhs_d"):
             getattr(L, name).argtypes = [C.c_void_p, dp, dp]
         L.or_compute_rhs_d_threads.argtypes = [C.c_void_p, dp, dp, C.c_int]
+        L.or_set_viscous.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.or_viscous_flux.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, dp]
+        L.or_ns_gradient.argtypes = [C.c_void_p, dp, dp, dp, dp]
+        L.or_ns_gradient.restype = C.c_int
+        L.or_compute_rhs_ns.argtypes = [C.c_void_p, dp, dp]
+        L.or_compute_rhs_ns.restype = C.c_int
         L.or_compute_rhs_c.argtypes = [C.c_void_p, dp, dp, dp, dp]
         L.or_geometry.argtypes = [C.c_void_p, dp, dp, dp]
         L.or_apply_minv.argtypes = [C.c_void_p, dp]
@@ -138,7 +144,8 @@ def bc_state(d, bc, sL, n, prim):
 class Oracle:
     """One DG discretisation (mesh + physics) on the CPU."""
 
-    def __init__(self, mesh, pde: int, flux: int, b_adv=(0.0, 0.0, 0.0), bc_prim=None):
+    def __init__(self, mesh, pde: int, flux: int, b_adv=(0.0, 0.0, 0.0), bc_prim=None, viscous=None):
+        """viscous = (mu, Pr): Navier-Stokes with BR2 (Euler contexts, periodic meshes)."""
         self.mesh = mesh
         self.d, self.p = mesh.d, mesh.p
         self.pde, self.flux = pde, flux
@@ -157,6 +164,8 @@ class Oracle:
         if not self.ctx:
             raise RuntimeError("or_create failed")
         self.ndof = mesh.E * self.Np * self.nv
+        if viscous is not None:
+            lib().or_set_viscous(self.ctx, float(viscous[0]), float(viscous[1]))
 
     def __del__(self):
         if getattr(self, "ctx", None):
@@ -197,6 +206,24 @@ class Oracle:
             raise RuntimeError("singular mass matrix")
         return r
 
+    def rhs_ns(self, u):
+        """Navier-Stokes (BR2) residual, no M^-1."""
+        r = np.zeros(self.ndof)
+        if lib().or_compute_rhs_ns(self.ctx, _d(np.ascontiguousarray(u, float)), _d(r)):
+            raise RuntimeError("or_compute_rhs_ns: viscous context with interior faces only")
+        return r
+
+    def ns_gradient(self, u):
+        """(g, gv, gf): full and volume-only weak gradient coefficients E x d x nv x Np, face lifts
+        F x 2 x d x nv x Np."""
+        m = self.mesh
+        eb = m.d * self.nv * self.Np
+        g, gv, gf = np.zeros(m.E * eb), np.zeros(m.E * eb), np.zeros(m.F * 2 * eb)
+        if lib().or_ns_gradient(self.ctx, _d(np.ascontiguousarray(u, float)), _d(g), _d(gv), _d(gf)):
+            raise RuntimeError("or_ns_gradient: interior faces only")
+        sh = (m.E, m.d, self.nv, self.Np)
+        return g.reshape(sh), gv.reshape(sh), gf.reshape(m.F, 2, m.d, self.nv, self.Np)
+
     def mass(self, e):
         M = np.zeros((self.Np, self.Np))
         lib().or_mass(self.ctx, e, _d(M))
@@ -229,3 +256,10 @@ class Oracle:
         out = np.zeros((self.mesh.E, nb, nb))
         lib().or_diag_blocks_cs(self.ctx, _d(np.ascontiguousarray(u, float)), h, _d(out))
         return out
+
+
+def viscous_flux(d, s, g, mu, Pr):
+    """Negated viscous flux fv[dim + d*eq] (or_viscous_flux)."""
+    fv = np.zeros(d * (d + 2))
+    lib().or_viscous_flux(d, _d(np.ascontiguousarray(s, float)), _d(np.ascontiguousarray(g, float)), mu, Pr, _d(fv))
+    return fv.reshape(d + 2, d)
