@@ -44,7 +44,7 @@ class DgxError(RuntimeError):
 
 class DgxConfig(C.Structure):
     _fields_ = [("d", C.c_int), ("p", C.c_int), ("pde", C.c_int), ("num_flux", C.c_int),
-                ("b_adv", C.c_double * 3), ("bc_prim", C.c_double * 5)]
+                ("b_adv", C.c_double * 3), ("bc_prim", C.c_double * 5), ("mu", C.c_double), ("Pr", C.c_double)]
 
 
 class DgxHostMesh(C.Structure):
@@ -154,10 +154,13 @@ def check(fn: str, code: int):
 class Context:
     """One DG discretisation on the device (owns geometry; state buffers are the caller's)."""
 
-    def __init__(self, d: int, p: int, pde: int, num_flux: int, b_adv=(0.0, 0.0, 0.0), bc_prim=None):
+    def __init__(self, d: int, p: int, pde: int, num_flux: int, b_adv=(0.0, 0.0, 0.0), bc_prim=None,
+                 mu: float = 0.0, Pr: float = 0.72):
+        """pde PDE_NAVIER_STOKES: constant viscosity mu and Prandtl number Pr (2-D, periodic meshes)."""
         L = load()
         cfg = DgxConfig()
         cfg.d, cfg.p, cfg.pde, cfg.num_flux = d, p, pde, num_flux
+        cfg.mu, cfg.Pr = float(mu), float(Pr)
         for k, v in enumerate(b_adv):
             cfg.b_adv[k] = float(v)
         if bc_prim is not None:

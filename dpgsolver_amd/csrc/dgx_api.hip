@@ -23,7 +23,7 @@ dgx_status hip_status(hipError_t e) { return e == hipSuccess ? DGX_OK : DGX_ERR_
 
 bool supported(int d, int p, int pde)
 {
-	if (d == 2) return p >= 1 && p <= 5;
+	if (d == 2) return p >= 1 && p <= 5;   // Navier-Stokes kernels: N = 2..6 as well
 	if (d == 3) return pde == DGX_PDE_EULER && p >= 1 && p <= 4;
 	return false;
 }
@@ -40,6 +40,7 @@ void free_mesh(Ctx& c)
 	release(c.xyz); release(c.faces); release(c.elem_faces); release(c.order); release(c.hmin); release(c.metrics_vc); release(c.metrics_vc32);
 	release(c.detJ_vc); release(c.detJ_vc32); release(c.nJ_fc); release(c.nJ_fc32); release(c.u0); release(c.rtmp);
 	release(c.color_elems);
+	release(c.ns_grad); release(c.ns_gface);
 	c.color_off.clear();
 	for (int& g : c.rhs_grid) g = 0;   // launch shapes are tuned per mesh
 	c.mesh_ready = false;
@@ -63,8 +64,13 @@ dgx_status dgx_create(dgx_ctx** out, const dgx_config* cfg)
 {
 	if (!out || !cfg) return DGX_ERR_ARG;
 	*out = nullptr;
-	if (cfg->pde != DGX_PDE_EULER && cfg->pde != DGX_PDE_ADVECTION) return DGX_ERR_ARG;
-	if (cfg->pde == DGX_PDE_EULER && cfg->num_flux != DGX_FLUX_LF && cfg->num_flux != DGX_FLUX_ROE) return DGX_ERR_ARG;
+	if (cfg->pde != DGX_PDE_EULER && cfg->pde != DGX_PDE_ADVECTION && cfg->pde != DGX_PDE_NAVIER_STOKES)
+		return DGX_ERR_ARG;
+	if (cfg->pde != DGX_PDE_ADVECTION && cfg->num_flux != DGX_FLUX_LF && cfg->num_flux != DGX_FLUX_ROE) return DGX_ERR_ARG;
+	if (cfg->pde == DGX_PDE_NAVIER_STOKES) {
+		if (cfg->d != 2) return DGX_ERR_UNSUPPORTED;
+		if (!(cfg->mu >= 0.0) || !(cfg->Pr > 0.0)) return DGX_ERR_ARG;
+	}
 	if (cfg->pde == DGX_PDE_ADVECTION && cfg->num_flux != DGX_FLUX_UPWIND) return DGX_ERR_ARG;
 	if (!supported(cfg->d, cfg->p, cfg->pde)) return DGX_ERR_UNSUPPORTED;
 	dgx_ctx* h = new (std::nothrow) dgx_ctx();
@@ -77,7 +83,7 @@ dgx_status dgx_create(dgx_ctx** out, const dgx_config* cfg)
 	c.Np = c.N * c.N * (c.d == 3 ? c.N : 1);
 	c.Nc = c.Np;
 	c.Nfc = (c.d == 3) ? c.N * c.N : c.N;
-	c.nv = (cfg->pde == DGX_PDE_EULER) ? c.d + 2 : 1;
+	c.nv = (cfg->pde == DGX_PDE_ADVECTION) ? 1 : c.d + 2;
 	if (hipGetDevice(&c.device) != hipSuccess) { delete h; return DGX_ERR_HIP; }
 	dgx::build_ops(c.d, c.p, c.h_ops64, c.h_ops32);
 	if (hipMalloc(&c.ops64, sizeof(dgx::Ops1D<double>)) != hipSuccess ||
@@ -144,6 +150,7 @@ dgx_status dgx_upload_mesh(dgx_ctx* h, const dgx_mesh* m)
 		if (bc == DGX_BC_INTERIOR) {
 			if (eR < 0 || eR >= m->E || fR < 0 || fR >= NFACE) return DGX_ERR_ARG;
 		} else {
+			if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;   // viscous BCs not built
 			if (c.cfg.pde != DGX_PDE_EULER || (bc != DGX_BC_RIEMANN && bc != DGX_BC_SLIPWALL)) return DGX_ERR_ARG;
 		}
 		dgx::ElemFace& L = ef[(size_t)eL * NFACE + fL];
@@ -203,6 +210,11 @@ dgx_status dgx_upload_mesh(dgx_ctx* h, const dgx_mesh* m)
 		DGX_TRY(hipMalloc(&c.color_elems, sizeof(int) * c.E));
 		DGX_TRY(hipMemcpy(c.color_elems, elems.data(), sizeof(int) * c.E, hipMemcpyHostToDevice));
 	}
+	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) {
+		// BR2 gradient at the GL nodes (E x d x nv x Nc) and its face traces (E x 2d x d x nv x Nfc)
+		DGX_TRY(hipMalloc(&c.ns_grad, sizeof(double) * (size_t)c.E * d * c.nv * c.Nc));
+		DGX_TRY(hipMalloc(&c.ns_gface, sizeof(double) * (size_t)c.E * NFACE * d * c.nv * c.Nfc));
+	}
 	DGX_TRY(dgx::launch_geometry(c));
 	DGX_TRY(dgx::launch_convert(c.metrics_vc, c.metrics_vc32, nm, c.stream));
 	DGX_TRY(dgx::launch_convert(c.detJ_vc, c.detJ_vc32, nj, c.stream));
@@ -235,6 +247,7 @@ dgx_status dgx_compute_rhs_f32(dgx_ctx* h, const float* u, float* r)
 {
 	if (!h || !u || !r) return DGX_ERR_ARG;
 	if (!h->c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (h->c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;   // FP64 only
 	return hip_status(dgx::launch_rhs32(h->c, u, r));
 }
 
@@ -280,7 +293,7 @@ dgx_status dgx_enforce_positivity(dgx_ctx* h, double* u)
 	if (!h || !u) return DGX_ERR_ARG;
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
-	if (c.cfg.pde != DGX_PDE_EULER) return DGX_OK;  // test_case.c:63-77: Euler/NS only
+	if (c.cfg.pde == DGX_PDE_ADVECTION) return DGX_OK;  // test_case.c:63-77: Euler/NS only
 	DGX_TRY(dgx::launch_fill_i32(c.stream, c.dstatus, 1, 0));
 	DGX_TRY(dgx::launch_positivity(c, u, c.dstatus));
 	int st = 0;
@@ -297,7 +310,7 @@ dgx_status dgx_ssp_rk3_step(dgx_ctx* h, double* u, double dt, int positivity, do
 	const size_t n = ndof(c);
 	if (!c.u0) DGX_TRY(hipMalloc(&c.u0, sizeof(double) * n));
 	if (!c.rtmp) DGX_TRY(hipMalloc(&c.rtmp, sizeof(double) * n));
-	const bool pos = positivity && c.cfg.pde == DGX_PDE_EULER;
+	const bool pos = positivity && c.cfg.pde != DGX_PDE_ADVECTION;
 	if (pos) DGX_TRY(dgx::launch_fill_i32(c.stream, c.dstatus, 1, 0));
 	for (int rk = 0; rk < 3; ++rk) {
 		DGX_TRY(dgx::launch_fill_u64(c.stream, c.dmax, 1, 0ull));
@@ -366,6 +379,7 @@ dgx_status dgx_graph_create_rhs(dgx_ctx* h, int kind, const void* u, void* r, dg
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
 	if (kind == DGX_GRAPH_RHS_MIXED && c.cfg.pde != DGX_PDE_EULER) return DGX_ERR_UNSUPPORTED;
+	if (kind == DGX_GRAPH_RHS_FP32 && c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;
 	dgx_graph* g = new (std::nothrow) dgx_graph();
 	if (!g) return DGX_ERR_HIP;
 	const dgx_status st = capture_graph(h, g, [&]() -> hipError_t {
@@ -394,7 +408,7 @@ dgx_status dgx_graph_create_rk3(dgx_ctx* h, double* u, double dt, int positivity
 	dgx_graph* g = new (std::nothrow) dgx_graph();
 	if (!g) return DGX_ERR_HIP;
 	g->rk3 = true;
-	g->positivity = positivity && c.cfg.pde == DGX_PDE_EULER;
+	g->positivity = positivity && c.cfg.pde != DGX_PDE_ADVECTION;
 	const dgx_status st = capture_graph(h, g, [&]() -> hipError_t {
 		for (int rk = 0; rk < 3; ++rk) {
 			DGX_HIP_CHECK(dgx::launch_fill_u64(c.stream, c.dmax, 1, 0ull));
@@ -473,6 +487,7 @@ dgx_status dgx_autotune_rhs(dgx_ctx* h, int kind, const void* u, void* r, int* g
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
 	if (kind == DGX_GRAPH_RHS_MIXED && c.cfg.pde != DGX_PDE_EULER) return DGX_ERR_UNSUPPORTED;
+	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;   // its kernels have a fixed launch shape
 	int dev = 0, ncu = 0;
 	DGX_TRY(hipGetDevice(&dev));
 	DGX_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -547,7 +562,7 @@ dgx_status dgx_assemble_diag_blocks(dgx_ctx* h, const double* u, const double* m
 	if (!h || !u || !blocks) return DGX_ERR_ARG;
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
-	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
+	if (c.d != 2 || c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;   // Euler linearisation only
 	const int nb = c.Np * c.nv;
 	for (int col = 0; col < nb; ++col) DGX_TRY(dgx::launch_diag_col(c, u, blocks, col));
 	if (mass_scale) DGX_TRY(dgx::launch_sub_mass(c, blocks, mass_scale, 0.0));
@@ -559,7 +574,7 @@ dgx_status dgx_assemble_face_blocks(dgx_ctx* h, const double* u, double* blocks)
 	if (!h || !u || !blocks) return DGX_ERR_ARG;
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
-	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
+	if (c.d != 2 || c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;   // Euler linearisation only
 	const int nfv = c.nv * c.N;   // face-node rows / columns of a 2-D face block
 	double* cT = nullptr;
 	DGX_TRY(hipMalloc(&cT, sizeof(double) * (size_t)c.E * 4 * nfv * nfv));
@@ -594,7 +609,7 @@ dgx_status dgx_newton_operator(dgx_ctx* h, const double* u, const double* mass_s
 	if (!h || !u || !mass_scale || !v || !out) return DGX_ERR_ARG;
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
-	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
+	if (c.d != 2 || c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;   // Euler linearisation only
 	return hip_status(dgx::launch_jvp(c, u, v, out, mass_scale));
 }
 
@@ -603,7 +618,7 @@ dgx_status dgx_jacobian_vector(dgx_ctx* h, const double* u, const double* v, dou
 	if (!h || !u || !v || !jv) return DGX_ERR_ARG;
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
-	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
+	if (c.d != 2 || c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;   // Euler linearisation only
 	return hip_status(dgx::launch_jvp(c, u, v, jv));
 }
 

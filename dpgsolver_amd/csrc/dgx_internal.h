@@ -59,6 +59,8 @@ struct Ctx {
 	int* color_elems = nullptr;     // E: elements grouped by colour (no two face neighbours share a colour)
 	std::vector<int> color_off;     // n_colors + 1 offsets into color_elems
 	dgx_mlp* mlp = nullptr;         // attached learned preconditioner (not owned)
+	double* ns_grad = nullptr;      // Navier-Stokes: BR2 gradient at the GL nodes (E x d x nv x Nc)
+	double* ns_gface = nullptr;     // Navier-Stokes: face gradient traces (E x 2d x d x nv x Nfc)
 	int rhs_grid[4] = { 0, 0, 0, 0 };   // residual launch grids by TUNE_* (0 = one workgroup per element block)
 	double* metrics_vc = nullptr;   // E*d*d*Nc  (m[(i + d*j)*Nc + n] = J dxi_j/dx_i)
 	float* metrics_vc32 = nullptr;
@@ -101,6 +103,7 @@ std::vector<int> morton_order(int d, int E, int Np, const double* xyz);
 hipError_t launch_geometry(Ctx& c);
 hipError_t launch_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned long long* dmax);
 hipError_t launch_rhs32(Ctx& c, const float* u, float* r);
+hipError_t launch_rhs_ns(Ctx& c, const double* u, double* r);   // 2-D Navier-Stokes (BR2), FP64
 hipError_t launch_rhs_mixed(Ctx& c, const double* u, float* r);
 hipError_t launch_minv(Ctx& c, double* r);
 hipError_t launch_positivity(Ctx& c, double* u, int* status);

@@ -35,7 +35,7 @@ void tensor_apply(int d, int N, const double* M, const double* in, double* out)
 	std::memcpy(out, a.data(), sizeof(double) * Np);
 }
 
-int nv_of(const dgx_config* cfg) { return cfg->pde == DGX_PDE_EULER ? cfg->d + 2 : 1; }
+int nv_of(const dgx_config* cfg) { return cfg->pde == DGX_PDE_ADVECTION ? 1 : cfg->d + 2; }
 
 }  // namespace
 

@@ -1,0 +1,417 @@
+// k_ns.hip -- 2-D Navier-Stokes residual with BR2 viscous terms (QUAD, interior faces), FP64.
+//
+// Two element-centric kernels, one thread per volume cubature (GL) node:
+//   k_ns_grad: the weak gradient of the state (compute_grad_coef_dg_T.c:171-307, 'e' branch), formed at the GL
+//     nodes.  With M = Ibar^T W J Ibar, M^-1 Ibar^T W x = Ibar^-1 (x / J), so the projected volume gradient is
+//     the pointwise (J grad s) / J at the GL nodes, and the face term M^-1 tw0_vt_fc (J_f n (s_num - s)) is the
+//     face lift of the residual kernel divided by W J:  G_f = Lift_side(c) W_f(a) J_f n (s_num - s)(a) / (W J).
+//     s_num is central, s_num - s_own = (s_nb - s_own) / 2.  Stored: the full gradient G = G_v + sum_faces G_f
+//     at the GL nodes (the volume flux reads it there: cv0_vs_vc grad_coef = Ibar Ibar^-1 G), and per face the
+//     trace of G_v + 1.01 NFMAX G_f(face) at the face GL nodes (constructor_partial_grad_fc_interp,
+//     compute_face_rlhs_dg_T.c:243-300; NFMAX = 4 in 2-D).  The trace of GL nodal values at xi = -1 / +1 is
+//     Lift_side = I^-T e_side (the GL Lagrange basis at the end points).
+//   k_rhs_ns: volume flux F_euler(s) + fv(s, G) (fv the negated viscous flux, compute_Flux_navier_stokes_0,
+//     flux_navier_stokes_T.c:1005-1040), weak divergence, then per face the inviscid numerical flux plus the
+//     central average n . (fv_L + fv_R) / 2 of the two sides' face gradients (numerical_flux_central_T.c:76-104),
+//     scaled by J_f and lifted; both sides of a face evaluate it from identical inputs in the left ordering.
+#include "dgx_internal.h"
+#include "dgx_device.h"
+
+namespace dgx {
+
+namespace {
+
+constexpr int kNsNV = 4;
+
+template <int N>
+constexpr int ns_epb()
+{
+	return (128 / (N * N)) > 0 ? 128 / (N * N) : 1;
+}
+
+// node of local face lf at face position a (faces r-, r+, s-, s+)
+template <int N>
+__device__ inline int ns_face_node(int lf, int a)
+{
+	const int fix = (lf & 1) ? N - 1 : 0;
+	return (lf >> 1) == 0 ? fix + N * a : a + N * fix;
+}
+
+// negated viscous flux fv[k + 2 * eq] of state s and gradient g[v + 4 * k] (2-D): tau with the -2/3 mu div V
+// part (compute_tau, flux_navier_stokes_T.c:778-799), scaled temperature gradient d(E/rho) - d(V^2)/2
+// (compute_dTs :904-926), energy flux gamma/Pr mu dTs + u . tau; constant viscosity
+__device__ inline void ns_viscous_flux2(const double* s, const double* g, double mu, double Pr, double* fv)
+{
+	const double rho = s[0], E = s[3], ri = 1.0 / rho;
+	const double u[2] = { s[1] * ri, s[2] * ri };
+	double du[2][2], tau[2][2], dTs[2];
+#pragma unroll
+	for (int k = 0; k < 2; ++k)
+#pragma unroll
+		for (int j = 0; j < 2; ++j) du[k][j] = ri * (g[(1 + k) + 4 * j] - u[k] * g[0 + 4 * j]);
+	const double divV = du[0][0] + du[1][1];
+#pragma unroll
+	for (int i = 0; i < 2; ++i)
+#pragma unroll
+		for (int j = 0; j < 2; ++j) tau[i][j] = mu * (du[i][j] + du[j][i]) - (i == j ? 2.0 / 3.0 * mu * divV : 0.0);
+#pragma unroll
+	for (int j = 0; j < 2; ++j) {
+		const double dV2 = 2.0 * (u[0] * du[0][j] + u[1] * du[1][j]);
+		dTs[j] = ri * ri * (g[3 + 4 * j] * rho - E * g[0 + 4 * j]) - 0.5 * dV2;
+	}
+#pragma unroll
+	for (int j = 0; j < 2; ++j) {
+		fv[j + 2 * 0] = 0.0;
+		fv[j + 2 * 1] = -tau[0][j];
+		fv[j + 2 * 2] = -tau[1][j];
+		fv[j + 2 * 3] = -(kGamma / Pr * mu * dTs[j] + u[0] * tau[j][0] + u[1] * tau[j][1]);
+	}
+}
+
+struct NsArgs {
+	const double* u;
+	double* out;           // k_rhs_ns: residual
+	double* gq;            // E x 2 x 4 x Np: full gradient at the GL nodes
+	double* gface;         // E x 4 x 2 x 4 x N: face gradient traces (own ordering, face GL nodes)
+	const double* met;     // E x 4 x Np
+	const double* detJ;    // E x Np
+	const double* nJ;      // F x N x 2
+	const ElemFace* ef;
+	const Ops1D<double>* ops;
+	double mu, Pr;
+	int E;
+};
+
+// shared per-element set-up: operators, the GLL state, its GL values and the GL face traces (own and neighbour)
+template <int N>
+struct NsBlock {
+	static constexpr int Np = N * N, EPB = ns_epb<N>();
+	double I[N * N], DhT[N * N], Lift[2][N], w[N];
+	int perm[8][N];
+	double U[EPB][kNsNV * Np];      // GLL nodal state
+	double V[EPB][kNsNV * Np];      // GL values
+	double TRo[EPB][4 * kNsNV * N]; // own face traces at the face GL nodes, own ordering
+	double TRn[EPB][4 * kNsNV * N]; // neighbour face traces, the neighbour's ordering
+	ElemFace ef[EPB][4];
+};
+
+template <int N>
+__device__ inline void ns_load(NsBlock<N>& B, const NsArgs& a, int sl, int t, int e, bool valid)
+{
+	constexpr int Np = N * N;
+	for (int i = threadIdx.x; i < N * N; i += blockDim.x) {
+		B.I[i] = a.ops->I[i];
+		B.DhT[i] = a.ops->DhT[i];
+	}
+	for (int i = threadIdx.x; i < N; i += blockDim.x) {
+		B.Lift[0][i] = a.ops->Lift[0][i];
+		B.Lift[1][i] = a.ops->Lift[1][i];
+		B.w[i] = a.ops->w[i];
+	}
+	for (int i = threadIdx.x; i < 8 * N; i += blockDim.x) B.perm[i / N][i % N] = a.ops->perm[i / N][i % N];
+	if (valid) {
+		if (t < 4) B.ef[sl][t] = a.ef[(size_t)e * 4 + t];
+		for (int v = 0; v < kNsNV; ++v) B.U[sl][v * Np + t] = a.u[((size_t)e * kNsNV + v) * Np + t];
+	}
+	__syncthreads();
+	// GL values: I along r, then along s (through V as scratch)
+	double tmp[kNsNV];
+	const int c0 = t % N, c1 = t / N;
+	if (valid) {
+#pragma unroll
+		for (int v = 0; v < kNsNV; ++v) {
+			double acc = 0.0;
+#pragma unroll
+			for (int m = 0; m < N; ++m) acc += B.I[c0 * N + m] * B.U[sl][v * Np + m + N * c1];
+			tmp[v] = acc;
+		}
+#pragma unroll
+		for (int v = 0; v < kNsNV; ++v) B.V[sl][v * Np + t] = tmp[v];
+	}
+	__syncthreads();
+	if (valid) {
+#pragma unroll
+		for (int v = 0; v < kNsNV; ++v) {
+			double acc = 0.0;
+#pragma unroll
+			for (int m = 0; m < N; ++m) acc += B.I[c1 * N + m] * B.V[sl][v * Np + c0 + N * m];
+			tmp[v] = acc;
+		}
+	}
+	__syncthreads();
+	if (valid) {
+#pragma unroll
+		for (int v = 0; v < kNsNV; ++v) B.V[sl][v * Np + t] = tmp[v];
+		// face traces: GLL face values (own from LDS, neighbour gathered), then I along the face
+		for (int item = t; item < 4 * N; item += Np) {
+			const int lf = item / N, i = item % N;
+			const ElemFace f = B.ef[sl][lf];
+			const int fn = f.packed & 15;
+			const int en = f.en >= 0 ? f.en : e;   // every face is interior (checked at mesh upload)
+#pragma unroll
+			for (int v = 0; v < kNsNV; ++v) {
+				double ao = 0.0, an = 0.0;
+#pragma unroll
+				for (int b = 0; b < N; ++b) {
+					ao += B.I[i * N + b] * B.U[sl][v * Np + ns_face_node<N>(lf, b)];
+					an += B.I[i * N + b] * a.u[((size_t)en * kNsNV + v) * Np + ns_face_node<N>(fn, b)];
+				}
+				B.TRo[sl][(lf * kNsNV + v) * N + i] = ao;
+				B.TRn[sl][(lf * kNsNV + v) * N + i] = an;
+			}
+		}
+	}
+	__syncthreads();
+}
+
+template <int N>
+__global__ void __launch_bounds__(128) k_ns_grad(NsArgs a)
+{
+	constexpr int Np = N * N, EPB = ns_epb<N>();
+	__shared__ NsBlock<N> B;
+	__shared__ double Gv[EPB][2 * kNsNV * Np];
+	__shared__ double QF[EPB][4 * 2 * kNsNV * N];   // J_f n_d (s_num - s_own) per face, own ordering
+	const int sl = threadIdx.x / Np, t = threadIdx.x % Np;
+	const int e = blockIdx.x * EPB + sl;
+	const bool valid = sl < EPB && e < a.E;
+	const int slc = sl < EPB ? sl : 0;
+	ns_load<N>(B, a, slc, t, valid ? e : 0, valid);
+	const int c0 = t % N, c1 = t / N;
+	double Jq = 1.0;
+	if (valid) {
+		Jq = a.detJ[(size_t)e * Np + t];
+		const double* m = a.met + (size_t)e * 4 * Np;
+		const double m00 = m[0 * Np + t], m10 = m[1 * Np + t], m01 = m[2 * Np + t], m11 = m[3 * Np + t];
+#pragma unroll
+		for (int v = 0; v < kNsNV; ++v) {
+			double d0 = 0.0, d1 = 0.0;   // d/dxi_0, d/dxi_1 at the GL node (GL-collocated derivative)
+#pragma unroll
+			for (int mm = 0; mm < N; ++mm) {
+				d0 += B.DhT[mm * N + c0] * B.V[slc][v * Np + mm + N * c1];
+				d1 += B.DhT[mm * N + c1] * B.V[slc][v * Np + c0 + N * mm];
+			}
+			// m[(i + 2 j)] = J dxi_j / dx_i:  J ds/dx_i = sum_j m(i, j) ds/dxi_j
+			Gv[slc][(0 * kNsNV + v) * Np + t] = (m00 * d0 + m01 * d1) / Jq;
+			Gv[slc][(1 * kNsNV + v) * Np + t] = (m10 * d0 + m11 * d1) / Jq;
+		}
+		// face products in the left ordering, stored at the own index
+		for (int item = t; item < 4 * N; item += Np) {
+			const int lf = item / N, i = item % N;
+			const ElemFace f = B.ef[slc][lf];
+			const int side = (f.packed >> 4) & 1, ord = (f.packed >> 8) & 7;
+			const int pi = B.perm[ord][i];
+			const double* own = B.TRo[slc] + lf * kNsNV * N;
+			const double* nb = B.TRn[slc] + lf * kNsNV * N;
+			const double* nJ = a.nJ + ((size_t)f.f * N + i) * 2;
+			const int dst = side == 0 ? i : pi;
+#pragma unroll
+			for (int v = 0; v < kNsNV; ++v) {
+				const double sL = side == 0 ? own[v * N + i] : nb[v * N + i];
+				const double sR = side == 0 ? nb[v * N + pi] : own[v * N + pi];
+				const double dl = 0.5 * (sR - sL);
+				QF[slc][((lf * 2 + 0) * kNsNV + v) * N + dst] = nJ[0] * dl;
+				QF[slc][((lf * 2 + 1) * kNsNV + v) * N + dst] = nJ[1] * dl;
+			}
+		}
+	}
+	__syncthreads();
+	if (valid) {
+		const double wq = B.w[c0] * B.w[c1];
+		const double sc = 1.0 / (wq * Jq);
+#pragma unroll
+		for (int d = 0; d < 2; ++d)
+#pragma unroll
+			for (int v = 0; v < kNsNV; ++v) {
+				double g = Gv[slc][(d * kNsNV + v) * Np + t];
+#pragma unroll
+				for (int lf = 0; lf < 4; ++lf) {
+					const int fd = lf >> 1, cn = fd == 0 ? c0 : c1, at = fd == 0 ? c1 : c0;
+					g += B.Lift[lf & 1][cn] * B.w[at] * QF[slc][((lf * 2 + d) * kNsNV + v) * N + at] * sc;
+				}
+				a.gq[(((size_t)e * 2 + d) * kNsNV + v) * Np + t] = g;
+			}
+		// face gradient traces of G_v + eta G_f(face): items (lf, d, v, a)
+		constexpr double eta = 1.01 * 4.0;
+		for (int item = t; item < 4 * 2 * kNsNV * N; item += Np) {
+			const int at = item % N, r = item / N, v = r % kNsNV, d = (r / kNsNV) % 2, lf = r / (2 * kNsNV);
+			const int fd = lf >> 1, es = lf & 1;   // normal direction and end (xi = -1 / +1) of the face
+			const double qf = QF[slc][((lf * 2 + d) * kNsNV + v) * N + at];
+			double acc = 0.0;
+#pragma unroll
+			for (int c = 0; c < N; ++c) {
+				const int q = fd == 0 ? c + N * at : at + N * c;
+				const double Jc = a.detJ[(size_t)e * Np + q];
+				const double gf = B.Lift[es][c] * B.w[at] * qf / (B.w[c] * B.w[at] * Jc);
+				acc += B.Lift[es][c] * (Gv[slc][(d * kNsNV + v) * Np + q] + eta * gf);
+			}
+			a.gface[((((size_t)e * 4 + lf) * 2 + d) * kNsNV + v) * N + at] = acc;
+		}
+	}
+}
+
+template <int N, int FLUX>
+__global__ void __launch_bounds__(128) k_rhs_ns(NsArgs a)
+{
+	constexpr int Np = N * N, EPB = ns_epb<N>();
+	__shared__ NsBlock<N> B;
+	__shared__ double X[EPB][2 * kNsNV * Np];      // W fr_d at the GL nodes
+	__shared__ double RES[EPB][4 * kNsNV * N];     // J_f nnf per face, own ordering
+	const int sl = threadIdx.x / Np, t = threadIdx.x % Np;
+	const int e = blockIdx.x * EPB + sl;
+	const bool valid = sl < EPB && e < a.E;
+	const int slc = sl < EPB ? sl : 0;
+	ns_load<N>(B, a, slc, t, valid ? e : 0, valid);
+	const int c0 = t % N, c1 = t / N;
+	if (valid) {
+		double s[kNsNV], g[2 * kNsNV], f[2 * kNsNV], fv[2 * kNsNV];
+#pragma unroll
+		for (int v = 0; v < kNsNV; ++v) s[v] = B.V[slc][v * Np + t];
+#pragma unroll
+		for (int d = 0; d < 2; ++d)
+#pragma unroll
+			for (int v = 0; v < kNsNV; ++v) g[v + kNsNV * d] = a.gq[(((size_t)e * 2 + d) * kNsNV + v) * Np + t];
+		euler_flux<double, 2>(s, f);
+		ns_viscous_flux2(s, g, a.mu, a.Pr, fv);
+#pragma unroll
+		for (int k = 0; k < 2 * kNsNV; ++k) f[k] += fv[k];
+		const double* m = a.met + (size_t)e * 4 * Np;
+		const double wq = B.w[c0] * B.w[c1];
+#pragma unroll
+		for (int d0 = 0; d0 < 2; ++d0)
+#pragma unroll
+			for (int v = 0; v < kNsNV; ++v) {
+				// fr_d0 = sum_d1 m(d1, d0) f_d1, m[(i + 2 j)] = J dxi_j / dx_i
+				const double fr = m[(0 + 2 * d0) * Np + t] * f[0 + 2 * v] + m[(1 + 2 * d0) * Np + t] * f[1 + 2 * v];
+				X[slc][(d0 * kNsNV + v) * Np + t] = wq * fr;
+			}
+		// face numerical flux, items in the left ordering (as the Euler kernel)
+		for (int item = t; item < 4 * N; item += Np) {
+			const int lf = item / N, i = item % N;
+			const ElemFace fc = B.ef[slc][lf];
+			const int side = (fc.packed >> 4) & 1, ord = (fc.packed >> 8) & 7, fn = fc.packed & 15;
+			const int pi = B.perm[ord][i];
+			const double* own = B.TRo[slc] + lf * kNsNV * N;
+			const double* nb = B.TRn[slc] + lf * kNsNV * N;
+			const double* go = a.gface + ((size_t)e * 4 + lf) * 2 * kNsNV * N;
+			const double* gn = a.gface + ((size_t)(fc.en >= 0 ? fc.en : e) * 4 + fn) * 2 * kNsNV * N;
+			double sL[kNsNV], sR[kNsNV], gL[2 * kNsNV], gR[2 * kNsNV];
+#pragma unroll
+			for (int v = 0; v < kNsNV; ++v) {
+				sL[v] = side == 0 ? own[v * N + i] : nb[v * N + i];
+				sR[v] = side == 0 ? nb[v * N + pi] : own[v * N + pi];
+#pragma unroll
+				for (int d = 0; d < 2; ++d) {
+					gL[v + kNsNV * d] = side == 0 ? go[(d * kNsNV + v) * N + i] : gn[(d * kNsNV + v) * N + i];
+					gR[v + kNsNV * d] = side == 0 ? gn[(d * kNsNV + v) * N + pi] : go[(d * kNsNV + v) * N + pi];
+				}
+			}
+			const double* nJ = a.nJ + ((size_t)fc.f * N + i) * 2;
+			const double Jf = sqrt(nJ[0] * nJ[0] + nJ[1] * nJ[1]);
+			const double nu[2] = { nJ[0] / Jf, nJ[1] / Jf };
+			double nnf[kNsNV], fvL[2 * kNsNV], fvR[2 * kNsNV];
+			if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<double, 2>(sL, sR, nu, nnf);
+			else numflux_roe<double, 2>(sL, sR, nu, nnf);
+			ns_viscous_flux2(sL, gL, a.mu, a.Pr, fvL);
+			ns_viscous_flux2(sR, gR, a.mu, a.Pr, fvR);
+			const int dst = side == 0 ? i : pi;
+#pragma unroll
+			for (int v = 0; v < kNsNV; ++v) {
+				const double vis = nu[0] * 0.5 * (fvL[0 + 2 * v] + fvR[0 + 2 * v]) + nu[1] * 0.5 * (fvL[1 + 2 * v] + fvR[1 + 2 * v]);
+				const double r = (nnf[v] + vis) * Jf;
+				RES[slc][(lf * kNsNV + v) * N + dst] = side == 0 ? r : -r;
+			}
+		}
+	}
+	__syncthreads();
+	double z[kNsNV];
+	if (valid) {
+		// weak divergence sum_d Dh_d^T (W fr_d), minus the face lifts, at the GL nodes
+#pragma unroll
+		for (int v = 0; v < kNsNV; ++v) {
+			double acc = 0.0;
+#pragma unroll
+			for (int mm = 0; mm < N; ++mm) {
+				acc += B.DhT[c0 * N + mm] * X[slc][(0 * kNsNV + v) * Np + mm + N * c1];
+				acc += B.DhT[c1 * N + mm] * X[slc][(1 * kNsNV + v) * Np + c0 + N * mm];
+			}
+#pragma unroll
+			for (int lf = 0; lf < 4; ++lf) {
+				const int fd = lf >> 1, cn = fd == 0 ? c0 : c1, at = fd == 0 ? c1 : c0;
+				acc -= B.Lift[lf & 1][cn] * B.w[at] * RES[slc][(lf * kNsNV + v) * N + at];
+			}
+			z[v] = acc;
+		}
+	}
+	__syncthreads();
+	if (valid)
+#pragma unroll
+		for (int v = 0; v < kNsNV; ++v) B.V[slc][v * Np + t] = z[v];
+	__syncthreads();
+	// back to the GLL test functions: Ibar^T = I^T along s, then along r
+	if (valid)
+#pragma unroll
+		for (int v = 0; v < kNsNV; ++v) {
+			double acc = 0.0;
+#pragma unroll
+			for (int mm = 0; mm < N; ++mm) acc += B.I[mm * N + c1] * B.V[slc][v * Np + c0 + N * mm];
+			z[v] = acc;
+		}
+	__syncthreads();
+	if (valid)
+#pragma unroll
+		for (int v = 0; v < kNsNV; ++v) B.V[slc][v * Np + t] = z[v];
+	__syncthreads();
+	if (valid)
+#pragma unroll
+		for (int v = 0; v < kNsNV; ++v) {
+			double acc = 0.0;
+#pragma unroll
+			for (int mm = 0; mm < N; ++mm) acc += B.I[mm * N + c0] * B.V[slc][v * Np + mm + N * c1];
+			a.out[((size_t)e * kNsNV + v) * Np + t] = acc;
+		}
+}
+
+template <int N>
+hipError_t run_ns(Ctx& c, const NsArgs& a)
+{
+	constexpr int EPB = ns_epb<N>(), Np = N * N;
+	const int grid = (c.E + EPB - 1) / EPB;
+	hipLaunchKernelGGL(k_ns_grad<N>, dim3(grid), dim3(EPB * Np), 0, c.stream, a);
+	DGX_HIP_CHECK(hipGetLastError());
+	if (c.cfg.num_flux == DGX_FLUX_LF)
+		hipLaunchKernelGGL((k_rhs_ns<N, DGX_FLUX_LF>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
+	else
+		hipLaunchKernelGGL((k_rhs_ns<N, DGX_FLUX_ROE>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
+	return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_rhs_ns(Ctx& c, const double* u, double* r)
+{
+	if (c.d != 2 || c.cfg.pde != DGX_PDE_NAVIER_STOKES || !c.ns_grad || !c.ns_gface) return hipErrorInvalidValue;
+	if (c.E == 0) return hipSuccess;
+	NsArgs a{};
+	a.u = u;
+	a.out = r;
+	a.gq = c.ns_grad;
+	a.gface = c.ns_gface;
+	a.met = c.metrics_vc;
+	a.detJ = c.detJ_vc;
+	a.nJ = c.nJ_fc;
+	a.ef = c.elem_faces;
+	a.ops = c.ops64;
+	a.mu = c.cfg.mu;
+	a.Pr = c.cfg.Pr;
+	a.E = c.E;
+	switch (c.N) {
+	case 2: return run_ns<2>(c, a);
+	case 3: return run_ns<3>(c, a);
+	case 4: return run_ns<4>(c, a);
+	case 5: return run_ns<5>(c, a);
+	case 6: return run_ns<6>(c, a);
+	default: return hipErrorInvalidValue;
+	}
+}
+
+}  // namespace dgx

@@ -25,6 +25,13 @@ static RhsArgs<double, double> make_args64(Ctx& c, const double* u, double* r, u
 
 hipError_t launch_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned long long* dmax)
 {
+	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) {
+		// gradient and residual kernels, then M^-1 and the max-norm for the explicit path
+		DGX_HIP_CHECK(launch_rhs_ns(c, u, r));
+		if (mode != MODE_RHS_MINV) return hipSuccess;
+		DGX_HIP_CHECK(launch_minv(c, r));
+		return dmax ? launch_maxabs(c, r, (size_t)c.E * c.Np * c.nv, dmax) : hipSuccess;
+	}
 	RhsArgs<double, double> a = make_args64(c, u, r, dmax);
 	a.grid = c.rhs_grid[mode == MODE_RHS ? TUNE_RHS64 : TUNE_RHS64_MINV];
 	if (a.grid > 0) return dispatch_rhs_persistent(c, a, mode);

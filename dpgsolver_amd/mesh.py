@@ -24,7 +24,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 BC_INTERIOR, BC_RIEMANN, BC_SLIPWALL = 0, 1, 2
-PDE_ADVECTION, PDE_EULER = 0, 1
+PDE_ADVECTION, PDE_EULER, PDE_NAVIER_STOKES = 0, 1, 2
 FLUX_UPWIND, FLUX_LF, FLUX_ROE = 0, 1, 2
 GAMMA, GM1 = 1.4, 0.4
 

@@ -32,7 +32,9 @@ enum {
 	DGX_ERR_NO_MESH = -7
 };
 
-enum { DGX_PDE_ADVECTION = 0, DGX_PDE_EULER = 1 };
+/* DGX_PDE_NAVIER_STOKES: Euler + BR2 viscous terms (2-D QUAD, interior faces only: periodic meshes; FP64
+ * residual and explicit path), constant viscosity mu and Prandtl number Pr from the config. */
+enum { DGX_PDE_ADVECTION = 0, DGX_PDE_EULER = 1, DGX_PDE_NAVIER_STOKES = 2 };
 /* num_flux_1st: upwind | Lax-Friedrichs | Roe-Pike (numerical_flux/definitions_numerical_flux.h:22-32) */
 enum { DGX_FLUX_UPWIND = 0, DGX_FLUX_LF = 1, DGX_FLUX_ROE = 2 };
 enum { DGX_BC_INTERIOR = 0, DGX_BC_RIEMANN = 1, DGX_BC_SLIPWALL = 2 };
@@ -44,6 +46,8 @@ typedef struct dgx_config {
 	int num_flux;       /* DGX_FLUX_* (num_flux_1st) */
 	double b_adv[3];    /* advection velocity (solution.data b_adv) */
 	double bc_prim[5];  /* free-stream primitive state for Riemann BCs (rho, u, v, [w], p) */
+	double mu;          /* DGX_PDE_NAVIER_STOKES: constant viscosity (compute_mu_constant_T, solution.data mu) */
+	double Pr;          /* DGX_PDE_NAVIER_STOKES: Prandtl number (solution.data Pr) */
 } dgx_config;
 
 typedef struct dgx_mesh {
