@@ -178,6 +178,47 @@ def time_to_steady(torch, dev, epochs=60):
     return out
 
 
+def ns_throughput(torch, dev, steps, warmup, n=448):
+    """2-D Navier-Stokes (BR2) FP64 residual on the QUAD part of the mixed-mesh workload: periodic [0,1]^2,
+    n x n elements, p = 3, Roe, Pr 0.72, Re 100 on the velocity amplitude 0.28 and unit length (mu = 0.0028);
+    state: the 3-D workload's formula in 2-D.  One evaluation = the gradient kernel + the residual kernel,
+    replayed as one HIP graph."""
+    from dpgsolver_amd import _lib, mesh as M
+
+    p, mu, Pr = 3, 0.28 / 100.0, 0.72
+    mesh = M.config4_mesh(n, p)
+    ctx = _lib.Context(2, p, M.PDE_NAVIER_STOKES, M.FLUX_ROE, mu=mu, Pr=Pr)
+    stream = torch.cuda.current_stream(dev)
+    ctx.set_stream(stream.cuda_stream)
+    ctx.upload_mesh(mesh.xyz, mesh.faces)
+    u = torch.from_numpy(M.config4_state(mesh)).to(dev)
+    r = torch.empty_like(u)
+    g = ctx.graph_rhs(_lib.GRAPH_RHS_FP64, u.data_ptr(), r.data_ptr())
+    for _ in range(warmup):
+        g.launch(1)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    g.launch(steps)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / steps
+    E, Np, nv, d, Nfc = mesh.E, 16, 4, 2, 4
+    ndof = E * Np * nv
+    # compulsory traffic: state in, residual out, FP64 geometry (metrics, det J, n J_f of the owned faces)
+    alg = E * (2 * nv * Np * 8 + Np * (d * d + 1) * 8 + d * Nfc * d * 8)
+    # plus the two-kernel design's gradient round trip (G at the GL nodes and the face traces, written, read)
+    grad_rt = E * 2 * (d * nv * Np * 8 + 2 * d * d * nv * Nfc * 8)
+    g.close()
+    ctx.close()
+    return {"workload": f"2-D Navier-Stokes BR2 p={p}, {n}x{n} periodic QUAD ({E} elements), Roe, Re 100, Pr 0.72",
+            "dtype": "f64", "value": round(ndof / (ms * 1e-3) / 1e9, 3), "unit": "GDOF/s", "ms_per_step": round(ms, 5),
+            "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "traffic": None},
+            "gradient_round_trip_bytes": grad_rt}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -297,6 +338,8 @@ def main():
     achmx = bytesmx / (msmx * 1e-3) / 1e9
     gdofmx = ndof / (msmx * 1e-3) / 1e9
 
+    ns = ns_throughput(torch, dev, args.steps, args.warmup)
+    print(f"navier-stokes residual: {ns['ms_per_step']:.4f} ms, {ns['value']} GDOF/s", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline()
     implicit = None if args.no_implicit else time_to_steady(torch, dev)
 
@@ -330,6 +373,7 @@ def main():
         "fp32_rel_err_vs_fp64": err32,
         "fp32_input_rounding_rel_err": err_in,
         "fp64_free_stream_max_abs": free_stream,
+        "navier_stokes": ns,
         "cpu_baseline": cpu,
         "time_to_1e-10": implicit,
     }

@@ -40,7 +40,7 @@ void free_mesh(Ctx& c)
 	release(c.xyz); release(c.faces); release(c.elem_faces); release(c.order); release(c.hmin); release(c.metrics_vc); release(c.metrics_vc32);
 	release(c.detJ_vc); release(c.detJ_vc32); release(c.nJ_fc); release(c.nJ_fc32); release(c.u0); release(c.rtmp);
 	release(c.color_elems);
-	release(c.ns_grad); release(c.ns_gface);
+	release(c.ns_gvt); release(c.ns_kappa);
 	c.color_off.clear();
 	for (int& g : c.rhs_grid) g = 0;   // launch shapes are tuned per mesh
 	c.mesh_ready = false;
@@ -211,9 +211,9 @@ dgx_status dgx_upload_mesh(dgx_ctx* h, const dgx_mesh* m)
 		DGX_TRY(hipMemcpy(c.color_elems, elems.data(), sizeof(int) * c.E, hipMemcpyHostToDevice));
 	}
 	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) {
-		// BR2 gradient at the GL nodes (E x d x nv x Nc) and its face traces (E x 2d x d x nv x Nfc)
-		DGX_TRY(hipMalloc(&c.ns_grad, sizeof(double) * (size_t)c.E * d * c.nv * c.Nc));
-		DGX_TRY(hipMalloc(&c.ns_gface, sizeof(double) * (size_t)c.E * NFACE * d * c.nv * c.Nfc));
+		// face traces of the volume gradient and the BR2 lift trace factors
+		DGX_TRY(hipMalloc(&c.ns_gvt, sizeof(double) * (size_t)c.E * NFACE * d * c.nv * c.Nfc));
+		DGX_TRY(hipMalloc(&c.ns_kappa, sizeof(double) * (size_t)c.E * NFACE * c.Nfc));
 	}
 	DGX_TRY(dgx::launch_geometry(c));
 	DGX_TRY(dgx::launch_convert(c.metrics_vc, c.metrics_vc32, nm, c.stream));

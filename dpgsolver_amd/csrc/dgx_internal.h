@@ -59,8 +59,8 @@ struct Ctx {
 	int* color_elems = nullptr;     // E: elements grouped by colour (no two face neighbours share a colour)
 	std::vector<int> color_off;     // n_colors + 1 offsets into color_elems
 	dgx_mlp* mlp = nullptr;         // attached learned preconditioner (not owned)
-	double* ns_grad = nullptr;      // Navier-Stokes: BR2 gradient at the GL nodes (E x d x nv x Nc)
-	double* ns_gface = nullptr;     // Navier-Stokes: face gradient traces (E x 2d x d x nv x Nfc)
+	double* ns_gvt = nullptr;       // Navier-Stokes: face traces of the volume gradient (E x 2d x d x nv x Nfc)
+	double* ns_kappa = nullptr;     // Navier-Stokes: BR2 lift trace factor per face node (E x 2d x Nfc)
 	int rhs_grid[4] = { 0, 0, 0, 0 };   // residual launch grids by TUNE_* (0 = one workgroup per element block)
 	double* metrics_vc = nullptr;   // E*d*d*Nc  (m[(i + d*j)*Nc + n] = J dxi_j/dx_i)
 	float* metrics_vc32 = nullptr;

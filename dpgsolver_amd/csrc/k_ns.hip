@@ -1,19 +1,22 @@
 // k_ns.hip -- 2-D Navier-Stokes residual with BR2 viscous terms (QUAD, interior faces), FP64.
 //
-// Two element-centric kernels, one thread per volume cubature (GL) node:
-//   k_ns_grad: the weak gradient of the state (compute_grad_coef_dg_T.c:171-307, 'e' branch), formed at the GL
-//     nodes.  With M = Ibar^T W J Ibar, M^-1 Ibar^T W x = Ibar^-1 (x / J), so the projected volume gradient is
-//     the pointwise (J grad s) / J at the GL nodes, and the face term M^-1 tw0_vt_fc (J_f n (s_num - s)) is the
-//     face lift of the residual kernel divided by W J:  G_f = Lift_side(c) W_f(a) J_f n (s_num - s)(a) / (W J).
-//     s_num is central, s_num - s_own = (s_nb - s_own) / 2.  Stored: the full gradient G = G_v + sum_faces G_f
-//     at the GL nodes (the volume flux reads it there: cv0_vs_vc grad_coef = Ibar Ibar^-1 G), and per face the
-//     trace of G_v + 1.01 NFMAX G_f(face) at the face GL nodes (constructor_partial_grad_fc_interp,
-//     compute_face_rlhs_dg_T.c:243-300; NFMAX = 4 in 2-D).  The trace of GL nodal values at xi = -1 / +1 is
-//     Lift_side = I^-T e_side (the GL Lagrange basis at the end points).
-//   k_rhs_ns: volume flux F_euler(s) + fv(s, G) (fv the negated viscous flux, compute_Flux_navier_stokes_0,
-//     flux_navier_stokes_T.c:1005-1040), weak divergence, then per face the inviscid numerical flux plus the
-//     central average n . (fv_L + fv_R) / 2 of the two sides' face gradients (numerical_flux_central_T.c:76-104),
-//     scaled by J_f and lifted; both sides of a face evaluate it from identical inputs in the left ordering.
+// The weak gradient (compute_grad_coef_dg_T.c:171-307, 'e' branch) is formed at the volume cubature (GL)
+// nodes.  With M = Ibar^T W J Ibar, M^-1 Ibar^T W x = Ibar^-1 (x / J), so the projected volume gradient is the
+// pointwise G_v = (J grad s) / J, and the face term M^-1 tw0_vt_fc (J_f n (s_num - s)) is the residual kernel's
+// face lift divided by W J:  G_f(c, a) = Lift_side(c) q(a) / (w_c J(c, a)),  q = J_f n (s_nb - s_own) / 2
+// (central s_num).  The volume flux uses G = G_v + sum_faces G_f at the GL nodes (cv0_vs_vc grad_coef =
+// Ibar Ibar^-1 G); each side of a face uses the trace of G_v + 1.01 NFMAX G_f(face) (NFMAX = 4 in 2-D;
+// constructor_partial_grad_fc_interp, compute_face_rlhs_dg_T.c:243-300).  The trace of GL nodal values at
+// xi = -1 / +1 is Lift_side = I^-T e_side, so the face part of that trace is q(a) kappa(a) with the geometric
+// factor kappa(a) = sum_c Lift_side(c)^2 / (w_c J(c, a)).
+//
+//   k_ns_gv:  per element, the face traces of G_v and kappa (its own state and geometry only).
+//   k_rhs_ns: G_v again, the face products q from the own and neighbour traces, G for the volume flux
+//     F_euler(s) + fv(s, G) (fv the negated viscous flux, compute_Flux_navier_stokes_0,
+//     flux_navier_stokes_T.c:1005-1040), the weak divergence, and per face the inviscid numerical flux plus the
+//     central n . (fv_L + fv_R) / 2 (numerical_flux_central_T.c:76-104) of the two sides' face gradients
+//     trace(G_v) + eta kappa q, scaled by J_f and lifted.  Both sides of a face evaluate it from identical
+//     inputs in the left ordering (the traces of G_v and kappa are read back from k_ns_gv's output by both).
 #include "dgx_internal.h"
 #include "dgx_device.h"
 
@@ -70,9 +73,9 @@ __device__ inline void ns_viscous_flux2(const double* s, const double* g, double
 
 struct NsArgs {
 	const double* u;
-	double* out;           // k_rhs_ns: residual
-	double* gq;            // E x 2 x 4 x Np: full gradient at the GL nodes
-	double* gface;         // E x 4 x 2 x 4 x N: face gradient traces (own ordering, face GL nodes)
+	double* out;           // residual
+	double* gvt;           // E x 4 x 2 x 4 x N: traces of G_v on the faces (own face ordering)
+	double* kap;           // E x 4 x N: kappa per face node
 	const double* met;     // E x 4 x Np
 	const double* detJ;    // E x Np
 	const double* nJ;      // F x N x 2
@@ -82,7 +85,7 @@ struct NsArgs {
 	int E;
 };
 
-// shared per-element set-up: operators, the GLL state, its GL values and the GL face traces (own and neighbour)
+// shared per-element set-up: operators, the GLL state, its GL values and (k_rhs_ns) the GL face traces
 template <int N>
 struct NsBlock {
 	static constexpr int Np = N * N, EPB = ns_epb<N>();
@@ -90,8 +93,6 @@ struct NsBlock {
 	int perm[8][N];
 	double U[EPB][kNsNV * Np];      // GLL nodal state
 	double V[EPB][kNsNV * Np];      // GL values
-	double TRo[EPB][4 * kNsNV * N]; // own face traces at the face GL nodes, own ordering
-	double TRn[EPB][4 * kNsNV * N]; // neighbour face traces, the neighbour's ordering
 	ElemFace ef[EPB][4];
 };
 
@@ -139,13 +140,98 @@ __device__ inline void ns_load(NsBlock<N>& B, const NsArgs& a, int sl, int t, in
 		}
 	}
 	__syncthreads();
-	if (valid) {
+	if (valid)
 #pragma unroll
 		for (int v = 0; v < kNsNV; ++v) B.V[sl][v * Np + t] = tmp[v];
+	__syncthreads();
+}
+
+// G_v = (J grad s) / J at GL node t (c0, c1): g[v + 4 d]; m[(i + 2 j)] = J dxi_j / dx_i
+template <int N>
+__device__ inline void ns_volume_gradient(const NsBlock<N>& B, const NsArgs& a, int sl, int t, int e, double* g)
+{
+	constexpr int Np = N * N;
+	const int c0 = t % N, c1 = t / N;
+	const double* m = a.met + (size_t)e * 4 * Np;
+	const double m00 = m[0 * Np + t], m10 = m[1 * Np + t], m01 = m[2 * Np + t], m11 = m[3 * Np + t];
+	const double Ji = 1.0 / a.detJ[(size_t)e * Np + t];
+#pragma unroll
+	for (int v = 0; v < kNsNV; ++v) {
+		double d0 = 0.0, d1 = 0.0;   // d/dxi_0, d/dxi_1 (GL-collocated derivative Dh[i][j] = DhT[j N + i])
+#pragma unroll
+		for (int mm = 0; mm < N; ++mm) {
+			d0 += B.DhT[mm * N + c0] * B.V[sl][v * Np + mm + N * c1];
+			d1 += B.DhT[mm * N + c1] * B.V[sl][v * Np + c0 + N * mm];
+		}
+		g[v + 4 * 0] = (m00 * d0 + m01 * d1) * Ji;
+		g[v + 4 * 1] = (m10 * d0 + m11 * d1) * Ji;
+	}
+}
+
+template <int N>
+__global__ void __launch_bounds__(128) k_ns_gv(NsArgs a)
+{
+	constexpr int Np = N * N, EPB = ns_epb<N>();
+	__shared__ NsBlock<N> B;
+	__shared__ double Gv[EPB][2 * kNsNV * Np];
+	__shared__ double WJi[EPB][Np];   // 1 / (w_c0 w_c1 J)
+	const int sl = threadIdx.x / Np, t = threadIdx.x % Np;
+	const int e = blockIdx.x * EPB + sl;
+	const bool valid = sl < EPB && e < a.E;
+	const int slc = sl < EPB ? sl : 0;
+	ns_load<N>(B, a, slc, t, valid ? e : 0, valid);
+	if (valid) {
+		double g[2 * kNsNV];
+		ns_volume_gradient<N>(B, a, slc, t, e, g);
+#pragma unroll
+		for (int k = 0; k < 2 * kNsNV; ++k) Gv[slc][(k / kNsNV * kNsNV + k % kNsNV) * Np + t] = g[k];
+		WJi[slc][t] = 1.0 / (B.w[t % N] * B.w[t / N] * a.detJ[(size_t)e * Np + t]);
+	}
+	__syncthreads();
+	if (valid) {
+		// traces of G_v: items (lf, d, v, a) in the output order
+		for (int item = t; item < 4 * 2 * kNsNV * N; item += Np) {
+			const int at = item % N, r = item / N, v = r % kNsNV, d = (r / kNsNV) % 2, lf = r / (2 * kNsNV);
+			const int fd = lf >> 1, es = lf & 1;   // normal direction and end (xi = -1 / +1) of the face
+			double acc = 0.0;
+#pragma unroll
+			for (int c = 0; c < N; ++c) acc += B.Lift[es][c] * Gv[slc][(d * kNsNV + v) * Np + (fd == 0 ? c + N * at : at + N * c)];
+			a.gvt[(size_t)e * 4 * 2 * kNsNV * N + item] = acc;
+		}
+		// kappa(a) = sum_c Lift(c)^2 / (w_c J(c, a)) = sum_c Lift(c)^2 w_a WJi(c, a)
+		for (int item = t; item < 4 * N; item += Np) {
+			const int at = item % N, lf = item / N, fd = lf >> 1, es = lf & 1;
+			double acc = 0.0;
+#pragma unroll
+			for (int c = 0; c < N; ++c)
+				acc += B.Lift[es][c] * B.Lift[es][c] * B.w[at] * WJi[slc][fd == 0 ? c + N * at : at + N * c];
+			a.kap[(size_t)e * 4 * N + item] = acc;
+		}
+	}
+}
+
+template <int N, int FLUX>
+__global__ void __launch_bounds__(128) k_rhs_ns(NsArgs a)
+{
+	constexpr int Np = N * N, EPB = ns_epb<N>();
+	constexpr double eta = 1.01 * 4.0;
+	__shared__ NsBlock<N> B;
+	__shared__ double TRo[EPB][4 * kNsNV * N];     // own face traces at the face GL nodes, own ordering
+	__shared__ double TRn[EPB][4 * kNsNV * N];     // neighbour face traces, the neighbour's ordering
+	__shared__ double QF[EPB][4 * 2 * kNsNV * N];  // q = J_f n_d (s_nb - s_own) / 2 per face, own ordering
+	__shared__ double X[EPB][2 * kNsNV * Np];      // W fr_d at the GL nodes
+	__shared__ double RES[EPB][4 * kNsNV * N];     // J_f nnf per face, own ordering
+	const int sl = threadIdx.x / Np, t = threadIdx.x % Np;
+	const int e = blockIdx.x * EPB + sl;
+	const bool valid = sl < EPB && e < a.E;
+	const int slc = sl < EPB ? sl : 0;
+	ns_load<N>(B, a, slc, t, valid ? e : 0, valid);
+	const int c0 = t % N, c1 = t / N;
+	if (valid) {
 		// face traces: GLL face values (own from LDS, neighbour gathered), then I along the face
 		for (int item = t; item < 4 * N; item += Np) {
 			const int lf = item / N, i = item % N;
-			const ElemFace f = B.ef[sl][lf];
+			const ElemFace f = B.ef[slc][lf];
 			const int fn = f.packed & 15;
 			const int en = f.en >= 0 ? f.en : e;   // every face is interior (checked at mesh upload)
 #pragma unroll
@@ -153,55 +239,24 @@ __device__ inline void ns_load(NsBlock<N>& B, const NsArgs& a, int sl, int t, in
 				double ao = 0.0, an = 0.0;
 #pragma unroll
 				for (int b = 0; b < N; ++b) {
-					ao += B.I[i * N + b] * B.U[sl][v * Np + ns_face_node<N>(lf, b)];
+					ao += B.I[i * N + b] * B.U[slc][v * Np + ns_face_node<N>(lf, b)];
 					an += B.I[i * N + b] * a.u[((size_t)en * kNsNV + v) * Np + ns_face_node<N>(fn, b)];
 				}
-				B.TRo[sl][(lf * kNsNV + v) * N + i] = ao;
-				B.TRn[sl][(lf * kNsNV + v) * N + i] = an;
+				TRo[slc][(lf * kNsNV + v) * N + i] = ao;
+				TRn[slc][(lf * kNsNV + v) * N + i] = an;
 			}
 		}
 	}
 	__syncthreads();
-}
-
-template <int N>
-__global__ void __launch_bounds__(128) k_ns_grad(NsArgs a)
-{
-	constexpr int Np = N * N, EPB = ns_epb<N>();
-	__shared__ NsBlock<N> B;
-	__shared__ double Gv[EPB][2 * kNsNV * Np];
-	__shared__ double QF[EPB][4 * 2 * kNsNV * N];   // J_f n_d (s_num - s_own) per face, own ordering
-	const int sl = threadIdx.x / Np, t = threadIdx.x % Np;
-	const int e = blockIdx.x * EPB + sl;
-	const bool valid = sl < EPB && e < a.E;
-	const int slc = sl < EPB ? sl : 0;
-	ns_load<N>(B, a, slc, t, valid ? e : 0, valid);
-	const int c0 = t % N, c1 = t / N;
-	double Jq = 1.0;
 	if (valid) {
-		Jq = a.detJ[(size_t)e * Np + t];
-		const double* m = a.met + (size_t)e * 4 * Np;
-		const double m00 = m[0 * Np + t], m10 = m[1 * Np + t], m01 = m[2 * Np + t], m11 = m[3 * Np + t];
-#pragma unroll
-		for (int v = 0; v < kNsNV; ++v) {
-			double d0 = 0.0, d1 = 0.0;   // d/dxi_0, d/dxi_1 at the GL node (GL-collocated derivative)
-#pragma unroll
-			for (int mm = 0; mm < N; ++mm) {
-				d0 += B.DhT[mm * N + c0] * B.V[slc][v * Np + mm + N * c1];
-				d1 += B.DhT[mm * N + c1] * B.V[slc][v * Np + c0 + N * mm];
-			}
-			// m[(i + 2 j)] = J dxi_j / dx_i:  J ds/dx_i = sum_j m(i, j) ds/dxi_j
-			Gv[slc][(0 * kNsNV + v) * Np + t] = (m00 * d0 + m01 * d1) / Jq;
-			Gv[slc][(1 * kNsNV + v) * Np + t] = (m10 * d0 + m11 * d1) / Jq;
-		}
 		// face products in the left ordering, stored at the own index
 		for (int item = t; item < 4 * N; item += Np) {
 			const int lf = item / N, i = item % N;
 			const ElemFace f = B.ef[slc][lf];
 			const int side = (f.packed >> 4) & 1, ord = (f.packed >> 8) & 7;
 			const int pi = B.perm[ord][i];
-			const double* own = B.TRo[slc] + lf * kNsNV * N;
-			const double* nb = B.TRn[slc] + lf * kNsNV * N;
+			const double* own = TRo[slc] + lf * kNsNV * N;
+			const double* nb = TRn[slc] + lf * kNsNV * N;
 			const double* nJ = a.nJ + ((size_t)f.f * N + i) * 2;
 			const int dst = side == 0 ? i : pi;
 #pragma unroll
@@ -216,66 +271,28 @@ __global__ void __launch_bounds__(128) k_ns_grad(NsArgs a)
 	}
 	__syncthreads();
 	if (valid) {
-		const double wq = B.w[c0] * B.w[c1];
-		const double sc = 1.0 / (wq * Jq);
-#pragma unroll
-		for (int d = 0; d < 2; ++d)
-#pragma unroll
-			for (int v = 0; v < kNsNV; ++v) {
-				double g = Gv[slc][(d * kNsNV + v) * Np + t];
-#pragma unroll
-				for (int lf = 0; lf < 4; ++lf) {
-					const int fd = lf >> 1, cn = fd == 0 ? c0 : c1, at = fd == 0 ? c1 : c0;
-					g += B.Lift[lf & 1][cn] * B.w[at] * QF[slc][((lf * 2 + d) * kNsNV + v) * N + at] * sc;
-				}
-				a.gq[(((size_t)e * 2 + d) * kNsNV + v) * Np + t] = g;
-			}
-		// face gradient traces of G_v + eta G_f(face): items (lf, d, v, a)
-		constexpr double eta = 1.01 * 4.0;
-		for (int item = t; item < 4 * 2 * kNsNV * N; item += Np) {
-			const int at = item % N, r = item / N, v = r % kNsNV, d = (r / kNsNV) % 2, lf = r / (2 * kNsNV);
-			const int fd = lf >> 1, es = lf & 1;   // normal direction and end (xi = -1 / +1) of the face
-			const double qf = QF[slc][((lf * 2 + d) * kNsNV + v) * N + at];
-			double acc = 0.0;
-#pragma unroll
-			for (int c = 0; c < N; ++c) {
-				const int q = fd == 0 ? c + N * at : at + N * c;
-				const double Jc = a.detJ[(size_t)e * Np + q];
-				const double gf = B.Lift[es][c] * B.w[at] * qf / (B.w[c] * B.w[at] * Jc);
-				acc += B.Lift[es][c] * (Gv[slc][(d * kNsNV + v) * Np + q] + eta * gf);
-			}
-			a.gface[((((size_t)e * 4 + lf) * 2 + d) * kNsNV + v) * N + at] = acc;
-		}
-	}
-}
-
-template <int N, int FLUX>
-__global__ void __launch_bounds__(128) k_rhs_ns(NsArgs a)
-{
-	constexpr int Np = N * N, EPB = ns_epb<N>();
-	__shared__ NsBlock<N> B;
-	__shared__ double X[EPB][2 * kNsNV * Np];      // W fr_d at the GL nodes
-	__shared__ double RES[EPB][4 * kNsNV * N];     // J_f nnf per face, own ordering
-	const int sl = threadIdx.x / Np, t = threadIdx.x % Np;
-	const int e = blockIdx.x * EPB + sl;
-	const bool valid = sl < EPB && e < a.E;
-	const int slc = sl < EPB ? sl : 0;
-	ns_load<N>(B, a, slc, t, valid ? e : 0, valid);
-	const int c0 = t % N, c1 = t / N;
-	if (valid) {
+		// G = G_v + sum_faces G_f at this node, then the volume flux
 		double s[kNsNV], g[2 * kNsNV], f[2 * kNsNV], fv[2 * kNsNV];
+		ns_volume_gradient<N>(B, a, slc, t, e, g);
+		const double wq = B.w[c0] * B.w[c1];
+		const double Jq = a.detJ[(size_t)e * Np + t];
+		const double wji = 1.0 / (wq * Jq);
+#pragma unroll
+		for (int lf = 0; lf < 4; ++lf) {
+			const int fd = lf >> 1, cn = fd == 0 ? c0 : c1, at = fd == 0 ? c1 : c0;
+			const double lw = B.Lift[lf & 1][cn] * B.w[at] * wji;
+#pragma unroll
+			for (int d = 0; d < 2; ++d)
+#pragma unroll
+				for (int v = 0; v < kNsNV; ++v) g[v + kNsNV * d] += lw * QF[slc][((lf * 2 + d) * kNsNV + v) * N + at];
+		}
 #pragma unroll
 		for (int v = 0; v < kNsNV; ++v) s[v] = B.V[slc][v * Np + t];
-#pragma unroll
-		for (int d = 0; d < 2; ++d)
-#pragma unroll
-			for (int v = 0; v < kNsNV; ++v) g[v + kNsNV * d] = a.gq[(((size_t)e * 2 + d) * kNsNV + v) * Np + t];
 		euler_flux<double, 2>(s, f);
 		ns_viscous_flux2(s, g, a.mu, a.Pr, fv);
 #pragma unroll
 		for (int k = 0; k < 2 * kNsNV; ++k) f[k] += fv[k];
 		const double* m = a.met + (size_t)e * 4 * Np;
-		const double wq = B.w[c0] * B.w[c1];
 #pragma unroll
 		for (int d0 = 0; d0 < 2; ++d0)
 #pragma unroll
@@ -289,23 +306,30 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs a)
 			const int lf = item / N, i = item % N;
 			const ElemFace fc = B.ef[slc][lf];
 			const int side = (fc.packed >> 4) & 1, ord = (fc.packed >> 8) & 7, fn = fc.packed & 15;
+			const int en = fc.en >= 0 ? fc.en : e;
 			const int pi = B.perm[ord][i];
-			const double* own = B.TRo[slc] + lf * kNsNV * N;
-			const double* nb = B.TRn[slc] + lf * kNsNV * N;
-			const double* go = a.gface + ((size_t)e * 4 + lf) * 2 * kNsNV * N;
-			const double* gn = a.gface + ((size_t)(fc.en >= 0 ? fc.en : e) * 4 + fn) * 2 * kNsNV * N;
+			const double* own = TRo[slc] + lf * kNsNV * N;
+			const double* nb = TRn[slc] + lf * kNsNV * N;
+			// left / right: element, local face, index of node i in that element's face ordering
+			const int eLf = side == 0 ? e : en, lfL = side == 0 ? lf : fn, iL = i;
+			const int eRf = side == 0 ? en : e, lfR = side == 0 ? fn : lf, iR = pi;
+			const double* tvL = a.gvt + ((size_t)eLf * 4 + lfL) * 2 * kNsNV * N;
+			const double* tvR = a.gvt + ((size_t)eRf * 4 + lfR) * 2 * kNsNV * N;
+			const double kL = a.kap[((size_t)eLf * 4 + lfL) * N + iL], kR = a.kap[((size_t)eRf * 4 + lfR) * N + iR];
+			const double* nJ = a.nJ + ((size_t)fc.f * N + i) * 2;
 			double sL[kNsNV], sR[kNsNV], gL[2 * kNsNV], gR[2 * kNsNV];
 #pragma unroll
 			for (int v = 0; v < kNsNV; ++v) {
 				sL[v] = side == 0 ? own[v * N + i] : nb[v * N + i];
 				sR[v] = side == 0 ? nb[v * N + pi] : own[v * N + pi];
+				const double dl = 0.5 * (sR[v] - sL[v]);
 #pragma unroll
 				for (int d = 0; d < 2; ++d) {
-					gL[v + kNsNV * d] = side == 0 ? go[(d * kNsNV + v) * N + i] : gn[(d * kNsNV + v) * N + i];
-					gR[v + kNsNV * d] = side == 0 ? gn[(d * kNsNV + v) * N + pi] : go[(d * kNsNV + v) * N + pi];
+					const double q = nJ[d] * dl;
+					gL[v + kNsNV * d] = tvL[(d * kNsNV + v) * N + iL] + eta * kL * q;
+					gR[v + kNsNV * d] = tvR[(d * kNsNV + v) * N + iR] + eta * kR * q;
 				}
 			}
-			const double* nJ = a.nJ + ((size_t)fc.f * N + i) * 2;
 			const double Jf = sqrt(nJ[0] * nJ[0] + nJ[1] * nJ[1]);
 			const double nu[2] = { nJ[0] / Jf, nJ[1] / Jf };
 			double nnf[kNsNV], fvL[2 * kNsNV], fvR[2 * kNsNV];
@@ -376,7 +400,7 @@ hipError_t run_ns(Ctx& c, const NsArgs& a)
 {
 	constexpr int EPB = ns_epb<N>(), Np = N * N;
 	const int grid = (c.E + EPB - 1) / EPB;
-	hipLaunchKernelGGL(k_ns_grad<N>, dim3(grid), dim3(EPB * Np), 0, c.stream, a);
+	hipLaunchKernelGGL(k_ns_gv<N>, dim3(grid), dim3(EPB * Np), 0, c.stream, a);
 	DGX_HIP_CHECK(hipGetLastError());
 	if (c.cfg.num_flux == DGX_FLUX_LF)
 		hipLaunchKernelGGL((k_rhs_ns<N, DGX_FLUX_LF>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
@@ -389,13 +413,13 @@ hipError_t run_ns(Ctx& c, const NsArgs& a)
 
 hipError_t launch_rhs_ns(Ctx& c, const double* u, double* r)
 {
-	if (c.d != 2 || c.cfg.pde != DGX_PDE_NAVIER_STOKES || !c.ns_grad || !c.ns_gface) return hipErrorInvalidValue;
+	if (c.d != 2 || c.cfg.pde != DGX_PDE_NAVIER_STOKES || !c.ns_gvt || !c.ns_kappa) return hipErrorInvalidValue;
 	if (c.E == 0) return hipSuccess;
 	NsArgs a{};
 	a.u = u;
 	a.out = r;
-	a.gq = c.ns_grad;
-	a.gface = c.ns_gface;
+	a.gvt = c.ns_gvt;
+	a.kap = c.ns_kappa;
 	a.met = c.metrics_vc;
 	a.detJ = c.detJ_vc;
 	a.nJ = c.nJ_fc;

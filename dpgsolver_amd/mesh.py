@@ -400,6 +400,17 @@ def smooth_state_3d(x: np.ndarray, seed: int = SEED) -> np.ndarray:
     return prim
 
 
+def config4_mesh(n: int = 448, p: int = 3) -> Mesh:
+    """2-D Navier-Stokes workload on QUADs only: periodic [0,1]^2, n x n elements, one order p (the
+    configuration's TRI half and per-element orders are not built)."""
+    return structured_tp_mesh(2, p, (n, n), (0.0, 0.0), (1.0, 1.0), (True, True))
+
+
+def config4_state(mesh: Mesh, seed: int = SEED) -> np.ndarray:
+    """The configuration-3 state formula in 2-D, sub-seed 4."""
+    return config3_state(mesh, seed + 1)
+
+
 def config3_state(mesh: Mesh, seed: int = SEED) -> np.ndarray:
     """smooth_state_3d in conserved variables, built directly in the packed [E][nv][Np] layout."""
     d = mesh.d
