@@ -194,24 +194,40 @@ def ns_throughput(torch, dev, steps, warmup, n=448):
     u = torch.from_numpy(M.config4_state(mesh)).to(dev)
     r = torch.empty_like(u)
     g = ctx.graph_rhs(_lib.GRAPH_RHS_FP64, u.data_ptr(), r.data_ptr())
-    for _ in range(warmup):
-        g.launch(1)
-    torch.cuda.synchronize()
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    g.launch(steps)
-    stop.record(stream)
-    torch.cuda.synchronize()
-    ms = start.elapsed_time(stop) / steps
+    u32 = u.float()
+    r32 = torch.empty_like(u32)
+    g32 = ctx.graph_rhs(_lib.GRAPH_RHS_FP32, u32.data_ptr(), r32.data_ptr())
+
+    def timed(graph):
+        for _ in range(warmup):
+            graph.launch(1)
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        graph.launch(steps)
+        stop.record(stream)
+        torch.cuda.synchronize()
+        return start.elapsed_time(stop) / steps
+
+    ms = timed(g)
+    ms32 = timed(g32)
+    err32 = ((r32.double() - r).abs().max() / r.abs().max()).item()
     E, Np, nv, d, Nfc = mesh.E, 16, 4, 2, 4
     ndof = E * Np * nv
     # compulsory traffic: state in, residual out, FP64 geometry (metrics, det J, n J_f of the owned faces)
     alg = E * (2 * nv * Np * 8 + Np * (d * d + 1) * 8 + d * Nfc * d * 8)
     # plus the two-kernel design's gradient round trip (G at the GL nodes and the face traces, written, read)
     grad_rt = E * 2 * (d * nv * Np * 8 + 2 * d * d * nv * Nfc * 8)
+    alg32 = E * (2 * nv * Np * 4 + Np * (d * d + 1) * 4 + d * Nfc * d * 4)
     g.close()
+    g32.close()
     ctx.close()
     return {"workload": f"2-D Navier-Stokes BR2 p={p}, {n}x{n} periodic QUAD ({E} elements), Roe, Re 100, Pr 0.72",
+            "fp32": {"value": round(ndof / (ms32 * 1e-3) / 1e9, 3), "ms_per_step": round(ms32, 5),
+                     "rel_err_vs_fp64": err32,
+                     "roofline": {"bound": "hbm", "achieved": round(alg32 / (ms32 * 1e-3) / 1e9, 1),
+                                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": round(alg32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None}},
             "dtype": "f64", "value": round(ndof / (ms * 1e-3) / 1e9, 3), "unit": "GDOF/s", "ms_per_step": round(ms, 5),
             "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),

@@ -247,7 +247,7 @@ dgx_status dgx_compute_rhs_f32(dgx_ctx* h, const float* u, float* r)
 {
 	if (!h || !u || !r) return DGX_ERR_ARG;
 	if (!h->c.mesh_ready) return DGX_ERR_NO_MESH;
-	if (h->c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;   // FP64 only
+	if (h->c.cfg.pde == DGX_PDE_NAVIER_STOKES) return hip_status(dgx::launch_rhs_ns32(h->c, u, r));
 	return hip_status(dgx::launch_rhs32(h->c, u, r));
 }
 
@@ -379,12 +379,13 @@ dgx_status dgx_graph_create_rhs(dgx_ctx* h, int kind, const void* u, void* r, dg
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
 	if (kind == DGX_GRAPH_RHS_MIXED && c.cfg.pde != DGX_PDE_EULER) return DGX_ERR_UNSUPPORTED;
-	if (kind == DGX_GRAPH_RHS_FP32 && c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;
 	dgx_graph* g = new (std::nothrow) dgx_graph();
 	if (!g) return DGX_ERR_HIP;
 	const dgx_status st = capture_graph(h, g, [&]() -> hipError_t {
 		if (kind == DGX_GRAPH_RHS_FP64)
 			return dgx::launch_rhs64(c, static_cast<const double*>(u), static_cast<double*>(r), dgx::MODE_RHS, nullptr);
+		if (kind == DGX_GRAPH_RHS_FP32 && c.cfg.pde == DGX_PDE_NAVIER_STOKES)
+			return dgx::launch_rhs_ns32(c, static_cast<const float*>(u), static_cast<float*>(r));
 		if (kind == DGX_GRAPH_RHS_FP32)
 			return dgx::launch_rhs32(c, static_cast<const float*>(u), static_cast<float*>(r));
 		return dgx::launch_rhs_mixed(c, static_cast<const double*>(u), static_cast<float*>(r));

@@ -142,14 +142,14 @@ template <typename T>
 __device__ inline T min_abs_real(T a, T b)
 {
 	T c = (fabs(rv(a)) < fabs(rv(b))) ? a : b;
-	if (rv(c) < 0.0) c = T(-1.0) * c;
+	if (rv(c) < 0) c = T(-1.0) * c;
 	return c;
 }
 template <typename T>
 __device__ inline T max_abs_real(T a, T b)
 {
 	T c = (fabs(rv(a)) > fabs(rv(b))) ? a : b;
-	if (rv(c) < 0.0) c = T(-1.0) * c;
+	if (rv(c) < 0) c = T(-1.0) * c;
 	return c;
 }
 
@@ -181,7 +181,7 @@ __device__ inline void numflux_roe(const T* WL, const T* WR, const T* n, T* nnf)
 
 	const T l1 = min_abs_real(VnL - c, Vn - c);
 	const T l5 = max_abs_real(VnR + c, Vn + c);
-	const T l234 = (rv(Vn) > 0.0) ? Vn : T(-1.0) * Vn;
+	const T l234 = (rv(Vn) > 0) ? Vn : T(-1.0) * Vn;
 
 	const T drho = rhoR - rhoL;
 	const T drhou = rhoR * uR - rhoL * uL, drhov = rhoR * vR - rhoL * vL, drhow = rhoR * wR - rhoL * wL;

@@ -1,4 +1,5 @@
-// k_ns.hip -- 2-D Navier-Stokes residual with BR2 viscous terms (QUAD, interior faces), FP64.
+// k_ns.hip -- 2-D Navier-Stokes residual with BR2 viscous terms (QUAD, interior faces), FP64 and FP32
+// (FP32: the FP32 state, geometry shadow and operator tables, as the FP32 Euler residual).
 //
 // The weak gradient (compute_grad_coef_dg_T.c:171-307, 'e' branch) is formed at the volume cubature (GL)
 // nodes.  With M = Ibar^T W J Ibar, M^-1 Ibar^T W x = Ibar^-1 (x / J), so the projected volume gradient is the
@@ -43,61 +44,63 @@ __device__ inline int ns_face_node(int lf, int a)
 // negated viscous flux fv[k + 2 * eq] of state s and gradient g[v + 4 * k] (2-D): tau with the -2/3 mu div V
 // part (compute_tau, flux_navier_stokes_T.c:778-799), scaled temperature gradient d(E/rho) - d(V^2)/2
 // (compute_dTs :904-926), energy flux gamma/Pr mu dTs + u . tau; constant viscosity
-__device__ inline void ns_viscous_flux2(const double* s, const double* g, double mu, double Pr, double* fv)
+template <typename S>
+__device__ inline void ns_viscous_flux2(const S* s, const S* g, S mu, S Pr, S* fv)
 {
-	const double rho = s[0], E = s[3], ri = 1.0 / rho;
-	const double u[2] = { s[1] * ri, s[2] * ri };
-	double du[2][2], tau[2][2], dTs[2];
+	const S rho = s[0], E = s[3], ri = trcp(rho);
+	const S u[2] = { s[1] * ri, s[2] * ri };
+	S du[2][2], tau[2][2], dTs[2];
 #pragma unroll
 	for (int k = 0; k < 2; ++k)
 #pragma unroll
 		for (int j = 0; j < 2; ++j) du[k][j] = ri * (g[(1 + k) + 4 * j] - u[k] * g[0 + 4 * j]);
-	const double divV = du[0][0] + du[1][1];
+	const S divV = du[0][0] + du[1][1];
 #pragma unroll
 	for (int i = 0; i < 2; ++i)
 #pragma unroll
-		for (int j = 0; j < 2; ++j) tau[i][j] = mu * (du[i][j] + du[j][i]) - (i == j ? 2.0 / 3.0 * mu * divV : 0.0);
+		for (int j = 0; j < 2; ++j) tau[i][j] = mu * (du[i][j] + du[j][i]) - (i == j ? S(2.0 / 3.0) * mu * divV : S(0.0));
 #pragma unroll
 	for (int j = 0; j < 2; ++j) {
-		const double dV2 = 2.0 * (u[0] * du[0][j] + u[1] * du[1][j]);
-		dTs[j] = ri * ri * (g[3 + 4 * j] * rho - E * g[0 + 4 * j]) - 0.5 * dV2;
+		const S dV2 = S(2.0) * (u[0] * du[0][j] + u[1] * du[1][j]);
+		dTs[j] = ri * ri * (g[3 + 4 * j] * rho - E * g[0 + 4 * j]) - S(0.5) * dV2;
 	}
 #pragma unroll
 	for (int j = 0; j < 2; ++j) {
-		fv[j + 2 * 0] = 0.0;
+		fv[j + 2 * 0] = S(0.0);
 		fv[j + 2 * 1] = -tau[0][j];
 		fv[j + 2 * 2] = -tau[1][j];
-		fv[j + 2 * 3] = -(kGamma / Pr * mu * dTs[j] + u[0] * tau[j][0] + u[1] * tau[j][1]);
+		fv[j + 2 * 3] = -(S(kGamma) / Pr * mu * dTs[j] + u[0] * tau[j][0] + u[1] * tau[j][1]);
 	}
 }
 
+template <typename S>
 struct NsArgs {
-	const double* u;
-	double* out;           // residual
-	double* gvt;           // E x 4 x 2 x 4 x N: traces of G_v on the faces (own face ordering)
-	double* kap;           // E x 4 x N: kappa per face node
-	const double* met;     // E x 4 x Np
-	const double* detJ;    // E x Np
-	const double* nJ;      // F x N x 2
+	const S* u;
+	S* out;           // residual
+	S* gvt;           // E x 4 x 2 x 4 x N: traces of G_v on the faces (own face ordering)
+	S* kap;           // E x 4 x N: kappa per face node
+	const S* met;     // E x 4 x Np
+	const S* detJ;    // E x Np
+	const S* nJ;      // F x N x 2
 	const ElemFace* ef;
-	const Ops1D<double>* ops;
-	double mu, Pr;
+	const Ops1D<S>* ops;
+	S mu, Pr;
 	int E;
 };
 
 // shared per-element set-up: operators, the GLL state, its GL values and (k_rhs_ns) the GL face traces
-template <int N>
+template <typename S, int N>
 struct NsBlock {
 	static constexpr int Np = N * N, EPB = ns_epb<N>();
-	double I[N * N], DhT[N * N], Lift[2][N], w[N];
+	S I[N * N], DhT[N * N], Lift[2][N], w[N];
 	int perm[8][N];
-	double U[EPB][kNsNV * Np];      // GLL nodal state
-	double V[EPB][kNsNV * Np];      // GL values
+	S U[EPB][kNsNV * Np];      // GLL nodal state
+	S V[EPB][kNsNV * Np];      // GL values
 	ElemFace ef[EPB][4];
 };
 
-template <int N>
-__device__ inline void ns_load(NsBlock<N>& B, const NsArgs& a, int sl, int t, int e, bool valid)
+template <typename S, int N>
+__device__ inline void ns_load(NsBlock<S, N>& B, const NsArgs<S>& a, int sl, int t, int e, bool valid)
 {
 	constexpr int Np = N * N;
 	for (int i = threadIdx.x; i < N * N; i += blockDim.x) {
@@ -116,12 +119,12 @@ __device__ inline void ns_load(NsBlock<N>& B, const NsArgs& a, int sl, int t, in
 	}
 	__syncthreads();
 	// GL values: I along r, then along s (through V as scratch)
-	double tmp[kNsNV];
+	S tmp[kNsNV];
 	const int c0 = t % N, c1 = t / N;
 	if (valid) {
 #pragma unroll
 		for (int v = 0; v < kNsNV; ++v) {
-			double acc = 0.0;
+			S acc = S(0.0);
 #pragma unroll
 			for (int m = 0; m < N; ++m) acc += B.I[c0 * N + m] * B.U[sl][v * Np + m + N * c1];
 			tmp[v] = acc;
@@ -133,7 +136,7 @@ __device__ inline void ns_load(NsBlock<N>& B, const NsArgs& a, int sl, int t, in
 	if (valid) {
 #pragma unroll
 		for (int v = 0; v < kNsNV; ++v) {
-			double acc = 0.0;
+			S acc = S(0.0);
 #pragma unroll
 			for (int m = 0; m < N; ++m) acc += B.I[c1 * N + m] * B.V[sl][v * Np + c0 + N * m];
 			tmp[v] = acc;
@@ -147,17 +150,17 @@ __device__ inline void ns_load(NsBlock<N>& B, const NsArgs& a, int sl, int t, in
 }
 
 // G_v = (J grad s) / J at GL node t (c0, c1): g[v + 4 d]; m[(i + 2 j)] = J dxi_j / dx_i
-template <int N>
-__device__ inline void ns_volume_gradient(const NsBlock<N>& B, const NsArgs& a, int sl, int t, int e, double* g)
+template <typename S, int N>
+__device__ inline void ns_volume_gradient(const NsBlock<S, N>& B, const NsArgs<S>& a, int sl, int t, int e, S* g)
 {
 	constexpr int Np = N * N;
 	const int c0 = t % N, c1 = t / N;
-	const double* m = a.met + (size_t)e * 4 * Np;
-	const double m00 = m[0 * Np + t], m10 = m[1 * Np + t], m01 = m[2 * Np + t], m11 = m[3 * Np + t];
-	const double Ji = 1.0 / a.detJ[(size_t)e * Np + t];
+	const S* m = a.met + (size_t)e * 4 * Np;
+	const S m00 = m[0 * Np + t], m10 = m[1 * Np + t], m01 = m[2 * Np + t], m11 = m[3 * Np + t];
+	const S Ji = trcp(a.detJ[(size_t)e * Np + t]);
 #pragma unroll
 	for (int v = 0; v < kNsNV; ++v) {
-		double d0 = 0.0, d1 = 0.0;   // d/dxi_0, d/dxi_1 (GL-collocated derivative Dh[i][j] = DhT[j N + i])
+		S d0 = S(0.0), d1 = S(0.0);   // d/dxi_0, d/dxi_1 (GL-collocated derivative Dh[i][j] = DhT[j N + i])
 #pragma unroll
 		for (int mm = 0; mm < N; ++mm) {
 			d0 += B.DhT[mm * N + c0] * B.V[sl][v * Np + mm + N * c1];
@@ -168,24 +171,24 @@ __device__ inline void ns_volume_gradient(const NsBlock<N>& B, const NsArgs& a, 
 	}
 }
 
-template <int N>
-__global__ void __launch_bounds__(128) k_ns_gv(NsArgs a)
+template <typename S, int N>
+__global__ void __launch_bounds__(128) k_ns_gv(NsArgs<S> a)
 {
 	constexpr int Np = N * N, EPB = ns_epb<N>();
-	__shared__ NsBlock<N> B;
-	__shared__ double Gv[EPB][2 * kNsNV * Np];
-	__shared__ double WJi[EPB][Np];   // 1 / (w_c0 w_c1 J)
+	__shared__ NsBlock<S, N> B;
+	__shared__ S Gv[EPB][2 * kNsNV * Np];
+	__shared__ S WJi[EPB][Np];   // 1 / (w_c0 w_c1 J)
 	const int sl = threadIdx.x / Np, t = threadIdx.x % Np;
 	const int e = blockIdx.x * EPB + sl;
 	const bool valid = sl < EPB && e < a.E;
 	const int slc = sl < EPB ? sl : 0;
-	ns_load<N>(B, a, slc, t, valid ? e : 0, valid);
+	ns_load<S, N>(B, a, slc, t, valid ? e : 0, valid);
 	if (valid) {
-		double g[2 * kNsNV];
-		ns_volume_gradient<N>(B, a, slc, t, e, g);
+		S g[2 * kNsNV];
+		ns_volume_gradient<S, N>(B, a, slc, t, e, g);
 #pragma unroll
 		for (int k = 0; k < 2 * kNsNV; ++k) Gv[slc][(k / kNsNV * kNsNV + k % kNsNV) * Np + t] = g[k];
-		WJi[slc][t] = 1.0 / (B.w[t % N] * B.w[t / N] * a.detJ[(size_t)e * Np + t]);
+		WJi[slc][t] = trcp(B.w[t % N] * B.w[t / N] * a.detJ[(size_t)e * Np + t]);
 	}
 	__syncthreads();
 	if (valid) {
@@ -193,7 +196,7 @@ __global__ void __launch_bounds__(128) k_ns_gv(NsArgs a)
 		for (int item = t; item < 4 * 2 * kNsNV * N; item += Np) {
 			const int at = item % N, r = item / N, v = r % kNsNV, d = (r / kNsNV) % 2, lf = r / (2 * kNsNV);
 			const int fd = lf >> 1, es = lf & 1;   // normal direction and end (xi = -1 / +1) of the face
-			double acc = 0.0;
+			S acc = S(0.0);
 #pragma unroll
 			for (int c = 0; c < N; ++c) acc += B.Lift[es][c] * Gv[slc][(d * kNsNV + v) * Np + (fd == 0 ? c + N * at : at + N * c)];
 			a.gvt[(size_t)e * 4 * 2 * kNsNV * N + item] = acc;
@@ -201,7 +204,7 @@ __global__ void __launch_bounds__(128) k_ns_gv(NsArgs a)
 		// kappa(a) = sum_c Lift(c)^2 / (w_c J(c, a)) = sum_c Lift(c)^2 w_a WJi(c, a)
 		for (int item = t; item < 4 * N; item += Np) {
 			const int at = item % N, lf = item / N, fd = lf >> 1, es = lf & 1;
-			double acc = 0.0;
+			S acc = S(0.0);
 #pragma unroll
 			for (int c = 0; c < N; ++c)
 				acc += B.Lift[es][c] * B.Lift[es][c] * B.w[at] * WJi[slc][fd == 0 ? c + N * at : at + N * c];
@@ -210,22 +213,22 @@ __global__ void __launch_bounds__(128) k_ns_gv(NsArgs a)
 	}
 }
 
-template <int N, int FLUX>
-__global__ void __launch_bounds__(128) k_rhs_ns(NsArgs a)
+template <typename S, int N, int FLUX>
+__global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 {
 	constexpr int Np = N * N, EPB = ns_epb<N>();
-	constexpr double eta = 1.01 * 4.0;
-	__shared__ NsBlock<N> B;
-	__shared__ double TRo[EPB][4 * kNsNV * N];     // own face traces at the face GL nodes, own ordering
-	__shared__ double TRn[EPB][4 * kNsNV * N];     // neighbour face traces, the neighbour's ordering
-	__shared__ double QF[EPB][4 * 2 * kNsNV * N];  // q = J_f n_d (s_nb - s_own) / 2 per face, own ordering
-	__shared__ double X[EPB][2 * kNsNV * Np];      // W fr_d at the GL nodes
-	__shared__ double RES[EPB][4 * kNsNV * N];     // J_f nnf per face, own ordering
+	constexpr S eta = S(1.01 * 4.0);
+	__shared__ NsBlock<S, N> B;
+	__shared__ S TRo[EPB][4 * kNsNV * N];     // own face traces at the face GL nodes, own ordering
+	__shared__ S TRn[EPB][4 * kNsNV * N];     // neighbour face traces, the neighbour's ordering
+	__shared__ S QF[EPB][4 * 2 * kNsNV * N];  // q = J_f n_d (s_nb - s_own) / 2 per face, own ordering
+	__shared__ S X[EPB][2 * kNsNV * Np];      // W fr_d at the GL nodes
+	__shared__ S RES[EPB][4 * kNsNV * N];     // J_f nnf per face, own ordering
 	const int sl = threadIdx.x / Np, t = threadIdx.x % Np;
 	const int e = blockIdx.x * EPB + sl;
 	const bool valid = sl < EPB && e < a.E;
 	const int slc = sl < EPB ? sl : 0;
-	ns_load<N>(B, a, slc, t, valid ? e : 0, valid);
+	ns_load<S, N>(B, a, slc, t, valid ? e : 0, valid);
 	const int c0 = t % N, c1 = t / N;
 	if (valid) {
 		// face traces: GLL face values (own from LDS, neighbour gathered), then I along the face
@@ -236,7 +239,7 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs a)
 			const int en = f.en >= 0 ? f.en : e;   // every face is interior (checked at mesh upload)
 #pragma unroll
 			for (int v = 0; v < kNsNV; ++v) {
-				double ao = 0.0, an = 0.0;
+				S ao = S(0.0), an = S(0.0);
 #pragma unroll
 				for (int b = 0; b < N; ++b) {
 					ao += B.I[i * N + b] * B.U[slc][v * Np + ns_face_node<N>(lf, b)];
@@ -255,15 +258,15 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs a)
 			const ElemFace f = B.ef[slc][lf];
 			const int side = (f.packed >> 4) & 1, ord = (f.packed >> 8) & 7;
 			const int pi = B.perm[ord][i];
-			const double* own = TRo[slc] + lf * kNsNV * N;
-			const double* nb = TRn[slc] + lf * kNsNV * N;
-			const double* nJ = a.nJ + ((size_t)f.f * N + i) * 2;
+			const S* own = TRo[slc] + lf * kNsNV * N;
+			const S* nb = TRn[slc] + lf * kNsNV * N;
+			const S* nJ = a.nJ + ((size_t)f.f * N + i) * 2;
 			const int dst = side == 0 ? i : pi;
 #pragma unroll
 			for (int v = 0; v < kNsNV; ++v) {
-				const double sL = side == 0 ? own[v * N + i] : nb[v * N + i];
-				const double sR = side == 0 ? nb[v * N + pi] : own[v * N + pi];
-				const double dl = 0.5 * (sR - sL);
+				const S sL = side == 0 ? own[v * N + i] : nb[v * N + i];
+				const S sR = side == 0 ? nb[v * N + pi] : own[v * N + pi];
+				const S dl = S(0.5) * (sR - sL);
 				QF[slc][((lf * 2 + 0) * kNsNV + v) * N + dst] = nJ[0] * dl;
 				QF[slc][((lf * 2 + 1) * kNsNV + v) * N + dst] = nJ[1] * dl;
 			}
@@ -272,15 +275,15 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs a)
 	__syncthreads();
 	if (valid) {
 		// G = G_v + sum_faces G_f at this node, then the volume flux
-		double s[kNsNV], g[2 * kNsNV], f[2 * kNsNV], fv[2 * kNsNV];
-		ns_volume_gradient<N>(B, a, slc, t, e, g);
-		const double wq = B.w[c0] * B.w[c1];
-		const double Jq = a.detJ[(size_t)e * Np + t];
-		const double wji = 1.0 / (wq * Jq);
+		S s[kNsNV], g[2 * kNsNV], f[2 * kNsNV], fv[2 * kNsNV];
+		ns_volume_gradient<S, N>(B, a, slc, t, e, g);
+		const S wq = B.w[c0] * B.w[c1];
+		const S Jq = a.detJ[(size_t)e * Np + t];
+		const S wji = trcp(wq * Jq);
 #pragma unroll
 		for (int lf = 0; lf < 4; ++lf) {
 			const int fd = lf >> 1, cn = fd == 0 ? c0 : c1, at = fd == 0 ? c1 : c0;
-			const double lw = B.Lift[lf & 1][cn] * B.w[at] * wji;
+			const S lw = B.Lift[lf & 1][cn] * B.w[at] * wji;
 #pragma unroll
 			for (int d = 0; d < 2; ++d)
 #pragma unroll
@@ -288,17 +291,17 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs a)
 		}
 #pragma unroll
 		for (int v = 0; v < kNsNV; ++v) s[v] = B.V[slc][v * Np + t];
-		euler_flux<double, 2>(s, f);
-		ns_viscous_flux2(s, g, a.mu, a.Pr, fv);
+		euler_flux<S, 2>(s, f);
+		ns_viscous_flux2<S>(s, g, a.mu, a.Pr, fv);
 #pragma unroll
 		for (int k = 0; k < 2 * kNsNV; ++k) f[k] += fv[k];
-		const double* m = a.met + (size_t)e * 4 * Np;
+		const S* m = a.met + (size_t)e * 4 * Np;
 #pragma unroll
 		for (int d0 = 0; d0 < 2; ++d0)
 #pragma unroll
 			for (int v = 0; v < kNsNV; ++v) {
 				// fr_d0 = sum_d1 m(d1, d0) f_d1, m[(i + 2 j)] = J dxi_j / dx_i
-				const double fr = m[(0 + 2 * d0) * Np + t] * f[0 + 2 * v] + m[(1 + 2 * d0) * Np + t] * f[1 + 2 * v];
+				const S fr = m[(0 + 2 * d0) * Np + t] * f[0 + 2 * v] + m[(1 + 2 * d0) * Np + t] * f[1 + 2 * v];
 				X[slc][(d0 * kNsNV + v) * Np + t] = wq * fr;
 			}
 		// face numerical flux, items in the left ordering (as the Euler kernel)
@@ -308,51 +311,52 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs a)
 			const int side = (fc.packed >> 4) & 1, ord = (fc.packed >> 8) & 7, fn = fc.packed & 15;
 			const int en = fc.en >= 0 ? fc.en : e;
 			const int pi = B.perm[ord][i];
-			const double* own = TRo[slc] + lf * kNsNV * N;
-			const double* nb = TRn[slc] + lf * kNsNV * N;
+			const S* own = TRo[slc] + lf * kNsNV * N;
+			const S* nb = TRn[slc] + lf * kNsNV * N;
 			// left / right: element, local face, index of node i in that element's face ordering
 			const int eLf = side == 0 ? e : en, lfL = side == 0 ? lf : fn, iL = i;
 			const int eRf = side == 0 ? en : e, lfR = side == 0 ? fn : lf, iR = pi;
-			const double* tvL = a.gvt + ((size_t)eLf * 4 + lfL) * 2 * kNsNV * N;
-			const double* tvR = a.gvt + ((size_t)eRf * 4 + lfR) * 2 * kNsNV * N;
-			const double kL = a.kap[((size_t)eLf * 4 + lfL) * N + iL], kR = a.kap[((size_t)eRf * 4 + lfR) * N + iR];
-			const double* nJ = a.nJ + ((size_t)fc.f * N + i) * 2;
-			double sL[kNsNV], sR[kNsNV], gL[2 * kNsNV], gR[2 * kNsNV];
+			const S* tvL = a.gvt + ((size_t)eLf * 4 + lfL) * 2 * kNsNV * N;
+			const S* tvR = a.gvt + ((size_t)eRf * 4 + lfR) * 2 * kNsNV * N;
+			const S kL = a.kap[((size_t)eLf * 4 + lfL) * N + iL], kR = a.kap[((size_t)eRf * 4 + lfR) * N + iR];
+			const S* nJ = a.nJ + ((size_t)fc.f * N + i) * 2;
+			S sL[kNsNV], sR[kNsNV], gL[2 * kNsNV], gR[2 * kNsNV];
 #pragma unroll
 			for (int v = 0; v < kNsNV; ++v) {
 				sL[v] = side == 0 ? own[v * N + i] : nb[v * N + i];
 				sR[v] = side == 0 ? nb[v * N + pi] : own[v * N + pi];
-				const double dl = 0.5 * (sR[v] - sL[v]);
+				const S dl = S(0.5) * (sR[v] - sL[v]);
 #pragma unroll
 				for (int d = 0; d < 2; ++d) {
-					const double q = nJ[d] * dl;
+					const S q = nJ[d] * dl;
 					gL[v + kNsNV * d] = tvL[(d * kNsNV + v) * N + iL] + eta * kL * q;
 					gR[v + kNsNV * d] = tvR[(d * kNsNV + v) * N + iR] + eta * kR * q;
 				}
 			}
-			const double Jf = sqrt(nJ[0] * nJ[0] + nJ[1] * nJ[1]);
-			const double nu[2] = { nJ[0] / Jf, nJ[1] / Jf };
-			double nnf[kNsNV], fvL[2 * kNsNV], fvR[2 * kNsNV];
-			if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<double, 2>(sL, sR, nu, nnf);
-			else numflux_roe<double, 2>(sL, sR, nu, nnf);
-			ns_viscous_flux2(sL, gL, a.mu, a.Pr, fvL);
-			ns_viscous_flux2(sR, gR, a.mu, a.Pr, fvR);
+			const S Jf = tsqrt(nJ[0] * nJ[0] + nJ[1] * nJ[1]);
+			const S Jfi = trcp(Jf);
+			const S nu[2] = { nJ[0] * Jfi, nJ[1] * Jfi };
+			S nnf[kNsNV], fvL[2 * kNsNV], fvR[2 * kNsNV];
+			if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<S, 2>(sL, sR, nu, nnf);
+			else numflux_roe<S, 2>(sL, sR, nu, nnf);
+			ns_viscous_flux2<S>(sL, gL, a.mu, a.Pr, fvL);
+			ns_viscous_flux2<S>(sR, gR, a.mu, a.Pr, fvR);
 			const int dst = side == 0 ? i : pi;
 #pragma unroll
 			for (int v = 0; v < kNsNV; ++v) {
-				const double vis = nu[0] * 0.5 * (fvL[0 + 2 * v] + fvR[0 + 2 * v]) + nu[1] * 0.5 * (fvL[1 + 2 * v] + fvR[1 + 2 * v]);
-				const double r = (nnf[v] + vis) * Jf;
+				const S vis = nu[0] * S(0.5) * (fvL[0 + 2 * v] + fvR[0 + 2 * v]) + nu[1] * S(0.5) * (fvL[1 + 2 * v] + fvR[1 + 2 * v]);
+				const S r = (nnf[v] + vis) * Jf;
 				RES[slc][(lf * kNsNV + v) * N + dst] = side == 0 ? r : -r;
 			}
 		}
 	}
 	__syncthreads();
-	double z[kNsNV];
+	S z[kNsNV];
 	if (valid) {
 		// weak divergence sum_d Dh_d^T (W fr_d), minus the face lifts, at the GL nodes
 #pragma unroll
 		for (int v = 0; v < kNsNV; ++v) {
-			double acc = 0.0;
+			S acc = S(0.0);
 #pragma unroll
 			for (int mm = 0; mm < N; ++mm) {
 				acc += B.DhT[c0 * N + mm] * X[slc][(0 * kNsNV + v) * Np + mm + N * c1];
@@ -375,7 +379,7 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs a)
 	if (valid)
 #pragma unroll
 		for (int v = 0; v < kNsNV; ++v) {
-			double acc = 0.0;
+			S acc = S(0.0);
 #pragma unroll
 			for (int mm = 0; mm < N; ++mm) acc += B.I[mm * N + c1] * B.V[slc][v * Np + c0 + N * mm];
 			z[v] = acc;
@@ -388,54 +392,65 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs a)
 	if (valid)
 #pragma unroll
 		for (int v = 0; v < kNsNV; ++v) {
-			double acc = 0.0;
+			S acc = S(0.0);
 #pragma unroll
 			for (int mm = 0; mm < N; ++mm) acc += B.I[mm * N + c0] * B.V[slc][v * Np + mm + N * c1];
 			a.out[((size_t)e * kNsNV + v) * Np + t] = acc;
 		}
 }
 
-template <int N>
-hipError_t run_ns(Ctx& c, const NsArgs& a)
+template <typename S, int N>
+hipError_t run_ns(Ctx& c, const NsArgs<S>& a)
 {
 	constexpr int EPB = ns_epb<N>(), Np = N * N;
 	const int grid = (c.E + EPB - 1) / EPB;
-	hipLaunchKernelGGL(k_ns_gv<N>, dim3(grid), dim3(EPB * Np), 0, c.stream, a);
+	hipLaunchKernelGGL((k_ns_gv<S, N>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
 	DGX_HIP_CHECK(hipGetLastError());
 	if (c.cfg.num_flux == DGX_FLUX_LF)
-		hipLaunchKernelGGL((k_rhs_ns<N, DGX_FLUX_LF>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
+		hipLaunchKernelGGL((k_rhs_ns<S, N, DGX_FLUX_LF>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
 	else
-		hipLaunchKernelGGL((k_rhs_ns<N, DGX_FLUX_ROE>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
+		hipLaunchKernelGGL((k_rhs_ns<S, N, DGX_FLUX_ROE>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
 	return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t launch_rhs_ns(Ctx& c, const double* u, double* r)
+template <typename S>
+static hipError_t launch_ns(Ctx& c, const S* u, S* r, const S* met, const S* detJ, const S* nJ, const Ops1D<S>* ops)
 {
 	if (c.d != 2 || c.cfg.pde != DGX_PDE_NAVIER_STOKES || !c.ns_gvt || !c.ns_kappa) return hipErrorInvalidValue;
 	if (c.E == 0) return hipSuccess;
-	NsArgs a{};
+	NsArgs<S> a{};
 	a.u = u;
 	a.out = r;
-	a.gvt = c.ns_gvt;
-	a.kap = c.ns_kappa;
-	a.met = c.metrics_vc;
-	a.detJ = c.detJ_vc;
-	a.nJ = c.nJ_fc;
+	a.gvt = reinterpret_cast<S*>(c.ns_gvt);     // FP32 uses the first half of the FP64-sized buffers
+	a.kap = reinterpret_cast<S*>(c.ns_kappa);
+	a.met = met;
+	a.detJ = detJ;
+	a.nJ = nJ;
 	a.ef = c.elem_faces;
-	a.ops = c.ops64;
-	a.mu = c.cfg.mu;
-	a.Pr = c.cfg.Pr;
+	a.ops = ops;
+	a.mu = S(c.cfg.mu);
+	a.Pr = S(c.cfg.Pr);
 	a.E = c.E;
 	switch (c.N) {
-	case 2: return run_ns<2>(c, a);
-	case 3: return run_ns<3>(c, a);
-	case 4: return run_ns<4>(c, a);
-	case 5: return run_ns<5>(c, a);
-	case 6: return run_ns<6>(c, a);
+	case 2: return run_ns<S, 2>(c, a);
+	case 3: return run_ns<S, 3>(c, a);
+	case 4: return run_ns<S, 4>(c, a);
+	case 5: return run_ns<S, 5>(c, a);
+	case 6: return run_ns<S, 6>(c, a);
 	default: return hipErrorInvalidValue;
 	}
+}
+
+hipError_t launch_rhs_ns(Ctx& c, const double* u, double* r)
+{
+	return launch_ns<double>(c, u, r, c.metrics_vc, c.detJ_vc, c.nJ_fc, c.ops64);
+}
+
+hipError_t launch_rhs_ns32(Ctx& c, const float* u, float* r)
+{
+	return launch_ns<float>(c, u, r, c.metrics_vc32, c.detJ_vc32, c.nJ_fc32, c.ops32);
 }
 
 }  // namespace dgx

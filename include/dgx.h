@@ -32,8 +32,8 @@ enum {
 	DGX_ERR_NO_MESH = -7
 };
 
-/* DGX_PDE_NAVIER_STOKES: Euler + BR2 viscous terms (2-D QUAD, interior faces only: periodic meshes; FP64
- * residual and explicit path), constant viscosity mu and Prandtl number Pr from the config. */
+/* DGX_PDE_NAVIER_STOKES: Euler + BR2 viscous terms (2-D QUAD, interior faces only: periodic meshes; FP64 and
+ * FP32 residual, FP64 explicit path), constant viscosity mu and Prandtl number Pr from the config. */
 enum { DGX_PDE_ADVECTION = 0, DGX_PDE_EULER = 1, DGX_PDE_NAVIER_STOKES = 2 };
 /* num_flux_1st: upwind | Lax-Friedrichs | Roe-Pike (numerical_flux/definitions_numerical_flux.h:22-32) */
 enum { DGX_FLUX_UPWIND = 0, DGX_FLUX_LF = 1, DGX_FLUX_ROE = 2 };

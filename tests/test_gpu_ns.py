@@ -5,7 +5,8 @@
   fraction of the residual (checked).
 * The explicit path (M^-1 R and its max-norm, an SSP-RK3 step) uses the same kernels.
 * The face coupling is conservative: the residual sums to zero per conserved variable on a periodic mesh.
-* Meshes with boundary faces, and the FP32 / linearised entry points, are refused with DGX_ERR_UNSUPPORTED.
+* The FP32 residual (FP32 state, geometry and operators) agrees with the FP64 one to FP32 rounding.
+* Meshes with boundary faces and the linearised entry points are refused with DGX_ERR_UNSUPPORTED.
 """
 import numpy as np
 import pytest
@@ -124,7 +125,25 @@ def test_ns_refuses_boundary_faces_and_unbuilt_paths(dev):
     u = torch.from_numpy(_state(mesh)).to(dev)
     r = torch.empty_like(u)
     with pytest.raises(_lib.DgxError, match="UNSUPPORTED"):
-        ctx.compute_rhs_f32(u.float().data_ptr(), r.float().data_ptr())
-    with pytest.raises(_lib.DgxError, match="UNSUPPORTED"):
         ctx.jacobian_vector(u.data_ptr(), u.data_ptr(), r.data_ptr())
+    ctx.close()
+
+
+@pytest.mark.parametrize("p", [2, 3, 5])
+def test_ns_fp32_residual_close_to_fp64(dev, p):
+    mesh = _mesh(4, p, rotate=np.random.default_rng(p).integers(0, 4, 16))
+    u = _state(mesh, seed=p)
+    ctx = _ctx(mesh, M.FLUX_ROE)
+    u64 = torch.from_numpy(u).to(dev)
+    u32 = u64.float()
+    r64 = torch.empty_like(u64)
+    r32 = torch.empty_like(u32)
+    ctx.compute_rhs(u64.data_ptr(), r64.data_ptr())
+    ctx.compute_rhs_f32(u32.data_ptr(), r32.data_ptr())
+    torch.cuda.synchronize()
+    err = ((r32.double() - r64).abs().max() / r64.abs().max()).item()
+    print(f"p={p}: FP32 vs FP64 rel err {err:.2e}")
+    assert err < 2e-4          # FP32 rounding of O(1) volume and face terms that cancel to the residual
+    tot = r32.double().reshape(mesh.E, 4, -1).sum(dim=(0, 2))
+    assert tot.abs().max().item() < 1e-5 * r32.double().abs().sum().item()
     ctx.close()
