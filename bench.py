@@ -212,6 +212,12 @@ def ns_throughput(torch, dev, steps, warmup, n=448):
     ms = timed(g)
     ms32 = timed(g32)
     err32 = ((r32.double() - r).abs().max() / r.abs().max()).item()
+    # input rounding alone: the FP64 residual of the FP32-rounded state
+    ur = u32.double()
+    rr = torch.empty_like(u)
+    ctx.compute_rhs(ur.data_ptr(), rr.data_ptr())
+    err_in = ((rr - r).abs().max() / r.abs().max()).item()
+    del ur, rr
     E, Np, nv, d, Nfc = mesh.E, 16, 4, 2, 4
     ndof = E * Np * nv
     # compulsory traffic: state in, residual out, FP64 geometry (metrics, det J, n J_f of the owned faces)
@@ -224,7 +230,7 @@ def ns_throughput(torch, dev, steps, warmup, n=448):
     ctx.close()
     return {"workload": f"2-D Navier-Stokes BR2 p={p}, {n}x{n} periodic QUAD ({E} elements), Roe, Re 100, Pr 0.72",
             "fp32": {"value": round(ndof / (ms32 * 1e-3) / 1e9, 3), "ms_per_step": round(ms32, 5),
-                     "rel_err_vs_fp64": err32,
+                     "rel_err_vs_fp64": err32, "input_rounding_rel_err": err_in,
                      "roofline": {"bound": "hbm", "achieved": round(alg32 / (ms32 * 1e-3) / 1e9, 1),
                                   "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                   "frac": round(alg32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None}},
@@ -245,6 +251,8 @@ def main():
     ap.add_argument("--no-implicit", action="store_true", help="skip the time-to-1e-10 steady-solve legs")
     ap.add_argument("--implicit-only", action="store_true",
                     help="run only the time-to-1e-10 legs and print their JSON (profiling aid)")
+    ap.add_argument("--ns-only", action="store_true",
+                    help="run only the Navier-Stokes residual leg and print its JSON (profiling aid)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 1 GiB read + 1 GiB write elementwise kernel (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -261,6 +269,9 @@ def main():
     dev = torch.device("cuda:0")
     if args.implicit_only:
         print(json.dumps({"time_to_1e-10": time_to_steady(torch, dev)}), flush=True)
+        return
+    if args.ns_only:
+        print(json.dumps({"navier_stokes": ns_throughput(torch, dev, args.steps, args.warmup)}), flush=True)
         return
     n, p = args.n, 4
     t_setup = time.perf_counter()
