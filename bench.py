@@ -56,10 +56,10 @@ def _state_on_device(torch, xyz_dev, seed):
     return out.reshape(-1)
 
 
-def _traffic_from_profiles(kernel_tag):
-    """HBM bytes per launch (calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE) from the committed summary, if present:
-    lines `traffic <tag> <bytes>` in profiles/rhs_hex_p4_64cube_traffic.md."""
-    path = os.path.join(ROOT, "profiles", "rhs_hex_p4_64cube_traffic.md")
+def _traffic_from_profiles(kernel_tag, summary="rhs_hex_p4_64cube_traffic.md"):
+    """HBM bytes per launch (calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE) from a committed summary, if present:
+    lines `traffic <tag> <bytes>` in profiles/<summary>."""
+    path = os.path.join(ROOT, "profiles", summary)
     try:
         for line in open(path):
             parts = line.split()
@@ -233,11 +233,12 @@ def ns_throughput(torch, dev, steps, warmup, n=448):
                      "rel_err_vs_fp64": err32, "input_rounding_rel_err": err_in,
                      "roofline": {"bound": "hbm", "achieved": round(alg32 / (ms32 * 1e-3) / 1e9, 1),
                                   "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                  "frac": round(alg32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None}},
+                                  "frac": round(alg32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                  "traffic": _traffic_from_profiles("ns_fp32", "ns_quad_448_p3_traffic.md")}},
             "dtype": "f64", "value": round(ndof / (ms * 1e-3) / 1e9, 3), "unit": "GDOF/s", "ms_per_step": round(ms, 5),
             "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                         "traffic": None},
+                         "traffic": _traffic_from_profiles("ns_fp64", "ns_quad_448_p3_traffic.md")},
             "gradient_round_trip_bytes": grad_rt}
 
 
