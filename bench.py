@@ -212,6 +212,11 @@ def ns_throughput(torch, dev, steps, warmup, n=448):
     ms = timed(g)
     ms32 = timed(g32)
     err32 = ((r32.double() - r).abs().max() / r.abs().max()).item()
+    rmx = torch.empty_like(u32)
+    gmx = ctx.graph_rhs(_lib.GRAPH_RHS_MIXED, u.data_ptr(), rmx.data_ptr())
+    msmx = timed(gmx)
+    errmx = ((rmx.double() - r).abs().max() / r.abs().max()).item()
+    gmx.close()
     # input rounding alone: the FP64 residual of the FP32-rounded state
     ur = u32.double()
     rr = torch.empty_like(u)
@@ -235,6 +240,8 @@ def ns_throughput(torch, dev, steps, warmup, n=448):
                                   "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                   "frac": round(alg32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                                   "traffic": _traffic_from_profiles("ns_fp32", "ns_quad_448_p3_traffic.md")}},
+            "mixed": {"value": round(ndof / (msmx * 1e-3) / 1e9, 3), "ms_per_step": round(msmx, 5),
+                      "rel_err_vs_fp64": errmx},
             "dtype": "f64", "value": round(ndof / (ms * 1e-3) / 1e9, 3), "unit": "GDOF/s", "ms_per_step": round(ms, 5),
             "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),

@@ -255,6 +255,7 @@ dgx_status dgx_compute_rhs_mixed(dgx_ctx* h, const double* u, float* r)
 {
 	if (!h || !u || !r) return DGX_ERR_ARG;
 	if (!h->c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (h->c.cfg.pde == DGX_PDE_NAVIER_STOKES) return hip_status(dgx::launch_rhs_ns_mixed(h->c, u, r));
 	if (h->c.cfg.pde != DGX_PDE_EULER) return DGX_ERR_UNSUPPORTED;
 	return hip_status(dgx::launch_rhs_mixed(h->c, u, r));
 }
@@ -378,7 +379,7 @@ dgx_status dgx_graph_create_rhs(dgx_ctx* h, int kind, const void* u, void* r, dg
 	if (!h || !u || !r || !out || kind < DGX_GRAPH_RHS_FP64 || kind > DGX_GRAPH_RHS_MIXED) return DGX_ERR_ARG;
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
-	if (kind == DGX_GRAPH_RHS_MIXED && c.cfg.pde != DGX_PDE_EULER) return DGX_ERR_UNSUPPORTED;
+	if (kind == DGX_GRAPH_RHS_MIXED && c.cfg.pde == DGX_PDE_ADVECTION) return DGX_ERR_UNSUPPORTED;
 	dgx_graph* g = new (std::nothrow) dgx_graph();
 	if (!g) return DGX_ERR_HIP;
 	const dgx_status st = capture_graph(h, g, [&]() -> hipError_t {
@@ -388,6 +389,8 @@ dgx_status dgx_graph_create_rhs(dgx_ctx* h, int kind, const void* u, void* r, dg
 			return dgx::launch_rhs_ns32(c, static_cast<const float*>(u), static_cast<float*>(r));
 		if (kind == DGX_GRAPH_RHS_FP32)
 			return dgx::launch_rhs32(c, static_cast<const float*>(u), static_cast<float*>(r));
+		if (c.cfg.pde == DGX_PDE_NAVIER_STOKES)
+			return dgx::launch_rhs_ns_mixed(c, static_cast<const double*>(u), static_cast<float*>(r));
 		return dgx::launch_rhs_mixed(c, static_cast<const double*>(u), static_cast<float*>(r));
 	});
 	if (st != DGX_OK) {

@@ -105,6 +105,7 @@ hipError_t launch_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned l
 hipError_t launch_rhs32(Ctx& c, const float* u, float* r);
 hipError_t launch_rhs_ns(Ctx& c, const double* u, double* r);   // 2-D Navier-Stokes (BR2), FP64
 hipError_t launch_rhs_ns32(Ctx& c, const float* u, float* r);    // the same in FP32
+hipError_t launch_rhs_ns_mixed(Ctx& c, const double* u, float* r);   // FP64 state, FP32 perturbation form
 hipError_t launch_rhs_mixed(Ctx& c, const double* u, float* r);
 hipError_t launch_minv(Ctx& c, double* r);
 hipError_t launch_positivity(Ctx& c, double* u, int* status);

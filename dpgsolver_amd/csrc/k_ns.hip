@@ -1,5 +1,6 @@
 // k_ns.hip -- 2-D Navier-Stokes residual with BR2 viscous terms (QUAD, interior faces), FP64 and FP32
-// (FP32: the FP32 state, geometry shadow and operator tables, as the FP32 Euler residual).
+// (FP32: the FP32 state, geometry shadow and operator tables, as the FP32 Euler residual), and the mixed
+// form (PERT: FP64 state in, FP32 contractions on u - c_e, FP64 pointwise fluxes, as the Euler mixed kernel).
 //
 // The weak gradient (compute_grad_coef_dg_T.c:171-307, 'e' branch) is formed at the volume cubature (GL)
 // nodes.  With M = Ibar^T W J Ibar, M^-1 Ibar^T W x = Ibar^-1 (x / J), so the projected volume gradient is the
@@ -76,6 +77,7 @@ __device__ inline void ns_viscous_flux2(const S* s, const S* g, S mu, S Pr, S* f
 template <typename S>
 struct NsArgs {
 	const S* u;
+	const double* u64;     // PERT: FP64 state (u unused)
 	S* out;           // residual
 	S* gvt;           // E x 4 x 2 x 4 x N: traces of G_v on the faces (own face ordering)
 	S* kap;           // E x 4 x N: kappa per face node
@@ -96,10 +98,13 @@ struct NsBlock {
 	int perm[8][N];
 	S U[EPB][kNsNV * Np];      // GLL nodal state
 	S V[EPB][kNsNV * Np];      // GL values
+	double C[EPB][kNsNV];      // PERT: the element's reference state c_e (its first GLL node)
 	ElemFace ef[EPB][4];
 };
 
-template <typename S, int N>
+// PERT: U = u - c_e from the FP64 state (the constant part has no gradient and its fluxes cancel exactly by
+// free-stream preservation), so FP32 rounding acts only on the within-element variation
+template <typename S, int N, bool PERT>
 __device__ inline void ns_load(NsBlock<S, N>& B, const NsArgs<S>& a, int sl, int t, int e, bool valid)
 {
 	constexpr int Np = N * N;
@@ -115,7 +120,13 @@ __device__ inline void ns_load(NsBlock<S, N>& B, const NsArgs<S>& a, int sl, int
 	for (int i = threadIdx.x; i < 8 * N; i += blockDim.x) B.perm[i / N][i % N] = a.ops->perm[i / N][i % N];
 	if (valid) {
 		if (t < 4) B.ef[sl][t] = a.ef[(size_t)e * 4 + t];
-		for (int v = 0; v < kNsNV; ++v) B.U[sl][v * Np + t] = a.u[((size_t)e * kNsNV + v) * Np + t];
+		if constexpr (PERT) {
+			if (t < kNsNV) B.C[sl][t] = a.u64[((size_t)e * kNsNV + t) * Np];
+			for (int v = 0; v < kNsNV; ++v)
+				B.U[sl][v * Np + t] = S(a.u64[((size_t)e * kNsNV + v) * Np + t] - a.u64[((size_t)e * kNsNV + v) * Np]);
+		} else {
+			for (int v = 0; v < kNsNV; ++v) B.U[sl][v * Np + t] = a.u[((size_t)e * kNsNV + v) * Np + t];
+		}
 	}
 	__syncthreads();
 	// GL values: I along r, then along s (through V as scratch)
@@ -171,7 +182,7 @@ __device__ inline void ns_volume_gradient(const NsBlock<S, N>& B, const NsArgs<S
 	}
 }
 
-template <typename S, int N>
+template <typename S, int N, bool PERT>
 __global__ void __launch_bounds__(128) k_ns_gv(NsArgs<S> a)
 {
 	constexpr int Np = N * N, EPB = ns_epb<N>();
@@ -182,7 +193,7 @@ __global__ void __launch_bounds__(128) k_ns_gv(NsArgs<S> a)
 	const int e = blockIdx.x * EPB + sl;
 	const bool valid = sl < EPB && e < a.E;
 	const int slc = sl < EPB ? sl : 0;
-	ns_load<S, N>(B, a, slc, t, valid ? e : 0, valid);
+	ns_load<S, N, PERT>(B, a, slc, t, valid ? e : 0, valid);
 	if (valid) {
 		S g[2 * kNsNV];
 		ns_volume_gradient<S, N>(B, a, slc, t, e, g);
@@ -213,7 +224,7 @@ __global__ void __launch_bounds__(128) k_ns_gv(NsArgs<S> a)
 	}
 }
 
-template <typename S, int N, int FLUX>
+template <typename S, int N, int FLUX, bool PERT>
 __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 {
 	constexpr int Np = N * N, EPB = ns_epb<N>();
@@ -228,7 +239,7 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 	const int e = blockIdx.x * EPB + sl;
 	const bool valid = sl < EPB && e < a.E;
 	const int slc = sl < EPB ? sl : 0;
-	ns_load<S, N>(B, a, slc, t, valid ? e : 0, valid);
+	ns_load<S, N, PERT>(B, a, slc, t, valid ? e : 0, valid);
 	const int c0 = t % N, c1 = t / N;
 	if (valid) {
 		// face traces: GLL face values (own from LDS, neighbour gathered), then I along the face
@@ -243,7 +254,10 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 #pragma unroll
 				for (int b = 0; b < N; ++b) {
 					ao += B.I[i * N + b] * B.U[slc][v * Np + ns_face_node<N>(lf, b)];
-					an += B.I[i * N + b] * a.u[((size_t)en * kNsNV + v) * Np + ns_face_node<N>(fn, b)];
+					if constexpr (PERT)
+						an += B.I[i * N + b] * S(a.u64[((size_t)en * kNsNV + v) * Np + ns_face_node<N>(fn, b)] - B.C[slc][v]);
+					else
+						an += B.I[i * N + b] * a.u[((size_t)en * kNsNV + v) * Np + ns_face_node<N>(fn, b)];
 				}
 				TRo[slc][(lf * kNsNV + v) * N + i] = ao;
 				TRn[slc][(lf * kNsNV + v) * N + i] = an;
@@ -289,12 +303,29 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 #pragma unroll
 				for (int v = 0; v < kNsNV; ++v) g[v + kNsNV * d] += lw * QF[slc][((lf * 2 + d) * kNsNV + v) * N + at];
 		}
+		if constexpr (PERT) {
+			// FP64 pointwise fluxes of c_e + V minus the Euler flux of c_e; the contractions stay in S
+			double sd[kNsNV], cd[kNsNV], gd[2 * kNsNV], fd[2 * kNsNV], fcd[2 * kNsNV], fvd[2 * kNsNV];
 #pragma unroll
-		for (int v = 0; v < kNsNV; ++v) s[v] = B.V[slc][v * Np + t];
-		euler_flux<S, 2>(s, f);
-		ns_viscous_flux2<S>(s, g, a.mu, a.Pr, fv);
+			for (int v = 0; v < kNsNV; ++v) {
+				cd[v] = B.C[slc][v];
+				sd[v] = cd[v] + (double)B.V[slc][v * Np + t];
+			}
 #pragma unroll
-		for (int k = 0; k < 2 * kNsNV; ++k) f[k] += fv[k];
+			for (int k = 0; k < 2 * kNsNV; ++k) gd[k] = (double)g[k];
+			euler_flux<double, 2>(sd, fd);
+			euler_flux<double, 2>(cd, fcd);
+			ns_viscous_flux2<double>(sd, gd, (double)a.mu, (double)a.Pr, fvd);
+#pragma unroll
+			for (int k = 0; k < 2 * kNsNV; ++k) f[k] = S(fd[k] - fcd[k] + fvd[k]);
+		} else {
+#pragma unroll
+			for (int v = 0; v < kNsNV; ++v) s[v] = B.V[slc][v * Np + t];
+			euler_flux<S, 2>(s, f);
+			ns_viscous_flux2<S>(s, g, a.mu, a.Pr, fv);
+#pragma unroll
+			for (int k = 0; k < 2 * kNsNV; ++k) f[k] += fv[k];
+		}
 		const S* m = a.met + (size_t)e * 4 * Np;
 #pragma unroll
 		for (int d0 = 0; d0 < 2; ++d0)
@@ -336,17 +367,52 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 			const S Jf = tsqrt(nJ[0] * nJ[0] + nJ[1] * nJ[1]);
 			const S Jfi = trcp(Jf);
 			const S nu[2] = { nJ[0] * Jfi, nJ[1] * Jfi };
-			S nnf[kNsNV], fvL[2 * kNsNV], fvR[2 * kNsNV];
-			if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<S, 2>(sL, sR, nu, nnf);
-			else numflux_roe<S, 2>(sL, sR, nu, nnf);
-			ns_viscous_flux2<S>(sL, gL, a.mu, a.Pr, fvL);
-			ns_viscous_flux2<S>(sR, gR, a.mu, a.Pr, fvR);
 			const int dst = side == 0 ? i : pi;
+			if constexpr (PERT) {
+				// FP64 states c_e + delta and fluxes; the normal Euler flux of c_e is subtracted before rounding
+				const double* c0 = B.C[slc];
+				double cL[kNsNV], cR[kNsNV], gLd[2 * kNsNV], gRd[2 * kNsNV], nd[2], fdn[kNsNV], cn[kNsNV];
+				double fvL[2 * kNsNV], fvR[2 * kNsNV], cu[2];
 #pragma unroll
-			for (int v = 0; v < kNsNV; ++v) {
-				const S vis = nu[0] * S(0.5) * (fvL[0 + 2 * v] + fvR[0 + 2 * v]) + nu[1] * S(0.5) * (fvL[1 + 2 * v] + fvR[1 + 2 * v]);
-				const S r = (nnf[v] + vis) * Jf;
-				RES[slc][(lf * kNsNV + v) * N + dst] = side == 0 ? r : -r;
+				for (int v = 0; v < kNsNV; ++v) {
+					cL[v] = c0[v] + (double)sL[v];
+					cR[v] = c0[v] + (double)sR[v];
+				}
+#pragma unroll
+				for (int k = 0; k < 2 * kNsNV; ++k) {
+					gLd[k] = (double)gL[k];
+					gRd[k] = (double)gR[k];
+				}
+				const double Jd = sqrt((double)nJ[0] * nJ[0] + (double)nJ[1] * nJ[1]);
+				nd[0] = (double)nJ[0] / Jd;
+				nd[1] = (double)nJ[1] / Jd;
+				if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<double, 2>(cL, cR, nd, fdn);
+				else numflux_roe<double, 2>(cL, cR, nd, fdn);
+				const double ci = 1.0 / c0[0];
+				cu[0] = c0[1] * ci;
+				cu[1] = c0[2] * ci;
+				const double pc = kGM1 * (c0[3] - 0.5 * c0[0] * (cu[0] * cu[0] + cu[1] * cu[1]));
+				euler_normal_flux<double, 2>(c0, cu, pc, nd, cn);
+				ns_viscous_flux2<double>(cL, gLd, (double)a.mu, (double)a.Pr, fvL);
+				ns_viscous_flux2<double>(cR, gRd, (double)a.mu, (double)a.Pr, fvR);
+#pragma unroll
+				for (int v = 0; v < kNsNV; ++v) {
+					const double vis = nd[0] * 0.5 * (fvL[0 + 2 * v] + fvR[0 + 2 * v]) + nd[1] * 0.5 * (fvL[1 + 2 * v] + fvR[1 + 2 * v]);
+					const S r = S((fdn[v] - cn[v] + vis) * Jd);
+					RES[slc][(lf * kNsNV + v) * N + dst] = side == 0 ? r : -r;
+				}
+			} else {
+				S nnf[kNsNV], fvL[2 * kNsNV], fvR[2 * kNsNV];
+				if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<S, 2>(sL, sR, nu, nnf);
+				else numflux_roe<S, 2>(sL, sR, nu, nnf);
+				ns_viscous_flux2<S>(sL, gL, a.mu, a.Pr, fvL);
+				ns_viscous_flux2<S>(sR, gR, a.mu, a.Pr, fvR);
+#pragma unroll
+				for (int v = 0; v < kNsNV; ++v) {
+					const S vis = nu[0] * S(0.5) * (fvL[0 + 2 * v] + fvR[0 + 2 * v]) + nu[1] * S(0.5) * (fvL[1 + 2 * v] + fvR[1 + 2 * v]);
+					const S r = (nnf[v] + vis) * Jf;
+					RES[slc][(lf * kNsNV + v) * N + dst] = side == 0 ? r : -r;
+				}
 			}
 		}
 	}
@@ -399,29 +465,31 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 		}
 }
 
-template <typename S, int N>
+template <typename S, int N, bool PERT>
 hipError_t run_ns(Ctx& c, const NsArgs<S>& a)
 {
 	constexpr int EPB = ns_epb<N>(), Np = N * N;
 	const int grid = (c.E + EPB - 1) / EPB;
-	hipLaunchKernelGGL((k_ns_gv<S, N>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
+	hipLaunchKernelGGL((k_ns_gv<S, N, PERT>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
 	DGX_HIP_CHECK(hipGetLastError());
 	if (c.cfg.num_flux == DGX_FLUX_LF)
-		hipLaunchKernelGGL((k_rhs_ns<S, N, DGX_FLUX_LF>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
+		hipLaunchKernelGGL((k_rhs_ns<S, N, DGX_FLUX_LF, PERT>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
 	else
-		hipLaunchKernelGGL((k_rhs_ns<S, N, DGX_FLUX_ROE>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
+		hipLaunchKernelGGL((k_rhs_ns<S, N, DGX_FLUX_ROE, PERT>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
 	return hipGetLastError();
 }
 
 }  // namespace
 
-template <typename S>
-static hipError_t launch_ns(Ctx& c, const S* u, S* r, const S* met, const S* detJ, const S* nJ, const Ops1D<S>* ops)
+template <typename S, bool PERT = false>
+static hipError_t launch_ns(Ctx& c, const S* u, S* r, const S* met, const S* detJ, const S* nJ, const Ops1D<S>* ops,
+                            const double* u64 = nullptr)
 {
 	if (c.d != 2 || c.cfg.pde != DGX_PDE_NAVIER_STOKES || !c.ns_gvt || !c.ns_kappa) return hipErrorInvalidValue;
 	if (c.E == 0) return hipSuccess;
 	NsArgs<S> a{};
 	a.u = u;
+	a.u64 = u64;
 	a.out = r;
 	a.gvt = reinterpret_cast<S*>(c.ns_gvt);     // FP32 uses the first half of the FP64-sized buffers
 	a.kap = reinterpret_cast<S*>(c.ns_kappa);
@@ -434,11 +502,11 @@ static hipError_t launch_ns(Ctx& c, const S* u, S* r, const S* met, const S* det
 	a.Pr = S(c.cfg.Pr);
 	a.E = c.E;
 	switch (c.N) {
-	case 2: return run_ns<S, 2>(c, a);
-	case 3: return run_ns<S, 3>(c, a);
-	case 4: return run_ns<S, 4>(c, a);
-	case 5: return run_ns<S, 5>(c, a);
-	case 6: return run_ns<S, 6>(c, a);
+	case 2: return run_ns<S, 2, PERT>(c, a);
+	case 3: return run_ns<S, 3, PERT>(c, a);
+	case 4: return run_ns<S, 4, PERT>(c, a);
+	case 5: return run_ns<S, 5, PERT>(c, a);
+	case 6: return run_ns<S, 6, PERT>(c, a);
 	default: return hipErrorInvalidValue;
 	}
 }
@@ -451,6 +519,11 @@ hipError_t launch_rhs_ns(Ctx& c, const double* u, double* r)
 hipError_t launch_rhs_ns32(Ctx& c, const float* u, float* r)
 {
 	return launch_ns<float>(c, u, r, c.metrics_vc32, c.detJ_vc32, c.nJ_fc32, c.ops32);
+}
+
+hipError_t launch_rhs_ns_mixed(Ctx& c, const double* u, float* r)
+{
+	return launch_ns<float, true>(c, nullptr, r, c.metrics_vc32, c.detJ_vc32, c.nJ_fc32, c.ops32, u);
 }
 
 }  // namespace dgx

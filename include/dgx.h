@@ -32,8 +32,8 @@ enum {
 	DGX_ERR_NO_MESH = -7
 };
 
-/* DGX_PDE_NAVIER_STOKES: Euler + BR2 viscous terms (2-D QUAD, interior faces only: periodic meshes; FP64 and
- * FP32 residual, FP64 explicit path), constant viscosity mu and Prandtl number Pr from the config. */
+/* DGX_PDE_NAVIER_STOKES: Euler + BR2 viscous terms (2-D QUAD, interior faces only: periodic meshes; FP64, FP32
+ * and mixed residual, FP64 explicit path), constant viscosity mu and Prandtl number Pr from the config. */
 enum { DGX_PDE_ADVECTION = 0, DGX_PDE_EULER = 1, DGX_PDE_NAVIER_STOKES = 2 };
 /* num_flux_1st: upwind | Lax-Friedrichs | Roe-Pike (numerical_flux/definitions_numerical_flux.h:22-32) */
 enum { DGX_FLUX_UPWIND = 0, DGX_FLUX_LF = 1, DGX_FLUX_ROE = 2 };
@@ -114,7 +114,7 @@ dgx_status dgx_compute_rhs(dgx_ctx* ctx, const double* u, double* r);
 /* FP32 residual (defect-correction inner residual): FP32 state, FP32 geometry shadow,
  * FP32 arithmetic.  Same semantics as dgx_compute_rhs. */
 dgx_status dgx_compute_rhs_f32(dgx_ctx* ctx, const float* u, float* r);
-/* FP32 residual of an FP64 state (Euler only), same semantics as dgx_compute_rhs: the kernel forms
+/* FP32 residual of an FP64 state (Euler and Navier-Stokes), same semantics as dgx_compute_rhs: the kernel forms
  * u - c_e and F(u) - F(c_e) per element (c_e = the state at the element's first node), evaluates the
  * pointwise physical and numerical fluxes in FP64 and runs the interpolations, contractions and lifts
  * in FP32 on those perturbations.  The F(c_e) part cancels exactly (free-stream preservation), so
