@@ -499,32 +499,50 @@ struct PrecondBlockSGS : Precond {
 // One Arnoldi column on the device (single thread): H(0..j, j) = h1 + h2, H(j+1, j) = |w|, the previous
 // Givens rotations applied to the new column, the new rotation, and the residual estimate |g(j+1)|.
 // st[0] = |g(j+1)|, st[1] = first column count at which |g| <= tol (or -1), st[2] = |w|.
-__global__ void k_arnoldi_step(double* H, double* cs, double* sn, double* g, const double* dcol, int j, int m,
-                               double tol, double* st)
+constexpr int kMaxRestart = 200;
+
+// Column j of the Hessenberg matrix from the Gram-Schmidt coefficients, the previous Givens rotations applied,
+// the new rotation formed, and the residual estimate |g[j+1]| (st[0]); st[1] = j + 1 once it is below tol.
+// The rotation chain is sequential: the column and the rotations are staged in LDS by all lanes first, so the
+// chain runs on LDS values instead of waiting on global loads at every step.
+__global__ void __launch_bounds__(64) k_arnoldi_step(double* H, double* cs, double* sn, double* g, const double* dcol,
+                                                    int j, int m, double tol, double* st)
 {
-	if (threadIdx.x != 0 || blockIdx.x != 0) return;
+	__shared__ double scol[kMaxRestart + 1], sc[kMaxRestart], ss[kMaxRestart];
+	const int t = threadIdx.x;
 	const double* h1 = dcol;
 	const double* h2 = dcol + (m + 1);
-	const double hn = sqrt(dcol[2 * (m + 1)]);
-	double* col = H + (size_t)j * (m + 1);   // column j, rows 0..m
-	for (int i = 0; i <= j; ++i) col[i] = h1[i] + h2[i];
-	col[j + 1] = hn;
-	for (int i = 0; i < j; ++i) {
-		const double a = col[i], b = col[i + 1];
-		col[i] = cs[i] * a + sn[i] * b;
-		col[i + 1] = -sn[i] * a + cs[i] * b;
+	for (int i = t; i <= j; i += blockDim.x) scol[i] = h1[i] + h2[i];
+	for (int i = t; i < j; i += blockDim.x) {
+		sc[i] = cs[i];
+		ss[i] = sn[i];
 	}
-	const double a = col[j], b = col[j + 1];
-	const double r = sqrt(a * a + b * b);
-	cs[j] = r > 0 ? a / r : 1.0;
-	sn[j] = r > 0 ? b / r : 0.0;
-	col[j] = r;
-	col[j + 1] = 0.0;
-	g[j + 1] = -sn[j] * g[j];
-	g[j] = cs[j] * g[j];
-	st[0] = fabs(g[j + 1]);
-	if (st[1] < 0.0 && (st[0] <= tol || hn == 0.0)) st[1] = (double)(j + 1);
-	st[2] = hn;
+	__syncthreads();
+	if (t == 0) {
+		const double hn = sqrt(dcol[2 * (m + 1)]);
+		double a = scol[0];
+		for (int i = 0; i < j; ++i) {
+			const double b = scol[i + 1];
+			scol[i] = sc[i] * a + ss[i] * b;
+			a = -ss[i] * a + sc[i] * b;
+		}
+		const double b = hn;
+		const double r = sqrt(a * a + b * b);
+		const double c = r > 0 ? a / r : 1.0, sj = r > 0 ? b / r : 0.0;
+		cs[j] = c;
+		sn[j] = sj;
+		scol[j] = r;
+		scol[j + 1] = 0.0;
+		const double gj = g[j];
+		g[j + 1] = -sj * gj;
+		g[j] = c * gj;
+		st[0] = fabs(g[j + 1]);
+		if (st[1] < 0.0 && (st[0] <= tol || hn == 0.0)) st[1] = (double)(j + 1);
+		st[2] = hn;
+	}
+	__syncthreads();
+	double* col = H + (size_t)j * (m + 1);   // column j, rows 0..m
+	for (int i = t; i <= j + 1; i += blockDim.x) col[i] = scol[i];
 }
 
 // g = (beta, 0, ...), st = (beta, -1, 1) at the start of a restart cycle
@@ -631,7 +649,7 @@ struct Gmres {
 				DGX_HIP_CHECK(blas.gemv_sub_dev(V, n, dh2, j + 1, w, n));
 				DGX_HIP_CHECK(blas.dots_dev(w, 0, 1, w, n, dnrm));
 				DGX_HIP_CHECK(blas.scale_by_rnorm(w, dnrm, V + (size_t)(j + 1) * n, n));
-				hipLaunchKernelGGL(k_arnoldi_step, dim3(1), dim3(1), 0, c.stream, dH, dcs, dsn, dg, dcol, j, m, tol, dst);
+				hipLaunchKernelGGL(k_arnoldi_step, dim3(1), dim3(64), 0, c.stream, dH, dcs, dsn, dg, dcol, j, m, tol, dst);
 				DGX_HIP_CHECK(hipGetLastError());
 				if ((j + 1) % kCheck == 0 || j + 1 == m || total + 1 >= max_it) {
 					double st[3];
