@@ -207,6 +207,40 @@ __global__ void k_gemv_sub(const double* __restrict__ V, size_t ldv, const doubl
 		w[i] -= s;
 	}
 }
+// The same reductions folded into their consumers (no k_dots_final launch): coefficient j is the sum of the
+// kChunks partials of row j in chunk order, exactly as k_dots_final forms it.
+__device__ inline double sum_partials(const double* __restrict__ partial, int j)
+{
+	double s = 0.0;
+	for (int c = 0; c < kChunks; ++c) s += partial[(size_t)j * kChunks + c];
+	return s;
+}
+// w -= V h, h reduced from the partials of V^T w by each workgroup (k <= 256)
+__global__ void __launch_bounds__(256) k_gemv_sub_partials(const double* __restrict__ V, size_t ldv,
+                                                           const double* __restrict__ partial, int k, double* w, size_t n)
+{
+	__shared__ double h[256];
+	if ((int)threadIdx.x < k) h[threadIdx.x] = sum_partials(partial, threadIdx.x);
+	__syncthreads();
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+		double s = 0.0;
+		for (int j = 0; j < k; ++j) s += V[(size_t)j * ldv + i] * h[j];
+		w[i] -= s;
+	}
+}
+// y = x / sqrt(|x|^2), |x|^2 reduced from its partials
+__global__ void k_scale_by_rnorm_partials(const double* __restrict__ x, const double* __restrict__ partial, double* y,
+                                          size_t n)
+{
+	__shared__ double a;
+	if (threadIdx.x == 0) {
+		const double nv = sum_partials(partial, 0);
+		a = nv > 0.0 ? 1.0 / sqrt(nv) : 0.0;
+	}
+	__syncthreads();
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+		y[i] = a * x[i];
+}
 // y = x / sqrt(nrm2[0])  (normalisation by a device-resident squared norm)
 __global__ void k_scale_by_rnorm(const double* __restrict__ x, const double* __restrict__ nrm2, double* y, size_t n)
 {
@@ -237,6 +271,8 @@ static int grid_for(size_t n) { return (int)((n + 255) / 256 < 4096 ? (n + 255) 
 struct Blas {
 	hipStream_t s;
 	double* partial = nullptr;  // kChunks * kmax
+	double* partial2 = nullptr; // kChunks * kmax: second Gram-Schmidt pass
+	double* partialn = nullptr; // kChunks: |w|^2
 	double* dvec = nullptr;     // kmax
 	std::vector<double> h;
 	int kmax = 0;
@@ -246,14 +282,16 @@ struct Blas {
 		kmax = k;
 		h.resize(k);
 		DGX_HIP_CHECK(hipMalloc(&partial, sizeof(double) * kChunks * k));
+		DGX_HIP_CHECK(hipMalloc(&partial2, sizeof(double) * kChunks * k));
+		DGX_HIP_CHECK(hipMalloc(&partialn, sizeof(double) * kChunks));
 		DGX_HIP_CHECK(hipMalloc(&dvec, sizeof(double) * k));
 		return hipSuccess;
 	}
 	void release()
 	{
-		if (partial) (void)hipFree(partial);
-		if (dvec) (void)hipFree(dvec);
-		partial = dvec = nullptr;
+		for (double* q : { partial, partial2, partialn, dvec })
+			if (q) (void)hipFree(q);
+		partial = partial2 = partialn = dvec = nullptr;
 	}
 	// device out[0..k) = V^T w (deterministic two-stage reduction, no host round trip)
 	hipError_t dots_dev(const double* V, size_t ldv, int k, const double* w, size_t n, double* out)
@@ -268,6 +306,24 @@ struct Blas {
 		DGX_HIP_CHECK(dots_dev(V, ldv, k, w, n, dvec));
 		DGX_HIP_CHECK(hipMemcpyAsync(hout, dvec, sizeof(double) * k, hipMemcpyDeviceToHost, s));
 		return hipStreamSynchronize(s);
+	}
+	// partials only (rows 0..k), into buf
+	hipError_t dots_partials(const double* V, size_t ldv, int k, const double* w, size_t n, double* buf)
+	{
+		hipLaunchKernelGGL(k_dots_partial, dim3(kChunks, k), dim3(256), 0, s, V, ldv, w, n, buf);
+		return hipGetLastError();
+	}
+	// w -= V h, h = the coefficients held as partials in buf (one workgroup per CU, each reduces them once)
+	hipError_t gemv_sub_partials(const double* V, size_t ldv, const double* buf, int k, double* w, size_t n)
+	{
+		const int grid = (int)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024);
+		hipLaunchKernelGGL(k_gemv_sub_partials, dim3(grid), dim3(256), 0, s, V, ldv, buf, k, w, n);
+		return hipGetLastError();
+	}
+	hipError_t scale_by_rnorm_partials(const double* x, const double* buf, double* y, size_t n)
+	{
+		hipLaunchKernelGGL(k_scale_by_rnorm_partials, dim3(grid_for(n)), dim3(256), 0, s, x, buf, y, n);
+		return hipGetLastError();
 	}
 	// w -= V h with h already on the device
 	hipError_t gemv_sub_dev(const double* V, size_t ldv, const double* hdev, int k, double* w, size_t n)
@@ -496,30 +552,28 @@ struct PrecondBlockSGS : Precond {
 
 
 // ------------------------------------------------------------------------------- GMRES
-// One Arnoldi column on the device (single thread): H(0..j, j) = h1 + h2, H(j+1, j) = |w|, the previous
-// Givens rotations applied to the new column, the new rotation, and the residual estimate |g(j+1)|.
-// st[0] = |g(j+1)|, st[1] = first column count at which |g| <= tol (or -1), st[2] = |w|.
 constexpr int kMaxRestart = 200;
 
-// Column j of the Hessenberg matrix from the Gram-Schmidt coefficients, the previous Givens rotations applied,
-// the new rotation formed, and the residual estimate |g[j+1]| (st[0]); st[1] = j + 1 once it is below tol.
+// One Arnoldi column on the device: H(0..j, j) = h1 + h2 (the two Gram-Schmidt passes), H(j+1, j) = |w|, the
+// previous Givens rotations applied to the new column, the new rotation, and the residual estimate.
+// st[0] = |g(j+1)|, st[1] = first column count at which |g| <= tol (or -1), st[2] = |w|.
 // The rotation chain is sequential: the column and the rotations are staged in LDS by all lanes first, so the
 // chain runs on LDS values instead of waiting on global loads at every step.
-__global__ void __launch_bounds__(64) k_arnoldi_step(double* H, double* cs, double* sn, double* g, const double* dcol,
-                                                    int j, int m, double tol, double* st)
+__global__ void __launch_bounds__(64) k_arnoldi_step(double* H, double* cs, double* sn, double* g,
+                                                    const double* __restrict__ p1, const double* __restrict__ p2,
+                                                    const double* __restrict__ pn, int j, int m, double tol, double* st)
 {
 	__shared__ double scol[kMaxRestart + 1], sc[kMaxRestart], ss[kMaxRestart];
 	const int t = threadIdx.x;
-	const double* h1 = dcol;
-	const double* h2 = dcol + (m + 1);
-	for (int i = t; i <= j; i += blockDim.x) scol[i] = h1[i] + h2[i];
+	// Gram-Schmidt coefficients of both passes and |w|^2, reduced from their partials
+	for (int i = t; i <= j; i += blockDim.x) scol[i] = sum_partials(p1, i) + sum_partials(p2, i);
 	for (int i = t; i < j; i += blockDim.x) {
 		sc[i] = cs[i];
 		ss[i] = sn[i];
 	}
 	__syncthreads();
 	if (t == 0) {
-		const double hn = sqrt(dcol[2 * (m + 1)]);
+		const double hn = sqrt(sum_partials(pn, 0));
 		double a = scol[0];
 		for (int i = 0; i < j; ++i) {
 			const double b = scol[i + 1];
@@ -562,8 +616,7 @@ struct Gmres {
 	double* x = nullptr;
 	size_t n = 0;
 	Blas blas;
-	std::vector<double> H, g, cs, sn, y, h1, h2, hcol;
-	double* dcol = nullptr;   // device: [h1 (m+1) | h2 (m+1) | |w|^2]
+	std::vector<double> H, g, cs, sn, y;
 	double* dls = nullptr;    // device: H ((m+1) x m, column-major) | cs (m) | sn (m) | g (m+1) | st (3)
 	hipError_t init(Ctx& c, int restart)
 	{
@@ -578,18 +631,14 @@ struct Gmres {
 		cs.assign(m, 0.0);
 		sn.assign(m, 0.0);
 		y.assign(m, 0.0);
-		h1.assign(m + 1, 0.0);
-		h2.assign(m + 1, 0.0);
-		hcol.assign(2 * (m + 1) + 1, 0.0);
-		DGX_HIP_CHECK(hipMalloc(&dcol, sizeof(double) * (2 * (m + 1) + 1)));
 		DGX_HIP_CHECK(hipMalloc(&dls, sizeof(double) * ls_size()));
 		return blas.init(c.stream, m + 1);
 	}
 	void release()
 	{
-		for (double* p : { V, w, z, x, dcol, dls })
+		for (double* p : { V, w, z, x, dls })
 			if (p) (void)hipFree(p);
-		V = w = z = x = dcol = dls = nullptr;
+		V = w = z = x = dls = nullptr;
 		blas.release();
 	}
 	size_t ls_size() const { return (size_t)(m + 1) * m + 2 * (size_t)m + (m + 1) + 3; }
@@ -618,9 +667,6 @@ struct Gmres {
 		double* dsn = dcs + m;
 		double* dg = dsn + m;
 		double* dst = dg + (m + 1);
-		double* dh1 = dcol;
-		double* dh2 = dcol + (m + 1);
-		double* dnrm = dcol + 2 * (m + 1);
 		std::vector<double> hls(ls_size());
 		int total = 0;
 		while (total < max_it) {
@@ -642,14 +688,16 @@ struct Gmres {
 			for (; j < m && total < max_it; ++j, ++total) {
 				DGX_HIP_CHECK(P.apply(c, V + (size_t)j * n, z));
 				DGX_HIP_CHECK(op(c, u, inv_dt, z, w));
-				// classical Gram-Schmidt, twice (CGS2), coefficients on the device
-				DGX_HIP_CHECK(blas.dots_dev(V, n, j + 1, w, n, dh1));
-				DGX_HIP_CHECK(blas.gemv_sub_dev(V, n, dh1, j + 1, w, n));
-				DGX_HIP_CHECK(blas.dots_dev(V, n, j + 1, w, n, dh2));
-				DGX_HIP_CHECK(blas.gemv_sub_dev(V, n, dh2, j + 1, w, n));
-				DGX_HIP_CHECK(blas.dots_dev(w, 0, 1, w, n, dnrm));
-				DGX_HIP_CHECK(blas.scale_by_rnorm(w, dnrm, V + (size_t)(j + 1) * n, n));
-				hipLaunchKernelGGL(k_arnoldi_step, dim3(1), dim3(64), 0, c.stream, dH, dcs, dsn, dg, dcol, j, m, tol, dst);
+				// classical Gram-Schmidt, twice (CGS2); the coefficients stay as partial sums on the device and
+				// each consumer reduces them
+				DGX_HIP_CHECK(blas.dots_partials(V, n, j + 1, w, n, blas.partial));
+				DGX_HIP_CHECK(blas.gemv_sub_partials(V, n, blas.partial, j + 1, w, n));
+				DGX_HIP_CHECK(blas.dots_partials(V, n, j + 1, w, n, blas.partial2));
+				DGX_HIP_CHECK(blas.gemv_sub_partials(V, n, blas.partial2, j + 1, w, n));
+				DGX_HIP_CHECK(blas.dots_partials(w, 0, 1, w, n, blas.partialn));
+				DGX_HIP_CHECK(blas.scale_by_rnorm_partials(w, blas.partialn, V + (size_t)(j + 1) * n, n));
+				hipLaunchKernelGGL(k_arnoldi_step, dim3(1), dim3(64), 0, c.stream, dH, dcs, dsn, dg, blas.partial,
+				                   blas.partial2, blas.partialn, j, m, tol, dst);
 				DGX_HIP_CHECK(hipGetLastError());
 				if ((j + 1) % kCheck == 0 || j + 1 == m || total + 1 >= max_it) {
 					double st[3];
