@@ -96,7 +96,8 @@ struct NsBlock {
 	static constexpr int Np = N * N, EPB = ns_epb<N>();
 	S I[N * N], DhT[N * N], Lift[2][N], w[N];
 	int perm[8][N];
-	S U[EPB][kNsNV * Np];      // GLL nodal state
+	static constexpr int UR = (kNsNV * Np > 4 * kNsNV * N) ? kNsNV * Np : 4 * kNsNV * N;
+	S U[EPB][UR];              // GLL nodal state (k_rhs_ns: then the face fluxes)
 	S V[EPB][kNsNV * Np];      // GL values
 	double C[EPB][kNsNV];      // PERT: the element's reference state c_e (its first GLL node)
 	ElemFace ef[EPB][4];
@@ -230,11 +231,16 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 	constexpr int Np = N * N, EPB = ns_epb<N>();
 	constexpr S eta = S(1.01 * 4.0);
 	__shared__ NsBlock<S, N> B;
-	__shared__ S TRo[EPB][4 * kNsNV * N];     // own face traces at the face GL nodes, own ordering
-	__shared__ S TRn[EPB][4 * kNsNV * N];     // neighbour face traces, the neighbour's ordering
+	// LDS shared between phases (occupancy): the face traces, dead once the face fluxes are formed, hold W fr_d
+	// for the volume divergence; the GLL state, dead once the traces are formed, holds the face fluxes
+	constexpr int TXS = (8 * kNsNV * N > 2 * kNsNV * Np) ? 8 * kNsNV * N : 2 * kNsNV * Np;
+	__shared__ S TX[EPB][TXS];
 	__shared__ S QF[EPB][4 * 2 * kNsNV * N];  // q = J_f n_d (s_nb - s_own) / 2 per face, own ordering
-	__shared__ S X[EPB][2 * kNsNV * Np];      // W fr_d at the GL nodes
-	__shared__ S RES[EPB][4 * kNsNV * N];     // J_f nnf per face, own ordering
+	S (*TRo)[TXS] = TX;                        // own face traces at the face GL nodes, own ordering: [0, 16 N)
+	// neighbour face traces, the neighbour's ordering: TX[slot] + 16 N
+	S (*X)[TXS] = TX;                          // W fr_d at the GL nodes (after the face phase)
+	constexpr int UR = NsBlock<S, N>::UR;
+	S (*RES)[UR] = B.U;                        // J_f nnf per face, own ordering (after the traces)
 	const int sl = threadIdx.x / Np, t = threadIdx.x % Np;
 	const int e = blockIdx.x * EPB + sl;
 	const bool valid = sl < EPB && e < a.E;
@@ -260,7 +266,7 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 						an += B.I[i * N + b] * a.u[((size_t)en * kNsNV + v) * Np + ns_face_node<N>(fn, b)];
 				}
 				TRo[slc][(lf * kNsNV + v) * N + i] = ao;
-				TRn[slc][(lf * kNsNV + v) * N + i] = an;
+				(TX[slc] + 4 * kNsNV * N)[(lf * kNsNV + v) * N + i] = an;
 			}
 		}
 	}
@@ -273,7 +279,7 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 			const int side = (f.packed >> 4) & 1, ord = (f.packed >> 8) & 7;
 			const int pi = B.perm[ord][i];
 			const S* own = TRo[slc] + lf * kNsNV * N;
-			const S* nb = TRn[slc] + lf * kNsNV * N;
+			const S* nb = (TX[slc] + 4 * kNsNV * N) + lf * kNsNV * N;
 			const S* nJ = a.nJ + ((size_t)f.f * N + i) * 2;
 			const int dst = side == 0 ? i : pi;
 #pragma unroll
@@ -288,53 +294,6 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 	}
 	__syncthreads();
 	if (valid) {
-		// G = G_v + sum_faces G_f at this node, then the volume flux
-		S s[kNsNV], g[2 * kNsNV], f[2 * kNsNV], fv[2 * kNsNV];
-		ns_volume_gradient<S, N>(B, a, slc, t, e, g);
-		const S wq = B.w[c0] * B.w[c1];
-		const S Jq = a.detJ[(size_t)e * Np + t];
-		const S wji = trcp(wq * Jq);
-#pragma unroll
-		for (int lf = 0; lf < 4; ++lf) {
-			const int fd = lf >> 1, cn = fd == 0 ? c0 : c1, at = fd == 0 ? c1 : c0;
-			const S lw = B.Lift[lf & 1][cn] * B.w[at] * wji;
-#pragma unroll
-			for (int d = 0; d < 2; ++d)
-#pragma unroll
-				for (int v = 0; v < kNsNV; ++v) g[v + kNsNV * d] += lw * QF[slc][((lf * 2 + d) * kNsNV + v) * N + at];
-		}
-		if constexpr (PERT) {
-			// FP64 pointwise fluxes of c_e + V minus the Euler flux of c_e; the contractions stay in S
-			double sd[kNsNV], cd[kNsNV], gd[2 * kNsNV], fd[2 * kNsNV], fcd[2 * kNsNV], fvd[2 * kNsNV];
-#pragma unroll
-			for (int v = 0; v < kNsNV; ++v) {
-				cd[v] = B.C[slc][v];
-				sd[v] = cd[v] + (double)B.V[slc][v * Np + t];
-			}
-#pragma unroll
-			for (int k = 0; k < 2 * kNsNV; ++k) gd[k] = (double)g[k];
-			euler_flux<double, 2>(sd, fd);
-			euler_flux<double, 2>(cd, fcd);
-			ns_viscous_flux2<double>(sd, gd, (double)a.mu, (double)a.Pr, fvd);
-#pragma unroll
-			for (int k = 0; k < 2 * kNsNV; ++k) f[k] = S(fd[k] - fcd[k] + fvd[k]);
-		} else {
-#pragma unroll
-			for (int v = 0; v < kNsNV; ++v) s[v] = B.V[slc][v * Np + t];
-			euler_flux<S, 2>(s, f);
-			ns_viscous_flux2<S>(s, g, a.mu, a.Pr, fv);
-#pragma unroll
-			for (int k = 0; k < 2 * kNsNV; ++k) f[k] += fv[k];
-		}
-		const S* m = a.met + (size_t)e * 4 * Np;
-#pragma unroll
-		for (int d0 = 0; d0 < 2; ++d0)
-#pragma unroll
-			for (int v = 0; v < kNsNV; ++v) {
-				// fr_d0 = sum_d1 m(d1, d0) f_d1, m[(i + 2 j)] = J dxi_j / dx_i
-				const S fr = m[(0 + 2 * d0) * Np + t] * f[0 + 2 * v] + m[(1 + 2 * d0) * Np + t] * f[1 + 2 * v];
-				X[slc][(d0 * kNsNV + v) * Np + t] = wq * fr;
-			}
 		// face numerical flux, items in the left ordering (as the Euler kernel)
 		for (int item = t; item < 4 * N; item += Np) {
 			const int lf = item / N, i = item % N;
@@ -343,7 +302,7 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 			const int en = fc.en >= 0 ? fc.en : e;
 			const int pi = B.perm[ord][i];
 			const S* own = TRo[slc] + lf * kNsNV * N;
-			const S* nb = TRn[slc] + lf * kNsNV * N;
+			const S* nb = (TX[slc] + 4 * kNsNV * N) + lf * kNsNV * N;
 			// left / right: element, local face, index of node i in that element's face ordering
 			const int eLf = side == 0 ? e : en, lfL = side == 0 ? lf : fn, iL = i;
 			const int eRf = side == 0 ? en : e, lfR = side == 0 ? fn : lf, iR = pi;
@@ -415,6 +374,56 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 				}
 			}
 		}
+	}
+	__syncthreads();   // the face traces are dead: X takes their place
+	if (valid) {
+		// G = G_v + sum_faces G_f at this node, then the volume flux
+		S s[kNsNV], g[2 * kNsNV], f[2 * kNsNV], fv[2 * kNsNV];
+		ns_volume_gradient<S, N>(B, a, slc, t, e, g);
+		const S wq = B.w[c0] * B.w[c1];
+		const S Jq = a.detJ[(size_t)e * Np + t];
+		const S wji = trcp(wq * Jq);
+#pragma unroll
+		for (int lf = 0; lf < 4; ++lf) {
+			const int fd = lf >> 1, cn = fd == 0 ? c0 : c1, at = fd == 0 ? c1 : c0;
+			const S lw = B.Lift[lf & 1][cn] * B.w[at] * wji;
+#pragma unroll
+			for (int d = 0; d < 2; ++d)
+#pragma unroll
+				for (int v = 0; v < kNsNV; ++v) g[v + kNsNV * d] += lw * QF[slc][((lf * 2 + d) * kNsNV + v) * N + at];
+		}
+		if constexpr (PERT) {
+			// FP64 pointwise fluxes of c_e + V minus the Euler flux of c_e; the contractions stay in S
+			double sd[kNsNV], cd[kNsNV], gd[2 * kNsNV], fd[2 * kNsNV], fcd[2 * kNsNV], fvd[2 * kNsNV];
+#pragma unroll
+			for (int v = 0; v < kNsNV; ++v) {
+				cd[v] = B.C[slc][v];
+				sd[v] = cd[v] + (double)B.V[slc][v * Np + t];
+			}
+#pragma unroll
+			for (int k = 0; k < 2 * kNsNV; ++k) gd[k] = (double)g[k];
+			euler_flux<double, 2>(sd, fd);
+			euler_flux<double, 2>(cd, fcd);
+			ns_viscous_flux2<double>(sd, gd, (double)a.mu, (double)a.Pr, fvd);
+#pragma unroll
+			for (int k = 0; k < 2 * kNsNV; ++k) f[k] = S(fd[k] - fcd[k] + fvd[k]);
+		} else {
+#pragma unroll
+			for (int v = 0; v < kNsNV; ++v) s[v] = B.V[slc][v * Np + t];
+			euler_flux<S, 2>(s, f);
+			ns_viscous_flux2<S>(s, g, a.mu, a.Pr, fv);
+#pragma unroll
+			for (int k = 0; k < 2 * kNsNV; ++k) f[k] += fv[k];
+		}
+		const S* m = a.met + (size_t)e * 4 * Np;
+#pragma unroll
+		for (int d0 = 0; d0 < 2; ++d0)
+#pragma unroll
+			for (int v = 0; v < kNsNV; ++v) {
+				// fr_d0 = sum_d1 m(d1, d0) f_d1, m[(i + 2 j)] = J dxi_j / dx_i
+				const S fr = m[(0 + 2 * d0) * Np + t] * f[0 + 2 * v] + m[(1 + 2 * d0) * Np + t] * f[1 + 2 * v];
+				X[slc][(d0 * kNsNV + v) * Np + t] = wq * fr;
+			}
 	}
 	__syncthreads();
 	S z[kNsNV];
