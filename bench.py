@@ -10,7 +10,9 @@ the FP64 residual is timed beside it and its error is stated against the FP64 pa
 
 CPU baseline: the FP64 oracle (a C restatement of the reference's dense per-element operator
 application) on one host core, timed on a bounded sample of the same workload (an 8^3 periodic
-mesh of the same curved HEX p=4 elements), and the same on all host cores (OpenMP, at most 16).
+mesh of the same curved HEX p=4 elements), and the same on all host cores (OpenMP, at most 16).  For the
+steady channel solve, the oracle's Newton-GMRES with LU block-Jacobi on one core, estimated from a bounded
+sample (cpu_steady_estimate).
 """
 from __future__ import annotations
 
@@ -106,7 +108,72 @@ def cpu_baseline(target_seconds=12.0):
     return out
 
 
-def time_to_steady(torch, dev, epochs=60):
+def cpu_steady_estimate(newton_steps, gmres_iters, restart=60, n_blocks=1024, n_iters=30):
+    """CPU time-to-1e-10 of the same Newton-GMRES with exact LU block-Jacobi, estimated from a bounded sample on
+    one host core: the oracle's complex-step diagonal blocks of n_blocks elements (A_e = dR_e/du_e - M_e/dt)
+    and their inverses, scaled to all elements, per Newton step; and n_iters right-preconditioned GMRES
+    iterations (block-inverse apply, complex-step J.v minus the mass term, two classical Gram-Schmidt passes
+    over a basis that grows to n_iters vectors), per GMRES iteration.  The estimate is
+    newton_steps x (residual + set-up) + gmres_iters x (one iteration at the average basis size of GMRES(60)),
+    with the Newton and GMRES counts of the GPU block-Jacobi leg."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import pyoracle
+    from threadpoolctl import threadpool_limits
+    from dpgsolver_amd import mesh as M
+
+    mesh = M.channel_mesh(64, 64, 3)
+    prim = M.freestream_prim(2)
+    orc = pyoracle.Oracle(mesh, M.PDE_EULER, M.FLUX_ROE, bc_prim=prim)
+    u = M.uniform_state(mesh, prim) * (1.0 + 1e-3 * np.random.default_rng(5).uniform(-1, 1, mesh.E * 64))
+    E, nb, n = mesh.E, 64, mesh.E * 64
+    dt = 0.1
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        orc.rhs(u)
+        t_res = time.perf_counter() - t0
+        idx = np.random.default_rng(0).choice(E, n_blocks, replace=False)
+        t0 = time.perf_counter()
+        blk = np.empty((n_blocks, nb, nb))
+        for k, e in enumerate(idx):
+            blk[k] = orc.jacobian_block(u, int(e), int(e)) - np.kron(np.eye(4), orc.mass(int(e))) / dt
+        inv = np.linalg.inv(blk)
+        t_setup = (time.perf_counter() - t0) * E / n_blocks
+        inv_all = np.tile(inv, (E // n_blocks + 1, 1, 1))[:E]
+        mass = np.stack([orc.mass(e) for e in range(E)])
+        V = np.zeros((n_iters + 1, n))
+        r0 = np.random.default_rng(1).standard_normal(n)
+        V[0] = r0 / np.linalg.norm(r0)
+        t0 = time.perf_counter()
+        for j in range(n_iters):
+            z = np.einsum("eij,ej->ei", inv_all, V[j].reshape(E, nb)).reshape(-1)
+            jre, jim = orc.rhs_complex(u, 1e-30 * z)
+            w = jim / 1e-30 - np.einsum("eab,evb->eva", mass, z.reshape(E, 4, 16)).reshape(-1) / dt
+            for _ in range(2):
+                h = V[: j + 1] @ w
+                w -= h @ V[: j + 1]
+            V[j + 1] = w / np.linalg.norm(w)
+        t_it = (time.perf_counter() - t0) / n_iters
+    # CGS2 grows linearly with the basis: the sampled iterations average (n_iters+1)/2 basis vectors, GMRES(restart)
+    # averages (restart+1)/2; add the difference at the measured cost of CGS2 over n_iters vectors
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        w = V[n_iters].copy()
+        for _ in range(2):
+            h = V[:n_iters] @ w
+            w -= h @ V[:n_iters]
+        t_cgs = time.perf_counter() - t0
+    t_it_full = t_it + t_cgs * ((restart + 1) / 2 - (n_iters + 1) / 2) / n_iters
+    total = newton_steps * (t_res + t_setup) + gmres_iters * t_it_full
+    return {"seconds_estimate": round(total, 1), "cores": 1, "kind": "port",
+            "per_newton_setup_s": round(t_res + t_setup, 3), "per_gmres_iter_s": round(t_it_full, 5),
+            "newton_steps": newton_steps, "gmres_iters": gmres_iters,
+            "sample": f"oracle complex-step diagonal blocks + inverses of {n_blocks} of {E} elements (scaled), "
+                      f"{n_iters} GMRES iterations (complex-step J.v, block-inverse apply, CGS2) on 1 core; "
+                      "counts from the GPU block-Jacobi solve"}
+
+
+def time_to_steady(torch, dev, epochs=60, cpu=True):
     """Time to max|R| < 1e-10 (FP64) on the steady channel workload: 2-D Euler, DG p=3, 64x64 QUAD channel
     [-1.5,1.5]x[0,0.8] with the bump y = 0.0625 exp(-25 x^2), inflow Mach 0.5, Roe flux, pseudo-time
     Newton-GMRES(60) (rtol 1e-4) from the free stream with cfl_initial 1 and the ramped CFL capped at 300.
@@ -173,6 +240,10 @@ def time_to_steady(torch, dev, epochs=60):
                         "loss_trained": loss, "structure": "Kronecker rank 8, MLP 96-256-256-2176 BF16/FP32"},
            "learned_vs_bj_wall": round(st_ln["seconds"] / st_bj["seconds"], 4),
            "steady_states_rel_diff": diff}
+    if cpu:
+        out["cpu_block_jacobi_estimate"] = cpu_steady_estimate(st_bj["newton_steps"], st_bj["gmres_iters"])
+        print(f"cpu block-Jacobi estimate: {out['cpu_block_jacobi_estimate']['seconds_estimate']} s",
+              file=sys.stderr, flush=True)
     mlp.close()
     ctx.close()
     return out
@@ -276,7 +347,7 @@ def main():
 
     dev = torch.device("cuda:0")
     if args.implicit_only:
-        print(json.dumps({"time_to_1e-10": time_to_steady(torch, dev)}), flush=True)
+        print(json.dumps({"time_to_1e-10": time_to_steady(torch, dev, cpu=not args.no_cpu)}), flush=True)
         return
     if args.ns_only:
         print(json.dumps({"navier_stokes": ns_throughput(torch, dev, args.steps, args.warmup)}), flush=True)
@@ -376,7 +447,7 @@ def main():
     ns = ns_throughput(torch, dev, args.steps, args.warmup)
     print(f"navier-stokes residual: {ns['ms_per_step']:.4f} ms, {ns['value']} GDOF/s", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline()
-    implicit = None if args.no_implicit else time_to_steady(torch, dev)
+    implicit = None if args.no_implicit else time_to_steady(torch, dev, cpu=not args.no_cpu)
 
     out = {
         "metric": "GDOF/s residual eval (FP64 vs FP32-MFMA); time-to-1e-10 steady residual",
