@@ -466,3 +466,148 @@ def freestream_prim(d: int, mach: float = 0.5) -> np.ndarray:
 def uniform_state(mesh: Mesh, prim: np.ndarray) -> np.ndarray:
     cons = prim_to_cons(prim[None, None, :], mesh.d)
     return pack_state(np.broadcast_to(cons, (mesh.E, mesh.Np, mesh.d + 2)).copy())
+
+
+# --------------------------------------------------------------------------- variable order (2-D QUAD)
+@dataclass
+class HpMesh:
+    """2-D QUAD mesh with one order per element (face_solver.c:123-131 gives a face the larger order of its
+    two volumes).  Elements are sorted by order (non-decreasing), so the elements of one order form one
+    contiguous group of the state vector, each in the uniform layout [var][node] of its own Np.
+    * ``p``     (E,) int32 -- element orders, non-decreasing.
+    * ``xyz``   list of (2, Np_e) float64 -- GLL geometry nodes per element, r-fastest.
+    * ``faces`` (F, 6) int32 -- the uniform mesh's face table (eL, fL, eR, fR, ord, bc).
+    * ``grid``  (E,) int -- structured-grid index of each element (before sorting by order)."""
+    p: np.ndarray
+    xyz: list
+    faces: np.ndarray
+    periods: tuple = ()
+    dims: tuple = ()
+    grid: np.ndarray = None
+    d: int = 2
+
+    @property
+    def E(self) -> int:
+        return len(self.p)
+
+    @property
+    def F(self) -> int:
+        return self.faces.shape[0]
+
+    def Np(self, e: int) -> int:
+        return (int(self.p[e]) + 1) ** 2
+
+    def offsets(self, nv: int) -> np.ndarray:
+        """(E+1,) start of each element's block in the flat state vector."""
+        sizes = (self.p.astype(np.int64) + 1) ** 2 * nv
+        return np.concatenate([[0], np.cumsum(sizes)])
+
+    def flat_xyz(self) -> np.ndarray:
+        return np.ascontiguousarray(np.concatenate([x.ravel() for x in self.xyz]))
+
+
+def structured_hp_mesh(p_elem, n: tuple, lo: tuple, hi: tuple, periodic: tuple, curve=None, rotate=None,
+                       bc_low=None, bc_high=None) -> HpMesh:
+    """Structured 2-D QUAD grid (i fastest) with per-element orders p_elem (grid order), reordered so that the
+    orders are non-decreasing (stable).  Each element carries its own isoparametric GLL geometry nodes; a face
+    is matched by its two end nodes, which every order shares (ord 0 or 2: same or reversed direction)."""
+    d = 2
+    p_grid = np.asarray(p_elem, dtype=np.int32)
+    E = int(np.prod(n))
+    assert p_grid.shape == (E,)
+    eidx = _tp_coords_var(n)
+    h = np.array([(hi[k] - lo[k]) / n[k] for k in range(d)])
+    base = np.array(lo)[None, :] + eidx * h[None, :]
+    xyz_grid = []
+    for e in range(E):
+        N = int(p_grid[e]) + 1
+        x1, _ = gauss_lobatto(N)
+        ref = x1[_tp_coords(N, d)]
+        pts = base[e][None, :] + 0.5 * (ref + 1.0) * h[None, :]
+        if curve is not None:
+            pts = curve(pts)
+        if rotate is not None and int(rotate[e]) % 4:
+            pts = _apply_rotation(d, N, pts[None], np.array([rotate[e]]))[0]
+        xyz_grid.append(np.ascontiguousarray(pts.T))
+    pairs, bfaces = [], []
+    strides = [int(np.prod(n[:k])) for k in range(d)]
+    all_e = np.arange(E)
+    for k in range(d):
+        ck = eidx[:, k]
+        inside = ck + 1 < n[k]
+        en = all_e + strides[k]
+        if periodic[k]:
+            en = np.where(inside, en, all_e - (n[k] - 1) * strides[k])
+            sel = np.ones(E, dtype=bool)
+        else:
+            sel = inside
+            bfaces += [(int(e), k, 1) for e in all_e[~inside]]
+            bfaces += [(int(e), k, 0) for e in all_e[ck == 0]]
+        pairs += list(zip(all_e[sel].tolist(), en[sel].tolist(), [k] * int(sel.sum())))
+    pairs.sort(key=lambda t: (t[0], t[2]))
+    per = np.array([(hi[k] - lo[k]) if periodic[k] else 0.0 for k in range(d)])
+
+    def wrap(a):
+        a = a.copy()
+        for k in range(d):
+            if per[k] > 0:
+                a[..., k] -= per[k] * np.round(a[..., k] / per[k])
+        return a
+
+    def ends(e, lf):
+        N = int(p_grid[e]) + 1
+        idx = face_node_volume_index(d, N, lf)
+        return xyz_grid[e][:, idx[0]], xyz_grid[e][:, idx[-1]]
+
+    def center(e, lf):
+        a, b = ends(e, lf)
+        return 0.5 * (a + b)
+
+    rows = []
+    for eL, eR, k in pairs:
+        fL = int(np.argmax([center(eL, lf)[k] for lf in range(4)]))
+        aL, bL = ends(eL, fL)
+        found = None
+        for fR in range(4):
+            if eL == eR and fR == fL:
+                continue
+            aR, bR = ends(eR, fR)
+            if np.abs(wrap(aL - aR)).max() < 1e-9 and np.abs(wrap(bL - bR)).max() < 1e-9:
+                found = (fR, 0)
+            elif np.abs(wrap(aL - bR)).max() < 1e-9 and np.abs(wrap(bL - aR)).max() < 1e-9:
+                found = (fR, 2)
+            if found:
+                break
+        if found is None:
+            raise ValueError(f"faces of elements {eL},{eR} do not match")
+        rows.append((eL, fL, eR, found[0], found[1], BC_INTERIOR))
+    if bfaces and rotate is not None:
+        raise ValueError("rotated elements are only supported on fully periodic meshes")
+    for e, k, side in bfaces:
+        rows.append((e, 2 * k + side, -1, -1, 0, (bc_high if side else bc_low)[k]))
+    # sort the elements by order and renumber the faces
+    order = np.argsort(p_grid, kind="stable")
+    new_of = np.empty(E, dtype=np.int64)
+    new_of[order] = np.arange(E)
+    faces = np.array(rows, dtype=np.int64).reshape(-1, 6)
+    faces[:, 0] = new_of[faces[:, 0]]
+    inter = faces[:, 2] >= 0
+    faces[inter, 2] = new_of[faces[inter, 2]]
+    return HpMesh(p=p_grid[order].copy(), xyz=[xyz_grid[e] for e in order], faces=faces.astype(np.int32),
+                  periods=tuple(per.tolist()), dims=tuple(n), grid=order)
+
+
+def hp_nodes_xyz(mesh: HpMesh) -> list:
+    """Per element (Np_e, 2) node coordinates."""
+    return [x.T for x in mesh.xyz]
+
+
+def hp_state(mesh: HpMesh, cons_fn) -> np.ndarray:
+    """Flat state: cons_fn(x (Np, 2)) -> (Np, nv) conserved values per element, packed [var][node]."""
+    return np.ascontiguousarray(np.concatenate([cons_fn(x).T.ravel() for x in hp_nodes_xyz(mesh)]))
+
+
+def config4_hp_orders(n: tuple, seed: int = SEED) -> np.ndarray:
+    """Per-element orders p ~ U{1..5} of the variable-order Navier-Stokes workload (seeded, grid order)."""
+    r = SplitMix64(seed + 4).uniform(int(np.prod(n)), 0.0, 1.0)
+    return (1 + np.minimum((r * 5).astype(np.int32), 4)).astype(np.int32)

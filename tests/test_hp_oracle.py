@@ -1,0 +1,94 @@
+"""Pins the variable-order oracle (oracle/hp_oracle.py) before it checks the GPU.
+
+* On meshes where every element has the same order it equals the dense-operator C oracle
+  (or_compute_rhs_d) on curved meshes with rotated elements: the volume term, metric conventions, face
+  traces, orientation permutations and lifts are the uniform path's.
+* Free stream: a constant state has zero residual on a straight-sided variable-order mesh with rotated
+  elements.  A transposed or mis-weighted projection at a nonconforming face, or a face cubature that does
+  not integrate the larger-order test functions, leaves a residual.
+* Conservation: the residual sums to zero per conserved variable on a curved periodic variable-order mesh
+  (both sides lift the same w J_f F-hat), which fails for a lift on the wrong side or a missing permutation.
+* Accuracy: M^-1 R converges to -div F of a smooth state at order p_min on checkerboard order patterns (the
+  face traces of the two orders differ by O(h^(p_min+1)), lifted by O(1/h)); uniform orders converge at
+  order p.
+The face order max(p_L, p_R) (face_solver.c:123-131) and the face geometry of the lower-order side
+(face_solver.c:95-107) are restated from the source text; no reference output pins them further.
+"""
+import numpy as np
+import pytest
+
+import hp_oracle as H
+import pyoracle as O
+from dpgsolver_amd import mesh as M
+
+
+def _curve(pts):
+    out = pts.copy()
+    x, y = pts[..., 0], pts[..., 1]
+    out[..., 0] = x + 0.03 * np.sin(2 * np.pi * y)
+    out[..., 1] = y + 0.03 * np.sin(2 * np.pi * x)
+    return out
+
+
+def _prim(x):
+    X, Y = x[:, 0], x[:, 1]
+    return M.prim_to_cons(np.stack([1 + 0.1 * np.sin(2 * np.pi * X), 0.3 + 0.1 * np.cos(2 * np.pi * Y),
+                                    -0.2 + 0.1 * np.sin(2 * np.pi * X),
+                                    1 + 0.1 * np.sin(2 * np.pi * (X + Y))], -1), 2)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+@pytest.mark.parametrize("flux", [M.FLUX_ROE, M.FLUX_LF])
+def test_uniform_order_equals_c_oracle(p, flux):
+    n = (4, 3)
+    rot = np.random.default_rng(p).integers(0, 4, 12)
+    hm = M.structured_hp_mesh(np.full(12, p), n, (0, 0), (1, 1), (True, True), curve=_curve, rotate=rot)
+    um = M.structured_tp_mesh(2, p, n, (0, 0), (1, 1), (True, True), curve=_curve, rotate=rot)
+    assert (hm.grid == np.arange(12)).all() and (hm.faces == um.faces).all()
+    u = M.pack_state(np.stack([_prim(x) for x in M.nodes_xyz(um)]))
+    r_c = O.Oracle(um, M.PDE_EULER, flux).rhs(u)
+    r_h = H.HpOracle(hm, M.PDE_EULER, flux).rhs(u)
+    assert np.abs(r_c - r_h).max() <= 1e-13 * np.abs(r_c).max()
+
+
+def test_free_stream_and_conservation_on_variable_order_meshes():
+    n = (4, 4)
+    pe = M.config4_hp_orders(n)
+    assert len(set(pe.tolist())) >= 3
+    rot = np.random.default_rng(1).integers(0, 4, 16)
+    const = lambda x: M.prim_to_cons(np.tile([1.1, 0.3, -0.2, 0.9], (x.shape[0], 1)), 2)
+    straight = M.structured_hp_mesh(pe, n, (0, 0), (1, 1), (True, True), rotate=rot)
+    orc = H.HpOracle(straight, M.PDE_EULER, M.FLUX_ROE)
+    assert np.abs(orc.rhs(M.hp_state(straight, const))).max() < 1e-13
+    curved = M.structured_hp_mesh(pe, n, (0, 0), (1, 1), (True, True), curve=_curve, rotate=rot)
+    orc = H.HpOracle(curved, M.PDE_EULER, M.FLUX_LF)
+    r = orc.rhs(M.hp_state(curved, _prim))
+    tot = sum(r[orc.off[e]:orc.off[e + 1]].reshape(4, -1).sum(1) for e in range(curved.E))
+    assert np.abs(tot).max() < 1e-14 * np.abs(r).sum()
+
+
+def _exact_rate(x):
+    """-div F of the smooth state, by central differences of the analytic flux (h = 1e-5)."""
+    h, out = 1e-5, np.zeros((x.shape[0], 4))
+    for k in range(2):
+        dx = np.zeros(2)
+        dx[k] = h
+        fp = np.stack([O.euler_flux(2, s).reshape(4, 2)[:, k] for s in _prim(x + dx)])
+        fm = np.stack([O.euler_flux(2, s).reshape(4, 2)[:, k] for s in _prim(x - dx)])
+        out -= (fp - fm) / (2 * h)
+    return out
+
+
+@pytest.mark.parametrize("pa,pb", [(2, 4), (3, 5)])
+def test_checkerboard_orders_converge_at_the_lower_order(pa, pb):
+    errs = []
+    for nn in (4, 8):
+        I, J = np.meshgrid(np.arange(nn), np.arange(nn), indexing="xy")
+        pe = np.where((I + J).ravel() % 2 == 0, pa, pb)
+        hm = M.structured_hp_mesh(pe, (nn, nn), (0, 0), (1, 1), (True, True))
+        orc = H.HpOracle(hm, M.PDE_EULER, M.FLUX_ROE)
+        mr = orc.apply_minv(orc.rhs(M.hp_state(hm, _prim)))
+        ex = M.hp_state(hm, _exact_rate)
+        errs.append(np.abs(mr - ex).max() / np.abs(ex).max())
+    # measured (n = 4, 8, 16): (2,4) 1.33, 0.41, 0.10 -> rates 1.7, 2.0; (3,5) 0.21, 0.036, 0.0047 -> 2.5, 2.9
+    assert np.log2(errs[0] / errs[1]) >= pa - 0.5, errs
