@@ -21,7 +21,7 @@ _ERRORS = {-1: "DGX_ERR_ARG", -2: "DGX_ERR_HIP", -3: "DGX_ERR_UNSUPPORTED", -4: 
 # every symbol include/dgx.h declares
 EXPORTS = [
     "dgx_version", "dgx_device_count", "dgx_create", "dgx_destroy", "dgx_set_stream", "dgx_synchronize",
-    "dgx_sizes", "dgx_upload_mesh", "dgx_get_geometry", "dgx_compute_rhs", "dgx_compute_rhs_f32", "dgx_compute_rhs_mixed",
+    "dgx_sizes", "dgx_upload_mesh", "dgx_upload_mesh_hp", "dgx_get_geometry", "dgx_compute_rhs", "dgx_compute_rhs_f32", "dgx_compute_rhs_mixed",
     "dgx_compute_rhs_explicit", "dgx_apply_minv", "dgx_enforce_positivity", "dgx_ssp_rk3_step",
     "dgx_graph_create_rhs", "dgx_graph_create_rk3", "dgx_graph_launch", "dgx_graph_sync", "dgx_graph_destroy",
     "dgx_set_rhs_grid", "dgx_autotune_rhs", "dgx_max_abs",
@@ -56,6 +56,11 @@ class DgxHostMesh(C.Structure):
 
 class DgxMesh(C.Structure):
     _fields_ = [("E", C.c_int), ("F", C.c_int), ("xyz", C.POINTER(C.c_double)), ("faces", C.POINTER(C.c_int))]
+
+
+class DgxMeshHp(C.Structure):
+    _fields_ = [("E", C.c_int), ("F", C.c_int), ("p", C.POINTER(C.c_int)), ("xyz", C.POINTER(C.c_double)),
+                ("faces", C.POINTER(C.c_int))]
 
 
 class ImplicitParams(C.Structure):
@@ -94,6 +99,7 @@ def load():
         "dgx_synchronize": [vp],
         "dgx_sizes": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_longlong)],
         "dgx_upload_mesh": [vp, C.POINTER(DgxMesh)],
+        "dgx_upload_mesh_hp": [vp, C.POINTER(DgxMeshHp)],
         "dgx_get_geometry": [vp, dp, dp, dp],
         "dgx_compute_rhs": [vp, vp, vp],
         "dgx_compute_rhs_f32": [vp, vp, vp],
@@ -185,7 +191,9 @@ class Context:
 
     @property
     def ndof(self) -> int:
-        return self.E * self.Np * self.nv
+        nd = C.c_longlong(0)
+        check("dgx_sizes", self.L.dgx_sizes(self.h, None, None, None, C.byref(nd)))
+        return int(nd.value) if nd.value > 0 else self.E * self.Np * self.nv
 
     def set_stream(self, stream_ptr: int):
         check("dgx_set_stream", self.L.dgx_set_stream(self.h, vp(stream_ptr)))
@@ -201,6 +209,19 @@ class Context:
         m.xyz = xyz.ctypes.data_as(dp)
         m.faces = faces.ctypes.data_as(C.POINTER(C.c_int))
         check("dgx_upload_mesh", self.L.dgx_upload_mesh(self.h, C.byref(m)))
+        self.E, self.F = m.E, m.F
+
+    def upload_mesh_hp(self, mesh):
+        """Variable-order 2-D mesh (mesh.HpMesh: orders non-decreasing, per-element geometry nodes)."""
+        p = np.ascontiguousarray(mesh.p, dtype=np.int32)
+        xyz = mesh.flat_xyz()
+        faces = np.ascontiguousarray(mesh.faces, dtype=np.int32)
+        m = DgxMeshHp()
+        m.E, m.F = int(p.size), int(faces.shape[0])
+        m.p = p.ctypes.data_as(C.POINTER(C.c_int))
+        m.xyz = xyz.ctypes.data_as(dp)
+        m.faces = faces.ctypes.data_as(C.POINTER(C.c_int))
+        check("dgx_upload_mesh_hp", self.L.dgx_upload_mesh_hp(self.h, C.byref(m)))
         self.E, self.F = m.E, m.F
 
     def geometry(self):

@@ -35,8 +35,20 @@ void release(T*& p)
 	p = nullptr;
 }
 
+void free_hp(Ctx& c)
+{
+	if (!c.hp) return;
+	dgx::HpData* H = c.hp;
+	for (dgx_ctx* g : H->groups) dgx_destroy(g);
+	release(H->faces); release(H->nJ); release(H->nJ32); release(H->z64); release(H->z32);
+	release(H->ops64); release(H->ops32);
+	delete H;
+	c.hp = nullptr;
+}
+
 void free_mesh(Ctx& c)
 {
+	free_hp(c);
 	release(c.xyz); release(c.faces); release(c.elem_faces); release(c.order); release(c.hmin); release(c.metrics_vc); release(c.metrics_vc32);
 	release(c.detJ_vc); release(c.detJ_vc32); release(c.nJ_fc); release(c.nJ_fc32); release(c.u0); release(c.rtmp);
 	release(c.color_elems);
@@ -46,7 +58,9 @@ void free_mesh(Ctx& c)
 	c.mesh_ready = false;
 }
 
-size_t ndof(const Ctx& c) { return (size_t)c.E * c.Np * c.nv; }
+size_t ndof(const Ctx& c) { return c.hp ? (size_t)c.hp->ndof : (size_t)c.E * c.Np * c.nv; }
+
+dgx_status upload_uniform(dgx_ctx* h, const dgx_mesh* m, bool nonconf_ok);
 
 }  // namespace
 
@@ -136,7 +150,15 @@ dgx_status dgx_sizes(const dgx_ctx* h, int* Np, int* nv, int* Nfc, long long* nd
 	return DGX_OK;
 }
 
-dgx_status dgx_upload_mesh(dgx_ctx* h, const dgx_mesh* m)
+dgx_status dgx_upload_mesh(dgx_ctx* h, const dgx_mesh* m) { return upload_uniform(h, m, false); }
+
+}  // extern "C"
+
+namespace {
+
+// The uniform-order upload; nonconf_ok admits faces marked kBcNonconforming (the order groups of a
+// variable-order mesh).
+dgx_status upload_uniform(dgx_ctx* h, const dgx_mesh* m, bool nonconf_ok)
 {
 	if (!h || !m || m->E <= 0 || m->F < 0 || !m->xyz || (m->F > 0 && !m->faces)) return DGX_ERR_ARG;
 	Ctx& c = h->c;
@@ -149,6 +171,8 @@ dgx_status dgx_upload_mesh(dgx_ctx* h, const dgx_mesh* m)
 		if (eL < 0 || eL >= m->E || fL < 0 || fL >= NFACE || ord < 0 || ord > 7) return DGX_ERR_ARG;
 		if (bc == DGX_BC_INTERIOR) {
 			if (eR < 0 || eR >= m->E || fR < 0 || fR >= NFACE) return DGX_ERR_ARG;
+		} else if (nonconf_ok && bc == dgx::kBcNonconforming) {
+			// face to an element of another order: its flux arrives lifted from the face kernel
 		} else {
 			if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;   // viscous BCs not built
 			if (c.cfg.pde != DGX_PDE_EULER || (bc != DGX_BC_RIEMANN && bc != DGX_BC_SLIPWALL)) return DGX_ERR_ARG;
@@ -224,11 +248,250 @@ dgx_status dgx_upload_mesh(dgx_ctx* h, const dgx_mesh* m)
 	return DGX_OK;
 }
 
+// Lagrange basis of the nodes xn (n) and its derivative at x.
+void lagrange_at(const double* xn, int n, double x, double* l, double* dl)
+{
+	for (int i = 0; i < n; ++i) {
+		double li = 1.0, dli = 0.0;
+		for (int j = 0; j < n; ++j) {
+			if (j == i) continue;
+			const double r = 1.0 / (xn[i] - xn[j]);
+			dli = dli * (x - xn[j]) * r + li * r;
+			li *= (x - xn[j]) * r;
+		}
+		l[i] = li;
+		dl[i] = dli;
+	}
+}
+
+void build_hp_ops(dgx::HpOps<double>& o)
+{
+	std::memset(&o, 0, sizeof(o));
+	double xl[dgx::kHpMaxN + 1][dgx::kHpMaxN], xg[dgx::kHpMaxN + 1][dgx::kHpMaxN], wg[dgx::kHpMaxN + 1][dgx::kHpMaxN], w[dgx::kHpMaxN];
+	for (int N = 2; N <= dgx::kHpMaxN; ++N) {
+		dgx::gauss_lobatto(N, xl[N], w);
+		dgx::gauss_legendre(N, xg[N], wg[N]);
+		for (int i = 0; i < N; ++i) {
+			o.w[N][i] = wg[N][i];
+			o.winv[N][i] = 1.0 / wg[N][i];
+		}
+	}
+	double l[dgx::kHpMaxN], dl[dgx::kHpMaxN];
+	for (int Ns = 2; Ns <= dgx::kHpMaxN; ++Ns)
+		for (int Nf = 2; Nf <= dgx::kHpMaxN; ++Nf)
+			for (int q = 0; q < Nf; ++q) {
+				lagrange_at(xl[Ns], Ns, xg[Nf][q], l, dl);
+				for (int i = 0; i < Ns; ++i) o.Ig[Ns][Nf][q * Ns + i] = l[i];
+				lagrange_at(xg[Ns], Ns, xg[Nf][q], l, dl);
+				for (int i = 0; i < Ns; ++i) o.Pg[Ns][Nf][q * Ns + i] = l[i];
+			}
+}
+
+int face_node_2d(int N, int lf, int i)
+{
+	const int fix = (lf & 1) ? N - 1 : 0;
+	return (lf >> 1) == 0 ? fix + N * i : i + N * fix;
+}
+
+// n J_f (outward) of local face lf of an element of N GLL nodes per direction at the face coordinates s
+// (face_solver.c:95-107 takes the face geometry from one volume): the rotated tangent of the face
+// interpolant, J grad xi = (y_eta, -x_eta) on xi faces, J grad eta = (-y_xi, x_xi) on eta faces.
+void outward_nj(const double* xyz, int N, int lf, const double* s, int ns, double* nj)
+{
+	double xl[dgx::kHpMaxN], w[dgx::kHpMaxN], l[dgx::kHpMaxN], dl[dgx::kHpMaxN];
+	dgx::gauss_lobatto(N, xl, w);
+	const int Np = N * N;
+	for (int q = 0; q < ns; ++q) {
+		lagrange_at(xl, N, s[q], l, dl);
+		double tx = 0.0, ty = 0.0;
+		for (int i = 0; i < N; ++i) {
+			const int vn = face_node_2d(N, lf, i);
+			tx += dl[i] * xyz[vn];
+			ty += dl[i] * xyz[Np + vn];
+		}
+		double nx, ny;
+		if ((lf >> 1) == 0) { nx = ty; ny = -tx; }
+		else { nx = -ty; ny = tx; }
+		const double sg = (lf & 1) ? 1.0 : -1.0;
+		nj[2 * q] = sg * nx;
+		nj[2 * q + 1] = sg * ny;
+	}
+}
+
+// Variable-order residuals: the faces between orders first (lifted fluxes), then each order group's residual
+// kernel on its contiguous part of the state (its faces to other orders read from the lifted fluxes).
+dgx_status hp_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned long long* dmax)
+{
+	dgx::HpData& H = *c.hp;
+	DGX_TRY(dgx::launch_hp_faces64(c, u));
+	for (size_t g = 0; g < H.groups.size(); ++g) {
+		Ctx& s = H.groups[g]->c;
+		s.stream = c.stream;
+		DGX_TRY(dgx::launch_rhs64(s, u + H.u_off[g], r + H.u_off[g], mode, dmax));
+	}
+	return DGX_OK;
+}
+
+dgx_status hp_rhs32(Ctx& c, const float* u, float* r)
+{
+	dgx::HpData& H = *c.hp;
+	DGX_TRY(dgx::launch_hp_faces32(c, u));
+	for (size_t g = 0; g < H.groups.size(); ++g) {
+		Ctx& s = H.groups[g]->c;
+		s.stream = c.stream;
+		DGX_TRY(dgx::launch_rhs32(s, u + H.u_off[g], r + H.u_off[g]));
+	}
+	return DGX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+dgx_status dgx_upload_mesh_hp(dgx_ctx* h, const dgx_mesh_hp* m)
+{
+	if (!h || !m || m->E <= 0 || m->F < 0 || !m->p || !m->xyz || (m->F > 0 && !m->faces)) return DGX_ERR_ARG;
+	Ctx& c = h->c;
+	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
+	if (c.cfg.pde != DGX_PDE_EULER) return DGX_ERR_UNSUPPORTED;
+	const int E = m->E, nv = c.nv;
+	// groups of equal order (the elements arrive sorted by order)
+	std::vector<int> grp(E), order, e_off;
+	for (int e = 0; e < E; ++e) {
+		const int p = m->p[e];
+		if (p < 1 || p > c.p || p + 1 > dgx::kHpMaxN) return DGX_ERR_ARG;
+		if (e > 0 && p < m->p[e - 1]) return DGX_ERR_ARG;
+		if (e == 0 || p != m->p[e - 1]) { order.push_back(p); e_off.push_back(e); }
+		grp[e] = (int)order.size() - 1;
+	}
+	e_off.push_back(E);
+	const int G = (int)order.size();
+	std::vector<long long> uoff(E + 1, 0), xoff(E + 1, 0);
+	for (int e = 0; e < E; ++e) {
+		const long long Np = (long long)(m->p[e] + 1) * (m->p[e] + 1);
+		uoff[e + 1] = uoff[e] + Np * nv;
+		xoff[e + 1] = xoff[e] + Np * 2;
+	}
+	std::vector<long long> z_off(G + 1, 0);
+	for (int g = 0; g < G; ++g)
+		z_off[g + 1] = z_off[g] + (long long)(e_off[g + 1] - e_off[g]) * 4 * nv * (order[g] + 1);
+	// split the face table: faces inside one group, faces between groups
+	std::vector<std::vector<int>> gfaces(G);
+	std::vector<dgx::HpFace> hf;
+	std::vector<double> nj;
+	double xg[dgx::kHpMaxN], wg[dgx::kHpMaxN], tmp[2 * dgx::kHpMaxN];
+	for (int f = 0; f < m->F; ++f) {
+		const int* r = m->faces + 6 * f;
+		const int eL = r[0], fL = r[1], eR = r[2], fR = r[3], ord = r[4], bc = r[5];
+		if (eL < 0 || eL >= E || fL < 0 || fL >= 4 || ord < 0 || ord > 7) return DGX_ERR_ARG;
+		const int gL = grp[eL];
+		auto push = [&](int g, int a0, int a1, int a2, int a3, int a4, int a5) {
+			const int row[6] = { a0, a1, a2, a3, a4, a5 };
+			gfaces[g].insert(gfaces[g].end(), row, row + 6);
+		};
+		if (bc != DGX_BC_INTERIOR) {
+			push(gL, eL - e_off[gL], fL, -1, -1, ord, bc);
+			continue;
+		}
+		if (eR < 0 || eR >= E || fR < 0 || fR >= 4) return DGX_ERR_ARG;
+		const int gR = grp[eR];
+		if (gL == gR) {
+			push(gL, eL - e_off[gL], fL, eR - e_off[gL], fR, ord, bc);
+			continue;
+		}
+		push(gL, eL - e_off[gL], fL, -1, -1, 0, dgx::kBcNonconforming);
+		push(gR, eR - e_off[gR], fR, -1, -1, 0, dgx::kBcNonconforming);
+		const int NL = order[gL] + 1, NR = order[gR] + 1, Nf = std::max(NL, NR);
+		dgx::HpFace x{};
+		x.uL = uoff[eL];
+		x.uR = uoff[eR];
+		x.zL = z_off[gL] + ((long long)(eL - e_off[gL]) * 4 + fL) * nv * NL;
+		x.zR = z_off[gR] + ((long long)(eR - e_off[gR]) * 4 + fR) * nv * NR;
+		x.info = NL | (NR << 3) | (fL << 6) | (fR << 8) | (ord << 10);
+		hf.push_back(x);
+		// face geometry from the lower-order side, in the left ordering, left outward
+		dgx::gauss_legendre(Nf, xg, wg);
+		const size_t base = nj.size();
+		nj.resize(base + 2 * dgx::kHpMaxN, 0.0);
+		if (NL < NR) {
+			outward_nj(m->xyz + xoff[eL], NL, fL, xg, Nf, nj.data() + base);
+		} else {
+			outward_nj(m->xyz + xoff[eR], NR, fR, xg, Nf, tmp);
+			for (int q = 0; q < Nf; ++q) {
+				const int qr = (ord & 2) ? Nf - 1 - q : q;
+				nj[base + 2 * q] = -tmp[2 * qr];
+				nj[base + 2 * q + 1] = -tmp[2 * qr + 1];
+			}
+		}
+	}
+	free_mesh(c);
+	c.hp = new (std::nothrow) dgx::HpData();
+	if (!c.hp) return DGX_ERR_ARG;
+	dgx::HpData& H = *c.hp;
+	H.order = order;
+	H.e_off = e_off;
+	H.E = E;
+	H.F = m->F;
+	H.Fnc = (int)hf.size();
+	H.ndof = uoff[E];
+	H.nz = z_off[G];
+	for (int g = 0; g < G; ++g) {
+		H.u_off.push_back(uoff[e_off[g]]);
+		H.z_off.push_back(z_off[g]);
+	}
+	DGX_TRY(hipMalloc(&H.z64, sizeof(double) * (size_t)std::max(H.nz, 1LL)));
+	DGX_TRY(hipMalloc(&H.z32, sizeof(float) * (size_t)std::max(H.nz, 1LL)));
+	DGX_TRY(hipMalloc(&H.faces, sizeof(dgx::HpFace) * std::max<size_t>(hf.size(), 1)));
+	DGX_TRY(hipMalloc(&H.nJ, sizeof(double) * std::max<size_t>(nj.size(), 1)));
+	DGX_TRY(hipMalloc(&H.nJ32, sizeof(float) * std::max<size_t>(nj.size(), 1)));
+	DGX_TRY(hipMalloc(&H.ops64, sizeof(dgx::HpOps<double>)));
+	DGX_TRY(hipMalloc(&H.ops32, sizeof(dgx::HpOps<float>)));
+	if (!hf.empty()) {
+		std::vector<float> nj32(nj.begin(), nj.end());
+		DGX_TRY(hipMemcpy(H.faces, hf.data(), sizeof(dgx::HpFace) * hf.size(), hipMemcpyHostToDevice));
+		DGX_TRY(hipMemcpy(H.nJ, nj.data(), sizeof(double) * nj.size(), hipMemcpyHostToDevice));
+		DGX_TRY(hipMemcpy(H.nJ32, nj32.data(), sizeof(float) * nj32.size(), hipMemcpyHostToDevice));
+	}
+	{
+		dgx::HpOps<double>* o64 = new dgx::HpOps<double>;
+		dgx::HpOps<float>* o32 = new dgx::HpOps<float>;
+		build_hp_ops(*o64);
+		const double* s64 = &o64->Ig[0][0][0];
+		float* d32 = &o32->Ig[0][0][0];
+		for (size_t i = 0; i < sizeof(dgx::HpOps<double>) / sizeof(double); ++i) d32[i] = (float)s64[i];
+		const hipError_t e1 = hipMemcpy(H.ops64, o64, sizeof(*o64), hipMemcpyHostToDevice);
+		const hipError_t e2 = hipMemcpy(H.ops32, o32, sizeof(*o32), hipMemcpyHostToDevice);
+		delete o64;
+		delete o32;
+		if (e1 != hipSuccess || e2 != hipSuccess) return DGX_ERR_HIP;
+	}
+	// one uniform-order context per group
+	for (int g = 0; g < G; ++g) {
+		dgx_config cfg = c.cfg;
+		cfg.p = order[g];
+		dgx_ctx* sub = nullptr;
+		dgx_status st = dgx_create(&sub, &cfg);
+		if (st != DGX_OK) return st;
+		H.groups.push_back(sub);
+		sub->c.stream = c.stream;
+		dgx_mesh sm{};
+		sm.E = e_off[g + 1] - e_off[g];
+		sm.F = (int)(gfaces[g].size() / 6);
+		sm.xyz = m->xyz + xoff[e_off[g]];
+		sm.faces = gfaces[g].empty() ? nullptr : gfaces[g].data();
+		st = upload_uniform(sub, &sm, true);
+		if (st != DGX_OK) return st;
+		sub->c.hp_z64 = H.z64 + H.z_off[g];
+		sub->c.hp_z32 = H.z32 + H.z_off[g];
+	}
+	return DGX_OK;
+}
+
 dgx_status dgx_get_geometry(dgx_ctx* h, double* met, double* dj, double* nJ)
 {
 	if (!h) return DGX_ERR_ARG;
 	Ctx& c = h->c;
-	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
 	DGX_TRY(hipStreamSynchronize(c.stream));
 	if (met) DGX_TRY(hipMemcpy(met, c.metrics_vc, sizeof(double) * (size_t)c.E * c.d * c.d * c.Nc, hipMemcpyDeviceToHost));
 	if (dj) DGX_TRY(hipMemcpy(dj, c.detJ_vc, sizeof(double) * (size_t)c.E * c.Nc, hipMemcpyDeviceToHost));
@@ -239,14 +502,16 @@ dgx_status dgx_get_geometry(dgx_ctx* h, double* met, double* dj, double* nJ)
 dgx_status dgx_compute_rhs(dgx_ctx* h, const double* u, double* r)
 {
 	if (!h || !u || !r) return DGX_ERR_ARG;
-	if (!h->c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (h->c.hp) return hp_rhs64(h->c, u, r, dgx::MODE_RHS, nullptr);
+	if (!h->c.mesh_ready) return h->c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
 	return hip_status(dgx::launch_rhs64(h->c, u, r, dgx::MODE_RHS, nullptr));
 }
 
 dgx_status dgx_compute_rhs_f32(dgx_ctx* h, const float* u, float* r)
 {
 	if (!h || !u || !r) return DGX_ERR_ARG;
-	if (!h->c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (h->c.hp) return hp_rhs32(h->c, u, r);
+	if (!h->c.mesh_ready) return h->c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
 	if (h->c.cfg.pde == DGX_PDE_NAVIER_STOKES) return hip_status(dgx::launch_rhs_ns32(h->c, u, r));
 	return hip_status(dgx::launch_rhs32(h->c, u, r));
 }
@@ -254,7 +519,7 @@ dgx_status dgx_compute_rhs_f32(dgx_ctx* h, const float* u, float* r)
 dgx_status dgx_compute_rhs_mixed(dgx_ctx* h, const double* u, float* r)
 {
 	if (!h || !u || !r) return DGX_ERR_ARG;
-	if (!h->c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (!h->c.mesh_ready) return h->c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
 	if (h->c.cfg.pde == DGX_PDE_NAVIER_STOKES) return hip_status(dgx::launch_rhs_ns_mixed(h->c, u, r));
 	if (h->c.cfg.pde != DGX_PDE_EULER) return DGX_ERR_UNSUPPORTED;
 	return hip_status(dgx::launch_rhs_mixed(h->c, u, r));
@@ -275,6 +540,12 @@ dgx_status dgx_compute_rhs_explicit(dgx_ctx* h, const double* u, double* r, doub
 {
 	if (!h || !u || !r) return DGX_ERR_ARG;
 	Ctx& c = h->c;
+	if (c.hp) {
+		DGX_TRY(dgx::launch_fill_u64(c.stream, c.dmax, 1, 0ull));
+		const dgx_status st = hp_rhs64(c, u, r, dgx::MODE_RHS_MINV, c.dmax);
+		if (st != DGX_OK) return st;
+		return max_abs_r ? read_max(c, max_abs_r) : DGX_OK;
+	}
 	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
 	DGX_TRY(dgx::launch_fill_u64(c.stream, c.dmax, 1, 0ull));
 	DGX_TRY(dgx::launch_rhs64(c, u, r, dgx::MODE_RHS_MINV, c.dmax));
@@ -285,7 +556,7 @@ dgx_status dgx_compute_rhs_explicit(dgx_ctx* h, const double* u, double* r, doub
 dgx_status dgx_apply_minv(dgx_ctx* h, double* r)
 {
 	if (!h || !r) return DGX_ERR_ARG;
-	if (!h->c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (!h->c.mesh_ready) return h->c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
 	return hip_status(dgx::launch_minv(h->c, r));
 }
 
@@ -293,7 +564,7 @@ dgx_status dgx_enforce_positivity(dgx_ctx* h, double* u)
 {
 	if (!h || !u) return DGX_ERR_ARG;
 	Ctx& c = h->c;
-	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
 	if (c.cfg.pde == DGX_PDE_ADVECTION) return DGX_OK;  // test_case.c:63-77: Euler/NS only
 	DGX_TRY(dgx::launch_fill_i32(c.stream, c.dstatus, 1, 0));
 	DGX_TRY(dgx::launch_positivity(c, u, c.dstatus));
@@ -307,7 +578,7 @@ dgx_status dgx_ssp_rk3_step(dgx_ctx* h, double* u, double dt, int positivity, do
 {
 	if (!h || !u) return DGX_ERR_ARG;
 	Ctx& c = h->c;
-	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (!c.mesh_ready && !c.hp) return DGX_ERR_NO_MESH;
 	const size_t n = ndof(c);
 	if (!c.u0) DGX_TRY(hipMalloc(&c.u0, sizeof(double) * n));
 	if (!c.rtmp) DGX_TRY(hipMalloc(&c.rtmp, sizeof(double) * n));
@@ -315,6 +586,19 @@ dgx_status dgx_ssp_rk3_step(dgx_ctx* h, double* u, double dt, int positivity, do
 	if (pos) DGX_TRY(dgx::launch_fill_i32(c.stream, c.dstatus, 1, 0));
 	for (int rk = 0; rk < 3; ++rk) {
 		DGX_TRY(dgx::launch_fill_u64(c.stream, c.dmax, 1, 0ull));
+		if (c.hp) {
+			// every group's residual from the stage's state, then every group's stage update
+			const dgx_status st = hp_rhs64(c, u, c.rtmp, dgx::MODE_RHS_MINV, c.dmax);
+			if (st != DGX_OK) return st;
+			dgx::HpData& H = *c.hp;
+			for (size_t g = 0; g < H.groups.size(); ++g) {
+				Ctx& s = H.groups[g]->c;
+				if (!s.u0) DGX_TRY(hipMalloc(&s.u0, sizeof(double) * ndof(s)));
+				DGX_TRY(dgx::launch_rk_stage(s, u + H.u_off[g], c.rtmp + H.u_off[g], dt, rk));
+				if (pos) DGX_TRY(dgx::launch_positivity(s, u + H.u_off[g], c.dstatus));
+			}
+			continue;
+		}
 		DGX_TRY(dgx::launch_rhs64(c, u, c.rtmp, dgx::MODE_RHS_MINV, c.dmax));
 		DGX_TRY(dgx::launch_rk_stage(c, u, c.rtmp, dt, rk));
 		if (pos) DGX_TRY(dgx::launch_positivity(c, u, c.dstatus));
@@ -378,7 +662,7 @@ dgx_status dgx_graph_create_rhs(dgx_ctx* h, int kind, const void* u, void* r, dg
 {
 	if (!h || !u || !r || !out || kind < DGX_GRAPH_RHS_FP64 || kind > DGX_GRAPH_RHS_MIXED) return DGX_ERR_ARG;
 	Ctx& c = h->c;
-	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
 	if (kind == DGX_GRAPH_RHS_MIXED && c.cfg.pde == DGX_PDE_ADVECTION) return DGX_ERR_UNSUPPORTED;
 	dgx_graph* g = new (std::nothrow) dgx_graph();
 	if (!g) return DGX_ERR_HIP;
@@ -405,7 +689,7 @@ dgx_status dgx_graph_create_rk3(dgx_ctx* h, double* u, double dt, int positivity
 {
 	if (!h || !u || !out) return DGX_ERR_ARG;
 	Ctx& c = h->c;
-	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
 	const size_t n = ndof(c);
 	if (!c.u0) DGX_TRY(hipMalloc(&c.u0, sizeof(double) * n));     // stage buffers exist before the capture
 	if (!c.rtmp) DGX_TRY(hipMalloc(&c.rtmp, sizeof(double) * n));
@@ -489,7 +773,7 @@ dgx_status dgx_autotune_rhs(dgx_ctx* h, int kind, const void* u, void* r, int* g
 {
 	if (!h || !u || !r || kind < DGX_GRAPH_RHS_FP64 || kind > DGX_TUNE_RHS_MINV) return DGX_ERR_ARG;
 	Ctx& c = h->c;
-	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
 	if (kind == DGX_GRAPH_RHS_MIXED && c.cfg.pde != DGX_PDE_EULER) return DGX_ERR_UNSUPPORTED;
 	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;   // its kernels have a fixed launch shape
 	int dev = 0, ncu = 0;
@@ -565,7 +849,7 @@ dgx_status dgx_assemble_diag_blocks(dgx_ctx* h, const double* u, const double* m
 {
 	if (!h || !u || !blocks) return DGX_ERR_ARG;
 	Ctx& c = h->c;
-	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
 	if (c.d != 2 || c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;   // Euler linearisation only
 	const int nb = c.Np * c.nv;
 	for (int col = 0; col < nb; ++col) DGX_TRY(dgx::launch_diag_col(c, u, blocks, col));
@@ -577,7 +861,7 @@ dgx_status dgx_assemble_face_blocks(dgx_ctx* h, const double* u, double* blocks)
 {
 	if (!h || !u || !blocks) return DGX_ERR_ARG;
 	Ctx& c = h->c;
-	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
 	if (c.d != 2 || c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;   // Euler linearisation only
 	const int nfv = c.nv * c.N;   // face-node rows / columns of a 2-D face block
 	double* cT = nullptr;
@@ -594,7 +878,7 @@ dgx_status dgx_cfl_dt(dgx_ctx* h, const double* u, double cfl, double* inv_dt)
 {
 	if (!h || !u || !inv_dt || !(cfl > 0.0)) return DGX_ERR_ARG;
 	Ctx& c = h->c;
-	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
 	if (c.d != 2 || c.nv != 4) return DGX_ERR_UNSUPPORTED;
 	return hip_status(dgx::launch_cfl_dt(c, u, cfl, inv_dt));
 }
@@ -603,7 +887,7 @@ dgx_status dgx_mass_sub(dgx_ctx* h, const double* x, const double* scale, double
 {
 	if (!h || !x || !y) return DGX_ERR_ARG;
 	Ctx& c = h->c;
-	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
 	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
 	return hip_status(dgx::launch_mass_sub(c, x, scale, y));
 }
@@ -612,7 +896,7 @@ dgx_status dgx_newton_operator(dgx_ctx* h, const double* u, const double* mass_s
 {
 	if (!h || !u || !mass_scale || !v || !out) return DGX_ERR_ARG;
 	Ctx& c = h->c;
-	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
 	if (c.d != 2 || c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;   // Euler linearisation only
 	return hip_status(dgx::launch_jvp(c, u, v, out, mass_scale));
 }
@@ -621,7 +905,7 @@ dgx_status dgx_jacobian_vector(dgx_ctx* h, const double* u, const double* v, dou
 {
 	if (!h || !u || !v || !jv) return DGX_ERR_ARG;
 	Ctx& c = h->c;
-	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
 	if (c.d != 2 || c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;   // Euler linearisation only
 	return hip_status(dgx::launch_jvp(c, u, v, jv));
 }

@@ -737,7 +737,7 @@ extern "C" dgx_status dgx_precond_apply(dgx_ctx* h, int precond, const double* u
 {
 	if (!h || !u || !inv_dt || !v || !z || v == z) return DGX_ERR_ARG;
 	Ctx& c = h->c;
-	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
 	if (c.d != 2 || c.cfg.pde != DGX_PDE_EULER) return DGX_ERR_UNSUPPORTED;
 	dgx::Precond* P = nullptr;
 	if (precond == DGX_PC_NONE) P = new dgx::PrecondIdentity();
@@ -757,7 +757,7 @@ extern "C" dgx_status dgx_solve_implicit(dgx_ctx* h, double* u, const dgx_implic
 {
 	if (!h || !u || !prm || !st) return DGX_ERR_ARG;
 	Ctx& c = h->c;
-	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
 	if (c.d != 2 || c.cfg.pde != DGX_PDE_EULER) return DGX_ERR_UNSUPPORTED;
 	if (prm->gmres_restart < 1 || prm->gmres_restart > 200 || prm->max_newton < 1 || !(prm->cfl_initial > 0.0))
 		return DGX_ERR_ARG;

@@ -35,6 +35,51 @@ struct Ops1D {
 	int perm[8][kMaxNF];     // face orientation permutations
 };
 
+// Variable-order 2-D meshes: orders 1..kHpMaxP, faces between elements of different orders.
+constexpr int kHpMaxN = 6;
+// bc code of an element's face to an element of another order (internal: the face kernel lifts its flux)
+constexpr int kBcNonconforming = 15;
+
+// 1-D operators between orders, N = p + 1 (index 0 and 1 unused):
+// Ig[Ns][Nf][q * Ns + i]: GLL_Ns Lagrange basis i at GL_Nf point q (traces to the face cubature);
+// Pg[Ne][Nf][q * Ne + a]: GL_Ne Lagrange basis a at GL_Nf point q (exact for the element's test functions);
+// w / winv: GL weights of each order and their inverses.
+template <typename T>
+struct HpOps {
+	T Ig[kHpMaxN + 1][kHpMaxN + 1][kHpMaxN * kHpMaxN];
+	T Pg[kHpMaxN + 1][kHpMaxN + 1][kHpMaxN * kHpMaxN];
+	T w[kHpMaxN + 1][kHpMaxN];
+	T winv[kHpMaxN + 1][kHpMaxN];
+};
+
+// One face between elements of different orders: first state value of each side's element, first value of
+// each side's lifted-flux block (own face, [var][own GL face point]), and NL | NR << 3 | lfL << 6 | lfR << 8 |
+// ord << 10.
+struct HpFace {
+	long long uL, uR, zL, zR;
+	int info, pad;
+};
+
+struct Ctx;
+// A variable-order mesh: one uniform-order context per order present; elements sorted by order, so each group
+// is a contiguous range of the state vector.
+struct HpData {
+	std::vector<dgx_ctx*> groups;
+	std::vector<int> order;          // p of each group
+	std::vector<int> e_off;          // first element of each group (groups + 1 entries)
+	std::vector<long long> u_off;    // first state value of each group
+	std::vector<long long> z_off;    // first lifted-flux value of each group
+	int E = 0, F = 0, Fnc = 0;
+	long long ndof = 0, nz = 0;
+	HpFace* faces = nullptr;         // Fnc
+	double* nJ = nullptr;            // Fnc x kHpMaxN x 2: n J_f at the face GL points (left ordering, left outward)
+	float* nJ32 = nullptr;
+	double* z64 = nullptr;           // lifted fluxes, FP64 and FP32 (each group: E_g x 4 x nv x N_g)
+	float* z32 = nullptr;
+	HpOps<double>* ops64 = nullptr;
+	HpOps<float>* ops32 = nullptr;
+};
+
 // Per-element face record: f, neighbour element (-1 on a boundary), and packed
 // (neighbour local face | side << 4 | ord << 8 | bc << 12).
 struct ElemFace {
@@ -62,6 +107,9 @@ struct Ctx {
 	double* ns_gvt = nullptr;       // Navier-Stokes: face traces of the volume gradient (E x 2d x d x nv x Nfc)
 	double* ns_kappa = nullptr;     // Navier-Stokes: BR2 lift trace factor per face node (E x 2d x Nfc)
 	int rhs_grid[4] = { 0, 0, 0, 0 };   // residual launch grids by TUNE_* (0 = one workgroup per element block)
+	HpData* hp = nullptr;           // variable-order mesh (then this context holds no uniform mesh)
+	const double* hp_z64 = nullptr; // order group of a variable-order mesh: its part of the lifted fluxes
+	const float* hp_z32 = nullptr;
 	double* metrics_vc = nullptr;   // E*d*d*Nc  (m[(i + d*j)*Nc + n] = J dxi_j/dx_i)
 	float* metrics_vc32 = nullptr;
 	double* detJ_vc = nullptr;      // E*Nc
@@ -107,6 +155,9 @@ hipError_t launch_rhs_ns(Ctx& c, const double* u, double* r);   // 2-D Navier-St
 hipError_t launch_rhs_ns32(Ctx& c, const float* u, float* r);    // the same in FP32
 hipError_t launch_rhs_ns_mixed(Ctx& c, const double* u, float* r);   // FP64 state, FP32 perturbation form
 hipError_t launch_rhs_mixed(Ctx& c, const double* u, float* r);
+// variable-order meshes: the lifted fluxes of the faces between orders (FP64 / FP32 state)
+hipError_t launch_hp_faces64(Ctx& c, const double* u);
+hipError_t launch_hp_faces32(Ctx& c, const float* u);
 hipError_t launch_minv(Ctx& c, double* r);
 hipError_t launch_positivity(Ctx& c, double* u, int* status);
 hipError_t launch_maxabs(Ctx& c, const double* r, size_t n, unsigned long long* dmax);

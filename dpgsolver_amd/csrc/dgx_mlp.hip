@@ -639,7 +639,7 @@ dgx_status dgx_mlp_features(dgx_ctx* h, const dgx_mlp* m, const double* u, const
                             float* sigma)
 {
 	if (!h || !m || !u || !inv_dt || !feat || !sigma) return DGX_ERR_ARG;
-	if (!h->c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (!h->c.mesh_ready) return h->c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
 	if (m->nb != h->c.Np * h->c.nv || m->Np != h->c.Np) return DGX_ERR_ARG;
 	return dgx::launch_features(h->c, u, inv_dt, feat, m->nf, sigma) == hipSuccess ? DGX_OK : DGX_ERR_HIP;
 }

@@ -66,6 +66,7 @@ struct RhsArgs {
 	int grid;             // workgroups of the launch: 0 = one per element block, else a persistent grid
 	unsigned long long* dmax;
 	const double* mass_scale;   // MODE_JVP: out = J v - mass_scale[e] M_e v when not NULL
+	const Real* zf;       // 2-D variable order: lifted fluxes of the faces to elements of another order
 };
 
 template <int D, int N>

@@ -21,6 +21,7 @@ hipError_t launch_rhs32(Ctx& c, const float* u, float* r)
 	for (int k = 0; k < 3; ++k) a.b_adv[k] = c.cfg.b_adv[k];
 	a.E = c.E;
 	a.dmax = nullptr;
+	a.zf = c.hp_z32;
 	a.grid = c.rhs_grid[TUNE_RHS32];
 	if (a.grid > 0) return dispatch_rhs_persistent(c, a, MODE_RHS);
 	return dispatch_rhs<float, float, MODE_RHS>(c.d, c.p, c.cfg.pde, c.cfg.num_flux, a, c.stream);

@@ -20,6 +20,7 @@ static RhsArgs<double, double> make_args64(Ctx& c, const double* u, double* r, u
 	a.E = c.E;
 	a.col = 0;
 	a.dmax = dmax;
+	a.zf = c.hp_z64;
 	return a;
 }
 
