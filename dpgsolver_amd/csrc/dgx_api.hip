@@ -323,6 +323,24 @@ void outward_nj(const double* xyz, int N, int lf, const double* s, int ns, doubl
 dgx_status hp_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned long long* dmax)
 {
 	dgx::HpData& H = *c.hp;
+	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) {
+		// every group's gradient traces, the faces between orders, every group's residual (+ M^-1, max-norm)
+		for (size_t g = 0; g < H.groups.size(); ++g) {
+			Ctx& s = H.groups[g]->c;
+			s.stream = c.stream;
+			DGX_TRY(dgx::launch_rhs_ns_phase(s, u + H.u_off[g], r + H.u_off[g], 1));
+		}
+		DGX_TRY(dgx::launch_hp_faces64(c, u));
+		for (size_t g = 0; g < H.groups.size(); ++g) {
+			Ctx& s = H.groups[g]->c;
+			double* rg = r + H.u_off[g];
+			DGX_TRY(dgx::launch_rhs_ns_phase(s, u + H.u_off[g], rg, 2));
+			if (mode != dgx::MODE_RHS_MINV) continue;
+			DGX_TRY(dgx::launch_minv(s, rg));
+			if (dmax) DGX_TRY(dgx::launch_maxabs(s, rg, ndof(s), dmax));
+		}
+		return DGX_OK;
+	}
 	DGX_TRY(dgx::launch_hp_faces64(c, u));
 	for (size_t g = 0; g < H.groups.size(); ++g) {
 		Ctx& s = H.groups[g]->c;
@@ -335,6 +353,17 @@ dgx_status hp_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned long 
 dgx_status hp_rhs32(Ctx& c, const float* u, float* r)
 {
 	dgx::HpData& H = *c.hp;
+	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) {
+		for (size_t g = 0; g < H.groups.size(); ++g) {
+			Ctx& s = H.groups[g]->c;
+			s.stream = c.stream;
+			DGX_TRY(dgx::launch_rhs_ns32_phase(s, u + H.u_off[g], r + H.u_off[g], 1));
+		}
+		DGX_TRY(dgx::launch_hp_faces32(c, u));
+		for (size_t g = 0; g < H.groups.size(); ++g)
+			DGX_TRY(dgx::launch_rhs_ns32_phase(H.groups[g]->c, u + H.u_off[g], r + H.u_off[g], 2));
+		return DGX_OK;
+	}
 	DGX_TRY(dgx::launch_hp_faces32(c, u));
 	for (size_t g = 0; g < H.groups.size(); ++g) {
 		Ctx& s = H.groups[g]->c;
@@ -353,8 +382,10 @@ dgx_status dgx_upload_mesh_hp(dgx_ctx* h, const dgx_mesh_hp* m)
 	if (!h || !m || m->E <= 0 || m->F < 0 || !m->p || !m->xyz || (m->F > 0 && !m->faces)) return DGX_ERR_ARG;
 	Ctx& c = h->c;
 	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
-	if (c.cfg.pde != DGX_PDE_EULER) return DGX_ERR_UNSUPPORTED;
+	if (c.cfg.pde != DGX_PDE_EULER && c.cfg.pde != DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;
 	const int E = m->E, nv = c.nv;
+	// per element face: the lifted flux [v][a] (Navier-Stokes: then the projected BR2 products [d][v][a])
+	const int zblk = (c.cfg.pde == DGX_PDE_NAVIER_STOKES) ? 3 * nv : nv;
 	// groups of equal order (the elements arrive sorted by order)
 	std::vector<int> grp(E), order, e_off;
 	for (int e = 0; e < E; ++e) {
@@ -374,7 +405,7 @@ dgx_status dgx_upload_mesh_hp(dgx_ctx* h, const dgx_mesh_hp* m)
 	}
 	std::vector<long long> z_off(G + 1, 0);
 	for (int g = 0; g < G; ++g)
-		z_off[g + 1] = z_off[g] + (long long)(e_off[g + 1] - e_off[g]) * 4 * nv * (order[g] + 1);
+		z_off[g + 1] = z_off[g] + (long long)(e_off[g + 1] - e_off[g]) * 4 * zblk * (order[g] + 1);
 	// split the face table: faces inside one group, faces between groups
 	std::vector<std::vector<int>> gfaces(G);
 	std::vector<dgx::HpFace> hf;
@@ -405,9 +436,11 @@ dgx_status dgx_upload_mesh_hp(dgx_ctx* h, const dgx_mesh_hp* m)
 		dgx::HpFace x{};
 		x.uL = uoff[eL];
 		x.uR = uoff[eR];
-		x.zL = z_off[gL] + ((long long)(eL - e_off[gL]) * 4 + fL) * nv * NL;
-		x.zR = z_off[gR] + ((long long)(eR - e_off[gR]) * 4 + fR) * nv * NR;
+		x.zL = z_off[gL] + ((long long)(eL - e_off[gL]) * 4 + fL) * zblk * NL;
+		x.zR = z_off[gR] + ((long long)(eR - e_off[gR]) * 4 + fR) * zblk * NR;
 		x.info = NL | (NR << 3) | (fL << 6) | (fR << 8) | (ord << 10);
+		x.eL = eL - e_off[gL];
+		x.eR = eR - e_off[gR];
 		hf.push_back(x);
 		// face geometry from the lower-order side, in the left ordering, left outward
 		dgx::gauss_legendre(Nf, xg, wg);

@@ -290,4 +290,36 @@ __device__ inline void euler_contravariant_flux(const T* s, const R* m, R wq, T 
 	}
 }
 
+// negated viscous flux fv[k + 2 * eq] of state s and gradient g[v + 4 * k] (2-D): tau with the -2/3 mu div V
+// part (compute_tau, flux_navier_stokes_T.c:778-799), scaled temperature gradient d(E/rho) - d(V^2)/2
+// (compute_dTs :904-926), energy flux gamma/Pr mu dTs + u . tau; constant viscosity
+template <typename S>
+__device__ inline void ns_viscous_flux2(const S* s, const S* g, S mu, S Pr, S* fv)
+{
+	const S rho = s[0], E = s[3], ri = trcp(rho);
+	const S u[2] = { s[1] * ri, s[2] * ri };
+	S du[2][2], tau[2][2], dTs[2];
+#pragma unroll
+	for (int k = 0; k < 2; ++k)
+#pragma unroll
+		for (int j = 0; j < 2; ++j) du[k][j] = ri * (g[(1 + k) + 4 * j] - u[k] * g[0 + 4 * j]);
+	const S divV = du[0][0] + du[1][1];
+#pragma unroll
+	for (int i = 0; i < 2; ++i)
+#pragma unroll
+		for (int j = 0; j < 2; ++j) tau[i][j] = mu * (du[i][j] + du[j][i]) - (i == j ? S(2.0 / 3.0) * mu * divV : S(0.0));
+#pragma unroll
+	for (int j = 0; j < 2; ++j) {
+		const S dV2 = S(2.0) * (u[0] * du[0][j] + u[1] * du[1][j]);
+		dTs[j] = ri * ri * (g[3 + 4 * j] * rho - E * g[0 + 4 * j]) - S(0.5) * dV2;
+	}
+#pragma unroll
+	for (int j = 0; j < 2; ++j) {
+		fv[j + 2 * 0] = S(0.0);
+		fv[j + 2 * 1] = -tau[0][j];
+		fv[j + 2 * 2] = -tau[1][j];
+		fv[j + 2 * 3] = -(S(kGamma) / Pr * mu * dTs[j] + u[0] * tau[j][0] + u[1] * tau[j][1]);
+	}
+}
+
 }  // namespace dgx

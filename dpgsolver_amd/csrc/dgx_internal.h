@@ -57,7 +57,9 @@ struct HpOps {
 // ord << 10.
 struct HpFace {
 	long long uL, uR, zL, zR;
-	int info, pad;
+	int info;
+	int eL, eR;          // each side's element index within its order group
+	int pad;
 };
 
 struct Ctx;
@@ -155,9 +157,12 @@ hipError_t launch_rhs_ns(Ctx& c, const double* u, double* r);   // 2-D Navier-St
 hipError_t launch_rhs_ns32(Ctx& c, const float* u, float* r);    // the same in FP32
 hipError_t launch_rhs_ns_mixed(Ctx& c, const double* u, float* r);   // FP64 state, FP32 perturbation form
 hipError_t launch_rhs_mixed(Ctx& c, const double* u, float* r);
-// variable-order meshes: the lifted fluxes of the faces between orders (FP64 / FP32 state)
+// variable-order meshes: the lifted fluxes of the faces between orders (FP64 / FP32 state); Navier-Stokes also
+// the projected BR2 face products, between the groups' gradient (phase 1) and residual (phase 2) kernels
 hipError_t launch_hp_faces64(Ctx& c, const double* u);
 hipError_t launch_hp_faces32(Ctx& c, const float* u);
+hipError_t launch_rhs_ns_phase(Ctx& c, const double* u, double* r, int phase);
+hipError_t launch_rhs_ns32_phase(Ctx& c, const float* u, float* r, int phase);
 hipError_t launch_minv(Ctx& c, double* r);
 hipError_t launch_positivity(Ctx& c, double* u, int* status);
 hipError_t launch_maxabs(Ctx& c, const double* r, size_t n, unsigned long long* dmax);

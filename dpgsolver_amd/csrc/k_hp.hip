@@ -116,11 +116,211 @@ __global__ void __launch_bounds__(64) k_hp_faces(HpFaceArgs<S> a)
 	}
 }
 
+// ------------------------------------------------------------------------------------------- Navier-Stokes
+// BR2 at a face between orders (compute_grad_coef_dg_T.c:171-307, compute_face_rlhs_dg_T.c:243-300): both sides
+// use q = n J_f (s_R - s_L) / 2 at the Nf face points; side e's face lift of the gradient is, at its own GL
+// nodes, Lift(c) y_e(a) / (w_c J(c, a)) with the projection y_e(a) = sum_q P_e[q][a] w_q q(q) / w_a, and its
+// trace at the face is kappa_e(a) y_e(a).  The face gradient at the Nf points is therefore
+// P_e (trace G_v + eta kappa_e y_e), and the face flux is the inviscid flux plus the central
+// n . (fv_L + fv_R) / 2 (numerical_flux_central_T.c:76-104).  Per element face the kernel writes the lifted
+// flux z [v][a] and then y [d][v][a], which the residual kernel of each order reads in place of its own face
+// traces.  Eight lanes per face (Nf <= 6 points), the stages staged in LDS.
+template <typename S>
+struct HpFaceNsArgs {
+	const S* u;
+	S* z;
+	const HpFace* faces;
+	const S* nJ;
+	const HpOps<S>* ops;
+	const S* gvt[kHpMaxN + 1];   // each order's face traces of G_v (E_g x 4 x 2 x 4 x N) and kappa (E_g x 4 x N)
+	const S* kap[kHpMaxN + 1];
+	S mu, Pr;
+	int F;
+};
+
+template <typename S, int FLUX>
+__global__ void __launch_bounds__(64) k_hp_faces_ns(HpFaceNsArgs<S> a)
+{
+	constexpr int NV = 4, NM = kHpMaxN, FPB = 8;
+	constexpr S eta = S(1.01 * 4.0);
+	__shared__ S uLs[FPB][NV][NM], uRs[FPB][NV][NM];           // GLL face nodes, own orderings
+	__shared__ S gvL[FPB][2 * NV][NM], gvR[FPB][2 * NV][NM];   // traces of G_v [d][v][a], own orderings
+	__shared__ S kL[FPB][NM], kR[FPB][NM];
+	__shared__ S sLq[FPB][NM][NV], sRq[FPB][NM][NV];           // face traces at the Nf points, left ordering
+	__shared__ S qf[FPB][NM][2 * NV];                          // w_q q [d][v] at the Nf points
+	__shared__ S yL[FPB][2 * NV][NM], yR[FPB][2 * NV][NM];     // projected products, own orderings
+	__shared__ S gq[FPB][NM][NV];                              // w_q J_f (F-hat + viscous)
+	const int slot = threadIdx.x >> 3, j = threadIdx.x & 7;
+	const int f = blockIdx.x * FPB + slot;
+	const bool valid = f < a.F;
+	HpFace hf{};
+	int NL = 0, NR = 0, lfL = 0, lfR = 0, Nf = 0;
+	bool flip = false;
+	if (valid) {
+		hf = a.faces[f];
+		NL = hf.info & 7;
+		NR = (hf.info >> 3) & 7;
+		lfL = (hf.info >> 6) & 3;
+		lfR = (hf.info >> 8) & 3;
+		flip = (hf.info >> 11) & 1;
+		Nf = NL > NR ? NL : NR;
+	}
+	const HpOps<S>* o = a.ops;
+	if (valid && j < NL) {
+		const S* p = a.u + hf.uL + face_node_2d(NL, lfL, j);
+		const S* t = a.gvt[NL] + ((size_t)hf.eL * 4 + lfL) * 2 * NV * NL;
+#pragma unroll
+		for (int v = 0; v < NV; ++v) uLs[slot][v][j] = p[v * NL * NL];
+#pragma unroll
+		for (int k = 0; k < 2 * NV; ++k) gvL[slot][k][j] = t[k * NL + j];
+		kL[slot][j] = a.kap[NL][((size_t)hf.eL * 4 + lfL) * NL + j];
+	}
+	if (valid && j < NR) {
+		const S* p = a.u + hf.uR + face_node_2d(NR, lfR, j);
+		const S* t = a.gvt[NR] + ((size_t)hf.eR * 4 + lfR) * 2 * NV * NR;
+#pragma unroll
+		for (int v = 0; v < NV; ++v) uRs[slot][v][j] = p[v * NR * NR];
+#pragma unroll
+		for (int k = 0; k < 2 * NV; ++k) gvR[slot][k][j] = t[k * NR + j];
+		kR[slot][j] = a.kap[NR][((size_t)hf.eR * 4 + lfR) * NR + j];
+	}
+	__syncthreads();
+	// traces at the face points (left ordering) and w q
+	if (valid && j < Nf) {
+		const int q = j, qr = flip ? Nf - 1 - q : q;
+		const S* IL = o->Ig[NL][Nf] + q * NL;
+		const S* IR = o->Ig[NR][Nf] + qr * NR;
+		const S* nj = a.nJ + ((size_t)f * NM + q) * 2;
+		const S wq = o->w[Nf][q];
+#pragma unroll
+		for (int v = 0; v < NV; ++v) {
+			S aL = S(0), aR = S(0);
+#pragma unroll
+			for (int i = 0; i < NM; ++i) {
+				if (i < NL) aL += IL[i] * uLs[slot][v][i];
+				if (i < NR) aR += IR[i] * uRs[slot][v][i];
+			}
+			sLq[slot][q][v] = aL;
+			sRq[slot][q][v] = aR;
+			const S dl = S(0.5) * (aR - aL);
+			qf[slot][q][v] = wq * (nj[0] * dl);
+			qf[slot][q][NV + v] = wq * (nj[1] * dl);
+		}
+	}
+	__syncthreads();
+	// projections to each side's own GL face points
+	if (valid && j < NL) {
+#pragma unroll
+		for (int k = 0; k < 2 * NV; ++k) {
+			S acc = S(0);
+#pragma unroll
+			for (int q = 0; q < NM; ++q)
+				if (q < Nf) acc += o->Pg[NL][Nf][q * NL + j] * qf[slot][q][k];
+			acc = acc * o->winv[NL][j];
+			yL[slot][k][j] = acc;
+			a.z[hf.zL + NV * NL + k * NL + j] = acc;
+		}
+	}
+	if (valid && j < NR) {
+#pragma unroll
+		for (int k = 0; k < 2 * NV; ++k) {
+			S acc = S(0);
+#pragma unroll
+			for (int q = 0; q < NM; ++q)
+				if (q < Nf) acc += o->Pg[NR][Nf][(flip ? Nf - 1 - q : q) * NR + j] * qf[slot][q][k];
+			acc = acc * o->winv[NR][j];
+			yR[slot][k][j] = acc;
+			a.z[hf.zR + NV * NR + k * NR + j] = acc;
+		}
+	}
+	__syncthreads();
+	// face gradients and the face flux at the Nf points
+	if (valid && j < Nf) {
+		const int q = j, qr = flip ? Nf - 1 - q : q;
+		const S* PL = o->Pg[NL][Nf] + q * NL;
+		const S* PR = o->Pg[NR][Nf] + qr * NR;
+		S gL[2 * NV], gR[2 * NV], sL[NV], sR[NV];
+#pragma unroll
+		for (int k = 0; k < 2 * NV; ++k) {
+			S aL = S(0), aR = S(0);
+#pragma unroll
+			for (int i = 0; i < NM; ++i) {
+				if (i < NL) aL += PL[i] * (gvL[slot][k][i] + eta * kL[slot][i] * yL[slot][k][i]);
+				if (i < NR) aR += PR[i] * (gvR[slot][k][i] + eta * kR[slot][i] * yR[slot][k][i]);
+			}
+			gL[k] = aL;
+			gR[k] = aR;
+		}
+#pragma unroll
+		for (int v = 0; v < NV; ++v) {
+			sL[v] = sLq[slot][q][v];
+			sR[v] = sRq[slot][q][v];
+		}
+		const S* nj = a.nJ + ((size_t)f * NM + q) * 2;
+		const S Jf = tsqrt(nj[0] * nj[0] + nj[1] * nj[1]);
+		const S Jfi = trcp(Jf);
+		const S nu[2] = { nj[0] * Jfi, nj[1] * Jfi };
+		S nnf[NV], fvL[2 * NV], fvR[2 * NV];
+		if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<S, 2>(sL, sR, nu, nnf);
+		else numflux_roe<S, 2>(sL, sR, nu, nnf);
+		ns_viscous_flux2<S>(sL, gL, a.mu, a.Pr, fvL);
+		ns_viscous_flux2<S>(sR, gR, a.mu, a.Pr, fvR);
+		const S wj = o->w[Nf][q] * Jf;
+#pragma unroll
+		for (int v = 0; v < NV; ++v) {
+			const S vis = nu[0] * S(0.5) * (fvL[0 + 2 * v] + fvR[0 + 2 * v]) + nu[1] * S(0.5) * (fvL[1 + 2 * v] + fvR[1 + 2 * v]);
+			gq[slot][q][v] = wj * (nnf[v] + vis);
+		}
+	}
+	__syncthreads();
+	// lifts to each side's own GL face points (the right side negated)
+	if (valid && j < NL) {
+#pragma unroll
+		for (int v = 0; v < NV; ++v) {
+			S acc = S(0);
+#pragma unroll
+			for (int q = 0; q < NM; ++q)
+				if (q < Nf) acc += o->Pg[NL][Nf][q * NL + j] * gq[slot][q][v];
+			a.z[hf.zL + v * NL + j] = acc * o->winv[NL][j];
+		}
+	}
+	if (valid && j < NR) {
+#pragma unroll
+		for (int v = 0; v < NV; ++v) {
+			S acc = S(0);
+#pragma unroll
+			for (int q = 0; q < NM; ++q)
+				if (q < Nf) acc += o->Pg[NR][Nf][(flip ? Nf - 1 - q : q) * NR + j] * gq[slot][q][v];
+			a.z[hf.zR + v * NR + j] = S(0) - acc * o->winv[NR][j];
+		}
+	}
+}
+
 template <typename S>
 hipError_t run_hp_faces(Ctx& c, const S* u, S* z, const S* nJ, const HpOps<S>* ops)
 {
 	const HpData& H = *c.hp;
 	if (H.Fnc == 0) return hipSuccess;
+	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) {
+		HpFaceNsArgs<S> a{};
+		a.u = u;
+		a.z = z;
+		a.faces = H.faces;
+		a.nJ = nJ;
+		a.ops = ops;
+		for (size_t g = 0; g < H.groups.size(); ++g) {
+			const Ctx& s = H.groups[g]->c;
+			a.gvt[s.N] = reinterpret_cast<const S*>(s.ns_gvt);   // FP32 uses the first half of the FP64 buffers
+			a.kap[s.N] = reinterpret_cast<const S*>(s.ns_kappa);
+		}
+		a.mu = S(c.cfg.mu);
+		a.Pr = S(c.cfg.Pr);
+		a.F = H.Fnc;
+		const int blocks = (H.Fnc + 7) / 8;
+		if (c.cfg.num_flux == DGX_FLUX_LF) hipLaunchKernelGGL((k_hp_faces_ns<S, DGX_FLUX_LF>), dim3(blocks), dim3(64), 0, c.stream, a);
+		else hipLaunchKernelGGL((k_hp_faces_ns<S, DGX_FLUX_ROE>), dim3(blocks), dim3(64), 0, c.stream, a);
+		return hipGetLastError();
+	}
 	HpFaceArgs<S> a{ u, z, H.faces, nJ, ops, H.Fnc };
 	const int blocks = (H.Fnc + 63) / 64;
 	if (c.cfg.num_flux == DGX_FLUX_LF) hipLaunchKernelGGL((k_hp_faces<S, DGX_FLUX_LF>), dim3(blocks), dim3(64), 0, c.stream, a);

@@ -42,38 +42,6 @@ __device__ inline int ns_face_node(int lf, int a)
 	return (lf >> 1) == 0 ? fix + N * a : a + N * fix;
 }
 
-// negated viscous flux fv[k + 2 * eq] of state s and gradient g[v + 4 * k] (2-D): tau with the -2/3 mu div V
-// part (compute_tau, flux_navier_stokes_T.c:778-799), scaled temperature gradient d(E/rho) - d(V^2)/2
-// (compute_dTs :904-926), energy flux gamma/Pr mu dTs + u . tau; constant viscosity
-template <typename S>
-__device__ inline void ns_viscous_flux2(const S* s, const S* g, S mu, S Pr, S* fv)
-{
-	const S rho = s[0], E = s[3], ri = trcp(rho);
-	const S u[2] = { s[1] * ri, s[2] * ri };
-	S du[2][2], tau[2][2], dTs[2];
-#pragma unroll
-	for (int k = 0; k < 2; ++k)
-#pragma unroll
-		for (int j = 0; j < 2; ++j) du[k][j] = ri * (g[(1 + k) + 4 * j] - u[k] * g[0 + 4 * j]);
-	const S divV = du[0][0] + du[1][1];
-#pragma unroll
-	for (int i = 0; i < 2; ++i)
-#pragma unroll
-		for (int j = 0; j < 2; ++j) tau[i][j] = mu * (du[i][j] + du[j][i]) - (i == j ? S(2.0 / 3.0) * mu * divV : S(0.0));
-#pragma unroll
-	for (int j = 0; j < 2; ++j) {
-		const S dV2 = S(2.0) * (u[0] * du[0][j] + u[1] * du[1][j]);
-		dTs[j] = ri * ri * (g[3 + 4 * j] * rho - E * g[0 + 4 * j]) - S(0.5) * dV2;
-	}
-#pragma unroll
-	for (int j = 0; j < 2; ++j) {
-		fv[j + 2 * 0] = S(0.0);
-		fv[j + 2 * 1] = -tau[0][j];
-		fv[j + 2 * 2] = -tau[1][j];
-		fv[j + 2 * 3] = -(S(kGamma) / Pr * mu * dTs[j] + u[0] * tau[j][0] + u[1] * tau[j][1]);
-	}
-}
-
 template <typename S>
 struct NsArgs {
 	const S* u;
@@ -88,6 +56,9 @@ struct NsArgs {
 	const Ops1D<S>* ops;
 	S mu, Pr;
 	int E;
+	// variable-order meshes: per element face to an element of another order, the lifted face flux [v][a]
+	// and then the projected face products q [d][v][a] (own GL face points and ordering)
+	const S* zf;
 };
 
 // shared per-element set-up: operators, the GLL state, its GL values and (k_rhs_ns) the GL face traces
@@ -278,6 +249,12 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 			const ElemFace f = B.ef[slc][lf];
 			const int side = (f.packed >> 4) & 1, ord = (f.packed >> 8) & 7;
 			const int pi = B.perm[ord][i];
+			if (((f.packed >> 12) & 15) == kBcNonconforming) {
+				const S* y = a.zf + ((size_t)e * 4 + lf) * 3 * kNsNV * N + kNsNV * N;
+#pragma unroll
+				for (int k = 0; k < 2 * kNsNV; ++k) QF[slc][(lf * 2 * kNsNV + k) * N + i] = y[k * N + i];
+				continue;
+			}
 			const S* own = TRo[slc] + lf * kNsNV * N;
 			const S* nb = (TX[slc] + 4 * kNsNV * N) + lf * kNsNV * N;
 			const S* nJ = a.nJ + ((size_t)f.f * N + i) * 2;
@@ -298,6 +275,12 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 		for (int item = t; item < 4 * N; item += Np) {
 			const int lf = item / N, i = item % N;
 			const ElemFace fc = B.ef[slc][lf];
+			if (((fc.packed >> 12) & 15) == kBcNonconforming) {
+				const S* z = a.zf + ((size_t)e * 4 + lf) * 3 * kNsNV * N;
+#pragma unroll
+				for (int v = 0; v < kNsNV; ++v) RES[slc][(lf * kNsNV + v) * N + i] = z[v * N + i];
+				continue;
+			}
 			const int side = (fc.packed >> 4) & 1, ord = (fc.packed >> 8) & 7, fn = fc.packed & 15;
 			const int en = fc.en >= 0 ? fc.en : e;
 			const int pi = B.perm[ord][i];
@@ -474,13 +457,18 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 		}
 }
 
+// phase 0: both kernels; 1: the gradient traces only; 2: the residual only (variable-order meshes run the
+// face kernel between orders in between)
 template <typename S, int N, bool PERT>
-hipError_t run_ns(Ctx& c, const NsArgs<S>& a)
+hipError_t run_ns(Ctx& c, const NsArgs<S>& a, int phase)
 {
 	constexpr int EPB = ns_epb<N>(), Np = N * N;
 	const int grid = (c.E + EPB - 1) / EPB;
-	hipLaunchKernelGGL((k_ns_gv<S, N, PERT>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
-	DGX_HIP_CHECK(hipGetLastError());
+	if (phase != 2) {
+		hipLaunchKernelGGL((k_ns_gv<S, N, PERT>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
+		DGX_HIP_CHECK(hipGetLastError());
+	}
+	if (phase == 1) return hipSuccess;
 	if (c.cfg.num_flux == DGX_FLUX_LF)
 		hipLaunchKernelGGL((k_rhs_ns<S, N, DGX_FLUX_LF, PERT>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
 	else
@@ -492,7 +480,7 @@ hipError_t run_ns(Ctx& c, const NsArgs<S>& a)
 
 template <typename S, bool PERT = false>
 static hipError_t launch_ns(Ctx& c, const S* u, S* r, const S* met, const S* detJ, const S* nJ, const Ops1D<S>* ops,
-                            const double* u64 = nullptr)
+                            const double* u64 = nullptr, int phase = 0)
 {
 	if (c.d != 2 || c.cfg.pde != DGX_PDE_NAVIER_STOKES || !c.ns_gvt || !c.ns_kappa) return hipErrorInvalidValue;
 	if (c.E == 0) return hipSuccess;
@@ -510,12 +498,14 @@ static hipError_t launch_ns(Ctx& c, const S* u, S* r, const S* met, const S* det
 	a.mu = S(c.cfg.mu);
 	a.Pr = S(c.cfg.Pr);
 	a.E = c.E;
+	if constexpr (sizeof(S) == sizeof(double)) a.zf = c.hp_z64;
+	else a.zf = c.hp_z32;
 	switch (c.N) {
-	case 2: return run_ns<S, 2, PERT>(c, a);
-	case 3: return run_ns<S, 3, PERT>(c, a);
-	case 4: return run_ns<S, 4, PERT>(c, a);
-	case 5: return run_ns<S, 5, PERT>(c, a);
-	case 6: return run_ns<S, 6, PERT>(c, a);
+	case 2: return run_ns<S, 2, PERT>(c, a, phase);
+	case 3: return run_ns<S, 3, PERT>(c, a, phase);
+	case 4: return run_ns<S, 4, PERT>(c, a, phase);
+	case 5: return run_ns<S, 5, PERT>(c, a, phase);
+	case 6: return run_ns<S, 6, PERT>(c, a, phase);
 	default: return hipErrorInvalidValue;
 	}
 }
@@ -528,6 +518,16 @@ hipError_t launch_rhs_ns(Ctx& c, const double* u, double* r)
 hipError_t launch_rhs_ns32(Ctx& c, const float* u, float* r)
 {
 	return launch_ns<float>(c, u, r, c.metrics_vc32, c.detJ_vc32, c.nJ_fc32, c.ops32);
+}
+
+hipError_t launch_rhs_ns_phase(Ctx& c, const double* u, double* r, int phase)
+{
+	return launch_ns<double>(c, u, r, c.metrics_vc, c.detJ_vc, c.nJ_fc, c.ops64, nullptr, phase);
+}
+
+hipError_t launch_rhs_ns32_phase(Ctx& c, const float* u, float* r, int phase)
+{
+	return launch_ns<float>(c, u, r, c.metrics_vc32, c.detJ_vc32, c.nJ_fc32, c.ops32, nullptr, phase);
 }
 
 hipError_t launch_rhs_ns_mixed(Ctx& c, const double* u, float* r)

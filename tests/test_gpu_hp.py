@@ -6,8 +6,10 @@
 * A mesh of one order through the variable-order path equals the uniform context bit for bit.
 * The explicit path (M^-1 R, its max-norm, SSP-RK3 stages) equals the oracle's, step by step.
 * FP32 agrees with FP64 to FP32 rounding; the residual sums to zero on a periodic mesh.
-* Unsorted orders, orders above the context's p and Navier-Stokes contexts are refused; entry points that
-  do not serve variable-order meshes return DGX_ERR_UNSUPPORTED.
+* Navier-Stokes (BR2): the groups' gradient kernels, the face kernel between orders (projected face products
+  and lifted fluxes), the groups' residual kernels equal the oracle's rhs_ns to 1e-12; FP32 to FP32 rounding.
+* Unsorted orders, orders above the context's p and viscous meshes with boundary faces are refused; entry
+  points that do not serve variable-order meshes return DGX_ERR_UNSUPPORTED.
 """
 import numpy as np
 import pytest
@@ -190,9 +192,11 @@ def test_hp_refusals(dev):
     with pytest.raises(_lib.DgxError, match="ARG"):
         ctx.upload_mesh_hp(mesh)                    # an order above the context's p
     ctx.close()
+    ch = M.structured_hp_mesh(pe, n, (0, 0), (1, 1), (False, False), bc_low=(M.BC_RIEMANN, M.BC_SLIPWALL),
+                              bc_high=(M.BC_RIEMANN, M.BC_SLIPWALL))
     ctx = _lib.Context(2, 5, M.PDE_NAVIER_STOKES, M.FLUX_ROE, mu=0.01)
     with pytest.raises(_lib.DgxError, match="UNSUPPORTED"):
-        ctx.upload_mesh_hp(mesh)
+        ctx.upload_mesh_hp(ch)                      # viscous boundary conditions are not built
     ctx.close()
     ctx = _ctx(mesh, M.FLUX_ROE)
     u = torch.from_numpy(M.hp_state(mesh, _state_fn(0))).to(dev)
@@ -202,3 +206,85 @@ def test_hp_refusals(dev):
     with pytest.raises(_lib.DgxError, match="UNSUPPORTED"):
         ctx.jacobian_vector(u.data_ptr(), u.data_ptr(), r.data_ptr())
     ctx.close()
+
+
+MU, PR = 0.02, 0.72
+NS_CASES = [(n, flux, curved, rot) for n in ((5, 4), (6, 6)) for flux in (M.FLUX_ROE, M.FLUX_LF)
+            for curved in (False, True) for rot in (False, True)]
+
+
+def _ns_ctx(mesh, flux):
+    from dpgsolver_amd import _lib
+
+    ctx = _lib.Context(2, 5, M.PDE_NAVIER_STOKES, flux, mu=MU, Pr=PR)
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    ctx.upload_mesh_hp(mesh)
+    return ctx
+
+
+@pytest.mark.parametrize("n,flux,curved,rot", NS_CASES,
+                         ids=[f"{n[0]}x{n[1]}_{'roe' if f == M.FLUX_ROE else 'lf'}{'_curved' if c else ''}"
+                              f"{'_rot' if r else ''}" for n, f, c, r in NS_CASES])
+def test_hp_ns_residual_matches_oracle(dev, n, flux, curved, rot):
+    E = n[0] * n[1]
+    pe = M.config4_hp_orders(n, seed=E + 5 * curved + 2 * rot)
+    rotate = np.random.default_rng(E + 1).integers(0, 4, E) if rot else None
+    mesh = M.structured_hp_mesh(pe, n, (0, 0), (1, 1), (True, True), curve=_curve if curved else None,
+                                rotate=rotate)
+    u = M.hp_state(mesh, _state_fn(E + 1))
+    orc = H.HpOracle(mesh, M.PDE_EULER, flux, viscous=(MU, PR))
+    ref = orc.rhs_ns(u)
+    assert np.abs(ref - orc.rhs(u)).max() > 1e-3 * np.abs(ref).max()   # the viscous part is not negligible
+    ctx = _ns_ctx(mesh, flux)
+    ud = torch.from_numpy(u).to(dev)
+    r = torch.full_like(ud, np.nan)
+    ctx.compute_rhs(ud.data_ptr(), r.data_ptr())
+    torch.cuda.synchronize()
+    got = r.cpu().numpy()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"{n} orders {np.bincount(pe, minlength=6)[1:]} flux={flux} curved={curved} rot={rot}: {err:.2e}")
+    assert err < 1e-12
+    off = mesh.offsets(4)
+    tot = sum(got[off[e]:off[e + 1]].reshape(4, -1).sum(1) for e in range(mesh.E))
+    assert np.abs(tot).max() < 1e-13 * np.abs(got).sum()
+    u32 = ud.float()
+    r32 = torch.full_like(u32, np.nan)
+    ctx.compute_rhs_f32(u32.data_ptr(), r32.data_ptr())
+    torch.cuda.synchronize()
+    e32 = ((r32.double() - r).abs().max() / r.abs().max()).item()
+    assert e32 < 2e-4, e32          # FP32 state rounding, amplified by the second derivatives
+    ctx.close()
+
+
+def test_hp_ns_explicit_path_and_uniform_equivalence(dev):
+    from dpgsolver_amd import _lib
+
+    n = (5, 5)
+    pe = M.config4_hp_orders(n, seed=9)
+    mesh = M.structured_hp_mesh(pe, n, (0, 0), (1, 1), (True, True), curve=_curve)
+    u = M.hp_state(mesh, _state_fn(8))
+    orc = H.HpOracle(mesh, M.PDE_EULER, M.FLUX_ROE, viscous=(MU, PR))
+    ctx = _ns_ctx(mesh, M.FLUX_ROE)
+    ud = torch.from_numpy(u).to(dev)
+    r = torch.empty_like(ud)
+    mx = ctx.compute_rhs_explicit(ud.data_ptr(), r.data_ptr())
+    torch.cuda.synchronize()
+    ref = orc.apply_minv(orc.rhs_ns(u))
+    assert np.abs(r.cpu().numpy() - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert abs(mx - np.abs(ref).max()) <= 1e-12 * np.abs(ref).max()
+    ctx.close()
+    # one order through the variable-order path equals the uniform Navier-Stokes context bit for bit
+    hm = M.structured_hp_mesh(np.full(20, 3), (5, 4), (0, 0), (1, 1), (True, True), curve=_curve)
+    um = M.structured_tp_mesh(2, 3, (5, 4), (0, 0), (1, 1), (True, True), curve=_curve)
+    uu = torch.from_numpy(M.hp_state(hm, _state_fn(3))).to(dev)
+    ch = _ns_ctx(hm, M.FLUX_ROE)
+    cu = _lib.Context(2, 3, M.PDE_NAVIER_STOKES, M.FLUX_ROE, mu=MU, Pr=PR)
+    cu.set_stream(torch.cuda.current_stream().cuda_stream)
+    cu.upload_mesh(um.xyz, um.faces)
+    rh, ru = torch.empty_like(uu), torch.empty_like(uu)
+    ch.compute_rhs(uu.data_ptr(), rh.data_ptr())
+    cu.compute_rhs(uu.data_ptr(), ru.data_ptr())
+    torch.cuda.synchronize()
+    assert torch.equal(rh, ru)
+    ch.close()
+    cu.close()
