@@ -320,6 +320,73 @@ def ns_throughput(torch, dev, steps, warmup, n=448):
             "gradient_round_trip_bytes": grad_rt}
 
 
+def hp_ns_throughput(torch, dev, steps, warmup):
+    """2-D Navier-Stokes (BR2) residual on the variable-order workload: the 500 x 300 periodic QUAD base grid
+    of [0,1]^2 with orders p ~ U{1..5} per element (seeded; the TRI split is not built), Roe, Pr 0.72, Re 100
+    (mu = 0.0028).  One evaluation = every order group's gradient kernel, the face kernel between orders and
+    every group's residual kernel, replayed as one HIP graph (FP64 and FP32)."""
+    import numpy as np
+    from dpgsolver_amd import _lib, mesh as M
+
+    mu, Pr = 0.28 / 100.0, 0.72
+    mesh = M.config4_hp_mesh()
+    ctx = _lib.Context(2, 5, M.PDE_NAVIER_STOKES, M.FLUX_ROE, mu=mu, Pr=Pr)
+    stream = torch.cuda.current_stream(dev)
+    ctx.set_stream(stream.cuda_stream)
+    ctx.upload_mesh_hp(mesh)
+    u = torch.from_numpy(M.config4_hp_state(mesh)).to(dev)
+    r = torch.empty_like(u)
+    u32 = u.float()
+    r32 = torch.empty_like(u32)
+    g = ctx.graph_rhs(_lib.GRAPH_RHS_FP64, u.data_ptr(), r.data_ptr())
+    g32 = ctx.graph_rhs(_lib.GRAPH_RHS_FP32, u32.data_ptr(), r32.data_ptr())
+
+    def timed(graph):
+        for _ in range(warmup):
+            graph.launch(1)
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        graph.launch(steps)
+        stop.record(stream)
+        torch.cuda.synchronize()
+        return start.elapsed_time(stop) / steps
+
+    ms = timed(g)
+    ms32 = timed(g32)
+    err32 = ((r32.double() - r).abs().max() / r.abs().max()).item()
+    ur = u32.double()
+    rr = torch.empty_like(u)
+    ctx.compute_rhs(ur.data_ptr(), rr.data_ptr())
+    torch.cuda.synchronize()
+    err_in = ((rr - r).abs().max() / r.abs().max()).item()
+    del ur, rr
+    nv, d = 4, 2
+    Np = (mesh.p.astype(np.int64) + 1) ** 2
+    N = mesh.p.astype(np.int64) + 1
+    ndof = int((Np * nv).sum())
+    # compulsory traffic per element: state in, residual out, geometry (metrics, det J, n J_f of its d owned faces)
+    alg = int((2 * nv * Np * 8 + Np * (d * d + 1) * 8 + d * N * d * 8).sum())
+    alg32 = int((2 * nv * Np * 4 + Np * (d * d + 1) * 4 + d * N * d * 4).sum())
+    fp, fo = mesh.p[mesh.faces[:, 0]], mesh.p[mesh.faces[:, 2]]
+    g.close()
+    g32.close()
+    ctx.close()
+    return {"workload": f"2-D Navier-Stokes BR2, 500x300 periodic QUAD ({mesh.E} elements), p ~ U{{1..5}} per "
+                        f"element, Roe, Re 100, Pr 0.72",
+            "elements_per_order": np.bincount(mesh.p, minlength=6)[1:].tolist(),
+            "faces_between_orders": int((fp != fo).sum()), "faces": int(mesh.F), "ndof": ndof,
+            "dtype": "f64", "value": round(ndof / (ms * 1e-3) / 1e9, 3), "unit": "GDOF/s",
+            "ms_per_step": round(ms, 5),
+            "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+            "fp32": {"value": round(ndof / (ms32 * 1e-3) / 1e9, 3), "ms_per_step": round(ms32, 5),
+                     "rel_err_vs_fp64": err32, "input_rounding_rel_err": err_in,
+                     "roofline": {"bound": "hbm", "achieved": round(alg32 / (ms32 * 1e-3) / 1e9, 1),
+                                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": round(alg32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -332,6 +399,8 @@ def main():
                     help="run only the time-to-1e-10 legs and print their JSON (profiling aid)")
     ap.add_argument("--ns-only", action="store_true",
                     help="run only the Navier-Stokes residual leg and print its JSON (profiling aid)")
+    ap.add_argument("--hp-only", action="store_true",
+                    help="run only the variable-order Navier-Stokes leg and print its JSON (profiling aid)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 1 GiB read + 1 GiB write elementwise kernel (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -351,6 +420,10 @@ def main():
         return
     if args.ns_only:
         print(json.dumps({"navier_stokes": ns_throughput(torch, dev, args.steps, args.warmup)}), flush=True)
+        return
+    if args.hp_only:
+        print(json.dumps({"navier_stokes_variable_order": hp_ns_throughput(torch, dev, args.steps, args.warmup)}),
+              flush=True)
         return
     n, p = args.n, 4
     t_setup = time.perf_counter()
@@ -446,6 +519,9 @@ def main():
 
     ns = ns_throughput(torch, dev, args.steps, args.warmup)
     print(f"navier-stokes residual: {ns['ms_per_step']:.4f} ms, {ns['value']} GDOF/s", file=sys.stderr, flush=True)
+    hp = hp_ns_throughput(torch, dev, args.steps, args.warmup)
+    print(f"variable-order navier-stokes residual: {hp['ms_per_step']:.4f} ms, {hp['value']} GDOF/s",
+          file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline()
     implicit = None if args.no_implicit else time_to_steady(torch, dev, cpu=not args.no_cpu)
 
@@ -480,6 +556,7 @@ def main():
         "fp32_input_rounding_rel_err": err_in,
         "fp64_free_stream_max_abs": free_stream,
         "navier_stokes": ns,
+        "navier_stokes_variable_order": hp,
         "cpu_baseline": cpu,
         "time_to_1e-10": implicit,
     }

@@ -457,10 +457,32 @@ dgx_status dgx_upload_mesh_hp(dgx_ctx* h, const dgx_mesh_hp* m)
 			}
 		}
 	}
+	// faces sorted by their (NL, NR) pair (stable): one kernel instance per pair, every wave on one pair
+	std::vector<int> fperm(hf.size());
+	for (size_t i = 0; i < hf.size(); ++i) fperm[i] = (int)i;
+	auto pair_key = [&](int i) { return (hf[i].info & 7) * 8 + ((hf[i].info >> 3) & 7); };
+	std::stable_sort(fperm.begin(), fperm.end(), [&](int a, int b) { return pair_key(a) < pair_key(b); });
+	{
+		std::vector<dgx::HpFace> hs(hf.size());
+		std::vector<double> ns(nj.size());
+		for (size_t i = 0; i < hf.size(); ++i) {
+			hs[i] = hf[fperm[i]];
+			std::copy(nj.begin() + (size_t)fperm[i] * 2 * dgx::kHpMaxN, nj.begin() + ((size_t)fperm[i] + 1) * 2 * dgx::kHpMaxN,
+			          ns.begin() + i * 2 * dgx::kHpMaxN);
+		}
+		hf.swap(hs);
+		nj.swap(ns);
+	}
 	free_mesh(c);
 	c.hp = new (std::nothrow) dgx::HpData();
 	if (!c.hp) return DGX_ERR_ARG;
 	dgx::HpData& H = *c.hp;
+	for (size_t i = 0; i < hf.size(); ++i) {
+		const int NL = hf[i].info & 7, NR = (hf[i].info >> 3) & 7;
+		if (H.pairs.empty() || H.pairs.back().NL != NL || H.pairs.back().NR != NR)
+			H.pairs.push_back(dgx::HpPair{ NL, NR, (int)i, 0 });
+		H.pairs.back().count += 1;
+	}
 	H.order = order;
 	H.e_off = e_off;
 	H.E = E;
@@ -695,7 +717,25 @@ dgx_status dgx_graph_create_rhs(dgx_ctx* h, int kind, const void* u, void* r, dg
 {
 	if (!h || !u || !r || !out || kind < DGX_GRAPH_RHS_FP64 || kind > DGX_GRAPH_RHS_MIXED) return DGX_ERR_ARG;
 	Ctx& c = h->c;
-	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
+	if (c.hp) {
+		// variable-order mesh: the face kernel between orders and every order group's launches in one graph
+		if (kind == DGX_GRAPH_RHS_MIXED) return DGX_ERR_UNSUPPORTED;
+		dgx_graph* g = new (std::nothrow) dgx_graph();
+		if (!g) return DGX_ERR_HIP;
+		const dgx_status st = capture_graph(h, g, [&]() -> hipError_t {
+			const dgx_status s = (kind == DGX_GRAPH_RHS_FP64)
+			    ? hp_rhs64(c, static_cast<const double*>(u), static_cast<double*>(r), dgx::MODE_RHS, nullptr)
+			    : hp_rhs32(c, static_cast<const float*>(u), static_cast<float*>(r));
+			return s == DGX_OK ? hipSuccess : hipErrorLaunchFailure;
+		});
+		if (st != DGX_OK) {
+			delete g;
+			return st;
+		}
+		*out = g;
+		return DGX_OK;
+	}
+	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
 	if (kind == DGX_GRAPH_RHS_MIXED && c.cfg.pde == DGX_PDE_ADVECTION) return DGX_ERR_UNSUPPORTED;
 	dgx_graph* g = new (std::nothrow) dgx_graph();
 	if (!g) return DGX_ERR_HIP;

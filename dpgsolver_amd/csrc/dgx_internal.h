@@ -62,7 +62,11 @@ struct HpFace {
 	int pad;
 };
 
-struct Ctx;
+// a run of faces between the same two orders (the faces are sorted by pair)
+struct HpPair {
+	int NL, NR, start, count;
+};
+
 // A variable-order mesh: one uniform-order context per order present; elements sorted by order, so each group
 // is a contiguous range of the state vector.
 struct HpData {
@@ -73,7 +77,8 @@ struct HpData {
 	std::vector<long long> z_off;    // first lifted-flux value of each group
 	int E = 0, F = 0, Fnc = 0;
 	long long ndof = 0, nz = 0;
-	HpFace* faces = nullptr;         // Fnc
+	std::vector<HpPair> pairs;       // runs of faces with one (NL, NR)
+	HpFace* faces = nullptr;         // Fnc, sorted by (NL, NR)
 	double* nJ = nullptr;            // Fnc x kHpMaxN x 2: n J_f at the face GL points (left ordering, left outward)
 	float* nJ32 = nullptr;
 	double* z64 = nullptr;           // lifted fluxes, FP64 and FP32 (each group: E_g x 4 x nv x N_g)

@@ -124,7 +124,8 @@ __global__ void __launch_bounds__(64) k_hp_faces(HpFaceArgs<S> a)
 // P_e (trace G_v + eta kappa_e y_e), and the face flux is the inviscid flux plus the central
 // n . (fv_L + fv_R) / 2 (numerical_flux_central_T.c:76-104).  Per element face the kernel writes the lifted
 // flux z [v][a] and then y [d][v][a], which the residual kernel of each order reads in place of its own face
-// traces.  Eight lanes per face (Nf <= 6 points), the stages staged in LDS.
+// traces.  One instance per (NL, NR) pair (the faces arrive sorted by pair), eight lanes per face (lane = face
+// point or own face node); the pair's operators, the face nodes and the cross-lane stages go through LDS.
 template <typename S>
 struct HpFaceNsArgs {
 	const S* u;
@@ -138,125 +139,127 @@ struct HpFaceNsArgs {
 	int F;
 };
 
-template <typename S, int FLUX>
-__global__ void __launch_bounds__(64) k_hp_faces_ns(HpFaceNsArgs<S> a)
+template <typename S, int FLUX, int NL, int NR>
+__global__ void __launch_bounds__(64) k_hp_faces_ns(HpFaceNsArgs<S> a, int f0, int nf)
 {
-	constexpr int NV = 4, NM = kHpMaxN, FPB = 8;
+	constexpr int NV = 4, FPB = 8, Nf = NL > NR ? NL : NR;
 	constexpr S eta = S(1.01 * 4.0);
-	__shared__ S uLs[FPB][NV][NM], uRs[FPB][NV][NM];           // GLL face nodes, own orderings
-	__shared__ S gvL[FPB][2 * NV][NM], gvR[FPB][2 * NV][NM];   // traces of G_v [d][v][a], own orderings
-	__shared__ S kL[FPB][NM], kR[FPB][NM];
-	__shared__ S sLq[FPB][NM][NV], sRq[FPB][NM][NV];           // face traces at the Nf points, left ordering
-	__shared__ S qf[FPB][NM][2 * NV];                          // w_q q [d][v] at the Nf points
-	__shared__ S yL[FPB][2 * NV][NM], yR[FPB][2 * NV][NM];     // projected products, own orderings
-	__shared__ S gq[FPB][NM][NV];                              // w_q J_f (F-hat + viscous)
+	// this pair's operators, staged once per block
+	__shared__ S IgL[Nf * NL], IgR[Nf * NR], PgL[Nf * NL], PgR[Nf * NR], wf[Nf], wiL[NL], wiR[NR];
+	// per face, aliased across the stages: the GLL face nodes (stage 1-2), then h = trace G_v + eta kappa y at
+	// the own face nodes (stage 3-4); w q (stage 2-3), then w J_f F-hat (stage 4-5)
+	constexpr int HS = 2 * NV * (NL + NR);
+	constexpr int QS = 2 * NV * Nf;
+	__shared__ S Hs[FPB][HS];
+	__shared__ S Qs[FPB][QS];
 	const int slot = threadIdx.x >> 3, j = threadIdx.x & 7;
-	const int f = blockIdx.x * FPB + slot;
-	const bool valid = f < a.F;
-	HpFace hf{};
-	int NL = 0, NR = 0, lfL = 0, lfR = 0, Nf = 0;
-	bool flip = false;
-	if (valid) {
-		hf = a.faces[f];
-		NL = hf.info & 7;
-		NR = (hf.info >> 3) & 7;
-		lfL = (hf.info >> 6) & 3;
-		lfR = (hf.info >> 8) & 3;
-		flip = (hf.info >> 11) & 1;
-		Nf = NL > NR ? NL : NR;
+	for (int i = threadIdx.x; i < Nf * NL; i += 64) {
+		IgL[i] = a.ops->Ig[NL][Nf][i];
+		PgL[i] = a.ops->Pg[NL][Nf][i];
 	}
-	const HpOps<S>* o = a.ops;
+	for (int i = threadIdx.x; i < Nf * NR; i += 64) {
+		IgR[i] = a.ops->Ig[NR][Nf][i];
+		PgR[i] = a.ops->Pg[NR][Nf][i];
+	}
+	if (threadIdx.x < Nf) wf[threadIdx.x] = a.ops->w[Nf][threadIdx.x];
+	if (threadIdx.x < NL) wiL[threadIdx.x] = a.ops->winv[NL][threadIdx.x];
+	if (threadIdx.x < NR) wiR[threadIdx.x] = a.ops->winv[NR][threadIdx.x];
+	const int fl = blockIdx.x * FPB + slot;
+	const bool valid = fl < nf;
+	const int f = f0 + (valid ? fl : 0);
+	const HpFace hf = a.faces[f];
+	const int lfL = (hf.info >> 6) & 3, lfR = (hf.info >> 8) & 3;
+	const bool flip = (hf.info >> 11) & 1;   // ord bit 1: the right face runs the other way
+	S* uLs = Hs[slot];                       // [v][i], i < NL
+	S* uRs = Hs[slot] + NV * NL;             // [v][i], i < NR
 	if (valid && j < NL) {
-		const S* p = a.u + hf.uL + face_node_2d(NL, lfL, j);
-		const S* t = a.gvt[NL] + ((size_t)hf.eL * 4 + lfL) * 2 * NV * NL;
 #pragma unroll
-		for (int v = 0; v < NV; ++v) uLs[slot][v][j] = p[v * NL * NL];
-#pragma unroll
-		for (int k = 0; k < 2 * NV; ++k) gvL[slot][k][j] = t[k * NL + j];
-		kL[slot][j] = a.kap[NL][((size_t)hf.eL * 4 + lfL) * NL + j];
+		for (int v = 0; v < NV; ++v) uLs[v * NL + j] = a.u[hf.uL + v * NL * NL + face_node_2d(NL, lfL, j)];
 	}
 	if (valid && j < NR) {
-		const S* p = a.u + hf.uR + face_node_2d(NR, lfR, j);
-		const S* t = a.gvt[NR] + ((size_t)hf.eR * 4 + lfR) * 2 * NV * NR;
 #pragma unroll
-		for (int v = 0; v < NV; ++v) uRs[slot][v][j] = p[v * NR * NR];
-#pragma unroll
-		for (int k = 0; k < 2 * NV; ++k) gvR[slot][k][j] = t[k * NR + j];
-		kR[slot][j] = a.kap[NR][((size_t)hf.eR * 4 + lfR) * NR + j];
+		for (int v = 0; v < NV; ++v) uRs[v * NR + j] = a.u[hf.uR + v * NR * NR + face_node_2d(NR, lfR, j)];
 	}
 	__syncthreads();
-	// traces at the face points (left ordering) and w q
+	// traces at this lane's face point (left ordering), kept in registers; w q into LDS
+	S sL[NV], sR[NV];
+	S* qf = Qs[slot];                        // [q][d * NV + v]
 	if (valid && j < Nf) {
 		const int q = j, qr = flip ? Nf - 1 - q : q;
-		const S* IL = o->Ig[NL][Nf] + q * NL;
-		const S* IR = o->Ig[NR][Nf] + qr * NR;
-		const S* nj = a.nJ + ((size_t)f * NM + q) * 2;
-		const S wq = o->w[Nf][q];
+		const S* nj = a.nJ + ((size_t)f * kHpMaxN + q) * 2;
+		const S wq = wf[q];
 #pragma unroll
 		for (int v = 0; v < NV; ++v) {
 			S aL = S(0), aR = S(0);
 #pragma unroll
-			for (int i = 0; i < NM; ++i) {
-				if (i < NL) aL += IL[i] * uLs[slot][v][i];
-				if (i < NR) aR += IR[i] * uRs[slot][v][i];
-			}
-			sLq[slot][q][v] = aL;
-			sRq[slot][q][v] = aR;
+			for (int i = 0; i < NL; ++i) aL += IgL[q * NL + i] * uLs[v * NL + i];
+#pragma unroll
+			for (int i = 0; i < NR; ++i) aR += IgR[qr * NR + i] * uRs[v * NR + i];
+			sL[v] = aL;
+			sR[v] = aR;
 			const S dl = S(0.5) * (aR - aL);
-			qf[slot][q][v] = wq * (nj[0] * dl);
-			qf[slot][q][NV + v] = wq * (nj[1] * dl);
+			qf[q * 2 * NV + v] = wq * (nj[0] * dl);
+			qf[q * 2 * NV + NV + v] = wq * (nj[1] * dl);
 		}
 	}
 	__syncthreads();
-	// projections to each side's own GL face points
+	// projections y to each side's own GL face points (written out for the residual kernels), and
+	// h = trace G_v + eta kappa y at the lane's own node (the face nodes in LDS are dead)
+	S hL[2 * NV], hR[2 * NV];
 	if (valid && j < NL) {
+		const S* t = a.gvt[NL] + ((size_t)hf.eL * 4 + lfL) * 2 * NV * NL;
+		const S kap = a.kap[NL][((size_t)hf.eL * 4 + lfL) * NL + j];
 #pragma unroll
 		for (int k = 0; k < 2 * NV; ++k) {
 			S acc = S(0);
 #pragma unroll
-			for (int q = 0; q < NM; ++q)
-				if (q < Nf) acc += o->Pg[NL][Nf][q * NL + j] * qf[slot][q][k];
-			acc = acc * o->winv[NL][j];
-			yL[slot][k][j] = acc;
+			for (int q = 0; q < Nf; ++q) acc += PgL[q * NL + j] * qf[q * 2 * NV + k];
+			acc = acc * wiL[j];
 			a.z[hf.zL + NV * NL + k * NL + j] = acc;
+			hL[k] = t[k * NL + j] + eta * kap * acc;
 		}
 	}
 	if (valid && j < NR) {
+		const S* t = a.gvt[NR] + ((size_t)hf.eR * 4 + lfR) * 2 * NV * NR;
+		const S kap = a.kap[NR][((size_t)hf.eR * 4 + lfR) * NR + j];
 #pragma unroll
 		for (int k = 0; k < 2 * NV; ++k) {
 			S acc = S(0);
 #pragma unroll
-			for (int q = 0; q < NM; ++q)
-				if (q < Nf) acc += o->Pg[NR][Nf][(flip ? Nf - 1 - q : q) * NR + j] * qf[slot][q][k];
-			acc = acc * o->winv[NR][j];
-			yR[slot][k][j] = acc;
+			for (int q = 0; q < Nf; ++q) acc += PgR[(flip ? Nf - 1 - q : q) * NR + j] * qf[q * 2 * NV + k];
+			acc = acc * wiR[j];
 			a.z[hf.zR + NV * NR + k * NR + j] = acc;
+			hR[k] = t[k * NR + j] + eta * kap * acc;
 		}
 	}
+	__syncthreads();   // w q and the face nodes are dead
+	S* hLs = Hs[slot];                       // [k][i], i < NL
+	S* hRs = Hs[slot] + 2 * NV * NL;         // [k][i], i < NR
+	if (valid && j < NL) {
+#pragma unroll
+		for (int k = 0; k < 2 * NV; ++k) hLs[k * NL + j] = hL[k];
+	}
+	if (valid && j < NR) {
+#pragma unroll
+		for (int k = 0; k < 2 * NV; ++k) hRs[k * NR + j] = hR[k];
+	}
 	__syncthreads();
-	// face gradients and the face flux at the Nf points
+	// face gradients P_e h_e and the face flux at this lane's point
+	S* gq = Qs[slot];                        // [q][v]
 	if (valid && j < Nf) {
 		const int q = j, qr = flip ? Nf - 1 - q : q;
-		const S* PL = o->Pg[NL][Nf] + q * NL;
-		const S* PR = o->Pg[NR][Nf] + qr * NR;
-		S gL[2 * NV], gR[2 * NV], sL[NV], sR[NV];
+		S gL[2 * NV], gR[2 * NV];
 #pragma unroll
 		for (int k = 0; k < 2 * NV; ++k) {
 			S aL = S(0), aR = S(0);
 #pragma unroll
-			for (int i = 0; i < NM; ++i) {
-				if (i < NL) aL += PL[i] * (gvL[slot][k][i] + eta * kL[slot][i] * yL[slot][k][i]);
-				if (i < NR) aR += PR[i] * (gvR[slot][k][i] + eta * kR[slot][i] * yR[slot][k][i]);
-			}
+			for (int i = 0; i < NL; ++i) aL += PgL[q * NL + i] * hLs[k * NL + i];
+#pragma unroll
+			for (int i = 0; i < NR; ++i) aR += PgR[qr * NR + i] * hRs[k * NR + i];
 			gL[k] = aL;
 			gR[k] = aR;
 		}
-#pragma unroll
-		for (int v = 0; v < NV; ++v) {
-			sL[v] = sLq[slot][q][v];
-			sR[v] = sRq[slot][q][v];
-		}
-		const S* nj = a.nJ + ((size_t)f * NM + q) * 2;
+		const S* nj = a.nJ + ((size_t)f * kHpMaxN + q) * 2;
 		const S Jf = tsqrt(nj[0] * nj[0] + nj[1] * nj[1]);
 		const S Jfi = trcp(Jf);
 		const S nu[2] = { nj[0] * Jfi, nj[1] * Jfi };
@@ -265,11 +268,11 @@ __global__ void __launch_bounds__(64) k_hp_faces_ns(HpFaceNsArgs<S> a)
 		else numflux_roe<S, 2>(sL, sR, nu, nnf);
 		ns_viscous_flux2<S>(sL, gL, a.mu, a.Pr, fvL);
 		ns_viscous_flux2<S>(sR, gR, a.mu, a.Pr, fvR);
-		const S wj = o->w[Nf][q] * Jf;
+		const S wj = wf[q] * Jf;
 #pragma unroll
 		for (int v = 0; v < NV; ++v) {
 			const S vis = nu[0] * S(0.5) * (fvL[0 + 2 * v] + fvR[0 + 2 * v]) + nu[1] * S(0.5) * (fvL[1 + 2 * v] + fvR[1 + 2 * v]);
-			gq[slot][q][v] = wj * (nnf[v] + vis);
+			gq[q * NV + v] = wj * (nnf[v] + vis);
 		}
 	}
 	__syncthreads();
@@ -279,9 +282,8 @@ __global__ void __launch_bounds__(64) k_hp_faces_ns(HpFaceNsArgs<S> a)
 		for (int v = 0; v < NV; ++v) {
 			S acc = S(0);
 #pragma unroll
-			for (int q = 0; q < NM; ++q)
-				if (q < Nf) acc += o->Pg[NL][Nf][q * NL + j] * gq[slot][q][v];
-			a.z[hf.zL + v * NL + j] = acc * o->winv[NL][j];
+			for (int q = 0; q < Nf; ++q) acc += PgL[q * NL + j] * gq[q * NV + v];
+			a.z[hf.zL + v * NL + j] = acc * wiL[j];
 		}
 	}
 	if (valid && j < NR) {
@@ -289,9 +291,8 @@ __global__ void __launch_bounds__(64) k_hp_faces_ns(HpFaceNsArgs<S> a)
 		for (int v = 0; v < NV; ++v) {
 			S acc = S(0);
 #pragma unroll
-			for (int q = 0; q < NM; ++q)
-				if (q < Nf) acc += o->Pg[NR][Nf][(flip ? Nf - 1 - q : q) * NR + j] * gq[slot][q][v];
-			a.z[hf.zR + v * NR + j] = S(0) - acc * o->winv[NR][j];
+			for (int q = 0; q < Nf; ++q) acc += PgR[(flip ? Nf - 1 - q : q) * NR + j] * gq[q * NV + v];
+			a.z[hf.zR + v * NR + j] = S(0) - acc * wiR[j];
 		}
 	}
 }
@@ -316,10 +317,26 @@ hipError_t run_hp_faces(Ctx& c, const S* u, S* z, const S* nJ, const HpOps<S>* o
 		a.mu = S(c.cfg.mu);
 		a.Pr = S(c.cfg.Pr);
 		a.F = H.Fnc;
-		const int blocks = (H.Fnc + 7) / 8;
-		if (c.cfg.num_flux == DGX_FLUX_LF) hipLaunchKernelGGL((k_hp_faces_ns<S, DGX_FLUX_LF>), dim3(blocks), dim3(64), 0, c.stream, a);
-		else hipLaunchKernelGGL((k_hp_faces_ns<S, DGX_FLUX_ROE>), dim3(blocks), dim3(64), 0, c.stream, a);
-		return hipGetLastError();
+		const bool lf = c.cfg.num_flux == DGX_FLUX_LF;
+		for (const HpPair& pr : H.pairs) {
+			const int blocks = (pr.count + 7) / 8;
+#define DGX_HP_PAIR(L, R)                                                                                              \
+	case L * 8 + R:                                                                                                    \
+		if (lf) hipLaunchKernelGGL((k_hp_faces_ns<S, DGX_FLUX_LF, L, R>), dim3(blocks), dim3(64), 0, c.stream, a, pr.start, pr.count); \
+		else hipLaunchKernelGGL((k_hp_faces_ns<S, DGX_FLUX_ROE, L, R>), dim3(blocks), dim3(64), 0, c.stream, a, pr.start, pr.count); \
+		break;
+			switch (pr.NL * 8 + pr.NR) {
+				DGX_HP_PAIR(2, 3) DGX_HP_PAIR(2, 4) DGX_HP_PAIR(2, 5) DGX_HP_PAIR(2, 6)
+				DGX_HP_PAIR(3, 2) DGX_HP_PAIR(3, 4) DGX_HP_PAIR(3, 5) DGX_HP_PAIR(3, 6)
+				DGX_HP_PAIR(4, 2) DGX_HP_PAIR(4, 3) DGX_HP_PAIR(4, 5) DGX_HP_PAIR(4, 6)
+				DGX_HP_PAIR(5, 2) DGX_HP_PAIR(5, 3) DGX_HP_PAIR(5, 4) DGX_HP_PAIR(5, 6)
+				DGX_HP_PAIR(6, 2) DGX_HP_PAIR(6, 3) DGX_HP_PAIR(6, 4) DGX_HP_PAIR(6, 5)
+			default: return hipErrorInvalidValue;
+			}
+#undef DGX_HP_PAIR
+			DGX_HIP_CHECK(hipGetLastError());
+		}
+		return hipSuccess;
 	}
 	HpFaceArgs<S> a{ u, z, H.faces, nJ, ops, H.Fnc };
 	const int blocks = (H.Fnc + 63) / 64;

@@ -401,8 +401,8 @@ def smooth_state_3d(x: np.ndarray, seed: int = SEED) -> np.ndarray:
 
 
 def config4_mesh(n: int = 448, p: int = 3) -> Mesh:
-    """2-D Navier-Stokes workload on QUADs only: periodic [0,1]^2, n x n elements, one order p (the
-    configuration's TRI half and per-element orders are not built)."""
+    """2-D Navier-Stokes workload on QUADs of one order p: periodic [0,1]^2, n x n elements (the per-element
+    orders are config4_hp_mesh; the configuration's TRI half is not built)."""
     return structured_tp_mesh(2, p, (n, n), (0.0, 0.0), (1.0, 1.0), (True, True))
 
 
@@ -518,17 +518,19 @@ def structured_hp_mesh(p_elem, n: tuple, lo: tuple, hi: tuple, periodic: tuple, 
     eidx = _tp_coords_var(n)
     h = np.array([(hi[k] - lo[k]) / n[k] for k in range(d)])
     base = np.array(lo)[None, :] + eidx * h[None, :]
-    xyz_grid = []
-    for e in range(E):
-        N = int(p_grid[e]) + 1
+    xyz_grid = [None] * E
+    for p in np.unique(p_grid):
+        N = int(p) + 1
         x1, _ = gauss_lobatto(N)
         ref = x1[_tp_coords(N, d)]
-        pts = base[e][None, :] + 0.5 * (ref + 1.0) * h[None, :]
+        sel = np.nonzero(p_grid == p)[0]
+        pts = base[sel][:, None, :] + 0.5 * (ref[None] + 1.0) * h[None, None, :]    # (E_p, Np, d)
         if curve is not None:
             pts = curve(pts)
-        if rotate is not None and int(rotate[e]) % 4:
-            pts = _apply_rotation(d, N, pts[None], np.array([rotate[e]]))[0]
-        xyz_grid.append(np.ascontiguousarray(pts.T))
+        if rotate is not None:
+            pts = _apply_rotation(d, N, pts, np.asarray(rotate)[sel])
+        for k, e in enumerate(sel):
+            xyz_grid[e] = np.ascontiguousarray(pts[k].T)
     pairs, bfaces = [], []
     strides = [int(np.prod(n[:k])) for k in range(d)]
     all_e = np.arange(E)
@@ -564,6 +566,10 @@ def structured_hp_mesh(p_elem, n: tuple, lo: tuple, hi: tuple, periodic: tuple, 
         return 0.5 * (a + b)
 
     rows = []
+    if rotate is None:
+        # unrotated structured numbering: the +k face of eL meets the -k face of eR in the same direction
+        rows = [(eL, 2 * k + 1, eR, 2 * k, 0, BC_INTERIOR) for (eL, eR, k) in pairs]
+        pairs = []
     for eL, eR, k in pairs:
         fL = int(np.argmax([center(eL, lf)[k] for lf in range(4)]))
         aL, bL = ends(eL, fL)
@@ -603,11 +609,28 @@ def hp_nodes_xyz(mesh: HpMesh) -> list:
 
 
 def hp_state(mesh: HpMesh, cons_fn) -> np.ndarray:
-    """Flat state: cons_fn(x (Np, 2)) -> (Np, nv) conserved values per element, packed [var][node]."""
-    return np.ascontiguousarray(np.concatenate([cons_fn(x).T.ravel() for x in hp_nodes_xyz(mesh)]))
+    """Flat state: cons_fn(x (n, 2)) -> (n, nv) conserved values (pointwise), packed per element [var][node]."""
+    out = []
+    for p in np.unique(mesh.p):
+        sel = np.nonzero(mesh.p == p)[0]                    # one contiguous group (orders are sorted)
+        x = np.stack([mesh.xyz[e].T for e in sel])          # (E_p, Np, 2)
+        c = cons_fn(x.reshape(-1, 2)).reshape(len(sel), x.shape[1], -1)
+        out.append(c.transpose(0, 2, 1).ravel())
+    return np.ascontiguousarray(np.concatenate(out))
 
 
 def config4_hp_orders(n: tuple, seed: int = SEED) -> np.ndarray:
     """Per-element orders p ~ U{1..5} of the variable-order Navier-Stokes workload (seeded, grid order)."""
     r = SplitMix64(seed + 4).uniform(int(np.prod(n)), 0.0, 1.0)
     return (1 + np.minimum((r * 5).astype(np.int32), 4)).astype(np.int32)
+
+
+def config4_hp_mesh(nx: int = 500, ny: int = 300, seed: int = SEED) -> HpMesh:
+    """Variable-order 2-D Navier-Stokes workload: the configuration's 500 x 300 base grid on periodic [0,1]^2,
+    all QUAD (its seeded TRI split is not built), orders p ~ U{1..5} per element (seeded)."""
+    return structured_hp_mesh(config4_hp_orders((nx, ny), seed), (nx, ny), (0.0, 0.0), (1.0, 1.0), (True, True))
+
+
+def config4_hp_state(mesh: HpMesh, seed: int = SEED) -> np.ndarray:
+    """The configuration-3 state formula in 2-D (as config4_state) at every element's own nodes."""
+    return hp_state(mesh, lambda x: prim_to_cons(smooth_state_3d(x, seed + 1), 2))

@@ -100,14 +100,16 @@ dgx_status dgx_sizes(const dgx_ctx* ctx, int* Np, int* nv, int* Nfc, long long* 
  * (src/simulation/geometry/geometry_T.c:165, 208-325, 734-801;
  *  deprecated/src/setup_geom_factors.c:157-243 for the 3-D curl form). Synchronous. */
 dgx_status dgx_upload_mesh(dgx_ctx* ctx, const dgx_mesh* mesh);
-/* Variable-order mesh (2-D QUAD, Euler): element e has order p[e] and its state block nv x (p[e]+1)^2 in
- * the layout of dgx_mesh, blocks in element order.  A face takes the larger order of its two volumes
- * (compute_face_reference_order, face_solver.c:123-131) and its geometry from the lower-order side
+/* Variable-order mesh (2-D QUAD, Euler and Navier-Stokes): element e has order p[e] and its state block
+ * nv x (p[e]+1)^2 in the layout of dgx_mesh, blocks in element order.  A face takes the larger order of its
+ * two volumes (compute_face_reference_order, face_solver.c:123-131) and its geometry from the lower-order side
  * (get_dominant_geom_vol_side_index, face_solver.c:95-107).  The context must be created with
  * dgx_config.p = the largest order allowed.  It then serves dgx_compute_rhs, dgx_compute_rhs_f32,
- * dgx_compute_rhs_explicit, dgx_ssp_rk3_step and dgx_sizes (ndof = the total); its other entry points return
- * DGX_ERR_UNSUPPORTED.  Replaces the per-(type, p) volume/face loops of compute_volume_rlhs_dg /
- * compute_face_rlhs_dg for meshes whose volumes differ in p_ref.  Synchronous. */
+ * dgx_compute_rhs_explicit, dgx_ssp_rk3_step, dgx_graph_create_rhs (FP64 / FP32: the face kernel between
+ * orders and every order group's launches in one graph) and dgx_sizes (ndof = the total); its other entry
+ * points return DGX_ERR_UNSUPPORTED.  Navier-Stokes: interior (periodic) faces only.  Replaces the per-(type,
+ * p) volume/face loops of compute_volume_rlhs_dg / compute_face_rlhs_dg for meshes whose volumes differ in
+ * p_ref.  Synchronous. */
 dgx_status dgx_upload_mesh_hp(dgx_ctx* ctx, const dgx_mesh_hp* mesh);
 /* gmsh 2.2 ASCII reader for 2-D QUAD meshes (mesh/mesh_readers.c, mesh_connectivity.c:285-465, mesh_periodic.c):
  * volumes in file order, $Periodic node correspondences, boundary tags from the line elements; geometry is

@@ -288,3 +288,37 @@ def test_hp_ns_explicit_path_and_uniform_equivalence(dev):
     assert torch.equal(rh, ru)
     ch.close()
     cu.close()
+
+
+def test_hp_graph_replays_equal_eager_calls(dev):
+    from dpgsolver_amd import _lib
+
+    n = (6, 5)
+    pe = M.config4_hp_orders(n, seed=21)
+    mesh = M.structured_hp_mesh(pe, n, (0, 0), (1, 1), (True, True), curve=_curve)
+    for pde in (M.PDE_EULER, M.PDE_NAVIER_STOKES):
+        ctx = _lib.Context(2, 5, pde, M.FLUX_LF, mu=MU, Pr=PR)
+        ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        ctx.upload_mesh_hp(mesh)
+        u64 = torch.from_numpy(M.hp_state(mesh, _state_fn(5))).to(dev)
+        u32 = u64.float()
+        outs = {_lib.GRAPH_RHS_FP64: torch.empty_like(u64), _lib.GRAPH_RHS_FP32: torch.empty_like(u32)}
+        ins = {_lib.GRAPH_RHS_FP64: u64, _lib.GRAPH_RHS_FP32: u32}
+        eager = {_lib.GRAPH_RHS_FP64: ctx.compute_rhs, _lib.GRAPH_RHS_FP32: ctx.compute_rhs_f32}
+        graphs = {k: ctx.graph_rhs(k, ins[k].data_ptr(), outs[k].data_ptr()) for k in outs}
+        for rep in range(3):
+            for k, g in graphs.items():
+                ref = torch.empty_like(outs[k])
+                eager[k](ins[k].data_ptr(), ref.data_ptr())
+                torch.cuda.synchronize()
+                outs[k].fill_(np.nan)
+                g.launch(1)
+                g.sync()
+                assert torch.equal(outs[k], ref), (pde, k, rep)
+            u64.mul_(1.0 + 1e-3)
+            u32.copy_(u64)
+        with pytest.raises(_lib.DgxError, match="UNSUPPORTED"):
+            ctx.graph_rhs(_lib.GRAPH_RHS_MIXED, u64.data_ptr(), outs[_lib.GRAPH_RHS_FP32].data_ptr())
+        for g in graphs.values():
+            g.close()
+        ctx.close()
