@@ -170,16 +170,18 @@ __global__ void __launch_bounds__(128) k_relaxed_update(double* u, const double*
 // ------------------------------------------------------------------------------- BLAS-1 (deterministic)
 constexpr int kChunks = 64;
 
-// partial[j*kChunks + c] = sum over chunk c of V_j . w   (j < k)
-__global__ void __launch_bounds__(256) k_dots_partial(const double* __restrict__ V, size_t ldv,
+// partial[j*kChunks + c] = sum over chunk c of V_j . w   (j < k); V in FP64, or the FP32 Krylov basis (the
+// products and sums in FP64)
+template <typename TV>
+__global__ void __launch_bounds__(256) k_dots_partial(const TV* __restrict__ V, size_t ldv,
                                                       const double* __restrict__ w, size_t n, double* partial)
 {
 	const int j = blockIdx.y, c = blockIdx.x;
 	const size_t chunk = (n + kChunks - 1) / kChunks;
 	const size_t lo = (size_t)c * chunk, hi = (lo + chunk < n) ? lo + chunk : n;
-	const double* vj = V + (size_t)j * ldv;
+	const TV* vj = V + (size_t)j * ldv;
 	double s = 0.0;
-	for (size_t i = lo + threadIdx.x; i < hi; i += blockDim.x) s += vj[i] * w[i];
+	for (size_t i = lo + threadIdx.x; i < hi; i += blockDim.x) s += (double)vj[i] * w[i];
 	__shared__ double red[256];
 	red[threadIdx.x] = s;
 	__syncthreads();
@@ -216,7 +218,8 @@ __device__ inline double sum_partials(const double* __restrict__ partial, int j)
 	return s;
 }
 // w -= V h, h reduced from the partials of V^T w by each workgroup (k <= 256)
-__global__ void __launch_bounds__(256) k_gemv_sub_partials(const double* __restrict__ V, size_t ldv,
+template <typename TV>
+__global__ void __launch_bounds__(256) k_gemv_sub_partials(const TV* __restrict__ V, size_t ldv,
                                                            const double* __restrict__ partial, int k, double* w, size_t n)
 {
 	__shared__ double h[256];
@@ -224,7 +227,7 @@ __global__ void __launch_bounds__(256) k_gemv_sub_partials(const double* __restr
 	__syncthreads();
 	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
 		double s = 0.0;
-		for (int j = 0; j < k; ++j) s += V[(size_t)j * ldv + i] * h[j];
+		for (int j = 0; j < k; ++j) s += (double)V[(size_t)j * ldv + i] * h[j];
 		w[i] -= s;
 	}
 }
@@ -241,6 +244,22 @@ __global__ void k_scale_by_rnorm_partials(const double* __restrict__ x, const do
 	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
 		y[i] = a * x[i];
 }
+// the next Krylov basis vector: y64 = x / |x| (FP64, the preconditioner's input) and its FP32 copy y32
+__global__ void k_scale_by_rnorm_partials2(const double* __restrict__ x, const double* __restrict__ partial,
+                                           float* y32, double* y64, size_t n)
+{
+	__shared__ double a;
+	if (threadIdx.x == 0) {
+		const double nv = sum_partials(partial, 0);
+		a = nv > 0.0 ? 1.0 / sqrt(nv) : 0.0;
+	}
+	__syncthreads();
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+		const double v = a * x[i];
+		y64[i] = v;
+		y32[i] = (float)v;
+	}
+}
 // y = x / sqrt(nrm2[0])  (normalisation by a device-resident squared norm)
 __global__ void k_scale_by_rnorm(const double* __restrict__ x, const double* __restrict__ nrm2, double* y, size_t n)
 {
@@ -256,12 +275,13 @@ __global__ void k_axpby(double a, const double* __restrict__ x, double b, double
 		y[i] = a * x[i] + (b == 0.0 ? 0.0 : b * y[i]);
 }
 // y = V c (k vectors), accumulate
-__global__ void k_gemv_add(const double* __restrict__ V, size_t ldv, const double* __restrict__ c, int k, double* y,
+template <typename TV>
+__global__ void k_gemv_add(const TV* __restrict__ V, size_t ldv, const double* __restrict__ c, int k, double* y,
                            size_t n)
 {
 	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
 		double s = 0.0;
-		for (int j = 0; j < k; ++j) s += V[(size_t)j * ldv + i] * c[j];
+		for (int j = 0; j < k; ++j) s += (double)V[(size_t)j * ldv + i] * c[j];
 		y[i] += s;
 	}
 }
@@ -296,7 +316,7 @@ struct Blas {
 	// device out[0..k) = V^T w (deterministic two-stage reduction, no host round trip)
 	hipError_t dots_dev(const double* V, size_t ldv, int k, const double* w, size_t n, double* out)
 	{
-		hipLaunchKernelGGL(k_dots_partial, dim3(kChunks, k), dim3(256), 0, s, V, ldv, w, n, partial);
+		hipLaunchKernelGGL(k_dots_partial<double>, dim3(kChunks, k), dim3(256), 0, s, V, ldv, w, n, partial);
 		hipLaunchKernelGGL(k_dots_final, dim3((k + 63) / 64), dim3(64), 0, s, partial, k, out);
 		return hipGetLastError();
 	}
@@ -308,16 +328,23 @@ struct Blas {
 		return hipStreamSynchronize(s);
 	}
 	// partials only (rows 0..k), into buf
-	hipError_t dots_partials(const double* V, size_t ldv, int k, const double* w, size_t n, double* buf)
+	template <typename TV>
+	hipError_t dots_partials(const TV* V, size_t ldv, int k, const double* w, size_t n, double* buf)
 	{
-		hipLaunchKernelGGL(k_dots_partial, dim3(kChunks, k), dim3(256), 0, s, V, ldv, w, n, buf);
+		hipLaunchKernelGGL(k_dots_partial<TV>, dim3(kChunks, k), dim3(256), 0, s, V, ldv, w, n, buf);
 		return hipGetLastError();
 	}
 	// w -= V h, h = the coefficients held as partials in buf (one workgroup per CU, each reduces them once)
-	hipError_t gemv_sub_partials(const double* V, size_t ldv, const double* buf, int k, double* w, size_t n)
+	template <typename TV>
+	hipError_t gemv_sub_partials(const TV* V, size_t ldv, const double* buf, int k, double* w, size_t n)
 	{
 		const int grid = (int)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024);
-		hipLaunchKernelGGL(k_gemv_sub_partials, dim3(grid), dim3(256), 0, s, V, ldv, buf, k, w, n);
+		hipLaunchKernelGGL(k_gemv_sub_partials<TV>, dim3(grid), dim3(256), 0, s, V, ldv, buf, k, w, n);
+		return hipGetLastError();
+	}
+	hipError_t scale_by_rnorm_partials2(const double* x, const double* buf, float* y32, double* y64, size_t n)
+	{
+		hipLaunchKernelGGL(k_scale_by_rnorm_partials2, dim3(grid_for(n)), dim3(256), 0, s, x, buf, y32, y64, n);
 		return hipGetLastError();
 	}
 	hipError_t scale_by_rnorm_partials(const double* x, const double* buf, double* y, size_t n)
@@ -342,10 +369,11 @@ struct Blas {
 		hipLaunchKernelGGL(k_gemv_sub, dim3(grid_for(n)), dim3(256), 0, s, V, ldv, dvec, k, w, n);
 		return hipGetLastError();
 	}
-	hipError_t gemv_add(const double* V, size_t ldv, const double* cc, int k, double* y, size_t n)
+	template <typename TV>
+	hipError_t gemv_add(const TV* V, size_t ldv, const double* cc, int k, double* y, size_t n)
 	{
 		DGX_HIP_CHECK(hipMemcpyAsync(dvec, cc, sizeof(double) * k, hipMemcpyHostToDevice, s));
-		hipLaunchKernelGGL(k_gemv_add, dim3(grid_for(n)), dim3(256), 0, s, V, ldv, dvec, k, y, n);
+		hipLaunchKernelGGL(k_gemv_add<TV>, dim3(grid_for(n)), dim3(256), 0, s, V, ldv, dvec, k, y, n);
 		return hipGetLastError();
 	}
 	hipError_t axpby(double a, const double* x, double b, double* y, size_t n)
@@ -610,7 +638,12 @@ __global__ void k_arnoldi_start(double* g, int m, double beta, double* st)
 struct Gmres {
 	int m = 60;
 	static constexpr int kCheck = 4;   // Arnoldi steps between host convergence checks
-	double* V = nullptr;  // (m+1) x n
+	// Krylov basis (m+1) x n in FP32: the Gram-Schmidt passes read it twice per iteration, and FP32 halves
+	// those bytes; every product and sum stays in FP64, and so do w, x and the newest basis vector vcur (the
+	// preconditioner's input).  The Arnoldi relation then holds to FP32 rounding (~1e-7 relative), far below the
+	// inner tolerance; the outer Newton residual decides convergence in FP64.
+	float* V = nullptr;
+	double* vcur = nullptr;
 	double* w = nullptr;
 	double* z = nullptr;
 	double* x = nullptr;
@@ -622,7 +655,8 @@ struct Gmres {
 	{
 		m = restart;
 		n = (size_t)c.E * c.Np * c.nv;
-		DGX_HIP_CHECK(hipMalloc(&V, sizeof(double) * (size_t)(m + 1) * n));
+		DGX_HIP_CHECK(hipMalloc(&V, sizeof(float) * (size_t)(m + 1) * n));
+		DGX_HIP_CHECK(hipMalloc(&vcur, sizeof(double) * n));
 		DGX_HIP_CHECK(hipMalloc(&w, sizeof(double) * n));
 		DGX_HIP_CHECK(hipMalloc(&z, sizeof(double) * n));
 		DGX_HIP_CHECK(hipMalloc(&x, sizeof(double) * n));
@@ -636,9 +670,11 @@ struct Gmres {
 	}
 	void release()
 	{
-		for (double* p : { V, w, z, x, dls })
+		if (V) (void)hipFree(V);
+		V = nullptr;
+		for (double* p : { vcur, w, z, x, dls })
 			if (p) (void)hipFree(p);
-		V = w = z = x = dls = nullptr;
+		vcur = w = z = x = dls = nullptr;
 		blas.release();
 	}
 	size_t ls_size() const { return (size_t)(m + 1) * m + 2 * (size_t)m + (m + 1) + 3; }
@@ -672,21 +708,22 @@ struct Gmres {
 		while (total < max_it) {
 			// r = b - A x  (x = 0 on the first cycle)
 			if (total == 0) {
-				DGX_HIP_CHECK(hipMemcpyAsync(V, b, sizeof(double) * n, hipMemcpyDeviceToDevice, c.stream));
+				DGX_HIP_CHECK(hipMemcpyAsync(vcur, b, sizeof(double) * n, hipMemcpyDeviceToDevice, c.stream));
 			} else {
 				DGX_HIP_CHECK(op(c, u, inv_dt, x, w));
-				DGX_HIP_CHECK(blas.axpby(1.0, b, 0.0, V, n));
-				DGX_HIP_CHECK(blas.axpby(-1.0, w, 1.0, V, n));
+				DGX_HIP_CHECK(blas.axpby(1.0, b, 0.0, vcur, n));
+				DGX_HIP_CHECK(blas.axpby(-1.0, w, 1.0, vcur, n));
 			}
 			double beta;
-			DGX_HIP_CHECK(blas.norm(V, n, &beta));
+			DGX_HIP_CHECK(blas.norm(vcur, n, &beta));
 			if (beta <= tol) { *converged = true; break; }
-			DGX_HIP_CHECK(blas.axpby(1.0 / beta, V, 0.0, V, n));
+			DGX_HIP_CHECK(blas.axpby(1.0 / beta, vcur, 0.0, vcur, n));
+			DGX_HIP_CHECK(launch_convert(vcur, V, n, c.stream));
 			hipLaunchKernelGGL(k_arnoldi_start, dim3((m + 64) / 64), dim3(64), 0, c.stream, dg, m, beta, dst);
 			DGX_HIP_CHECK(hipGetLastError());
 			int j = 0, jconv = -1;
 			for (; j < m && total < max_it; ++j, ++total) {
-				DGX_HIP_CHECK(P.apply(c, V + (size_t)j * n, z));
+				DGX_HIP_CHECK(P.apply(c, vcur, z));   // vcur = v_j in FP64
 				DGX_HIP_CHECK(op(c, u, inv_dt, z, w));
 				// classical Gram-Schmidt, twice (CGS2); the coefficients stay as partial sums on the device and
 				// each consumer reduces them
@@ -695,7 +732,7 @@ struct Gmres {
 				DGX_HIP_CHECK(blas.dots_partials(V, n, j + 1, w, n, blas.partial2));
 				DGX_HIP_CHECK(blas.gemv_sub_partials(V, n, blas.partial2, j + 1, w, n));
 				DGX_HIP_CHECK(blas.dots_partials(w, 0, 1, w, n, blas.partialn));
-				DGX_HIP_CHECK(blas.scale_by_rnorm_partials(w, blas.partialn, V + (size_t)(j + 1) * n, n));
+				DGX_HIP_CHECK(blas.scale_by_rnorm_partials2(w, blas.partialn, V + (size_t)(j + 1) * n, vcur, n));
 				hipLaunchKernelGGL(k_arnoldi_step, dim3(1), dim3(64), 0, c.stream, dH, dcs, dsn, dg, blas.partial,
 				                   blas.partial2, blas.partialn, j, m, tol, dst);
 				DGX_HIP_CHECK(hipGetLastError());
@@ -717,7 +754,7 @@ struct Gmres {
 				for (int k = i + 1; k < jj; ++k) sacc -= Hh[(size_t)k * (m + 1) + i] * y[k];
 				y[i] = sacc / Hh[(size_t)i * (m + 1) + i];
 			}
-			DGX_HIP_CHECK(blas.axpby(0.0, V, 0.0, w, n));
+			DGX_HIP_CHECK(blas.axpby(0.0, b, 0.0, w, n));
 			DGX_HIP_CHECK(blas.gemv_add(V, n, y.data(), jj, w, n));
 			DGX_HIP_CHECK(P.apply(c, w, z));
 			DGX_HIP_CHECK(blas.axpby(1.0, z, 1.0, x, n));
