@@ -180,7 +180,7 @@ def time_to_steady(torch, dev, epochs=60, cpu=True):
     Legs: exact block-Jacobi (FP64 inverse of every diagonal block); the learned preconditioner (Kronecker
     rank 8, trained here on the diagonal blocks captured from the block-Jacobi run) with the exact
     block-Jacobi as fallback when GMRES has not converged after 50 iterations; multicolour block symmetric
-    Gauss-Seidel over the diagonal and face blocks."""
+    Gauss-Seidel over the diagonal and face blocks, with the blocks stored in FP64 and in FP32."""
     from dpgsolver_amd import _lib, mesh as M
 
     mesh = M.channel_mesh(64, 64, 3)
@@ -231,11 +231,17 @@ def time_to_steady(torch, dev, epochs=60, cpu=True):
     st_sgs, u_sgs = solve(_lib.PC_BLOCK_SGS)
     print(f"block-SGS: {st_sgs['seconds']:.3f} s, newton {st_sgs['newton_steps']}, gmres {st_sgs['gmres_iters']}",
           file=sys.stderr, flush=True)
+    st_s32, u_s32 = solve(_lib.PC_BLOCK_SGS32)
+    print(f"block-SGS (FP32 blocks): {st_s32['seconds']:.3f} s, newton {st_s32['newton_steps']}, "
+          f"gmres {st_s32['gmres_iters']}", file=sys.stderr, flush=True)
     diff = ((u_ln - u_bj).abs().max() / u_bj.abs().max()).item()
     out = {"workload": "2-D Euler DG p=3, 64x64 curved-bump channel, Mach 0.5, Roe; Newton-GMRES(60) to 1e-10",
            "block_jacobi": leg(st_bj), "learned_with_fallback": leg(st_ln), "block_sgs": leg(st_sgs),
            "sgs_vs_bj_wall": round(st_sgs["seconds"] / st_bj["seconds"], 4),
            "sgs_steady_states_rel_diff": ((u_sgs - u_bj).abs().max() / u_bj.abs().max()).item(),
+           "block_sgs_fp32_blocks": leg(st_s32),
+           "sgs32_vs_bj_wall": round(st_s32["seconds"] / st_bj["seconds"], 4),
+           "sgs32_steady_states_rel_diff": ((u_s32 - u_bj).abs().max() / u_bj.abs().max()).item(),
            "training": {"blocks": n, "epochs": epochs, "seconds": round(t_train, 3), "loss_untrained": l0,
                         "loss_trained": loss, "structure": "Kronecker rank 8, MLP 96-256-256-2176 BF16/FP32"},
            "learned_vs_bj_wall": round(st_ln["seconds"] / st_bj["seconds"], 4),

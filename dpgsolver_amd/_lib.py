@@ -32,7 +32,7 @@ EXPORTS = [
     "dgx_mlp_features", "dgx_mlp_loss_grad", "dgx_mlp_train", "dgx_mlp_predict", "dgx_attach_mlp",
 ]
 
-PC_NONE, PC_BLOCK_JACOBI, PC_LEARNED, PC_BLOCK_SGS = 0, 1, 2, 3
+PC_NONE, PC_BLOCK_JACOBI, PC_LEARNED, PC_BLOCK_SGS, PC_BLOCK_SGS32 = 0, 1, 2, 3, 4
 HISTORY_MAX = 256
 
 

@@ -446,8 +446,10 @@ __device__ inline int sgs_face_node(int lf, int a, int N)
 // compactly, transposed, on the nfv = nv N face-node rows and columns (the MODE_OFFD layout), so a face
 // costs an nfv x nfv mat-vec on the neighbour's trace.  Elements of one colour share no face, so their
 // updates are independent.  One 64-lane workgroup per element; nb <= 64 and 4 nfv <= 64.
+// TB: the stored blocks' type (FP64, or FP32 with the products and sums in FP64).
+template <typename TB>
 __global__ void __launch_bounds__(64) k_bgs_color(const int* __restrict__ elems, const ElemFace* __restrict__ ef,
-                                                  const double* __restrict__ invT, const double* __restrict__ offT,
+                                                  const TB* __restrict__ invT, const TB* __restrict__ offT,
                                                   const double* __restrict__ rhs, double* z, int N, int nv)
 {
 	__shared__ double sz[64], sf[64], sr[64];
@@ -461,9 +463,9 @@ __global__ void __launch_bounds__(64) k_bgs_color(const int* __restrict__ elems,
 	__syncthreads();
 	if (t < 4 * nfv) {   // face contributions on the element's face-node rows
 		const int lf = t / nfv, row = t - lf * nfv;
-		const double* C = offT + ((size_t)e * 4 + lf) * nfv * nfv;
+		const TB* C = offT + ((size_t)e * 4 + lf) * nfv * nfv;
 		double acc = 0.0;
-		for (int c = 0; c < nfv; ++c) acc += C[(size_t)c * nfv + row] * sz[lf * nfv + c];
+		for (int c = 0; c < nfv; ++c) acc += (double)C[(size_t)c * nfv + row] * sz[lf * nfv + c];
 		sf[t] = acc;
 	}
 	__syncthreads();
@@ -478,9 +480,9 @@ __global__ void __launch_bounds__(64) k_bgs_color(const int* __restrict__ elems,
 	}
 	__syncthreads();
 	if (t < nb) {
-		const double* M = invT + (size_t)e * nb * nb;
+		const TB* M = invT + (size_t)e * nb * nb;
 		double acc = 0.0;
-		for (int k = 0; k < nb; ++k) acc += M[(size_t)k * nb + t] * sr[k];
+		for (int k = 0; k < nb; ++k) acc += (double)M[(size_t)k * nb + t] * sr[k];
 		z[(size_t)e * nb + t] = acc;
 	}
 }
@@ -545,9 +547,15 @@ struct PrecondBlockJacobi : Precond {
 struct PrecondBlockSGS : Precond {
 	PrecondBlockJacobi diag;   // assembles and inverts A_ee - M_e/dt
 	double* offT = nullptr;    // E * 4 * nfv^2 compact transposed face blocks
+	bool fp32 = false;         // sweep over FP32 copies of the inverse and face blocks
+	float* inv32 = nullptr;
+	float* off32 = nullptr;
+	explicit PrecondBlockSGS(bool use_fp32 = false) : fp32(use_fp32) {}
 	~PrecondBlockSGS() override
 	{
 		if (offT) (void)hipFree(offT);
+		if (inv32) (void)hipFree(inv32);
+		if (off32) (void)hipFree(off32);
 	}
 	hipError_t setup(Ctx& c, const double* u, const double* inv_dt) override
 	{
@@ -556,14 +564,23 @@ struct PrecondBlockSGS : Precond {
 		DGX_HIP_CHECK(diag.setup(c, u, inv_dt));
 		if (!offT) DGX_HIP_CHECK(hipMalloc(&offT, sizeof(double) * (size_t)c.E * 4 * nfv * nfv));
 		for (int col = 0; col < nfv; ++col) DGX_HIP_CHECK(launch_offd_col(c, u, offT, col));
-		return hipSuccess;
+		if (!fp32) return hipSuccess;
+		const size_t ni = (size_t)c.E * nb * nb, no = (size_t)c.E * 4 * nfv * nfv;
+		if (!inv32) DGX_HIP_CHECK(hipMalloc(&inv32, sizeof(float) * ni));
+		if (!off32) DGX_HIP_CHECK(hipMalloc(&off32, sizeof(float) * no));
+		DGX_HIP_CHECK(launch_convert(diag.blocks, inv32, ni, c.stream));
+		return launch_convert(offT, off32, no, c.stream);
 	}
 	hipError_t sweep(Ctx& c, const double* v, double* z, int k)
 	{
 		const int n0 = c.color_off[k], cnt = c.color_off[k + 1] - n0;
 		if (cnt == 0) return hipSuccess;
-		hipLaunchKernelGGL(k_bgs_color, dim3(cnt), dim3(64), 0, c.stream, c.color_elems + n0, c.elem_faces,
-		                   diag.blocks, offT, v, z, c.N, c.nv);
+		if (fp32)
+			hipLaunchKernelGGL(k_bgs_color<float>, dim3(cnt), dim3(64), 0, c.stream, c.color_elems + n0, c.elem_faces,
+			                   inv32, off32, v, z, c.N, c.nv);
+		else
+			hipLaunchKernelGGL(k_bgs_color<double>, dim3(cnt), dim3(64), 0, c.stream, c.color_elems + n0, c.elem_faces,
+			                   diag.blocks, offT, v, z, c.N, c.nv);
 		return hipGetLastError();
 	}
 	hipError_t apply(Ctx& c, const double* v, double* z) override
@@ -780,6 +797,7 @@ extern "C" dgx_status dgx_precond_apply(dgx_ctx* h, int precond, const double* u
 	if (precond == DGX_PC_NONE) P = new dgx::PrecondIdentity();
 	else if (precond == DGX_PC_BLOCK_JACOBI) P = new dgx::PrecondBlockJacobi();
 	else if (precond == DGX_PC_BLOCK_SGS) P = new dgx::PrecondBlockSGS();
+	else if (precond == DGX_PC_BLOCK_SGS32) P = new dgx::PrecondBlockSGS(true);
 	else if (precond == DGX_PC_LEARNED) P = dgx::make_learned_precond(c);
 	if (!P) return precond == DGX_PC_LEARNED ? DGX_ERR_UNSUPPORTED : DGX_ERR_ARG;
 	hipError_t e = P->setup(c, u, inv_dt);
@@ -823,6 +841,7 @@ extern "C" dgx_status dgx_solve_implicit(dgx_ctx* h, double* u, const dgx_implic
 			P = bj;
 		}
 		else if (prm->precond == DGX_PC_BLOCK_SGS) P = new dgx::PrecondBlockSGS();
+		else if (prm->precond == DGX_PC_BLOCK_SGS32) P = new dgx::PrecondBlockSGS(true);
 		else if (prm->precond == DGX_PC_LEARNED) {
 			P = dgx::make_learned_precond(c);
 			if (prm->learned_fallback_iters > 0) Pfb = new dgx::PrecondBlockJacobi();

@@ -233,8 +233,10 @@ dgx_status dgx_block_inv_apply(dgx_ctx* ctx, const double* inv, const double* x,
  * dR/du - M/dt.  DGX_PC_BLOCK_SGS (nb <= 64): one forward and one backward multicolour block Gauss-Seidel sweep
  * over the diagonal and face blocks from z = 0, the coupled-block counterpart of the reference's default
  * PETSc ILU(1) with RCM ordering (external/petsc/options_files/petsc_options_gmres_default.txt:6-8), which
- * factors the assembled matrix sequentially.  DGX_PC_LEARNED: the MLP below. */
-enum { DGX_PC_NONE = 0, DGX_PC_BLOCK_JACOBI = 1, DGX_PC_LEARNED = 2, DGX_PC_BLOCK_SGS = 3 };
+ * factors the assembled matrix sequentially.  DGX_PC_BLOCK_SGS32: the same sweeps with the inverse diagonal
+ * blocks and the face blocks stored in FP32 (half the bytes the sweeps read; products and sums in FP64, the
+ * vectors FP64).  DGX_PC_LEARNED: the MLP below. */
+enum { DGX_PC_NONE = 0, DGX_PC_BLOCK_JACOBI = 1, DGX_PC_LEARNED = 2, DGX_PC_BLOCK_SGS = 3, DGX_PC_BLOCK_SGS32 = 4 };
 /* z = P^-1 v for one preconditioner set up at state u with the pseudo-time scale inv_dt (E doubles, as
  * dgx_cfl_dt writes it): the operator that dgx_solve_implicit applies inside GMRES, exposed for callers that
  * run their own Krylov solver.  All pointers device, v and z distinct; synchronous. */

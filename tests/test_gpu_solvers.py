@@ -271,6 +271,13 @@ def test_preconditioner_apply_matches_numpy(dev, case):
                 z[e] = np.linalg.solve(D[e], r)
     got = zs.cpu().numpy()
     assert _rel(got, z.ravel()) < 1e-11
+    # the same sweeps over FP32 copies of the blocks (products in FP64): FP32 storage rounding only
+    z32 = torch.empty_like(ud)
+    ctx.precond_apply(_lib.PC_BLOCK_SGS32, ud.data_ptr(), inv_dt.data_ptr(), vd.data_ptr(), z32.data_ptr())
+    torch.cuda.synchronize()
+    e32 = _rel(z32.cpu().numpy(), z.ravel())
+    print(f"{name}: FP32-stored SGS vs FP64 sweeps {e32:.2e}")
+    assert 0 < e32 < 1e-5
     A = np.zeros((mesh.E * nb, mesh.E * nb))
     for e in range(mesh.E):
         A[e * nb:(e + 1) * nb, e * nb:(e + 1) * nb] += D[e]
@@ -359,14 +366,15 @@ def test_steady_channel_newton_gmres_block_jacobi(dev):
 
 def test_steady_channel_block_sgs_fewer_iterations(dev):
     """The multicolour block SGS preconditioner reaches the same steady state as block-Jacobi with fewer
-    GMRES iterations (it carries the face coupling that block-Jacobi drops)."""
+    GMRES iterations (it carries the face coupling that block-Jacobi drops); so does its variant with the
+    blocks stored in FP32."""
     from dpgsolver_amd import _lib
 
     mesh = M.channel_mesh(12, 6, 2)
     prim = M.freestream_prim(2)
     ctx = _ctx(mesh, M.PDE_EULER, M.FLUX_ROE, bc_prim=prim)
     out = {}
-    for pc in (_lib.PC_BLOCK_JACOBI, _lib.PC_BLOCK_SGS):
+    for pc in (_lib.PC_BLOCK_JACOBI, _lib.PC_BLOCK_SGS, _lib.PC_BLOCK_SGS32):
         ud = torch.from_numpy(M.uniform_state(mesh, prim)).to(dev)
         st = ctx.solve_implicit(ud.data_ptr(), cfl_initial=10.0, exit_tol=1e-10, max_newton=80, precond=pc)
         print("pc", pc, "newton", st["newton_steps"], "gmres", st["gmres_iters"], st["seconds"])
@@ -375,4 +383,7 @@ def test_steady_channel_block_sgs_fewer_iterations(dev):
     assert out[_lib.PC_BLOCK_SGS][0]["gmres_iters"] < out[_lib.PC_BLOCK_JACOBI][0]["gmres_iters"]
     a, b = out[_lib.PC_BLOCK_SGS][1], out[_lib.PC_BLOCK_JACOBI][1]
     assert _rel(a, b) < 1e-7
+    # FP32-stored factors: the same steady state, about as few iterations as the FP64 sweeps
+    assert _rel(out[_lib.PC_BLOCK_SGS32][1], b) < 1e-7
+    assert out[_lib.PC_BLOCK_SGS32][0]["gmres_iters"] <= 1.1 * out[_lib.PC_BLOCK_SGS][0]["gmres_iters"] + 5
     ctx.close()
