@@ -330,7 +330,7 @@ def hp_ns_throughput(torch, dev, steps, warmup):
     """2-D Navier-Stokes (BR2) residual on the variable-order workload: the 500 x 300 periodic QUAD base grid
     of [0,1]^2 with orders p ~ U{1..5} per element (seeded; the TRI split is not built), Roe, Pr 0.72, Re 100
     (mu = 0.0028).  One evaluation = every order group's gradient kernel, the face kernel between orders and
-    every group's residual kernel, replayed as one HIP graph (FP64 and FP32)."""
+    every group's residual kernel, replayed as one HIP graph (FP64, FP32 and the mixed form)."""
     import numpy as np
     from dpgsolver_amd import _lib, mesh as M
 
@@ -361,6 +361,11 @@ def hp_ns_throughput(torch, dev, steps, warmup):
     ms = timed(g)
     ms32 = timed(g32)
     err32 = ((r32.double() - r).abs().max() / r.abs().max()).item()
+    rmx = torch.empty_like(u32)
+    gmx = ctx.graph_rhs(_lib.GRAPH_RHS_MIXED, u.data_ptr(), rmx.data_ptr())
+    msmx = timed(gmx)
+    errmx = ((rmx.double() - r).abs().max() / r.abs().max()).item()
+    gmx.close()
     ur = u32.double()
     rr = torch.empty_like(u)
     ctx.compute_rhs(ur.data_ptr(), rr.data_ptr())
@@ -390,7 +395,9 @@ def hp_ns_throughput(torch, dev, steps, warmup):
                      "rel_err_vs_fp64": err32, "input_rounding_rel_err": err_in,
                      "roofline": {"bound": "hbm", "achieved": round(alg32 / (ms32 * 1e-3) / 1e9, 1),
                                   "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                  "frac": round(alg32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}}
+                                  "frac": round(alg32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}},
+            "mixed": {"value": round(ndof / (msmx * 1e-3) / 1e9, 3), "ms_per_step": round(msmx, 5),
+                      "rel_err_vs_fp64": errmx}}
 
 
 def main():

@@ -40,7 +40,7 @@ void free_hp(Ctx& c)
 	if (!c.hp) return;
 	dgx::HpData* H = c.hp;
 	for (dgx_ctx* g : H->groups) dgx_destroy(g);
-	release(H->faces); release(H->nJ); release(H->nJ32); release(H->z64); release(H->z32);
+	release(H->faces); release(H->nJ); release(H->nJ32); release(H->nJhi); release(H->z64); release(H->z32);
 	release(H->ops64); release(H->ops32);
 	delete H;
 	c.hp = nullptr;
@@ -186,9 +186,13 @@ dgx_status upload_uniform(dgx_ctx* h, const dgx_mesh* m, bool nonconf_ok)
 			R = dgx::ElemFace{ f, eL, fL | (1 << 4) | (ord << 8), 0 };
 		}
 	}
-	for (const auto& x : ef)
+	bool nonconf = false;
+	for (const auto& x : ef) {
 		if (x.f < 0) return DGX_ERR_ARG;  // open mesh: every local face needs a face record
+		if (((x.packed >> 12) & 15) == dgx::kBcNonconforming) nonconf = true;
+	}
 	free_mesh(c);
+	c.has_nonconforming = nonconf;
 	c.E = m->E;
 	c.F = m->F;
 	const size_t nx = (size_t)c.E * d * c.Np, nm = (size_t)c.E * d * d * c.Nc, nj = (size_t)c.E * c.Nc,
@@ -373,6 +377,30 @@ dgx_status hp_rhs32(Ctx& c, const float* u, float* r)
 	return DGX_OK;
 }
 
+// Mixed form: FP64 state in, FP32 residual; the faces between orders in FP64, the groups' mixed kernels.
+dgx_status hp_rhs_mixed(Ctx& c, const double* u, float* r)
+{
+	dgx::HpData& H = *c.hp;
+	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) {
+		for (size_t g = 0; g < H.groups.size(); ++g) {
+			Ctx& s = H.groups[g]->c;
+			s.stream = c.stream;
+			DGX_TRY(dgx::launch_rhs_ns_mixed_phase(s, u + H.u_off[g], r + H.u_off[g], 1));
+		}
+		DGX_TRY(dgx::launch_hp_faces_mixed(c, u));
+		for (size_t g = 0; g < H.groups.size(); ++g)
+			DGX_TRY(dgx::launch_rhs_ns_mixed_phase(H.groups[g]->c, u + H.u_off[g], r + H.u_off[g], 2));
+		return DGX_OK;
+	}
+	DGX_TRY(dgx::launch_hp_faces_mixed(c, u));
+	for (size_t g = 0; g < H.groups.size(); ++g) {
+		Ctx& s = H.groups[g]->c;
+		s.stream = c.stream;
+		DGX_TRY(dgx::launch_rhs_mixed(s, u + H.u_off[g], r + H.u_off[g]));
+	}
+	return DGX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -409,7 +437,7 @@ dgx_status dgx_upload_mesh_hp(dgx_ctx* h, const dgx_mesh_hp* m)
 	// split the face table: faces inside one group, faces between groups
 	std::vector<std::vector<int>> gfaces(G);
 	std::vector<dgx::HpFace> hf;
-	std::vector<double> nj;
+	std::vector<double> nj, njh;   // n J_f of the face (lower-order side) and of the higher-order side's own face
 	double xg[dgx::kHpMaxN], wg[dgx::kHpMaxN], tmp[2 * dgx::kHpMaxN];
 	for (int f = 0; f < m->F; ++f) {
 		const int* r = m->faces + 6 * f;
@@ -444,17 +472,18 @@ dgx_status dgx_upload_mesh_hp(dgx_ctx* h, const dgx_mesh_hp* m)
 		hf.push_back(x);
 		// face geometry from the lower-order side, in the left ordering, left outward
 		dgx::gauss_legendre(Nf, xg, wg);
+		// (the higher-order side's own, used by the mixed form to cancel that element's constant part exactly)
 		const size_t base = nj.size();
 		nj.resize(base + 2 * dgx::kHpMaxN, 0.0);
-		if (NL < NR) {
-			outward_nj(m->xyz + xoff[eL], NL, fL, xg, Nf, nj.data() + base);
-		} else {
-			outward_nj(m->xyz + xoff[eR], NR, fR, xg, Nf, tmp);
-			for (int q = 0; q < Nf; ++q) {
-				const int qr = (ord & 2) ? Nf - 1 - q : q;
-				nj[base + 2 * q] = -tmp[2 * qr];
-				nj[base + 2 * q + 1] = -tmp[2 * qr + 1];
-			}
+		njh.resize(base + 2 * dgx::kHpMaxN, 0.0);
+		double* lo = (NL < NR) ? nj.data() + base : njh.data() + base;   // the left side's own geometry
+		double* ro = (NL < NR) ? njh.data() + base : nj.data() + base;   // the right side's, left orientation
+		outward_nj(m->xyz + xoff[eL], NL, fL, xg, Nf, lo);
+		outward_nj(m->xyz + xoff[eR], NR, fR, xg, Nf, tmp);
+		for (int q = 0; q < Nf; ++q) {
+			const int qr = (ord & 2) ? Nf - 1 - q : q;
+			ro[2 * q] = -tmp[2 * qr];
+			ro[2 * q + 1] = -tmp[2 * qr + 1];
 		}
 	}
 	// faces sorted by their (NL, NR) pair (stable): one kernel instance per pair, every wave on one pair
@@ -464,14 +493,16 @@ dgx_status dgx_upload_mesh_hp(dgx_ctx* h, const dgx_mesh_hp* m)
 	std::stable_sort(fperm.begin(), fperm.end(), [&](int a, int b) { return pair_key(a) < pair_key(b); });
 	{
 		std::vector<dgx::HpFace> hs(hf.size());
-		std::vector<double> ns(nj.size());
+		std::vector<double> ns(nj.size()), nh(njh.size());
 		for (size_t i = 0; i < hf.size(); ++i) {
 			hs[i] = hf[fperm[i]];
-			std::copy(nj.begin() + (size_t)fperm[i] * 2 * dgx::kHpMaxN, nj.begin() + ((size_t)fperm[i] + 1) * 2 * dgx::kHpMaxN,
-			          ns.begin() + i * 2 * dgx::kHpMaxN);
+			const size_t src = (size_t)fperm[i] * 2 * dgx::kHpMaxN, dst = i * 2 * dgx::kHpMaxN;
+			std::copy(nj.begin() + src, nj.begin() + src + 2 * dgx::kHpMaxN, ns.begin() + dst);
+			std::copy(njh.begin() + src, njh.begin() + src + 2 * dgx::kHpMaxN, nh.begin() + dst);
 		}
 		hf.swap(hs);
 		nj.swap(ns);
+		njh.swap(nh);
 	}
 	free_mesh(c);
 	c.hp = new (std::nothrow) dgx::HpData();
@@ -499,6 +530,7 @@ dgx_status dgx_upload_mesh_hp(dgx_ctx* h, const dgx_mesh_hp* m)
 	DGX_TRY(hipMalloc(&H.faces, sizeof(dgx::HpFace) * std::max<size_t>(hf.size(), 1)));
 	DGX_TRY(hipMalloc(&H.nJ, sizeof(double) * std::max<size_t>(nj.size(), 1)));
 	DGX_TRY(hipMalloc(&H.nJ32, sizeof(float) * std::max<size_t>(nj.size(), 1)));
+	DGX_TRY(hipMalloc(&H.nJhi, sizeof(double) * std::max<size_t>(njh.size(), 1)));
 	DGX_TRY(hipMalloc(&H.ops64, sizeof(dgx::HpOps<double>)));
 	DGX_TRY(hipMalloc(&H.ops32, sizeof(dgx::HpOps<float>)));
 	if (!hf.empty()) {
@@ -506,6 +538,7 @@ dgx_status dgx_upload_mesh_hp(dgx_ctx* h, const dgx_mesh_hp* m)
 		DGX_TRY(hipMemcpy(H.faces, hf.data(), sizeof(dgx::HpFace) * hf.size(), hipMemcpyHostToDevice));
 		DGX_TRY(hipMemcpy(H.nJ, nj.data(), sizeof(double) * nj.size(), hipMemcpyHostToDevice));
 		DGX_TRY(hipMemcpy(H.nJ32, nj32.data(), sizeof(float) * nj32.size(), hipMemcpyHostToDevice));
+		DGX_TRY(hipMemcpy(H.nJhi, njh.data(), sizeof(double) * njh.size(), hipMemcpyHostToDevice));
 	}
 	{
 		dgx::HpOps<double>* o64 = new dgx::HpOps<double>;
@@ -574,6 +607,7 @@ dgx_status dgx_compute_rhs_f32(dgx_ctx* h, const float* u, float* r)
 dgx_status dgx_compute_rhs_mixed(dgx_ctx* h, const double* u, float* r)
 {
 	if (!h || !u || !r) return DGX_ERR_ARG;
+	if (h->c.hp) return hp_rhs_mixed(h->c, u, r);
 	if (!h->c.mesh_ready) return h->c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
 	if (h->c.cfg.pde == DGX_PDE_NAVIER_STOKES) return hip_status(dgx::launch_rhs_ns_mixed(h->c, u, r));
 	if (h->c.cfg.pde != DGX_PDE_EULER) return DGX_ERR_UNSUPPORTED;
@@ -719,13 +753,14 @@ dgx_status dgx_graph_create_rhs(dgx_ctx* h, int kind, const void* u, void* r, dg
 	Ctx& c = h->c;
 	if (c.hp) {
 		// variable-order mesh: the face kernel between orders and every order group's launches in one graph
-		if (kind == DGX_GRAPH_RHS_MIXED) return DGX_ERR_UNSUPPORTED;
 		dgx_graph* g = new (std::nothrow) dgx_graph();
 		if (!g) return DGX_ERR_HIP;
 		const dgx_status st = capture_graph(h, g, [&]() -> hipError_t {
 			const dgx_status s = (kind == DGX_GRAPH_RHS_FP64)
 			    ? hp_rhs64(c, static_cast<const double*>(u), static_cast<double*>(r), dgx::MODE_RHS, nullptr)
-			    : hp_rhs32(c, static_cast<const float*>(u), static_cast<float*>(r));
+			    : (kind == DGX_GRAPH_RHS_FP32)
+			    ? hp_rhs32(c, static_cast<const float*>(u), static_cast<float*>(r))
+			    : hp_rhs_mixed(c, static_cast<const double*>(u), static_cast<float*>(r));
 			return s == DGX_OK ? hipSuccess : hipErrorLaunchFailure;
 		});
 		if (st != DGX_OK) {

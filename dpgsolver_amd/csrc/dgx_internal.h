@@ -80,6 +80,7 @@ struct HpData {
 	std::vector<HpPair> pairs;       // runs of faces with one (NL, NR)
 	HpFace* faces = nullptr;         // Fnc, sorted by (NL, NR)
 	double* nJ = nullptr;            // Fnc x kHpMaxN x 2: n J_f at the face GL points (left ordering, left outward)
+	double* nJhi = nullptr;          // the same from the higher-order side's own face (mixed form)
 	float* nJ32 = nullptr;
 	double* z64 = nullptr;           // lifted fluxes, FP64 and FP32 (each group: E_g x 4 x nv x N_g)
 	float* z32 = nullptr;
@@ -117,6 +118,7 @@ struct Ctx {
 	HpData* hp = nullptr;           // variable-order mesh (then this context holds no uniform mesh)
 	const double* hp_z64 = nullptr; // order group of a variable-order mesh: its part of the lifted fluxes
 	const float* hp_z32 = nullptr;
+	bool has_nonconforming = false; // some element face is marked kBcNonconforming (then hp_z* must be set)
 	double* metrics_vc = nullptr;   // E*d*d*Nc  (m[(i + d*j)*Nc + n] = J dxi_j/dx_i)
 	float* metrics_vc32 = nullptr;
 	double* detJ_vc = nullptr;      // E*Nc
@@ -168,6 +170,8 @@ hipError_t launch_hp_faces64(Ctx& c, const double* u);
 hipError_t launch_hp_faces32(Ctx& c, const float* u);
 hipError_t launch_rhs_ns_phase(Ctx& c, const double* u, double* r, int phase);
 hipError_t launch_rhs_ns32_phase(Ctx& c, const float* u, float* r, int phase);
+hipError_t launch_rhs_ns_mixed_phase(Ctx& c, const double* u, float* r, int phase);
+hipError_t launch_hp_faces_mixed(Ctx& c, const double* u);   // FP64 state and fluxes, FP32 lifted results
 hipError_t launch_minv(Ctx& c, double* r);
 hipError_t launch_positivity(Ctx& c, double* u, int* status);
 hipError_t launch_maxabs(Ctx& c, const double* r, size_t n, unsigned long long* dmax);

@@ -9,6 +9,7 @@ namespace dgx {
 template <int MODE>
 static hipError_t dispatch_dual(Ctx& c, const RhsArgs<Dual, double>& a)
 {
+	if (c.has_nonconforming) return hipErrorInvalidValue;   // no linearisation of faces between orders
 	const int d = c.d, p = c.p, pde = c.cfg.pde, flux = c.cfg.num_flux;
 #define DGX_DCASE(NN)                                                                                    \
 	if (d == 2 && p + 1 == NN) {                                                                         \

@@ -500,6 +500,7 @@ static hipError_t launch_ns(Ctx& c, const S* u, S* r, const S* met, const S* det
 	a.E = c.E;
 	if constexpr (sizeof(S) == sizeof(double)) a.zf = c.hp_z64;
 	else a.zf = c.hp_z32;
+	if (c.has_nonconforming && !a.zf) return hipErrorInvalidValue;
 	switch (c.N) {
 	case 2: return run_ns<S, 2, PERT>(c, a, phase);
 	case 3: return run_ns<S, 3, PERT>(c, a, phase);
@@ -528,6 +529,11 @@ hipError_t launch_rhs_ns_phase(Ctx& c, const double* u, double* r, int phase)
 hipError_t launch_rhs_ns32_phase(Ctx& c, const float* u, float* r, int phase)
 {
 	return launch_ns<float>(c, u, r, c.metrics_vc32, c.detJ_vc32, c.nJ_fc32, c.ops32, nullptr, phase);
+}
+
+hipError_t launch_rhs_ns_mixed_phase(Ctx& c, const double* u, float* r, int phase)
+{
+	return launch_ns<float, true>(c, nullptr, r, c.metrics_vc32, c.detJ_vc32, c.nJ_fc32, c.ops32, u, phase);
 }
 
 hipError_t launch_rhs_ns_mixed(Ctx& c, const double* u, float* r)

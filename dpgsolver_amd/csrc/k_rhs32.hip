@@ -22,6 +22,7 @@ hipError_t launch_rhs32(Ctx& c, const float* u, float* r)
 	a.E = c.E;
 	a.dmax = nullptr;
 	a.zf = c.hp_z32;
+	if (c.has_nonconforming && !a.zf) return hipErrorInvalidValue;
 	a.grid = c.rhs_grid[TUNE_RHS32];
 	if (a.grid > 0) return dispatch_rhs_persistent(c, a, MODE_RHS);
 	return dispatch_rhs<float, float, MODE_RHS>(c.d, c.p, c.cfg.pde, c.cfg.num_flux, a, c.stream);
@@ -45,6 +46,8 @@ hipError_t launch_rhs_mixed(Ctx& c, const double* u, float* r)
 	for (int k = 0; k < 3; ++k) a.b_adv[k] = c.cfg.b_adv[k];
 	a.E = c.E;
 	a.dmax = nullptr;
+	a.zf = c.hp_z32;
+	if (c.has_nonconforming && !a.zf) return hipErrorInvalidValue;
 	a.grid = c.rhs_grid[TUNE_MIXED];
 	if (a.grid > 0) return dispatch_rhs_persistent(c, a, MODE_PERT);
 	return dispatch_rhs<float, float, MODE_PERT>(c.d, c.p, c.cfg.pde, c.cfg.num_flux, a, c.stream);

@@ -34,6 +34,7 @@ hipError_t launch_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned l
 		return dmax ? launch_maxabs(c, r, (size_t)c.E * c.Np * c.nv, dmax) : hipSuccess;
 	}
 	RhsArgs<double, double> a = make_args64(c, u, r, dmax);
+	if (c.has_nonconforming && !a.zf) return hipErrorInvalidValue;
 	a.grid = c.rhs_grid[mode == MODE_RHS ? TUNE_RHS64 : TUNE_RHS64_MINV];
 	if (a.grid > 0) return dispatch_rhs_persistent(c, a, mode);
 	if (mode == MODE_RHS) return dispatch_rhs<double, double, MODE_RHS>(c.d, c.p, c.cfg.pde, c.cfg.num_flux, a, c.stream);

@@ -105,8 +105,8 @@ dgx_status dgx_upload_mesh(dgx_ctx* ctx, const dgx_mesh* mesh);
  * two volumes (compute_face_reference_order, face_solver.c:123-131) and its geometry from the lower-order side
  * (get_dominant_geom_vol_side_index, face_solver.c:95-107).  The context must be created with
  * dgx_config.p = the largest order allowed.  It then serves dgx_compute_rhs, dgx_compute_rhs_f32,
- * dgx_compute_rhs_explicit, dgx_ssp_rk3_step, dgx_graph_create_rhs (FP64 / FP32: the face kernel between
- * orders and every order group's launches in one graph) and dgx_sizes (ndof = the total); its other entry
+ * dgx_compute_rhs_mixed, dgx_compute_rhs_explicit, dgx_ssp_rk3_step, dgx_graph_create_rhs (the face kernel
+ * between orders and every order group's launches in one graph) and dgx_sizes (ndof = the total); its other entry
  * points return DGX_ERR_UNSUPPORTED.  Navier-Stokes: interior (periodic) faces only.  Replaces the per-(type,
  * p) volume/face loops of compute_volume_rlhs_dg / compute_face_rlhs_dg for meshes whose volumes differ in
  * p_ref.  Synchronous. */
