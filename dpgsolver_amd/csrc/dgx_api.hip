@@ -405,7 +405,7 @@ dgx_status hp_rhs_mixed(Ctx& c, const double* u, float* r)
 
 extern "C" {
 
-dgx_status dgx_upload_mesh_hp(dgx_ctx* h, const dgx_mesh_hp* m)
+static dgx_status upload_hp_impl(dgx_ctx* h, const dgx_mesh_hp* m)
 {
 	if (!h || !m || m->E <= 0 || m->F < 0 || !m->p || !m->xyz || (m->F > 0 && !m->faces)) return DGX_ERR_ARG;
 	Ctx& c = h->c;
@@ -572,7 +572,16 @@ dgx_status dgx_upload_mesh_hp(dgx_ctx* h, const dgx_mesh_hp* m)
 		sub->c.hp_z64 = H.z64 + H.z_off[g];
 		sub->c.hp_z32 = H.z32 + H.z_off[g];
 	}
+	H.ready = true;
 	return DGX_OK;
+}
+
+dgx_status dgx_upload_mesh_hp(dgx_ctx* h, const dgx_mesh_hp* m)
+{
+	const dgx_status st = upload_hp_impl(h, m);
+	// a failure after the previous mesh was released leaves no half-built variable-order state behind
+	if (st != DGX_OK && h && h->c.hp && !h->c.hp->ready) free_mesh(h->c);
+	return st;
 }
 
 dgx_status dgx_get_geometry(dgx_ctx* h, double* met, double* dj, double* nJ)

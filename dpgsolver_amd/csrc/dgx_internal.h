@@ -75,6 +75,7 @@ struct HpData {
 	std::vector<int> e_off;          // first element of each group (groups + 1 entries)
 	std::vector<long long> u_off;    // first state value of each group
 	std::vector<long long> z_off;    // first lifted-flux value of each group
+	bool ready = false;              // every group uploaded
 	int E = 0, F = 0, Fnc = 0;
 	long long ndof = 0, nz = 0;
 	std::vector<HpPair> pairs;       // runs of faces with one (NL, NR)

@@ -335,3 +335,23 @@ def test_hp_graph_replays_equal_eager_calls(dev):
         for g in graphs.values():
             g.close()
         ctx.close()
+
+
+def test_hp_failed_upload_leaves_no_mesh(dev):
+    """An upload that fails in a group (a viscous mesh with boundary faces) leaves the context without a mesh:
+    the residual then reports DGX_ERR_NO_MESH instead of running on a half-built variable-order state."""
+    from dpgsolver_amd import _lib
+
+    n = (4, 4)
+    pe = M.config4_hp_orders(n)
+    ch = M.structured_hp_mesh(pe, n, (0, 0), (1, 1), (False, False), bc_low=(M.BC_RIEMANN, M.BC_SLIPWALL),
+                              bc_high=(M.BC_RIEMANN, M.BC_SLIPWALL))
+    ctx = _lib.Context(2, 5, M.PDE_NAVIER_STOKES, M.FLUX_ROE, mu=MU, Pr=PR)
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    with pytest.raises(_lib.DgxError, match="UNSUPPORTED"):
+        ctx.upload_mesh_hp(ch)
+    u = torch.from_numpy(M.hp_state(ch, _state_fn(1))).to(dev)
+    r = torch.empty_like(u)
+    with pytest.raises(_lib.DgxError, match="NO_MESH"):
+        ctx.compute_rhs(u.data_ptr(), r.data_ptr())
+    ctx.close()
