@@ -285,6 +285,15 @@ def test_hp_ns_explicit_path_and_uniform_equivalence(dev):
     ref = orc.apply_minv(orc.rhs_ns(u))
     assert np.abs(r.cpu().numpy() - ref).max() <= 1e-12 * np.abs(ref).max()
     assert abs(mx - np.abs(ref).max()) <= 1e-12 * np.abs(ref).max()
+    # one SSP-RK3 step (Gottlieb 2001 eq. 4.2, no limiter) against the same stages on the oracle
+    dt = 1e-3
+    f = lambda w: orc.apply_minv(orc.rhs_ns(w))
+    s1 = u + dt * f(u)
+    s2 = 0.75 * u + 0.25 * (s1 + dt * f(s1))
+    uref = u / 3.0 + 2.0 / 3.0 * (s2 + dt * f(s2))
+    ctx.ssp_rk3_step(ud.data_ptr(), dt, positivity=False)
+    torch.cuda.synchronize()
+    assert np.abs(ud.cpu().numpy() - uref).max() <= 1e-12 * np.abs(uref).max()
     ctx.close()
     # one order through the variable-order path equals the uniform Navier-Stokes context bit for bit
     hm = M.structured_hp_mesh(np.full(20, 3), (5, 4), (0, 0), (1, 1), (True, True), curve=_curve)
