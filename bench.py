@@ -400,6 +400,46 @@ def hp_ns_throughput(torch, dev, steps, warmup):
                       "rel_err_vs_fp64": errmx}}
 
 
+def hp_euler_throughput(torch, dev, steps, warmup):
+    """2-D Euler residual on the same variable-order grid (500 x 300 periodic QUAD, p ~ U{1..5}, Roe): the face
+    kernel between orders and every order group's residual kernel as one HIP graph, FP64, FP32 and mixed."""
+    from dpgsolver_amd import _lib, mesh as M
+
+    mesh = M.config4_hp_mesh()
+    ctx = _lib.Context(2, 5, M.PDE_EULER, M.FLUX_ROE)
+    stream = torch.cuda.current_stream(dev)
+    ctx.set_stream(stream.cuda_stream)
+    ctx.upload_mesh_hp(mesh)
+    u = torch.from_numpy(M.config4_hp_state(mesh)).to(dev)
+    u32 = u.float()
+    outs = {_lib.GRAPH_RHS_FP64: torch.empty_like(u), _lib.GRAPH_RHS_FP32: torch.empty_like(u32),
+            _lib.GRAPH_RHS_MIXED: torch.empty_like(u32)}
+    ins = {_lib.GRAPH_RHS_FP64: u, _lib.GRAPH_RHS_FP32: u32, _lib.GRAPH_RHS_MIXED: u}
+    ms = {}
+    for kind, out in outs.items():
+        g = ctx.graph_rhs(kind, ins[kind].data_ptr(), out.data_ptr())
+        for _ in range(warmup):
+            g.launch(1)
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        g.launch(steps)
+        stop.record(stream)
+        torch.cuda.synchronize()
+        ms[kind] = start.elapsed_time(stop) / steps
+        g.close()
+    r = outs[_lib.GRAPH_RHS_FP64]
+    err = {k: ((outs[k].double() - r).abs().max() / r.abs().max()).item()
+           for k in (_lib.GRAPH_RHS_FP32, _lib.GRAPH_RHS_MIXED)}
+    ndof = u.numel()
+    ctx.close()
+    leg = lambda k: {"value": round(ndof / (ms[k] * 1e-3) / 1e9, 3), "ms_per_step": round(ms[k], 5)}
+    return {"workload": "2-D Euler, 500x300 periodic QUAD, p ~ U{1..5} per element, Roe", "ndof": ndof,
+            "fp64": leg(_lib.GRAPH_RHS_FP64),
+            "fp32": {**leg(_lib.GRAPH_RHS_FP32), "rel_err_vs_fp64": err[_lib.GRAPH_RHS_FP32]},
+            "mixed": {**leg(_lib.GRAPH_RHS_MIXED), "rel_err_vs_fp64": err[_lib.GRAPH_RHS_MIXED]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -435,7 +475,8 @@ def main():
         print(json.dumps({"navier_stokes": ns_throughput(torch, dev, args.steps, args.warmup)}), flush=True)
         return
     if args.hp_only:
-        print(json.dumps({"navier_stokes_variable_order": hp_ns_throughput(torch, dev, args.steps, args.warmup)}),
+        print(json.dumps({"navier_stokes_variable_order": hp_ns_throughput(torch, dev, args.steps, args.warmup),
+                          "euler_variable_order": hp_euler_throughput(torch, dev, args.steps, args.warmup)}),
               flush=True)
         return
     n, p = args.n, 4
@@ -535,6 +576,9 @@ def main():
     hp = hp_ns_throughput(torch, dev, args.steps, args.warmup)
     print(f"variable-order navier-stokes residual: {hp['ms_per_step']:.4f} ms, {hp['value']} GDOF/s",
           file=sys.stderr, flush=True)
+    hpe = hp_euler_throughput(torch, dev, args.steps, args.warmup)
+    print(f"variable-order euler residual: {hpe['fp64']['ms_per_step']:.4f} ms, {hpe['fp64']['value']} GDOF/s",
+          file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline()
     implicit = None if args.no_implicit else time_to_steady(torch, dev, cpu=not args.no_cpu)
 
@@ -570,6 +614,7 @@ def main():
         "fp64_free_stream_max_abs": free_stream,
         "navier_stokes": ns,
         "navier_stokes_variable_order": hp,
+        "euler_variable_order": hpe,
         "cpu_baseline": cpu,
         "time_to_1e-10": implicit,
     }

@@ -1,7 +1,8 @@
 // k_hp.hip -- faces between elements of different orders on 2-D variable-order meshes.
 //
 // A face takes the larger order of its two volumes (face_solver.c:123-131), so its cubature is the GL rule of
-// Nf = max(NL, NR) points.  Per face:
+// Nf = max(NL, NR) points.  Per face (one kernel instance per (NL, NR) pair, the faces sorted by pair at upload,
+// eight lanes per face):
 //   1. both sides' GLL face nodes (NL and NR per variable) are gathered from the state,
 //   2. interpolated to the Nf face GL points (get_operator__cv0_vs_fc_T with the face order,
 //      compute_face_rlhs_T.c:67-80), the right side read in the left ordering,
@@ -59,106 +60,107 @@ __device__ inline void ref_normal_flux(const C* c, const C* n, C* fn)
 	euler_normal_flux<C, 2>(c, cu, pc, n, fn);
 }
 
-// Euler: one thread per face, one pass over the face points accumulating both sides' lifts.
-template <typename S, int FLUX, bool PERT>
-__global__ void __launch_bounds__(64) k_hp_faces(HpFaceArgs<S> a)
+// Euler: one instance per (NL, NR) pair, eight lanes per face: the lanes load the face nodes, evaluate the flux
+// at one face point each and lift to one own face node each, through LDS.
+template <typename S, int FLUX, int NL, int NR, bool PERT>
+__global__ void __launch_bounds__(64) k_hp_faces(HpFaceArgs<S> a, int f0, int nf)
 {
 	using C = typename std::conditional<PERT, double, S>::type;
-	constexpr int NV = 4, NM = kHpMaxN;
-	const int f = blockIdx.x * blockDim.x + threadIdx.x;
-	if (f >= a.F) return;
+	constexpr int NV = 4, FPB = 8, Nf = NL > NR ? NL : NR;
 	const C* U;
 	const C* NJ;
 	const HpOps<C>* o;
 	if constexpr (PERT) { U = a.u64; NJ = a.nJ64; o = a.ops64; }
 	else { U = a.u; NJ = a.nJ; o = a.ops; }
-	const HpFace hf = a.faces[f];
-	const int NL = hf.info & 7, NR = (hf.info >> 3) & 7, lfL = (hf.info >> 6) & 3, lfR = (hf.info >> 8) & 3;
-	const bool flip = (hf.info >> 11) & 1;   // ord bit 1: the right face runs the other way
-	const int Nf = NL > NR ? NL : NR;
-	C uL[NV][NM], uR[NV][NM], zL[NM][NV], zR[NM][NV], cL[NV], cR[NV];
-#pragma unroll
-	for (int i = 0; i < NM; ++i) {
-		if (i < NL) {
-			const C* p = U + hf.uL + face_node_2d(NL, lfL, i);
-#pragma unroll
-			for (int v = 0; v < NV; ++v) uL[v][i] = p[v * NL * NL];
-		}
-		if (i < NR) {
-			const C* p = U + hf.uR + face_node_2d(NR, lfR, i);
-#pragma unroll
-			for (int v = 0; v < NV; ++v) uR[v][i] = p[v * NR * NR];
-		}
-#pragma unroll
-		for (int v = 0; v < NV; ++v) zL[i][v] = zR[i][v] = C(0);
+	__shared__ C IgL[Nf * NL], IgR[Nf * NR], PgL[Nf * NL], PgR[Nf * NR], wf[Nf], wiL[NL], wiR[NR];
+	__shared__ C uLs[FPB][NV * NL], uRs[FPB][NV * NR];   // GLL face nodes [v][i], own orderings
+	__shared__ C gqL[FPB][Nf * NV], gqR[FPB][Nf * NV];   // w J_f F-hat per side (the mixed form: minus F(c_e))
+	const int slot = threadIdx.x >> 3, j = threadIdx.x & 7;
+	for (int i = threadIdx.x; i < Nf * NL; i += 64) {
+		IgL[i] = o->Ig[NL][Nf][i];
+		PgL[i] = o->Pg[NL][Nf][i];
 	}
-	if constexpr (PERT) {
+	for (int i = threadIdx.x; i < Nf * NR; i += 64) {
+		IgR[i] = o->Ig[NR][Nf][i];
+		PgR[i] = o->Pg[NR][Nf][i];
+	}
+	if (threadIdx.x < Nf) wf[threadIdx.x] = o->w[Nf][threadIdx.x];
+	if (threadIdx.x < NL) wiL[threadIdx.x] = o->winv[NL][threadIdx.x];
+	if (threadIdx.x < NR) wiR[threadIdx.x] = o->winv[NR][threadIdx.x];
+	const int fl = blockIdx.x * FPB + slot;
+	const bool valid = fl < nf;
+	const int f = f0 + (valid ? fl : 0);
+	const HpFace hf = a.faces[f];
+	const int lfL = (hf.info >> 6) & 3, lfR = (hf.info >> 8) & 3;
+	const bool flip = (hf.info >> 11) & 1;   // ord bit 1: the right face runs the other way
+	if (valid && j < NL) {
+#pragma unroll
+		for (int v = 0; v < NV; ++v) uLs[slot][v * NL + j] = U[hf.uL + v * NL * NL + face_node_2d(NL, lfL, j)];
+	}
+	if (valid && j < NR) {
+#pragma unroll
+		for (int v = 0; v < NV; ++v) uRs[slot][v * NR + j] = U[hf.uR + v * NR * NR + face_node_2d(NR, lfR, j)];
+	}
+	__syncthreads();
+	if (valid && j < Nf) {
+		const int q = j, qr = flip ? Nf - 1 - q : q;
+		C sL[NV], sR[NV];
 #pragma unroll
 		for (int v = 0; v < NV; ++v) {
-			cL[v] = U[hf.uL + v * NL * NL];
-			cR[v] = U[hf.uR + v * NR * NR];
-		}
-	}
+			C aL = C(0), aR = C(0);
 #pragma unroll
-	for (int q = 0; q < NM; ++q) {
-		if (q < Nf) {
-			const int qr = flip ? Nf - 1 - q : q;
-			const C* IL = o->Ig[NL][Nf] + q * NL;
-			const C* IR = o->Ig[NR][Nf] + qr * NR;
-			C sL[NV], sR[NV];
+			for (int i = 0; i < NL; ++i) aL += IgL[q * NL + i] * uLs[slot][v * NL + i];
+#pragma unroll
+			for (int i = 0; i < NR; ++i) aR += IgR[qr * NR + i] * uRs[slot][v * NR + i];
+			sL[v] = aL;
+			sR[v] = aR;
+		}
+		const C* nj = NJ + ((size_t)f * kHpMaxN + q) * 2;
+		const C Jf = tsqrt(nj[0] * nj[0] + nj[1] * nj[1]);
+		const C Jfi = trcp(Jf);
+		const C nu[2] = { nj[0] * Jfi, nj[1] * Jfi };
+		C nnf[NV];
+		if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<C, 2>(sL, sR, nu, nnf);
+		else numflux_roe<C, 2>(sL, sR, nu, nnf);
+		const C wj = wf[q] * Jf;
+		if constexpr (PERT) {
+			C cL[NV], cR[NV], fcL[NV], fcR[NV];
 #pragma unroll
 			for (int v = 0; v < NV; ++v) {
-				C aL = C(0), aR = C(0);
-#pragma unroll
-				for (int i = 0; i < NM; ++i) {
-					if (i < NL) aL += IL[i] * uL[v][i];
-					if (i < NR) aR += IR[i] * uR[v][i];
-				}
-				sL[v] = aL;
-				sR[v] = aR;
+				cL[v] = U[hf.uL + v * NL * NL];
+				cR[v] = U[hf.uR + v * NR * NR];
 			}
-			const C* nj = NJ + ((size_t)f * NM + q) * 2;
-			const C Jf = tsqrt(nj[0] * nj[0] + nj[1] * nj[1]);
-			const C Jfi = trcp(Jf);
-			const C nu[2] = { nj[0] * Jfi, nj[1] * Jfi };
-			C nnf[NV], gL[NV], gR[NV];
-			if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<C, 2>(sL, sR, nu, nnf);
-			else numflux_roe<C, 2>(sL, sR, nu, nnf);
-			const C wj = o->w[Nf][q] * Jf;
-			if constexpr (PERT) {
-				const double* nh = a.nJhi + ((size_t)f * NM + q) * 2;
-				C fcL[NV], fcR[NV];
-				ref_normal_flux<C>(cL, (NL < NR) ? nj : nh, fcL);   // each side's own n J (left orientation)
-				ref_normal_flux<C>(cR, (NL < NR) ? nh : nj, fcR);
-				const C wq = o->w[Nf][q];
+			const double* nh = a.nJhi + ((size_t)f * kHpMaxN + q) * 2;
+			ref_normal_flux<C>(cL, (NL < NR) ? nj : nh, fcL);   // each side's own n J (left orientation)
+			ref_normal_flux<C>(cR, (NL < NR) ? nh : nj, fcR);
 #pragma unroll
-				for (int v = 0; v < NV; ++v) {
-					gL[v] = wj * nnf[v] - wq * fcL[v];
-					gR[v] = wj * nnf[v] - wq * fcR[v];
-				}
-			} else {
-#pragma unroll
-				for (int v = 0; v < NV; ++v) gL[v] = gR[v] = wj * nnf[v];
+			for (int v = 0; v < NV; ++v) {
+				gqL[slot][q * NV + v] = wj * nnf[v] - wf[q] * fcL[v];
+				gqR[slot][q * NV + v] = wj * nnf[v] - wf[q] * fcR[v];
 			}
-			const C* PL = o->Pg[NL][Nf] + q * NL;
-			const C* PR = o->Pg[NR][Nf] + qr * NR;
+		} else {
 #pragma unroll
-			for (int p = 0; p < NM; ++p) {
-#pragma unroll
-				for (int v = 0; v < NV; ++v) {
-					if (p < NL) zL[p][v] += PL[p] * gL[v];
-					if (p < NR) zR[p][v] += PR[p] * gR[v];
-				}
-			}
+			for (int v = 0; v < NV; ++v) gqL[slot][q * NV + v] = gqR[slot][q * NV + v] = wj * nnf[v];
 		}
 	}
-	// each side's own GL face points (the right side negated, in its own ordering)
-#pragma unroll
-	for (int p = 0; p < NM; ++p) {
+	__syncthreads();
+	// lifts to each side's own GL face points (the right side negated, in its own ordering)
+	if (valid && j < NL) {
 #pragma unroll
 		for (int v = 0; v < NV; ++v) {
-			if (p < NL) a.z[hf.zL + v * NL + p] = S(zL[p][v] * o->winv[NL][p]);
-			if (p < NR) a.z[hf.zR + v * NR + p] = S(C(0) - zR[p][v] * o->winv[NR][p]);
+			C acc = C(0);
+#pragma unroll
+			for (int q = 0; q < Nf; ++q) acc += PgL[q * NL + j] * gqL[slot][q * NV + v];
+			a.z[hf.zL + v * NL + j] = S(acc * wiL[j]);
+		}
+	}
+	if (valid && j < NR) {
+#pragma unroll
+		for (int v = 0; v < NV; ++v) {
+			C acc = C(0);
+#pragma unroll
+			for (int q = 0; q < Nf; ++q) acc += PgR[(flip ? Nf - 1 - q : q) * NR + j] * gqR[slot][q * NV + v];
+			a.z[hf.zR + v * NR + j] = S(C(0) - acc * wiR[j]);
 		}
 	}
 }
@@ -420,10 +422,26 @@ hipError_t run_hp_faces(Ctx& c, const S* u, const double* u64, S* z, const S* nJ
 		return hipSuccess;
 	}
 	HpFaceArgs<S> a{ u, u64, z, H.faces, nJ, H.nJ, H.nJhi, ops, H.ops64, H.Fnc };
-	const int blocks = (H.Fnc + 63) / 64;
-	if (c.cfg.num_flux == DGX_FLUX_LF) hipLaunchKernelGGL((k_hp_faces<S, DGX_FLUX_LF, PERT>), dim3(blocks), dim3(64), 0, c.stream, a);
-	else hipLaunchKernelGGL((k_hp_faces<S, DGX_FLUX_ROE, PERT>), dim3(blocks), dim3(64), 0, c.stream, a);
-	return hipGetLastError();
+	const bool lf = c.cfg.num_flux == DGX_FLUX_LF;
+	for (const HpPair& pr : H.pairs) {
+		const int blocks = (pr.count + 7) / 8;
+#define DGX_HP_PAIR(L, R)                                                                                              \
+	case L * 8 + R:                                                                                                    \
+		if (lf) hipLaunchKernelGGL((k_hp_faces<S, DGX_FLUX_LF, L, R, PERT>), dim3(blocks), dim3(64), 0, c.stream, a, pr.start, pr.count); \
+		else hipLaunchKernelGGL((k_hp_faces<S, DGX_FLUX_ROE, L, R, PERT>), dim3(blocks), dim3(64), 0, c.stream, a, pr.start, pr.count); \
+		break;
+		switch (pr.NL * 8 + pr.NR) {
+			DGX_HP_PAIR(2, 3) DGX_HP_PAIR(2, 4) DGX_HP_PAIR(2, 5) DGX_HP_PAIR(2, 6)
+			DGX_HP_PAIR(3, 2) DGX_HP_PAIR(3, 4) DGX_HP_PAIR(3, 5) DGX_HP_PAIR(3, 6)
+			DGX_HP_PAIR(4, 2) DGX_HP_PAIR(4, 3) DGX_HP_PAIR(4, 5) DGX_HP_PAIR(4, 6)
+			DGX_HP_PAIR(5, 2) DGX_HP_PAIR(5, 3) DGX_HP_PAIR(5, 4) DGX_HP_PAIR(5, 6)
+			DGX_HP_PAIR(6, 2) DGX_HP_PAIR(6, 3) DGX_HP_PAIR(6, 4) DGX_HP_PAIR(6, 5)
+		default: return hipErrorInvalidValue;
+		}
+#undef DGX_HP_PAIR
+		DGX_HIP_CHECK(hipGetLastError());
+	}
+	return hipSuccess;
 }
 
 }  // namespace
