@@ -321,14 +321,24 @@ __global__ void k_tanh_backward(const float* __restrict__ h, float* dh, size_t n
 	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
 		dh[i] *= (1.f - h[i] * h[i]);
 }
-// out[j] = sum_b g[b][j]  (deterministic, one thread per column)
-__global__ void k_colsum(const float* __restrict__ g, int B, int N, float* out)
+// out[j] = sum_b g[b][j]: 64 columns per workgroup, the rows split over 16 slices whose partial sums are added
+// in a fixed order (deterministic); a row of 64 columns is one coalesced load per slice
+__global__ void __launch_bounds__(1024) k_colsum(const float* __restrict__ g, int B, int N, float* out)
 {
-	const int j = blockIdx.x * blockDim.x + threadIdx.x;
-	if (j >= N) return;
+	__shared__ float part[16][64];
+	const int c = threadIdx.x & 63, sl = threadIdx.x >> 6;
+	const int j = blockIdx.x * 64 + c;
 	float s = 0.f;
-	for (int b = 0; b < B; ++b) s += g[(size_t)b * N + j];
-	out[j] = s;
+	if (j < N)
+		for (int b = sl; b < B; b += 16) s += g[(size_t)b * N + j];
+	part[sl][c] = s;
+	__syncthreads();
+	if (sl == 0 && j < N) {
+		float t = part[0][c];
+#pragma unroll
+		for (int k = 1; k < 16; ++k) t += part[k][c];
+		out[j] = t;
+	}
 }
 __global__ void k_adam(float* w, const float* __restrict__ g, float* m, float* v, size_t n, float lr, float b1,
                        float b2, float c1, float c2, float eps)
@@ -412,17 +422,17 @@ static hipError_t mlp_backward(dgx_mlp* m, const float* X, int B)
 	float *gW1 = m->gw + m->off[0], *gb1 = m->gw + m->off[1], *gW2 = m->gw + m->off[2], *gb2 = m->gw + m->off[3],
 	      *gW3 = m->gw + m->off[4], *gb3 = m->gw + m->off[5];
 	DGX_HIP_CHECK(gemm(m->s, m->go, 1, m->nout, m->h2, m->nhid, 1, gW3, m->nhid, m->nout, m->nhid, B, 0, nullptr, 0));
-	hipLaunchKernelGGL(k_colsum, dim3((m->nout + 255) / 256), dim3(256), 0, m->s, m->go, B, m->nout, gb3);
+	hipLaunchKernelGGL(k_colsum, dim3((m->nout + 63) / 64), dim3(1024), 0, m->s, m->go, B, m->nout, gb3);
 	DGX_HIP_CHECK(gemm(m->s, m->go, m->nout, 1, W3, m->nhid, 1, m->gh2, m->nhid, B, m->nhid, m->nout, 0, nullptr, 0));
 	hipLaunchKernelGGL(k_tanh_backward, dim3(gridn((size_t)B * m->nhid)), dim3(256), 0, m->s, m->h2, m->gh2,
 	                   (size_t)B * m->nhid);
 	DGX_HIP_CHECK(gemm(m->s, m->gh2, 1, m->nhid, m->h1, m->nhid, 1, gW2, m->nhid, m->nhid, m->nhid, B, 0, nullptr, 0));
-	hipLaunchKernelGGL(k_colsum, dim3((m->nhid + 255) / 256), dim3(256), 0, m->s, m->gh2, B, m->nhid, gb2);
+	hipLaunchKernelGGL(k_colsum, dim3((m->nhid + 63) / 64), dim3(1024), 0, m->s, m->gh2, B, m->nhid, gb2);
 	DGX_HIP_CHECK(gemm(m->s, m->gh2, m->nhid, 1, W2, m->nhid, 1, m->gh1, m->nhid, B, m->nhid, m->nhid, 0, nullptr, 0));
 	hipLaunchKernelGGL(k_tanh_backward, dim3(gridn((size_t)B * m->nhid)), dim3(256), 0, m->s, m->h1, m->gh1,
 	                   (size_t)B * m->nhid);
 	DGX_HIP_CHECK(gemm(m->s, m->gh1, 1, m->nhid, X, m->nf, 1, gW1, m->nf, m->nhid, m->nf, B, 0, nullptr, 0));
-	hipLaunchKernelGGL(k_colsum, dim3((m->nhid + 255) / 256), dim3(256), 0, m->s, m->gh1, B, m->nhid, gb1);
+	hipLaunchKernelGGL(k_colsum, dim3((m->nhid + 63) / 64), dim3(1024), 0, m->s, m->gh1, B, m->nhid, gb1);
 	return hipGetLastError();
 }
 
