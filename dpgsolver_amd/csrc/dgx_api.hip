@@ -42,6 +42,11 @@ void free_hp(Ctx& c)
 	for (dgx_ctx* g : H->groups) dgx_destroy(g);
 	release(H->faces); release(H->nJ); release(H->nJ32); release(H->nJhi); release(H->z64); release(H->z32);
 	release(H->ops64); release(H->ops32);
+	for (int k = 0; k < dgx::HpData::kLanes; ++k) {
+		if (H->lane[k]) (void)hipStreamDestroy(H->lane[k]);
+		if (H->join_ev[k]) (void)hipEventDestroy(H->join_ev[k]);
+	}
+	if (H->fork_ev) (void)hipEventDestroy(H->fork_ev);
 	delete H;
 	c.hp = nullptr;
 }
@@ -322,58 +327,75 @@ void outward_nj(const double* xyz, int N, int lf, const double* s, int ns, doubl
 	}
 }
 
-// Variable-order residuals: the faces between orders first (lifted fluxes), then each order group's residual
-// kernel on its contiguous part of the state (its faces to other orders read from the lifted fluxes).
+// Enqueues launch(g, s) for every order group on the group's side stream; the group contexts keep the
+// caller's stream outside these calls.
+template <typename F>
+hipError_t on_groups(Ctx& c, F launch)
+{
+	dgx::HpData& H = *c.hp;
+	for (size_t g = 0; g < H.groups.size(); ++g) {
+		Ctx& s = H.groups[g]->c;
+		s.stream = H.lane[H.group_lane[g]];
+		const hipError_t e = launch(g, s);
+		s.stream = c.stream;
+		if (e != hipSuccess) return e;
+	}
+	return hipSuccess;
+}
+
+// Every side stream waits for everything enqueued so far on the side streams.
+hipError_t hp_barrier(dgx::HpData& H, hipStream_t main)
+{
+	const hipError_t e = dgx::hp_join(H, main);
+	return e != hipSuccess ? e : dgx::hp_fork(H, main);
+}
+
+// Variable-order residuals. Navier-Stokes: every group's gradient traces, the faces between orders (lifted
+// gradients and fluxes), every group's residual (+ M^-1 and max-norm). Euler: the faces between orders, then
+// every group's residual. The launches of one phase write disjoint outputs and run concurrently on the side
+// streams; the phases are ordered by barriers.
 dgx_status hp_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned long long* dmax)
 {
 	dgx::HpData& H = *c.hp;
+	DGX_TRY(dgx::hp_fork(H, c.stream));
 	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) {
-		// every group's gradient traces, the faces between orders, every group's residual (+ M^-1, max-norm)
-		for (size_t g = 0; g < H.groups.size(); ++g) {
-			Ctx& s = H.groups[g]->c;
-			s.stream = c.stream;
-			DGX_TRY(dgx::launch_rhs_ns_phase(s, u + H.u_off[g], r + H.u_off[g], 1));
-		}
+		DGX_TRY(on_groups(c, [&](size_t g, Ctx& s) { return dgx::launch_rhs_ns_phase(s, u + H.u_off[g], r + H.u_off[g], 1); }));
+		DGX_TRY(hp_barrier(H, c.stream));
 		DGX_TRY(dgx::launch_hp_faces64(c, u));
-		for (size_t g = 0; g < H.groups.size(); ++g) {
-			Ctx& s = H.groups[g]->c;
+		DGX_TRY(hp_barrier(H, c.stream));
+		DGX_TRY(on_groups(c, [&](size_t g, Ctx& s) -> hipError_t {
 			double* rg = r + H.u_off[g];
-			DGX_TRY(dgx::launch_rhs_ns_phase(s, u + H.u_off[g], rg, 2));
-			if (mode != dgx::MODE_RHS_MINV) continue;
-			DGX_TRY(dgx::launch_minv(s, rg));
-			if (dmax) DGX_TRY(dgx::launch_maxabs(s, rg, ndof(s), dmax));
-		}
-		return DGX_OK;
+			hipError_t e = dgx::launch_rhs_ns_phase(s, u + H.u_off[g], rg, 2);
+			if (e != hipSuccess || mode != dgx::MODE_RHS_MINV) return e;
+			e = dgx::launch_minv(s, rg);
+			if (e == hipSuccess && dmax) e = dgx::launch_maxabs(s, rg, ndof(s), dmax);
+			return e;
+		}));
+	} else {
+		DGX_TRY(dgx::launch_hp_faces64(c, u));
+		DGX_TRY(hp_barrier(H, c.stream));
+		DGX_TRY(on_groups(c, [&](size_t g, Ctx& s) { return dgx::launch_rhs64(s, u + H.u_off[g], r + H.u_off[g], mode, dmax); }));
 	}
-	DGX_TRY(dgx::launch_hp_faces64(c, u));
-	for (size_t g = 0; g < H.groups.size(); ++g) {
-		Ctx& s = H.groups[g]->c;
-		s.stream = c.stream;
-		DGX_TRY(dgx::launch_rhs64(s, u + H.u_off[g], r + H.u_off[g], mode, dmax));
-	}
+	DGX_TRY(dgx::hp_join(H, c.stream));
 	return DGX_OK;
 }
 
 dgx_status hp_rhs32(Ctx& c, const float* u, float* r)
 {
 	dgx::HpData& H = *c.hp;
+	DGX_TRY(dgx::hp_fork(H, c.stream));
 	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) {
-		for (size_t g = 0; g < H.groups.size(); ++g) {
-			Ctx& s = H.groups[g]->c;
-			s.stream = c.stream;
-			DGX_TRY(dgx::launch_rhs_ns32_phase(s, u + H.u_off[g], r + H.u_off[g], 1));
-		}
+		DGX_TRY(on_groups(c, [&](size_t g, Ctx& s) { return dgx::launch_rhs_ns32_phase(s, u + H.u_off[g], r + H.u_off[g], 1); }));
+		DGX_TRY(hp_barrier(H, c.stream));
 		DGX_TRY(dgx::launch_hp_faces32(c, u));
-		for (size_t g = 0; g < H.groups.size(); ++g)
-			DGX_TRY(dgx::launch_rhs_ns32_phase(H.groups[g]->c, u + H.u_off[g], r + H.u_off[g], 2));
-		return DGX_OK;
+		DGX_TRY(hp_barrier(H, c.stream));
+		DGX_TRY(on_groups(c, [&](size_t g, Ctx& s) { return dgx::launch_rhs_ns32_phase(s, u + H.u_off[g], r + H.u_off[g], 2); }));
+	} else {
+		DGX_TRY(dgx::launch_hp_faces32(c, u));
+		DGX_TRY(hp_barrier(H, c.stream));
+		DGX_TRY(on_groups(c, [&](size_t g, Ctx& s) { return dgx::launch_rhs32(s, u + H.u_off[g], r + H.u_off[g]); }));
 	}
-	DGX_TRY(dgx::launch_hp_faces32(c, u));
-	for (size_t g = 0; g < H.groups.size(); ++g) {
-		Ctx& s = H.groups[g]->c;
-		s.stream = c.stream;
-		DGX_TRY(dgx::launch_rhs32(s, u + H.u_off[g], r + H.u_off[g]));
-	}
+	DGX_TRY(dgx::hp_join(H, c.stream));
 	return DGX_OK;
 }
 
@@ -381,23 +403,19 @@ dgx_status hp_rhs32(Ctx& c, const float* u, float* r)
 dgx_status hp_rhs_mixed(Ctx& c, const double* u, float* r)
 {
 	dgx::HpData& H = *c.hp;
+	DGX_TRY(dgx::hp_fork(H, c.stream));
 	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) {
-		for (size_t g = 0; g < H.groups.size(); ++g) {
-			Ctx& s = H.groups[g]->c;
-			s.stream = c.stream;
-			DGX_TRY(dgx::launch_rhs_ns_mixed_phase(s, u + H.u_off[g], r + H.u_off[g], 1));
-		}
+		DGX_TRY(on_groups(c, [&](size_t g, Ctx& s) { return dgx::launch_rhs_ns_mixed_phase(s, u + H.u_off[g], r + H.u_off[g], 1); }));
+		DGX_TRY(hp_barrier(H, c.stream));
 		DGX_TRY(dgx::launch_hp_faces_mixed(c, u));
-		for (size_t g = 0; g < H.groups.size(); ++g)
-			DGX_TRY(dgx::launch_rhs_ns_mixed_phase(H.groups[g]->c, u + H.u_off[g], r + H.u_off[g], 2));
-		return DGX_OK;
+		DGX_TRY(hp_barrier(H, c.stream));
+		DGX_TRY(on_groups(c, [&](size_t g, Ctx& s) { return dgx::launch_rhs_ns_mixed_phase(s, u + H.u_off[g], r + H.u_off[g], 2); }));
+	} else {
+		DGX_TRY(dgx::launch_hp_faces_mixed(c, u));
+		DGX_TRY(hp_barrier(H, c.stream));
+		DGX_TRY(on_groups(c, [&](size_t g, Ctx& s) { return dgx::launch_rhs_mixed(s, u + H.u_off[g], r + H.u_off[g]); }));
 	}
-	DGX_TRY(dgx::launch_hp_faces_mixed(c, u));
-	for (size_t g = 0; g < H.groups.size(); ++g) {
-		Ctx& s = H.groups[g]->c;
-		s.stream = c.stream;
-		DGX_TRY(dgx::launch_rhs_mixed(s, u + H.u_off[g], r + H.u_off[g]));
-	}
+	DGX_TRY(dgx::hp_join(H, c.stream));
 	return DGX_OK;
 }
 
@@ -572,6 +590,34 @@ static dgx_status upload_hp_impl(dgx_ctx* h, const dgx_mesh_hp* m)
 		sub->c.hp_z64 = H.z64 + H.z_off[g];
 		sub->c.hp_z32 = H.z32 + H.z_off[g];
 	}
+	// side streams, and the groups and face pairs spread over them longest first (cost ~ items x N^2)
+	for (int k = 0; k < dgx::HpData::kLanes; ++k) {
+		DGX_TRY(hipStreamCreateWithFlags(&H.lane[k], hipStreamNonBlocking));
+		DGX_TRY(hipEventCreateWithFlags(&H.join_ev[k], hipEventDisableTiming));
+	}
+	DGX_TRY(hipEventCreateWithFlags(&H.fork_ev, hipEventDisableTiming));
+	auto balance = [](const std::vector<double>& cost) {
+		std::vector<int> idx(cost.size()), lane(cost.size(), 0);
+		for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
+		std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+		double load[dgx::HpData::kLanes] = {};
+		for (int i : idx) {
+			int k = 0;
+			for (int j = 1; j < dgx::HpData::kLanes; ++j)
+				if (load[j] < load[k]) k = j;
+			lane[i] = k;
+			load[k] += cost[i];
+		}
+		return lane;
+	};
+	std::vector<double> gcost(G), pcost(H.pairs.size());
+	for (int g = 0; g < G; ++g) gcost[g] = (double)(e_off[g + 1] - e_off[g]) * (order[g] + 1) * (order[g] + 1);
+	for (size_t i = 0; i < H.pairs.size(); ++i) {
+		const int n = std::max(H.pairs[i].NL, H.pairs[i].NR);
+		pcost[i] = (double)H.pairs[i].count * n * n;
+	}
+	H.group_lane = balance(gcost);
+	H.pair_lane = balance(pcost);
 	H.ready = true;
 	return DGX_OK;
 }

@@ -87,7 +87,20 @@ struct HpData {
 	float* z32 = nullptr;
 	HpOps<double>* ops64 = nullptr;
 	HpOps<float>* ops32 = nullptr;
+	// Side streams (default priority) over which the independent launches of one phase are spread: the
+	// groups' kernels and the per-pair face kernels write disjoint outputs, so they run concurrently between
+	// a fork (each side stream waits for the caller's stream) and a join (the caller's stream waits for each).
+	static constexpr int kLanes = 4;
+	hipStream_t lane[kLanes] = {};
+	hipEvent_t fork_ev = nullptr;
+	hipEvent_t join_ev[kLanes] = {};
+	std::vector<int> group_lane;     // side stream of each group (longest-first balance of E_g N^2)
+	std::vector<int> pair_lane;      // side stream of each face pair (longest-first balance of count max(NL,NR)^2)
 };
+
+// Fork / join of the variable-order side streams on `main` (the context's stream, or the capturing stream).
+hipError_t hp_fork(HpData& H, hipStream_t main);
+hipError_t hp_join(HpData& H, hipStream_t main);
 
 // Per-element face record: f, neighbour element (-1 on a boundary), and packed
 // (neighbour local face | side << 4 | ord << 8 | bc << 12).

@@ -401,12 +401,14 @@ hipError_t run_hp_faces(Ctx& c, const S* u, const double* u64, S* z, const S* nJ
 		a.Pr = c.cfg.Pr;
 		a.F = H.Fnc;
 		const bool lf = c.cfg.num_flux == DGX_FLUX_LF;
-		for (const HpPair& pr : H.pairs) {
+		for (size_t ip = 0; ip < H.pairs.size(); ++ip) {
+			const HpPair& pr = H.pairs[ip];
+			const hipStream_t st = H.lane[H.pair_lane[ip]];
 			const int blocks = (pr.count + 7) / 8;
 #define DGX_HP_PAIR(L, R)                                                                                              \
 	case L * 8 + R:                                                                                                    \
-		if (lf) hipLaunchKernelGGL((k_hp_faces_ns<S, DGX_FLUX_LF, L, R, PERT>), dim3(blocks), dim3(64), 0, c.stream, a, pr.start, pr.count); \
-		else hipLaunchKernelGGL((k_hp_faces_ns<S, DGX_FLUX_ROE, L, R, PERT>), dim3(blocks), dim3(64), 0, c.stream, a, pr.start, pr.count); \
+		if (lf) hipLaunchKernelGGL((k_hp_faces_ns<S, DGX_FLUX_LF, L, R, PERT>), dim3(blocks), dim3(64), 0, st, a, pr.start, pr.count); \
+		else hipLaunchKernelGGL((k_hp_faces_ns<S, DGX_FLUX_ROE, L, R, PERT>), dim3(blocks), dim3(64), 0, st, a, pr.start, pr.count); \
 		break;
 			switch (pr.NL * 8 + pr.NR) {
 				DGX_HP_PAIR(2, 3) DGX_HP_PAIR(2, 4) DGX_HP_PAIR(2, 5) DGX_HP_PAIR(2, 6)
@@ -423,12 +425,14 @@ hipError_t run_hp_faces(Ctx& c, const S* u, const double* u64, S* z, const S* nJ
 	}
 	HpFaceArgs<S> a{ u, u64, z, H.faces, nJ, H.nJ, H.nJhi, ops, H.ops64, H.Fnc };
 	const bool lf = c.cfg.num_flux == DGX_FLUX_LF;
-	for (const HpPair& pr : H.pairs) {
+	for (size_t ip = 0; ip < H.pairs.size(); ++ip) {
+		const HpPair& pr = H.pairs[ip];
+		const hipStream_t st = H.lane[H.pair_lane[ip]];
 		const int blocks = (pr.count + 7) / 8;
 #define DGX_HP_PAIR(L, R)                                                                                              \
 	case L * 8 + R:                                                                                                    \
-		if (lf) hipLaunchKernelGGL((k_hp_faces<S, DGX_FLUX_LF, L, R, PERT>), dim3(blocks), dim3(64), 0, c.stream, a, pr.start, pr.count); \
-		else hipLaunchKernelGGL((k_hp_faces<S, DGX_FLUX_ROE, L, R, PERT>), dim3(blocks), dim3(64), 0, c.stream, a, pr.start, pr.count); \
+		if (lf) hipLaunchKernelGGL((k_hp_faces<S, DGX_FLUX_LF, L, R, PERT>), dim3(blocks), dim3(64), 0, st, a, pr.start, pr.count); \
+		else hipLaunchKernelGGL((k_hp_faces<S, DGX_FLUX_ROE, L, R, PERT>), dim3(blocks), dim3(64), 0, st, a, pr.start, pr.count); \
 		break;
 		switch (pr.NL * 8 + pr.NR) {
 			DGX_HP_PAIR(2, 3) DGX_HP_PAIR(2, 4) DGX_HP_PAIR(2, 5) DGX_HP_PAIR(2, 6)
@@ -446,6 +450,24 @@ hipError_t run_hp_faces(Ctx& c, const S* u, const double* u64, S* z, const S* nJ
 
 }  // namespace
 
+hipError_t hp_fork(HpData& H, hipStream_t main)
+{
+	DGX_HIP_CHECK(hipEventRecord(H.fork_ev, main));
+	for (int k = 0; k < HpData::kLanes; ++k) DGX_HIP_CHECK(hipStreamWaitEvent(H.lane[k], H.fork_ev, 0));
+	return hipSuccess;
+}
+
+hipError_t hp_join(HpData& H, hipStream_t main)
+{
+	for (int k = 0; k < HpData::kLanes; ++k) {
+		DGX_HIP_CHECK(hipEventRecord(H.join_ev[k], H.lane[k]));
+		DGX_HIP_CHECK(hipStreamWaitEvent(main, H.join_ev[k], 0));
+	}
+	return hipSuccess;
+}
+
+// The face kernels between orders, one launch per (NL, NR) pair on that pair's side stream; the caller forks
+// the side streams before and joins them after.
 hipError_t launch_hp_faces64(Ctx& c, const double* u)
 {
 	return run_hp_faces<double, false>(c, u, nullptr, c.hp->z64, c.hp->nJ, c.hp->ops64);

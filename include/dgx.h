@@ -107,7 +107,9 @@ dgx_status dgx_upload_mesh(dgx_ctx* ctx, const dgx_mesh* mesh);
  * dgx_config.p = the largest order allowed.  It then serves dgx_compute_rhs, dgx_compute_rhs_f32,
  * dgx_compute_rhs_mixed, dgx_compute_rhs_explicit, dgx_ssp_rk3_step, dgx_graph_create_rhs (the face kernel
  * between orders and every order group's launches in one graph) and dgx_sizes (ndof = the total); its other entry
- * points return DGX_ERR_UNSUPPORTED.  Navier-Stokes: interior (periodic) faces only.  Replaces the per-(type,
+ * points return DGX_ERR_UNSUPPORTED.  Navier-Stokes: interior (periodic) faces only.  The context then owns
+ * four side streams over which each residual spreads the independent per-order and per-pair launches, forked
+ * from and joined back to the context's stream within the call.  Replaces the per-(type,
  * p) volume/face loops of compute_volume_rlhs_dg / compute_face_rlhs_dg for meshes whose volumes differ in
  * p_ref.  Synchronous. */
 dgx_status dgx_upload_mesh_hp(dgx_ctx* ctx, const dgx_mesh_hp* mesh);
