@@ -203,6 +203,16 @@ def time_to_steady(torch, dev, epochs=60, cpu=True):
         st = ctx.solve_implicit(u.data_ptr(), precond=pc, **{**kw, **extra})
         return st, u
 
+    def _training_mfma(n, epochs, seconds, batch=256, nf=96, nh=256, no=2176):
+        # BF16 MFMA GEMMs per mini-batch: forward (3 layers), weight gradients (3) and input gradients (2)
+        fwd = 2 * batch * (nf * nh + nh * nh + nh * no)
+        step = 2 * fwd + 2 * batch * (no * nh + nh * nh)
+        flop = step * epochs * ((n + batch - 1) // batch)
+        ach = flop / seconds / 1e12
+        return {"gemm_tflop": round(flop / 1e12, 3), "achieved_tflops": round(ach, 3), "peak_tflops": 2500.0,
+                "frac": round(ach / 2500.0, 6),
+                "note": "GEMM FLOPs over the whole training wall time (loss, bias-gradient and Adam kernels included)"}
+
     def leg(st):
         return {"seconds": round(st["seconds"], 4), "newton_steps": st["newton_steps"],
                 "gmres_iters": st["gmres_iters"], "converged": bool(st["converged"]), "max_rhs": st["max_rhs"],
@@ -243,7 +253,8 @@ def time_to_steady(torch, dev, epochs=60, cpu=True):
            "sgs32_vs_bj_wall": round(st_s32["seconds"] / st_bj["seconds"], 4),
            "sgs32_steady_states_rel_diff": ((u_s32 - u_bj).abs().max() / u_bj.abs().max()).item(),
            "training": {"blocks": n, "epochs": epochs, "seconds": round(t_train, 3), "loss_untrained": l0,
-                        "loss_trained": loss, "structure": "Kronecker rank 8, MLP 96-256-256-2176 BF16/FP32"},
+                        "loss_trained": loss, "structure": "Kronecker rank 8, MLP 96-256-256-2176 BF16/FP32",
+                        "mfma": _training_mfma(n, epochs, t_train)},
            "learned_vs_bj_wall": round(st_ln["seconds"] / st_bj["seconds"], 4),
            "steady_states_rel_diff": diff}
     if cpu:

@@ -47,7 +47,24 @@ __global__ void __launch_bounds__(64) k_gemm_bf16(const float* __restrict__ A, l
 	const int i0 = blockIdx.y * 16, j0 = blockIdx.x * 16;
 	const int row = i0 + (l & 15), col = j0 + (l & 15), kq = (l >> 4) * 4;
 	f32x4 acc = { 0.f, 0.f, 0.f, 0.f };
-	for (int k0 = 0; k0 < K; k0 += 16) {
+	int k0 = 0;
+	// four k-steps of operands in flight before their MFMAs (the long-K products are load-latency bound;
+	// eight measured slower); the accumulation order is the one of the single-step loop below
+	for (; k0 + 64 <= K; k0 += 64) {
+		bf16x4 a[4], b[4];
+#pragma unroll
+		for (int u = 0; u < 4; ++u) {
+#pragma unroll
+			for (int t = 0; t < 4; ++t) {
+				const int k = k0 + 16 * u + kq + t;
+				a[u][t] = f2bf(row < M ? A[(long)row * sa0 + (long)k * sa1] : 0.f);
+				b[u][t] = f2bf(col < N ? B[(long)k * sb0 + (long)col * sb1] : 0.f);
+			}
+		}
+#pragma unroll
+		for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[u], b[u], acc, 0, 0, 0);
+	}
+	for (; k0 < K; k0 += 16) {
 		bf16x4 a, b;
 #pragma unroll
 		for (int t = 0; t < 4; ++t) {
