@@ -198,13 +198,19 @@ __global__ void __launch_bounds__(256) k_block_loss_kron(const float* __restrict
                                                          double* loss_partial, float gscale)
 {
 	constexpr int NB = NV * NPN, NS = NV * NV, NQ = NPN * NPN, NO = RK * (NS + NQ);
-	__shared__ float sA[NB * NB], sP[NB * NB], sR[NB * NB];
+	// 64 x 64 blocks: the products run as 4 x 4 output tiles per thread (256 threads); other sizes one output
+	// per thread-iteration; rows padded to NB + 1 floats against LDS bank conflicts; every output is summed
+	// over k in ascending order
+	constexpr bool TILE = (NB == 64);
+	constexpr int LD = NB + 1;
+	__shared__ float sA[NB * LD], sP[NB * LD], sR[NB * LD];
 	__shared__ float sO[NO];
 	__shared__ double red[256];
 	const int e = blockIdx.x, t = threadIdx.x;
+	const int ti = t / 16, tj = t % 16;   // output rows 4 ti .. 4 ti + 3, columns 4 tj .. 4 tj + 3
 	const float sg = sigma ? sigma[e] : 1.f;
 	const float* o = out + (size_t)e * NO;
-	for (int i = t; i < NB * NB; i += blockDim.x) sA[i] = (float)A[(size_t)e * NB * NB + i];
+	for (int i = t; i < NB * NB; i += blockDim.x) sA[(i / NB) * LD + i % NB] = (float)A[(size_t)e * NB * NB + i];
 	for (int i = t; i < NO; i += blockDim.x) sO[i] = o[i];
 	__syncthreads();
 	const float* S = sO;
@@ -214,16 +220,43 @@ __global__ void __launch_bounds__(256) k_block_loss_kron(const float* __restrict
 		const int a = row / NPN, i = row % NPN, b = col / NPN, j = col % NPN;
 		float p = 0.f;
 		for (int r = 0; r < RK; ++r) p += S[r * NS + a * NV + b] * Q[r * NQ + i * NPN + j];
-		sP[ij] = sg * p;
+		sP[row * LD + col] = sg * p;
 	}
 	__syncthreads();
+	// R = P A - I
 	double lsum = 0.0;
-	for (int ij = t; ij < NB * NB; ij += blockDim.x) {
-		const int i = ij / NB, j = ij % NB;
-		float acc = (i == j) ? -1.f : 0.f;
-		for (int k = 0; k < NB; ++k) acc += sP[i * NB + k] * sA[k * NB + j];
-		sR[ij] = acc;
-		lsum += (double)acc * acc;
+	if constexpr (!TILE) {
+		for (int ij = t; ij < NB * NB; ij += blockDim.x) {
+			const int i = ij / NB, j = ij % NB;
+			float acc = (i == j) ? -1.f : 0.f;
+			for (int k = 0; k < NB; ++k) acc += sP[i * LD + k] * sA[k * LD + j];
+			sR[i * LD + j] = acc;
+			lsum += (double)acc * acc;
+		}
+	} else {
+		float acc[4][4];
+#pragma unroll
+		for (int r = 0; r < 4; ++r)
+#pragma unroll
+			for (int c = 0; c < 4; ++c) acc[r][c] = (4 * ti + r == 4 * tj + c) ? -1.f : 0.f;
+		for (int k = 0; k < NB; ++k) {
+			float pa[4], ab[4];
+#pragma unroll
+			for (int r = 0; r < 4; ++r) pa[r] = sP[(4 * ti + r) * LD + k];
+#pragma unroll
+			for (int c = 0; c < 4; ++c) ab[c] = sA[k * LD + 4 * tj + c];
+#pragma unroll
+			for (int r = 0; r < 4; ++r)
+#pragma unroll
+				for (int c = 0; c < 4; ++c) acc[r][c] += pa[r] * ab[c];
+		}
+#pragma unroll
+		for (int r = 0; r < 4; ++r)
+#pragma unroll
+			for (int c = 0; c < 4; ++c) {
+				sR[(4 * ti + r) * LD + 4 * tj + c] = acc[r][c];
+				lsum += (double)acc[r][c] * acc[r][c];
+			}
 	}
 	red[t] = lsum;
 	__syncthreads();
@@ -233,11 +266,35 @@ __global__ void __launch_bounds__(256) k_block_loss_kron(const float* __restrict
 	}
 	if (t == 0) loss_partial[e] = red[0] / NB;
 	if (!grad) return;
-	for (int ij = t; ij < NB * NB; ij += blockDim.x) {
-		const int i = ij / NB, j = ij % NB;
-		float acc = 0.f;
-		for (int k = 0; k < NB; ++k) acc += sR[i * NB + k] * sA[j * NB + k];
-		sP[ij] = acc * (2.0f / NB) * gscale * sg;
+	// G = sigma * 2 R A^T / nb * gscale  (into sP, no longer needed)
+	if constexpr (!TILE) {
+		for (int ij = t; ij < NB * NB; ij += blockDim.x) {
+			const int i = ij / NB, j = ij % NB;
+			float acc = 0.f;
+			for (int k = 0; k < NB; ++k) acc += sR[i * LD + k] * sA[j * LD + k];
+			sP[i * LD + j] = acc * (2.0f / NB) * gscale * sg;
+		}
+	} else {
+		float acc[4][4];
+#pragma unroll
+		for (int r = 0; r < 4; ++r)
+#pragma unroll
+			for (int c = 0; c < 4; ++c) acc[r][c] = 0.f;
+		for (int k = 0; k < NB; ++k) {
+			float ra[4], aa[4];
+#pragma unroll
+			for (int r = 0; r < 4; ++r) ra[r] = sR[(4 * ti + r) * LD + k];
+#pragma unroll
+			for (int c = 0; c < 4; ++c) aa[c] = sA[(4 * tj + c) * LD + k];
+#pragma unroll
+			for (int r = 0; r < 4; ++r)
+#pragma unroll
+				for (int c = 0; c < 4; ++c) acc[r][c] += ra[r] * aa[c];
+		}
+#pragma unroll
+		for (int r = 0; r < 4; ++r)
+#pragma unroll
+			for (int c = 0; c < 4; ++c) sP[(4 * ti + r) * LD + 4 * tj + c] = acc[r][c] * (2.0f / NB) * gscale * sg;
 	}
 	__syncthreads();
 	float* g = grad + (size_t)e * NO;
@@ -245,14 +302,14 @@ __global__ void __launch_bounds__(256) k_block_loss_kron(const float* __restrict
 		const int r = x / NS, a = (x % NS) / NV, b = x % NV;
 		float acc = 0.f;
 		for (int i = 0; i < NPN; ++i)
-			for (int j = 0; j < NPN; ++j) acc += sP[(a * NPN + i) * NB + b * NPN + j] * Q[r * NQ + i * NPN + j];
+			for (int j = 0; j < NPN; ++j) acc += sP[(a * NPN + i) * LD + b * NPN + j] * Q[r * NQ + i * NPN + j];
 		g[x] = acc;
 	}
 	for (int x = t; x < RK * NQ; x += blockDim.x) {
 		const int r = x / NQ, i = (x % NQ) / NPN, j = x % NPN;
 		float acc = 0.f;
 		for (int a = 0; a < NV; ++a)
-			for (int b = 0; b < NV; ++b) acc += sP[(a * NPN + i) * NB + b * NPN + j] * S[r * NS + a * NV + b];
+			for (int b = 0; b < NV; ++b) acc += sP[(a * NPN + i) * LD + b * NPN + j] * S[r * NS + a * NV + b];
 		g[RK * NS + x] = acc;
 	}
 }
