@@ -152,7 +152,7 @@ def test_learned_preconditioner_trains_on_solver_blocks_and_converges(dev, kind)
     feat = torch.empty(cap * m.nf, dtype=torch.float32, device=dev)
     sig = torch.empty(cap, dtype=torch.float32, device=dev)
 
-    # 1. exact LU block-Jacobi (the fallback and the judge), capturing its blocks
+    # 1. exact LU block-Jacobi (the fallback and the method the learned one is measured against), capturing its blocks
     u_bj = torch.from_numpy(u0).to(dev)
     st_bj = ctx.solve_implicit(u_bj.data_ptr(), cfl_initial=1.0, cfl_max=300.0, exit_tol=1e-10, max_newton=120,
                                precond=_lib.PC_BLOCK_JACOBI,
