@@ -346,6 +346,51 @@ def test_hp_graph_replays_equal_eager_calls(dev):
         ctx.close()
 
 
+def test_hp_residual_on_a_caller_stream(dev):
+    """A variable-order residual spreads its launches over the context's side streams and joins them back to the
+    caller's stream: on a caller stream other than the default one, work enqueued on that stream right after
+    the call (a copy of the residual) sees the whole residual, eagerly and through a graph, bitwise equal to
+    the same call on the default stream."""
+    from dpgsolver_amd import _lib
+
+    n = (6, 5)
+    pe = M.config4_hp_orders(n, seed=23)
+    mesh = M.structured_hp_mesh(pe, n, (0, 0), (1, 1), (True, True), curve=_curve)
+    side = torch.cuda.Stream(device=dev)
+    for pde in (M.PDE_EULER, M.PDE_NAVIER_STOKES):
+        ctx = _lib.Context(2, 5, pde, M.FLUX_ROE, mu=MU, Pr=PR)
+        ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        ctx.upload_mesh_hp(mesh)
+        u64 = torch.from_numpy(M.hp_state(mesh, _state_fn(7))).to(dev)
+        u32 = u64.float()
+        calls = [(ctx.compute_rhs, u64, torch.float64), (ctx.compute_rhs_f32, u32, torch.float32),
+                 (ctx.compute_rhs_mixed, u64, torch.float32)]
+        refs = []
+        for fn, ui, dt in calls:
+            r = torch.empty(u64.shape, dtype=dt, device=dev)
+            fn(ui.data_ptr(), r.data_ptr())
+            refs.append(r)
+        torch.cuda.synchronize()
+        side.wait_stream(torch.cuda.current_stream())
+        ctx.set_stream(side.cuda_stream)
+        with torch.cuda.stream(side):
+            for rep in range(2):
+                for (fn, ui, dt), ref in zip(calls, refs):
+                    r = torch.full(u64.shape, float("nan"), dtype=dt, device=dev)
+                    fn(ui.data_ptr(), r.data_ptr())
+                    seen = r.clone()   # enqueued on the caller stream right after the residual
+                    side.synchronize()
+                    assert torch.equal(seen, ref), (pde, fn.__name__, rep)
+            out = torch.full(u64.shape, float("nan"), dtype=torch.float64, device=dev)
+            g = ctx.graph_rhs(_lib.GRAPH_RHS_FP64, u64.data_ptr(), out.data_ptr())
+            g.launch(1)
+            seen = out.clone()
+            side.synchronize()
+            assert torch.equal(seen, refs[0]), (pde, "graph")
+            g.close()
+        ctx.close()
+
+
 def test_hp_failed_upload_leaves_no_mesh(dev):
     """An upload that fails in a group (a viscous mesh with boundary faces) leaves the context without a mesh:
     the residual then reports DGX_ERR_NO_MESH instead of running on a half-built variable-order state."""
