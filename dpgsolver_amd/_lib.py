@@ -20,7 +20,7 @@ _ERRORS = {-1: "DGX_ERR_ARG", -2: "DGX_ERR_HIP", -3: "DGX_ERR_UNSUPPORTED", -4: 
 
 # every symbol include/dgx.h declares
 EXPORTS = [
-    "dgx_version", "dgx_device_count", "dgx_create", "dgx_destroy", "dgx_set_stream", "dgx_synchronize",
+    "dgx_face_permutation", "dgx_version", "dgx_device_count", "dgx_create", "dgx_destroy", "dgx_set_stream", "dgx_synchronize",
     "dgx_sizes", "dgx_upload_mesh", "dgx_upload_mesh_hp", "dgx_get_geometry", "dgx_compute_rhs", "dgx_compute_rhs_f32", "dgx_compute_rhs_mixed",
     "dgx_compute_rhs_explicit", "dgx_apply_minv", "dgx_enforce_positivity", "dgx_ssp_rk3_step",
     "dgx_graph_create_rhs", "dgx_graph_create_rk3", "dgx_graph_launch", "dgx_graph_sync", "dgx_graph_destroy",
@@ -92,6 +92,7 @@ def load():
     L = C.CDLL(LIB_PATH)
     L.dgx_version.restype = C.c_char_p
     sig = {
+        "dgx_face_permutation": [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)],
         "dgx_device_count": [C.POINTER(C.c_int)],
         "dgx_create": [C.POINTER(vp), C.POINTER(DgxConfig)],
         "dgx_destroy": [vp],
@@ -431,6 +432,15 @@ class MLP:
 
     def attach(self, ctx: "Context"):
         check("dgx_attach_mlp", self.L.dgx_attach_mlp(ctx.h, self.h))
+
+
+def face_permutation(d: int, p: int, ord_: int) -> np.ndarray:
+    """dgx_face_permutation (host only): perm[i] = the right-face node that coincides with left-face node i."""
+    L = load()
+    n = (p + 1) ** (d - 1)
+    out = np.zeros(n, dtype=np.int32)
+    check("dgx_face_permutation", L.dgx_face_permutation(d, p, ord_, out.ctypes.data_as(C.POINTER(C.c_int))))
+    return out
 
 
 def read_gmsh(path: str, p: int) -> dict:

@@ -172,8 +172,11 @@ dgx_status upload_uniform(dgx_ctx* h, const dgx_mesh* m, bool nonconf_ok)
 	std::vector<dgx::ElemFace> ef((size_t)m->E * NFACE, dgx::ElemFace{ -1, -1, -1, 0 });
 	for (int f = 0; f < m->F; ++f) {
 		const int* r = m->faces + 6 * f;
-		const int eL = r[0], fL = r[1], eR = r[2], fR = r[3], ord = r[4], bc = r[5];
-		if (eL < 0 || eL >= m->E || fL < 0 || fL >= NFACE || ord < 0 || ord > 7) return DGX_ERR_ARG;
+		const int eL = r[0], fL = r[1], eR = r[2], fR = r[3], bc = r[5];
+		if (eL < 0 || eL >= m->E || fL < 0 || fL >= NFACE) return DGX_ERR_ARG;
+		// the reference's ind_ord -> permutation bits; boundary faces carry none (the reference writes -1)
+		const int ord = (bc == DGX_BC_INTERIOR) ? dgx::ord_bits(d, r[4]) : 0;
+		if (ord < 0 || (bc != DGX_BC_INTERIOR && r[4] != -1 && dgx::ord_bits(d, r[4]) < 0)) return DGX_ERR_ARG;
 		if (bc == DGX_BC_INTERIOR) {
 			if (eR < 0 || eR >= m->E || fR < 0 || fR >= NFACE) return DGX_ERR_ARG;
 		} else if (nonconf_ok && bc == dgx::kBcNonconforming) {
@@ -460,7 +463,12 @@ static dgx_status upload_hp_impl(dgx_ctx* h, const dgx_mesh_hp* m)
 	for (int f = 0; f < m->F; ++f) {
 		const int* r = m->faces + 6 * f;
 		const int eL = r[0], fL = r[1], eR = r[2], fR = r[3], ord = r[4], bc = r[5];
-		if (eL < 0 || eL >= E || fL < 0 || fL >= 4 || ord < 0 || ord > 7) return DGX_ERR_ARG;
+		if (eL < 0 || eL >= E || fL < 0 || fL >= 4) return DGX_ERR_ARG;
+		// ord stays the reference's code in the group face tables (upload_uniform translates it); the faces
+		// between orders use its permutation bits
+		if ((bc == DGX_BC_INTERIOR && dgx::ord_bits(2, ord) < 0) || (bc != DGX_BC_INTERIOR && ord != -1 && ord != 0))
+			return DGX_ERR_ARG;
+		const int obits = (bc == DGX_BC_INTERIOR) ? dgx::ord_bits(2, ord) : 0;
 		const int gL = grp[eL];
 		auto push = [&](int g, int a0, int a1, int a2, int a3, int a4, int a5) {
 			const int row[6] = { a0, a1, a2, a3, a4, a5 };
@@ -484,7 +492,7 @@ static dgx_status upload_hp_impl(dgx_ctx* h, const dgx_mesh_hp* m)
 		x.uR = uoff[eR];
 		x.zL = z_off[gL] + ((long long)(eL - e_off[gL]) * 4 + fL) * zblk * NL;
 		x.zR = z_off[gR] + ((long long)(eR - e_off[gR]) * 4 + fR) * zblk * NR;
-		x.info = NL | (NR << 3) | (fL << 6) | (fR << 8) | (ord << 10);
+		x.info = NL | (NR << 3) | (fL << 6) | (fR << 8) | (obits << 10);
 		x.eL = eL - e_off[gL];
 		x.eR = eR - e_off[gR];
 		hf.push_back(x);
@@ -499,7 +507,7 @@ static dgx_status upload_hp_impl(dgx_ctx* h, const dgx_mesh_hp* m)
 		outward_nj(m->xyz + xoff[eL], NL, fL, xg, Nf, lo);
 		outward_nj(m->xyz + xoff[eR], NR, fR, xg, Nf, tmp);
 		for (int q = 0; q < Nf; ++q) {
-			const int qr = (ord & 2) ? Nf - 1 - q : q;
+			const int qr = (obits & 2) ? Nf - 1 - q : q;
 			ro[2 * q] = -tmp[2 * qr];
 			ro[2 * q + 1] = -tmp[2 * qr + 1];
 		}

@@ -32,8 +32,22 @@ struct Ops1D {
 	T Lift[2][kMaxN];        // I^{-T} e_side : face lift expressed at the GL volume nodes
 	T w[kMaxN];              // GL weights
 	T wgll[kMaxN];           // GLL weights
-	int perm[8][kMaxNF];     // face orientation permutations
+	int perm[8][kMaxNF];     // face orientation permutations, indexed by ord_bits()
 };
+
+// The reference's face orientation index ind_ord at the ABI (the case of the face node correspondence,
+// src/element/nodes_correspondence.c) -> the orientation bits that index Ops1D::perm (bit0 transpose, bit1
+// reverse the first face coordinate, bit2 reverse the second); -1 for a code that means nothing in dimension d.
+// LINE faces (d = 2): 0 aligned, 1 reversed.  QUAD faces (d = 3): cases 0-3 reverse the first and/or second
+// coordinate, cases 4-7 are the transposed numbering, case 5 = transpose then reverse the first, case 6 =
+// transpose then reverse the second.
+inline int ord_bits(int d, int ind_ord)
+{
+	if (d == 1) return ind_ord == 0 ? 0 : -1;
+	if (d == 2) return ind_ord == 0 ? 0 : (ind_ord == 1 ? 2 : -1);
+	static const int kBits[8] = { 0, 2, 4, 6, 1, 3, 5, 7 };
+	return (ind_ord >= 0 && ind_ord < 8) ? kBits[ind_ord] : -1;
+}
 
 // Variable-order 2-D meshes: orders 1..kHpMaxP, faces between elements of different orders.
 constexpr int kHpMaxN = 6;

@@ -206,7 +206,7 @@ dgx_status dgx_mesh_read_gmsh(const char* path, int p, dgx_host_mesh* out)
 				v2lfwp[f1] = v2lf[f1];
 			}
 			// same direction along the face (after the periodic map) or reversed
-			const int ord = (cor[f0][0] == cor[f1][0]) ? 0 : 2;
+			const int ord = (cor[f0][0] == cor[f1][0]) ? 0 : 1;   // ind_ord: LINE face aligned or reversed
 			const int row[6] = { f0 / 4, f0 % 4, f1 / 4, f1 % 4, ord, DGX_BC_INTERIOR };
 			faces.insert(faces.end(), row, row + 6);
 		} else if (fs.size() == 1) {

@@ -261,3 +261,17 @@ std::vector<double> min_edge_lengths(int d, int N, int E, const double* xyz)
 }
 
 }  // namespace dgx
+
+// Host-only: the face node permutation the library applies for an orientation code at the ABI.
+extern "C" dgx_status dgx_face_permutation(int d, int p, int ord, int* perm)
+{
+	if ((d != 2 && d != 3) || p < 1 || p > dgx::kMaxN - 1 || !perm) return DGX_ERR_ARG;
+	const int bits = dgx::ord_bits(d, ord);
+	if (bits < 0) return DGX_ERR_ARG;
+	dgx::Ops1D<double> o;
+	dgx::Ops1D<float> o32;
+	dgx::build_ops(d, p, o, o32);
+	const int N = p + 1, NF = (d == 3) ? N * N : N;
+	for (int i = 0; i < NF; ++i) perm[i] = o.perm[bits][i];
+	return DGX_OK;
+}

@@ -9,8 +9,8 @@ generated in memory and flattened into the arrays the C-ABI takes:
 * ``xyz``   (E, d, Np) float64 -- isoparametric geometry nodes at the GLL solution
   nodes, r-fastest ordering (element_operators_tp.c:90-131).
 * ``faces`` (F, 6) int32 -- ``eL, fL, eR, fR, ord, bc`` with TP local faces
-  ``2*dim + side`` (side 0 = -1, 1 = +1) and ``ord`` the orientation bits
-  (bit0 transpose, bit1 flip first face coordinate, bit2 flip second).
+  ``2*dim + side`` (side 0 = -1, 1 = +1) and ``ord`` the reference's face
+  orientation index ind_ord (``face_perm``).
 
 Face node ordering: the face coordinates are the remaining volume dimensions in
 increasing order, first fastest.  ``perm[ord][i]`` is the right-face index of the
@@ -75,28 +75,32 @@ def face_node_volume_index(d: int, N: int, lf: int) -> np.ndarray:
 
 
 def face_perm(d: int, N: int, ord_: int) -> np.ndarray:
+    """perm[i] = the right-face node that coincides with left-face node i, for the reference's orientation index
+    ind_ord (the face node correspondence of src/element/nodes_correspondence.c: LINE faces 0 aligned, 1
+    reversed; QUAD faces, node (a, b) with a fastest: cases 4-7 transposed, cases 1, 3, 6, 7 reverse a, cases
+    2, 3, 5, 7 reverse b)."""
     nf = N ** (d - 1)
     perm = np.empty(nf, dtype=np.int64)
     if d == 1:
         perm[0] = 0
         return perm
     if d == 2:
-        for a in range(N):
-            perm[a] = N - 1 - a if ord_ & 2 else a
+        if ord_ not in (0, 1):
+            raise ValueError(f"LINE face orientation {ord_} (0 or 1)")
+        perm[:] = np.arange(N)[::-1] if ord_ == 1 else np.arange(N)
         return perm
+    if not 0 <= ord_ < 8:
+        raise ValueError(f"QUAD face orientation {ord_} (0..7)")
     for b in range(N):
         for a in range(N):
-            ap, bp = (b, a) if ord_ & 1 else (a, b)
-            if ord_ & 2:
-                ap = N - 1 - ap
-            if ord_ & 4:
-                bp = N - 1 - bp
-            perm[a + N * b] = ap + N * bp
+            a2 = N - 1 - a if ord_ in (1, 3, 6, 7) else a
+            b2 = N - 1 - b if ord_ in (2, 3, 5, 7) else b
+            perm[a + N * b] = (b2 + N * a2) if ord_ >= 4 else (a2 + N * b2)
     return perm
 
 
 def n_orientations(d: int) -> int:
-    return {1: 1, 2: 4, 3: 8}[d]
+    return {1: 1, 2: 2, 3: 8}[d]
 
 
 # --------------------------------------------------------------------------- rng
@@ -510,7 +514,7 @@ def structured_hp_mesh(p_elem, n: tuple, lo: tuple, hi: tuple, periodic: tuple, 
                        bc_low=None, bc_high=None) -> HpMesh:
     """Structured 2-D QUAD grid (i fastest) with per-element orders p_elem (grid order), reordered so that the
     orders are non-decreasing (stable).  Each element carries its own isoparametric GLL geometry nodes; a face
-    is matched by its two end nodes, which every order shares (ord 0 or 2: same or reversed direction)."""
+    is matched by its two end nodes, which every order shares (ord 0 or 1: same or reversed direction)."""
     d = 2
     p_grid = np.asarray(p_elem, dtype=np.int32)
     E = int(np.prod(n))
@@ -581,7 +585,7 @@ def structured_hp_mesh(p_elem, n: tuple, lo: tuple, hi: tuple, periodic: tuple, 
             if np.abs(wrap(aL - aR)).max() < 1e-9 and np.abs(wrap(bL - bR)).max() < 1e-9:
                 found = (fR, 0)
             elif np.abs(wrap(aL - bR)).max() < 1e-9 and np.abs(wrap(bL - aR)).max() < 1e-9:
-                found = (fR, 2)
+                found = (fR, 1)
             if found:
                 break
         if found is None:

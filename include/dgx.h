@@ -54,7 +54,18 @@ typedef struct dgx_mesh {
 	int E;              /* elements (QUAD/HEX, one p) */
 	int F;              /* faces */
 	const double* xyz;  /* HOST, E x d x Np: isoparametric geometry nodes at the GLL solution nodes */
-	const int* faces;   /* HOST, F x 6: eL, fL, eR, fR, ord, bc  (face.h:32-70 neigh_info) */
+	const int* faces;   /* HOST, F x 6: eL, fL, eR, fR, ord, bc  (face.h:32-70 neigh_info): eL/fL = neigh_info[0]
+	                       {volume, ind_lf}, eR/fR = neigh_info[1]; ord = the face orientation index ind_ord, the
+	                       case of the reference's face node correspondence (constructor_face_corr,
+	                       element/nodes_correspondence.c; input/testing/unit/nodes/nodes_correspondence.data)
+	                       that takes the right trace into the left ordering, s_l[i] = s_r[perm[i]]
+	                       (permute_Multiarray_T_fc, compute_face_rlhs_T.c:312-318; dgx_face_permutation gives
+	                       perm).  2-D: 0 aligned, 1 reversed (Face::neigh_info[0].ind_ord, equal to
+	                       neigh_info[1].ind_ord for LINE faces).  3-D: 0..7, Face::neigh_info[0].ind_ord (from
+	                       the left-to-right vertex match, face.c:498-540; neigh_info[1] holds the inverse case,
+	                       which differs only for cases 5 and 6).
+	                       Boundary faces: -1 (as the reference stores them) or any valid code, ignored.  Any
+	                       other value: DGX_ERR_ARG. */
 } dgx_mesh;
 
 /* 2-D QUAD mesh with one order per element (Solver_Volume_T::p_ref, volume_solver_T.h:31-104). */
@@ -80,6 +91,11 @@ typedef struct dgx_host_mesh {
 	int n_unsupported_bc; /* boundary faces whose tag is neither Riemann (x101) nor slip wall (x102) */
 	int n_inverted;       /* elements numbered clockwise (det J <= 0 in the straight map) */
 } dgx_host_mesh;
+
+/* Host only: perm[i] (N^(d-1) ints, N = p + 1) = the right-face node that coincides with left-face node i for
+ * orientation code ord of the face table, as the library applies it (constructor_face_corr,
+ * element/nodes_correspondence.c).  DGX_ERR_ARG for a code that means nothing in dimension d. */
+dgx_status dgx_face_permutation(int d, int p, int ord, int* perm);
 
 /* Library/version and device helpers. */
 const char* dgx_version(void);

@@ -250,21 +250,28 @@ static int face_vol_node(int d, int N, int lf, int a, int b)
 	return idx[0] + N * (idx[1] + N * idx[2]);
 }
 
+/* Face node correspondence for the reference's orientation index ind_ord (constructor_face_corr,
+ * src/element/nodes_correspondence.c; golden table input/testing/unit/nodes/nodes_correspondence.data):
+ * perm[i] is the right-face node that coincides with left-face node i, so a right trace is taken into the
+ * left ordering as s_l[i] = s_r[perm[i]] (permute_Multiarray_T_fc, compute_face_rlhs_T.c:312-318).
+ * LINE faces (2-D): 0 aligned, 1 reversed.  QUAD faces (3-D), face node (a, b) with a fastest: cases 4-7 start
+ * from the transposed numbering, cases 1, 3, 6, 7 reverse within each row of N (the first face coordinate)
+ * and cases 2, 3, 5, 7 reverse the order of the rows (the second). */
 void or_face_perm(const or_ctx* c, int ord, int* perm)
 {
 	const int N = c->N;
 	if (c->d == 1) { perm[0] = 0; return; }
 	if (c->d == 2) {
-		for (int a = 0; a < N; ++a) perm[a] = (ord & 2) ? N - 1 - a : a;
+		for (int a = 0; a < N; ++a) perm[a] = (ord == 1) ? N - 1 - a : a;
 		return;
 	}
+	const int transposed = (ord >= 4);
+	const int rev_rows = (ord == 1 || ord == 3 || ord == 6 || ord == 7);
+	const int rev_blocks = (ord == 2 || ord == 3 || ord == 5 || ord == 7);
 	for (int b = 0; b < N; ++b)
 		for (int a = 0; a < N; ++a) {
-			int ap = a, bp = b;
-			if (ord & 1) { ap = b; bp = a; }
-			if (ord & 2) ap = N - 1 - ap;
-			if (ord & 4) bp = N - 1 - bp;
-			perm[a + N * b] = ap + N * bp;
+			const int a2 = rev_rows ? N - 1 - a : a, b2 = rev_blocks ? N - 1 - b : b;
+			perm[a + N * b] = transposed ? (b2 + N * a2) : (a2 + N * b2);
 		}
 }
 
@@ -457,7 +464,7 @@ or_ctx* or_create(int d, int p, int pde, int flux, int E, int F, const double* x
 	}
 	(void)face_vol_node;
 
-	c->n_ord = (d == 3) ? 8 : (d == 2 ? 4 : 1);
+	c->n_ord = (d == 3) ? 8 : (d == 2 ? 2 : 1);
 	c->perm = (int*)malloc(sizeof(int) * c->n_ord * Nfc);
 	for (int o = 0; o < c->n_ord; ++o) or_face_perm(c, o, c->perm + (size_t)o * Nfc);
 

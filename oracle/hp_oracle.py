@@ -126,7 +126,7 @@ class HpOracle:
         NR = NL if bc != 0 else int(m.p[eR]) + 1
         Nf = max(NL, NR)
         xq, wq = O.gauss_legendre(Nf)
-        perm = (Nf - 1 - np.arange(Nf)) if ord_ & 2 else np.arange(Nf)   # right index of left point q
+        perm = (Nf - 1 - np.arange(Nf)) if ord_ == 1 else np.arange(Nf)   # right index of left point q
         IL, _ = O.lagrange(ops(NL).xl, xq)                   # (Nf, NL): left face basis at the face points
         UL = self.element(u, eL)[:, face_nodes(NL, fL)]      # (nv, NL)
         sL = (IL @ UL.T)                                     # (Nf, nv) left ordering
@@ -182,7 +182,7 @@ class HpOracle:
         NL, NR = int(m.p[eL]) + 1, int(m.p[eR]) + 1
         Nf = max(NL, NR)
         xq, wq = O.gauss_legendre(Nf)
-        perm = (Nf - 1 - np.arange(Nf)) if ord_ & 2 else np.arange(Nf)
+        perm = (Nf - 1 - np.arange(Nf)) if ord_ == 1 else np.arange(Nf)
         IL, _ = O.lagrange(ops(NL).xl, xq)
         IR, _ = O.lagrange(ops(NR).xl, xq)
         if NL <= NR:

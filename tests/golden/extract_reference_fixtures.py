@@ -1,12 +1,18 @@
-"""Writes tests/golden/{nodes_tp,operators_tp}.json from the reference's unit-test data files.
+"""Writes tests/golden/*.json from the reference's unit- and integration-test data files.
 
 Reads (data only, no reference code is run):
   input/testing/unit/nodes/nodes_tp.data          (used by src/testing/unit/test_unit_nodes.c)
   input/testing/unit/operators/operators_tp.data  (used by src/testing/unit/test_unit_operators_tp.c:88-170)
+  input/testing/unit/nodes/nodes_correspondence.data (face node permutations per ind_ord,
+                                                  src/element/nodes_correspondence.c)
+  input/testing/integration/fe/*.fe.data          (Face::neigh_info{ind_v, ind_lf, ind_ord} of the shipped meshes,
+                                                  compared by src/testing/support/test_support_fe_init.c:222)
 under the DPGSolver checkout given as argv[1] (default /root/reference).
 """
+import glob
 import json
 import os
+import re
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -84,6 +90,55 @@ def operators(path):
     return out
 
 
+def _read_vector_i_list(lines, i):
+    """lines[i] is 'container Multiarray_Vector_i'; returns (list of int rows, next index)."""
+    assert lines[i].strip() == "container Multiarray_Vector_i", lines[i]
+    n = int(lines[i + 2].split()[1])
+    assert lines[i + 3].strip() == "ext_0/data"
+    rows = []
+    for k in range(n):
+        vals = [int(v) for v in lines[i + 4 + k].split()]
+        assert vals[0] == len(vals) - 1
+        rows.append(vals[1:])
+    return rows, i + 4 + n
+
+
+def correspondence(path):
+    lines = open(path).read().splitlines()
+    out = {}
+    for i, s in enumerate(lines):
+        s = s.strip()
+        if re.fullmatch(r"(point|line|quad|tri)_\d+:", s):
+            out[s[:-1]], _ = _read_vector_i_list(lines, i + 1)
+    return out
+
+
+def fe_faces(path):
+    """Per face: [boundary, v0, lf0, ord0, v1, lf1, ord1] from a .fe.data file."""
+    lines = [x.strip() for x in open(path).read().splitlines()]
+    out = []
+    i = 0
+    while i < len(lines):
+        if lines[i].startswith("Face "):
+            bnd = int(lines[i + 1].split()[1])
+            rec = [bnd]
+            j = i + 1
+            for side in (0, 1):
+                while lines[j] != f"Neigh_Info {side}":
+                    j += 1
+                kv = {}
+                for k in range(1, 6):
+                    key, val = lines[j + k].split()
+                    kv[key] = int(val)
+                rec += [kv["ind_v"], kv["ind_lf"], kv["ind_ord"]]
+                j += 6
+            out.append(rec)
+            i = j
+        else:
+            i += 1
+    return out
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     unit = os.path.join(ref, "input", "testing", "unit")
@@ -93,7 +148,17 @@ def main():
         json.dump({"source": "input/testing/unit/nodes/nodes_tp.data", "nodes": n}, f, indent=0)
     with open(os.path.join(HERE, "operators_tp.json"), "w") as f:
         json.dump({"source": "input/testing/unit/operators/operators_tp.data", "operators": o}, f, indent=0)
-    print("nodes:", sorted(n), "operators:", sorted(o))
+    c = correspondence(os.path.join(unit, "nodes", "nodes_correspondence.data"))
+    with open(os.path.join(HERE, "nodes_correspondence.json"), "w") as f:
+        json.dump({"source": "input/testing/unit/nodes/nodes_correspondence.data", "corr": c}, f, indent=0)
+    fe = {}
+    for path in sorted(glob.glob(os.path.join(ref, "input", "testing", "integration", "fe", "*.fe.data"))):
+        fe[os.path.basename(path)[:-len(".fe.data")]] = fe_faces(path)
+    with open(os.path.join(HERE, "fe_faces.json"), "w") as f:
+        json.dump({"source": "input/testing/integration/fe/*.fe.data",
+                   "columns": ["boundary", "ind_v0", "ind_lf0", "ind_ord0", "ind_v1", "ind_lf1", "ind_ord1"],
+                   "faces": fe}, f, indent=0)
+    print("nodes:", sorted(n), "operators:", sorted(o), "corr:", sorted(c), "fe:", sorted(fe))
 
 
 if __name__ == "__main__":
