@@ -855,7 +855,9 @@ extern "C" dgx_status dgx_solve_implicit(dgx_ctx* h, double* u, const dgx_implic
 			if (dgx::launch_fill_u64(c.stream, c.dmax, 1, 0ull) != hipSuccess ||
 			    dgx::launch_maxabs(c, r, n, c.dmax) != hipSuccess) { fail(DGX_ERR_HIP); break; }
 			unsigned long long bits = 0;
-			if (hipMemcpy(&bits, c.dmax, sizeof(bits), hipMemcpyDeviceToHost) != hipSuccess) { fail(DGX_ERR_HIP); break; }
+			// read on the stream that wrote it (a non-blocking caller stream does not order with the null stream)
+			if (hipMemcpyAsync(&bits, c.dmax, sizeof(bits), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+			    hipStreamSynchronize(c.stream) != hipSuccess) { fail(DGX_ERR_HIP); break; }
 			std::memcpy(&mx, &bits, sizeof(mx));
 			if (!(mx == mx) || mx > 1e300) { fail(DGX_ERR_NONFINITE); break; }
 			if (it == 0) max_rhs0 = mx;
@@ -901,7 +903,8 @@ extern "C" dgx_status dgx_solve_implicit(dgx_ctx* h, double* u, const dgx_implic
 			    dgx::launch_relaxed_update(c, u, gm.x, status) != hipSuccess ||
 			    dgx::launch_positivity(c, u, status) != hipSuccess) { fail(DGX_ERR_HIP); break; }
 			int bad = 0;
-			if (hipMemcpy(&bad, status, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) { fail(DGX_ERR_HIP); break; }
+			if (hipMemcpyAsync(&bad, status, sizeof(int), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+			    hipStreamSynchronize(c.stream) != hipSuccess) { fail(DGX_ERR_HIP); break; }
 			if (bad) { fail(DGX_ERR_POSITIVITY); break; }
 			st->newton_steps = it + 1;
 		}

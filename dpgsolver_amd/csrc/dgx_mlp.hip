@@ -708,13 +708,17 @@ dgx_status dgx_mlp_sizes(const dgx_mlp* m, int* nf, int* nout, long long* nparam
 dgx_status dgx_mlp_get_params(dgx_mlp* m, float* host)
 {
 	if (!m || !host) return DGX_ERR_ARG;
-	return hipMemcpy(host, m->w, sizeof(float) * m->nparam, hipMemcpyDeviceToHost) == hipSuccess ? DGX_OK : DGX_ERR_HIP;
+	// on the model's stream: the training kernels that update m->w run there
+	if (hipMemcpyAsync(host, m->w, sizeof(float) * m->nparam, hipMemcpyDeviceToHost, m->s) != hipSuccess ||
+	    hipStreamSynchronize(m->s) != hipSuccess)
+		return DGX_ERR_HIP;
+	return DGX_OK;
 }
 
 dgx_status dgx_mlp_set_params(dgx_mlp* m, const float* host)
 {
 	if (!m || !host) return DGX_ERR_ARG;
-	if (hipMemcpy(m->w, host, sizeof(float) * m->nparam, hipMemcpyHostToDevice) != hipSuccess) return DGX_ERR_HIP;
+	if (hipMemcpyAsync(m->w, host, sizeof(float) * m->nparam, hipMemcpyHostToDevice, m->s) != hipSuccess) return DGX_ERR_HIP;
 	if (dgx::mlp_sync_bf16(m) != hipSuccess || hipStreamSynchronize(m->s) != hipSuccess) return DGX_ERR_HIP;
 	return DGX_OK;
 }

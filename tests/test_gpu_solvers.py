@@ -387,3 +387,64 @@ def test_steady_channel_block_sgs_fewer_iterations(dev):
     assert _rel(out[_lib.PC_BLOCK_SGS32][1], b) < 1e-7
     assert out[_lib.PC_BLOCK_SGS32][0]["gmres_iters"] <= 1.1 * out[_lib.PC_BLOCK_SGS][0]["gmres_iters"] + 5
     ctx.close()
+
+
+def test_steady_solve_on_a_nonblocking_stream_matches_default_stream(dev):
+    """The solver reads its norms and flags on the context's stream: a non-blocking caller stream (which does
+    not order with the null stream) gives the same solve as the default stream."""
+    from dpgsolver_amd import _lib
+
+    mesh = M.channel_mesh(12, 6, 2)
+    prim = M.freestream_prim(2)
+    out = []
+    for stream in (torch.cuda.default_stream(dev), torch.cuda.Stream(dev)):
+        with torch.cuda.stream(stream):
+            ctx = _lib.Context(2, 2, M.PDE_EULER, M.FLUX_ROE, bc_prim=prim)
+            ctx.set_stream(stream.cuda_stream)
+            ctx.upload_mesh(mesh.xyz, mesh.faces)
+            ud = torch.from_numpy(M.uniform_state(mesh, prim)).to(dev)
+            st = ctx.solve_implicit(ud.data_ptr(), cfl_initial=10.0, exit_tol=1e-10, max_newton=80,
+                                    precond=_lib.PC_BLOCK_JACOBI)
+            stream.synchronize()
+            out.append((st, ud.cpu().numpy()))
+            ctx.close()
+    (s0, u0), (s1, u1) = out
+    assert s0["converged"] == 1 and s1["converged"] == 1
+    assert (s0["newton_steps"], s0["gmres_iters"]) == (s1["newton_steps"], s1["gmres_iters"])
+    assert s0["history"] == s1["history"]
+    np.testing.assert_array_equal(u0, u1)
+
+
+def test_nonfinite_state_is_reported_not_solved(dev):
+    """A NaN in the state makes the residual norm non-finite: DGX_ERR_NONFINITE, no Newton step taken
+    (the reference would carry the NaN on; the library stops with a status code)."""
+    from dpgsolver_amd import _lib
+
+    mesh = M.channel_mesh(6, 4, 2)
+    prim = M.freestream_prim(2)
+    ctx = _ctx(mesh, M.PDE_EULER, M.FLUX_ROE, bc_prim=prim)
+    u = M.uniform_state(mesh, prim)
+    u[17] = np.nan
+    ud = torch.from_numpy(u).to(dev)
+    st = ctx.solve_implicit(ud.data_ptr(), max_newton=5, raise_on_error=False)
+    assert st["status"] == -5 and st["newton_steps"] == 0 and st["converged"] == 0
+    ctx.close()
+
+
+def test_gmres_cap_is_counted_and_newton_continues(dev):
+    """GMRES hitting its iteration cap is PETSc's DIVERGED_ITS: counted, and the Newton loop goes on
+    (solve_implicit.c:668-672); the solve ends DGX_ERR_NOT_CONVERGED when the Newton steps run out."""
+    from dpgsolver_amd import _lib
+
+    mesh = M.channel_mesh(12, 6, 2)
+    prim = M.freestream_prim(2)
+    ctx = _ctx(mesh, M.PDE_EULER, M.FLUX_ROE, bc_prim=prim)
+    u0, _ = _channel_state(mesh, amp=0.05)
+    ud = torch.from_numpy(u0).to(dev)
+    st = ctx.solve_implicit(ud.data_ptr(), cfl_initial=10.0, exit_tol=1e-14, max_newton=4, gmres_max_iter=2,
+                            gmres_rtol=1e-12, precond=_lib.PC_BLOCK_JACOBI, raise_on_error=False)
+    assert st["status"] == -6
+    assert st["newton_steps"] == 4 and st["gmres_not_converged"] == 4
+    assert st["gmres_per_step"] == [2, 2, 2, 2]
+    assert st["history"][3] < st["history"][0]        # the capped steps still reduce the residual
+    ctx.close()
