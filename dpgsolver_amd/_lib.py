@@ -21,7 +21,7 @@ _ERRORS = {-1: "DGX_ERR_ARG", -2: "DGX_ERR_HIP", -3: "DGX_ERR_UNSUPPORTED", -4: 
 # every symbol include/dgx.h declares
 EXPORTS = [
     "dgx_face_permutation", "dgx_version", "dgx_device_count", "dgx_create", "dgx_destroy", "dgx_set_stream", "dgx_synchronize",
-    "dgx_sizes", "dgx_upload_mesh", "dgx_upload_mesh_hp", "dgx_get_geometry", "dgx_compute_rhs", "dgx_compute_rhs_f32", "dgx_compute_rhs_mixed",
+    "dgx_sizes", "dgx_upload_mesh", "dgx_upload_mesh_hp", "dgx_get_geometry", "dgx_compute_rhs", "dgx_compute_volume_rhs", "dgx_compute_face_rhs", "dgx_compute_rhs_f32", "dgx_compute_rhs_mixed",
     "dgx_compute_rhs_explicit", "dgx_apply_minv", "dgx_enforce_positivity", "dgx_ssp_rk3_step",
     "dgx_graph_create_rhs", "dgx_graph_create_rk3", "dgx_graph_launch", "dgx_graph_sync", "dgx_graph_destroy",
     "dgx_set_rhs_grid", "dgx_autotune_rhs", "dgx_max_abs",
@@ -103,6 +103,8 @@ def load():
         "dgx_upload_mesh_hp": [vp, C.POINTER(DgxMeshHp)],
         "dgx_get_geometry": [vp, dp, dp, dp],
         "dgx_compute_rhs": [vp, vp, vp],
+        "dgx_compute_volume_rhs": [vp, vp, vp],
+        "dgx_compute_face_rhs": [vp, vp, vp],
         "dgx_compute_rhs_f32": [vp, vp, vp],
         "dgx_compute_rhs_mixed": [vp, vp, vp],
         "dgx_compute_rhs_explicit": [vp, vp, vp, dp],
@@ -236,6 +238,14 @@ class Context:
     # --- device-pointer entry points
     def compute_rhs(self, u: int, r: int):
         check("dgx_compute_rhs", self.L.dgx_compute_rhs(self.h, vp(u), vp(r)))
+
+    def compute_volume_rhs(self, u: int, r: int):
+        """r += the volume term (compute_volume_rlhs_dg)."""
+        check("dgx_compute_volume_rhs", self.L.dgx_compute_volume_rhs(self.h, vp(u), vp(r)))
+
+    def compute_face_rhs(self, u: int, r: int):
+        """r += the face term (compute_face_rlhs_dg)."""
+        check("dgx_compute_face_rhs", self.L.dgx_compute_face_rhs(self.h, vp(u), vp(r)))
 
     def compute_rhs_f32(self, u: int, r: int):
         check("dgx_compute_rhs_f32", self.L.dgx_compute_rhs_f32(self.h, vp(u), vp(r)))

@@ -688,6 +688,20 @@ static dgx_status read_max(Ctx& c, double* out)
 	return (v != v || v > 1.7e308) ? DGX_ERR_NONFINITE : DGX_OK;
 }
 
+static dgx_status rhs_part(dgx_ctx* h, const double* u, double* r, bool volume)
+{
+	if (!h || !u || !r) return DGX_ERR_ARG;
+	Ctx& c = h->c;
+	if (c.hp) return DGX_ERR_UNSUPPORTED;
+	if (!c.mesh_ready) return DGX_ERR_NO_MESH;
+	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;
+	return hip_status(dgx::launch_rhs_part64(c, u, r, volume));
+}
+
+dgx_status dgx_compute_volume_rhs(dgx_ctx* h, const double* u, double* r) { return rhs_part(h, u, r, true); }
+
+dgx_status dgx_compute_face_rhs(dgx_ctx* h, const double* u, double* r) { return rhs_part(h, u, r, false); }
+
 dgx_status dgx_compute_rhs_explicit(dgx_ctx* h, const double* u, double* r, double* max_abs_r)
 {
 	if (!h || !u || !r) return DGX_ERR_ARG;

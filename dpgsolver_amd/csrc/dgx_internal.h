@@ -187,6 +187,8 @@ std::vector<int> morton_order(int d, int E, int Np, const double* xyz);
 // launchers (each returns hipError_t)
 hipError_t launch_geometry(Ctx& c);
 hipError_t launch_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned long long* dmax);
+// r += the volume term (volume = true) or the face term alone, FP64 (k_rhs_parts.hip)
+hipError_t launch_rhs_part64(Ctx& c, const double* u, double* r, bool volume);
 hipError_t launch_rhs32(Ctx& c, const float* u, float* r);
 hipError_t launch_rhs_ns(Ctx& c, const double* u, double* r);   // 2-D Navier-Stokes (BR2), FP64
 hipError_t launch_rhs_ns32(Ctx& c, const float* u, float* r);    // the same in FP32

@@ -44,7 +44,9 @@ __device__ inline Dual operator*(double a, Dual b) { return Dual(a * b.v, a * b.
 // carries none, so only the face flux of the face shared with n has a derivative; the faces are lifted
 // one at a time and the face-node rows rho = v * NF + a (node face_vol_node(lf, a)) are stored
 // transposed: out[((e * 2d + lf) * NFV + col) * NFV + rho].
-enum { MODE_JVP = 5, MODE_DIAG = 6, MODE_PERT = 7, MODE_OFFD = 8 };
+// MODE_VOL / MODE_FACE: the volume term alone (compute_volume_rlhs_dg) or the face term alone
+// (compute_face_rlhs_dg), added into out (the reference accumulates both into a pre-zeroed rhs).
+enum { MODE_JVP = 5, MODE_DIAG = 6, MODE_PERT = 7, MODE_OFFD = 8, MODE_VOL = 9, MODE_FACE = 10 };
 
 template <typename S, typename R>
 struct RhsArgs {

@@ -149,6 +149,16 @@ dgx_status dgx_get_geometry(dgx_ctx* ctx, double* metrics_vc, double* detJ_vc, d
  * 'e' parameters (solvers/dg/compute_volume_rlhs_dg_T.c:53-72, compute_face_rlhs_dg_T.c:88-114).
  * r is fully overwritten (no pre-zeroing needed, solve_T.c:159-182). */
 dgx_status dgx_compute_rhs(dgx_ctx* ctx, const double* u, double* r);
+/* r += the volume term alone: compute_volume_rlhs_dg with 'e' parameters (solvers/dg/compute_volume_rlhs_dg_T.c:53-72,
+ * compute_volume_rlhs_T.c:145-177), rhs += sum_d tw1_vt_vc[d] fr_d.  Like the reference, it accumulates into r:
+ * the caller zeroes r first (solve_T.c:159-182).  FP64, uniform-order Euler and advection contexts. */
+dgx_status dgx_compute_volume_rhs(dgx_ctx* ctx, const double* u, double* r);
+/* r += the face term alone: compute_face_rlhs_dg with 'e' parameters (solvers/dg/compute_face_rlhs_dg_T.c:88-114,
+ * compute_face_rlhs_T.c:297-359), rhs -= tw0_vt_fc (J_f nnf) on both sides of every face, boundary faces
+ * included.  Accumulates like dgx_compute_volume_rhs; the two calls on a zeroed r give dgx_compute_rhs's r to
+ * rounding (the reference's linearization harness calls them in this order,
+ * testing/integration/test_integration_linearization.c:270-271). */
+dgx_status dgx_compute_face_rhs(dgx_ctx* ctx, const double* u, double* r);
 /* FP32 residual (defect-correction inner residual): FP32 state, FP32 geometry shadow,
  * FP32 arithmetic.  Same semantics as dgx_compute_rhs. */
 dgx_status dgx_compute_rhs_f32(dgx_ctx* ctx, const float* u, float* r);

@@ -115,6 +115,38 @@ def test_fp64_residual_matches_oracle(dev, case):
     ctx.close()
 
 
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_volume_and_face_terms_separately_match_oracle(dev, case):
+    """dgx_compute_volume_rhs / dgx_compute_face_rhs (compute_volume_rlhs_dg / compute_face_rlhs_dg) against the
+    oracle's or_volume_rhs_d / or_face_rhs_d; both accumulate into r like the reference, and the two calls on a
+    zeroed r give the full residual."""
+    name, mesh, pde, flux, b, bcp, u = case
+    print(f"case {name}: E={mesh.E}", flush=True)
+    ctx = _ctx(mesh, pde, flux, b, bcp)
+    orc = O.Oracle(mesh, pde, flux, b, bcp)
+    ud = torch.from_numpy(u).to(dev)
+    vol_ref, face_ref = orc.rhs(u, "volume"), orc.rhs(u, "face")
+    scale = max(np.abs(vol_ref).max(), np.abs(face_ref).max())
+    rv = torch.zeros_like(ud)
+    ctx.compute_volume_rhs(ud.data_ptr(), rv.data_ptr())
+    rf = torch.zeros_like(ud)
+    ctx.compute_face_rhs(ud.data_ptr(), rf.data_ptr())
+    offset = torch.full_like(ud, 0.25)                   # accumulation: r += term
+    ctx.compute_face_rhs(ud.data_ptr(), offset.data_ptr())
+    both = torch.zeros_like(ud)
+    ctx.compute_volume_rhs(ud.data_ptr(), both.data_ptr())
+    ctx.compute_face_rhs(ud.data_ptr(), both.data_ptr())
+    full = torch.empty_like(ud)
+    ctx.compute_rhs(ud.data_ptr(), full.data_ptr())
+    torch.cuda.synchronize()
+    assert np.abs(rv.cpu().numpy() - vol_ref).max() < TOL64 * scale
+    assert np.abs(rf.cpu().numpy() - face_ref).max() < TOL64 * scale
+    # r += f: only the rounding of 0.25 + f (half an ulp of |0.25 + f|) separates it from 0.25 + rf
+    assert np.abs((offset - 0.25 - rf).cpu().numpy()).max() <= 2.0 ** -52 * (0.25 + rf.abs().max().item())
+    assert np.abs((both - full).cpu().numpy()).max() < TOL64 * scale
+    ctx.close()
+
+
 @pytest.mark.parametrize("case", CASES[:6], ids=[c[0] for c in CASES[:6]])
 def test_explicit_rhs_with_mass_inverse(dev, case):
     name, mesh, pde, flux, b, bcp, u = case
