@@ -22,7 +22,7 @@ _ERRORS = {-1: "DGX_ERR_ARG", -2: "DGX_ERR_HIP", -3: "DGX_ERR_UNSUPPORTED", -4: 
 EXPORTS = [
     "dgx_face_permutation", "dgx_version", "dgx_device_count", "dgx_create", "dgx_destroy", "dgx_set_stream", "dgx_synchronize",
     "dgx_sizes", "dgx_upload_mesh", "dgx_upload_mesh_hp", "dgx_get_geometry", "dgx_compute_rhs", "dgx_compute_volume_rhs", "dgx_compute_face_rhs", "dgx_compute_rhs_f32", "dgx_compute_rhs_mixed",
-    "dgx_compute_rhs_explicit", "dgx_apply_minv", "dgx_enforce_positivity", "dgx_ssp_rk3_step",
+    "dgx_compute_rhs_explicit", "dgx_apply_minv", "dgx_enforce_positivity", "dgx_ssp_rk3_step", "dgx_ssp_rk3_step_noalloc",
     "dgx_graph_create_rhs", "dgx_graph_create_rk3", "dgx_graph_launch", "dgx_graph_sync", "dgx_graph_destroy",
     "dgx_set_rhs_grid", "dgx_autotune_rhs", "dgx_max_abs",
     "dgx_convert_f64_f32", "dgx_convert_f32_f64", "dgx_assemble_diag_blocks", "dgx_jacobian_vector",
@@ -111,6 +111,7 @@ def load():
         "dgx_apply_minv": [vp, vp],
         "dgx_enforce_positivity": [vp, vp],
         "dgx_ssp_rk3_step": [vp, vp, C.c_double, C.c_int, dp],
+        "dgx_ssp_rk3_step_noalloc": [vp, vp, C.c_double, C.c_int, dp],
         "dgx_graph_create_rhs": [vp, C.c_int, vp, vp, C.POINTER(vp)],
         "dgx_graph_create_rk3": [vp, vp, C.c_double, C.c_int, C.POINTER(vp)],
         "dgx_graph_launch": [vp, C.c_int],

@@ -104,6 +104,9 @@ dgx_status dgx_create(dgx_ctx** out, const dgx_config* cfg)
 	c.Nfc = (c.d == 3) ? c.N * c.N : c.N;
 	c.nv = (cfg->pde == DGX_PDE_ADVECTION) ? 1 : c.d + 2;
 	if (hipGetDevice(&c.device) != hipSuccess) { delete h; return DGX_ERR_HIP; }
+	// the no-allocation entry point's static scratch array on this device (looked up once per device; a
+	// failure here only disables dgx_ssp_rk3_step_noalloc)
+	if (dgx::noalloc_scratch(c.device, &c.noalloc_scratch) != hipSuccess) c.noalloc_scratch = nullptr;
 	dgx::build_ops(c.d, c.p, c.h_ops64, c.h_ops32);
 	if (hipMalloc(&c.ops64, sizeof(dgx::Ops1D<double>)) != hipSuccess ||
 	    hipMalloc(&c.ops32, sizeof(dgx::Ops1D<float>)) != hipSuccess ||

@@ -157,6 +157,7 @@ struct Ctx {
 	Ops1D<float>* ops32 = nullptr;
 	Ops1D<double> h_ops64{};
 	Ops1D<float> h_ops32{};
+	void* noalloc_scratch = nullptr; // device address of the static scratch array on this context's device
 	double* u0 = nullptr;           // RK stage storage (E*Np*nv)
 	double* rtmp = nullptr;         // residual scratch (E*Np*nv)
 	unsigned long long* dmax = nullptr;  // device reduction slot(s)
@@ -208,6 +209,11 @@ hipError_t launch_maxabs(Ctx& c, const double* r, size_t n, unsigned long long* 
 hipError_t launch_convert(const double* src, float* dst, size_t n, hipStream_t s);
 hipError_t launch_convert_back(const float* src, double* dst, size_t n, hipStream_t s);
 hipError_t launch_rk_stage(Ctx& c, double* u, const double* r, double dt, int stage);
+// the same with the stage-0 copy in u0 (E*Np*nv doubles) instead of c.u0
+hipError_t launch_rk_stage_buf(Ctx& c, double* u, double* u0, const double* r, double dt, int stage);
+// dgx_noalloc.hip: the device address of the static scratch array on device `dev` (the current device);
+// looked up once per device, then cached
+hipError_t noalloc_scratch(int dev, void** ptr);
 hipError_t launch_sub_mass(Ctx& c, double* blocks, const double* scale_e, double scale);
 hipError_t launch_mass_sub(Ctx& c, const double* x, const double* scale, double* y);
 hipError_t launch_cfl_dt(Ctx& c, const double* u, double cfl, double* inv_dt);

@@ -393,12 +393,16 @@ __global__ void k_rk_stage(double* __restrict__ u, double* __restrict__ u0, cons
 		}
 	}
 }
-hipError_t launch_rk_stage(Ctx& c, double* u, const double* r, double dt, int stage)
+hipError_t launch_rk_stage_buf(Ctx& c, double* u, double* u0, const double* r, double dt, int stage)
 {
 	const size_t n = (size_t)c.E * c.Np * c.nv;
 	const int grid = (int)((n + 255) / 256 < 8192 ? (n + 255) / 256 : 8192);
-	if (grid) hipLaunchKernelGGL(k_rk_stage, dim3(grid), dim3(256), 0, c.stream, u, c.u0, r, dt, stage, n);
+	if (grid) hipLaunchKernelGGL(k_rk_stage, dim3(grid), dim3(256), 0, c.stream, u, u0, r, dt, stage, n);
 	return hipGetLastError();
+}
+hipError_t launch_rk_stage(Ctx& c, double* u, const double* r, double dt, int stage)
+{
+	return launch_rk_stage_buf(c, u, c.u0, r, dt, stage);
 }
 
 // ------------------------------------------------------------------------------- mass blocks

@@ -181,6 +181,16 @@ dgx_status dgx_enforce_positivity(dgx_ctx* ctx, double* u);
  * max|M^-1 R| of the last stage.  Synchronous when max_abs_r is given. */
 dgx_status dgx_ssp_rk3_step(dgx_ctx* ctx, double* u, double dt, int enforce_positivity, double* max_abs_r);
 
+/* dgx_ssp_rk3_step without any allocation: no counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, tight time loops, harnesses that forbid hipMalloc): the stage-0
+ * copy, the residual, the max-norm slot and the limiter flag live in a static device array of 64 MiB per GPU
+ * (held until the process ends) instead of buffers the library allocates on the first dgx_ssp_rk3_step.  Same
+ * kernels and results as dgx_ssp_rk3_step.  Uniform-order Euler or advection contexts with at most 4,194,272
+ * state values (E * Np * nv: two stage buffers, each rounded up to 256 B, after a 256 B head); larger:
+ * DGX_ERR_ARG; variable-order or Navier-Stokes contexts: DGX_ERR_UNSUPPORTED.  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (other streams or host threads) race. */
+dgx_status dgx_ssp_rk3_step_noalloc(dgx_ctx* ctx, double* u, double dt, int enforce_positivity, double* max_abs_r);
+
 /* HIP graphs of the launch sequences above, recorded once and replayed with one hipGraphLaunch each: the
  * explicit time loop (solve_explicit.c:78-118 calling time_step_ssp_rk_33 every step) and repeated residual
  * evaluations then cost no per-kernel launch overhead.  A graph binds the pointers and dt it was recorded
