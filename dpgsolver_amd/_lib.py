@@ -20,7 +20,7 @@ _ERRORS = {-1: "DGX_ERR_ARG", -2: "DGX_ERR_HIP", -3: "DGX_ERR_UNSUPPORTED", -4: 
 
 # every symbol include/dgx.h declares
 EXPORTS = [
-    "dgx_face_permutation", "dgx_version", "dgx_device_count", "dgx_create", "dgx_destroy", "dgx_set_stream", "dgx_synchronize",
+    "dgx_face_permutation", "dgx_reference_nodes", "dgx_tri_rule", "dgx_upload_mesh_mixed", "dgx_version", "dgx_device_count", "dgx_create", "dgx_destroy", "dgx_set_stream", "dgx_synchronize",
     "dgx_sizes", "dgx_upload_mesh", "dgx_upload_mesh_hp", "dgx_get_geometry", "dgx_compute_rhs", "dgx_compute_volume_rhs", "dgx_compute_face_rhs", "dgx_compute_rhs_f32", "dgx_compute_rhs_mixed",
     "dgx_compute_rhs_explicit", "dgx_apply_minv", "dgx_enforce_positivity", "dgx_ssp_rk3_step", "dgx_ssp_rk3_step_noalloc",
     "dgx_graph_create_rhs", "dgx_graph_create_rk3", "dgx_graph_launch", "dgx_graph_sync", "dgx_graph_destroy",
@@ -58,6 +58,11 @@ class DgxMesh(C.Structure):
     _fields_ = [("E", C.c_int), ("F", C.c_int), ("xyz", C.POINTER(C.c_double)), ("faces", C.POINTER(C.c_int))]
 
 
+class DgxMeshMixed(C.Structure):
+    _fields_ = [("E", C.c_int), ("F", C.c_int), ("type", C.POINTER(C.c_int)), ("p", C.POINTER(C.c_int)),
+                ("xyz", C.POINTER(C.c_double)), ("faces", C.POINTER(C.c_int))]
+
+
 class DgxMeshHp(C.Structure):
     _fields_ = [("E", C.c_int), ("F", C.c_int), ("p", C.POINTER(C.c_int)), ("xyz", C.POINTER(C.c_double)),
                 ("faces", C.POINTER(C.c_int))]
@@ -93,6 +98,9 @@ def load():
     L.dgx_version.restype = C.c_char_p
     sig = {
         "dgx_face_permutation": [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)],
+        "dgx_reference_nodes": [C.c_int, C.c_int, dp, dp, C.POINTER(C.c_int)],
+        "dgx_tri_rule": [C.c_int, C.c_int, dp, dp, C.POINTER(C.c_int)],
+        "dgx_upload_mesh_mixed": [vp, C.c_void_p],
         "dgx_device_count": [C.POINTER(C.c_int)],
         "dgx_create": [C.POINTER(vp), C.POINTER(DgxConfig)],
         "dgx_destroy": [vp],
@@ -226,6 +234,21 @@ class Context:
         m.xyz = xyz.ctypes.data_as(dp)
         m.faces = faces.ctypes.data_as(C.POINTER(C.c_int))
         check("dgx_upload_mesh_hp", self.L.dgx_upload_mesh_hp(self.h, C.byref(m)))
+        self.E, self.F = m.E, m.F
+
+    def upload_mesh_mixed(self, mesh):
+        """Mixed TRI / QUAD 2-D mesh (mesh.MixedMesh: sorted by (type, p), per-element geometry nodes)."""
+        kind = np.ascontiguousarray(mesh.kind, dtype=np.int32)
+        p = np.ascontiguousarray(mesh.p, dtype=np.int32)
+        xyz = mesh.flat_xyz()
+        faces = np.ascontiguousarray(mesh.faces, dtype=np.int32)
+        m = DgxMeshMixed()
+        m.E, m.F = int(p.size), int(faces.shape[0])
+        m.type = kind.ctypes.data_as(C.POINTER(C.c_int))
+        m.p = p.ctypes.data_as(C.POINTER(C.c_int))
+        m.xyz = xyz.ctypes.data_as(dp)
+        m.faces = faces.ctypes.data_as(C.POINTER(C.c_int))
+        check("dgx_upload_mesh_mixed", self.L.dgx_upload_mesh_mixed(self.h, C.byref(m)))
         self.E, self.F = m.E, m.F
 
     def geometry(self):
@@ -452,6 +475,26 @@ def face_permutation(d: int, p: int, ord_: int) -> np.ndarray:
     out = np.zeros(n, dtype=np.int32)
     check("dgx_face_permutation", L.dgx_face_permutation(d, p, ord_, out.ctypes.data_as(C.POINTER(C.c_int))))
     return out
+
+
+def reference_nodes(kind: int, p: int):
+    """dgx_reference_nodes (host only): (rst (Np, 2), w (Np,)) of the solution nodes of type kind (0 QUAD, 1 TRI)."""
+    L = load()
+    n = C.c_int()
+    check("dgx_reference_nodes", L.dgx_reference_nodes(kind, p, None, None, C.byref(n)))
+    rst, w = np.zeros(2 * n.value), np.zeros(n.value)
+    check("dgx_reference_nodes", L.dgx_reference_nodes(kind, p, rst.ctypes.data_as(dp), w.ctypes.data_as(dp), None))
+    return rst.reshape(-1, 2), w
+
+
+def tri_rule(kind: int, q: int):
+    """dgx_tri_rule (host only): the reference's TRI rule (kind 0 WSH order q, 1 WV degree q): (rst (n, 2), w)."""
+    L = load()
+    n = C.c_int()
+    check("dgx_tri_rule", L.dgx_tri_rule(kind, q, None, None, C.byref(n)))
+    rst, w = np.zeros(2 * n.value), np.zeros(n.value)
+    check("dgx_tri_rule", L.dgx_tri_rule(kind, q, rst.ctypes.data_as(dp), w.ctypes.data_as(dp), None))
+    return rst.reshape(-1, 2), w
 
 
 def read_gmsh(path: str, p: int) -> dict:

@@ -42,6 +42,8 @@ void free_hp(Ctx& c)
 	for (dgx_ctx* g : H->groups) dgx_destroy(g);
 	release(H->faces); release(H->nJ); release(H->nJ32); release(H->nJhi); release(H->z64); release(H->z32);
 	release(H->ops64); release(H->ops32);
+	for (dgx::TriGroup& t : H->tri) { release(t.met); release(t.detJ); }
+	release(H->tri_ops64); release(H->mfaces); release(H->mnJ);
 	for (int k = 0; k < dgx::HpData::kLanes; ++k) {
 		if (H->lane[k]) (void)hipStreamDestroy(H->lane[k]);
 		if (H->join_ev[k]) (void)hipEventDestroy(H->join_ev[k]);
@@ -378,9 +380,16 @@ dgx_status hp_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned long 
 			return e;
 		}));
 	} else {
+		if (!H.tri.empty() && mode != dgx::MODE_RHS) {
+			(void)dgx::hp_join(H, c.stream);
+			return DGX_ERR_UNSUPPORTED;   // TRI elements: the residual without M^-1 only
+		}
 		DGX_TRY(dgx::launch_hp_faces64(c, u));
+		DGX_TRY(dgx::launch_mix_faces64(H, c, u, H.lane[dgx::HpData::kLanes - 1]));
 		DGX_TRY(hp_barrier(H, c.stream));
 		DGX_TRY(on_groups(c, [&](size_t g, Ctx& s) { return dgx::launch_rhs64(s, u + H.u_off[g], r + H.u_off[g], mode, dmax); }));
+		for (size_t t = 0; t < H.tri.size(); ++t)
+			DGX_TRY(dgx::launch_tri_rhs64(H.tri[t], H.tri_ops64, u, r, H.z64, H.lane[t % dgx::HpData::kLanes]));
 	}
 	DGX_TRY(dgx::hp_join(H, c.stream));
 	return DGX_OK;
@@ -389,6 +398,7 @@ dgx_status hp_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned long 
 dgx_status hp_rhs32(Ctx& c, const float* u, float* r)
 {
 	dgx::HpData& H = *c.hp;
+	if (!H.tri.empty()) return DGX_ERR_UNSUPPORTED;   // TRI elements: FP64 only
 	DGX_TRY(dgx::hp_fork(H, c.stream));
 	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) {
 		DGX_TRY(on_groups(c, [&](size_t g, Ctx& s) { return dgx::launch_rhs_ns32_phase(s, u + H.u_off[g], r + H.u_off[g], 1); }));
@@ -409,6 +419,7 @@ dgx_status hp_rhs32(Ctx& c, const float* u, float* r)
 dgx_status hp_rhs_mixed(Ctx& c, const double* u, float* r)
 {
 	dgx::HpData& H = *c.hp;
+	if (!H.tri.empty()) return DGX_ERR_UNSUPPORTED;   // TRI elements: FP64 only
 	DGX_TRY(dgx::hp_fork(H, c.stream));
 	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) {
 		DGX_TRY(on_groups(c, [&](size_t g, Ctx& s) { return dgx::launch_rhs_ns_mixed_phase(s, u + H.u_off[g], r + H.u_off[g], 1); }));
@@ -429,44 +440,74 @@ dgx_status hp_rhs_mixed(Ctx& c, const double* u, float* r)
 
 extern "C" {
 
-static dgx_status upload_hp_impl(dgx_ctx* h, const dgx_mesh_hp* m)
+// Variable-order 2-D mesh of QUAD (type 0) and, when type is given, TRI (type 1) elements sorted by (type, p).
+static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, const int* pe, const double* xyz,
+                                 const int* faces)
 {
-	if (!h || !m || m->E <= 0 || m->F < 0 || !m->p || !m->xyz || (m->F > 0 && !m->faces)) return DGX_ERR_ARG;
+	if (!h || E <= 0 || F < 0 || !pe || !xyz || (F > 0 && !faces)) return DGX_ERR_ARG;
 	Ctx& c = h->c;
 	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
 	if (c.cfg.pde != DGX_PDE_EULER && c.cfg.pde != DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;
-	const int E = m->E, nv = c.nv;
+	const int nv = c.nv;
 	// per element face: the lifted flux [v][a] (Navier-Stokes: then the projected BR2 products [d][v][a])
 	const int zblk = (c.cfg.pde == DGX_PDE_NAVIER_STOKES) ? 3 * nv : nv;
-	// groups of equal order (the elements arrive sorted by order)
-	std::vector<int> grp(E), order, e_off;
+	auto kind = [&](int e) { return type ? type[e] : 0; };
+	// QUAD groups of equal order and TRI groups of equal order (the elements arrive sorted by (type, p))
+	std::vector<int> grp(E), order, e_off;   // QUAD groups: index into order / e_off; TRI elements: TRI group
+	std::vector<int> tri_p, tri_e0;
+	bool any_tri = false;
 	for (int e = 0; e < E; ++e) {
-		const int p = m->p[e];
-		if (p < 1 || p > c.p || p + 1 > dgx::kHpMaxN) return DGX_ERR_ARG;
-		if (e > 0 && p < m->p[e - 1]) return DGX_ERR_ARG;
-		if (e == 0 || p != m->p[e - 1]) { order.push_back(p); e_off.push_back(e); }
-		grp[e] = (int)order.size() - 1;
+		const int p = pe[e], k = kind(e);
+		if (k != 0 && k != 1) return DGX_ERR_ARG;
+		if (p < 1 || p > c.p || p + 1 > dgx::kHpMaxN || (k == 1 && p > dgx::kTriMaxP)) return DGX_ERR_ARG;
+		if (e > 0 && k * 8 + p < kind(e - 1) * 8 + pe[e - 1]) return DGX_ERR_ARG;
+		const bool start = (e == 0 || p != pe[e - 1] || k != kind(e - 1));
+		if (k == 0) {
+			if (start) { order.push_back(p); e_off.push_back(e); }
+			grp[e] = (int)order.size() - 1;
+		} else {
+			any_tri = true;
+			if (start) { tri_p.push_back(p); tri_e0.push_back(e); }
+			grp[e] = (int)tri_p.size() - 1;
+		}
 	}
-	e_off.push_back(E);
-	const int G = (int)order.size();
+	if (any_tri && c.cfg.pde != DGX_PDE_EULER) return DGX_ERR_UNSUPPORTED;   // TRI: Euler only
+	const int G = (int)order.size(), T = (int)tri_p.size();
+	e_off.push_back(G ? (any_tri ? tri_e0[0] : E) : 0);
+	tri_e0.push_back(E);
+	auto npe = [&](int e) { return kind(e) ? (pe[e] + 1) * (pe[e] + 2) / 2 : (pe[e] + 1) * (pe[e] + 1); };
 	std::vector<long long> uoff(E + 1, 0), xoff(E + 1, 0);
 	for (int e = 0; e < E; ++e) {
-		const long long Np = (long long)(m->p[e] + 1) * (m->p[e] + 1);
-		uoff[e + 1] = uoff[e] + Np * nv;
-		xoff[e + 1] = xoff[e] + Np * 2;
+		uoff[e + 1] = uoff[e] + (long long)npe(e) * nv;
+		xoff[e + 1] = xoff[e] + (long long)npe(e) * 2;
 	}
-	std::vector<long long> z_off(G + 1, 0);
+	std::vector<long long> z_off(G + 1, 0), tz_off(T + 1, 0);
 	for (int g = 0; g < G; ++g)
 		z_off[g + 1] = z_off[g] + (long long)(e_off[g + 1] - e_off[g]) * 4 * zblk * (order[g] + 1);
-	// split the face table: faces inside one group, faces between groups
+	tz_off[0] = z_off[G];
+	for (int t = 0; t < T; ++t)
+		tz_off[t + 1] = tz_off[t] + (long long)(tri_e0[t + 1] - tri_e0[t]) * 3 * nv * (tri_p[t] + 1);
+	// lifted-flux block of local face lf of element e (its own GL face points, [v][a])
+	auto zblock = [&](int e, int lf) -> long long {
+		const int N = pe[e] + 1;
+		if (kind(e)) return tz_off[grp[e]] + ((long long)(e - tri_e0[grp[e]]) * 3 + lf) * nv * N;
+		return z_off[grp[e]] + ((long long)(e - e_off[grp[e]]) * 4 + lf) * zblk * N;
+	};
+	// outward n J_f of element e's face lf at face coordinates s (its own map)
+	auto side_nj = [&](int e, int lf, const double* s, int nq, double* out) {
+		if (kind(e)) dgx::tri_face_nj(pe[e], lf, xyz + xoff[e], s, nq, out);
+		else outward_nj(xyz + xoff[e], pe[e] + 1, lf, s, nq, out);
+	};
+	// split the face table: faces inside one group, faces between QUAD groups, faces with a TRI side
 	std::vector<std::vector<int>> gfaces(G);
 	std::vector<dgx::HpFace> hf;
-	std::vector<double> nj, njh;   // n J_f of the face (lower-order side) and of the higher-order side's own face
+	std::vector<dgx::MixFace> mf;
+	std::vector<double> nj, njh, mnj;   // n J_f of the face (lower-order side) and of the higher-order side's own face
 	double xg[dgx::kHpMaxN], wg[dgx::kHpMaxN], tmp[2 * dgx::kHpMaxN];
-	for (int f = 0; f < m->F; ++f) {
-		const int* r = m->faces + 6 * f;
+	for (int f = 0; f < F; ++f) {
+		const int* r = faces + 6 * f;
 		const int eL = r[0], fL = r[1], eR = r[2], fR = r[3], ord = r[4], bc = r[5];
-		if (eL < 0 || eL >= E || fL < 0 || fL >= 4) return DGX_ERR_ARG;
+		if (eL < 0 || eL >= E || fL < 0 || fL >= (kind(eL) ? 3 : 4)) return DGX_ERR_ARG;
 		// ord stays the reference's code in the group face tables (upload_uniform translates it); the faces
 		// between orders use its permutation bits
 		if ((bc == DGX_BC_INTERIOR && dgx::ord_bits(2, ord) < 0) || (bc != DGX_BC_INTERIOR && ord != -1 && ord != 0))
@@ -477,11 +518,49 @@ static dgx_status upload_hp_impl(dgx_ctx* h, const dgx_mesh_hp* m)
 			const int row[6] = { a0, a1, a2, a3, a4, a5 };
 			gfaces[g].insert(gfaces[g].end(), row, row + 6);
 		};
+		if (bc == DGX_BC_INTERIOR && (eR < 0 || eR >= E || fR < 0 || fR >= (kind(eR) ? 3 : 4))) return DGX_ERR_ARG;
+		if (kind(eL) || (bc == DGX_BC_INTERIOR && kind(eR))) {
+			// a TRI side: evaluated by the mixed face kernel, QUAD sides read its lifted flux
+			if (bc != DGX_BC_INTERIOR && (bc != DGX_BC_RIEMANN && bc != DGX_BC_SLIPWALL)) return DGX_ERR_ARG;
+			dgx::MixFace x{};
+			const bool inter = (bc == DGX_BC_INTERIOR);
+			x.uL = uoff[eL];
+			x.zL = zblock(eL, fL);
+			x.kL = kind(eL);
+			x.NL = pe[eL] + 1;
+			x.lfL = fL;
+			x.bc = bc;
+			x.rev = (obits & 2) ? 1 : 0;
+			if (!kind(eL)) push(gL, eL - e_off[gL], fL, -1, -1, 0, dgx::kBcNonconforming);
+			if (inter) {
+				x.uR = uoff[eR];
+				x.zR = zblock(eR, fR);
+				x.kR = kind(eR);
+				x.NR = pe[eR] + 1;
+				x.lfR = fR;
+				if (!kind(eR)) push(grp[eR], eR - e_off[grp[eR]], fR, -1, -1, 0, dgx::kBcNonconforming);
+			} else {
+				x.NR = x.NL;
+			}
+			mf.push_back(x);
+			const int Nf = std::max(x.NL, x.NR);
+			dgx::gauss_legendre(Nf, xg, wg);
+			const size_t base = mnj.size();
+			mnj.resize(base + 2 * dgx::kHpMaxN, 0.0);
+			if (!inter || x.NL <= x.NR) {
+				side_nj(eL, fL, xg, Nf, mnj.data() + base);
+			} else {
+				double sr[dgx::kHpMaxN];
+				for (int q = 0; q < Nf; ++q) sr[q] = x.rev ? -xg[q] : xg[q];
+				side_nj(eR, fR, sr, Nf, tmp);
+				for (int q = 0; q < 2 * Nf; ++q) mnj[base + q] = -tmp[q];
+			}
+			continue;
+		}
 		if (bc != DGX_BC_INTERIOR) {
 			push(gL, eL - e_off[gL], fL, -1, -1, ord, bc);
 			continue;
 		}
-		if (eR < 0 || eR >= E || fR < 0 || fR >= 4) return DGX_ERR_ARG;
 		const int gR = grp[eR];
 		if (gL == gR) {
 			push(gL, eL - e_off[gL], fL, eR - e_off[gL], fR, ord, bc);
@@ -507,8 +586,8 @@ static dgx_status upload_hp_impl(dgx_ctx* h, const dgx_mesh_hp* m)
 		njh.resize(base + 2 * dgx::kHpMaxN, 0.0);
 		double* lo = (NL < NR) ? nj.data() + base : njh.data() + base;   // the left side's own geometry
 		double* ro = (NL < NR) ? njh.data() + base : nj.data() + base;   // the right side's, left orientation
-		outward_nj(m->xyz + xoff[eL], NL, fL, xg, Nf, lo);
-		outward_nj(m->xyz + xoff[eR], NR, fR, xg, Nf, tmp);
+		outward_nj(xyz + xoff[eL], NL, fL, xg, Nf, lo);
+		outward_nj(xyz + xoff[eR], NR, fR, xg, Nf, tmp);
 		for (int q = 0; q < Nf; ++q) {
 			const int qr = (obits & 2) ? Nf - 1 - q : q;
 			ro[2 * q] = -tmp[2 * qr];
@@ -546,10 +625,10 @@ static dgx_status upload_hp_impl(dgx_ctx* h, const dgx_mesh_hp* m)
 	H.order = order;
 	H.e_off = e_off;
 	H.E = E;
-	H.F = m->F;
+	H.F = F;
 	H.Fnc = (int)hf.size();
 	H.ndof = uoff[E];
-	H.nz = z_off[G];
+	H.nz = tz_off[T];
 	for (int g = 0; g < G; ++g) {
 		H.u_off.push_back(uoff[e_off[g]]);
 		H.z_off.push_back(z_off[g]);
@@ -582,7 +661,43 @@ static dgx_status upload_hp_impl(dgx_ctx* h, const dgx_mesh_hp* m)
 		delete o32;
 		if (e1 != hipSuccess || e2 != hipSuccess) return DGX_ERR_HIP;
 	}
-	// one uniform-order context per group
+	// TRI groups: operators of every order, each element's metrics and det J at the cubature; the mixed faces
+	if (T > 0) {
+		std::vector<dgx::TriOps<double>> tops(dgx::kTriMaxP + 1);
+		for (int p = 1; p <= dgx::kTriMaxP; ++p)
+			if (!dgx::build_tri_ops(p, tops[p])) return DGX_ERR_UNSUPPORTED;
+		DGX_TRY(hipMalloc(&H.tri_ops64, sizeof(dgx::TriOps<double>) * tops.size()));
+		DGX_TRY(hipMemcpy(H.tri_ops64, tops.data(), sizeof(dgx::TriOps<double>) * tops.size(), hipMemcpyHostToDevice));
+		for (int t = 0; t < T; ++t) {
+			dgx::TriGroup tg;
+			tg.p = tri_p[t];
+			tg.e0 = tri_e0[t];
+			tg.E = tri_e0[t + 1] - tri_e0[t];
+			tg.u_off = uoff[tg.e0];
+			tg.z_off = tz_off[t];
+			const dgx::TriOps<double>& o = tops[tg.p];
+			std::vector<double> met((size_t)tg.E * 4 * o.Nc), dj((size_t)tg.E * o.Nc);
+			for (int k = 0; k < tg.E; ++k) {
+				dgx::tri_geometry(o, xyz + xoff[tg.e0 + k], &met[(size_t)k * 4 * o.Nc], &dj[(size_t)k * o.Nc]);
+				for (int q = 0; q < o.Nc; ++q)
+					if (!(dj[(size_t)k * o.Nc + q] > 0.0)) return DGX_ERR_ARG;   // clockwise or degenerate element
+			}
+			H.tri.push_back(tg);
+			dgx::TriGroup& hg = H.tri.back();
+			DGX_TRY(hipMalloc(&hg.met, sizeof(double) * met.size()));
+			DGX_TRY(hipMalloc(&hg.detJ, sizeof(double) * dj.size()));
+			DGX_TRY(hipMemcpy(hg.met, met.data(), sizeof(double) * met.size(), hipMemcpyHostToDevice));
+			DGX_TRY(hipMemcpy(hg.detJ, dj.data(), sizeof(double) * dj.size(), hipMemcpyHostToDevice));
+		}
+	}
+	H.Fmix = (int)mf.size();
+	if (!mf.empty()) {
+		DGX_TRY(hipMalloc(&H.mfaces, sizeof(dgx::MixFace) * mf.size()));
+		DGX_TRY(hipMalloc(&H.mnJ, sizeof(double) * mnj.size()));
+		DGX_TRY(hipMemcpy(H.mfaces, mf.data(), sizeof(dgx::MixFace) * mf.size(), hipMemcpyHostToDevice));
+		DGX_TRY(hipMemcpy(H.mnJ, mnj.data(), sizeof(double) * mnj.size(), hipMemcpyHostToDevice));
+	}
+	// one uniform-order context per QUAD group
 	for (int g = 0; g < G; ++g) {
 		dgx_config cfg = c.cfg;
 		cfg.p = order[g];
@@ -594,7 +709,7 @@ static dgx_status upload_hp_impl(dgx_ctx* h, const dgx_mesh_hp* m)
 		dgx_mesh sm{};
 		sm.E = e_off[g + 1] - e_off[g];
 		sm.F = (int)(gfaces[g].size() / 6);
-		sm.xyz = m->xyz + xoff[e_off[g]];
+		sm.xyz = xyz + xoff[e_off[g]];
 		sm.faces = gfaces[g].empty() ? nullptr : gfaces[g].data();
 		st = upload_uniform(sub, &sm, true);
 		if (st != DGX_OK) return st;
@@ -635,8 +750,16 @@ static dgx_status upload_hp_impl(dgx_ctx* h, const dgx_mesh_hp* m)
 
 dgx_status dgx_upload_mesh_hp(dgx_ctx* h, const dgx_mesh_hp* m)
 {
-	const dgx_status st = upload_hp_impl(h, m);
+	const dgx_status st = m ? upload_hp_impl(h, m->E, m->F, nullptr, m->p, m->xyz, m->faces) : DGX_ERR_ARG;
 	// a failure after the previous mesh was released leaves no half-built variable-order state behind
+	if (st != DGX_OK && h && h->c.hp && !h->c.hp->ready) free_mesh(h->c);
+	return st;
+}
+
+dgx_status dgx_upload_mesh_mixed(dgx_ctx* h, const dgx_mesh_mixed* m)
+{
+	const dgx_status st = (m && m->type) ? upload_hp_impl(h, m->E, m->F, m->type, m->p, m->xyz, m->faces)
+	                                     : DGX_ERR_ARG;
 	if (st != DGX_OK && h && h->c.hp && !h->c.hp->ready) free_mesh(h->c);
 	return st;
 }

@@ -76,6 +76,49 @@ struct HpFace {
 	int pad;
 };
 
+// TRI elements of the reference's mixed 2-D meshes (dgx_tri.cpp): Lagrange basis on the WSH nodes of order p,
+// WV cubature of degree 2p, GL face points of the element's own order.  Row-major [out][in] unless noted.
+constexpr int kTriMaxP = 5;
+constexpr int kTriMaxNp = 21;   // (p+1)(p+2)/2 at p = 5
+constexpr int kTriMaxNc = 25;   // WV10
+template <typename T>
+struct TriOps {
+	T I[kTriMaxNc * kTriMaxNp];        // basis at the cubature [q][m]
+	T Dr[kTriMaxNc * kTriMaxNp];       // d/dr, d/ds of the basis at the cubature [q][m]
+	T Ds[kTriMaxNc * kTriMaxNp];
+	T DrW[kTriMaxNp * kTriMaxNc];      // transposed and weighted: [m][q] = d phi_m/dr (q) w_q
+	T DsW[kTriMaxNp * kTriMaxNc];
+	T w[kTriMaxNc];                    // cubature weights
+	T Ifc[3][kHpMaxN * kTriMaxNp];     // basis at the element's own GL points of face lf [a][m]
+	T tw0[3][kTriMaxNp * kHpMaxN];     // test functions times GL weights at those points [m][a]
+	int p, Np, Nc, N;
+};
+bool build_tri_ops(int p, TriOps<double>& o);
+void tri_geometry(const TriOps<double>& o, const double* xyz, double* met, double* detJ);
+void tri_face_nj(int p, int lf, const double* xyz, const double* s, int nq, double* nj);
+
+// A face with a TRI element on at least one side, evaluated by k_mix_faces: each side's trace at its own GL face
+// points (TRI: from all its nodes; QUAD: from its GLL face nodes), interpolated to the face's GL points, the
+// numerical flux there, and each side's lifted flux z = P^T w (J_f F) / w_own at its own points, own ordering.
+struct MixFace {
+	long long uL, uR;    // first state value of each side's element
+	long long zL, zR;    // first lifted-flux value of each side's own face block [v][a]
+	int kL, kR;          // element type of each side: 0 QUAD, 1 TRI
+	int NL, NR;          // p + 1 of each side
+	int lfL, lfR;        // local faces
+	int rev;             // 1: the right face runs the other way (ind_ord 1)
+	int bc;              // DGX_BC_* (boundary faces: the right side is unused)
+};
+
+// The TRI elements of one order of a mixed mesh (a contiguous range of the state vector).
+struct TriGroup {
+	int p = 0, E = 0, e0 = 0;
+	long long u_off = 0;             // first state value
+	long long z_off = 0;             // first lifted-flux value (E x 3 x nv x (p + 1))
+	double* met = nullptr;           // E x 4 x Nc: J-scaled cofactors (y_s, -x_s, -y_r, x_r) at the cubature
+	double* detJ = nullptr;          // E x Nc
+};
+
 // a run of faces between the same two orders (the faces are sorted by pair)
 struct HpPair {
 	int NL, NR, start, count;
@@ -108,6 +151,12 @@ struct HpData {
 	hipStream_t lane[kLanes] = {};
 	hipEvent_t fork_ev = nullptr;
 	hipEvent_t join_ev[kLanes] = {};
+	// TRI groups and the faces that touch a TRI element (mixed meshes)
+	std::vector<TriGroup> tri;
+	TriOps<double>* tri_ops64 = nullptr;   // device: kTriMaxP + 1 entries, indexed by p
+	MixFace* mfaces = nullptr;             // Fmix
+	double* mnJ = nullptr;                 // Fmix x kHpMaxN x 2: n J_f at the face GL points (left ordering)
+	int Fmix = 0;
 	std::vector<int> group_lane;     // side stream of each group (longest-first balance of E_g N^2)
 	std::vector<int> pair_lane;      // side stream of each face pair (longest-first balance of count max(NL,NR)^2)
 };
@@ -203,6 +252,10 @@ hipError_t launch_rhs_ns_phase(Ctx& c, const double* u, double* r, int phase);
 hipError_t launch_rhs_ns32_phase(Ctx& c, const float* u, float* r, int phase);
 hipError_t launch_rhs_ns_mixed_phase(Ctx& c, const double* u, float* r, int phase);
 hipError_t launch_hp_faces_mixed(Ctx& c, const double* u);   // FP64 state and fluxes, FP32 lifted results
+// mixed TRI / QUAD meshes (k_tri.hip): one TRI group's residual from its lifted face fluxes; the faces with a TRI side
+hipError_t launch_tri_rhs64(const TriGroup& g, const TriOps<double>* ops, const double* u, double* r, const double* z,
+                            hipStream_t st);
+hipError_t launch_mix_faces64(const HpData& H, const Ctx& c, const double* u, hipStream_t st);
 hipError_t launch_minv(Ctx& c, double* r);
 hipError_t launch_positivity(Ctx& c, double* u, int* status);
 hipError_t launch_maxabs(Ctx& c, const double* r, size_t n, unsigned long long* dmax);

@@ -638,3 +638,170 @@ def config4_hp_mesh(nx: int = 500, ny: int = 300, seed: int = SEED) -> HpMesh:
 def config4_hp_state(mesh: HpMesh, seed: int = SEED) -> np.ndarray:
     """The configuration-3 state formula in 2-D (as config4_state) at every element's own nodes."""
     return hp_state(mesh, lambda x: prim_to_cons(smooth_state_3d(x, seed + 1), 2))
+
+
+# --------------------------------------------------------------------------- mixed TRI / QUAD (2-D)
+QUAD_T, TRI_T = 0, 1
+_TRI_F_VE = ((1, 2), (0, 2), (0, 1))        # TRI local face lf runs from vertex f_ve[lf][0] to f_ve[lf][1]
+_QUAD_F_VE = ((0, 2), (1, 3), (0, 1), (2, 3))   # QUAD vertices (0,0) (1,0) (0,1) (1,1), face coordinate increasing
+
+
+def tri_reference_nodes(p: int) -> np.ndarray:
+    """(Np, 3) barycentric coordinates of the order-p TRI solution nodes (WSH, the reference's order), from the
+    library (dgx_reference_nodes): the state at the ABI is nodal at these points."""
+    from . import _lib
+
+    rst, _ = _lib.reference_nodes(TRI_T, p)
+    sq3 = math.sqrt(3.0)
+    V = np.array([[-1.0, 1.0, 0.0], [-1.0 / sq3, -1.0 / sq3, 2.0 / sq3], [1.0, 1.0, 1.0]])
+    return np.linalg.solve(V, np.vstack([rst.T, np.ones(len(rst))])).T
+
+
+@dataclass
+class MixedMesh:
+    """2-D mesh of QUAD (type 0, GLL nodes) and TRI (type 1, WSH nodes) elements, sorted by (type, p) so that
+    every (type, p) group is one contiguous range of the state.
+    * ``kind``  (E,) int32, ``p`` (E,) int32;
+    * ``xyz``   list of (2, Np_e) float64 geometry nodes at each element's solution nodes;
+    * ``faces`` (F, 6) int32 (eL, fL, eR, fR, ord, bc), ord the reference's ind_ord (0 aligned, 1 reversed)."""
+    kind: np.ndarray
+    p: np.ndarray
+    xyz: list
+    faces: np.ndarray
+    d: int = 2
+
+    @property
+    def E(self) -> int:
+        return len(self.p)
+
+    @property
+    def F(self) -> int:
+        return self.faces.shape[0]
+
+    def Np(self, e: int) -> int:
+        p = int(self.p[e])
+        return (p + 1) * (p + 2) // 2 if self.kind[e] == TRI_T else (p + 1) ** 2
+
+    def offsets(self, nv: int) -> np.ndarray:
+        sizes = np.array([self.Np(e) * nv for e in range(self.E)], dtype=np.int64)
+        return np.concatenate([[0], np.cumsum(sizes)])
+
+    def flat_xyz(self) -> np.ndarray:
+        return np.ascontiguousarray(np.concatenate([x.ravel() for x in self.xyz]))
+
+
+def structured_mixed_mesh(n: tuple, split, p_elem, lo=(0.0, 0.0), hi=(1.0, 1.0), periodic=(True, True),
+                          curve=None, bc_low=None, bc_high=None) -> MixedMesh:
+    """Structured nx x ny grid (i fastest) whose cells with split[cell] are cut along the (i, j)-(i+1, j+1)
+    diagonal into the counter-clockwise TRIs (c00, c10, c11) and (c00, c11, c01).  p_elem gives the order of each
+    element in creation order (a QUAD cell: one; a split cell: two, A then B).  Faces are matched on the integer
+    grid vertices (periodic directions wrap), the earlier-created element is the left side, ord 1 when the two
+    sides run the other way; unmatched faces take bc_low / bc_high of their boundary (BC_* per direction).
+    Geometry: each element's solution nodes mapped by the straight cell / triangle map, then by curve."""
+    nx, ny = n
+    split = np.asarray(split, bool).reshape(-1)
+    h = ((hi[0] - lo[0]) / nx, (hi[1] - lo[1]) / ny)
+    elems = []   # (kind, vertices as integer grid points)
+    for j in range(ny):
+        for i in range(nx):
+            c00, c10, c01, c11 = (i, j), (i + 1, j), (i, j + 1), (i + 1, j + 1)
+            if split[i + nx * j]:
+                elems.append((TRI_T, (c00, c10, c11)))
+                elems.append((TRI_T, (c00, c11, c01)))
+            else:
+                elems.append((QUAD_T, (c00, c10, c01, c11)))
+    p_elem = np.asarray(p_elem, dtype=np.int32)
+    if len(p_elem) != len(elems):
+        raise ValueError(f"{len(elems)} elements, {len(p_elem)} orders")
+
+    def wrap(v):
+        x, y = v
+        if periodic[0]:
+            x %= nx
+        if periodic[1]:
+            y %= ny
+        return (x, y)
+
+    def phys(pts):
+        out = np.empty_like(pts)
+        out[..., 0] = lo[0] + pts[..., 0] * h[0]
+        out[..., 1] = lo[1] + pts[..., 1] * h[1]
+        return curve(out) if curve is not None else out
+
+    xyz = []
+    for (kind, vs), p in zip(elems, p_elem):
+        V = np.array(vs, float)
+        if kind == TRI_T:
+            pts = tri_reference_nodes(int(p)) @ V
+        else:
+            x1, _ = gauss_lobatto(int(p) + 1)
+            t = 0.5 * (x1 + 1.0)
+            a, b = np.meshgrid(t, t)                      # a: r (fastest), b: s
+            a, b = a.ravel(), b.ravel()
+            pts = (np.outer((1 - a) * (1 - b), V[0]) + np.outer(a * (1 - b), V[1]) + np.outer((1 - a) * b, V[2])
+                   + np.outer(a * b, V[3]))
+        xyz.append(np.ascontiguousarray(phys(pts).T))
+    # faces: match on wrapped integer endpoints
+    open_faces, rows = {}, []
+    for e, (kind, vs) in enumerate(elems):
+        fve = _TRI_F_VE if kind == TRI_T else _QUAD_F_VE
+        for lf, (a, b) in enumerate(fve):
+            A, B = wrap(vs[a]), wrap(vs[b])
+            key = frozenset((A, B))
+            if key in open_faces:
+                e0, lf0, A0, B0 = open_faces.pop(key)
+                rows.append([e0, lf0, e, lf, 0 if (A0 == A and B0 == B) else 1, BC_INTERIOR])
+            else:
+                open_faces[key] = (e, lf, A, B)
+    for (e, lf, A, B) in open_faces.values():
+        kind, vs = elems[e]
+        fve = _TRI_F_VE if kind == TRI_T else _QUAD_F_VE
+        va, vb = vs[fve[lf][0]], vs[fve[lf][1]]
+        if va[0] == vb[0]:
+            k, side = 0, (va[0] == nx)
+        else:
+            k, side = 1, (va[1] == ny)
+        bc = (bc_high if side else bc_low)
+        if bc is None:
+            raise ValueError(f"unmatched face {e}/{lf} on a boundary without a condition")
+        rows.append([e, lf, -1, -1, 0, bc[k]])
+    # sort the elements by (type, p), stable, and renumber the faces
+    kinds = np.array([k for k, _ in elems], dtype=np.int32)
+    order = np.lexsort((p_elem, kinds))
+    new_of = np.empty(len(elems), dtype=np.int64)
+    new_of[order] = np.arange(len(elems))
+    faces = np.array(rows, dtype=np.int64).reshape(-1, 6)
+    faces[:, 0] = new_of[faces[:, 0]]
+    inter = faces[:, 2] >= 0
+    faces[inter, 2] = new_of[faces[inter, 2]]
+    return MixedMesh(kind=kinds[order].copy(), p=p_elem[order].copy(), xyz=[xyz[e] for e in order],
+                     faces=faces.astype(np.int32))
+
+
+def mixed_state(mesh: MixedMesh, cons_fn) -> np.ndarray:
+    """Flat state: cons_fn(x (n, 2)) -> (n, nv) conserved values at every element's own nodes, [var][node]."""
+    out = []
+    for e in range(mesh.E):
+        c = cons_fn(mesh.xyz[e].T)
+        out.append(c.T.ravel())
+    return np.ascontiguousarray(np.concatenate(out))
+
+
+def config4_mixed_split(nx: int, ny: int, seed: int = SEED) -> np.ndarray:
+    """The seeded third of the cells that configuration 4 cuts into two TRIs (100 k TRI + 100 k QUAD on 500 x 300)."""
+    r = SplitMix64(seed + 40).uniform(nx * ny, 0.0, 1.0)
+    return r < 1.0 / 3.0
+
+
+def config4_mixed_mesh(nx: int = 500, ny: int = 300, seed: int = SEED) -> MixedMesh:
+    """Configuration 4's mesh: the 500 x 300 periodic base grid of [0,1]^2 with a seeded third of its cells cut into
+    two TRIs, orders p ~ U{1..5} per element (seeded)."""
+    split = config4_mixed_split(nx, ny, seed)
+    n_el = int(nx * ny + split.sum())
+    r = SplitMix64(seed + 41).uniform(n_el, 0.0, 1.0)
+    p = (1 + np.minimum((r * 5).astype(np.int32), 4)).astype(np.int32)
+    return structured_mixed_mesh((nx, ny), split, p)
+
+
+def config4_mixed_state(mesh: MixedMesh, seed: int = SEED) -> np.ndarray:
+    return mixed_state(mesh, lambda x: prim_to_cons(smooth_state_3d(x, seed + 1), 2))

@@ -77,6 +77,22 @@ typedef struct dgx_mesh_hp {
 	const int* faces;   /* HOST, F x 6 as dgx_mesh.faces */
 } dgx_mesh_hp;
 
+/* 2-D mesh of QUAD and TRI elements of one order each (the reference's mixed meshes, mesh_type mixed): element e
+ * has type[e] (0 QUAD: Lagrange basis on the (p+1)^2 GLL nodes; 1 TRI: Lagrange basis on the (p+1)(p+2)/2 WSH
+ * nodes of the reference's equilateral triangle, interp_si WSH, element/nodes.c:628-852; dgx_reference_nodes
+ * gives both node sets) and order p[e]; the elements are sorted by (type, p), so every (type, p) group is one
+ * contiguous range of the state.  xyz: per element 2 x Np_e geometry nodes at the solution nodes (isoparametric),
+ * element after element.  faces: F x 6 as dgx_mesh.faces; TRI local face lf is opposite vertex lf and runs from
+ * vertex (1, 2, 0)[lf] to (2, 2, 1)[lf] (element.c:454-463); ord 1 = the right face runs the other way. */
+typedef struct dgx_mesh_mixed {
+	int E;
+	int F;
+	const int* type;    /* HOST, E: 0 QUAD, 1 TRI */
+	const int* p;       /* HOST, E: orders 1..5 */
+	const double* xyz;  /* HOST, per element 2 x Np_e */
+	const int* faces;   /* HOST, F x 6 */
+} dgx_mesh_mixed;
+
 typedef struct dgx_ctx dgx_ctx;
 typedef struct dgx_mlp dgx_mlp;
 
@@ -129,6 +145,22 @@ dgx_status dgx_upload_mesh(dgx_ctx* ctx, const dgx_mesh* mesh);
  * p) volume/face loops of compute_volume_rlhs_dg / compute_face_rlhs_dg for meshes whose volumes differ in
  * p_ref.  Synchronous. */
 dgx_status dgx_upload_mesh_hp(dgx_ctx* ctx, const dgx_mesh_hp* mesh);
+/* Mixed TRI / QUAD mesh (Euler).  TRI groups: the volume term with the element's dense operators at the WV
+ * cubature of degree 2p (compute_volume_rlhs_T.c:145-177; nodes_operators.c:327-345, 436-520); every face with a
+ * TRI side is evaluated once at the GL points of the face order max(p_L, p_R) (compute_face_reference_order,
+ * face_solver.c:123-131), geometry from the lower-order side, and lifted with each side's own test functions.
+ * QUAD groups and faces between QUADs as in dgx_upload_mesh_hp.  Serves dgx_compute_rhs (FP64), dgx_graph_create_rhs
+ * (DGX_GRAPH_RHS_FP64) and dgx_sizes; with TRI elements present the FP32, mixed and explicit entry points return
+ * DGX_ERR_UNSUPPORTED, as does a Navier-Stokes context.  Elements of negative Jacobian: DGX_ERR_ARG.
+ * Synchronous. */
+dgx_status dgx_upload_mesh_mixed(dgx_ctx* ctx, const dgx_mesh_mixed* mesh);
+/* Host only: the solution nodes of the reference element of type (0 QUAD, 1 TRI) and order p (1..5): rst (2 Np,
+ * (r, s) pairs), weights w (Np; the WSH rule's weights on the triangle of area sqrt(3), or the GLL tensor weights),
+ * *np = Np; any output may be NULL. */
+dgx_status dgx_reference_nodes(int type, int p, double* rst, double* w, int* np);
+/* Host only: the reference's symmetric TRI rule kind<q> (kind 0: WSH of order q, 1: WV of degree q) expanded as the
+ * reference expands it (nodes.c:628-852): *n points rst (2 n) and weights w (n); NULL outputs are skipped. */
+dgx_status dgx_tri_rule(int kind, int q, double* rst, double* w, int* n);
 /* gmsh 2.2 ASCII reader for 2-D QUAD meshes (mesh/mesh_readers.c, mesh_connectivity.c:285-465, mesh_periodic.c):
  * volumes in file order, $Periodic node correspondences, boundary tags from the line elements; geometry is
  * the straight-sided map at the GLL nodes of degree p.  TRI or 3-D volumes: DGX_ERR_UNSUPPORTED.  Host only. */

@@ -7,6 +7,9 @@ Reads (data only, no reference code is run):
                                                   src/element/nodes_correspondence.c)
   input/testing/integration/fe/*.fe.data          (Face::neigh_info{ind_v, ind_lf, ind_ord} of the shipped meshes,
                                                   compared by src/testing/support/test_support_fe_init.c:222)
+  input/testing/unit/nodes/nodes_si.data          (2-D simplex nodes, used by src/testing/unit/test_unit_nodes.c)
+  input/nodes/tri/WSH{1..5}.txt, WV{2..10}.txt    (the symmetric TRI node and cubature tables the reference reads
+                                                  at run time, src/element/nodes.c:628-852)
 under the DPGSolver checkout given as argv[1] (default /root/reference).
 """
 import glob
@@ -139,6 +142,45 @@ def fe_faces(path):
     return out
 
 
+def simplex_nodes(path):
+    """Nodes_d2_* blocks of nodes_si.data: {name: {"rst": rows, "w": weights or None}}."""
+    lines = open(path).read().splitlines()
+    out = {}
+    i = 0
+    while i < len(lines):
+        st = lines[i].strip()
+        if st.startswith("Nodes_d2_"):
+            has_w = int(lines[i + 2].split(":")[1])
+            assert lines[i + 3].strip() == "rst:"
+            rst, j = _read_matrix(lines, i + 4)
+            w = None
+            if has_w:
+                assert lines[j].strip() == "w:"
+                w, j = _read_matrix(lines, j + 1)
+                w = [x[0] for x in w]
+            out[st] = {"rst": rst, "w": w}
+            i = j
+        else:
+            i += 1
+    return out
+
+
+def tri_table(path):
+    """One input/nodes/tri/*.txt file: symmetry orbits as the reference stores them."""
+    lines = [x for x in open(path).read().splitlines()]
+    ngroups, nsymms = (int(v) for v in lines[0].split())
+    symms = [[int(v) for v in lines[1 + k].split()] for k in range(nsymms)]
+    k = 1 + nsymms
+    while not lines[k].strip().startswith("Barycentric"):
+        k += 1
+    bary = [[float(v) for v in lines[k + 1 + g].split()] for g in range(ngroups)]
+    k += 1 + ngroups
+    while not lines[k].strip().startswith("Weights"):
+        k += 1
+    w = [float(lines[k + 1 + g].split()[0]) for g in range(ngroups)]
+    return {"symms": symms, "bary": bary, "w": w}
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     unit = os.path.join(ref, "input", "testing", "unit")
@@ -158,7 +200,16 @@ def main():
         json.dump({"source": "input/testing/integration/fe/*.fe.data",
                    "columns": ["boundary", "ind_v0", "ind_lf0", "ind_ord0", "ind_v1", "ind_lf1", "ind_ord1"],
                    "faces": fe}, f, indent=0)
-    print("nodes:", sorted(n), "operators:", sorted(o), "corr:", sorted(c), "fe:", sorted(fe))
+    si = simplex_nodes(os.path.join(unit, "nodes", "nodes_si.data"))
+    tables = {}
+    for kind, orders in (("WSH", range(1, 6)), ("WV", range(2, 11, 2))):
+        for q in orders:
+            tables[f"{kind}{q}"] = tri_table(os.path.join(ref, "input", "nodes", "tri", f"{kind}{q}.txt"))
+    with open(os.path.join(HERE, "tri_nodes.json"), "w") as f:
+        json.dump({"source": "input/testing/unit/nodes/nodes_si.data (golden), input/nodes/tri/*.txt (tables)",
+                   "golden": si, "tables": tables}, f, indent=0)
+    print("nodes:", sorted(n), "operators:", sorted(o), "corr:", sorted(c), "fe:", sorted(fe), "si:", sorted(si),
+          "tri tables:", sorted(tables))
 
 
 if __name__ == "__main__":

@@ -1,0 +1,150 @@
+"""GPU parity on mixed TRI / QUAD meshes (dgx_upload_mesh_mixed) against oracle/mixed_oracle.py, FP64 to 1e-12.
+
+Cases: one order everywhere (p = 1..5) on a checkerboard of split cells, random orders 1..5 per element on
+curved meshes, an all-TRI mesh, and a non-periodic box with Riemann and slip-wall faces on TRI and QUAD sides;
+Roe-Pike and Lax-Friedrichs; the error is taken relative to the larger of max |R| and max |volume term|.
+Also: graph replays equal the eager call, and the entry points not built for TRI elements refuse the mesh
+instead of computing something else.
+"""
+import numpy as np
+import pytest
+
+import mixed_oracle as MO
+from dpgsolver_amd import mesh as M
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+TOL64 = 1e-12
+
+
+@pytest.fixture(scope="module")
+def dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda:0")
+
+
+def _curve(amp=0.03):
+    def f(pts):
+        out = pts.copy()
+        out[..., 0] = pts[..., 0] + amp * np.sin(2 * np.pi * pts[..., 1])
+        out[..., 1] = pts[..., 1] + amp * np.sin(2 * np.pi * pts[..., 0])
+        return out
+    return f
+
+
+def _cases():
+    out = []
+    for p in range(1, 6):
+        n = 4
+        split = (np.add.outer(np.arange(n), np.arange(n)) % 2 == 0).ravel()
+        m = M.structured_mixed_mesh((n, n), split, np.full(n * n + split.sum(), p, np.int32))
+        out.append((f"checker_p{p}_roe", m, M.FLUX_ROE, None))
+    rng = np.random.default_rng(7)
+    for flux, name in ((M.FLUX_ROE, "roe"), (M.FLUX_LF, "lf")):
+        split = rng.random(30) < 0.5
+        p = rng.integers(1, 6, int(30 + split.sum())).astype(np.int32)
+        m = M.structured_mixed_mesh((6, 5), split, p, curve=_curve())
+        out.append((f"random_orders_curved_{name}", m, flux, None))
+    split = np.ones(12, bool)
+    m = M.structured_mixed_mesh((4, 3), split, rng.integers(1, 6, 24).astype(np.int32))
+    out.append(("all_tri_random_orders", m, M.FLUX_ROE, None))
+    prim = M.freestream_prim(2)
+    split = rng.random(20) < 0.5
+    p = rng.integers(1, 6, int(20 + split.sum())).astype(np.int32)
+    m = M.structured_mixed_mesh((5, 4), split, p, lo=(-1.0, 0.0), hi=(1.0, 0.8), periodic=(False, False),
+                                bc_low=(M.BC_RIEMANN, M.BC_SLIPWALL), bc_high=(M.BC_RIEMANN, M.BC_SLIPWALL))
+    out.append(("box_riemann_slipwall", m, M.FLUX_ROE, prim))
+    return out
+
+
+CASES = _cases()
+
+
+def _state(mesh, bc_prim):
+    if bc_prim is None:
+        return M.mixed_state(mesh, lambda x: M.prim_to_cons(M.smooth_state_3d(x), 2))
+    pert = lambda x: 0.02 * np.sin(3 * x[:, :1]) * np.cos(2 * x[:, 1:2]) * np.array([1, 1, 0.5, 1])
+    return M.mixed_state(mesh, lambda x: M.prim_to_cons(bc_prim[None, :] + pert(x), 2))
+
+
+def _ctx(mesh, flux, bc_prim):
+    from dpgsolver_amd import _lib
+
+    ctx = _lib.Context(2, int(mesh.p.max()), M.PDE_EULER, flux, bc_prim=bc_prim)
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    ctx.upload_mesh_mixed(mesh)
+    return ctx
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_mixed_mesh_residual_matches_oracle(dev, case):
+    name, mesh, flux, bc_prim = case
+    print(f"case {name}: E={mesh.E} (TRI {int(mesh.kind.sum())}), F={mesh.F}", flush=True)
+    u = _state(mesh, bc_prim)
+    ctx = _ctx(mesh, flux, bc_prim)
+    ud = torch.from_numpy(u).to(dev)
+    rd = torch.full_like(ud, float("nan"))
+    ctx.compute_rhs(ud.data_ptr(), rd.data_ptr())
+    torch.cuda.synchronize()
+    orc = MO.MixedOracle(mesh, flux, bc_prim=bc_prim)
+    ref = orc.rhs(u)
+    got = rd.cpu().numpy()
+    assert np.isfinite(got).all()
+    # relative to the terms that cancel in R: near the free stream (the boundary case) R is ~1% of them, and two
+    # exact-to-rounding constructions of the TRI operators (orthogonal basis in double, monomials in long double)
+    # already differ by 1.2e-12 of max |R| there
+    scale = max(np.abs(orc.rhs(u, "volume")).max(), np.abs(ref).max())
+    err = np.abs(got - ref).max() / scale
+    assert err < TOL64, err
+    ctx.close()
+
+
+def test_mixed_mesh_free_stream_and_graph_replays(dev):
+    from dpgsolver_amd import _lib
+
+    name, mesh, flux, _ = CASES[5]
+    ctx = _ctx(mesh, flux, None)
+    prim = np.array([1.0, 0.3, -0.2, 1.0])
+    straight = M.structured_mixed_mesh((6, 5), np.arange(30) % 3 == 0, np.full(40, 3, np.int32))
+    cs = _ctx(straight, flux, None)
+    uf = torch.from_numpy(M.mixed_state(straight, lambda x: np.broadcast_to(M.prim_to_cons(prim, 2), (len(x), 4)))).to(dev)
+    rf = torch.empty_like(uf)
+    cs.compute_rhs(uf.data_ptr(), rf.data_ptr())
+    torch.cuda.synchronize()
+    assert rf.abs().max().item() < 1e-13
+    cs.close()
+    u = torch.from_numpy(_state(mesh, None)).to(dev)
+    r_eager = torch.empty_like(u)
+    ctx.compute_rhs(u.data_ptr(), r_eager.data_ptr())
+    r = torch.full_like(u, float("nan"))
+    g = ctx.graph_rhs(_lib.GRAPH_RHS_FP64, u.data_ptr(), r.data_ptr())
+    for _ in range(3):
+        r.fill_(float("nan"))
+        g.launch(1)
+        g.sync()
+        assert torch.equal(r, r_eager)
+        torch.cuda.synchronize()
+    g.close()
+    ctx.close()
+
+
+def test_entry_points_not_built_for_tri_refuse(dev):
+    from dpgsolver_amd import _lib
+
+    name, mesh, flux, _ = CASES[0]
+    ctx = _ctx(mesh, flux, None)
+    u = torch.from_numpy(_state(mesh, None)).to(dev)
+    r = torch.empty_like(u)
+    u32, r32 = u.float(), torch.empty(u.numel(), dtype=torch.float32, device=dev)
+    for call in (lambda: ctx.compute_rhs_f32(u32.data_ptr(), r32.data_ptr()),
+                 lambda: ctx.compute_rhs_mixed(u.data_ptr(), r32.data_ptr()),
+                 lambda: ctx.compute_rhs_explicit(u.data_ptr(), r.data_ptr())):
+        with pytest.raises(_lib.DgxError) as ei:
+            call()
+        assert ei.value.code == -3                  # DGX_ERR_UNSUPPORTED
+    ctx.close()
+    ns = _lib.Context(2, 5, M.PDE_NAVIER_STOKES, M.FLUX_ROE, mu=0.01, Pr=0.72)
+    with pytest.raises(_lib.DgxError):
+        ns.upload_mesh_mixed(mesh)
+    ns.close()
