@@ -453,6 +453,52 @@ def hp_euler_throughput(torch, dev, steps, warmup):
             "mixed": {**leg(_lib.GRAPH_RHS_MIXED), "rel_err_vs_fp64": err[_lib.GRAPH_RHS_MIXED]}}
 
 
+def mixed_euler_throughput(torch, dev, steps, warmup):
+    """2-D Euler FP64 residual on configuration 4's mesh: the 500 x 300 periodic base grid of [0,1]^2 with a seeded
+    third of its cells cut into two TRIs (99,980 TRI + 100,010 QUAD), p ~ U{1..5} per element (seeded), Roe; the
+    state is the 3-D workload's formula in 2-D.  One evaluation = the face kernels (between QUAD orders, and every
+    face with a TRI side) and every (type, p) group's residual kernel, replayed as one HIP graph."""
+    import numpy as np
+    from dpgsolver_amd import _lib, mesh as M
+
+    mesh = M.config4_mixed_mesh()
+    ctx = _lib.Context(2, 5, M.PDE_EULER, M.FLUX_ROE)
+    stream = torch.cuda.current_stream(dev)
+    ctx.set_stream(stream.cuda_stream)
+    ctx.upload_mesh_mixed(mesh)
+    u = torch.from_numpy(M.config4_mixed_state(mesh)).to(dev)
+    r = torch.empty_like(u)
+    g = ctx.graph_rhs(_lib.GRAPH_RHS_FP64, u.data_ptr(), r.data_ptr())
+    for _ in range(warmup):
+        g.launch(1)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    g.launch(steps)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / steps
+    g.close()
+    ctx.close()
+    tri = mesh.kind == 1
+    p = mesh.p.astype(np.int64)
+    Np = np.where(tri, (p + 1) * (p + 2) // 2, (p + 1) ** 2)
+    Nc = np.where(tri, np.array([0, 3, 6, 12, 16, 25])[p], (p + 1) ** 2)
+    # compulsory bytes: state in and residual out, the volume geometry at the cubature (QUAD: 4 metrics and det J;
+    # TRI: 4 metrics), n J_f at the GL points of every face (2 values)
+    geo = np.where(tri, 4 * Nc, 5 * Nc) * 8
+    fmax = np.maximum(mesh.p[mesh.faces[:, 0]], np.where(mesh.faces[:, 2] >= 0, mesh.p[mesh.faces[:, 2]], 0)) + 1
+    alg = int((2 * 4 * Np * 8 + geo).sum() + (fmax * 2 * 8).sum())
+    ndof = u.numel()
+    return {"workload": "2-D Euler, configuration 4's mesh: 500x300 periodic grid, a seeded third of the cells cut "
+                        "into two TRIs, p ~ U{1..5} per element, Roe", "elements": int(mesh.E),
+            "tri_elements": int(tri.sum()), "faces": int(mesh.F), "ndof": ndof, "dtype": "f64",
+            "value": round(ndof / (ms * 1e-3) / 1e9, 3), "unit": "GDOF/s", "ms_per_step": round(ms, 5),
+            "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "traffic": None}}
+
+
 def config2_throughput(torch, dev, steps, warmup):
     """2-D Euler, DG p=3 on the 128 x 128 periodic QUAD mesh of [0,10]^2, isentropic vortex (strength 5, radius 1,
     Mach 0.5 along x) plus a seeded 1e-3 perturbation, Roe-Pike: the FP64 and mixed residuals as graph replays,
@@ -567,6 +613,8 @@ def main():
                     help="run only the Navier-Stokes residual leg and print its JSON (profiling aid)")
     ap.add_argument("--hp-only", action="store_true",
                     help="run only the variable-order Navier-Stokes leg and print its JSON (profiling aid)")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="run only the mixed TRI/QUAD Euler leg and print its JSON (profiling aid)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 1 GiB read + 1 GiB write elementwise kernel (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -586,6 +634,10 @@ def main():
         return
     if args.ns_only:
         print(json.dumps({"navier_stokes": ns_throughput(torch, dev, args.steps, args.warmup)}), flush=True)
+        return
+    if args.mixed_only:
+        print(json.dumps({"euler_mixed_tri_quad": mixed_euler_throughput(torch, dev, args.steps, args.warmup)}),
+              flush=True)
         return
     if args.hp_only:
         print(json.dumps({"navier_stokes_variable_order": hp_ns_throughput(torch, dev, args.steps, args.warmup),
@@ -693,6 +745,9 @@ def main():
     hpe = hp_euler_throughput(torch, dev, args.steps, args.warmup)
     print(f"variable-order euler residual: {hpe['fp64']['ms_per_step']:.4f} ms, {hpe['fp64']['value']} GDOF/s",
           file=sys.stderr, flush=True)
+    mx4 = mixed_euler_throughput(torch, dev, args.steps, args.warmup)
+    print(f"mixed TRI/QUAD euler residual: {mx4['ms_per_step']:.4f} ms, {mx4['value']} GDOF/s", file=sys.stderr,
+          flush=True)
     c2 = config2_throughput(torch, dev, args.steps, args.warmup)
     print(f"config-2 euler residual: fp64 {c2['fp64']['value']} GDOF/s, mixed {c2['mixed']['value']} GDOF/s, "
           f"rk3 {c2['ssp_rk3']['ms_per_step']} ms/step", file=sys.stderr, flush=True)
@@ -732,6 +787,7 @@ def main():
         "fp32_ms_per_step_eager_launch": round(ms32_eager, 5),
         "launch_grids": tuned,
         "fp64_free_stream_max_abs": free_stream,
+        "euler_mixed_tri_quad": mx4,
         "euler_vortex_128": c2,
         "advection_rk3_16": c1,
         "navier_stokes": ns,

@@ -728,19 +728,26 @@ def structured_mixed_mesh(n: tuple, split, p_elem, lo=(0.0, 0.0), hi=(1.0, 1.0),
         out[..., 1] = lo[1] + pts[..., 1] * h[1]
         return curve(out) if curve is not None else out
 
-    xyz = []
-    for (kind, vs), p in zip(elems, p_elem):
-        V = np.array(vs, float)
+    # vertex weights of every element's solution nodes, per (type, p): barycentric (TRI) or bilinear (QUAD)
+    weights = {}
+    for kind, p in set(zip((k for k, _ in elems), p_elem.tolist())):
         if kind == TRI_T:
-            pts = tri_reference_nodes(int(p)) @ V
+            weights[(kind, p)] = tri_reference_nodes(p)
         else:
-            x1, _ = gauss_lobatto(int(p) + 1)
+            x1, _ = gauss_lobatto(p + 1)
             t = 0.5 * (x1 + 1.0)
             a, b = np.meshgrid(t, t)                      # a: r (fastest), b: s
             a, b = a.ravel(), b.ravel()
-            pts = (np.outer((1 - a) * (1 - b), V[0]) + np.outer(a * (1 - b), V[1]) + np.outer((1 - a) * b, V[2])
-                   + np.outer(a * b, V[3]))
-        xyz.append(np.ascontiguousarray(phys(pts).T))
+            weights[(kind, p)] = np.stack([(1 - a) * (1 - b), a * (1 - b), (1 - a) * b, a * b], 1)
+    xyz = [None] * len(elems)
+    groups = {}
+    for e, ((kind, vs), p) in enumerate(zip(elems, p_elem.tolist())):
+        groups.setdefault((kind, p), []).append(e)
+    for key, ids in groups.items():
+        V = np.array([elems[e][1] for e in ids], float)          # (n, nvert, 2)
+        pts = phys(np.einsum("qv,evd->eqd", weights[key], V))   # (n, Np, 2)
+        for k, e in enumerate(ids):
+            xyz[e] = np.ascontiguousarray(pts[k].T)
     # faces: match on wrapped integer endpoints
     open_faces, rows = {}, []
     for e, (kind, vs) in enumerate(elems):
@@ -779,11 +786,15 @@ def structured_mixed_mesh(n: tuple, split, p_elem, lo=(0.0, 0.0), hi=(1.0, 1.0),
 
 
 def mixed_state(mesh: MixedMesh, cons_fn) -> np.ndarray:
-    """Flat state: cons_fn(x (n, 2)) -> (n, nv) conserved values at every element's own nodes, [var][node]."""
+    """Flat state: cons_fn(x (n, 2)) -> (n, nv) conserved values at every element's own nodes, [var][node]
+    (evaluated per (type, p) group: the elements are sorted by it)."""
     out = []
-    for e in range(mesh.E):
-        c = cons_fn(mesh.xyz[e].T)
-        out.append(c.T.ravel())
+    key = mesh.kind.astype(np.int64) * 8 + mesh.p
+    starts = np.concatenate([[0], np.nonzero(np.diff(key))[0] + 1, [mesh.E]])
+    for a, b in zip(starts[:-1], starts[1:]):
+        x = np.stack([mesh.xyz[e].T for e in range(a, b)])          # (n, Np, 2)
+        c = cons_fn(x.reshape(-1, 2)).reshape(b - a, x.shape[1], -1)
+        out.append(c.transpose(0, 2, 1).ravel())
     return np.ascontiguousarray(np.concatenate(out))
 
 
