@@ -42,7 +42,7 @@ void free_hp(Ctx& c)
 	for (dgx_ctx* g : H->groups) dgx_destroy(g);
 	release(H->faces); release(H->nJ); release(H->nJ32); release(H->nJhi); release(H->z64); release(H->z32);
 	release(H->ops64); release(H->ops32);
-	for (dgx::TriGroup& t : H->tri) { release(t.met); release(t.detJ); }
+	for (dgx::TriGroup& t : H->tri) { release(t.met); release(t.detJ); release(t.minv); release(t.K); release(t.gv); release(t.gvt); }
 	release(H->tri_ops64); release(H->mfaces); release(H->mnJ);
 	for (int k = 0; k < dgx::HpData::kLanes; ++k) {
 		if (H->lane[k]) (void)hipStreamDestroy(H->lane[k]);
@@ -366,11 +366,21 @@ dgx_status hp_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned long 
 {
 	dgx::HpData& H = *c.hp;
 	DGX_TRY(dgx::hp_fork(H, c.stream));
+	if (!H.tri.empty() && mode != dgx::MODE_RHS) {
+		(void)dgx::hp_join(H, c.stream);
+		return DGX_ERR_UNSUPPORTED;   // TRI elements: the residual without M^-1 only
+	}
 	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) {
+		const double mu = c.cfg.mu, Pr = c.cfg.Pr;
 		DGX_TRY(on_groups(c, [&](size_t g, Ctx& s) { return dgx::launch_rhs_ns_phase(s, u + H.u_off[g], r + H.u_off[g], 1); }));
+		for (size_t t = 0; t < H.tri.size(); ++t)
+			DGX_TRY(dgx::launch_tri_ns_phase64(H.tri[t], H.tri_ops64, u, r, H.z64, mu, Pr, 1, H.lane[t % dgx::HpData::kLanes]));
 		DGX_TRY(hp_barrier(H, c.stream));
 		DGX_TRY(dgx::launch_hp_faces64(c, u));
+		DGX_TRY(dgx::launch_mix_faces_ns64(H, c, u, H.lane[dgx::HpData::kLanes - 1]));
 		DGX_TRY(hp_barrier(H, c.stream));
+		for (size_t t = 0; t < H.tri.size(); ++t)
+			DGX_TRY(dgx::launch_tri_ns_phase64(H.tri[t], H.tri_ops64, u, r, H.z64, mu, Pr, 2, H.lane[t % dgx::HpData::kLanes]));
 		DGX_TRY(on_groups(c, [&](size_t g, Ctx& s) -> hipError_t {
 			double* rg = r + H.u_off[g];
 			hipError_t e = dgx::launch_rhs_ns_phase(s, u + H.u_off[g], rg, 2);
@@ -380,10 +390,6 @@ dgx_status hp_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned long 
 			return e;
 		}));
 	} else {
-		if (!H.tri.empty() && mode != dgx::MODE_RHS) {
-			(void)dgx::hp_join(H, c.stream);
-			return DGX_ERR_UNSUPPORTED;   // TRI elements: the residual without M^-1 only
-		}
 		DGX_TRY(dgx::launch_hp_faces64(c, u));
 		DGX_TRY(dgx::launch_mix_faces64(H, c, u, H.lane[dgx::HpData::kLanes - 1]));
 		DGX_TRY(hp_barrier(H, c.stream));
@@ -471,7 +477,6 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 			grp[e] = (int)tri_p.size() - 1;
 		}
 	}
-	if (any_tri && c.cfg.pde != DGX_PDE_EULER) return DGX_ERR_UNSUPPORTED;   // TRI: Euler only
 	const int G = (int)order.size(), T = (int)tri_p.size();
 	e_off.push_back(G ? (any_tri ? tri_e0[0] : E) : 0);
 	tri_e0.push_back(E);
@@ -486,11 +491,11 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 		z_off[g + 1] = z_off[g] + (long long)(e_off[g + 1] - e_off[g]) * 4 * zblk * (order[g] + 1);
 	tz_off[0] = z_off[G];
 	for (int t = 0; t < T; ++t)
-		tz_off[t + 1] = tz_off[t] + (long long)(tri_e0[t + 1] - tri_e0[t]) * 3 * nv * (tri_p[t] + 1);
-	// lifted-flux block of local face lf of element e (its own GL face points, [v][a])
+		tz_off[t + 1] = tz_off[t] + (long long)(tri_e0[t + 1] - tri_e0[t]) * 3 * zblk * (tri_p[t] + 1);
+	// lifted-flux block of local face lf of element e (its own GL face points, [v][a]; Navier-Stokes: then y)
 	auto zblock = [&](int e, int lf) -> long long {
 		const int N = pe[e] + 1;
-		if (kind(e)) return tz_off[grp[e]] + ((long long)(e - tri_e0[grp[e]]) * 3 + lf) * nv * N;
+		if (kind(e)) return tz_off[grp[e]] + ((long long)(e - tri_e0[grp[e]]) * 3 + lf) * zblk * N;
 		return z_off[grp[e]] + ((long long)(e - e_off[grp[e]]) * 4 + lf) * zblk * N;
 	};
 	// outward n J_f of element e's face lf at face coordinates s (its own map)
@@ -522,6 +527,7 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 		if (kind(eL) || (bc == DGX_BC_INTERIOR && kind(eR))) {
 			// a TRI side: evaluated by the mixed face kernel, QUAD sides read its lifted flux
 			if (bc != DGX_BC_INTERIOR && (bc != DGX_BC_RIEMANN && bc != DGX_BC_SLIPWALL)) return DGX_ERR_ARG;
+			if (bc != DGX_BC_INTERIOR && c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;   // viscous BCs not built
 			dgx::MixFace x{};
 			const bool inter = (bc == DGX_BC_INTERIOR);
 			x.uL = uoff[eL];
@@ -529,6 +535,8 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 			x.kL = kind(eL);
 			x.NL = pe[eL] + 1;
 			x.lfL = fL;
+			x.eL = kind(eL) ? eL - tri_e0[gL] : eL - e_off[gL];
+			x.tL = kind(eL) ? gL : -1;
 			x.bc = bc;
 			x.rev = (obits & 2) ? 1 : 0;
 			if (!kind(eL)) push(gL, eL - e_off[gL], fL, -1, -1, 0, dgx::kBcNonconforming);
@@ -538,6 +546,8 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 				x.kR = kind(eR);
 				x.NR = pe[eR] + 1;
 				x.lfR = fR;
+				x.eR = kind(eR) ? eR - tri_e0[grp[eR]] : eR - e_off[grp[eR]];
+				x.tR = kind(eR) ? grp[eR] : -1;
 				if (!kind(eR)) push(grp[eR], eR - e_off[grp[eR]], fR, -1, -1, 0, dgx::kBcNonconforming);
 			} else {
 				x.NR = x.NL;
@@ -688,6 +698,21 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 			DGX_TRY(hipMalloc(&hg.detJ, sizeof(double) * dj.size()));
 			DGX_TRY(hipMemcpy(hg.met, met.data(), sizeof(double) * met.size(), hipMemcpyHostToDevice));
 			DGX_TRY(hipMemcpy(hg.detJ, dj.data(), sizeof(double) * dj.size(), hipMemcpyHostToDevice));
+			if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) {
+				// BR2: each element's M^-1 and the face-trace operators K_f = Ifc_f M^-1 tw0_f of its lifts;
+				// the volume gradient and its face traces are written per evaluation
+				const int Np = o.Np, N = o.N;
+				std::vector<double> mi((size_t)tg.E * Np * Np), K((size_t)tg.E * 3 * N * N);
+				for (int k = 0; k < tg.E; ++k)
+					if (!dgx::tri_mass_data(o, &dj[(size_t)k * o.Nc], &mi[(size_t)k * Np * Np], &K[(size_t)k * 3 * N * N]))
+						return DGX_ERR_ARG;
+				DGX_TRY(hipMalloc(&hg.minv, sizeof(double) * mi.size()));
+				DGX_TRY(hipMalloc(&hg.K, sizeof(double) * K.size()));
+				DGX_TRY(hipMalloc(&hg.gv, sizeof(double) * (size_t)tg.E * 2 * nv * Np));
+				DGX_TRY(hipMalloc(&hg.gvt, sizeof(double) * (size_t)tg.E * 3 * 2 * nv * N));
+				DGX_TRY(hipMemcpy(hg.minv, mi.data(), sizeof(double) * mi.size(), hipMemcpyHostToDevice));
+				DGX_TRY(hipMemcpy(hg.K, K.data(), sizeof(double) * K.size(), hipMemcpyHostToDevice));
+			}
 		}
 	}
 	H.Fmix = (int)mf.size();

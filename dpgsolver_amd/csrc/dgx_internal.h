@@ -96,27 +96,35 @@ struct TriOps {
 bool build_tri_ops(int p, TriOps<double>& o);
 void tri_geometry(const TriOps<double>& o, const double* xyz, double* met, double* detJ);
 void tri_face_nj(int p, int lf, const double* xyz, const double* s, int nq, double* nj);
+bool tri_mass_data(const TriOps<double>& o, const double* detJ, double* minv, double* K);
 
 // A face with a TRI element on at least one side, evaluated by k_mix_faces: each side's trace at its own GL face
 // points (TRI: from all its nodes; QUAD: from its GLL face nodes), interpolated to the face's GL points, the
 // numerical flux there, and each side's lifted flux z = P^T w (J_f F) / w_own at its own points, own ordering.
 struct MixFace {
 	long long uL, uR;    // first state value of each side's element
-	long long zL, zR;    // first lifted-flux value of each side's own face block [v][a]
+	long long zL, zR;    // first lifted-flux value of each side's own face block [v][a] (Navier-Stokes: then y [d][v][a])
 	int kL, kR;          // element type of each side: 0 QUAD, 1 TRI
 	int NL, NR;          // p + 1 of each side
 	int lfL, lfR;        // local faces
 	int rev;             // 1: the right face runs the other way (ind_ord 1)
 	int bc;              // DGX_BC_* (boundary faces: the right side is unused)
+	int eL, eR;          // each side's element index within its (type, p) group
+	int tL, tR;          // Navier-Stokes, TRI sides: the TRI group index
 };
 
 // The TRI elements of one order of a mixed mesh (a contiguous range of the state vector).
 struct TriGroup {
 	int p = 0, E = 0, e0 = 0;
 	long long u_off = 0;             // first state value
-	long long z_off = 0;             // first lifted-flux value (E x 3 x nv x (p + 1))
+	long long z_off = 0;             // first lifted-flux value (E x 3 x zblk x (p + 1), zblk = nv or 3 nv)
 	double* met = nullptr;           // E x 4 x Nc: J-scaled cofactors (y_s, -x_s, -y_r, x_r) at the cubature
 	double* detJ = nullptr;          // E x Nc
+	// Navier-Stokes (BR2)
+	double* minv = nullptr;          // E x Np x Np: inverse mass matrices
+	double* K = nullptr;             // E x 3 x N x N: lift traces Ifc_f M^-1 tw0_f per local face
+	double* gv = nullptr;            // E x 2 x nv x Np: volume part of the weak gradient (written each residual)
+	double* gvt = nullptr;           // E x 3 x 2 x nv x N: its traces at the own GL face points
 };
 
 // a run of faces between the same two orders (the faces are sorted by pair)
@@ -256,6 +264,11 @@ hipError_t launch_hp_faces_mixed(Ctx& c, const double* u);   // FP64 state and f
 hipError_t launch_tri_rhs64(const TriGroup& g, const TriOps<double>* ops, const double* u, double* r, const double* z,
                             hipStream_t st);
 hipError_t launch_mix_faces64(const HpData& H, const Ctx& c, const double* u, hipStream_t st);
+// Navier-Stokes on mixed meshes: a TRI group's volume gradient and its traces (phase 1) or its residual (phase 2),
+// and the faces with a TRI side (the BR2 products y, then the lifted inviscid + viscous fluxes)
+hipError_t launch_tri_ns_phase64(const TriGroup& g, const TriOps<double>* ops, const double* u, double* r,
+                                 const double* z, double mu, double Pr, int phase, hipStream_t st);
+hipError_t launch_mix_faces_ns64(const HpData& H, const Ctx& c, const double* u, hipStream_t st);
 hipError_t launch_minv(Ctx& c, double* r);
 hipError_t launch_positivity(Ctx& c, double* u, int* status);
 hipError_t launch_maxabs(Ctx& c, const double* r, size_t n, unsigned long long* dmax);

@@ -215,6 +215,56 @@ void tri_geometry(const TriOps<double>& o, const double* xyz, double* met, doubl
 	}
 }
 
+// The element's mass matrix M = I^T diag(w det J) I (constructor_mass_T, volume_solver_T.c:115-145), its inverse
+// (Np x Np, row-major) and, per local face, the BR2 lift trace K_f = Ifc_f M^-1 tw0_f (N x N): the trace at the
+// element's own GL face points of the lift M^-1 tw0_f y of face values y.  false if M is singular.
+bool tri_mass_data(const TriOps<double>& o, const double* detJ, double* minv, double* K)
+{
+	const int Np = o.Np, Nc = o.Nc, N = o.N;
+	std::vector<double> M((size_t)Np * 2 * Np, 0.0);
+	for (int i = 0; i < Np; ++i)
+		for (int j = 0; j < Np; ++j) {
+			double acc = 0.0;
+			for (int q = 0; q < Nc; ++q) acc += o.I[q * Np + i] * o.w[q] * detJ[q] * o.I[q * Np + j];
+			M[(size_t)i * 2 * Np + j] = acc;
+		}
+	for (int i = 0; i < Np; ++i) M[(size_t)i * 2 * Np + Np + i] = 1.0;
+	for (int c = 0; c < Np; ++c) {   // Gauss-Jordan, partial pivoting (M is symmetric positive definite)
+		int piv = c;
+		for (int i = c + 1; i < Np; ++i)
+			if (std::fabs(M[(size_t)i * 2 * Np + c]) > std::fabs(M[(size_t)piv * 2 * Np + c])) piv = i;
+		if (!(M[(size_t)piv * 2 * Np + c] != 0.0)) return false;
+		if (piv != c)
+			for (int j = 0; j < 2 * Np; ++j) std::swap(M[(size_t)c * 2 * Np + j], M[(size_t)piv * 2 * Np + j]);
+		const double d = 1.0 / M[(size_t)c * 2 * Np + c];
+		for (int j = 0; j < 2 * Np; ++j) M[(size_t)c * 2 * Np + j] *= d;
+		for (int i = 0; i < Np; ++i) {
+			if (i == c) continue;
+			const double f = M[(size_t)i * 2 * Np + c];
+			if (f == 0.0) continue;
+			for (int j = 0; j < 2 * Np; ++j) M[(size_t)i * 2 * Np + j] -= f * M[(size_t)c * 2 * Np + j];
+		}
+	}
+	for (int i = 0; i < Np; ++i)
+		for (int j = 0; j < Np; ++j) minv[i * Np + j] = M[(size_t)i * 2 * Np + Np + j];
+	std::vector<double> L((size_t)Np * N);
+	for (int lf = 0; lf < 3; ++lf) {
+		for (int m = 0; m < Np; ++m)
+			for (int b = 0; b < N; ++b) {
+				double acc = 0.0;
+				for (int n = 0; n < Np; ++n) acc += minv[m * Np + n] * o.tw0[lf][n * N + b];
+				L[(size_t)m * N + b] = acc;
+			}
+		for (int a = 0; a < N; ++a)
+			for (int b = 0; b < N; ++b) {
+				double acc = 0.0;
+				for (int m = 0; m < Np; ++m) acc += o.Ifc[lf][a * Np + m] * L[(size_t)m * N + b];
+				K[(lf * N + a) * N + b] = acc;
+			}
+	}
+	return true;
+}
+
 // outward n J_f of local face lf of a TRI element of order p at face coordinates s[0..nq) (the tangent of its map
 // turned clockwise, reversed on face 1, which runs clockwise on a counter-clockwise element)
 void tri_face_nj(int p, int lf, const double* xyz, const double* s, int nq, double* nj)

@@ -3,6 +3,8 @@
 Cases: one order everywhere (p = 1..5) on a checkerboard of split cells, random orders 1..5 per element on
 curved meshes, an all-TRI mesh, and a non-periodic box with Riemann and slip-wall faces on TRI and QUAD sides;
 Roe-Pike and Lax-Friedrichs; the error is taken relative to the larger of max |R| and max |volume term|.
+Navier-Stokes (BR2) on periodic mixed meshes against MixedOracle.rhs_ns, same bar and scale: checkerboards at
+p = 1, 3, 5, random orders on curved meshes with both fluxes, an all-TRI curved mesh; free stream, conservation.
 Also: graph replays equal the eager call, and the entry points not built for TRI elements refuse the mesh
 instead of computing something else.
 """
@@ -144,7 +146,100 @@ def test_entry_points_not_built_for_tri_refuse(dev):
             call()
         assert ei.value.code == -3                  # DGX_ERR_UNSUPPORTED
     ctx.close()
-    ns = _lib.Context(2, 5, M.PDE_NAVIER_STOKES, M.FLUX_ROE, mu=0.01, Pr=0.72)
-    with pytest.raises(_lib.DgxError):
-        ns.upload_mesh_mixed(mesh)
+    # Navier-Stokes: no viscous boundary conditions, and the residual without M^-1 only
+    ns = _lib.Context(2, 5, M.PDE_NAVIER_STOKES, M.FLUX_ROE, mu=MU, Pr=PR)
+    with pytest.raises(_lib.DgxError) as ei:
+        ns.upload_mesh_mixed(CASES[-1][1])
+    assert ei.value.code == -3
+    ns.upload_mesh_mixed(mesh)
+    with pytest.raises(_lib.DgxError) as ei:
+        ns.compute_rhs_explicit(u.data_ptr(), r.data_ptr())
+    assert ei.value.code == -3
     ns.close()
+
+
+# ---------------------------------------------------------------------------------------------- Navier-Stokes
+MU, PR = 0.05, 0.72
+
+
+def _ns_cases():
+    out = []
+    for p in (1, 3, 5):
+        n = 4
+        split = (np.add.outer(np.arange(n), np.arange(n)) % 2 == 0).ravel()
+        m = M.structured_mixed_mesh((n, n), split, np.full(n * n + split.sum(), p, np.int32))
+        out.append((f"ns_checker_p{p}_roe", m, M.FLUX_ROE))
+    rng = np.random.default_rng(17)
+    for flux, name in ((M.FLUX_ROE, "roe"), (M.FLUX_LF, "lf")):
+        split = rng.random(30) < 0.5
+        p = rng.integers(1, 6, int(30 + split.sum())).astype(np.int32)
+        out.append((f"ns_random_orders_curved_{name}", M.structured_mixed_mesh((6, 5), split, p, curve=_curve()), flux))
+    m = M.structured_mixed_mesh((4, 3), np.ones(12, bool), rng.integers(1, 6, 24).astype(np.int32), curve=_curve())
+    out.append(("ns_all_tri_random_orders_curved", m, M.FLUX_ROE))
+    return out
+
+
+NS_CASES = _ns_cases()
+
+
+def _ns_ctx(mesh, flux):
+    from dpgsolver_amd import _lib
+
+    ctx = _lib.Context(2, int(mesh.p.max()), M.PDE_NAVIER_STOKES, flux, mu=MU, Pr=PR)
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    ctx.upload_mesh_mixed(mesh)
+    return ctx
+
+
+@pytest.mark.parametrize("case", NS_CASES, ids=[c[0] for c in NS_CASES])
+def test_mixed_mesh_ns_residual_matches_oracle(dev, case):
+    name, mesh, flux = case
+    print(f"case {name}: E={mesh.E} (TRI {int(mesh.kind.sum())}), F={mesh.F}", flush=True)
+    u = _state(mesh, None)
+    orc = MO.MixedOracle(mesh, flux, viscous=(MU, PR))
+    ref = orc.rhs_ns(u)
+    assert np.abs(ref - orc.rhs(u)).max() > 1e-3 * np.abs(ref).max()     # the viscous part is not negligible
+    ctx = _ns_ctx(mesh, flux)
+    ud = torch.from_numpy(u).to(dev)
+    rd = torch.full_like(ud, float("nan"))
+    ctx.compute_rhs(ud.data_ptr(), rd.data_ptr())
+    torch.cuda.synchronize()
+    got = rd.cpu().numpy()
+    assert np.isfinite(got).all()
+    scale = max(np.abs(orc.rhs(u, "volume")).max(), np.abs(ref).max())
+    err = np.abs(got - ref).max() / scale
+    print(f"  rel err {err:.2e}", flush=True)
+    assert err < TOL64, err
+    off = orc.off
+    tot = sum(got[off[e]:off[e + 1]].reshape(4, -1).sum(1) for e in range(mesh.E))
+    assert np.abs(tot).max() < 1e-13 * np.abs(got).sum()
+    ctx.close()
+
+
+def test_mixed_mesh_ns_free_stream_and_graph_replays(dev):
+    from dpgsolver_amd import _lib
+
+    straight = M.structured_mixed_mesh((6, 5), np.arange(30) % 3 == 0, np.full(40, 3, np.int32))
+    cs = _ns_ctx(straight, M.FLUX_ROE)
+    prim = np.array([1.0, 0.3, -0.2, 1.0])
+    uf = torch.from_numpy(M.mixed_state(straight, lambda x: np.broadcast_to(M.prim_to_cons(prim, 2), (len(x), 4)))).to(dev)
+    rf = torch.empty_like(uf)
+    cs.compute_rhs(uf.data_ptr(), rf.data_ptr())
+    torch.cuda.synchronize()
+    assert rf.abs().max().item() < 1e-13
+    cs.close()
+    name, mesh, flux = NS_CASES[3]
+    ctx = _ns_ctx(mesh, flux)
+    u = torch.from_numpy(_state(mesh, None)).to(dev)
+    r_eager = torch.empty_like(u)
+    ctx.compute_rhs(u.data_ptr(), r_eager.data_ptr())
+    r = torch.full_like(u, float("nan"))
+    g = ctx.graph_rhs(_lib.GRAPH_RHS_FP64, u.data_ptr(), r.data_ptr())
+    for _ in range(3):
+        r.fill_(float("nan"))
+        g.launch(1)
+        g.sync()
+        assert torch.equal(r, r_eager)
+        torch.cuda.synchronize()
+    g.close()
+    ctx.close()

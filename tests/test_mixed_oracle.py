@@ -181,3 +181,63 @@ def test_mixed_residual_converges_to_the_divergence(p, curved):
         errs.append(worst)
     rate = math.log2(errs[0] / errs[1])
     assert rate >= p - 0.5, (errs, rate)
+
+
+# ---------------------------------------------------------------------------------------------- Navier-Stokes
+MU, PR = 0.05, 0.72
+
+
+def test_mixed_ns_oracle_equals_hp_oracle_on_quad_meshes():
+    rng = np.random.default_rng(32)
+    n = (4, 3)
+    p = rng.integers(1, 6, 12).astype(np.int32)
+    mm = M.structured_mixed_mesh(n, np.zeros(12, bool), p, curve=_curve())
+    hm = M.structured_hp_mesh(p, n, (0.0, 0.0), (1.0, 1.0), (True, True), curve=_curve())
+    state = lambda x: M.prim_to_cons(M.smooth_state_3d(x), 2)
+    um, uh = M.mixed_state(mm, state), M.hp_state(hm, state)
+    # Lax-Friedrichs: the two mesh builders list some faces with left and right swapped, and the reference's
+    # Roe-Pike flux with its entropy fix is not invariant under that swap (F(a, b, n) != -F(b, a, -n))
+    rm = MO.MixedOracle(mm, M.FLUX_LF, viscous=(MU, PR)).rhs_ns(um)
+    rh = HO.HpOracle(hm, 1, M.FLUX_LF, viscous=(MU, PR)).rhs_ns(uh)
+    assert np.abs(rm - rh).max() < 1e-13 * np.abs(rh).max()
+    assert np.abs(rh - HO.HpOracle(hm, 1, M.FLUX_LF).rhs(uh)).max() > 1e-3 * np.abs(rh).max()
+
+
+def test_mixed_ns_constants_and_conservation():
+    prim = np.array([1.1, 0.3, -0.2, 0.9])
+    const = lambda x: np.broadcast_to(M.prim_to_cons(prim, 2), (len(x), 4))
+    straight = _mixed(seed=8)
+    orc = MO.MixedOracle(straight, M.FLUX_ROE, viscous=(MU, PR))
+    assert np.abs(orc.rhs_ns(M.mixed_state(straight, const))).max() < 1e-13
+    curved = _mixed(seed=9, curve=_curve())
+    orc = MO.MixedOracle(curved, M.FLUX_LF, viscous=(MU, PR))
+    r = orc.rhs_ns(M.mixed_state(curved, lambda x: M.prim_to_cons(_smooth_prim(x), 2)))
+    tot = sum(r[orc.off[e]:orc.off[e + 1]].reshape(4, -1).sum(1) for e in range(curved.E))
+    assert np.abs(tot).max() < 1e-14 * np.abs(r).sum()
+
+
+def test_mixed_ns_gradient_of_a_linear_state_and_its_integral():
+    """The BR2 gradient G of u = a + b . x is b exactly in every element with no face on the periodic wrap (the
+    state is continuous elsewhere, so every lift there vanishes); and on any periodic mesh the integrals of G
+    sum to zero (1^T M_e G_e = int grad u + int_boundary n (u* - u), and the averaged traces u* cancel between
+    the two sides), which fails for a lift of the wrong sign or a side's own trace in place of u*."""
+    mesh = _mixed(n=(5, 4), seed=10)
+    orc = MO.MixedOracle(mesh, M.FLUX_ROE, viscous=(MU, PR))
+    b = np.array([[0.3, -0.7], [1.1, 0.2], [-0.4, 0.5], [0.9, 1.3]])          # (nv, d)
+    lin = lambda x: np.array([1.0, 0.5, -0.2, 2.0]) + x @ b.T
+    G, _, _ = orc.ns_gradient(M.mixed_state(mesh, lin))
+    wrap = set()
+    for eL, fL, eR, fR, ord_, bc in mesh.faces:
+        cL, cR = mesh.xyz[eL].mean(1), mesh.xyz[eR].mean(1)
+        if np.abs(cL - cR).max() > 0.5:
+            wrap.update((int(eL), int(eR)))
+    inner = [e for e in range(mesh.E) if e not in wrap]
+    assert len(inner) >= 6 and len(wrap) > 0
+    for e in inner:
+        np.testing.assert_allclose(G[e], np.broadcast_to(b.T[:, :, None], G[e].shape), atol=1e-12, rtol=0)
+    u = M.mixed_state(mesh, lambda x: M.prim_to_cons(_smooth_prim(x), 2))
+    G, gv, _ = orc.ns_gradient(u)
+    tot = sum(np.einsum("ij,dvj->dv", orc.mass(e), G[e]) for e in range(mesh.E))
+    vol = sum(np.abs(np.einsum("ij,dvj->dv", orc.mass(e), gv[e])).sum() for e in range(mesh.E))
+    assert np.abs(tot).max() < 1e-14 * vol, (tot, vol)
+    assert vol > 1e-2
