@@ -453,16 +453,22 @@ def hp_euler_throughput(torch, dev, steps, warmup):
             "mixed": {**leg(_lib.GRAPH_RHS_MIXED), "rel_err_vs_fp64": err[_lib.GRAPH_RHS_MIXED]}}
 
 
-def mixed_euler_throughput(torch, dev, steps, warmup):
-    """2-D Euler FP64 residual on configuration 4's mesh: the 500 x 300 periodic base grid of [0,1]^2 with a seeded
-    third of its cells cut into two TRIs (99,980 TRI + 100,010 QUAD), p ~ U{1..5} per element (seeded), Roe; the
-    state is the 3-D workload's formula in 2-D.  One evaluation = the face kernels (between QUAD orders, and every
-    face with a TRI side) and every (type, p) group's residual kernel, replayed as one HIP graph."""
+def mixed_throughput(torch, dev, steps, warmup, pde):
+    """2-D FP64 residual on configuration 4's mesh: the 500 x 300 periodic base grid of [0,1]^2 with a seeded third
+    of its cells cut into two TRIs (99,980 TRI + 100,010 QUAD), p ~ U{1..5} per element (seeded), Roe; the state is
+    the 3-D workload's formula in 2-D.  Euler, or Navier-Stokes (BR2) with Re 100, Pr 0.72 (mu = 0.0028), the
+    configuration's own equations.  One evaluation = (Navier-Stokes: every group's gradient kernel,) the face
+    kernels (between QUAD orders, and every face with a TRI side) and every (type, p) group's residual kernel,
+    replayed as one HIP graph."""
     import numpy as np
     from dpgsolver_amd import _lib, mesh as M
 
+    ns = pde == "navier_stokes"
     mesh = M.config4_mixed_mesh()
-    ctx = _lib.Context(2, 5, M.PDE_EULER, M.FLUX_ROE)
+    if ns:
+        ctx = _lib.Context(2, 5, M.PDE_NAVIER_STOKES, M.FLUX_ROE, mu=0.28 / 100.0, Pr=0.72)
+    else:
+        ctx = _lib.Context(2, 5, M.PDE_EULER, M.FLUX_ROE)
     stream = torch.cuda.current_stream(dev)
     ctx.set_stream(stream.cuda_stream)
     ctx.upload_mesh_mixed(mesh)
@@ -485,13 +491,17 @@ def mixed_euler_throughput(torch, dev, steps, warmup):
     Np = np.where(tri, (p + 1) * (p + 2) // 2, (p + 1) ** 2)
     Nc = np.where(tri, np.array([0, 3, 6, 12, 16, 25])[p], (p + 1) ** 2)
     # compulsory bytes: state in and residual out, the volume geometry at the cubature (QUAD: 4 metrics and det J;
-    # TRI: 4 metrics), n J_f at the GL points of every face (2 values)
+    # TRI: 4 metrics), n J_f at the GL points of every face (2 values); Navier-Stokes TRI: also each element's
+    # M^-1 (Np^2) and the three face-trace operators of its lifts (3 (p+1)^2)
     geo = np.where(tri, 4 * Nc, 5 * Nc) * 8
+    if ns:
+        geo = geo + np.where(tri, Np * Np + 3 * (p + 1) ** 2, 0) * 8
     fmax = np.maximum(mesh.p[mesh.faces[:, 0]], np.where(mesh.faces[:, 2] >= 0, mesh.p[mesh.faces[:, 2]], 0)) + 1
     alg = int((2 * 4 * Np * 8 + geo).sum() + (fmax * 2 * 8).sum())
     ndof = u.numel()
-    return {"workload": "2-D Euler, configuration 4's mesh: 500x300 periodic grid, a seeded third of the cells cut "
-                        "into two TRIs, p ~ U{1..5} per element, Roe", "elements": int(mesh.E),
+    eq = "Navier-Stokes BR2 (Re 100, Pr 0.72)" if ns else "Euler"
+    return {"workload": f"2-D {eq}, configuration 4's mesh: 500x300 periodic grid, a seeded third of the cells "
+                        "cut into two TRIs, p ~ U{1..5} per element, Roe", "elements": int(mesh.E),
             "tri_elements": int(tri.sum()), "faces": int(mesh.F), "ndof": ndof, "dtype": "f64",
             "value": round(ndof / (ms * 1e-3) / 1e9, 3), "unit": "GDOF/s", "ms_per_step": round(ms, 5),
             "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
@@ -614,7 +624,7 @@ def main():
     ap.add_argument("--hp-only", action="store_true",
                     help="run only the variable-order Navier-Stokes leg and print its JSON (profiling aid)")
     ap.add_argument("--mixed-only", action="store_true",
-                    help="run only the mixed TRI/QUAD Euler leg and print its JSON (profiling aid)")
+                    help="run only the mixed TRI/QUAD legs and print their JSON (profiling aid)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 1 GiB read + 1 GiB write elementwise kernel (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -636,8 +646,9 @@ def main():
         print(json.dumps({"navier_stokes": ns_throughput(torch, dev, args.steps, args.warmup)}), flush=True)
         return
     if args.mixed_only:
-        print(json.dumps({"euler_mixed_tri_quad": mixed_euler_throughput(torch, dev, args.steps, args.warmup)}),
-              flush=True)
+        print(json.dumps({"euler_mixed_tri_quad": mixed_throughput(torch, dev, args.steps, args.warmup, "euler"),
+                          "navier_stokes_mixed_tri_quad": mixed_throughput(torch, dev, args.steps, args.warmup,
+                                                                           "navier_stokes")}), flush=True)
         return
     if args.hp_only:
         print(json.dumps({"navier_stokes_variable_order": hp_ns_throughput(torch, dev, args.steps, args.warmup),
@@ -745,9 +756,12 @@ def main():
     hpe = hp_euler_throughput(torch, dev, args.steps, args.warmup)
     print(f"variable-order euler residual: {hpe['fp64']['ms_per_step']:.4f} ms, {hpe['fp64']['value']} GDOF/s",
           file=sys.stderr, flush=True)
-    mx4 = mixed_euler_throughput(torch, dev, args.steps, args.warmup)
+    mx4 = mixed_throughput(torch, dev, args.steps, args.warmup, "euler")
     print(f"mixed TRI/QUAD euler residual: {mx4['ms_per_step']:.4f} ms, {mx4['value']} GDOF/s", file=sys.stderr,
           flush=True)
+    mx4ns = mixed_throughput(torch, dev, args.steps, args.warmup, "navier_stokes")
+    print(f"mixed TRI/QUAD navier-stokes residual: {mx4ns['ms_per_step']:.4f} ms, {mx4ns['value']} GDOF/s",
+          file=sys.stderr, flush=True)
     c2 = config2_throughput(torch, dev, args.steps, args.warmup)
     print(f"config-2 euler residual: fp64 {c2['fp64']['value']} GDOF/s, mixed {c2['mixed']['value']} GDOF/s, "
           f"rk3 {c2['ssp_rk3']['ms_per_step']} ms/step", file=sys.stderr, flush=True)
@@ -788,6 +802,7 @@ def main():
         "launch_grids": tuned,
         "fp64_free_stream_max_abs": free_stream,
         "euler_mixed_tri_quad": mx4,
+        "navier_stokes_mixed_tri_quad": mx4ns,
         "euler_vortex_128": c2,
         "advection_rk3_16": c1,
         "navier_stokes": ns,
