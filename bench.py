@@ -491,11 +491,11 @@ def mixed_throughput(torch, dev, steps, warmup, pde):
     Np = np.where(tri, (p + 1) * (p + 2) // 2, (p + 1) ** 2)
     Nc = np.where(tri, np.array([0, 3, 6, 12, 16, 25])[p], (p + 1) ** 2)
     # compulsory bytes: state in and residual out, the volume geometry at the cubature (QUAD: 4 metrics and det J;
-    # TRI: 4 metrics), n J_f at the GL points of every face (2 values); Navier-Stokes TRI: also each element's
-    # M^-1 (Np^2) and the three face-trace operators of its lifts (3 (p+1)^2)
+    # TRI: 4 metrics), n J_f at the GL points of every face (2 values); Navier-Stokes on these straight (affine)
+    # TRIs: also 1 / det J per element (their M^-1 and lift traces are the reference ones scaled by it)
     geo = np.where(tri, 4 * Nc, 5 * Nc) * 8
     if ns:
-        geo = geo + np.where(tri, Np * Np + 3 * (p + 1) ** 2, 0) * 8
+        geo = geo + np.where(tri, 1, 0) * 8
     fmax = np.maximum(mesh.p[mesh.faces[:, 0]], np.where(mesh.faces[:, 2] >= 0, mesh.p[mesh.faces[:, 2]], 0)) + 1
     alg = int((2 * 4 * Np * 8 + geo).sum() + (fmax * 2 * 8).sum())
     ndof = u.numel()

@@ -42,7 +42,10 @@ void free_hp(Ctx& c)
 	for (dgx_ctx* g : H->groups) dgx_destroy(g);
 	release(H->faces); release(H->nJ); release(H->nJ32); release(H->nJhi); release(H->z64); release(H->z32);
 	release(H->ops64); release(H->ops32);
-	for (dgx::TriGroup& t : H->tri) { release(t.met); release(t.detJ); release(t.minv); release(t.K); release(t.gv); release(t.gvt); }
+	for (dgx::TriGroup& t : H->tri) {
+		release(t.met); release(t.detJ); release(t.tr); release(t.jinv);
+		release(t.minv); release(t.K); release(t.gv); release(t.gvt);
+	}
 	release(H->tri_ops64); release(H->mfaces); release(H->mnJ);
 	for (int k = 0; k < dgx::HpData::kLanes; ++k) {
 		if (H->lane[k]) (void)hipStreamDestroy(H->lane[k]);
@@ -391,7 +394,12 @@ dgx_status hp_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned long 
 		}));
 	} else {
 		DGX_TRY(dgx::launch_hp_faces64(c, u));
-		DGX_TRY(dgx::launch_mix_faces64(H, c, u, H.lane[dgx::HpData::kLanes - 1]));
+		if (!H.tri.empty()) {   // the TRI traces (beside the faces between QUAD orders), then the faces with a TRI side
+			for (size_t t = 0; t < H.tri.size(); ++t)
+				DGX_TRY(dgx::launch_tri_trace64(H.tri[t], H.tri_ops64, u, H.lane[t % dgx::HpData::kLanes]));
+			DGX_TRY(hp_barrier(H, c.stream));
+			DGX_TRY(dgx::launch_mix_faces64(H, c, u, H.lane[dgx::HpData::kLanes - 1]));
+		}
 		DGX_TRY(hp_barrier(H, c.stream));
 		DGX_TRY(on_groups(c, [&](size_t g, Ctx& s) { return dgx::launch_rhs64(s, u + H.u_off[g], r + H.u_off[g], mode, dmax); }));
 		for (size_t t = 0; t < H.tri.size(); ++t)
@@ -604,6 +612,28 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 			ro[2 * q + 1] = -tmp[2 * qr + 1];
 		}
 	}
+	// faces with a TRI side sorted by (Nf, kL, NL, kR, NR) (stable): one launch per Nf, waves of one shape
+	std::vector<dgx::MixRun> mruns;
+	{
+		auto mkey = [&](const dgx::MixFace& x) {
+			return ((std::max(x.NL, x.NR) * 2 + x.kL) * 8 + x.NL) * 16 + x.kR * 8 + x.NR;
+		};
+		std::vector<int> mp(mf.size());
+		for (size_t i = 0; i < mf.size(); ++i) mp[i] = (int)i;
+		std::stable_sort(mp.begin(), mp.end(), [&](int a, int b) { return mkey(mf[a]) < mkey(mf[b]); });
+		std::vector<dgx::MixFace> ms(mf.size());
+		std::vector<double> mn(mnj.size());
+		for (size_t i = 0; i < mf.size(); ++i) {
+			ms[i] = mf[mp[i]];
+			const size_t src = (size_t)mp[i] * 2 * dgx::kHpMaxN, dst = i * 2 * dgx::kHpMaxN;
+			std::copy(mnj.begin() + src, mnj.begin() + src + 2 * dgx::kHpMaxN, mn.begin() + dst);
+			const int Nf = std::max(ms[i].NL, ms[i].NR);
+			if (mruns.empty() || mruns.back().Nf != Nf) mruns.push_back(dgx::MixRun{ Nf, (int)i, 0 });
+			mruns.back().count += 1;
+		}
+		mf.swap(ms);
+		mnj.swap(mn);
+	}
 	// faces sorted by their (NL, NR) pair (stable): one kernel instance per pair, every wave on one pair
 	std::vector<int> fperm(hf.size());
 	for (size_t i = 0; i < hf.size(); ++i) fperm[i] = (int)i;
@@ -694,28 +724,51 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 			}
 			H.tri.push_back(tg);
 			dgx::TriGroup& hg = H.tri.back();
+			const int Np = o.Np, N = o.N;
 			DGX_TRY(hipMalloc(&hg.met, sizeof(double) * met.size()));
 			DGX_TRY(hipMalloc(&hg.detJ, sizeof(double) * dj.size()));
+			DGX_TRY(hipMalloc(&hg.tr, sizeof(double) * (size_t)tg.E * 3 * nv * N));
 			DGX_TRY(hipMemcpy(hg.met, met.data(), sizeof(double) * met.size(), hipMemcpyHostToDevice));
 			DGX_TRY(hipMemcpy(hg.detJ, dj.data(), sizeof(double) * dj.size(), hipMemcpyHostToDevice));
 			if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) {
-				// BR2: each element's M^-1 and the face-trace operators K_f = Ifc_f M^-1 tw0_f of its lifts;
-				// the volume gradient and its face traces are written per evaluation
-				const int Np = o.Np, N = o.N;
-				std::vector<double> mi((size_t)tg.E * Np * Np), K((size_t)tg.E * 3 * N * N);
-				for (int k = 0; k < tg.E; ++k)
-					if (!dgx::tri_mass_data(o, &dj[(size_t)k * o.Nc], &mi[(size_t)k * Np * Np], &K[(size_t)k * 3 * N * N]))
-						return DGX_ERR_ARG;
-				DGX_TRY(hipMalloc(&hg.minv, sizeof(double) * mi.size()));
-				DGX_TRY(hipMalloc(&hg.K, sizeof(double) * K.size()));
+				// BR2: M^-1 and the face-trace operators K_f = Ifc_f M^-1 tw0_f of the lifts.  An affine element
+				// (det J constant at the cubature) has the reference ones divided by det J; if every element of
+				// the group is affine only 1 / det J is stored, else each element's own matrices.  The volume
+				// gradient and its face traces are written per evaluation.
+				bool affine = true;
+				std::vector<double> jinv(tg.E);
+				for (int k = 0; k < tg.E && affine; ++k) {
+					const double* d = &dj[(size_t)k * o.Nc];
+					double lo = d[0], hi = d[0], sum = 0.0;
+					for (int q = 0; q < o.Nc; ++q) {
+						lo = std::min(lo, d[q]);
+						hi = std::max(hi, d[q]);
+						sum += d[q];
+					}
+					affine = (hi - lo) <= 1e-14 * hi;
+					jinv[k] = (double)o.Nc / sum;
+				}
+				hg.affine = affine ? 1 : 0;
+				if (affine) {
+					DGX_TRY(hipMalloc(&hg.jinv, sizeof(double) * jinv.size()));
+					DGX_TRY(hipMemcpy(hg.jinv, jinv.data(), sizeof(double) * jinv.size(), hipMemcpyHostToDevice));
+				} else {
+					std::vector<double> mi((size_t)tg.E * Np * Np), K((size_t)tg.E * 3 * N * N);
+					for (int k = 0; k < tg.E; ++k)
+						if (!dgx::tri_mass_data(o, &dj[(size_t)k * o.Nc], &mi[(size_t)k * Np * Np], &K[(size_t)k * 3 * N * N]))
+							return DGX_ERR_ARG;
+					DGX_TRY(hipMalloc(&hg.minv, sizeof(double) * mi.size()));
+					DGX_TRY(hipMalloc(&hg.K, sizeof(double) * K.size()));
+					DGX_TRY(hipMemcpy(hg.minv, mi.data(), sizeof(double) * mi.size(), hipMemcpyHostToDevice));
+					DGX_TRY(hipMemcpy(hg.K, K.data(), sizeof(double) * K.size(), hipMemcpyHostToDevice));
+				}
 				DGX_TRY(hipMalloc(&hg.gv, sizeof(double) * (size_t)tg.E * 2 * nv * Np));
 				DGX_TRY(hipMalloc(&hg.gvt, sizeof(double) * (size_t)tg.E * 3 * 2 * nv * N));
-				DGX_TRY(hipMemcpy(hg.minv, mi.data(), sizeof(double) * mi.size(), hipMemcpyHostToDevice));
-				DGX_TRY(hipMemcpy(hg.K, K.data(), sizeof(double) * K.size(), hipMemcpyHostToDevice));
 			}
 		}
 	}
 	H.Fmix = (int)mf.size();
+	H.mruns = mruns;
 	if (!mf.empty()) {
 		DGX_TRY(hipMalloc(&H.mfaces, sizeof(dgx::MixFace) * mf.size()));
 		DGX_TRY(hipMalloc(&H.mnJ, sizeof(double) * mnj.size()));

@@ -91,6 +91,10 @@ struct TriOps {
 	T w[kTriMaxNc];                    // cubature weights
 	T Ifc[3][kHpMaxN * kTriMaxNp];     // basis at the element's own GL points of face lf [a][m]
 	T tw0[3][kTriMaxNp * kHpMaxN];     // test functions times GL weights at those points [m][a]
+	// the inverse mass matrix and the lift traces K_f = Ifc_f M^-1 tw0_f of an element with det J = 1 (an affine
+	// element's are these divided by its det J)
+	T Minv[kTriMaxNp * kTriMaxNp];
+	T K[3][kHpMaxN * kHpMaxN];
 	int p, Np, Nc, N;
 };
 bool build_tri_ops(int p, TriOps<double>& o);
@@ -98,9 +102,10 @@ void tri_geometry(const TriOps<double>& o, const double* xyz, double* met, doubl
 void tri_face_nj(int p, int lf, const double* xyz, const double* s, int nq, double* nj);
 bool tri_mass_data(const TriOps<double>& o, const double* detJ, double* minv, double* K);
 
-// A face with a TRI element on at least one side, evaluated by k_mix_faces: each side's trace at its own GL face
-// points (TRI: from all its nodes; QUAD: from its GLL face nodes), interpolated to the face's GL points, the
-// numerical flux there, and each side's lifted flux z = P^T w (J_f F) / w_own at its own points, own ordering.
+// A face with a TRI element on at least one side, evaluated by k_mix_faces: each side's values at its own face
+// points (TRI: its trace at its GL face points, written per element beforehand; QUAD: its GLL face nodes),
+// interpolated to the face's GL points, the numerical flux there, and each side's lifted flux
+// z = P^T w (J_f F) / w_own at its own GL points, own ordering.  The faces are sorted by (Nf, kL, NL, kR, NR).
 struct MixFace {
 	long long uL, uR;    // first state value of each side's element
 	long long zL, zR;    // first lifted-flux value of each side's own face block [v][a] (Navier-Stokes: then y [d][v][a])
@@ -110,7 +115,12 @@ struct MixFace {
 	int rev;             // 1: the right face runs the other way (ind_ord 1)
 	int bc;              // DGX_BC_* (boundary faces: the right side is unused)
 	int eL, eR;          // each side's element index within its (type, p) group
-	int tL, tR;          // Navier-Stokes, TRI sides: the TRI group index
+	int tL, tR;          // TRI sides: the TRI group index
+};
+
+// a run of mixed faces with the same number of face points (one launch each)
+struct MixRun {
+	int Nf, start, count;
 };
 
 // The TRI elements of one order of a mixed mesh (a contiguous range of the state vector).
@@ -120,7 +130,11 @@ struct TriGroup {
 	long long z_off = 0;             // first lifted-flux value (E x 3 x zblk x (p + 1), zblk = nv or 3 nv)
 	double* met = nullptr;           // E x 4 x Nc: J-scaled cofactors (y_s, -x_s, -y_r, x_r) at the cubature
 	double* detJ = nullptr;          // E x Nc
-	// Navier-Stokes (BR2)
+	double* tr = nullptr;            // E x 3 x nv x N: the state's traces at the own GL face points (written each residual)
+	// Navier-Stokes (BR2): every element affine (det J constant): 1 / det J per element with the reference
+	// M^-1 and K_f; otherwise each element's own
+	int affine = 0;
+	double* jinv = nullptr;          // E
 	double* minv = nullptr;          // E x Np x Np: inverse mass matrices
 	double* K = nullptr;             // E x 3 x N x N: lift traces Ifc_f M^-1 tw0_f per local face
 	double* gv = nullptr;            // E x 2 x nv x Np: volume part of the weak gradient (written each residual)
@@ -162,9 +176,10 @@ struct HpData {
 	// TRI groups and the faces that touch a TRI element (mixed meshes)
 	std::vector<TriGroup> tri;
 	TriOps<double>* tri_ops64 = nullptr;   // device: kTriMaxP + 1 entries, indexed by p
-	MixFace* mfaces = nullptr;             // Fmix
+	MixFace* mfaces = nullptr;             // Fmix, sorted by (Nf, kL, NL, kR, NR)
 	double* mnJ = nullptr;                 // Fmix x kHpMaxN x 2: n J_f at the face GL points (left ordering)
 	int Fmix = 0;
+	std::vector<MixRun> mruns;             // runs of mixed faces with one Nf
 	std::vector<int> group_lane;     // side stream of each group (longest-first balance of E_g N^2)
 	std::vector<int> pair_lane;      // side stream of each face pair (longest-first balance of count max(NL,NR)^2)
 };
@@ -268,6 +283,8 @@ hipError_t launch_mix_faces64(const HpData& H, const Ctx& c, const double* u, hi
 // and the faces with a TRI side (the BR2 products y, then the lifted inviscid + viscous fluxes)
 hipError_t launch_tri_ns_phase64(const TriGroup& g, const TriOps<double>* ops, const double* u, double* r,
                                  const double* z, double mu, double Pr, int phase, hipStream_t st);
+// Euler: a TRI group's state traces at its own GL face points (before the mixed faces)
+hipError_t launch_tri_trace64(const TriGroup& g, const TriOps<double>* ops, const double* u, hipStream_t st);
 hipError_t launch_mix_faces_ns64(const HpData& H, const Ctx& c, const double* u, hipStream_t st);
 hipError_t launch_minv(Ctx& c, double* r);
 hipError_t launch_positivity(Ctx& c, double* u, int* status);

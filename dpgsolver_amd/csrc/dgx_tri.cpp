@@ -11,6 +11,7 @@
 // that the double operators are rounded once (Np <= 21).
 #include <cmath>
 #include <cstring>
+#include <algorithm>
 #include <vector>
 
 #include "dgx_internal.h"
@@ -191,6 +192,10 @@ bool build_tri_ops(int p, TriOps<double>& o)
 				o.tw0[lf][m * N + a] = phi[m] * wg[a];
 			}
 		}
+	std::vector<double> one(Nc, 1.0), mi((size_t)Np * Np), K((size_t)3 * N * N);
+	if (!tri_mass_data(o, one.data(), mi.data(), K.data())) return false;
+	std::copy(mi.begin(), mi.end(), o.Minv);
+	for (int lf = 0; lf < 3; ++lf) std::copy(K.begin() + lf * N * N, K.begin() + (lf + 1) * N * N, o.K[lf]);
 	return true;
 }
 

@@ -1,20 +1,40 @@
-// k_tri.hip -- TRI elements of 2-D mixed meshes and the faces that touch them (FP64 Euler).
+// k_tri.hip -- TRI elements of 2-D mixed meshes and the faces that touch them (FP64, Euler and BR2 Navier-Stokes).
 //
-// TRI group residual (k_tri_rhs): the volume term of compute_volume_rlhs_dg with the element's dense operators
-// (compute_volume_rlhs_T.c:145-177: s = cv0_vs_vc u, fr_d = sum_j m_{jd} f_j, rhs += sum_d tw1_vt_vc[d] fr_d), and
-// the face term of every face read as the lifted flux z that k_mix_faces wrote at the element's own GL face
-// points (rhs -= tw0_vt_fc z, compute_face_rlhs_T.c:297-359).  One element per wavefront, four per workgroup,
-// the order's operators staged in LDS once per workgroup over a strided element loop.
+// TRI group kernels.  A TRI's operators are dense (every WSH node contributes to every cubature and face point),
+// so a group kernel is a batch of small dense products: per element s = I u at the WV cubature, the
+// contravariant fluxes, R = sum_d tw1_d fr_d - sum_f tw0_f z_f (compute_volume_rlhs_T.c:145-177,
+// compute_face_rlhs_T.c:297-359).  One workgroup of 256 lanes takes a batch of EPB elements (as many as give
+// every lane a cubature point, within an LDS budget): the batch's state and face blocks are read contiguously
+// into LDS, each stage maps the lanes over the (element, point) pairs of the whole batch, and the order's
+// operators are staged in LDS once per workgroup over a strided batch loop.
+//   k_tri_trace: the state's traces at each face's own GL points (Euler, before the face kernels);
+//   k_tri_rhs:   the Euler residual from the lifted face fluxes z;
+//   k_tri_gv:    (BR2) the volume part of the weak gradient G_v = M^-1 I^T W (J grad s)
+//                (compute_grad_coef_dg_T.c:171-307), its face traces, and the state traces;
+//   k_tri_ns:    G = G_v + sum_f M^-1 tw0_f y_f, the volume flux F_euler + fv (flux_navier_stokes_T.c:1005-1040)
+//                and the residual.
+// M^-1 and the lift traces K_f = Ifc_f M^-1 tw0_f: the reference ones divided by det J on an affine group, each
+// element's own otherwise.
 //
-// Faces with a TRI side (k_mix_faces): one face per thread.  Each side's trace at its own GL points (TRI: the
-// dense trace of all its nodes; QUAD: its GLL face nodes), interpolated to the face's GL points (order
-// max(p_L, p_R), compute_face_reference_order face_solver.c:123-131), the right side in the left ordering; the
-// numerical flux with n J_f of the lower-order side (get_dominant_geom_vol_side_index, face_solver.c:95-107);
-// then each side's z = P^T (w J_f F) / w_own at its own GL points (the right side negated, own ordering), which
-// the element's own test functions lift exactly (they are polynomials of its order along the face).
+// Faces with a TRI side (k_mixf, k_mixf_ns): eight lanes per face, eight faces per 64-lane workgroup, one
+// launch per face-point count Nf = max(p_L, p_R) + 1 (compute_face_reference_order, face_solver.c:123-131; the
+// faces arrive sorted by (Nf, types, orders), so a workgroup rarely mixes shapes).  Each side's own values
+// (TRI: its traces; QUAD: its GLL face nodes) go through LDS and are interpolated to the face's GL points (the
+// right side in the left ordering); the numerical flux uses n J_f of the lower-order side
+// (get_dominant_geom_vol_side_index, face_solver.c:95-107); each side's z = P^T (w J_f F) / w_own at its own
+// GL points (the right side negated, own ordering) is lifted exactly by its own test functions.  BR2: both
+// sides use q = n J_f (s_R - s_L) / 2, each side's projection y = P^T w q / w_own (written after z for the
+// residual kernels), its face gradient h = trace G_v + eta (lift trace) y with eta = 1.01 NFMAX = 4.04
+// (compute_face_rlhs_dg_T.c:243-300; TRI: K_f y, QUAD: kappa y), and the central n . (fv_L + fv_R) / 2
+// (numerical_flux_central_T.c:76-104) added to the inviscid flux.
 #include "k_residual.h"
 
 namespace dgx {
+
+namespace {
+
+constexpr int kNV = 4;
+constexpr int kTriLds = 6144;   // LDS budget of one TRI workgroup, in doubles (48 KB)
 
 __device__ inline int quad_face_node(int N, int lf, int i)
 {
@@ -22,601 +42,765 @@ __device__ inline int quad_face_node(int N, int lf, int i)
 	return fd == 0 ? fix + N * i : i + N * fix;
 }
 
-template <int P>
-constexpr int tri_nc()
+constexpr int tri_np(int P) { return (P + 1) * (P + 2) / 2; }
+constexpr int tri_nc(int P) { return P == 1 ? 3 : P == 2 ? 6 : P == 3 ? 12 : P == 4 ? 16 : 25; }   // WV of degree 2p
+
+// elements per workgroup: enough to give every lane a point, within the LDS budget
+constexpr int tri_epb(int lanes_per_elem, int ops, int per_elem)
 {
-	return P == 1 ? 3 : P == 2 ? 6 : P == 3 ? 12 : P == 4 ? 16 : 25;   // WV of degree 2p
+	const int fill = 256 / lanes_per_elem > 0 ? 256 / lanes_per_elem : 1;
+	const int fit = (kTriLds - ops) / per_elem;
+	return fill < fit ? fill : (fit > 0 ? fit : 1);
 }
 
-struct TriRhsArgs {
-	const double* u;     // the group's state, E x nv x Np
-	double* r;           // the group's residual
-	const double* met;   // E x 4 x Nc
-	const double* z;     // the group's lifted face fluxes, E x 3 x nv x N
+// each kernel's batch size (LDS: operators, then per element)
+constexpr int epb_trace(int P) { return tri_epb(3 * (P + 1), 3 * (P + 1) * tri_np(P), kNV * tri_np(P)); }
+constexpr int epb_rhs(int P)
+{
+	return tri_epb(tri_nc(P), 3 * tri_nc(P) * tri_np(P) + 3 * tri_np(P) * (P + 1),
+	               kNV * tri_np(P) + 2 * kNV * tri_nc(P) + 3 * kNV * (P + 1));
+}
+constexpr int epb_gv(int P)
+{
+	return tri_epb(tri_nc(P), 3 * tri_nc(P) * tri_np(P) + 3 * (P + 1) * tri_np(P) + tri_nc(P) + tri_np(P) * tri_np(P),
+	               kNV * tri_np(P) + 2 * kNV * tri_nc(P) + 2 * kNV * tri_np(P));
+}
+constexpr int epb_ns(int P)
+{
+	return tri_epb(tri_nc(P), 3 * tri_nc(P) * tri_np(P) + 3 * tri_np(P) * (P + 1) + tri_np(P) * tri_np(P),
+	               kNV * tri_np(P) + 9 * kNV * (P + 1) + 2 * kNV * tri_np(P) + 2 * kNV * tri_nc(P));
+}
+
+struct TriArgs {
+	const double* u;       // the group's state, E x nv x Np
+	double* r;             // the group's residual
+	const double* met;     // E x 4 x Nc
+	const double* z;       // the group's face blocks, E x 3 x zblk x N
+	double* tr;            // E x 3 x nv x N
+	double* gv;            // E x 2 x nv x Np
+	double* gvt;           // E x 3 x 2 x nv x N
+	const double* minv;    // curved groups: E x Np x Np
+	const double* jinv;    // affine groups: E
 	const TriOps<double>* ops;
-	int E;
+	double mu, Pr;
+	int E, affine;
 };
 
 template <int P>
-__global__ void __launch_bounds__(256) k_tri_rhs(TriRhsArgs a)
+__global__ void __launch_bounds__(256) k_tri_trace(TriArgs a)
 {
-	constexpr int Np = (P + 1) * (P + 2) / 2, Nc = tri_nc<P>(), N = P + 1, NV = 4, W = 4;
+	constexpr int Np = tri_np(P), N = P + 1, F3 = 3 * N, EPB = epb_trace(P);
+	__shared__ double sIfc[F3 * Np];
+	__shared__ double sU[EPB * kNV * Np];
+	const int t = threadIdx.x;
+	for (int i = t; i < F3 * Np; i += 256) sIfc[i] = a.ops->Ifc[i / (N * Np)][i % (N * Np)];
+	for (int e0 = blockIdx.x * EPB; e0 < a.E; e0 += gridDim.x * EPB) {
+		const int ne = min(EPB, a.E - e0);
+		__syncthreads();
+		for (int i = t; i < ne * kNV * Np; i += 256) sU[i] = a.u[(size_t)e0 * kNV * Np + i];
+		__syncthreads();
+		for (int i = t; i < ne * F3; i += 256) {
+			const int sl = i / F3, k = i - sl * F3, lf = k / N, b = k - lf * N;
+			const double* U = sU + sl * kNV * Np;
+			double acc[kNV] = {};
+#pragma unroll
+			for (int m = 0; m < Np; ++m) {
+				const double c = sIfc[k * Np + m];
+#pragma unroll
+				for (int v = 0; v < kNV; ++v) acc[v] += c * U[v * Np + m];
+			}
+			double* out = a.tr + ((size_t)(e0 + sl) * 3 + lf) * kNV * N + b;
+#pragma unroll
+			for (int v = 0; v < kNV; ++v) out[v * N] = acc[v];
+		}
+	}
+}
+
+template <int P>
+__global__ void __launch_bounds__(256) k_tri_rhs(TriArgs a)
+{
+	constexpr int Np = tri_np(P), Nc = tri_nc(P), N = P + 1, ZB = kNV * N, EPB = epb_rhs(P);
 	__shared__ double sI[Nc * Np], sDrW[Np * Nc], sDsW[Np * Nc], sTw0[3 * Np * N];
-	__shared__ double sU[W][NV * Np], sFR[W][2 * NV * Nc], sZ[W][3 * NV * N];
-	for (int i = threadIdx.x; i < Nc * Np; i += blockDim.x) {
+	__shared__ double sU[EPB * kNV * Np], sFR[EPB * 2 * kNV * Nc], sZ[EPB * 3 * ZB];
+	const int t = threadIdx.x;
+	for (int i = t; i < Nc * Np; i += 256) {
 		sI[i] = a.ops->I[i];
 		sDrW[i] = a.ops->DrW[i];
 		sDsW[i] = a.ops->DsW[i];
 	}
-	for (int i = threadIdx.x; i < 3 * Np * N; i += blockDim.x) sTw0[i] = a.ops->tw0[i / (Np * N)][i % (Np * N)];
-	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-	for (int base = blockIdx.x * W; base < a.E; base += gridDim.x * W) {   // the same trip count for every wave
-		const int e = base + wave;
-		const bool valid = e < a.E;
-		__syncthreads();   // the operators are staged / the previous element's buffers are read
-		if (valid) {
-			for (int i = lane; i < NV * Np; i += 64) sU[wave][i] = a.u[(size_t)e * NV * Np + i];
-			for (int i = lane; i < 3 * NV * N; i += 64) sZ[wave][i] = a.z[(size_t)e * 3 * NV * N + i];
-		}
+	for (int i = t; i < 3 * Np * N; i += 256) sTw0[i] = a.ops->tw0[i / (Np * N)][i % (Np * N)];
+	for (int e0 = blockIdx.x * EPB; e0 < a.E; e0 += gridDim.x * EPB) {
+		const int ne = min(EPB, a.E - e0);
+		__syncthreads();   // the operators are staged / the previous batch's buffers are read
+		for (int i = t; i < ne * kNV * Np; i += 256) sU[i] = a.u[(size_t)e0 * kNV * Np + i];
+		for (int i = t; i < ne * 3 * ZB; i += 256) sZ[i] = a.z[(size_t)e0 * 3 * ZB + i];
 		__syncthreads();
-		if (valid && lane < Nc) {
-			const int q = lane;
-			double s[NV];
+		for (int i = t; i < ne * Nc; i += 256) {
+			const int sl = i / Nc, q = i - sl * Nc;
+			const double* U = sU + sl * kNV * Np;
+			double s[kNV] = {};
 #pragma unroll
-			for (int v = 0; v < NV; ++v) {
-				double acc = 0.0;
+			for (int m = 0; m < Np; ++m) {
+				const double c = sI[q * Np + m];
 #pragma unroll
-				for (int m = 0; m < Np; ++m) acc += sI[q * Np + m] * sU[wave][v * Np + m];
-				s[v] = acc;
+				for (int v = 0; v < kNV; ++v) s[v] += c * U[v * Np + m];
 			}
-			const double* mp = a.met + (size_t)e * 4 * Nc + q;
+			const double* mp = a.met + (size_t)(e0 + sl) * 4 * Nc + q;
 			const double m0 = mp[0], m1 = mp[Nc], m2 = mp[2 * Nc], m3 = mp[3 * Nc];
-			double f[2 * NV];
+			double f[2 * kNV];
 			euler_flux<double, 2>(s, f);
+			double* FR = sFR + sl * 2 * kNV * Nc;
 #pragma unroll
-			for (int v = 0; v < NV; ++v) {
-				sFR[wave][v * Nc + q] = m0 * f[2 * v] + m1 * f[2 * v + 1];
-				sFR[wave][(NV + v) * Nc + q] = m2 * f[2 * v] + m3 * f[2 * v + 1];
+			for (int v = 0; v < kNV; ++v) {
+				FR[v * Nc + q] = m0 * f[2 * v] + m1 * f[2 * v + 1];
+				FR[(kNV + v) * Nc + q] = m2 * f[2 * v] + m3 * f[2 * v + 1];
 			}
 		}
 		__syncthreads();
-		if (valid && lane < Np) {
-			const int m = lane;
+		for (int i = t; i < ne * Np; i += 256) {
+			const int sl = i / Np, m = i - sl * Np;
+			const double* FR = sFR + sl * 2 * kNV * Nc;
+			const double* Z = sZ + sl * 3 * ZB;
+			double acc[kNV] = {};
 #pragma unroll
-			for (int v = 0; v < NV; ++v) {
-				double acc = 0.0;
+			for (int q = 0; q < Nc; ++q) {
+				const double cr = sDrW[m * Nc + q], cs = sDsW[m * Nc + q];
 #pragma unroll
-				for (int q = 0; q < Nc; ++q)
-					acc += sDrW[m * Nc + q] * sFR[wave][v * Nc + q] + sDsW[m * Nc + q] * sFR[wave][(NV + v) * Nc + q];
-#pragma unroll
-				for (int lf = 0; lf < 3; ++lf)
-#pragma unroll
-					for (int k = 0; k < N; ++k) acc -= sTw0[(lf * Np + m) * N + k] * sZ[wave][(lf * NV + v) * N + k];
-				a.r[(size_t)e * NV * Np + v * Np + m] = acc;
+				for (int v = 0; v < kNV; ++v) acc[v] += cr * FR[v * Nc + q] + cs * FR[(kNV + v) * Nc + q];
 			}
+#pragma unroll
+			for (int lf = 0; lf < 3; ++lf)
+#pragma unroll
+				for (int b = 0; b < N; ++b) {
+					const double c = sTw0[(lf * Np + m) * N + b];
+#pragma unroll
+					for (int v = 0; v < kNV; ++v) acc[v] -= c * Z[(lf * kNV + v) * N + b];
+				}
+			double* R = a.r + (size_t)(e0 + sl) * kNV * Np + m;
+#pragma unroll
+			for (int v = 0; v < kNV; ++v) R[v * Np] = acc[v];
 		}
 	}
+}
+
+// M^-1 row m of element e (affine groups: the reference row in LDS and 1 / det J)
+template <int Np>
+__device__ inline void tri_minv_row(const TriArgs& a, const double* sMinv, int e, int m, const double*& row,
+                                    double& scale)
+{
+	if (a.affine) {
+		row = sMinv + m * Np;
+		scale = a.jinv[e];
+	} else {
+		row = a.minv + (size_t)e * Np * Np + m * Np;
+		scale = 1.0;
+	}
+}
+
+template <int P>
+__global__ void __launch_bounds__(256) k_tri_gv(TriArgs a)
+{
+	constexpr int Np = tri_np(P), Nc = tri_nc(P), N = P + 1, F3 = 3 * N, K8 = 2 * kNV, EPB = epb_gv(P);
+	__shared__ double sI[Nc * Np], sDr[Nc * Np], sDs[Nc * Np], sIfc[F3 * Np], sW[Nc], sMinv[Np * Np];
+	__shared__ double sU[EPB * kNV * Np], sX[EPB * K8 * Nc], sB[EPB * K8 * Np];
+	const int t = threadIdx.x;
+	for (int i = t; i < Nc * Np; i += 256) {
+		sI[i] = a.ops->I[i];
+		sDr[i] = a.ops->Dr[i];
+		sDs[i] = a.ops->Ds[i];
+	}
+	for (int i = t; i < F3 * Np; i += 256) sIfc[i] = a.ops->Ifc[i / (N * Np)][i % (N * Np)];
+	for (int i = t; i < Nc; i += 256) sW[i] = a.ops->w[i];
+	for (int i = t; i < Np * Np; i += 256) sMinv[i] = a.ops->Minv[i];
+	for (int e0 = blockIdx.x * EPB; e0 < a.E; e0 += gridDim.x * EPB) {
+		const int ne = min(EPB, a.E - e0);
+		__syncthreads();
+		for (int i = t; i < ne * kNV * Np; i += 256) sU[i] = a.u[(size_t)e0 * kNV * Np + i];
+		__syncthreads();
+		for (int i = t; i < ne * Nc; i += 256) {   // W J grad s at the cubature
+			const int sl = i / Nc, q = i - sl * Nc;
+			const double* U = sU + sl * kNV * Np;
+			double sr[kNV] = {}, ss[kNV] = {};
+#pragma unroll
+			for (int m = 0; m < Np; ++m) {
+				const double cr = sDr[q * Np + m], cs = sDs[q * Np + m];
+#pragma unroll
+				for (int v = 0; v < kNV; ++v) {
+					sr[v] += cr * U[v * Np + m];
+					ss[v] += cs * U[v * Np + m];
+				}
+			}
+			const double* mp = a.met + (size_t)(e0 + sl) * 4 * Nc + q;
+			const double m0 = mp[0], m1 = mp[Nc], m2 = mp[2 * Nc], m3 = mp[3 * Nc], w = sW[q];
+			double* X = sX + sl * K8 * Nc;
+#pragma unroll
+			for (int v = 0; v < kNV; ++v) {
+				X[v * Nc + q] = w * (m0 * sr[v] + m2 * ss[v]);
+				X[(kNV + v) * Nc + q] = w * (m1 * sr[v] + m3 * ss[v]);
+			}
+		}
+		__syncthreads();
+		for (int i = t; i < ne * Np; i += 256) {   // I^T (W J grad s)
+			const int sl = i / Np, m = i - sl * Np;
+			const double* X = sX + sl * K8 * Nc;
+			double acc[K8] = {};
+#pragma unroll
+			for (int q = 0; q < Nc; ++q) {
+				const double c = sI[q * Np + m];
+#pragma unroll
+				for (int k = 0; k < K8; ++k) acc[k] += c * X[k * Nc + q];
+			}
+			double* B = sB + sl * K8 * Np;
+#pragma unroll
+			for (int k = 0; k < K8; ++k) B[k * Np + m] = acc[k];
+		}
+		__syncthreads();
+		for (int i = t; i < ne * Np; i += 256) {   // G_v = M^-1 (...), kept in sX for the traces
+			const int sl = i / Np, m = i - sl * Np, e = e0 + sl;
+			const double* row;
+			double sc;
+			tri_minv_row<Np>(a, sMinv, e, m, row, sc);
+			const double* B = sB + sl * K8 * Np;
+			double g[K8] = {};
+#pragma unroll
+			for (int n = 0; n < Np; ++n) {
+				const double c = row[n];
+#pragma unroll
+				for (int k = 0; k < K8; ++k) g[k] += c * B[k * Np + n];
+			}
+			double* G = sX + sl * K8 * Nc;
+#pragma unroll
+			for (int k = 0; k < K8; ++k) {
+				g[k] *= sc;
+				G[k * Np + m] = g[k];
+				a.gv[((size_t)e * K8 + k) * Np + m] = g[k];
+			}
+		}
+		__syncthreads();
+		for (int i = t; i < ne * F3; i += 256) {   // traces of G_v and of the state at the own GL face points
+			const int sl = i / F3, k0 = i - sl * F3, lf = k0 / N, b = k0 - lf * N;
+			const double* G = sX + sl * K8 * Nc;
+			const double* U = sU + sl * kNV * Np;
+			double acc[K8] = {}, st[kNV] = {};
+#pragma unroll
+			for (int m = 0; m < Np; ++m) {
+				const double c = sIfc[k0 * Np + m];
+#pragma unroll
+				for (int k = 0; k < K8; ++k) acc[k] += c * G[k * Np + m];
+#pragma unroll
+				for (int v = 0; v < kNV; ++v) st[v] += c * U[v * Np + m];
+			}
+			const size_t blk = (size_t)(e0 + sl) * 3 + lf;
+#pragma unroll
+			for (int k = 0; k < K8; ++k) a.gvt[(blk * K8 + k) * N + b] = acc[k];
+#pragma unroll
+			for (int v = 0; v < kNV; ++v) a.tr[(blk * kNV + v) * N + b] = st[v];
+		}
+	}
+}
+
+template <int P>
+__global__ void __launch_bounds__(256) k_tri_ns(TriArgs a)
+{
+	constexpr int Np = tri_np(P), Nc = tri_nc(P), N = P + 1, K8 = 2 * kNV, ZB = 3 * kNV * N, EPB = epb_ns(P);
+	__shared__ double sI[Nc * Np], sDrW[Np * Nc], sDsW[Np * Nc], sTw0[3 * Np * N], sMinv[Np * Np];
+	__shared__ double sU[EPB * kNV * Np], sZ[EPB * 3 * ZB], sG[EPB * K8 * Np], sFR[EPB * K8 * Nc];
+	const int t = threadIdx.x;
+	for (int i = t; i < Nc * Np; i += 256) {
+		sI[i] = a.ops->I[i];
+		sDrW[i] = a.ops->DrW[i];
+		sDsW[i] = a.ops->DsW[i];
+	}
+	for (int i = t; i < 3 * Np * N; i += 256) sTw0[i] = a.ops->tw0[i / (Np * N)][i % (Np * N)];
+	for (int i = t; i < Np * Np; i += 256) sMinv[i] = a.ops->Minv[i];
+	const double mu = a.mu, Pr = a.Pr;
+	for (int e0 = blockIdx.x * EPB; e0 < a.E; e0 += gridDim.x * EPB) {
+		const int ne = min(EPB, a.E - e0);
+		__syncthreads();
+		for (int i = t; i < ne * kNV * Np; i += 256) sU[i] = a.u[(size_t)e0 * kNV * Np + i];
+		for (int i = t; i < ne * 3 * ZB; i += 256) sZ[i] = a.z[(size_t)e0 * 3 * ZB + i];
+		__syncthreads();
+		// the face lifts of the gradient: sum_f tw0_f y_f at the nodes (into sFR, free until the fluxes)
+		for (int i = t; i < ne * Np; i += 256) {
+			const int sl = i / Np, m = i - sl * Np;
+			const double* Z = sZ + sl * 3 * ZB;
+			double acc[K8] = {};
+#pragma unroll
+			for (int lf = 0; lf < 3; ++lf)
+#pragma unroll
+				for (int b = 0; b < N; ++b) {
+					const double c = sTw0[(lf * Np + m) * N + b];
+#pragma unroll
+					for (int k = 0; k < K8; ++k) acc[k] += c * Z[lf * ZB + kNV * N + k * N + b];
+				}
+			double* T = sFR + sl * K8 * Nc;
+#pragma unroll
+			for (int k = 0; k < K8; ++k) T[k * Np + m] = acc[k];
+		}
+		__syncthreads();
+		for (int i = t; i < ne * Np; i += 256) {   // G = G_v + M^-1 t
+			const int sl = i / Np, m = i - sl * Np, e = e0 + sl;
+			const double* row;
+			double sc;
+			tri_minv_row<Np>(a, sMinv, e, m, row, sc);
+			const double* T = sFR + sl * K8 * Nc;
+			double acc[K8] = {};
+#pragma unroll
+			for (int n = 0; n < Np; ++n) {
+				const double c = row[n];
+#pragma unroll
+				for (int k = 0; k < K8; ++k) acc[k] += c * T[k * Np + n];
+			}
+			double* G = sG + sl * K8 * Np;
+#pragma unroll
+			for (int k = 0; k < K8; ++k) G[k * Np + m] = a.gv[((size_t)e * K8 + k) * Np + m] + sc * acc[k];
+		}
+		__syncthreads();
+		for (int i = t; i < ne * Nc; i += 256) {
+			const int sl = i / Nc, q = i - sl * Nc;
+			const double* U = sU + sl * kNV * Np;
+			const double* G = sG + sl * K8 * Np;
+			double s[kNV] = {}, gq[K8] = {};
+#pragma unroll
+			for (int m = 0; m < Np; ++m) {
+				const double c = sI[q * Np + m];
+#pragma unroll
+				for (int v = 0; v < kNV; ++v) s[v] += c * U[v * Np + m];
+#pragma unroll
+				for (int k = 0; k < K8; ++k) gq[k] += c * G[k * Np + m];
+			}
+			const double* mp = a.met + (size_t)(e0 + sl) * 4 * Nc + q;
+			const double m0 = mp[0], m1 = mp[Nc], m2 = mp[2 * Nc], m3 = mp[3 * Nc];
+			double f[K8], fv[K8];
+			euler_flux<double, 2>(s, f);
+			ns_viscous_flux2<double>(s, gq, mu, Pr, fv);
+			double* FR = sFR + sl * K8 * Nc;
+#pragma unroll
+			for (int v = 0; v < kNV; ++v) {
+				const double f0 = f[2 * v] + fv[2 * v], f1 = f[2 * v + 1] + fv[2 * v + 1];
+				FR[v * Nc + q] = m0 * f0 + m1 * f1;
+				FR[(kNV + v) * Nc + q] = m2 * f0 + m3 * f1;
+			}
+		}
+		__syncthreads();
+		for (int i = t; i < ne * Np; i += 256) {
+			const int sl = i / Np, m = i - sl * Np;
+			const double* FR = sFR + sl * K8 * Nc;
+			const double* Z = sZ + sl * 3 * ZB;
+			double acc[kNV] = {};
+#pragma unroll
+			for (int q = 0; q < Nc; ++q) {
+				const double cr = sDrW[m * Nc + q], cs = sDsW[m * Nc + q];
+#pragma unroll
+				for (int v = 0; v < kNV; ++v) acc[v] += cr * FR[v * Nc + q] + cs * FR[(kNV + v) * Nc + q];
+			}
+#pragma unroll
+			for (int lf = 0; lf < 3; ++lf)
+#pragma unroll
+				for (int b = 0; b < N; ++b) {
+					const double c = sTw0[(lf * Np + m) * N + b];
+#pragma unroll
+					for (int v = 0; v < kNV; ++v) acc[v] -= c * Z[lf * ZB + v * N + b];
+				}
+			double* R = a.r + (size_t)(e0 + sl) * kNV * Np + m;
+#pragma unroll
+			for (int v = 0; v < kNV; ++v) R[v * Np] = acc[v];
+		}
+	}
+}
+
+// ------------------------------------------------------------------------------------------------ faces
+struct MixArgs {
+	const MixFace* faces;              // the whole sorted list
+	const double* nJ;                  // Fmix x kHpMaxN x 2
+	const double* u;                   // the whole state
+	double* z;                         // the whole lifted-flux buffer
+	const HpOps<double>* hops;
+	const TriOps<double>* tops;        // indexed by p
+	const double* ttr[kTriMaxP];       // TRI groups: state traces
+	const double* tgvt[kTriMaxP];      // TRI groups: traces of G_v
+	const double* tK[kTriMaxP];        // curved TRI groups: each element's K_f (affine groups: null)
+	const double* tjinv[kTriMaxP];     // affine TRI groups: 1 / det J
+	const double* qgvt[kHpMaxN + 1];   // QUAD groups by N: traces of G_v and kappa
+	const double* qkap[kHpMaxN + 1];
+	double bc_prim[5];
+	double mu, Pr;
+	int start, count;
+};
+
+// value j of variable v of one side at its own face points (TRI: trace at its GL points; QUAD: GLL face node)
+__device__ inline double side_value(const MixArgs& a, int kind, int N, int lf, long long uoff, int el, int tg, int v,
+                                    int j)
+{
+	if (kind) return a.ttr[tg][(((size_t)el * 3 + lf) * kNV + v) * N + j];
+	return a.u[uoff + (long long)v * N * N + quad_face_node(N, lf, j)];
+}
+
+template <int NF>
+struct MixOps {
+	double Ig[NF + 1][NF * NF];   // own GLL points -> the NF face points, by own N ([q * N + i])
+	double Pg[NF + 1][NF * NF];   // own GL points -> the NF face points
+	double winv[NF + 1][NF];
+	double w[NF];
+};
+
+template <int NF>
+__device__ inline void stage_mix_ops(MixOps<NF>& o, const HpOps<double>* h)
+{
+	for (int i = threadIdx.x; i < (NF + 1) * NF * NF; i += blockDim.x) {
+		const int n = i / (NF * NF), k = i - n * NF * NF;
+		o.Ig[n][k] = h->Ig[n][NF][k];
+		o.Pg[n][k] = h->Pg[n][NF][k];
+	}
+	for (int i = threadIdx.x; i < (NF + 1) * NF; i += blockDim.x) o.winv[i / NF][i % NF] = h->winv[i / NF][i % NF];
+	for (int i = threadIdx.x; i < NF; i += blockDim.x) o.w[i] = h->w[NF][i];
+}
+
+// one side's state at face point q from its own values S[v * NF + i] (TRI: from GL points; QUAD: from GLL)
+template <int NF>
+__device__ inline void side_at_point(const MixOps<NF>& o, int kind, int Ns, int q, const double* S, double* out)
+{
+	const double* op = kind ? o.Pg[Ns] : o.Ig[Ns];
+#pragma unroll
+	for (int v = 0; v < kNV; ++v) {
+		double acc = 0.0;
+#pragma unroll
+		for (int i = 0; i < NF; ++i)
+			if (i < Ns) acc += op[q * Ns + i] * S[v * NF + i];
+		out[v] = acc;
+	}
+}
+
+// z = P^T g / w_own at own point j of one side (g: [q][v] at the NF face points, in the left ordering)
+template <int NF>
+__device__ inline void lift_side(const MixOps<NF>& o, int Ns, int j, bool rev, bool neg, const double* g, double* z)
+{
+	const double* Pg = o.Pg[Ns];
+#pragma unroll
+	for (int v = 0; v < kNV; ++v) {
+		double acc = 0.0;
+#pragma unroll
+		for (int q = 0; q < NF; ++q) acc += Pg[(rev ? NF - 1 - q : q) * Ns + j] * g[q * kNV + v];
+		z[v * Ns + j] = neg ? 0.0 - acc * o.winv[Ns][j] : acc * o.winv[Ns][j];
+	}
+}
+
+template <int NF, int FLUX>
+__global__ void __launch_bounds__(64) k_mixf(MixArgs a)
+{
+	constexpr int FPB = 8;
+	__shared__ MixOps<NF> o;
+	__shared__ double sOwn[FPB][2][kNV * NF];
+	__shared__ double sG[FPB][NF * kNV];
+	stage_mix_ops<NF>(o, a.hops);
+	const int slot = threadIdx.x >> 3, j = threadIdx.x & 7;
+	const int fl = blockIdx.x * FPB + slot;
+	const bool valid = fl < a.count;
+	const int fi = a.start + (valid ? fl : 0);
+	const MixFace f = a.faces[fi];
+	const bool inter = f.bc == DGX_BC_INTERIOR;
+	if (valid && j < f.NL)
+#pragma unroll
+		for (int v = 0; v < kNV; ++v) sOwn[slot][0][v * NF + j] = side_value(a, f.kL, f.NL, f.lfL, f.uL, f.eL, f.tL, v, j);
+	if (valid && inter && j < f.NR)
+#pragma unroll
+		for (int v = 0; v < kNV; ++v) sOwn[slot][1][v * NF + j] = side_value(a, f.kR, f.NR, f.lfR, f.uR, f.eR, f.tR, v, j);
+	__syncthreads();
+	if (valid && j < NF) {
+		const int q = j;
+		const double* nj = a.nJ + ((size_t)fi * kHpMaxN + q) * 2;
+		const double Jf = sqrt(nj[0] * nj[0] + nj[1] * nj[1]);
+		const double nu[2] = { nj[0] / Jf, nj[1] / Jf };
+		double sL[kNV], sR[kNV], nnf[kNV];
+		side_at_point<NF>(o, f.kL, f.NL, q, sOwn[slot][0], sL);
+		if (inter) side_at_point<NF>(o, f.kR, f.NR, f.rev ? NF - 1 - q : q, sOwn[slot][1], sR);
+		else bc_state<double, 2>(f.bc, sL, nu, a.bc_prim, sR);
+		if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<double, 2>(sL, sR, nu, nnf);
+		else numflux_roe<double, 2>(sL, sR, nu, nnf);
+#pragma unroll
+		for (int v = 0; v < kNV; ++v) sG[slot][q * kNV + v] = o.w[q] * Jf * nnf[v];
+	}
+	__syncthreads();
+	if (valid && j < f.NL) lift_side<NF>(o, f.NL, j, false, false, sG[slot], a.z + f.zL);
+	if (valid && inter && j < f.NR) lift_side<NF>(o, f.NR, j, f.rev != 0, true, sG[slot], a.z + f.zR);
+}
+
+// a side's face gradient at its own GL point j: trace G_v + eta (lift trace) y, with y = Y[k * NF + b]
+template <int NF>
+__device__ inline void side_face_gradient(const MixArgs& a, int kind, int Ns, int lf, int el, int tg, int j,
+                                          const double* Y, double* h)
+{
+	constexpr double eta = 1.01 * 4.0;
+	if (kind) {
+		const double* t = a.tgvt[tg] + ((size_t)el * 3 + lf) * 2 * kNV * Ns;
+		const double* K;
+		double sc;
+		if (a.tK[tg]) {
+			K = a.tK[tg] + ((size_t)el * 3 + lf) * Ns * Ns + j * Ns;
+			sc = 1.0;
+		} else {
+			K = a.tops[Ns - 1].K[lf] + j * Ns;
+			sc = a.tjinv[tg][el];
+		}
+#pragma unroll
+		for (int k = 0; k < 2 * kNV; ++k) {
+			double acc = 0.0;
+#pragma unroll
+			for (int b = 0; b < NF; ++b)
+				if (b < Ns) acc += K[b] * Y[k * NF + b];
+			h[k] = t[k * Ns + j] + eta * (sc * acc);
+		}
+		return;
+	}
+	const double* t = a.qgvt[Ns] + ((size_t)el * 4 + lf) * 2 * kNV * Ns;
+	const double kap = a.qkap[Ns][((size_t)el * 4 + lf) * Ns + j];
+#pragma unroll
+	for (int k = 0; k < 2 * kNV; ++k) h[k] = t[k * Ns + j] + eta * kap * Y[k * NF + j];
+}
+
+template <int NF, int FLUX>
+__global__ void __launch_bounds__(64) k_mixf_ns(MixArgs a)
+{
+	constexpr int FPB = 8, K8 = 2 * kNV;
+	__shared__ MixOps<NF> o;
+	__shared__ double sOwn[FPB][2][kNV * NF];
+	__shared__ double sQ[FPB][NF * K8];        // w q at the face points [q][k], then w J_f F [q][v]
+	__shared__ double sY[FPB][2][K8 * NF];     // each side's y [k][j], then its face gradient h
+	stage_mix_ops<NF>(o, a.hops);
+	const int slot = threadIdx.x >> 3, j = threadIdx.x & 7;
+	const int fl = blockIdx.x * FPB + slot;
+	const bool valid = fl < a.count;
+	const int fi = a.start + (valid ? fl : 0);
+	const MixFace f = a.faces[fi];
+	if (valid && j < f.NL)
+#pragma unroll
+		for (int v = 0; v < kNV; ++v) sOwn[slot][0][v * NF + j] = side_value(a, f.kL, f.NL, f.lfL, f.uL, f.eL, f.tL, v, j);
+	if (valid && j < f.NR)
+#pragma unroll
+		for (int v = 0; v < kNV; ++v) sOwn[slot][1][v * NF + j] = side_value(a, f.kR, f.NR, f.lfR, f.uR, f.eR, f.tR, v, j);
+	__syncthreads();
+	double sL[kNV], sR[kNV];
+	const int q = j, qr = f.rev ? NF - 1 - q : q;
+	if (valid && j < NF) {
+		side_at_point<NF>(o, f.kL, f.NL, q, sOwn[slot][0], sL);
+		side_at_point<NF>(o, f.kR, f.NR, qr, sOwn[slot][1], sR);
+		const double* nj = a.nJ + ((size_t)fi * kHpMaxN + q) * 2;
+#pragma unroll
+		for (int v = 0; v < kNV; ++v) {
+			const double dl = 0.5 * (sR[v] - sL[v]);
+			sQ[slot][q * K8 + v] = o.w[q] * (nj[0] * dl);
+			sQ[slot][q * K8 + kNV + v] = o.w[q] * (nj[1] * dl);
+		}
+	}
+	__syncthreads();
+	// projections y to each side's own GL points (written for the residual kernels, after the lifted flux)
+	if (valid && j < f.NL) {
+		const double* Pg = o.Pg[f.NL];
+#pragma unroll
+		for (int k = 0; k < K8; ++k) {
+			double acc = 0.0;
+#pragma unroll
+			for (int qq = 0; qq < NF; ++qq) acc += Pg[qq * f.NL + j] * sQ[slot][qq * K8 + k];
+			acc *= o.winv[f.NL][j];
+			sY[slot][0][k * NF + j] = acc;
+			a.z[f.zL + kNV * f.NL + k * f.NL + j] = acc;
+		}
+	}
+	if (valid && j < f.NR) {
+		const double* Pg = o.Pg[f.NR];
+#pragma unroll
+		for (int k = 0; k < K8; ++k) {
+			double acc = 0.0;
+#pragma unroll
+			for (int qq = 0; qq < NF; ++qq) acc += Pg[(f.rev ? NF - 1 - qq : qq) * f.NR + j] * sQ[slot][qq * K8 + k];
+			acc *= o.winv[f.NR][j];
+			sY[slot][1][k * NF + j] = acc;
+			a.z[f.zR + kNV * f.NR + k * f.NR + j] = acc;
+		}
+	}
+	__syncthreads();
+	double hL[K8], hR[K8];
+	if (valid && j < f.NL) side_face_gradient<NF>(a, f.kL, f.NL, f.lfL, f.eL, f.tL, j, sY[slot][0], hL);
+	if (valid && j < f.NR) side_face_gradient<NF>(a, f.kR, f.NR, f.lfR, f.eR, f.tR, j, sY[slot][1], hR);
+	__syncthreads();   // every y is read
+	if (valid && j < f.NL)
+#pragma unroll
+		for (int k = 0; k < K8; ++k) sY[slot][0][k * NF + j] = hL[k];
+	if (valid && j < f.NR)
+#pragma unroll
+		for (int k = 0; k < K8; ++k) sY[slot][1][k * NF + j] = hR[k];
+	__syncthreads();
+	if (valid && j < NF) {   // the face gradients P h at this face point, and the flux
+		double gL[K8], gR[K8];
+		const double* PL = o.Pg[f.NL];
+		const double* PR = o.Pg[f.NR];
+#pragma unroll
+		for (int k = 0; k < K8; ++k) {
+			double aL = 0.0, aR = 0.0;
+#pragma unroll
+			for (int i = 0; i < NF; ++i) {
+				if (i < f.NL) aL += PL[q * f.NL + i] * sY[slot][0][k * NF + i];
+				if (i < f.NR) aR += PR[qr * f.NR + i] * sY[slot][1][k * NF + i];
+			}
+			gL[k] = aL;
+			gR[k] = aR;
+		}
+		const double* nj = a.nJ + ((size_t)fi * kHpMaxN + q) * 2;
+		const double Jf = sqrt(nj[0] * nj[0] + nj[1] * nj[1]);
+		const double nu[2] = { nj[0] / Jf, nj[1] / Jf };
+		double nnf[kNV], fvL[K8], fvR[K8];
+		if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<double, 2>(sL, sR, nu, nnf);
+		else numflux_roe<double, 2>(sL, sR, nu, nnf);
+		ns_viscous_flux2<double>(sL, gL, a.mu, a.Pr, fvL);
+		ns_viscous_flux2<double>(sR, gR, a.mu, a.Pr, fvR);
+		const double wj = o.w[q] * Jf;
+#pragma unroll
+		for (int v = 0; v < kNV; ++v) {
+			const double vis = nu[0] * 0.5 * (fvL[0 + 2 * v] + fvR[0 + 2 * v]) + nu[1] * 0.5 * (fvL[1 + 2 * v] + fvR[1 + 2 * v]);
+			sQ[slot][q * kNV + v] = wj * (nnf[v] + vis);
+		}
+	}
+	__syncthreads();
+	if (valid && j < f.NL) lift_side<NF>(o, f.NL, j, false, false, sQ[slot], a.z + f.zL);
+	if (valid && j < f.NR) lift_side<NF>(o, f.NR, j, f.rev != 0, true, sQ[slot], a.z + f.zR);
+}
+
+TriArgs tri_args(const TriGroup& g, const TriOps<double>* ops, const double* u, double* r, const double* z)
+{
+	TriArgs a{};
+	a.u = u + g.u_off;
+	a.r = r ? r + g.u_off : nullptr;
+	a.met = g.met;
+	a.z = z ? z + g.z_off : nullptr;
+	a.tr = g.tr;
+	a.gv = g.gv;
+	a.gvt = g.gvt;
+	a.minv = g.minv;
+	a.jinv = g.jinv;
+	a.ops = ops + g.p;
+	a.E = g.E;
+	a.affine = g.affine;
+	return a;
+}
+
+// one pass of batches over the group (the kernels loop over batches when the grid is capped)
+int tri_grid(int E, int epb)
+{
+	return std::max(1, std::min((E + epb - 1) / epb, 16384));
+}
+
+MixArgs mix_args(const HpData& H, const Ctx& c, const double* u)
+{
+	MixArgs a{};
+	a.faces = H.mfaces;
+	a.nJ = H.mnJ;
+	a.u = u;
+	a.z = H.z64;
+	a.hops = H.ops64;
+	a.tops = H.tri_ops64;
+	for (size_t t = 0; t < H.tri.size() && t < (size_t)kTriMaxP; ++t) {
+		a.ttr[t] = H.tri[t].tr;
+		a.tgvt[t] = H.tri[t].gvt;
+		a.tK[t] = H.tri[t].affine ? nullptr : H.tri[t].K;
+		a.tjinv[t] = H.tri[t].jinv;
+	}
+	for (size_t g = 0; g < H.groups.size(); ++g) {
+		const Ctx& s = H.groups[g]->c;
+		a.qgvt[s.N] = s.ns_gvt;
+		a.qkap[s.N] = s.ns_kappa;
+	}
+	for (int k = 0; k < 5; ++k) a.bc_prim[k] = c.cfg.bc_prim[k];
+	a.mu = c.cfg.mu;
+	a.Pr = c.cfg.Pr;
+	return a;
+}
+
+// the faces with a TRI side, one launch per run of equal Nf, spread over the mesh's side streams
+hipError_t launch_mix(const HpData& H, const Ctx& c, const double* u, bool ns)
+{
+	MixArgs a = mix_args(H, c, u);
+	const bool lf = c.cfg.num_flux == DGX_FLUX_LF;
+	for (size_t i = 0; i < H.mruns.size(); ++i) {
+		const MixRun& run = H.mruns[i];
+		a.start = run.start;
+		a.count = run.count;
+		const hipStream_t st = H.lane[(i + 1) % HpData::kLanes];
+		const dim3 grid((run.count + 7) / 8);
+#define DGX_MIX(NFV)                                                                                           \
+	case NFV:                                                                                                  \
+		if (ns) {                                                                                              \
+			if (lf) hipLaunchKernelGGL((k_mixf_ns<NFV, DGX_FLUX_LF>), grid, dim3(64), 0, st, a);               \
+			else hipLaunchKernelGGL((k_mixf_ns<NFV, DGX_FLUX_ROE>), grid, dim3(64), 0, st, a);                 \
+		} else {                                                                                               \
+			if (lf) hipLaunchKernelGGL((k_mixf<NFV, DGX_FLUX_LF>), grid, dim3(64), 0, st, a);                  \
+			else hipLaunchKernelGGL((k_mixf<NFV, DGX_FLUX_ROE>), grid, dim3(64), 0, st, a);                    \
+		}                                                                                                      \
+		break;
+		switch (run.Nf) {
+			DGX_MIX(2) DGX_MIX(3) DGX_MIX(4) DGX_MIX(5) DGX_MIX(6)
+		default: return hipErrorInvalidValue;
+		}
+#undef DGX_MIX
+		const hipError_t e = hipGetLastError();
+		if (e != hipSuccess) return e;
+	}
+	return hipSuccess;
+}
+
+}  // namespace
+
+#define DGX_TRI_SWITCH(KERNEL, EPB_OF)                                                                     \
+	switch (g.p) {                                                                                         \
+	case 1: hipLaunchKernelGGL(KERNEL<1>, dim3(tri_grid(g.E, EPB_OF(1))), dim3(256), 0, st, a); break;     \
+	case 2: hipLaunchKernelGGL(KERNEL<2>, dim3(tri_grid(g.E, EPB_OF(2))), dim3(256), 0, st, a); break;     \
+	case 3: hipLaunchKernelGGL(KERNEL<3>, dim3(tri_grid(g.E, EPB_OF(3))), dim3(256), 0, st, a); break;     \
+	case 4: hipLaunchKernelGGL(KERNEL<4>, dim3(tri_grid(g.E, EPB_OF(4))), dim3(256), 0, st, a); break;     \
+	case 5: hipLaunchKernelGGL(KERNEL<5>, dim3(tri_grid(g.E, EPB_OF(5))), dim3(256), 0, st, a); break;     \
+	default: return hipErrorInvalidValue;                                                                  \
+	}
+
+hipError_t launch_tri_trace64(const TriGroup& g, const TriOps<double>* ops, const double* u, hipStream_t st)
+{
+	if (g.E == 0) return hipSuccess;
+	TriArgs a = tri_args(g, ops, u, nullptr, nullptr);
+	DGX_TRI_SWITCH(k_tri_trace, epb_trace)
+	return hipGetLastError();
 }
 
 hipError_t launch_tri_rhs64(const TriGroup& g, const TriOps<double>* ops, const double* u, double* r,
                             const double* z, hipStream_t st)
 {
 	if (g.E == 0) return hipSuccess;
-	TriRhsArgs a{ u + g.u_off, r + g.u_off, g.met, z + g.z_off, ops + g.p, g.E };
-	const int grid = std::min((g.E + 3) / 4, 4096);
-	switch (g.p) {
-	case 1: hipLaunchKernelGGL(k_tri_rhs<1>, dim3(grid), dim3(256), 0, st, a); break;
-	case 2: hipLaunchKernelGGL(k_tri_rhs<2>, dim3(grid), dim3(256), 0, st, a); break;
-	case 3: hipLaunchKernelGGL(k_tri_rhs<3>, dim3(grid), dim3(256), 0, st, a); break;
-	case 4: hipLaunchKernelGGL(k_tri_rhs<4>, dim3(grid), dim3(256), 0, st, a); break;
-	case 5: hipLaunchKernelGGL(k_tri_rhs<5>, dim3(grid), dim3(256), 0, st, a); break;
-	default: return hipErrorInvalidValue;
-	}
+	TriArgs a = tri_args(g, ops, u, r, z);
+	DGX_TRI_SWITCH(k_tri_rhs, epb_rhs)
 	return hipGetLastError();
-}
-
-struct MixFaceArgs {
-	const MixFace* faces;
-	const double* nJ;              // Fmix x kHpMaxN x 2
-	const double* u;               // the whole state
-	double* z;                     // the whole lifted-flux buffer
-	const TriOps<double>* tops;    // indexed by p
-	const HpOps<double>* hops;
-	double bc_prim[5];
-	int F;
-};
-
-// one side's trace at the face's Nf GL points, in that side's own face ordering
-__device__ inline void side_trace(const MixFaceArgs& a, int kind, int Ns, int lf, long long uoff, int Nf,
-                                  double (*out)[4])
-{
-	constexpr int NV = 4;
-	const double* U = a.u + uoff;
-	if (kind == 1) {
-		const TriOps<double>& o = a.tops[Ns - 1];
-		const int Np = o.Np;
-		double t[kHpMaxN][NV];
-		for (int k = 0; k < Ns; ++k)
-#pragma unroll
-			for (int v = 0; v < NV; ++v) {
-				double acc = 0.0;
-				for (int m = 0; m < Np; ++m) acc += o.Ifc[lf][k * Np + m] * U[v * Np + m];
-				t[k][v] = acc;
-			}
-		const double* Pg = a.hops->Pg[Ns][Nf];   // own GL points -> face GL points
-		for (int q = 0; q < Nf; ++q)
-#pragma unroll
-			for (int v = 0; v < NV; ++v) {
-				double acc = 0.0;
-				for (int k = 0; k < Ns; ++k) acc += Pg[q * Ns + k] * t[k][v];
-				out[q][v] = acc;
-			}
-		return;
-	}
-	const double* Ig = a.hops->Ig[Ns][Nf];       // GLL face nodes -> face GL points
-	const int Npq = Ns * Ns;
-	for (int q = 0; q < Nf; ++q)
-#pragma unroll
-		for (int v = 0; v < NV; ++v) {
-			double acc = 0.0;
-			for (int i = 0; i < Ns; ++i) acc += Ig[q * Ns + i] * U[v * Npq + quad_face_node(Ns, lf, i)];
-			out[q][v] = acc;
-		}
-}
-
-template <int FLUX>
-__global__ void __launch_bounds__(128) k_mix_faces(MixFaceArgs a)
-{
-	constexpr int NV = 4;
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= a.F) return;
-	const MixFace f = a.faces[i];
-	const bool inter = (f.bc == DGX_BC_INTERIOR);
-	const int Nf = inter ? (f.NL > f.NR ? f.NL : f.NR) : f.NL;
-	double sL[kHpMaxN][NV], sRo[kHpMaxN][NV], g[kHpMaxN][NV];
-	side_trace(a, f.kL, f.NL, f.lfL, f.uL, Nf, sL);
-	if (inter) side_trace(a, f.kR, f.NR, f.lfR, f.uR, Nf, sRo);
-	const double* wf = a.hops->w[Nf];
-	for (int q = 0; q < Nf; ++q) {
-		const double* nj = a.nJ + ((size_t)i * kHpMaxN + q) * 2;
-		const double Jf = sqrt(nj[0] * nj[0] + nj[1] * nj[1]);
-		const double nu[2] = { nj[0] / Jf, nj[1] / Jf };
-		double sR[NV], nnf[NV];
-		if (inter) {
-			const int qr = f.rev ? Nf - 1 - q : q;
-#pragma unroll
-			for (int v = 0; v < NV; ++v) sR[v] = sRo[qr][v];
-		} else {
-			bc_state<double, 2>(f.bc, sL[q], nu, a.bc_prim, sR);
-		}
-		if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<double, 2>(sL[q], sR, nu, nnf);
-		else numflux_roe<double, 2>(sL[q], sR, nu, nnf);
-#pragma unroll
-		for (int v = 0; v < NV; ++v) g[q][v] = wf[q] * Jf * nnf[v];
-	}
-	// lifts to each side's own GL face points: z = P^T g / w_own (the right side negated, in its own ordering)
-	{
-		const double* Pg = a.hops->Pg[f.NL][Nf];
-		for (int k = 0; k < f.NL; ++k)
-#pragma unroll
-			for (int v = 0; v < NV; ++v) {
-				double acc = 0.0;
-				for (int q = 0; q < Nf; ++q) acc += Pg[q * f.NL + k] * g[q][v];
-				a.z[f.zL + v * f.NL + k] = acc * a.hops->winv[f.NL][k];
-			}
-	}
-	if (inter) {
-		const double* Pg = a.hops->Pg[f.NR][Nf];
-		for (int k = 0; k < f.NR; ++k)
-#pragma unroll
-			for (int v = 0; v < NV; ++v) {
-				double acc = 0.0;
-				for (int q = 0; q < Nf; ++q) acc += Pg[(f.rev ? Nf - 1 - q : q) * f.NR + k] * g[q][v];
-				a.z[f.zR + v * f.NR + k] = 0.0 - acc * a.hops->winv[f.NR][k];
-			}
-	}
-}
-
-hipError_t launch_mix_faces64(const HpData& H, const Ctx& c, const double* u, hipStream_t st)
-{
-	if (H.Fmix == 0) return hipSuccess;
-	MixFaceArgs a{};
-	a.faces = H.mfaces;
-	a.nJ = H.mnJ;
-	a.u = u;
-	a.z = H.z64;
-	a.tops = H.tri_ops64;
-	a.hops = H.ops64;
-	for (int k = 0; k < 5; ++k) a.bc_prim[k] = c.cfg.bc_prim[k];
-	a.F = H.Fmix;
-	const int grid = (H.Fmix + 127) / 128;
-	if (c.cfg.num_flux == DGX_FLUX_LF) hipLaunchKernelGGL(k_mix_faces<DGX_FLUX_LF>, dim3(grid), dim3(128), 0, st, a);
-	else hipLaunchKernelGGL(k_mix_faces<DGX_FLUX_ROE>, dim3(grid), dim3(128), 0, st, a);
-	return hipGetLastError();
-}
-
-
-// ------------------------------------------------------------------------------------------- Navier-Stokes (BR2)
-// TRI groups with the BR2 viscous terms (compute_grad_coef_dg_T.c:171-307, compute_face_rlhs_dg_T.c:243-300):
-//   k_tri_gv:  the volume part of the weak gradient G_v = M^-1 I^T W (J grad s) per element, and its traces at the
-//              element's own GL face points (read by the face kernel);
-//   k_mix_faces_ns: per face with a TRI side, q = n J_f (s_R - s_L) / 2 at the face's GL points, each side's
-//              projection y = P^T w q / w_own at its own GL points (written for the residual kernels after the
-//              lifted flux), its face gradient h = trace G_v + eta (lift trace) y with eta = 1.01 NFMAX = 4.04
-//              (TRI: the dense K_f = Ifc_f M^-1 tw0_f; QUAD: its diagonal kappa), the inviscid flux plus the
-//              central n . (fv_L + fv_R) / 2 (numerical_flux_central_T.c:76-104), lifted as for Euler;
-//   k_tri_ns:  G = G_v + sum_f M^-1 tw0_f y_f at the cubature, F_euler + fv (flux_navier_stokes_T.c:1005-1040) in
-//              the volume term, and the lifted face fluxes.
-struct TriNsArgs {
-	const double* u;     // the group's state
-	double* r;           // the group's residual
-	const double* met;
-	const double* minv;  // E x Np x Np
-	double* gv;          // E x 2 x nv x Np
-	double* gvt;         // E x 3 x 2 x nv x N
-	const double* z;     // E x 3 x (3 nv N): lifted flux [v][a], then y [d][v][a]
-	const TriOps<double>* ops;
-	double mu, Pr;
-	int E;
-};
-
-template <int P>
-__global__ void __launch_bounds__(256) k_tri_gv(TriNsArgs a)
-{
-	constexpr int Np = (P + 1) * (P + 2) / 2, Nc = tri_nc<P>(), N = P + 1, NV = 4, W = 4;
-	__shared__ double sI[Nc * Np], sDr[Nc * Np], sDs[Nc * Np], sIfc[3 * N * Np], sWq[Nc];
-	__shared__ double sU[W][NV * Np], sX[W][2 * NV * Nc], sB[W][2 * NV * Np];
-	for (int i = threadIdx.x; i < Nc * Np; i += blockDim.x) {
-		sI[i] = a.ops->I[i];
-		sDr[i] = a.ops->Dr[i];
-		sDs[i] = a.ops->Ds[i];
-	}
-	for (int i = threadIdx.x; i < 3 * N * Np; i += blockDim.x) sIfc[i] = a.ops->Ifc[i / (N * Np)][i % (N * Np)];
-	for (int i = threadIdx.x; i < Nc; i += blockDim.x) sWq[i] = a.ops->w[i];
-	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-	for (int base = blockIdx.x * W; base < a.E; base += gridDim.x * W) {
-		const int e = base + wave;
-		const bool valid = e < a.E;
-		__syncthreads();
-		if (valid)
-			for (int i = lane; i < NV * Np; i += 64) sU[wave][i] = a.u[(size_t)e * NV * Np + i];
-		__syncthreads();
-		if (valid && lane < Nc) {   // W J grad s at the cubature
-			const int q = lane;
-			const double* mp = a.met + (size_t)e * 4 * Nc + q;
-			const double m0 = mp[0], m1 = mp[Nc], m2 = mp[2 * Nc], m3 = mp[3 * Nc];
-#pragma unroll
-			for (int v = 0; v < NV; ++v) {
-				double sr = 0.0, ss = 0.0;
-#pragma unroll
-				for (int m = 0; m < Np; ++m) {
-					sr += sDr[q * Np + m] * sU[wave][v * Np + m];
-					ss += sDs[q * Np + m] * sU[wave][v * Np + m];
-				}
-				sX[wave][v * Nc + q] = sWq[q] * (m0 * sr + m2 * ss);
-				sX[wave][(NV + v) * Nc + q] = sWq[q] * (m1 * sr + m3 * ss);
-			}
-		}
-		__syncthreads();
-		if (valid && lane < Np) {   // I^T (W J grad s)
-			const int m = lane;
-#pragma unroll
-			for (int k = 0; k < 2 * NV; ++k) {
-				double acc = 0.0;
-#pragma unroll
-				for (int q = 0; q < Nc; ++q) acc += sI[q * Np + m] * sX[wave][k * Nc + q];
-				sB[wave][k * Np + m] = acc;
-			}
-		}
-		__syncthreads();
-		double g[2 * NV];
-		if (valid && lane < Np) {   // G_v = M^-1 (...)
-			const int m = lane;
-			const double* mi = a.minv + (size_t)e * Np * Np + m * Np;
-#pragma unroll
-			for (int k = 0; k < 2 * NV; ++k) g[k] = 0.0;
-			for (int n = 0; n < Np; ++n) {
-				const double c = mi[n];
-#pragma unroll
-				for (int k = 0; k < 2 * NV; ++k) g[k] += c * sB[wave][k * Np + n];
-			}
-		}
-		__syncthreads();
-		if (valid && lane < Np) {
-#pragma unroll
-			for (int k = 0; k < 2 * NV; ++k) {
-				sB[wave][k * Np + lane] = g[k];
-				a.gv[((size_t)e * 2 * NV + k) * Np + lane] = g[k];
-			}
-		}
-		__syncthreads();
-		if (valid && lane < 3 * N) {   // traces at the own GL face points
-			const int lf = lane / N, k0 = lane - lf * N;
-#pragma unroll
-			for (int k = 0; k < 2 * NV; ++k) {
-				double acc = 0.0;
-#pragma unroll
-				for (int m = 0; m < Np; ++m) acc += sIfc[(lf * N + k0) * Np + m] * sB[wave][k * Np + m];
-				a.gvt[(((size_t)e * 3 + lf) * 2 * NV + k) * N + k0] = acc;
-			}
-		}
-	}
-}
-
-template <int P>
-__global__ void __launch_bounds__(256) k_tri_ns(TriNsArgs a)
-{
-	constexpr int Np = (P + 1) * (P + 2) / 2, Nc = tri_nc<P>(), N = P + 1, NV = 4, W = 4, ZB = 3 * NV * N;
-	__shared__ double sI[Nc * Np], sDrW[Np * Nc], sDsW[Np * Nc], sTw0[3 * Np * N];
-	__shared__ double sU[W][NV * Np], sFR[W][2 * NV * Nc], sZ[W][3 * ZB], sG[W][2 * NV * Np];
-	for (int i = threadIdx.x; i < Nc * Np; i += blockDim.x) {
-		sI[i] = a.ops->I[i];
-		sDrW[i] = a.ops->DrW[i];
-		sDsW[i] = a.ops->DsW[i];
-	}
-	for (int i = threadIdx.x; i < 3 * Np * N; i += blockDim.x) sTw0[i] = a.ops->tw0[i / (Np * N)][i % (Np * N)];
-	const double mu = a.mu, Pr = a.Pr;
-	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-	for (int base = blockIdx.x * W; base < a.E; base += gridDim.x * W) {
-		const int e = base + wave;
-		const bool valid = e < a.E;
-		__syncthreads();
-		if (valid) {
-			for (int i = lane; i < NV * Np; i += 64) sU[wave][i] = a.u[(size_t)e * NV * Np + i];
-			for (int i = lane; i < 3 * ZB; i += 64) sZ[wave][i] = a.z[(size_t)e * 3 * ZB + i];
-		}
-		__syncthreads();
-		// the face lifts of the gradient: t = sum_f tw0_f y_f at the nodes (into sFR, free until the fluxes)
-		if (valid && lane < Np) {
-			const int m = lane;
-#pragma unroll
-			for (int k = 0; k < 2 * NV; ++k) {
-				double acc = 0.0;
-#pragma unroll
-				for (int lf = 0; lf < 3; ++lf)
-#pragma unroll
-					for (int b = 0; b < N; ++b) acc += sTw0[(lf * Np + m) * N + b] * sZ[wave][lf * ZB + NV * N + k * N + b];
-				sFR[wave][k * Np + m] = acc;
-			}
-		}
-		__syncthreads();
-		if (valid && lane < Np) {   // G = G_v + M^-1 t
-			const int m = lane;
-			const double* mi = a.minv + (size_t)e * Np * Np + m * Np;
-			double g[2 * NV];
-#pragma unroll
-			for (int k = 0; k < 2 * NV; ++k) g[k] = a.gv[((size_t)e * 2 * NV + k) * Np + m];
-			for (int n = 0; n < Np; ++n) {
-				const double c = mi[n];
-#pragma unroll
-				for (int k = 0; k < 2 * NV; ++k) g[k] += c * sFR[wave][k * Np + n];
-			}
-#pragma unroll
-			for (int k = 0; k < 2 * NV; ++k) sG[wave][k * Np + m] = g[k];
-		}
-		__syncthreads();
-		if (valid && lane < Nc) {
-			const int q = lane;
-			double s[NV], gq[2 * NV];
-#pragma unroll
-			for (int v = 0; v < NV; ++v) {
-				double acc = 0.0;
-#pragma unroll
-				for (int m = 0; m < Np; ++m) acc += sI[q * Np + m] * sU[wave][v * Np + m];
-				s[v] = acc;
-			}
-#pragma unroll
-			for (int k = 0; k < 2 * NV; ++k) {
-				double acc = 0.0;
-#pragma unroll
-				for (int m = 0; m < Np; ++m) acc += sI[q * Np + m] * sG[wave][k * Np + m];
-				gq[k] = acc;
-			}
-			const double* mp = a.met + (size_t)e * 4 * Nc + q;
-			const double m0 = mp[0], m1 = mp[Nc], m2 = mp[2 * Nc], m3 = mp[3 * Nc];
-			double f[2 * NV], fv[2 * NV];
-			euler_flux<double, 2>(s, f);
-			ns_viscous_flux2<double>(s, gq, mu, Pr, fv);
-#pragma unroll
-			for (int k = 0; k < 2 * NV; ++k) f[k] += fv[k];
-#pragma unroll
-			for (int v = 0; v < NV; ++v) {
-				sFR[wave][v * Nc + q] = m0 * f[2 * v] + m1 * f[2 * v + 1];
-				sFR[wave][(NV + v) * Nc + q] = m2 * f[2 * v] + m3 * f[2 * v + 1];
-			}
-		}
-		__syncthreads();
-		if (valid && lane < Np) {
-			const int m = lane;
-#pragma unroll
-			for (int v = 0; v < NV; ++v) {
-				double acc = 0.0;
-#pragma unroll
-				for (int q = 0; q < Nc; ++q)
-					acc += sDrW[m * Nc + q] * sFR[wave][v * Nc + q] + sDsW[m * Nc + q] * sFR[wave][(NV + v) * Nc + q];
-#pragma unroll
-				for (int lf = 0; lf < 3; ++lf)
-#pragma unroll
-					for (int b = 0; b < N; ++b) acc -= sTw0[(lf * Np + m) * N + b] * sZ[wave][lf * ZB + v * N + b];
-				a.r[(size_t)e * NV * Np + v * Np + m] = acc;
-			}
-		}
-	}
-}
-
-struct MixFaceNsArgs {
-	MixFaceArgs base;
-	const double* qgvt[kHpMaxN + 1];   // QUAD groups by N: face traces of G_v (E_g x 4 x 2 x nv x N) and kappa
-	const double* qkap[kHpMaxN + 1];
-	const double* tgvt[kTriMaxP];      // TRI groups by group index: gvt and K
-	const double* tK[kTriMaxP];
-	double mu, Pr;
-};
-
-// one side's face gradient h at its own GL points from its projection y: trace G_v + eta (lift trace) y
-__device__ inline void side_face_gradient(const MixFaceNsArgs& a, int kind, int Ns, int lf, int el, int tg,
-                                          const double (*y)[8], double (*h)[8])
-{
-	constexpr int NV = 4;
-	const double eta = 1.01 * 4.0;
-	if (kind == 1) {
-		const double* t = a.tgvt[tg] + ((size_t)el * 3 + lf) * 2 * NV * Ns;
-		const double* K = a.tK[tg] + ((size_t)el * 3 + lf) * Ns * Ns;
-		for (int k0 = 0; k0 < Ns; ++k0)
-			for (int k = 0; k < 2 * NV; ++k) {
-				double acc = 0.0;
-				for (int b = 0; b < Ns; ++b) acc += K[k0 * Ns + b] * y[b][k];
-				h[k0][k] = t[k * Ns + k0] + eta * acc;
-			}
-		return;
-	}
-	const double* t = a.qgvt[Ns] + ((size_t)el * 4 + lf) * 2 * NV * Ns;
-	const double* kap = a.qkap[Ns] + ((size_t)el * 4 + lf) * Ns;
-	for (int k0 = 0; k0 < Ns; ++k0)
-		for (int k = 0; k < 2 * NV; ++k) h[k0][k] = t[k * Ns + k0] + eta * kap[k0] * y[k0][k];
-}
-
-template <int FLUX>
-__global__ void __launch_bounds__(128) k_mix_faces_ns(MixFaceNsArgs A)
-{
-	constexpr int NV = 4;
-	const MixFaceArgs& a = A.base;
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= a.F) return;
-	const MixFace f = a.faces[i];
-	const int Nf = f.NL > f.NR ? f.NL : f.NR;
-	double sL[kHpMaxN][NV], sRo[kHpMaxN][NV], qv[kHpMaxN][2 * NV];
-	side_trace(a, f.kL, f.NL, f.lfL, f.uL, Nf, sL);
-	side_trace(a, f.kR, f.NR, f.lfR, f.uR, Nf, sRo);
-	const double* wf = a.hops->w[Nf];
-	for (int q = 0; q < Nf; ++q) {   // w q, q = n J_f (s_R - s_L) / 2, left ordering
-		const double* nj = a.nJ + ((size_t)i * kHpMaxN + q) * 2;
-		const int qr = f.rev ? Nf - 1 - q : q;
-#pragma unroll
-		for (int v = 0; v < NV; ++v) {
-			const double dl = 0.5 * (sRo[qr][v] - sL[q][v]);
-			qv[q][v] = wf[q] * (nj[0] * dl);
-			qv[q][NV + v] = wf[q] * (nj[1] * dl);
-		}
-	}
-	// projections to each side's own points (written after the lifted flux), and the face gradients there
-	double yL[kHpMaxN][8], yR[kHpMaxN][8], hL[kHpMaxN][8], hR[kHpMaxN][8];
-	{
-		const double* Pg = a.hops->Pg[f.NL][Nf];
-		for (int k0 = 0; k0 < f.NL; ++k0)
-			for (int k = 0; k < 2 * NV; ++k) {
-				double acc = 0.0;
-				for (int q = 0; q < Nf; ++q) acc += Pg[q * f.NL + k0] * qv[q][k];
-				yL[k0][k] = acc * a.hops->winv[f.NL][k0];
-				a.z[f.zL + NV * f.NL + k * f.NL + k0] = yL[k0][k];
-			}
-		const double* PgR = a.hops->Pg[f.NR][Nf];
-		for (int k0 = 0; k0 < f.NR; ++k0)
-			for (int k = 0; k < 2 * NV; ++k) {
-				double acc = 0.0;
-				for (int q = 0; q < Nf; ++q) acc += PgR[(f.rev ? Nf - 1 - q : q) * f.NR + k0] * qv[q][k];
-				yR[k0][k] = acc * a.hops->winv[f.NR][k0];
-				a.z[f.zR + NV * f.NR + k * f.NR + k0] = yR[k0][k];
-			}
-	}
-	side_face_gradient(A, f.kL, f.NL, f.lfL, f.eL, f.tL, yL, hL);
-	side_face_gradient(A, f.kR, f.NR, f.lfR, f.eR, f.tR, yR, hR);
-	double g[kHpMaxN][NV];
-	const double mu = A.mu, Pr = A.Pr;
-	const double* PgL = a.hops->Pg[f.NL][Nf];
-	const double* PgR = a.hops->Pg[f.NR][Nf];
-	for (int q = 0; q < Nf; ++q) {
-		const int qr = f.rev ? Nf - 1 - q : q;
-		double gL[2 * NV], gR[2 * NV];
-		for (int k = 0; k < 2 * NV; ++k) {
-			double aL = 0.0, aR = 0.0;
-			for (int k0 = 0; k0 < f.NL; ++k0) aL += PgL[q * f.NL + k0] * hL[k0][k];
-			for (int k0 = 0; k0 < f.NR; ++k0) aR += PgR[qr * f.NR + k0] * hR[k0][k];
-			gL[k] = aL;
-			gR[k] = aR;
-		}
-		const double* nj = a.nJ + ((size_t)i * kHpMaxN + q) * 2;
-		const double Jf = sqrt(nj[0] * nj[0] + nj[1] * nj[1]);
-		const double nu[2] = { nj[0] / Jf, nj[1] / Jf };
-		double sR[NV], nnf[NV], fvL[2 * NV], fvR[2 * NV];
-#pragma unroll
-		for (int v = 0; v < NV; ++v) sR[v] = sRo[qr][v];
-		if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<double, 2>(sL[q], sR, nu, nnf);
-		else numflux_roe<double, 2>(sL[q], sR, nu, nnf);
-		ns_viscous_flux2<double>(sL[q], gL, mu, Pr, fvL);
-		ns_viscous_flux2<double>(sR, gR, mu, Pr, fvR);
-#pragma unroll
-		for (int v = 0; v < NV; ++v) {
-			const double vis = 0.5 * ((fvL[2 * v] + fvR[2 * v]) * nu[0] + (fvL[2 * v + 1] + fvR[2 * v + 1]) * nu[1]);
-			g[q][v] = wf[q] * Jf * (nnf[v] + vis);
-		}
-	}
-	{
-		const double* Pg = a.hops->Pg[f.NL][Nf];
-		for (int k = 0; k < f.NL; ++k)
-#pragma unroll
-			for (int v = 0; v < NV; ++v) {
-				double acc = 0.0;
-				for (int q = 0; q < Nf; ++q) acc += Pg[q * f.NL + k] * g[q][v];
-				a.z[f.zL + v * f.NL + k] = acc * a.hops->winv[f.NL][k];
-			}
-		const double* Pr_ = a.hops->Pg[f.NR][Nf];
-		for (int k = 0; k < f.NR; ++k)
-#pragma unroll
-			for (int v = 0; v < NV; ++v) {
-				double acc = 0.0;
-				for (int q = 0; q < Nf; ++q) acc += Pr_[(f.rev ? Nf - 1 - q : q) * f.NR + k] * g[q][v];
-				a.z[f.zR + v * f.NR + k] = 0.0 - acc * a.hops->winv[f.NR][k];
-			}
-	}
 }
 
 hipError_t launch_tri_ns_phase64(const TriGroup& g, const TriOps<double>* ops, const double* u, double* r,
                                  const double* z, double mu, double Pr, int phase, hipStream_t st)
 {
 	if (g.E == 0) return hipSuccess;
-	TriNsArgs a{ u + g.u_off, r + g.u_off, g.met, g.minv, g.gv, g.gvt, z + g.z_off, ops + g.p, mu, Pr, g.E };
-	const int grid = std::min((g.E + 3) / 4, 4096);
-#define DGX_TRI_NS(PP)                                                                                     \
-	case PP:                                                                                               \
-		if (phase == 1) hipLaunchKernelGGL(k_tri_gv<PP>, dim3(grid), dim3(256), 0, st, a);                  \
-		else hipLaunchKernelGGL(k_tri_ns<PP>, dim3(grid), dim3(256), 0, st, a);                             \
-		break;
-	switch (g.p) {
-		DGX_TRI_NS(1) DGX_TRI_NS(2) DGX_TRI_NS(3) DGX_TRI_NS(4) DGX_TRI_NS(5)
-	default: return hipErrorInvalidValue;
+	TriArgs a = tri_args(g, ops, u, r, z);
+	a.mu = mu;
+	a.Pr = Pr;
+	if (phase == 1) {
+		DGX_TRI_SWITCH(k_tri_gv, epb_gv)
+	} else {
+		DGX_TRI_SWITCH(k_tri_ns, epb_ns)
 	}
-#undef DGX_TRI_NS
 	return hipGetLastError();
 }
 
-hipError_t launch_mix_faces_ns64(const HpData& H, const Ctx& c, const double* u, hipStream_t st)
+#undef DGX_TRI_SWITCH
+
+hipError_t launch_mix_faces64(const HpData& H, const Ctx& c, const double* u, hipStream_t)
 {
-	if (H.Fmix == 0) return hipSuccess;
-	MixFaceNsArgs A{};
-	MixFaceArgs& a = A.base;
-	a.faces = H.mfaces;
-	a.nJ = H.mnJ;
-	a.u = u;
-	a.z = H.z64;
-	a.tops = H.tri_ops64;
-	a.hops = H.ops64;
-	a.F = H.Fmix;
-	for (size_t g = 0; g < H.groups.size(); ++g) {
-		const Ctx& s = H.groups[g]->c;
-		A.qgvt[s.N] = s.ns_gvt;
-		A.qkap[s.N] = s.ns_kappa;
-	}
-	for (size_t t = 0; t < H.tri.size() && t < (size_t)kTriMaxP; ++t) {
-		A.tgvt[t] = H.tri[t].gvt;
-		A.tK[t] = H.tri[t].K;
-	}
-	A.mu = c.cfg.mu;
-	A.Pr = c.cfg.Pr;
-	const int grid = (H.Fmix + 127) / 128;
-	if (c.cfg.num_flux == DGX_FLUX_LF) hipLaunchKernelGGL(k_mix_faces_ns<DGX_FLUX_LF>, dim3(grid), dim3(128), 0, st, A);
-	else hipLaunchKernelGGL(k_mix_faces_ns<DGX_FLUX_ROE>, dim3(grid), dim3(128), 0, st, A);
-	return hipGetLastError();
+	return launch_mix(H, c, u, false);
+}
+
+hipError_t launch_mix_faces_ns64(const HpData& H, const Ctx& c, const double* u, hipStream_t)
+{
+	return launch_mix(H, c, u, true);
 }
 
 }  // namespace dgx
