@@ -75,10 +75,8 @@ int bc_code(int tag)
 
 extern "C" {
 
-dgx_status dgx_mesh_read_gmsh(const char* path, int p, dgx_host_mesh* out)
+static dgx_status read_gmsh(const char* path, int p, dgx_host_mesh* out)
 {
-	if (!path || !out || p < 1 || p > 5) return DGX_ERR_ARG;
-	std::memset(out, 0, sizeof(*out));
 	std::ifstream in(path);
 	if (!in) return DGX_ERR_ARG;
 	std::map<int, std::array<double, 2>> nodes;
@@ -96,9 +94,9 @@ dgx_status dgx_mesh_read_gmsh(const char* path, int p, dgx_host_mesh* out)
 			int n = 0;
 			in >> n;
 			for (int i = 0; i < n; ++i) {
-				int id;
-				double x, y, z;
-				in >> id >> x >> y >> z;
+				int id = 0;
+				double x = 0.0, y = 0.0, z = 0.0;
+				if (!(in >> id >> x >> y >> z)) return DGX_ERR_ARG;   // truncated or malformed node line
 				nodes[id] = { x, y };
 			}
 		} else if (line.rfind("$Elements", 0) == 0) {
@@ -108,14 +106,22 @@ dgx_status dgx_mesh_read_gmsh(const char* path, int p, dgx_host_mesh* out)
 			for (int i = 0; i < n; ++i) {
 				std::getline(in, line);
 				std::istringstream ss(line);
-				int id, type, ntag;
-				ss >> id >> type >> ntag;
+				int id = 0, type = 0, ntag = -1;
+				if (!(ss >> id >> type >> ntag) || ntag < 0 || ntag > 64) return DGX_ERR_ARG;
 				Elem e;
 				e.type = type;
 				e.tags.resize(ntag);
-				for (int t = 0; t < ntag; ++t) ss >> e.tags[t];
+				for (int t = 0; t < ntag; ++t)
+					if (!(ss >> e.tags[t])) return DGX_ERR_ARG;
 				int v;
 				while (ss >> v) e.nodes.push_back(v);
+				// vertices per gmsh element type: 15 point, 1 line, 2 triangle, 3 quadrangle, 4 tetrahedron,
+				// 5 hexahedron (first-order types only)
+				const int need = type == 15 ? 1 : type == 1 ? 2 : type == 2 ? 3 : type == 3 ? 4 : type == 4 ? 4 : type == 5 ? 8 : -1;
+				if (need < 0) return DGX_ERR_UNSUPPORTED;
+				if ((int)e.nodes.size() < need) return DGX_ERR_ARG;   // truncated element line
+				for (int k = 0; k < need; ++k)
+					if (!nodes.count(e.nodes[k])) return DGX_ERR_ARG;   // a vertex the $Nodes section lacks
 				elems.push_back(e);
 			}
 		}
@@ -287,6 +293,18 @@ void dgx_mesh_free(dgx_host_mesh* m)
 	std::free(m->v_to_lf);
 	std::free(m->v_to_lf_wp);
 	std::memset(m, 0, sizeof(*m));
+}
+
+dgx_status dgx_mesh_read_gmsh(const char* path, int p, dgx_host_mesh* out)
+{
+	if (!path || !out || p < 1 || p > 5) return DGX_ERR_ARG;
+	std::memset(out, 0, sizeof(*out));
+	try {
+		return read_gmsh(path, p, out);
+	} catch (...) {   // a malformed file (std::out_of_range, std::bad_alloc): a status code, never an abort
+		dgx_mesh_free(out);
+		return DGX_ERR_ARG;
+	}
 }
 
 }  // extern "C"

@@ -10,6 +10,7 @@
 // The Lagrange operators are formed through monomials r^a s^b with the Vandermonde inverse in long double, so
 // that the double operators are rounded once (Np <= 21).
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <algorithm>
 #include <vector>
@@ -327,6 +328,7 @@ extern "C" dgx_status dgx_reference_nodes(int type, int p, double* rst, double* 
 // Host only: the cubature of the reference's TRI rule <kind><q> (kind 0: WSH, 1: WV) as the reference expands it.
 extern "C" dgx_status dgx_tri_rule(int kind, int q, double* rst, double* w, int* n)
 {
+	if (kind != 0 && kind != 1) return DGX_ERR_ARG;
 	const TriRule* t = find_rule(kind == 0 ? "WSH" : "WV", q);
 	if (!t) return DGX_ERR_ARG;
 	std::vector<double> rs, ww;
