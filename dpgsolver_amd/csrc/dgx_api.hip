@@ -338,17 +338,19 @@ void outward_nj(const double* xyz, int N, int lf, const double* s, int ns, doubl
 	}
 }
 
-// Enqueues launch(g, s) for every order group on the group's side stream; the group contexts keep the
-// caller's stream outside these calls.
+// Enqueues launch(g, s) for every order group on the group's side stream.  Each group context gets its own
+// stream back afterwards (not the caller's current one: during a graph capture that is the private capture
+// stream, destroyed when the capture ends, and a group context must never keep it).
 template <typename F>
 hipError_t on_groups(Ctx& c, F launch)
 {
 	dgx::HpData& H = *c.hp;
 	for (size_t g = 0; g < H.groups.size(); ++g) {
 		Ctx& s = H.groups[g]->c;
+		const hipStream_t own = s.stream;
 		s.stream = H.lane[H.group_lane[g]];
 		const hipError_t e = launch(g, s);
-		s.stream = c.stream;
+		s.stream = own;
 		if (e != hipSuccess) return e;
 	}
 	return hipSuccess;
@@ -361,10 +363,11 @@ hipError_t hp_barrier(dgx::HpData& H, hipStream_t main)
 	return e != hipSuccess ? e : dgx::hp_fork(H, main);
 }
 
-// Variable-order residuals. Navier-Stokes: every group's gradient traces, the faces between orders (lifted
-// gradients and fluxes), every group's residual (+ M^-1 and max-norm). Euler: the faces between orders, then
-// every group's residual. The launches of one phase write disjoint outputs and run concurrently on the side
-// streams; the phases are ordered by barriers.
+// Variable-order and mixed-type residuals (FP64). Navier-Stokes: every group's gradient traces (TRI: and state
+// traces), the faces between orders or types (lifted gradients and fluxes; one launch per face-point count),
+// every group's residual (+ M^-1 and max-norm). Euler: the TRI state traces, the faces, every group's residual.
+// The launches of one phase write disjoint outputs and run concurrently on the side streams; the phases are
+// ordered by barriers.
 dgx_status hp_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned long long* dmax)
 {
 	dgx::HpData& H = *c.hp;
@@ -377,13 +380,13 @@ dgx_status hp_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned long 
 		const double mu = c.cfg.mu, Pr = c.cfg.Pr;
 		DGX_TRY(on_groups(c, [&](size_t g, Ctx& s) { return dgx::launch_rhs_ns_phase(s, u + H.u_off[g], r + H.u_off[g], 1); }));
 		for (size_t t = 0; t < H.tri.size(); ++t)
-			DGX_TRY(dgx::launch_tri_ns_phase64(H.tri[t], H.tri_ops64, u, r, H.z64, mu, Pr, 1, H.lane[t % dgx::HpData::kLanes]));
+			DGX_TRY(dgx::launch_tri_ns_phase64(H.tri[t], H.tri_ops64, u, r, H.z64, mu, Pr, 1, H.lane[H.tri_lane[t]]));
 		DGX_TRY(hp_barrier(H, c.stream));
-		DGX_TRY(dgx::launch_hp_faces64(c, u));
-		DGX_TRY(dgx::launch_mix_faces_ns64(H, c, u, H.lane[dgx::HpData::kLanes - 1]));
+		if (H.tri.empty()) DGX_TRY(dgx::launch_hp_faces64(c, u));
+		else DGX_TRY(dgx::launch_mix_faces_ns64(H, c, u, H.lane[dgx::HpData::kLanes - 1]));
 		DGX_TRY(hp_barrier(H, c.stream));
 		for (size_t t = 0; t < H.tri.size(); ++t)
-			DGX_TRY(dgx::launch_tri_ns_phase64(H.tri[t], H.tri_ops64, u, r, H.z64, mu, Pr, 2, H.lane[t % dgx::HpData::kLanes]));
+			DGX_TRY(dgx::launch_tri_ns_phase64(H.tri[t], H.tri_ops64, u, r, H.z64, mu, Pr, 2, H.lane[H.tri_lane[t]]));
 		DGX_TRY(on_groups(c, [&](size_t g, Ctx& s) -> hipError_t {
 			double* rg = r + H.u_off[g];
 			hipError_t e = dgx::launch_rhs_ns_phase(s, u + H.u_off[g], rg, 2);
@@ -393,17 +396,18 @@ dgx_status hp_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned long 
 			return e;
 		}));
 	} else {
-		DGX_TRY(dgx::launch_hp_faces64(c, u));
-		if (!H.tri.empty()) {   // the TRI traces (beside the faces between QUAD orders), then the faces with a TRI side
+		if (H.tri.empty()) {
+			DGX_TRY(dgx::launch_hp_faces64(c, u));
+		} else {   // the TRI traces, then the faces
 			for (size_t t = 0; t < H.tri.size(); ++t)
-				DGX_TRY(dgx::launch_tri_trace64(H.tri[t], H.tri_ops64, u, H.lane[t % dgx::HpData::kLanes]));
+				DGX_TRY(dgx::launch_tri_trace64(H.tri[t], H.tri_ops64, u, H.lane[H.tri_lane[t]]));
 			DGX_TRY(hp_barrier(H, c.stream));
 			DGX_TRY(dgx::launch_mix_faces64(H, c, u, H.lane[dgx::HpData::kLanes - 1]));
 		}
 		DGX_TRY(hp_barrier(H, c.stream));
 		DGX_TRY(on_groups(c, [&](size_t g, Ctx& s) { return dgx::launch_rhs64(s, u + H.u_off[g], r + H.u_off[g], mode, dmax); }));
 		for (size_t t = 0; t < H.tri.size(); ++t)
-			DGX_TRY(dgx::launch_tri_rhs64(H.tri[t], H.tri_ops64, u, r, H.z64, H.lane[t % dgx::HpData::kLanes]));
+			DGX_TRY(dgx::launch_tri_rhs64(H.tri[t], H.tri_ops64, u, r, H.z64, H.lane[H.tri_lane[t]]));
 	}
 	DGX_TRY(dgx::hp_join(H, c.stream));
 	return DGX_OK;
@@ -611,6 +615,25 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 			ro[2 * q] = -tmp[2 * qr];
 			ro[2 * q + 1] = -tmp[2 * qr + 1];
 		}
+		// on a mesh with TRI elements, the same face for the FP64 face kernels, which then take every face
+		// between orders or types by Nf (on QUAD-only meshes the per-pair kernels measured faster)
+		if (!any_tri) continue;
+		dgx::MixFace m{};
+		m.uL = x.uL;
+		m.uR = x.uR;
+		m.zL = x.zL;
+		m.zR = x.zR;
+		m.NL = NL;
+		m.NR = NR;
+		m.lfL = fL;
+		m.lfR = fR;
+		m.rev = (obits & 2) ? 1 : 0;
+		m.bc = DGX_BC_INTERIOR;
+		m.eL = x.eL;
+		m.eR = x.eR;
+		m.tL = m.tR = -1;
+		mf.push_back(m);
+		mnj.insert(mnj.end(), nj.begin() + base, nj.begin() + base + 2 * dgx::kHpMaxN);
 	}
 	// faces with a TRI side sorted by (Nf, kL, NL, kR, NR) (stable): one launch per Nf, waves of one shape
 	std::vector<dgx::MixRun> mruns;
@@ -814,14 +837,25 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 		}
 		return lane;
 	};
-	std::vector<double> gcost(G), pcost(H.pairs.size());
+	// the QUAD and TRI groups of one phase share the streams: one balance over both (a TRI element's dense
+	// products cost about Np Nc / 3 against N^2 per QUAD element, from the measured kernel times)
+	std::vector<double> gcost(G + T), pcost(H.pairs.size());
 	for (int g = 0; g < G; ++g) gcost[g] = (double)(e_off[g + 1] - e_off[g]) * (order[g] + 1) * (order[g] + 1);
+	for (int t = 0; t < T; ++t) {
+		const int p = tri_p[t], np = (p + 1) * (p + 2) / 2, nc = p == 1 ? 3 : p == 2 ? 6 : p == 3 ? 12 : p == 4 ? 16 : 25;
+		gcost[G + t] = (double)(tri_e0[t + 1] - tri_e0[t]) * np * nc / 3.0;
+	}
 	for (size_t i = 0; i < H.pairs.size(); ++i) {
 		const int n = std::max(H.pairs[i].NL, H.pairs[i].NR);
 		pcost[i] = (double)H.pairs[i].count * n * n;
 	}
-	H.group_lane = balance(gcost);
+	const std::vector<int> lanes = balance(gcost);
+	H.group_lane.assign(lanes.begin(), lanes.begin() + G);
+	H.tri_lane.assign(lanes.begin() + G, lanes.end());
 	H.pair_lane = balance(pcost);
+	std::vector<double> mcost(H.mruns.size());
+	for (size_t i = 0; i < H.mruns.size(); ++i) mcost[i] = (double)H.mruns[i].count * H.mruns[i].Nf * H.mruns[i].Nf;
+	H.mrun_lane = balance(mcost);
 	H.ready = true;
 	return DGX_OK;
 }

@@ -180,7 +180,9 @@ struct HpData {
 	double* mnJ = nullptr;                 // Fmix x kHpMaxN x 2: n J_f at the face GL points (left ordering)
 	int Fmix = 0;
 	std::vector<MixRun> mruns;             // runs of mixed faces with one Nf
-	std::vector<int> group_lane;     // side stream of each group (longest-first balance of E_g N^2)
+	std::vector<int> mrun_lane;            // side stream of each run (longest-first balance of count Nf^2)
+	std::vector<int> group_lane;     // side stream of each QUAD group and of each TRI group (one longest-first
+	std::vector<int> tri_lane;       // balance of the groups' costs)
 	std::vector<int> pair_lane;      // side stream of each face pair (longest-first balance of count max(NL,NR)^2)
 };
 

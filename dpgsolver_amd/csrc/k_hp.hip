@@ -450,6 +450,10 @@ hipError_t run_hp_faces(Ctx& c, const S* u, const double* u64, S* z, const S* nJ
 
 }  // namespace
 
+// Fork and join of the side streams around one phase.  Both also run inside graph captures (dgx_graph_create_rhs),
+// so each event is recorded on this call, in the current capture, before any stream waits on it: a wait on an
+// event whose last record belonged to an earlier capture (whose graph may since have been destroyed) would make
+// the capture depend on nodes of that other graph.
 hipError_t hp_fork(HpData& H, hipStream_t main)
 {
 	DGX_HIP_CHECK(hipEventRecord(H.fork_ev, main));

@@ -34,7 +34,8 @@ namespace dgx {
 namespace {
 
 constexpr int kNV = 4;
-constexpr int kTriLds = 6144;   // LDS budget of one TRI workgroup, in doubles (48 KB)
+constexpr int kTriLds = 6144;     // LDS budget of one TRI workgroup, in doubles (48 KB)
+constexpr int kTriLdsNs = 8192;   // the Navier-Stokes kernels (64 KB; their registers allow two per CU anyway)
 
 __device__ inline int quad_face_node(int N, int lf, int i)
 {
@@ -46,10 +47,10 @@ constexpr int tri_np(int P) { return (P + 1) * (P + 2) / 2; }
 constexpr int tri_nc(int P) { return P == 1 ? 3 : P == 2 ? 6 : P == 3 ? 12 : P == 4 ? 16 : 25; }   // WV of degree 2p
 
 // elements per workgroup: enough to give every lane a point, within the LDS budget
-constexpr int tri_epb(int lanes_per_elem, int ops, int per_elem)
+constexpr int tri_epb(int lanes_per_elem, int ops, int per_elem, int budget = kTriLds)
 {
 	const int fill = 256 / lanes_per_elem > 0 ? 256 / lanes_per_elem : 1;
-	const int fit = (kTriLds - ops) / per_elem;
+	const int fit = (budget - ops) / per_elem;
 	return fill < fit ? fill : (fit > 0 ? fit : 1);
 }
 
@@ -58,17 +59,18 @@ constexpr int epb_trace(int P) { return tri_epb(3 * (P + 1), 3 * (P + 1) * tri_n
 constexpr int epb_rhs(int P)
 {
 	return tri_epb(tri_nc(P), 3 * tri_nc(P) * tri_np(P) + 3 * tri_np(P) * (P + 1),
-	               kNV * tri_np(P) + 2 * kNV * tri_nc(P) + 3 * kNV * (P + 1));
+	               kNV * tri_np(P) + 2 * kNV * tri_nc(P) + 3 * kNV * (P + 1) + 4 * tri_nc(P));
 }
 constexpr int epb_gv(int P)
 {
 	return tri_epb(tri_nc(P), 3 * tri_nc(P) * tri_np(P) + 3 * (P + 1) * tri_np(P) + tri_nc(P) + tri_np(P) * tri_np(P),
-	               kNV * tri_np(P) + 2 * kNV * tri_nc(P) + 2 * kNV * tri_np(P));
+	               kNV * tri_np(P) + 2 * kNV * tri_nc(P) + 2 * kNV * tri_np(P) + 4 * tri_nc(P), kTriLdsNs);
 }
 constexpr int epb_ns(int P)
 {
 	return tri_epb(tri_nc(P), 3 * tri_nc(P) * tri_np(P) + 3 * tri_np(P) * (P + 1) + tri_np(P) * tri_np(P),
-	               kNV * tri_np(P) + 9 * kNV * (P + 1) + 2 * kNV * tri_np(P) + 2 * kNV * tri_nc(P));
+	               kNV * tri_np(P) + 9 * kNV * (P + 1) + 2 * kNV * tri_np(P) + 2 * kNV * tri_nc(P) + 4 * tri_nc(P),
+	               kTriLdsNs);
 }
 
 struct TriArgs {
@@ -121,7 +123,7 @@ __global__ void __launch_bounds__(256) k_tri_rhs(TriArgs a)
 {
 	constexpr int Np = tri_np(P), Nc = tri_nc(P), N = P + 1, ZB = kNV * N, EPB = epb_rhs(P);
 	__shared__ double sI[Nc * Np], sDrW[Np * Nc], sDsW[Np * Nc], sTw0[3 * Np * N];
-	__shared__ double sU[EPB * kNV * Np], sFR[EPB * 2 * kNV * Nc], sZ[EPB * 3 * ZB];
+	__shared__ double sU[EPB * kNV * Np], sFR[EPB * 2 * kNV * Nc], sZ[EPB * 3 * ZB], sM[EPB * 4 * Nc];
 	const int t = threadIdx.x;
 	for (int i = t; i < Nc * Np; i += 256) {
 		sI[i] = a.ops->I[i];
@@ -132,8 +134,10 @@ __global__ void __launch_bounds__(256) k_tri_rhs(TriArgs a)
 	for (int e0 = blockIdx.x * EPB; e0 < a.E; e0 += gridDim.x * EPB) {
 		const int ne = min(EPB, a.E - e0);
 		__syncthreads();   // the operators are staged / the previous batch's buffers are read
+		// every global read of the batch at once (one round trip), contiguous
 		for (int i = t; i < ne * kNV * Np; i += 256) sU[i] = a.u[(size_t)e0 * kNV * Np + i];
 		for (int i = t; i < ne * 3 * ZB; i += 256) sZ[i] = a.z[(size_t)e0 * 3 * ZB + i];
+		for (int i = t; i < ne * 4 * Nc; i += 256) sM[i] = a.met[(size_t)e0 * 4 * Nc + i];
 		__syncthreads();
 		for (int i = t; i < ne * Nc; i += 256) {
 			const int sl = i / Nc, q = i - sl * Nc;
@@ -145,7 +149,7 @@ __global__ void __launch_bounds__(256) k_tri_rhs(TriArgs a)
 #pragma unroll
 				for (int v = 0; v < kNV; ++v) s[v] += c * U[v * Np + m];
 			}
-			const double* mp = a.met + (size_t)(e0 + sl) * 4 * Nc + q;
+			const double* mp = sM + sl * 4 * Nc + q;
 			const double m0 = mp[0], m1 = mp[Nc], m2 = mp[2 * Nc], m3 = mp[3 * Nc];
 			double f[2 * kNV];
 			euler_flux<double, 2>(s, f);
@@ -202,7 +206,7 @@ __global__ void __launch_bounds__(256) k_tri_gv(TriArgs a)
 {
 	constexpr int Np = tri_np(P), Nc = tri_nc(P), N = P + 1, F3 = 3 * N, K8 = 2 * kNV, EPB = epb_gv(P);
 	__shared__ double sI[Nc * Np], sDr[Nc * Np], sDs[Nc * Np], sIfc[F3 * Np], sW[Nc], sMinv[Np * Np];
-	__shared__ double sU[EPB * kNV * Np], sX[EPB * K8 * Nc], sB[EPB * K8 * Np];
+	__shared__ double sU[EPB * kNV * Np], sX[EPB * K8 * Nc], sB[EPB * K8 * Np], sM[EPB * 4 * Nc];
 	const int t = threadIdx.x;
 	for (int i = t; i < Nc * Np; i += 256) {
 		sI[i] = a.ops->I[i];
@@ -216,6 +220,7 @@ __global__ void __launch_bounds__(256) k_tri_gv(TriArgs a)
 		const int ne = min(EPB, a.E - e0);
 		__syncthreads();
 		for (int i = t; i < ne * kNV * Np; i += 256) sU[i] = a.u[(size_t)e0 * kNV * Np + i];
+		for (int i = t; i < ne * 4 * Nc; i += 256) sM[i] = a.met[(size_t)e0 * 4 * Nc + i];
 		__syncthreads();
 		for (int i = t; i < ne * Nc; i += 256) {   // W J grad s at the cubature
 			const int sl = i / Nc, q = i - sl * Nc;
@@ -230,7 +235,7 @@ __global__ void __launch_bounds__(256) k_tri_gv(TriArgs a)
 					ss[v] += cs * U[v * Np + m];
 				}
 			}
-			const double* mp = a.met + (size_t)(e0 + sl) * 4 * Nc + q;
+			const double* mp = sM + sl * 4 * Nc + q;
 			const double m0 = mp[0], m1 = mp[Nc], m2 = mp[2 * Nc], m3 = mp[3 * Nc], w = sW[q];
 			double* X = sX + sl * K8 * Nc;
 #pragma unroll
@@ -304,7 +309,7 @@ __global__ void __launch_bounds__(256) k_tri_ns(TriArgs a)
 {
 	constexpr int Np = tri_np(P), Nc = tri_nc(P), N = P + 1, K8 = 2 * kNV, ZB = 3 * kNV * N, EPB = epb_ns(P);
 	__shared__ double sI[Nc * Np], sDrW[Np * Nc], sDsW[Np * Nc], sTw0[3 * Np * N], sMinv[Np * Np];
-	__shared__ double sU[EPB * kNV * Np], sZ[EPB * 3 * ZB], sG[EPB * K8 * Np], sFR[EPB * K8 * Nc];
+	__shared__ double sU[EPB * kNV * Np], sZ[EPB * 3 * ZB], sG[EPB * K8 * Np], sFR[EPB * K8 * Nc], sM[EPB * 4 * Nc];
 	const int t = threadIdx.x;
 	for (int i = t; i < Nc * Np; i += 256) {
 		sI[i] = a.ops->I[i];
@@ -317,8 +322,11 @@ __global__ void __launch_bounds__(256) k_tri_ns(TriArgs a)
 	for (int e0 = blockIdx.x * EPB; e0 < a.E; e0 += gridDim.x * EPB) {
 		const int ne = min(EPB, a.E - e0);
 		__syncthreads();
+		// every global read of the batch at once (one round trip), contiguous; G starts as G_v
 		for (int i = t; i < ne * kNV * Np; i += 256) sU[i] = a.u[(size_t)e0 * kNV * Np + i];
 		for (int i = t; i < ne * 3 * ZB; i += 256) sZ[i] = a.z[(size_t)e0 * 3 * ZB + i];
+		for (int i = t; i < ne * K8 * Np; i += 256) sG[i] = a.gv[(size_t)e0 * K8 * Np + i];
+		for (int i = t; i < ne * 4 * Nc; i += 256) sM[i] = a.met[(size_t)e0 * 4 * Nc + i];
 		__syncthreads();
 		// the face lifts of the gradient: sum_f tw0_f y_f at the nodes (into sFR, free until the fluxes)
 		for (int i = t; i < ne * Np; i += 256) {
@@ -353,7 +361,7 @@ __global__ void __launch_bounds__(256) k_tri_ns(TriArgs a)
 			}
 			double* G = sG + sl * K8 * Np;
 #pragma unroll
-			for (int k = 0; k < K8; ++k) G[k * Np + m] = a.gv[((size_t)e * K8 + k) * Np + m] + sc * acc[k];
+			for (int k = 0; k < K8; ++k) G[k * Np + m] += sc * acc[k];
 		}
 		__syncthreads();
 		for (int i = t; i < ne * Nc; i += 256) {
@@ -369,7 +377,7 @@ __global__ void __launch_bounds__(256) k_tri_ns(TriArgs a)
 #pragma unroll
 				for (int k = 0; k < K8; ++k) gq[k] += c * G[k * Np + m];
 			}
-			const double* mp = a.met + (size_t)(e0 + sl) * 4 * Nc + q;
+			const double* mp = sM + sl * 4 * Nc + q;
 			const double m0 = mp[0], m1 = mp[Nc], m2 = mp[2 * Nc], m3 = mp[3 * Nc];
 			double f[K8], fv[K8];
 			euler_flux<double, 2>(s, f);
@@ -485,6 +493,14 @@ __device__ inline void lift_side(const MixOps<NF>& o, int Ns, int j, bool rev, b
 	}
 }
 
+// one side's values at its own face points, loaded into registers by lane j (j < Ns)
+__device__ inline void load_own(const MixArgs& a, int kind, int Ns, int lf, long long uoff, int el, int tg, int j,
+                                double* own)
+{
+#pragma unroll
+	for (int v = 0; v < kNV; ++v) own[v] = side_value(a, kind, Ns, lf, uoff, el, tg, v, j);
+}
+
 template <int NF, int FLUX>
 __global__ void __launch_bounds__(64) k_mixf(MixArgs a)
 {
@@ -492,25 +508,33 @@ __global__ void __launch_bounds__(64) k_mixf(MixArgs a)
 	__shared__ MixOps<NF> o;
 	__shared__ double sOwn[FPB][2][kNV * NF];
 	__shared__ double sG[FPB][NF * kNV];
-	stage_mix_ops<NF>(o, a.hops);
 	const int slot = threadIdx.x >> 3, j = threadIdx.x & 7;
 	const int fl = blockIdx.x * FPB + slot;
 	const bool valid = fl < a.count;
 	const int fi = a.start + (valid ? fl : 0);
 	const MixFace f = a.faces[fi];
 	const bool inter = f.bc == DGX_BC_INTERIOR;
+	// every global load of this lane first (one round trip), then the operators
+	double oL[kNV], oR[kNV], nj0 = 1.0, nj1 = 0.0;
+	if (valid && j < f.NL) load_own(a, f.kL, f.NL, f.lfL, f.uL, f.eL, f.tL, j, oL);
+	if (valid && inter && j < f.NR) load_own(a, f.kR, f.NR, f.lfR, f.uR, f.eR, f.tR, j, oR);
+	if (valid && j < NF) {
+		const double* nj = a.nJ + ((size_t)fi * kHpMaxN + j) * 2;
+		nj0 = nj[0];
+		nj1 = nj[1];
+	}
+	stage_mix_ops<NF>(o, a.hops);
 	if (valid && j < f.NL)
 #pragma unroll
-		for (int v = 0; v < kNV; ++v) sOwn[slot][0][v * NF + j] = side_value(a, f.kL, f.NL, f.lfL, f.uL, f.eL, f.tL, v, j);
+		for (int v = 0; v < kNV; ++v) sOwn[slot][0][v * NF + j] = oL[v];
 	if (valid && inter && j < f.NR)
 #pragma unroll
-		for (int v = 0; v < kNV; ++v) sOwn[slot][1][v * NF + j] = side_value(a, f.kR, f.NR, f.lfR, f.uR, f.eR, f.tR, v, j);
+		for (int v = 0; v < kNV; ++v) sOwn[slot][1][v * NF + j] = oR[v];
 	__syncthreads();
 	if (valid && j < NF) {
 		const int q = j;
-		const double* nj = a.nJ + ((size_t)fi * kHpMaxN + q) * 2;
-		const double Jf = sqrt(nj[0] * nj[0] + nj[1] * nj[1]);
-		const double nu[2] = { nj[0] / Jf, nj[1] / Jf };
+		const double Jf = sqrt(nj0 * nj0 + nj1 * nj1);
+		const double nu[2] = { nj0 / Jf, nj1 / Jf };
 		double sL[kNV], sR[kNV], nnf[kNV];
 		side_at_point<NF>(o, f.kL, f.NL, q, sOwn[slot][0], sL);
 		if (inter) side_at_point<NF>(o, f.kR, f.NR, f.rev ? NF - 1 - q : q, sOwn[slot][1], sR);
@@ -525,16 +549,17 @@ __global__ void __launch_bounds__(64) k_mixf(MixArgs a)
 	if (valid && inter && j < f.NR) lift_side<NF>(o, f.NR, j, f.rev != 0, true, sG[slot], a.z + f.zR);
 }
 
-// a side's face gradient at its own GL point j: trace G_v + eta (lift trace) y, with y = Y[k * NF + b]
+// what a side's face gradient at its own GL point j needs from memory: the trace of G_v there (t[k]) and its
+// row of the lift-trace operator (TRI: K_f row j times sc; QUAD: kappa in kr[0])
 template <int NF>
-__device__ inline void side_face_gradient(const MixArgs& a, int kind, int Ns, int lf, int el, int tg, int j,
-                                          const double* Y, double* h)
+__device__ inline void load_side_grad(const MixArgs& a, int kind, int Ns, int lf, int el, int tg, int j, double* t,
+                                      double* kr, double& sc)
 {
-	constexpr double eta = 1.01 * 4.0;
 	if (kind) {
-		const double* t = a.tgvt[tg] + ((size_t)el * 3 + lf) * 2 * kNV * Ns;
+		const double* tp = a.tgvt[tg] + ((size_t)el * 3 + lf) * 2 * kNV * Ns + j;
+#pragma unroll
+		for (int k = 0; k < 2 * kNV; ++k) t[k] = tp[k * Ns];
 		const double* K;
-		double sc;
 		if (a.tK[tg]) {
 			K = a.tK[tg] + ((size_t)el * 3 + lf) * Ns * Ns + j * Ns;
 			sc = 1.0;
@@ -543,19 +568,35 @@ __device__ inline void side_face_gradient(const MixArgs& a, int kind, int Ns, in
 			sc = a.tjinv[tg][el];
 		}
 #pragma unroll
+		for (int b = 0; b < NF; ++b) kr[b] = b < Ns ? K[b] : 0.0;
+		return;
+	}
+	const double* tp = a.qgvt[Ns] + ((size_t)el * 4 + lf) * 2 * kNV * Ns + j;
+#pragma unroll
+	for (int k = 0; k < 2 * kNV; ++k) t[k] = tp[k * Ns];
+	kr[0] = a.qkap[Ns][((size_t)el * 4 + lf) * Ns + j];
+	sc = 1.0;
+}
+
+// the face gradient h = trace G_v + eta (lift trace) y at own point j, y = Y[k * NF + b]
+template <int NF>
+__device__ inline void side_face_gradient(int kind, int Ns, int j, const double* t, const double* kr, double sc,
+                                          const double* Y, double* h)
+{
+	constexpr double eta = 1.01 * 4.0;
+	if (kind) {
+#pragma unroll
 		for (int k = 0; k < 2 * kNV; ++k) {
 			double acc = 0.0;
 #pragma unroll
 			for (int b = 0; b < NF; ++b)
-				if (b < Ns) acc += K[b] * Y[k * NF + b];
-			h[k] = t[k * Ns + j] + eta * (sc * acc);
+				if (b < Ns) acc += kr[b] * Y[k * NF + b];
+			h[k] = t[k] + eta * (sc * acc);
 		}
 		return;
 	}
-	const double* t = a.qgvt[Ns] + ((size_t)el * 4 + lf) * 2 * kNV * Ns;
-	const double kap = a.qkap[Ns][((size_t)el * 4 + lf) * Ns + j];
 #pragma unroll
-	for (int k = 0; k < 2 * kNV; ++k) h[k] = t[k * Ns + j] + eta * kap * Y[k * NF + j];
+	for (int k = 0; k < 2 * kNV; ++k) h[k] = t[k] + eta * kr[0] * Y[k * NF + j];
 }
 
 template <int NF, int FLUX>
@@ -566,30 +607,45 @@ __global__ void __launch_bounds__(64) k_mixf_ns(MixArgs a)
 	__shared__ double sOwn[FPB][2][kNV * NF];
 	__shared__ double sQ[FPB][NF * K8];        // w q at the face points [q][k], then w J_f F [q][v]
 	__shared__ double sY[FPB][2][K8 * NF];     // each side's y [k][j], then its face gradient h
-	stage_mix_ops<NF>(o, a.hops);
 	const int slot = threadIdx.x >> 3, j = threadIdx.x & 7;
 	const int fl = blockIdx.x * FPB + slot;
 	const bool valid = fl < a.count;
 	const int fi = a.start + (valid ? fl : 0);
 	const MixFace f = a.faces[fi];
+	// every global load of this lane first (one round trip), then the operators
+	double oL[kNV], oR[kNV], nj0 = 1.0, nj1 = 0.0;
+	double tL[K8], tR[K8], kL[NF], kR[NF], scL = 1.0, scR = 1.0;
+	if (valid && j < f.NL) {
+		load_own(a, f.kL, f.NL, f.lfL, f.uL, f.eL, f.tL, j, oL);
+		load_side_grad<NF>(a, f.kL, f.NL, f.lfL, f.eL, f.tL, j, tL, kL, scL);
+	}
+	if (valid && j < f.NR) {
+		load_own(a, f.kR, f.NR, f.lfR, f.uR, f.eR, f.tR, j, oR);
+		load_side_grad<NF>(a, f.kR, f.NR, f.lfR, f.eR, f.tR, j, tR, kR, scR);
+	}
+	if (valid && j < NF) {
+		const double* nj = a.nJ + ((size_t)fi * kHpMaxN + j) * 2;
+		nj0 = nj[0];
+		nj1 = nj[1];
+	}
+	stage_mix_ops<NF>(o, a.hops);
 	if (valid && j < f.NL)
 #pragma unroll
-		for (int v = 0; v < kNV; ++v) sOwn[slot][0][v * NF + j] = side_value(a, f.kL, f.NL, f.lfL, f.uL, f.eL, f.tL, v, j);
+		for (int v = 0; v < kNV; ++v) sOwn[slot][0][v * NF + j] = oL[v];
 	if (valid && j < f.NR)
 #pragma unroll
-		for (int v = 0; v < kNV; ++v) sOwn[slot][1][v * NF + j] = side_value(a, f.kR, f.NR, f.lfR, f.uR, f.eR, f.tR, v, j);
+		for (int v = 0; v < kNV; ++v) sOwn[slot][1][v * NF + j] = oR[v];
 	__syncthreads();
 	double sL[kNV], sR[kNV];
 	const int q = j, qr = f.rev ? NF - 1 - q : q;
 	if (valid && j < NF) {
 		side_at_point<NF>(o, f.kL, f.NL, q, sOwn[slot][0], sL);
 		side_at_point<NF>(o, f.kR, f.NR, qr, sOwn[slot][1], sR);
-		const double* nj = a.nJ + ((size_t)fi * kHpMaxN + q) * 2;
 #pragma unroll
 		for (int v = 0; v < kNV; ++v) {
 			const double dl = 0.5 * (sR[v] - sL[v]);
-			sQ[slot][q * K8 + v] = o.w[q] * (nj[0] * dl);
-			sQ[slot][q * K8 + kNV + v] = o.w[q] * (nj[1] * dl);
+			sQ[slot][q * K8 + v] = o.w[q] * (nj0 * dl);
+			sQ[slot][q * K8 + kNV + v] = o.w[q] * (nj1 * dl);
 		}
 	}
 	__syncthreads();
@@ -620,8 +676,8 @@ __global__ void __launch_bounds__(64) k_mixf_ns(MixArgs a)
 	}
 	__syncthreads();
 	double hL[K8], hR[K8];
-	if (valid && j < f.NL) side_face_gradient<NF>(a, f.kL, f.NL, f.lfL, f.eL, f.tL, j, sY[slot][0], hL);
-	if (valid && j < f.NR) side_face_gradient<NF>(a, f.kR, f.NR, f.lfR, f.eR, f.tR, j, sY[slot][1], hR);
+	if (valid && j < f.NL) side_face_gradient<NF>(f.kL, f.NL, j, tL, kL, scL, sY[slot][0], hL);
+	if (valid && j < f.NR) side_face_gradient<NF>(f.kR, f.NR, j, tR, kR, scR, sY[slot][1], hR);
 	__syncthreads();   // every y is read
 	if (valid && j < f.NL)
 #pragma unroll
@@ -645,9 +701,8 @@ __global__ void __launch_bounds__(64) k_mixf_ns(MixArgs a)
 			gL[k] = aL;
 			gR[k] = aR;
 		}
-		const double* nj = a.nJ + ((size_t)fi * kHpMaxN + q) * 2;
-		const double Jf = sqrt(nj[0] * nj[0] + nj[1] * nj[1]);
-		const double nu[2] = { nj[0] / Jf, nj[1] / Jf };
+		const double Jf = sqrt(nj0 * nj0 + nj1 * nj1);
+		const double nu[2] = { nj0 / Jf, nj1 / Jf };
 		double nnf[kNV], fvL[K8], fvR[K8];
 		if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<double, 2>(sL, sR, nu, nnf);
 		else numflux_roe<double, 2>(sL, sR, nu, nnf);
@@ -724,7 +779,7 @@ hipError_t launch_mix(const HpData& H, const Ctx& c, const double* u, bool ns)
 		const MixRun& run = H.mruns[i];
 		a.start = run.start;
 		a.count = run.count;
-		const hipStream_t st = H.lane[(i + 1) % HpData::kLanes];
+		const hipStream_t st = H.lane[H.mrun_lane[i]];
 		const dim3 grid((run.count + 7) / 8);
 #define DGX_MIX(NFV)                                                                                           \
 	case NFV:                                                                                                  \
