@@ -180,8 +180,17 @@ def test_fp32_residual_close_to_fp64(dev):
     ctx.compute_rhs_f32(u32.data_ptr(), r32.data_ptr())
     torch.cuda.synchronize()
     err = _rel(r32.double().cpu().numpy(), r64.cpu().numpy())
-    print(f"FP32 vs FP64 relative error (inf norm): {err:.3e}")
-    assert err < 1e-4
+    # the floor any FP32-state kernel has: the FP64 residual of the state rounded to FP32 (R is ~60x smaller than
+    # the volume and face terms that cancel in it); the kernel's own FP32 arithmetic adds rounding of the same
+    # size (measured 2.8x the floor on the 64^3 workload)
+    rr = torch.empty_like(ud)
+    ur = u32.double()
+    ctx.compute_rhs(ur.data_ptr(), rr.data_ptr())
+    torch.cuda.synchronize()
+    floor = _rel(rr.cpu().numpy(), r64.cpu().numpy())
+    print(f"FP32 vs FP64 relative error (inf norm): {err:.3e}, input rounding alone {floor:.3e}")
+    assert floor > 0.0
+    assert err < 6.0 * floor, (err, floor)
     ctx.close()
 
 
