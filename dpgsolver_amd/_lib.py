@@ -27,7 +27,8 @@ EXPORTS = [
     "dgx_set_rhs_grid", "dgx_autotune_rhs", "dgx_max_abs",
     "dgx_convert_f64_f32", "dgx_convert_f32_f64", "dgx_assemble_diag_blocks", "dgx_jacobian_vector",
     "dgx_assemble_face_blocks", "dgx_newton_operator", "dgx_block_lu_factor", "dgx_block_lu_solve", "dgx_block_inverse", "dgx_block_inv_apply", "dgx_cfl_dt", "dgx_mass_sub", "dgx_precond_apply", "dgx_solve_implicit",
-    "dgx_mesh_read_gmsh", "dgx_mesh_free", "dgx_restart_write_gmsh", "dgx_restart_read_solution",
+    "dgx_mesh_read_gmsh", "dgx_mesh_free", "dgx_mesh_read_gmsh_mixed", "dgx_mesh_mixed_free",
+    "dgx_restart_write_gmsh", "dgx_restart_read_solution",
     "dgx_mlp_create", "dgx_mlp_destroy", "dgx_mlp_sizes", "dgx_mlp_get_params", "dgx_mlp_set_params",
     "dgx_mlp_features", "dgx_mlp_loss_grad", "dgx_mlp_train", "dgx_mlp_predict", "dgx_attach_mlp",
 ]
@@ -49,6 +50,13 @@ class DgxConfig(C.Structure):
 
 class DgxHostMesh(C.Structure):
     _fields_ = [("d", C.c_int), ("p", C.c_int), ("E", C.c_int), ("F", C.c_int),
+                ("xyz", C.POINTER(C.c_double)), ("faces", C.POINTER(C.c_int)), ("v_to_v", C.POINTER(C.c_int)),
+                ("v_to_lf", C.POINTER(C.c_int)), ("v_to_lf_wp", C.POINTER(C.c_int)),
+                ("n_unsupported_bc", C.c_int), ("n_inverted", C.c_int)]
+
+
+class DgxHostMeshMixed(C.Structure):
+    _fields_ = [("E", C.c_int), ("F", C.c_int), ("p", C.c_int), ("type", C.POINTER(C.c_int)),
                 ("xyz", C.POINTER(C.c_double)), ("faces", C.POINTER(C.c_int)), ("v_to_v", C.POINTER(C.c_int)),
                 ("v_to_lf", C.POINTER(C.c_int)), ("v_to_lf_wp", C.POINTER(C.c_int)),
                 ("n_unsupported_bc", C.c_int), ("n_inverted", C.c_int)]
@@ -139,6 +147,7 @@ def load():
         "dgx_newton_operator": [vp, vp, vp, vp, vp],
         "dgx_precond_apply": [vp, C.c_int, vp, vp, vp, vp],
         "dgx_mesh_read_gmsh": [C.c_char_p, C.c_int, C.c_void_p],
+        "dgx_mesh_read_gmsh_mixed": [C.c_char_p, C.c_int, C.c_void_p],
         "dgx_restart_write_gmsh": [C.c_char_p, C.POINTER(DgxConfig), C.POINTER(DgxMesh), dp],
         "dgx_restart_read_solution": [C.c_char_p, C.POINTER(DgxConfig), C.c_int, dp],
         "dgx_mlp_create": [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong],
@@ -516,6 +525,33 @@ def read_gmsh(path: str, p: int) -> dict:
                "n_unsupported_bc": m.n_unsupported_bc, "n_inverted": m.n_inverted}
     finally:
         L.dgx_mesh_free(C.byref(m))
+    return out
+
+
+def read_gmsh_mixed(path: str, p: int) -> dict:
+    """dgx_mesh_read_gmsh_mixed: a gmsh 2.2 TRI / QUAD / mixed mesh, volumes in file order (type, xyz as a list
+    of (2, Np_e) arrays, faces, and v_to_v / v_to_lf / v_to_lf_wp with 4 entries per volume, a TRI's 4th -1)."""
+    L = load()
+    L.dgx_mesh_mixed_free.argtypes = [C.c_void_p]
+    L.dgx_mesh_mixed_free.restype = None
+    L.dgx_mesh_read_gmsh_mixed.argtypes = [C.c_char_p, C.c_int, C.c_void_p]
+    L.dgx_mesh_read_gmsh_mixed.restype = C.c_int
+    m = DgxHostMeshMixed()
+    check("dgx_mesh_read_gmsh_mixed", L.dgx_mesh_read_gmsh_mixed(str(path).encode(), p, C.byref(m)))
+    try:
+        typ = np.ctypeslib.as_array(m.type, (m.E,)).copy()
+        npe = np.where(typ == 1, (p + 1) * (p + 2) // 2, (p + 1) ** 2)
+        flat = np.ctypeslib.as_array(m.xyz, (int(2 * npe.sum()),)).copy()
+        off = np.concatenate([[0], np.cumsum(2 * npe)])
+        out = {"p": m.p, "E": m.E, "F": m.F, "type": typ,
+               "xyz": [flat[off[e]:off[e + 1]].reshape(2, -1) for e in range(m.E)],
+               "faces": np.ctypeslib.as_array(m.faces, (m.F, 6)).copy(),
+               "v_to_v": np.ctypeslib.as_array(m.v_to_v, (m.E, 4)).copy(),
+               "v_to_lf": np.ctypeslib.as_array(m.v_to_lf, (m.E, 4)).copy(),
+               "v_to_lf_wp": np.ctypeslib.as_array(m.v_to_lf_wp, (m.E, 4)).copy(),
+               "n_unsupported_bc": m.n_unsupported_bc, "n_inverted": m.n_inverted}
+    finally:
+        L.dgx_mesh_mixed_free(C.byref(m))
     return out
 
 

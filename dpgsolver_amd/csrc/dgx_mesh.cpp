@@ -1,10 +1,11 @@
-// dgx_mesh.cpp -- gmsh 2.2 ASCII mesh ingest for 2-D QUAD meshes (host only, no HIP calls).
+// dgx_mesh.cpp -- gmsh 2.2 ASCII mesh ingest for 2-D QUAD, TRI and mixed meshes (host only, no HIP calls).
 //
 // Follows the reference's reader and connectivity (src/simulation/mesh/mesh_readers.c,
 // mesh_connectivity.c:285-465, mesh_periodic.c):
 //  * volumes are the 2-D elements in file order; gmsh's counter-clockwise QUAD vertices (a, b, c, d) are
-//    taken in tensor-product order (a, b, d, c), r fastest;
-//  * local faces 0..3 are r = -1, r = +1, s = -1, s = +1 with their vertices in increasing face coordinate;
+//    taken in tensor-product order (a, b, d, c), r fastest; TRI vertices (a, b, c) as they are;
+//  * QUAD local faces 0..3 are r = -1, r = +1, s = -1, s = +1 with their vertices in increasing face coordinate;
+//    TRI local face f is the edge opposite vertex f, vertices (1, 2), (0, 2), (0, 1);
 //  * periodic faces are the boundary line elements with a periodic tag (x051..x056 translation, x061/x062
 //    reflected in y); masters (x051, x053, x055, x061) and slaves (x052, x054, x056, x062) are paired in the
 //    order of their centroids along the face (the slave's reflected for x062), and a volume face lying on a
@@ -12,7 +13,8 @@
 //  * faces match when their vertex sets then agree; v_to_lf_wp keeps only matches whose raw vertex sets agree
 //    and holds the periodic boundary tags elsewhere;
 //  * unmatched faces take the physical tag of the boundary line element with the same vertices (add_bc_info).
-// The geometry is the straight-sided (bilinear) map of the vertices evaluated at the GLL nodes of degree p.
+// The geometry is the straight-sided map of the vertices (QUAD bilinear, TRI affine) evaluated at the element's
+// solution nodes of degree p (QUAD: GLL tensor nodes; TRI: the WSH nodes of dgx_reference_nodes).
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -73,9 +75,16 @@ int bc_code(int tag)
 
 }  // namespace
 
-extern "C" {
 
-static dgx_status read_gmsh(const char* path, int p, dgx_host_mesh* out)
+// what the readers return, before it is copied into the C structures
+struct ReadOut {
+	std::vector<int> type;                      // per volume: 0 QUAD, 1 TRI
+	std::vector<double> xyz;                    // per volume 2 x Np, concatenated
+	std::vector<int> faces, v2v, v2lf, v2lfwp;  // v2*: 4 per volume (a TRI's 4th is -1)
+	int n_unsupported = 0, n_inverted = 0;
+};
+
+static dgx_status read_any(const char* path, int p, bool allow_tri, ReadOut& R)
 {
 	std::ifstream in(path);
 	if (!in) return DGX_ERR_ARG;
@@ -128,7 +137,7 @@ static dgx_status read_gmsh(const char* path, int p, dgx_host_mesh* out)
 	}
 	if (nodes.empty() || elems.empty()) return DGX_ERR_ARG;
 	for (const Elem& e : elems)
-		if (e.type == 2 || e.type == 4 || e.type == 5) return DGX_ERR_UNSUPPORTED;   // TRI / 3-D volumes
+		if ((e.type == 2 && !allow_tri) || e.type == 4 || e.type == 5) return DGX_ERR_UNSUPPORTED;   // TRI / 3-D
 	auto key = [](std::array<int, 2> x) { return std::make_pair(std::min(x[0], x[1]), std::max(x[0], x[1])); };
 
 	// periodic boundary faces: pair masters and slaves by (direction, centroid along the face)
@@ -173,25 +182,37 @@ static dgx_status read_gmsh(const char* path, int p, dgx_host_mesh* out)
 		slave_map[key(S.v)] = mp;
 	}
 
-	// volumes and their local faces
-	std::vector<std::array<int, 4>> vol;   // tensor-product vertex order
-	for (const Elem& e : elems)
-		if (e.type == 3) vol.push_back({ e.nodes[0], e.nodes[1], e.nodes[3], e.nodes[2] });
+	// volumes and their local faces (four face slots per volume; a TRI uses three)
+	std::vector<std::array<int, 4>> vol;   // QUAD: tensor-product vertex order; TRI: (a, b, c, -1)
+	std::vector<int> vtype;
+	for (const Elem& e : elems) {
+		if (e.type == 3) {
+			vol.push_back({ e.nodes[0], e.nodes[1], e.nodes[3], e.nodes[2] });
+			vtype.push_back(0);
+		} else if (e.type == 2) {
+			vol.push_back({ e.nodes[0], e.nodes[1], e.nodes[2], -1 });
+			vtype.push_back(1);
+		}
+	}
 	const int E = (int)vol.size();
 	if (E == 0) return DGX_ERR_UNSUPPORTED;
-	static const int fv[4][2] = { { 0, 2 }, { 1, 3 }, { 0, 1 }, { 2, 3 } };
+	static const int fvq[4][2] = { { 0, 2 }, { 1, 3 }, { 0, 1 }, { 2, 3 } };
+	static const int fvt[3][2] = { { 1, 2 }, { 0, 2 }, { 0, 1 } };
+	auto nfaces = [&](int v) { return vtype[v] ? 3 : 4; };
 	const int nf = 4 * E;
-	std::vector<std::array<int, 2>> raw(nf), cor(nf);
+	std::vector<std::array<int, 2>> raw(nf, { -1, -1 }), cor(nf, { -1, -1 });
 	for (int v = 0; v < E; ++v)
-		for (int lf = 0; lf < 4; ++lf) {
-			const int a = vol[v][fv[lf][0]], b = vol[v][fv[lf][1]];
+		for (int lf = 0; lf < nfaces(v); ++lf) {
+			const int* fv = vtype[v] ? fvt[lf] : fvq[lf];
+			const int a = vol[v][fv[0]], b = vol[v][fv[1]];
 			raw[4 * v + lf] = { a, b };
 			auto it = slave_map.find(key({ a, b }));
 			if (it == slave_map.end()) cor[4 * v + lf] = { a, b };
 			else cor[4 * v + lf] = { it->second.at(a), it->second.at(b) };
 		}
 	std::map<std::pair<int, int>, std::vector<int>> groups;
-	for (int f = 0; f < nf; ++f) groups[key(cor[f])].push_back(f);
+	for (int f = 0; f < nf; ++f)
+		if (f % 4 < nfaces(f / 4)) groups[key(cor[f])].push_back(f);
 	std::map<std::pair<int, int>, int> btag;   // boundary line elements by raw vertex pair
 	for (const Elem& e : elems)
 		if (e.type == 1 && !e.tags.empty()) btag[key({ e.nodes[0], e.nodes[1] })] = e.tags[0];
@@ -230,59 +251,114 @@ static dgx_status read_gmsh(const char* path, int p, dgx_host_mesh* out)
 	}
 	// periodic boundary tags in v_to_lf_wp (add_bc_info with include_periodic)
 	for (int f = 0; f < nf; ++f)
-		if (v2lfwp[f] == -1) {
+		if (f % 4 < nfaces(f / 4) && v2lfwp[f] == -1) {
 			auto it = btag.find(key(raw[f]));
 			if (it != btag.end()) v2lfwp[f] = it->second;
 		}
 
-	// straight-sided geometry at the GLL nodes of degree p
-	const int N = p + 1, Np = N * N;
+	// straight-sided geometry at the solution nodes of degree p
+	const int N = p + 1, NpQ = N * N, NpT = N * (N + 1) / 2;
 	std::vector<double> r;
 	gll_nodes(N, r);
-	double* xyz = static_cast<double*>(std::malloc(sizeof(double) * (size_t)E * 2 * Np));
-	int* fout = static_cast<int*>(std::malloc(sizeof(int) * faces.size()));
-	int* a = static_cast<int*>(std::malloc(sizeof(int) * nf));
-	int* b = static_cast<int*>(std::malloc(sizeof(int) * nf));
-	int* c = static_cast<int*>(std::malloc(sizeof(int) * nf));
-	if (!xyz || !fout || !a || !b || !c) {
-		std::free(xyz); std::free(fout); std::free(a); std::free(b); std::free(c);
-		return DGX_ERR_ARG;
-	}
+	std::vector<double> rst(2 * (size_t)NpT);
+	if (allow_tri && dgx_reference_nodes(1, p, rst.data(), nullptr, nullptr) != DGX_OK) return DGX_ERR_UNSUPPORTED;
 	int n_inverted = 0;
 	for (int v = 0; v < E; ++v) {
-		double X[4][2];
-		for (int k = 0; k < 4; ++k) {
+		double X[4][2] = {};
+		for (int k = 0; k < (vtype[v] ? 3 : 4); ++k) {
 			X[k][0] = nodes[vol[v][k]][0];
 			X[k][1] = nodes[vol[v][k]][1];
 		}
-		for (int j = 0; j < N; ++j)
-			for (int i = 0; i < N; ++i) {
-				const double s0 = 0.5 * (1 - r[i]), s1 = 0.5 * (1 + r[i]), t0 = 0.5 * (1 - r[j]), t1 = 0.5 * (1 + r[j]);
+		const size_t base = R.xyz.size();
+		if (vtype[v] == 0) {
+			R.xyz.resize(base + 2 * (size_t)NpQ);
+			for (int j2 = 0; j2 < N; ++j2)
+				for (int i2 = 0; i2 < N; ++i2) {
+					const double s0 = 0.5 * (1 - r[i2]), s1 = 0.5 * (1 + r[i2]), t0 = 0.5 * (1 - r[j2]), t1 = 0.5 * (1 + r[j2]);
+					for (int dd = 0; dd < 2; ++dd)
+						R.xyz[base + (size_t)dd * NpQ + i2 + N * j2] =
+						    s0 * t0 * X[0][dd] + s1 * t0 * X[1][dd] + s0 * t1 * X[2][dd] + s1 * t1 * X[3][dd];
+				}
+			// orientation: det J at the first vertex of the bilinear map
+			const double j00 = X[1][0] - X[0][0], j01 = X[2][0] - X[0][0], j10 = X[1][1] - X[0][1], j11 = X[2][1] - X[0][1];
+			if (j00 * j11 - j01 * j10 <= 0.0) ++n_inverted;
+		} else {
+			// barycentric coordinates on the reference triangle (-1, -1/sqrt3), (1, -1/sqrt3), (0, 2/sqrt3)
+			R.xyz.resize(base + 2 * (size_t)NpT);
+			const double s3 = std::sqrt(3.0);
+			for (int k = 0; k < NpT; ++k) {
+				const double l2 = (rst[2 * k + 1] + 1.0 / s3) / s3;
+				const double l1 = 0.5 * ((rst[2 * k] + 1.0) - l2);
+				const double l0 = 1.0 - l1 - l2;
 				for (int dd = 0; dd < 2; ++dd)
-					xyz[((size_t)v * 2 + dd) * Np + i + N * j] =
-					    s0 * t0 * X[0][dd] + s1 * t0 * X[1][dd] + s0 * t1 * X[2][dd] + s1 * t1 * X[3][dd];
+					R.xyz[base + (size_t)dd * NpT + k] = l0 * X[0][dd] + l1 * X[1][dd] + l2 * X[2][dd];
 			}
-		// orientation: det J at the first vertex of the bilinear map
-		const double j00 = X[1][0] - X[0][0], j01 = X[2][0] - X[0][0], j10 = X[1][1] - X[0][1], j11 = X[2][1] - X[0][1];
-		if (j00 * j11 - j01 * j10 <= 0.0) ++n_inverted;
+			const double cr = (X[1][0] - X[0][0]) * (X[2][1] - X[0][1]) - (X[1][1] - X[0][1]) * (X[2][0] - X[0][0]);
+			if (cr <= 0.0) ++n_inverted;
+		}
 	}
-	std::copy(faces.begin(), faces.end(), fout);
-	std::copy(v2v.begin(), v2v.end(), a);
-	std::copy(v2lf.begin(), v2lf.end(), b);
-	std::copy(v2lfwp.begin(), v2lfwp.end(), c);
+	R.type = vtype;
+	R.faces = faces;
+	R.v2v = v2v;
+	R.v2lf = v2lf;
+	R.v2lfwp = v2lfwp;
+	for (int v = 0; v < E; ++v)
+		if (vtype[v]) R.v2v[4 * v + 3] = R.v2lf[4 * v + 3] = R.v2lfwp[4 * v + 3] = -1;
+	R.n_unsupported = n_unsupported;
+	R.n_inverted = n_inverted;
+	return DGX_OK;
+}
+
+template <typename T>
+static T* dup(const std::vector<T>& v)
+{
+	T* out = static_cast<T*>(std::malloc(sizeof(T) * std::max<size_t>(v.size(), 1)));
+	if (out) std::copy(v.begin(), v.end(), out);
+	return out;
+}
+
+static dgx_status read_gmsh(const char* path, int p, dgx_host_mesh* out)
+{
+	ReadOut R;
+	const dgx_status st = read_any(path, p, false, R);
+	if (st != DGX_OK) return st;
+	const int E = (int)R.type.size();
 	out->d = 2;
 	out->p = p;
 	out->E = E;
-	out->F = (int)(faces.size() / 6);
-	out->xyz = xyz;
-	out->faces = fout;
-	out->v_to_v = a;
-	out->v_to_lf = b;
-	out->v_to_lf_wp = c;
-	out->n_unsupported_bc = n_unsupported;
-	out->n_inverted = n_inverted;
+	out->F = (int)(R.faces.size() / 6);
+	out->xyz = dup(R.xyz);
+	out->faces = dup(R.faces);
+	out->v_to_v = dup(R.v2v);
+	out->v_to_lf = dup(R.v2lf);
+	out->v_to_lf_wp = dup(R.v2lfwp);
+	out->n_unsupported_bc = R.n_unsupported;
+	out->n_inverted = R.n_inverted;
+	if (!out->xyz || !out->faces || !out->v_to_v || !out->v_to_lf || !out->v_to_lf_wp) return DGX_ERR_ARG;
 	return DGX_OK;
 }
+
+static dgx_status read_gmsh_mixed(const char* path, int p, dgx_host_mesh_mixed* out)
+{
+	ReadOut R;
+	const dgx_status st = read_any(path, p, true, R);
+	if (st != DGX_OK) return st;
+	out->E = (int)R.type.size();
+	out->F = (int)(R.faces.size() / 6);
+	out->p = p;
+	out->type = dup(R.type);
+	out->xyz = dup(R.xyz);
+	out->faces = dup(R.faces);
+	out->v_to_v = dup(R.v2v);
+	out->v_to_lf = dup(R.v2lf);
+	out->v_to_lf_wp = dup(R.v2lfwp);
+	out->n_unsupported_bc = R.n_unsupported;
+	out->n_inverted = R.n_inverted;
+	if (!out->type || !out->xyz || !out->faces || !out->v_to_v || !out->v_to_lf || !out->v_to_lf_wp) return DGX_ERR_ARG;
+	return DGX_OK;
+}
+
+extern "C" {
 
 void dgx_mesh_free(dgx_host_mesh* m)
 {
@@ -299,12 +375,40 @@ dgx_status dgx_mesh_read_gmsh(const char* path, int p, dgx_host_mesh* out)
 {
 	if (!path || !out || p < 1 || p > 5) return DGX_ERR_ARG;
 	std::memset(out, 0, sizeof(*out));
+	dgx_status st;
 	try {
-		return read_gmsh(path, p, out);
+		st = read_gmsh(path, p, out);
 	} catch (...) {   // a malformed file (std::out_of_range, std::bad_alloc): a status code, never an abort
-		dgx_mesh_free(out);
-		return DGX_ERR_ARG;
+		st = DGX_ERR_ARG;
 	}
+	if (st != DGX_OK) dgx_mesh_free(out);
+	return st;
+}
+
+void dgx_mesh_mixed_free(dgx_host_mesh_mixed* m)
+{
+	if (!m) return;
+	std::free(m->type);
+	std::free(m->xyz);
+	std::free(m->faces);
+	std::free(m->v_to_v);
+	std::free(m->v_to_lf);
+	std::free(m->v_to_lf_wp);
+	std::memset(m, 0, sizeof(*m));
+}
+
+dgx_status dgx_mesh_read_gmsh_mixed(const char* path, int p, dgx_host_mesh_mixed* out)
+{
+	if (!path || !out || p < 1 || p > 5) return DGX_ERR_ARG;
+	std::memset(out, 0, sizeof(*out));
+	dgx_status st;
+	try {
+		st = read_gmsh_mixed(path, p, out);
+	} catch (...) {
+		st = DGX_ERR_ARG;
+	}
+	if (st != DGX_OK) dgx_mesh_mixed_free(out);
+	return st;
 }
 
 }  // extern "C"

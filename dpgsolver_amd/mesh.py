@@ -690,6 +690,25 @@ class MixedMesh:
         return np.ascontiguousarray(np.concatenate([x.ravel() for x in self.xyz]))
 
 
+def mixed_mesh_from_gmsh(path: str, p: int):
+    """A gmsh 2.2 TRI / QUAD / mixed mesh (dgx_mesh_read_gmsh_mixed) as a MixedMesh of order p sorted by type
+    (QUAD first, as dgx_upload_mesh_mixed wants), and the file index of each sorted element."""
+    from . import _lib
+
+    m = _lib.read_gmsh_mixed(path, p)
+    kind = np.asarray(m["type"], np.int32)
+    order = np.argsort(kind, kind="stable")              # sorted position -> file index
+    where = np.empty_like(order)
+    where[order] = np.arange(len(order))                 # file index -> sorted position
+    faces = np.array(m["faces"], np.int32)
+    faces[:, 0] = where[faces[:, 0]]
+    inter = faces[:, 2] >= 0
+    faces[inter, 2] = where[faces[inter, 2]]
+    mesh = MixedMesh(kind=kind[order], p=np.full(len(order), p, np.int32), xyz=[m["xyz"][e] for e in order],
+                     faces=faces)
+    return mesh, order
+
+
 def structured_mixed_mesh(n: tuple, split, p_elem, lo=(0.0, 0.0), hi=(1.0, 1.0), periodic=(True, True),
                           curve=None, bc_low=None, bc_high=None) -> MixedMesh:
     """Structured nx x ny grid (i fastest) whose cells with split[cell] are cut along the (i, j)-(i+1, j+1)

@@ -166,6 +166,25 @@ dgx_status dgx_tri_rule(int kind, int q, double* rst, double* w, int* n);
  * the straight-sided map at the GLL nodes of degree p.  TRI or 3-D volumes: DGX_ERR_UNSUPPORTED.  Host only. */
 dgx_status dgx_mesh_read_gmsh(const char* path, int p, dgx_host_mesh* out);
 void dgx_mesh_free(dgx_host_mesh* m);
+/* A gmsh 2.2 mesh of TRI and QUAD elements (the reference's mixed 2-D meshes, mesh_readers.c,
+ * mesh_connectivity.c:285-465): the volumes in file order.  TRI local face f is the edge opposite vertex f,
+ * its vertices (1, 2), (0, 2), (0, 1) in increasing face coordinate; a TRI's geometry is the affine map of its
+ * vertices at the WSH solution nodes.  v_to_v / v_to_lf / v_to_lf_wp hold 4 entries per volume (a TRI's 4th is
+ * -1).  dgx_upload_mesh_mixed wants the volumes sorted by (type, p): reorder them and renumber the face table
+ * first.  Host only; release with dgx_mesh_mixed_free. */
+typedef struct dgx_host_mesh_mixed {
+	int E, F, p;
+	int* type;            /* E: 0 QUAD, 1 TRI */
+	double* xyz;          /* per volume 2 x Np (x then y at its solution nodes), concatenated */
+	int* faces;           /* F x 6 {eL, fL, eR, fR, ord, bc} */
+	int* v_to_v;          /* E x 4 */
+	int* v_to_lf;         /* E x 4 */
+	int* v_to_lf_wp;      /* E x 4 */
+	int n_unsupported_bc;
+	int n_inverted;
+} dgx_host_mesh_mixed;
+dgx_status dgx_mesh_read_gmsh_mixed(const char* path, int p, dgx_host_mesh_mixed* out);
+void dgx_mesh_mixed_free(dgx_host_mesh_mixed* m);
 /* The reference's restart file (restart_writers.c:57-171, 301-412): gmsh 2.2 vertices, boundary faces tagged
  * 10101 (Riemann) / 10102 (slip wall), volumes, and $SolutionCoefficients with each volume's Bezier
  * coefficients (dof fastest).  u: HOST state in the dgx_compute_rhs layout.  Host only. */

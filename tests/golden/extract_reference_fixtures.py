@@ -10,12 +10,15 @@ Reads (data only, no reference code is run):
   input/testing/unit/nodes/nodes_si.data          (2-D simplex nodes, used by src/testing/unit/test_unit_nodes.c)
   input/nodes/tri/WSH{1..5}.txt, WV{2..10}.txt    (the symmetric TRI node and cubature tables the reference reads
                                                   at run time, src/element/nodes.c:628-852)
+  input/testing/integration/mesh/*.msh(.data)     (the 2-D test meshes and the connectivity the reference's mesh
+                                                  integration test compares, copied as data into tests/golden/mesh)
 under the DPGSolver checkout given as argv[1] (default /root/reference).
 """
 import glob
 import json
 import os
 import re
+import shutil
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -208,8 +211,12 @@ def main():
     with open(os.path.join(HERE, "tri_nodes.json"), "w") as f:
         json.dump({"source": "input/testing/unit/nodes/nodes_si.data (golden), input/nodes/tri/*.txt (tables)",
                    "golden": si, "tables": tables}, f, indent=0)
+    meshes = []
+    for path in sorted(glob.glob(os.path.join(ref, "input", "testing", "integration", "mesh", "*2d*.msh*"))):
+        shutil.copyfile(path, os.path.join(HERE, "mesh", os.path.basename(path)))
+        meshes.append(os.path.basename(path))
     print("nodes:", sorted(n), "operators:", sorted(o), "corr:", sorted(c), "fe:", sorted(fe), "si:", sorted(si),
-          "tri tables:", sorted(tables))
+          "tri tables:", sorted(tables), "meshes:", meshes)
 
 
 if __name__ == "__main__":
