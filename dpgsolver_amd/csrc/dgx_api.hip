@@ -372,10 +372,11 @@ dgx_status hp_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned long 
 {
 	dgx::HpData& H = *c.hp;
 	DGX_TRY(dgx::hp_fork(H, c.stream));
-	if (!H.tri.empty() && mode != dgx::MODE_RHS) {
+	if (!H.tri.empty() && mode != dgx::MODE_RHS && mode != dgx::MODE_RHS_MINV) {
 		(void)dgx::hp_join(H, c.stream);
-		return DGX_ERR_UNSUPPORTED;   // TRI elements: the residual without M^-1 only
+		return DGX_ERR_UNSUPPORTED;
 	}
+	const bool minv = (mode == dgx::MODE_RHS_MINV);
 	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) {
 		const double mu = c.cfg.mu, Pr = c.cfg.Pr;
 		DGX_TRY(on_groups(c, [&](size_t g, Ctx& s) { return dgx::launch_rhs_ns_phase(s, u + H.u_off[g], r + H.u_off[g], 1); }));
@@ -385,8 +386,10 @@ dgx_status hp_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned long 
 		if (H.tri.empty()) DGX_TRY(dgx::launch_hp_faces64(c, u));
 		else DGX_TRY(dgx::launch_mix_faces_ns64(H, c, u, H.lane[dgx::HpData::kLanes - 1]));
 		DGX_TRY(hp_barrier(H, c.stream));
-		for (size_t t = 0; t < H.tri.size(); ++t)
+		for (size_t t = 0; t < H.tri.size(); ++t) {
 			DGX_TRY(dgx::launch_tri_ns_phase64(H.tri[t], H.tri_ops64, u, r, H.z64, mu, Pr, 2, H.lane[H.tri_lane[t]]));
+			if (minv) DGX_TRY(dgx::launch_tri_minv64(H.tri[t], H.tri_ops64, r, dmax, H.lane[H.tri_lane[t]]));
+		}
 		DGX_TRY(on_groups(c, [&](size_t g, Ctx& s) -> hipError_t {
 			double* rg = r + H.u_off[g];
 			hipError_t e = dgx::launch_rhs_ns_phase(s, u + H.u_off[g], rg, 2);
@@ -406,8 +409,10 @@ dgx_status hp_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned long 
 		}
 		DGX_TRY(hp_barrier(H, c.stream));
 		DGX_TRY(on_groups(c, [&](size_t g, Ctx& s) { return dgx::launch_rhs64(s, u + H.u_off[g], r + H.u_off[g], mode, dmax); }));
-		for (size_t t = 0; t < H.tri.size(); ++t)
+		for (size_t t = 0; t < H.tri.size(); ++t) {
 			DGX_TRY(dgx::launch_tri_rhs64(H.tri[t], H.tri_ops64, u, r, H.z64, H.lane[H.tri_lane[t]]));
+			if (minv) DGX_TRY(dgx::launch_tri_minv64(H.tri[t], H.tri_ops64, r, dmax, H.lane[H.tri_lane[t]]));
+		}
 	}
 	DGX_TRY(dgx::hp_join(H, c.stream));
 	return DGX_OK;
@@ -753,11 +758,11 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 			DGX_TRY(hipMalloc(&hg.tr, sizeof(double) * (size_t)tg.E * 3 * nv * N));
 			DGX_TRY(hipMemcpy(hg.met, met.data(), sizeof(double) * met.size(), hipMemcpyHostToDevice));
 			DGX_TRY(hipMemcpy(hg.detJ, dj.data(), sizeof(double) * dj.size(), hipMemcpyHostToDevice));
-			if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) {
-				// BR2: M^-1 and the face-trace operators K_f = Ifc_f M^-1 tw0_f of the lifts.  An affine element
-				// (det J constant at the cubature) has the reference ones divided by det J; if every element of
-				// the group is affine only 1 / det J is stored, else each element's own matrices.  The volume
-				// gradient and its face traces are written per evaluation.
+			{
+				// M^-1 (the explicit path; BR2: also the face-trace operators K_f = Ifc_f M^-1 tw0_f of the
+				// lifts).  An affine element (det J constant at the cubature) has the reference ones divided by
+				// det J; if every element of the group is affine only 1 / det J is stored, else each element's
+				// own matrices.  The BR2 volume gradient and its face traces are written per evaluation.
 				bool affine = true;
 				std::vector<double> jinv(tg.E);
 				for (int k = 0; k < tg.E && affine; ++k) {
@@ -785,8 +790,10 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 					DGX_TRY(hipMemcpy(hg.minv, mi.data(), sizeof(double) * mi.size(), hipMemcpyHostToDevice));
 					DGX_TRY(hipMemcpy(hg.K, K.data(), sizeof(double) * K.size(), hipMemcpyHostToDevice));
 				}
-				DGX_TRY(hipMalloc(&hg.gv, sizeof(double) * (size_t)tg.E * 2 * nv * Np));
-				DGX_TRY(hipMalloc(&hg.gvt, sizeof(double) * (size_t)tg.E * 3 * 2 * nv * N));
+				if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) {
+					DGX_TRY(hipMalloc(&hg.gv, sizeof(double) * (size_t)tg.E * 2 * nv * Np));
+					DGX_TRY(hipMalloc(&hg.gvt, sizeof(double) * (size_t)tg.E * 3 * 2 * nv * N));
+				}
 			}
 		}
 	}
@@ -987,6 +994,7 @@ dgx_status dgx_ssp_rk3_step(dgx_ctx* h, double* u, double dt, int positivity, do
 	if (!c.u0) DGX_TRY(hipMalloc(&c.u0, sizeof(double) * n));
 	if (!c.rtmp) DGX_TRY(hipMalloc(&c.rtmp, sizeof(double) * n));
 	const bool pos = positivity && c.cfg.pde != DGX_PDE_ADVECTION;
+	if (pos && c.hp && !c.hp->tri.empty()) return DGX_ERR_UNSUPPORTED;   // the limiter is built for QUAD/HEX only
 	if (pos) DGX_TRY(dgx::launch_fill_i32(c.stream, c.dstatus, 1, 0));
 	for (int rk = 0; rk < 3; ++rk) {
 		DGX_TRY(dgx::launch_fill_u64(c.stream, c.dmax, 1, 0ull));
@@ -1000,6 +1008,10 @@ dgx_status dgx_ssp_rk3_step(dgx_ctx* h, double* u, double dt, int positivity, do
 				if (!s.u0) DGX_TRY(hipMalloc(&s.u0, sizeof(double) * ndof(s)));
 				DGX_TRY(dgx::launch_rk_stage(s, u + H.u_off[g], c.rtmp + H.u_off[g], dt, rk));
 				if (pos) DGX_TRY(dgx::launch_positivity(s, u + H.u_off[g], c.dstatus));
+			}
+			for (const dgx::TriGroup& t : H.tri) {   // TRI groups: the stage copy in the mesh-wide buffer
+				const size_t nt = (size_t)t.E * c.nv * ((t.p + 1) * (t.p + 2) / 2);
+				DGX_TRY(dgx::launch_rk_stage_n(c.stream, u + t.u_off, c.u0 + t.u_off, c.rtmp + t.u_off, dt, rk, nt));
 			}
 			continue;
 		}

@@ -296,6 +296,11 @@ hipError_t launch_convert_back(const float* src, double* dst, size_t n, hipStrea
 hipError_t launch_rk_stage(Ctx& c, double* u, const double* r, double dt, int stage);
 // the same with the stage-0 copy in u0 (E*Np*nv doubles) instead of c.u0
 hipError_t launch_rk_stage_buf(Ctx& c, double* u, double* u0, const double* r, double dt, int stage);
+// the same over n values on stream st (a TRI group's range of a mixed mesh)
+hipError_t launch_rk_stage_n(hipStream_t st, double* u, double* u0, const double* r, double dt, int stage, size_t n);
+// r <- M^-1 r on one TRI group (and max |M^-1 r| into dmax when dmax is not NULL)
+hipError_t launch_tri_minv64(const TriGroup& g, const TriOps<double>* ops, double* r, unsigned long long* dmax,
+                             hipStream_t st);
 // dgx_noalloc.hip: the device address of the static scratch array on device `dev` (the current device);
 // looked up once per device, then cached
 hipError_t noalloc_scratch(int dev, void** ptr);

@@ -404,6 +404,12 @@ hipError_t launch_rk_stage(Ctx& c, double* u, const double* r, double dt, int st
 {
 	return launch_rk_stage_buf(c, u, c.u0, r, dt, stage);
 }
+hipError_t launch_rk_stage_n(hipStream_t st, double* u, double* u0, const double* r, double dt, int stage, size_t n)
+{
+	const int grid = (int)((n + 255) / 256 < 8192 ? (n + 255) / 256 : 8192);
+	if (grid) hipLaunchKernelGGL(k_rk_stage, dim3(grid), dim3(256), 0, st, u, u0, r, dt, stage, n);
+	return hipGetLastError();
+}
 
 // ------------------------------------------------------------------------------- mass blocks
 // blocks[e] -= scale * (M_e (x) I_nv), M_e[i][j] = sum_q Ibar[q][i] w_q J_q Ibar[q][j]

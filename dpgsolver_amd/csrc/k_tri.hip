@@ -417,6 +417,56 @@ __global__ void __launch_bounds__(256) k_tri_ns(TriArgs a)
 	}
 }
 
+// r <- M^-1 r per element (scale_rhs_by_m_inv_std, solve_dg.c:265-272), and max |M^-1 r| into dmax (the bits of a
+// non-negative double order as the double does; NaN propagates as the largest)
+constexpr int epb_minv(int P) { return tri_epb(tri_np(P), tri_np(P) * tri_np(P), kNV * tri_np(P)); }
+
+template <int P>
+__global__ void __launch_bounds__(256) k_tri_minv(TriArgs a, unsigned long long* dmax)
+{
+	constexpr int Np = tri_np(P), EPB = epb_minv(P);
+	__shared__ double sMinv[Np * Np];
+	__shared__ double sR[EPB * kNV * Np];
+	const int t = threadIdx.x;
+	for (int i = t; i < Np * Np; i += 256) sMinv[i] = a.ops->Minv[i];
+	double lmax = 0.0;
+	for (int e0 = blockIdx.x * EPB; e0 < a.E; e0 += gridDim.x * EPB) {
+		const int ne = min(EPB, a.E - e0);
+		__syncthreads();
+		for (int i = t; i < ne * kNV * Np; i += 256) sR[i] = a.r[(size_t)e0 * kNV * Np + i];
+		__syncthreads();
+		for (int i = t; i < ne * Np; i += 256) {
+			const int sl = i / Np, m = i - sl * Np, e = e0 + sl;
+			const double* row;
+			double sc;
+			tri_minv_row<Np>(a, sMinv, e, m, row, sc);
+			const double* R = sR + sl * kNV * Np;
+			double acc[kNV] = {};
+#pragma unroll
+			for (int n = 0; n < Np; ++n) {
+				const double c = row[n];
+#pragma unroll
+				for (int v = 0; v < kNV; ++v) acc[v] += c * R[v * Np + n];
+			}
+			double* out = a.r + (size_t)e * kNV * Np + m;
+#pragma unroll
+			for (int v = 0; v < kNV; ++v) {
+				const double x = sc * acc[v];
+				out[v * Np] = x;
+				const double ax = fabs(x);
+				lmax = (ax > lmax || ax != ax) ? ax : lmax;
+			}
+		}
+	}
+	if (dmax) {
+		for (int off = 32; off > 0; off >>= 1) {
+			const double o = __shfl_down(lmax, off);
+			lmax = (o > lmax || o != o) ? o : lmax;
+		}
+		if ((threadIdx.x & 63) == 0) atomicMax(dmax, (unsigned long long)__double_as_longlong(lmax));
+	}
+}
+
 // ------------------------------------------------------------------------------------------------ faces
 struct MixArgs {
 	const MixFace* faces;              // the whole sorted list
@@ -842,6 +892,22 @@ hipError_t launch_tri_ns_phase64(const TriGroup& g, const TriOps<double>* ops, c
 		DGX_TRI_SWITCH(k_tri_gv, epb_gv)
 	} else {
 		DGX_TRI_SWITCH(k_tri_ns, epb_ns)
+	}
+	return hipGetLastError();
+}
+
+hipError_t launch_tri_minv64(const TriGroup& g, const TriOps<double>* ops, double* r, unsigned long long* dmax,
+                             hipStream_t st)
+{
+	if (g.E == 0) return hipSuccess;
+	TriArgs a = tri_args(g, ops, r, r, nullptr);
+	switch (g.p) {
+	case 1: hipLaunchKernelGGL(k_tri_minv<1>, dim3(tri_grid(g.E, epb_minv(1))), dim3(256), 0, st, a, dmax); break;
+	case 2: hipLaunchKernelGGL(k_tri_minv<2>, dim3(tri_grid(g.E, epb_minv(2))), dim3(256), 0, st, a, dmax); break;
+	case 3: hipLaunchKernelGGL(k_tri_minv<3>, dim3(tri_grid(g.E, epb_minv(3))), dim3(256), 0, st, a, dmax); break;
+	case 4: hipLaunchKernelGGL(k_tri_minv<4>, dim3(tri_grid(g.E, epb_minv(4))), dim3(256), 0, st, a, dmax); break;
+	case 5: hipLaunchKernelGGL(k_tri_minv<5>, dim3(tri_grid(g.E, epb_minv(5))), dim3(256), 0, st, a, dmax); break;
+	default: return hipErrorInvalidValue;
 	}
 	return hipGetLastError();
 }

@@ -5,6 +5,8 @@ curved meshes, an all-TRI mesh, and a non-periodic box with Riemann and slip-wal
 Roe-Pike and Lax-Friedrichs; the error is taken relative to the larger of max |R| and max |volume term|.
 Navier-Stokes (BR2) on periodic mixed meshes against MixedOracle.rhs_ns, same bar and scale: checkerboards at
 p = 1, 3, 5, random orders on curved meshes with both fluxes, an all-TRI curved mesh; free stream, conservation.
+Explicit path: M^-1 R (dgx_compute_rhs_explicit, the TRI mass inverse from the reference's scaled by 1/det J on
+affine elements) against the oracle's dense solve, with its max-norm; three SSP-RK3 steps against the oracle's.
 Also: graph replays equal the eager call, and the entry points not built for TRI elements refuse the mesh
 instead of computing something else.
 """
@@ -141,19 +143,15 @@ def test_entry_points_not_built_for_tri_refuse(dev):
     u32, r32 = u.float(), torch.empty(u.numel(), dtype=torch.float32, device=dev)
     for call in (lambda: ctx.compute_rhs_f32(u32.data_ptr(), r32.data_ptr()),
                  lambda: ctx.compute_rhs_mixed(u.data_ptr(), r32.data_ptr()),
-                 lambda: ctx.compute_rhs_explicit(u.data_ptr(), r.data_ptr())):
+                 lambda: ctx.ssp_rk3_step(u.data_ptr(), 1e-4, positivity=True)):
         with pytest.raises(_lib.DgxError) as ei:
             call()
         assert ei.value.code == -3                  # DGX_ERR_UNSUPPORTED
     ctx.close()
-    # Navier-Stokes: no viscous boundary conditions, and the residual without M^-1 only
+    # Navier-Stokes: no viscous boundary conditions
     ns = _lib.Context(2, 5, M.PDE_NAVIER_STOKES, M.FLUX_ROE, mu=MU, Pr=PR)
     with pytest.raises(_lib.DgxError) as ei:
         ns.upload_mesh_mixed(CASES[-1][1])
-    assert ei.value.code == -3
-    ns.upload_mesh_mixed(mesh)
-    with pytest.raises(_lib.DgxError) as ei:
-        ns.compute_rhs_explicit(u.data_ptr(), r.data_ptr())
     assert ei.value.code == -3
     ns.close()
 
@@ -274,4 +272,52 @@ def test_ingested_mixed_mesh_residual_matches_oracle(dev, name, p):
     err = np.abs(got - ref).max() / scale
     print(f"  rel err {err:.2e}", flush=True)
     assert err < TOL64, err
+    ctx.close()
+
+
+# ---------------------------------------------------------------------------------------------- explicit path
+EXPLICIT = [CASES[1], CASES[4], CASES[6], CASES[7]]
+
+
+@pytest.mark.parametrize("case", EXPLICIT, ids=[c[0] for c in EXPLICIT])
+def test_mixed_mesh_explicit_residual_matches_oracle(dev, case):
+    name, mesh, flux, bc_prim = case
+    u = _state(mesh, bc_prim)
+    ctx = _ctx(mesh, flux, bc_prim)
+    ud = torch.from_numpy(u).to(dev)
+    rd = torch.full_like(ud, float("nan"))
+    mx = ctx.compute_rhs_explicit(ud.data_ptr(), rd.data_ptr())
+    torch.cuda.synchronize()
+    orc = MO.MixedOracle(mesh, flux, bc_prim=bc_prim)
+    ref = orc.apply_minv(orc.rhs(u))
+    got = rd.cpu().numpy()
+    scale = max(np.abs(orc.apply_minv(orc.rhs(u, "volume"))).max(), np.abs(ref).max())
+    err = np.abs(got - ref).max() / scale
+    print(f"case {name}: M^-1 R rel err {err:.2e}", flush=True)
+    assert err < TOL64, err
+    assert mx == np.abs(got).max()
+    ctx.close()
+
+
+def test_mixed_mesh_ssp_rk3_matches_oracle(dev):
+    name, mesh, flux, _ = CASES[5]
+    u = _state(mesh, None)
+    ctx = _ctx(mesh, flux, None)
+    ud = torch.from_numpy(u).to(dev)
+    orc = MO.MixedOracle(mesh, flux)
+    dt = 2e-4
+    ref = u.copy()
+    for _ in range(3):
+        ctx.ssp_rk3_step(ud.data_ptr(), dt, positivity=False)
+        u0 = ref
+        f = lambda w: orc.apply_minv(orc.rhs(w))
+        u1 = u0 + dt * f(u0)
+        u2 = 0.75 * u0 + 0.25 * (u1 + dt * f(u1))
+        ref = (u0 + 2.0 * (u2 + dt * f(u2))) / 3.0
+    torch.cuda.synchronize()
+    got = ud.cpu().numpy()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"SSP-RK3 x3 on {name}: rel err {err:.2e}, change {np.abs(ref - u).max():.2e}", flush=True)
+    assert np.abs(ref - u).max() > 1e-6
+    assert err < 1e-13
     ctx.close()
