@@ -27,6 +27,9 @@
 // residual kernels), its face gradient h = trace G_v + eta (lift trace) y with eta = 1.01 NFMAX = 4.04
 // (compute_face_rlhs_dg_T.c:243-300; TRI: K_f y, QUAD: kappa y), and the central n . (fv_L + fv_R) / 2
 // (numerical_flux_central_T.c:76-104) added to the inviscid flux.
+#include <cstdlib>
+#include <cstring>
+
 #include "k_residual.h"
 
 namespace dgx {
@@ -184,6 +187,131 @@ __global__ void __launch_bounds__(256) k_tri_rhs(TriArgs a)
 #pragma unroll
 			for (int v = 0; v < kNV; ++v) R[v * Np] = acc[v];
 		}
+	}
+}
+
+// The same residual with the dense products on the matrix cores (v_mfma_f64_16x16x4f64, FP64 in and out).
+// A wavefront takes four elements; the 16 rows of a tile are (variable v, element slot e) as i = e + 4 v, so that
+// with the gfx950 layout (A: lane l holds A[l % 16][l / 16]; B: lane l holds B[l / 16][l % 16]; D: lane l,
+// register r holds D[4 r + l / 16][l % 16]) each lane ends up with all four variables of element l / 16 at one
+// cubature point or node l % 16 of the tile:
+//   S^T (16 x Nc) = U^T (16 x Np) I^T                    the state at the cubature, then the fluxes in registers;
+//   R^T (16 x Np) = FR_r^T DrW^T + FR_s^T DsW^T - Z^T tw0^T   the weak divergence and the face lifts,
+// with the fluxes turned from the D layout to the A layout through LDS.  Operand B (the order's operators) is
+// staged in LDS in fragment order (one double per lane per instruction); the state and the face blocks are read
+// straight into A fragments.  Padding rows / columns / k-slots are zero, so they add nothing.
+typedef double dgx_d4 __attribute__((ext_vector_type(4)));
+
+template <int P>
+struct TriMfma {
+	static constexpr int Np = tri_np(P), Nc = tri_nc(P), N = P + 1;
+	static constexpr int KS1 = (Np + 3) / 4, NT1 = (Nc + 15) / 16;              // S: k over nodes, tiles over points
+	static constexpr int KS2 = (Nc + 3) / 4, QP = 4 * KS2, MT = (Np + 15) / 16;  // R: k over points, tiles over nodes
+	static constexpr int KF = (3 * N + 3) / 4;                                  // R: k over the face points
+};
+
+template <int P>
+__global__ void __launch_bounds__(256) k_tri_rhs_mfma(TriArgs a)
+{
+	using T = TriMfma<P>;
+	constexpr int Np = T::Np, Nc = T::Nc, N = T::N, KS1 = T::KS1, NT1 = T::NT1, KS2 = T::KS2, QP = T::QP;
+	constexpr int MT = T::MT, KF = T::KF;
+	__shared__ double sB1[KS1 * NT1 * 64], sBr[KS2 * MT * 64], sBs[KS2 * MT * 64], sBf[KF * MT * 64];
+	__shared__ double sFR[4][4 * 2 * kNV * QP];   // per wavefront: [e][d][v][q]
+	const int t = threadIdx.x;
+	for (int i = t; i < KS1 * NT1 * 64; i += 256) {
+		const int ln = i & 63, f = i >> 6, kk = f / NT1, nt = f - kk * NT1;
+		const int q = 16 * nt + (ln & 15), m = 4 * kk + (ln >> 4);
+		sB1[i] = (q < Nc && m < Np) ? a.ops->I[q * Np + m] : 0.0;
+	}
+	for (int i = t; i < KS2 * MT * 64; i += 256) {
+		const int ln = i & 63, f = i >> 6, kk = f / MT, mt = f - kk * MT;
+		const int m = 16 * mt + (ln & 15), q = 4 * kk + (ln >> 4);
+		const bool ok = m < Np && q < Nc;
+		sBr[i] = ok ? a.ops->DrW[m * Nc + q] : 0.0;
+		sBs[i] = ok ? a.ops->DsW[m * Nc + q] : 0.0;
+	}
+	for (int i = t; i < KF * MT * 64; i += 256) {
+		const int ln = i & 63, f = i >> 6, kf = f / MT, mt = f - kf * MT;
+		const int m = 16 * mt + (ln & 15), kap = 4 * kf + (ln >> 4);
+		sBf[i] = (m < Np && kap < 3 * N) ? 0.0 - a.ops->tw0[kap / N][m * N + kap % N] : 0.0;
+	}
+	__syncthreads();
+	const int wave = t >> 6, l = t & 63;
+	const int ea = (l & 15) & 3, va = (l & 15) >> 2, ka = l >> 4;   // A fragment: row l % 16, k slot l / 16
+	const int ed = l >> 4, jd = l & 15;                              // D fragment: element l / 16, column l % 16
+	double* F = sFR[wave];
+	const int ngroups = (a.E + 3) / 4;
+	for (int g0 = blockIdx.x * 4; g0 < ngroups; g0 += gridDim.x * 4) {   // the same trip count for every wave
+		const int grp = g0 + wave;
+		const bool gvalid = grp < ngroups;
+		const int e0 = grp * 4;
+		const int eA = gvalid ? min(e0 + ea, a.E - 1) : 0;   // a padding slot reads a real element (never stored)
+		const int eD = gvalid ? min(e0 + ed, a.E - 1) : 0;
+		double ua[KS1], za[KF];
+#pragma unroll
+		for (int kk = 0; kk < KS1; ++kk) {
+			const int m = 4 * kk + ka;
+			ua[kk] = (gvalid && m < Np) ? a.u[(size_t)eA * kNV * Np + va * Np + m] : 0.0;
+		}
+#pragma unroll
+		for (int kf = 0; kf < KF; ++kf) {
+			const int kap = 4 * kf + ka;
+			za[kf] = (gvalid && kap < 3 * N) ? a.z[(size_t)eA * 3 * kNV * N + ((kap / N) * kNV + va) * N + kap % N] : 0.0;
+		}
+		// the state at the cubature, the fluxes there in registers, then into LDS in the A layout
+#pragma unroll
+		for (int nt = 0; nt < NT1; ++nt) {
+			dgx_d4 acc = { 0.0, 0.0, 0.0, 0.0 };
+#pragma unroll
+			for (int kk = 0; kk < KS1; ++kk)
+				acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ua[kk], sB1[(kk * NT1 + nt) * 64 + l], acc, 0, 0, 0);
+			const int q = 16 * nt + jd;
+			if (q < QP) {
+				double fr0[kNV] = {}, fr1[kNV] = {};
+				if (gvalid && q < Nc) {
+					const double s[kNV] = { acc[0], acc[1], acc[2], acc[3] };
+					const double* mp = a.met + (size_t)eD * 4 * Nc + q;
+					const double m0 = mp[0], m1 = mp[Nc], m2 = mp[2 * Nc], m3 = mp[3 * Nc];
+					double f[2 * kNV];
+					euler_flux<double, 2>(s, f);
+#pragma unroll
+					for (int v = 0; v < kNV; ++v) {
+						fr0[v] = m0 * f[2 * v] + m1 * f[2 * v + 1];
+						fr1[v] = m2 * f[2 * v] + m3 * f[2 * v + 1];
+					}
+				}
+#pragma unroll
+				for (int v = 0; v < kNV; ++v) {
+					F[((ed * 2 + 0) * kNV + v) * QP + q] = fr0[v];
+					F[((ed * 2 + 1) * kNV + v) * QP + q] = fr1[v];
+				}
+			}
+		}
+		__syncthreads();
+		// the weak divergence and the face lifts, tile by tile of nodes
+#pragma unroll
+		for (int mt = 0; mt < MT; ++mt) {
+			dgx_d4 acc = { 0.0, 0.0, 0.0, 0.0 };
+#pragma unroll
+			for (int kk = 0; kk < KS2; ++kk) {
+				const int q = 4 * kk + ka;
+				const double ar = F[((ea * 2 + 0) * kNV + va) * QP + q];
+				const double as = F[((ea * 2 + 1) * kNV + va) * QP + q];
+				acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, sBr[(kk * MT + mt) * 64 + l], acc, 0, 0, 0);
+				acc = __builtin_amdgcn_mfma_f64_16x16x4f64(as, sBs[(kk * MT + mt) * 64 + l], acc, 0, 0, 0);
+			}
+#pragma unroll
+			for (int kf = 0; kf < KF; ++kf)
+				acc = __builtin_amdgcn_mfma_f64_16x16x4f64(za[kf], sBf[(kf * MT + mt) * 64 + l], acc, 0, 0, 0);
+			const int m = 16 * mt + jd;
+			if (gvalid && m < Np && e0 + ed < a.E) {
+				double* R = a.r + (size_t)(e0 + ed) * kNV * Np + m;
+#pragma unroll
+				for (int v = 0; v < kNV; ++v) R[v * Np] = acc[v];
+			}
+		}
+		__syncthreads();   // the fluxes are read before the next group overwrites them
 	}
 }
 
@@ -872,12 +1000,28 @@ hipError_t launch_tri_trace64(const TriGroup& g, const TriOps<double>* ops, cons
 	return hipGetLastError();
 }
 
+// TRI residual kernel: the matrix-core form unless DGX_TRI_KERNEL=valu (the VALU form, for comparison)
+static bool tri_use_mfma()
+{
+	static const bool mfma = [] {
+		const char* e = std::getenv("DGX_TRI_KERNEL");
+		return !(e && std::strcmp(e, "valu") == 0);
+	}();
+	return mfma;
+}
+
+constexpr int epb_mfma(int) { return 16; }   // four wavefronts of four elements
+
 hipError_t launch_tri_rhs64(const TriGroup& g, const TriOps<double>* ops, const double* u, double* r,
                             const double* z, hipStream_t st)
 {
 	if (g.E == 0) return hipSuccess;
 	TriArgs a = tri_args(g, ops, u, r, z);
-	DGX_TRI_SWITCH(k_tri_rhs, epb_rhs)
+	if (tri_use_mfma()) {
+		DGX_TRI_SWITCH(k_tri_rhs_mfma, epb_mfma)
+	} else {
+		DGX_TRI_SWITCH(k_tri_rhs, epb_rhs)
+	}
 	return hipGetLastError();
 }
 
