@@ -763,18 +763,50 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 				// lifts).  An affine element (det J constant at the cubature) has the reference ones divided by
 				// det J; if every element of the group is affine only 1 / det J is stored, else each element's
 				// own matrices.  The BR2 volume gradient and its face traces are written per evaluation.
+				// affine: the least-squares affine map of the reference nodes onto the element's nodes reproduces
+				// every node to 1e-12 of the element's size (det J computed at the cubature from the nodes varies
+				// by rounding alone up to ~6e-13 at p = 5, so it cannot decide); its det J is then exact
 				bool affine = true;
-				std::vector<double> jinv(tg.E);
+				std::vector<double> jinv(tg.E), rst(2 * (size_t)Np);
+				if (dgx_reference_nodes(1, tg.p, rst.data(), nullptr, nullptr) != DGX_OK) return DGX_ERR_ARG;
+				double nm[3][3] = {};   // normal equations of the fit in (r, s, 1)
+				for (int k = 0; k < Np; ++k) {
+					const double b3[3] = { rst[2 * k], rst[2 * k + 1], 1.0 };
+					for (int i = 0; i < 3; ++i)
+						for (int j = 0; j < 3; ++j) nm[i][j] += b3[i] * b3[j];
+				}
+				const double det3 = nm[0][0] * (nm[1][1] * nm[2][2] - nm[1][2] * nm[2][1]) -
+				                    nm[0][1] * (nm[1][0] * nm[2][2] - nm[1][2] * nm[2][0]) +
+				                    nm[0][2] * (nm[1][0] * nm[2][1] - nm[1][1] * nm[2][0]);
 				for (int k = 0; k < tg.E && affine; ++k) {
-					const double* d = &dj[(size_t)k * o.Nc];
-					double lo = d[0], hi = d[0], sum = 0.0;
-					for (int q = 0; q < o.Nc; ++q) {
-						lo = std::min(lo, d[q]);
-						hi = std::max(hi, d[q]);
-						sum += d[q];
+					const double* x = xyz + xoff[tg.e0 + k];   // 2 x Np
+					double coef[2][3];
+					for (int dd = 0; dd < 2; ++dd) {
+						double rhs[3] = {};
+						for (int m = 0; m < Np; ++m) {
+							rhs[0] += rst[2 * m] * x[dd * Np + m];
+							rhs[1] += rst[2 * m + 1] * x[dd * Np + m];
+							rhs[2] += x[dd * Np + m];
+						}
+						for (int c3 = 0; c3 < 3; ++c3) {   // Cramer's rule on the 3 x 3 normal equations
+							double M3[3][3];
+							for (int i = 0; i < 3; ++i)
+								for (int j = 0; j < 3; ++j) M3[i][j] = (j == c3) ? rhs[i] : nm[i][j];
+							coef[dd][c3] = (M3[0][0] * (M3[1][1] * M3[2][2] - M3[1][2] * M3[2][1]) -
+							                M3[0][1] * (M3[1][0] * M3[2][2] - M3[1][2] * M3[2][0]) +
+							                M3[0][2] * (M3[1][0] * M3[2][1] - M3[1][1] * M3[2][0])) / det3;
+						}
 					}
-					affine = (hi - lo) <= 1e-14 * hi;
-					jinv[k] = (double)o.Nc / sum;
+					double size = 0.0, worst = 0.0;
+					for (int m = 0; m < Np; ++m)
+						for (int dd = 0; dd < 2; ++dd) {
+							const double fit = coef[dd][0] * rst[2 * m] + coef[dd][1] * rst[2 * m + 1] + coef[dd][2];
+							worst = std::max(worst, std::fabs(fit - x[dd * Np + m]));
+							size = std::max(size, std::fabs(x[dd * Np + m] - x[dd * Np]));
+						}
+					const double dA = coef[0][0] * coef[1][1] - coef[0][1] * coef[1][0];
+					affine = worst <= 1e-12 * size && dA > 0.0;
+					jinv[k] = 1.0 / dA;
 				}
 				hg.affine = affine ? 1 : 0;
 				if (affine) {

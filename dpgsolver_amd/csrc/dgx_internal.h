@@ -95,6 +95,8 @@ struct TriOps {
 	// element's are these divided by its det J)
 	T Minv[kTriMaxNp * kTriMaxNp];
 	T K[3][kHpMaxN * kHpMaxN];
+	// the lifts of the three faces' projections y at the cubature, I M^-1 tw0_f side by side: [q][f N + b]
+	T L[kTriMaxNc * 3 * kHpMaxN];
 	int p, Np, Nc, N;
 };
 bool build_tri_ops(int p, TriOps<double>& o);

@@ -197,6 +197,17 @@ bool build_tri_ops(int p, TriOps<double>& o)
 	if (!tri_mass_data(o, one.data(), mi.data(), K.data())) return false;
 	std::copy(mi.begin(), mi.end(), o.Minv);
 	for (int lf = 0; lf < 3; ++lf) std::copy(K.begin() + lf * N * N, K.begin() + (lf + 1) * N * N, o.K[lf]);
+	for (int q = 0; q < Nc; ++q)
+		for (int lf = 0; lf < 3; ++lf)
+			for (int b = 0; b < N; ++b) {
+				long double acc = 0.0L;
+				for (int m = 0; m < Np; ++m) {
+					long double mt = 0.0L;   // (M^-1 tw0_f)[m][b]
+					for (int n = 0; n < Np; ++n) mt += (long double)mi[(size_t)m * Np + n] * o.tw0[lf][n * N + b];
+					acc += (long double)o.I[q * Np + m] * mt;
+				}
+				o.L[q * 3 * N + lf * N + b] = (double)acc;
+			}
 	return true;
 }
 

@@ -315,6 +315,142 @@ __global__ void __launch_bounds__(256) k_tri_rhs_mfma(TriArgs a)
 	}
 }
 
+// The Navier-Stokes TRI residual on the matrix cores (affine groups).  With the tiles of k_tri_rhs_mfma, the state
+// S^T = U^T I^T and each component d of the gradient at the cubature,
+//   G_d^T = Gv_d^T I^T + (Y_d^T / det J) L^T,   L = I M^-1 tw0_f side by side over the three faces,
+// are three (variable, element) tiles, so a lane holds the state and the whole gradient of one element at one
+// cubature point: F_euler + fv is evaluated there, and the divergence and face lifts follow as for Euler.
+template <int P>
+__global__ void __launch_bounds__(256) k_tri_ns_mfma(TriArgs a)
+{
+	using T = TriMfma<P>;
+	constexpr int Np = T::Np, Nc = T::Nc, N = T::N, KS1 = T::KS1, NT1 = T::NT1, KS2 = T::KS2, QP = T::QP;
+	constexpr int MT = T::MT, KF = T::KF, ZB = 3 * kNV * N;
+	__shared__ double sB1[KS1 * NT1 * 64], sBL[KF * NT1 * 64], sBr[KS2 * MT * 64], sBs[KS2 * MT * 64],
+	    sBf[KF * MT * 64];
+	__shared__ double sFR[4][4 * 2 * kNV * QP];
+	const int t = threadIdx.x;
+	for (int i = t; i < KS1 * NT1 * 64; i += 256) {
+		const int ln = i & 63, f = i >> 6, kk = f / NT1, nt = f - kk * NT1;
+		const int q = 16 * nt + (ln & 15), m = 4 * kk + (ln >> 4);
+		sB1[i] = (q < Nc && m < Np) ? a.ops->I[q * Np + m] : 0.0;
+	}
+	for (int i = t; i < KF * NT1 * 64; i += 256) {
+		const int ln = i & 63, f = i >> 6, kf = f / NT1, nt = f - kf * NT1;
+		const int q = 16 * nt + (ln & 15), kap = 4 * kf + (ln >> 4);
+		sBL[i] = (q < Nc && kap < 3 * N) ? a.ops->L[q * 3 * N + kap] : 0.0;
+	}
+	for (int i = t; i < KS2 * MT * 64; i += 256) {
+		const int ln = i & 63, f = i >> 6, kk = f / MT, mt = f - kk * MT;
+		const int m = 16 * mt + (ln & 15), q = 4 * kk + (ln >> 4);
+		const bool ok = m < Np && q < Nc;
+		sBr[i] = ok ? a.ops->DrW[m * Nc + q] : 0.0;
+		sBs[i] = ok ? a.ops->DsW[m * Nc + q] : 0.0;
+	}
+	for (int i = t; i < KF * MT * 64; i += 256) {
+		const int ln = i & 63, f = i >> 6, kf = f / MT, mt = f - kf * MT;
+		const int m = 16 * mt + (ln & 15), kap = 4 * kf + (ln >> 4);
+		sBf[i] = (m < Np && kap < 3 * N) ? 0.0 - a.ops->tw0[kap / N][m * N + kap % N] : 0.0;
+	}
+	__syncthreads();
+	const double mu = a.mu, Pr = a.Pr;
+	const int wave = t >> 6, l = t & 63;
+	const int ea = (l & 15) & 3, va = (l & 15) >> 2, ka = l >> 4;
+	const int ed = l >> 4, jd = l & 15;
+	double* F = sFR[wave];
+	const int ngroups = (a.E + 3) / 4;
+	for (int g0 = blockIdx.x * 4; g0 < ngroups; g0 += gridDim.x * 4) {
+		const int grp = g0 + wave;
+		const bool gvalid = grp < ngroups;
+		const int e0 = grp * 4;
+		const int eA = gvalid ? min(e0 + ea, a.E - 1) : 0;
+		const int eD = gvalid ? min(e0 + ed, a.E - 1) : 0;
+		const double js = gvalid ? a.jinv[eA] : 0.0;
+		double ua[KS1], g0a[KS1], g1a[KS1], y0a[KF], y1a[KF], za[KF];
+#pragma unroll
+		for (int kk = 0; kk < KS1; ++kk) {
+			const int m = 4 * kk + ka;
+			const bool ok = gvalid && m < Np;
+			ua[kk] = ok ? a.u[(size_t)eA * kNV * Np + va * Np + m] : 0.0;
+			g0a[kk] = ok ? a.gv[((size_t)eA * 2 * kNV + va) * Np + m] : 0.0;
+			g1a[kk] = ok ? a.gv[((size_t)eA * 2 * kNV + kNV + va) * Np + m] : 0.0;
+		}
+#pragma unroll
+		for (int kf = 0; kf < KF; ++kf) {
+			const int kap = 4 * kf + ka;
+			const bool ok = gvalid && kap < 3 * N;
+			const double* zf = a.z + (size_t)eA * 3 * ZB + (kap / N) * ZB + kap % N;
+			za[kf] = ok ? zf[va * N] : 0.0;
+			y0a[kf] = ok ? js * zf[kNV * N + va * N] : 0.0;
+			y1a[kf] = ok ? js * zf[kNV * N + (kNV + va) * N] : 0.0;
+		}
+#pragma unroll
+		for (int nt = 0; nt < NT1; ++nt) {
+			dgx_d4 as = { 0.0, 0.0, 0.0, 0.0 }, ag0 = as, ag1 = as;
+#pragma unroll
+			for (int kk = 0; kk < KS1; ++kk) {
+				const double b = sB1[(kk * NT1 + nt) * 64 + l];
+				as = __builtin_amdgcn_mfma_f64_16x16x4f64(ua[kk], b, as, 0, 0, 0);
+				ag0 = __builtin_amdgcn_mfma_f64_16x16x4f64(g0a[kk], b, ag0, 0, 0, 0);
+				ag1 = __builtin_amdgcn_mfma_f64_16x16x4f64(g1a[kk], b, ag1, 0, 0, 0);
+			}
+#pragma unroll
+			for (int kf = 0; kf < KF; ++kf) {
+				const double b = sBL[(kf * NT1 + nt) * 64 + l];
+				ag0 = __builtin_amdgcn_mfma_f64_16x16x4f64(y0a[kf], b, ag0, 0, 0, 0);
+				ag1 = __builtin_amdgcn_mfma_f64_16x16x4f64(y1a[kf], b, ag1, 0, 0, 0);
+			}
+			const int q = 16 * nt + jd;
+			if (q < QP) {
+				double fr0[kNV] = {}, fr1[kNV] = {};
+				if (gvalid && q < Nc) {
+					const double s[kNV] = { as[0], as[1], as[2], as[3] };
+					const double gq[2 * kNV] = { ag0[0], ag0[1], ag0[2], ag0[3], ag1[0], ag1[1], ag1[2], ag1[3] };
+					const double* mp = a.met + (size_t)eD * 4 * Nc + q;
+					const double m0 = mp[0], m1 = mp[Nc], m2 = mp[2 * Nc], m3 = mp[3 * Nc];
+					double f[2 * kNV], fv[2 * kNV];
+					euler_flux<double, 2>(s, f);
+					ns_viscous_flux2<double>(s, gq, mu, Pr, fv);
+#pragma unroll
+					for (int v = 0; v < kNV; ++v) {
+						const double f0 = f[2 * v] + fv[2 * v], f1 = f[2 * v + 1] + fv[2 * v + 1];
+						fr0[v] = m0 * f0 + m1 * f1;
+						fr1[v] = m2 * f0 + m3 * f1;
+					}
+				}
+#pragma unroll
+				for (int v = 0; v < kNV; ++v) {
+					F[((ed * 2 + 0) * kNV + v) * QP + q] = fr0[v];
+					F[((ed * 2 + 1) * kNV + v) * QP + q] = fr1[v];
+				}
+			}
+		}
+		__syncthreads();
+#pragma unroll
+		for (int mt = 0; mt < MT; ++mt) {
+			dgx_d4 acc = { 0.0, 0.0, 0.0, 0.0 };
+#pragma unroll
+			for (int kk = 0; kk < KS2; ++kk) {
+				const int q = 4 * kk + ka;
+				const double ar = F[((ea * 2 + 0) * kNV + va) * QP + q];
+				const double as2 = F[((ea * 2 + 1) * kNV + va) * QP + q];
+				acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, sBr[(kk * MT + mt) * 64 + l], acc, 0, 0, 0);
+				acc = __builtin_amdgcn_mfma_f64_16x16x4f64(as2, sBs[(kk * MT + mt) * 64 + l], acc, 0, 0, 0);
+			}
+#pragma unroll
+			for (int kf = 0; kf < KF; ++kf)
+				acc = __builtin_amdgcn_mfma_f64_16x16x4f64(za[kf], sBf[(kf * MT + mt) * 64 + l], acc, 0, 0, 0);
+			const int m = 16 * mt + jd;
+			if (gvalid && m < Np && e0 + ed < a.E) {
+				double* R = a.r + (size_t)(e0 + ed) * kNV * Np + m;
+#pragma unroll
+				for (int v = 0; v < kNV; ++v) R[v * Np] = acc[v];
+			}
+		}
+		__syncthreads();
+	}
+}
+
 // M^-1 row m of element e (affine groups: the reference row in LDS and 1 / det J)
 template <int Np>
 __device__ inline void tri_minv_row(const TriArgs& a, const double* sMinv, int e, int m, const double*& row,
@@ -1034,6 +1170,8 @@ hipError_t launch_tri_ns_phase64(const TriGroup& g, const TriOps<double>* ops, c
 	a.Pr = Pr;
 	if (phase == 1) {
 		DGX_TRI_SWITCH(k_tri_gv, epb_gv)
+	} else if (g.affine && tri_use_mfma()) {
+		DGX_TRI_SWITCH(k_tri_ns_mfma, epb_mfma)
 	} else {
 		DGX_TRI_SWITCH(k_tri_ns, epb_ns)
 	}
