@@ -16,9 +16,12 @@
 // M^-1 and the lift traces K_f = Ifc_f M^-1 tw0_f: the reference ones divided by det J on an affine group, each
 // element's own otherwise.
 //
-// Faces with a TRI side (k_mixf, k_mixf_ns): eight lanes per face, eight faces per 64-lane workgroup, one
-// launch per face-point count Nf = max(p_L, p_R) + 1 (compute_face_reference_order, face_solver.c:123-131; the
-// faces arrive sorted by (Nf, types, orders), so a workgroup rarely mixes shapes).  Each side's own values
+// Faces with a TRI side (k_mixf, k_mixf_ns): 2, 4 or 8 lanes per face (the next power of two >= Nf), four
+// wavefronts per workgroup sharing one copy of the operators in LDS, each face's LDS steps synchronised within
+// its wavefront; one launch per face-point count Nf = max(p_L, p_R) + 1 (compute_face_reference_order,
+// face_solver.c:123-131; the faces arrive sorted by (Nf, types, orders), so a wavefront rarely mixes shapes).
+// The Navier-Stokes kernel forms the two sides' face gradients one after the other in the same LDS block, which
+// keeps its LDS per face at 16 Nf doubles.  Each side's own values
 // (TRI: its traces; QUAD: its GLL face nodes) go through LDS and are interpolated to the face's GL points (the
 // right side in the left ordering); the numerical flux uses n J_f of the lower-order side
 // (get_dominant_geom_vol_side_index, face_solver.c:95-107); each side's z = P^T (w J_f F) / w_own at its own
@@ -758,23 +761,38 @@ __device__ inline double side_value(const MixArgs& a, int kind, int N, int lf, l
 	return a.u[uoff + (long long)v * N * N + quad_face_node(N, lf, j)];
 }
 
+// lanes per face: the next power of two >= the face-point count; four wavefronts per workgroup share one copy
+// of the operators, and the faces of a wavefront are synchronised within the wavefront only
+constexpr int mix_lpf(int NF) { return NF <= 2 ? 2 : NF <= 4 ? 4 : 8; }
+constexpr int kMixWaves = 4;
+constexpr int mix_fpb(int NF) { return kMixWaves * (64 / mix_lpf(NF)); }
+
+// orders the LDS writes and reads of one wavefront (every lane of a face is in the same wavefront)
+__device__ inline void wave_sync()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// the operators of one face-point count NF, for the own point counts N = 2..NF (index N - 2)
 template <int NF>
 struct MixOps {
-	double Ig[NF + 1][NF * NF];   // own GLL points -> the NF face points, by own N ([q * N + i])
-	double Pg[NF + 1][NF * NF];   // own GL points -> the NF face points
-	double winv[NF + 1][NF];
+	double Ig[NF - 1][NF * NF];   // own GLL points -> the NF face points ([q * N + i])
+	double Pg[NF - 1][NF * NF];   // own GL points -> the NF face points
+	double winv[NF - 1][NF];
 	double w[NF];
 };
 
 template <int NF>
 __device__ inline void stage_mix_ops(MixOps<NF>& o, const HpOps<double>* h)
 {
-	for (int i = threadIdx.x; i < (NF + 1) * NF * NF; i += blockDim.x) {
+	for (int i = threadIdx.x; i < (NF - 1) * NF * NF; i += blockDim.x) {
 		const int n = i / (NF * NF), k = i - n * NF * NF;
-		o.Ig[n][k] = h->Ig[n][NF][k];
-		o.Pg[n][k] = h->Pg[n][NF][k];
+		o.Ig[n][k] = h->Ig[n + 2][NF][k];
+		o.Pg[n][k] = h->Pg[n + 2][NF][k];
 	}
-	for (int i = threadIdx.x; i < (NF + 1) * NF; i += blockDim.x) o.winv[i / NF][i % NF] = h->winv[i / NF][i % NF];
+	for (int i = threadIdx.x; i < (NF - 1) * NF; i += blockDim.x) o.winv[i / NF][i % NF] = h->winv[i / NF + 2][i % NF];
 	for (int i = threadIdx.x; i < NF; i += blockDim.x) o.w[i] = h->w[NF][i];
 }
 
@@ -782,7 +800,7 @@ __device__ inline void stage_mix_ops(MixOps<NF>& o, const HpOps<double>* h)
 template <int NF>
 __device__ inline void side_at_point(const MixOps<NF>& o, int kind, int Ns, int q, const double* S, double* out)
 {
-	const double* op = kind ? o.Pg[Ns] : o.Ig[Ns];
+	const double* op = kind ? o.Pg[Ns - 2] : o.Ig[Ns - 2];
 #pragma unroll
 	for (int v = 0; v < kNV; ++v) {
 		double acc = 0.0;
@@ -797,13 +815,13 @@ __device__ inline void side_at_point(const MixOps<NF>& o, int kind, int Ns, int 
 template <int NF>
 __device__ inline void lift_side(const MixOps<NF>& o, int Ns, int j, bool rev, bool neg, const double* g, double* z)
 {
-	const double* Pg = o.Pg[Ns];
+	const double* Pg = o.Pg[Ns - 2];
 #pragma unroll
 	for (int v = 0; v < kNV; ++v) {
 		double acc = 0.0;
 #pragma unroll
 		for (int q = 0; q < NF; ++q) acc += Pg[(rev ? NF - 1 - q : q) * Ns + j] * g[q * kNV + v];
-		z[v * Ns + j] = neg ? 0.0 - acc * o.winv[Ns][j] : acc * o.winv[Ns][j];
+		z[v * Ns + j] = neg ? 0.0 - acc * o.winv[Ns - 2][j] : acc * o.winv[Ns - 2][j];
 	}
 }
 
@@ -816,13 +834,12 @@ __device__ inline void load_own(const MixArgs& a, int kind, int Ns, int lf, long
 }
 
 template <int NF, int FLUX>
-__global__ void __launch_bounds__(64) k_mixf(MixArgs a)
+__global__ void __launch_bounds__(256) k_mixf(MixArgs a)
 {
-	constexpr int FPB = 8;
+	constexpr int LPF = mix_lpf(NF), FPB = mix_fpb(NF);
 	__shared__ MixOps<NF> o;
-	__shared__ double sOwn[FPB][2][kNV * NF];
-	__shared__ double sG[FPB][NF * kNV];
-	const int slot = threadIdx.x >> 3, j = threadIdx.x & 7;
+	__shared__ double sF[FPB][2 * kNV * NF];   // both sides' own values [side][v][i], then w J_f F [q][v]
+	const int slot = threadIdx.x / LPF, j = threadIdx.x % LPF;
 	const int fl = blockIdx.x * FPB + slot;
 	const bool valid = fl < a.count;
 	const int fi = a.start + (valid ? fl : 0);
@@ -838,29 +855,35 @@ __global__ void __launch_bounds__(64) k_mixf(MixArgs a)
 		nj1 = nj[1];
 	}
 	stage_mix_ops<NF>(o, a.hops);
+	double* F = sF[slot];
 	if (valid && j < f.NL)
 #pragma unroll
-		for (int v = 0; v < kNV; ++v) sOwn[slot][0][v * NF + j] = oL[v];
+		for (int v = 0; v < kNV; ++v) F[v * NF + j] = oL[v];
 	if (valid && inter && j < f.NR)
 #pragma unroll
-		for (int v = 0; v < kNV; ++v) sOwn[slot][1][v * NF + j] = oR[v];
-	__syncthreads();
+		for (int v = 0; v < kNV; ++v) F[kNV * NF + v * NF + j] = oR[v];
+	__syncthreads();   // the operators and every face's own values
+	const int q = j;
+	double g[kNV];
 	if (valid && j < NF) {
-		const int q = j;
 		const double Jf = sqrt(nj0 * nj0 + nj1 * nj1);
 		const double nu[2] = { nj0 / Jf, nj1 / Jf };
 		double sL[kNV], sR[kNV], nnf[kNV];
-		side_at_point<NF>(o, f.kL, f.NL, q, sOwn[slot][0], sL);
-		if (inter) side_at_point<NF>(o, f.kR, f.NR, f.rev ? NF - 1 - q : q, sOwn[slot][1], sR);
+		side_at_point<NF>(o, f.kL, f.NL, q, F, sL);
+		if (inter) side_at_point<NF>(o, f.kR, f.NR, f.rev ? NF - 1 - q : q, F + kNV * NF, sR);
 		else bc_state<double, 2>(f.bc, sL, nu, a.bc_prim, sR);
 		if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<double, 2>(sL, sR, nu, nnf);
 		else numflux_roe<double, 2>(sL, sR, nu, nnf);
 #pragma unroll
-		for (int v = 0; v < kNV; ++v) sG[slot][q * kNV + v] = o.w[q] * Jf * nnf[v];
+		for (int v = 0; v < kNV; ++v) g[v] = o.w[q] * Jf * nnf[v];
 	}
-	__syncthreads();
-	if (valid && j < f.NL) lift_side<NF>(o, f.NL, j, false, false, sG[slot], a.z + f.zL);
-	if (valid && inter && j < f.NR) lift_side<NF>(o, f.NR, j, f.rev != 0, true, sG[slot], a.z + f.zR);
+	wave_sync();   // the own values are read
+	if (valid && j < NF)
+#pragma unroll
+		for (int v = 0; v < kNV; ++v) F[q * kNV + v] = g[v];
+	wave_sync();
+	if (valid && j < f.NL) lift_side<NF>(o, f.NL, j, false, false, F, a.z + f.zL);
+	if (valid && inter && j < f.NR) lift_side<NF>(o, f.NR, j, f.rev != 0, true, F, a.z + f.zR);
 }
 
 // what a side's face gradient at its own GL point j needs from memory: the trace of G_v there (t[k]) and its
@@ -914,124 +937,139 @@ __device__ inline void side_face_gradient(int kind, int Ns, int j, const double*
 }
 
 template <int NF, int FLUX>
-__global__ void __launch_bounds__(64) k_mixf_ns(MixArgs a)
+__global__ void __launch_bounds__(256) k_mixf_ns(MixArgs a)
 {
-	constexpr int FPB = 8, K8 = 2 * kNV;
+	constexpr int LPF = mix_lpf(NF), FPB = mix_fpb(NF), K8 = 2 * kNV;
 	__shared__ MixOps<NF> o;
-	__shared__ double sOwn[FPB][2][kNV * NF];
-	__shared__ double sQ[FPB][NF * K8];        // w q at the face points [q][k], then w J_f F [q][v]
-	__shared__ double sY[FPB][2][K8 * NF];     // each side's y [k][j], then its face gradient h
-	const int slot = threadIdx.x >> 3, j = threadIdx.x & 7;
+	__shared__ double sY[FPB][2][K8 * NF];   // both sides' own values [side][v][i]; then each side's y [k][i], then h
+	__shared__ double sQ[FPB][NF * K8];      // w q at the face points [q][k]; then w J_f F [q][v]
+	const int slot = threadIdx.x / LPF, j = threadIdx.x % LPF;
 	const int fl = blockIdx.x * FPB + slot;
 	const bool valid = fl < a.count;
 	const int fi = a.start + (valid ? fl : 0);
 	const MixFace f = a.faces[fi];
-	// every global load of this lane first (one round trip), then the operators
+	// the own values and normals first, then the operators
 	double oL[kNV], oR[kNV], nj0 = 1.0, nj1 = 0.0;
 	double tL[K8], tR[K8], kL[NF], kR[NF], scL = 1.0, scR = 1.0;
-	if (valid && j < f.NL) {
-		load_own(a, f.kL, f.NL, f.lfL, f.uL, f.eL, f.tL, j, oL);
-		load_side_grad<NF>(a, f.kL, f.NL, f.lfL, f.eL, f.tL, j, tL, kL, scL);
-	}
-	if (valid && j < f.NR) {
-		load_own(a, f.kR, f.NR, f.lfR, f.uR, f.eR, f.tR, j, oR);
-		load_side_grad<NF>(a, f.kR, f.NR, f.lfR, f.eR, f.tR, j, tR, kR, scR);
-	}
-	if (valid && j < NF) {
+	const bool ownL = valid && j < f.NL, ownR = valid && j < f.NR, point = valid && j < NF;
+	if (ownL) load_own(a, f.kL, f.NL, f.lfL, f.uL, f.eL, f.tL, j, oL);
+	if (ownR) load_own(a, f.kR, f.NR, f.lfR, f.uR, f.eR, f.tR, j, oR);
+	if (point) {
 		const double* nj = a.nJ + ((size_t)fi * kHpMaxN + j) * 2;
 		nj0 = nj[0];
 		nj1 = nj[1];
 	}
 	stage_mix_ops<NF>(o, a.hops);
-	if (valid && j < f.NL)
+	double* YL = sY[slot][0];
+	double* YR = sY[slot][1];
+	double* Q = sQ[slot];
+	if (ownL)
 #pragma unroll
-		for (int v = 0; v < kNV; ++v) sOwn[slot][0][v * NF + j] = oL[v];
-	if (valid && j < f.NR)
+		for (int v = 0; v < kNV; ++v) YL[v * NF + j] = oL[v];
+	if (ownR)
 #pragma unroll
-		for (int v = 0; v < kNV; ++v) sOwn[slot][1][v * NF + j] = oR[v];
-	__syncthreads();
+		for (int v = 0; v < kNV; ++v) YL[kNV * NF + v * NF + j] = oR[v];
+	__syncthreads();   // the operators and every face's own values
+	// the gradient traces and lift-trace rows, needed from the face-gradient step on
+	if (ownL) load_side_grad<NF>(a, f.kL, f.NL, f.lfL, f.eL, f.tL, j, tL, kL, scL);
+	if (ownR) load_side_grad<NF>(a, f.kR, f.NR, f.lfR, f.eR, f.tR, j, tR, kR, scR);
 	double sL[kNV], sR[kNV];
 	const int q = j, qr = f.rev ? NF - 1 - q : q;
-	if (valid && j < NF) {
-		side_at_point<NF>(o, f.kL, f.NL, q, sOwn[slot][0], sL);
-		side_at_point<NF>(o, f.kR, f.NR, qr, sOwn[slot][1], sR);
+	if (point) {
+		side_at_point<NF>(o, f.kL, f.NL, q, YL, sL);
+		side_at_point<NF>(o, f.kR, f.NR, qr, YL + kNV * NF, sR);
 #pragma unroll
 		for (int v = 0; v < kNV; ++v) {
 			const double dl = 0.5 * (sR[v] - sL[v]);
-			sQ[slot][q * K8 + v] = o.w[q] * (nj0 * dl);
-			sQ[slot][q * K8 + kNV + v] = o.w[q] * (nj1 * dl);
+			Q[q * K8 + v] = o.w[q] * (nj0 * dl);
+			Q[q * K8 + kNV + v] = o.w[q] * (nj1 * dl);
 		}
 	}
-	__syncthreads();
+	wave_sync();   // the own values are read and every q is written
 	// projections y to each side's own GL points (written for the residual kernels, after the lifted flux)
-	if (valid && j < f.NL) {
-		const double* Pg = o.Pg[f.NL];
+	if (ownL) {
+		const double* Pg = o.Pg[f.NL - 2];
 #pragma unroll
 		for (int k = 0; k < K8; ++k) {
 			double acc = 0.0;
 #pragma unroll
-			for (int qq = 0; qq < NF; ++qq) acc += Pg[qq * f.NL + j] * sQ[slot][qq * K8 + k];
-			acc *= o.winv[f.NL][j];
-			sY[slot][0][k * NF + j] = acc;
+			for (int qq = 0; qq < NF; ++qq) acc += Pg[qq * f.NL + j] * Q[qq * K8 + k];
+			acc *= o.winv[f.NL - 2][j];
+			YL[k * NF + j] = acc;
 			a.z[f.zL + kNV * f.NL + k * f.NL + j] = acc;
 		}
 	}
-	if (valid && j < f.NR) {
-		const double* Pg = o.Pg[f.NR];
+	if (ownR) {
+		const double* Pg = o.Pg[f.NR - 2];
 #pragma unroll
 		for (int k = 0; k < K8; ++k) {
 			double acc = 0.0;
 #pragma unroll
-			for (int qq = 0; qq < NF; ++qq) acc += Pg[(f.rev ? NF - 1 - qq : qq) * f.NR + j] * sQ[slot][qq * K8 + k];
-			acc *= o.winv[f.NR][j];
-			sY[slot][1][k * NF + j] = acc;
+			for (int qq = 0; qq < NF; ++qq) acc += Pg[(f.rev ? NF - 1 - qq : qq) * f.NR + j] * Q[qq * K8 + k];
+			acc *= o.winv[f.NR - 2][j];
+			YR[k * NF + j] = acc;
 			a.z[f.zR + kNV * f.NR + k * f.NR + j] = acc;
 		}
 	}
-	__syncthreads();
+	wave_sync();
 	double hL[K8], hR[K8];
-	if (valid && j < f.NL) side_face_gradient<NF>(f.kL, f.NL, j, tL, kL, scL, sY[slot][0], hL);
-	if (valid && j < f.NR) side_face_gradient<NF>(f.kR, f.NR, j, tR, kR, scR, sY[slot][1], hR);
-	__syncthreads();   // every y is read
-	if (valid && j < f.NL)
+	if (ownL) side_face_gradient<NF>(f.kL, f.NL, j, tL, kL, scL, YL, hL);
+	if (ownR) side_face_gradient<NF>(f.kR, f.NR, j, tR, kR, scR, YR, hR);
+	wave_sync();   // every y is read
+	if (ownL)
 #pragma unroll
-		for (int k = 0; k < K8; ++k) sY[slot][0][k * NF + j] = hL[k];
-	if (valid && j < f.NR)
+		for (int k = 0; k < K8; ++k) YL[k * NF + j] = hL[k];
+	if (ownR)
 #pragma unroll
-		for (int k = 0; k < K8; ++k) sY[slot][1][k * NF + j] = hR[k];
-	__syncthreads();
-	if (valid && j < NF) {   // the face gradients P h at this face point, and the flux
-		double gL[K8], gR[K8];
-		const double* PL = o.Pg[f.NL];
-		const double* PR = o.Pg[f.NR];
-#pragma unroll
-		for (int k = 0; k < K8; ++k) {
-			double aL = 0.0, aR = 0.0;
-#pragma unroll
-			for (int i = 0; i < NF; ++i) {
-				if (i < f.NL) aL += PL[q * f.NL + i] * sY[slot][0][k * NF + i];
-				if (i < f.NR) aR += PR[qr * f.NR + i] * sY[slot][1][k * NF + i];
-			}
-			gL[k] = aL;
-			gR[k] = aR;
-		}
+		for (int k = 0; k < K8; ++k) YR[k * NF + j] = hR[k];
+	wave_sync();
+	double fq[kNV];
+	if (point) {   // the flux at this face point: inviscid first, then each side's face gradient P h and viscous flux
 		const double Jf = sqrt(nj0 * nj0 + nj1 * nj1);
 		const double nu[2] = { nj0 / Jf, nj1 / Jf };
 		double nnf[kNV], fvL[K8], fvR[K8];
 		if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<double, 2>(sL, sR, nu, nnf);
 		else numflux_roe<double, 2>(sL, sR, nu, nnf);
-		ns_viscous_flux2<double>(sL, gL, a.mu, a.Pr, fvL);
-		ns_viscous_flux2<double>(sR, gR, a.mu, a.Pr, fvR);
+		{
+			double g[K8];
+			const double* P = o.Pg[f.NL - 2];
+#pragma unroll
+			for (int k = 0; k < K8; ++k) {
+				double acc = 0.0;
+#pragma unroll
+				for (int i = 0; i < NF; ++i)
+					if (i < f.NL) acc += P[q * f.NL + i] * YL[k * NF + i];
+				g[k] = acc;
+			}
+			ns_viscous_flux2<double>(sL, g, a.mu, a.Pr, fvL);
+		}
+		{
+			double g[K8];
+			const double* P = o.Pg[f.NR - 2];
+#pragma unroll
+			for (int k = 0; k < K8; ++k) {
+				double acc = 0.0;
+#pragma unroll
+				for (int i = 0; i < NF; ++i)
+					if (i < f.NR) acc += P[qr * f.NR + i] * YR[k * NF + i];
+				g[k] = acc;
+			}
+			ns_viscous_flux2<double>(sR, g, a.mu, a.Pr, fvR);
+		}
 		const double wj = o.w[q] * Jf;
 #pragma unroll
 		for (int v = 0; v < kNV; ++v) {
 			const double vis = nu[0] * 0.5 * (fvL[0 + 2 * v] + fvR[0 + 2 * v]) + nu[1] * 0.5 * (fvL[1 + 2 * v] + fvR[1 + 2 * v]);
-			sQ[slot][q * kNV + v] = wj * (nnf[v] + vis);
+			fq[v] = wj * (nnf[v] + vis);
 		}
 	}
-	__syncthreads();
-	if (valid && j < f.NL) lift_side<NF>(o, f.NL, j, false, false, sQ[slot], a.z + f.zL);
-	if (valid && j < f.NR) lift_side<NF>(o, f.NR, j, f.rev != 0, true, sQ[slot], a.z + f.zR);
+	// every q was read before the y step's sync
+	if (point)
+#pragma unroll
+		for (int v = 0; v < kNV; ++v) Q[q * kNV + v] = fq[v];
+	wave_sync();
+	if (ownL) lift_side<NF>(o, f.NL, j, false, false, Q, a.z + f.zL);
+	if (ownR) lift_side<NF>(o, f.NR, j, f.rev != 0, true, Q, a.z + f.zR);
 }
 
 TriArgs tri_args(const TriGroup& g, const TriOps<double>* ops, const double* u, double* r, const double* z)
@@ -1094,17 +1132,18 @@ hipError_t launch_mix(const HpData& H, const Ctx& c, const double* u, bool ns)
 		a.start = run.start;
 		a.count = run.count;
 		const hipStream_t st = H.lane[H.mrun_lane[i]];
-		const dim3 grid((run.count + 7) / 8);
 #define DGX_MIX(NFV)                                                                                           \
-	case NFV:                                                                                                  \
+	case NFV: {                                                                                                \
+		const dim3 grid((run.count + mix_fpb(NFV) - 1) / mix_fpb(NFV)), block(64 * kMixWaves);                 \
 		if (ns) {                                                                                              \
-			if (lf) hipLaunchKernelGGL((k_mixf_ns<NFV, DGX_FLUX_LF>), grid, dim3(64), 0, st, a);               \
-			else hipLaunchKernelGGL((k_mixf_ns<NFV, DGX_FLUX_ROE>), grid, dim3(64), 0, st, a);                 \
+			if (lf) hipLaunchKernelGGL((k_mixf_ns<NFV, DGX_FLUX_LF>), grid, block, 0, st, a);                  \
+			else hipLaunchKernelGGL((k_mixf_ns<NFV, DGX_FLUX_ROE>), grid, block, 0, st, a);                    \
 		} else {                                                                                               \
-			if (lf) hipLaunchKernelGGL((k_mixf<NFV, DGX_FLUX_LF>), grid, dim3(64), 0, st, a);                  \
-			else hipLaunchKernelGGL((k_mixf<NFV, DGX_FLUX_ROE>), grid, dim3(64), 0, st, a);                    \
+			if (lf) hipLaunchKernelGGL((k_mixf<NFV, DGX_FLUX_LF>), grid, block, 0, st, a);                     \
+			else hipLaunchKernelGGL((k_mixf<NFV, DGX_FLUX_ROE>), grid, block, 0, st, a);                       \
 		}                                                                                                      \
-		break;
+		break;                                                                                                 \
+	}
 		switch (run.Nf) {
 			DGX_MIX(2) DGX_MIX(3) DGX_MIX(4) DGX_MIX(5) DGX_MIX(6)
 		default: return hipErrorInvalidValue;
