@@ -457,7 +457,7 @@ def hp_euler_throughput(torch, dev, steps, warmup):
 
 
 def mixed_throughput(torch, dev, steps, warmup, pde):
-    """2-D FP64 residual on configuration 4's mesh: the 500 x 300 periodic base grid of [0,1]^2 with a seeded third
+    """2-D FP64 and FP32 residuals on configuration 4's mesh: the 500 x 300 periodic base grid of [0,1]^2 with a seeded third
     of its cells cut into two TRIs (99,980 TRI + 100,010 QUAD), p ~ U{1..5} per element (seeded), Roe; the state is
     the 3-D workload's formula in 2-D.  Euler, or Navier-Stokes (BR2) with Re 100, Pr 0.72 (mu = 0.0028), the
     configuration's own equations.  One evaluation = (Navier-Stokes: every group's gradient kernel,) the face
@@ -477,17 +477,33 @@ def mixed_throughput(torch, dev, steps, warmup, pde):
     ctx.upload_mesh_mixed(mesh)
     u = torch.from_numpy(M.config4_mixed_state(mesh)).to(dev)
     r = torch.empty_like(u)
-    g = ctx.graph_rhs(_lib.GRAPH_RHS_FP64, u.data_ptr(), r.data_ptr())
-    for _ in range(warmup):
-        g.launch(1)
+    u32 = u.float()
+    r32 = torch.empty_like(u32)
+
+    def timed(kind, ui, ri):
+        g = ctx.graph_rhs(kind, ui.data_ptr(), ri.data_ptr())
+        for _ in range(warmup):
+            g.launch(1)
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        g.launch(steps)
+        stop.record(stream)
+        torch.cuda.synchronize()
+        g.close()
+        return start.elapsed_time(stop) / steps
+
+    ms = timed(_lib.GRAPH_RHS_FP64, u, r)
+    ms32 = timed(_lib.GRAPH_RHS_FP32, u32, r32)
+    # FP32 error against FP64, and the part that rounding the state alone to FP32 causes
+    rr = torch.empty_like(u)
+    u64r = u32.double()
+    ctx.compute_rhs(u64r.data_ptr(), rr.data_ptr())
     torch.cuda.synchronize()
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    g.launch(steps)
-    stop.record(stream)
-    torch.cuda.synchronize()
-    ms = start.elapsed_time(stop) / steps
-    g.close()
+    scale = r.abs().max().item()
+    err32 = (r32.double() - r).abs().max().item() / scale
+    err_in = (rr - r).abs().max().item() / scale
+    del rr, u64r
     ctx.close()
     tri = mesh.kind == 1
     p = mesh.p.astype(np.int64)
@@ -509,7 +525,13 @@ def mixed_throughput(torch, dev, steps, warmup, pde):
             "value": round(ndof / (ms * 1e-3) / 1e9, 3), "unit": "GDOF/s", "ms_per_step": round(ms, 5),
             "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                         "traffic": None}}
+                         "traffic": None},
+            # FP32 state, geometry shadows and operators: the same values in half the bytes
+            "fp32": {"value": round(ndof / (ms32 * 1e-3) / 1e9, 3), "ms_per_step": round(ms32, 5),
+                     "rel_err_vs_fp64": err32, "input_rounding_rel_err": err_in,
+                     "roofline": {"bound": "hbm", "achieved": round(alg / 2 / (ms32 * 1e-3) / 1e9, 1),
+                                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": round(alg / 2 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None}}}
 
 
 def config2_throughput(torch, dev, steps, warmup):
@@ -760,10 +782,10 @@ def main():
     print(f"variable-order euler residual: {hpe['fp64']['ms_per_step']:.4f} ms, {hpe['fp64']['value']} GDOF/s",
           file=sys.stderr, flush=True)
     mx4 = mixed_throughput(torch, dev, args.steps, args.warmup, "euler")
-    print(f"mixed TRI/QUAD euler residual: {mx4['ms_per_step']:.4f} ms, {mx4['value']} GDOF/s", file=sys.stderr,
+    print(f"mixed TRI/QUAD euler residual: {mx4['ms_per_step']:.4f} ms, {mx4['value']} GDOF/s (FP32 {mx4['fp32']['ms_per_step']:.4f} ms)", file=sys.stderr,
           flush=True)
     mx4ns = mixed_throughput(torch, dev, args.steps, args.warmup, "navier_stokes")
-    print(f"mixed TRI/QUAD navier-stokes residual: {mx4ns['ms_per_step']:.4f} ms, {mx4ns['value']} GDOF/s",
+    print(f"mixed TRI/QUAD navier-stokes residual: {mx4ns['ms_per_step']:.4f} ms, {mx4ns['value']} GDOF/s (FP32 {mx4ns['fp32']['ms_per_step']:.4f} ms)",
           file=sys.stderr, flush=True)
     c2 = config2_throughput(torch, dev, args.steps, args.warmup)
     print(f"config-2 euler residual: fp64 {c2['fp64']['value']} GDOF/s, mixed {c2['mixed']['value']} GDOF/s, "

@@ -3,6 +3,7 @@
 
 #include <cstdio>
 #include <algorithm>
+#include <cstddef>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -35,6 +36,15 @@ void release(T*& p)
 	p = nullptr;
 }
 
+// a device copy of n doubles rounded to FP32
+hipError_t upload_f32(float** dst, const double* src, size_t n)
+{
+	std::vector<float> h(src, src + n);
+	hipError_t e = hipMalloc(dst, sizeof(float) * std::max<size_t>(n, 1));
+	if (e == hipSuccess && n) e = hipMemcpy(*dst, h.data(), sizeof(float) * n, hipMemcpyHostToDevice);
+	return e;
+}
+
 void free_hp(Ctx& c)
 {
 	if (!c.hp) return;
@@ -45,8 +55,9 @@ void free_hp(Ctx& c)
 	for (dgx::TriGroup& t : H->tri) {
 		release(t.met); release(t.detJ); release(t.tr); release(t.jinv);
 		release(t.minv); release(t.K); release(t.gv); release(t.gvt);
+		release(t.met32); release(t.jinv32); release(t.minv32); release(t.K32);
 	}
-	release(H->tri_ops64); release(H->mfaces); release(H->mnJ);
+	release(H->tri_ops64); release(H->tri_ops32); release(H->mfaces); release(H->mnJ); release(H->mnJ32);
 	for (int k = 0; k < dgx::HpData::kLanes; ++k) {
 		if (H->lane[k]) (void)hipStreamDestroy(H->lane[k]);
 		if (H->join_ev[k]) (void)hipEventDestroy(H->join_ev[k]);
@@ -421,18 +432,32 @@ dgx_status hp_rhs64(Ctx& c, const double* u, double* r, int mode, unsigned long 
 dgx_status hp_rhs32(Ctx& c, const float* u, float* r)
 {
 	dgx::HpData& H = *c.hp;
-	if (!H.tri.empty()) return DGX_ERR_UNSUPPORTED;   // TRI elements: FP64 only
 	DGX_TRY(dgx::hp_fork(H, c.stream));
 	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) {
+		const double mu = c.cfg.mu, Pr = c.cfg.Pr;
 		DGX_TRY(on_groups(c, [&](size_t g, Ctx& s) { return dgx::launch_rhs_ns32_phase(s, u + H.u_off[g], r + H.u_off[g], 1); }));
+		for (size_t t = 0; t < H.tri.size(); ++t)
+			DGX_TRY(dgx::launch_tri_ns_phase32(H.tri[t], H.tri_ops32, u, r, H.z32, mu, Pr, 1, H.lane[H.tri_lane[t]]));
 		DGX_TRY(hp_barrier(H, c.stream));
-		DGX_TRY(dgx::launch_hp_faces32(c, u));
+		if (H.tri.empty()) DGX_TRY(dgx::launch_hp_faces32(c, u));
+		else DGX_TRY(dgx::launch_mix_faces32(H, c, u, true));
 		DGX_TRY(hp_barrier(H, c.stream));
+		for (size_t t = 0; t < H.tri.size(); ++t)
+			DGX_TRY(dgx::launch_tri_ns_phase32(H.tri[t], H.tri_ops32, u, r, H.z32, mu, Pr, 2, H.lane[H.tri_lane[t]]));
 		DGX_TRY(on_groups(c, [&](size_t g, Ctx& s) { return dgx::launch_rhs_ns32_phase(s, u + H.u_off[g], r + H.u_off[g], 2); }));
 	} else {
-		DGX_TRY(dgx::launch_hp_faces32(c, u));
+		if (H.tri.empty()) {
+			DGX_TRY(dgx::launch_hp_faces32(c, u));
+		} else {   // the TRI traces, then the faces
+			for (size_t t = 0; t < H.tri.size(); ++t)
+				DGX_TRY(dgx::launch_tri_trace32(H.tri[t], H.tri_ops32, u, H.lane[H.tri_lane[t]]));
+			DGX_TRY(hp_barrier(H, c.stream));
+			DGX_TRY(dgx::launch_mix_faces32(H, c, u, false));
+		}
 		DGX_TRY(hp_barrier(H, c.stream));
 		DGX_TRY(on_groups(c, [&](size_t g, Ctx& s) { return dgx::launch_rhs32(s, u + H.u_off[g], r + H.u_off[g]); }));
+		for (size_t t = 0; t < H.tri.size(); ++t)
+			DGX_TRY(dgx::launch_tri_rhs32(H.tri[t], H.tri_ops32, u, r, H.z32, H.lane[H.tri_lane[t]]));
 	}
 	DGX_TRY(dgx::hp_join(H, c.stream));
 	return DGX_OK;
@@ -736,6 +761,23 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 			if (!dgx::build_tri_ops(p, tops[p])) return DGX_ERR_UNSUPPORTED;
 		DGX_TRY(hipMalloc(&H.tri_ops64, sizeof(dgx::TriOps<double>) * tops.size()));
 		DGX_TRY(hipMemcpy(H.tri_ops64, tops.data(), sizeof(dgx::TriOps<double>) * tops.size(), hipMemcpyHostToDevice));
+		{
+			// the FP32 operators: every table rounded (the tables precede the sizes in TriOps)
+			std::vector<dgx::TriOps<float>> t32(tops.size());
+			constexpr size_t nt = offsetof(dgx::TriOps<double>, p) / sizeof(double);
+			static_assert(offsetof(dgx::TriOps<float>, p) == nt * sizeof(float), "TriOps tables are contiguous");
+			for (size_t k = 0; k < tops.size(); ++k) {
+				const double* src = &tops[k].I[0];
+				float* dst = &t32[k].I[0];
+				for (size_t i = 0; i < nt; ++i) dst[i] = (float)src[i];
+				t32[k].p = tops[k].p;
+				t32[k].Np = tops[k].Np;
+				t32[k].Nc = tops[k].Nc;
+				t32[k].N = tops[k].N;
+			}
+			DGX_TRY(hipMalloc(&H.tri_ops32, sizeof(dgx::TriOps<float>) * t32.size()));
+			DGX_TRY(hipMemcpy(H.tri_ops32, t32.data(), sizeof(dgx::TriOps<float>) * t32.size(), hipMemcpyHostToDevice));
+		}
 		for (int t = 0; t < T; ++t) {
 			dgx::TriGroup tg;
 			tg.p = tri_p[t];
@@ -757,6 +799,7 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 			DGX_TRY(hipMalloc(&hg.detJ, sizeof(double) * dj.size()));
 			DGX_TRY(hipMalloc(&hg.tr, sizeof(double) * (size_t)tg.E * 3 * nv * N));
 			DGX_TRY(hipMemcpy(hg.met, met.data(), sizeof(double) * met.size(), hipMemcpyHostToDevice));
+			DGX_TRY(upload_f32(&hg.met32, met.data(), met.size()));
 			DGX_TRY(hipMemcpy(hg.detJ, dj.data(), sizeof(double) * dj.size(), hipMemcpyHostToDevice));
 			{
 				// M^-1 (the explicit path; BR2: also the face-trace operators K_f = Ifc_f M^-1 tw0_f of the
@@ -812,6 +855,7 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 				if (affine) {
 					DGX_TRY(hipMalloc(&hg.jinv, sizeof(double) * jinv.size()));
 					DGX_TRY(hipMemcpy(hg.jinv, jinv.data(), sizeof(double) * jinv.size(), hipMemcpyHostToDevice));
+					DGX_TRY(upload_f32(&hg.jinv32, jinv.data(), jinv.size()));
 				} else {
 					std::vector<double> mi((size_t)tg.E * Np * Np), K((size_t)tg.E * 3 * N * N);
 					for (int k = 0; k < tg.E; ++k)
@@ -821,6 +865,8 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 					DGX_TRY(hipMalloc(&hg.K, sizeof(double) * K.size()));
 					DGX_TRY(hipMemcpy(hg.minv, mi.data(), sizeof(double) * mi.size(), hipMemcpyHostToDevice));
 					DGX_TRY(hipMemcpy(hg.K, K.data(), sizeof(double) * K.size(), hipMemcpyHostToDevice));
+					DGX_TRY(upload_f32(&hg.minv32, mi.data(), mi.size()));
+					DGX_TRY(upload_f32(&hg.K32, K.data(), K.size()));
 				}
 				if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) {
 					DGX_TRY(hipMalloc(&hg.gv, sizeof(double) * (size_t)tg.E * 2 * nv * Np));
@@ -836,6 +882,7 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 		DGX_TRY(hipMalloc(&H.mnJ, sizeof(double) * mnj.size()));
 		DGX_TRY(hipMemcpy(H.mfaces, mf.data(), sizeof(dgx::MixFace) * mf.size(), hipMemcpyHostToDevice));
 		DGX_TRY(hipMemcpy(H.mnJ, mnj.data(), sizeof(double) * mnj.size(), hipMemcpyHostToDevice));
+		DGX_TRY(upload_f32(&H.mnJ32, mnj.data(), mnj.size()));
 	}
 	// one uniform-order context per QUAD group
 	for (int g = 0; g < G; ++g) {

@@ -141,6 +141,12 @@ struct TriGroup {
 	double* K = nullptr;             // E x 3 x N x N: lift traces Ifc_f M^-1 tw0_f per local face
 	double* gv = nullptr;            // E x 2 x nv x Np: volume part of the weak gradient (written each residual)
 	double* gvt = nullptr;           // E x 3 x 2 x nv x N: its traces at the own GL face points
+	// FP32 shadows of the geometry and mass data (the FP32 residual; its traces and gradients use the first half
+	// of tr, gv and gvt)
+	float* met32 = nullptr;
+	float* jinv32 = nullptr;
+	float* minv32 = nullptr;
+	float* K32 = nullptr;
 };
 
 // a run of faces between the same two orders (the faces are sorted by pair)
@@ -178,8 +184,10 @@ struct HpData {
 	// TRI groups and the faces that touch a TRI element (mixed meshes)
 	std::vector<TriGroup> tri;
 	TriOps<double>* tri_ops64 = nullptr;   // device: kTriMaxP + 1 entries, indexed by p
+	TriOps<float>* tri_ops32 = nullptr;
 	MixFace* mfaces = nullptr;             // Fmix, sorted by (Nf, kL, NL, kR, NR)
 	double* mnJ = nullptr;                 // Fmix x kHpMaxN x 2: n J_f at the face GL points (left ordering)
+	float* mnJ32 = nullptr;
 	int Fmix = 0;
 	std::vector<MixRun> mruns;             // runs of mixed faces with one Nf
 	std::vector<int> mrun_lane;            // side stream of each run (longest-first balance of count Nf^2)
@@ -290,6 +298,13 @@ hipError_t launch_tri_ns_phase64(const TriGroup& g, const TriOps<double>* ops, c
 // Euler: a TRI group's state traces at its own GL face points (before the mixed faces)
 hipError_t launch_tri_trace64(const TriGroup& g, const TriOps<double>* ops, const double* u, hipStream_t st);
 hipError_t launch_mix_faces_ns64(const HpData& H, const Ctx& c, const double* u, hipStream_t st);
+// the FP32 residual on mixed meshes (FP32 state, geometry shadows and operators; VALU kernels)
+hipError_t launch_tri_trace32(const TriGroup& g, const TriOps<float>* ops, const float* u, hipStream_t st);
+hipError_t launch_tri_rhs32(const TriGroup& g, const TriOps<float>* ops, const float* u, float* r, const float* z,
+                            hipStream_t st);
+hipError_t launch_tri_ns_phase32(const TriGroup& g, const TriOps<float>* ops, const float* u, float* r,
+                                 const float* z, double mu, double Pr, int phase, hipStream_t st);
+hipError_t launch_mix_faces32(const HpData& H, const Ctx& c, const float* u, bool ns);
 hipError_t launch_minv(Ctx& c, double* r);
 hipError_t launch_positivity(Ctx& c, double* u, int* status);
 hipError_t launch_maxabs(Ctx& c, const double* r, size_t n, unsigned long long* dmax);

@@ -32,6 +32,7 @@
 // (numerical_flux_central_T.c:76-104) added to the inviscid flux.
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include "k_residual.h"
 
@@ -79,27 +80,29 @@ constexpr int epb_ns(int P)
 	               kTriLdsNs);
 }
 
-struct TriArgs {
-	const double* u;       // the group's state, E x nv x Np
-	double* r;             // the group's residual
-	const double* met;     // E x 4 x Nc
-	const double* z;       // the group's face blocks, E x 3 x zblk x N
-	double* tr;            // E x 3 x nv x N
-	double* gv;            // E x 2 x nv x Np
-	double* gvt;           // E x 3 x 2 x nv x N
-	const double* minv;    // curved groups: E x Np x Np
-	const double* jinv;    // affine groups: E
-	const TriOps<double>* ops;
+template <typename T>
+struct TriArgsT {
+	const T* u;            // the group's state, E x nv x Np
+	T* r;                  // the group's residual
+	const T* met;          // E x 4 x Nc
+	const T* z;            // the group's face blocks, E x 3 x zblk x N
+	T* tr;                 // E x 3 x nv x N
+	T* gv;                 // E x 2 x nv x Np
+	T* gvt;                // E x 3 x 2 x nv x N
+	const T* minv;         // curved groups: E x Np x Np
+	const T* jinv;         // affine groups: E
+	const TriOps<T>* ops;
 	double mu, Pr;
 	int E, affine;
 };
+using TriArgs = TriArgsT<double>;
 
-template <int P>
-__global__ void __launch_bounds__(256) k_tri_trace(TriArgs a)
+template <typename T, int P>
+__global__ void __launch_bounds__(256) k_tri_trace(TriArgsT<T> a)
 {
 	constexpr int Np = tri_np(P), N = P + 1, F3 = 3 * N, EPB = epb_trace(P);
-	__shared__ double sIfc[F3 * Np];
-	__shared__ double sU[EPB * kNV * Np];
+	__shared__ T sIfc[F3 * Np];
+	__shared__ T sU[EPB * kNV * Np];
 	const int t = threadIdx.x;
 	for (int i = t; i < F3 * Np; i += 256) sIfc[i] = a.ops->Ifc[i / (N * Np)][i % (N * Np)];
 	for (int e0 = blockIdx.x * EPB; e0 < a.E; e0 += gridDim.x * EPB) {
@@ -109,27 +112,27 @@ __global__ void __launch_bounds__(256) k_tri_trace(TriArgs a)
 		__syncthreads();
 		for (int i = t; i < ne * F3; i += 256) {
 			const int sl = i / F3, k = i - sl * F3, lf = k / N, b = k - lf * N;
-			const double* U = sU + sl * kNV * Np;
-			double acc[kNV] = {};
+			const T* U = sU + sl * kNV * Np;
+			T acc[kNV] = {};
 #pragma unroll
 			for (int m = 0; m < Np; ++m) {
-				const double c = sIfc[k * Np + m];
+				const T c = sIfc[k * Np + m];
 #pragma unroll
 				for (int v = 0; v < kNV; ++v) acc[v] += c * U[v * Np + m];
 			}
-			double* out = a.tr + ((size_t)(e0 + sl) * 3 + lf) * kNV * N + b;
+			T* out = a.tr + ((size_t)(e0 + sl) * 3 + lf) * kNV * N + b;
 #pragma unroll
 			for (int v = 0; v < kNV; ++v) out[v * N] = acc[v];
 		}
 	}
 }
 
-template <int P>
-__global__ void __launch_bounds__(256) k_tri_rhs(TriArgs a)
+template <typename T, int P>
+__global__ void __launch_bounds__(256) k_tri_rhs(TriArgsT<T> a)
 {
 	constexpr int Np = tri_np(P), Nc = tri_nc(P), N = P + 1, ZB = kNV * N, EPB = epb_rhs(P);
-	__shared__ double sI[Nc * Np], sDrW[Np * Nc], sDsW[Np * Nc], sTw0[3 * Np * N];
-	__shared__ double sU[EPB * kNV * Np], sFR[EPB * 2 * kNV * Nc], sZ[EPB * 3 * ZB], sM[EPB * 4 * Nc];
+	__shared__ T sI[Nc * Np], sDrW[Np * Nc], sDsW[Np * Nc], sTw0[3 * Np * N];
+	__shared__ T sU[EPB * kNV * Np], sFR[EPB * 2 * kNV * Nc], sZ[EPB * 3 * ZB], sM[EPB * 4 * Nc];
 	const int t = threadIdx.x;
 	for (int i = t; i < Nc * Np; i += 256) {
 		sI[i] = a.ops->I[i];
@@ -147,19 +150,19 @@ __global__ void __launch_bounds__(256) k_tri_rhs(TriArgs a)
 		__syncthreads();
 		for (int i = t; i < ne * Nc; i += 256) {
 			const int sl = i / Nc, q = i - sl * Nc;
-			const double* U = sU + sl * kNV * Np;
-			double s[kNV] = {};
+			const T* U = sU + sl * kNV * Np;
+			T s[kNV] = {};
 #pragma unroll
 			for (int m = 0; m < Np; ++m) {
-				const double c = sI[q * Np + m];
+				const T c = sI[q * Np + m];
 #pragma unroll
 				for (int v = 0; v < kNV; ++v) s[v] += c * U[v * Np + m];
 			}
-			const double* mp = sM + sl * 4 * Nc + q;
-			const double m0 = mp[0], m1 = mp[Nc], m2 = mp[2 * Nc], m3 = mp[3 * Nc];
-			double f[2 * kNV];
-			euler_flux<double, 2>(s, f);
-			double* FR = sFR + sl * 2 * kNV * Nc;
+			const T* mp = sM + sl * 4 * Nc + q;
+			const T m0 = mp[0], m1 = mp[Nc], m2 = mp[2 * Nc], m3 = mp[3 * Nc];
+			T f[2 * kNV];
+			euler_flux<T, 2>(s, f);
+			T* FR = sFR + sl * 2 * kNV * Nc;
 #pragma unroll
 			for (int v = 0; v < kNV; ++v) {
 				FR[v * Nc + q] = m0 * f[2 * v] + m1 * f[2 * v + 1];
@@ -169,12 +172,12 @@ __global__ void __launch_bounds__(256) k_tri_rhs(TriArgs a)
 		__syncthreads();
 		for (int i = t; i < ne * Np; i += 256) {
 			const int sl = i / Np, m = i - sl * Np;
-			const double* FR = sFR + sl * 2 * kNV * Nc;
-			const double* Z = sZ + sl * 3 * ZB;
-			double acc[kNV] = {};
+			const T* FR = sFR + sl * 2 * kNV * Nc;
+			const T* Z = sZ + sl * 3 * ZB;
+			T acc[kNV] = {};
 #pragma unroll
 			for (int q = 0; q < Nc; ++q) {
-				const double cr = sDrW[m * Nc + q], cs = sDsW[m * Nc + q];
+				const T cr = sDrW[m * Nc + q], cs = sDsW[m * Nc + q];
 #pragma unroll
 				for (int v = 0; v < kNV; ++v) acc[v] += cr * FR[v * Nc + q] + cs * FR[(kNV + v) * Nc + q];
 			}
@@ -182,11 +185,11 @@ __global__ void __launch_bounds__(256) k_tri_rhs(TriArgs a)
 			for (int lf = 0; lf < 3; ++lf)
 #pragma unroll
 				for (int b = 0; b < N; ++b) {
-					const double c = sTw0[(lf * Np + m) * N + b];
+					const T c = sTw0[(lf * Np + m) * N + b];
 #pragma unroll
 					for (int v = 0; v < kNV; ++v) acc[v] -= c * Z[(lf * kNV + v) * N + b];
 				}
-			double* R = a.r + (size_t)(e0 + sl) * kNV * Np + m;
+			T* R = a.r + (size_t)(e0 + sl) * kNV * Np + m;
 #pragma unroll
 			for (int v = 0; v < kNV; ++v) R[v * Np] = acc[v];
 		}
@@ -455,9 +458,9 @@ __global__ void __launch_bounds__(256) k_tri_ns_mfma(TriArgs a)
 }
 
 // M^-1 row m of element e (affine groups: the reference row in LDS and 1 / det J)
-template <int Np>
-__device__ inline void tri_minv_row(const TriArgs& a, const double* sMinv, int e, int m, const double*& row,
-                                    double& scale)
+template <int Np, typename T>
+__device__ inline void tri_minv_row(const TriArgsT<T>& a, const T* sMinv, int e, int m, const T*& row,
+                                    T& scale)
 {
 	if (a.affine) {
 		row = sMinv + m * Np;
@@ -468,12 +471,12 @@ __device__ inline void tri_minv_row(const TriArgs& a, const double* sMinv, int e
 	}
 }
 
-template <int P>
-__global__ void __launch_bounds__(256) k_tri_gv(TriArgs a)
+template <typename T, int P>
+__global__ void __launch_bounds__(256) k_tri_gv(TriArgsT<T> a)
 {
 	constexpr int Np = tri_np(P), Nc = tri_nc(P), N = P + 1, F3 = 3 * N, K8 = 2 * kNV, EPB = epb_gv(P);
-	__shared__ double sI[Nc * Np], sDr[Nc * Np], sDs[Nc * Np], sIfc[F3 * Np], sW[Nc], sMinv[Np * Np];
-	__shared__ double sU[EPB * kNV * Np], sX[EPB * K8 * Nc], sB[EPB * K8 * Np], sM[EPB * 4 * Nc];
+	__shared__ T sI[Nc * Np], sDr[Nc * Np], sDs[Nc * Np], sIfc[F3 * Np], sW[Nc], sMinv[Np * Np];
+	__shared__ T sU[EPB * kNV * Np], sX[EPB * K8 * Nc], sB[EPB * K8 * Np], sM[EPB * 4 * Nc];
 	const int t = threadIdx.x;
 	for (int i = t; i < Nc * Np; i += 256) {
 		sI[i] = a.ops->I[i];
@@ -491,20 +494,20 @@ __global__ void __launch_bounds__(256) k_tri_gv(TriArgs a)
 		__syncthreads();
 		for (int i = t; i < ne * Nc; i += 256) {   // W J grad s at the cubature
 			const int sl = i / Nc, q = i - sl * Nc;
-			const double* U = sU + sl * kNV * Np;
-			double sr[kNV] = {}, ss[kNV] = {};
+			const T* U = sU + sl * kNV * Np;
+			T sr[kNV] = {}, ss[kNV] = {};
 #pragma unroll
 			for (int m = 0; m < Np; ++m) {
-				const double cr = sDr[q * Np + m], cs = sDs[q * Np + m];
+				const T cr = sDr[q * Np + m], cs = sDs[q * Np + m];
 #pragma unroll
 				for (int v = 0; v < kNV; ++v) {
 					sr[v] += cr * U[v * Np + m];
 					ss[v] += cs * U[v * Np + m];
 				}
 			}
-			const double* mp = sM + sl * 4 * Nc + q;
-			const double m0 = mp[0], m1 = mp[Nc], m2 = mp[2 * Nc], m3 = mp[3 * Nc], w = sW[q];
-			double* X = sX + sl * K8 * Nc;
+			const T* mp = sM + sl * 4 * Nc + q;
+			const T m0 = mp[0], m1 = mp[Nc], m2 = mp[2 * Nc], m3 = mp[3 * Nc], w = sW[q];
+			T* X = sX + sl * K8 * Nc;
 #pragma unroll
 			for (int v = 0; v < kNV; ++v) {
 				X[v * Nc + q] = w * (m0 * sr[v] + m2 * ss[v]);
@@ -514,33 +517,33 @@ __global__ void __launch_bounds__(256) k_tri_gv(TriArgs a)
 		__syncthreads();
 		for (int i = t; i < ne * Np; i += 256) {   // I^T (W J grad s)
 			const int sl = i / Np, m = i - sl * Np;
-			const double* X = sX + sl * K8 * Nc;
-			double acc[K8] = {};
+			const T* X = sX + sl * K8 * Nc;
+			T acc[K8] = {};
 #pragma unroll
 			for (int q = 0; q < Nc; ++q) {
-				const double c = sI[q * Np + m];
+				const T c = sI[q * Np + m];
 #pragma unroll
 				for (int k = 0; k < K8; ++k) acc[k] += c * X[k * Nc + q];
 			}
-			double* B = sB + sl * K8 * Np;
+			T* B = sB + sl * K8 * Np;
 #pragma unroll
 			for (int k = 0; k < K8; ++k) B[k * Np + m] = acc[k];
 		}
 		__syncthreads();
 		for (int i = t; i < ne * Np; i += 256) {   // G_v = M^-1 (...), kept in sX for the traces
 			const int sl = i / Np, m = i - sl * Np, e = e0 + sl;
-			const double* row;
-			double sc;
+			const T* row;
+			T sc;
 			tri_minv_row<Np>(a, sMinv, e, m, row, sc);
-			const double* B = sB + sl * K8 * Np;
-			double g[K8] = {};
+			const T* B = sB + sl * K8 * Np;
+			T g[K8] = {};
 #pragma unroll
 			for (int n = 0; n < Np; ++n) {
-				const double c = row[n];
+				const T c = row[n];
 #pragma unroll
 				for (int k = 0; k < K8; ++k) g[k] += c * B[k * Np + n];
 			}
-			double* G = sX + sl * K8 * Nc;
+			T* G = sX + sl * K8 * Nc;
 #pragma unroll
 			for (int k = 0; k < K8; ++k) {
 				g[k] *= sc;
@@ -551,12 +554,12 @@ __global__ void __launch_bounds__(256) k_tri_gv(TriArgs a)
 		__syncthreads();
 		for (int i = t; i < ne * F3; i += 256) {   // traces of G_v and of the state at the own GL face points
 			const int sl = i / F3, k0 = i - sl * F3, lf = k0 / N, b = k0 - lf * N;
-			const double* G = sX + sl * K8 * Nc;
-			const double* U = sU + sl * kNV * Np;
-			double acc[K8] = {}, st[kNV] = {};
+			const T* G = sX + sl * K8 * Nc;
+			const T* U = sU + sl * kNV * Np;
+			T acc[K8] = {}, st[kNV] = {};
 #pragma unroll
 			for (int m = 0; m < Np; ++m) {
-				const double c = sIfc[k0 * Np + m];
+				const T c = sIfc[k0 * Np + m];
 #pragma unroll
 				for (int k = 0; k < K8; ++k) acc[k] += c * G[k * Np + m];
 #pragma unroll
@@ -571,12 +574,12 @@ __global__ void __launch_bounds__(256) k_tri_gv(TriArgs a)
 	}
 }
 
-template <int P>
-__global__ void __launch_bounds__(256) k_tri_ns(TriArgs a)
+template <typename T, int P>
+__global__ void __launch_bounds__(256) k_tri_ns(TriArgsT<T> a)
 {
 	constexpr int Np = tri_np(P), Nc = tri_nc(P), N = P + 1, K8 = 2 * kNV, ZB = 3 * kNV * N, EPB = epb_ns(P);
-	__shared__ double sI[Nc * Np], sDrW[Np * Nc], sDsW[Np * Nc], sTw0[3 * Np * N], sMinv[Np * Np];
-	__shared__ double sU[EPB * kNV * Np], sZ[EPB * 3 * ZB], sG[EPB * K8 * Np], sFR[EPB * K8 * Nc], sM[EPB * 4 * Nc];
+	__shared__ T sI[Nc * Np], sDrW[Np * Nc], sDsW[Np * Nc], sTw0[3 * Np * N], sMinv[Np * Np];
+	__shared__ T sU[EPB * kNV * Np], sZ[EPB * 3 * ZB], sG[EPB * K8 * Np], sFR[EPB * K8 * Nc], sM[EPB * 4 * Nc];
 	const int t = threadIdx.x;
 	for (int i = t; i < Nc * Np; i += 256) {
 		sI[i] = a.ops->I[i];
@@ -585,7 +588,7 @@ __global__ void __launch_bounds__(256) k_tri_ns(TriArgs a)
 	}
 	for (int i = t; i < 3 * Np * N; i += 256) sTw0[i] = a.ops->tw0[i / (Np * N)][i % (Np * N)];
 	for (int i = t; i < Np * Np; i += 256) sMinv[i] = a.ops->Minv[i];
-	const double mu = a.mu, Pr = a.Pr;
+	const T mu = a.mu, Pr = a.Pr;
 	for (int e0 = blockIdx.x * EPB; e0 < a.E; e0 += gridDim.x * EPB) {
 		const int ne = min(EPB, a.E - e0);
 		__syncthreads();
@@ -598,61 +601,61 @@ __global__ void __launch_bounds__(256) k_tri_ns(TriArgs a)
 		// the face lifts of the gradient: sum_f tw0_f y_f at the nodes (into sFR, free until the fluxes)
 		for (int i = t; i < ne * Np; i += 256) {
 			const int sl = i / Np, m = i - sl * Np;
-			const double* Z = sZ + sl * 3 * ZB;
-			double acc[K8] = {};
+			const T* Z = sZ + sl * 3 * ZB;
+			T acc[K8] = {};
 #pragma unroll
 			for (int lf = 0; lf < 3; ++lf)
 #pragma unroll
 				for (int b = 0; b < N; ++b) {
-					const double c = sTw0[(lf * Np + m) * N + b];
+					const T c = sTw0[(lf * Np + m) * N + b];
 #pragma unroll
 					for (int k = 0; k < K8; ++k) acc[k] += c * Z[lf * ZB + kNV * N + k * N + b];
 				}
-			double* T = sFR + sl * K8 * Nc;
+			T* Lt = sFR + sl * K8 * Nc;
 #pragma unroll
-			for (int k = 0; k < K8; ++k) T[k * Np + m] = acc[k];
+			for (int k = 0; k < K8; ++k) Lt[k * Np + m] = acc[k];
 		}
 		__syncthreads();
 		for (int i = t; i < ne * Np; i += 256) {   // G = G_v + M^-1 t
 			const int sl = i / Np, m = i - sl * Np, e = e0 + sl;
-			const double* row;
-			double sc;
+			const T* row;
+			T sc;
 			tri_minv_row<Np>(a, sMinv, e, m, row, sc);
-			const double* T = sFR + sl * K8 * Nc;
-			double acc[K8] = {};
+			const T* Lt = sFR + sl * K8 * Nc;
+			T acc[K8] = {};
 #pragma unroll
 			for (int n = 0; n < Np; ++n) {
-				const double c = row[n];
+				const T c = row[n];
 #pragma unroll
-				for (int k = 0; k < K8; ++k) acc[k] += c * T[k * Np + n];
+				for (int k = 0; k < K8; ++k) acc[k] += c * Lt[k * Np + n];
 			}
-			double* G = sG + sl * K8 * Np;
+			T* G = sG + sl * K8 * Np;
 #pragma unroll
 			for (int k = 0; k < K8; ++k) G[k * Np + m] += sc * acc[k];
 		}
 		__syncthreads();
 		for (int i = t; i < ne * Nc; i += 256) {
 			const int sl = i / Nc, q = i - sl * Nc;
-			const double* U = sU + sl * kNV * Np;
-			const double* G = sG + sl * K8 * Np;
-			double s[kNV] = {}, gq[K8] = {};
+			const T* U = sU + sl * kNV * Np;
+			const T* G = sG + sl * K8 * Np;
+			T s[kNV] = {}, gq[K8] = {};
 #pragma unroll
 			for (int m = 0; m < Np; ++m) {
-				const double c = sI[q * Np + m];
+				const T c = sI[q * Np + m];
 #pragma unroll
 				for (int v = 0; v < kNV; ++v) s[v] += c * U[v * Np + m];
 #pragma unroll
 				for (int k = 0; k < K8; ++k) gq[k] += c * G[k * Np + m];
 			}
-			const double* mp = sM + sl * 4 * Nc + q;
-			const double m0 = mp[0], m1 = mp[Nc], m2 = mp[2 * Nc], m3 = mp[3 * Nc];
-			double f[K8], fv[K8];
-			euler_flux<double, 2>(s, f);
-			ns_viscous_flux2<double>(s, gq, mu, Pr, fv);
-			double* FR = sFR + sl * K8 * Nc;
+			const T* mp = sM + sl * 4 * Nc + q;
+			const T m0 = mp[0], m1 = mp[Nc], m2 = mp[2 * Nc], m3 = mp[3 * Nc];
+			T f[K8], fv[K8];
+			euler_flux<T, 2>(s, f);
+			ns_viscous_flux2<T>(s, gq, mu, Pr, fv);
+			T* FR = sFR + sl * K8 * Nc;
 #pragma unroll
 			for (int v = 0; v < kNV; ++v) {
-				const double f0 = f[2 * v] + fv[2 * v], f1 = f[2 * v + 1] + fv[2 * v + 1];
+				const T f0 = f[2 * v] + fv[2 * v], f1 = f[2 * v + 1] + fv[2 * v + 1];
 				FR[v * Nc + q] = m0 * f0 + m1 * f1;
 				FR[(kNV + v) * Nc + q] = m2 * f0 + m3 * f1;
 			}
@@ -660,12 +663,12 @@ __global__ void __launch_bounds__(256) k_tri_ns(TriArgs a)
 		__syncthreads();
 		for (int i = t; i < ne * Np; i += 256) {
 			const int sl = i / Np, m = i - sl * Np;
-			const double* FR = sFR + sl * K8 * Nc;
-			const double* Z = sZ + sl * 3 * ZB;
-			double acc[kNV] = {};
+			const T* FR = sFR + sl * K8 * Nc;
+			const T* Z = sZ + sl * 3 * ZB;
+			T acc[kNV] = {};
 #pragma unroll
 			for (int q = 0; q < Nc; ++q) {
-				const double cr = sDrW[m * Nc + q], cs = sDsW[m * Nc + q];
+				const T cr = sDrW[m * Nc + q], cs = sDsW[m * Nc + q];
 #pragma unroll
 				for (int v = 0; v < kNV; ++v) acc[v] += cr * FR[v * Nc + q] + cs * FR[(kNV + v) * Nc + q];
 			}
@@ -673,11 +676,11 @@ __global__ void __launch_bounds__(256) k_tri_ns(TriArgs a)
 			for (int lf = 0; lf < 3; ++lf)
 #pragma unroll
 				for (int b = 0; b < N; ++b) {
-					const double c = sTw0[(lf * Np + m) * N + b];
+					const T c = sTw0[(lf * Np + m) * N + b];
 #pragma unroll
 					for (int v = 0; v < kNV; ++v) acc[v] -= c * Z[lf * ZB + v * N + b];
 				}
-			double* R = a.r + (size_t)(e0 + sl) * kNV * Np + m;
+			T* R = a.r + (size_t)(e0 + sl) * kNV * Np + m;
 #pragma unroll
 			for (int v = 0; v < kNV; ++v) R[v * Np] = acc[v];
 		}
@@ -735,26 +738,28 @@ __global__ void __launch_bounds__(256) k_tri_minv(TriArgs a, unsigned long long*
 }
 
 // ------------------------------------------------------------------------------------------------ faces
-struct MixArgs {
+template <typename T>
+struct MixArgsT {
 	const MixFace* faces;              // the whole sorted list
-	const double* nJ;                  // Fmix x kHpMaxN x 2
-	const double* u;                   // the whole state
-	double* z;                         // the whole lifted-flux buffer
-	const HpOps<double>* hops;
-	const TriOps<double>* tops;        // indexed by p
-	const double* ttr[kTriMaxP];       // TRI groups: state traces
-	const double* tgvt[kTriMaxP];      // TRI groups: traces of G_v
-	const double* tK[kTriMaxP];        // curved TRI groups: each element's K_f (affine groups: null)
-	const double* tjinv[kTriMaxP];     // affine TRI groups: 1 / det J
-	const double* qgvt[kHpMaxN + 1];   // QUAD groups by N: traces of G_v and kappa
-	const double* qkap[kHpMaxN + 1];
+	const T* nJ;                  // Fmix x kHpMaxN x 2
+	const T* u;                   // the whole state
+	T* z;                         // the whole lifted-flux buffer
+	const HpOps<T>* hops;
+	const TriOps<T>* tops;        // indexed by p
+	const T* ttr[kTriMaxP];       // TRI groups: state traces
+	const T* tgvt[kTriMaxP];      // TRI groups: traces of G_v
+	const T* tK[kTriMaxP];        // curved TRI groups: each element's K_f (affine groups: null)
+	const T* tjinv[kTriMaxP];     // affine TRI groups: 1 / det J
+	const T* qgvt[kHpMaxN + 1];   // QUAD groups by N: traces of G_v and kappa
+	const T* qkap[kHpMaxN + 1];
 	double bc_prim[5];
 	double mu, Pr;
 	int start, count;
 };
 
 // value j of variable v of one side at its own face points (TRI: trace at its GL points; QUAD: GLL face node)
-__device__ inline double side_value(const MixArgs& a, int kind, int N, int lf, long long uoff, int el, int tg, int v,
+template <typename T>
+__device__ inline T side_value(const MixArgsT<T>& a, int kind, int N, int lf, long long uoff, int el, int tg, int v,
                                     int j)
 {
 	if (kind) return a.ttr[tg][(((size_t)el * 3 + lf) * kNV + v) * N + j];
@@ -776,16 +781,16 @@ __device__ inline void wave_sync()
 }
 
 // the operators of one face-point count NF, for the own point counts N = 2..NF (index N - 2)
-template <int NF>
+template <typename T, int NF>
 struct MixOps {
-	double Ig[NF - 1][NF * NF];   // own GLL points -> the NF face points ([q * N + i])
-	double Pg[NF - 1][NF * NF];   // own GL points -> the NF face points
-	double winv[NF - 1][NF];
-	double w[NF];
+	T Ig[NF - 1][NF * NF];   // own GLL points -> the NF face points ([q * N + i])
+	T Pg[NF - 1][NF * NF];   // own GL points -> the NF face points
+	T winv[NF - 1][NF];
+	T w[NF];
 };
 
-template <int NF>
-__device__ inline void stage_mix_ops(MixOps<NF>& o, const HpOps<double>* h)
+template <int NF, typename T>
+__device__ inline void stage_mix_ops(MixOps<T, NF>& o, const HpOps<T>* h)
 {
 	for (int i = threadIdx.x; i < (NF - 1) * NF * NF; i += blockDim.x) {
 		const int n = i / (NF * NF), k = i - n * NF * NF;
@@ -797,13 +802,13 @@ __device__ inline void stage_mix_ops(MixOps<NF>& o, const HpOps<double>* h)
 }
 
 // one side's state at face point q from its own values S[v * NF + i] (TRI: from GL points; QUAD: from GLL)
-template <int NF>
-__device__ inline void side_at_point(const MixOps<NF>& o, int kind, int Ns, int q, const double* S, double* out)
+template <int NF, typename T>
+__device__ inline void side_at_point(const MixOps<T, NF>& o, int kind, int Ns, int q, const T* S, T* out)
 {
-	const double* op = kind ? o.Pg[Ns - 2] : o.Ig[Ns - 2];
+	const T* op = kind ? o.Pg[Ns - 2] : o.Ig[Ns - 2];
 #pragma unroll
 	for (int v = 0; v < kNV; ++v) {
-		double acc = 0.0;
+		T acc = 0.0;
 #pragma unroll
 		for (int i = 0; i < NF; ++i)
 			if (i < Ns) acc += op[q * Ns + i] * S[v * NF + i];
@@ -812,33 +817,34 @@ __device__ inline void side_at_point(const MixOps<NF>& o, int kind, int Ns, int 
 }
 
 // z = P^T g / w_own at own point j of one side (g: [q][v] at the NF face points, in the left ordering)
-template <int NF>
-__device__ inline void lift_side(const MixOps<NF>& o, int Ns, int j, bool rev, bool neg, const double* g, double* z)
+template <int NF, typename T>
+__device__ inline void lift_side(const MixOps<T, NF>& o, int Ns, int j, bool rev, bool neg, const T* g, T* z)
 {
-	const double* Pg = o.Pg[Ns - 2];
+	const T* Pg = o.Pg[Ns - 2];
 #pragma unroll
 	for (int v = 0; v < kNV; ++v) {
-		double acc = 0.0;
+		T acc = 0.0;
 #pragma unroll
 		for (int q = 0; q < NF; ++q) acc += Pg[(rev ? NF - 1 - q : q) * Ns + j] * g[q * kNV + v];
-		z[v * Ns + j] = neg ? 0.0 - acc * o.winv[Ns - 2][j] : acc * o.winv[Ns - 2][j];
+		z[v * Ns + j] = neg ? T(0) - acc * o.winv[Ns - 2][j] : acc * o.winv[Ns - 2][j];
 	}
 }
 
 // one side's values at its own face points, loaded into registers by lane j (j < Ns)
-__device__ inline void load_own(const MixArgs& a, int kind, int Ns, int lf, long long uoff, int el, int tg, int j,
-                                double* own)
+template <typename T>
+__device__ inline void load_own(const MixArgsT<T>& a, int kind, int Ns, int lf, long long uoff, int el, int tg, int j,
+                                T* own)
 {
 #pragma unroll
 	for (int v = 0; v < kNV; ++v) own[v] = side_value(a, kind, Ns, lf, uoff, el, tg, v, j);
 }
 
-template <int NF, int FLUX>
-__global__ void __launch_bounds__(256) k_mixf(MixArgs a)
+template <typename T, int NF, int FLUX>
+__global__ void __launch_bounds__(256) k_mixf(MixArgsT<T> a)
 {
 	constexpr int LPF = mix_lpf(NF), FPB = mix_fpb(NF);
-	__shared__ MixOps<NF> o;
-	__shared__ double sF[FPB][2 * kNV * NF];   // both sides' own values [side][v][i], then w J_f F [q][v]
+	__shared__ MixOps<T, NF> o;
+	__shared__ T sF[FPB][2 * kNV * NF];   // both sides' own values [side][v][i], then w J_f F [q][v]
 	const int slot = threadIdx.x / LPF, j = threadIdx.x % LPF;
 	const int fl = blockIdx.x * FPB + slot;
 	const bool valid = fl < a.count;
@@ -846,16 +852,16 @@ __global__ void __launch_bounds__(256) k_mixf(MixArgs a)
 	const MixFace f = a.faces[fi];
 	const bool inter = f.bc == DGX_BC_INTERIOR;
 	// every global load of this lane first (one round trip), then the operators
-	double oL[kNV], oR[kNV], nj0 = 1.0, nj1 = 0.0;
+	T oL[kNV], oR[kNV], nj0 = 1.0, nj1 = 0.0;
 	if (valid && j < f.NL) load_own(a, f.kL, f.NL, f.lfL, f.uL, f.eL, f.tL, j, oL);
 	if (valid && inter && j < f.NR) load_own(a, f.kR, f.NR, f.lfR, f.uR, f.eR, f.tR, j, oR);
 	if (valid && j < NF) {
-		const double* nj = a.nJ + ((size_t)fi * kHpMaxN + j) * 2;
+		const T* nj = a.nJ + ((size_t)fi * kHpMaxN + j) * 2;
 		nj0 = nj[0];
 		nj1 = nj[1];
 	}
 	stage_mix_ops<NF>(o, a.hops);
-	double* F = sF[slot];
+	T* F = sF[slot];
 	if (valid && j < f.NL)
 #pragma unroll
 		for (int v = 0; v < kNV; ++v) F[v * NF + j] = oL[v];
@@ -864,16 +870,16 @@ __global__ void __launch_bounds__(256) k_mixf(MixArgs a)
 		for (int v = 0; v < kNV; ++v) F[kNV * NF + v * NF + j] = oR[v];
 	__syncthreads();   // the operators and every face's own values
 	const int q = j;
-	double g[kNV];
+	T g[kNV];
 	if (valid && j < NF) {
-		const double Jf = sqrt(nj0 * nj0 + nj1 * nj1);
-		const double nu[2] = { nj0 / Jf, nj1 / Jf };
-		double sL[kNV], sR[kNV], nnf[kNV];
+		const T Jf = sqrt(nj0 * nj0 + nj1 * nj1);
+		const T nu[2] = { nj0 / Jf, nj1 / Jf };
+		T sL[kNV], sR[kNV], nnf[kNV];
 		side_at_point<NF>(o, f.kL, f.NL, q, F, sL);
 		if (inter) side_at_point<NF>(o, f.kR, f.NR, f.rev ? NF - 1 - q : q, F + kNV * NF, sR);
-		else bc_state<double, 2>(f.bc, sL, nu, a.bc_prim, sR);
-		if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<double, 2>(sL, sR, nu, nnf);
-		else numflux_roe<double, 2>(sL, sR, nu, nnf);
+		else bc_state<T, 2>(f.bc, sL, nu, a.bc_prim, sR);
+		if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<T, 2>(sL, sR, nu, nnf);
+		else numflux_roe<T, 2>(sL, sR, nu, nnf);
 #pragma unroll
 		for (int v = 0; v < kNV; ++v) g[v] = o.w[q] * Jf * nnf[v];
 	}
@@ -888,43 +894,43 @@ __global__ void __launch_bounds__(256) k_mixf(MixArgs a)
 
 // what a side's face gradient at its own GL point j needs from memory: the trace of G_v there (t[k]) and its
 // row of the lift-trace operator (TRI: K_f row j times sc; QUAD: kappa in kr[0])
-template <int NF>
-__device__ inline void load_side_grad(const MixArgs& a, int kind, int Ns, int lf, int el, int tg, int j, double* t,
-                                      double* kr, double& sc)
+template <int NF, typename T>
+__device__ inline void load_side_grad(const MixArgsT<T>& a, int kind, int Ns, int lf, int el, int tg, int j, T* t,
+                                      T* kr, T& sc)
 {
 	if (kind) {
-		const double* tp = a.tgvt[tg] + ((size_t)el * 3 + lf) * 2 * kNV * Ns + j;
+		const T* tp = a.tgvt[tg] + ((size_t)el * 3 + lf) * 2 * kNV * Ns + j;
 #pragma unroll
 		for (int k = 0; k < 2 * kNV; ++k) t[k] = tp[k * Ns];
-		const double* K;
+		const T* K;
 		if (a.tK[tg]) {
 			K = a.tK[tg] + ((size_t)el * 3 + lf) * Ns * Ns + j * Ns;
-			sc = 1.0;
+			sc = T(1);
 		} else {
 			K = a.tops[Ns - 1].K[lf] + j * Ns;
 			sc = a.tjinv[tg][el];
 		}
 #pragma unroll
-		for (int b = 0; b < NF; ++b) kr[b] = b < Ns ? K[b] : 0.0;
+		for (int b = 0; b < NF; ++b) kr[b] = b < Ns ? K[b] : T(0);
 		return;
 	}
-	const double* tp = a.qgvt[Ns] + ((size_t)el * 4 + lf) * 2 * kNV * Ns + j;
+	const T* tp = a.qgvt[Ns] + ((size_t)el * 4 + lf) * 2 * kNV * Ns + j;
 #pragma unroll
 	for (int k = 0; k < 2 * kNV; ++k) t[k] = tp[k * Ns];
 	kr[0] = a.qkap[Ns][((size_t)el * 4 + lf) * Ns + j];
-	sc = 1.0;
+	sc = T(1);
 }
 
 // the face gradient h = trace G_v + eta (lift trace) y at own point j, y = Y[k * NF + b]
-template <int NF>
-__device__ inline void side_face_gradient(int kind, int Ns, int j, const double* t, const double* kr, double sc,
-                                          const double* Y, double* h)
+template <int NF, typename T>
+__device__ inline void side_face_gradient(int kind, int Ns, int j, const T* t, const T* kr, T sc,
+                                          const T* Y, T* h)
 {
-	constexpr double eta = 1.01 * 4.0;
+	constexpr T eta = T(1.01 * 4.0);
 	if (kind) {
 #pragma unroll
 		for (int k = 0; k < 2 * kNV; ++k) {
-			double acc = 0.0;
+			T acc = 0.0;
 #pragma unroll
 			for (int b = 0; b < NF; ++b)
 				if (b < Ns) acc += kr[b] * Y[k * NF + b];
@@ -936,33 +942,33 @@ __device__ inline void side_face_gradient(int kind, int Ns, int j, const double*
 	for (int k = 0; k < 2 * kNV; ++k) h[k] = t[k] + eta * kr[0] * Y[k * NF + j];
 }
 
-template <int NF, int FLUX>
-__global__ void __launch_bounds__(256) k_mixf_ns(MixArgs a)
+template <typename T, int NF, int FLUX>
+__global__ void __launch_bounds__(256) k_mixf_ns(MixArgsT<T> a)
 {
 	constexpr int LPF = mix_lpf(NF), FPB = mix_fpb(NF), K8 = 2 * kNV;
-	__shared__ MixOps<NF> o;
-	__shared__ double sY[FPB][2][K8 * NF];   // both sides' own values [side][v][i]; then each side's y [k][i], then h
-	__shared__ double sQ[FPB][NF * K8];      // w q at the face points [q][k]; then w J_f F [q][v]
+	__shared__ MixOps<T, NF> o;
+	__shared__ T sY[FPB][2][K8 * NF];   // both sides' own values [side][v][i]; then each side's y [k][i], then h
+	__shared__ T sQ[FPB][NF * K8];      // w q at the face points [q][k]; then w J_f F [q][v]
 	const int slot = threadIdx.x / LPF, j = threadIdx.x % LPF;
 	const int fl = blockIdx.x * FPB + slot;
 	const bool valid = fl < a.count;
 	const int fi = a.start + (valid ? fl : 0);
 	const MixFace f = a.faces[fi];
 	// the own values and normals first, then the operators
-	double oL[kNV], oR[kNV], nj0 = 1.0, nj1 = 0.0;
-	double tL[K8], tR[K8], kL[NF], kR[NF], scL = 1.0, scR = 1.0;
+	T oL[kNV], oR[kNV], nj0 = 1.0, nj1 = 0.0;
+	T tL[K8], tR[K8], kL[NF], kR[NF], scL = 1.0, scR = 1.0;
 	const bool ownL = valid && j < f.NL, ownR = valid && j < f.NR, point = valid && j < NF;
 	if (ownL) load_own(a, f.kL, f.NL, f.lfL, f.uL, f.eL, f.tL, j, oL);
 	if (ownR) load_own(a, f.kR, f.NR, f.lfR, f.uR, f.eR, f.tR, j, oR);
 	if (point) {
-		const double* nj = a.nJ + ((size_t)fi * kHpMaxN + j) * 2;
+		const T* nj = a.nJ + ((size_t)fi * kHpMaxN + j) * 2;
 		nj0 = nj[0];
 		nj1 = nj[1];
 	}
 	stage_mix_ops<NF>(o, a.hops);
-	double* YL = sY[slot][0];
-	double* YR = sY[slot][1];
-	double* Q = sQ[slot];
+	T* YL = sY[slot][0];
+	T* YR = sY[slot][1];
+	T* Q = sQ[slot];
 	if (ownL)
 #pragma unroll
 		for (int v = 0; v < kNV; ++v) YL[v * NF + j] = oL[v];
@@ -973,14 +979,14 @@ __global__ void __launch_bounds__(256) k_mixf_ns(MixArgs a)
 	// the gradient traces and lift-trace rows, needed from the face-gradient step on
 	if (ownL) load_side_grad<NF>(a, f.kL, f.NL, f.lfL, f.eL, f.tL, j, tL, kL, scL);
 	if (ownR) load_side_grad<NF>(a, f.kR, f.NR, f.lfR, f.eR, f.tR, j, tR, kR, scR);
-	double sL[kNV], sR[kNV];
+	T sL[kNV], sR[kNV];
 	const int q = j, qr = f.rev ? NF - 1 - q : q;
 	if (point) {
 		side_at_point<NF>(o, f.kL, f.NL, q, YL, sL);
 		side_at_point<NF>(o, f.kR, f.NR, qr, YL + kNV * NF, sR);
 #pragma unroll
 		for (int v = 0; v < kNV; ++v) {
-			const double dl = 0.5 * (sR[v] - sL[v]);
+			const T dl = T(0.5) * (sR[v] - sL[v]);
 			Q[q * K8 + v] = o.w[q] * (nj0 * dl);
 			Q[q * K8 + kNV + v] = o.w[q] * (nj1 * dl);
 		}
@@ -988,10 +994,10 @@ __global__ void __launch_bounds__(256) k_mixf_ns(MixArgs a)
 	wave_sync();   // the own values are read and every q is written
 	// projections y to each side's own GL points (written for the residual kernels, after the lifted flux)
 	if (ownL) {
-		const double* Pg = o.Pg[f.NL - 2];
+		const T* Pg = o.Pg[f.NL - 2];
 #pragma unroll
 		for (int k = 0; k < K8; ++k) {
-			double acc = 0.0;
+			T acc = 0.0;
 #pragma unroll
 			for (int qq = 0; qq < NF; ++qq) acc += Pg[qq * f.NL + j] * Q[qq * K8 + k];
 			acc *= o.winv[f.NL - 2][j];
@@ -1000,10 +1006,10 @@ __global__ void __launch_bounds__(256) k_mixf_ns(MixArgs a)
 		}
 	}
 	if (ownR) {
-		const double* Pg = o.Pg[f.NR - 2];
+		const T* Pg = o.Pg[f.NR - 2];
 #pragma unroll
 		for (int k = 0; k < K8; ++k) {
-			double acc = 0.0;
+			T acc = 0.0;
 #pragma unroll
 			for (int qq = 0; qq < NF; ++qq) acc += Pg[(f.rev ? NF - 1 - qq : qq) * f.NR + j] * Q[qq * K8 + k];
 			acc *= o.winv[f.NR - 2][j];
@@ -1012,7 +1018,7 @@ __global__ void __launch_bounds__(256) k_mixf_ns(MixArgs a)
 		}
 	}
 	wave_sync();
-	double hL[K8], hR[K8];
+	T hL[K8], hR[K8];
 	if (ownL) side_face_gradient<NF>(f.kL, f.NL, j, tL, kL, scL, YL, hL);
 	if (ownR) side_face_gradient<NF>(f.kR, f.NR, j, tR, kR, scR, YR, hR);
 	wave_sync();   // every y is read
@@ -1023,43 +1029,43 @@ __global__ void __launch_bounds__(256) k_mixf_ns(MixArgs a)
 #pragma unroll
 		for (int k = 0; k < K8; ++k) YR[k * NF + j] = hR[k];
 	wave_sync();
-	double fq[kNV];
+	T fq[kNV];
 	if (point) {   // the flux at this face point: inviscid first, then each side's face gradient P h and viscous flux
-		const double Jf = sqrt(nj0 * nj0 + nj1 * nj1);
-		const double nu[2] = { nj0 / Jf, nj1 / Jf };
-		double nnf[kNV], fvL[K8], fvR[K8];
-		if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<double, 2>(sL, sR, nu, nnf);
-		else numflux_roe<double, 2>(sL, sR, nu, nnf);
+		const T Jf = sqrt(nj0 * nj0 + nj1 * nj1);
+		const T nu[2] = { nj0 / Jf, nj1 / Jf };
+		T nnf[kNV], fvL[K8], fvR[K8];
+		if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<T, 2>(sL, sR, nu, nnf);
+		else numflux_roe<T, 2>(sL, sR, nu, nnf);
 		{
-			double g[K8];
-			const double* P = o.Pg[f.NL - 2];
+			T g[K8];
+			const T* P = o.Pg[f.NL - 2];
 #pragma unroll
 			for (int k = 0; k < K8; ++k) {
-				double acc = 0.0;
+				T acc = 0.0;
 #pragma unroll
 				for (int i = 0; i < NF; ++i)
 					if (i < f.NL) acc += P[q * f.NL + i] * YL[k * NF + i];
 				g[k] = acc;
 			}
-			ns_viscous_flux2<double>(sL, g, a.mu, a.Pr, fvL);
+			ns_viscous_flux2<T>(sL, g, a.mu, a.Pr, fvL);
 		}
 		{
-			double g[K8];
-			const double* P = o.Pg[f.NR - 2];
+			T g[K8];
+			const T* P = o.Pg[f.NR - 2];
 #pragma unroll
 			for (int k = 0; k < K8; ++k) {
-				double acc = 0.0;
+				T acc = 0.0;
 #pragma unroll
 				for (int i = 0; i < NF; ++i)
 					if (i < f.NR) acc += P[qr * f.NR + i] * YR[k * NF + i];
 				g[k] = acc;
 			}
-			ns_viscous_flux2<double>(sR, g, a.mu, a.Pr, fvR);
+			ns_viscous_flux2<T>(sR, g, a.mu, a.Pr, fvR);
 		}
-		const double wj = o.w[q] * Jf;
+		const T wj = o.w[q] * Jf;
 #pragma unroll
 		for (int v = 0; v < kNV; ++v) {
-			const double vis = nu[0] * 0.5 * (fvL[0 + 2 * v] + fvR[0 + 2 * v]) + nu[1] * 0.5 * (fvL[1 + 2 * v] + fvR[1 + 2 * v]);
+			const T vis = nu[0] * T(0.5) * (fvL[0 + 2 * v] + fvR[0 + 2 * v]) + nu[1] * T(0.5) * (fvL[1 + 2 * v] + fvR[1 + 2 * v]);
 			fq[v] = wj * (nnf[v] + vis);
 		}
 	}
@@ -1072,18 +1078,27 @@ __global__ void __launch_bounds__(256) k_mixf_ns(MixArgs a)
 	if (ownR) lift_side<NF>(o, f.NR, j, f.rev != 0, true, Q, a.z + f.zR);
 }
 
-TriArgs tri_args(const TriGroup& g, const TriOps<double>* ops, const double* u, double* r, const double* z)
+// a TRI group's buffers in the precision T (FP32: the shadows of the geometry and mass data; the per-evaluation
+// traces and gradients use the first half of the FP64-sized buffers)
+template <typename T>
+TriArgsT<T> tri_args(const TriGroup& g, const TriOps<T>* ops, const T* u, T* r, const T* z)
 {
-	TriArgs a{};
+	TriArgsT<T> a{};
 	a.u = u + g.u_off;
 	a.r = r ? r + g.u_off : nullptr;
-	a.met = g.met;
 	a.z = z ? z + g.z_off : nullptr;
-	a.tr = g.tr;
-	a.gv = g.gv;
-	a.gvt = g.gvt;
-	a.minv = g.minv;
-	a.jinv = g.jinv;
+	a.tr = reinterpret_cast<T*>(g.tr);
+	a.gv = reinterpret_cast<T*>(g.gv);
+	a.gvt = reinterpret_cast<T*>(g.gvt);
+	if constexpr (std::is_same<T, double>::value) {
+		a.met = g.met;
+		a.minv = g.minv;
+		a.jinv = g.jinv;
+	} else {
+		a.met = g.met32;
+		a.minv = g.minv32;
+		a.jinv = g.jinv32;
+	}
 	a.ops = ops + g.p;
 	a.E = g.E;
 	a.affine = g.affine;
@@ -1096,25 +1111,40 @@ int tri_grid(int E, int epb)
 	return std::max(1, std::min((E + epb - 1) / epb, 16384));
 }
 
-MixArgs mix_args(const HpData& H, const Ctx& c, const double* u)
+template <typename T>
+MixArgsT<T> mix_args(const HpData& H, const Ctx& c, const T* u)
 {
-	MixArgs a{};
+	constexpr bool f64 = std::is_same<T, double>::value;
+	MixArgsT<T> a{};
 	a.faces = H.mfaces;
-	a.nJ = H.mnJ;
 	a.u = u;
-	a.z = H.z64;
-	a.hops = H.ops64;
-	a.tops = H.tri_ops64;
-	for (size_t t = 0; t < H.tri.size() && t < (size_t)kTriMaxP; ++t) {
-		a.ttr[t] = H.tri[t].tr;
-		a.tgvt[t] = H.tri[t].gvt;
-		a.tK[t] = H.tri[t].affine ? nullptr : H.tri[t].K;
-		a.tjinv[t] = H.tri[t].jinv;
+	if constexpr (f64) {
+		a.nJ = H.mnJ;
+		a.z = H.z64;
+		a.hops = H.ops64;
+		a.tops = H.tri_ops64;
+	} else {
+		a.nJ = H.mnJ32;
+		a.z = H.z32;
+		a.hops = H.ops32;
+		a.tops = H.tri_ops32;
 	}
-	for (size_t g = 0; g < H.groups.size(); ++g) {
+	for (size_t t = 0; t < H.tri.size() && t < (size_t)kTriMaxP; ++t) {
+		const TriGroup& g = H.tri[t];
+		a.ttr[t] = reinterpret_cast<const T*>(g.tr);
+		a.tgvt[t] = reinterpret_cast<const T*>(g.gvt);
+		if constexpr (f64) {
+			a.tK[t] = g.affine ? nullptr : g.K;
+			a.tjinv[t] = g.jinv;
+		} else {
+			a.tK[t] = g.affine ? nullptr : g.K32;
+			a.tjinv[t] = g.jinv32;
+		}
+	}
+	for (size_t g = 0; g < H.groups.size(); ++g) {   // FP32 uses the first half of the FP64-sized buffers
 		const Ctx& s = H.groups[g]->c;
-		a.qgvt[s.N] = s.ns_gvt;
-		a.qkap[s.N] = s.ns_kappa;
+		a.qgvt[s.N] = reinterpret_cast<const T*>(s.ns_gvt);
+		a.qkap[s.N] = reinterpret_cast<const T*>(s.ns_kappa);
 	}
 	for (int k = 0; k < 5; ++k) a.bc_prim[k] = c.cfg.bc_prim[k];
 	a.mu = c.cfg.mu;
@@ -1123,9 +1153,10 @@ MixArgs mix_args(const HpData& H, const Ctx& c, const double* u)
 }
 
 // the faces with a TRI side, one launch per run of equal Nf, spread over the mesh's side streams
-hipError_t launch_mix(const HpData& H, const Ctx& c, const double* u, bool ns)
+template <typename T>
+hipError_t launch_mix(const HpData& H, const Ctx& c, const T* u, bool ns)
 {
-	MixArgs a = mix_args(H, c, u);
+	MixArgsT<T> a = mix_args<T>(H, c, u);
 	const bool lf = c.cfg.num_flux == DGX_FLUX_LF;
 	for (size_t i = 0; i < H.mruns.size(); ++i) {
 		const MixRun& run = H.mruns[i];
@@ -1136,11 +1167,11 @@ hipError_t launch_mix(const HpData& H, const Ctx& c, const double* u, bool ns)
 	case NFV: {                                                                                                \
 		const dim3 grid((run.count + mix_fpb(NFV) - 1) / mix_fpb(NFV)), block(64 * kMixWaves);                 \
 		if (ns) {                                                                                              \
-			if (lf) hipLaunchKernelGGL((k_mixf_ns<NFV, DGX_FLUX_LF>), grid, block, 0, st, a);                  \
-			else hipLaunchKernelGGL((k_mixf_ns<NFV, DGX_FLUX_ROE>), grid, block, 0, st, a);                    \
+			if (lf) hipLaunchKernelGGL((k_mixf_ns<T, NFV, DGX_FLUX_LF>), grid, block, 0, st, a);               \
+			else hipLaunchKernelGGL((k_mixf_ns<T, NFV, DGX_FLUX_ROE>), grid, block, 0, st, a);                 \
 		} else {                                                                                               \
-			if (lf) hipLaunchKernelGGL((k_mixf<NFV, DGX_FLUX_LF>), grid, block, 0, st, a);                     \
-			else hipLaunchKernelGGL((k_mixf<NFV, DGX_FLUX_ROE>), grid, block, 0, st, a);                       \
+			if (lf) hipLaunchKernelGGL((k_mixf<T, NFV, DGX_FLUX_LF>), grid, block, 0, st, a);                  \
+			else hipLaunchKernelGGL((k_mixf<T, NFV, DGX_FLUX_ROE>), grid, block, 0, st, a);                    \
 		}                                                                                                      \
 		break;                                                                                                 \
 	}
@@ -1166,12 +1197,30 @@ hipError_t launch_mix(const HpData& H, const Ctx& c, const double* u, bool ns)
 	case 5: hipLaunchKernelGGL(KERNEL<5>, dim3(tri_grid(g.E, EPB_OF(5))), dim3(256), 0, st, a); break;     \
 	default: return hipErrorInvalidValue;                                                                  \
 	}
+// the same for the kernels templated on the scalar type
+#define DGX_TRI_SWITCH_T(KERNEL, T, EPB_OF)                                                                    \
+	switch (g.p) {                                                                                             \
+	case 1: hipLaunchKernelGGL((KERNEL<T, 1>), dim3(tri_grid(g.E, EPB_OF(1))), dim3(256), 0, st, a); break;    \
+	case 2: hipLaunchKernelGGL((KERNEL<T, 2>), dim3(tri_grid(g.E, EPB_OF(2))), dim3(256), 0, st, a); break;    \
+	case 3: hipLaunchKernelGGL((KERNEL<T, 3>), dim3(tri_grid(g.E, EPB_OF(3))), dim3(256), 0, st, a); break;    \
+	case 4: hipLaunchKernelGGL((KERNEL<T, 4>), dim3(tri_grid(g.E, EPB_OF(4))), dim3(256), 0, st, a); break;    \
+	case 5: hipLaunchKernelGGL((KERNEL<T, 5>), dim3(tri_grid(g.E, EPB_OF(5))), dim3(256), 0, st, a); break;    \
+	default: return hipErrorInvalidValue;                                                                      \
+	}
 
 hipError_t launch_tri_trace64(const TriGroup& g, const TriOps<double>* ops, const double* u, hipStream_t st)
 {
 	if (g.E == 0) return hipSuccess;
-	TriArgs a = tri_args(g, ops, u, nullptr, nullptr);
-	DGX_TRI_SWITCH(k_tri_trace, epb_trace)
+	TriArgs a = tri_args<double>(g, ops, u, nullptr, nullptr);
+	DGX_TRI_SWITCH_T(k_tri_trace, double, epb_trace)
+	return hipGetLastError();
+}
+
+hipError_t launch_tri_trace32(const TriGroup& g, const TriOps<float>* ops, const float* u, hipStream_t st)
+{
+	if (g.E == 0) return hipSuccess;
+	TriArgsT<float> a = tri_args<float>(g, ops, u, nullptr, nullptr);
+	DGX_TRI_SWITCH_T(k_tri_trace, float, epb_trace)
 	return hipGetLastError();
 }
 
@@ -1191,12 +1240,21 @@ hipError_t launch_tri_rhs64(const TriGroup& g, const TriOps<double>* ops, const 
                             const double* z, hipStream_t st)
 {
 	if (g.E == 0) return hipSuccess;
-	TriArgs a = tri_args(g, ops, u, r, z);
+	TriArgs a = tri_args<double>(g, ops, u, r, z);
 	if (tri_use_mfma()) {
 		DGX_TRI_SWITCH(k_tri_rhs_mfma, epb_mfma)
 	} else {
-		DGX_TRI_SWITCH(k_tri_rhs, epb_rhs)
+		DGX_TRI_SWITCH_T(k_tri_rhs, double, epb_rhs)
 	}
+	return hipGetLastError();
+}
+
+hipError_t launch_tri_rhs32(const TriGroup& g, const TriOps<float>* ops, const float* u, float* r, const float* z,
+                            hipStream_t st)
+{
+	if (g.E == 0) return hipSuccess;
+	TriArgsT<float> a = tri_args<float>(g, ops, u, r, z);
+	DGX_TRI_SWITCH_T(k_tri_rhs, float, epb_rhs)
 	return hipGetLastError();
 }
 
@@ -1204,15 +1262,30 @@ hipError_t launch_tri_ns_phase64(const TriGroup& g, const TriOps<double>* ops, c
                                  const double* z, double mu, double Pr, int phase, hipStream_t st)
 {
 	if (g.E == 0) return hipSuccess;
-	TriArgs a = tri_args(g, ops, u, r, z);
+	TriArgs a = tri_args<double>(g, ops, u, r, z);
 	a.mu = mu;
 	a.Pr = Pr;
 	if (phase == 1) {
-		DGX_TRI_SWITCH(k_tri_gv, epb_gv)
+		DGX_TRI_SWITCH_T(k_tri_gv, double, epb_gv)
 	} else if (g.affine && tri_use_mfma()) {
 		DGX_TRI_SWITCH(k_tri_ns_mfma, epb_mfma)
 	} else {
-		DGX_TRI_SWITCH(k_tri_ns, epb_ns)
+		DGX_TRI_SWITCH_T(k_tri_ns, double, epb_ns)
+	}
+	return hipGetLastError();
+}
+
+hipError_t launch_tri_ns_phase32(const TriGroup& g, const TriOps<float>* ops, const float* u, float* r,
+                                 const float* z, double mu, double Pr, int phase, hipStream_t st)
+{
+	if (g.E == 0) return hipSuccess;
+	TriArgsT<float> a = tri_args<float>(g, ops, u, r, z);
+	a.mu = mu;
+	a.Pr = Pr;
+	if (phase == 1) {
+		DGX_TRI_SWITCH_T(k_tri_gv, float, epb_gv)
+	} else {
+		DGX_TRI_SWITCH_T(k_tri_ns, float, epb_ns)
 	}
 	return hipGetLastError();
 }
@@ -1221,7 +1294,7 @@ hipError_t launch_tri_minv64(const TriGroup& g, const TriOps<double>* ops, doubl
                              hipStream_t st)
 {
 	if (g.E == 0) return hipSuccess;
-	TriArgs a = tri_args(g, ops, r, r, nullptr);
+	TriArgs a = tri_args<double>(g, ops, r, r, nullptr);
 	switch (g.p) {
 	case 1: hipLaunchKernelGGL(k_tri_minv<1>, dim3(tri_grid(g.E, epb_minv(1))), dim3(256), 0, st, a, dmax); break;
 	case 2: hipLaunchKernelGGL(k_tri_minv<2>, dim3(tri_grid(g.E, epb_minv(2))), dim3(256), 0, st, a, dmax); break;
@@ -1234,6 +1307,7 @@ hipError_t launch_tri_minv64(const TriGroup& g, const TriOps<double>* ops, doubl
 }
 
 #undef DGX_TRI_SWITCH
+#undef DGX_TRI_SWITCH_T
 
 hipError_t launch_mix_faces64(const HpData& H, const Ctx& c, const double* u, hipStream_t)
 {
@@ -1243,6 +1317,11 @@ hipError_t launch_mix_faces64(const HpData& H, const Ctx& c, const double* u, hi
 hipError_t launch_mix_faces_ns64(const HpData& H, const Ctx& c, const double* u, hipStream_t)
 {
 	return launch_mix(H, c, u, true);
+}
+
+hipError_t launch_mix_faces32(const HpData& H, const Ctx& c, const float* u, bool ns)
+{
+	return launch_mix(H, c, u, ns);
 }
 
 }  // namespace dgx

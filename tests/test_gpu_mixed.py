@@ -7,6 +7,8 @@ Navier-Stokes (BR2) on periodic mixed meshes against MixedOracle.rhs_ns, same ba
 p = 1, 3, 5, random orders on curved meshes with both fluxes, an all-TRI curved mesh; free stream, conservation.
 Explicit path: M^-1 R (dgx_compute_rhs_explicit, the TRI mass inverse from the reference's scaled by 1/det J on
 affine elements) against the oracle's dense solve, with its max-norm; three SSP-RK3 steps against the oracle's.
+FP32 (dgx_compute_rhs_f32: FP32 state, geometry and operators) against the FP64 residual, bounded by a multiple
+of the error that rounding the state alone to FP32 causes (the FP64 residual of the rounded state).
 Also: graph replays equal the eager call, and the entry points not built for TRI elements refuse the mesh
 instead of computing something else.
 """
@@ -139,10 +141,8 @@ def test_entry_points_not_built_for_tri_refuse(dev):
     name, mesh, flux, _ = CASES[0]
     ctx = _ctx(mesh, flux, None)
     u = torch.from_numpy(_state(mesh, None)).to(dev)
-    r = torch.empty_like(u)
-    u32, r32 = u.float(), torch.empty(u.numel(), dtype=torch.float32, device=dev)
-    for call in (lambda: ctx.compute_rhs_f32(u32.data_ptr(), r32.data_ptr()),
-                 lambda: ctx.compute_rhs_mixed(u.data_ptr(), r32.data_ptr()),
+    r32 = torch.empty(u.numel(), dtype=torch.float32, device=dev)
+    for call in (lambda: ctx.compute_rhs_mixed(u.data_ptr(), r32.data_ptr()),
                  lambda: ctx.ssp_rk3_step(u.data_ptr(), 1e-4, positivity=True)):
         with pytest.raises(_lib.DgxError) as ei:
             call()
@@ -320,4 +320,46 @@ def test_mixed_mesh_ssp_rk3_matches_oracle(dev):
     print(f"SSP-RK3 x3 on {name}: rel err {err:.2e}, change {np.abs(ref - u).max():.2e}", flush=True)
     assert np.abs(ref - u).max() > 1e-6
     assert err < 1e-13
+    ctx.close()
+
+
+# ---------------------------------------------------------------------------------------------- FP32
+F32_CASES = [(c[0], c[1], c[2], c[3], M.PDE_EULER) for c in CASES] + [(c[0], c[1], c[2], None, M.PDE_NAVIER_STOKES)
+                                                                     for c in NS_CASES]
+
+
+@pytest.mark.parametrize("case", F32_CASES, ids=[c[0] for c in F32_CASES])
+def test_mixed_mesh_fp32_residual(dev, case):
+    """FP32 residual on mixed meshes: within 8x the input-rounding floor of the FP64 residual (the FP64 residual
+    of the FP32-rounded state against that of the state), both relative to max |R|; graph replays of the FP32
+    residual equal its eager call."""
+    from dpgsolver_amd import _lib
+
+    name, mesh, flux, bc_prim, pde = case
+    ctx = _ctx(mesh, flux, bc_prim) if pde == M.PDE_EULER else _ns_ctx(mesh, flux)
+    ud = torch.from_numpy(_state(mesh, bc_prim)).to(dev)
+    r64 = torch.empty_like(ud)
+    ctx.compute_rhs(ud.data_ptr(), r64.data_ptr())
+    u32 = ud.float()
+    u64r = u32.double()
+    r64r = torch.empty_like(ud)
+    ctx.compute_rhs(u64r.data_ptr(), r64r.data_ptr())
+    r32 = torch.full_like(u32, float("nan"))
+    ctx.compute_rhs_f32(u32.data_ptr(), r32.data_ptr())
+    torch.cuda.synchronize()
+    scale = r64.abs().max().item()
+    e32 = (r32.double() - r64).abs().max().item() / scale
+    floor = (r64r - r64).abs().max().item() / scale
+    print(f"case {name}: FP32 rel err {e32:.2e}, input-rounding floor {floor:.2e}", flush=True)
+    assert torch.isfinite(r32).all()
+    assert e32 < 8 * floor, (e32, floor)
+    r = torch.full_like(u32, float("nan"))
+    g = ctx.graph_rhs(_lib.GRAPH_RHS_FP32, u32.data_ptr(), r.data_ptr())
+    for _ in range(2):
+        r.fill_(float("nan"))
+        g.launch(1)
+        g.sync()
+        assert torch.equal(r, r32)
+        torch.cuda.synchronize()
+    g.close()
     ctx.close()
