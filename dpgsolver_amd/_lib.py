@@ -45,7 +45,8 @@ class DgxError(RuntimeError):
 
 class DgxConfig(C.Structure):
     _fields_ = [("d", C.c_int), ("p", C.c_int), ("pde", C.c_int), ("num_flux", C.c_int),
-                ("b_adv", C.c_double * 3), ("bc_prim", C.c_double * 5), ("mu", C.c_double), ("Pr", C.c_double)]
+                ("b_adv", C.c_double * 3), ("bc_prim", C.c_double * 5), ("mu", C.c_double), ("Pr", C.c_double),
+                ("wall_nf_E", C.c_double), ("wall_omega", C.c_double), ("wall_rho", C.c_double), ("wall_E", C.c_double)]
 
 
 class DgxHostMesh(C.Structure):
@@ -182,12 +183,15 @@ class Context:
     """One DG discretisation on the device (owns geometry; state buffers are the caller's)."""
 
     def __init__(self, d: int, p: int, pde: int, num_flux: int, b_adv=(0.0, 0.0, 0.0), bc_prim=None,
-                 mu: float = 0.0, Pr: float = 0.72):
-        """pde PDE_NAVIER_STOKES: constant viscosity mu and Prandtl number Pr (2-D, periodic meshes)."""
+                 mu: float = 0.0, Pr: float = 0.72, walls=None):
+        """pde PDE_NAVIER_STOKES: constant viscosity mu and Prandtl number Pr (2-D); walls = (nf_E, omega, rho_b,
+        E_b), the no-slip wall data (dgx_config wall_nf_E, wall_omega, wall_rho, wall_E)."""
         L = load()
         cfg = DgxConfig()
         cfg.d, cfg.p, cfg.pde, cfg.num_flux = d, p, pde, num_flux
         cfg.mu, cfg.Pr = float(mu), float(Pr)
+        if walls is not None:
+            cfg.wall_nf_E, cfg.wall_omega, cfg.wall_rho, cfg.wall_E = (float(w) for w in walls)
         for k, v in enumerate(b_adv):
             cfg.b_adv[k] = float(v)
         if bc_prim is not None:

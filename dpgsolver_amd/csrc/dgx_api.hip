@@ -73,7 +73,7 @@ void free_mesh(Ctx& c)
 	release(c.xyz); release(c.faces); release(c.elem_faces); release(c.order); release(c.hmin); release(c.metrics_vc); release(c.metrics_vc32);
 	release(c.detJ_vc); release(c.detJ_vc32); release(c.nJ_fc); release(c.nJ_fc32); release(c.u0); release(c.rtmp);
 	release(c.color_elems);
-	release(c.ns_gvt); release(c.ns_kappa);
+	release(c.ns_gvt); release(c.ns_kappa); release(c.wall_xy);
 	c.color_off.clear();
 	for (int& g : c.rhs_grid) g = 0;   // launch shapes are tuned per mesh
 	c.mesh_ready = false;
@@ -82,6 +82,8 @@ void free_mesh(Ctx& c)
 size_t ndof(const Ctx& c) { return c.hp ? (size_t)c.hp->ndof : (size_t)c.E * c.Np * c.nv; }
 
 dgx_status upload_uniform(dgx_ctx* h, const dgx_mesh* m, bool nonconf_ok);
+void lagrange_at(const double* xn, int n, double x, double* l, double* dl);
+int face_node_2d(int N, int lf, int i);
 
 }  // namespace
 
@@ -201,8 +203,17 @@ dgx_status upload_uniform(dgx_ctx* h, const dgx_mesh* m, bool nonconf_ok)
 		} else if (nonconf_ok && bc == dgx::kBcNonconforming) {
 			// face to an element of another order: its flux arrives lifted from the face kernel
 		} else {
-			if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;   // viscous BCs not built
-			if (c.cfg.pde != DGX_PDE_EULER || (bc != DGX_BC_RIEMANN && bc != DGX_BC_SLIPWALL)) return DGX_ERR_ARG;
+			// Euler: Riemann and slip walls; Navier-Stokes (2-D): Riemann and the no-slip walls
+			// (set_function_pointers_num_flux_bc_{euler,navier_stokes}, face_solver_T.c:251-298)
+			const bool noslip = bc == DGX_BC_NOSLIP_ADIABATIC || bc == DGX_BC_NOSLIP_DIABATIC || bc == DGX_BC_NOSLIP_ROTATING;
+			if (bc != DGX_BC_RIEMANN && bc != DGX_BC_SLIPWALL && !noslip) return DGX_ERR_ARG;
+			if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) {
+				if (d != 2 || bc == DGX_BC_SLIPWALL) return DGX_ERR_UNSUPPORTED;
+			} else if (c.cfg.pde != DGX_PDE_EULER) {
+				return DGX_ERR_ARG;
+			} else if (noslip) {
+				return DGX_ERR_UNSUPPORTED;
+			}
 		}
 		dgx::ElemFace& L = ef[(size_t)eL * NFACE + fL];
 		if (L.f >= 0) return DGX_ERR_ARG;  // local face used twice
@@ -269,6 +280,35 @@ dgx_status upload_uniform(dgx_ctx* h, const dgx_mesh* m, bool nonconf_ok)
 		// face traces of the volume gradient and the BR2 lift trace factors
 		DGX_TRY(hipMalloc(&c.ns_gvt, sizeof(double) * (size_t)c.E * NFACE * d * c.nv * c.Nfc));
 		DGX_TRY(hipMalloc(&c.ns_kappa, sizeof(double) * (size_t)c.E * NFACE * c.Nfc));
+		// rotating walls: the physical coordinates of the face GL points (the wall velocity omega (-y, x) there),
+		// interpolated from the element's GLL nodes on the face
+		bool rotating = false;
+		for (int f = 0; f < m->F; ++f) rotating |= m->faces[6 * f + 5] == DGX_BC_NOSLIP_ROTATING;
+		if (rotating) {
+			const int N = c.N, Np = c.Np;
+			double xl[dgx::kMaxN], wl[dgx::kMaxN], xg[dgx::kMaxN], wg[dgx::kMaxN], l[dgx::kMaxN], dl[dgx::kMaxN];
+			dgx::gauss_lobatto(N, xl, wl);
+			dgx::gauss_legendre(N, xg, wg);
+			std::vector<double> xy((size_t)c.F * N * 2, 0.0);
+			for (int f = 0; f < m->F; ++f) {
+				const int* r = m->faces + 6 * f;
+				if (r[5] != DGX_BC_NOSLIP_ROTATING) continue;
+				const double* xe = m->xyz + (size_t)r[0] * 2 * Np;
+				for (int q = 0; q < N; ++q) {
+					lagrange_at(xl, N, xg[q], l, dl);
+					double px = 0.0, py = 0.0;
+					for (int i = 0; i < N; ++i) {
+						const int vn = face_node_2d(N, r[1], i);
+						px += l[i] * xe[vn];
+						py += l[i] * xe[Np + vn];
+					}
+					xy[((size_t)f * N + q) * 2] = px;
+					xy[((size_t)f * N + q) * 2 + 1] = py;
+				}
+			}
+			DGX_TRY(hipMalloc(&c.wall_xy, sizeof(double) * xy.size()));
+			DGX_TRY(hipMemcpy(c.wall_xy, xy.data(), sizeof(double) * xy.size(), hipMemcpyHostToDevice));
+		}
 	}
 	DGX_TRY(dgx::launch_geometry(c));
 	DGX_TRY(dgx::launch_convert(c.metrics_vc, c.metrics_vc32, nm, c.stream));
@@ -610,6 +650,8 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 			continue;
 		}
 		if (bc != DGX_BC_INTERIOR) {
+			// Navier-Stokes walls and Riemann faces are built for uniform-order meshes only
+			if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;
 			push(gL, eL - e_off[gL], fL, -1, -1, ord, bc);
 			continue;
 		}

@@ -226,6 +226,7 @@ struct Ctx {
 	dgx_mlp* mlp = nullptr;         // attached learned preconditioner (not owned)
 	double* ns_gvt = nullptr;       // Navier-Stokes: face traces of the volume gradient (E x 2d x d x nv x Nfc)
 	double* ns_kappa = nullptr;     // Navier-Stokes: BR2 lift trace factor per face node (E x 2d x Nfc)
+	double* wall_xy = nullptr;      // Navier-Stokes with rotating walls: face GL point coordinates (F x Nfc x 2)
 	int rhs_grid[4] = { 0, 0, 0, 0 };   // residual launch grids by TUNE_* (0 = one workgroup per element block)
 	HpData* hp = nullptr;           // variable-order mesh (then this context holds no uniform mesh)
 	const double* hp_z64 = nullptr; // order group of a variable-order mesh: its part of the lifted fluxes

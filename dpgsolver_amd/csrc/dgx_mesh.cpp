@@ -69,6 +69,9 @@ int bc_code(int tag)
 	switch (tag % 10000) {
 	case 101: return DGX_BC_RIEMANN;
 	case 102: return DGX_BC_SLIPWALL;
+	case 111: return DGX_BC_NOSLIP_ADIABATIC;
+	case 112: return DGX_BC_NOSLIP_DIABATIC;
+	case 113: return DGX_BC_NOSLIP_ROTATING;
 	default: return -1;
 	}
 }

@@ -1,4 +1,5 @@
-// k_ns.hip -- 2-D Navier-Stokes residual with BR2 viscous terms (QUAD, interior faces), FP64 and FP32
+// k_ns.hip -- 2-D Navier-Stokes residual with BR2 viscous terms (QUAD; interior, Riemann and no-slip wall faces),
+// FP64 and FP32
 // (FP32: the FP32 state, geometry shadow and operator tables, as the FP32 Euler residual), and the mixed
 // form (PERT: FP64 state in, FP32 contractions on u - c_e, FP64 pointwise fluxes, as the Euler mixed kernel).
 //
@@ -19,6 +20,10 @@
 //     central n . (fv_L + fv_R) / 2 (numerical_flux_central_T.c:76-104) of the two sides' face gradients
 //     trace(G_v) + eta kappa q, scaled by J_f and lifted.  Both sides of a face evaluate it from identical
 //     inputs in the left ordering (the traces of G_v and kappa are read back from k_ns_gv's output by both).
+//     A boundary face takes the boundary state (ns_wall_state) as its right trace, so q = J_f n (s_b - s_L) / 2
+//     (compute_g_coef_related_boundary, compute_grad_coef_dg_T.c:527-575), the interior face gradient as the
+//     boundary one, and on the no-slip flux walls the provided energy flux nf_E in place of the energy
+//     equation's numerical flux.
 #include "dgx_internal.h"
 #include "dgx_device.h"
 
@@ -59,7 +64,39 @@ struct NsArgs {
 	// variable-order meshes: per element face to an element of another order, the lifted face flux [v][a]
 	// and then the projected face products q [d][v][a] (own GL face points and ordering)
 	const S* zf;
+	// boundary faces: the Riemann free stream, the no-slip wall data and (rotating walls) the face GL point
+	// coordinates, F x N x 2
+	double bc_prim[5];
+	double wall_nfE, wall_omega, wall_rho, wall_E;
+	const double* wall_xy;
 };
+
+// Boundary (ghost) state of a boundary face node, in FP64: Riemann (boundary_euler_T.c:99-180); the no-slip
+// adiabatic / diabatic walls keep rho and E and negate the momentum (-rho u_l + 2 rho u_ex, u_ex = 0;
+// constructor_Boundary_Value_T_navier_stokes_no_slip_flux_general, boundary_navier_stokes_T.c:260-290); the
+// rotating wall is 2 s_ex - s_l with s_ex = (rho_b, rho_b u_ex, E_b), u_ex = omega (-y, x) (no_slip_all_general
+// :200-240, compute_uvw_ex_rotating :505-520).  The boundary gradient is the interior one (grad_from_internal).
+template <typename S>
+__device__ inline void ns_wall_state(const NsArgs<S>& a, int bc, const double* sL, const double* n, const double* xy,
+                                     double* sB)
+{
+	if (bc == DGX_BC_RIEMANN) {
+		bc_state<double, 2>(bc, sL, n, a.bc_prim, sB);
+		return;
+	}
+	if (bc == DGX_BC_NOSLIP_ROTATING) {
+		const double ux = -a.wall_omega * xy[1], uy = a.wall_omega * xy[0];
+		sB[0] = -sL[0] + 2.0 * a.wall_rho;
+		sB[1] = -sL[1] + 2.0 * a.wall_rho * ux;
+		sB[2] = -sL[2] + 2.0 * a.wall_rho * uy;
+		sB[3] = -sL[3] + 2.0 * a.wall_E;
+		return;
+	}
+	sB[0] = sL[0];
+	sB[1] = -sL[1];
+	sB[2] = -sL[2];
+	sB[3] = sL[3];
+}
 
 // shared per-element set-up: operators, the GLL state, its GL values and (k_rhs_ns) the GL face traces
 template <typename S, int N>
@@ -223,8 +260,10 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 		for (int item = t; item < 4 * N; item += Np) {
 			const int lf = item / N, i = item % N;
 			const ElemFace f = B.ef[slc][lf];
-			const int fn = f.packed & 15;
-			const int en = f.en >= 0 ? f.en : e;   // every face is interior (checked at mesh upload)
+			const int fn = f.packed & 15, bc = (f.packed >> 12) & 15;
+			const int en = f.en >= 0 ? f.en : e;   // boundary faces: the own element (its value is replaced below)
+			S* nbt = TX[slc] + 4 * kNsNV * N;
+			S aos[kNsNV];
 #pragma unroll
 			for (int v = 0; v < kNsNV; ++v) {
 				S ao = S(0.0), an = S(0.0);
@@ -237,7 +276,25 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 						an += B.I[i * N + b] * a.u[((size_t)en * kNsNV + v) * Np + ns_face_node<N>(fn, b)];
 				}
 				TRo[slc][(lf * kNsNV + v) * N + i] = ao;
-				(TX[slc] + 4 * kNsNV * N)[(lf * kNsNV + v) * N + i] = an;
+				nbt[(lf * kNsNV + v) * N + i] = an;
+				aos[v] = ao;
+			}
+			if (f.en < 0 && bc != kBcNonconforming) {
+				// boundary face: the boundary state from the own trace (PERT: from c_e + delta, back to a delta)
+				double sl[kNsNV], sb[kNsNV], nd[2], xy[2] = { 0.0, 0.0 };
+#pragma unroll
+				for (int v = 0; v < kNsNV; ++v) sl[v] = PERT ? B.C[slc][v] + (double)aos[v] : (double)aos[v];
+				const S* nJ = a.nJ + ((size_t)f.f * N + i) * 2;
+				const double Jd = sqrt((double)nJ[0] * nJ[0] + (double)nJ[1] * nJ[1]);
+				nd[0] = (double)nJ[0] / Jd;
+				nd[1] = (double)nJ[1] / Jd;
+				if (bc == DGX_BC_NOSLIP_ROTATING) {
+					xy[0] = a.wall_xy[((size_t)f.f * N + i) * 2];
+					xy[1] = a.wall_xy[((size_t)f.f * N + i) * 2 + 1];
+				}
+				ns_wall_state<S>(a, bc, sl, nd, xy, sb);
+#pragma unroll
+				for (int v = 0; v < kNsNV; ++v) nbt[(lf * kNsNV + v) * N + i] = PERT ? S(sb[v] - B.C[slc][v]) : S(sb[v]);
 			}
 		}
 	}
@@ -294,6 +351,9 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 			const S kL = a.kap[((size_t)eLf * 4 + lfL) * N + iL], kR = a.kap[((size_t)eRf * 4 + lfR) * N + iR];
 			const S* nJ = a.nJ + ((size_t)fc.f * N + i) * 2;
 			S sL[kNsNV], sR[kNsNV], gL[2 * kNsNV], gR[2 * kNsNV];
+			const bool wall = fc.en < 0;   // boundary face: the boundary gradient is the interior one
+			const int bcf = (fc.packed >> 12) & 15;
+			const bool nf_provided = bcf == DGX_BC_NOSLIP_ADIABATIC || bcf == DGX_BC_NOSLIP_DIABATIC;
 #pragma unroll
 			for (int v = 0; v < kNsNV; ++v) {
 				sL[v] = side == 0 ? own[v * N + i] : nb[v * N + i];
@@ -303,7 +363,7 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 				for (int d = 0; d < 2; ++d) {
 					const S q = nJ[d] * dl;
 					gL[v + kNsNV * d] = tvL[(d * kNsNV + v) * N + iL] + eta * kL * q;
-					gR[v + kNsNV * d] = tvR[(d * kNsNV + v) * N + iR] + eta * kR * q;
+					gR[v + kNsNV * d] = wall ? gL[v + kNsNV * d] : tvR[(d * kNsNV + v) * N + iR] + eta * kR * q;
 				}
 			}
 			const S Jf = tsqrt(nJ[0] * nJ[0] + nJ[1] * nJ[1]);
@@ -340,7 +400,9 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 #pragma unroll
 				for (int v = 0; v < kNsNV; ++v) {
 					const double vis = nd[0] * 0.5 * (fvL[0 + 2 * v] + fvR[0 + 2 * v]) + nd[1] * 0.5 * (fvL[1 + 2 * v] + fvR[1 + 2 * v]);
-					const S r = S((fdn[v] - cn[v] + vis) * Jd);
+					// no-slip flux walls: the energy flux is the provided one (numerical_flux_central_T.c:222-230)
+					const double tot = (nf_provided && v == kNsNV - 1) ? a.wall_nfE : fdn[v] + vis;
+					const S r = S((tot - cn[v]) * Jd);
 					RES[slc][(lf * kNsNV + v) * N + dst] = side == 0 ? r : -r;
 				}
 			} else {
@@ -352,7 +414,9 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 #pragma unroll
 				for (int v = 0; v < kNsNV; ++v) {
 					const S vis = nu[0] * S(0.5) * (fvL[0 + 2 * v] + fvR[0 + 2 * v]) + nu[1] * S(0.5) * (fvL[1 + 2 * v] + fvR[1 + 2 * v]);
-					const S r = (nnf[v] + vis) * Jf;
+					// no-slip flux walls: the energy flux is the provided one (numerical_flux_central_T.c:222-230)
+					const S tot = (nf_provided && v == kNsNV - 1) ? S(a.wall_nfE) : nnf[v] + vis;
+					const S r = tot * Jf;
 					RES[slc][(lf * kNsNV + v) * N + dst] = side == 0 ? r : -r;
 				}
 			}
@@ -500,6 +564,12 @@ static hipError_t launch_ns(Ctx& c, const S* u, S* r, const S* met, const S* det
 	a.E = c.E;
 	if constexpr (sizeof(S) == sizeof(double)) a.zf = c.hp_z64;
 	else a.zf = c.hp_z32;
+	for (int k = 0; k < 5; ++k) a.bc_prim[k] = c.cfg.bc_prim[k];
+	a.wall_nfE = c.cfg.wall_nf_E;
+	a.wall_omega = c.cfg.wall_omega;
+	a.wall_rho = c.cfg.wall_rho;
+	a.wall_E = c.cfg.wall_E;
+	a.wall_xy = c.wall_xy;
 	if (c.has_nonconforming && !a.zf) return hipErrorInvalidValue;
 	switch (c.N) {
 	case 2: return run_ns<S, 2, PERT>(c, a, phase);

@@ -24,6 +24,9 @@ from dataclasses import dataclass, field
 import numpy as np
 
 BC_INTERIOR, BC_RIEMANN, BC_SLIPWALL = 0, 1, 2
+# Navier-Stokes no-slip walls (definitions_bc.h: BC_NOSLIP_ADIABATIC 111, BC_NOSLIP_DIABATIC 112,
+# BC_NOSLIP_ALL_ROTATING 113)
+BC_NOSLIP_ADIABATIC, BC_NOSLIP_DIABATIC, BC_NOSLIP_ROTATING = 3, 4, 5
 PDE_ADVECTION, PDE_EULER, PDE_NAVIER_STOKES = 0, 1, 2
 FLUX_UPWIND, FLUX_LF, FLUX_ROE = 0, 1, 2
 GAMMA, GM1 = 1.4, 0.4
@@ -263,6 +266,51 @@ def structured_tp_mesh(d: int, p: int, n: tuple, lo: tuple, hi: tuple, periodic:
         rows.append((e, 2 * k + side, -1, -1, 0, (bc_high if side else bc_low)[k]))
     faces = np.array(rows, dtype=np.int32).reshape(-1, 6)
     return Mesh(d, p, xyz, faces, periods, tuple(n))
+
+
+def annulus_mesh(p: int, nr: int, nt: int, r_i: float = 0.5, r_o: float = 1.0, bc_inner: int = BC_NOSLIP_ROTATING,
+                 bc_outer: int = BC_NOSLIP_ADIABATIC) -> Mesh:
+    """QUAD mesh of the annulus r_i <= r <= r_o (the reference's Taylor-Couette geometry, n-cylinder_hollow with
+    geometry_parameters.geo r_i = 0.5, r_o = 1), nr elements across and nt around, periodic in theta; the
+    isoparametric GLL nodes lie on the exact circles.  Local xi runs outward, eta counter-clockwise; the faces on
+    the inner / outer circle carry bc_inner / bc_outer."""
+    def curve(pts):
+        r = r_i + (r_o - r_i) * pts[..., 0]
+        th = 2.0 * np.pi * pts[..., 1]
+        return np.stack([r * np.cos(th), r * np.sin(th)], -1)
+    return structured_tp_mesh(2, p, (nr, nt), (0.0, 0.0), (1.0, 1.0), (False, True), curve=curve,
+                              bc_low=(bc_inner, BC_INTERIOR), bc_high=(bc_outer, BC_INTERIOR))
+
+
+def taylor_couette_state(mesh: Mesh, mu: float, Pr: float, omega: float = 1.0, r_i: float = 0.5, r_o: float = 1.0,
+                         dTdr_o: float = 0.0, t_i: float = 1.0, p_i: float = 1.0):
+    """The reference's Taylor-Couette test state (solution_taylor_couette_T.c, specific gas constant 1) at the
+    mesh nodes: V_theta = c (1/r - r/r_o^2) with c = omega / (1/r_i^2 - 1/r_o^2), the conduction-dissipation
+    temperature profile with dT/dr = dTdr_o at r_o, uniform pressure p_i, rho = p / T.  Returns (state, rate):
+    rate is -div F of that state, rho V^2 / r e_r in the momentum equations and 0 otherwise (the uniform
+    pressure leaves the centripetal term unbalanced; the theta-momentum and energy equations hold exactly)."""
+    x = nodes_xyz(mesh)
+    X, Y = x[..., 0], x[..., 1]
+    kappa = mu * GAMMA / GM1 / Pr
+    c = omega / (1.0 / r_i ** 2 - 1.0 / r_o ** 2)
+    r, th = np.sqrt(X * X + Y * Y), np.arctan2(Y, X)
+    Vt = c * (1.0 / r - r / r_o ** 2)
+    t = (t_i - 2.0 * c * c / r_o ** 2 * mu / kappa * np.log(r / r_i) - c * c * mu / kappa * (1.0 / r ** 2 - 1.0 / r_i ** 2)
+         + dTdr_o * r_o * np.log(r / r_i))
+    rho = p_i / t
+    prim = np.stack([rho, -np.sin(th) * Vt, np.cos(th) * Vt, p_i + 0 * X], -1)
+    rate = np.stack([0 * X, rho * Vt ** 2 / r * np.cos(th), rho * Vt ** 2 / r * np.sin(th), 0 * X], -1)
+    return pack_state(prim_to_cons(prim, 2)), pack_state(rate)
+
+
+def taylor_couette_walls(mu: float, Pr: float, omega: float = 1.0, r_i: float = 0.5, dTdr_o: float = 0.0,
+                         t_i: float = 1.0, p_i: float = 1.0) -> tuple:
+    """(nf_E, omega, rho_b, E_b) of the Taylor-Couette walls: the outer wall's normal viscous energy flux
+    -kappa dT/dn (get_normal_flux_Energy, kappa = mu Cp / Pr), the inner wall's rotation, density p_i / t_i and
+    total energy p_i / (gamma - 1) + rho (omega r_i)^2 / 2 (solution.data: rho_b 1, E_b 2.625)."""
+    kappa = mu * GAMMA / GM1 / Pr
+    rho_b = p_i / t_i
+    return (-kappa * dTdr_o, omega, rho_b, p_i / GM1 + 0.5 * rho_b * (omega * r_i) ** 2)
 
 
 def _tp_coords_var(n: tuple) -> np.ndarray:

@@ -32,12 +32,18 @@ enum {
 	DGX_ERR_NO_MESH = -7
 };
 
-/* DGX_PDE_NAVIER_STOKES: Euler + BR2 viscous terms (2-D QUAD, interior faces only: periodic meshes; FP64, FP32
- * and mixed residual, FP64 explicit path), constant viscosity mu and Prandtl number Pr from the config. */
+/* DGX_PDE_NAVIER_STOKES: Euler + BR2 viscous terms (2-D QUAD; FP64, FP32 and mixed residual, FP64 explicit
+ * path), constant viscosity mu and Prandtl number Pr from the config.  Boundary faces: Riemann and the no-slip
+ * walls (uniform-order meshes; variable-order and mixed meshes take interior faces only). */
 enum { DGX_PDE_ADVECTION = 0, DGX_PDE_EULER = 1, DGX_PDE_NAVIER_STOKES = 2 };
 /* num_flux_1st: upwind | Lax-Friedrichs | Roe-Pike (numerical_flux/definitions_numerical_flux.h:22-32) */
 enum { DGX_FLUX_UPWIND = 0, DGX_FLUX_LF = 1, DGX_FLUX_ROE = 2 };
-enum { DGX_BC_INTERIOR = 0, DGX_BC_RIEMANN = 1, DGX_BC_SLIPWALL = 2 };
+/* Boundary conditions (definitions_bc.h: BC_RIEMANN 1, BC_SLIPWALL 2; the Navier-Stokes no-slip walls
+ * BC_NOSLIP_ADIABATIC 111, BC_NOSLIP_DIABATIC 112, BC_NOSLIP_ALL_ROTATING 113, boundary_navier_stokes_T.c).  The
+ * slip wall is an Euler condition, the no-slip walls Navier-Stokes ones (face_solver_T.c:251-298): another
+ * combination is refused with DGX_ERR_UNSUPPORTED. */
+enum { DGX_BC_INTERIOR = 0, DGX_BC_RIEMANN = 1, DGX_BC_SLIPWALL = 2,
+       DGX_BC_NOSLIP_ADIABATIC = 3, DGX_BC_NOSLIP_DIABATIC = 4, DGX_BC_NOSLIP_ROTATING = 5 };
 
 typedef struct dgx_config {
 	int d;              /* 2 or 3 (DIM, a compile-time constant in the reference) */
@@ -48,6 +54,14 @@ typedef struct dgx_config {
 	double bc_prim[5];  /* free-stream primitive state for Riemann BCs (rho, u, v, [w], p) */
 	double mu;          /* DGX_PDE_NAVIER_STOKES: constant viscosity (compute_mu_constant_T, solution.data mu) */
 	double Pr;          /* DGX_PDE_NAVIER_STOKES: Prandtl number (solution.data Pr) */
+	/* no-slip wall data (solution.data; boundary_navier_stokes_T.c): */
+	double wall_nf_E;   /* DGX_BC_NOSLIP_ADIABATIC / _DIABATIC: the normal viscous energy flux, which replaces the
+	                       energy equation's numerical flux there (get_normal_flux_Energy: 0 adiabatic,
+	                       -mu Cp / Pr dTdn diabatic); pass 0 for adiabatic walls */
+	double wall_omega;  /* DGX_BC_NOSLIP_ROTATING: angular velocity about the origin (omega); the wall velocity at
+	                       (x, y) is omega (-y, x) */
+	double wall_rho;    /* DGX_BC_NOSLIP_ROTATING: wall density and total energy (rho_b, E_b) */
+	double wall_E;
 } dgx_config;
 
 typedef struct dgx_mesh {

@@ -37,6 +37,9 @@ struct or_ctx {
 	int* ef_list;        /* (face << 1) | side */
 	int viscous;         /* Navier-Stokes: Euler + BR2 viscous terms (or_set_viscous) */
 	double mu, Pr;       /* constant viscosity (compute_mu_constant_T) and Prandtl number */
+	double wall_nfE;     /* no-slip flux walls: the normal viscous energy flux nf_E (0 adiabatic) */
+	double wall_omega;   /* rotating no-slip walls: angular velocity, and the wall density and total energy */
+	double wall_rho, wall_E;
 };
 
 /* ------------------------------------------------------------------------------------------------ */
@@ -658,6 +661,77 @@ void or_set_viscous(or_ctx* c, double mu, double Pr)
 	c->Pr = Pr;
 }
 
+void or_set_walls(or_ctx* c, double nf_E, double omega, double rho_b, double E_b)
+{
+	c->wall_nfE = nf_E;
+	c->wall_omega = omega;
+	c->wall_rho = rho_b;
+	c->wall_E = E_b;
+}
+
+static int ns_bc_supported(int bc)
+{
+	return bc == OR_BC_RIEMANN || bc == OR_BC_NOSLIP_ADIABATIC || bc == OR_BC_NOSLIP_DIABATIC || bc == OR_BC_NOSLIP_ROTATING;
+}
+
+/* physical coordinates at the face cubature nodes of local face lf of element e (isoparametric): xy[n * d + k] */
+static void face_xyz(const or_ctx* c, int e, int lf, double* xy)
+{
+	const int d = c->d, Np = c->Np, Nfc = c->Nfc;
+	const double* op = c->cv0_vs_fc + (size_t)lf * Nfc * Np;
+	for (int k = 0; k < d; ++k)
+		for (int n = 0; n < Nfc; ++n) {
+			double acc = 0.0;
+			for (int m = 0; m < Np; ++m) acc += op[n * Np + m] * c->xyz[((size_t)e * d + k) * Np + m];
+			xy[n * d + k] = acc;
+		}
+}
+
+/* Boundary (ghost) state of a Navier-Stokes boundary face node:
+ *   Riemann: the Euler Riemann state (boundary_euler_T.c:99-180);
+ *   no-slip adiabatic / diabatic walls (constructor_Boundary_Value_T_navier_stokes_no_slip_flux_general,
+ *     boundary_navier_stokes_T.c:260-290): rho and E from inside, momentum -rho u_l + 2 rho u_ex with u_ex = 0;
+ *   rotating no-slip wall (no_slip_all_general, :200-240): s_b = 2 s_ex - s_l, s_ex = (rho_b, rho_b u_ex, E_b),
+ *     u_ex = (-sin th, cos th) omega r at the node (compute_uvw_ex_rotating, :505-520). */
+static void ns_bc_state(const or_ctx* c, int bc, const double* sL, const double* n, const double* xy, double* sB)
+{
+	const int d = c->d;
+	if (bc == OR_BC_RIEMANN) {
+		bc_state_d(d, bc, sL, n, c->bc_prim, sB);
+		return;
+	}
+	double uvw[3] = { 0.0, 0.0, 0.0 };
+	if (bc == OR_BC_NOSLIP_ROTATING) {
+		const double r = sqrt(xy[0] * xy[0] + xy[1] * xy[1]), th = atan2(xy[1], xy[0]), Vt = c->wall_omega * r;
+		uvw[0] = -sin(th) * Vt;
+		uvw[1] = cos(th) * Vt;
+		sB[0] = -sL[0] + 2.0 * c->wall_rho;
+		for (int k = 0; k < d; ++k) sB[1 + k] = -sL[1 + k] + 2.0 * c->wall_rho * uvw[k];
+		sB[d + 1] = -sL[d + 1] + 2.0 * c->wall_E;
+		return;
+	}
+	sB[0] = sL[0];
+	sB[d + 1] = sL[d + 1];
+	for (int k = 0; k < d; ++k) sB[1 + k] = -sL[1 + k] + 2.0 * sB[0] * uvw[k];
+}
+
+/* the boundary states of face f at its cubature nodes, in the left (own) ordering */
+static void ns_face_bc_states(const or_ctx* c, int f, const double* sL, double* sB)
+{
+	const int d = c->d, Nfc = c->Nfc, nv = c->nv;
+	const int* fc = c->faces + 6 * f;
+	double xy[3 * 16];
+	face_xyz(c, fc[0], fc[1], xy);
+	const double* nJ = c->nJ_fc + (size_t)f * Nfc * d;
+	for (int n = 0; n < Nfc; ++n) {
+		double vn[3] = { 0.0, 0.0, 0.0 }, J2 = 0.0;
+		for (int k = 0; k < d; ++k) J2 += nJ[n * d + k] * nJ[n * d + k];
+		const double Jf = sqrt(J2);
+		for (int k = 0; k < d; ++k) vn[k] = nJ[n * d + k] / Jf;
+		ns_bc_state(c, fc[5], &sL[n * nv], vn, &xy[n * d], &sB[n * nv]);
+	}
+}
+
 static int ensure_minv(or_ctx* c)
 {
 	const int Np = c->Np;
@@ -720,13 +794,14 @@ static void trace_fc(const or_ctx* c, int e, int lf, const double* coef, double*
  *   gf[f][side][dim] = M^-1 tw0_vt_fc (J_f n_dim (s_num - s_side)), s_num central: s_num - s_L = (s_R - s_L)/2,
  *                  the right side using the same product permuted into its face ordering (:240-256)
  *   g[e]          = gv[e] + sum over the faces of e of gf (add_face_grad_coef_f_to_volumes :505-527).
- * Layouts: gv, g: E x d x nv x Np; gf: F x 2 x d x nv x Np.  Interior faces only (returns -1 otherwise). */
+ * Layouts: gv, g: E x d x nv x Np; gf: F x 2 x d x nv x Np (boundary faces: side 0 only).  Returns -1 for a
+ * boundary code other than Riemann and the no-slip walls. */
 int or_ns_gradient(or_ctx* c, const double* u, double* g, double* gv, double* gf)
 {
 	const int d = c->d, Np = c->Np, Nc = c->Nc, Nfc = c->Nfc, nv = c->nv;
 	if (ensure_minv(c)) return -1;
 	for (int f = 0; f < c->F; ++f)
-		if (c->faces[6 * f + 5] != OR_BC_INTERIOR) return -1;
+		if (c->faces[6 * f + 5] != OR_BC_INTERIOR && !ns_bc_supported(c->faces[6 * f + 5])) return -1;
 	const size_t eb = (size_t)d * nv * Np;
 	double* gx = (double*)malloc(sizeof(double) * Nc * Np);
 	double* t1 = (double*)malloc(sizeof(double) * Nc);
@@ -767,14 +842,20 @@ int or_ns_gradient(or_ctx* c, const double* u, double* g, double* gv, double* gf
 	double* sRr = (double*)malloc(sizeof(double) * Nfc * nv);
 	double* q = (double*)malloc(sizeof(double) * Nfc * nv);
 	double* qs = (double*)malloc(sizeof(double) * Nfc * nv);
+	int ident[16];
+	for (int n = 0; n < Nfc; ++n) ident[n] = n;
 	for (int f = 0; f < c->F; ++f) {
 		const int* fc = c->faces + 6 * f;
 		const int eL = fc[0], fL = fc[1], eR = fc[2], fR = fc[3], ord = fc[4];
+		const int inter = fc[5] == OR_BC_INTERIOR;
 		trace_fc(c, eL, fL, u + (size_t)eL * nv * Np, sL);
-		trace_fc(c, eR, fR, u + (size_t)eR * nv * Np, sRr);
-		const int* perm = c->perm + (size_t)ord * Nfc;
+		/* boundary faces: s_num = (s_L + s_b) / 2 with the boundary state (compute_g_coef_related_boundary,
+		 * compute_grad_coef_dg_T.c:527-575), lifted into the left element only */
+		if (inter) trace_fc(c, eR, fR, u + (size_t)eR * nv * Np, sRr);
+		else ns_face_bc_states(c, f, sL, sRr);
+		const int* perm = inter ? c->perm + (size_t)ord * Nfc : ident;
 		const double* nJ = c->nJ_fc + (size_t)f * Nfc * d;
-		for (int side = 0; side < 2; ++side) {
+		for (int side = 0; side < (inter ? 2 : 1); ++side) {
 			const int e = side ? eR : eL, lf = side ? fR : fL;
 			const double* Mi = c->minv + (size_t)e * Np * Np;
 			const double* tw0 = c->tw0_vt_fc + (size_t)lf * Np * Nfc;
@@ -815,7 +896,8 @@ int or_ns_gradient(or_ctx* c, const double* u, double* g, double* gv, double* gf
  *   face:   nnf = nnf_euler(s_L, s_R, n) + n . (fv(s_L, g_L) + fv(s_R, g_R)) / 2 (central, numerical_flux_central_T.c
  *           :76-104), g_side = cv0_vr_fc (gv + 1.01 NFMAX gf[side]) (constructor_partial_grad_fc_interp,
  *           compute_face_rlhs_dg_T.c:243-300), NFMAX = 2d faces; then scaled by J_f and lifted as the Euler faces.
- * Interior faces only; returns -1 otherwise. */
+ * Boundary faces (Riemann and the no-slip walls, ns_bc_state): s_R = the boundary state, g_R = g_L; returns -1 for
+ * other boundary codes (the reference has no slip-wall condition for Navier-Stokes, face_solver_T.c:275-298). */
 int or_compute_rhs_ns(or_ctx* c, const double* u, double* rhs)
 {
 	const int d = c->d, Np = c->Np, Nc = c->Nc, Nfc = c->Nfc, nv = c->nv;
@@ -877,21 +959,28 @@ int or_compute_rhs_ns(or_ctx* c, const double* u, double* rhs)
 	double* gRr = (double*)malloc(sizeof(double) * Nfc * nv * d);
 	double* tmp = (double*)malloc(sizeof(double) * Nfc * nv);
 	double* nnf = (double*)malloc(sizeof(double) * Nfc * nv);
+	int ident[16];
+	for (int n = 0; n < Nfc; ++n) ident[n] = n;
 	for (int f = 0; f < c->F; ++f) {
 		const int* fc = c->faces + 6 * f;
-		const int eL = fc[0], fL = fc[1], eR = fc[2], fR = fc[3], ord = fc[4];
-		const int* perm = c->perm + (size_t)ord * Nfc;
+		const int eL = fc[0], fL = fc[1], eR = fc[2], fR = fc[3], ord = fc[4], bc = fc[5];
+		const int inter = bc == OR_BC_INTERIOR;
+		const int* perm = inter ? c->perm + (size_t)ord * Nfc : ident;
 		trace_fc(c, eL, fL, u + (size_t)eL * nv * Np, sL);
-		trace_fc(c, eR, fR, u + (size_t)eR * nv * Np, sRr);
+		if (inter) trace_fc(c, eR, fR, u + (size_t)eR * nv * Np, sRr);
+		else ns_face_bc_states(c, f, sL, sRr);
 		for (size_t i = 0; i < eb; ++i) {
 			gLc[i] = gv[(size_t)eL * eb + i] + eta * gf[((size_t)f * 2 + 0) * eb + i];
-			gRc[i] = gv[(size_t)eR * eb + i] + eta * gf[((size_t)f * 2 + 1) * eb + i];
+			if (inter) gRc[i] = gv[(size_t)eR * eb + i] + eta * gf[((size_t)f * 2 + 1) * eb + i];
 		}
 		for (int dim = 0; dim < d; ++dim) {
 			trace_fc(c, eL, fL, gLc + (size_t)dim * nv * Np, tmp);
 			for (int n = 0; n < Nfc; ++n)
 				for (int v = 0; v < nv; ++v) gL[n * nv * d + v + nv * dim] = tmp[n * nv + v];
-			trace_fc(c, eR, fR, gRc + (size_t)dim * nv * Np, tmp);
+			if (inter) trace_fc(c, eR, fR, gRc + (size_t)dim * nv * Np, tmp);
+			/* boundary faces: the boundary gradient is the interior partial gradient
+			 * (constructor_Boundary_Value_T_grad_from_internal, boundary_T.c:103-132; the no-slip flux walls
+			 * copy it too, their energy entry being replaced by the provided flux below) */
 			for (int n = 0; n < Nfc; ++n)
 				for (int v = 0; v < nv; ++v) gRr[n * nv * d + v + nv * dim] = tmp[n * nv + v];
 		}
@@ -908,11 +997,16 @@ int or_compute_rhs_ns(or_ctx* c, const double* u, double* rhs)
 			for (int eq = 0; eq < nv; ++eq) {
 				double acc = 0.0;
 				for (int k = 0; k < d; ++k) acc += vn[k] * 0.5 * (fvL[k + d * eq] + fvR[k + d * eq]);
-				nnf[n * nv + eq] = (nnf[n * nv + eq] + acc) * Jf;
+				nnf[n * nv + eq] = nnf[n * nv + eq] + acc;
 			}
+			/* no-slip flux walls: the energy equation's normal flux is the provided nf_E, replacing the inviscid
+			 * and viscous parts (set_Numerical_Flux_Energy_member after compute_Numerical_Flux_12,
+			 * numerical_flux_central_T.c:222-230, 250-262; numerical_flux_T.c:150-156) */
+			if (bc == OR_BC_NOSLIP_ADIABATIC || bc == OR_BC_NOSLIP_DIABATIC) nnf[n * nv + nv - 1] = c->wall_nfE;
+			for (int eq = 0; eq < nv; ++eq) nnf[n * nv + eq] *= Jf;
 		}
 		face_side_d(c, f, 0, nnf, rhs);
-		face_side_d(c, f, 1, nnf, rhs);
+		if (inter) face_side_d(c, f, 1, nnf, rhs);
 	}
 	free(s_vc); free(g_vc); free(fr); free(sL); free(sRr); free(gLc); free(gRc); free(gL); free(gRr); free(tmp); free(nnf);
 	free(g); free(gv); free(gf);

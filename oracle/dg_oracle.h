@@ -52,7 +52,10 @@ extern "C" {
 
 enum { OR_PDE_ADVECTION = 0, OR_PDE_EULER = 1 };
 enum { OR_FLUX_UPWIND = 0, OR_FLUX_LF = 1, OR_FLUX_ROE = 2 };
-enum { OR_BC_INTERIOR = 0, OR_BC_RIEMANN = 1, OR_BC_SLIPWALL = 2 };
+enum { OR_BC_INTERIOR = 0, OR_BC_RIEMANN = 1, OR_BC_SLIPWALL = 2,
+       /* Navier-Stokes no-slip walls (definitions_bc.h: BC_NOSLIP_ADIABATIC 111, BC_NOSLIP_DIABATIC 112,
+        * BC_NOSLIP_ALL_ROTATING 113) */
+       OR_BC_NOSLIP_ADIABATIC = 3, OR_BC_NOSLIP_DIABATIC = 4, OR_BC_NOSLIP_ROTATING = 5 };
 
 typedef struct or_ctx or_ctx;
 
@@ -81,9 +84,12 @@ void or_face_perm(const or_ctx* c, int ord, int* perm);
 
 /* residual (volume + face, no M^-1) -- compute_rlhs_common_dg */
 void or_compute_rhs_d(const or_ctx* c, const double* u, double* rhs);
-/* Navier-Stokes with BR2 (real only, interior faces only: periodic meshes).  or_set_viscous switches an Euler
+/* Navier-Stokes with BR2 (real only; interior, Riemann and no-slip wall faces).  or_set_viscous switches an Euler
  * context to Navier-Stokes with constant viscosity mu and Prandtl number Pr. */
 void or_set_viscous(or_ctx* c, double mu, double Pr);
+/* wall data of the no-slip conditions: nf_E the normal viscous energy flux of the adiabatic (0) and diabatic
+ * walls (get_normal_flux_Energy: -mu Cp / Pr dT/dn); omega, rho_b, E_b of the rotating wall (solution.data) */
+void or_set_walls(or_ctx* c, double nf_E, double omega, double rho_b, double E_b);
 /* negated viscous flux fv[dim + d*eq] for state s[nv] and gradient g[v + nv*dim] */
 void or_viscous_flux(int d, const double* s, const double* g, double mu, double Pr, double* fv);
 /* BR2 weak gradient coefficients: g, gv: E x d x nv x Np; gf: F x 2 x d x nv x Np.  0 or -1 */

@@ -38,6 +38,7 @@ def lib():
             getattr(L, name).argtypes = [C.c_void_p, dp, dp]
         L.or_compute_rhs_d_threads.argtypes = [C.c_void_p, dp, dp, C.c_int]
         L.or_set_viscous.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.or_set_walls.argtypes = [C.c_void_p] + [C.c_double] * 4
         L.or_viscous_flux.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, dp]
         L.or_ns_gradient.argtypes = [C.c_void_p, dp, dp, dp, dp]
         L.or_ns_gradient.restype = C.c_int
@@ -144,8 +145,9 @@ def bc_state(d, bc, sL, n, prim):
 class Oracle:
     """One DG discretisation (mesh + physics) on the CPU."""
 
-    def __init__(self, mesh, pde: int, flux: int, b_adv=(0.0, 0.0, 0.0), bc_prim=None, viscous=None):
-        """viscous = (mu, Pr): Navier-Stokes with BR2 (Euler contexts, periodic meshes)."""
+    def __init__(self, mesh, pde: int, flux: int, b_adv=(0.0, 0.0, 0.0), bc_prim=None, viscous=None, walls=None):
+        """viscous = (mu, Pr): Navier-Stokes with BR2 (Euler contexts; interior, Riemann and no-slip wall faces).
+        walls = (nf_E, omega, rho_b, E_b): the no-slip wall data (or_set_walls)."""
         self.mesh = mesh
         self.d, self.p = mesh.d, mesh.p
         self.pde, self.flux = pde, flux
@@ -166,6 +168,8 @@ class Oracle:
         self.ndof = mesh.E * self.Np * self.nv
         if viscous is not None:
             lib().or_set_viscous(self.ctx, float(viscous[0]), float(viscous[1]))
+        if walls is not None:
+            lib().or_set_walls(self.ctx, *[float(w) for w in walls])
 
     def __del__(self):
         if getattr(self, "ctx", None):
@@ -210,7 +214,7 @@ class Oracle:
         """Navier-Stokes (BR2) residual, no M^-1."""
         r = np.zeros(self.ndof)
         if lib().or_compute_rhs_ns(self.ctx, _d(np.ascontiguousarray(u, float)), _d(r)):
-            raise RuntimeError("or_compute_rhs_ns: viscous context with interior faces only")
+            raise RuntimeError("or_compute_rhs_ns: viscous context with interior, Riemann and no-slip wall faces only")
         return r
 
     def ns_gradient(self, u):
@@ -220,7 +224,7 @@ class Oracle:
         eb = m.d * self.nv * self.Np
         g, gv, gf = np.zeros(m.E * eb), np.zeros(m.E * eb), np.zeros(m.F * 2 * eb)
         if lib().or_ns_gradient(self.ctx, _d(np.ascontiguousarray(u, float)), _d(g), _d(gv), _d(gf)):
-            raise RuntimeError("or_ns_gradient: interior faces only")
+            raise RuntimeError("or_ns_gradient: interior, Riemann and no-slip wall faces only")
         sh = (m.E, m.d, self.nv, self.Np)
         return g.reshape(sh), gv.reshape(sh), gf.reshape(m.F, 2, m.d, self.nv, self.Np)
 

@@ -8,7 +8,10 @@
 * The FP32 residual (FP32 state, geometry and operators) agrees with the FP64 one to FP32 rounding, and the
   mixed residual (FP64 state, FP32 contractions on the perturbation from the element's first node) closer
   still (the gap widens with refinement: 4x at 4x4, see the bench's 448x448 leg).
-* Meshes with boundary faces and the linearised entry points are refused with DGX_ERR_UNSUPPORTED.
+* Boundary faces: the no-slip walls (adiabatic, diabatic, rotating) on the Taylor-Couette annulus and Riemann
+  faces on a box equal the oracle to 1e-12 (FP64 residual, M^-1 R), p = 1..5, both fluxes; the FP32 and mixed
+  forms stay within a multiple of the input-rounding floor.
+* Slip walls (an Euler condition) and the linearised entry points are refused with DGX_ERR_UNSUPPORTED.
 """
 import numpy as np
 import pytest
@@ -50,10 +53,10 @@ def _state(mesh, seed=0):
     return M.pack_state(M.prim_to_cons(np.stack([rho, u, v, p], -1), 2))
 
 
-def _ctx(mesh, flux):
+def _ctx(mesh, flux, walls=None, bc_prim=None):
     from dpgsolver_amd import _lib
 
-    ctx = _lib.Context(2, mesh.p, M.PDE_NAVIER_STOKES, flux, mu=MU, Pr=PR)
+    ctx = _lib.Context(2, mesh.p, M.PDE_NAVIER_STOKES, flux, mu=MU, Pr=PR, walls=walls, bc_prim=bc_prim)
     ctx.set_stream(torch.cuda.current_stream().cuda_stream)
     ctx.upload_mesh(mesh.xyz, mesh.faces)
     return ctx
@@ -156,3 +159,89 @@ def test_ns_fp32_residual_close_to_fp64(dev, p):
     tot = r32.double().reshape(mesh.E, 4, -1).sum(dim=(0, 2))
     assert tot.abs().max().item() < 1e-5 * r32.double().abs().sum().item()
     ctx.close()
+
+
+# ---------------------------------------------------------------------------------------------- boundary faces
+WALL_CASES = [(p, n, dT, flux) for (p, n) in ((1, 3), (2, 3), (3, 2), (4, 2), (5, 2)) for dT in (0.0, 1.0)
+              for flux in (M.FLUX_ROE, M.FLUX_LF)]
+
+
+def _wall_state(mesh, dTdr_o):
+    """The Taylor-Couette state plus a smooth perturbation (every wall term active, R far from zero)."""
+    u, _ = M.taylor_couette_state(mesh, MU, PR, dTdr_o=dTdr_o)
+    x = M.nodes_xyz(mesh)
+    pert = 0.02 * np.sin(3 * x[..., 0]) * np.cos(2 * x[..., 1])
+    return u * (1.0 + M.pack_state(np.stack([pert, 2 * pert, -pert, 0.5 * pert], -1)))
+
+
+@pytest.mark.parametrize("p,n,dTdr_o,flux", WALL_CASES,
+                         ids=[f"p{p}_n{n}_{'diabatic' if dT else 'adiabatic'}_{'roe' if f == M.FLUX_ROE else 'lf'}"
+                              for p, n, dT, f in WALL_CASES])
+def test_ns_no_slip_walls_match_oracle(dev, p, n, dTdr_o, flux):
+    bc_outer = M.BC_NOSLIP_DIABATIC if dTdr_o else M.BC_NOSLIP_ADIABATIC
+    mesh = M.annulus_mesh(p, n, 4 * n, bc_outer=bc_outer)
+    walls = M.taylor_couette_walls(MU, PR, dTdr_o=dTdr_o)
+    u = _wall_state(mesh, dTdr_o)
+    print(f"annulus p={p} {n}x{4 * n} outer={'diabatic' if dTdr_o else 'adiabatic'} flux={flux}", flush=True)
+    orc = O.Oracle(mesh, M.PDE_EULER, flux, viscous=(MU, PR), walls=walls)
+    ref = orc.rhs_ns(u)
+    ctx = _ctx(mesh, flux, walls=walls)
+    ud = torch.from_numpy(u).to(dev)
+    r = torch.full_like(ud, float("nan"))
+    ctx.compute_rhs(ud.data_ptr(), r.data_ptr())
+    torch.cuda.synchronize()
+    got = r.cpu().numpy()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"  FP64 rel err {err:.2e}", flush=True)
+    assert err < 1e-12
+    # explicit path: M^-1 R and its max-norm
+    rx = torch.full_like(ud, float("nan"))
+    mx = ctx.compute_rhs_explicit(ud.data_ptr(), rx.data_ptr())
+    torch.cuda.synchronize()
+    refx = orc.apply_minv(ref)
+    assert np.abs(rx.cpu().numpy() - refx).max() <= 1e-12 * np.abs(refx).max()
+    assert abs(mx - np.abs(refx).max()) <= 1e-12 * np.abs(refx).max()
+    # FP32 and mixed forms against FP64, bounded by the input-rounding floor
+    u32 = ud.float()
+    rr = torch.empty_like(ud)
+    ctx.compute_rhs(u32.double().data_ptr(), rr.data_ptr())
+    r32 = torch.full_like(u32, float("nan"))
+    rmx = torch.full_like(u32, float("nan"))
+    ctx.compute_rhs_f32(u32.data_ptr(), r32.data_ptr())
+    ctx.compute_rhs_mixed(ud.data_ptr(), rmx.data_ptr())
+    torch.cuda.synchronize()
+    scale = r.abs().max().item()
+    floor = (rr - r).abs().max().item() / scale
+    e32 = (r32.double() - r).abs().max().item() / scale
+    emx = (rmx.double() - r).abs().max().item() / scale
+    print(f"  FP32 {e32:.2e}, mixed {emx:.2e}, input-rounding floor {floor:.2e}", flush=True)
+    assert e32 < 8 * floor and emx < e32
+    ctx.close()
+
+
+def test_ns_riemann_box_matches_oracle(dev):
+    prim = M.freestream_prim(2)
+    curve = lambda pts: np.stack([pts[..., 0] + 0.03 * np.sin(2 * np.pi * pts[..., 1]),
+                                  pts[..., 1] + 0.03 * np.sin(2 * np.pi * pts[..., 0])], -1)
+    mesh = M.structured_tp_mesh(2, 3, (5, 4), (0.0, 0.0), (1.0, 0.8), (False, False), curve=curve,
+                                bc_low=(M.BC_RIEMANN, M.BC_RIEMANN), bc_high=(M.BC_RIEMANN, M.BC_RIEMANN))
+    x = M.nodes_xyz(mesh)
+    pert = 0.02 * np.sin(3 * x[..., 0]) * np.cos(2 * x[..., 1])
+    u = M.pack_state(M.prim_to_cons(prim[None, None, :] * (1.0 + np.stack([pert, pert, -pert, 0.5 * pert], -1)), 2))
+    for flux in (M.FLUX_ROE, M.FLUX_LF):
+        orc = O.Oracle(mesh, M.PDE_EULER, flux, bc_prim=prim, viscous=(MU, PR))
+        ref = orc.rhs_ns(u)
+        ctx = _ctx(mesh, flux, bc_prim=prim)
+        ud = torch.from_numpy(u).to(dev)
+        r = torch.full_like(ud, float("nan"))
+        ctx.compute_rhs(ud.data_ptr(), r.data_ptr())
+        torch.cuda.synchronize()
+        err = np.abs(r.cpu().numpy() - ref).max() / np.abs(ref).max()
+        print(f"Riemann box flux={flux}: rel err {err:.2e}", flush=True)
+        assert err < 1e-12
+        # a free stream stays a free stream
+        uf = torch.from_numpy(M.uniform_state(mesh, prim)).to(dev)
+        ctx.compute_rhs(uf.data_ptr(), r.data_ptr())
+        torch.cuda.synchronize()
+        assert r.abs().max().item() < 1e-13
+        ctx.close()

@@ -10,7 +10,15 @@ The reference holds no golden values for the viscous path, so each check is one 
 * M^-1 (R_NS - R_Euler), the viscous part of the residual, converges to the closed-form viscous rates of
   three fields that isolate each term: a shear layer u = A sin(2 pi y) (tau_xy and the work term),
   a static density variation at constant pressure (heat conduction gamma/Pr mu e_xx), and a compression
-  u = A sin(2 pi x) (tau_xx with the -2/3 mu div V part).
+  u = A sin(2 pi x) (tau_xx with the -2/3 mu div V part);
+* the no-slip walls (or_set_walls): on the reference's Taylor-Couette annulus (r 0.5 to 1, inner cylinder
+  rotating, BC_NOSLIP_ALL_ROTATING; outer wall adiabatic or diabatic, BC_NOSLIP_ADIABATIC / _DIABATIC), the
+  state of the reference's own test solution (solution_taylor_couette_T.c: V_theta = c (1/r - r/r_o^2), the
+  conduction-dissipation temperature profile, uniform pressure) satisfies the theta-momentum and energy
+  equations exactly, so M^-1 R converges to rho V^2 / r e_r (the centripetal term the uniform pressure leaves)
+  in the momentum components and to 0 in mass and energy; a reversed wall rotation or a wrong-sign wall heat
+  flux does not converge;
+* Riemann faces in a Navier-Stokes context keep a free stream exactly.
 """
 import numpy as np
 import pytest
@@ -143,3 +151,45 @@ def test_viscous_flux_vanishes_for_uniform_state_and_is_linear_in_mu():
     assert not f1[0].any()                    # no viscous mass flux
     # tau is symmetric: the x-momentum flux in y equals the y-momentum flux in x
     assert abs(f1[1, 1] - f1[2, 0]) < 1e-14
+
+
+# ---------------------------------------------------------------------------------------------- no-slip walls
+def _tc_error(n, p, dTdr_o, walls=None, mu=0.1):
+    bc_outer = M.BC_NOSLIP_ADIABATIC if dTdr_o == 0.0 else M.BC_NOSLIP_DIABATIC
+    mesh = M.annulus_mesh(p, n, 4 * n, bc_outer=bc_outer)
+    u, rate = M.taylor_couette_state(mesh, mu, PR, dTdr_o=dTdr_o)
+    orc = O.Oracle(mesh, M.PDE_EULER, M.FLUX_ROE, viscous=(mu, PR),
+                   walls=walls or M.taylor_couette_walls(mu, PR, dTdr_o=dTdr_o))
+    got = orc.apply_minv(orc.rhs_ns(u))
+    return np.abs(got - rate).reshape(mesh.E, 4, -1).max(axis=(0, 2))
+
+
+@pytest.mark.parametrize("dTdr_o", [0.0, 1.0], ids=["adiabatic_outer", "diabatic_outer"])
+def test_no_slip_walls_converge_on_taylor_couette(dTdr_o):
+    e = [_tc_error(n, 3, dTdr_o) for n in (2, 4, 8)]
+    # measured (p = 3, mu = 0.1, n = 2, 4, 8; mass, momentum, energy): adiabatic 1.1e-2 / 1.4e-3 / 1.8e-4,
+    # 0.149 / 0.056 / 0.018, 0.078 / 0.023 / 0.0073; diabatic 1.4e-2 / 1.9e-3 / 2.4e-4, 0.45 / 0.20 / 0.074,
+    # 2.0 / 0.94 / 0.35 (|rate| <= 0.5): every component at least halves per refinement
+    for k in range(2):
+        assert (e[k] / e[k + 1] >= 2.0).all(), e
+    assert e[2][:3].max() < 0.1 and e[2][3] < 0.5, e
+
+
+def test_no_slip_wall_plants_fail():
+    good = _tc_error(4, 3, 1.0)
+    walls = list(M.taylor_couette_walls(0.1, PR, dTdr_o=1.0))
+    wrong_rotation = _tc_error(4, 3, 1.0, walls=(walls[0], -walls[1], walls[2], walls[3]))
+    wrong_heat_flux = _tc_error(4, 3, 1.0, walls=(-walls[0], walls[1], walls[2], walls[3]))
+    assert wrong_rotation[1] > 100 * good[1], (wrong_rotation, good)
+    assert wrong_heat_flux[3] > 50 * good[3], (wrong_heat_flux, good)
+
+
+def test_riemann_faces_keep_a_free_stream_in_navier_stokes():
+    prim = M.freestream_prim(2)
+    curve = lambda pts: np.stack([pts[..., 0] + 0.03 * np.sin(2 * np.pi * pts[..., 1]),
+                                  pts[..., 1] + 0.03 * np.sin(2 * np.pi * pts[..., 0])], -1)
+    mesh = M.structured_tp_mesh(2, 3, (4, 3), (0.0, 0.0), (1.0, 0.8), (False, False), curve=curve,
+                                bc_low=(M.BC_RIEMANN, M.BC_RIEMANN), bc_high=(M.BC_RIEMANN, M.BC_RIEMANN))
+    orc = O.Oracle(mesh, M.PDE_EULER, M.FLUX_ROE, bc_prim=prim, viscous=(MU, PR))
+    r = orc.rhs_ns(M.uniform_state(mesh, prim))
+    assert np.abs(r).max() < 1e-13
