@@ -178,6 +178,63 @@ def cpu_steady_estimate(newton_steps, gmres_iters, restart=60, n_blocks=1024, n_
                       "counts from the GPU block-Jacobi solve"}
 
 
+def cpu_steady_measured(gpu_compare=True):
+    """The same steady solve measured on the host's cores: the oracle's restatement of the pseudo-time
+    Newton-GMRES(60) with exact LU block-Jacobi (oracle/steady_oracle.py: threaded residual, complex-step J.v and
+    diagonal blocks, Gauss-Jordan block inverses, CGS2), with the parameters of the GPU block-Jacobi leg, from the
+    free stream to max|R| < 1e-10, on min(16, available) cores.  Then (gpu_compare) the GPU block-Jacobi solve of
+    the same problem, and the difference of the two converged states."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import pyoracle
+    import steady_oracle
+    from threadpoolctl import threadpool_limits
+    from dpgsolver_amd import mesh as M
+
+    avail = len(os.sched_getaffinity(0))
+    nt = min(16, avail)
+    mesh = M.channel_mesh(64, 64, 3)
+    prim = M.freestream_prim(2)
+    orc = pyoracle.Oracle(mesh, M.PDE_EULER, M.FLUX_ROE, bc_prim=prim)
+    u = np.ascontiguousarray(M.uniform_state(mesh, prim))
+    print(f"cpu steady solve on {nt} threads ({avail} cores available)", file=sys.stderr, flush=True)
+    with threadpool_limits(limits=1):
+        st = steady_oracle.solve_implicit(
+            orc, u, nt, cfl_initial=1.0, cfl_max=300.0, exit_tol=1e-10, max_newton=100, restart=60, rtol=1e-4,
+            max_iter=1000, progress=lambda it, mx, its, t: print(f"  cpu newton {it}: max|R| {mx:.3e}, gmres {its}, "
+                                                                 f"{t:.1f} s", file=sys.stderr, flush=True))
+    print(f"cpu block-Jacobi: {st['seconds']:.1f} s, newton {st['newton_steps']}, gmres {st['gmres_iters']}",
+          file=sys.stderr, flush=True)
+    out = {"workload": "2-D Euler DG p=3, 64x64 curved-bump channel, Mach 0.5, Roe; Newton-GMRES(60) to 1e-10",
+           "cpu": {"seconds": round(st["seconds"], 2), "newton_steps": st["newton_steps"],
+                   "gmres_iters": st["gmres_iters"], "converged": st["converged"], "max_rhs": st["max_rhs"],
+                   "gmres_per_step": st["gmres_per_step"], "cores": nt, "host_cores_available": avail,
+                   "kind": "port",
+                   "what": "oracle restatement of solve_implicit with exact LU block-Jacobi, every residual, "
+                           "complex-step product, block and Gram-Schmidt pass on OpenMP threads"}}
+    if gpu_compare:
+        import torch
+        from dpgsolver_amd import _lib
+        dev = torch.device("cuda:0")
+        ctx = _lib.Context(2, 3, M.PDE_EULER, M.FLUX_ROE, bc_prim=prim)
+        ctx.set_stream(torch.cuda.current_stream(dev).cuda_stream)
+        ctx.upload_mesh(mesh.xyz, mesh.faces)
+        kw = dict(precond=_lib.PC_BLOCK_JACOBI, cfl_initial=1.0, cfl_max=300.0, exit_tol=1e-10, max_newton=100,
+                  gmres_restart=60, gmres_rtol=1e-4, gmres_max_iter=1000, raise_on_error=False)
+        ug = torch.from_numpy(M.uniform_state(mesh, prim)).to(dev)
+        ctx.solve_implicit(ug.data_ptr(), **{**kw, "max_newton": 2})     # warm-up
+        ug = torch.from_numpy(M.uniform_state(mesh, prim)).to(dev)
+        sg = ctx.solve_implicit(ug.data_ptr(), **kw)
+        ctx.close()
+        ugh = ug.cpu().numpy()
+        out["gpu"] = {"seconds": round(sg["seconds"], 4), "newton_steps": sg["newton_steps"],
+                      "gmres_iters": sg["gmres_iters"], "converged": bool(sg["converged"]),
+                      "gmres_per_step": sg["gmres_per_step"]}
+        out["steady_states_rel_diff"] = float(np.abs(ugh - u).max() / np.abs(u).max())
+        out["cpu_vs_gpu_wall"] = round(st["seconds"] / sg["seconds"], 1)
+    return out
+
+
 def time_to_steady(torch, dev, epochs=60, cpu=True):
     """Time to max|R| < 1e-10 (FP64) on the steady channel workload: 2-D Euler, DG p=3, 64x64 QUAD channel
     [-1.5,1.5]x[0,0.8] with the bump y = 0.0625 exp(-25 x^2), inflow Mach 0.5, Roe flux, pseudo-time
@@ -650,6 +707,8 @@ def main():
                     help="run only the variable-order Navier-Stokes leg and print its JSON (profiling aid)")
     ap.add_argument("--mixed-only", action="store_true",
                     help="run only the mixed TRI/QUAD legs and print their JSON (profiling aid)")
+    ap.add_argument("--cpu-steady", action="store_true",
+                    help="run only the measured CPU time-to-1e-10 (host cores) beside the GPU solve and print its JSON")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 1 GiB read + 1 GiB write elementwise kernel (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -657,6 +716,10 @@ def main():
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "sharding is a design, not a measured path.", file=sys.stderr)
         sys.exit(2)
+
+    if args.cpu_steady:
+        print(json.dumps({"cpu_time_to_1e-10": cpu_steady_measured()}), flush=True)
+        return
 
     import numpy as np
     import torch

@@ -546,13 +546,9 @@ void or_compute_rhs_d(const or_ctx* c, const double* u, double* rhs)
  * buffer, then lifted colour by colour: faces of one colour (greedy over the face list) share no element, so
  * their lifts touch disjoint residual blocks.  Each element receives the same additions in the same order as
  * or_compute_rhs_d up to the face order, so the result agrees to rounding. */
-void or_compute_rhs_d_threads(const or_ctx* c, const double* u, double* rhs, int nthreads)
+/* greedy face colouring: faces of one colour share no element (their lifts touch disjoint residual blocks) */
+static int face_colors(const or_ctx* c, int* color)
 {
-	const size_t blk = (size_t)c->Np * c->nv, fsz = (size_t)c->Nfc * c->nv;
-	if (nthreads < 1) nthreads = 1;
-	memset(rhs, 0, sizeof(double) * (size_t)c->E * blk);
-	double* nnf = (double*)malloc(sizeof(double) * (size_t)c->F * fsz);
-	int* color = (int*)malloc(sizeof(int) * (size_t)c->F);
 	unsigned long long* used = (unsigned long long*)calloc((size_t)c->E, sizeof(unsigned long long));
 	int ncol = 0;
 	for (int f = 0; f < c->F; ++f) {
@@ -566,6 +562,17 @@ void or_compute_rhs_d_threads(const or_ctx* c, const double* u, double* rhs, int
 		if (k + 1 > ncol) ncol = k + 1;
 	}
 	free(used);
+	return ncol;
+}
+
+void or_compute_rhs_d_threads(const or_ctx* c, const double* u, double* rhs, int nthreads)
+{
+	const size_t blk = (size_t)c->Np * c->nv, fsz = (size_t)c->Nfc * c->nv;
+	if (nthreads < 1) nthreads = 1;
+	memset(rhs, 0, sizeof(double) * (size_t)c->E * blk);
+	double* nnf = (double*)malloc(sizeof(double) * (size_t)c->F * fsz);
+	int* color = (int*)malloc(sizeof(int) * (size_t)c->F);
+	const int ncol = face_colors(c, color);
 #pragma omp parallel num_threads(nthreads)
 	{
 #pragma omp for schedule(static)
@@ -607,6 +614,40 @@ void or_compute_rhs_c(const or_ctx* c, const double* ure, const double* uim, dou
 	face_rhs_c(c, u, r);
 	for (size_t i = 0; i < n; ++i) { rre[i] = creal(r[i]); rim[i] = cimag(r[i]); }
 	free(u); free(r);
+}
+
+/* The complex twin on nthreads threads (the CPU Newton-GMRES's complex-step J.v), organised as
+ * or_compute_rhs_d_threads. */
+void or_compute_rhs_c_threads(const or_ctx* c, const double* ure, const double* uim, double* rre, double* rim,
+                              int nthreads)
+{
+	const size_t n = (size_t)c->E * c->Np * c->nv, fsz = (size_t)c->Nfc * c->nv;
+	if (nthreads < 1) nthreads = 1;
+	double complex* u = (double complex*)malloc(sizeof(double complex) * n);
+	double complex* r = (double complex*)calloc(n, sizeof(double complex));
+	double complex* nnf = (double complex*)malloc(sizeof(double complex) * (size_t)c->F * fsz);
+	int* color = (int*)malloc(sizeof(int) * (size_t)c->F);
+	const int ncol = face_colors(c, color);
+#pragma omp parallel num_threads(nthreads)
+	{
+#pragma omp for schedule(static)
+		for (size_t i = 0; i < n; ++i) u[i] = ure[i] + I * (uim ? uim[i] : 0.0);
+#pragma omp for schedule(static)
+		for (int e = 0; e < c->E; ++e) volume_elem_c(c, e, u, r);
+#pragma omp for schedule(static)
+		for (int f = 0; f < c->F; ++f) face_nnf_c(c, f, u, nnf + (size_t)f * fsz);
+		for (int k = 0; k < ncol; ++k) {
+#pragma omp for schedule(static)
+			for (int f = 0; f < c->F; ++f) {
+				if (color[f] != k) continue;
+				face_side_c(c, f, 0, nnf + (size_t)f * fsz, r);
+				if (c->faces[6 * f + 5] == OR_BC_INTERIOR) face_side_c(c, f, 1, nnf + (size_t)f * fsz, r);
+			}
+		}
+#pragma omp for schedule(static)
+		for (size_t i = 0; i < n; ++i) { rre[i] = creal(r[i]); rim[i] = cimag(r[i]); }
+	}
+	free(u); free(r); free(nnf); free(color);
 }
 
 /* mass matrix cv0^T diag(w detJ) cv0 -- volume_solver_T.c:115-131 */
@@ -1151,4 +1192,188 @@ void or_diag_blocks_cs(const or_ctx* c, const double* u, double h, double* block
 {
 	const int nb = c->Np * c->nv;
 	for (int e = 0; e < c->E; ++e) or_jacobian_block_cs(c, u, e, e, h, blocks + (size_t)e * nb * nb);
+}
+
+/* The same on nthreads threads, each with its own complex copy of the state (a perturbed column is visible to
+ * the faces of the neighbours, so the copies cannot be shared). */
+void or_diag_blocks_cs_threads(const or_ctx* c, const double* u, double h, double* blocks, int nthreads)
+{
+	const size_t ntot = (size_t)c->E * c->Np * c->nv;
+	const int nb = c->Np * c->nv;
+	if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads)
+	{
+		double complex* uc = (double complex*)malloc(sizeof(double complex) * ntot);
+		double complex* rc = (double complex*)calloc(ntot, sizeof(double complex));
+		for (size_t i = 0; i < ntot; ++i) uc[i] = u[i];
+#pragma omp for schedule(dynamic, 16)
+		for (int e = 0; e < c->E; ++e) {
+			double* blk = blocks + (size_t)e * nb * nb;
+			for (int j = 0; j < nb; ++j) {
+				const size_t gj = (size_t)e * nb + j;
+				uc[gj] = u[gj] + I * h;
+				rhs_local_c(c, e, uc, rc);
+				uc[gj] = u[gj];
+				for (int i = 0; i < nb; ++i) blk[(size_t)i * nb + j] = cimag(rc[(size_t)e * nb + i]) / h;
+			}
+		}
+		free(uc); free(rc);
+	}
+}
+
+/* Bezier coefficients of the nv variables of element e (ccSB0): b[v * Np + i] */
+static void bezier_coefs(const or_ctx* c, const double* ue, double* b)
+{
+	const int Np = c->Np, nv = c->nv;
+	for (int v = 0; v < nv; ++v)
+		for (int i = 0; i < Np; ++i) {
+			double acc = 0.0;
+			for (int j = 0; j < Np; ++j) acc += c->ccSB[i * Np + j] * ue[v * Np + j];
+			b[v * Np + i] = acc;
+		}
+}
+
+/* 1 / dt per element for the pseudo-time step: dt = cfl * h_min / max(|V| + c) over the Bezier coefficients of
+ * the state (compute_dt_cfl_constrained, solve_dg.c:284-319), h_min the shortest straight edge between the
+ * element's vertices (compute_min_length_measure). */
+void or_cfl_inv_dt(const or_ctx* c, const double* u, double cfl, double* inv_dt)
+{
+	const int d = c->d, Np = c->Np, nv = c->nv, N = c->N;
+	double* b = (double*)malloc(sizeof(double) * Np * nv);
+	for (int e = 0; e < c->E; ++e) {
+		bezier_coefs(c, u + (size_t)e * Np * nv, b);
+		double m = 0.0;
+		for (int i = 0; i < Np; ++i) {
+			const double rho = b[i], ri = 1.0 / rho;
+			double V2 = 0.0;
+			for (int k = 0; k < d; ++k) { const double w = b[(1 + k) * Np + i] * ri; V2 += w * w; }
+			const double p = OR_GM1 * (b[(d + 1) * Np + i] - 0.5 * rho * V2);
+			const double ws = sqrt(V2) + sqrt(OR_GAMMA * p * ri);
+			if (ws > m) m = ws;
+		}
+		double hmin = 1e300;
+		for (int a = 0; a < (1 << d); ++a)
+			for (int k = 0; k < d; ++k) {
+				const int bb = a ^ (1 << k);
+				if (bb < a) continue;
+				int na = 0, nb2 = 0, st = 1;
+				for (int q = 0; q < d; ++q) {
+					na += ((a >> q) & 1) * (N - 1) * st;
+					nb2 += ((bb >> q) & 1) * (N - 1) * st;
+					st *= N;
+				}
+				double s2 = 0.0;
+				for (int x = 0; x < d; ++x) {
+					const double t = c->xyz[((size_t)e * d + x) * Np + na] - c->xyz[((size_t)e * d + x) * Np + nb2];
+					s2 += t * t;
+				}
+				if (sqrt(s2) < hmin) hmin = sqrt(s2);
+			}
+		inv_dt[e] = m / (cfl * hmin);
+	}
+	free(b);
+}
+
+/* u_e += relax_e du_e with relax halved until the means of the Bezier coefficients of rho and p exceed EPS_PHYS
+ * (compute_relaxation, solve_implicit.c:823-873).  Returns 0, or -1 when an element needs relax < EPS. */
+int or_relaxed_update(const or_ctx* c, double* u, const double* du)
+{
+	const int d = c->d, Np = c->Np, nv = c->nv;
+	double* bu = (double*)malloc(sizeof(double) * Np * nv);
+	double* bd = (double*)malloc(sizeof(double) * Np * nv);
+	int status = 0;
+	for (int e = 0; e < c->E; ++e) {
+		double* ue = u + (size_t)e * Np * nv;
+		const double* de = du + (size_t)e * Np * nv;
+		bezier_coefs(c, ue, bu);
+		bezier_coefs(c, de, bd);
+		double relax = 1.0;
+		for (;;) {
+			double ar = 0.0, ap = 0.0;
+			for (int i = 0; i < Np; ++i) {
+				const double rho = bu[i] + relax * bd[i];
+				double V2 = 0.0;
+				for (int k = 0; k < d; ++k) {
+					const double w = (bu[(1 + k) * Np + i] + relax * bd[(1 + k) * Np + i]) / rho;
+					V2 += w * w;
+				}
+				ar += rho;
+				ap += OR_GM1 * ((bu[(d + 1) * Np + i] + relax * bd[(d + 1) * Np + i]) - 0.5 * rho * V2);
+			}
+			ar /= Np;
+			ap /= Np;
+			if (!(ar < OR_EPS_PHYS || ap < OR_EPS_PHYS)) break;
+			relax *= 0.5;
+			if (relax < OR_EPS) { status = -1; break; }
+		}
+		for (int i = 0; i < Np * nv; ++i) ue[i] += relax * de[i];
+	}
+	free(bu); free(bd);
+	return status;
+}
+
+/* Ainv_e = A_e^-1 for E dense nb x nb blocks (row-major; Gauss-Jordan with partial pivoting), on nthreads
+ * threads.  Returns -1 if a block is singular. */
+int or_block_inverse(int E, int nb, const double* A, double* Ainv, int nthreads)
+{
+	int bad = 0;
+	if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads)
+	{
+		double* a = (double*)malloc(sizeof(double) * (size_t)nb * nb);
+#pragma omp for schedule(static) reduction(|| : bad)
+		for (int e = 0; e < E; ++e) {
+			memcpy(a, A + (size_t)e * nb * nb, sizeof(double) * (size_t)nb * nb);
+			double* x = Ainv + (size_t)e * nb * nb;
+			if (invert(nb, a, x)) bad = 1;
+		}
+		free(a);
+	}
+	return bad ? -1 : 0;
+}
+
+/* h_j = V_j . w for the first k rows of V (k x n, row-major), on nthreads threads */
+void or_dots(int k, long long n, const double* V, const double* w, double* h, int nthreads)
+{
+	if (nthreads < 1) nthreads = 1;
+	for (int j = 0; j < k; ++j) h[j] = 0.0;
+#pragma omp parallel num_threads(nthreads)
+	{
+		double loc[256];
+		for (int j = 0; j < k; ++j) loc[j] = 0.0;
+#pragma omp for schedule(static)
+		for (long long i = 0; i < n; ++i)
+			for (int j = 0; j < k; ++j) loc[j] += V[(size_t)j * n + i] * w[i];
+#pragma omp critical
+		for (int j = 0; j < k; ++j) h[j] += loc[j];
+	}
+}
+
+/* w -= sum_j h_j V_j over the first k rows of V, on nthreads threads */
+void or_gemv_sub(int k, long long n, const double* V, const double* h, double* w, int nthreads)
+{
+	if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+	for (long long i = 0; i < n; ++i) {
+		double acc = w[i];
+		for (int j = 0; j < k; ++j) acc -= h[j] * V[(size_t)j * n + i];
+		w[i] = acc;
+	}
+}
+
+/* z_e = A_e v_e for E dense nb x nb blocks (row-major), on nthreads threads */
+void or_block_apply(int E, int nb, const double* A, const double* v, double* z, int nthreads)
+{
+	if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+	for (int e = 0; e < E; ++e) {
+		const double* Ae = A + (size_t)e * nb * nb;
+		const double* ve = v + (size_t)e * nb;
+		double* ze = z + (size_t)e * nb;
+		for (int i = 0; i < nb; ++i) {
+			double acc = 0.0;
+			for (int j = 0; j < nb; ++j) acc += Ae[(size_t)i * nb + j] * ve[j];
+			ze[i] = acc;
+		}
+	}
 }

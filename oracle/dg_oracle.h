@@ -100,6 +100,17 @@ int or_compute_rhs_ns(or_ctx* c, const double* u, double* rhs);
 void or_compute_rhs_d_threads(const or_ctx* c, const double* u, double* rhs, int nthreads);
 /* complex twin: u = ure + i*uim */
 void or_compute_rhs_c(const or_ctx* c, const double* ure, const double* uim, double* rre, double* rim);
+void or_compute_rhs_c_threads(const or_ctx* c, const double* ure, const double* uim, double* rre, double* rim,
+                              int nthreads);
+/* pieces of the CPU Newton-GMRES (solve_implicit.c): element diagonal blocks by complex step on nthreads threads,
+ * the per-element 1/dt of the CFL-constrained pseudo-time step, the relaxed update, dense block mat-vecs */
+void or_diag_blocks_cs_threads(const or_ctx* c, const double* u, double h, double* blocks, int nthreads);
+void or_cfl_inv_dt(const or_ctx* c, const double* u, double cfl, double* inv_dt);
+int or_relaxed_update(const or_ctx* c, double* u, const double* du);
+void or_block_apply(int E, int nb, const double* A, const double* v, double* z, int nthreads);
+int or_block_inverse(int E, int nb, const double* A, double* Ainv, int nthreads);
+void or_dots(int k, long long n, const double* V, const double* w, double* h, int nthreads);
+void or_gemv_sub(int k, long long n, const double* V, const double* h, double* w, int nthreads);
 /* volume-only and face-only parts */
 void or_volume_rhs_d(const or_ctx* c, const double* u, double* rhs);
 void or_face_rhs_d(const or_ctx* c, const double* u, double* rhs);

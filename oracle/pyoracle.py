@@ -64,6 +64,16 @@ def lib():
         L.or_bc_state_d.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp]
         L.or_jacobian_block_cs.argtypes = [C.c_void_p, dp, C.c_int, C.c_int, C.c_double, dp]
         L.or_diag_blocks_cs.argtypes = [C.c_void_p, dp, C.c_double, dp]
+        L.or_diag_blocks_cs_threads.argtypes = [C.c_void_p, dp, C.c_double, dp, C.c_int]
+        L.or_compute_rhs_c_threads.argtypes = [C.c_void_p, dp, dp, dp, dp, C.c_int]
+        L.or_cfl_inv_dt.argtypes = [C.c_void_p, dp, C.c_double, dp]
+        L.or_relaxed_update.argtypes = [C.c_void_p, dp, dp]
+        L.or_relaxed_update.restype = C.c_int
+        L.or_block_apply.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_int]
+        L.or_block_inverse.argtypes = [C.c_int, C.c_int, dp, dp, C.c_int]
+        L.or_block_inverse.restype = C.c_int
+        L.or_dots.argtypes = [C.c_int, C.c_longlong, dp, dp, dp, C.c_int]
+        L.or_gemv_sub.argtypes = [C.c_int, C.c_longlong, dp, dp, dp, C.c_int]
         L.or_face_perm.argtypes = [C.c_void_p, C.c_int, ip]
         _lib = L
     return _lib
@@ -255,11 +265,29 @@ class Oracle:
         lib().or_jacobian_block_cs(self.ctx, _d(np.ascontiguousarray(u, float)), e_row, e_col, h, _d(blk))
         return blk
 
-    def diag_blocks(self, u, h=1e-30):
+    def diag_blocks(self, u, h=1e-30, nthreads=1):
         nb = self.Np * self.nv
         out = np.zeros((self.mesh.E, nb, nb))
-        lib().or_diag_blocks_cs(self.ctx, _d(np.ascontiguousarray(u, float)), h, _d(out))
+        if nthreads > 1:
+            lib().or_diag_blocks_cs_threads(self.ctx, _d(np.ascontiguousarray(u, float)), h, _d(out), int(nthreads))
+        else:
+            lib().or_diag_blocks_cs(self.ctx, _d(np.ascontiguousarray(u, float)), h, _d(out))
         return out
+
+    def rhs_complex_threads(self, u_re, u_im, nthreads):
+        rre, rim = np.zeros(self.ndof), np.zeros(self.ndof)
+        lib().or_compute_rhs_c_threads(self.ctx, _d(np.ascontiguousarray(u_re, float)),
+                                       _d(np.ascontiguousarray(u_im, float)), _d(rre), _d(rim), int(nthreads))
+        return rre, rim
+
+    def cfl_inv_dt(self, u, cfl):
+        out = np.zeros(self.mesh.E)
+        lib().or_cfl_inv_dt(self.ctx, _d(np.ascontiguousarray(u, float)), float(cfl), _d(out))
+        return out
+
+    def relaxed_update(self, u, du):
+        """u += relax_e du_e in place (u: contiguous float64); returns the status (0, or -1: relax < EPS)."""
+        return lib().or_relaxed_update(self.ctx, _d(u), _d(np.ascontiguousarray(du, float)))
 
 
 def viscous_flux(d, s, g, mu, Pr):
@@ -267,3 +295,33 @@ def viscous_flux(d, s, g, mu, Pr):
     fv = np.zeros(d * (d + 2))
     lib().or_viscous_flux(d, _d(np.ascontiguousarray(s, float)), _d(np.ascontiguousarray(g, float)), mu, Pr, _d(fv))
     return fv.reshape(d + 2, d)
+
+
+def block_apply(A, v, nthreads=1):
+    """z_e = A_e v_e for a stack of dense blocks A (E, nb, nb) and v (E * nb,)."""
+    E, nb, _ = A.shape
+    z = np.zeros(E * nb)
+    lib().or_block_apply(E, nb, _d(np.ascontiguousarray(A, float)), _d(np.ascontiguousarray(v, float)), _d(z),
+                         int(nthreads))
+    return z
+
+
+def block_inverse(A, nthreads=1):
+    """Inverses of a stack of dense blocks (E, nb, nb), Gauss-Jordan with partial pivoting."""
+    E, nb, _ = A.shape
+    out = np.zeros_like(A)
+    if lib().or_block_inverse(E, nb, _d(np.ascontiguousarray(A, float)), _d(out), int(nthreads)):
+        raise np.linalg.LinAlgError("singular block")
+    return out
+
+
+def dots(V, k, w, nthreads=1):
+    """V[:k] @ w (V: (m, n) C-contiguous)."""
+    h = np.zeros(k)
+    lib().or_dots(k, V.shape[1], _d(V), _d(w), _d(h), int(nthreads))
+    return h
+
+
+def gemv_sub(V, k, h, w, nthreads=1):
+    """w -= h @ V[:k] in place."""
+    lib().or_gemv_sub(k, V.shape[1], _d(V), _d(np.ascontiguousarray(h, float)), _d(w), int(nthreads))
