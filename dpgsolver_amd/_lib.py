@@ -82,7 +82,8 @@ class ImplicitParams(C.Structure):
                 ("max_newton", C.c_int), ("gmres_restart", C.c_int), ("gmres_rtol", C.c_double),
                 ("gmres_max_iter", C.c_int), ("precond", C.c_int), ("cfl_max", C.c_double),
                 ("learned_fallback_iters", C.c_int), ("capture_blocks", C.c_void_p),
-                ("capture_features", C.c_void_p), ("capture_sigma", C.c_void_p), ("capture_capacity", C.c_longlong)]
+                ("capture_features", C.c_void_p), ("capture_sigma", C.c_void_p), ("capture_capacity", C.c_longlong),
+                ("jvp_fp32", C.c_int)]
 
 
 class ImplicitStats(C.Structure):
@@ -363,11 +364,13 @@ class Context:
 
     def solve_implicit(self, u: int, cfl_initial=10.0, exit_tol=1e-10, exit_ratio=0.0, max_newton=100,
                        gmres_restart=60, gmres_rtol=1e-4, gmres_max_iter=600, precond=PC_BLOCK_JACOBI,
-                       raise_on_error=True, capture=None, cfl_max=0.0, learned_fallback_iters=0):
-        """capture: optional (blocks_ptr, features_ptr, sigma_ptr, capacity_elements) device buffers."""
+                       raise_on_error=True, capture=None, cfl_max=0.0, learned_fallback_iters=0, jvp_fp32=False):
+        """capture: optional (blocks_ptr, features_ptr, sigma_ptr, capacity_elements) device buffers; jvp_fp32: the
+        GMRES operator in FP32 dual numbers (the FP64 residual still decides convergence)."""
         cb, cf, cs, cc = capture if capture else (0, 0, 0, 0)
         p = ImplicitParams(cfl_initial, exit_tol, exit_ratio, max_newton, gmres_restart, gmres_rtol, gmres_max_iter,
-                           precond, cfl_max, learned_fallback_iters, cb or None, cf or None, cs or None, cc)
+                           precond, cfl_max, learned_fallback_iters, cb or None, cf or None, cs or None, cc,
+                           1 if jvp_fp32 else 0)
         st = ImplicitStats()
         rc = self.L.dgx_solve_implicit(self.h, vp(u), C.byref(p), C.byref(st))
         if raise_on_error and rc != DGX_OK:

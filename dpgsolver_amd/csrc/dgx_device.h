@@ -38,9 +38,32 @@ __device__ inline Dual& operator+=(Dual& a, Dual b) { a = a + b; return a; }
 __device__ inline Dual& operator-=(Dual& a, Dual b) { a = a - b; return a; }
 __device__ inline Dual& operator*=(Dual& a, Dual b) { a = a * b; return a; }
 
+// The same in FP32 (value and derivative): the inner Jacobian-vector product of the implicit solve, read from and
+// written to FP64 vectors.
+struct Dual32 {
+	float v, d;
+	Dual32() = default;
+	__host__ __device__ Dual32(double a) : v((float)a), d(0.0f) {}
+	__host__ __device__ Dual32(double a, double b) : v((float)a), d((float)b) {}
+	__host__ __device__ Dual32(float a, float b) : v(a), d(b) {}
+};
+__device__ inline Dual32 operator+(Dual32 a, Dual32 b) { return Dual32(a.v + b.v, a.d + b.d); }
+__device__ inline Dual32 operator-(Dual32 a, Dual32 b) { return Dual32(a.v - b.v, a.d - b.d); }
+__device__ inline Dual32 operator-(Dual32 a) { return Dual32(-a.v, -a.d); }
+__device__ inline Dual32 operator*(Dual32 a, Dual32 b) { return Dual32(a.v * b.v, a.d * b.v + a.v * b.d); }
+__device__ inline Dual32 operator/(Dual32 a, Dual32 b)
+{
+	const float q = a.v / b.v;
+	return Dual32(q, (a.d - q * b.d) / b.v);
+}
+__device__ inline Dual32& operator+=(Dual32& a, Dual32 b) { a = a + b; return a; }
+__device__ inline Dual32& operator-=(Dual32& a, Dual32 b) { a = a - b; return a; }
+__device__ inline Dual32& operator*=(Dual32& a, Dual32 b) { a = a * b; return a; }
+
 __device__ inline double rv(double a) { return a; }
 __device__ inline float rv(float a) { return a; }
 __device__ inline double rv(Dual a) { return a.v; }
+__device__ inline float rv(Dual32 a) { return a.v; }
 
 __device__ inline double tsqrt(double a) { return sqrt(a); }
 // FP32: the hardware square root and reciprocal (1 ulp), not the IEEE-rounded sequences
@@ -50,12 +73,22 @@ __device__ inline Dual tsqrt(Dual a)
 	const double s = sqrt(a.v);
 	return Dual(s, a.d / (2.0 * s));
 }
+__device__ inline Dual32 tsqrt(Dual32 a)
+{
+	const float s = __builtin_amdgcn_sqrtf(a.v);
+	return Dual32(s, a.d * (0.5f * __builtin_amdgcn_rcpf(s)));
+}
 __device__ inline double tpow(double a, double b) { return pow(a, b); }
 __device__ inline float tpow(float a, double b) { return powf(a, (float)b); }
 __device__ inline Dual tpow(Dual a, double b)
 {
 	const double pv = pow(a.v, b);
 	return Dual(pv, b * pow(a.v, b - 1.0) * a.d);
+}
+__device__ inline Dual32 tpow(Dual32 a, double b)
+{
+	const float pv = powf(a.v, (float)b);
+	return Dual32(pv, (float)b * powf(a.v, (float)(b - 1.0)) * a.d);
 }
 __device__ inline double trcp(double a) { return 1.0 / a; }
 __device__ inline float trcp(float a) { return __builtin_amdgcn_rcpf(a); }
@@ -64,13 +97,20 @@ __device__ inline Dual trcp(Dual a)
 	const double r = 1.0 / a.v;
 	return Dual(r, -a.d * r * r);
 }
+__device__ inline Dual32 trcp(Dual32 a)
+{
+	const float r = __builtin_amdgcn_rcpf(a.v);
+	return Dual32(r, -a.d * r * r);
+}
 // drop the derivative part (neighbour traces held fixed for diagonal blocks)
 __device__ inline double strip(double a) { return a; }
 __device__ inline float strip(float a) { return a; }
 __device__ inline Dual strip(Dual a) { return Dual(a.v, 0.0); }
+__device__ inline Dual32 strip(Dual32 a) { return Dual32(a.v, 0.0f); }
 
 template <typename T> struct RealOf { using type = T; };
 template <> struct RealOf<Dual> { using type = double; };
+template <> struct RealOf<Dual32> { using type = double; };   // FP64 vectors in and out
 
 constexpr __host__ __device__ int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
 

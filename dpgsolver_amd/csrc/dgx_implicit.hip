@@ -695,11 +695,12 @@ struct Gmres {
 		blas.release();
 	}
 	size_t ls_size() const { return (size_t)(m + 1) * m + 2 * (size_t)m + (m + 1) + 3; }
+	bool jvp32 = false;   // the operator in FP32 dual numbers
 	// w = A v with A = dR/du(u) - M/dt
 	hipError_t op(Ctx& c, const double* u, const double* inv_dt, const double* v, double* out)
 	{
 		// dual-number J v with the -M/dt term added in the same kernel (at the GL nodes, before Ibar^T)
-		return launch_jvp(c, u, v, out, inv_dt);
+		return jvp32 ? launch_jvp32(c, u, v, out, inv_dt) : launch_jvp(c, u, v, out, inv_dt);
 	}
 	// right-preconditioned restarted GMRES on A x = b, x0 = 0; returns iterations, sets converged.
 	// The Hessenberg column and its Givens rotation are formed on the device; the host reads the residual
@@ -832,6 +833,7 @@ extern "C" dgx_status dgx_solve_implicit(dgx_ctx* h, double* u, const dgx_implic
 			fail(DGX_ERR_HIP);
 			break;
 		}
+		gm.jvp32 = prm->jvp_fp32 != 0;
 		if (prm->precond == DGX_PC_BLOCK_JACOBI) {
 			auto* bj = new dgx::PrecondBlockJacobi();
 			bj->cap_blocks = prm->capture_blocks;

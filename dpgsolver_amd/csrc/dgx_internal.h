@@ -339,6 +339,8 @@ hipError_t launch_offd_col(Ctx& c, const double* u, double* blocks, int col);
 hipError_t launch_expand_face_blocks(Ctx& c, const double* cT, double* full);   // compact -> full nb x nb (2-D)
 // jv = J v, or J v - mass_scale[e] M_e v when mass_scale is not NULL (the pseudo-time Newton operator)
 hipError_t launch_jvp(Ctx& c, const double* u, const double* v, double* jv, const double* mass_scale = nullptr);
+// the same in FP32 dual numbers (FP32 geometry and operators; FP64 u, v and jv)
+hipError_t launch_jvp32(Ctx& c, const double* u, const double* v, double* jv, const double* mass_scale = nullptr);
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, int n, unsigned long long v);
 hipError_t launch_fill_i32(hipStream_t s, int* p, int n, int v);
 

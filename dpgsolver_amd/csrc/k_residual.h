@@ -29,6 +29,7 @@
 namespace dgx {
 
 __device__ inline Dual operator*(double a, Dual b) { return Dual(a * b.v, a * b.d); }
+__device__ inline Dual32 operator*(float a, Dual32 b) { return Dual32(a * b.v, a * b.d); }
 
 // MODE_PERT: FP32 residual of an FP64 state in per-element perturbation form.  With c_e the state at
 // the element's first node, the kernel interpolates and contracts u - c_e and F(u) - F(c_e) in FP32

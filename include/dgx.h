@@ -365,6 +365,10 @@ typedef struct dgx_implicit_params {
 	float* capture_features;   /* nf = round_up(nb + Np + 1, 16) floats per element */
 	float* capture_sigma;      /* 1 float per element */
 	long long capture_capacity;
+	int jvp_fp32;         /* 1: the GMRES operator dR/du v - M v / dt in FP32 dual numbers (FP32 geometry and
+	                         operators, FP64 vectors): the inner defect-correction product; the FP64 residual of
+	                         each Newton step still decides convergence.  0: FP64 (no counterpart: the reference
+	                         assembles the FP64 Jacobian) */
 } dgx_implicit_params;
 typedef struct dgx_implicit_stats {
 	int newton_steps;

@@ -389,6 +389,32 @@ def test_steady_channel_block_sgs_fewer_iterations(dev):
     ctx.close()
 
 
+def test_steady_channel_fp32_inner_operator(dev):
+    """The GMRES operator in FP32 dual numbers (jvp_fp32: the inner defect-correction product over the FP32
+    geometry), with the FP64 Newton residual deciding convergence: the same steady state as the FP64 operator,
+    the same Newton steps, and GMRES iterations within 5%."""
+    from dpgsolver_amd import _lib
+
+    mesh = M.channel_mesh(12, 6, 2)
+    prim = M.freestream_prim(2)
+    ctx = _ctx(mesh, M.PDE_EULER, M.FLUX_ROE, bc_prim=prim)
+    out = {}
+    for fp32 in (False, True):
+        ud = torch.from_numpy(M.uniform_state(mesh, prim)).to(dev)
+        st = ctx.solve_implicit(ud.data_ptr(), cfl_initial=10.0, exit_tol=1e-10, max_newton=80,
+                                precond=_lib.PC_BLOCK_JACOBI, jvp_fp32=fp32)
+        print("jvp_fp32", fp32, "newton", st["newton_steps"], "gmres", st["gmres_iters"], st["seconds"])
+        assert st["converged"] == 1 and st["max_rhs"] < 1e-10
+        out[fp32] = (st, ud.cpu().numpy())
+    (s64, u64), (s32, u32) = out[False], out[True]
+    d = _rel(u32, u64)
+    print(f"steady states rel diff {d:.2e}")
+    assert d < 1e-9
+    assert s32["newton_steps"] == s64["newton_steps"]
+    assert abs(s32["gmres_iters"] - s64["gmres_iters"]) <= 0.05 * s64["gmres_iters"] + 4
+    ctx.close()
+
+
 def test_steady_solve_on_a_nonblocking_stream_matches_default_stream(dev):
     """The solver reads its norms and flags on the context's stream: a non-blocking caller stream (which does
     not order with the null stream) gives the same solve as the default stream."""
