@@ -58,6 +58,7 @@ void free_hp(Ctx& c)
 		release(t.met32); release(t.jinv32); release(t.minv32); release(t.K32);
 	}
 	release(H->tri_ops64); release(H->tri_ops32); release(H->mfaces); release(H->mnJ); release(H->mnJ32);
+	release(H->mxy);
 	for (int k = 0; k < dgx::HpData::kLanes; ++k) {
 		if (H->lane[k]) (void)hipStreamDestroy(H->lane[k]);
 		if (H->join_ev[k]) (void)hipEventDestroy(H->join_ev[k]);
@@ -589,6 +590,7 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 	std::vector<std::vector<int>> gfaces(G);
 	std::vector<dgx::HpFace> hf;
 	std::vector<dgx::MixFace> mf;
+	std::vector<int> mface_elem;   // per mixed face: its (TRI) element when it is a rotating wall, else -1
 	std::vector<double> nj, njh, mnj;   // n J_f of the face (lower-order side) and of the higher-order side's own face
 	double xg[dgx::kHpMaxN], wg[dgx::kHpMaxN], tmp[2 * dgx::kHpMaxN];
 	for (int f = 0; f < F; ++f) {
@@ -606,10 +608,14 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 			gfaces[g].insert(gfaces[g].end(), row, row + 6);
 		};
 		if (bc == DGX_BC_INTERIOR && (eR < 0 || eR >= E || fR < 0 || fR >= (kind(eR) ? 3 : 4))) return DGX_ERR_ARG;
+		// boundary codes: Euler takes Riemann and slip walls, Navier-Stokes Riemann and the no-slip walls
+		if (bc != DGX_BC_INTERIOR) {
+			const bool noslip = bc == DGX_BC_NOSLIP_ADIABATIC || bc == DGX_BC_NOSLIP_DIABATIC || bc == DGX_BC_NOSLIP_ROTATING;
+			if (bc != DGX_BC_RIEMANN && bc != DGX_BC_SLIPWALL && !noslip) return DGX_ERR_ARG;
+			if ((c.cfg.pde == DGX_PDE_NAVIER_STOKES) ? bc == DGX_BC_SLIPWALL : noslip) return DGX_ERR_UNSUPPORTED;
+		}
 		if (kind(eL) || (bc == DGX_BC_INTERIOR && kind(eR))) {
 			// a TRI side: evaluated by the mixed face kernel, QUAD sides read its lifted flux
-			if (bc != DGX_BC_INTERIOR && (bc != DGX_BC_RIEMANN && bc != DGX_BC_SLIPWALL)) return DGX_ERR_ARG;
-			if (bc != DGX_BC_INTERIOR && c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;   // viscous BCs not built
 			dgx::MixFace x{};
 			const bool inter = (bc == DGX_BC_INTERIOR);
 			x.uL = uoff[eL];
@@ -635,6 +641,7 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 				x.NR = x.NL;
 			}
 			mf.push_back(x);
+			mface_elem.push_back(bc == DGX_BC_NOSLIP_ROTATING ? eL : -1);
 			const int Nf = std::max(x.NL, x.NR);
 			dgx::gauss_legendre(Nf, xg, wg);
 			const size_t base = mnj.size();
@@ -650,8 +657,6 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 			continue;
 		}
 		if (bc != DGX_BC_INTERIOR) {
-			// Navier-Stokes walls and Riemann faces are built for uniform-order meshes only
-			if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;
 			push(gL, eL - e_off[gL], fL, -1, -1, ord, bc);
 			continue;
 		}
@@ -705,6 +710,7 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 		m.eR = x.eR;
 		m.tL = m.tR = -1;
 		mf.push_back(m);
+		mface_elem.push_back(-1);
 		mnj.insert(mnj.end(), nj.begin() + base, nj.begin() + base + 2 * dgx::kHpMaxN);
 	}
 	// faces with a TRI side sorted by (Nf, kL, NL, kR, NR) (stable): one launch per Nf, waves of one shape
@@ -718,8 +724,10 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 		std::stable_sort(mp.begin(), mp.end(), [&](int a, int b) { return mkey(mf[a]) < mkey(mf[b]); });
 		std::vector<dgx::MixFace> ms(mf.size());
 		std::vector<double> mn(mnj.size());
+		std::vector<int> me(mf.size());
 		for (size_t i = 0; i < mf.size(); ++i) {
 			ms[i] = mf[mp[i]];
+			me[i] = mface_elem[mp[i]];
 			const size_t src = (size_t)mp[i] * 2 * dgx::kHpMaxN, dst = i * 2 * dgx::kHpMaxN;
 			std::copy(mnj.begin() + src, mnj.begin() + src + 2 * dgx::kHpMaxN, mn.begin() + dst);
 			const int Nf = std::max(ms[i].NL, ms[i].NR);
@@ -728,6 +736,7 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 		}
 		mf.swap(ms);
 		mnj.swap(mn);
+		mface_elem.swap(me);
 	}
 	// faces sorted by their (NL, NR) pair (stable): one kernel instance per pair, every wave on one pair
 	std::vector<int> fperm(hf.size());
@@ -916,6 +925,35 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 				}
 			}
 		}
+	}
+	// rotating walls on TRI faces: the coordinates of the face's GL points (its own, the face having one side)
+	if (std::any_of(mface_elem.begin(), mface_elem.end(), [](int e) { return e >= 0; })) {
+		std::vector<double> xy(mf.size() * dgx::kHpMaxN * 2, 0.0);
+		std::vector<dgx::TriOps<double>> tcache(dgx::kTriMaxP + 1);
+		std::vector<bool> built(dgx::kTriMaxP + 1, false);
+		for (size_t i = 0; i < mf.size(); ++i) {
+			const int e = mface_elem[i];
+			if (e < 0) continue;
+			if (!built[pe[e]]) {
+				if (!dgx::build_tri_ops(pe[e], tcache[pe[e]])) return DGX_ERR_UNSUPPORTED;
+				built[pe[e]] = true;
+			}
+			const dgx::TriOps<double>& to = tcache[pe[e]];
+			const double* xe = xyz + xoff[e];
+			const int N = pe[e] + 1, Np = to.Np;
+			for (int q = 0; q < N; ++q) {
+				double px = 0.0, py = 0.0;
+				for (int m = 0; m < Np; ++m) {
+					const double l = to.Ifc[mf[i].lfL][q * Np + m];
+					px += l * xe[m];
+					py += l * xe[Np + m];
+				}
+				xy[(i * dgx::kHpMaxN + q) * 2] = px;
+				xy[(i * dgx::kHpMaxN + q) * 2 + 1] = py;
+			}
+		}
+		DGX_TRY(hipMalloc(&H.mxy, sizeof(double) * xy.size()));
+		DGX_TRY(hipMemcpy(H.mxy, xy.data(), sizeof(double) * xy.size(), hipMemcpyHostToDevice));
 	}
 	H.Fmix = (int)mf.size();
 	H.mruns = mruns;

@@ -362,4 +362,37 @@ __device__ inline void ns_viscous_flux2(const S* s, const S* g, S mu, S Pr, S* f
 	}
 }
 
+// Navier-Stokes wall data: the provided normal viscous energy flux of the no-slip flux walls, and the rotating
+// wall's angular velocity, density and total energy (dgx_config wall_*)
+struct NsWalls {
+	double nfE, omega, rho, E;
+};
+
+// Boundary (ghost) state of a 2-D Navier-Stokes boundary face node: Riemann (boundary_euler_T.c:99-180); the
+// no-slip adiabatic / diabatic walls keep rho and E and negate the momentum (-rho u_l + 2 rho u_ex, u_ex = 0;
+// constructor_Boundary_Value_T_navier_stokes_no_slip_flux_general, boundary_navier_stokes_T.c:260-290); the
+// rotating wall is 2 s_ex - s_l with s_ex = (rho_b, rho_b u_ex, E_b), u_ex = omega (-y, x) at the node xy
+// (no_slip_all_general :200-240, compute_uvw_ex_rotating :505-520).  The boundary gradient is the interior one.
+template <typename T>
+__device__ inline void ns_wall_state(int bc, const T* sL, const T* n, const double* bc_prim, const NsWalls& w,
+                                     const double* xy, T* sB)
+{
+	if (bc == DGX_BC_RIEMANN) {
+		bc_state<T, 2>(bc, sL, n, bc_prim, sB);
+		return;
+	}
+	if (bc == DGX_BC_NOSLIP_ROTATING) {
+		const double ux = -w.omega * xy[1], uy = w.omega * xy[0];
+		sB[0] = -sL[0] + T(2.0 * w.rho);
+		sB[1] = -sL[1] + T(2.0 * w.rho * ux);
+		sB[2] = -sL[2] + T(2.0 * w.rho * uy);
+		sB[3] = -sL[3] + T(2.0 * w.E);
+		return;
+	}
+	sB[0] = sL[0];
+	sB[1] = -sL[1];
+	sB[2] = -sL[2];
+	sB[3] = sL[3];
+}
+
 }  // namespace dgx

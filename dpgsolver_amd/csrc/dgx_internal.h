@@ -188,6 +188,7 @@ struct HpData {
 	MixFace* mfaces = nullptr;             // Fmix, sorted by (Nf, kL, NL, kR, NR)
 	double* mnJ = nullptr;                 // Fmix x kHpMaxN x 2: n J_f at the face GL points (left ordering)
 	float* mnJ32 = nullptr;
+	double* mxy = nullptr;                 // Navier-Stokes rotating walls: GL point coordinates (Fmix x kHpMaxN x 2)
 	int Fmix = 0;
 	std::vector<MixRun> mruns;             // runs of mixed faces with one Nf
 	std::vector<int> mrun_lane;            // side stream of each run (longest-first balance of count Nf^2)

@@ -67,36 +67,10 @@ struct NsArgs {
 	// boundary faces: the Riemann free stream, the no-slip wall data and (rotating walls) the face GL point
 	// coordinates, F x N x 2
 	double bc_prim[5];
-	double wall_nfE, wall_omega, wall_rho, wall_E;
+	NsWalls walls;
 	const double* wall_xy;
 };
 
-// Boundary (ghost) state of a boundary face node, in FP64: Riemann (boundary_euler_T.c:99-180); the no-slip
-// adiabatic / diabatic walls keep rho and E and negate the momentum (-rho u_l + 2 rho u_ex, u_ex = 0;
-// constructor_Boundary_Value_T_navier_stokes_no_slip_flux_general, boundary_navier_stokes_T.c:260-290); the
-// rotating wall is 2 s_ex - s_l with s_ex = (rho_b, rho_b u_ex, E_b), u_ex = omega (-y, x) (no_slip_all_general
-// :200-240, compute_uvw_ex_rotating :505-520).  The boundary gradient is the interior one (grad_from_internal).
-template <typename S>
-__device__ inline void ns_wall_state(const NsArgs<S>& a, int bc, const double* sL, const double* n, const double* xy,
-                                     double* sB)
-{
-	if (bc == DGX_BC_RIEMANN) {
-		bc_state<double, 2>(bc, sL, n, a.bc_prim, sB);
-		return;
-	}
-	if (bc == DGX_BC_NOSLIP_ROTATING) {
-		const double ux = -a.wall_omega * xy[1], uy = a.wall_omega * xy[0];
-		sB[0] = -sL[0] + 2.0 * a.wall_rho;
-		sB[1] = -sL[1] + 2.0 * a.wall_rho * ux;
-		sB[2] = -sL[2] + 2.0 * a.wall_rho * uy;
-		sB[3] = -sL[3] + 2.0 * a.wall_E;
-		return;
-	}
-	sB[0] = sL[0];
-	sB[1] = -sL[1];
-	sB[2] = -sL[2];
-	sB[3] = sL[3];
-}
 
 // shared per-element set-up: operators, the GLL state, its GL values and (k_rhs_ns) the GL face traces
 template <typename S, int N>
@@ -292,7 +266,7 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 					xy[0] = a.wall_xy[((size_t)f.f * N + i) * 2];
 					xy[1] = a.wall_xy[((size_t)f.f * N + i) * 2 + 1];
 				}
-				ns_wall_state<S>(a, bc, sl, nd, xy, sb);
+				ns_wall_state<double>(bc, sl, nd, a.bc_prim, a.walls, xy, sb);
 #pragma unroll
 				for (int v = 0; v < kNsNV; ++v) nbt[(lf * kNsNV + v) * N + i] = PERT ? S(sb[v] - B.C[slc][v]) : S(sb[v]);
 			}
@@ -401,7 +375,7 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 				for (int v = 0; v < kNsNV; ++v) {
 					const double vis = nd[0] * 0.5 * (fvL[0 + 2 * v] + fvR[0 + 2 * v]) + nd[1] * 0.5 * (fvL[1 + 2 * v] + fvR[1 + 2 * v]);
 					// no-slip flux walls: the energy flux is the provided one (numerical_flux_central_T.c:222-230)
-					const double tot = (nf_provided && v == kNsNV - 1) ? a.wall_nfE : fdn[v] + vis;
+					const double tot = (nf_provided && v == kNsNV - 1) ? a.walls.nfE : fdn[v] + vis;
 					const S r = S((tot - cn[v]) * Jd);
 					RES[slc][(lf * kNsNV + v) * N + dst] = side == 0 ? r : -r;
 				}
@@ -415,7 +389,7 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 				for (int v = 0; v < kNsNV; ++v) {
 					const S vis = nu[0] * S(0.5) * (fvL[0 + 2 * v] + fvR[0 + 2 * v]) + nu[1] * S(0.5) * (fvL[1 + 2 * v] + fvR[1 + 2 * v]);
 					// no-slip flux walls: the energy flux is the provided one (numerical_flux_central_T.c:222-230)
-					const S tot = (nf_provided && v == kNsNV - 1) ? S(a.wall_nfE) : nnf[v] + vis;
+					const S tot = (nf_provided && v == kNsNV - 1) ? S(a.walls.nfE) : nnf[v] + vis;
 					const S r = tot * Jf;
 					RES[slc][(lf * kNsNV + v) * N + dst] = side == 0 ? r : -r;
 				}
@@ -565,10 +539,7 @@ static hipError_t launch_ns(Ctx& c, const S* u, S* r, const S* met, const S* det
 	if constexpr (sizeof(S) == sizeof(double)) a.zf = c.hp_z64;
 	else a.zf = c.hp_z32;
 	for (int k = 0; k < 5; ++k) a.bc_prim[k] = c.cfg.bc_prim[k];
-	a.wall_nfE = c.cfg.wall_nf_E;
-	a.wall_omega = c.cfg.wall_omega;
-	a.wall_rho = c.cfg.wall_rho;
-	a.wall_E = c.cfg.wall_E;
+	a.walls = NsWalls{ c.cfg.wall_nf_E, c.cfg.wall_omega, c.cfg.wall_rho, c.cfg.wall_E };
 	a.wall_xy = c.wall_xy;
 	if (c.has_nonconforming && !a.zf) return hipErrorInvalidValue;
 	switch (c.N) {

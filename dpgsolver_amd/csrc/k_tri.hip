@@ -754,6 +754,8 @@ struct MixArgsT {
 	const T* qkap[kHpMaxN + 1];
 	double bc_prim[5];
 	double mu, Pr;
+	NsWalls walls;                     // Navier-Stokes wall data
+	const double* wxy;                 // rotating walls: coordinates of each face's GL points (Fmix x kHpMaxN x 2)
 	int start, count;
 };
 
@@ -957,7 +959,8 @@ __global__ void __launch_bounds__(256) k_mixf_ns(MixArgsT<T> a)
 	// the own values and normals first, then the operators
 	T oL[kNV], oR[kNV], nj0 = 1.0, nj1 = 0.0;
 	T tL[K8], tR[K8], kL[NF], kR[NF], scL = 1.0, scR = 1.0;
-	const bool ownL = valid && j < f.NL, ownR = valid && j < f.NR, point = valid && j < NF;
+	const bool inter = f.bc == DGX_BC_INTERIOR;   // a boundary face has a TRI left side only
+	const bool ownL = valid && j < f.NL, ownR = valid && inter && j < f.NR, point = valid && j < NF;
 	if (ownL) load_own(a, f.kL, f.NL, f.lfL, f.uL, f.eL, f.tL, j, oL);
 	if (ownR) load_own(a, f.kR, f.NR, f.lfR, f.uR, f.eR, f.tR, j, oR);
 	if (point) {
@@ -983,7 +986,18 @@ __global__ void __launch_bounds__(256) k_mixf_ns(MixArgsT<T> a)
 	const int q = j, qr = f.rev ? NF - 1 - q : q;
 	if (point) {
 		side_at_point<NF>(o, f.kL, f.NL, q, YL, sL);
-		side_at_point<NF>(o, f.kR, f.NR, qr, YL + kNV * NF, sR);
+		if (inter) {
+			side_at_point<NF>(o, f.kR, f.NR, qr, YL + kNV * NF, sR);
+		} else {   // the boundary state (Riemann or a no-slip wall) at this point
+			const T Jf = sqrt(nj0 * nj0 + nj1 * nj1);
+			const T nu[2] = { nj0 / Jf, nj1 / Jf };
+			double xy[2] = { 0.0, 0.0 };
+			if (f.bc == DGX_BC_NOSLIP_ROTATING) {
+				xy[0] = a.wxy[((size_t)fi * kHpMaxN + q) * 2];
+				xy[1] = a.wxy[((size_t)fi * kHpMaxN + q) * 2 + 1];
+			}
+			ns_wall_state<T>(f.bc, sL, nu, a.bc_prim, a.walls, xy, sR);
+		}
 #pragma unroll
 		for (int v = 0; v < kNV; ++v) {
 			const T dl = T(0.5) * (sR[v] - sL[v]);
@@ -1048,8 +1062,10 @@ __global__ void __launch_bounds__(256) k_mixf_ns(MixArgsT<T> a)
 				g[k] = acc;
 			}
 			ns_viscous_flux2<T>(sL, g, a.mu, a.Pr, fvL);
+			// boundary faces: the boundary gradient is the interior one (grad_from_internal)
+			if (!inter) ns_viscous_flux2<T>(sR, g, a.mu, a.Pr, fvR);
 		}
-		{
+		if (inter) {
 			T g[K8];
 			const T* P = o.Pg[f.NR - 2];
 #pragma unroll
@@ -1063,10 +1079,12 @@ __global__ void __launch_bounds__(256) k_mixf_ns(MixArgsT<T> a)
 			ns_viscous_flux2<T>(sR, g, a.mu, a.Pr, fvR);
 		}
 		const T wj = o.w[q] * Jf;
+		// no-slip flux walls: the energy flux is the provided one (numerical_flux_central_T.c:222-230)
+		const bool nf_provided = f.bc == DGX_BC_NOSLIP_ADIABATIC || f.bc == DGX_BC_NOSLIP_DIABATIC;
 #pragma unroll
 		for (int v = 0; v < kNV; ++v) {
 			const T vis = nu[0] * T(0.5) * (fvL[0 + 2 * v] + fvR[0 + 2 * v]) + nu[1] * T(0.5) * (fvL[1 + 2 * v] + fvR[1 + 2 * v]);
-			fq[v] = wj * (nnf[v] + vis);
+			fq[v] = wj * ((nf_provided && v == kNV - 1) ? T(a.walls.nfE) : nnf[v] + vis);
 		}
 	}
 	// every q was read before the y step's sync
@@ -1149,6 +1167,8 @@ MixArgsT<T> mix_args(const HpData& H, const Ctx& c, const T* u)
 	for (int k = 0; k < 5; ++k) a.bc_prim[k] = c.cfg.bc_prim[k];
 	a.mu = c.cfg.mu;
 	a.Pr = c.cfg.Pr;
+	a.walls = NsWalls{ c.cfg.wall_nf_E, c.cfg.wall_omega, c.cfg.wall_rho, c.cfg.wall_E };
+	a.wxy = H.mxy;
 	return a;
 }
 

@@ -282,14 +282,14 @@ def annulus_mesh(p: int, nr: int, nt: int, r_i: float = 0.5, r_o: float = 1.0, b
                               bc_low=(bc_inner, BC_INTERIOR), bc_high=(bc_outer, BC_INTERIOR))
 
 
-def taylor_couette_state(mesh: Mesh, mu: float, Pr: float, omega: float = 1.0, r_i: float = 0.5, r_o: float = 1.0,
-                         dTdr_o: float = 0.0, t_i: float = 1.0, p_i: float = 1.0):
-    """The reference's Taylor-Couette test state (solution_taylor_couette_T.c, specific gas constant 1) at the
-    mesh nodes: V_theta = c (1/r - r/r_o^2) with c = omega / (1/r_i^2 - 1/r_o^2), the conduction-dissipation
-    temperature profile with dT/dr = dTdr_o at r_o, uniform pressure p_i, rho = p / T.  Returns (state, rate):
-    rate is -div F of that state, rho V^2 / r e_r in the momentum equations and 0 otherwise (the uniform
-    pressure leaves the centripetal term unbalanced; the theta-momentum and energy equations hold exactly)."""
-    x = nodes_xyz(mesh)
+def taylor_couette_fields(x: np.ndarray, mu: float, Pr: float, omega: float = 1.0, r_i: float = 0.5, r_o: float = 1.0,
+                          dTdr_o: float = 0.0, t_i: float = 1.0, p_i: float = 1.0):
+    """The reference's Taylor-Couette test state (solution_taylor_couette_T.c, specific gas constant 1) at points
+    x (..., 2): V_theta = c (1/r - r/r_o^2) with c = omega / (1/r_i^2 - 1/r_o^2), the conduction-dissipation
+    temperature profile with dT/dr = dTdr_o at r_o, uniform pressure p_i, rho = p / T.  Returns (conserved state,
+    rate), both (..., 4): rate is -div F of that state, rho V^2 / r e_r in the momentum equations and 0 otherwise
+    (the uniform pressure leaves the centripetal term unbalanced; the theta-momentum and energy equations hold
+    exactly)."""
     X, Y = x[..., 0], x[..., 1]
     kappa = mu * GAMMA / GM1 / Pr
     c = omega / (1.0 / r_i ** 2 - 1.0 / r_o ** 2)
@@ -300,7 +300,13 @@ def taylor_couette_state(mesh: Mesh, mu: float, Pr: float, omega: float = 1.0, r
     rho = p_i / t
     prim = np.stack([rho, -np.sin(th) * Vt, np.cos(th) * Vt, p_i + 0 * X], -1)
     rate = np.stack([0 * X, rho * Vt ** 2 / r * np.cos(th), rho * Vt ** 2 / r * np.sin(th), 0 * X], -1)
-    return pack_state(prim_to_cons(prim, 2)), pack_state(rate)
+    return prim_to_cons(prim, 2), rate
+
+
+def taylor_couette_state(mesh: Mesh, mu: float, Pr: float, **kw):
+    """taylor_couette_fields at the nodes of a QUAD mesh, packed: (state, rate)."""
+    cons, rate = taylor_couette_fields(nodes_xyz(mesh), mu, Pr, **kw)
+    return pack_state(cons), pack_state(rate)
 
 
 def taylor_couette_walls(mu: float, Pr: float, omega: float = 1.0, r_i: float = 0.5, dTdr_o: float = 0.0,
@@ -850,6 +856,25 @@ def structured_mixed_mesh(n: tuple, split, p_elem, lo=(0.0, 0.0), hi=(1.0, 1.0),
     faces[inter, 2] = new_of[faces[inter, 2]]
     return MixedMesh(kind=kinds[order].copy(), p=p_elem[order].copy(), xyz=[xyz[e] for e in order],
                      faces=faces.astype(np.int32))
+
+
+def annulus_mixed_mesh(n: tuple, split, p_elem, r_i: float = 0.5, r_o: float = 1.0,
+                       bc_inner: int = BC_NOSLIP_ROTATING, bc_outer: int = BC_NOSLIP_ADIABATIC) -> MixedMesh:
+    """Mixed TRI/QUAD mesh of the annulus r_i <= r <= r_o (the reference's Taylor-Couette geometry): the
+    structured_mixed_mesh of n = (nr, ntheta) cells, periodic around, mapped onto the annulus (the solution nodes
+    on the exact circles); the inner / outer circle's faces carry bc_inner / bc_outer."""
+    def curve(pts):
+        r = r_i + (r_o - r_i) * pts[..., 0]
+        th = 2.0 * np.pi * pts[..., 1]
+        return np.stack([r * np.cos(th), r * np.sin(th)], -1)
+    return structured_mixed_mesh(n, split, p_elem, lo=(0.0, 0.0), hi=(1.0, 1.0), periodic=(False, True), curve=curve,
+                                 bc_low=(bc_inner, BC_INTERIOR), bc_high=(bc_outer, BC_INTERIOR))
+
+
+def taylor_couette_mixed_state(mesh: MixedMesh, mu: float, Pr: float, **kw):
+    """taylor_couette_fields at the nodes of a mixed mesh: (state, rate) in the mixed_state layout."""
+    return (mixed_state(mesh, lambda x: taylor_couette_fields(x, mu, Pr, **kw)[0]),
+            mixed_state(mesh, lambda x: taylor_couette_fields(x, mu, Pr, **kw)[1]))
 
 
 def mixed_state(mesh: MixedMesh, cons_fn) -> np.ndarray:

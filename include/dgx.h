@@ -34,7 +34,7 @@ enum {
 
 /* DGX_PDE_NAVIER_STOKES: Euler + BR2 viscous terms (2-D QUAD; FP64, FP32 and mixed residual, FP64 explicit
  * path), constant viscosity mu and Prandtl number Pr from the config.  Boundary faces: Riemann and the no-slip
- * walls (uniform-order meshes; variable-order and mixed meshes take interior faces only). */
+ * walls, on uniform-order, variable-order and mixed TRI/QUAD meshes. */
 enum { DGX_PDE_ADVECTION = 0, DGX_PDE_EULER = 1, DGX_PDE_NAVIER_STOKES = 2 };
 /* num_flux_1st: upwind | Lax-Friedrichs | Roe-Pike (numerical_flux/definitions_numerical_flux.h:22-32) */
 enum { DGX_FLUX_UPWIND = 0, DGX_FLUX_LF = 1, DGX_FLUX_ROE = 2 };

@@ -183,8 +183,10 @@ class MixedOracle:
     """Euler residual (and BR2 Navier-Stokes with viscous=(mu, Pr)) on a mixed mesh (dpgsolver_amd.mesh.MixedMesh:
     kind, p, xyz per element at its solution nodes, faces {eL, fL, eR, fR, ord, bc}, offsets(nv))."""
 
-    def __init__(self, mesh, flux, bc_prim=None, viscous=None):
+    def __init__(self, mesh, flux, bc_prim=None, viscous=None, walls=None):
+        """walls = (nf_E, omega, rho_b, E_b): the Navier-Stokes no-slip wall data (dgx_config wall_*)."""
         self.mesh, self.flux, self.viscous = mesh, flux, viscous
+        self.walls = (0.0, 0.0, 0.0, 0.0) if walls is None else tuple(float(w) for w in walls)
         self.nv = 4
         self.bc_prim = np.zeros(5) if bc_prim is None else np.resize(np.asarray(bc_prim, float), 5)
         self.off = mesh.offsets(self.nv)
@@ -252,6 +254,7 @@ class MixedOracle:
         out = dict(eL=eL, fL=fL, eR=eR, fR=fR, bc=bc, Nf=Nf, wq=wq, IL=IL, sL=sL)
         if bc != 0:
             out["nJ"] = nJL
+            out["xy"] = IL @ m.xyz[eL].T                                # (Nf, 2) the face points (isoparametric)
             return out
         sR_pts = -sq if ord_ == 1 else sq                               # the right face coordinate of point q
         IR, nJR = self._side(eR, fR, sR_pts)
@@ -291,6 +294,29 @@ class MixedOracle:
             r[self.off[eR]:self.off[eR + 1]] += (fs["IR"].T @ gq).T.ravel()
 
     # ---------------------------------------------------------------------------------------------- BR2
+    def _ns_boundary_states(self, fs):
+        """Navier-Stokes boundary states at a boundary face's points: Riemann (the Euler state); the no-slip
+        adiabatic / diabatic walls keep rho and E and negate the momentum (boundary_navier_stokes_T.c:260-290);
+        the rotating wall is 2 s_ex - s_l, s_ex = (rho_b, rho_b u_ex, E_b) with u_ex = omega r (-sin th, cos th)
+        at the point (:200-240, :505-520)."""
+        nf_E, omega, rho_b, E_b = self.walls
+        n = fs["nJ"] / np.sqrt((fs["nJ"] ** 2).sum(0))
+        out = np.empty_like(fs["sL"])
+        for q in range(fs["Nf"]):
+            sl = fs["sL"][q]
+            if fs["bc"] == 1:
+                out[q] = O.bc_state(2, 1, sl, n[:, q], self.bc_prim)
+            elif fs["bc"] == 5:
+                x, y = fs["xy"][q]
+                r, th = np.hypot(x, y), np.arctan2(y, x)
+                uex = np.array([-np.sin(th), np.cos(th)]) * omega * r
+                out[q] = np.concatenate([[2 * rho_b - sl[0]], 2 * rho_b * uex - sl[1:3], [2 * E_b - sl[3]]])
+            elif fs["bc"] in (3, 4):
+                out[q] = np.array([sl[0], -sl[1], -sl[2], sl[3]])
+            else:
+                raise ValueError(f"boundary code {fs['bc']} has no Navier-Stokes condition")
+        return out
+
     def ns_gradient(self, u):
         """(gfull, gv, gf): per element the weak gradient (d, nv, Np_e), its volume part, and per face the lifts
         of both sides (compute_grad_coef_dg_T.c:171-307, central s_num)."""
@@ -308,10 +334,11 @@ class MixedOracle:
         gf = []
         for row in m.faces:
             fs = self._face_setup(u, row)
-            assert fs["bc"] == 0, "Navier-Stokes: interior faces only"
+            if fs["bc"] != 0:   # s_num = (s_L + s_b) / 2 (compute_g_coef_related_boundary), left side only
+                fs["sR"] = self._ns_boundary_states(fs)
             dl = 0.5 * (fs["sR"] - fs["sL"])
             pair = []
-            for side in (0, 1):
+            for side in ((0, 1) if fs["bc"] == 0 else (0,)):
                 e, I = (fs["eL"], fs["IL"]) if side == 0 else (fs["eR"], fs["IR"])
                 Minv = np.linalg.inv(self.mass(e))
                 lift = np.stack([(Minv @ (I.T @ ((fs["wq"] * fs["nJ"][dim])[:, None] * dl))).T for dim in range(2)])
@@ -331,7 +358,11 @@ class MixedOracle:
         for k, row in enumerate(self.mesh.faces):
             fs = self._face_setup(u, row)
             gL = np.einsum("qi,dvi->qdv", fs["IL"], gv[fs["eL"]] + eta * gf[k][0])
-            gR = np.einsum("qi,dvi->qdv", fs["IR"], gv[fs["eR"]] + eta * gf[k][1])
+            if fs["bc"] == 0:
+                gR = np.einsum("qi,dvi->qdv", fs["IR"], gv[fs["eR"]] + eta * gf[k][1])
+            else:               # the boundary state, and the interior gradient as the boundary one
+                fs["sR"] = self._ns_boundary_states(fs)
+                gR = gL
             Jf = np.sqrt((fs["nJ"] ** 2).sum(0))
             n = fs["nJ"] / Jf
             tot = np.empty((fs["Nf"], 4))
@@ -340,5 +371,7 @@ class MixedOracle:
                 fvL = O.viscous_flux(2, fs["sL"][q], gL[q].reshape(-1), mu, Pr)
                 fvR = O.viscous_flux(2, fs["sR"][q], gR[q].reshape(-1), mu, Pr)
                 tot[q] = inv + 0.5 * ((fvL + fvR) @ n[:, q])
+                if fs["bc"] in (3, 4):   # no-slip flux walls: the provided energy flux (numerical_flux_central_T.c)
+                    tot[q][3] = self.walls[0]
             self._lift(r, fs, (fs["wq"] * Jf)[:, None] * tot)
         return r

@@ -205,7 +205,7 @@ def test_hp_refusals(dev):
                               bc_high=(M.BC_RIEMANN, M.BC_SLIPWALL))
     ctx = _lib.Context(2, 5, M.PDE_NAVIER_STOKES, M.FLUX_ROE, mu=0.01)
     with pytest.raises(_lib.DgxError, match="UNSUPPORTED"):
-        ctx.upload_mesh_hp(ch)                      # viscous boundary conditions are not built
+        ctx.upload_mesh_hp(ch)                      # the slip wall is an Euler condition
     ctx.close()
     ctx = _ctx(mesh, M.FLUX_ROE)
     u = torch.from_numpy(M.hp_state(mesh, _state_fn(0))).to(dev)

@@ -7,6 +7,8 @@ Navier-Stokes (BR2) on periodic mixed meshes against MixedOracle.rhs_ns, same ba
 p = 1, 3, 5, random orders on curved meshes with both fluxes, an all-TRI curved mesh; free stream, conservation.
 Explicit path: M^-1 R (dgx_compute_rhs_explicit, the TRI mass inverse from the reference's scaled by 1/det J on
 affine elements) against the oracle's dense solve, with its max-norm; three SSP-RK3 steps against the oracle's.
+Navier-Stokes walls (Riemann-free Taylor-Couette annuli: rotating inner, adiabatic or diabatic outer wall) on
+mixed and QUAD variable-order meshes against the oracle at the same bar.
 FP32 (dgx_compute_rhs_f32: FP32 state, geometry and operators) against the FP64 residual, bounded by a multiple
 of the error that rounding the state alone to FP32 causes (the FP64 residual of the rounded state).
 Also: graph replays equal the eager call, and the entry points not built for TRI elements refuse the mesh
@@ -148,7 +150,7 @@ def test_entry_points_not_built_for_tri_refuse(dev):
             call()
         assert ei.value.code == -3                  # DGX_ERR_UNSUPPORTED
     ctx.close()
-    # Navier-Stokes: no viscous boundary conditions
+    # Navier-Stokes: the slip wall is an Euler condition (the box case has slip walls)
     ns = _lib.Context(2, 5, M.PDE_NAVIER_STOKES, M.FLUX_ROE, mu=MU, Pr=PR)
     with pytest.raises(_lib.DgxError) as ei:
         ns.upload_mesh_mixed(CASES[-1][1])
@@ -362,4 +364,65 @@ def test_mixed_mesh_fp32_residual(dev, case):
         assert torch.equal(r, r32)
         torch.cuda.synchronize()
     g.close()
+    ctx.close()
+
+
+# ---------------------------------------------------------------------------------------------- walls
+def _wall_cases():
+    out = []
+    rng = np.random.default_rng(23)
+    for flux, fname in ((M.FLUX_ROE, "roe"), (M.FLUX_LF, "lf")):
+        for bco, dT, wname in ((M.BC_NOSLIP_ADIABATIC, 0.0, "adiabatic"), (M.BC_NOSLIP_DIABATIC, 1.0, "diabatic")):
+            n = (3, 10)
+            split = rng.random(30) < 0.5
+            p = rng.integers(1, 6, int(30 + split.sum())).astype(np.int32)
+            out.append((f"annulus_random_orders_{wname}_{fname}", M.annulus_mixed_mesh(n, split, p, bc_outer=bco),
+                        flux, dT))
+    # QUAD only, variable order: the walls are the order groups' own boundary faces
+    p = rng.integers(1, 6, 30).astype(np.int32)
+    out.append(("annulus_quad_variable_order_diabatic",
+                M.annulus_mixed_mesh((3, 10), np.zeros(30, bool), p, bc_outer=M.BC_NOSLIP_DIABATIC), M.FLUX_ROE, 1.0))
+    return out
+
+
+WALL_CASES = _wall_cases()
+
+
+@pytest.mark.parametrize("case", WALL_CASES, ids=[c[0] for c in WALL_CASES])
+def test_mixed_mesh_ns_walls_match_oracle(dev, case):
+    from dpgsolver_amd import _lib
+
+    name, mesh, flux, dTdr_o = case
+    mu, Pr = 0.05, 0.72
+    walls = M.taylor_couette_walls(mu, Pr, dTdr_o=dTdr_o)
+    print(f"case {name}: E={mesh.E} (TRI {int(mesh.kind.sum())}), F={mesh.F}", flush=True)
+    u, _ = M.taylor_couette_mixed_state(mesh, mu, Pr, dTdr_o=dTdr_o)
+    u = u * (1.0 + 0.01 * np.sin(np.arange(u.size)))          # every wall term active, R far from zero
+    orc = MO.MixedOracle(mesh, flux, viscous=(mu, Pr), walls=walls)
+    ref = orc.rhs_ns(u)
+    ctx = _lib.Context(2, int(mesh.p.max()), M.PDE_NAVIER_STOKES, flux, mu=mu, Pr=Pr, walls=walls)
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    ctx.upload_mesh_mixed(mesh)
+    ud = torch.from_numpy(u).to(dev)
+    rd = torch.full_like(ud, float("nan"))
+    ctx.compute_rhs(ud.data_ptr(), rd.data_ptr())
+    torch.cuda.synchronize()
+    got = rd.cpu().numpy()
+    assert np.isfinite(got).all()
+    scale = max(np.abs(orc.rhs(u, "volume")).max(), np.abs(ref).max())
+    err = np.abs(got - ref).max() / scale
+    print(f"  rel err {err:.2e}", flush=True)
+    assert err < TOL64, err
+    # FP32 against FP64, bounded by the input-rounding floor
+    u32 = ud.float()
+    rr = torch.empty_like(ud)
+    ctx.compute_rhs(u32.double().data_ptr(), rr.data_ptr())
+    r32 = torch.full_like(u32, float("nan"))
+    ctx.compute_rhs_f32(u32.data_ptr(), r32.data_ptr())
+    torch.cuda.synchronize()
+    s = rd.abs().max().item()
+    e32 = (r32.double() - rd).abs().max().item() / s
+    floor = (rr - rd).abs().max().item() / s
+    print(f"  FP32 {e32:.2e}, input-rounding floor {floor:.2e}", flush=True)
+    assert e32 < 8 * floor
     ctx.close()

@@ -241,3 +241,38 @@ def test_mixed_ns_gradient_of_a_linear_state_and_its_integral():
     vol = sum(np.abs(np.einsum("ij,dvj->dv", orc.mass(e), gv[e])).sum() for e in range(mesh.E))
     assert np.abs(tot).max() < 1e-14 * vol, (tot, vol)
     assert vol > 1e-2
+
+
+# ---------------------------------------------------------------------------------------------- no-slip walls
+def test_mixed_ns_walls_equal_the_c_oracle_on_quad_annuli():
+    """The mixed oracle's wall faces (its own restatement of the boundary states, gradients and energy flux)
+    equal the C oracle's, pinned on Taylor-Couette (tests/test_ns_oracle.py), on all-QUAD annuli."""
+    import pyoracle as O
+
+    mu, Pr = 0.1, 0.72
+    for p in (2, 3):
+        for bco, dT in ((M.BC_NOSLIP_ADIABATIC, 0.0), (M.BC_NOSLIP_DIABATIC, 1.0)):
+            qm = M.annulus_mesh(p, 2, 8, bc_outer=bco)
+            mm = M.annulus_mixed_mesh((2, 8), np.zeros(16, bool), np.full(16, p, np.int32), bc_outer=bco)
+            walls = M.taylor_couette_walls(mu, Pr, dTdr_o=dT)
+            u, _ = M.taylor_couette_state(qm, mu, Pr, dTdr_o=dT)
+            um, _ = M.taylor_couette_mixed_state(mm, mu, Pr, dTdr_o=dT)
+            assert np.abs(u - um).max() < 1e-14
+            rq = O.Oracle(qm, M.PDE_EULER, M.FLUX_ROE, viscous=(mu, Pr), walls=walls).rhs_ns(u)
+            rm = MO.MixedOracle(mm, M.FLUX_ROE, viscous=(mu, Pr), walls=walls).rhs_ns(um)
+            # measured 3e-14 to 7e-13 (two constructions of the same operators)
+            assert np.abs(rq - rm).max() < 2e-12 * np.abs(rq).max(), (p, bco)
+
+
+def test_mixed_ns_walls_converge_on_a_mixed_annulus():
+    """Taylor-Couette on annuli with half the cells cut into TRIs: M^-1 R converges to the state's -div F
+    (measured p = 3: 0.354, then 0.149)."""
+    mu, Pr = 0.1, 0.72
+    errs = []
+    for n in ((2, 8), (4, 16)):
+        split = np.random.default_rng(1).random(n[0] * n[1]) < 0.5
+        mm = M.annulus_mixed_mesh(n, split, np.full(n[0] * n[1] + split.sum(), 3, np.int32))
+        u, rate = M.taylor_couette_mixed_state(mm, mu, Pr)
+        orc = MO.MixedOracle(mm, M.FLUX_ROE, viscous=(mu, Pr), walls=M.taylor_couette_walls(mu, Pr))
+        errs.append(np.abs(orc.apply_minv(orc.rhs_ns(u)) - rate).max())
+    assert errs[0] / errs[1] >= 2.0, errs
