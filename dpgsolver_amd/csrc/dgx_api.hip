@@ -74,7 +74,7 @@ void free_mesh(Ctx& c)
 	release(c.xyz); release(c.faces); release(c.elem_faces); release(c.order); release(c.hmin); release(c.metrics_vc); release(c.metrics_vc32);
 	release(c.detJ_vc); release(c.detJ_vc32); release(c.nJ_fc); release(c.nJ_fc32); release(c.u0); release(c.rtmp);
 	release(c.color_elems);
-	release(c.ns_gvt); release(c.ns_kappa); release(c.wall_xy);
+	release(c.ns_gvt); release(c.ns_kappa); release(c.wall_xy); release(c.ns_gvt_dual); release(c.ns_kappa_dual);
 	c.color_off.clear();
 	for (int& g : c.rhs_grid) g = 0;   // launch shapes are tuned per mesh
 	c.mesh_ready = false;
@@ -1444,7 +1444,7 @@ dgx_status dgx_assemble_diag_blocks(dgx_ctx* h, const double* u, const double* m
 	if (!h || !u || !blocks) return DGX_ERR_ARG;
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
-	if (c.d != 2 || c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;   // Euler linearisation only
+	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
 	const int nb = c.Np * c.nv;
 	for (int col = 0; col < nb; ++col) DGX_TRY(dgx::launch_diag_col(c, u, blocks, col));
 	if (mass_scale) DGX_TRY(dgx::launch_sub_mass(c, blocks, mass_scale, 0.0));
@@ -1456,7 +1456,9 @@ dgx_status dgx_assemble_face_blocks(dgx_ctx* h, const double* u, double* blocks)
 	if (!h || !u || !blocks) return DGX_ERR_ARG;
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
-	if (c.d != 2 || c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;   // Euler linearisation only
+	// the face blocks of the Euler and advection residuals (the BR2 residual couples beyond the face neighbours'
+	// face nodes)
+	if (c.d != 2 || c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;
 	const int nfv = c.nv * c.N;   // face-node rows / columns of a 2-D face block
 	double* cT = nullptr;
 	DGX_TRY(hipMalloc(&cT, sizeof(double) * (size_t)c.E * 4 * nfv * nfv));
@@ -1491,7 +1493,7 @@ dgx_status dgx_newton_operator(dgx_ctx* h, const double* u, const double* mass_s
 	if (!h || !u || !mass_scale || !v || !out) return DGX_ERR_ARG;
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
-	if (c.d != 2 || c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;   // Euler linearisation only
+	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
 	return hip_status(dgx::launch_jvp(c, u, v, out, mass_scale));
 }
 
@@ -1500,7 +1502,7 @@ dgx_status dgx_jacobian_vector(dgx_ctx* h, const double* u, const double* v, dou
 	if (!h || !u || !v || !jv) return DGX_ERR_ARG;
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
-	if (c.d != 2 || c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;   // Euler linearisation only
+	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
 	return hip_status(dgx::launch_jvp(c, u, v, jv));
 }
 

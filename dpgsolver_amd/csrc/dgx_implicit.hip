@@ -77,10 +77,11 @@ __global__ void __launch_bounds__(128) k_mass_sub(const double* __restrict__ x, 
 	for (int i = threadIdx.x; i < NV * Np; i += blockDim.x) y[(size_t)e * NV * Np + i] -= s * cur[i];
 }
 
-// per-element dt = cfl * h_min / max(|V| + c) over the Bezier coefficients (solve_dg.c:284-319)
+// per-element dt = cfl * min(h_min / max(|V| + c), h_min^2 / mu) over the Bezier coefficients (the viscous
+// bound with mu > 0, the Navier-Stokes constant viscosity; solve_dg.c:284-319)
 template <int D, int N>
 __global__ void __launch_bounds__(128) k_cfl_dt(const double* __restrict__ u, const double* __restrict__ hmin,
-                                                const Ops1D<double>* ops, int E, double cfl, double* inv_dt)
+                                                const Ops1D<double>* ops, int E, double cfl, double mu, double* inv_dt)
 {
 	constexpr int Np = ipow(N, D), NV = D + 2;
 	__shared__ double B0[NV * Np], B1[NV * Np], sBinv[N * N], sWs[Np];
@@ -107,7 +108,7 @@ __global__ void __launch_bounds__(128) k_cfl_dt(const double* __restrict__ u, co
 	if (threadIdx.x == 0) {
 		double m = 0.0;
 		for (int i = 0; i < Np; ++i) m = fmax(m, sWs[i]);
-		inv_dt[e] = m / (cfl * hmin[e]);
+		inv_dt[e] = fmax(m / (cfl * hmin[e]), mu / (cfl * hmin[e] * hmin[e]));
 	}
 }
 
@@ -414,7 +415,7 @@ hipError_t launch_mass_sub(Ctx& c, const double* x, const double* scale, double*
 hipError_t launch_cfl_dt(Ctx& c, const double* u, double cfl, double* inv_dt)
 {
 	if (c.d != 2 || c.nv != 4) return hipErrorInvalidValue;
-#define ARGS u, c.hmin, c.ops64, c.E, cfl, inv_dt
+#define ARGS u, c.hmin, c.ops64, c.E, cfl, (c.cfg.pde == DGX_PDE_NAVIER_STOKES ? c.cfg.mu : 0.0), inv_dt
 	DGX_PER_ELEM2(k_cfl_dt, 64, )
 #undef ARGS
 	return hipGetLastError();
@@ -793,7 +794,10 @@ extern "C" dgx_status dgx_precond_apply(dgx_ctx* h, int precond, const double* u
 	if (!h || !u || !inv_dt || !v || !z || v == z) return DGX_ERR_ARG;
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
-	if (c.d != 2 || c.cfg.pde != DGX_PDE_EULER) return DGX_ERR_UNSUPPORTED;
+	if (c.d != 2 || c.cfg.pde == DGX_PDE_ADVECTION) return DGX_ERR_UNSUPPORTED;
+	// Navier-Stokes: the diagonal blocks only (the BR2 residual has no face-block linearisation here)
+	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES && precond != DGX_PC_NONE && precond != DGX_PC_BLOCK_JACOBI)
+		return DGX_ERR_UNSUPPORTED;
 	dgx::Precond* P = nullptr;
 	if (precond == DGX_PC_NONE) P = new dgx::PrecondIdentity();
 	else if (precond == DGX_PC_BLOCK_JACOBI) P = new dgx::PrecondBlockJacobi();
@@ -814,7 +818,11 @@ extern "C" dgx_status dgx_solve_implicit(dgx_ctx* h, double* u, const dgx_implic
 	if (!h || !u || !prm || !st) return DGX_ERR_ARG;
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
-	if (c.d != 2 || c.cfg.pde != DGX_PDE_EULER) return DGX_ERR_UNSUPPORTED;
+	if (c.d != 2 || c.cfg.pde == DGX_PDE_ADVECTION) return DGX_ERR_UNSUPPORTED;
+	// Navier-Stokes: block-Jacobi or no preconditioner, the FP64 Jacobian-vector product
+	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES &&
+	    ((prm->precond != DGX_PC_NONE && prm->precond != DGX_PC_BLOCK_JACOBI) || prm->jvp_fp32))
+		return DGX_ERR_UNSUPPORTED;
 	if (prm->gmres_restart < 1 || prm->gmres_restart > 200 || prm->max_newton < 1 || !(prm->cfl_initial > 0.0))
 		return DGX_ERR_ARG;
 	std::memset(st, 0, sizeof(*st));

@@ -228,6 +228,8 @@ struct Ctx {
 	double* ns_gvt = nullptr;       // Navier-Stokes: face traces of the volume gradient (E x 2d x d x nv x Nfc)
 	double* ns_kappa = nullptr;     // Navier-Stokes: BR2 lift trace factor per face node (E x 2d x Nfc)
 	double* wall_xy = nullptr;      // Navier-Stokes with rotating walls: face GL point coordinates (F x Nfc x 2)
+	void* ns_gvt_dual = nullptr;    // Navier-Stokes linearisation: ns_gvt and ns_kappa in dual numbers (first use)
+	void* ns_kappa_dual = nullptr;
 	int rhs_grid[4] = { 0, 0, 0, 0 };   // residual launch grids by TUNE_* (0 = one workgroup per element block)
 	HpData* hp = nullptr;           // variable-order mesh (then this context holds no uniform mesh)
 	const double* hp_z64 = nullptr; // order group of a variable-order mesh: its part of the lifted fluxes
@@ -280,6 +282,9 @@ hipError_t launch_rhs32(Ctx& c, const float* u, float* r);
 hipError_t launch_rhs_ns(Ctx& c, const double* u, double* r);   // 2-D Navier-Stokes (BR2), FP64
 hipError_t launch_rhs_ns32(Ctx& c, const float* u, float* r);    // the same in FP32
 hipError_t launch_rhs_ns_mixed(Ctx& c, const double* u, float* r);   // FP64 state, FP32 perturbation form
+// its linearisation (dual numbers): jv = J v (or J v - mass_scale[e] M_e v), one column of every diagonal block
+hipError_t launch_ns_jvp(Ctx& c, const double* u, const double* v, double* jv, const double* mass_scale);
+hipError_t launch_ns_diag_col(Ctx& c, const double* u, double* blocks, int col);
 hipError_t launch_rhs_mixed(Ctx& c, const double* u, float* r);
 // variable-order meshes: the lifted fluxes of the faces between orders (FP64 / FP32 state); Navier-Stokes also
 // the projected BR2 face products, between the groups' gradient (phase 1) and residual (phase 2) kernels

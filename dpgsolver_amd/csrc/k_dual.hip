@@ -1,6 +1,6 @@
 // k_dual.hip -- forward-mode (dual number) instantiations of the fused residual kernel:
 // exact Jacobian-vector products (FP64, and FP32 for the inner operator of the implicit solve), columns of the
-// element diagonal blocks and of the face blocks
+// element diagonal blocks and of the face blocks (Navier-Stokes: J v and the diagonal blocks from k_ns.hip)
 // (the quantities constructor_lhs_v_1_T / constructor_lhs_f_1_T assemble,
 //  compute_volume_rlhs_T.c:179-220, compute_face_rlhs_T.c:163-220).
 #include "k_residual.h"
@@ -11,6 +11,7 @@ template <int MODE, typename S = Dual, typename R = double>
 static hipError_t dispatch_dual(Ctx& c, const RhsArgs<S, R>& a)
 {
 	if (c.has_nonconforming) return hipErrorInvalidValue;   // no linearisation of faces between orders
+	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) return hipErrorInvalidValue;   // its own kernels (k_ns.hip)
 	const int d = c.d, p = c.p, pde = c.cfg.pde, flux = c.cfg.num_flux;
 #define DGX_DCASE(NN)                                                                                    \
 	if (d == 2 && p + 1 == NN) {                                                                         \
@@ -45,6 +46,7 @@ static RhsArgs<Dual, double> make_dual(Ctx& c, const double* u, const double* v,
 
 hipError_t launch_jvp(Ctx& c, const double* u, const double* v, double* jv, const double* mass_scale)
 {
+	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) return launch_ns_jvp(c, u, v, jv, mass_scale);
 	RhsArgs<Dual, double> a = make_dual(c, u, v, jv, 0);
 	a.mass_scale = mass_scale;
 	return dispatch_dual<MODE_JVP>(c, a);
@@ -72,6 +74,7 @@ hipError_t launch_jvp32(Ctx& c, const double* u, const double* v, double* jv, co
 
 hipError_t launch_diag_col(Ctx& c, const double* u, double* blocks, int col)
 {
+	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) return launch_ns_diag_col(c, u, blocks, col);
 	return dispatch_dual<MODE_DIAG>(c, make_dual(c, u, nullptr, blocks, col));
 }
 

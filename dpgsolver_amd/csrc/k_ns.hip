@@ -1,5 +1,5 @@
 // k_ns.hip -- 2-D Navier-Stokes residual with BR2 viscous terms (QUAD; interior, Riemann and no-slip wall faces),
-// FP64 and FP32
+// FP64 and FP32, and its linearisation in dual numbers (J v, diagonal-block columns)
 // (FP32: the FP32 state, geometry shadow and operator tables, as the FP32 Euler residual), and the mixed
 // form (PERT: FP64 state in, FP32 contractions on u - c_e, FP64 pointwise fluxes, as the Euler mixed kernel).
 //
@@ -33,11 +33,18 @@ namespace {
 
 constexpr int kNsNV = 4;
 
-template <int N>
+// elements per workgroup of up to 128 lanes (dual numbers at p <= 2: up to 64 lanes, for the LDS)
+template <typename S, int N>
 constexpr int ns_epb()
 {
-	return (128 / (N * N)) > 0 ? 128 / (N * N) : 1;
+	constexpr int lanes = (sizeof(S) > 8 && N <= 3) ? 64 : 128;
+	return (lanes / (N * N)) > 0 ? lanes / (N * N) : 1;
 }
+
+// what the kernels compute: the residual, J v (dual numbers seeded with v; with mass_scale, J v - s_e M_e v), or
+// one column of every element's diagonal block dR_e/du_e (dual numbers seeded with the local dof col, the
+// neighbours' traces and gradient traces held fixed)
+constexpr int kNsRhs = 0, kNsJvp = 1, kNsDiag = 2;
 
 // node of local face lf at face position a (faces r-, r+, s-, s+)
 template <int N>
@@ -49,18 +56,22 @@ __device__ inline int ns_face_node(int lf, int a)
 
 template <typename S>
 struct NsArgs {
-	const S* u;
+	using G = typename RealOf<S>::type;   // precision of the state, geometry and operators (dual numbers: FP64)
+	const G* u;
 	const double* u64;     // PERT: FP64 state (u unused)
-	S* out;           // residual
+	const G* v;            // kNsJvp: the direction
+	G* out;           // residual; kNsJvp: J v; kNsDiag: the diagonal blocks, E x nb x nb
 	S* gvt;           // E x 4 x 2 x 4 x N: traces of G_v on the faces (own face ordering)
 	S* kap;           // E x 4 x N: kappa per face node
-	const S* met;     // E x 4 x Np
-	const S* detJ;    // E x Np
-	const S* nJ;      // F x N x 2
+	const G* met;     // E x 4 x Np
+	const G* detJ;    // E x Np
+	const G* nJ;      // F x N x 2
 	const ElemFace* ef;
-	const Ops1D<S>* ops;
+	const Ops1D<G>* ops;
 	S mu, Pr;
 	int E;
+	int col;                    // kNsDiag: the seeded local dof
+	const double* mass_scale;   // kNsJvp: s_e of the -s_e M_e v term (or NULL)
 	// variable-order meshes: per element face to an element of another order, the lifted face flux [v][a]
 	// and then the projected face products q [d][v][a] (own GL face points and ordering)
 	const S* zf;
@@ -75,7 +86,7 @@ struct NsArgs {
 // shared per-element set-up: operators, the GLL state, its GL values and (k_rhs_ns) the GL face traces
 template <typename S, int N>
 struct NsBlock {
-	static constexpr int Np = N * N, EPB = ns_epb<N>();
+	static constexpr int Np = N * N, EPB = ns_epb<S, N>();
 	S I[N * N], DhT[N * N], Lift[2][N], w[N];
 	int perm[8][N];
 	static constexpr int UR = (kNsNV * Np > 4 * kNsNV * N) ? kNsNV * Np : 4 * kNsNV * N;
@@ -87,7 +98,7 @@ struct NsBlock {
 
 // PERT: U = u - c_e from the FP64 state (the constant part has no gradient and its fluxes cancel exactly by
 // free-stream preservation), so FP32 rounding acts only on the within-element variation
-template <typename S, int N, bool PERT>
+template <typename S, int N, bool PERT, int MODE>
 __device__ inline void ns_load(NsBlock<S, N>& B, const NsArgs<S>& a, int sl, int t, int e, bool valid)
 {
 	constexpr int Np = N * N;
@@ -108,7 +119,12 @@ __device__ inline void ns_load(NsBlock<S, N>& B, const NsArgs<S>& a, int sl, int
 			for (int v = 0; v < kNsNV; ++v)
 				B.U[sl][v * Np + t] = S(a.u64[((size_t)e * kNsNV + v) * Np + t] - a.u64[((size_t)e * kNsNV + v) * Np]);
 		} else {
-			for (int v = 0; v < kNsNV; ++v) B.U[sl][v * Np + t] = a.u[((size_t)e * kNsNV + v) * Np + t];
+			for (int v = 0; v < kNsNV; ++v) {
+				const size_t gi = ((size_t)e * kNsNV + v) * Np + t;
+				if constexpr (MODE == kNsJvp) B.U[sl][v * Np + t] = S(a.u[gi], a.v[gi]);
+				else if constexpr (MODE == kNsDiag) B.U[sl][v * Np + t] = S(a.u[gi], (v * Np + t == a.col) ? 1.0 : 0.0);
+				else B.U[sl][v * Np + t] = a.u[gi];
+			}
 		}
 	}
 	__syncthreads();
@@ -149,7 +165,7 @@ __device__ inline void ns_volume_gradient(const NsBlock<S, N>& B, const NsArgs<S
 {
 	constexpr int Np = N * N;
 	const int c0 = t % N, c1 = t / N;
-	const S* m = a.met + (size_t)e * 4 * Np;
+	const typename NsArgs<S>::G* m = a.met + (size_t)e * 4 * Np;
 	const S m00 = m[0 * Np + t], m10 = m[1 * Np + t], m01 = m[2 * Np + t], m11 = m[3 * Np + t];
 	const S Ji = trcp(a.detJ[(size_t)e * Np + t]);
 #pragma unroll
@@ -165,10 +181,10 @@ __device__ inline void ns_volume_gradient(const NsBlock<S, N>& B, const NsArgs<S
 	}
 }
 
-template <typename S, int N, bool PERT>
+template <typename S, int N, bool PERT, int MODE>
 __global__ void __launch_bounds__(128) k_ns_gv(NsArgs<S> a)
 {
-	constexpr int Np = N * N, EPB = ns_epb<N>();
+	constexpr int Np = N * N, EPB = ns_epb<S, N>();
 	__shared__ NsBlock<S, N> B;
 	__shared__ S Gv[EPB][2 * kNsNV * Np];
 	__shared__ S WJi[EPB][Np];   // 1 / (w_c0 w_c1 J)
@@ -176,7 +192,7 @@ __global__ void __launch_bounds__(128) k_ns_gv(NsArgs<S> a)
 	const int e = blockIdx.x * EPB + sl;
 	const bool valid = sl < EPB && e < a.E;
 	const int slc = sl < EPB ? sl : 0;
-	ns_load<S, N, PERT>(B, a, slc, t, valid ? e : 0, valid);
+	ns_load<S, N, PERT, MODE>(B, a, slc, t, valid ? e : 0, valid);
 	if (valid) {
 		S g[2 * kNsNV];
 		ns_volume_gradient<S, N>(B, a, slc, t, e, g);
@@ -207,11 +223,11 @@ __global__ void __launch_bounds__(128) k_ns_gv(NsArgs<S> a)
 	}
 }
 
-template <typename S, int N, int FLUX, bool PERT>
+template <typename S, int N, int FLUX, bool PERT, int MODE>
 __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 {
-	constexpr int Np = N * N, EPB = ns_epb<N>();
-	constexpr S eta = S(1.01 * 4.0);
+	constexpr int Np = N * N, EPB = ns_epb<S, N>();
+	const S eta = S(1.01 * 4.0);
 	__shared__ NsBlock<S, N> B;
 	// LDS shared between phases (occupancy): the face traces, dead once the face fluxes are formed, hold W fr_d
 	// for the volume divergence; the GLL state, dead once the traces are formed, holds the face fluxes
@@ -227,7 +243,7 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 	const int e = blockIdx.x * EPB + sl;
 	const bool valid = sl < EPB && e < a.E;
 	const int slc = sl < EPB ? sl : 0;
-	ns_load<S, N, PERT>(B, a, slc, t, valid ? e : 0, valid);
+	ns_load<S, N, PERT, MODE>(B, a, slc, t, valid ? e : 0, valid);
 	const int c0 = t % N, c1 = t / N;
 	if (valid) {
 		// face traces: GLL face values (own from LDS, neighbour gathered), then I along the face
@@ -244,31 +260,40 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 #pragma unroll
 				for (int b = 0; b < N; ++b) {
 					ao += B.I[i * N + b] * B.U[slc][v * Np + ns_face_node<N>(lf, b)];
-					if constexpr (PERT)
-						an += B.I[i * N + b] * S(a.u64[((size_t)en * kNsNV + v) * Np + ns_face_node<N>(fn, b)] - B.C[slc][v]);
-					else
-						an += B.I[i * N + b] * a.u[((size_t)en * kNsNV + v) * Np + ns_face_node<N>(fn, b)];
+					const size_t gn = ((size_t)en * kNsNV + v) * Np + ns_face_node<N>(fn, b);
+					if constexpr (PERT) an += B.I[i * N + b] * S(a.u64[gn] - B.C[slc][v]);
+					else if constexpr (MODE == kNsJvp) an += B.I[i * N + b] * S(a.u[gn], a.v[gn]);
+					else an += B.I[i * N + b] * S(a.u[gn]);   // kNsDiag: the neighbour held fixed
 				}
 				TRo[slc][(lf * kNsNV + v) * N + i] = ao;
 				nbt[(lf * kNsNV + v) * N + i] = an;
 				aos[v] = ao;
 			}
 			if (f.en < 0 && bc != kBcNonconforming) {
-				// boundary face: the boundary state from the own trace (PERT: from c_e + delta, back to a delta)
-				double sl[kNsNV], sb[kNsNV], nd[2], xy[2] = { 0.0, 0.0 };
-#pragma unroll
-				for (int v = 0; v < kNsNV; ++v) sl[v] = PERT ? B.C[slc][v] + (double)aos[v] : (double)aos[v];
-				const S* nJ = a.nJ + ((size_t)f.f * N + i) * 2;
+				// boundary face: the boundary state from the own trace (PERT: in FP64 from c_e + delta, back to a
+				// delta; dual numbers: with its derivative)
+				const typename NsArgs<S>::G* nJ = a.nJ + ((size_t)f.f * N + i) * 2;
 				const double Jd = sqrt((double)nJ[0] * nJ[0] + (double)nJ[1] * nJ[1]);
-				nd[0] = (double)nJ[0] / Jd;
-				nd[1] = (double)nJ[1] / Jd;
+				double xy[2] = { 0.0, 0.0 };
 				if (bc == DGX_BC_NOSLIP_ROTATING) {
 					xy[0] = a.wall_xy[((size_t)f.f * N + i) * 2];
 					xy[1] = a.wall_xy[((size_t)f.f * N + i) * 2 + 1];
 				}
-				ns_wall_state<double>(bc, sl, nd, a.bc_prim, a.walls, xy, sb);
+				if constexpr (PERT) {
+					double sl[kNsNV], sb[kNsNV];
+					const double nd[2] = { (double)nJ[0] / Jd, (double)nJ[1] / Jd };
 #pragma unroll
-				for (int v = 0; v < kNsNV; ++v) nbt[(lf * kNsNV + v) * N + i] = PERT ? S(sb[v] - B.C[slc][v]) : S(sb[v]);
+					for (int v = 0; v < kNsNV; ++v) sl[v] = B.C[slc][v] + (double)aos[v];
+					ns_wall_state<double>(bc, sl, nd, a.bc_prim, a.walls, xy, sb);
+#pragma unroll
+					for (int v = 0; v < kNsNV; ++v) nbt[(lf * kNsNV + v) * N + i] = S(sb[v] - B.C[slc][v]);
+				} else {
+					S sb[kNsNV];
+					const S ns2[2] = { S((double)nJ[0] / Jd), S((double)nJ[1] / Jd) };
+					ns_wall_state<S>(bc, aos, ns2, a.bc_prim, a.walls, xy, sb);
+#pragma unroll
+					for (int v = 0; v < kNsNV; ++v) nbt[(lf * kNsNV + v) * N + i] = sb[v];
+				}
 			}
 		}
 	}
@@ -288,7 +313,7 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 			}
 			const S* own = TRo[slc] + lf * kNsNV * N;
 			const S* nb = (TX[slc] + 4 * kNsNV * N) + lf * kNsNV * N;
-			const S* nJ = a.nJ + ((size_t)f.f * N + i) * 2;
+			const typename NsArgs<S>::G* nJ = a.nJ + ((size_t)f.f * N + i) * 2;
 			const int dst = side == 0 ? i : pi;
 #pragma unroll
 			for (int v = 0; v < kNsNV; ++v) {
@@ -323,7 +348,7 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 			const S* tvL = a.gvt + ((size_t)eLf * 4 + lfL) * 2 * kNsNV * N;
 			const S* tvR = a.gvt + ((size_t)eRf * 4 + lfR) * 2 * kNsNV * N;
 			const S kL = a.kap[((size_t)eLf * 4 + lfL) * N + iL], kR = a.kap[((size_t)eRf * 4 + lfR) * N + iR];
-			const S* nJ = a.nJ + ((size_t)fc.f * N + i) * 2;
+			const typename NsArgs<S>::G* nJ = a.nJ + ((size_t)fc.f * N + i) * 2;
 			S sL[kNsNV], sR[kNsNV], gL[2 * kNsNV], gR[2 * kNsNV];
 			const bool wall = fc.en < 0;   // boundary face: the boundary gradient is the interior one
 			const int bcf = (fc.packed >> 12) & 15;
@@ -336,8 +361,14 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 #pragma unroll
 				for (int d = 0; d < 2; ++d) {
 					const S q = nJ[d] * dl;
-					gL[v + kNsNV * d] = tvL[(d * kNsNV + v) * N + iL] + eta * kL * q;
-					gR[v + kNsNV * d] = wall ? gL[v + kNsNV * d] : tvR[(d * kNsNV + v) * N + iR] + eta * kR * q;
+					S tL = tvL[(d * kNsNV + v) * N + iL], tR = tvR[(d * kNsNV + v) * N + iR];
+					if constexpr (MODE == kNsDiag) {
+						// the neighbour's gradient trace held fixed (its derivative is for its own seeded dof)
+						if (side == 0) tR = strip(tR);
+						else tL = strip(tL);
+					}
+					gL[v + kNsNV * d] = tL + eta * kL * q;
+					gR[v + kNsNV * d] = wall ? gL[v + kNsNV * d] : tR + eta * kR * q;
 				}
 			}
 			const S Jf = tsqrt(nJ[0] * nJ[0] + nJ[1] * nJ[1]);
@@ -436,7 +467,7 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 #pragma unroll
 			for (int k = 0; k < 2 * kNsNV; ++k) f[k] += fv[k];
 		}
-		const S* m = a.met + (size_t)e * 4 * Np;
+		const typename NsArgs<S>::G* m = a.met + (size_t)e * 4 * Np;
 #pragma unroll
 		for (int d0 = 0; d0 < 2; ++d0)
 #pragma unroll
@@ -462,6 +493,11 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 			for (int lf = 0; lf < 4; ++lf) {
 				const int fd = lf >> 1, cn = fd == 0 ? c0 : c1, at = fd == 0 ? c1 : c0;
 				acc -= B.Lift[lf & 1][cn] * B.w[at] * RES[slc][(lf * kNsNV + v) * N + at];
+			}
+			if constexpr (MODE == kNsJvp) {
+				// - s_e M_e v: M = Ibar^T W J Ibar, so - s_e W J (Ibar v) here, before Ibar^T (V holds Ibar u + i Ibar v)
+				if (a.mass_scale)
+					acc.d -= a.mass_scale[e] * (rv(B.w[c0]) * rv(B.w[c1]) * a.detJ[(size_t)e * Np + t]) * B.V[slc][v * Np + t].d;
 			}
 			z[v] = acc;
 		}
@@ -491,34 +527,40 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 			S acc = S(0.0);
 #pragma unroll
 			for (int mm = 0; mm < N; ++mm) acc += B.I[mm * N + c0] * B.V[slc][v * Np + mm + N * c1];
-			a.out[((size_t)e * kNsNV + v) * Np + t] = acc;
+			if constexpr (MODE == kNsJvp) a.out[((size_t)e * kNsNV + v) * Np + t] = acc.d;
+			else if constexpr (MODE == kNsDiag)
+				a.out[(size_t)e * (kNsNV * Np) * (kNsNV * Np) + (size_t)(v * Np + t) * (kNsNV * Np) + a.col] = acc.d;
+			else a.out[((size_t)e * kNsNV + v) * Np + t] = acc;
 		}
 }
 
 // phase 0: both kernels; 1: the gradient traces only; 2: the residual only (variable-order meshes run the
 // face kernel between orders in between)
-template <typename S, int N, bool PERT>
+template <typename S, int N, bool PERT, int MODE>
 hipError_t run_ns(Ctx& c, const NsArgs<S>& a, int phase)
 {
-	constexpr int EPB = ns_epb<N>(), Np = N * N;
+	constexpr int EPB = ns_epb<S, N>(), Np = N * N;
 	const int grid = (c.E + EPB - 1) / EPB;
 	if (phase != 2) {
-		hipLaunchKernelGGL((k_ns_gv<S, N, PERT>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
+		hipLaunchKernelGGL((k_ns_gv<S, N, PERT, MODE>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
 		DGX_HIP_CHECK(hipGetLastError());
 	}
 	if (phase == 1) return hipSuccess;
 	if (c.cfg.num_flux == DGX_FLUX_LF)
-		hipLaunchKernelGGL((k_rhs_ns<S, N, DGX_FLUX_LF, PERT>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
+		hipLaunchKernelGGL((k_rhs_ns<S, N, DGX_FLUX_LF, PERT, MODE>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
 	else
-		hipLaunchKernelGGL((k_rhs_ns<S, N, DGX_FLUX_ROE, PERT>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
+		hipLaunchKernelGGL((k_rhs_ns<S, N, DGX_FLUX_ROE, PERT, MODE>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
 	return hipGetLastError();
 }
 
 }  // namespace
 
-template <typename S, bool PERT = false>
-static hipError_t launch_ns(Ctx& c, const S* u, S* r, const S* met, const S* detJ, const S* nJ, const Ops1D<S>* ops,
-                            const double* u64 = nullptr, int phase = 0)
+template <typename S, bool PERT = false, int MODE = kNsRhs>
+static hipError_t launch_ns(Ctx& c, const typename NsArgs<S>::G* u, typename NsArgs<S>::G* r,
+                            const typename NsArgs<S>::G* met, const typename NsArgs<S>::G* detJ,
+                            const typename NsArgs<S>::G* nJ, const Ops1D<typename NsArgs<S>::G>* ops,
+                            const double* u64 = nullptr, int phase = 0, const double* v = nullptr, int col = 0,
+                            const double* mass_scale = nullptr)
 {
 	if (c.d != 2 || c.cfg.pde != DGX_PDE_NAVIER_STOKES || !c.ns_gvt || !c.ns_kappa) return hipErrorInvalidValue;
 	if (c.E == 0) return hipSuccess;
@@ -526,8 +568,20 @@ static hipError_t launch_ns(Ctx& c, const S* u, S* r, const S* met, const S* det
 	a.u = u;
 	a.u64 = u64;
 	a.out = r;
-	a.gvt = reinterpret_cast<S*>(c.ns_gvt);     // FP32 uses the first half of the FP64-sized buffers
-	a.kap = reinterpret_cast<S*>(c.ns_kappa);
+	if constexpr (sizeof(S) > sizeof(double)) {
+		// dual numbers: their own gradient-trace buffers, allocated on first use
+		const size_t ng = (size_t)c.E * 4 * 2 * c.nv * c.Nfc, nk = (size_t)c.E * 4 * c.Nfc;
+		if (!c.ns_gvt_dual) DGX_HIP_CHECK(hipMalloc(&c.ns_gvt_dual, sizeof(S) * ng));
+		if (!c.ns_kappa_dual) DGX_HIP_CHECK(hipMalloc(&c.ns_kappa_dual, sizeof(S) * nk));
+		a.gvt = reinterpret_cast<S*>(c.ns_gvt_dual);
+		a.kap = reinterpret_cast<S*>(c.ns_kappa_dual);
+		a.v = v;
+		a.col = col;
+		a.mass_scale = mass_scale;
+	} else {
+		a.gvt = reinterpret_cast<S*>(c.ns_gvt);     // FP32 uses the first half of the FP64-sized buffers
+		a.kap = reinterpret_cast<S*>(c.ns_kappa);
+	}
 	a.met = met;
 	a.detJ = detJ;
 	a.nJ = nJ;
@@ -536,18 +590,19 @@ static hipError_t launch_ns(Ctx& c, const S* u, S* r, const S* met, const S* det
 	a.mu = S(c.cfg.mu);
 	a.Pr = S(c.cfg.Pr);
 	a.E = c.E;
-	if constexpr (sizeof(S) == sizeof(double)) a.zf = c.hp_z64;
-	else a.zf = c.hp_z32;
+	if constexpr (sizeof(S) == sizeof(float)) a.zf = c.hp_z32;
+	else if constexpr (sizeof(S) == sizeof(double)) a.zf = c.hp_z64;
+	else a.zf = nullptr;   // dual numbers: no faces between orders
 	for (int k = 0; k < 5; ++k) a.bc_prim[k] = c.cfg.bc_prim[k];
 	a.walls = NsWalls{ c.cfg.wall_nf_E, c.cfg.wall_omega, c.cfg.wall_rho, c.cfg.wall_E };
 	a.wall_xy = c.wall_xy;
-	if (c.has_nonconforming && !a.zf) return hipErrorInvalidValue;
+	if (c.has_nonconforming && (!a.zf || MODE != kNsRhs)) return hipErrorInvalidValue;   // no linearisation there
 	switch (c.N) {
-	case 2: return run_ns<S, 2, PERT>(c, a, phase);
-	case 3: return run_ns<S, 3, PERT>(c, a, phase);
-	case 4: return run_ns<S, 4, PERT>(c, a, phase);
-	case 5: return run_ns<S, 5, PERT>(c, a, phase);
-	case 6: return run_ns<S, 6, PERT>(c, a, phase);
+	case 2: return run_ns<S, 2, PERT, MODE>(c, a, phase);
+	case 3: return run_ns<S, 3, PERT, MODE>(c, a, phase);
+	case 4: return run_ns<S, 4, PERT, MODE>(c, a, phase);
+	case 5: return run_ns<S, 5, PERT, MODE>(c, a, phase);
+	case 6: return run_ns<S, 6, PERT, MODE>(c, a, phase);
 	default: return hipErrorInvalidValue;
 	}
 }
@@ -580,6 +635,20 @@ hipError_t launch_rhs_ns_mixed_phase(Ctx& c, const double* u, float* r, int phas
 hipError_t launch_rhs_ns_mixed(Ctx& c, const double* u, float* r)
 {
 	return launch_ns<float, true>(c, nullptr, r, c.metrics_vc32, c.detJ_vc32, c.nJ_fc32, c.ops32, u);
+}
+
+// the Navier-Stokes linearisation (the BR2 gradient differentiated with the rest, compute_rlhs_dg's 'i' branch):
+// J v (with mass_scale: J v - s_e M_e v), and one column of every diagonal block
+hipError_t launch_ns_jvp(Ctx& c, const double* u, const double* v, double* jv, const double* mass_scale)
+{
+	return launch_ns<Dual, false, kNsJvp>(c, u, jv, c.metrics_vc, c.detJ_vc, c.nJ_fc, c.ops64, nullptr, 0, v, 0,
+	                                      mass_scale);
+}
+
+hipError_t launch_ns_diag_col(Ctx& c, const double* u, double* blocks, int col)
+{
+	return launch_ns<Dual, false, kNsDiag>(c, u, blocks, c.metrics_vc, c.detJ_vc, c.nJ_fc, c.ops64, nullptr, 0,
+	                                       nullptr, col);
 }
 
 }  // namespace dgx

@@ -213,6 +213,11 @@ static int invert(int n, const double* A, double* Ainv)
 /* ------------------------------------------------------------------------------------------------ */
 /* templated half                                                                                     */
 
+/* real-only helpers of the templated Navier-Stokes part, defined below */
+static int ensure_minv(or_ctx* c);
+static void face_xyz(const or_ctx* c, int e, int lf, double* xy);
+static int ns_bc_supported(int bc);
+
 #define Type double
 #define FN(name) name##_d
 #define SQRT_T sqrt
@@ -728,51 +733,6 @@ static void face_xyz(const or_ctx* c, int e, int lf, double* xy)
 		}
 }
 
-/* Boundary (ghost) state of a Navier-Stokes boundary face node:
- *   Riemann: the Euler Riemann state (boundary_euler_T.c:99-180);
- *   no-slip adiabatic / diabatic walls (constructor_Boundary_Value_T_navier_stokes_no_slip_flux_general,
- *     boundary_navier_stokes_T.c:260-290): rho and E from inside, momentum -rho u_l + 2 rho u_ex with u_ex = 0;
- *   rotating no-slip wall (no_slip_all_general, :200-240): s_b = 2 s_ex - s_l, s_ex = (rho_b, rho_b u_ex, E_b),
- *     u_ex = (-sin th, cos th) omega r at the node (compute_uvw_ex_rotating, :505-520). */
-static void ns_bc_state(const or_ctx* c, int bc, const double* sL, const double* n, const double* xy, double* sB)
-{
-	const int d = c->d;
-	if (bc == OR_BC_RIEMANN) {
-		bc_state_d(d, bc, sL, n, c->bc_prim, sB);
-		return;
-	}
-	double uvw[3] = { 0.0, 0.0, 0.0 };
-	if (bc == OR_BC_NOSLIP_ROTATING) {
-		const double r = sqrt(xy[0] * xy[0] + xy[1] * xy[1]), th = atan2(xy[1], xy[0]), Vt = c->wall_omega * r;
-		uvw[0] = -sin(th) * Vt;
-		uvw[1] = cos(th) * Vt;
-		sB[0] = -sL[0] + 2.0 * c->wall_rho;
-		for (int k = 0; k < d; ++k) sB[1 + k] = -sL[1 + k] + 2.0 * c->wall_rho * uvw[k];
-		sB[d + 1] = -sL[d + 1] + 2.0 * c->wall_E;
-		return;
-	}
-	sB[0] = sL[0];
-	sB[d + 1] = sL[d + 1];
-	for (int k = 0; k < d; ++k) sB[1 + k] = -sL[1 + k] + 2.0 * sB[0] * uvw[k];
-}
-
-/* the boundary states of face f at its cubature nodes, in the left (own) ordering */
-static void ns_face_bc_states(const or_ctx* c, int f, const double* sL, double* sB)
-{
-	const int d = c->d, Nfc = c->Nfc, nv = c->nv;
-	const int* fc = c->faces + 6 * f;
-	double xy[3 * 16];
-	face_xyz(c, fc[0], fc[1], xy);
-	const double* nJ = c->nJ_fc + (size_t)f * Nfc * d;
-	for (int n = 0; n < Nfc; ++n) {
-		double vn[3] = { 0.0, 0.0, 0.0 }, J2 = 0.0;
-		for (int k = 0; k < d; ++k) J2 += nJ[n * d + k] * nJ[n * d + k];
-		const double Jf = sqrt(J2);
-		for (int k = 0; k < d; ++k) vn[k] = nJ[n * d + k] / Jf;
-		ns_bc_state(c, fc[5], &sL[n * nv], vn, &xy[n * d], &sB[n * nv]);
-	}
-}
-
 static int ensure_minv(or_ctx* c)
 {
 	const int Np = c->Np;
@@ -787,271 +747,33 @@ static int ensure_minv(or_ctx* c)
 	return 0;
 }
 
-/* Negated viscous flux fv[dim + d*eq] (added to the Euler flux) of state s and gradient g[v + nv*dim]:
- * compute_Flux_navier_stokes_0 (flux_navier_stokes_T.c:1005-1040) with tau (compute_tau :778-799),
- * velocity gradients du_k/dx_j = (d(rho u_k)/dx_j - u_k drho/dx_j)/rho and the scaled temperature gradient
- * dTs_j = d(E/rho)/dx_j - 1/2 d(V^2)/dx_j (compute_dTs :904-926); constant viscosity mu. */
 void or_viscous_flux(int d, const double* s, const double* g, double mu, double Pr, double* fv)
 {
-	const int nv = d + 2;
-	const double rho = s[0], E = s[d + 1], ri = 1.0 / rho;
-	double uvw[3] = { 0.0, 0.0, 0.0 }, du[3][3] = { { 0.0 } }, tau[3][3] = { { 0.0 } }, dTs[3] = { 0.0 };
-	for (int k = 0; k < d; ++k) uvw[k] = s[1 + k] * ri;
-	for (int k = 0; k < d; ++k)
-		for (int j = 0; j < d; ++j) du[k][j] = ri * (g[(1 + k) + nv * j] - uvw[k] * g[0 + nv * j]);
-	double divV = 0.0;
-	for (int k = 0; k < d; ++k) divV += du[k][k];
-	for (int i = 0; i < d; ++i)
-		for (int j = 0; j < d; ++j) tau[i][j] = mu * (du[i][j] + du[j][i]) - (i == j ? 2.0 / 3.0 * mu * divV : 0.0);
-	for (int j = 0; j < d; ++j) {
-		double dV2 = 0.0;
-		for (int k = 0; k < d; ++k) dV2 += 2.0 * uvw[k] * du[k][j];
-		dTs[j] = ri * ri * (g[(d + 1) + nv * j] * rho - E * g[0 + nv * j]) - 0.5 * dV2;
-	}
-	for (int j = 0; j < d; ++j) {
-		fv[j + d * 0] = 0.0;
-		for (int i = 0; i < d; ++i) fv[j + d * (1 + i)] = -tau[i][j];
-		double ut = 0.0;
-		for (int k = 0; k < d; ++k) ut += uvw[k] * tau[j][k];
-		fv[j + d * (d + 1)] = -(OR_GAMMA / Pr * mu * dTs[j] + ut);
-	}
+	viscous_flux_d(d, s, g, mu, Pr, fv);
 }
 
-/* state at the face cubature nodes of local face lf of element e: out[n * nv + v] */
-static void trace_fc(const or_ctx* c, int e, int lf, const double* coef, double* out)
-{
-	const int Np = c->Np, Nfc = c->Nfc, nv = c->nv;
-	const double* op = c->cv0_vs_fc + (size_t)lf * Nfc * Np;
-	for (int v = 0; v < nv; ++v)
-		for (int n = 0; n < Nfc; ++n) {
-			double acc = 0.0;
-			for (int m = 0; m < Np; ++m) acc += op[n * Np + m] * coef[(size_t)v * Np + m];
-			out[n * nv + v] = acc;
-		}
-}
-
-/* Weak gradient coefficients (compute_grad_coef_dg_T.c:171-307, 'e' branch).
- *   gv[e][dim]    = M^-1 tw0_vt_vc grad_xyz[dim] s,  grad_xyz[dim] = sum_k diag(J dxi_k/dx_dim) cv1_vs_vc[k]
- *   gf[f][side][dim] = M^-1 tw0_vt_fc (J_f n_dim (s_num - s_side)), s_num central: s_num - s_L = (s_R - s_L)/2,
- *                  the right side using the same product permuted into its face ordering (:240-256)
- *   g[e]          = gv[e] + sum over the faces of e of gf (add_face_grad_coef_f_to_volumes :505-527).
- * Layouts: gv, g: E x d x nv x Np; gf: F x 2 x d x nv x Np (boundary faces: side 0 only).  Returns -1 for a
- * boundary code other than Riemann and the no-slip walls. */
 int or_ns_gradient(or_ctx* c, const double* u, double* g, double* gv, double* gf)
 {
-	const int d = c->d, Np = c->Np, Nc = c->Nc, Nfc = c->Nfc, nv = c->nv;
-	if (ensure_minv(c)) return -1;
-	for (int f = 0; f < c->F; ++f)
-		if (c->faces[6 * f + 5] != OR_BC_INTERIOR && !ns_bc_supported(c->faces[6 * f + 5])) return -1;
-	const size_t eb = (size_t)d * nv * Np;
-	double* gx = (double*)malloc(sizeof(double) * Nc * Np);
-	double* t1 = (double*)malloc(sizeof(double) * Nc);
-	double* t2 = (double*)malloc(sizeof(double) * Np);
-	for (int e = 0; e < c->E; ++e) {
-		const double* met = c->metrics_vc + (size_t)e * d * d * Nc;
-		const double* Mi = c->minv + (size_t)e * Np * Np;
-		const double* ue = u + (size_t)e * nv * Np;
-		for (int dim = 0; dim < d; ++dim) {
-			for (int n = 0; n < Nc; ++n)
-				for (int m = 0; m < Np; ++m) {
-					double acc = 0.0;
-					for (int k = 0; k < d; ++k) acc += met[(dim + d * k) * Nc + n] * c->cv1_vs_vc[(size_t)k * Nc * Np + n * Np + m];
-					gx[n * Np + m] = acc;
-				}
-			for (int v = 0; v < nv; ++v) {
-				for (int n = 0; n < Nc; ++n) {
-					double acc = 0.0;
-					for (int m = 0; m < Np; ++m) acc += gx[n * Np + m] * ue[v * Np + m];
-					t1[n] = acc * c->w_vc[n];
-				}
-				for (int i = 0; i < Np; ++i) {
-					double acc = 0.0;
-					for (int n = 0; n < Nc; ++n) acc += c->cv0_vs_vc[n * Np + i] * t1[n];
-					t2[i] = acc;
-				}
-				double* out = gv + (size_t)e * eb + ((size_t)dim * nv + v) * Np;
-				for (int i = 0; i < Np; ++i) {
-					double acc = 0.0;
-					for (int j = 0; j < Np; ++j) acc += Mi[i * Np + j] * t2[j];
-					out[i] = acc;
-				}
-			}
-		}
-	}
-	memcpy(g, gv, sizeof(double) * (size_t)c->E * eb);
-	double* sL = (double*)malloc(sizeof(double) * Nfc * nv);
-	double* sRr = (double*)malloc(sizeof(double) * Nfc * nv);
-	double* q = (double*)malloc(sizeof(double) * Nfc * nv);
-	double* qs = (double*)malloc(sizeof(double) * Nfc * nv);
-	int ident[16];
-	for (int n = 0; n < Nfc; ++n) ident[n] = n;
-	for (int f = 0; f < c->F; ++f) {
-		const int* fc = c->faces + 6 * f;
-		const int eL = fc[0], fL = fc[1], eR = fc[2], fR = fc[3], ord = fc[4];
-		const int inter = fc[5] == OR_BC_INTERIOR;
-		trace_fc(c, eL, fL, u + (size_t)eL * nv * Np, sL);
-		/* boundary faces: s_num = (s_L + s_b) / 2 with the boundary state (compute_g_coef_related_boundary,
-		 * compute_grad_coef_dg_T.c:527-575), lifted into the left element only */
-		if (inter) trace_fc(c, eR, fR, u + (size_t)eR * nv * Np, sRr);
-		else ns_face_bc_states(c, f, sL, sRr);
-		const int* perm = inter ? c->perm + (size_t)ord * Nfc : ident;
-		const double* nJ = c->nJ_fc + (size_t)f * Nfc * d;
-		for (int side = 0; side < (inter ? 2 : 1); ++side) {
-			const int e = side ? eR : eL, lf = side ? fR : fL;
-			const double* Mi = c->minv + (size_t)e * Np * Np;
-			const double* tw0 = c->tw0_vt_fc + (size_t)lf * Np * Nfc;
-			for (int dim = 0; dim < d; ++dim) {
-				/* q = J_f n_dim (s_R - s_L) / 2 in the left ordering */
-				for (int n = 0; n < Nfc; ++n)
-					for (int v = 0; v < nv; ++v)
-						q[n * nv + v] = nJ[n * d + dim] * 0.5 * (sRr[perm[n] * nv + v] - sL[n * nv + v]);
-				if (side)   /* into the right face ordering */
-					for (int n = 0; n < Nfc; ++n)
-						for (int v = 0; v < nv; ++v) qs[perm[n] * nv + v] = q[n * nv + v];
-				else
-					memcpy(qs, q, sizeof(double) * Nfc * nv);
-				for (int v = 0; v < nv; ++v) {
-					for (int i = 0; i < Np; ++i) {
-						double acc = 0.0;
-						for (int n = 0; n < Nfc; ++n) acc += tw0[i * Nfc + n] * qs[n * nv + v];
-						t2[i] = acc;
-					}
-					double* out = gf + (((size_t)f * 2 + side) * d + dim) * nv * Np + (size_t)v * Np;
-					double* ge = g + (size_t)e * eb + ((size_t)dim * nv + v) * Np;
-					for (int i = 0; i < Np; ++i) {
-						double acc = 0.0;
-						for (int j = 0; j < Np; ++j) acc += Mi[i * Np + j] * t2[j];
-						out[i] = acc;
-						ge[i] += acc;
-					}
-				}
-			}
-		}
-	}
-	free(gx); free(t1); free(t2); free(sL); free(sRr); free(q); free(qs);
-	return 0;
+	return ns_gradient_d(c, u, g, gv, gf);
 }
 
-/* Navier-Stokes residual (compute_rlhs_dg order: gradient, volume, face; solve_dg.c:182-189):
- *   volume: f = F_euler(s) + fv(s, g) at the volume cubature, g interpolated from the full gradient coefficients;
- *   face:   nnf = nnf_euler(s_L, s_R, n) + n . (fv(s_L, g_L) + fv(s_R, g_R)) / 2 (central, numerical_flux_central_T.c
- *           :76-104), g_side = cv0_vr_fc (gv + 1.01 NFMAX gf[side]) (constructor_partial_grad_fc_interp,
- *           compute_face_rlhs_dg_T.c:243-300), NFMAX = 2d faces; then scaled by J_f and lifted as the Euler faces.
- * Boundary faces (Riemann and the no-slip walls, ns_bc_state): s_R = the boundary state, g_R = g_L; returns -1 for
- * other boundary codes (the reference has no slip-wall condition for Navier-Stokes, face_solver_T.c:275-298). */
 int or_compute_rhs_ns(or_ctx* c, const double* u, double* rhs)
 {
-	const int d = c->d, Np = c->Np, Nc = c->Nc, Nfc = c->Nfc, nv = c->nv;
-	if (!c->viscous || c->pde != OR_PDE_EULER) return -1;
-	const size_t eb = (size_t)d * nv * Np;
-	double* g = (double*)malloc(sizeof(double) * (size_t)c->E * eb);
-	double* gv = (double*)malloc(sizeof(double) * (size_t)c->E * eb);
-	double* gf = (double*)malloc(sizeof(double) * (size_t)c->F * 2 * eb);
-	if (or_ns_gradient(c, u, g, gv, gf)) { free(g); free(gv); free(gf); return -1; }
-	memset(rhs, 0, sizeof(double) * (size_t)c->E * Np * nv);
-	const double eta = 1.01 * 2 * d;
-	double* s_vc = (double*)malloc(sizeof(double) * Nc * nv);
-	double* g_vc = (double*)malloc(sizeof(double) * Nc * nv * d);
-	double* fr = (double*)malloc(sizeof(double) * Nc * nv * d);
-	for (int e = 0; e < c->E; ++e) {
-		const double* ue = u + (size_t)e * Np * nv;
-		double* re = rhs + (size_t)e * Np * nv;
-		const double* met = c->metrics_vc + (size_t)e * d * d * Nc;
-		for (int v = 0; v < nv; ++v)
-			for (int n = 0; n < Nc; ++n) {
-				double acc = 0.0;
-				for (int m = 0; m < Np; ++m) acc += c->cv0_vs_vc[n * Np + m] * ue[v * Np + m];
-				s_vc[n * nv + v] = acc;
-				for (int dim = 0; dim < d; ++dim) {
-					const double* gc = g + (size_t)e * eb + ((size_t)dim * nv + v) * Np;
-					double a2 = 0.0;
-					for (int m = 0; m < Np; ++m) a2 += c->cv0_vs_vc[n * Np + m] * gc[m];
-					g_vc[n * nv * d + v + nv * dim] = a2;
-				}
-			}
-		for (int n = 0; n < Nc; ++n) {
-			double f[15], fv[15];
-			euler_flux_d(d, &s_vc[n * nv], f);
-			or_viscous_flux(d, &s_vc[n * nv], &g_vc[n * nv * d], c->mu, c->Pr, fv);
-			for (int k = 0; k < d * nv; ++k) f[k] += fv[k];
-			for (int eq = 0; eq < nv; ++eq)
-				for (int d0 = 0; d0 < d; ++d0) {
-					double acc = 0.0;
-					for (int d1 = 0; d1 < d; ++d1) acc += met[(d1 + d * d0) * Nc + n] * f[d1 + d * eq];
-					fr[(eq * d + d0) * Nc + n] = acc;
-				}
-		}
-		for (int eq = 0; eq < nv; ++eq)
-			for (int dim = 0; dim < d; ++dim) {
-				const double* tw1 = c->tw1_vt_vc + (size_t)dim * Np * Nc;
-				const double* frd = &fr[(eq * d + dim) * Nc];
-				for (int m = 0; m < Np; ++m) {
-					double acc = 0.0;
-					for (int n = 0; n < Nc; ++n) acc += tw1[m * Nc + n] * frd[n];
-					re[eq * Np + m] += acc;
-				}
-			}
-	}
-	double* sL = (double*)malloc(sizeof(double) * Nfc * nv);
-	double* sRr = (double*)malloc(sizeof(double) * Nfc * nv);
-	double* gLc = (double*)malloc(sizeof(double) * eb);
-	double* gRc = (double*)malloc(sizeof(double) * eb);
-	double* gL = (double*)malloc(sizeof(double) * Nfc * nv * d);
-	double* gRr = (double*)malloc(sizeof(double) * Nfc * nv * d);
-	double* tmp = (double*)malloc(sizeof(double) * Nfc * nv);
-	double* nnf = (double*)malloc(sizeof(double) * Nfc * nv);
-	int ident[16];
-	for (int n = 0; n < Nfc; ++n) ident[n] = n;
-	for (int f = 0; f < c->F; ++f) {
-		const int* fc = c->faces + 6 * f;
-		const int eL = fc[0], fL = fc[1], eR = fc[2], fR = fc[3], ord = fc[4], bc = fc[5];
-		const int inter = bc == OR_BC_INTERIOR;
-		const int* perm = inter ? c->perm + (size_t)ord * Nfc : ident;
-		trace_fc(c, eL, fL, u + (size_t)eL * nv * Np, sL);
-		if (inter) trace_fc(c, eR, fR, u + (size_t)eR * nv * Np, sRr);
-		else ns_face_bc_states(c, f, sL, sRr);
-		for (size_t i = 0; i < eb; ++i) {
-			gLc[i] = gv[(size_t)eL * eb + i] + eta * gf[((size_t)f * 2 + 0) * eb + i];
-			if (inter) gRc[i] = gv[(size_t)eR * eb + i] + eta * gf[((size_t)f * 2 + 1) * eb + i];
-		}
-		for (int dim = 0; dim < d; ++dim) {
-			trace_fc(c, eL, fL, gLc + (size_t)dim * nv * Np, tmp);
-			for (int n = 0; n < Nfc; ++n)
-				for (int v = 0; v < nv; ++v) gL[n * nv * d + v + nv * dim] = tmp[n * nv + v];
-			if (inter) trace_fc(c, eR, fR, gRc + (size_t)dim * nv * Np, tmp);
-			/* boundary faces: the boundary gradient is the interior partial gradient
-			 * (constructor_Boundary_Value_T_grad_from_internal, boundary_T.c:103-132; the no-slip flux walls
-			 * copy it too, their energy entry being replaced by the provided flux below) */
-			for (int n = 0; n < Nfc; ++n)
-				for (int v = 0; v < nv; ++v) gRr[n * nv * d + v + nv * dim] = tmp[n * nv + v];
-		}
-		const double* nJ = c->nJ_fc + (size_t)f * Nfc * d;
-		for (int n = 0; n < Nfc; ++n) {
-			double vn[3] = { 0.0, 0.0, 0.0 }, J2 = 0.0, sR[5], fvL[15], fvR[15];
-			for (int k = 0; k < d; ++k) J2 += nJ[n * d + k] * nJ[n * d + k];
-			const double Jf = sqrt(J2);
-			for (int k = 0; k < d; ++k) vn[k] = nJ[n * d + k] / Jf;
-			for (int v = 0; v < nv; ++v) sR[v] = sRr[perm[n] * nv + v];
-			numflux_d(d, c->pde, c->flux, &sL[n * nv], sR, vn, c->b_adv, &nnf[n * nv]);
-			or_viscous_flux(d, &sL[n * nv], &gL[n * nv * d], c->mu, c->Pr, fvL);
-			or_viscous_flux(d, sR, &gRr[perm[n] * nv * d], c->mu, c->Pr, fvR);
-			for (int eq = 0; eq < nv; ++eq) {
-				double acc = 0.0;
-				for (int k = 0; k < d; ++k) acc += vn[k] * 0.5 * (fvL[k + d * eq] + fvR[k + d * eq]);
-				nnf[n * nv + eq] = nnf[n * nv + eq] + acc;
-			}
-			/* no-slip flux walls: the energy equation's normal flux is the provided nf_E, replacing the inviscid
-			 * and viscous parts (set_Numerical_Flux_Energy_member after compute_Numerical_Flux_12,
-			 * numerical_flux_central_T.c:222-230, 250-262; numerical_flux_T.c:150-156) */
-			if (bc == OR_BC_NOSLIP_ADIABATIC || bc == OR_BC_NOSLIP_DIABATIC) nnf[n * nv + nv - 1] = c->wall_nfE;
-			for (int eq = 0; eq < nv; ++eq) nnf[n * nv + eq] *= Jf;
-		}
-		face_side_d(c, f, 0, nnf, rhs);
-		if (inter) face_side_d(c, f, 1, nnf, rhs);
-	}
-	free(s_vc); free(g_vc); free(fr); free(sL); free(sRr); free(gLc); free(gRc); free(gL); free(gRr); free(tmp); free(nnf);
-	free(g); free(gv); free(gf);
-	return 0;
+	return compute_rhs_ns_d(c, u, rhs);
+}
+
+/* the complex twin (complex-step derivatives of the BR2 residual: J v = Im R(u + i h v) / h) */
+int or_compute_rhs_ns_c(or_ctx* c, const double* ure, const double* uim, double* rre, double* rim)
+{
+	const size_t n = (size_t)c->E * c->Np * c->nv;
+	double complex* u = (double complex*)calloc(n ? n : 1, sizeof(double complex));
+	double complex* r = (double complex*)malloc(sizeof(double complex) * (n ? n : 1));
+	for (size_t i = 0; i < n; ++i) u[i] = ure[i] + I * (uim ? uim[i] : 0.0);
+	const int st = compute_rhs_ns_c(c, u, r);
+	if (!st)
+		for (size_t i = 0; i < n; ++i) { rre[i] = creal(r[i]); rim[i] = cimag(r[i]); }
+	free(u); free(r);
+	return st;
 }
 
 double or_compute_rhs_explicit(or_ctx* c, const double* u, double* rhs)
@@ -1270,6 +992,8 @@ void or_cfl_inv_dt(const or_ctx* c, const double* u, double cfl, double* inv_dt)
 				if (sqrt(s2) < hmin) hmin = sqrt(s2);
 			}
 		inv_dt[e] = m / (cfl * hmin);
+		/* Navier-Stokes: the viscous bound dt <= cfl h^2 / mu (solve_dg.c:300-318) */
+		if (c->viscous && c->mu / (cfl * hmin * hmin) > inv_dt[e]) inv_dt[e] = c->mu / (cfl * hmin * hmin);
 	}
 	free(b);
 }

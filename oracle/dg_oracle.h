@@ -84,7 +84,7 @@ void or_face_perm(const or_ctx* c, int ord, int* perm);
 
 /* residual (volume + face, no M^-1) -- compute_rlhs_common_dg */
 void or_compute_rhs_d(const or_ctx* c, const double* u, double* rhs);
-/* Navier-Stokes with BR2 (real only; interior, Riemann and no-slip wall faces).  or_set_viscous switches an Euler
+/* Navier-Stokes with BR2 (real and complex; interior, Riemann and no-slip wall faces).  or_set_viscous switches an Euler
  * context to Navier-Stokes with constant viscosity mu and Prandtl number Pr. */
 void or_set_viscous(or_ctx* c, double mu, double Pr);
 /* wall data of the no-slip conditions: nf_E the normal viscous energy flux of the adiabatic (0) and diabatic
@@ -96,6 +96,8 @@ void or_viscous_flux(int d, const double* s, const double* g, double mu, double 
 int or_ns_gradient(or_ctx* c, const double* u, double* g, double* gv, double* gf);
 /* full Navier-Stokes residual (no M^-1).  0 or -1 */
 int or_compute_rhs_ns(or_ctx* c, const double* u, double* rhs);
+/* its complex twin: R(ure + i uim) = rre + i rim (complex-step derivatives of the BR2 residual).  0 or -1 */
+int or_compute_rhs_ns_c(or_ctx* c, const double* ure, const double* uim, double* rre, double* rim);
 /* the same on nthreads OpenMP threads (faces lifted by colour); agrees with or_compute_rhs_d to rounding */
 void or_compute_rhs_d_threads(const or_ctx* c, const double* u, double* rhs, int nthreads);
 /* complex twin: u = ure + i*uim */

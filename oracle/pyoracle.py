@@ -44,6 +44,8 @@ def lib():
         L.or_ns_gradient.restype = C.c_int
         L.or_compute_rhs_ns.argtypes = [C.c_void_p, dp, dp]
         L.or_compute_rhs_ns.restype = C.c_int
+        L.or_compute_rhs_ns_c.argtypes = [C.c_void_p, dp, dp, dp, dp]
+        L.or_compute_rhs_ns_c.restype = C.c_int
         L.or_compute_rhs_c.argtypes = [C.c_void_p, dp, dp, dp, dp]
         L.or_geometry.argtypes = [C.c_void_p, dp, dp, dp]
         L.or_apply_minv.argtypes = [C.c_void_p, dp]
@@ -226,6 +228,30 @@ class Oracle:
         if lib().or_compute_rhs_ns(self.ctx, _d(np.ascontiguousarray(u, float)), _d(r)):
             raise RuntimeError("or_compute_rhs_ns: viscous context with interior, Riemann and no-slip wall faces only")
         return r
+
+    def rhs_ns_complex(self, u_re, u_im):
+        """The Navier-Stokes residual at a complex state (its imaginary part carries complex-step derivatives)."""
+        rre, rim = np.zeros(self.ndof), np.zeros(self.ndof)
+        if lib().or_compute_rhs_ns_c(self.ctx, _d(np.ascontiguousarray(u_re, float)),
+                                     _d(np.ascontiguousarray(u_im, float)), _d(rre), _d(rim)):
+            raise RuntimeError("or_compute_rhs_ns_c: viscous context with interior, Riemann and no-slip wall faces only")
+        return rre, rim
+
+    def jvp_ns(self, u, v, h=1e-30):
+        """J v of the Navier-Stokes residual by complex step: Im R(u + i h v) / h."""
+        return self.rhs_ns_complex(u, h * np.asarray(v, float))[1] / h
+
+    def ns_diag_block(self, u, e, h=1e-30):
+        """The diagonal block dR_e/du_e (nb x nb, nb = nv Np, rows and columns in the e x nv x Np order) by
+        complex step, one local dof at a time."""
+        nb = self.nv * self.Np
+        B = np.empty((nb, nb))
+        im = np.zeros(self.ndof)
+        for col in range(nb):
+            im[:] = 0.0
+            im[e * nb + col] = h
+            B[:, col] = self.rhs_ns_complex(u, im)[1][e * nb:(e + 1) * nb] / h
+        return B
 
     def ns_gradient(self, u):
         """(g, gv, gf): full and volume-only weak gradient coefficients E x d x nv x Np, face lifts

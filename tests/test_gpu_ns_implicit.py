@@ -1,0 +1,202 @@
+"""GPU parity of the Navier-Stokes (BR2) linearisation and of its implicit steady solve.
+
+* J v (dgx_jacobian_vector: dual numbers through the gradient-trace and residual kernels, the BR2 gradient
+  differentiated with the rest) equals the oracle's complex step of the BR2 residual (or_compute_rhs_ns_c) to
+  1e-12 relative: curved periodic meshes p = 1..5 (Lax-Friedrichs and Roe), the Taylor-Couette annulus with its
+  rotating, adiabatic and diabatic no-slip walls, and a Riemann box.
+* The diagonal blocks dR_e/du_e (dgx_assemble_diag_blocks: one dual-number column per launch, the neighbours'
+  states and gradient traces held fixed) equal the oracle's complex-step blocks to 1e-12, with and without the
+  -s_e M_e mass term.
+* dgx_newton_operator (J v - s_e M_e v in one launch) equals dgx_jacobian_vector followed by dgx_mass_sub.
+* The pseudo-time step takes the viscous bound dt <= cfl h^2 / mu (solve_dg.c:300-318) where it is the smaller.
+* A Newton-GMRES steady solve with block-Jacobi reaches max |R| < 1e-10 on the Taylor-Couette annulus; the
+  oracle's residual of the converged state agrees, and the steady velocity is the Couette profile.
+* The Navier-Stokes paths with no linearisation here (block SGS, the learned preconditioner, the FP32 J v, the
+  face blocks) are refused with DGX_ERR_UNSUPPORTED.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from dpgsolver_amd import mesh as M
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+MU, PR = 0.02, 0.72
+
+
+@pytest.fixture(scope="module")
+def dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda:0")
+
+
+def _ctx(mesh, flux, walls=None, bc_prim=None, mu=MU):
+    from dpgsolver_amd import _lib
+
+    ctx = _lib.Context(2, mesh.p, M.PDE_NAVIER_STOKES, flux, mu=mu, Pr=PR, walls=walls, bc_prim=bc_prim)
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    ctx.upload_mesh(mesh.xyz, mesh.faces)
+    return ctx
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _perturbed(mesh, u, amp=0.02):
+    x = M.nodes_xyz(mesh)
+    pert = amp * np.sin(3 * x[..., 0]) * np.cos(2 * x[..., 1])
+    return u * (1.0 + M.pack_state(np.stack([pert, 2 * pert, -pert, 0.5 * pert], -1)))
+
+
+def _cases():
+    curve = lambda pts: np.stack([pts[..., 0] + 0.03 * np.sin(2 * np.pi * pts[..., 1]),
+                                  pts[..., 1] + 0.03 * np.sin(2 * np.pi * pts[..., 0])], -1)
+    out = []
+    for p, n in ((1, 4), (2, 3), (3, 3), (4, 2), (5, 2)):
+        for flux in (M.FLUX_LF, M.FLUX_ROE):
+            mesh = M.structured_tp_mesh(2, p, (n, n), (0.0, 0.0), (1.0, 1.0), (True, True), curve=curve,
+                                        rotate=np.random.default_rng(p).integers(0, 4, n * n))
+            x = M.nodes_xyz(mesh)
+            X, Y = x[..., 0], x[..., 1]
+            prim = np.stack([1 + 0.1 * np.sin(2 * np.pi * X), 0.3 + 0.2 * np.sin(2 * np.pi * Y),
+                             -0.1 + 0.2 * np.cos(2 * np.pi * X), 1 + 0.1 * np.sin(2 * np.pi * (X + Y))], -1)
+            out.append((f"periodic_p{p}_{'lf' if flux == M.FLUX_LF else 'roe'}", mesh, flux, {},
+                        M.pack_state(M.prim_to_cons(prim, 2))))
+    for p, bc_outer, dT in ((2, M.BC_NOSLIP_ADIABATIC, 0.0), (3, M.BC_NOSLIP_DIABATIC, 1.0)):
+        mesh = M.annulus_mesh(p, 2, 8, bc_outer=bc_outer)
+        u, _ = M.taylor_couette_state(mesh, MU, PR, dTdr_o=dT)
+        out.append((f"annulus_p{p}_{'diabatic' if dT else 'adiabatic'}", mesh, M.FLUX_ROE,
+                    {"walls": M.taylor_couette_walls(MU, PR, dTdr_o=dT)}, _perturbed(mesh, u)))
+    prim = np.array([1.0, 0.3, 0.2, 1.0])
+    mesh = M.structured_tp_mesh(2, 2, (3, 3), (0.0, 0.0), (1.0, 0.8), (False, False),
+                                bc_low=(M.BC_RIEMANN, M.BC_RIEMANN), bc_high=(M.BC_RIEMANN, M.BC_RIEMANN))
+    out.append(("riemann_box_p2", mesh, M.FLUX_LF, {"bc_prim": prim},
+                _perturbed(mesh, M.uniform_state(mesh, prim))))
+    return out
+
+
+CASES = _cases()
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_ns_jacobian_vector_matches_complex_step(dev, case):
+    name, mesh, flux, kw, u = case
+    print(name, flush=True)
+    ctx = _ctx(mesh, flux, **kw)
+    v = np.random.default_rng(9).standard_normal(u.size) * np.abs(u)
+    ud, vd = torch.from_numpy(u).to(dev), torch.from_numpy(v).to(dev)
+    jv = torch.full_like(ud, float("nan"))
+    ctx.jacobian_vector(ud.data_ptr(), vd.data_ptr(), jv.data_ptr())
+    torch.cuda.synchronize()
+    ref = O.Oracle(mesh, M.PDE_EULER, flux, viscous=(MU, PR), **kw).jvp_ns(u, v)
+    err = _rel(jv.cpu().numpy(), ref)
+    print(f"  J v rel err {err:.1e}", flush=True)
+    assert err < 1e-12
+    # J v - s_e M_e v in one launch
+    sc = np.random.default_rng(19).uniform(0.5, 30.0, mesh.E)
+    sd = torch.from_numpy(sc).to(dev)
+    a = torch.full_like(ud, float("nan"))
+    ctx.newton_operator(ud.data_ptr(), sd.data_ptr(), vd.data_ptr(), a.data_ptr())
+    ctx.mass_sub(vd.data_ptr(), sd.data_ptr(), jv.data_ptr())
+    torch.cuda.synchronize()
+    assert _rel(a.cpu().numpy(), jv.cpu().numpy()) < 1e-13
+    ctx.close()
+
+
+BLOCK_CASES = [c for c in CASES if c[0] in ("periodic_p2_roe", "periodic_p3_lf", "annulus_p2_adiabatic",
+                                             "annulus_p3_diabatic", "riemann_box_p2")]
+
+
+@pytest.mark.parametrize("case", BLOCK_CASES, ids=[c[0] for c in BLOCK_CASES])
+def test_ns_diag_blocks_match_complex_step(dev, case):
+    name, mesh, flux, kw, u = case
+    print(name, flush=True)
+    ctx = _ctx(mesh, flux, **kw)
+    nb = ctx.Np * ctx.nv
+    ud = torch.from_numpy(u).to(dev)
+    blocks = torch.full((mesh.E * nb * nb,), float("nan"), dtype=torch.float64, device=dev)
+    ctx.assemble_diag_blocks(ud.data_ptr(), None, blocks.data_ptr())
+    torch.cuda.synchronize()
+    g = blocks.cpu().numpy().reshape(mesh.E, nb, nb)
+    orc = O.Oracle(mesh, M.PDE_EULER, flux, viscous=(MU, PR), **kw)
+    s = np.linspace(0.5, 2.0, mesh.E)
+    sd = torch.from_numpy(s).to(dev)
+    ctx.assemble_diag_blocks(ud.data_ptr(), sd.data_ptr(), blocks.data_ptr())
+    torch.cuda.synchronize()
+    g2 = blocks.cpu().numpy().reshape(mesh.E, nb, nb)
+    for e in sorted({0, mesh.E // 2, mesh.E - 1}):
+        ref = orc.ns_diag_block(u, e)
+        err = np.abs(g[e] - ref).max() / np.abs(ref).max()
+        print(f"  element {e}: block rel err {err:.1e}", flush=True)
+        assert err < 1e-12
+        Mm = np.kron(np.eye(ctx.nv), orc.mass(e))
+        assert np.abs(g2[e] - (ref - s[e] * Mm)).max() <= 1e-12 * np.abs(ref).max()
+    ctx.close()
+
+
+def test_ns_cfl_dt_takes_the_viscous_bound(dev):
+    mesh = M.structured_tp_mesh(2, 2, (4, 4), (0.0, 0.0), (1.0, 1.0), (True, True))
+    prim = np.array([1.0, 0.0, 0.0, 1.0])
+    u = M.uniform_state(mesh, prim)
+    h, cfl, c = 0.25, 3.0, np.sqrt(1.4)
+    for mu in (1e-3, 5.0):   # the acoustic bound, then the viscous one
+        ctx = _ctx(mesh, M.FLUX_ROE, mu=mu)
+        ud = torch.from_numpy(u).to(dev)
+        inv = torch.empty(mesh.E, dtype=torch.float64, device=dev)
+        ctx.cfl_dt(ud.data_ptr(), cfl, inv.data_ptr())
+        torch.cuda.synchronize()
+        want = max(c / (cfl * h), mu / (cfl * h * h))
+        np.testing.assert_allclose(inv.cpu().numpy(), want, rtol=1e-13)
+        ctx.close()
+
+
+def test_ns_steady_taylor_couette_newton_gmres(dev):
+    from dpgsolver_amd import _lib
+
+    mu = 0.1
+    walls = M.taylor_couette_walls(mu, PR)
+    mesh = M.annulus_mesh(2, 3, 12)
+    u0, _ = M.taylor_couette_state(mesh, mu, PR)
+    ctx = _ctx(mesh, M.FLUX_ROE, walls=walls, mu=mu)
+    ud = torch.from_numpy(_perturbed(mesh, u0, 0.05)).to(dev)
+    st = ctx.solve_implicit(ud.data_ptr(), cfl_initial=5.0, exit_tol=1e-10, max_newton=100,
+                            precond=_lib.PC_BLOCK_JACOBI)
+    print("newton", st["newton_steps"], "gmres", st["gmres_iters"], "max_rhs", st["max_rhs"], st["seconds"],
+          flush=True)
+    assert st["converged"] == 1 and st["max_rhs"] < 1e-10
+    u = ud.cpu().numpy()
+    r = O.Oracle(mesh, M.PDE_EULER, M.FLUX_ROE, viscous=(mu, PR), walls=walls).rhs_ns(u)
+    assert np.abs(r).max() < 1e-9          # the oracle agrees that this is the steady state
+    # the steady velocity is the Couette profile V_theta = c (1/r - r/r_o^2); the pressure takes up rho V^2 / r
+    x = M.nodes_xyz(mesh)
+    cons = u.reshape(mesh.E, 4, -1).transpose(0, 2, 1)
+    X, Y = x[..., 0], x[..., 1]
+    R = np.hypot(X, Y)
+    vt = (-Y * cons[..., 1] + X * cons[..., 2]) / (R * cons[..., 0])
+    vr = (X * cons[..., 1] + Y * cons[..., 2]) / (R * cons[..., 0])
+    c = 0.5 / (1.0 / 0.5 - 0.5)
+    exact = c * (1.0 / R - R)
+    print(f"V_theta error {np.abs(vt - exact).max():.2e}, max |V_r| {np.abs(vr).max():.2e}", flush=True)
+    assert np.abs(vt - exact).max() < 0.02 * exact.max() and np.abs(vr).max() < 0.02 * exact.max()
+    ctx.close()
+
+
+def test_ns_unlinearised_paths_are_refused(dev):
+    from dpgsolver_amd import _lib
+
+    mesh = M.structured_tp_mesh(2, 2, (3, 3), (0.0, 0.0), (1.0, 1.0), (True, True))
+    ctx = _ctx(mesh, M.FLUX_ROE)
+    u = torch.from_numpy(M.uniform_state(mesh, np.array([1.0, 0.3, 0.1, 1.0]))).to(dev)
+    for kw in ({"precond": _lib.PC_BLOCK_SGS}, {"precond": _lib.PC_BLOCK_SGS32}, {"precond": _lib.PC_LEARNED},
+               {"precond": _lib.PC_BLOCK_JACOBI, "jvp_fp32": True}):
+        with pytest.raises(_lib.DgxError, match="UNSUPPORTED"):
+            ctx.solve_implicit(u.data_ptr(), **kw)
+    nb = ctx.Np * ctx.nv
+    out = torch.empty(mesh.E * 4 * nb * nb, dtype=torch.float64, device=dev)
+    with pytest.raises(_lib.DgxError, match="UNSUPPORTED"):
+        ctx.assemble_face_blocks(u.data_ptr(), out.data_ptr())
+    ctx.close()

@@ -18,7 +18,13 @@ The reference holds no golden values for the viscous path, so each check is one 
   equations exactly, so M^-1 R converges to rho V^2 / r e_r (the centripetal term the uniform pressure leaves)
   in the momentum components and to 0 in mass and energy; a reversed wall rotation or a wrong-sign wall heat
   flux does not converge;
-* Riemann faces in a Navier-Stokes context keep a free stream exactly.
+* Riemann faces in a Navier-Stokes context keep a free stream exactly;
+* the complex twin (or_compute_rhs_ns_c, the complex-step derivatives that pin the GPU linearisation): its real
+  part is the residual bit for bit; with mu = 0 its derivative equals the Euler residual's complex step on a
+  periodic mesh; and the derivative of the viscous part R_NS(mu) - R_NS(0), a smooth function (the numerical
+  fluxes' kinks, max and |.|, cancel in the difference), equals Richardson-extrapolated central differences to
+  1e-8 on periodic, no-slip wall and Riemann meshes.  Differences of the full residual cannot pin it: a face
+  node whose max(|V| + c) or Roe |lambda| changes branch within h gives them O(h) errors.
 """
 import numpy as np
 import pytest
@@ -193,3 +199,69 @@ def test_riemann_faces_keep_a_free_stream_in_navier_stokes():
     orc = O.Oracle(mesh, M.PDE_EULER, M.FLUX_ROE, bc_prim=prim, viscous=(MU, PR))
     r = orc.rhs_ns(M.uniform_state(mesh, prim))
     assert np.abs(r).max() < 1e-13
+
+
+# ---------------------------------------------------------------------------------------------- complex twin
+def _cs_cases():
+    prim_fn = lambda X, Y: np.stack([1 + 0.1 * np.sin(2 * np.pi * X), 0.3 + 0.2 * np.sin(2 * np.pi * Y),
+                                     -0.1 + 0.2 * np.cos(2 * np.pi * X), 1 + 0.1 * np.sin(2 * np.pi * (X + Y))], -1)
+    out = []
+    for p, flux in ((1, M.FLUX_LF), (2, M.FLUX_ROE), (3, M.FLUX_LF)):
+        mesh = _periodic(3, p, curved=True)
+        out.append((f"periodic_p{p}", mesh, flux, {}, _state(mesh, prim_fn)))
+    for bc_outer, dT in ((M.BC_NOSLIP_ADIABATIC, 0.0), (M.BC_NOSLIP_DIABATIC, 1.0)):
+        mesh = M.annulus_mesh(2, 2, 8, bc_outer=bc_outer)
+        u, _ = M.taylor_couette_state(mesh, MU, PR, dTdr_o=dT)
+        x = M.nodes_xyz(mesh)
+        pert = 0.02 * np.sin(3 * x[..., 0]) * np.cos(2 * x[..., 1])
+        u = u * (1.0 + M.pack_state(np.stack([pert, 2 * pert, -pert, 0.5 * pert], -1)))
+        out.append((f"annulus_{'diabatic' if dT else 'adiabatic'}", mesh, M.FLUX_ROE,
+                    {"walls": M.taylor_couette_walls(MU, PR, dTdr_o=dT)}, u))
+    # oblique subsonic free stream: the normal velocity keeps its sign on every boundary face (at V.n = 0 the
+    # Riemann state switches between its inflow and outflow branches, a jump no difference quotient follows)
+    prim = np.array([1.0, 0.3, 0.2, 1.0])
+    mesh = M.structured_tp_mesh(2, 2, (3, 3), (0.0, 0.0), (1.0, 0.8), (False, False),
+                                bc_low=(M.BC_RIEMANN, M.BC_RIEMANN), bc_high=(M.BC_RIEMANN, M.BC_RIEMANN))
+    x = M.nodes_xyz(mesh)
+    pert = 0.02 * np.sin(3 * x[..., 0]) * np.cos(2 * x[..., 1])
+    u = M.pack_state(M.prim_to_cons(prim[None, None, :] * (1.0 + np.stack([pert, pert, -pert, 0.5 * pert], -1)), 2))
+    out.append(("riemann_box", mesh, M.FLUX_LF, {"bc_prim": prim}, u))
+    return out
+
+
+CS_CASES = _cs_cases()
+
+
+@pytest.mark.parametrize("case", CS_CASES, ids=[c[0] for c in CS_CASES])
+def test_ns_complex_twin_real_part_is_the_residual(case):
+    _, mesh, flux, kw, u = case
+    orc = O.Oracle(mesh, M.PDE_EULER, flux, viscous=(MU, PR), **kw)
+    rre, rim = orc.rhs_ns_complex(u, np.zeros_like(u))
+    assert np.array_equal(rre, orc.rhs_ns(u)) and not rim.any()
+
+
+@pytest.mark.parametrize("case", CS_CASES, ids=[c[0] for c in CS_CASES])
+def test_ns_complex_step_of_the_viscous_part_matches_richardson_differences(case):
+    _, mesh, flux, kw, u = case
+    visc = O.Oracle(mesh, M.PDE_EULER, flux, viscous=(MU, PR), **kw)
+    invisc = O.Oracle(mesh, M.PDE_EULER, flux, viscous=(0.0, PR), **kw)
+    v = np.random.default_rng(3).standard_normal(u.size) * np.abs(u)
+    cs = visc.jvp_ns(u, v) - invisc.jvp_ns(u, v)
+    rv = lambda w: visc.rhs_ns(w) - invisc.rhs_ns(w)
+    dif = lambda h: (rv(u + h * v) - rv(u - h * v)) / (2 * h)
+    h = 1e-3
+    rich = (4 * dif(h / 2) - dif(h)) / 3
+    err = np.abs(cs - rich).max() / np.abs(cs).max()
+    print(f"{case[0]}: complex step vs Richardson {err:.1e}")
+    assert err < 1e-8
+
+
+def test_ns_complex_step_without_viscosity_is_the_euler_one():
+    mesh = _periodic(3, 2, curved=True)
+    u = _state(mesh, lambda X, Y: np.stack([1 + 0.1 * np.sin(2 * np.pi * X), 0.3 + 0.0 * X,
+                                            -0.1 + 0.2 * np.cos(2 * np.pi * X), 1 + 0.05 * np.cos(2 * np.pi * Y)], -1))
+    v = np.random.default_rng(4).standard_normal(u.size)
+    for flux in (M.FLUX_LF, M.FLUX_ROE):
+        ns = O.Oracle(mesh, M.PDE_EULER, flux, viscous=(0.0, PR)).jvp_ns(u, v)
+        eu = O.Oracle(mesh, M.PDE_EULER, flux).rhs_complex(u, 1e-30 * v)[1] / 1e-30
+        assert np.abs(ns - eu).max() <= 1e-13 * np.abs(eu).max()
