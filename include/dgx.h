@@ -294,11 +294,14 @@ dgx_status dgx_max_abs(dgx_ctx* ctx, const double* x, long long n, double* out);
 dgx_status dgx_convert_f64_f32(dgx_ctx* ctx, const double* src, float* dst, long long n);
 dgx_status dgx_convert_f32_f64(dgx_ctx* ctx, const float* src, double* dst, long long n);
 
-/* Implicit path.
+/* Implicit path (2-D QUAD contexts of one order: advection, Euler, and Navier-Stokes where stated).
  * Diagonal blocks A_e = dR_e/du_e - mass_scale[e] * M_e (mass_scale: device array of E, or NULL for none), (Np*nv)^2 each, row-major, row eq*Np+node,
  * column var*Np+node: volume lhs_v_1 + face lhs_ll/lhs_rr contributions + CFL term
  * (constructor_lhs_v_1_T compute_volume_rlhs_T.c:179-220, constructor_lhs_f_1_T
- *  compute_face_rlhs_T.c:163-220, compute_CFL_ramping solve_dg.c:223-248 with mass_scale = 1/dt). */
+ *  compute_face_rlhs_T.c:163-220, compute_CFL_ramping solve_dg.c:223-248 with mass_scale = 1/dt).
+ * Navier-Stokes: the exact dR_e/du_e of the BR2 residual, the element's own lifted gradients differentiated with
+ * the rest and the neighbours' states and gradient traces held fixed (the 'i' branch of compute_rlhs_dg,
+ * compute_face_rlhs_dg_T.c:243-300). */
 dgx_status dgx_assemble_diag_blocks(dgx_ctx* ctx, const double* u, const double* mass_scale, double* blocks);
 /* Face (off-diagonal) blocks: blocks[(e * 2d + lf) * nb^2 + row * nb + col] = dR_e[row] / du_n[col], n the
  * neighbour across local face lf (col in n's local numbering var*Np+node); zero for boundary faces.  Only the
@@ -306,15 +309,16 @@ dgx_status dgx_assemble_diag_blocks(dgx_ctx* ctx, const double* u, const double*
  * with the element's face nodes); they are computed compactly, nv*N dual-number launches, and scattered.
  * These are lhs_lr / lhs_rl of constructor_lhs_f_1_T (compute_face_rlhs_T.c:163-220, added at
  * compute_face_rlhs_dg.c:198-213); with the diagonal blocks they are the whole block-sparse Jacobian. */
-dgx_status dgx_assemble_face_blocks(dgx_ctx* ctx, const double* u, double* blocks);
-/* CFL-ramped pseudo-time step per element, returned as inv_dt[e] = max wave speed / (cfl * h_min)
- * (compute_dt_cfl_constrained / compute_min_length_measure, solve_dg.c:284-379). */
+dgx_status dgx_assemble_face_blocks(dgx_ctx* ctx, const double* u, double* blocks);   /* not Navier-Stokes */
+/* CFL-ramped pseudo-time step per element, returned as inv_dt[e] = max wave speed / (cfl * h_min), and for
+ * Navier-Stokes the larger of that and mu / (cfl * h_min^2) (compute_dt_cfl_constrained /
+ * compute_min_length_measure, solve_dg.c:284-379). */
 dgx_status dgx_cfl_dt(dgx_ctx* ctx, const double* u, double cfl, double* inv_dt);
 /* y_e -= scale[e] * M_e x_e (scale NULL = 1): the -M/dt term of the CFL-ramped system. */
 dgx_status dgx_mass_sub(dgx_ctx* ctx, const double* x, const double* scale, double* y);
 /* Exact Jacobian-vector product jv = (dR/du)(u) v by forward-mode (dual-number) evaluation of the
- * residual: the matrix-free operator of the Newton system (replaces the PETSc AIJ Mat assembled by
- * add_to_petsc_Mat, solve.c:321-338). */
+ * residual (Navier-Stokes: of the BR2 residual, gradients included): the matrix-free operator of the Newton
+ * system (replaces the PETSc AIJ Mat assembled by add_to_petsc_Mat, solve.c:321-338). */
 dgx_status dgx_jacobian_vector(dgx_ctx* ctx, const double* u, const double* v, double* jv);
 /* out = (dR/du)(u) v - mass_scale[e] M_e v_e (mass_scale: device, E doubles): the operator of the pseudo-time
  * Newton system, J v and the -M/dt term in one dual-number launch (the GMRES operator of dgx_solve_implicit). */
@@ -331,7 +335,8 @@ dgx_status dgx_block_inv_apply(dgx_ctx* ctx, const double* inv, const double* x,
 
 /* Steady solve by pseudo-time Newton-GMRES (solve_implicit, solve_implicit.c:95-110, 232-281, 393-470).
  * Names mirror the reference's test_case.data keys (cfl_initial, exit_tol_i, exit_ratio_i) and the PETSc
- * options file (GMRES restart 60, rtol 1e-4).  u (device) is updated in place. */
+ * options file (GMRES restart 60, rtol 1e-4).  u (device) is updated in place.  Navier-Stokes contexts:
+ * DGX_PC_BLOCK_JACOBI or DGX_PC_NONE with the FP64 operator (others: DGX_ERR_UNSUPPORTED). */
 /* Preconditioners of the GMRES solves.  DGX_PC_BLOCK_JACOBI: exact inverses of the element diagonal blocks of
  * dR/du - M/dt.  DGX_PC_BLOCK_SGS (nb <= 64): one forward and one backward multicolour block Gauss-Seidel sweep
  * over the diagonal and face blocks from z = 0, the coupled-block counterpart of the reference's default
