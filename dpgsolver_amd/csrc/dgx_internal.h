@@ -97,6 +97,8 @@ struct TriOps {
 	T K[3][kHpMaxN * kHpMaxN];
 	// the lifts of the three faces' projections y at the cubature, I M^-1 tw0_f side by side: [q][f N + b]
 	T L[kTriMaxNc * 3 * kHpMaxN];
+	// M^-1 I^T of an element with det J = 1, [m][q]: the volume gradient's projection of the cubature values
+	T MIt[kTriMaxNp * kTriMaxNc];
 	int p, Np, Nc, N;
 };
 bool build_tri_ops(int p, TriOps<double>& o);

@@ -208,6 +208,12 @@ bool build_tri_ops(int p, TriOps<double>& o)
 				}
 				o.L[q * 3 * N + lf * N + b] = (double)acc;
 			}
+	for (int m = 0; m < Np; ++m)
+		for (int q = 0; q < Nc; ++q) {
+			long double acc = 0.0L;
+			for (int n = 0; n < Np; ++n) acc += (long double)mi[(size_t)m * Np + n] * o.I[q * Np + n];
+			o.MIt[m * Nc + q] = (double)acc;
+		}
 	return true;
 }
 

@@ -10,7 +10,8 @@
 //   k_tri_trace: the state's traces at each face's own GL points (Euler, before the face kernels);
 //   k_tri_rhs:   the Euler residual from the lifted face fluxes z;
 //   k_tri_gv:    (BR2) the volume part of the weak gradient G_v = M^-1 I^T W (J grad s)
-//                (compute_grad_coef_dg_T.c:171-307), its face traces, and the state traces;
+//                (compute_grad_coef_dg_T.c:171-307), its face traces, and the state traces (affine FP64 groups:
+//                k_tri_gv_mfma on the matrix cores);
 //   k_tri_ns:    G = G_v + sum_f M^-1 tw0_f y_f, the volume flux F_euler + fv (flux_navier_stokes_T.c:1005-1040)
 //                and the residual.
 // M^-1 and the lift traces K_f = Ifc_f M^-1 tw0_f: the reference ones divided by det J on an affine group, each
@@ -454,6 +455,156 @@ __global__ void __launch_bounds__(256) k_tri_ns_mfma(TriArgs a)
 			}
 		}
 		__syncthreads();
+	}
+}
+
+// The BR2 volume gradient of an affine TRI group on the matrix cores (FP64).  With the (variable, element) tiles of
+// k_tri_rhs_mfma:
+//   SR^T, SS^T (16 x Nc) = U^T Dr^T, U^T Ds^T        the reference derivatives at the cubature; a lane then holds the
+//       four variables of one element at one point and forms X_d = w (J grad s)_d with the cofactors;
+//   G_d^T (16 x Np) = (X_d^T / det J) (M^-1 I^T)^T  the volume gradient G_v at the nodes (written out);
+//   T^T (16 x 3N) = U^T Ifc^T, G_d^T Ifc^T             the state and gradient traces at the own GL face points,
+// with X and then G turned from the D layout to the A layout through a per-wavefront LDS block.
+template <int P>
+__global__ void __launch_bounds__(256) k_tri_gv_mfma(TriArgs a)
+{
+	using T = TriMfma<P>;
+	constexpr int Np = T::Np, Nc = T::Nc, N = T::N, KS1 = T::KS1, NT1 = T::NT1, KS2 = T::KS2, QP = T::QP;
+	constexpr int MT = T::MT, F3 = 3 * N, FT = (F3 + 15) / 16, NPP = 4 * KS1, K8 = 2 * kNV;
+	constexpr int XS = 4 * 2 * kNV * QP, GS = 4 * 2 * kNV * NPP, WS = XS > GS ? XS : GS;
+	__shared__ double sBr[KS1 * NT1 * 64], sBs[KS1 * NT1 * 64], sBK[KS2 * MT * 64], sBF[KS1 * FT * 64];
+	__shared__ double sWq[QP];
+	__shared__ double sX[4][WS];   // per wavefront: X [e][d][v][q], then G [e][d][v][m]
+	const int t = threadIdx.x;
+	for (int i = t; i < KS1 * NT1 * 64; i += 256) {
+		const int ln = i & 63, f = i >> 6, kk = f / NT1, nt = f - kk * NT1;
+		const int q = 16 * nt + (ln & 15), m = 4 * kk + (ln >> 4);
+		const bool ok = q < Nc && m < Np;
+		sBr[i] = ok ? a.ops->Dr[q * Np + m] : 0.0;
+		sBs[i] = ok ? a.ops->Ds[q * Np + m] : 0.0;
+	}
+	for (int i = t; i < KS2 * MT * 64; i += 256) {
+		const int ln = i & 63, f = i >> 6, kk = f / MT, mt = f - kk * MT;
+		const int m = 16 * mt + (ln & 15), q = 4 * kk + (ln >> 4);
+		sBK[i] = (m < Np && q < Nc) ? a.ops->MIt[m * Nc + q] : 0.0;
+	}
+	for (int i = t; i < KS1 * FT * 64; i += 256) {
+		const int ln = i & 63, f = i >> 6, kk = f / FT, ft = f - kk * FT;
+		const int k0 = 16 * ft + (ln & 15), m = 4 * kk + (ln >> 4);
+		sBF[i] = (k0 < F3 && m < Np) ? a.ops->Ifc[k0 / N][(k0 % N) * Np + m] : 0.0;
+	}
+	for (int i = t; i < QP; i += 256) sWq[i] = i < Nc ? a.ops->w[i] : 0.0;
+	__syncthreads();
+	const int wave = t >> 6, l = t & 63;
+	const int ea = (l & 15) & 3, va = (l & 15) >> 2, ka = l >> 4;   // A fragment: row l % 16, k slot l / 16
+	const int ed = l >> 4, jd = l & 15;                              // D fragment: element l / 16, column l % 16
+	double* W = sX[wave];
+	const int ngroups = (a.E + 3) / 4;
+	for (int g0 = blockIdx.x * 4; g0 < ngroups; g0 += gridDim.x * 4) {   // the same trip count for every wave
+		const int grp = g0 + wave;
+		const bool gvalid = grp < ngroups;
+		const int e0 = grp * 4;
+		const int eA = gvalid ? min(e0 + ea, a.E - 1) : 0;   // a padding slot reads a real element (never stored)
+		const int eD = gvalid ? min(e0 + ed, a.E - 1) : 0;
+		const bool store = gvalid && e0 + ed < a.E;
+		double ua[KS1];
+#pragma unroll
+		for (int kk = 0; kk < KS1; ++kk) {
+			const int m = 4 * kk + ka;
+			ua[kk] = (gvalid && m < Np) ? a.u[(size_t)eA * kNV * Np + va * Np + m] : 0.0;
+		}
+		// W J grad s at the cubature, into LDS in the A layout
+#pragma unroll
+		for (int nt = 0; nt < NT1; ++nt) {
+			dgx_d4 ar = { 0.0, 0.0, 0.0, 0.0 }, as = ar;
+#pragma unroll
+			for (int kk = 0; kk < KS1; ++kk) {
+				ar = __builtin_amdgcn_mfma_f64_16x16x4f64(ua[kk], sBr[(kk * NT1 + nt) * 64 + l], ar, 0, 0, 0);
+				as = __builtin_amdgcn_mfma_f64_16x16x4f64(ua[kk], sBs[(kk * NT1 + nt) * 64 + l], as, 0, 0, 0);
+			}
+			const int q = 16 * nt + jd;
+			if (q < QP) {
+				double x0[kNV] = {}, x1[kNV] = {};
+				if (gvalid && q < Nc) {
+					const double* mp = a.met + (size_t)eD * 4 * Nc + q;
+					const double m0 = mp[0], m1 = mp[Nc], m2 = mp[2 * Nc], m3 = mp[3 * Nc], w = sWq[q];
+#pragma unroll
+					for (int v = 0; v < kNV; ++v) {
+						x0[v] = w * (m0 * ar[v] + m2 * as[v]);
+						x1[v] = w * (m1 * ar[v] + m3 * as[v]);
+					}
+				}
+#pragma unroll
+				for (int v = 0; v < kNV; ++v) {
+					W[((ed * 2 + 0) * kNV + v) * QP + q] = x0[v];
+					W[((ed * 2 + 1) * kNV + v) * QP + q] = x1[v];
+				}
+			}
+		}
+		__syncthreads();
+		// G_v = (M^-1 I^T X) / det J at the nodes
+		dgx_d4 g[MT][2];
+		const double js = gvalid ? a.jinv[eD] : 0.0;
+#pragma unroll
+		for (int mt = 0; mt < MT; ++mt) {
+			dgx_d4 a0 = { 0.0, 0.0, 0.0, 0.0 }, a1 = a0;
+#pragma unroll
+			for (int kk = 0; kk < KS2; ++kk) {
+				const int q = 4 * kk + ka;
+				const double b = sBK[(kk * MT + mt) * 64 + l];
+				a0 = __builtin_amdgcn_mfma_f64_16x16x4f64(W[((ea * 2 + 0) * kNV + va) * QP + q], b, a0, 0, 0, 0);
+				a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(W[((ea * 2 + 1) * kNV + va) * QP + q], b, a1, 0, 0, 0);
+			}
+#pragma unroll
+			for (int v = 0; v < kNV; ++v) {
+				g[mt][0][v] = a0[v] * js;
+				g[mt][1][v] = a1[v] * js;
+			}
+		}
+		__syncthreads();   // X is read by every lane of the wavefront before G takes its place
+#pragma unroll
+		for (int mt = 0; mt < MT; ++mt) {
+			const int m = 16 * mt + jd;
+			if (m < NPP) {
+#pragma unroll
+				for (int d = 0; d < 2; ++d)
+#pragma unroll
+					for (int v = 0; v < kNV; ++v) W[((ed * 2 + d) * kNV + v) * NPP + m] = m < Np ? g[mt][d][v] : 0.0;
+			}
+			if (store && m < Np) {
+				double* G = a.gv + (size_t)(e0 + ed) * K8 * Np + m;
+#pragma unroll
+				for (int d = 0; d < 2; ++d)
+#pragma unroll
+					for (int v = 0; v < kNV; ++v) G[(d * kNV + v) * Np] = g[mt][d][v];
+			}
+		}
+		__syncthreads();
+		// the state and G_v at the own GL face points
+#pragma unroll
+		for (int ft = 0; ft < FT; ++ft) {
+			dgx_d4 at = { 0.0, 0.0, 0.0, 0.0 }, ag0 = at, ag1 = at;
+#pragma unroll
+			for (int kk = 0; kk < KS1; ++kk) {
+				const int m = 4 * kk + ka;
+				const double b = sBF[(kk * FT + ft) * 64 + l];
+				at = __builtin_amdgcn_mfma_f64_16x16x4f64(ua[kk], b, at, 0, 0, 0);
+				ag0 = __builtin_amdgcn_mfma_f64_16x16x4f64(W[((ea * 2 + 0) * kNV + va) * NPP + m], b, ag0, 0, 0, 0);
+				ag1 = __builtin_amdgcn_mfma_f64_16x16x4f64(W[((ea * 2 + 1) * kNV + va) * NPP + m], b, ag1, 0, 0, 0);
+			}
+			const int k0 = 16 * ft + jd;
+			if (store && k0 < F3) {
+				const int lf = k0 / N, b = k0 - lf * N;
+				const size_t blk = (size_t)(e0 + ed) * 3 + lf;
+#pragma unroll
+				for (int v = 0; v < kNV; ++v) {
+					a.tr[(blk * kNV + v) * N + b] = at[v];
+					a.gvt[(blk * K8 + v) * N + b] = ag0[v];
+					a.gvt[(blk * K8 + kNV + v) * N + b] = ag1[v];
+				}
+			}
+		}
+		__syncthreads();   // G is read before the next group's X overwrites it
 	}
 }
 
@@ -1286,7 +1437,11 @@ hipError_t launch_tri_ns_phase64(const TriGroup& g, const TriOps<double>* ops, c
 	a.mu = mu;
 	a.Pr = Pr;
 	if (phase == 1) {
-		DGX_TRI_SWITCH_T(k_tri_gv, double, epb_gv)
+		if (g.affine && tri_use_mfma()) {
+			DGX_TRI_SWITCH(k_tri_gv_mfma, epb_mfma)
+		} else {
+			DGX_TRI_SWITCH_T(k_tri_gv, double, epb_gv)
+		}
 	} else if (g.affine && tri_use_mfma()) {
 		DGX_TRI_SWITCH(k_tri_ns_mfma, epb_mfma)
 	} else {
