@@ -1456,9 +1456,13 @@ dgx_status dgx_assemble_face_blocks(dgx_ctx* h, const double* u, double* blocks)
 	if (!h || !u || !blocks) return DGX_ERR_ARG;
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
-	// the face blocks of the Euler and advection residuals (the BR2 residual couples beyond the face neighbours'
-	// face nodes)
-	if (c.d != 2 || c.cfg.pde == DGX_PDE_NAVIER_STOKES) return DGX_ERR_UNSUPPORTED;
+	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
+	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) {
+		// BR2: full blocks (the neighbour's whole state enters through its lifted gradient), one column per launch
+		const int nb = c.Np * c.nv;
+		for (int col = 0; col < nb; ++col) DGX_TRY(dgx::launch_ns_offd_col(c, u, blocks, col, false));
+		return DGX_OK;
+	}
 	const int nfv = c.nv * c.N;   // face-node rows / columns of a 2-D face block
 	double* cT = nullptr;
 	DGX_TRY(hipMalloc(&cT, sizeof(double) * (size_t)c.E * 4 * nfv * nfv));

@@ -488,6 +488,41 @@ __global__ void __launch_bounds__(64) k_bgs_color(const int* __restrict__ elems,
 	}
 }
 
+// The same colour sweep with full face blocks (the BR2 residual: the neighbour's whole state enters through its
+// lifted gradient): offT holds dR_e/du_n transposed per local face, [e][lf][col][row], nb <= 64.
+template <typename TB>
+__global__ void __launch_bounds__(64) k_bgs_full_color(const int* __restrict__ elems, const ElemFace* __restrict__ ef,
+                                                       const TB* __restrict__ invT, const TB* __restrict__ offT,
+                                                       const double* __restrict__ rhs, double* z, int nb)
+{
+	__shared__ double sz[4][64], sr[64];
+	const int e = elems[blockIdx.x], t = threadIdx.x;
+#pragma unroll
+	for (int lf = 0; lf < 4; ++lf) {
+		const ElemFace f = ef[(size_t)e * 4 + lf];
+		sz[lf][t] = (f.en >= 0 && t < nb) ? z[(size_t)f.en * nb + t] : 0.0;
+	}
+	__syncthreads();
+	if (t < nb) {
+		double r = rhs[(size_t)e * nb + t];
+#pragma unroll
+		for (int lf = 0; lf < 4; ++lf) {
+			const TB* C = offT + ((size_t)e * 4 + lf) * nb * nb;
+			double acc = 0.0;
+			for (int c = 0; c < nb; ++c) acc += (double)C[(size_t)c * nb + t] * sz[lf][c];
+			r -= acc;
+		}
+		sr[t] = r;
+	}
+	__syncthreads();
+	if (t < nb) {
+		const TB* M = invT + (size_t)e * nb * nb;
+		double acc = 0.0;
+		for (int k = 0; k < nb; ++k) acc += (double)M[(size_t)k * nb + t] * sr[k];
+		z[(size_t)e * nb + t] = acc;
+	}
+}
+
 struct PrecondIdentity : Precond {
 	hipError_t setup(Ctx&, const double*, const double*) override { return hipSuccess; }
 	hipError_t apply(Ctx& c, const double* v, double* z) override
@@ -547,7 +582,7 @@ struct PrecondBlockJacobi : Precond {
 // colours, then backward, from z = 0.  A fixed linear operator, so plain right-preconditioned GMRES applies.
 struct PrecondBlockSGS : Precond {
 	PrecondBlockJacobi diag;   // assembles and inverts A_ee - M_e/dt
-	double* offT = nullptr;    // E * 4 * nfv^2 compact transposed face blocks
+	double* offT = nullptr;    // E * 4 * nfv^2 compact transposed face blocks (Navier-Stokes: E * 4 * nb^2, full)
 	bool fp32 = false;         // sweep over FP32 copies of the inverse and face blocks
 	float* inv32 = nullptr;
 	float* off32 = nullptr;
@@ -558,15 +593,21 @@ struct PrecondBlockSGS : Precond {
 		if (inv32) (void)hipFree(inv32);
 		if (off32) (void)hipFree(off32);
 	}
+	static bool full(const Ctx& c) { return c.cfg.pde == DGX_PDE_NAVIER_STOKES; }
 	hipError_t setup(Ctx& c, const double* u, const double* inv_dt) override
 	{
 		const int nb = c.Np * c.nv, nfv = c.nv * c.N;
 		if (c.d != 2 || nb > kMaxInvBlock || 4 * nfv > 64) return hipErrorInvalidValue;
 		DGX_HIP_CHECK(diag.setup(c, u, inv_dt));
-		if (!offT) DGX_HIP_CHECK(hipMalloc(&offT, sizeof(double) * (size_t)c.E * 4 * nfv * nfv));
-		for (int col = 0; col < nfv; ++col) DGX_HIP_CHECK(launch_offd_col(c, u, offT, col));
+		// the face blocks: compact on the face nodes (Euler, advection), full for BR2
+		const int fb = full(c) ? nb : nfv;
+		if (!offT) DGX_HIP_CHECK(hipMalloc(&offT, sizeof(double) * (size_t)c.E * 4 * fb * fb));
+		if (full(c))
+			for (int col = 0; col < nb; ++col) DGX_HIP_CHECK(launch_ns_offd_col(c, u, offT, col, true));
+		else
+			for (int col = 0; col < nfv; ++col) DGX_HIP_CHECK(launch_offd_col(c, u, offT, col));
 		if (!fp32) return hipSuccess;
-		const size_t ni = (size_t)c.E * nb * nb, no = (size_t)c.E * 4 * nfv * nfv;
+		const size_t ni = (size_t)c.E * nb * nb, no = (size_t)c.E * 4 * fb * fb;
 		if (!inv32) DGX_HIP_CHECK(hipMalloc(&inv32, sizeof(float) * ni));
 		if (!off32) DGX_HIP_CHECK(hipMalloc(&off32, sizeof(float) * no));
 		DGX_HIP_CHECK(launch_convert(diag.blocks, inv32, ni, c.stream));
@@ -576,6 +617,16 @@ struct PrecondBlockSGS : Precond {
 	{
 		const int n0 = c.color_off[k], cnt = c.color_off[k + 1] - n0;
 		if (cnt == 0) return hipSuccess;
+		if (full(c)) {
+			const int nb = c.Np * c.nv;
+			if (fp32)
+				hipLaunchKernelGGL(k_bgs_full_color<float>, dim3(cnt), dim3(64), 0, c.stream, c.color_elems + n0,
+				                   c.elem_faces, inv32, off32, v, z, nb);
+			else
+				hipLaunchKernelGGL(k_bgs_full_color<double>, dim3(cnt), dim3(64), 0, c.stream, c.color_elems + n0,
+				                   c.elem_faces, diag.blocks, offT, v, z, nb);
+			return hipGetLastError();
+		}
 		if (fp32)
 			hipLaunchKernelGGL(k_bgs_color<float>, dim3(cnt), dim3(64), 0, c.stream, c.color_elems + n0, c.elem_faces,
 			                   inv32, off32, v, z, c.N, c.nv);
@@ -795,9 +846,8 @@ extern "C" dgx_status dgx_precond_apply(dgx_ctx* h, int precond, const double* u
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
 	if (c.d != 2 || c.cfg.pde == DGX_PDE_ADVECTION) return DGX_ERR_UNSUPPORTED;
-	// Navier-Stokes: the diagonal blocks only (the BR2 residual has no face-block linearisation here)
-	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES && precond != DGX_PC_NONE && precond != DGX_PC_BLOCK_JACOBI)
-		return DGX_ERR_UNSUPPORTED;
+	// Navier-Stokes: no learned preconditioner (it is trained on Euler blocks)
+	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES && precond == DGX_PC_LEARNED) return DGX_ERR_UNSUPPORTED;
 	dgx::Precond* P = nullptr;
 	if (precond == DGX_PC_NONE) P = new dgx::PrecondIdentity();
 	else if (precond == DGX_PC_BLOCK_JACOBI) P = new dgx::PrecondBlockJacobi();
@@ -819,9 +869,8 @@ extern "C" dgx_status dgx_solve_implicit(dgx_ctx* h, double* u, const dgx_implic
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
 	if (c.d != 2 || c.cfg.pde == DGX_PDE_ADVECTION) return DGX_ERR_UNSUPPORTED;
-	// Navier-Stokes: block-Jacobi or no preconditioner, the FP64 Jacobian-vector product
-	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES &&
-	    ((prm->precond != DGX_PC_NONE && prm->precond != DGX_PC_BLOCK_JACOBI) || prm->jvp_fp32))
+	// Navier-Stokes: the FP64 Jacobian-vector product, no learned preconditioner
+	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES && (prm->precond == DGX_PC_LEARNED || prm->jvp_fp32))
 		return DGX_ERR_UNSUPPORTED;
 	if (prm->gmres_restart < 1 || prm->gmres_restart > 200 || prm->max_newton < 1 || !(prm->cfl_initial > 0.0))
 		return DGX_ERR_ARG;

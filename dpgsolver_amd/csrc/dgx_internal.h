@@ -287,6 +287,7 @@ hipError_t launch_rhs_ns_mixed(Ctx& c, const double* u, float* r);   // FP64 sta
 // its linearisation (dual numbers): jv = J v (or J v - mass_scale[e] M_e v), one column of every diagonal block
 hipError_t launch_ns_jvp(Ctx& c, const double* u, const double* v, double* jv, const double* mass_scale);
 hipError_t launch_ns_diag_col(Ctx& c, const double* u, double* blocks, int col);
+hipError_t launch_ns_offd_col(Ctx& c, const double* u, double* blocks, int col, bool transposed);
 hipError_t launch_rhs_mixed(Ctx& c, const double* u, float* r);
 // variable-order meshes: the lifted fluxes of the faces between orders (FP64 / FP32 state); Navier-Stokes also
 // the projected BR2 face products, between the groups' gradient (phase 1) and residual (phase 2) kernels

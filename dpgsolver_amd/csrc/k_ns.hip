@@ -41,10 +41,12 @@ constexpr int ns_epb()
 	return (lanes / (N * N)) > 0 ? lanes / (N * N) : 1;
 }
 
-// what the kernels compute: the residual, J v (dual numbers seeded with v; with mass_scale, J v - s_e M_e v), or
+// what the kernels compute: the residual, J v (dual numbers seeded with v; with mass_scale, J v - s_e M_e v),
 // one column of every element's diagonal block dR_e/du_e (dual numbers seeded with the local dof col, the
-// neighbours' traces and gradient traces held fixed)
-constexpr int kNsRhs = 0, kNsJvp = 1, kNsDiag = 2;
+// neighbours' traces and gradient traces held fixed), or one column of every face block dR_e/du_n, n the
+// neighbour across local face blockIdx.y (the neighbour's dof col seeded, through its trace and its gradient
+// trace on that face; the element's own state and the other neighbours held fixed)
+constexpr int kNsRhs = 0, kNsJvp = 1, kNsDiag = 2, kNsOffd = 3;
 
 // node of local face lf at face position a (faces r-, r+, s-, s+)
 template <int N>
@@ -70,7 +72,8 @@ struct NsArgs {
 	const Ops1D<G>* ops;
 	S mu, Pr;
 	int E;
-	int col;                    // kNsDiag: the seeded local dof
+	int col;                    // kNsDiag, kNsOffd: the seeded local dof
+	int offd_transposed;        // kNsOffd: out[((e 4 + lf) nb + col) nb + row] (else [.. row) nb + col])
 	const double* mass_scale;   // kNsJvp: s_e of the -s_e M_e v term (or NULL)
 	// variable-order meshes: per element face to an element of another order, the lifted face flux [v][a]
 	// and then the projected face products q [d][v][a] (own GL face points and ordering)
@@ -263,6 +266,9 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 					const size_t gn = ((size_t)en * kNsNV + v) * Np + ns_face_node<N>(fn, b);
 					if constexpr (PERT) an += B.I[i * N + b] * S(a.u64[gn] - B.C[slc][v]);
 					else if constexpr (MODE == kNsJvp) an += B.I[i * N + b] * S(a.u[gn], a.v[gn]);
+					else if constexpr (MODE == kNsOffd)   // the seeded face's neighbour dof col
+						an += B.I[i * N + b] *
+						      S(a.u[gn], (lf == (int)blockIdx.y && v * Np + ns_face_node<N>(fn, b) == a.col) ? 1.0 : 0.0);
 					else an += B.I[i * N + b] * S(a.u[gn]);   // kNsDiag: the neighbour held fixed
 				}
 				TRo[slc][(lf * kNsNV + v) * N + i] = ao;
@@ -366,6 +372,16 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 						// the neighbour's gradient trace held fixed (its derivative is for its own seeded dof)
 						if (side == 0) tR = strip(tR);
 						else tL = strip(tL);
+					} else if constexpr (MODE == kNsOffd) {
+						// the own gradient trace held fixed, the neighbour's kept on the seeded face only
+						const bool keep = lf == (int)blockIdx.y;
+						if (side == 0) {
+							tL = strip(tL);
+							if (!keep) tR = strip(tR);
+						} else {
+							tR = strip(tR);
+							if (!keep) tL = strip(tL);
+						}
 					}
 					gL[v + kNsNV * d] = tL + eta * kL * q;
 					gR[v + kNsNV * d] = wall ? gL[v + kNsNV * d] : tR + eta * kR * q;
@@ -530,7 +546,13 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 			if constexpr (MODE == kNsJvp) a.out[((size_t)e * kNsNV + v) * Np + t] = acc.d;
 			else if constexpr (MODE == kNsDiag)
 				a.out[(size_t)e * (kNsNV * Np) * (kNsNV * Np) + (size_t)(v * Np + t) * (kNsNV * Np) + a.col] = acc.d;
-			else a.out[((size_t)e * kNsNV + v) * Np + t] = acc;
+			else if constexpr (MODE == kNsOffd) {
+				constexpr int nb = kNsNV * Np;
+				const size_t blk = ((size_t)e * 4 + blockIdx.y) * nb;
+				const int row = v * Np + t;
+				if (a.offd_transposed) a.out[(blk + a.col) * nb + row] = acc.d;
+				else a.out[(blk + row) * nb + a.col] = acc.d;
+			} else a.out[((size_t)e * kNsNV + v) * Np + t] = acc;
 		}
 }
 
@@ -541,15 +563,19 @@ hipError_t run_ns(Ctx& c, const NsArgs<S>& a, int phase)
 {
 	constexpr int EPB = ns_epb<S, N>(), Np = N * N;
 	const int grid = (c.E + EPB - 1) / EPB;
+	// kNsOffd: every element's gradient traces seeded at its own dof col (the kNsDiag gradient kernel), then one
+	// residual launch per local face (blockIdx.y) seeded through that face's neighbour
+	constexpr int GVMODE = (MODE == kNsOffd) ? kNsDiag : MODE;
+	const dim3 rgrid(grid, MODE == kNsOffd ? 4 : 1);
 	if (phase != 2) {
-		hipLaunchKernelGGL((k_ns_gv<S, N, PERT, MODE>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
+		hipLaunchKernelGGL((k_ns_gv<S, N, PERT, GVMODE>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
 		DGX_HIP_CHECK(hipGetLastError());
 	}
 	if (phase == 1) return hipSuccess;
 	if (c.cfg.num_flux == DGX_FLUX_LF)
-		hipLaunchKernelGGL((k_rhs_ns<S, N, DGX_FLUX_LF, PERT, MODE>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
+		hipLaunchKernelGGL((k_rhs_ns<S, N, DGX_FLUX_LF, PERT, MODE>), rgrid, dim3(EPB * Np), 0, c.stream, a);
 	else
-		hipLaunchKernelGGL((k_rhs_ns<S, N, DGX_FLUX_ROE, PERT, MODE>), dim3(grid), dim3(EPB * Np), 0, c.stream, a);
+		hipLaunchKernelGGL((k_rhs_ns<S, N, DGX_FLUX_ROE, PERT, MODE>), rgrid, dim3(EPB * Np), 0, c.stream, a);
 	return hipGetLastError();
 }
 
@@ -560,7 +586,7 @@ static hipError_t launch_ns(Ctx& c, const typename NsArgs<S>::G* u, typename NsA
                             const typename NsArgs<S>::G* met, const typename NsArgs<S>::G* detJ,
                             const typename NsArgs<S>::G* nJ, const Ops1D<typename NsArgs<S>::G>* ops,
                             const double* u64 = nullptr, int phase = 0, const double* v = nullptr, int col = 0,
-                            const double* mass_scale = nullptr)
+                            const double* mass_scale = nullptr, int offd_transposed = 0)
 {
 	if (c.d != 2 || c.cfg.pde != DGX_PDE_NAVIER_STOKES || !c.ns_gvt || !c.ns_kappa) return hipErrorInvalidValue;
 	if (c.E == 0) return hipSuccess;
@@ -578,6 +604,7 @@ static hipError_t launch_ns(Ctx& c, const typename NsArgs<S>::G* u, typename NsA
 		a.v = v;
 		a.col = col;
 		a.mass_scale = mass_scale;
+		a.offd_transposed = offd_transposed;
 	} else {
 		a.gvt = reinterpret_cast<S*>(c.ns_gvt);     // FP32 uses the first half of the FP64-sized buffers
 		a.kap = reinterpret_cast<S*>(c.ns_kappa);
@@ -649,6 +676,14 @@ hipError_t launch_ns_diag_col(Ctx& c, const double* u, double* blocks, int col)
 {
 	return launch_ns<Dual, false, kNsDiag>(c, u, blocks, c.metrics_vc, c.detJ_vc, c.nJ_fc, c.ops64, nullptr, 0,
 	                                       nullptr, col);
+}
+
+// column col of every face block dR_e/du_n (E x 4 x nb x nb; the BR2 residual couples the neighbour's whole
+// state through its lifted gradient, so the blocks are full), transposed ([e][lf][col][row]) or not
+hipError_t launch_ns_offd_col(Ctx& c, const double* u, double* blocks, int col, bool transposed)
+{
+	return launch_ns<Dual, false, kNsOffd>(c, u, blocks, c.metrics_vc, c.detJ_vc, c.nJ_fc, c.ops64, nullptr, 0,
+	                                       nullptr, col, nullptr, transposed ? 1 : 0);
 }
 
 }  // namespace dgx

@@ -253,6 +253,25 @@ class Oracle:
             B[:, col] = self.rhs_ns_complex(u, im)[1][e * nb:(e + 1) * nb] / h
         return B
 
+    def ns_face_block(self, u, e, lf, h=1e-30):
+        """The face block dR_e/du_n (nb x nb, rows and columns in the e x nv x Np order), n the neighbour across
+        local face lf of element e, by complex step, one of the neighbour's dofs at a time; zero on a boundary
+        face."""
+        nb = self.nv * self.Np
+        f = self.mesh.faces
+        hit = [(int(r[2]) if (r[0] == e and r[1] == lf) else int(r[0])) for r in f
+               if (r[0] == e and r[1] == lf) or (r[2] == e and r[3] == lf)]
+        B = np.zeros((nb, nb))
+        if not hit or hit[0] < 0:
+            return B
+        n = hit[0]
+        im = np.zeros(self.ndof)
+        for col in range(nb):
+            im[:] = 0.0
+            im[n * nb + col] = h
+            B[:, col] = self.rhs_ns_complex(u, im)[1][e * nb:(e + 1) * nb] / h
+        return B
+
     def ns_gradient(self, u):
         """(g, gv, gf): full and volume-only weak gradient coefficients E x d x nv x Np, face lifts
         F x 2 x d x nv x Np."""

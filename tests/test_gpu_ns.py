@@ -11,7 +11,7 @@
 * Boundary faces: the no-slip walls (adiabatic, diabatic, rotating) on the Taylor-Couette annulus and Riemann
   faces on a box equal the oracle to 1e-12 (FP64 residual, M^-1 R), p = 1..5, both fluxes; the FP32 and mixed
   forms stay within a multiple of the input-rounding floor.
-* Slip walls (an Euler condition) and the face blocks (no BR2 face-block linearisation) are refused with
+* Slip walls (an Euler condition) and the learned preconditioner (trained on Euler blocks) are refused with
   DGX_ERR_UNSUPPORTED.
 """
 import numpy as np
@@ -129,10 +129,8 @@ def test_ns_refuses_boundary_faces_and_unbuilt_paths(dev):
     mesh = _mesh(3, 2)
     ctx = _ctx(mesh, M.FLUX_ROE)
     u = torch.from_numpy(_state(mesh)).to(dev)
-    nb = ctx.Np * ctx.nv
-    out = torch.empty(mesh.E * 4 * nb * nb, dtype=torch.float64, device=dev)
     with pytest.raises(_lib.DgxError, match="UNSUPPORTED"):
-        ctx.assemble_face_blocks(u.data_ptr(), out.data_ptr())
+        ctx.solve_implicit(u.data_ptr(), precond=_lib.PC_LEARNED)
     ctx.close()
 
 

@@ -265,3 +265,28 @@ def test_ns_complex_step_without_viscosity_is_the_euler_one():
         ns = O.Oracle(mesh, M.PDE_EULER, flux, viscous=(0.0, PR)).jvp_ns(u, v)
         eu = O.Oracle(mesh, M.PDE_EULER, flux).rhs_complex(u, 1e-30 * v)[1] / 1e-30
         assert np.abs(ns - eu).max() <= 1e-13 * np.abs(eu).max()
+
+
+def test_ns_complex_step_blocks_rebuild_the_jacobian():
+    """The oracle's complex-step diagonal and face blocks (ns_diag_block, ns_face_block) are the whole BR2
+    Jacobian: D_e v_e + sum_faces B_e,n v_n equals the complex-step J v for every element (the BR2 residual of e
+    depends on e and its face neighbours only), on a curved periodic mesh and on the no-slip annulus."""
+    for name, mesh, flux, kw, u in (CS_CASES[0], CS_CASES[3]):
+        orc = O.Oracle(mesh, M.PDE_EULER, flux, viscous=(MU, PR), **kw)
+        nb = orc.nv * orc.Np
+        v = np.random.default_rng(6).standard_normal(u.size)
+        jv = orc.jvp_ns(u, v).reshape(mesh.E, nb)
+        V = v.reshape(mesh.E, nb)
+        nbr = {}
+        for eL, fL, eR, fR, _, _ in mesh.faces:
+            if eR >= 0:
+                nbr[(int(eL), int(fL))] = int(eR)
+                nbr[(int(eR), int(fR))] = int(eL)
+        for e in (0, mesh.E // 2, mesh.E - 1):
+            got = orc.ns_diag_block(u, e) @ V[e]
+            for lf in range(4):
+                if (e, lf) in nbr:
+                    got = got + orc.ns_face_block(u, e, lf) @ V[nbr[(e, lf)]]
+                else:
+                    assert not orc.ns_face_block(u, e, lf).any()
+            assert np.abs(got - jv[e]).max() <= 1e-12 * np.abs(jv[e]).max(), (name, e)
