@@ -336,6 +336,48 @@ def time_to_steady(torch, dev, epochs=60, cpu=True):
     return out
 
 
+def ns_time_to_steady(torch, dev, max_newton=150):
+    """Time to max|R| < 1e-10 (FP64) of the steady Navier-Stokes Taylor-Couette flow: the reference's annulus
+    (r 0.5 to 1, inner cylinder rotating at omega 1 with the no_slip_all wall, outer wall adiabatic), mu 1e-3 and
+    Pr 0.72 of its solution.data (Re about 250), BR2 with the Roe flux, DG p=3 on 16 x 64 curved QUADs (65,536
+    DOF), from the fluid at rest, pseudo-time Newton-GMRES(60) (rtol 1e-4) with cfl_initial 1 and the ramped CFL
+    capped at 300.  Legs: multicolour block SGS over the diagonal and full BR2 face blocks (FP64 and FP32-stored),
+    and block-Jacobi, each given at most max_newton Newton steps."""
+    import numpy as np
+
+    from dpgsolver_amd import _lib, mesh as M
+
+    mu, Pr = 1e-3, 0.72
+    mesh = M.annulus_mesh(3, 16, 64)
+    ctx = _lib.Context(2, 3, M.PDE_NAVIER_STOKES, M.FLUX_ROE, mu=mu, Pr=Pr, walls=M.taylor_couette_walls(mu, Pr))
+    ctx.set_stream(torch.cuda.current_stream(dev).cuda_stream)
+    ctx.upload_mesh(mesh.xyz, mesh.faces)
+    u0 = M.uniform_state(mesh, np.array([1.0, 0.0, 0.0, 1.0]))
+    kw = dict(cfl_initial=1.0, cfl_max=300.0, exit_tol=1e-10, max_newton=max_newton, gmres_restart=60,
+              gmres_rtol=1e-4, gmres_max_iter=1000, raise_on_error=False)
+
+    def solve(pc, **extra):
+        u = torch.from_numpy(u0).to(dev)
+        st = ctx.solve_implicit(u.data_ptr(), precond=pc, **{**kw, **extra})
+        return {"seconds": round(st["seconds"], 4), "newton_steps": st["newton_steps"],
+                "gmres_iters": st["gmres_iters"], "converged": bool(st["converged"]), "max_rhs": st["max_rhs"]}, u
+
+    solve(_lib.PC_BLOCK_SGS, max_newton=2)      # warm-up
+    sgs, u_sgs = solve(_lib.PC_BLOCK_SGS)
+    print(f"navier-stokes taylor-couette, block SGS: {sgs['seconds']:.3f} s, newton {sgs['newton_steps']}, "
+          f"gmres {sgs['gmres_iters']}", file=sys.stderr, flush=True)
+    s32, u_s32 = solve(_lib.PC_BLOCK_SGS32)
+    bj, _ = solve(_lib.PC_BLOCK_JACOBI)
+    print(f"navier-stokes taylor-couette, block-Jacobi: converged {bj['converged']}, max_rhs {bj['max_rhs']:.2e} "
+          f"after {bj['newton_steps']} steps", file=sys.stderr, flush=True)
+    ctx.close()
+    return {"workload": "2-D Navier-Stokes BR2 (Roe), Taylor-Couette annulus r 0.5-1, mu 1e-3, Pr 0.72, omega 1, "
+                        "DG p=3 on 16x64 curved QUADs (65536 DOF), from rest; Newton-GMRES(60) to 1e-10, CFL cap 300",
+            "block_sgs": sgs, "block_sgs_fp32_blocks": {**s32, "steady_states_rel_diff":
+                                                        ((u_s32 - u_sgs).abs().max() / u_sgs.abs().max()).item()},
+            "block_jacobi": bj}
+
+
 def ns_throughput(torch, dev, steps, warmup, n=448):
     """2-D Navier-Stokes (BR2) FP64 residual on the QUAD part of the mixed-mesh workload: periodic [0,1]^2,
     n x n elements, p = 3, Roe, Pr 0.72, Re 100 on the velocity amplitude 0.28 and unit length (mu = 0.0028);
@@ -736,7 +778,8 @@ def main():
 
     dev = torch.device("cuda:0")
     if args.implicit_only:
-        print(json.dumps({"time_to_1e-10": time_to_steady(torch, dev, cpu=not args.no_cpu)}), flush=True)
+        print(json.dumps({"time_to_1e-10": time_to_steady(torch, dev, cpu=not args.no_cpu),
+                          "navier_stokes_time_to_1e-10": ns_time_to_steady(torch, dev)}), flush=True)
         return
     if args.ns_only:
         print(json.dumps({"navier_stokes": ns_throughput(torch, dev, args.steps, args.warmup)}), flush=True)
@@ -865,6 +908,7 @@ def main():
     print(f"config-1 rk3: {c1['gpu_steps_per_s']} steps/s", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline()
     implicit = None if args.no_implicit else time_to_steady(torch, dev, cpu=not args.no_cpu)
+    ns_steady = None if args.no_implicit else ns_time_to_steady(torch, dev)
 
     out = {
         "metric": "GDOF/s residual eval (FP64 vs FP32-MFMA); time-to-1e-10 steady residual",
@@ -906,6 +950,7 @@ def main():
         "euler_variable_order": hpe,
         "cpu_baseline": cpu,
         "time_to_1e-10": implicit,
+        "navier_stokes_time_to_1e-10": ns_steady,
     }
     print(json.dumps(out), flush=True)
     for g in (g32, g64, gmx):

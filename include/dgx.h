@@ -309,7 +309,9 @@ dgx_status dgx_assemble_diag_blocks(dgx_ctx* ctx, const double* u, const double*
  * with the element's face nodes); they are computed compactly, nv*N dual-number launches, and scattered.
  * These are lhs_lr / lhs_rl of constructor_lhs_f_1_T (compute_face_rlhs_T.c:163-220, added at
  * compute_face_rlhs_dg.c:198-213); with the diagonal blocks they are the whole block-sparse Jacobian. */
-dgx_status dgx_assemble_face_blocks(dgx_ctx* ctx, const double* u, double* blocks);   /* not Navier-Stokes */
+/* Navier-Stokes: every row and column of a face block is non-zero (the neighbour's whole state enters the BR2
+ * residual through its lifted gradient, compute_volume_rlhs_dg.c:220-261); nb dual-number launches. */
+dgx_status dgx_assemble_face_blocks(dgx_ctx* ctx, const double* u, double* blocks);
 /* CFL-ramped pseudo-time step per element, returned as inv_dt[e] = max wave speed / (cfl * h_min), and for
  * Navier-Stokes the larger of that and mu / (cfl * h_min^2) (compute_dt_cfl_constrained /
  * compute_min_length_measure, solve_dg.c:284-379). */
@@ -335,8 +337,8 @@ dgx_status dgx_block_inv_apply(dgx_ctx* ctx, const double* inv, const double* x,
 
 /* Steady solve by pseudo-time Newton-GMRES (solve_implicit, solve_implicit.c:95-110, 232-281, 393-470).
  * Names mirror the reference's test_case.data keys (cfl_initial, exit_tol_i, exit_ratio_i) and the PETSc
- * options file (GMRES restart 60, rtol 1e-4).  u (device) is updated in place.  Navier-Stokes contexts:
- * DGX_PC_BLOCK_JACOBI or DGX_PC_NONE with the FP64 operator (others: DGX_ERR_UNSUPPORTED). */
+ * options file (GMRES restart 60, rtol 1e-4).  u (device) is updated in place.  Navier-Stokes contexts: the
+ * FP64 operator, any preconditioner but DGX_PC_LEARNED (block SGS over the full BR2 face blocks, nb <= 64). */
 /* Preconditioners of the GMRES solves.  DGX_PC_BLOCK_JACOBI: exact inverses of the element diagonal blocks of
  * dR/du - M/dt.  DGX_PC_BLOCK_SGS (nb <= 64): one forward and one backward multicolour block Gauss-Seidel sweep
  * over the diagonal and face blocks from z = 0, the coupled-block counterpart of the reference's default
