@@ -26,7 +26,7 @@ EXPORTS = [
     "dgx_graph_create_rhs", "dgx_graph_create_rk3", "dgx_graph_launch", "dgx_graph_sync", "dgx_graph_destroy",
     "dgx_set_rhs_grid", "dgx_autotune_rhs", "dgx_max_abs",
     "dgx_convert_f64_f32", "dgx_convert_f32_f64", "dgx_assemble_diag_blocks", "dgx_jacobian_vector",
-    "dgx_assemble_face_blocks", "dgx_newton_operator", "dgx_block_lu_factor", "dgx_block_lu_solve", "dgx_block_inverse", "dgx_block_inv_apply", "dgx_cfl_dt", "dgx_mass_sub", "dgx_precond_apply", "dgx_solve_implicit",
+    "dgx_assemble_face_blocks", "dgx_jacobian_bsr_pattern", "dgx_assemble_jacobian_bsr", "dgx_newton_operator", "dgx_block_lu_factor", "dgx_block_lu_solve", "dgx_block_inverse", "dgx_block_inv_apply", "dgx_cfl_dt", "dgx_mass_sub", "dgx_precond_apply", "dgx_solve_implicit",
     "dgx_mesh_read_gmsh", "dgx_mesh_free", "dgx_mesh_read_gmsh_mixed", "dgx_mesh_mixed_free",
     "dgx_restart_write_gmsh", "dgx_restart_read_solution",
     "dgx_mlp_create", "dgx_mlp_destroy", "dgx_mlp_sizes", "dgx_mlp_get_params", "dgx_mlp_set_params",
@@ -146,6 +146,8 @@ def load():
         "dgx_solve_implicit": [vp, vp, C.POINTER(ImplicitParams), C.POINTER(ImplicitStats)],
         "dgx_jacobian_vector": [vp, vp, vp, vp],
         "dgx_assemble_face_blocks": [vp, vp, vp],
+        "dgx_jacobian_bsr_pattern": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_longlong)],
+        "dgx_assemble_jacobian_bsr": [vp, vp, vp, vp],
         "dgx_newton_operator": [vp, vp, vp, vp, vp],
         "dgx_precond_apply": [vp, C.c_int, vp, vp, vp, vp],
         "dgx_mesh_read_gmsh": [C.c_char_p, C.c_int, C.c_void_p],
@@ -355,6 +357,23 @@ class Context:
 
     def assemble_face_blocks(self, u: int, blocks: int):
         check("dgx_assemble_face_blocks", self.L.dgx_assemble_face_blocks(self.h, vp(u), vp(blocks)))
+
+    def jacobian_bsr_pattern(self):
+        """(row_ptr, col_idx) of the block-sparse Newton matrix (numpy int32 arrays)."""
+        import numpy as np
+
+        n = C.c_longlong(0)
+        check("dgx_jacobian_bsr_pattern", self.L.dgx_jacobian_bsr_pattern(self.h, None, None, C.byref(n)))
+        rp = np.zeros(self.E + 1, dtype=np.int32)
+        ci = np.zeros(max(n.value, 1), dtype=np.int32)
+        check("dgx_jacobian_bsr_pattern",
+              self.L.dgx_jacobian_bsr_pattern(self.h, rp.ctypes.data_as(C.POINTER(C.c_int)),
+                                              ci.ctypes.data_as(C.POINTER(C.c_int)), C.byref(n)))
+        return rp, ci[: n.value]
+
+    def assemble_jacobian_bsr(self, u: int, mass_scale: int | None, values: int):
+        check("dgx_assemble_jacobian_bsr",
+              self.L.dgx_assemble_jacobian_bsr(self.h, vp(u), vp(mass_scale or 0), vp(values)))
 
     def cfl_dt(self, u: int, cfl: float, inv_dt: int):
         check("dgx_cfl_dt", self.L.dgx_cfl_dt(self.h, vp(u), cfl, vp(inv_dt)))

@@ -1474,6 +1474,103 @@ dgx_status dgx_assemble_face_blocks(dgx_ctx* h, const double* u, double* blocks)
 	return hip_status(e != hipSuccess ? e : es);
 }
 
+namespace {
+
+// the block pattern of the Newton matrix (constructor_nnz_dg_T, solve_dg_T.c:41-74): per element row, its own
+// column and its face neighbours', ascending and without repeats; diag[e] and face[lf * E + e] give each
+// element's and each local face's block slot (-1: a boundary face)
+dgx_status bsr_pattern(Ctx& c, std::vector<int>& row_ptr, std::vector<int>& col, std::vector<int>& diag,
+                       std::vector<int>& face)
+{
+	const int E = c.E, nf = 2 * c.d;
+	std::vector<dgx::ElemFace> ef((size_t)E * nf);
+	if (E > 0 && hipMemcpy(ef.data(), c.elem_faces, sizeof(dgx::ElemFace) * ef.size(), hipMemcpyDeviceToHost) != hipSuccess)
+		return DGX_ERR_HIP;
+	row_ptr.assign(E + 1, 0);
+	col.clear();
+	diag.assign(E, -1);
+	face.assign((size_t)nf * E, -1);
+	for (int e = 0; e < E; ++e) {
+		std::vector<int> cs{ e };
+		for (int lf = 0; lf < nf; ++lf)
+			if (ef[(size_t)e * nf + lf].en >= 0) cs.push_back(ef[(size_t)e * nf + lf].en);
+		std::sort(cs.begin(), cs.end());
+		cs.erase(std::unique(cs.begin(), cs.end()), cs.end());
+		const int base = (int)col.size();
+		for (int k = 0; k < (int)cs.size(); ++k) {
+			if (cs[k] == e) diag[e] = base + k;
+			col.push_back(cs[k]);
+		}
+		for (int lf = 0; lf < nf; ++lf) {
+			const int en = ef[(size_t)e * nf + lf].en;
+			if (en >= 0) face[(size_t)lf * E + e] = base + (int)(std::lower_bound(cs.begin(), cs.end(), en) - cs.begin());
+		}
+		row_ptr[e + 1] = (int)col.size();
+	}
+	return DGX_OK;
+}
+
+}  // namespace
+
+dgx_status dgx_jacobian_bsr_pattern(dgx_ctx* h, int* row_ptr, int* col_idx, long long* nnzb)
+{
+	if (!h || !nnzb) return DGX_ERR_ARG;
+	Ctx& c = h->c;
+	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
+	std::vector<int> rp, col, diag, face;
+	const dgx_status ps = bsr_pattern(c, rp, col, diag, face);
+	if (ps != DGX_OK) return ps;
+	*nnzb = (long long)col.size();
+	if (row_ptr) std::copy(rp.begin(), rp.end(), row_ptr);
+	if (col_idx) std::copy(col.begin(), col.end(), col_idx);
+	return DGX_OK;
+}
+
+dgx_status dgx_assemble_jacobian_bsr(dgx_ctx* h, const double* u, const double* mass_scale, double* values)
+{
+	if (!h || !u || !values) return DGX_ERR_ARG;
+	Ctx& c = h->c;
+	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
+	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
+	std::vector<int> rp, col, diag, face;
+	const dgx_status ps = bsr_pattern(c, rp, col, diag, face);
+	if (ps != DGX_OK) return ps;
+	const int E = c.E, nb = c.Np * c.nv, nb2 = nb * nb;
+	double *dblk = nullptr, *fblk = nullptr;
+	int* slots = nullptr;
+	dgx_status st = DGX_OK;
+	do {
+		if (hipMalloc(&dblk, sizeof(double) * (size_t)E * nb2) != hipSuccess ||
+		    hipMalloc(&fblk, sizeof(double) * (size_t)E * 4 * nb2) != hipSuccess ||
+		    hipMalloc(&slots, sizeof(int) * (size_t)5 * E) != hipSuccess) { st = DGX_ERR_HIP; break; }
+		if (hipMemcpyAsync(slots, diag.data(), sizeof(int) * E, hipMemcpyHostToDevice, c.stream) != hipSuccess ||
+		    hipMemcpyAsync(slots + E, face.data(), sizeof(int) * 4 * (size_t)E, hipMemcpyHostToDevice, c.stream) != hipSuccess) {
+			st = DGX_ERR_HIP;
+			break;
+		}
+		if ((st = dgx_assemble_diag_blocks(h, u, mass_scale, dblk)) != DGX_OK) break;
+		if ((st = dgx_assemble_face_blocks(h, u, fblk)) != DGX_OK) break;
+		// the face blocks are added into their slots: every slot starts at zero
+		for (size_t off = 0, n = col.size() * (size_t)nb2; off < n && st == DGX_OK; off += (size_t)1 << 28)
+			if (dgx::launch_fill_u64(c.stream, reinterpret_cast<unsigned long long*>(values + off),
+			                         (int)std::min(n - off, (size_t)1 << 28), 0ull) != hipSuccess)
+				st = DGX_ERR_HIP;
+		if (st != DGX_OK) break;
+		// each block once: the own block, then the faces one local face at a time (a neighbour reached through two
+		// faces, or the element itself on a periodic mesh one element wide, gets their sum)
+		if (dgx::launch_bsr_place(c.stream, dblk, (size_t)nb2, 0, slots, values, E, nb2, false) != hipSuccess) { st = DGX_ERR_HIP; break; }
+		for (int lf = 0; lf < 4 && st == DGX_OK; ++lf)
+			if (dgx::launch_bsr_place(c.stream, fblk, (size_t)4 * nb2, (size_t)lf * nb2, slots + (size_t)(lf + 1) * E, values,
+			                          E, nb2, true) != hipSuccess)
+				st = DGX_ERR_HIP;
+	} while (false);
+	if (hipStreamSynchronize(c.stream) != hipSuccess && st == DGX_OK) st = DGX_ERR_HIP;
+	if (dblk) (void)hipFree(dblk);
+	if (fblk) (void)hipFree(fblk);
+	if (slots) (void)hipFree(slots);
+	return st;
+}
+
 dgx_status dgx_cfl_dt(dgx_ctx* h, const double* u, double cfl, double* inv_dt)
 {
 	if (!h || !u || !inv_dt || !(cfl > 0.0)) return DGX_ERR_ARG;

@@ -346,6 +346,9 @@ hipError_t launch_diag_col(Ctx& c, const double* u, double* blocks, int col);
 // nv * N^(d-1), rows and columns on the face nodes (see MODE_OFFD)
 hipError_t launch_offd_col(Ctx& c, const double* u, double* blocks, int col);
 hipError_t launch_expand_face_blocks(Ctx& c, const double* cT, double* full);   // compact -> full nb x nb (2-D)
+// block-sparse assembly: values[slot[e]] (+)= src[e * stride + off] (slot < 0: skipped)
+hipError_t launch_bsr_place(hipStream_t s, const double* src, size_t stride, size_t off, const int* slot, double* vals,
+                            int E, int nb2, bool add);
 // jv = J v, or J v - mass_scale[e] M_e v when mass_scale is not NULL (the pseudo-time Newton operator)
 hipError_t launch_jvp(Ctx& c, const double* u, const double* v, double* jv, const double* mass_scale = nullptr);
 // the same in FP32 dual numbers (FP32 geometry and operators; FP64 u, v and jv)

@@ -661,6 +661,24 @@ hipError_t launch_expand_face_blocks(Ctx& c, const double* cT, double* full)
 	return hipGetLastError();
 }
 
+// values[slot[e]] = (or +=) src[e * stride + off], nb2 values per block, for every element with slot[e] >= 0
+__global__ void __launch_bounds__(256) k_bsr_place(const double* __restrict__ src, size_t stride, size_t off,
+                                                   const int* __restrict__ slot, double* vals, int nb2, int add)
+{
+	const int e = blockIdx.x, sl = slot[e];
+	if (sl < 0) return;
+	const double* a = src + (size_t)e * stride + off;
+	double* b = vals + (size_t)sl * nb2;
+	for (int i = threadIdx.x; i < nb2; i += blockDim.x) b[i] = add ? b[i] + a[i] : a[i];
+}
+hipError_t launch_bsr_place(hipStream_t s, const double* src, size_t stride, size_t off, const int* slot, double* vals,
+                            int E, int nb2, bool add)
+{
+	if (E == 0) return hipSuccess;
+	hipLaunchKernelGGL(k_bsr_place, dim3(E), dim3(256), 0, s, src, stride, off, slot, vals, nb2, add ? 1 : 0);
+	return hipGetLastError();
+}
+
 hipError_t launch_block_lu(hipStream_t s, double* blocks, int* piv, int E, int nb)
 {
 	if (E == 0) return hipSuccess;

@@ -312,6 +312,14 @@ dgx_status dgx_assemble_diag_blocks(dgx_ctx* ctx, const double* u, const double*
 /* Navier-Stokes: every row and column of a face block is non-zero (the neighbour's whole state enters the BR2
  * residual through its lifted gradient, compute_volume_rlhs_dg.c:220-261); nb dual-number launches. */
 dgx_status dgx_assemble_face_blocks(dgx_ctx* ctx, const double* u, double* blocks);
+/* The Newton matrix dR/du - mass_scale[e] M_e (mass_scale NULL: dR/du) in block-sparse row form, the sparsity of
+ * the reference's AIJ matrix (constructor_nnz_dg_T, solve_dg_T.c:41-74; filled by add_to_petsc_Mat, solve.c:321-338):
+ * block row e holds the element's own block and one block per face neighbour, columns ascending, nb = nv Np.
+ * dgx_jacobian_bsr_pattern: host arrays row_ptr[E + 1] and col_idx[nnzb] (either may be NULL to query nnzb).
+ * dgx_assemble_jacobian_bsr: device values[nnzb][nb][nb], row-major blocks in the pattern's order (2-D; the
+ * diagonal and face blocks above, a neighbour reached through two faces getting their sum); synchronous. */
+dgx_status dgx_jacobian_bsr_pattern(dgx_ctx* ctx, int* row_ptr, int* col_idx, long long* nnzb);
+dgx_status dgx_assemble_jacobian_bsr(dgx_ctx* ctx, const double* u, const double* mass_scale, double* values);
 /* CFL-ramped pseudo-time step per element, returned as inv_dt[e] = max wave speed / (cfl * h_min), and for
  * Navier-Stokes the larger of that and mu / (cfl * h_min^2) (compute_dt_cfl_constrained /
  * compute_min_length_measure, solve_dg.c:284-379). */

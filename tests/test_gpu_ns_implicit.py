@@ -194,6 +194,32 @@ def test_ns_face_blocks_match_complex_step(dev, case):
     ctx.close()
 
 
+@pytest.mark.parametrize("case", FACE_CASES[2:], ids=[c[0] for c in FACE_CASES[2:]])
+def test_ns_jacobian_bsr_is_the_newton_operator(dev, case):
+    name, mesh, flux, kw, u = case
+    ctx = _ctx(mesh, flux, **kw)
+    nb = ctx.Np * ctx.nv
+    rp, ci = ctx.jacobian_bsr_pattern()
+    ud = torch.from_numpy(u).to(dev)
+    sc = np.random.default_rng(3).uniform(0.5, 5.0, mesh.E)
+    sd = torch.from_numpy(sc).to(dev)
+    vals = torch.full((len(ci) * nb * nb,), float("nan"), dtype=torch.float64, device=dev)
+    ctx.assemble_jacobian_bsr(ud.data_ptr(), sd.data_ptr(), vals.data_ptr())
+    v = np.random.default_rng(4).standard_normal(u.size)
+    vd = torch.from_numpy(v).to(dev)
+    ref = torch.empty_like(ud)
+    ctx.newton_operator(ud.data_ptr(), sd.data_ptr(), vd.data_ptr(), ref.data_ptr())
+    torch.cuda.synchronize()
+    W = vals.cpu().numpy().reshape(-1, nb, nb)
+    V = v.reshape(mesh.E, nb)
+    got = np.zeros_like(V)
+    for e in range(mesh.E):
+        for k in range(rp[e], rp[e + 1]):
+            got[e] += W[k] @ V[ci[k]]
+    assert _rel(got.ravel(), ref.cpu().numpy()) < 1e-12
+    ctx.close()
+
+
 def _greedy_colours(mesh, nbr):
     """Smallest colour unused by the already coloured face neighbours, elements in index order."""
     col = [-1] * mesh.E

@@ -6,6 +6,7 @@
   test_integration_linearization.c:377, 4e-13).
 * Exact Jacobian-vector products against complex step of the full oracle residual.
 * Batched block LU against numpy; CFL dt against its closed form on a uniform state.
+* The block-sparse Newton matrix (the reference's AIJ pattern) applied to v equals the matrix-free operator.
 * A steady channel solve to 1e-10, whose converged state is re-checked by the oracle residual.
 """
 import numpy as np
@@ -213,6 +214,54 @@ def test_newton_operator_fuses_the_mass_term(dev, case):
     ctx.mass_sub(vd.data_ptr(), sd.data_ptr(), b.data_ptr())
     torch.cuda.synchronize()
     assert _rel(a.cpu().numpy(), b.cpu().numpy()) < 1e-13
+    ctx.close()
+
+
+def _bsr_apply(rp, ci, vals, v, nb):
+    V = v.reshape(-1, nb)
+    out = np.zeros_like(V)
+    for e in range(len(rp) - 1):
+        for k in range(rp[e], rp[e + 1]):
+            out[e] += vals[k] @ V[ci[k]]
+    return out.ravel()
+
+
+BSR_CASES = JAC + [("periodic_2x1_lf_p2", M.structured_tp_mesh(2, 2, (2, 1), (0, 0), (1, 1), (True, True)),
+                    M.FLUX_LF, None, None)]
+
+
+@pytest.mark.parametrize("case", BSR_CASES, ids=[c[0] for c in BSR_CASES])
+def test_jacobian_bsr_is_the_newton_operator(dev, case):
+    """dgx_assemble_jacobian_bsr in the pattern of dgx_jacobian_bsr_pattern (the reference's AIJ sparsity: each
+    element's own block and its face neighbours', columns ascending) applied to v equals J v - s_e M_e v
+    (dgx_newton_operator) and, without the mass term, J v; on a periodic mesh two elements wide and one high a
+    neighbour reached through two faces, and the element itself, get the summed blocks."""
+    name, mesh, flux, bcp, u = case
+    if u is None:
+        u = M.pack_state(M.prim_to_cons(M.smooth_state_3d(M.nodes_xyz(mesh)), 2))
+    ctx = _ctx(mesh, M.PDE_EULER, flux, bc_prim=bcp)
+    nb = ctx.Np * ctx.nv
+    rp, ci = ctx.jacobian_bsr_pattern()
+    nbr = _neighbours(mesh)
+    for e in range(mesh.E):
+        want = sorted({e} | {n for (ee, lf), n in nbr.items() if ee == e})
+        assert list(ci[rp[e]:rp[e + 1]]) == want
+    ud = torch.from_numpy(u).to(dev)
+    sc = np.random.default_rng(3).uniform(0.5, 5.0, mesh.E)
+    sd = torch.from_numpy(sc).to(dev)
+    vals = torch.full((len(ci) * nb * nb,), float("nan"), dtype=torch.float64, device=dev)
+    v = np.random.default_rng(4).standard_normal(u.size)
+    vd = torch.from_numpy(v).to(dev)
+    ref = torch.empty_like(ud)
+    for mass in (sd, None):
+        ctx.assemble_jacobian_bsr(ud.data_ptr(), mass.data_ptr() if mass is not None else None, vals.data_ptr())
+        if mass is not None:
+            ctx.newton_operator(ud.data_ptr(), sd.data_ptr(), vd.data_ptr(), ref.data_ptr())
+        else:
+            ctx.jacobian_vector(ud.data_ptr(), vd.data_ptr(), ref.data_ptr())
+        torch.cuda.synchronize()
+        got = _bsr_apply(rp, ci, vals.cpu().numpy().reshape(-1, nb, nb), v, nb)
+        assert _rel(got, ref.cpu().numpy()) < 1e-12, name
     ctx.close()
 
 
