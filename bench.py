@@ -513,7 +513,8 @@ def hp_ns_throughput(torch, dev, steps, warmup):
             "dtype": "f64", "value": round(ndof / (ms * 1e-3) / 1e9, 3), "unit": "GDOF/s",
             "ms_per_step": round(ms, 5),
             "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+                         "unit": "GB/s", "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "traffic": _traffic_from_profiles("hp_ns_fp64", "variable_order_500x300_traffic.md")},
             "fp32": {"value": round(ndof / (ms32 * 1e-3) / 1e9, 3), "ms_per_step": round(ms32, 5),
                      "rel_err_vs_fp64": err32, "input_rounding_rel_err": err_in,
                      "roofline": {"bound": "hbm", "achieved": round(alg32 / (ms32 * 1e-3) / 1e9, 1),
@@ -632,7 +633,8 @@ def mixed_throughput(torch, dev, steps, warmup, pde):
             "value": round(ndof / (ms * 1e-3) / 1e9, 3), "unit": "GDOF/s", "ms_per_step": round(ms, 5),
             "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                         "traffic": None},
+                         "traffic": _traffic_from_profiles("mixed_ns_fp64" if ns else "mixed_euler_fp64",
+                                                           "mixed_tri_quad_500x300_traffic.md")},
             # FP32 state, geometry shadows and operators: the same values in half the bytes
             "fp32": {"value": round(ndof / (ms32 * 1e-3) / 1e9, 3), "ms_per_step": round(ms32, 5),
                      "rel_err_vs_fp64": err32, "input_rounding_rel_err": err_in,
