@@ -1444,7 +1444,7 @@ dgx_status dgx_assemble_diag_blocks(dgx_ctx* h, const double* u, const double* m
 	if (!h || !u || !blocks) return DGX_ERR_ARG;
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
-	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
+	if (c.d != 2 && c.d != 3) return DGX_ERR_UNSUPPORTED;   // 3-D: Euler and advection
 	const int nb = c.Np * c.nv;
 	for (int col = 0; col < nb; ++col) DGX_TRY(dgx::launch_diag_col(c, u, blocks, col));
 	if (mass_scale) DGX_TRY(dgx::launch_sub_mass(c, blocks, mass_scale, 0.0));
@@ -1576,7 +1576,7 @@ dgx_status dgx_cfl_dt(dgx_ctx* h, const double* u, double cfl, double* inv_dt)
 	if (!h || !u || !inv_dt || !(cfl > 0.0)) return DGX_ERR_ARG;
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
-	if (c.d != 2 || c.nv != 4) return DGX_ERR_UNSUPPORTED;
+	if (c.nv != c.d + 2) return DGX_ERR_UNSUPPORTED;   // Euler and Navier-Stokes, 2-D and 3-D
 	return hip_status(dgx::launch_cfl_dt(c, u, cfl, inv_dt));
 }
 
@@ -1585,7 +1585,7 @@ dgx_status dgx_mass_sub(dgx_ctx* h, const double* x, const double* scale, double
 	if (!h || !x || !y) return DGX_ERR_ARG;
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
-	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
+	if (c.d != 2 && c.d != 3) return DGX_ERR_UNSUPPORTED;
 	return hip_status(dgx::launch_mass_sub(c, x, scale, y));
 }
 
@@ -1594,7 +1594,7 @@ dgx_status dgx_newton_operator(dgx_ctx* h, const double* u, const double* mass_s
 	if (!h || !u || !mass_scale || !v || !out) return DGX_ERR_ARG;
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
-	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
+	if (c.d != 2 && c.d != 3) return DGX_ERR_UNSUPPORTED;
 	return hip_status(dgx::launch_jvp(c, u, v, out, mass_scale));
 }
 
@@ -1603,7 +1603,7 @@ dgx_status dgx_jacobian_vector(dgx_ctx* h, const double* u, const double* v, dou
 	if (!h || !u || !v || !jv) return DGX_ERR_ARG;
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
-	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
+	if (c.d != 2 && c.d != 3) return DGX_ERR_UNSUPPORTED;
 	return hip_status(dgx::launch_jvp(c, u, v, jv));
 }
 

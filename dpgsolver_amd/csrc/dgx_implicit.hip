@@ -391,6 +391,7 @@ struct Blas {
 	}
 };
 
+// one workgroup per element, a lane per node (HEX p = 4: 128 lanes for its 125 nodes)
 #define DGX_PER_ELEM2(KNAME, BLK, ...)                                                               \
 	switch (c.d * 16 + c.N) {                                                                       \
 	case 2 * 16 + 2: hipLaunchKernelGGL((KNAME<2, 2 __VA_ARGS__>), dim3(c.E), dim3(BLK), 0, c.stream, ARGS); break; \
@@ -398,23 +399,28 @@ struct Blas {
 	case 2 * 16 + 4: hipLaunchKernelGGL((KNAME<2, 4 __VA_ARGS__>), dim3(c.E), dim3(BLK), 0, c.stream, ARGS); break; \
 	case 2 * 16 + 5: hipLaunchKernelGGL((KNAME<2, 5 __VA_ARGS__>), dim3(c.E), dim3(BLK), 0, c.stream, ARGS); break; \
 	case 2 * 16 + 6: hipLaunchKernelGGL((KNAME<2, 6 __VA_ARGS__>), dim3(c.E), dim3(BLK), 0, c.stream, ARGS); break; \
+	case 3 * 16 + 2: hipLaunchKernelGGL((KNAME<3, 2 __VA_ARGS__>), dim3(c.E), dim3(64), 0, c.stream, ARGS); break;  \
+	case 3 * 16 + 3: hipLaunchKernelGGL((KNAME<3, 3 __VA_ARGS__>), dim3(c.E), dim3(64), 0, c.stream, ARGS); break;  \
+	case 3 * 16 + 4: hipLaunchKernelGGL((KNAME<3, 4 __VA_ARGS__>), dim3(c.E), dim3(64), 0, c.stream, ARGS); break;  \
+	case 3 * 16 + 5: hipLaunchKernelGGL((KNAME<3, 5 __VA_ARGS__>), dim3(c.E), dim3(128), 0, c.stream, ARGS); break; \
 	default: return hipErrorInvalidValue;                                                       \
 	}
 
 hipError_t launch_mass_sub(Ctx& c, const double* x, const double* scale, double* y)
 {
 	if (c.E == 0) return hipSuccess;
-	if (c.d != 2) return hipErrorInvalidValue;
+	if (c.d != 2 && c.d != 3) return hipErrorInvalidValue;
 #define ARGS x, c.detJ_vc, scale, c.ops64, c.E, y
 	if (c.nv == 1) { DGX_PER_ELEM2(k_mass_sub, 64, , 1) }
-	else { DGX_PER_ELEM2(k_mass_sub, 64, , 4) }
+	else if (c.nv == 4) { DGX_PER_ELEM2(k_mass_sub, 64, , 4) }
+	else { DGX_PER_ELEM2(k_mass_sub, 64, , 5) }
 #undef ARGS
 	return hipGetLastError();
 }
 
 hipError_t launch_cfl_dt(Ctx& c, const double* u, double cfl, double* inv_dt)
 {
-	if (c.d != 2 || c.nv != 4) return hipErrorInvalidValue;
+	if (c.nv != c.d + 2) return hipErrorInvalidValue;   // Euler and Navier-Stokes
 #define ARGS u, c.hmin, c.ops64, c.E, cfl, (c.cfg.pde == DGX_PDE_NAVIER_STOKES ? c.cfg.mu : 0.0), inv_dt
 	DGX_PER_ELEM2(k_cfl_dt, 64, )
 #undef ARGS
@@ -423,7 +429,7 @@ hipError_t launch_cfl_dt(Ctx& c, const double* u, double cfl, double* inv_dt)
 
 hipError_t launch_relaxed_update(Ctx& c, double* u, const double* du, int* status)
 {
-	if (c.d != 2 || c.nv != 4) return hipErrorInvalidValue;
+	if (c.nv != c.d + 2) return hipErrorInvalidValue;
 #define ARGS u, du, c.ops64, c.E, status, (double*)nullptr
 	DGX_PER_ELEM2(k_relaxed_update, 64, )
 #undef ARGS
@@ -839,15 +845,26 @@ struct Gmres {
 
 using dgx::Ctx;
 
+// 3-D (HEX Euler): the FP64 J v, and no preconditioner or block-Jacobi with blocks the batched LU takes
+// (nb = 5 (p + 1)^3 <= 90: p = 1); the face blocks, and so block SGS, are 2-D
+static bool implicit_3d_ok(const Ctx& c, int precond, int jvp_fp32)
+{
+	if (c.d != 3) return true;
+	if (jvp_fp32) return false;
+	if (precond == DGX_PC_NONE) return true;
+	return precond == DGX_PC_BLOCK_JACOBI && c.Np * c.nv <= 90;
+}
+
 extern "C" dgx_status dgx_precond_apply(dgx_ctx* h, int precond, const double* u, const double* inv_dt,
                                         const double* v, double* z)
 {
 	if (!h || !u || !inv_dt || !v || !z || v == z) return DGX_ERR_ARG;
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
-	if (c.d != 2 || c.cfg.pde == DGX_PDE_ADVECTION) return DGX_ERR_UNSUPPORTED;
+	if ((c.d != 2 && c.d != 3) || c.cfg.pde == DGX_PDE_ADVECTION) return DGX_ERR_UNSUPPORTED;
 	// Navier-Stokes: no learned preconditioner (it is trained on Euler blocks)
 	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES && precond == DGX_PC_LEARNED) return DGX_ERR_UNSUPPORTED;
+	if (!implicit_3d_ok(c, precond, 0)) return DGX_ERR_UNSUPPORTED;
 	dgx::Precond* P = nullptr;
 	if (precond == DGX_PC_NONE) P = new dgx::PrecondIdentity();
 	else if (precond == DGX_PC_BLOCK_JACOBI) P = new dgx::PrecondBlockJacobi();
@@ -868,10 +885,11 @@ extern "C" dgx_status dgx_solve_implicit(dgx_ctx* h, double* u, const dgx_implic
 	if (!h || !u || !prm || !st) return DGX_ERR_ARG;
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
-	if (c.d != 2 || c.cfg.pde == DGX_PDE_ADVECTION) return DGX_ERR_UNSUPPORTED;
+	if ((c.d != 2 && c.d != 3) || c.cfg.pde == DGX_PDE_ADVECTION) return DGX_ERR_UNSUPPORTED;
 	// Navier-Stokes: the FP64 Jacobian-vector product, no learned preconditioner
 	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES && (prm->precond == DGX_PC_LEARNED || prm->jvp_fp32))
 		return DGX_ERR_UNSUPPORTED;
+	if (!implicit_3d_ok(c, prm->precond, prm->jvp_fp32)) return DGX_ERR_UNSUPPORTED;
 	if (prm->gmres_restart < 1 || prm->gmres_restart > 200 || prm->max_newton < 1 || !(prm->cfl_initial > 0.0))
 		return DGX_ERR_ARG;
 	std::memset(st, 0, sizeof(*st));

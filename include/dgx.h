@@ -294,7 +294,9 @@ dgx_status dgx_max_abs(dgx_ctx* ctx, const double* x, long long n, double* out);
 dgx_status dgx_convert_f64_f32(dgx_ctx* ctx, const double* src, float* dst, long long n);
 dgx_status dgx_convert_f32_f64(dgx_ctx* ctx, const float* src, double* dst, long long n);
 
-/* Implicit path (2-D QUAD contexts of one order: advection, Euler, and Navier-Stokes where stated).
+/* Implicit path (contexts of one order: 2-D QUAD advection, Euler, and Navier-Stokes where stated; 3-D HEX Euler
+ * for J v, the Newton operator, the diagonal blocks, the pseudo-time step and the solve with block-Jacobi up to
+ * nb = 90 or no preconditioner; the face blocks are 2-D).
  * Diagonal blocks A_e = dR_e/du_e - mass_scale[e] * M_e (mass_scale: device array of E, or NULL for none), (Np*nv)^2 each, row-major, row eq*Np+node,
  * column var*Np+node: volume lhs_v_1 + face lhs_ll/lhs_rr contributions + CFL term
  * (constructor_lhs_v_1_T compute_volume_rlhs_T.c:179-220, constructor_lhs_f_1_T
