@@ -495,38 +495,40 @@ __global__ void __launch_bounds__(64) k_bgs_color(const int* __restrict__ elems,
 }
 
 // The same colour sweep with full face blocks (the BR2 residual: the neighbour's whole state enters through its
-// lifted gradient): offT holds dR_e/du_n transposed per local face, [e][lf][col][row], nb <= 64.
+// lifted gradient): offT holds dR_e/du_n transposed per local face, [e][lf][col][row], nb <= 64.  Four wavefronts
+// per element, one per local face for the face products and one quarter of the columns each for the inverse, so
+// that four times as many loads are in flight as with one wavefront (the sweep reads 4 nb^2 + nb^2 values per
+// element and is bound by load latency, not bytes).
 template <typename TB>
-__global__ void __launch_bounds__(64) k_bgs_full_color(const int* __restrict__ elems, const ElemFace* __restrict__ ef,
-                                                       const TB* __restrict__ invT, const TB* __restrict__ offT,
-                                                       const double* __restrict__ rhs, double* z, int nb)
+__global__ void __launch_bounds__(256) k_bgs_full_color(const int* __restrict__ elems, const ElemFace* __restrict__ ef,
+                                                        const TB* __restrict__ invT, const TB* __restrict__ offT,
+                                                        const double* __restrict__ rhs, double* z, int nb)
 {
-	__shared__ double sz[4][64], sr[64];
-	const int e = elems[blockIdx.x], t = threadIdx.x;
-#pragma unroll
-	for (int lf = 0; lf < 4; ++lf) {
+	__shared__ double sz[4][64], sf[4][64], sr[64];
+	const int e = elems[blockIdx.x], t = threadIdx.x, lf = t >> 6, row = t & 63;
+	{
 		const ElemFace f = ef[(size_t)e * 4 + lf];
-		sz[lf][t] = (f.en >= 0 && t < nb) ? z[(size_t)f.en * nb + t] : 0.0;
+		sz[lf][row] = (f.en >= 0 && row < nb) ? z[(size_t)f.en * nb + row] : 0.0;
 	}
 	__syncthreads();
-	if (t < nb) {
-		double r = rhs[(size_t)e * nb + t];
-#pragma unroll
-		for (int lf = 0; lf < 4; ++lf) {
-			const TB* C = offT + ((size_t)e * 4 + lf) * nb * nb;
-			double acc = 0.0;
-			for (int c = 0; c < nb; ++c) acc += (double)C[(size_t)c * nb + t] * sz[lf][c];
-			r -= acc;
-		}
-		sr[t] = r;
-	}
-	__syncthreads();
-	if (t < nb) {
-		const TB* M = invT + (size_t)e * nb * nb;
+	if (row < nb) {   // face lf's product, row `row`
+		const TB* C = offT + ((size_t)e * 4 + lf) * nb * nb;
 		double acc = 0.0;
-		for (int k = 0; k < nb; ++k) acc += (double)M[(size_t)k * nb + t] * sr[k];
-		z[(size_t)e * nb + t] = acc;
+		for (int c = 0; c < nb; ++c) acc += (double)C[(size_t)c * nb + row] * sz[lf][c];
+		sf[lf][row] = acc;
 	}
+	__syncthreads();
+	if (t < nb) sr[t] = rhs[(size_t)e * nb + t] - ((sf[0][t] + sf[1][t]) + (sf[2][t] + sf[3][t]));
+	__syncthreads();
+	if (row < nb) {   // quarter lf of the columns of A_ee^-1 r
+		const TB* M = invT + (size_t)e * nb * nb;
+		const int k0 = lf * ((nb + 3) / 4), k1 = min(nb, k0 + (nb + 3) / 4);
+		double acc = 0.0;
+		for (int k = k0; k < k1; ++k) acc += (double)M[(size_t)k * nb + row] * sr[k];
+		sf[lf][row] = acc;
+	}
+	__syncthreads();
+	if (t < nb) z[(size_t)e * nb + t] = (sf[0][t] + sf[1][t]) + (sf[2][t] + sf[3][t]);
 }
 
 struct PrecondIdentity : Precond {
@@ -626,10 +628,10 @@ struct PrecondBlockSGS : Precond {
 		if (full(c)) {
 			const int nb = c.Np * c.nv;
 			if (fp32)
-				hipLaunchKernelGGL(k_bgs_full_color<float>, dim3(cnt), dim3(64), 0, c.stream, c.color_elems + n0,
+				hipLaunchKernelGGL(k_bgs_full_color<float>, dim3(cnt), dim3(256), 0, c.stream, c.color_elems + n0,
 				                   c.elem_faces, inv32, off32, v, z, nb);
 			else
-				hipLaunchKernelGGL(k_bgs_full_color<double>, dim3(cnt), dim3(64), 0, c.stream, c.color_elems + n0,
+				hipLaunchKernelGGL(k_bgs_full_color<double>, dim3(cnt), dim3(256), 0, c.stream, c.color_elems + n0,
 				                   c.elem_faces, diag.blocks, offT, v, z, nb);
 			return hipGetLastError();
 		}
