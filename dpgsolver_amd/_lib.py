@@ -46,7 +46,8 @@ class DgxError(RuntimeError):
 class DgxConfig(C.Structure):
     _fields_ = [("d", C.c_int), ("p", C.c_int), ("pde", C.c_int), ("num_flux", C.c_int),
                 ("b_adv", C.c_double * 3), ("bc_prim", C.c_double * 5), ("mu", C.c_double), ("Pr", C.c_double),
-                ("wall_nf_E", C.c_double), ("wall_omega", C.c_double), ("wall_rho", C.c_double), ("wall_E", C.c_double)]
+                ("wall_nf_E", C.c_double), ("wall_omega", C.c_double), ("wall_rho", C.c_double), ("wall_E", C.c_double),
+                ("viscosity_type", C.c_int), ("r_s", C.c_double)]
 
 
 class DgxHostMesh(C.Structure):
@@ -186,13 +187,15 @@ class Context:
     """One DG discretisation on the device (owns geometry; state buffers are the caller's)."""
 
     def __init__(self, d: int, p: int, pde: int, num_flux: int, b_adv=(0.0, 0.0, 0.0), bc_prim=None,
-                 mu: float = 0.0, Pr: float = 0.72, walls=None):
-        """pde PDE_NAVIER_STOKES: constant viscosity mu and Prandtl number Pr (2-D); walls = (nf_E, omega, rho_b,
-        E_b), the no-slip wall data (dgx_config wall_nf_E, wall_omega, wall_rho, wall_E)."""
+                 mu: float = 0.0, Pr: float = 0.72, walls=None, sutherland: bool = False, r_s: float = 1.0):
+        """pde PDE_NAVIER_STOKES: viscosity mu (constant, or with sutherland the reference value of Sutherland's
+        formula at T = p / (rho r_s)) and Prandtl number Pr (2-D); walls = (nf_E, omega, rho_b, E_b), the no-slip
+        wall data (dgx_config wall_nf_E, wall_omega, wall_rho, wall_E)."""
         L = load()
         cfg = DgxConfig()
         cfg.d, cfg.p, cfg.pde, cfg.num_flux = d, p, pde, num_flux
         cfg.mu, cfg.Pr = float(mu), float(Pr)
+        cfg.viscosity_type, cfg.r_s = (1 if sutherland else 0), float(r_s)
         if walls is not None:
             cfg.wall_nf_E, cfg.wall_omega, cfg.wall_rho, cfg.wall_E = (float(w) for w in walls)
         for k, v in enumerate(b_adv):

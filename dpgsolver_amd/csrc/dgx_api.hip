@@ -108,6 +108,8 @@ dgx_status dgx_create(dgx_ctx** out, const dgx_config* cfg)
 	if (cfg->pde == DGX_PDE_NAVIER_STOKES) {
 		if (cfg->d != 2) return DGX_ERR_UNSUPPORTED;
 		if (!(cfg->mu >= 0.0) || !(cfg->Pr > 0.0)) return DGX_ERR_ARG;
+		if (cfg->viscosity_type != DGX_VISC_CONSTANT && cfg->viscosity_type != DGX_VISC_SUTHERLAND) return DGX_ERR_ARG;
+		if (cfg->viscosity_type == DGX_VISC_SUTHERLAND && !(cfg->r_s >= 0.0)) return DGX_ERR_ARG;
 	}
 	if (cfg->pde == DGX_PDE_ADVECTION && cfg->num_flux != DGX_FLUX_UPWIND) return DGX_ERR_ARG;
 	if (!supported(cfg->d, cfg->p, cfg->pde)) return DGX_ERR_UNSUPPORTED;
@@ -1026,8 +1028,15 @@ static dgx_status upload_hp_impl(dgx_ctx* h, int E, int F, const int* type, cons
 	return DGX_OK;
 }
 
+// Sutherland's viscosity is built for the uniform-order QUAD kernels only
+static bool sutherland_ns(const dgx_ctx* h)
+{
+	return h && h->c.cfg.pde == DGX_PDE_NAVIER_STOKES && h->c.cfg.viscosity_type == DGX_VISC_SUTHERLAND;
+}
+
 dgx_status dgx_upload_mesh_hp(dgx_ctx* h, const dgx_mesh_hp* m)
 {
+	if (sutherland_ns(h)) return DGX_ERR_UNSUPPORTED;
 	const dgx_status st = m ? upload_hp_impl(h, m->E, m->F, nullptr, m->p, m->xyz, m->faces) : DGX_ERR_ARG;
 	// a failure after the previous mesh was released leaves no half-built variable-order state behind
 	if (st != DGX_OK && h && h->c.hp && !h->c.hp->ready) free_mesh(h->c);
@@ -1036,6 +1045,7 @@ dgx_status dgx_upload_mesh_hp(dgx_ctx* h, const dgx_mesh_hp* m)
 
 dgx_status dgx_upload_mesh_mixed(dgx_ctx* h, const dgx_mesh_mixed* m)
 {
+	if (sutherland_ns(h)) return DGX_ERR_UNSUPPORTED;
 	const dgx_status st = (m && m->type) ? upload_hp_impl(h, m->E, m->F, m->type, m->p, m->xyz, m->faces)
 	                                     : DGX_ERR_ARG;
 	if (st != DGX_OK && h && h->c.hp && !h->c.hp->ready) free_mesh(h->c);

@@ -362,6 +362,20 @@ __device__ inline void ns_viscous_flux2(const S* s, const S* g, S mu, S Pr, S* f
 	}
 }
 
+// Viscosity at a state: constant mu, or Sutherland's formula mu_s = 8e-2 mu sqrt(T) / (1 + 112 / T) with
+// T = p / (rho r_s) (compute_mu_sutherland_T, solution_navier_stokes_T.c:246-255; C1_SUTHERLAND, C2_SUTHERLAND,
+// solution_navier_stokes.h:30-33)
+template <typename S>
+__device__ inline S ns_mu(const S* s, S mu, int sutherland, double r_s)
+{
+	if (!sutherland) return mu;
+	const S ri = trcp(s[0]);
+	const S V2 = (s[1] * s[1] + s[2] * s[2]) * ri * ri;
+	const S p = S(kGM1) * (s[3] - S(0.5) * s[0] * V2);
+	const S T = p * ri * S(1.0 / r_s);
+	return S(8e-2) * mu * tsqrt(T) * trcp(S(1.0) + S(112.0) * trcp(T));
+}
+
 // Navier-Stokes wall data: the provided normal viscous energy flux of the no-slip flux walls, and the rotating
 // wall's angular velocity, density and total energy (dgx_config wall_*)
 struct NsWalls {

@@ -81,10 +81,11 @@ __global__ void __launch_bounds__(128) k_mass_sub(const double* __restrict__ x, 
 // bound with mu > 0, the Navier-Stokes constant viscosity; solve_dg.c:284-319)
 template <int D, int N>
 __global__ void __launch_bounds__(128) k_cfl_dt(const double* __restrict__ u, const double* __restrict__ hmin,
-                                                const Ops1D<double>* ops, int E, double cfl, double mu, double* inv_dt)
+                                                const Ops1D<double>* ops, int E, double cfl, double mu,
+                                                int sutherland, double r_s, double* inv_dt)
 {
 	constexpr int Np = ipow(N, D), NV = D + 2;
-	__shared__ double B0[NV * Np], B1[NV * Np], sBinv[N * N], sWs[Np];
+	__shared__ double B0[NV * Np], B1[NV * Np], sBinv[N * N], sWs[Np], sMu[Np];
 	const int e = blockIdx.x, n = threadIdx.x;
 	for (int i = threadIdx.x; i < N * N; i += blockDim.x) sBinv[i] = ops->Binv[i];
 	for (int i = threadIdx.x; i < NV * Np; i += blockDim.x) B0[i] = u[(size_t)e * NV * Np + i];
@@ -103,12 +104,18 @@ __global__ void __launch_bounds__(128) k_cfl_dt(const double* __restrict__ u, co
 		for (int k = 0; k < D; ++k) { const double w = cur[(1 + k) * Np + n] * ri; V2 += w * w; }
 		const double p = kGM1 * (cur[(D + 1) * Np + n] - 0.5 * rho * V2);
 		sWs[n] = sqrt(V2) + sqrt(kGamma * p * ri);
+		// the largest viscosity over the coefficients (compute_viscosity, solve_dg.c:300-307)
+		const double T = p * ri / r_s;
+		sMu[n] = sutherland ? 8e-2 * mu * sqrt(T) / (1.0 + 112.0 / T) : mu;
 	}
 	__syncthreads();
 	if (threadIdx.x == 0) {
-		double m = 0.0;
-		for (int i = 0; i < Np; ++i) m = fmax(m, sWs[i]);
-		inv_dt[e] = fmax(m / (cfl * hmin[e]), mu / (cfl * hmin[e] * hmin[e]));
+		double m = 0.0, mmu = 0.0;
+		for (int i = 0; i < Np; ++i) {
+			m = fmax(m, sWs[i]);
+			mmu = fmax(mmu, sMu[i]);
+		}
+		inv_dt[e] = fmax(m / (cfl * hmin[e]), mmu / (cfl * hmin[e] * hmin[e]));
 	}
 }
 
@@ -421,7 +428,9 @@ hipError_t launch_mass_sub(Ctx& c, const double* x, const double* scale, double*
 hipError_t launch_cfl_dt(Ctx& c, const double* u, double cfl, double* inv_dt)
 {
 	if (c.nv != c.d + 2) return hipErrorInvalidValue;   // Euler and Navier-Stokes
-#define ARGS u, c.hmin, c.ops64, c.E, cfl, (c.cfg.pde == DGX_PDE_NAVIER_STOKES ? c.cfg.mu : 0.0), inv_dt
+#define ARGS u, c.hmin, c.ops64, c.E, cfl, (c.cfg.pde == DGX_PDE_NAVIER_STOKES ? c.cfg.mu : 0.0),                   \
+             (c.cfg.pde == DGX_PDE_NAVIER_STOKES && c.cfg.viscosity_type == DGX_VISC_SUTHERLAND) ? 1 : 0,          \
+             (c.cfg.r_s > 0.0 ? c.cfg.r_s : 1.0), inv_dt
 	DGX_PER_ELEM2(k_cfl_dt, 64, )
 #undef ARGS
 	return hipGetLastError();

@@ -62,7 +62,15 @@ typedef struct dgx_config {
 	                       (x, y) is omega (-y, x) */
 	double wall_rho;    /* DGX_BC_NOSLIP_ROTATING: wall density and total energy (rho_b, E_b) */
 	double wall_E;
+	int viscosity_type; /* DGX_VISC_CONSTANT (0: mu) or DGX_VISC_SUTHERLAND (solution.data viscosity_type) */
+	double r_s;         /* DGX_VISC_SUTHERLAND: specific gas constant of T = p / (rho r_s) (solution.data r_s;
+	                       0 means 1) */
 } dgx_config;
+/* Viscosity of the Navier-Stokes flux: constant mu (compute_mu_constant_T), or Sutherland's formula
+ * mu = 8e-2 mu sqrt(T) / (1 + 112 / T) (compute_mu_sutherland_T, solution_navier_stokes_T.c:246-255; uniform-order
+ * QUAD meshes, FP64 / FP32 / mixed residuals and the linearisation; variable-order and mixed TRI/QUAD meshes
+ * refuse it with DGX_ERR_UNSUPPORTED). */
+enum { DGX_VISC_CONSTANT = 0, DGX_VISC_SUTHERLAND = 1 };
 
 typedef struct dgx_mesh {
 	int E;              /* elements (QUAD/HEX, one p) */

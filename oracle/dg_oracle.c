@@ -40,6 +40,8 @@ struct or_ctx {
 	double wall_nfE;     /* no-slip flux walls: the normal viscous energy flux nf_E (0 adiabatic) */
 	double wall_omega;   /* rotating no-slip walls: angular velocity, and the wall density and total energy */
 	double wall_rho, wall_E;
+	int sutherland;      /* viscosity by Sutherland's formula (or_set_sutherland) */
+	double r_s;
 };
 
 /* ------------------------------------------------------------------------------------------------ */
@@ -715,6 +717,12 @@ void or_set_walls(or_ctx* c, double nf_E, double omega, double rho_b, double E_b
 	c->wall_E = E_b;
 }
 
+void or_set_sutherland(or_ctx* c, double r_s)
+{
+	c->sutherland = 1;
+	c->r_s = r_s;
+}
+
 static int ns_bc_supported(int bc)
 {
 	return bc == OR_BC_RIEMANN || bc == OR_BC_NOSLIP_ADIABATIC || bc == OR_BC_NOSLIP_DIABATIC || bc == OR_BC_NOSLIP_ROTATING;
@@ -992,8 +1000,21 @@ void or_cfl_inv_dt(const or_ctx* c, const double* u, double cfl, double* inv_dt)
 				if (sqrt(s2) < hmin) hmin = sqrt(s2);
 			}
 		inv_dt[e] = m / (cfl * hmin);
-		/* Navier-Stokes: the viscous bound dt <= cfl h^2 / mu (solve_dg.c:300-318) */
-		if (c->viscous && c->mu / (cfl * hmin * hmin) > inv_dt[e]) inv_dt[e] = c->mu / (cfl * hmin * hmin);
+		/* Navier-Stokes: the viscous bound dt <= cfl h^2 / mu_max, mu_max over the Bezier coefficients
+		 * (solve_dg.c:300-318) */
+		if (c->viscous) {
+			double mmu = c->mu;
+			if (c->sutherland) {
+				mmu = 0.0;
+				for (int i = 0; i < Np; ++i) {
+					double s[5];
+					for (int v = 0; v < nv; ++v) s[v] = b[v * Np + i];
+					const double m = mu_of_d(c, s);
+					if (m > mmu) mmu = m;
+				}
+			}
+			if (mmu / (cfl * hmin * hmin) > inv_dt[e]) inv_dt[e] = mmu / (cfl * hmin * hmin);
+		}
 	}
 	free(b);
 }

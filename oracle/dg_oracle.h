@@ -90,6 +90,9 @@ void or_set_viscous(or_ctx* c, double mu, double Pr);
 /* wall data of the no-slip conditions: nf_E the normal viscous energy flux of the adiabatic (0) and diabatic
  * walls (get_normal_flux_Energy: -mu Cp / Pr dT/dn); omega, rho_b, E_b of the rotating wall (solution.data) */
 void or_set_walls(or_ctx* c, double nf_E, double omega, double rho_b, double E_b);
+/* viscosity by Sutherland's formula mu_s = 8e-2 mu sqrt(T) / (1 + 112 / T), T = p / (rho r_s), instead of the
+ * constant mu (compute_mu_sutherland_T, solution_navier_stokes_T.c:246-255) */
+void or_set_sutherland(or_ctx* c, double r_s);
 /* negated viscous flux fv[dim + d*eq] for state s[nv] and gradient g[v + nv*dim] */
 void or_viscous_flux(int d, const double* s, const double* g, double mu, double Pr, double* fv);
 /* BR2 weak gradient coefficients: g, gv: E x d x nv x Np; gf: F x 2 x d x nv x Np.  0 or -1 */

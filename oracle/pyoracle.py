@@ -38,6 +38,7 @@ def lib():
             getattr(L, name).argtypes = [C.c_void_p, dp, dp]
         L.or_compute_rhs_d_threads.argtypes = [C.c_void_p, dp, dp, C.c_int]
         L.or_set_viscous.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.or_set_sutherland.argtypes = [C.c_void_p, C.c_double]
         L.or_set_walls.argtypes = [C.c_void_p] + [C.c_double] * 4
         L.or_viscous_flux.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, dp]
         L.or_ns_gradient.argtypes = [C.c_void_p, dp, dp, dp, dp]
@@ -157,7 +158,8 @@ def bc_state(d, bc, sL, n, prim):
 class Oracle:
     """One DG discretisation (mesh + physics) on the CPU."""
 
-    def __init__(self, mesh, pde: int, flux: int, b_adv=(0.0, 0.0, 0.0), bc_prim=None, viscous=None, walls=None):
+    def __init__(self, mesh, pde: int, flux: int, b_adv=(0.0, 0.0, 0.0), bc_prim=None, viscous=None, walls=None,
+                 sutherland_r_s=None):
         """viscous = (mu, Pr): Navier-Stokes with BR2 (Euler contexts; interior, Riemann and no-slip wall faces).
         walls = (nf_E, omega, rho_b, E_b): the no-slip wall data (or_set_walls)."""
         self.mesh = mesh
@@ -180,6 +182,8 @@ class Oracle:
         self.ndof = mesh.E * self.Np * self.nv
         if viscous is not None:
             lib().or_set_viscous(self.ctx, float(viscous[0]), float(viscous[1]))
+            if sutherland_r_s is not None:
+                lib().or_set_sutherland(self.ctx, float(sutherland_r_s))
         if walls is not None:
             lib().or_set_walls(self.ctx, *[float(w) for w in walls])
 

@@ -285,6 +285,51 @@ def test_ns_block_sgs_apply_matches_numpy(dev, case):
     ctx.close()
 
 
+@pytest.mark.parametrize("flux", [M.FLUX_ROE, M.FLUX_LF])
+def test_ns_sutherland_residual_and_linearisation_match_oracle(dev, flux):
+    """Sutherland's viscosity (DGX_VISC_SUTHERLAND, r_s 0.4): the FP64 residual, J v and the diagonal blocks equal
+    the oracle to 1e-12, on a curved periodic mesh and on the no-slip annulus; the mixed (perturbation) residual
+    stays within 1e-5 of FP64; variable-order and mixed TRI/QUAD meshes refuse it."""
+    from dpgsolver_amd import _lib
+
+    for name, mesh, fl, kw, u in (CASES[2], CASES[10]):
+        # the annulus keeps its Roe flux: the outer wall's nodes are at rest, where the Lax-Friedrichs max(|V| + c)
+        # has no derivative (|V| at V = 0; the reference's dmaxV_ds divides by V there too,
+        # numerical_flux_euler_T.c:1653-1676)
+        flux = flux if name.startswith("periodic") else fl
+        ctx = _lib.Context(2, mesh.p, M.PDE_NAVIER_STOKES, flux, mu=MU, Pr=PR, sutherland=True, r_s=0.4, **kw)
+        ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        ctx.upload_mesh(mesh.xyz, mesh.faces)
+        orc = O.Oracle(mesh, M.PDE_EULER, flux, viscous=(MU, PR), sutherland_r_s=0.4, **kw)
+        ud = torch.from_numpy(u).to(dev)
+        r = torch.full_like(ud, float("nan"))
+        ctx.compute_rhs(ud.data_ptr(), r.data_ptr())
+        v = np.random.default_rng(2).standard_normal(u.size) * np.abs(u)
+        vd = torch.from_numpy(v).to(dev)
+        jv = torch.full_like(ud, float("nan"))
+        ctx.jacobian_vector(ud.data_ptr(), vd.data_ptr(), jv.data_ptr())
+        rmx = torch.full_like(ud, float("nan"), dtype=torch.float32)
+        ctx.compute_rhs_mixed(ud.data_ptr(), rmx.data_ptr())
+        nb = ctx.Np * ctx.nv
+        blocks = torch.full((mesh.E * nb * nb,), float("nan"), dtype=torch.float64, device=dev)
+        ctx.assemble_diag_blocks(ud.data_ptr(), None, blocks.data_ptr())
+        torch.cuda.synchronize()
+        ref = orc.rhs_ns(u)
+        e_r = _rel(r.cpu().numpy(), ref)
+        e_jv = _rel(jv.cpu().numpy(), orc.jvp_ns(u, v))
+        e_mx = _rel(rmx.double().cpu().numpy(), ref)
+        blk = orc.ns_diag_block(u, 0)
+        e_b = np.abs(blocks.cpu().numpy().reshape(mesh.E, nb, nb)[0] - blk).max() / np.abs(blk).max()
+        print(f"{name} flux {flux}: residual {e_r:.1e}, J v {e_jv:.1e}, block {e_b:.1e}, mixed {e_mx:.1e}", flush=True)
+        assert e_r < 1e-12 and e_jv < 1e-12 and e_b < 1e-12 and e_mx < 1e-5
+        ctx.close()
+    ctx = _lib.Context(2, 2, M.PDE_NAVIER_STOKES, flux, mu=MU, Pr=PR, sutherland=True)
+    hm = M.structured_hp_mesh(np.array([1, 2, 2, 1]), (2, 2), (0.0, 0.0), (1.0, 1.0), (True, True))
+    with pytest.raises(_lib.DgxError, match="UNSUPPORTED"):
+        ctx.upload_mesh_hp(hm)
+    ctx.close()
+
+
 def test_ns_cfl_dt_takes_the_viscous_bound(dev):
     mesh = M.structured_tp_mesh(2, 2, (4, 4), (0.0, 0.0), (1.0, 1.0), (True, True))
     prim = np.array([1.0, 0.0, 0.0, 1.0])

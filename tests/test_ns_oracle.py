@@ -19,6 +19,11 @@ The reference holds no golden values for the viscous path, so each check is one 
   in the momentum components and to 0 in mass and energy; a reversed wall rotation or a wrong-sign wall heat
   flux does not converge;
 * Riemann faces in a Navier-Stokes context keep a free stream exactly;
+* Sutherland's viscosity (or_set_sutherland): at a state of uniform nodal temperature the residual is, to the
+  temperature's deviation at the cubature points, the constant-viscosity residual with mu_s(T0) = 8e-2 mu
+  sqrt(T0) / (1 + 112 / T0), evaluated here from the published formula (Toro eq. 1.56 with the reference's
+  constants), and a wrong constant or exponent is well outside that; its complex step matches Richardson
+  differences as the constant one does;
 * the complex twin (or_compute_rhs_ns_c, the complex-step derivatives that pin the GPU linearisation): its real
   part is the residual bit for bit; with mu = 0 its derivative equals the Euler residual's complex step on a
   periodic mesh; and the derivative of the viscous part R_NS(mu) - R_NS(0), a smooth function (the numerical
@@ -290,3 +295,42 @@ def test_ns_complex_step_blocks_rebuild_the_jacobian():
                 else:
                     assert not orc.ns_face_block(u, e, lf).any()
             assert np.abs(got - jv[e]).max() <= 1e-12 * np.abs(jv[e]).max(), (name, e)
+
+
+# ---------------------------------------------------------------------------------------------- Sutherland
+def _sutherland_mu(mu, T):
+    return 8e-2 * mu * np.sqrt(T) / (1.0 + 112.0 / T)
+
+
+@pytest.mark.parametrize("r_s", [1.0, 0.4])
+def test_sutherland_at_nodal_uniform_temperature_is_nearly_constant_viscosity(r_s):
+    """At a state whose nodal temperature is uniform (T0 = 2.5), the Sutherland residual is the constant-viscosity
+    residual with mu_s(T0) up to the temperature's deviation at the cubature and face points (the interpolated
+    conserved variables do not keep p / rho exactly): measured 0.15-0.2% of the viscous part.  A formula with the
+    square root left out (58%) or the constant 110.4 in place of 112 (1.4%) is well outside that."""
+    mesh = _periodic(3, 3, curved=True)
+    T0, mu = 2.5, 20.0
+
+    def prim(X, Y):
+        rho = 1.0 + 0.2 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y)
+        return np.stack([rho, 0.3 + 0.2 * np.sin(2 * np.pi * Y), -0.1 + 0.2 * np.cos(2 * np.pi * X), rho * r_s * T0], -1)
+    u = _state(mesh, prim)
+    suth = O.Oracle(mesh, M.PDE_EULER, M.FLUX_ROE, viscous=(mu, PR), sutherland_r_s=r_s).rhs_ns(u)
+    const = lambda m: O.Oracle(mesh, M.PDE_EULER, M.FLUX_ROE, viscous=(m, PR)).rhs_ns(u)
+    ref = const(_sutherland_mu(mu, T0))
+    vis = np.abs(ref - const(0.0)).max()
+    assert np.abs(suth - ref).max() <= 3e-3 * vis
+    for plant in (8e-2 * mu * T0 / (1.0 + 112.0 / T0), 8e-2 * mu * np.sqrt(T0) / (1.0 + 110.4 / T0)):
+        assert np.abs(suth - const(plant)).max() > 1e-2 * vis
+
+
+def test_sutherland_complex_step_matches_richardson_differences():
+    _, mesh, flux, kw, u = CS_CASES[1]
+    visc = O.Oracle(mesh, M.PDE_EULER, flux, viscous=(MU, PR), sutherland_r_s=0.4, **kw)
+    invisc = O.Oracle(mesh, M.PDE_EULER, flux, viscous=(0.0, PR), **kw)
+    v = np.random.default_rng(8).standard_normal(u.size) * np.abs(u)
+    cs = visc.jvp_ns(u, v) - invisc.jvp_ns(u, v)
+    rv = lambda w: visc.rhs_ns(w) - invisc.rhs_ns(w)
+    dif = lambda h: (rv(u + h * v) - rv(u - h * v)) / (2 * h)
+    rich = (4 * dif(5e-4) - dif(1e-3)) / 3
+    assert np.abs(cs - rich).max() < 1e-8 * np.abs(cs).max()
