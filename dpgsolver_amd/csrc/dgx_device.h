@@ -364,11 +364,12 @@ __device__ inline void ns_viscous_flux2(const S* s, const S* g, S mu, S Pr, S* f
 
 // Viscosity at a state: constant mu, or Sutherland's formula mu_s = 8e-2 mu sqrt(T) / (1 + 112 / T) with
 // T = p / (rho r_s) (compute_mu_sutherland_T, solution_navier_stokes_T.c:246-255; C1_SUTHERLAND, C2_SUTHERLAND,
-// solution_navier_stokes.h:30-33)
-template <typename S>
-__device__ inline S ns_mu(const S* s, S mu, int sutherland, double r_s)
+// solution_navier_stokes.h:30-33). A compile-time switch: a runtime branch here doubled the residual kernels'
+// VGPRs and cost the constant-viscosity kernels a third of their occupancy.
+template <bool SUTHERLAND, typename S>
+__device__ inline S ns_mu(const S* s, S mu, double r_s)
 {
-	if (!sutherland) return mu;
+	if constexpr (!SUTHERLAND) return mu;
 	const S ri = trcp(s[0]);
 	const S V2 = (s[1] * s[1] + s[2] * s[2]) * ri * ri;
 	const S p = S(kGM1) * (s[3] - S(0.5) * s[0] * V2);

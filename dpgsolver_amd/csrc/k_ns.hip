@@ -71,7 +71,7 @@ struct NsArgs {
 	const ElemFace* ef;
 	const Ops1D<G>* ops;
 	S mu, Pr;
-	int sutherland;             // viscosity by Sutherland's formula (ns_mu) instead of constant mu
+	int sutherland;             // viscosity by Sutherland's formula: selects the k_rhs_ns<..., SUTH = true> kernels
 	double r_s;
 	int E;
 	int col;                    // kNsDiag, kNsOffd: the seeded local dof
@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(128) k_ns_gv(NsArgs<S> a)
 	}
 }
 
-template <typename S, int N, int FLUX, bool PERT, int MODE>
+template <typename S, int N, int FLUX, bool PERT, int MODE, bool SUTH>
 __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 {
 	constexpr int Np = N * N, EPB = ns_epb<S, N>();
@@ -418,8 +418,8 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 				cu[1] = c0[2] * ci;
 				const double pc = kGM1 * (c0[3] - 0.5 * c0[0] * (cu[0] * cu[0] + cu[1] * cu[1]));
 				euler_normal_flux<double, 2>(c0, cu, pc, nd, cn);
-				ns_viscous_flux2<double>(cL, gLd, ns_mu<double>(cL, (double)a.mu, a.sutherland, a.r_s), (double)a.Pr, fvL);
-				ns_viscous_flux2<double>(cR, gRd, ns_mu<double>(cR, (double)a.mu, a.sutherland, a.r_s), (double)a.Pr, fvR);
+				ns_viscous_flux2<double>(cL, gLd, ns_mu<SUTH, double>(cL, (double)a.mu, a.r_s), (double)a.Pr, fvL);
+				ns_viscous_flux2<double>(cR, gRd, ns_mu<SUTH, double>(cR, (double)a.mu, a.r_s), (double)a.Pr, fvR);
 #pragma unroll
 				for (int v = 0; v < kNsNV; ++v) {
 					const double vis = nd[0] * 0.5 * (fvL[0 + 2 * v] + fvR[0 + 2 * v]) + nd[1] * 0.5 * (fvL[1 + 2 * v] + fvR[1 + 2 * v]);
@@ -432,8 +432,8 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 				S nnf[kNsNV], fvL[2 * kNsNV], fvR[2 * kNsNV];
 				if constexpr (FLUX == DGX_FLUX_LF) numflux_lf<S, 2>(sL, sR, nu, nnf);
 				else numflux_roe<S, 2>(sL, sR, nu, nnf);
-				ns_viscous_flux2<S>(sL, gL, ns_mu<S>(sL, a.mu, a.sutherland, a.r_s), a.Pr, fvL);
-				ns_viscous_flux2<S>(sR, gR, ns_mu<S>(sR, a.mu, a.sutherland, a.r_s), a.Pr, fvR);
+				ns_viscous_flux2<S>(sL, gL, ns_mu<SUTH, S>(sL, a.mu, a.r_s), a.Pr, fvL);
+				ns_viscous_flux2<S>(sR, gR, ns_mu<SUTH, S>(sR, a.mu, a.r_s), a.Pr, fvR);
 #pragma unroll
 				for (int v = 0; v < kNsNV; ++v) {
 					const S vis = nu[0] * S(0.5) * (fvL[0 + 2 * v] + fvR[0 + 2 * v]) + nu[1] * S(0.5) * (fvL[1 + 2 * v] + fvR[1 + 2 * v]);
@@ -474,14 +474,14 @@ __global__ void __launch_bounds__(128) k_rhs_ns(NsArgs<S> a)
 			for (int k = 0; k < 2 * kNsNV; ++k) gd[k] = (double)g[k];
 			euler_flux<double, 2>(sd, fd);
 			euler_flux<double, 2>(cd, fcd);
-			ns_viscous_flux2<double>(sd, gd, ns_mu<double>(sd, (double)a.mu, a.sutherland, a.r_s), (double)a.Pr, fvd);
+			ns_viscous_flux2<double>(sd, gd, ns_mu<SUTH, double>(sd, (double)a.mu, a.r_s), (double)a.Pr, fvd);
 #pragma unroll
 			for (int k = 0; k < 2 * kNsNV; ++k) f[k] = S(fd[k] - fcd[k] + fvd[k]);
 		} else {
 #pragma unroll
 			for (int v = 0; v < kNsNV; ++v) s[v] = B.V[slc][v * Np + t];
 			euler_flux<S, 2>(s, f);
-			ns_viscous_flux2<S>(s, g, ns_mu<S>(s, a.mu, a.sutherland, a.r_s), a.Pr, fv);
+			ns_viscous_flux2<S>(s, g, ns_mu<SUTH, S>(s, a.mu, a.r_s), a.Pr, fv);
 #pragma unroll
 			for (int k = 0; k < 2 * kNsNV; ++k) f[k] += fv[k];
 		}
@@ -574,10 +574,14 @@ hipError_t run_ns(Ctx& c, const NsArgs<S>& a, int phase)
 		DGX_HIP_CHECK(hipGetLastError());
 	}
 	if (phase == 1) return hipSuccess;
-	if (c.cfg.num_flux == DGX_FLUX_LF)
-		hipLaunchKernelGGL((k_rhs_ns<S, N, DGX_FLUX_LF, PERT, MODE>), rgrid, dim3(EPB * Np), 0, c.stream, a);
-	else
-		hipLaunchKernelGGL((k_rhs_ns<S, N, DGX_FLUX_ROE, PERT, MODE>), rgrid, dim3(EPB * Np), 0, c.stream, a);
+	const bool lf = c.cfg.num_flux == DGX_FLUX_LF;
+	if (a.sutherland) {
+		if (lf) hipLaunchKernelGGL((k_rhs_ns<S, N, DGX_FLUX_LF, PERT, MODE, true>), rgrid, dim3(EPB * Np), 0, c.stream, a);
+		else hipLaunchKernelGGL((k_rhs_ns<S, N, DGX_FLUX_ROE, PERT, MODE, true>), rgrid, dim3(EPB * Np), 0, c.stream, a);
+	} else {
+		if (lf) hipLaunchKernelGGL((k_rhs_ns<S, N, DGX_FLUX_LF, PERT, MODE, false>), rgrid, dim3(EPB * Np), 0, c.stream, a);
+		else hipLaunchKernelGGL((k_rhs_ns<S, N, DGX_FLUX_ROE, PERT, MODE, false>), rgrid, dim3(EPB * Np), 0, c.stream, a);
+	}
 	return hipGetLastError();
 }
 
