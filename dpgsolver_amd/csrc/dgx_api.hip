@@ -1466,16 +1466,17 @@ dgx_status dgx_assemble_face_blocks(dgx_ctx* h, const double* u, double* blocks)
 	if (!h || !u || !blocks) return DGX_ERR_ARG;
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
-	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
+	if (c.d != 2 && c.d != 3) return DGX_ERR_UNSUPPORTED;
 	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) {
+		if (c.d != 2) return DGX_ERR_UNSUPPORTED;
 		// BR2: full blocks (the neighbour's whole state enters through its lifted gradient), one column per launch
 		const int nb = c.Np * c.nv;
 		for (int col = 0; col < nb; ++col) DGX_TRY(dgx::launch_ns_offd_col(c, u, blocks, col, false));
 		return DGX_OK;
 	}
-	const int nfv = c.nv * c.N;   // face-node rows / columns of a 2-D face block
+	const int nfv = c.nv * (c.d == 2 ? c.N : c.N * c.N);   // face-node rows / columns of a face block
 	double* cT = nullptr;
-	DGX_TRY(hipMalloc(&cT, sizeof(double) * (size_t)c.E * 4 * nfv * nfv));
+	DGX_TRY(hipMalloc(&cT, sizeof(double) * (size_t)c.E * 2 * c.d * nfv * nfv));
 	hipError_t e = hipSuccess;
 	for (int col = 0; col < nfv && e == hipSuccess; ++col) e = dgx::launch_offd_col(c, u, cT, col);
 	if (e == hipSuccess) e = dgx::launch_expand_face_blocks(c, cT, blocks);
@@ -1541,20 +1542,20 @@ dgx_status dgx_assemble_jacobian_bsr(dgx_ctx* h, const double* u, const double* 
 	if (!h || !u || !values) return DGX_ERR_ARG;
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
-	if (c.d != 2) return DGX_ERR_UNSUPPORTED;
+	if (c.d != 2 && (c.d != 3 || c.cfg.pde == DGX_PDE_NAVIER_STOKES)) return DGX_ERR_UNSUPPORTED;
 	std::vector<int> rp, col, diag, face;
 	const dgx_status ps = bsr_pattern(c, rp, col, diag, face);
 	if (ps != DGX_OK) return ps;
-	const int E = c.E, nb = c.Np * c.nv, nb2 = nb * nb;
+	const int E = c.E, nb = c.Np * c.nv, nb2 = nb * nb, nf = 2 * c.d;
 	double *dblk = nullptr, *fblk = nullptr;
 	int* slots = nullptr;
 	dgx_status st = DGX_OK;
 	do {
 		if (hipMalloc(&dblk, sizeof(double) * (size_t)E * nb2) != hipSuccess ||
-		    hipMalloc(&fblk, sizeof(double) * (size_t)E * 4 * nb2) != hipSuccess ||
-		    hipMalloc(&slots, sizeof(int) * (size_t)5 * E) != hipSuccess) { st = DGX_ERR_HIP; break; }
+		    hipMalloc(&fblk, sizeof(double) * (size_t)E * nf * nb2) != hipSuccess ||
+		    hipMalloc(&slots, sizeof(int) * (size_t)(nf + 1) * E) != hipSuccess) { st = DGX_ERR_HIP; break; }
 		if (hipMemcpyAsync(slots, diag.data(), sizeof(int) * E, hipMemcpyHostToDevice, c.stream) != hipSuccess ||
-		    hipMemcpyAsync(slots + E, face.data(), sizeof(int) * 4 * (size_t)E, hipMemcpyHostToDevice, c.stream) != hipSuccess) {
+		    hipMemcpyAsync(slots + E, face.data(), sizeof(int) * nf * (size_t)E, hipMemcpyHostToDevice, c.stream) != hipSuccess) {
 			st = DGX_ERR_HIP;
 			break;
 		}
@@ -1569,8 +1570,8 @@ dgx_status dgx_assemble_jacobian_bsr(dgx_ctx* h, const double* u, const double* 
 		// each block once: the own block, then the faces one local face at a time (a neighbour reached through two
 		// faces, or the element itself on a periodic mesh one element wide, gets their sum)
 		if (dgx::launch_bsr_place(c.stream, dblk, (size_t)nb2, 0, slots, values, E, nb2, false) != hipSuccess) { st = DGX_ERR_HIP; break; }
-		for (int lf = 0; lf < 4 && st == DGX_OK; ++lf)
-			if (dgx::launch_bsr_place(c.stream, fblk, (size_t)4 * nb2, (size_t)lf * nb2, slots + (size_t)(lf + 1) * E, values,
+		for (int lf = 0; lf < nf && st == DGX_OK; ++lf)
+			if (dgx::launch_bsr_place(c.stream, fblk, (size_t)nf * nb2, (size_t)lf * nb2, slots + (size_t)(lf + 1) * E, values,
 			                          E, nb2, true) != hipSuccess)
 				st = DGX_ERR_HIP;
 	} while (false);

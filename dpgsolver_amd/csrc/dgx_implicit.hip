@@ -451,47 +451,66 @@ hipError_t launch_relaxed_update(Ctx& c, double* u, const double* du, int* statu
 namespace dgx {
 
 // 2-D face-node map: node of local face lf at face position a (faces r-, r+, s-, s+)
+template <int D>
 __device__ inline int sgs_face_node(int lf, int a, int N)
 {
-	const int fix = (lf & 1) ? N - 1 : 0;
-	return (lf >> 1) == 0 ? fix + N * a : a + N * fix;
+	const int fd = lf >> 1, fix = (lf & 1) ? N - 1 : 0;
+	if constexpr (D == 2) return fd == 0 ? fix + N * a : a + N * fix;
+	const int i = a % N, j = a / N;   // HEX face position a = i + N j over the two remaining directions
+	if (fd == 0) return fix + N * (i + N * j);
+	if (fd == 1) return i + N * (fix + N * j);
+	return i + N * (j + N * fix);
 }
 
-// One colour of a block Gauss-Seidel sweep on a 2-D mesh: for every element e of the colour,
+// One colour of a block Gauss-Seidel sweep: for every element e of the colour,
 // z_e = A_ee^-1 (v_e - sum_faces A_e,n z_n).  A_ee^-1 is stored transposed (nb x nb); the face blocks A_e,n
-// compactly, transposed, on the nfv = nv N face-node rows and columns (the MODE_OFFD layout), so a face
+// compactly, transposed, on the nfv = nv N^(D-1) face-node rows and columns (the MODE_OFFD layout), so a face
 // costs an nfv x nfv mat-vec on the neighbour's trace.  Elements of one colour share no face, so their
-// updates are independent.  One 64-lane workgroup per element; nb <= 64 and 4 nfv <= 64.
+// updates are independent.  One workgroup of 64 (QUAD) or 128 (HEX) lanes per element; nb <= 64 and
+// 2 D nfv <= the lanes.
 // TB: the stored blocks' type (FP64, or FP32 with the products and sums in FP64).
-template <typename TB>
-__global__ void __launch_bounds__(64) k_bgs_color(const int* __restrict__ elems, const ElemFace* __restrict__ ef,
-                                                  const TB* __restrict__ invT, const TB* __restrict__ offT,
-                                                  const double* __restrict__ rhs, double* z, int N, int nv)
+template <typename TB, int D>
+__global__ void __launch_bounds__(D == 2 ? 64 : 128) k_bgs_color(const int* __restrict__ elems,
+                                                               const ElemFace* __restrict__ ef,
+                                                               const TB* __restrict__ invT, const TB* __restrict__ offT,
+                                                               const double* __restrict__ rhs, double* z, int N, int nv)
 {
-	__shared__ double sz[64], sf[64], sr[64];
-	const int Np = N * N, nb = Np * nv, nfv = nv * N;
+	constexpr int NFACE = 2 * D, NT = (D == 2) ? 64 : 128;
+	__shared__ double sz[NT], sf[NT], sr[64];
+	const int NF = (D == 2) ? N : N * N, Np = NF * N, nb = Np * nv, nfv = nv * NF;
 	const int e = elems[blockIdx.x], t = threadIdx.x;
-	if (t < 4 * nfv) {   // neighbour traces, in each neighbour's own face-node order
-		const int lf = t / nfv, c = t - lf * nfv, w = c / N;
-		const ElemFace f = ef[(size_t)e * 4 + lf];
-		sz[t] = (f.en >= 0) ? z[(size_t)f.en * nb + w * Np + sgs_face_node(f.packed & 15, c - w * N, N)] : 0.0;
+	if (t < NFACE * nfv) {   // neighbour traces, in each neighbour's own face-node order
+		const int lf = t / nfv, c = t - lf * nfv, w = c / NF;
+		const ElemFace f = ef[(size_t)e * NFACE + lf];
+		sz[t] = (f.en >= 0) ? z[(size_t)f.en * nb + w * Np + sgs_face_node<D>(f.packed & 15, c - w * NF, N)] : 0.0;
 	}
 	__syncthreads();
-	if (t < 4 * nfv) {   // face contributions on the element's face-node rows
+	if (t < NFACE * nfv) {   // face contributions on the element's face-node rows
 		const int lf = t / nfv, row = t - lf * nfv;
-		const TB* C = offT + ((size_t)e * 4 + lf) * nfv * nfv;
+		const TB* C = offT + ((size_t)e * NFACE + lf) * nfv * nfv;
 		double acc = 0.0;
 		for (int c = 0; c < nfv; ++c) acc += (double)C[(size_t)c * nfv + row] * sz[lf * nfv + c];
 		sf[t] = acc;
 	}
 	__syncthreads();
 	if (t < nb) {
-		const int v = t / Np, node = t - v * Np, i0 = node % N, i1 = node / N;
+		const int v = t / Np, node = t - v * Np, i0 = node % N;
 		double r = rhs[(size_t)e * nb + t];
-		if (i0 == 0) r -= sf[0 * nfv + v * N + i1];
-		if (i0 == N - 1) r -= sf[1 * nfv + v * N + i1];
-		if (i1 == 0) r -= sf[2 * nfv + v * N + i0];
-		if (i1 == N - 1) r -= sf[3 * nfv + v * N + i0];
+		if constexpr (D == 2) {
+			const int i1 = node / N;
+			if (i0 == 0) r -= sf[0 * nfv + v * N + i1];
+			if (i0 == N - 1) r -= sf[1 * nfv + v * N + i1];
+			if (i1 == 0) r -= sf[2 * nfv + v * N + i0];
+			if (i1 == N - 1) r -= sf[3 * nfv + v * N + i0];
+		} else {
+			const int i1 = (node / N) % N, i2 = node / (N * N);
+			if (i0 == 0) r -= sf[0 * nfv + v * NF + i1 + N * i2];
+			if (i0 == N - 1) r -= sf[1 * nfv + v * NF + i1 + N * i2];
+			if (i1 == 0) r -= sf[2 * nfv + v * NF + i0 + N * i2];
+			if (i1 == N - 1) r -= sf[3 * nfv + v * NF + i0 + N * i2];
+			if (i2 == 0) r -= sf[4 * nfv + v * NF + i0 + N * i1];
+			if (i2 == N - 1) r -= sf[5 * nfv + v * NF + i0 + N * i1];
+		}
 		sr[t] = r;
 	}
 	__syncthreads();
@@ -613,18 +632,19 @@ struct PrecondBlockSGS : Precond {
 	static bool full(const Ctx& c) { return c.cfg.pde == DGX_PDE_NAVIER_STOKES; }
 	hipError_t setup(Ctx& c, const double* u, const double* inv_dt) override
 	{
-		const int nb = c.Np * c.nv, nfv = c.nv * c.N;
-		if (c.d != 2 || nb > kMaxInvBlock || 4 * nfv > 64) return hipErrorInvalidValue;
+		const int nb = c.Np * c.nv, nf = 2 * c.d, nfv = c.nv * (c.d == 2 ? c.N : c.N * c.N);
+		if ((c.d != 2 && c.d != 3) || (full(c) && c.d != 2) || nb > kMaxInvBlock || nf * nfv > (c.d == 2 ? 64 : 128))
+			return hipErrorInvalidValue;
 		DGX_HIP_CHECK(diag.setup(c, u, inv_dt));
 		// the face blocks: compact on the face nodes (Euler, advection), full for BR2
 		const int fb = full(c) ? nb : nfv;
-		if (!offT) DGX_HIP_CHECK(hipMalloc(&offT, sizeof(double) * (size_t)c.E * 4 * fb * fb));
+		if (!offT) DGX_HIP_CHECK(hipMalloc(&offT, sizeof(double) * (size_t)c.E * nf * fb * fb));
 		if (full(c))
 			for (int col = 0; col < nb; ++col) DGX_HIP_CHECK(launch_ns_offd_col(c, u, offT, col, true));
 		else
 			for (int col = 0; col < nfv; ++col) DGX_HIP_CHECK(launch_offd_col(c, u, offT, col));
 		if (!fp32) return hipSuccess;
-		const size_t ni = (size_t)c.E * nb * nb, no = (size_t)c.E * 4 * fb * fb;
+		const size_t ni = (size_t)c.E * nb * nb, no = (size_t)c.E * nf * fb * fb;
 		if (!inv32) DGX_HIP_CHECK(hipMalloc(&inv32, sizeof(float) * ni));
 		if (!off32) DGX_HIP_CHECK(hipMalloc(&off32, sizeof(float) * no));
 		DGX_HIP_CHECK(launch_convert(diag.blocks, inv32, ni, c.stream));
@@ -644,11 +664,18 @@ struct PrecondBlockSGS : Precond {
 				                   c.elem_faces, diag.blocks, offT, v, z, nb);
 			return hipGetLastError();
 		}
-		if (fp32)
-			hipLaunchKernelGGL(k_bgs_color<float>, dim3(cnt), dim3(64), 0, c.stream, c.color_elems + n0, c.elem_faces,
+		if (c.d == 3) {
+			if (fp32)
+				hipLaunchKernelGGL((k_bgs_color<float, 3>), dim3(cnt), dim3(128), 0, c.stream, c.color_elems + n0,
+				                   c.elem_faces, inv32, off32, v, z, c.N, c.nv);
+			else
+				hipLaunchKernelGGL((k_bgs_color<double, 3>), dim3(cnt), dim3(128), 0, c.stream, c.color_elems + n0,
+				                   c.elem_faces, diag.blocks, offT, v, z, c.N, c.nv);
+		} else if (fp32)
+			hipLaunchKernelGGL((k_bgs_color<float, 2>), dim3(cnt), dim3(64), 0, c.stream, c.color_elems + n0, c.elem_faces,
 			                   inv32, off32, v, z, c.N, c.nv);
 		else
-			hipLaunchKernelGGL(k_bgs_color<double>, dim3(cnt), dim3(64), 0, c.stream, c.color_elems + n0, c.elem_faces,
+			hipLaunchKernelGGL((k_bgs_color<double, 2>), dim3(cnt), dim3(64), 0, c.stream, c.color_elems + n0, c.elem_faces,
 			                   diag.blocks, offT, v, z, c.N, c.nv);
 		return hipGetLastError();
 	}
@@ -863,6 +890,8 @@ static bool implicit_3d_ok(const Ctx& c, int precond, int jvp_fp32)
 	if (c.d != 3) return true;
 	if (jvp_fp32) return false;
 	if (precond == DGX_PC_NONE) return true;
+	if (precond == DGX_PC_BLOCK_SGS || precond == DGX_PC_BLOCK_SGS32)   // in-LDS inverses, 6 nv N^2 <= 128 (p = 1)
+		return c.Np * c.nv <= dgx::kMaxInvBlock && 6 * c.nv * c.N * c.N <= 128;
 	return precond == DGX_PC_BLOCK_JACOBI && c.Np * c.nv <= 90;
 }
 

@@ -622,22 +622,27 @@ hipError_t launch_block_inv_apply(hipStream_t s, const double* invT, const doubl
 	return hipGetLastError();
 }
 
-// 2-D face-node map: node of local face lf at face position a (faces r-, r+, s-, s+)
-__device__ inline int face_node_2d(int lf, int a, int N)
+// Face-node map: volume node of local face lf at face position i (faces r-, r+, s-, s+[, t-, t+]; on a HEX face
+// i = a + N b, a and b the two remaining directions in increasing order)
+__device__ inline int face_node_dd(int d, int lf, int i, int N)
 {
-	const int fix = (lf & 1) ? N - 1 : 0;
-	return (lf >> 1) == 0 ? fix + N * a : a + N * fix;
+	const int fd = lf >> 1, fix = (lf & 1) ? N - 1 : 0;
+	if (d == 2) return fd == 0 ? fix + N * i : i + N * fix;
+	const int a = i % N, b = i / N;
+	if (fd == 0) return fix + N * (a + N * b);
+	if (fd == 1) return a + N * (fix + N * b);
+	return a + N * (b + N * fix);
 }
 
-// Compact transposed 2-D face blocks (nfv = nv N face-node rows and columns, [col][row], as the MODE_OFFD
-// launches store them) scattered into full row-major nb x nb blocks, zero off the face nodes and on
-// boundary faces.  One workgroup per (element, local face).
+// Compact transposed face blocks (nfv = nv N^(d-1) face-node rows and columns, [col][row], as the MODE_OFFD
+// launches store them) scattered into full row-major nb x nb blocks, zero off the face nodes and on boundary
+// faces.  One workgroup per (element, local face).
 __global__ void k_expand_face_blocks(const double* __restrict__ cT, const ElemFace* __restrict__ ef, double* full,
-                                     int N, int nv)
+                                     int d, int N, int nv)
 {
-	const int Np = N * N, nb = Np * nv, nfv = nv * N;
+	const int NF = (d == 2) ? N : N * N, Np = NF * N, nb = Np * nv, nfv = nv * NF;
 	const size_t b = blockIdx.x;
-	const int lf = (int)(b % 4);
+	const int lf = (int)(b % (2 * d));
 	const ElemFace f = ef[b];
 	double* out = full + b * nb * nb;
 	for (int k = threadIdx.x; k < nb * nb; k += blockDim.x) out[k] = 0.0;
@@ -646,18 +651,18 @@ __global__ void k_expand_face_blocks(const double* __restrict__ cT, const ElemFa
 	const int fn = f.packed & 15;
 	for (int k = threadIdx.x; k < nfv * nfv; k += blockDim.x) {
 		const int col = k / nfv, row = k - col * nfv;
-		const int v = row / N, w = col / N;
-		out[(size_t)(v * Np + face_node_2d(lf, row - v * N, N)) * nb + w * Np + face_node_2d(fn, col - w * N, N)] =
+		const int v = row / NF, w = col / NF;
+		out[(size_t)(v * Np + face_node_dd(d, lf, row - v * NF, N)) * nb + w * Np + face_node_dd(d, fn, col - w * NF, N)] =
 			cT[b * nfv * nfv + k];
 	}
 }
 
 hipError_t launch_expand_face_blocks(Ctx& c, const double* cT, double* full)
 {
-	if (c.d != 2) return hipErrorInvalidValue;
+	if (c.d != 2 && c.d != 3) return hipErrorInvalidValue;
 	if (c.E == 0) return hipSuccess;
-	hipLaunchKernelGGL(k_expand_face_blocks, dim3((unsigned)(c.E * 4)), dim3(256), 0, c.stream, cT, c.elem_faces, full,
-	                   c.N, c.nv);
+	hipLaunchKernelGGL(k_expand_face_blocks, dim3((unsigned)(c.E * 2 * c.d)), dim3(256), 0, c.stream, cT, c.elem_faces,
+	                   full, c.d, c.N, c.nv);
 	return hipGetLastError();
 }
 

@@ -21,17 +21,15 @@ static hipError_t dispatch_dual(Ctx& c, const RhsArgs<S, R>& a)
 	}
 	DGX_DCASE(2) DGX_DCASE(3) DGX_DCASE(4) DGX_DCASE(5)
 #undef DGX_DCASE
-	// 3-D (HEX): J v and the diagonal blocks (the face blocks are 2-D only)
-	if constexpr (MODE != MODE_OFFD) {
+	// 3-D (HEX): J v, the diagonal blocks and the face blocks, FP64 dual numbers
 #define DGX_DCASE3(NN)                                                                                   \
-		if (d == 3 && p + 1 == NN) {                                                                     \
-			if (pde == DGX_PDE_ADVECTION) return run_rhs<S, R, 3, NN, DGX_PDE_ADVECTION, DGX_FLUX_UPWIND, MODE>(a, c.stream); \
-			if (flux == DGX_FLUX_LF) return run_rhs<S, R, 3, NN, DGX_PDE_EULER, DGX_FLUX_LF, MODE>(a, c.stream);            \
-			return run_rhs<S, R, 3, NN, DGX_PDE_EULER, DGX_FLUX_ROE, MODE>(a, c.stream);                                    \
-		}
-		if constexpr (sizeof(S) == 2 * sizeof(double)) { DGX_DCASE3(2) DGX_DCASE3(3) DGX_DCASE3(4) DGX_DCASE3(5) }
-#undef DGX_DCASE3
+	if (d == 3 && p + 1 == NN) {                                                                         \
+		if (pde == DGX_PDE_ADVECTION) return run_rhs<S, R, 3, NN, DGX_PDE_ADVECTION, DGX_FLUX_UPWIND, MODE>(a, c.stream); \
+		if (flux == DGX_FLUX_LF) return run_rhs<S, R, 3, NN, DGX_PDE_EULER, DGX_FLUX_LF, MODE>(a, c.stream);            \
+		return run_rhs<S, R, 3, NN, DGX_PDE_EULER, DGX_FLUX_ROE, MODE>(a, c.stream);                                    \
 	}
+	if constexpr (sizeof(S) == 2 * sizeof(double)) { DGX_DCASE3(2) DGX_DCASE3(3) DGX_DCASE3(4) DGX_DCASE3(5) }
+#undef DGX_DCASE3
 	return hipErrorInvalidValue;
 }
 

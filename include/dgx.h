@@ -302,9 +302,10 @@ dgx_status dgx_max_abs(dgx_ctx* ctx, const double* x, long long n, double* out);
 dgx_status dgx_convert_f64_f32(dgx_ctx* ctx, const double* src, float* dst, long long n);
 dgx_status dgx_convert_f32_f64(dgx_ctx* ctx, const float* src, double* dst, long long n);
 
-/* Implicit path (contexts of one order: 2-D QUAD advection, Euler, and Navier-Stokes where stated; 3-D HEX Euler
- * for J v, the Newton operator, the diagonal blocks, the pseudo-time step and the solve with block-Jacobi up to
- * nb = 90 or no preconditioner; the face blocks are 2-D).
+/* Implicit path (contexts of one order: 2-D QUAD advection, Euler, and Navier-Stokes where stated; 3-D HEX
+ * advection and Euler for J v, the Newton operator, the diagonal and face blocks, the block-sparse export, the
+ * pseudo-time step and the solve with block-Jacobi up to nb = 90, block SGS up to nb = 64 (p = 1), or no
+ * preconditioner).
  * Diagonal blocks A_e = dR_e/du_e - mass_scale[e] * M_e (mass_scale: device array of E, or NULL for none), (Np*nv)^2 each, row-major, row eq*Np+node,
  * column var*Np+node: volume lhs_v_1 + face lhs_ll/lhs_rr contributions + CFL term
  * (constructor_lhs_v_1_T compute_volume_rlhs_T.c:179-220, constructor_lhs_f_1_T
@@ -315,19 +316,21 @@ dgx_status dgx_convert_f32_f64(dgx_ctx* ctx, const float* src, double* dst, long
 dgx_status dgx_assemble_diag_blocks(dgx_ctx* ctx, const double* u, const double* mass_scale, double* blocks);
 /* Face (off-diagonal) blocks: blocks[(e * 2d + lf) * nb^2 + row * nb + col] = dR_e[row] / du_n[col], n the
  * neighbour across local face lf (col in n's local numbering var*Np+node); zero for boundary faces.  Only the
- * nv*N face-node rows and columns are non-zero (the neighbour enters through its trace, the face integral tests
- * with the element's face nodes); they are computed compactly, nv*N dual-number launches, and scattered.
+ * nv*N^(d-1) face-node rows and columns are non-zero (the neighbour enters through its trace, the face integral
+ * tests with the element's face nodes); they are computed compactly, nv*N^(d-1) dual-number launches, and
+ * scattered.
  * These are lhs_lr / lhs_rl of constructor_lhs_f_1_T (compute_face_rlhs_T.c:163-220, added at
  * compute_face_rlhs_dg.c:198-213); with the diagonal blocks they are the whole block-sparse Jacobian. */
-/* Navier-Stokes: every row and column of a face block is non-zero (the neighbour's whole state enters the BR2
+/* Navier-Stokes (2-D): every row and column of a face block is non-zero (the neighbour's whole state enters the BR2
  * residual through its lifted gradient, compute_volume_rlhs_dg.c:220-261); nb dual-number launches. */
 dgx_status dgx_assemble_face_blocks(dgx_ctx* ctx, const double* u, double* blocks);
 /* The Newton matrix dR/du - mass_scale[e] M_e (mass_scale NULL: dR/du) in block-sparse row form, the sparsity of
  * the reference's AIJ matrix (constructor_nnz_dg_T, solve_dg_T.c:41-74; filled by add_to_petsc_Mat, solve.c:321-338):
  * block row e holds the element's own block and one block per face neighbour, columns ascending, nb = nv Np.
  * dgx_jacobian_bsr_pattern: host arrays row_ptr[E + 1] and col_idx[nnzb] (either may be NULL to query nnzb).
- * dgx_assemble_jacobian_bsr: device values[nnzb][nb][nb], row-major blocks in the pattern's order (2-D; the
- * diagonal and face blocks above, a neighbour reached through two faces getting their sum); synchronous. */
+ * dgx_assemble_jacobian_bsr: device values[nnzb][nb][nb], row-major blocks in the pattern's order (2-D, and 3-D
+ * advection and Euler; the diagonal and face blocks above, a neighbour reached through two faces getting their
+ * sum); synchronous. */
 dgx_status dgx_jacobian_bsr_pattern(dgx_ctx* ctx, int* row_ptr, int* col_idx, long long* nnzb);
 dgx_status dgx_assemble_jacobian_bsr(dgx_ctx* ctx, const double* u, const double* mass_scale, double* values);
 /* CFL-ramped pseudo-time step per element, returned as inv_dt[e] = max wave speed / (cfl * h_min), and for
@@ -358,7 +361,7 @@ dgx_status dgx_block_inv_apply(dgx_ctx* ctx, const double* inv, const double* x,
  * options file (GMRES restart 60, rtol 1e-4).  u (device) is updated in place.  Navier-Stokes contexts: the
  * FP64 operator, any preconditioner but DGX_PC_LEARNED (block SGS over the full BR2 face blocks, nb <= 64). */
 /* Preconditioners of the GMRES solves.  DGX_PC_BLOCK_JACOBI: exact inverses of the element diagonal blocks of
- * dR/du - M/dt.  DGX_PC_BLOCK_SGS (nb <= 64): one forward and one backward multicolour block Gauss-Seidel sweep
+ * dR/du - M/dt.  DGX_PC_BLOCK_SGS (nb <= 64; HEX: p = 1): one forward and one backward multicolour block Gauss-Seidel sweep
  * over the diagonal and face blocks from z = 0, the coupled-block counterpart of the reference's default
  * PETSc ILU(1) with RCM ordering (external/petsc/options_files/petsc_options_gmres_default.txt:6-8), which
  * factors the assembled matrix sequentially.  DGX_PC_BLOCK_SGS32: the same sweeps with the inverse diagonal

@@ -7,9 +7,12 @@
   linearisation check (test_integration_linearization.c:377), with and without the -s_e M_e term.
 * dgx_newton_operator equals dgx_jacobian_vector followed by dgx_mass_sub; the pseudo-time step has its closed
   form on a uniform state.
-* A steady solve on a 3-D box with Riemann faces (p = 1, block-Jacobi and no preconditioner) reaches max |R|
-  < 1e-10 from a perturbed free stream; the oracle's residual of the result agrees.
-* 3-D paths with no counterpart here (block SGS, FP32 J v, blocks the batched LU cannot take) return
+* A steady solve on a 3-D box with Riemann faces (p = 1; block-Jacobi, block SGS with FP64 and FP32-stored
+  blocks, and no preconditioner) reaches max |R| < 1e-10 from a perturbed free stream; the oracle's residual of
+  the result agrees, and block SGS takes fewer GMRES iterations than block-Jacobi.
+* The 3-D face blocks, the block-sparse export and the SGS sweeps are checked against the oracle and numpy with
+  the 2-D ones (test_gpu_solvers.py, the hex_* cases).
+* 3-D paths with no counterpart here (FP32 J v, blocks the batched LU or the SGS sweep cannot take) return
   DGX_ERR_UNSUPPORTED.
 """
 import numpy as np
@@ -132,15 +135,18 @@ def test_steady_box_3d_newton_gmres(dev):
 
     mesh, u0, prim = _box()
     ctx = _ctx(mesh, M.FLUX_ROE, bc_prim=prim)
-    for pc in (_lib.PC_BLOCK_JACOBI, _lib.PC_NONE):
+    its = {}
+    for pc in (_lib.PC_BLOCK_JACOBI, _lib.PC_NONE, _lib.PC_BLOCK_SGS, _lib.PC_BLOCK_SGS32):
         ud = torch.from_numpy(u0).to(dev)
         st = ctx.solve_implicit(ud.data_ptr(), cfl_initial=5.0, cfl_max=300.0, exit_tol=1e-10, max_newton=60,
                                 precond=pc)
         print("3-D box, precond", pc, "newton", st["newton_steps"], "gmres", st["gmres_iters"], st["seconds"],
               flush=True)
         assert st["converged"] == 1 and st["max_rhs"] < 1e-10
+        its[pc] = st["gmres_iters"]
         r = O.Oracle(mesh, M.PDE_EULER, M.FLUX_ROE, bc_prim=prim).rhs(ud.cpu().numpy())
         assert np.abs(r).max() < 1e-9
+    assert its[_lib.PC_BLOCK_SGS] < its[_lib.PC_BLOCK_JACOBI]
     ctx.close()
 
 
@@ -150,17 +156,13 @@ def test_3d_unsupported_paths_are_refused(dev):
     mesh, u0, prim = _box()
     ctx = _ctx(mesh, M.FLUX_ROE, bc_prim=prim)
     ud = torch.from_numpy(u0).to(dev)
-    for kw in ({"precond": _lib.PC_BLOCK_SGS}, {"precond": _lib.PC_BLOCK_JACOBI, "jvp_fp32": True}):
-        with pytest.raises(_lib.DgxError, match="UNSUPPORTED"):
-            ctx.solve_implicit(ud.data_ptr(), **kw)
-    nb = ctx.Np * ctx.nv
-    out = torch.empty(mesh.E * 6 * nb * nb, dtype=torch.float64, device=dev)
     with pytest.raises(_lib.DgxError, match="UNSUPPORTED"):
-        ctx.assemble_face_blocks(ud.data_ptr(), out.data_ptr())
+        ctx.solve_implicit(ud.data_ptr(), precond=_lib.PC_BLOCK_JACOBI, jvp_fp32=True)
     ctx.close()
-    mesh2, u2 = _curved(2, 2, 3)    # nb = 135: more than the batched LU takes
+    mesh2, u2 = _curved(2, 2, 3)    # nb = 135: more than the batched LU and the SGS sweep take
     ctx = _ctx(mesh2, M.FLUX_ROE)
     ud = torch.from_numpy(u2).to(dev)
-    with pytest.raises(_lib.DgxError, match="UNSUPPORTED"):
-        ctx.solve_implicit(ud.data_ptr(), precond=_lib.PC_BLOCK_JACOBI)
+    for pc in (_lib.PC_BLOCK_JACOBI, _lib.PC_BLOCK_SGS, _lib.PC_BLOCK_SGS32):
+        with pytest.raises(_lib.DgxError, match="UNSUPPORTED"):
+            ctx.solve_implicit(ud.data_ptr(), precond=pc)
     ctx.close()

@@ -101,6 +101,21 @@ def _jac_cases():
     m = M.channel_mesh(4, 3, 2)
     u, prim = _channel_state(m)
     out.append(("channel_bc_roe_p2", m, M.FLUX_ROE, prim, u))
+    # HEX p = 1 (nb = 40): a curved periodic mesh with rotated element numberings (every face orientation), and a
+    # box with Riemann faces on two sides
+    curve = lambda pts: pts + 0.03 * np.sin(2 * np.pi * pts[..., [1, 2, 0]])
+    m = M.structured_tp_mesh(3, 1, (3, 3, 3), (0, 0, 0), (1, 1, 1), (True, True, True), curve=curve,
+                             rotate=np.random.default_rng(4).integers(0, 8, 27))
+    u = M.pack_state(M.prim_to_cons(M.smooth_state_3d(M.nodes_xyz(m)), 3))
+    out.append(("hex_rotated_roe_p1", m, M.FLUX_ROE, None, u))
+    prim = M.freestream_prim(3)
+    R = M.BC_RIEMANN
+    m = M.structured_tp_mesh(3, 1, (3, 2, 2), (0, 0, 0), (1, 0.75, 0.75), (False, True, True), bc_low=(R, R, R),
+                             bc_high=(R, R, R))
+    x = M.nodes_xyz(m)
+    pert = 0.02 * np.sin(3 * x[..., 0]) * np.cos(2 * x[..., 1]) * np.cos(x[..., 2])
+    u = M.pack_state(M.prim_to_cons(prim[None, None, :] * (1.0 + np.stack([pert, pert, -pert, pert, 0.5 * pert], -1)), 3))
+    out.append(("hex_box_lf_p1", m, M.FLUX_LF, prim, u))
     return out
 
 
@@ -227,7 +242,9 @@ def _bsr_apply(rp, ci, vals, v, nb):
 
 
 BSR_CASES = JAC + [("periodic_2x1_lf_p2", M.structured_tp_mesh(2, 2, (2, 1), (0, 0), (1, 1), (True, True)),
-                    M.FLUX_LF, None, None)]
+                    M.FLUX_LF, None, None),
+                   ("hex_periodic_2x2x2_roe_p2", M.structured_tp_mesh(3, 2, (2, 2, 2), (0, 0, 0), (1, 1, 1),
+                                                                    (True, True, True)), M.FLUX_ROE, None, None)]
 
 
 @pytest.mark.parametrize("case", BSR_CASES, ids=[c[0] for c in BSR_CASES])
@@ -238,7 +255,7 @@ def test_jacobian_bsr_is_the_newton_operator(dev, case):
     neighbour reached through two faces, and the element itself, get the summed blocks."""
     name, mesh, flux, bcp, u = case
     if u is None:
-        u = M.pack_state(M.prim_to_cons(M.smooth_state_3d(M.nodes_xyz(mesh)), 2))
+        u = M.pack_state(M.prim_to_cons(M.smooth_state_3d(M.nodes_xyz(mesh)), mesh.d))
     ctx = _ctx(mesh, M.PDE_EULER, flux, bc_prim=bcp)
     nb = ctx.Np * ctx.nv
     rp, ci = ctx.jacobian_bsr_pattern()
