@@ -1632,13 +1632,13 @@ dgx_status dgx_block_inv_apply(dgx_ctx* h, const double* inv, const double* x, d
 
 dgx_status dgx_block_lu_factor(dgx_ctx* h, double* blocks, int* piv, int E, int nb)
 {
-	if (!h || !blocks || !piv || E < 0 || nb <= 0 || nb > 90) return DGX_ERR_ARG;  // nb^2 doubles in LDS
+	if (!h || !blocks || !piv || E < 0 || nb <= 0 || nb > dgx::kMaxLuBlock) return DGX_ERR_ARG;  // nb^2 doubles in LDS
 	return hip_status(dgx::launch_block_lu(h->c.stream, blocks, piv, E, nb));
 }
 
 dgx_status dgx_block_lu_solve(dgx_ctx* h, const double* lu, const int* piv, double* x, int E, int nb)
 {
-	if (!h || !lu || !piv || !x || E < 0 || nb <= 0 || nb > 90) return DGX_ERR_ARG;
+	if (!h || !lu || !piv || !x || E < 0 || nb <= 0 || nb > dgx::kMaxLuBlock) return DGX_ERR_ARG;
 	return hip_status(dgx::launch_block_lu_solve(h->c.stream, lu, piv, x, E, nb));
 }
 

@@ -883,8 +883,8 @@ struct Gmres {
 
 using dgx::Ctx;
 
-// 3-D (HEX Euler): the FP64 J v, and no preconditioner or block-Jacobi with blocks the batched LU takes
-// (nb = 5 (p + 1)^3 <= 90: p = 1); the face blocks, and so block SGS, are 2-D
+// 3-D (HEX Euler): the FP64 J v, and no preconditioner, block-Jacobi with blocks the batched LU takes
+// (nb = 5 (p + 1)^3 <= 143: p <= 2), or block SGS with blocks the in-LDS inverse and the sweep take (p = 1)
 static bool implicit_3d_ok(const Ctx& c, int precond, int jvp_fp32)
 {
 	if (c.d != 3) return true;
@@ -892,7 +892,7 @@ static bool implicit_3d_ok(const Ctx& c, int precond, int jvp_fp32)
 	if (precond == DGX_PC_NONE) return true;
 	if (precond == DGX_PC_BLOCK_SGS || precond == DGX_PC_BLOCK_SGS32)   // in-LDS inverses, 6 nv N^2 <= 128 (p = 1)
 		return c.Np * c.nv <= dgx::kMaxInvBlock && 6 * c.nv * c.N * c.N <= 128;
-	return precond == DGX_PC_BLOCK_JACOBI && c.Np * c.nv <= 90;
+	return precond == DGX_PC_BLOCK_JACOBI && c.Np * c.nv <= dgx::kMaxLuBlock;
 }
 
 extern "C" dgx_status dgx_precond_apply(dgx_ctx* h, int precond, const double* u, const double* inv_dt,

@@ -336,6 +336,8 @@ hipError_t launch_mass_sub(Ctx& c, const double* x, const double* scale, double*
 hipError_t launch_cfl_dt(Ctx& c, const double* u, double cfl, double* inv_dt);
 hipError_t launch_relaxed_update(Ctx& c, double* u, const double* du, int* status);
 std::vector<double> min_edge_lengths(int d, int N, int E, const double* xyz);
+constexpr int kMaxLuBlock = 143;   // largest block of the in-LDS batched LU: nb^2 doubles in the 160 KB of LDS a
+                                   // gfx950 workgroup may hold (HEX p = 2: nb = 135)
 constexpr int kMaxInvBlock = 64;   // largest element block (nb = Np * nv) of the in-LDS batched inverse
 hipError_t launch_block_inverse(hipStream_t s, double* blocks, int E, int nb, int* status);
 hipError_t launch_block_inv_apply(hipStream_t s, const double* invT, const double* v, double* z, int E, int nb);

@@ -686,8 +686,16 @@ hipError_t launch_bsr_place(hipStream_t s, const double* src, size_t stride, siz
 
 hipError_t launch_block_lu(hipStream_t s, double* blocks, int* piv, int E, int nb)
 {
+	if (nb <= 0 || nb > kMaxLuBlock) return hipErrorInvalidValue;
 	if (E == 0) return hipSuccess;
-	hipLaunchKernelGGL(k_block_lu, dim3(E), dim3(256), sizeof(double) * nb * nb, s, blocks, piv, nb);
+	const size_t lds = sizeof(double) * nb * nb;
+	if (lds > 65536) {   // beyond 64 KB the launch asks for the larger dynamic LDS once per process
+		static hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_block_lu),
+		                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+		                                             (int)(sizeof(double) * kMaxLuBlock * kMaxLuBlock));
+		if (attr != hipSuccess) return attr;
+	}
+	hipLaunchKernelGGL(k_block_lu, dim3(E), dim3(256), lds, s, blocks, piv, nb);
 	return hipGetLastError();
 }
 hipError_t launch_block_lu_solve(hipStream_t s, const double* lu, const int* piv, double* x, int E, int nb)

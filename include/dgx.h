@@ -304,8 +304,8 @@ dgx_status dgx_convert_f32_f64(dgx_ctx* ctx, const float* src, double* dst, long
 
 /* Implicit path (contexts of one order: 2-D QUAD advection, Euler, and Navier-Stokes where stated; 3-D HEX
  * advection and Euler for J v, the Newton operator, the diagonal and face blocks, the block-sparse export, the
- * pseudo-time step and the solve with block-Jacobi up to nb = 90, block SGS up to nb = 64 (p = 1), or no
- * preconditioner).
+ * pseudo-time step and the solve with block-Jacobi up to nb = 143 (p = 2), block SGS up to nb = 64 (p = 1), or
+ * no preconditioner).
  * Diagonal blocks A_e = dR_e/du_e - mass_scale[e] * M_e (mass_scale: device array of E, or NULL for none), (Np*nv)^2 each, row-major, row eq*Np+node,
  * column var*Np+node: volume lhs_v_1 + face lhs_ll/lhs_rr contributions + CFL term
  * (constructor_lhs_v_1_T compute_volume_rlhs_T.c:179-220, constructor_lhs_f_1_T
@@ -346,7 +346,8 @@ dgx_status dgx_jacobian_vector(dgx_ctx* ctx, const double* u, const double* v, d
 /* out = (dR/du)(u) v - mass_scale[e] M_e v_e (mass_scale: device, E doubles): the operator of the pseudo-time
  * Newton system, J v and the -M/dt term in one dual-number launch (the GMRES operator of dgx_solve_implicit). */
 dgx_status dgx_newton_operator(dgx_ctx* ctx, const double* u, const double* mass_scale, const double* v, double* out);
-/* Batched LU (partial pivoting) of E blocks of size nb <= 90 in place; piv: E*nb ints (device). */
+/* Batched LU (partial pivoting) of E blocks of size nb <= 143 in place (a block in LDS); piv: E*nb ints
+ * (device). */
 dgx_status dgx_block_lu_factor(dgx_ctx* ctx, double* blocks, int* piv, int E, int nb);
 /* x <- A_e^{-1} x for every element block (x: E*nb). */
 dgx_status dgx_block_lu_solve(dgx_ctx* ctx, const double* lu, const int* piv, double* x, int E, int nb);

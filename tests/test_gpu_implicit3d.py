@@ -8,8 +8,9 @@
 * dgx_newton_operator equals dgx_jacobian_vector followed by dgx_mass_sub; the pseudo-time step has its closed
   form on a uniform state.
 * A steady solve on a 3-D box with Riemann faces (p = 1; block-Jacobi, block SGS with FP64 and FP32-stored
-  blocks, and no preconditioner) reaches max |R| < 1e-10 from a perturbed free stream; the oracle's residual of
-  the result agrees, and block SGS takes fewer GMRES iterations than block-Jacobi.
+  blocks, and no preconditioner; p = 2 with block-Jacobi through the batched LU) reaches max |R| < 1e-10 from a
+  perturbed free stream; the oracle's residual of the result agrees, and block SGS takes fewer GMRES iterations
+  than block-Jacobi.
 * The 3-D face blocks, the block-sparse export and the SGS sweeps are checked against the oracle and numpy with
   the 2-D ones (test_gpu_solvers.py, the hex_* cases).
 * 3-D paths with no counterpart here (FP32 J v, blocks the batched LU or the SGS sweep cannot take) return
@@ -119,10 +120,10 @@ def test_cfl_dt_3d_closed_form(dev):
     ctx.close()
 
 
-def _box():
+def _box(p=1):
     prim = M.freestream_prim(3)
     R = M.BC_RIEMANN
-    mesh = M.structured_tp_mesh(3, 1, (4, 3, 3), (0.0, 0.0, 0.0), (1.0, 0.75, 0.75), (False, True, True),
+    mesh = M.structured_tp_mesh(3, p, (4, 3, 3), (0.0, 0.0, 0.0), (1.0, 0.75, 0.75), (False, True, True),
                                 bc_low=(R, R, R), bc_high=(R, R, R))
     x = M.nodes_xyz(mesh)
     pert = 0.02 * np.sin(3 * x[..., 0]) * np.cos(2 * x[..., 1]) * np.cos(x[..., 2])
@@ -150,6 +151,22 @@ def test_steady_box_3d_newton_gmres(dev):
     ctx.close()
 
 
+def test_steady_box_3d_p2_block_jacobi(dev):
+    """p = 2 (nb = 135): block-Jacobi through the batched LU with a 146 KB block in LDS per workgroup."""
+    from dpgsolver_amd import _lib
+
+    mesh, u0, prim = _box(2)
+    ctx = _ctx(mesh, M.FLUX_ROE, bc_prim=prim)
+    ud = torch.from_numpy(u0).to(dev)
+    st = ctx.solve_implicit(ud.data_ptr(), cfl_initial=5.0, cfl_max=300.0, exit_tol=1e-10, max_newton=60,
+                            precond=_lib.PC_BLOCK_JACOBI)
+    print("3-D box p=2, block-Jacobi: newton", st["newton_steps"], "gmres", st["gmres_iters"], st["seconds"], flush=True)
+    assert st["converged"] == 1 and st["max_rhs"] < 1e-10
+    r = O.Oracle(mesh, M.PDE_EULER, M.FLUX_ROE, bc_prim=prim).rhs(ud.cpu().numpy())
+    assert np.abs(r).max() < 1e-9
+    ctx.close()
+
+
 def test_3d_unsupported_paths_are_refused(dev):
     from dpgsolver_amd import _lib
 
@@ -159,10 +176,16 @@ def test_3d_unsupported_paths_are_refused(dev):
     with pytest.raises(_lib.DgxError, match="UNSUPPORTED"):
         ctx.solve_implicit(ud.data_ptr(), precond=_lib.PC_BLOCK_JACOBI, jvp_fp32=True)
     ctx.close()
-    mesh2, u2 = _curved(2, 2, 3)    # nb = 135: more than the batched LU and the SGS sweep take
+    mesh2, u2 = _curved(2, 2, 3)    # nb = 135: more than the SGS sweep takes
     ctx = _ctx(mesh2, M.FLUX_ROE)
     ud = torch.from_numpy(u2).to(dev)
-    for pc in (_lib.PC_BLOCK_JACOBI, _lib.PC_BLOCK_SGS, _lib.PC_BLOCK_SGS32):
+    for pc in (_lib.PC_BLOCK_SGS, _lib.PC_BLOCK_SGS32):
         with pytest.raises(_lib.DgxError, match="UNSUPPORTED"):
             ctx.solve_implicit(ud.data_ptr(), precond=pc)
+    ctx.close()
+    mesh3, u3 = _curved(3, 2, 3)    # nb = 320: more than the batched LU takes
+    ctx = _ctx(mesh3, M.FLUX_ROE)
+    ud = torch.from_numpy(u3).to(dev)
+    with pytest.raises(_lib.DgxError, match="UNSUPPORTED"):
+        ctx.solve_implicit(ud.data_ptr(), precond=_lib.PC_BLOCK_JACOBI)
     ctx.close()

@@ -358,20 +358,29 @@ def test_block_lu_solve_matches_numpy(dev):
     from dpgsolver_amd import _lib
 
     rng = np.random.default_rng(3)
+    ctx = _lib.Context(2, 3, M.PDE_EULER, M.FLUX_ROE)
+    # nb = 135 (HEX p = 2) and 143 take more than 64 KB of LDS per block: gfx950 workgroups hold up to 160 KB
+    for E, nb in ((37, 64), (19, 135), (5, 143)):
+        # eigenvalues within 1 of 4 at every size (circular law), so cond(A) stays O(1) and 1e-12 means the solve;
+        # two columns swapped so that the first pivot is not on the diagonal
+        A = rng.standard_normal((E, nb, nb)) / np.sqrt(nb) + 4 * np.eye(nb)
+        A[:, :, [0, 5]] = A[:, :, [5, 0]]
+        x = rng.standard_normal((E, nb))
+        Ad = torch.from_numpy(A.copy()).to(dev)
+        piv = torch.zeros(E * nb, dtype=torch.int32, device=dev)
+        xd = torch.from_numpy(x.copy()).to(dev)
+        ctx.block_lu_factor(Ad.data_ptr(), piv.data_ptr(), E, nb)
+        ctx.block_lu_solve(Ad.data_ptr(), piv.data_ptr(), xd.data_ptr(), E, nb)
+        torch.cuda.synchronize()
+        ref = np.linalg.solve(A, x[..., None])[..., 0]
+        assert _rel(xd.cpu().numpy(), ref) < 1e-12, nb
+    # nb = 144: nb^2 doubles exceed the LDS; refused on the host before any launch (the buffers are 5 x 143^2)
+    with pytest.raises(_lib.DgxError):
+        ctx.block_lu_factor(Ad.data_ptr(), piv.data_ptr(), 1, 144)
     E, nb = 37, 64
     A = rng.standard_normal((E, nb, nb)) + 4 * np.eye(nb)
     x = rng.standard_normal((E, nb))
-    ctx = _lib.Context(2, 3, M.PDE_EULER, M.FLUX_ROE)
-    Ad = torch.from_numpy(A.copy()).to(dev)
-    piv = torch.zeros(E * nb, dtype=torch.int32, device=dev)
     xd = torch.from_numpy(x.copy()).to(dev)
-    ctx.block_lu_factor(Ad.data_ptr(), piv.data_ptr(), E, nb)
-    ctx.block_lu_solve(Ad.data_ptr(), piv.data_ptr(), xd.data_ptr(), E, nb)
-    torch.cuda.synchronize()
-    ref = np.linalg.solve(A, x[..., None])[..., 0]
-    assert _rel(xd.cpu().numpy(), ref) < 1e-12
-    with pytest.raises(_lib.DgxError):
-        ctx.block_lu_factor(Ad.data_ptr(), piv.data_ptr(), E, 91)   # nb^2 doubles must fit in LDS
     # the explicit (transposed) inverse used by the block-Jacobi preconditioner, with pivoting exercised
     A2 = A.copy()
     A2[:, [0, 5]] = A2[:, [5, 0]]          # zero-free but pivot-demanding leading columns
