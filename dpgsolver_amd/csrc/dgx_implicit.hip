@@ -466,8 +466,8 @@ __device__ inline int sgs_face_node(int lf, int a, int N)
 // z_e = A_ee^-1 (v_e - sum_faces A_e,n z_n).  A_ee^-1 is stored transposed (nb x nb); the face blocks A_e,n
 // compactly, transposed, on the nfv = nv N^(D-1) face-node rows and columns (the MODE_OFFD layout), so a face
 // costs an nfv x nfv mat-vec on the neighbour's trace.  Elements of one colour share no face, so their
-// updates are independent.  One workgroup of 64 (QUAD) or 128 (HEX) lanes per element; nb <= 64 and
-// 2 D nfv <= the lanes.
+// updates are independent.  One workgroup of 64 (QUAD) or 128 (HEX) lanes per element, rows and face dofs beyond
+// the lanes looping; nb <= kMaxInvBlock and 2 D nfv <= kMaxSgsFace.
 // TB: the stored blocks' type (FP64, or FP32 with the products and sums in FP64).
 template <typename TB, int D>
 __global__ void __launch_bounds__(D == 2 ? 64 : 128) k_bgs_color(const int* __restrict__ elems,
@@ -476,26 +476,26 @@ __global__ void __launch_bounds__(D == 2 ? 64 : 128) k_bgs_color(const int* __re
                                                                const double* __restrict__ rhs, double* z, int N, int nv)
 {
 	constexpr int NFACE = 2 * D, NT = (D == 2) ? 64 : 128;
-	__shared__ double sz[NT], sf[NT], sr[64];
+	__shared__ double sz[kMaxSgsFace], sf[kMaxSgsFace], sr[kMaxInvBlock];
 	const int NF = (D == 2) ? N : N * N, Np = NF * N, nb = Np * nv, nfv = nv * NF;
 	const int e = elems[blockIdx.x], t = threadIdx.x;
-	if (t < NFACE * nfv) {   // neighbour traces, in each neighbour's own face-node order
-		const int lf = t / nfv, c = t - lf * nfv, w = c / NF;
+	for (int it = t; it < NFACE * nfv; it += NT) {   // neighbour traces, in each neighbour's own face-node order
+		const int lf = it / nfv, c = it - lf * nfv, w = c / NF;
 		const ElemFace f = ef[(size_t)e * NFACE + lf];
-		sz[t] = (f.en >= 0) ? z[(size_t)f.en * nb + w * Np + sgs_face_node<D>(f.packed & 15, c - w * NF, N)] : 0.0;
+		sz[it] = (f.en >= 0) ? z[(size_t)f.en * nb + w * Np + sgs_face_node<D>(f.packed & 15, c - w * NF, N)] : 0.0;
 	}
 	__syncthreads();
-	if (t < NFACE * nfv) {   // face contributions on the element's face-node rows
-		const int lf = t / nfv, row = t - lf * nfv;
+	for (int it = t; it < NFACE * nfv; it += NT) {   // face contributions on the element's face-node rows
+		const int lf = it / nfv, row = it - lf * nfv;
 		const TB* C = offT + ((size_t)e * NFACE + lf) * nfv * nfv;
 		double acc = 0.0;
 		for (int c = 0; c < nfv; ++c) acc += (double)C[(size_t)c * nfv + row] * sz[lf * nfv + c];
-		sf[t] = acc;
+		sf[it] = acc;
 	}
 	__syncthreads();
-	if (t < nb) {
-		const int v = t / Np, node = t - v * Np, i0 = node % N;
-		double r = rhs[(size_t)e * nb + t];
+	for (int i = t; i < nb; i += NT) {
+		const int v = i / Np, node = i - v * Np, i0 = node % N;
+		double r = rhs[(size_t)e * nb + i];
 		if constexpr (D == 2) {
 			const int i1 = node / N;
 			if (i0 == 0) r -= sf[0 * nfv + v * N + i1];
@@ -511,44 +511,44 @@ __global__ void __launch_bounds__(D == 2 ? 64 : 128) k_bgs_color(const int* __re
 			if (i2 == 0) r -= sf[4 * nfv + v * NF + i0 + N * i1];
 			if (i2 == N - 1) r -= sf[5 * nfv + v * NF + i0 + N * i1];
 		}
-		sr[t] = r;
+		sr[i] = r;
 	}
 	__syncthreads();
-	if (t < nb) {
+	for (int i = t; i < nb; i += NT) {
 		const TB* M = invT + (size_t)e * nb * nb;
 		double acc = 0.0;
-		for (int k = 0; k < nb; ++k) acc += (double)M[(size_t)k * nb + t] * sr[k];
-		z[(size_t)e * nb + t] = acc;
+		for (int k = 0; k < nb; ++k) acc += (double)M[(size_t)k * nb + i] * sr[k];
+		z[(size_t)e * nb + i] = acc;
 	}
 }
 
 // The same colour sweep with full face blocks (the BR2 residual: the neighbour's whole state enters through its
-// lifted gradient): offT holds dR_e/du_n transposed per local face, [e][lf][col][row], nb <= 64.  Four wavefronts
-// per element, one per local face for the face products and one quarter of the columns each for the inverse, so
-// that four times as many loads are in flight as with one wavefront (the sweep reads 4 nb^2 + nb^2 values per
-// element and is bound by load latency, not bytes).
+// lifted gradient): offT holds dR_e/du_n transposed per local face, [e][lf][col][row], nb <= kMaxInvBlock.  Four
+// wavefronts per element, one per local face for the face products and one quarter of the columns each for the
+// inverse, so that four times as many loads are in flight as with one wavefront (the sweep reads 4 nb^2 + nb^2
+// values per element and is bound by load latency, not bytes); rows beyond 64 loop.
 template <typename TB>
 __global__ void __launch_bounds__(256) k_bgs_full_color(const int* __restrict__ elems, const ElemFace* __restrict__ ef,
                                                         const TB* __restrict__ invT, const TB* __restrict__ offT,
                                                         const double* __restrict__ rhs, double* z, int nb)
 {
-	__shared__ double sz[4][64], sf[4][64], sr[64];
-	const int e = elems[blockIdx.x], t = threadIdx.x, lf = t >> 6, row = t & 63;
+	__shared__ double sz[4][kMaxInvBlock], sf[4][kMaxInvBlock], sr[kMaxInvBlock];
+	const int e = elems[blockIdx.x], t = threadIdx.x, lf = t >> 6, l = t & 63;
 	{
 		const ElemFace f = ef[(size_t)e * 4 + lf];
-		sz[lf][row] = (f.en >= 0 && row < nb) ? z[(size_t)f.en * nb + row] : 0.0;
+		for (int row = l; row < nb; row += 64) sz[lf][row] = (f.en >= 0) ? z[(size_t)f.en * nb + row] : 0.0;
 	}
 	__syncthreads();
-	if (row < nb) {   // face lf's product, row `row`
+	for (int row = l; row < nb; row += 64) {   // face lf's product, row `row`
 		const TB* C = offT + ((size_t)e * 4 + lf) * nb * nb;
 		double acc = 0.0;
 		for (int c = 0; c < nb; ++c) acc += (double)C[(size_t)c * nb + row] * sz[lf][c];
 		sf[lf][row] = acc;
 	}
 	__syncthreads();
-	if (t < nb) sr[t] = rhs[(size_t)e * nb + t] - ((sf[0][t] + sf[1][t]) + (sf[2][t] + sf[3][t]));
+	for (int i = t; i < nb; i += 256) sr[i] = rhs[(size_t)e * nb + i] - ((sf[0][i] + sf[1][i]) + (sf[2][i] + sf[3][i]));
 	__syncthreads();
-	if (row < nb) {   // quarter lf of the columns of A_ee^-1 r
+	for (int row = l; row < nb; row += 64) {   // quarter lf of the columns of A_ee^-1 r
 		const TB* M = invT + (size_t)e * nb * nb;
 		const int k0 = lf * ((nb + 3) / 4), k1 = min(nb, k0 + (nb + 3) / 4);
 		double acc = 0.0;
@@ -556,7 +556,7 @@ __global__ void __launch_bounds__(256) k_bgs_full_color(const int* __restrict__ 
 		sf[lf][row] = acc;
 	}
 	__syncthreads();
-	if (t < nb) z[(size_t)e * nb + t] = (sf[0][t] + sf[1][t]) + (sf[2][t] + sf[3][t]);
+	for (int i = t; i < nb; i += 256) z[(size_t)e * nb + i] = (sf[0][i] + sf[1][i]) + (sf[2][i] + sf[3][i]);
 }
 
 struct PrecondIdentity : Precond {
@@ -600,7 +600,7 @@ struct PrecondBlockJacobi : Precond {
 			                              cap_sigma + (size_t)cap_count));
 			cap_count += c.E;
 		}
-		// blocks up to 64 x 64: explicit inverse (one coalesced mat-vec per application); larger: LU
+		// blocks up to 142 x 142: explicit inverse (one coalesced mat-vec per application); larger: LU
 		if (nb <= kMaxInvBlock) return launch_block_inverse(c.stream, blocks, c.E, nb, nullptr);
 		return launch_block_lu(c.stream, blocks, piv, c.E, nb);
 	}
@@ -633,7 +633,7 @@ struct PrecondBlockSGS : Precond {
 	hipError_t setup(Ctx& c, const double* u, const double* inv_dt) override
 	{
 		const int nb = c.Np * c.nv, nf = 2 * c.d, nfv = c.nv * (c.d == 2 ? c.N : c.N * c.N);
-		if ((c.d != 2 && c.d != 3) || (full(c) && c.d != 2) || nb > kMaxInvBlock || nf * nfv > (c.d == 2 ? 64 : 128))
+		if ((c.d != 2 && c.d != 3) || (full(c) && c.d != 2) || nb > kMaxInvBlock || nf * nfv > kMaxSgsFace)
 			return hipErrorInvalidValue;
 		DGX_HIP_CHECK(diag.setup(c, u, inv_dt));
 		// the face blocks: compact on the face nodes (Euler, advection), full for BR2
@@ -884,14 +884,14 @@ struct Gmres {
 using dgx::Ctx;
 
 // 3-D (HEX Euler): the FP64 J v, and no preconditioner, block-Jacobi with blocks the batched LU takes
-// (nb = 5 (p + 1)^3 <= 143: p <= 2), or block SGS with blocks the in-LDS inverse and the sweep take (p = 1)
+// (nb = 5 (p + 1)^3 <= 143: p <= 2), or block SGS with blocks the in-LDS inverse and the sweep take (p <= 2)
 static bool implicit_3d_ok(const Ctx& c, int precond, int jvp_fp32)
 {
 	if (c.d != 3) return true;
 	if (jvp_fp32) return false;
 	if (precond == DGX_PC_NONE) return true;
-	if (precond == DGX_PC_BLOCK_SGS || precond == DGX_PC_BLOCK_SGS32)   // in-LDS inverses, 6 nv N^2 <= 128 (p = 1)
-		return c.Np * c.nv <= dgx::kMaxInvBlock && 6 * c.nv * c.N * c.N <= 128;
+	if (precond == DGX_PC_BLOCK_SGS || precond == DGX_PC_BLOCK_SGS32)   // in-LDS inverses (nb <= 142: p <= 2)
+		return c.Np * c.nv <= dgx::kMaxInvBlock && 6 * c.nv * c.N * c.N <= dgx::kMaxSgsFace;
 	return precond == DGX_PC_BLOCK_JACOBI && c.Np * c.nv <= dgx::kMaxLuBlock;
 }
 

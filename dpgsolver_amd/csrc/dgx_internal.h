@@ -338,7 +338,9 @@ hipError_t launch_relaxed_update(Ctx& c, double* u, const double* du, int* statu
 std::vector<double> min_edge_lengths(int d, int N, int E, const double* xyz);
 constexpr int kMaxLuBlock = 143;   // largest block of the in-LDS batched LU: nb^2 doubles in the 160 KB of LDS a
                                    // gfx950 workgroup may hold (HEX p = 2: nb = 135)
-constexpr int kMaxInvBlock = 64;   // largest element block (nb = Np * nv) of the in-LDS batched inverse
+constexpr int kMaxInvBlock = 142;  // largest element block (nb = Np * nv) of the in-LDS batched inverse: nb^2 + nb
+                                   // doubles and the pivot list within the 160 KB of a gfx950 workgroup
+constexpr int kMaxSgsFace = 288;   // face dofs (2 d nv N^(d-1)) of one element in the block SGS sweep (HEX p = 2: 270)
 hipError_t launch_block_inverse(hipStream_t s, double* blocks, int E, int nb, int* status);
 hipError_t launch_block_inv_apply(hipStream_t s, const double* invT, const double* v, double* z, int E, int nb);
 hipError_t launch_block_lu(hipStream_t s, double* blocks, int* piv, int E, int nb);

@@ -611,8 +611,15 @@ __global__ void __launch_bounds__(64) k_block_inv_apply(const double* __restrict
 hipError_t launch_block_inverse(hipStream_t s, double* blocks, int E, int nb, int* status)
 {
 	if (E == 0) return hipSuccess;
-	if (nb > kMaxInvBlock) return hipErrorInvalidValue;
-	hipLaunchKernelGGL(k_block_inverse, dim3(E), dim3(256), sizeof(double) * (nb * nb + nb), s, blocks, nb, status);
+	if (nb <= 0 || nb > kMaxInvBlock) return hipErrorInvalidValue;
+	const size_t lds = sizeof(double) * (nb * nb + nb);
+	if (lds > 65536) {   // beyond 64 KB the launch asks for the larger dynamic LDS once per process
+		static hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_block_inverse),
+		                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+		                                             (int)(sizeof(double) * (kMaxInvBlock * kMaxInvBlock + kMaxInvBlock)));
+		if (attr != hipSuccess) return attr;
+	}
+	hipLaunchKernelGGL(k_block_inverse, dim3(E), dim3(256), lds, s, blocks, nb, status);
 	return hipGetLastError();
 }
 hipError_t launch_block_inv_apply(hipStream_t s, const double* invT, const double* v, double* z, int E, int nb)

@@ -304,7 +304,7 @@ dgx_status dgx_convert_f32_f64(dgx_ctx* ctx, const float* src, double* dst, long
 
 /* Implicit path (contexts of one order: 2-D QUAD advection, Euler, and Navier-Stokes where stated; 3-D HEX
  * advection and Euler for J v, the Newton operator, the diagonal and face blocks, the block-sparse export, the
- * pseudo-time step and the solve with block-Jacobi up to nb = 143 (p = 2), block SGS up to nb = 64 (p = 1), or
+ * pseudo-time step and the solve with block-Jacobi up to nb = 143 or block SGS up to nb = 142 (both p <= 2), or
  * no preconditioner).
  * Diagonal blocks A_e = dR_e/du_e - mass_scale[e] * M_e (mass_scale: device array of E, or NULL for none), (Np*nv)^2 each, row-major, row eq*Np+node,
  * column var*Np+node: volume lhs_v_1 + face lhs_ll/lhs_rr contributions + CFL term
@@ -351,7 +351,7 @@ dgx_status dgx_newton_operator(dgx_ctx* ctx, const double* u, const double* mass
 dgx_status dgx_block_lu_factor(dgx_ctx* ctx, double* blocks, int* piv, int E, int nb);
 /* x <- A_e^{-1} x for every element block (x: E*nb). */
 dgx_status dgx_block_lu_solve(dgx_ctx* ctx, const double* lu, const int* piv, double* x, int E, int nb);
-/* In-place inverse of E blocks of size nb <= 64 (Gauss-Jordan, partial pivoting), stored transposed:
+/* In-place inverse of E blocks of size nb <= 142 (Gauss-Jordan, partial pivoting, a block in LDS), stored transposed:
  * blocks[e][j * nb + i] = (A_e^-1)[i][j].  The block-Jacobi preconditioner applies it as one mat-vec. */
 dgx_status dgx_block_inverse(dgx_ctx* ctx, double* blocks, int E, int nb);
 /* y_e = A_e^-1 x_e from the transposed inverses of dgx_block_inverse (x, y: E*nb, distinct). */
@@ -360,9 +360,9 @@ dgx_status dgx_block_inv_apply(dgx_ctx* ctx, const double* inv, const double* x,
 /* Steady solve by pseudo-time Newton-GMRES (solve_implicit, solve_implicit.c:95-110, 232-281, 393-470).
  * Names mirror the reference's test_case.data keys (cfl_initial, exit_tol_i, exit_ratio_i) and the PETSc
  * options file (GMRES restart 60, rtol 1e-4).  u (device) is updated in place.  Navier-Stokes contexts: the
- * FP64 operator, any preconditioner but DGX_PC_LEARNED (block SGS over the full BR2 face blocks, nb <= 64). */
+ * FP64 operator, any preconditioner but DGX_PC_LEARNED (block SGS over the full BR2 face blocks, nb <= 142). */
 /* Preconditioners of the GMRES solves.  DGX_PC_BLOCK_JACOBI: exact inverses of the element diagonal blocks of
- * dR/du - M/dt.  DGX_PC_BLOCK_SGS (nb <= 64; HEX: p = 1): one forward and one backward multicolour block Gauss-Seidel sweep
+ * dR/du - M/dt.  DGX_PC_BLOCK_SGS (nb <= 142: QUAD Euler p <= 4, HEX p <= 2): one forward and one backward multicolour block Gauss-Seidel sweep
  * over the diagonal and face blocks from z = 0, the coupled-block counterpart of the reference's default
  * PETSc ILU(1) with RCM ordering (external/petsc/options_files/petsc_options_gmres_default.txt:6-8), which
  * factors the assembled matrix sequentially.  DGX_PC_BLOCK_SGS32: the same sweeps with the inverse diagonal

@@ -8,12 +8,12 @@
 * dgx_newton_operator equals dgx_jacobian_vector followed by dgx_mass_sub; the pseudo-time step has its closed
   form on a uniform state.
 * A steady solve on a 3-D box with Riemann faces (p = 1; block-Jacobi, block SGS with FP64 and FP32-stored
-  blocks, and no preconditioner; p = 2 with block-Jacobi through the batched LU) reaches max |R| < 1e-10 from a
-  perturbed free stream; the oracle's residual of the result agrees, and block SGS takes fewer GMRES iterations
+  blocks, and no preconditioner; p = 2 with block-Jacobi and block SGS over 135 x 135 blocks) reaches max |R|
+  < 1e-10 from a perturbed free stream; the oracle's residual of the result agrees, and block SGS takes fewer GMRES iterations
   than block-Jacobi.
 * The 3-D face blocks, the block-sparse export and the SGS sweeps are checked against the oracle and numpy with
   the 2-D ones (test_gpu_solvers.py, the hex_* cases).
-* 3-D paths with no counterpart here (FP32 J v, blocks the batched LU or the SGS sweep cannot take) return
+* 3-D paths with no counterpart here (FP32 J v, blocks beyond the in-LDS inverse and LU: p >= 3) return
   DGX_ERR_UNSUPPORTED.
 """
 import numpy as np
@@ -151,19 +151,25 @@ def test_steady_box_3d_newton_gmres(dev):
     ctx.close()
 
 
-def test_steady_box_3d_p2_block_jacobi(dev):
-    """p = 2 (nb = 135): block-Jacobi through the batched LU with a 146 KB block in LDS per workgroup."""
+def test_steady_box_3d_p2_block_jacobi_and_sgs(dev):
+    """p = 2 (nb = 135): block-Jacobi and block SGS over in-LDS inverses of 146 KB blocks (270 face dofs per
+    element in the SGS sweep)."""
     from dpgsolver_amd import _lib
 
     mesh, u0, prim = _box(2)
     ctx = _ctx(mesh, M.FLUX_ROE, bc_prim=prim)
-    ud = torch.from_numpy(u0).to(dev)
-    st = ctx.solve_implicit(ud.data_ptr(), cfl_initial=5.0, cfl_max=300.0, exit_tol=1e-10, max_newton=60,
-                            precond=_lib.PC_BLOCK_JACOBI)
-    print("3-D box p=2, block-Jacobi: newton", st["newton_steps"], "gmres", st["gmres_iters"], st["seconds"], flush=True)
-    assert st["converged"] == 1 and st["max_rhs"] < 1e-10
-    r = O.Oracle(mesh, M.PDE_EULER, M.FLUX_ROE, bc_prim=prim).rhs(ud.cpu().numpy())
-    assert np.abs(r).max() < 1e-9
+    its = {}
+    for pc in (_lib.PC_BLOCK_JACOBI, _lib.PC_BLOCK_SGS):
+        ud = torch.from_numpy(u0).to(dev)
+        st = ctx.solve_implicit(ud.data_ptr(), cfl_initial=5.0, cfl_max=300.0, exit_tol=1e-10, max_newton=60,
+                                precond=pc)
+        print("3-D box p=2, precond", pc, "newton", st["newton_steps"], "gmres", st["gmres_iters"], st["seconds"],
+              flush=True)
+        assert st["converged"] == 1 and st["max_rhs"] < 1e-10
+        its[pc] = st["gmres_iters"]
+        r = O.Oracle(mesh, M.PDE_EULER, M.FLUX_ROE, bc_prim=prim).rhs(ud.cpu().numpy())
+        assert np.abs(r).max() < 1e-9
+    assert its[_lib.PC_BLOCK_SGS] < its[_lib.PC_BLOCK_JACOBI]
     ctx.close()
 
 
@@ -176,16 +182,10 @@ def test_3d_unsupported_paths_are_refused(dev):
     with pytest.raises(_lib.DgxError, match="UNSUPPORTED"):
         ctx.solve_implicit(ud.data_ptr(), precond=_lib.PC_BLOCK_JACOBI, jvp_fp32=True)
     ctx.close()
-    mesh2, u2 = _curved(2, 2, 3)    # nb = 135: more than the SGS sweep takes
-    ctx = _ctx(mesh2, M.FLUX_ROE)
-    ud = torch.from_numpy(u2).to(dev)
-    for pc in (_lib.PC_BLOCK_SGS, _lib.PC_BLOCK_SGS32):
-        with pytest.raises(_lib.DgxError, match="UNSUPPORTED"):
-            ctx.solve_implicit(ud.data_ptr(), precond=pc)
-    ctx.close()
-    mesh3, u3 = _curved(3, 2, 3)    # nb = 320: more than the batched LU takes
+    mesh3, u3 = _curved(3, 2, 3)    # nb = 320: more than the batched LU and the in-LDS inverse take
     ctx = _ctx(mesh3, M.FLUX_ROE)
     ud = torch.from_numpy(u3).to(dev)
-    with pytest.raises(_lib.DgxError, match="UNSUPPORTED"):
-        ctx.solve_implicit(ud.data_ptr(), precond=_lib.PC_BLOCK_JACOBI)
+    for pc in (_lib.PC_BLOCK_JACOBI, _lib.PC_BLOCK_SGS, _lib.PC_BLOCK_SGS32):
+        with pytest.raises(_lib.DgxError, match="UNSUPPORTED"):
+            ctx.solve_implicit(ud.data_ptr(), precond=pc)
     ctx.close()

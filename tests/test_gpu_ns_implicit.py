@@ -158,6 +158,23 @@ FACE_CASES = [c for c in CASES if c[0] in ("periodic_p1_roe", "periodic_p2_lf", 
                                             "riemann_box_p2")]
 
 
+def _periodic_p4_3x3():
+    """p = 4 (nb = 100, more than 64 rows per block in the full-block SGS sweep) on a 3 x 3 periodic mesh, so that
+    every face has its own neighbour."""
+    curve = lambda pts: np.stack([pts[..., 0] + 0.03 * np.sin(2 * np.pi * pts[..., 1]),
+                                  pts[..., 1] + 0.03 * np.sin(2 * np.pi * pts[..., 0])], -1)
+    mesh = M.structured_tp_mesh(2, 4, (3, 3), (0.0, 0.0), (1.0, 1.0), (True, True), curve=curve,
+                                rotate=np.random.default_rng(44).integers(0, 4, 9))
+    x = M.nodes_xyz(mesh)
+    X, Y = x[..., 0], x[..., 1]
+    prim = np.stack([1 + 0.1 * np.sin(2 * np.pi * X), 0.3 + 0.2 * np.sin(2 * np.pi * Y),
+                     -0.1 + 0.2 * np.cos(2 * np.pi * X), 1 + 0.1 * np.sin(2 * np.pi * (X + Y))], -1)
+    return ("periodic_3x3_p4_roe", mesh, M.FLUX_ROE, {}, M.pack_state(M.prim_to_cons(prim, 2)))
+
+
+FACE_CASES = FACE_CASES + [_periodic_p4_3x3()]
+
+
 @pytest.mark.parametrize("case", FACE_CASES, ids=[c[0] for c in FACE_CASES])
 def test_ns_face_blocks_match_complex_step(dev, case):
     name, mesh, flux, kw, u = case

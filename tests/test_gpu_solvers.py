@@ -116,6 +116,15 @@ def _jac_cases():
     pert = 0.02 * np.sin(3 * x[..., 0]) * np.cos(2 * x[..., 1]) * np.cos(x[..., 2])
     u = M.pack_state(M.prim_to_cons(prim[None, None, :] * (1.0 + np.stack([pert, pert, -pert, pert, 0.5 * pert], -1)), 3))
     out.append(("hex_box_lf_p1", m, M.FLUX_LF, prim, u))
+    # the largest blocks of the in-LDS inverse: QUAD Euler p = 4 (nb = 100) and HEX p = 2 (nb = 135, 270 face dofs)
+    m = M.structured_tp_mesh(2, 4, (3, 3), (0, 0), (1, 1), (True, True),
+                             rotate=np.random.default_rng(6).integers(0, 4, 9))
+    u = M.pack_state(M.prim_to_cons(M.smooth_state_3d(M.nodes_xyz(m)), 2))
+    out.append(("rotated_roe_p4", m, M.FLUX_ROE, None, u))
+    m = M.structured_tp_mesh(3, 2, (2, 2, 2), (0, 0, 0), (1, 1, 1), (True, True, True), curve=curve,
+                             rotate=np.random.default_rng(7).integers(0, 8, 8))
+    u = M.pack_state(M.prim_to_cons(M.smooth_state_3d(M.nodes_xyz(m)), 3))
+    out.append(("hex_rotated_lf_p2", m, M.FLUX_LF, None, u))
     return out
 
 
@@ -377,26 +386,26 @@ def test_block_lu_solve_matches_numpy(dev):
     # nb = 144: nb^2 doubles exceed the LDS; refused on the host before any launch (the buffers are 5 x 143^2)
     with pytest.raises(_lib.DgxError):
         ctx.block_lu_factor(Ad.data_ptr(), piv.data_ptr(), 1, 144)
-    E, nb = 37, 64
-    A = rng.standard_normal((E, nb, nb)) + 4 * np.eye(nb)
-    x = rng.standard_normal((E, nb))
-    xd = torch.from_numpy(x.copy()).to(dev)
-    # the explicit (transposed) inverse used by the block-Jacobi preconditioner, with pivoting exercised
-    A2 = A.copy()
-    A2[:, [0, 5]] = A2[:, [5, 0]]          # zero-free but pivot-demanding leading columns
-    A2[:, 0, 0] = 1e-8
-    Ai = torch.from_numpy(A2.copy()).to(dev)
-    ctx.block_inverse(Ai.data_ptr(), E, nb)
-    y = torch.empty_like(xd)
-    x2 = torch.from_numpy(x.copy()).to(dev)
-    ctx.block_inv_apply(Ai.data_ptr(), x2.data_ptr(), y.data_ptr(), E, nb)
-    torch.cuda.synchronize()
-    inv = np.transpose(Ai.cpu().numpy().reshape(E, nb, nb), (0, 2, 1))
-    np.testing.assert_allclose(inv, np.linalg.inv(A2), rtol=0, atol=1e-12 * np.abs(np.linalg.inv(A2)).max())
-    ref2 = np.linalg.solve(A2, x[..., None])[..., 0]
-    assert _rel(y.cpu().numpy(), ref2) < 1e-12
-    with pytest.raises(_lib.DgxError):
-        ctx.block_inverse(Ai.data_ptr(), E, 65)
+    # the explicit (transposed) inverse used by the block-Jacobi and block SGS preconditioners, with pivoting
+    # exercised; nb = 135 and 142 take more than 64 KB of LDS per block
+    for E, nb in ((37, 64), (9, 135), (4, 142)):
+        A = rng.standard_normal((E, nb, nb)) / np.sqrt(nb) + 4 * np.eye(nb)
+        x = rng.standard_normal((E, nb))
+        A2 = A.copy()
+        A2[:, [0, 5]] = A2[:, [5, 0]]          # zero-free but pivot-demanding leading columns
+        A2[:, 0, 0] = 1e-8
+        Ai = torch.from_numpy(A2.copy()).to(dev)
+        ctx.block_inverse(Ai.data_ptr(), E, nb)
+        y = torch.empty((E, nb), dtype=torch.float64, device=dev)
+        x2 = torch.from_numpy(x.copy()).to(dev)
+        ctx.block_inv_apply(Ai.data_ptr(), x2.data_ptr(), y.data_ptr(), E, nb)
+        torch.cuda.synchronize()
+        inv = np.transpose(Ai.cpu().numpy().reshape(E, nb, nb), (0, 2, 1))
+        np.testing.assert_allclose(inv, np.linalg.inv(A2), rtol=0, atol=1e-12 * np.abs(np.linalg.inv(A2)).max())
+        ref2 = np.linalg.solve(A2, x[..., None])[..., 0]
+        assert _rel(y.cpu().numpy(), ref2) < 1e-12, nb
+    with pytest.raises(_lib.DgxError):   # refused on the host before any launch (the buffer is 4 x 142^2)
+        ctx.block_inverse(Ai.data_ptr(), 1, 143)
     ctx.close()
 
 
