@@ -341,8 +341,9 @@ def ns_time_to_steady(torch, dev, max_newton=150):
     (r 0.5 to 1, inner cylinder rotating at omega 1 with the no_slip_all wall, outer wall adiabatic), mu 1e-3 and
     Pr 0.72 of its solution.data (Re about 250), BR2 with the Roe flux, DG p=3 on 16 x 64 curved QUADs (65,536
     DOF), from the fluid at rest, pseudo-time Newton-GMRES(60) (rtol 1e-4) with cfl_initial 1 and the ramped CFL
-    capped at 300.  Legs: multicolour block SGS over the diagonal and full BR2 face blocks (FP64 and FP32-stored),
-    and block-Jacobi, each given at most max_newton Newton steps."""
+    capped at 300.  Legs: multicolour block SGS over the diagonal and full BR2 face blocks (FP64 and FP32-stored,
+    and FP32-stored with the GMRES operator in FP32 dual numbers), and block-Jacobi, each given at most max_newton
+    Newton steps."""
     import numpy as np
 
     from dpgsolver_amd import _lib, mesh as M
@@ -367,6 +368,9 @@ def ns_time_to_steady(torch, dev, max_newton=150):
     print(f"navier-stokes taylor-couette, block SGS: {sgs['seconds']:.3f} s, newton {sgs['newton_steps']}, "
           f"gmres {sgs['gmres_iters']}", file=sys.stderr, flush=True)
     s32, u_s32 = solve(_lib.PC_BLOCK_SGS32)
+    j32, u_j32 = solve(_lib.PC_BLOCK_SGS32, jvp_fp32=True)
+    print(f"navier-stokes taylor-couette, block SGS (FP32 blocks, FP32 operator): {j32['seconds']:.3f} s, "
+          f"newton {j32['newton_steps']}, gmres {j32['gmres_iters']}", file=sys.stderr, flush=True)
     bj, _ = solve(_lib.PC_BLOCK_JACOBI)
     print(f"navier-stokes taylor-couette, block-Jacobi: converged {bj['converged']}, max_rhs {bj['max_rhs']:.2e} "
           f"after {bj['newton_steps']} steps", file=sys.stderr, flush=True)
@@ -375,6 +379,8 @@ def ns_time_to_steady(torch, dev, max_newton=150):
                         "DG p=3 on 16x64 curved QUADs (65536 DOF), from rest; Newton-GMRES(60) to 1e-10, CFL cap 300",
             "block_sgs": sgs, "block_sgs_fp32_blocks": {**s32, "steady_states_rel_diff":
                                                         ((u_s32 - u_sgs).abs().max() / u_sgs.abs().max()).item()},
+            "block_sgs_fp32_blocks_fp32_jvp": {**j32, "steady_states_rel_diff":
+                                               ((u_j32 - u_sgs).abs().max() / u_sgs.abs().max()).item()},
             "block_jacobi": bj}
 
 

@@ -926,9 +926,8 @@ extern "C" dgx_status dgx_solve_implicit(dgx_ctx* h, double* u, const dgx_implic
 	Ctx& c = h->c;
 	if (!c.mesh_ready) return c.hp ? DGX_ERR_UNSUPPORTED : DGX_ERR_NO_MESH;
 	if ((c.d != 2 && c.d != 3) || c.cfg.pde == DGX_PDE_ADVECTION) return DGX_ERR_UNSUPPORTED;
-	// Navier-Stokes: the FP64 Jacobian-vector product, no learned preconditioner
-	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES && (prm->precond == DGX_PC_LEARNED || prm->jvp_fp32))
-		return DGX_ERR_UNSUPPORTED;
+	// Navier-Stokes: no learned preconditioner (it is trained on Euler blocks)
+	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES && prm->precond == DGX_PC_LEARNED) return DGX_ERR_UNSUPPORTED;
 	if (!implicit_3d_ok(c, prm->precond, prm->jvp_fp32)) return DGX_ERR_UNSUPPORTED;
 	if (prm->gmres_restart < 1 || prm->gmres_restart > 200 || prm->max_newton < 1 || !(prm->cfl_initial > 0.0))
 		return DGX_ERR_ARG;

@@ -286,6 +286,7 @@ hipError_t launch_rhs_ns32(Ctx& c, const float* u, float* r);    // the same in 
 hipError_t launch_rhs_ns_mixed(Ctx& c, const double* u, float* r);   // FP64 state, FP32 perturbation form
 // its linearisation (dual numbers): jv = J v (or J v - mass_scale[e] M_e v), one column of every diagonal block
 hipError_t launch_ns_jvp(Ctx& c, const double* u, const double* v, double* jv, const double* mass_scale);
+hipError_t launch_ns_jvp32(Ctx& c, const double* u, const double* v, double* jv, const double* mass_scale);
 hipError_t launch_ns_diag_col(Ctx& c, const double* u, double* blocks, int col);
 hipError_t launch_ns_offd_col(Ctx& c, const double* u, double* blocks, int col, bool transposed);
 hipError_t launch_rhs_mixed(Ctx& c, const double* u, float* r);

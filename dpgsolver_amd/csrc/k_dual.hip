@@ -64,6 +64,7 @@ hipError_t launch_jvp(Ctx& c, const double* u, const double* v, double* jv, cons
 // the same product in FP32 dual numbers over the FP32 geometry and operator shadows, FP64 vectors in and out
 hipError_t launch_jvp32(Ctx& c, const double* u, const double* v, double* jv, const double* mass_scale)
 {
+	if (c.cfg.pde == DGX_PDE_NAVIER_STOKES) return launch_ns_jvp32(c, u, v, jv, mass_scale);
 	RhsArgs<Dual32, float> a{};
 	a.u = u;
 	a.v = v;

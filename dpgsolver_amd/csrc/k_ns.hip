@@ -24,6 +24,8 @@
 //     (compute_g_coef_related_boundary, compute_grad_coef_dg_T.c:527-575), the interior face gradient as the
 //     boundary one, and on the no-slip flux walls the provided energy flux nf_E in place of the energy
 //     equation's numerical flux.
+#include <type_traits>
+
 #include "dgx_internal.h"
 #include "dgx_device.h"
 
@@ -600,13 +602,18 @@ static hipError_t launch_ns(Ctx& c, const typename NsArgs<S>::G* u, typename NsA
 	a.u = u;
 	a.u64 = u64;
 	a.out = r;
-	if constexpr (sizeof(S) > sizeof(double)) {
-		// dual numbers: their own gradient-trace buffers, allocated on first use
-		const size_t ng = (size_t)c.E * 4 * 2 * c.nv * c.Nfc, nk = (size_t)c.E * 4 * c.Nfc;
-		if (!c.ns_gvt_dual) DGX_HIP_CHECK(hipMalloc(&c.ns_gvt_dual, sizeof(S) * ng));
-		if (!c.ns_kappa_dual) DGX_HIP_CHECK(hipMalloc(&c.ns_kappa_dual, sizeof(S) * nk));
-		a.gvt = reinterpret_cast<S*>(c.ns_gvt_dual);
-		a.kap = reinterpret_cast<S*>(c.ns_kappa_dual);
+	if constexpr (std::is_same<S, Dual>::value || std::is_same<S, Dual32>::value) {
+		if constexpr (std::is_same<S, Dual>::value) {
+			// FP64 dual numbers: their own gradient-trace buffers, allocated on first use
+			const size_t ng = (size_t)c.E * 4 * 2 * c.nv * c.Nfc, nk = (size_t)c.E * 4 * c.Nfc;
+			if (!c.ns_gvt_dual) DGX_HIP_CHECK(hipMalloc(&c.ns_gvt_dual, sizeof(S) * ng));
+			if (!c.ns_kappa_dual) DGX_HIP_CHECK(hipMalloc(&c.ns_kappa_dual, sizeof(S) * nk));
+			a.gvt = reinterpret_cast<S*>(c.ns_gvt_dual);
+			a.kap = reinterpret_cast<S*>(c.ns_kappa_dual);
+		} else {   // FP32 dual numbers are 8 bytes: the FP64-sized buffers hold them
+			a.gvt = reinterpret_cast<S*>(c.ns_gvt);
+			a.kap = reinterpret_cast<S*>(c.ns_kappa);
+		}
 		a.v = v;
 		a.col = col;
 		a.mass_scale = mass_scale;
@@ -625,8 +632,8 @@ static hipError_t launch_ns(Ctx& c, const typename NsArgs<S>::G* u, typename NsA
 	a.sutherland = c.cfg.viscosity_type == DGX_VISC_SUTHERLAND;
 	a.r_s = c.cfg.r_s > 0.0 ? c.cfg.r_s : 1.0;
 	a.E = c.E;
-	if constexpr (sizeof(S) == sizeof(float)) a.zf = c.hp_z32;
-	else if constexpr (sizeof(S) == sizeof(double)) a.zf = c.hp_z64;
+	if constexpr (std::is_same<S, float>::value) a.zf = c.hp_z32;
+	else if constexpr (std::is_same<S, double>::value) a.zf = c.hp_z64;
 	else a.zf = nullptr;   // dual numbers: no faces between orders
 	for (int k = 0; k < 5; ++k) a.bc_prim[k] = c.cfg.bc_prim[k];
 	a.walls = NsWalls{ c.cfg.wall_nf_E, c.cfg.wall_omega, c.cfg.wall_rho, c.cfg.wall_E };
@@ -678,6 +685,14 @@ hipError_t launch_ns_jvp(Ctx& c, const double* u, const double* v, double* jv, c
 {
 	return launch_ns<Dual, false, kNsJvp>(c, u, jv, c.metrics_vc, c.detJ_vc, c.nJ_fc, c.ops64, nullptr, 0, v, 0,
 	                                      mass_scale);
+}
+
+// the same product in FP32 dual numbers (FP64 state, direction, geometry and result): the FP32 inner operator of
+// the implicit solve
+hipError_t launch_ns_jvp32(Ctx& c, const double* u, const double* v, double* jv, const double* mass_scale)
+{
+	return launch_ns<Dual32, false, kNsJvp>(c, u, jv, c.metrics_vc, c.detJ_vc, c.nJ_fc, c.ops64, nullptr, 0, v, 0,
+	                                        mass_scale);
 }
 
 hipError_t launch_ns_diag_col(Ctx& c, const double* u, double* blocks, int col)

@@ -360,7 +360,8 @@ dgx_status dgx_block_inv_apply(dgx_ctx* ctx, const double* inv, const double* x,
 /* Steady solve by pseudo-time Newton-GMRES (solve_implicit, solve_implicit.c:95-110, 232-281, 393-470).
  * Names mirror the reference's test_case.data keys (cfl_initial, exit_tol_i, exit_ratio_i) and the PETSc
  * options file (GMRES restart 60, rtol 1e-4).  u (device) is updated in place.  Navier-Stokes contexts: the
- * FP64 operator, any preconditioner but DGX_PC_LEARNED (block SGS over the full BR2 face blocks, nb <= 142). */
+ * FP64 or FP32 operator, any preconditioner but DGX_PC_LEARNED (block SGS over the full BR2 face blocks,
+ * nb <= 142). */
 /* Preconditioners of the GMRES solves.  DGX_PC_BLOCK_JACOBI: exact inverses of the element diagonal blocks of
  * dR/du - M/dt.  DGX_PC_BLOCK_SGS (nb <= 142: QUAD Euler p <= 4, HEX p <= 2): one forward and one backward multicolour block Gauss-Seidel sweep
  * over the diagonal and face blocks from z = 0, the coupled-block counterpart of the reference's default
@@ -394,8 +395,9 @@ typedef struct dgx_implicit_params {
 	float* capture_features;   /* nf = round_up(nb + Np + 1, 16) floats per element */
 	float* capture_sigma;      /* 1 float per element */
 	long long capture_capacity;
-	int jvp_fp32;         /* 1: the GMRES operator dR/du v - M v / dt in FP32 dual numbers (FP32 geometry and
-	                         operators, FP64 vectors): the inner defect-correction product; the FP64 residual of
+	int jvp_fp32;         /* 1: the GMRES operator dR/du v - M v / dt in FP32 dual numbers (FP64 vectors; Euler:
+	                         FP32 geometry and operators, Navier-Stokes: FP64 ones converted as they are read): the
+	                         inner defect-correction product; the FP64 residual of
 	                         each Newton step still decides convergence.  0: FP64 (no counterpart: the reference
 	                         assembles the FP64 Jacobian) */
 } dgx_implicit_params;
