@@ -1,4 +1,5 @@
-// dgx_mesh.cpp -- gmsh 2.2 ASCII mesh ingest for 2-D QUAD, TRI and mixed meshes (host only, no HIP calls).
+// dgx_mesh.cpp -- gmsh 2.2 ASCII mesh ingest for 2-D QUAD, TRI and mixed meshes and 3-D HEX meshes (host only,
+// no HIP calls).
 //
 // Follows the reference's reader and connectivity (src/simulation/mesh/mesh_readers.c,
 // mesh_connectivity.c:285-465, mesh_periodic.c):
@@ -341,6 +342,211 @@ static dgx_status read_gmsh(const char* path, int p, dgx_host_mesh* out)
 	return DGX_OK;
 }
 
+// 3-D: first-order HEX volumes (gmsh type 5), QUAD boundary faces (type 3) carrying the physical tags, and
+// periodic pairs tagged 51/52 (x), 53/54 (y), 55/56 (z), master first (mesh_periodic.c).  gmsh's HEX vertices
+// (bottom face counter-clockwise, then the top face) are taken in tensor-product order (r fastest, then s, t):
+// (0, 1, 3, 2, 4, 5, 7, 6).  Local faces 0..5 are r = -1, r = +1, s = -1, s = +1, t = -1, t = +1, each with its
+// corners in face order (a, b), a fastest: (s, t) on r faces, (r, t) on s faces, (r, s) on t faces.  A face's
+// orientation code is the one of the eight QUAD-face codes whose corner permutation (dgx_face_permutation at
+// p = 1) carries the left face's corners onto the right face's.  The geometry is the trilinear map of the
+// vertices at the GLL nodes of degree p.
+static dgx_status read_hex(const char* path, int p, dgx_host_mesh* out)
+{
+	std::ifstream in(path);
+	if (!in) return DGX_ERR_ARG;
+	std::map<int, std::array<double, 3>> nodes;
+	std::vector<Elem> elems;
+	std::string line;
+	while (std::getline(in, line)) {
+		if (line.rfind("$Nodes", 0) == 0) {
+			int n = 0;
+			in >> n;
+			for (int i = 0; i < n; ++i) {
+				int id = 0;
+				double x = 0.0, y = 0.0, z = 0.0;
+				if (!(in >> id >> x >> y >> z)) return DGX_ERR_ARG;
+				nodes[id] = { x, y, z };
+			}
+		} else if (line.rfind("$Elements", 0) == 0) {
+			int n = 0;
+			in >> n;
+			std::getline(in, line);
+			for (int i = 0; i < n; ++i) {
+				std::getline(in, line);
+				std::istringstream ss(line);
+				int id = 0, type = 0, ntag = -1;
+				if (!(ss >> id >> type >> ntag) || ntag < 0 || ntag > 64) return DGX_ERR_ARG;
+				Elem e;
+				e.type = type;
+				e.tags.resize(ntag);
+				for (int t = 0; t < ntag; ++t)
+					if (!(ss >> e.tags[t])) return DGX_ERR_ARG;
+				int v;
+				while (ss >> v) e.nodes.push_back(v);
+				const int need = type == 15 ? 1 : type == 1 ? 2 : type == 3 ? 4 : type == 5 ? 8 : -1;
+				if (need < 0) return DGX_ERR_UNSUPPORTED;   // TET, PRISM, PYR, second order
+				if ((int)e.nodes.size() < need) return DGX_ERR_ARG;
+				for (int k = 0; k < need; ++k)
+					if (!nodes.count(e.nodes[k])) return DGX_ERR_ARG;
+				e.nodes.resize(need);
+				elems.push_back(e);
+			}
+		}
+	}
+	using Key = std::array<int, 4>;
+	auto key4 = [](Key k) { std::sort(k.begin(), k.end()); return k; };
+	std::vector<std::array<int, 8>> vol;
+	std::map<Key, int> btag;
+	std::vector<Key> masters[3], slaves[3];
+	for (const Elem& e : elems) {
+		if (e.type == 5) {
+			const std::vector<int>& n = e.nodes;
+			vol.push_back({ n[0], n[1], n[3], n[2], n[4], n[5], n[7], n[6] });
+		} else if (e.type == 3 && !e.tags.empty()) {
+			const Key q = { e.nodes[0], e.nodes[1], e.nodes[2], e.nodes[3] };
+			btag[key4(q)] = e.tags[0];
+			const int base = e.tags[0] % 10000;
+			if (base >= 51 && base <= 56) ((base % 2) ? masters : slaves)[(base - 51) / 2].push_back(q);
+			else if (base == 61 || base == 62) return DGX_ERR_UNSUPPORTED;
+		}
+	}
+	const int E = (int)vol.size();
+	if (E == 0) return DGX_ERR_UNSUPPORTED;
+	// periodic slave vertex -> master vertex, matched on the two coordinates along the face
+	double lo[3] = { 1e300, 1e300, 1e300 }, hi[3] = { -1e300, -1e300, -1e300 };
+	for (const auto& nd : nodes)
+		for (int c = 0; c < 3; ++c) { lo[c] = std::min(lo[c], nd.second[c]); hi[c] = std::max(hi[c], nd.second[c]); }
+	const double tol = 1e-9 * std::max({ hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-300 });
+	std::map<Key, std::map<int, int>> slave_map;
+	for (int dir = 0; dir < 3; ++dir) {
+		if (masters[dir].size() != slaves[dir].size()) return DGX_ERR_ARG;
+		const int t0 = (dir + 1) % 3, t1 = (dir + 2) % 3;
+		auto same = [&](int a, int b) {
+			return std::fabs(nodes[a][t0] - nodes[b][t0]) <= tol && std::fabs(nodes[a][t1] - nodes[b][t1]) <= tol;
+		};
+		for (const Key& S : slaves[dir]) {
+			bool found = false;
+			for (const Key& Mq : masters[dir]) {
+				std::map<int, int> mp;
+				for (int a : S)
+					for (int b : Mq)
+						if (same(a, b)) mp[a] = b;
+				if ((int)mp.size() == 4) {
+					slave_map[key4(S)] = mp;
+					found = true;
+					break;
+				}
+			}
+			if (!found) return DGX_ERR_ARG;   // a slave face with no master counterpart
+		}
+	}
+	// local face corners (a + 2 b) as tensor-product vertex indices
+	static const int fc[6][4] = { { 0, 2, 4, 6 }, { 1, 3, 5, 7 }, { 0, 1, 4, 5 },
+	                              { 2, 3, 6, 7 }, { 0, 1, 2, 3 }, { 4, 5, 6, 7 } };
+	const int nf = 6 * E;
+	std::vector<Key> raw(nf), cor(nf);
+	for (int v = 0; v < E; ++v)
+		for (int lf = 0; lf < 6; ++lf) {
+			Key r;
+			for (int k = 0; k < 4; ++k) r[k] = vol[v][fc[lf][k]];
+			raw[6 * v + lf] = r;
+			auto it = slave_map.find(key4(r));
+			Key c = r;
+			if (it != slave_map.end())
+				for (int k = 0; k < 4; ++k) c[k] = it->second.at(r[k]);
+			cor[6 * v + lf] = c;
+		}
+	std::map<Key, std::vector<int>> groups;
+	for (int f = 0; f < nf; ++f) groups[key4(cor[f])].push_back(f);
+	int perms[8][4];
+	for (int o = 0; o < 8; ++o)
+		if (dgx_face_permutation(3, 1, o, perms[o]) != DGX_OK) return DGX_ERR_ARG;
+	std::vector<int> v2v(nf, -1), v2lf(nf, -1), v2lfwp(nf, -1), faces;
+	int n_unsupported = 0;
+	for (const auto& g : groups) {
+		const std::vector<int>& fs = g.second;
+		if (fs.size() == 2) {
+			const int f0 = std::min(fs[0], fs[1]), f1 = std::max(fs[0], fs[1]);
+			v2v[f0] = f1 / 6;
+			v2lf[f0] = f1 % 6;
+			v2v[f1] = f0 / 6;
+			v2lf[f1] = f0 % 6;
+			if (key4(raw[f0]) == key4(raw[f1])) {
+				v2lfwp[f0] = v2lf[f0];
+				v2lfwp[f1] = v2lf[f1];
+			}
+			int ord = -1;
+			for (int o = 0; o < 8 && ord < 0; ++o) {
+				bool ok = true;
+				for (int k = 0; k < 4 && ok; ++k) ok = cor[f1][perms[o][k]] == cor[f0][k];
+				if (ok) ord = o;
+			}
+			if (ord < 0) return DGX_ERR_ARG;   // the two faces' corners do not coincide
+			const int row[6] = { f0 / 6, f0 % 6, f1 / 6, f1 % 6, ord, DGX_BC_INTERIOR };
+			faces.insert(faces.end(), row, row + 6);
+		} else if (fs.size() == 1) {
+			const int f = fs[0];
+			auto it = btag.find(key4(raw[f]));
+			const int tag = (it == btag.end()) ? -1 : it->second;
+			v2lf[f] = tag;
+			const int bc = bc_code(tag);
+			if (bc < 0) ++n_unsupported;
+			const int row[6] = { f / 6, f % 6, -1, -1, -1, bc < 0 ? DGX_BC_RIEMANN : bc };
+			faces.insert(faces.end(), row, row + 6);
+		} else {
+			return DGX_ERR_UNSUPPORTED;   // non-manifold face
+		}
+	}
+	for (int f = 0; f < nf; ++f)
+		if (v2lfwp[f] == -1) {
+			auto it = btag.find(key4(raw[f]));
+			if (it != btag.end()) v2lfwp[f] = it->second;
+		}
+	const int N = p + 1, Np = N * N * N;
+	std::vector<double> r, xyz((size_t)E * 3 * Np);
+	gll_nodes(N, r);
+	int n_inverted = 0;
+	for (int v = 0; v < E; ++v) {
+		double X[8][3];
+		for (int k = 0; k < 8; ++k)
+			for (int c = 0; c < 3; ++c) X[k][c] = nodes[vol[v][k]][c];
+		for (int k2 = 0; k2 < N; ++k2)
+			for (int j2 = 0; j2 < N; ++j2)
+				for (int i2 = 0; i2 < N; ++i2) {
+					const double w[2][3] = { { 0.5 * (1 - r[i2]), 0.5 * (1 - r[j2]), 0.5 * (1 - r[k2]) },
+					                         { 0.5 * (1 + r[i2]), 0.5 * (1 + r[j2]), 0.5 * (1 + r[k2]) } };
+					for (int c = 0; c < 3; ++c) {
+						double x = 0.0;
+						for (int k = 0; k < 8; ++k) x += w[k & 1][0] * w[(k >> 1) & 1][1] * w[k >> 2][2] * X[k][c];
+						xyz[((size_t)v * 3 + c) * Np + i2 + N * (j2 + N * k2)] = x;
+					}
+				}
+		double J[3][3];
+		for (int c = 0; c < 3; ++c) {
+			J[c][0] = X[1][c] - X[0][c];
+			J[c][1] = X[2][c] - X[0][c];
+			J[c][2] = X[4][c] - X[0][c];
+		}
+		const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+		                   J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+		                   J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+		if (det <= 0.0) ++n_inverted;
+	}
+	out->d = 3;
+	out->p = p;
+	out->E = E;
+	out->F = (int)(faces.size() / 6);
+	out->xyz = dup(xyz);
+	out->faces = dup(faces);
+	out->v_to_v = dup(v2v);
+	out->v_to_lf = dup(v2lf);
+	out->v_to_lf_wp = dup(v2lfwp);
+	out->n_unsupported_bc = n_unsupported;
+	out->n_inverted = n_inverted;
+	if (!out->xyz || !out->faces || !out->v_to_v || !out->v_to_lf || !out->v_to_lf_wp) return DGX_ERR_ARG;
+	return DGX_OK;
+}
+
 static dgx_status read_gmsh_mixed(const char* path, int p, dgx_host_mesh_mixed* out)
 {
 	ReadOut R;
@@ -381,6 +587,11 @@ dgx_status dgx_mesh_read_gmsh(const char* path, int p, dgx_host_mesh* out)
 	dgx_status st;
 	try {
 		st = read_gmsh(path, p, out);
+		if (st == DGX_ERR_UNSUPPORTED) {   // not a 2-D QUAD mesh: a HEX mesh?
+			dgx_mesh_free(out);
+			std::memset(out, 0, sizeof(*out));
+			st = read_hex(path, p, out);
+		}
 	} catch (...) {   // a malformed file (std::out_of_range, std::bad_alloc): a status code, never an abort
 		st = DGX_ERR_ARG;
 	}

@@ -183,9 +183,11 @@ dgx_status dgx_reference_nodes(int type, int p, double* rst, double* w, int* np)
 /* Host only: the reference's symmetric TRI rule kind<q> (kind 0: WSH of order q, 1: WV of degree q) expanded as the
  * reference expands it (nodes.c:628-852): *n points rst (2 n) and weights w (n); NULL outputs are skipped. */
 dgx_status dgx_tri_rule(int kind, int q, double* rst, double* w, int* n);
-/* gmsh 2.2 ASCII reader for 2-D QUAD meshes (mesh/mesh_readers.c, mesh_connectivity.c:285-465, mesh_periodic.c):
- * volumes in file order, $Periodic node correspondences, boundary tags from the line elements; geometry is
- * the straight-sided map at the GLL nodes of degree p.  TRI or 3-D volumes: DGX_ERR_UNSUPPORTED.  Host only. */
+/* gmsh 2.2 ASCII reader for 2-D QUAD and 3-D HEX meshes (mesh/mesh_readers.c, mesh_connectivity.c:285-465,
+ * mesh_periodic.c): volumes in file order, periodic pairs from the tagged boundary faces (51/52, 53/54, 55/56
+ * master/slave in x, y, z), boundary tags from the line (2-D) or QUAD (3-D) elements, each interior face's
+ * ind_ord from its corners; geometry is the straight-sided (bilinear / trilinear) map at the GLL nodes of degree
+ * p.  TRI, TET, prism and pyramid volumes: DGX_ERR_UNSUPPORTED.  Host only. */
 dgx_status dgx_mesh_read_gmsh(const char* path, int p, dgx_host_mesh* out);
 void dgx_mesh_free(dgx_host_mesh* m);
 /* A gmsh 2.2 mesh of TRI and QUAD elements (the reference's mixed 2-D meshes, mesh_readers.c,

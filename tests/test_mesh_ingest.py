@@ -176,3 +176,103 @@ def test_mixed_faces_meet_at_the_same_points(name):
             X = vols[order[e]]
             assert X.shape == (3, 2)
             np.testing.assert_allclose(mesh.xyz[e] @ w / w.sum(), X.mean(axis=0), atol=1e-13)
+
+
+# ----------------------------------------------------------------------------- 3-D HEX meshes
+def _cube_rotations():
+    """The 24 orientation-preserving symmetries of the unit cube as maps of corner coordinates (i0, i1, i2)."""
+    import itertools
+
+    out = []
+    for perm in itertools.permutations(range(3)):
+        sgn = np.linalg.det(np.eye(3)[list(perm)])
+        for flips in itertools.product((0, 1), repeat=3):
+            if sgn * (-1) ** sum(flips) > 0:
+                out.append((perm, flips))
+    return out
+
+
+def _write_hex_gmsh(path, n, periodic, seed):
+    """A gmsh 2.2 file of the n^3 unit-cube HEX mesh: each element's vertices listed after a random cube rotation,
+    boundary QUADs tagged 51..56 (periodic master/slave per direction) or 10101 (Riemann)."""
+    from dpgsolver_amd import mesh as M
+
+    bc = None if periodic else (M.BC_RIEMANN,) * 3
+    m1 = M.structured_tp_mesh(3, 1, (n, n, n), (0, 0, 0), (1, 1, 1), (periodic,) * 3, bc_low=bc, bc_high=bc)
+    corners = np.transpose(m1.xyz, (0, 2, 1))              # (E, 8, 3) in tensor-product order
+    ids, pts = {}, []
+    def nid(x):
+        k = tuple(np.round(x, 12))
+        if k not in ids:
+            ids[k] = len(pts) + 1
+            pts.append(x)
+        return ids[k]
+    rots = _cube_rotations()
+    rng = np.random.default_rng(seed)
+    gm = (0, 1, 3, 2, 4, 5, 7, 6)                          # gmsh HEX vertex j = tensor-product corner gm[j]
+    hexes = []
+    for e in range(m1.E):
+        perm, flips = rots[rng.integers(len(rots))]
+        vs = []
+        for j in range(8):
+            c = [(gm[j] >> k) & 1 for k in range(3)]
+            c = [c[perm[k]] ^ flips[k] for k in range(3)]
+            vs.append(nid(corners[e, c[0] + 2 * c[1] + 4 * c[2]]))
+        hexes.append(vs)
+    quads = []
+    for dirn in range(3):
+        for side in (0, 1):
+            for e in range(m1.E):
+                sel = [k for k in range(8) if ((k >> dirn) & 1) == side]
+                X = corners[e, sel]
+                if np.allclose(X[:, dirn], float(side)):
+                    tag = (51 + 2 * dirn + side) if periodic else 10101
+                    quads.append((tag, [nid(x) for x in X[[0, 1, 3, 2]]]))
+    with open(path, "w") as f:
+        f.write("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n%d\n" % len(pts))
+        for i, x in enumerate(pts):
+            f.write("%d %.17g %.17g %.17g\n" % (i + 1, *x))
+        f.write("$EndNodes\n$Elements\n%d\n" % (len(quads) + len(hexes)))
+        k = 1
+        for tag, vs in quads:
+            f.write("%d 3 2 %d %d %s\n" % (k, tag, tag, " ".join(map(str, vs))))
+            k += 1
+        for vs in hexes:
+            f.write("%d 5 2 1 1 %s\n" % (k, " ".join(map(str, vs))))
+            k += 1
+        f.write("$EndElements\n")
+
+
+@pytest.mark.parametrize("periodic", [True, False], ids=["periodic", "riemann"])
+def test_hex_gmsh_reader_matches_the_generated_mesh(tmp_path, periodic):
+    """dgx_mesh_read_gmsh on a HEX mesh whose elements list their vertices after random cube rotations (every
+    face orientation code occurs): the same elements (node sets), and the oracle's residual of a smooth state on the
+    read mesh equals the one on the generated mesh at every node to 1e-12 -- a wrong orientation code or
+    neighbour changes the face fluxes."""
+    import pyoracle as O
+
+    from dpgsolver_amd import _lib, mesh as M
+
+    n, p = 3, 2
+    path = str(tmp_path / "hex.msh")
+    _write_hex_gmsh(path, n, periodic, seed=11)
+    m = _lib.read_gmsh(path, p)
+    assert m["d"] == 3 and m["E"] == n ** 3 and m["n_inverted"] == 0 and m["n_unsupported_bc"] == 0
+    ref = M.structured_tp_mesh(3, p, (n, n, n), (0, 0, 0), (1, 1, 1), (periodic,) * 3,
+                               bc_low=None if periodic else (M.BC_RIEMANN,) * 3,
+                               bc_high=None if periodic else (M.BC_RIEMANN,) * 3)
+    got = M.Mesh(3, p, m["xyz"], m["faces"].astype(np.int32))
+    interior = m["faces"][:, 5] == M.BC_INTERIOR
+    assert len(set(m["faces"][interior, 4])) > 4              # many orientation codes occur
+    prim = M.freestream_prim(3)
+    u_of = lambda mesh: M.pack_state(M.prim_to_cons(M.smooth_state_3d(M.nodes_xyz(mesh)), 3))
+    kw = {} if periodic else {"bc_prim": prim}
+    r_got = O.Oracle(got, M.PDE_EULER, M.FLUX_ROE, **kw).rhs(u_of(got)).reshape(n ** 3, 5, -1)
+    r_ref = O.Oracle(ref, M.PDE_EULER, M.FLUX_ROE, **kw).rhs(u_of(ref)).reshape(n ** 3, 5, -1)
+    scale = np.abs(r_ref).max()
+    for e in range(n ** 3):
+        xg, xr = m["xyz"][e].T, ref.xyz[e].T                   # (Np, 3)
+        d2 = ((xg[:, None, :] - xr[None, :, :]) ** 2).sum(-1)
+        j = d2.argmin(1)
+        assert d2[np.arange(len(j)), j].max() < 1e-24 and len(set(j)) == len(j), e
+        assert np.abs(r_got[e] - r_ref[e][:, j]).max() <= 1e-12 * scale, e
