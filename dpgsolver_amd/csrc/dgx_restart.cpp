@@ -92,6 +92,10 @@ dgx_status dgx_restart_write_gmsh(const char* path, const dgx_config* cfg, const
 		const int* r = mesh->faces + 6 * fc;
 		if (r[2] >= 0) continue;
 		const int e = r[0], lf = r[1], dir = lf >> 1, side = lf & 1;
+		if (e < 0 || e >= mesh->E || lf < 0 || lf >= 2 * d) {   // a face table of another dimension or size
+			std::fclose(f);
+			return DGX_ERR_ARG;
+		}
 		const int tag = 10000 + (r[5] == DGX_BC_SLIPWALL ? 102 : 101);
 		std::vector<int> fvs;
 		for (int k = 0; k < nc; ++k)

@@ -84,7 +84,7 @@ void check_mesh(const std::string& path, const std::string& tmpdir)
 			continue;
 		}
 		expect(st == DGX_OK, "dgx_mesh_read_gmsh", path);
-		if (st != DGX_OK) {
+		if (st != DGX_OK || m.d != 2) {   // HEX meshes: read (and so checked) only; the round trip below is 2-D
 			dgx_mesh_free(&m);
 			continue;
 		}
